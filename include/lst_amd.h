@@ -1,0 +1,128 @@
+/* C ABI of the MI355X Line Segment Tracking engine (liblst_amd.so).
+ *
+ * Drop-in boundary for the reference's LST path: the entry points below
+ * replace, one for one, the reference interfaces cited on each declaration
+ * (paths relative to the SegmentLinking/TrackLooper tree).  Plain pointers and
+ * sizes only; the engine owns its device memory, the caller owns its inputs
+ * (read during the call only) and output buffers.
+ *
+ * Threading: an lst_es handle is immutable after creation and may be shared by
+ * any number of events; an lst_event handle is single-threaded.
+ */
+#ifndef LST_AMD_H
+#define LST_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lst_es lst_es;        /* geometry / event setup (LSTESDeviceData) */
+typedef struct lst_event lst_event;  /* one event's device state (SDL::Event)    */
+
+/* Status codes.  LST_W_TRUNCATED is a warning bit OR-ed into LST_OK when a
+ * capacity truncated objects exactly as the reference would (the reference
+ * only prints a banner, SDL/Event.cc:237-244,759-781). */
+typedef enum {
+  LST_OK = 0,
+  LST_E_IO = 1,        /* geometry file missing or unreadable (reference: exit(2), SDL/LSTESData.cc:31-38) */
+  LST_E_FORMAT = 2,    /* malformed geometry file (reference: std::runtime_error, SDL/ModuleConnectionMap.cc:14-46) */
+  LST_E_INPUT = 3,     /* bad input, e.g. a detId absent from the geometry (reference: indexes mapIdx[-1], SDL/Hit.h:245-246) */
+  LST_E_ORDER = 4,     /* stage called out of order */
+  LST_E_DEVICE = 5,    /* HIP runtime error */
+  LST_E_CAPACITY = 6,  /* output buffer too small */
+  LST_W_TRUNCATED = 0x100
+} lst_status;
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* lst_last_error(void);
+
+/* Replaces SDL::loadAndFillESHost() + SDL::loadAndFillESDevice(queue, host)
+ * (SDL/LSTESData.h:79-80, SDL/LSTESData.cc:78-113).  The geometry directory
+ * and pT cut are parameters (the reference hard-codes pt0.8 under $LST_BASE,
+ * SDL/LSTESData.cc:46-57,98-99).  module_map_stride = 0 uses the maximum
+ * fan-out of the geometry (the reference fixes 40, SDL/Constants.h:108, which
+ * the pt0.6 map overflows). */
+lst_status lst_es_create(const char* geom_dir, float pt_cut, uint32_t module_map_stride, lst_es** out);
+void lst_es_destroy(lst_es* es);
+/* Geometry scalars: "nModules", "nLowerModules", "nPixels", "pixelModuleIndex", "stride". */
+uint32_t lst_es_scalar(const lst_es* es, const char* name);
+
+/* T5 selection modes (reference build flags, bin/sdl_make_tracklooper:157-162). */
+enum { LST_T5_PLAIN_CHI2 = 0 /* -DUSE_RZCHI2 -DUSE_RPHICHI2 */, LST_T5_DNN = 1 /* -DUSE_RZCHI2 -DUSE_T5_DNN */ };
+
+/* Replaces SDL::Event<Acc>(verbose, queue, deviceESData) (SDL/Event.h:100-111).
+ * hip_stream is a hipStream_t (NULL: the engine creates its own).  Device
+ * arenas are allocated once here and reused by every event run on the handle. */
+lst_status lst_event_create(lst_es* es, void* hip_stream, int t5_mode, lst_event** out);
+void lst_event_destroy(lst_event* ev);
+
+/* Replaces SDL::Event::addHitToEvent(x, y, z, detId, idxInNtuple)
+ * (SDL/Event.h:115-119, SDL/Event.cc:151-215).  on_device != 0: the pointers
+ * are device pointers (inputs already resident in HBM). */
+lst_status lst_event_add_hits(lst_event* ev, const float* x, const float* y, const float* z, const uint32_t* det_id,
+                              const uint32_t* idx_in_ntuple, uint32_t n_hits, int on_device);
+
+/* The 18 vectors of SDL::Event::addPixelSegmentToEvent (SDL/Event.h:120-137,
+ * SDL/Event.cc:217-391), as arrays of n_pls entries. */
+typedef struct {
+  const uint32_t *hit_indices0, *hit_indices1, *hit_indices2, *hit_indices3;
+  const float *delta_phi_change, *pt_in, *pt_err, *px, *py, *pz, *eta, *eta_err, *phi;
+  const int32_t* charge;
+  const uint32_t* seed_idx;
+  const int32_t* superbin;
+  const int8_t* pixel_type;
+  const int8_t* is_quad;
+  uint32_t n_pls;
+} lst_pixel_segments;
+lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments* pls, int on_device);
+
+/* Stage calls, in this order (SDL/Event.h:138-150, SDL/LST.cc:74-148). */
+lst_status lst_event_create_mini_doublets(lst_event* ev);        /* createMiniDoublets */
+lst_status lst_event_create_segments(lst_event* ev);             /* createSegmentsWithModuleMap */
+lst_status lst_event_create_triplets(lst_event* ev);             /* createTriplets */
+
+/* Replaces SDL::Event::resetEvent (SDL/Event.cc:45-149); keeps the arenas. */
+lst_status lst_event_reset(lst_event* ev);
+/* Waits for the event's stream. */
+lst_status lst_event_synchronize(lst_event* ev);
+
+/* Replaces SDL::Event::getNumberOf* (SDL/Event.h:152-183): "hits", "pixelSegments",
+ * "miniDoublets", "segments", "triplets" (outer-tracker totals). */
+uint32_t lst_event_count(lst_event* ev, const char* what);
+
+/* Replaces the host-copy getters SDL::Event::get{Hits,MiniDoublets,Segments,
+ * Triplets,Ranges}() (SDL/Event.h:185-204) field by field: copies the named
+ * SoA field ("hits.xs", "mds.anchorHitIndices", "segs.mdIndices",
+ * "t3.circleRadius", "ranges.mdRanges", ...) into dst.  With dst == NULL only
+ * *dtype (0 f32, 1 u32, 2 i32, 3 u16, 4 i16, 5 u8, 6 i8) and *count are set. */
+lst_status lst_event_get_field(lst_event* ev, const char* name, void* dst, size_t dst_bytes, int* dtype,
+                               size_t* count);
+
+/* Arguments of SDL::LST<Acc>::run (SDL/LST.h:29-48); see_hitIdx is flattened
+ * with n_seeds + 1 offsets. */
+typedef struct {
+  const float *see_px, *see_py, *see_pz, *see_dxy, *see_dz, *see_ptErr, *see_etaErr;
+  const float *see_stateTrajGlbX, *see_stateTrajGlbY, *see_stateTrajGlbZ;
+  const float *see_stateTrajGlbPx, *see_stateTrajGlbPy, *see_stateTrajGlbPz;
+  const int32_t* see_q;
+  const int32_t* see_hitIdx;
+  const uint32_t* see_hitIdx_offsets;
+  uint32_t n_seeds;
+  const uint32_t* ph2_detId;
+  const float *ph2_x, *ph2_y, *ph2_z;
+  uint32_t n_hits;
+} lst_input;
+
+/* Replaces SDL::LST<Acc>::run (SDL/LST.cc:12-161) up to and including
+ * `last_stage` (0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,
+ * 4 triplets): prepareInput on the host (SDL/LST.cc:175-365), then the stage
+ * sequence on the device. */
+lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LST_AMD_H */

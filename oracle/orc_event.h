@@ -1,0 +1,87 @@
+// ORACLE — test infrastructure only (see orc_common.h).
+// Per-event SoA state of the serial restatement.  Field names and layouts
+// follow the reference structs (SDL/Hit.h:13-56, SDL/Module.h:21-85,
+// SDL/MiniDoublet.h:16-190, SDL/Segment.h:17-175, SDL/Triplet.h:17-151,
+// SDL/Quintuplet.h:20-141, SDL/PixelTriplet.h:20-132,1871-1960,
+// SDL/TrackCandidate.h:20-109) so tests can compare them by name.
+#pragma once
+#include "orc_common.h"
+
+namespace orc {
+
+struct Config {
+  float ptCut = 0.8f;    // SDL/Constants.h:102-105 (PT_CUT)
+  int t5Mode = 0;        // 0: USE_RZCHI2+USE_RPHICHI2 ("plain"), 1: USE_RZCHI2+USE_T5_DNN
+  bool tcPlsTriplets = false;   // TC_PLS_TRIPLETS
+  bool noPlsDupClean = false;   // NOPLSDUPCLEAN
+};
+
+struct Input {  // what SDL::Event::addHitToEvent/addPixelSegmentToEvent receive
+  std::vector<float> x, y, z;
+  std::vector<unsigned> detId, idx;
+  std::vector<unsigned> hitIdx0, hitIdx1, hitIdx2, hitIdx3;
+  std::vector<float> dPhiChange, ptIn, ptErr, px, py, pz, eta, etaErr, phi;
+  std::vector<int> charge, superbin;
+  std::vector<unsigned> seedIdx;
+  std::vector<int8_t> pixelType;
+  std::vector<char> isQuad;
+};
+
+struct Event {
+  const Geometry* g = nullptr;
+  Config cfg;
+  // hits
+  unsigned nHits = 0;
+  std::vector<float> xs, ys, zs, rts, phis, etas, highEdgeXs, highEdgeYs, lowEdgeXs, lowEdgeYs;
+  std::vector<unsigned> detid, idxs;
+  std::vector<uint16_t> hitModuleIndices;
+  std::vector<int> hitRanges, hitRangesLower, hitRangesUpper;
+  std::vector<int8_t> hitRangesnLower, hitRangesnUpper;
+  // object ranges
+  std::vector<int> mdRanges, segmentRanges, tripletRanges, quintupletRanges;
+  std::vector<int> miniDoubletModuleIndices, miniDoubletModuleOccupancy;
+  std::vector<int> segmentModuleIndices, segmentModuleOccupancy;
+  std::vector<int> tripletModuleIndices, tripletModuleOccupancy;
+  std::vector<int> quintupletModuleIndices, quintupletModuleOccupancy;
+  uint16_t nEligibleT5Modules = 0;
+  std::vector<uint16_t> indicesOfEligibleT5Modules;
+  unsigned nTotalMDs = 0, nTotalSegs = 0, nTotalTrips = 0, nTotalQuints = 0;
+  // mini-doublets
+  unsigned mdCapacity = 0;
+  std::vector<unsigned> nMDs, totOccupancyMDs, mdAnchorHitIndices, mdOuterHitIndices;
+  std::vector<uint16_t> mdModuleIndices;
+  std::vector<float> mdDphichanges, mdDzs, mdDphis, mdShiftedXs, mdShiftedYs, mdShiftedZs,
+      mdNoShiftedDzs, mdNoShiftedDphis, mdNoShiftedDphiChanges;
+  std::vector<float> anchorX, anchorY, anchorZ, anchorRt, anchorPhi, anchorEta, anchorHighEdgeX,
+      anchorHighEdgeY, anchorLowEdgeX, anchorLowEdgeY, anchorHighEdgePhi, anchorLowEdgePhi;
+  std::vector<float> outerX, outerY, outerZ, outerRt, outerPhi, outerEta, outerHighEdgeX,
+      outerHighEdgeY, outerLowEdgeX, outerLowEdgeY;
+  // segments (OT + pLS)
+  unsigned segCapacity = 0, nPLS = 0;
+  std::vector<unsigned> nSegments, totOccupancySegments, segMdIndices, segInnerAnchorHit, segOuterAnchorHit;
+  std::vector<uint16_t> segInnerLowerModule, segOuterLowerModule;
+  std::vector<float> segDPhis, segDPhiMins, segDPhiMaxs, segDPhiChanges, segDPhiChangeMins, segDPhiChangeMaxs;
+  // pLS-only arrays, indexed by pLS number
+  std::vector<unsigned> plsSeedIdx, plsHitsIdxs;  // plsHitsIdxs: 4 per pLS (x,y,z,w)
+  std::vector<int> plsCharge, plsSuperbin;
+  std::vector<int8_t> plsPixelType;
+  std::vector<char> plsIsQuad, plsIsDup;
+  std::vector<uint8_t> plsPartOfPT5;
+  std::vector<float> plsPtIn, plsPtErr, plsPx, plsPy, plsPz, plsEtaErr, plsEta, plsPhi, plsScore,
+      plsCircleCenterX, plsCircleCenterY, plsCircleRadius;
+  // triplets
+  unsigned t3Capacity = 0;
+  std::vector<unsigned> nTriplets, totOccupancyTriplets, t3SegmentIndices, t3HitIndices;
+  std::vector<uint16_t> t3LowerModuleIndices;
+  std::vector<uint8_t> t3LogicalLayers, t3PartOfPT5, t3PartOfT5, t3PartOfPT3;
+  std::vector<float> t3BetaIn, t3CircleRadius, t3CircleCenterX, t3CircleCenterY;
+};
+
+bool event_add_hits(Event& ev, const Input& in);
+void event_add_pixel_segments(Event& ev, const Input& in);
+void event_create_mini_doublets(Event& ev);
+void event_create_segments(Event& ev);
+void event_create_triplets(Event& ev);
+float computeRadiusFromThreeAnchorHits(float x1, float y1, float x2, float y2, float x3, float y3, float& g, float& f);
+
+}  // namespace orc

@@ -1,0 +1,125 @@
+"""Stage-by-stage comparison of the HIP engine against the oracle.
+
+Only the slots the reference defines are compared: for every module, objects
+[moduleIndices[m], moduleIndices[m] + n[m]).  Integer fields must be identical;
+float fields must agree within the 1e-5 relative tolerance the engine is held to
+(in practice they are bit-identical, and the report counts the exceptions).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FLOAT_RTOL = 1e-5
+
+HIT_FIELDS = ["xs", "ys", "zs", "moduleIndices", "idxs", "detid", "rts", "phis", "etas", "highEdgeXs", "highEdgeYs",
+              "lowEdgeXs", "lowEdgeYs"]
+MD_FIELDS = ["anchorHitIndices", "outerHitIndices", "moduleIndices", "dphichanges", "dzs", "dphis", "shiftedXs",
+             "shiftedYs", "shiftedZs", "noShiftedDzs", "noShiftedDphis", "noShiftedDphiChanges", "anchorX", "anchorY",
+             "anchorZ", "anchorRt", "anchorPhi", "anchorEta", "anchorHighEdgeX", "anchorHighEdgeY", "anchorLowEdgeX",
+             "anchorLowEdgeY", "anchorHighEdgePhi", "anchorLowEdgePhi", "outerX", "outerY", "outerZ", "outerRt",
+             "outerPhi", "outerEta", "outerHighEdgeX", "outerHighEdgeY", "outerLowEdgeX", "outerLowEdgeY"]
+SEG_FIELDS = {"mdIndices": 2, "innerLowerModuleIndices": 1, "outerLowerModuleIndices": 1,
+              "innerMiniDoubletAnchorHitIndices": 1, "outerMiniDoubletAnchorHitIndices": 1, "dPhis": 1,
+              "dPhiMins": 1, "dPhiMaxs": 1, "dPhiChanges": 1, "dPhiChangeMins": 1, "dPhiChangeMaxs": 1}
+PLS_FIELDS = {"seedIdx": 1, "pLSHitsIdxs": 4, "charge": 1, "superbin": 1, "pixelType": 1, "isQuad": 1, "ptIn": 1,
+              "ptErr": 1, "px": 1, "py": 1, "pz": 1, "etaErr": 1, "eta": 1, "phi": 1, "score": 1,
+              "circleCenterX": 1, "circleCenterY": 1, "circleRadius": 1}
+T3_FIELDS = {"segmentIndices": 2, "lowerModuleIndices": 3, "logicalLayers": 3, "hitIndices": 6, "betaIn": 1,
+             "circleRadius": 1, "circleCenterX": 1, "circleCenterY": 1}
+
+
+class Report:
+    def __init__(self):
+        self.lines = []
+        self.float_inexact = 0
+        self.float_compared = 0
+
+    def check(self, name, a, b):
+        a = np.asarray(a)
+        b = np.asarray(b)
+        assert a.shape == b.shape, f"{name}: shape {a.shape} vs {b.shape}"
+        if a.dtype.kind == "f":
+            self.float_compared += a.size
+            same = (a == b) | (np.isnan(a) & np.isnan(b))
+            n_bad = int((~same).sum())
+            self.float_inexact += n_bad
+            if n_bad:
+                denom = np.maximum(np.abs(b), 1e-30)
+                rel = np.where(same, 0.0, np.abs(a.astype(np.float64) - b) / denom)
+                worst = float(np.nanmax(rel))
+                self.lines.append(f"{name}: {n_bad} of {a.size} not bit-identical, worst rel {worst:.3g}")
+                assert worst <= FLOAT_RTOL, f"{name}: relative difference {worst:.3g} > {FLOAT_RTOL}"
+        else:
+            bad = np.nonzero(a != b)[0]
+            assert len(bad) == 0, f"{name}: {len(bad)} of {a.size} differ; first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+
+
+def slots(start, count, mult=1):
+    """Flat element indices of objects [start, start+count) with `mult` entries each."""
+    start = np.asarray(start, np.int64)
+    count = np.asarray(count, np.int64)
+    keep = count > 0
+    start, count = start[keep], count[keep]
+    if len(start) == 0:
+        return np.zeros(0, np.int64)
+    obj = np.repeat(start - np.concatenate([[0], np.cumsum(count)[:-1]]), count) + np.arange(count.sum())
+    return (obj[:, None] * mult + np.arange(mult)[None, :]).ravel()
+
+
+def compare(prod, orc, upto: str, report: Report | None = None, n_lower: int = 13200) -> Report:
+    """Compare every stage up to `upto` ("hits", "pls", "md", "ls", "t3")."""
+    r = report or Report()
+    order = ["hits", "pls", "md", "ls", "t3"]
+    last = order.index(upto)
+    n_hits = orc.scalar("nHits")
+    n_pls = orc.scalar("nPLS")
+    for f in HIT_FIELDS:
+        r.check("hits." + f, prod.field("hits." + f)[:n_hits], orc.field("hits." + f)[:n_hits])
+    for f in ("hitRanges", "hitRangesLower", "hitRangesUpper", "hitRangesnLower", "hitRangesnUpper"):
+        r.check("hits." + f, prod.field("hits." + f), orc.field("hits." + f))
+    if last >= 1:
+        for f in ("miniDoubletModuleIndices", "miniDoubletModuleOccupancy", "segmentModuleIndices",
+                  "segmentModuleOccupancy"):
+            r.check("ranges." + f, prod.field("ranges." + f), orc.field("ranges." + f))
+        for f, m in PLS_FIELDS.items():
+            r.check("segs." + f, prod.field("segs." + f)[:n_pls * m], orc.field("segs." + f)[:n_pls * m])
+        pix = n_lower
+        md0 = orc.field("ranges.miniDoubletModuleIndices")[pix]
+        s0 = orc.field("ranges.segmentModuleIndices")[pix]
+        idx = np.arange(md0, md0 + 2 * n_pls)
+        for f in MD_FIELDS:
+            r.check("mds(pixel)." + f, prod.field("mds." + f)[idx], orc.field("mds." + f)[idx])
+        for f, m in SEG_FIELDS.items():
+            if f in ("dPhis", "dPhiMins", "dPhiMaxs", "dPhiChangeMins", "dPhiChangeMaxs"):
+                continue  # not written for pLS by the reference (SDL/Segment.h:392-411)
+            sl = slots([s0], [n_pls], m)
+            r.check("segs(pixel)." + f, prod.field("segs." + f)[sl], orc.field("segs." + f)[sl])
+    if last >= 2:
+        n = orc.field("mds.nMDs")
+        r.check("mds.nMDs", prod.field("mds.nMDs"), n)
+        r.check("mds.totOccupancyMDs", prod.field("mds.totOccupancyMDs"), orc.field("mds.totOccupancyMDs"))
+        r.check("ranges.mdRanges", prod.field("ranges.mdRanges"), orc.field("ranges.mdRanges"))
+        sl = slots(orc.field("ranges.miniDoubletModuleIndices")[:n_lower], n[:n_lower])
+        for f in MD_FIELDS:
+            r.check("mds." + f, prod.field("mds." + f)[sl], orc.field("mds." + f)[sl])
+    if last >= 3:
+        n = orc.field("segs.nSegments")
+        r.check("segs.nSegments", prod.field("segs.nSegments"), n)
+        r.check("segs.totOccupancySegments", prod.field("segs.totOccupancySegments"),
+                orc.field("segs.totOccupancySegments"))
+        r.check("ranges.segmentRanges", prod.field("ranges.segmentRanges"), orc.field("ranges.segmentRanges"))
+        start = orc.field("ranges.segmentModuleIndices")[:n_lower]
+        for f, m in SEG_FIELDS.items():
+            sl = slots(start, n[:n_lower], m)
+            r.check("segs." + f, prod.field("segs." + f)[sl], orc.field("segs." + f)[sl])
+    if last >= 4:
+        n = orc.field("t3.nTriplets")
+        r.check("t3.nTriplets", prod.field("t3.nTriplets"), n)
+        r.check("t3.totOccupancyTriplets", prod.field("t3.totOccupancyTriplets"), orc.field("t3.totOccupancyTriplets"))
+        for f in ("tripletModuleIndices", "tripletModuleOccupancy", "tripletRanges"):
+            r.check("ranges." + f, prod.field("ranges." + f), orc.field("ranges." + f))
+        start = orc.field("ranges.tripletModuleIndices")
+        for f, m in T3_FIELDS.items():
+            sl = slots(start, n, m)
+            r.check("t3." + f, prod.field("t3." + f)[sl], orc.field("t3." + f)[sl])
+    return r

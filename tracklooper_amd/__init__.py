@@ -1,0 +1,198 @@
+"""MI355X-native Line Segment Tracking engine.
+
+Python mirror of the reference host interface for the LST path
+(``SDL::loadAndFillES*``, ``SDL::Event``, ``SDL::LST::run``), calling the HIP
+engine in ``_lib/liblst_amd.so`` through its C ABI (``include/lst_amd.h``).
+There is no CPU fallback: importing works anywhere, but creating an event
+setup requires the built library and a GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import synth  # noqa: F401  (seeded synthetic events)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "liblst_amd.so")
+DEFAULT_GEOMETRY = os.path.join(HERE, "data", "OT800_IT615_pt0.8")
+
+_DT = {0: np.float32, 1: np.uint32, 2: np.int32, 3: np.uint16, 4: np.int16, 5: np.uint8, 6: np.int8}
+STAGES = {"hits": 0, "pls": 1, "md": 2, "ls": 3, "t3": 4}
+LST_T5_PLAIN_CHI2, LST_T5_DNN = 0, 1
+_lib = None
+
+
+class LSTError(RuntimeError):
+    pass
+
+
+class _PixelSegments(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "hit_indices0", "hit_indices1", "hit_indices2", "hit_indices3", "delta_phi_change", "pt_in", "pt_err",
+        "px", "py", "pz", "eta", "eta_err", "phi", "charge", "seed_idx", "superbin", "pixel_type", "is_quad")] + [
+        ("n_pls", C.c_uint32)]
+
+
+class _Input(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr", "see_stateTrajGlbX",
+        "see_stateTrajGlbY", "see_stateTrajGlbZ", "see_stateTrajGlbPx", "see_stateTrajGlbPy", "see_stateTrajGlbPz",
+        "see_q", "see_hitIdx", "see_hitIdx_offsets")] + [("n_seeds", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")] + [("n_hits", C.c_uint32)]
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load liblst_amd.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LSTError(f"{LIB_PATH} is missing: run `python -m tracklooper_amd.build` (HIP, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.lst_last_error.restype = C.c_char_p
+        L.lst_es_create.argtypes = [C.c_char_p, C.c_float, C.c_uint32, C.POINTER(P)]
+        L.lst_es_destroy.argtypes = [P]
+        L.lst_es_scalar.argtypes = [P, C.c_char_p]
+        L.lst_es_scalar.restype = C.c_uint32
+        L.lst_event_create.argtypes = [P, P, C.c_int, C.POINTER(P)]
+        L.lst_event_destroy.argtypes = [P]
+        L.lst_event_add_hits.argtypes = [P, P, P, P, P, P, C.c_uint32, C.c_int]
+        L.lst_event_add_pixel_segments.argtypes = [P, C.POINTER(_PixelSegments), C.c_int]
+        for n in ("create_mini_doublets", "create_segments", "create_triplets", "reset", "synchronize"):
+            getattr(L, "lst_event_" + n).argtypes = [P]
+        L.lst_event_count.argtypes = [P, C.c_char_p]
+        L.lst_event_count.restype = C.c_uint32
+        L.lst_event_get_field.argtypes = [P, C.c_char_p, P, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
+        L.lst_run.argtypes = [P, C.POINTER(_Input), C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status & 0xFF:
+        raise LSTError(f"lst status {status}: {lib().lst_last_error().decode()}")
+    return status
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class LSTESData:
+    """Geometry resident in HBM (``SDL::loadAndFillESDevice``, SDL/LSTESData.cc:87-113)."""
+
+    def __init__(self, geom_dir: str = DEFAULT_GEOMETRY, pt_cut: float = 0.8, module_map_stride: int = 0):
+        self.h = C.c_void_p()
+        _check(lib().lst_es_create(geom_dir.encode(), pt_cut, module_map_stride, C.byref(self.h)))
+        self.geom_dir = geom_dir
+        self.pt_cut = pt_cut
+
+    def scalar(self, name: str) -> int:
+        return lib().lst_es_scalar(self.h, name.encode())
+
+    def close(self):
+        if self.h:
+            lib().lst_es_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    """``SDL::Event`` (SDL/Event.h:26-205) backed by the HIP engine."""
+
+    def __init__(self, es: LSTESData, stream: int | None = None, t5_mode: int = LST_T5_PLAIN_CHI2):
+        self.es = es
+        self.h = C.c_void_p()
+        _check(lib().lst_event_create(es.h, stream, t5_mode, C.byref(self.h)))
+        self._keep = []
+
+    # --- SDL::Event stage API -------------------------------------------------
+    def addHitToEvent(self, x, y, z, detId, idxInNtuple):
+        arrs = [np.ascontiguousarray(a, t) for a, t in ((x, np.float32), (y, np.float32), (z, np.float32),
+                                                        (detId, np.uint32), (idxInNtuple, np.uint32))]
+        self._keep = arrs
+        _check(lib().lst_event_add_hits(self.h, *[_p(a) for a in arrs], len(arrs[0]), 0))
+
+    def addPixelSegmentToEvent(self, hitIndices0, hitIndices1, hitIndices2, hitIndices3, dPhiChange, ptIn, ptErr,
+                               px, py, pz, eta, etaErr, phi, charge, seedIdx, superbin, pixelType, isQuad):
+        types = [np.uint32] * 4 + [np.float32] * 9 + [np.int32, np.uint32, np.int32, np.int8, np.int8]
+        vals = [hitIndices0, hitIndices1, hitIndices2, hitIndices3, dPhiChange, ptIn, ptErr, px, py, pz, eta, etaErr,
+                phi, charge, seedIdx, superbin, pixelType, isQuad]
+        arrs = [np.ascontiguousarray(a, t) for a, t in zip(vals, types)]
+        self._keep = arrs
+        ps = _PixelSegments(*[_p(a) for a in arrs], len(arrs[5]))
+        return _check(lib().lst_event_add_pixel_segments(self.h, C.byref(ps), 0))
+
+    def createMiniDoublets(self):
+        _check(lib().lst_event_create_mini_doublets(self.h))
+
+    def createSegmentsWithModuleMap(self):
+        _check(lib().lst_event_create_segments(self.h))
+
+    def createTriplets(self):
+        _check(lib().lst_event_create_triplets(self.h))
+
+    def resetEvent(self):
+        _check(lib().lst_event_reset(self.h))
+
+    def synchronize(self):
+        _check(lib().lst_event_synchronize(self.h))
+
+    def getNumberOfMiniDoublets(self):
+        return lib().lst_event_count(self.h, b"miniDoublets")
+
+    def getNumberOfSegments(self):
+        return lib().lst_event_count(self.h, b"segments")
+
+    def getNumberOfTriplets(self):
+        return lib().lst_event_count(self.h, b"triplets")
+
+    def getNumberOfPixelSegments(self):
+        return lib().lst_event_count(self.h, b"pixelSegments")
+
+    # --- host copies of SoA fields (SDL::Event::get*, SDL/Event.h:185-204) -----
+    def field(self, name: str) -> np.ndarray:
+        dt, n = C.c_int(), C.c_size_t()
+        _check(lib().lst_event_get_field(self.h, name.encode(), None, 0, C.byref(dt), C.byref(n)))
+        out = np.empty(n.value, _DT[dt.value])
+        _check(lib().lst_event_get_field(self.h, name.encode(), _p(out), out.nbytes, C.byref(dt), C.byref(n)))
+        return out
+
+    # --- SDL::LST::run ----------------------------------------------------------
+    def run_lst(self, ev: dict, upto: str = "t3"):
+        """``SDL::LST::run`` (SDL/LST.cc:12-161) on one synthetic/real event dict."""
+        a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
+        self._keep = list(a.values())
+        inp = _Input(
+            *[_p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
+                                 "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ",
+                                 "see_stateTrajGlbPx", "see_stateTrajGlbPy", "see_stateTrajGlbPz", "see_q",
+                                 "see_hitIdx", "see_hitIdx_off")],
+            len(a["see_px"]),
+            *[_p(a[k]) for k in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")],
+            len(a["ph2_x"]))
+        _check(lib().lst_run(self.h, C.byref(inp), STAGES[upto]))
+        return self
+
+    def close(self):
+        if self.h:
+            lib().lst_event_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,440 @@
+// Event orchestration and the C ABI of the LST engine.
+//
+// Device memory: one arena per event handle, carved once into the SoA of every
+// object kind at reference capacities and reused by every event run on the
+// handle (the reference allocates and frees ~0.4 GB per event,
+// SDL/Event.cc:45-149).  No stage makes a host round trip: capacities that the
+// reference reads back (SDL/Event.cc:252-320,520-541) are computed on the
+// device or fixed by the geometry.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lst_amd.h"
+#include "es.h"
+#include "prepare.h"
+#include "stages.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+lst_status fail(lst_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_OK(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return fail(LST_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+enum DType { F32 = 0, U32 = 1, I32 = 2, U16 = 3, I16 = 4, U8 = 5, I8 = 6 };
+template <class T> constexpr int dtype_of();
+template <> constexpr int dtype_of<float>() { return F32; }
+template <> constexpr int dtype_of<uint32_t>() { return U32; }
+template <> constexpr int dtype_of<int32_t>() { return I32; }
+template <> constexpr int dtype_of<uint16_t>() { return U16; }
+template <> constexpr int dtype_of<int16_t>() { return I16; }
+template <> constexpr int dtype_of<uint8_t>() { return U8; }
+template <> constexpr int dtype_of<int8_t>() { return I8; }
+
+struct FieldRef {
+  void* ptr;
+  int dtype;
+  size_t elem;
+  size_t count;
+};
+
+// Bump allocator over one device allocation; 256-byte aligned sub-buffers.
+struct Carve {
+  size_t off = 0;
+  std::vector<std::pair<void**, size_t>> todo;
+  template <class T>
+  void take(T*& p, size_t n) {
+    todo.push_back({reinterpret_cast<void**>(&p), off});
+    off += (n * sizeof(T) + 255) & ~size_t(255);
+  }
+  void bind(char* base) {
+    for (auto& t : todo) *t.first = base + t.second;
+  }
+};
+
+}  // namespace
+
+struct lst_es {
+  lst::HostGeometry host;
+  lst::DeviceGeometry dev;
+  float ptCut = 0.8f;
+};
+
+struct lst_event {
+  lst_es* es = nullptr;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  int t5Mode = 0;
+  int stage = -1;  // last completed stage
+  lst::EventView v{};
+  // capacities
+  uint32_t hitCap = 0, mdCap = 0, segCap = 0, plsCap = lst::kNMaxPixelSegments, t3Cap = 0;
+  char* arena = nullptr;
+  size_t arenaBytes = 0;
+  uint32_t* plsHitIdx[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* plsDPhiChange = nullptr;
+  std::vector<std::pair<std::string, FieldRef>> fields;
+  lst::PreparedEvent prepared;
+
+  lst_status allocate(uint32_t nHitsNeeded);
+  lst_status add_hits(const float* x, const float* y, const float* z, const uint32_t* det, const uint32_t* idx,
+                      uint32_t n, bool onDevice);
+  lst_status add_pixel_segments(const lst_pixel_segments& p, bool onDevice);
+  const FieldRef* field(const char* name);
+};
+
+lst_status lst_event::allocate(uint32_t nHitsNeeded) {
+  if (arena && nHitsNeeded <= hitCap) return LST_OK;
+  if (arena) {
+    HIP_OK(hipStreamSynchronize(stream));
+    HIP_OK(hipFree(arena));
+    arena = nullptr;
+  }
+  const lst::DeviceGeometry& g = es->dev;
+  const uint32_t nMod = g.view.nModules, nL = g.view.nLowerModules;
+  hitCap = nHitsNeeded < (1u << 18) ? (1u << 18) : nHitsNeeded + nHitsNeeded / 4;
+  mdCap = g.nTotalMDCap + lst::kNMaxPixelMDs;
+  segCap = g.nTotalSegCap + lst::kNMaxPixelSegments;
+  t3Cap = (uint32_t)g.t3CapBound;
+  Carve c;
+  lst::EventView& e = v;
+#define CARVE(TY, NM, MU) c.take(e.H.NM, (size_t)hitCap * MU);
+  LST_HIT_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.D.NM, (size_t)mdCap * MU);
+  LST_MD_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.S.NM, (size_t)segCap * MU);
+  LST_SEG_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.P.NM, (size_t)plsCap * MU);
+  LST_PLS_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.T.NM, (size_t)(t3Cap ? t3Cap : 1) * MU);
+  LST_T3_FIELDS(CARVE)
+#undef CARVE
+  c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
+  c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);
+  c.take(e.R.mdRanges, 2 * nMod); c.take(e.R.segmentRanges, 2 * nMod); c.take(e.R.tripletRanges, 2 * nMod);
+  c.take(e.R.quintupletRanges, 2 * nMod);
+  c.take(e.R.tripletModuleIndices, nL); c.take(e.R.tripletModuleOccupancy, nL);
+  c.take(e.R.nMDs, nL + 1); c.take(e.R.totOccupancyMDs, nL + 1);
+  c.take(e.R.nSegments, nL + 1); c.take(e.R.totOccupancySegments, nL + 1);
+  c.take(e.R.nTriplets, nL); c.take(e.R.totOccupancyTriplets, nL);
+  c.take(e.R.scalars, lst::kNScalars);
+  for (int k = 0; k < 4; k++) c.take(plsHitIdx[k], plsCap);
+  c.take(plsDPhiChange, plsCap);
+  arenaBytes = c.off;
+  HIP_OK(hipMalloc(&arena, arenaBytes));
+  c.bind(arena);
+  e.M = g.view;
+  e.mdModuleIndices = g.mdIndices;
+  e.mdModuleOccupancy = g.mdOccupancy;
+  e.segModuleIndices = g.segIndices;
+  e.segModuleOccupancy = g.segOccupancy;
+  e.t3OccIfAny = g.t3OccIfAny;
+  e.pixelModule = g.pixelModuleIndex;
+  e.ptCut = es->ptCut;
+
+  // by-name field table (host getters)
+  fields.clear();
+  auto add = [&](const std::string& name, auto* p, size_t n) {
+    using T = std::remove_pointer_t<decltype(p)>;
+    fields.push_back({name, FieldRef{(void*)p, dtype_of<T>(), sizeof(T), n}});
+  };
+#define REG(PFX, OBJ, CAP) [&] { (void)0; }
+#define FIELD(TY, NM, MU) add(prefix + #NM, cur->NM, capacity * MU);
+  {
+    std::string prefix = "hits.";
+    size_t capacity = hitCap;
+    auto* cur = &e.H;
+    LST_HIT_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "mds.";
+    size_t capacity = mdCap;
+    auto* cur = &e.D;
+    LST_MD_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "segs.";
+    size_t capacity = segCap;
+    auto* cur = &e.S;
+    LST_SEG_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "segs.";
+    size_t capacity = plsCap;
+    auto* cur = &e.P;
+    LST_PLS_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "t3.";
+    size_t capacity = t3Cap;
+    auto* cur = &e.T;
+    LST_T3_FIELDS(FIELD)
+  }
+#undef FIELD
+#undef REG
+  add("hits.hitRanges", e.R.hitRanges, 2 * nMod);
+  add("hits.hitRangesLower", e.R.hitRangesLower, nMod);
+  add("hits.hitRangesUpper", e.R.hitRangesUpper, nMod);
+  add("hits.hitRangesnLower", e.R.hitRangesnLower, nMod);
+  add("hits.hitRangesnUpper", e.R.hitRangesnUpper, nMod);
+  add("ranges.mdRanges", e.R.mdRanges, 2 * nMod);
+  add("ranges.segmentRanges", e.R.segmentRanges, 2 * nMod);
+  add("ranges.tripletRanges", e.R.tripletRanges, 2 * nMod);
+  add("ranges.quintupletRanges", e.R.quintupletRanges, 2 * nMod);
+  add("ranges.miniDoubletModuleIndices", const_cast<int32_t*>(g.mdIndices), nL + 1);
+  add("ranges.miniDoubletModuleOccupancy", const_cast<int32_t*>(g.mdOccupancy), nL + 1);
+  add("ranges.segmentModuleIndices", const_cast<int32_t*>(g.segIndices), nL + 1);
+  add("ranges.segmentModuleOccupancy", const_cast<int32_t*>(g.segOccupancy), nL + 1);
+  add("ranges.tripletModuleIndices", e.R.tripletModuleIndices, nL);
+  add("ranges.tripletModuleOccupancy", e.R.tripletModuleOccupancy, nL);
+  add("mds.nMDs", e.R.nMDs, nL + 1);
+  add("mds.totOccupancyMDs", e.R.totOccupancyMDs, nL + 1);
+  add("segs.nSegments", e.R.nSegments, nL + 1);
+  add("segs.totOccupancySegments", e.R.totOccupancySegments, nL + 1);
+  add("t3.nTriplets", e.R.nTriplets, nL);
+  add("t3.totOccupancyTriplets", e.R.totOccupancyTriplets, nL);
+  add("scalars", e.R.scalars, lst::kNScalars);
+  return LST_OK;
+}
+
+const FieldRef* lst_event::field(const char* name) {
+  for (auto& f : fields)
+    if (f.first == name) return &f.second;
+  return nullptr;
+}
+
+lst_status lst_event::add_hits(const float* x, const float* y, const float* z, const uint32_t* det,
+                               const uint32_t* idx, uint32_t n, bool onDevice) {
+  lst_status s = allocate(n);
+  if (s != LST_OK) return s;
+  const hipMemcpyKind k = onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (n) {
+    HIP_OK(hipMemcpyAsync(v.H.xs, x, n * sizeof(float), k, stream));
+    HIP_OK(hipMemcpyAsync(v.H.ys, y, n * sizeof(float), k, stream));
+    HIP_OK(hipMemcpyAsync(v.H.zs, z, n * sizeof(float), k, stream));
+    HIP_OK(hipMemcpyAsync(v.H.detid, det, n * sizeof(uint32_t), k, stream));
+    HIP_OK(hipMemcpyAsync(v.H.idxs, idx, n * sizeof(uint32_t), k, stream));
+  }
+  v.nHits = n;
+  v.nPLS = 0;
+  lst::launch_hits(v, stream);
+  HIP_OK(hipGetLastError());
+  stage = 0;
+  return LST_OK;
+}
+
+lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDevice) {
+  if (stage != 0) return fail(LST_E_ORDER, "addPixelSegmentToEvent must follow addHitToEvent");
+  uint32_t n = p.n_pls;
+  lst_status warn = LST_OK;
+  if (n > lst::kNMaxPixelSegments) {  // SDL/Event.cc:237-244
+    n = lst::kNMaxPixelSegments;
+    warn = LST_W_TRUNCATED;
+  }
+  const hipMemcpyKind k = onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (n) {
+    HIP_OK(hipMemcpyAsync(plsHitIdx[0], p.hit_indices0, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(plsHitIdx[1], p.hit_indices1, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(plsHitIdx[2], p.hit_indices2, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(plsHitIdx[3], p.hit_indices3, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(plsDPhiChange, p.delta_phi_change, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.ptIn, p.pt_in, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.ptErr, p.pt_err, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.px, p.px, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.py, p.py, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.pz, p.pz, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.eta, p.eta, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.etaErr, p.eta_err, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.phi, p.phi, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.charge, p.charge, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.seedIdx, p.seed_idx, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.superbin, p.superbin, n * 4, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.pixelType, p.pixel_type, n, k, stream));
+    HIP_OK(hipMemcpyAsync(v.P.isQuad, p.is_quad, n, k, stream));
+  }
+  v.nPLS = n;
+  const uint32_t* hi[4] = {plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]};
+  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream);
+  HIP_OK(hipGetLastError());
+  stage = 1;
+  return warn;
+}
+
+extern "C" {
+
+const char* lst_last_error(void) { return g_err.c_str(); }
+
+lst_status lst_es_create(const char* geom_dir, float pt_cut, uint32_t stride, lst_es** out) {
+  *out = nullptr;
+  auto es = std::make_unique<lst_es>();
+  std::string err;
+  if (!lst::load_host_geometry(geom_dir ? geom_dir : "", stride, es->host, err))
+    return fail(err.rfind("cannot read", 0) == 0 ? LST_E_IO : LST_E_FORMAT, err);
+  if (es->host.stride > 159) return fail(LST_E_FORMAT, "module fan-out above 159 is not supported");
+  if (!lst::upload_geometry(es->host, es->dev, err)) return fail(LST_E_DEVICE, err);
+  es->ptCut = pt_cut;
+  *out = es.release();
+  return LST_OK;
+}
+
+void lst_es_destroy(lst_es* es) {
+  if (!es) return;
+  lst::free_geometry(es->dev);
+  delete es;
+}
+
+uint32_t lst_es_scalar(const lst_es* es, const char* name) {
+  if (!es || !name) return 0;
+  const auto& h = es->host;
+  if (!std::strcmp(name, "nModules")) return h.nModules;
+  if (!std::strcmp(name, "nLowerModules")) return h.nLowerModules;
+  if (!std::strcmp(name, "nPixels")) return h.nPixels;
+  if (!std::strcmp(name, "pixelModuleIndex")) return h.pixelModuleIndex;
+  if (!std::strcmp(name, "stride")) return h.stride;
+  return 0;
+}
+
+lst_status lst_event_create(lst_es* es, void* hip_stream, int t5_mode, lst_event** out) {
+  *out = nullptr;
+  if (!es) return fail(LST_E_INPUT, "null event setup");
+  auto ev = std::make_unique<lst_event>();
+  ev->es = es;
+  ev->t5Mode = t5_mode;
+  if (hip_stream) {
+    ev->stream = static_cast<hipStream_t>(hip_stream);
+  } else {
+    HIP_OK(hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking));
+    ev->ownStream = true;
+  }
+  lst_status s = ev->allocate(0);
+  if (s != LST_OK) return s;
+  *out = ev.release();
+  return LST_OK;
+}
+
+void lst_event_destroy(lst_event* ev) {
+  if (!ev) return;
+  if (ev->stream) hipStreamSynchronize(ev->stream);
+  if (ev->arena) hipFree(ev->arena);
+  if (ev->ownStream) hipStreamDestroy(ev->stream);
+  delete ev;
+}
+
+lst_status lst_event_add_hits(lst_event* ev, const float* x, const float* y, const float* z, const uint32_t* det_id,
+                              const uint32_t* idx, uint32_t n, int on_device) {
+  if (!ev) return fail(LST_E_INPUT, "null event");
+  return ev->add_hits(x, y, z, det_id, idx, n, on_device != 0);
+}
+
+lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments* p, int on_device) {
+  if (!ev || !p) return fail(LST_E_INPUT, "null argument");
+  return ev->add_pixel_segments(*p, on_device != 0);
+}
+
+lst_status lst_event_create_mini_doublets(lst_event* ev) {
+  if (!ev || ev->stage != 1) return fail(LST_E_ORDER, "createMiniDoublets needs hits and pixel segments");
+  lst::launch_mini_doublets(ev->v, ev->stream);
+  HIP_OK(hipGetLastError());
+  ev->stage = 2;
+  return LST_OK;
+}
+
+lst_status lst_event_create_segments(lst_event* ev) {
+  if (!ev || ev->stage != 2) return fail(LST_E_ORDER, "createSegmentsWithModuleMap needs mini-doublets");
+  lst::launch_segments(ev->v, ev->stream);
+  HIP_OK(hipGetLastError());
+  ev->stage = 3;
+  return LST_OK;
+}
+
+lst_status lst_event_create_triplets(lst_event* ev) {
+  if (!ev || ev->stage != 3) return fail(LST_E_ORDER, "createTriplets needs segments");
+  lst::launch_triplets(ev->v, ev->stream);
+  HIP_OK(hipGetLastError());
+  ev->stage = 4;
+  return LST_OK;
+}
+
+lst_status lst_event_reset(lst_event* ev) {
+  if (!ev) return fail(LST_E_INPUT, "null event");
+  ev->stage = -1;
+  ev->v.nHits = 0;
+  ev->v.nPLS = 0;
+  return LST_OK;
+}
+
+lst_status lst_event_synchronize(lst_event* ev) {
+  if (!ev) return fail(LST_E_INPUT, "null event");
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  return LST_OK;
+}
+
+uint32_t lst_event_count(lst_event* ev, const char* what) {
+  if (!ev || !what) return 0;
+  if (!std::strcmp(what, "hits")) return ev->v.nHits;
+  if (!std::strcmp(what, "pixelSegments")) return ev->v.nPLS;
+  const uint32_t nL = ev->v.M.nLowerModules;
+  const char* key = nullptr;
+  uint32_t n = nL;
+  if (!std::strcmp(what, "miniDoublets")) key = "mds.nMDs";
+  else if (!std::strcmp(what, "segments")) key = "segs.nSegments";
+  else if (!std::strcmp(what, "triplets")) key = "t3.nTriplets";
+  if (!key) return 0;
+  const FieldRef* f = ev->field(key);
+  std::vector<uint32_t> h(n);
+  if (hipMemcpyAsync(h.data(), f->ptr, n * 4, hipMemcpyDeviceToHost, ev->stream) != hipSuccess) return 0;
+  if (hipStreamSynchronize(ev->stream) != hipSuccess) return 0;
+  uint64_t tot = 0;
+  for (auto x : h) tot += x;
+  return (uint32_t)tot;
+}
+
+lst_status lst_event_get_field(lst_event* ev, const char* name, void* dst, size_t dst_bytes, int* dtype,
+                               size_t* count) {
+  if (!ev || !name) return fail(LST_E_INPUT, "null argument");
+  const FieldRef* f = ev->field(name);
+  if (!f) return fail(LST_E_INPUT, std::string("unknown field ") + name);
+  if (dtype) *dtype = f->dtype;
+  if (count) *count = f->count;
+  if (!dst) return LST_OK;
+  const size_t bytes = f->count * f->elem;
+  if (dst_bytes < bytes) return fail(LST_E_CAPACITY, "destination too small");
+  HIP_OK(hipMemcpyAsync(dst, f->ptr, bytes, hipMemcpyDeviceToHost, ev->stream));
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  return LST_OK;
+}
+
+lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
+  if (!ev || !in) return fail(LST_E_INPUT, "null argument");
+  lst::prepare_event(*in, ev->prepared);
+  const auto& p = ev->prepared;
+  lst_status s = ev->add_hits(p.x.data(), p.y.data(), p.z.data(), p.detId.data(), p.idx.data(),
+                              (uint32_t)p.x.size(), false);
+  if (s != LST_OK || last_stage < 1) return s;
+  const lst_pixel_segments pv = p.view();
+  lst_status warn = ev->add_pixel_segments(pv, false);
+  if ((warn & 0xff) != LST_OK || last_stage < 2) return warn;
+  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_triplets(ev)) != LST_OK) return s;
+  return warn;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Field lists of the per-event device SoA.  One X-macro list per object kind
+// drives the struct members, the arena carving and the by-name getters, so the
+// three can never disagree.  Names follow the reference structs
+// (SDL/Hit.h:13-56, SDL/MiniDoublet.h:16-53, SDL/Segment.h:17-50,
+// SDL/Triplet.h:17-31) so tests compare fields by name against the oracle.
+#pragma once
+#include <cstdint>
+
+// X(type, name, per-object multiplicity)
+#define LST_HIT_FIELDS(X)                                                                                  \
+  X(float, xs, 1) X(float, ys, 1) X(float, zs, 1) X(uint16_t, moduleIndices, 1) X(uint32_t, idxs, 1)      \
+  X(uint32_t, detid, 1) X(float, rts, 1) X(float, phis, 1) X(float, etas, 1) X(float, highEdgeXs, 1)      \
+  X(float, highEdgeYs, 1) X(float, lowEdgeXs, 1) X(float, lowEdgeYs, 1)
+
+#define LST_MD_FIELDS(X)                                                                                   \
+  X(uint32_t, anchorHitIndices, 1) X(uint32_t, outerHitIndices, 1) X(uint16_t, moduleIndices, 1)           \
+  X(float, dphichanges, 1) X(float, dzs, 1) X(float, dphis, 1) X(float, shiftedXs, 1)                      \
+  X(float, shiftedYs, 1) X(float, shiftedZs, 1) X(float, noShiftedDzs, 1) X(float, noShiftedDphis, 1)      \
+  X(float, noShiftedDphiChanges, 1) X(float, anchorX, 1) X(float, anchorY, 1) X(float, anchorZ, 1)         \
+  X(float, anchorRt, 1) X(float, anchorPhi, 1) X(float, anchorEta, 1) X(float, anchorHighEdgeX, 1)         \
+  X(float, anchorHighEdgeY, 1) X(float, anchorLowEdgeX, 1) X(float, anchorLowEdgeY, 1)                     \
+  X(float, anchorHighEdgePhi, 1) X(float, anchorLowEdgePhi, 1) X(float, outerX, 1) X(float, outerY, 1)     \
+  X(float, outerZ, 1) X(float, outerRt, 1) X(float, outerPhi, 1) X(float, outerEta, 1)                     \
+  X(float, outerHighEdgeX, 1) X(float, outerHighEdgeY, 1) X(float, outerLowEdgeX, 1)                       \
+  X(float, outerLowEdgeY, 1)
+
+#define LST_SEG_FIELDS(X)                                                                                  \
+  X(uint32_t, mdIndices, 2) X(uint16_t, innerLowerModuleIndices, 1) X(uint16_t, outerLowerModuleIndices, 1) \
+  X(uint32_t, innerMiniDoubletAnchorHitIndices, 1) X(uint32_t, outerMiniDoubletAnchorHitIndices, 1)         \
+  X(float, dPhis, 1) X(float, dPhiMins, 1) X(float, dPhiMaxs, 1) X(float, dPhiChanges, 1)                  \
+  X(float, dPhiChangeMins, 1) X(float, dPhiChangeMaxs, 1)
+
+// pLS-only arrays, indexed by pLS number (SDL/Segment.h:33-55)
+#define LST_PLS_FIELDS(X)                                                                                  \
+  X(uint32_t, seedIdx, 1) X(uint32_t, pLSHitsIdxs, 4) X(int32_t, charge, 1) X(int32_t, superbin, 1)        \
+  X(int8_t, pixelType, 1) X(int8_t, isQuad, 1) X(int8_t, isDup, 1) X(uint8_t, partOfPT5, 1)                \
+  X(float, ptIn, 1) X(float, ptErr, 1) X(float, px, 1) X(float, py, 1) X(float, pz, 1) X(float, etaErr, 1) \
+  X(float, eta, 1) X(float, phi, 1) X(float, score, 1) X(float, circleCenterX, 1)                          \
+  X(float, circleCenterY, 1) X(float, circleRadius, 1)
+
+#define LST_T3_FIELDS(X)                                                                                   \
+  X(uint32_t, segmentIndices, 2) X(uint16_t, lowerModuleIndices, 3) X(uint8_t, logicalLayers, 3)           \
+  X(uint32_t, hitIndices, 6) X(float, betaIn, 1) X(float, circleRadius, 1) X(float, circleCenterX, 1)      \
+  X(float, circleCenterY, 1) X(uint8_t, partOfPT5, 1) X(uint8_t, partOfT5, 1) X(uint8_t, partOfPT3, 1)
+
+#define LST_DECL_FIELD(T, N, M) T* N;
+
+namespace lst {
+struct HitsSoA { LST_HIT_FIELDS(LST_DECL_FIELD) };
+struct MDsSoA { LST_MD_FIELDS(LST_DECL_FIELD) };
+struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
+struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
+struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
+
+// Per-module counters and ranges (SDL/Module.h:21-85, SDL/Hit.h:28-32).
+struct RangesSoA {
+  int32_t* hitRanges;        // [2 nModules]
+  int32_t* hitRangesLower;   // [nModules]
+  int32_t* hitRangesUpper;
+  int8_t* hitRangesnLower;
+  int8_t* hitRangesnUpper;
+  int32_t* mdRanges;         // [2 nModules]
+  int32_t* segmentRanges;
+  int32_t* tripletRanges;
+  int32_t* quintupletRanges;
+  int32_t* tripletModuleIndices;    // [nLower]
+  int32_t* tripletModuleOccupancy;  // [nLower]
+  uint32_t* nMDs;            // [nLower + 1]
+  uint32_t* totOccupancyMDs;
+  uint32_t* nSegments;       // [nLower + 1]
+  uint32_t* totOccupancySegments;
+  uint32_t* nTriplets;       // [nLower]
+  uint32_t* totOccupancyTriplets;
+  uint32_t* scalars;         // [16] device-side totals (see Scalar)
+};
+enum Scalar : int { kNTotalTrips = 0, kBadHits = 1, kNScalars = 16 };
+}  // namespace lst

@@ -1,0 +1,32 @@
+// Host-callable launchers of the LST stage kernels (one file per stage group).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "es.h"
+#include "soa.h"
+
+namespace lst {
+
+struct EventView {  // everything a stage kernel needs, by value
+  ModulesView M;
+  HitsSoA H;
+  MDsSoA D;
+  SegsSoA S;
+  PLSSoA P;
+  T3SoA T;
+  RangesSoA R;
+  const int32_t* mdModuleIndices;    // [nLower + 1], geometry-only
+  const int32_t* mdModuleOccupancy;  // [nLower + 1]
+  const int32_t* segModuleIndices;   // [nLower + 1]
+  const int32_t* segModuleOccupancy; // [nLower + 1]
+  const int32_t* t3OccIfAny;         // [nLower] T3 capacity of a module that has segments
+  uint32_t nHits, nPLS, pixelModule;
+  float ptCut;
+};
+
+void launch_hits(const EventView& v, hipStream_t s);
+void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s);
+void launch_mini_doublets(const EventView& v, hipStream_t s);
+void launch_segments(const EventView& v, hipStream_t s);
+void launch_triplets(const EventView& v, hipStream_t s);
+
+}  // namespace lst

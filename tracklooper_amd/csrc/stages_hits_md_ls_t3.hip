@@ -1,0 +1,492 @@
+// Stage kernels: hits, pixel line segments, mini-doublets, line segments,
+// triplets.
+//
+// Builders run one wave64 per lower module (the unit that owns an output
+// range).  Candidates of the module are linearised in the reference's serial
+// loop order (SDL/MiniDoublet.h:911-925, SDL/Segment.h:810-838,
+// SDL/Triplet.h:886-913), evaluated 64 at a time, and the passing ones are
+// ranked with ballot+mbcnt, so object slots equal the reference CPU slots and
+// per-module capacity truncation keeps exactly the first `cap` objects.
+#include <climits>
+
+#include "algo_md_ls_t3.h"
+#include "stages.h"
+#include "wave.h"
+
+namespace lst {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int find_module(const ModulesView& M, uint32_t det) {
+  // binary search over the sorted detIds (SDL/Hit.h:172-192)
+  int lo = 0, hi = (int)M.nModules - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    uint32_t t = M.mapdetId[mid];
+    if (t == det) return M.mapIdx[mid];
+    if (t > det) hi = mid - 1;
+    else lo = mid + 1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int find_endcap(const ModulesView& M, uint32_t det) {
+  int lo = 0, hi = (int)M.nEndCapMap - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    uint32_t t = M.geoMapDetId[mid];
+    if (t == det) return mid;
+    if (t > det) hi = mid - 1;
+    else lo = mid + 1;
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------------ hits
+__global__ void k_event_init(EventView v) {
+  const uint32_t n2 = 2 * v.M.nModules;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
+    v.R.hitRanges[i] = (i & 1) ? -1 : INT_MAX;
+    v.R.mdRanges[i] = -1;
+    v.R.segmentRanges[i] = -1;
+    v.R.tripletRanges[i] = -1;
+    v.R.quintupletRanges[i] = -1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kNScalars) v.R.scalars[threadIdx.x] = 0;
+}
+
+// SDL/Hit.h:217-270
+__global__ void k_hits(EventView v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v.nHits; i += gridDim.x * blockDim.x) {
+    const float x = v.H.xs[i], y = v.H.ys[i], z = v.H.zs[i];
+    const uint32_t det = v.H.detid[i];
+    const float rt = m_sqrt(x * x + y * y);
+    v.H.rts[i] = rt;
+    v.H.phis[i] = phi_of(x, y);
+    v.H.etas[i] = ((z > 0) - (z < 0)) * m_acosh(m_sqrt(x * x + y * y + z * z) / rt);
+    float hx = 0.f, hy = 0.f, lx = 0.f, ly = 0.f;
+    const int mod = find_module(v.M, det);
+    if (mod < 0) {
+      v.H.moduleIndices[i] = 0xFFFF;
+      atomicOr(&v.R.scalars[kBadHits], 1u);
+    } else {
+      v.H.moduleIndices[i] = mod;
+      if (v.M.subdets[mod] == kEndcap && v.M.moduleType[mod] == kTwoS) {
+        const int e = find_endcap(v.M, det);
+        const float p = e >= 0 ? v.M.geoMapPhi[e] : 0.f;
+        const float c = m_cos(p), s = m_sin(p);
+        hx = x + 2.5f * c; lx = x - 2.5f * c;
+        hy = y + 2.5f * s; ly = y - 2.5f * s;
+      }
+      atomicMin(&v.R.hitRanges[2 * mod], (int)i);
+      atomicMax(&v.R.hitRanges[2 * mod + 1], (int)i);
+    }
+    v.H.highEdgeXs[i] = hx; v.H.highEdgeYs[i] = hy; v.H.lowEdgeXs[i] = lx; v.H.lowEdgeYs[i] = ly;
+  }
+}
+
+// SDL/Hit.h:194-215
+__global__ void k_module_ranges(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nModules; m += gridDim.x * blockDim.x) {
+    const int hi = v.R.hitRanges[2 * m + 1];
+    int32_t lower = -1, upper = -1;
+    int8_t nlo = -1, nup = -1;
+    if (m < v.M.nLowerModules) {
+      const uint32_t up = v.M.partnerModuleIndices[m];
+      const int hiUp = v.R.hitRanges[2 * up + 1];
+      if (hi != -1 && hiUp != -1) {
+        lower = v.R.hitRanges[2 * m];
+        upper = v.R.hitRanges[2 * up];
+        nlo = hi - lower + 1;
+        nup = hiUp - upper + 1;
+      }
+    }
+    v.R.hitRangesLower[m] = lower; v.R.hitRangesUpper[m] = upper;
+    v.R.hitRangesnLower[m] = nlo; v.R.hitRangesnUpper[m] = nup;
+  }
+}
+__global__ void k_hit_ranges_fix(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nModules; m += gridDim.x * blockDim.x)
+    if (v.R.hitRanges[2 * m + 1] == -1) v.R.hitRanges[2 * m] = -1;
+}
+
+// ----------------------------------------------------------- mini-doublet store
+// SDL/MiniDoublet.h:193-262
+__device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, uint32_t upperHit, uint16_t mod,
+                                         bool anchorIsUpper, const MDOut& o, uint32_t idx) {
+  const MDsSoA& D = v.D;
+  const HitsSoA& H = v.H;
+  const uint32_t a = anchorIsUpper ? upperHit : lowerHit;
+  const uint32_t b = anchorIsUpper ? lowerHit : upperHit;
+  D.moduleIndices[idx] = mod;
+  D.anchorHitIndices[idx] = a;
+  D.outerHitIndices[idx] = b;
+  D.dphichanges[idx] = o.dphichange; D.dphis[idx] = o.dphi; D.dzs[idx] = o.dz;
+  D.shiftedXs[idx] = o.sx; D.shiftedYs[idx] = o.sy; D.shiftedZs[idx] = o.sz;
+  D.noShiftedDzs[idx] = o.nsdz; D.noShiftedDphis[idx] = o.nsdphi; D.noShiftedDphiChanges[idx] = o.nsdphichange;
+  const float ahx = H.highEdgeXs[a], ahy = H.highEdgeYs[a], alx = H.lowEdgeXs[a], aly = H.lowEdgeYs[a];
+  D.anchorX[idx] = H.xs[a]; D.anchorY[idx] = H.ys[a]; D.anchorZ[idx] = H.zs[a];
+  D.anchorRt[idx] = H.rts[a]; D.anchorPhi[idx] = H.phis[a]; D.anchorEta[idx] = H.etas[a];
+  D.anchorHighEdgeX[idx] = ahx; D.anchorHighEdgeY[idx] = ahy; D.anchorLowEdgeX[idx] = alx; D.anchorLowEdgeY[idx] = aly;
+  D.anchorHighEdgePhi[idx] = m_atan2(ahy, ahx);
+  D.anchorLowEdgePhi[idx] = m_atan2(aly, alx);
+  D.outerX[idx] = H.xs[b]; D.outerY[idx] = H.ys[b]; D.outerZ[idx] = H.zs[b];
+  D.outerRt[idx] = H.rts[b]; D.outerPhi[idx] = H.phis[b]; D.outerEta[idx] = H.etas[b];
+  D.outerHighEdgeX[idx] = H.highEdgeXs[b]; D.outerHighEdgeY[idx] = H.highEdgeYs[b];
+  D.outerLowEdgeX[idx] = H.lowEdgeXs[b]; D.outerLowEdgeY[idx] = H.lowEdgeYs[b];
+}
+
+// ------------------------------------------------------------- pixel segments
+// SDL/Segment.h:1031-1117 and addPixelSegmentToMemory SDL/Segment.h:392-452
+struct PixelInputs {
+  const uint32_t* h0;
+  const uint32_t* h1;
+  const uint32_t* h2;
+  const uint32_t* h3;
+  const float* dPhiChange;
+};
+
+__global__ void k_pixel_segments(EventView v, PixelInputs in) {
+  const uint32_t pix = v.pixelModule;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    v.R.nSegments[pix] = v.nPLS; v.R.totOccupancySegments[pix] = v.nPLS;
+    v.R.nMDs[pix] = 2 * v.nPLS; v.R.totOccupancyMDs[pix] = 2 * v.nPLS;
+  }
+  const MDOut zero{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < v.nPLS; t += gridDim.x * blockDim.x) {
+    const uint32_t innerMD = v.mdModuleIndices[pix] + 2 * t, outerMD = innerMD + 1;
+    const uint32_t seg = v.segModuleIndices[pix] + t;
+    const uint32_t h0 = in.h0[t], h1 = in.h1[t], h2 = in.h2[t], h3 = in.h3[t];
+    store_md(v, h0, h1, pix, false, zero, innerMD);
+    store_md(v, h2, h3, pix, false, zero, outerMD);
+    // line through the PCA pseudo-hit with slope sinh(eta) and its residual at r3LH
+    const float slope = m_sinh(v.H.ys[h1]);
+    const float icpt = v.H.zs[h0] - slope * v.H.rts[h0];
+    float score = (v.H.rts[h2] * slope + icpt) - v.H.zs[h2];
+    score = score * score;
+    v.S.mdIndices[2 * seg] = innerMD; v.S.mdIndices[2 * seg + 1] = outerMD;
+    v.S.innerLowerModuleIndices[seg] = pix; v.S.outerLowerModuleIndices[seg] = pix;
+    v.S.innerMiniDoubletAnchorHitIndices[seg] = h0; v.S.outerMiniDoubletAnchorHitIndices[seg] = h2;
+    v.S.dPhiChanges[seg] = in.dPhiChange[t];
+    v.P.isDup[t] = 0;
+    v.P.partOfPT5[t] = 0;
+    v.P.score[t] = score;
+    v.P.pLSHitsIdxs[4 * t + 0] = v.H.idxs[h0];
+    v.P.pLSHitsIdxs[4 * t + 1] = v.H.idxs[h2];
+    v.P.pLSHitsIdxs[4 * t + 2] = v.H.idxs[h1];
+    v.P.pLSHitsIdxs[4 * t + 3] = v.H.idxs[h3];
+    // two circle centres at distance R from the PCA point; keep the one closer to r3LH
+    const float R = v.H.xs[h1] / (2 * k2Rinv1GeVf);
+    const float ph = v.H.zs[h1];
+    const float ax = v.H.xs[h0], ay = v.H.ys[h0];
+    const float cx0 = ax + R * m_sin(ph), cx1 = ax - R * m_sin(ph);
+    const float cy0 = ay - R * m_cos(ph), cy1 = ay + R * m_cos(ph);
+    const float bx = v.H.xs[h2], by = v.H.ys[h2];
+    const float d0 = m_abs(m_sqrt((bx - cx0) * (bx - cx0) + (by - cy0) * (by - cy0)) - R);
+    const float d1 = m_abs(m_sqrt((bx - cx1) * (bx - cx1) + (by - cy1) * (by - cy1)) - R);
+    // first minimum below SDL_INF wins, as the reference's scan over the two centres
+    const float best0 = d0 < kSdlInf ? d0 : kSdlInf;
+    const bool second = d1 < best0;
+    v.P.circleCenterX[t] = second ? cx1 : cx0;
+    v.P.circleCenterY[t] = second ? cy1 : cy0;
+    v.P.circleRadius[t] = R;
+  }
+}
+
+// --------------------------------------------------------------- mini-doublets
+__global__ __launch_bounds__(kWave) void k_mini_doublets(EventView v) {
+  __shared__ float sx[256], sy[256], sz[256], srt[256];
+  const uint32_t m = blockIdx.x;
+  const unsigned lane = threadIdx.x;
+  const int loBase = v.R.hitRangesLower[m];
+  const int cap = v.mdModuleOccupancy[m];
+  unsigned count = 0;
+  if (loBase != -1) {
+    const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+    const int upBase = v.R.hitRangesUpper[m];
+    for (int i = lane; i < nLo; i += kWave) {
+      sx[i] = v.H.xs[loBase + i]; sy[i] = v.H.ys[loBase + i]; sz[i] = v.H.zs[loBase + i]; srt[i] = v.H.rts[loBase + i];
+    }
+    for (int i = lane; i < nUp; i += kWave) {
+      const int j = 128 + i;
+      sx[j] = v.H.xs[upBase + i]; sy[j] = v.H.ys[upBase + i]; sz[j] = v.H.zs[upBase + i]; srt[j] = v.H.rts[upBase + i];
+    }
+    __syncthreads();
+    const ModCtx c = load_mod(v.M, m);
+    const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
+    const uint32_t base = v.mdModuleIndices[m];
+    const int limit = nLo * nUp;
+    for (int c0 = 0; c0 < limit; c0 += kWave) {
+      const int k = c0 + lane;
+      bool pass = false;
+      MDOut o;
+      int li = 0, ui = 0;
+      if (k < limit) {
+        li = k / nUp;
+        ui = k - li * nUp;
+        const int u = 128 + ui;
+        pass = c.subdet == kBarrel
+                   ? md_barrel(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], o)
+                   : md_endcap(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], o);
+      }
+      unsigned n;
+      const unsigned rank = count + wave_rank(pass, n);
+      if (pass && (int)rank < cap) store_md(v, loBase + li, upBase + ui, m, anchorIsUpper, o, base + rank);
+      count += n;
+    }
+  }
+  if (lane == 0) {
+    v.R.totOccupancyMDs[m] = count;
+    v.R.nMDs[m] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// SDL/MiniDoublet.h:1084-1104
+__global__ void k_md_ranges(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+    const uint32_t n = v.R.nMDs[m];
+    if (n != 0 && v.R.hitRanges[2 * m] != -1) {
+      v.R.mdRanges[2 * m] = v.mdModuleIndices[m];
+      v.R.mdRanges[2 * m + 1] = v.mdModuleIndices[m] + n - 1;
+    }
+  }
+}
+
+// --------------------------------------------------------------- line segments
+__global__ __launch_bounds__(kWave) void k_segments(EventView v) {
+  __shared__ uint32_t pre[160];
+  __shared__ uint16_t outMod[160];
+  __shared__ uint32_t outCnt[160];
+  const uint32_t m = blockIdx.x;
+  const unsigned lane = threadIdx.x;
+  const uint32_t nIn = v.R.nMDs[m];
+  const int cap = v.segModuleOccupancy[m];
+  unsigned count = 0;
+  if (nIn != 0) {
+    const uint32_t nConn = v.M.nConnectedModules[m];
+    for (uint32_t k = lane; k < nConn; k += kWave) {
+      const uint16_t om = v.M.moduleMap[(size_t)m * v.M.stride + k];
+      outMod[k] = om;
+      outCnt[k] = v.R.nMDs[om];
+    }
+    __syncthreads();
+    if (lane == 0) {
+      uint32_t run = 0;
+      for (uint32_t k = 0; k < nConn; k++) { pre[k] = run; run += nIn * outCnt[k]; }
+      pre[nConn] = run;
+    }
+    __syncthreads();
+    const uint32_t total = pre[nConn];
+    const ModCtx ci = load_mod(v.M, m);
+    const uint32_t inBase = v.R.mdRanges[2 * m];
+    const uint32_t segBase = v.segModuleIndices[m];
+    for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+      const uint32_t L = c0 + lane;
+      bool pass = false;
+      SegOut o{0, 0, 0, 0, 0, 0};
+      uint32_t inMD = 0, outMD = 0;
+      uint16_t om = 0;
+      if (L < total) {
+        uint32_t k = 0;
+        while (k + 1 < nConn && pre[k + 1] <= L) k++;
+        om = outMod[k];
+        const uint32_t local = L - pre[k];
+        const uint32_t nOut = outCnt[k];
+        inMD = inBase + local / nOut;
+        outMD = v.R.mdRanges[2 * om] + local % nOut;
+        const ModCtx co = load_mod(v.M, om);
+        const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
+        pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, o)
+                                                              : seg_endcap(ci, co, v.ptCut, a, b, o);
+      }
+      unsigned n;
+      const unsigned rank = count + wave_rank(pass, n);
+      if (pass && (int)rank < cap) {
+        const uint32_t s = segBase + rank;
+        v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
+        v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
+        v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
+        v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
+        v.S.dPhis[s] = o.dPhi; v.S.dPhiMins[s] = o.dPhiMin; v.S.dPhiMaxs[s] = o.dPhiMax;
+        v.S.dPhiChanges[s] = o.dPhiChange; v.S.dPhiChangeMins[s] = o.dPhiChangeMin;
+        v.S.dPhiChangeMaxs[s] = o.dPhiChangeMax;
+      }
+      count += n;
+    }
+  }
+  if (lane == 0) {
+    v.R.totOccupancySegments[m] = count;
+    v.R.nSegments[m] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// SDL/Segment.h:1010-1029
+__global__ void k_segment_ranges(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+    const uint32_t n = v.R.nSegments[m];
+    if (n != 0) {
+      v.R.segmentRanges[2 * m] = v.segModuleIndices[m];
+      v.R.segmentRanges[2 * m + 1] = v.segModuleIndices[m] + n - 1;
+    }
+  }
+}
+
+// -------------------------------------------------------------------- triplets
+// capacity offsets: a module with segments gets its tabulated capacity, others 0
+// (SDL/Triplet.h:1007-1102); exclusive scan in module order.
+__global__ __launch_bounds__(1024) void k_triplet_offsets(EventView v) {
+  __shared__ unsigned lds[17];
+  unsigned carry = 0;
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t b = 0; b < nL; b += 1024) {
+    const uint32_t m = b + threadIdx.x;
+    const int occ = (m < nL && v.R.nSegments[m] != 0) ? v.t3OccIfAny[m] : 0;
+    unsigned total;
+    const unsigned ex = block_exclusive_scan<1024>((unsigned)occ, lds, total);
+    if (m < nL) {
+      v.R.tripletModuleOccupancy[m] = occ;
+      v.R.tripletModuleIndices[m] = carry + ex;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) v.R.scalars[kNTotalTrips] = carry;
+}
+
+__global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
+  __shared__ uint32_t pre[1025];
+  const uint32_t m1 = blockIdx.x;
+  const unsigned lane = threadIdx.x;
+  const uint32_t nIn = v.R.nSegments[m1];
+  const int cap = v.R.tripletModuleOccupancy[m1];
+  unsigned count = 0;
+  if (nIn != 0 && v.M.nConnectedModules[m1] != 0 && nIn <= 1024) {
+    const uint32_t segBase = v.segModuleIndices[m1];
+    uint32_t run = 0;
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
+      const uint32_t a = a0 + lane;
+      const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
+      const uint32_t incl = wave_inclusive_scan(cnt);
+      if (a < nIn) pre[a + 1] = run + incl;
+      run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) pre[0] = 0;
+    __syncthreads();
+    const uint32_t total = run;
+    const bool psIn = v.M.moduleType[m1] == kPS;
+    const int l1 = v.M.sdlLayers[m1];
+    const int16_t s1 = v.M.subdets[m1];
+    const uint32_t t3Base = v.R.tripletModuleIndices[m1];
+    for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+      const uint32_t L = c0 + lane;
+      bool pass = false;
+      float betaIn = 0, radius = 0, gx = 0, gy = 0;
+      uint32_t inSeg = 0, outSeg = 0, md1 = 0, md2 = 0, md3 = 0;
+      uint16_t m2 = 0, m3 = 0;
+      if (L < total) {
+        // largest a with pre[a] <= L
+        uint32_t lo = 0, hi = nIn;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (pre[mid] <= L) lo = mid;
+          else hi = mid;
+        }
+        inSeg = segBase + lo;
+        m2 = v.S.outerLowerModuleIndices[inSeg];
+        outSeg = v.segModuleIndices[m2] + (L - pre[lo]);
+        m3 = v.S.outerLowerModuleIndices[outSeg];
+        md1 = v.S.mdIndices[2 * inSeg];
+        md2 = v.S.mdIndices[2 * outSeg];
+        md3 = v.S.mdIndices[2 * outSeg + 1];
+        if (v.S.mdIndices[2 * inSeg + 1] == md2) {  // shared middle mini-doublet (SDL/Triplet.h:823-825)
+          T3Geo p;
+          p.x1 = v.D.anchorX[md1]; p.y1 = v.D.anchorY[md1]; p.z1 = v.D.anchorZ[md1]; p.r1 = v.D.anchorRt[md1];
+          p.phi1 = v.D.anchorPhi[md1];
+          p.x2 = v.D.anchorX[md2]; p.y2 = v.D.anchorY[md2]; p.z2 = v.D.anchorZ[md2]; p.r2 = v.D.anchorRt[md2];
+          p.x3 = v.D.anchorX[md3]; p.y3 = v.D.anchorY[md3]; p.z3 = v.D.anchorZ[md3]; p.r3 = v.D.anchorRt[md3];
+          if (t3_rz(l1, v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) {
+            const int16_t s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
+            const bool psOut = v.M.moduleType[m3] == kPS;
+            const float alphaIn = v.S.dPhiChanges[inSeg];
+            if (s1 == kBarrel && s2 == kBarrel && s3 == kBarrel) pass = t3_bbb(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
+            else if (s1 == kBarrel && s3 == kEndcap) pass = t3_bbe(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
+            else if (s1 == kEndcap && s2 == kEndcap && s3 == kEndcap)
+              pass = t3_eee(psIn, psOut, v.ptCut, p, alphaIn, v.S.dPhiChangeMins[inSeg], v.S.dPhiChangeMaxs[inSeg],
+                            betaIn);
+            if (pass) radius = circle_radius3(p.x1, p.y1, p.x2, p.y2, p.x3, p.y3, gx, gy);
+          }
+        }
+      }
+      unsigned n;
+      const unsigned rank = count + wave_rank(pass, n);
+      if (pass && (int)rank < cap) {
+        const uint32_t t = t3Base + rank;
+        v.T.segmentIndices[2 * t] = inSeg; v.T.segmentIndices[2 * t + 1] = outSeg;
+        v.T.lowerModuleIndices[3 * t] = m1; v.T.lowerModuleIndices[3 * t + 1] = m2;
+        v.T.lowerModuleIndices[3 * t + 2] = m3;
+        v.T.betaIn[t] = betaIn; v.T.circleRadius[t] = radius; v.T.circleCenterX[t] = gx; v.T.circleCenterY[t] = gy;
+        v.T.logicalLayers[3 * t] = v.M.layers[m1] + (s1 == kEndcap) * 6;
+        v.T.logicalLayers[3 * t + 1] = v.M.layers[m2] + (v.M.subdets[m2] == kEndcap) * 6;
+        v.T.logicalLayers[3 * t + 2] = v.M.layers[m3] + (v.M.subdets[m3] == kEndcap) * 6;
+        const uint32_t mdB = v.S.mdIndices[2 * inSeg + 1];
+        v.T.hitIndices[6 * t + 0] = v.D.anchorHitIndices[md1]; v.T.hitIndices[6 * t + 1] = v.D.outerHitIndices[md1];
+        v.T.hitIndices[6 * t + 2] = v.D.anchorHitIndices[mdB]; v.T.hitIndices[6 * t + 3] = v.D.outerHitIndices[mdB];
+        v.T.hitIndices[6 * t + 4] = v.D.anchorHitIndices[md3]; v.T.hitIndices[6 * t + 5] = v.D.outerHitIndices[md3];
+        v.T.partOfPT5[t] = 0; v.T.partOfT5[t] = 0; v.T.partOfPT3[t] = 0;
+      }
+      count += n;
+    }
+  }
+  if (lane == 0) {
+    v.R.totOccupancyTriplets[m1] = count;
+    v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// SDL/Triplet.h:1104-1123
+__global__ void k_triplet_ranges(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+    const uint32_t n = v.R.nTriplets[m];
+    if (n != 0) {
+      v.R.tripletRanges[2 * m] = v.R.tripletModuleIndices[m];
+      v.R.tripletRanges[2 * m + 1] = v.R.tripletModuleIndices[m] + n - 1;
+    }
+  }
+}
+
+inline unsigned grid_for(uint32_t n, unsigned block) {
+  unsigned g = (n + block - 1) / block;
+  return g == 0 ? 1 : (g > 4096 ? 4096 : g);
+}
+
+}  // namespace
+
+void launch_hits(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_event_init, dim3(grid_for(2 * v.M.nModules, 256)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_hits, dim3(grid_for(v.nHits, 256)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_module_ranges, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_hit_ranges_fix, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
+}
+
+void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s) {
+  PixelInputs in{hitIdx[0], hitIdx[1], hitIdx[2], hitIdx[3], dPhiChange};
+  hipLaunchKernelGGL(k_pixel_segments, dim3(grid_for(v.nPLS, 256)), dim3(256), 0, s, v, in);
+}
+
+void launch_mini_doublets(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
+}
+
+void launch_segments(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
+}
+
+void launch_triplets(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
+}
+
+}  // namespace lst
