@@ -249,6 +249,12 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
     warn = LST_W_TRUNCATED;
   }
   const hipMemcpyKind k = onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (!onDevice) {  // pseudo-hit indices must address hits of this event
+    const uint32_t* hi[4] = {p.hit_indices0, p.hit_indices1, p.hit_indices2, p.hit_indices3};
+    for (int q = 0; q < 4; q++)
+      for (uint32_t t = 0; t < n; t++)
+        if (hi[q][t] >= v.nHits) return fail(LST_E_INPUT, "pixel segment hit index out of range");
+  }
   if (n) {
     HIP_OK(hipMemcpyAsync(plsHitIdx[0], p.hit_indices0, n * 4, k, stream));
     HIP_OK(hipMemcpyAsync(plsHitIdx[1], p.hit_indices1, n * 4, k, stream));
