@@ -1,0 +1,183 @@
+"""Throughput benchmark of the MI355X LST engine.
+
+One step = the LST chain on one synthetic PU200-like event (about 7000
+helices with pT = 0.4 + Exp(0.5) GeV plus 200 hard tracks, 1% noise hits,
+OT800_IT615_pt0.8 geometry), inputs already resident in HBM (prepared once on
+the host and staged; generation and staging are outside the timed region).
+Prints ONE JSON line (see the `result` dict) with the roofline of the
+dominant kernel and the oracle's CPU throughput on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC set"
+STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets"]
+KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets"]
+LAST_STAGE = 4  # triplets: the stages built so far
+HBM_PEAK_GBS = 8000.0
+
+
+def _input_struct(lst, ev):
+    import numpy as np
+    a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    inp = lst._Input(
+        *[p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
+                            "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ", "see_stateTrajGlbPx",
+                            "see_stateTrajGlbPy", "see_stateTrajGlbPz", "see_q", "see_hitIdx", "see_hitIdx_off")],
+        len(a["see_px"]), *[p(a[k]) for k in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")], len(a["ph2_x"]))
+    return inp, a
+
+
+def algorithmic_bytes(counts: dict) -> dict:
+    """Minimal HBM bytes per launch of each builder kernel: inputs read once, outputs written once.
+
+    Record sizes are the fields each kernel reads/writes:
+      hits kernel  : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
+      MD builder   : read x,y,z,rt of the module-pair hits (16 B/hit), write 134 B/MD
+      LS builder   : read 44 B of each mini-doublet, write 44 B/segment
+      T3 builder   : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
+    """
+    h, md, ls, t3 = counts["hits"], counts["mds"], counts["segs"], counts["t3"]
+    return {
+        "k_hits": 46 * h,
+        "k_mini_doublets": 16 * h + 134 * md,
+        "k_segments": 44 * md + 44 * ls,
+        "k_triplets": 22 * ls + 20 * md + 60 * t3,
+    }
+
+
+def cpu_baseline(events, budget_s: float = 20.0) -> dict:
+    """The oracle (serial C++ restatement, 1 thread) on a bounded sample of the same events."""
+    from oracle import orc
+    from tracklooper_amd import synth
+    g = orc.Geometry(synth.geometry_dir())
+    n, t0 = 0, time.perf_counter()
+    stage = ["hits", "pls", "md", "ls", "t3"][LAST_STAGE]
+    while time.perf_counter() - t0 < budget_s and n < 4 * len(events):
+        orc.Event(g).run_lst(events[n % len(events)], stage)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"{n} PU200-like events through the same stages, serial oracle, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--events", type=int, default=4, help="distinct seeded events cycled through")
+    ap.add_argument("--n-pu", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (each box has one MI355X); --gpus {args.gpus} is not supported.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+
+    t_gen = time.perf_counter()
+    events = [synth.generate_event("pu", i, n_pu=args.n_pu) for i in range(args.events)]
+    t_gen = time.perf_counter() - t_gen
+    es = lst.LSTESData(synth.geometry_dir())
+    ev = lst.Event(es)
+    L = lst.lib()
+    L.lst_event_stage_input.argtypes = [C.c_void_p, C.POINTER(lst._Input), C.c_uint32]
+    L.lst_event_run_staged.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
+    L.lst_event_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.lst_event_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.lst_event_stream.argtypes = [C.c_void_p]
+    L.lst_event_stream.restype = C.c_void_p
+    for i, e in enumerate(events):
+        inp, keep = _input_struct(lst, e)
+        lst._check(L.lst_event_stage_input(ev.h, C.byref(inp), i))
+    stream = torch.cuda.ExternalStream(L.lst_event_stream(ev.h))
+    torch.cuda.synchronize()
+
+    def step(i):
+        lst._check(L.lst_event_run_staged(ev.h, i % len(events), LAST_STAGE))
+
+    for i in range(args.warmup):
+        step(i)
+    ev.synchronize()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for i in range(args.steps):
+        step(i)
+    stop.record(stream)
+    ev.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    ms_per_step = wall * 1e3 / args.steps
+
+    # per-kernel device time of the main kernel of every stage, and object counts per event
+    lst._check(L.lst_event_set_timing(ev.h, 1))
+    ktimes = np.zeros((len(events), 16), np.float32)
+    counts = []
+    for i in range(len(events)):
+        step(i)
+        lst._check(L.lst_event_kernel_times(ev.h, ktimes[i].ctypes.data_as(C.c_void_p), 16))
+        counts.append({"hits": int(lst.lib().lst_event_count(ev.h, b"hits")),
+                       "mds": int(ev.field("mds.nMDs").sum()),
+                       "segs": int(ev.field("segs.nSegments").sum()),
+                       "t3": int(ev.field("t3.nTriplets").sum())})
+    lst._check(L.lst_event_set_timing(ev.h, 0))
+    avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
+    per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}
+    dom = max(per_kernel, key=per_kernel.get)
+    mean_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
+    abytes = algorithmic_bytes(mean_counts)
+    dom_bytes = abytes.get(dom, 0.0)
+    achieved = dom_bytes / (per_kernel[dom] * 1e-3) / 1e9 if per_kernel[dom] > 0 else 0.0
+
+    result = {
+        "metric": METRIC,
+        "value": args.steps / wall,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"per-event path, one PU{args.n_pu}-like synthetic event per step "
+                        f"({args.events} seeded events cycled), stages hits..triplets",
+            "geometry": "OT800_IT615_pt0.8",
+            "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],
+            "mean_segments": mean_counts["segs"], "mean_triplets": mean_counts["t3"],
+            "gpu_ms_per_step": gpu_ms / args.steps,
+            "kernel_ms": per_kernel,
+        },
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": dom_bytes},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(events, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

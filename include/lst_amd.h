@@ -122,6 +122,23 @@ typedef struct {
  * sequence on the device. */
 lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage);
 
+/* Throughput path (no reference counterpart; the reference preloads inputs on
+ * the host, bin/sdl.cc:352-384): prepare an event on the host once and keep it
+ * resident in HBM in input slot `slot` of the event handle ... */
+lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t slot);
+/* ... then run the chain on it with no host work and no host round trip
+ * (asynchronous on the event's stream; same results as lst_run). */
+lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
+
+/* Per-kernel device timing of the builder kernels (hipEvents on the event's
+ * stream).  After a run with timing on, lst_event_kernel_times writes the
+ * durations in ms of: 0 hits, 1 pixel segments, 2 mini-doublets,
+ * 3 segments, 4 triplets (n <= 16 entries; synchronises the stream). */
+lst_status lst_event_set_timing(lst_event* ev, int on);
+lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n);
+/* The hipStream_t the event runs on. */
+void* lst_event_stream(lst_event* ev);
+
 #ifdef __cplusplus
 }
 #endif

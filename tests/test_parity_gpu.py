@@ -75,3 +75,25 @@ def test_repeat_runs_identical(engine, oracle_geometry):
     e.run_lst(ev1, "t3")
     e.run_lst(ev0, "t3")
     assert len(a) > 0 and np.array_equal(a, t3_hits())
+
+
+def test_staged_device_path_matches(engine, oracle_geometry):
+    """The HBM-resident throughput path gives exactly the lst_run results."""
+    import ctypes as C
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    from tests.parity import compare
+    sys_path_bench = __import__("bench")
+    _, es = engine
+    ev = synth.generate_event("pu", 2, n_pu=100)
+    e = lst.Event(es)
+    L = lst.lib()
+    L.lst_event_stage_input.argtypes = [C.c_void_p, C.POINTER(lst._Input), C.c_uint32]
+    L.lst_event_run_staged.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
+    inp, keep = sys_path_bench._input_struct(lst, ev)
+    lst._check(L.lst_event_stage_input(e.h, C.byref(inp), 3))
+    lst._check(L.lst_event_run_staged(e.h, 3, 4))
+    e.synchronize()
+    from oracle import orc
+    ref = orc.Event(oracle_geometry).run_lst(ev, "t3")
+    compare(e, ref, "t3")

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -88,6 +89,24 @@ struct lst_event {
   float* plsDPhiChange = nullptr;
   std::vector<std::pair<std::string, FieldRef>> fields;
   lst::PreparedEvent prepared;
+  // timing of the main kernel of each stage
+  bool timing = false;
+  lst::KernelTimer timers[16];
+  lst::KernelTimer timer(int k) const { return timing ? timers[k] : lst::KernelTimer{}; }
+  // device-resident prepared inputs (throughput path)
+  struct Staged {
+    char* mem = nullptr;
+    size_t bytes = 0;
+    uint32_t nHits = 0, nPLS = 0;
+    float *x = nullptr, *y = nullptr, *z = nullptr;
+    uint32_t *det = nullptr, *idx = nullptr;
+    uint32_t *h[4] = {nullptr, nullptr, nullptr, nullptr}, *seed = nullptr;
+    float *dphi = nullptr, *pt = nullptr, *pterr = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr,
+          *eta = nullptr, *etaerr = nullptr, *phi = nullptr;
+    int32_t *charge = nullptr, *superbin = nullptr;
+    int8_t *ptype = nullptr, *quad = nullptr;
+  };
+  std::vector<Staged> staged;
 
   lst_status allocate(uint32_t nHitsNeeded);
   lst_status add_hits(const float* x, const float* y, const float* z, const uint32_t* det, const uint32_t* idx,
@@ -234,7 +253,7 @@ lst_status lst_event::add_hits(const float* x, const float* y, const float* z, c
   }
   v.nHits = n;
   v.nPLS = 0;
-  lst::launch_hits(v, stream);
+  lst::launch_hits(v, stream, timer(0));
   HIP_OK(hipGetLastError());
   stage = 0;
   return LST_OK;
@@ -277,7 +296,7 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
   }
   v.nPLS = n;
   const uint32_t* hi[4] = {plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]};
-  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream);
+  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream, timer(1));
   HIP_OK(hipGetLastError());
   stage = 1;
   return warn;
@@ -339,6 +358,12 @@ void lst_event_destroy(lst_event* ev) {
   if (!ev) return;
   if (ev->stream) hipStreamSynchronize(ev->stream);
   if (ev->arena) hipFree(ev->arena);
+  for (auto& st : ev->staged)
+    if (st.mem) hipFree(st.mem);
+  for (auto& t : ev->timers) {
+    if (t.start) hipEventDestroy(t.start);
+    if (t.stop) hipEventDestroy(t.stop);
+  }
   if (ev->ownStream) hipStreamDestroy(ev->stream);
   delete ev;
 }
@@ -356,7 +381,7 @@ lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments*
 
 lst_status lst_event_create_mini_doublets(lst_event* ev) {
   if (!ev || ev->stage != 1) return fail(LST_E_ORDER, "createMiniDoublets needs hits and pixel segments");
-  lst::launch_mini_doublets(ev->v, ev->stream);
+  lst::launch_mini_doublets(ev->v, ev->stream, ev->timer(2));
   HIP_OK(hipGetLastError());
   ev->stage = 2;
   return LST_OK;
@@ -364,7 +389,7 @@ lst_status lst_event_create_mini_doublets(lst_event* ev) {
 
 lst_status lst_event_create_segments(lst_event* ev) {
   if (!ev || ev->stage != 2) return fail(LST_E_ORDER, "createSegmentsWithModuleMap needs mini-doublets");
-  lst::launch_segments(ev->v, ev->stream);
+  lst::launch_segments(ev->v, ev->stream, ev->timer(3));
   HIP_OK(hipGetLastError());
   ev->stage = 3;
   return LST_OK;
@@ -372,7 +397,7 @@ lst_status lst_event_create_segments(lst_event* ev) {
 
 lst_status lst_event_create_triplets(lst_event* ev) {
   if (!ev || ev->stage != 3) return fail(LST_E_ORDER, "createTriplets needs segments");
-  lst::launch_triplets(ev->v, ev->stream);
+  lst::launch_triplets(ev->v, ev->stream, ev->timer(4));
   HIP_OK(hipGetLastError());
   ev->stage = 4;
   return LST_OK;
@@ -442,5 +467,86 @@ lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
   if ((s = lst_event_create_triplets(ev)) != LST_OK) return s;
   return warn;
 }
+
+lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t slot) {
+  if (!ev || !in) return fail(LST_E_INPUT, "null argument");
+  lst::PreparedEvent p;
+  lst::prepare_event(*in, p);
+  if (ev->staged.size() <= slot) ev->staged.resize(slot + 1);
+  lst_event::Staged& st = ev->staged[slot];
+  if (st.mem) {
+    HIP_OK(hipStreamSynchronize(ev->stream));
+    HIP_OK(hipFree(st.mem));
+    st = lst_event::Staged();
+  }
+  const uint32_t nh = p.x.size();
+  const uint32_t np = std::min<uint32_t>(p.ptIn.size(), lst::kNMaxPixelSegments);
+  Carve c;
+  c.take(st.x, nh); c.take(st.y, nh); c.take(st.z, nh); c.take(st.det, nh); c.take(st.idx, nh);
+  for (int k = 0; k < 4; k++) c.take(st.h[k], np);
+  c.take(st.seed, np); c.take(st.dphi, np); c.take(st.pt, np); c.take(st.pterr, np); c.take(st.px, np);
+  c.take(st.py, np); c.take(st.pz, np); c.take(st.eta, np); c.take(st.etaerr, np); c.take(st.phi, np);
+  c.take(st.charge, np); c.take(st.superbin, np); c.take(st.ptype, np); c.take(st.quad, np);
+  HIP_OK(hipMalloc(&st.mem, c.off ? c.off : 256));
+  st.bytes = c.off;
+  c.bind(st.mem);
+  st.nHits = nh;
+  st.nPLS = np;
+  auto up = [&](void* d, const void* h, size_t b) { return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess; };
+  HIP_OK(up(st.x, p.x.data(), nh * 4)); HIP_OK(up(st.y, p.y.data(), nh * 4)); HIP_OK(up(st.z, p.z.data(), nh * 4));
+  HIP_OK(up(st.det, p.detId.data(), nh * 4)); HIP_OK(up(st.idx, p.idx.data(), nh * 4));
+  HIP_OK(up(st.h[0], p.h0.data(), np * 4)); HIP_OK(up(st.h[1], p.h1.data(), np * 4));
+  HIP_OK(up(st.h[2], p.h2.data(), np * 4)); HIP_OK(up(st.h[3], p.h3.data(), np * 4));
+  HIP_OK(up(st.seed, p.seedIdx.data(), np * 4)); HIP_OK(up(st.dphi, p.dPhiChange.data(), np * 4));
+  HIP_OK(up(st.pt, p.ptIn.data(), np * 4)); HIP_OK(up(st.pterr, p.ptErr.data(), np * 4));
+  HIP_OK(up(st.px, p.px.data(), np * 4)); HIP_OK(up(st.py, p.py.data(), np * 4)); HIP_OK(up(st.pz, p.pz.data(), np * 4));
+  HIP_OK(up(st.eta, p.eta.data(), np * 4)); HIP_OK(up(st.etaerr, p.etaErr.data(), np * 4));
+  HIP_OK(up(st.phi, p.phi.data(), np * 4)); HIP_OK(up(st.charge, p.charge.data(), np * 4));
+  HIP_OK(up(st.superbin, p.superbin.data(), np * 4)); HIP_OK(up(st.ptype, p.pixelType.data(), np));
+  HIP_OK(up(st.quad, p.isQuad.data(), np));
+  return ev->allocate(nh);
+}
+
+lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
+  if (!ev || slot >= ev->staged.size() || !ev->staged[slot].mem) return fail(LST_E_INPUT, "input slot not staged");
+  const lst_event::Staged& st = ev->staged[slot];
+  lst_status s = ev->add_hits(st.x, st.y, st.z, st.det, st.idx, st.nHits, true);
+  if (s != LST_OK || last_stage < 1) return s;
+  lst_pixel_segments p;
+  p.hit_indices0 = st.h[0]; p.hit_indices1 = st.h[1]; p.hit_indices2 = st.h[2]; p.hit_indices3 = st.h[3];
+  p.delta_phi_change = st.dphi; p.pt_in = st.pt; p.pt_err = st.pterr; p.px = st.px; p.py = st.py; p.pz = st.pz;
+  p.eta = st.eta; p.eta_err = st.etaerr; p.phi = st.phi; p.charge = st.charge; p.seed_idx = st.seed;
+  p.superbin = st.superbin; p.pixel_type = st.ptype; p.is_quad = st.quad; p.n_pls = st.nPLS;
+  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s;
+  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
+  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
+  return lst_event_create_triplets(ev);
+}
+
+lst_status lst_event_set_timing(lst_event* ev, int on) {
+  if (!ev) return fail(LST_E_INPUT, "null event");
+  if (on && !ev->timers[0].start)
+    for (auto& t : ev->timers) {
+      HIP_OK(hipEventCreate(&t.start));
+      HIP_OK(hipEventCreate(&t.stop));
+    }
+  ev->timing = on != 0;
+  return LST_OK;
+}
+
+lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n) {
+  if (!ev || !ms) return fail(LST_E_INPUT, "null argument");
+  if (!ev->timers[0].start) return fail(LST_E_ORDER, "timing was never enabled");
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  for (int k = 0; k < n && k < 16; k++) {
+    ms[k] = 0.f;
+    if (hipEventQuery(ev->timers[k].stop) == hipSuccess &&
+        hipEventElapsedTime(&ms[k], ev->timers[k].start, ev->timers[k].stop) != hipSuccess)
+      ms[k] = 0.f;
+  }
+  return LST_OK;
+}
+
+void* lst_event_stream(lst_event* ev) { return ev ? (void*)ev->stream : nullptr; }
 
 }  // extern "C"

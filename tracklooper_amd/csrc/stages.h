@@ -23,10 +23,18 @@ struct EventView {  // everything a stage kernel needs, by value
   float ptCut;
 };
 
-void launch_hits(const EventView& v, hipStream_t s);
-void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s);
-void launch_mini_doublets(const EventView& v, hipStream_t s);
-void launch_segments(const EventView& v, hipStream_t s);
-void launch_triplets(const EventView& v, hipStream_t s);
+// Optional hipEvent pair recorded around the stage's main kernel (timing).
+struct KernelTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  void begin(hipStream_t s) const { if (start) hipEventRecord(start, s); }
+  void end(hipStream_t s) const { if (stop) hipEventRecord(stop, s); }
+};
+
+void launch_hits(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
+                           const KernelTimer& t = {});
+void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 
 }  // namespace lst

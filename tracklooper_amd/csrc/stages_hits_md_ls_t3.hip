@@ -461,31 +461,42 @@ inline unsigned grid_for(uint32_t n, unsigned block) {
 
 }  // namespace
 
-void launch_hits(const EventView& v, hipStream_t s) {
+void launch_hits(const EventView& v, hipStream_t s, const KernelTimer& t) {
   hipLaunchKernelGGL(k_event_init, dim3(grid_for(2 * v.M.nModules, 256)), dim3(256), 0, s, v);
+  t.begin(s);
   hipLaunchKernelGGL(k_hits, dim3(grid_for(v.nHits, 256)), dim3(256), 0, s, v);
+  t.end(s);
   hipLaunchKernelGGL(k_module_ranges, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_hit_ranges_fix, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
 }
 
-void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s) {
+void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
+                           const KernelTimer& t) {
   PixelInputs in{hitIdx[0], hitIdx[1], hitIdx[2], hitIdx[3], dPhiChange};
+  t.begin(s);
   hipLaunchKernelGGL(k_pixel_segments, dim3(grid_for(v.nPLS, 256)), dim3(256), 0, s, v, in);
+  t.end(s);
 }
 
-void launch_mini_doublets(const EventView& v, hipStream_t s) {
+void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+  t.begin(s);
   hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  t.end(s);
   hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
-void launch_segments(const EventView& v, hipStream_t s) {
+void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t) {
+  t.begin(s);
   hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  t.end(s);
   hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
-void launch_triplets(const EventView& v, hipStream_t s) {
+void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
+  t.begin(s);
   hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  t.end(s);
   hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
