@@ -1,0 +1,1030 @@
+// ORACLE — test infrastructure only (see orc_common.h).
+// Quintuplets (SDL/Quintuplet.h:143-3235, SDL/Event.cc:903-991) and the
+// after-build duplicate removal (SDL/Kernels.h:202-249), serial order, plain
+// chi2 mode (-DUSE_RZCHI2 -DUSE_RPHICHI2).
+//
+// Overload note: SDL/Quintuplet.h:412 calls an unqualified cos() on a float in
+// namespace SDL; with <cmath> only, the CPU build resolves it to the global
+// double cos(double), so Py is computed in double there and rounded once.
+#include "orc_event.h"
+#include "orc_t5.h"
+
+namespace orc {
+namespace {
+
+bool checkIntervalOverlap(float aMin, float aMax, float bMin, float bMax) {  // Quintuplet.h:143-148
+  return ((aMin <= bMin) && (bMin < aMax)) || ((bMin < aMin) && (aMin < bMax));
+}
+
+// SDL/Quintuplet.h:218-298 (90% chi2 constraint); the (1,2,3,4,x) cases are unreachable there
+bool passChiSquaredConstraint(int l1, int l2, int l3, int l4, int l5, float chi2) {
+  if (l1 == 7 && l2 == 8 && l3 == 9) {
+    if (l4 == 10 && l5 == 11) return chi2 < 0.01788f;
+    if (l4 == 10 && l5 == 16) return chi2 < 0.04725f;
+    if (l4 == 15 && l5 == 16) return chi2 < 0.04725f;
+  } else if (l1 == 1 && l2 == 7 && l3 == 8) {
+    if (l4 == 9 && l5 == 10) return chi2 < 0.01788f;
+    if (l4 == 9 && l5 == 15) return chi2 < 0.08234f;
+  } else if (l1 == 1 && l2 == 2 && l3 == 7) {
+    if (l4 == 8 && l5 == 9) return chi2 < 0.02360f;
+    if (l4 == 8 && l5 == 14) return chi2 < 0.07167f;
+    if (l4 == 13 && l5 == 14) return chi2 < 0.08234f;
+  } else if (l1 == 1 && l2 == 2 && l3 == 3) {
+    if (l4 == 7 && l5 == 8) return chi2 < 0.01026f;
+    if (l4 == 7 && l5 == 13) return chi2 < 0.06238f;
+    if (l4 == 12 && l5 == 13) return chi2 < 0.06238f;
+  } else if (l1 == 2 && l2 == 7 && l3 == 8) {
+    if (l4 == 9 && l5 == 10) return chi2 < 0.00512f;
+    if (l4 == 9 && l5 == 15) return chi2 < 0.04112f;
+    if (l4 == 14 && l5 == 15) return chi2 < 0.06238f;
+  } else if (l1 == 2 && l2 == 3 && l3 == 7) {
+    if (l4 == 8 && l5 == 14) return chi2 < 0.07167f;
+    if (l4 == 13 && l5 == 14) return chi2 < 0.06238f;
+  } else if (l1 == 2 && l2 == 3 && l3 == 4) {
+    if (l4 == 5 && l5 == 6) return chi2 < 0.08234f;
+    if (l4 == 5 && l5 == 12) return chi2 < 0.10870f;
+    if (l4 == 12 && l5 == 13) return chi2 < 0.10870f;
+  } else if (l1 == 3 && l2 == 7 && l3 == 8 && l4 == 14 && l5 == 15) {
+    return chi2 < 0.09461f;
+  } else if (l1 == 3 && l2 == 4 && l3 == 5 && l4 == 12 && l5 == 13) {
+    return chi2 < 0.09461f;
+  }
+  return true;
+}
+
+// SDL/Quintuplet.h:302-735
+bool passT5RZConstraint(const Event& ev, const unsigned md[5], const uint16_t mod[5], float& rzChiSquared,
+                        float inner_pt, float innerRadius, float g, float f, bool& TightCutFlag) {
+  const Geometry& G = *ev.g;
+  const float rt1 = ev.anchorRt[md[0]] / 100, rt2 = ev.anchorRt[md[1]] / 100, rt3 = ev.anchorRt[md[2]] / 100,
+              rt4 = ev.anchorRt[md[3]] / 100, rt5 = ev.anchorRt[md[4]] / 100;
+  const float z1 = ev.anchorZ[md[0]] / 100, z2 = ev.anchorZ[md[1]] / 100, z3 = ev.anchorZ[md[2]] / 100,
+              z4 = ev.anchorZ[md[3]] / 100, z5 = ev.anchorZ[md[4]] / 100;
+  const int layer1 = G.sdlLayers[mod[0]], layer2 = G.sdlLayers[mod[1]], layer3 = G.sdlLayers[mod[2]],
+            layer4 = G.sdlLayers[mod[3]], layer5 = G.sdlLayers[mod[4]];
+  const int moduleType1 = G.moduleType[mod[0]], moduleType2 = G.moduleType[mod[1]], moduleType3 = G.moduleType[mod[2]],
+            moduleType4 = G.moduleType[mod[3]], moduleType5 = G.moduleType[mod[4]];
+  const float x1 = ev.anchorX[md[0]] / 100, x2 = ev.anchorX[md[1]] / 100, x3 = ev.anchorX[md[2]] / 100,
+              x4 = ev.anchorX[md[3]] / 100;
+  const float y1 = ev.anchorY[md[0]] / 100, y2 = ev.anchorY[md[1]] / 100, y3 = ev.anchorY[md[2]] / 100,
+              y4 = ev.anchorY[md[3]] / 100;
+  float residual = 0, error = 0;
+  float x_center = g / 100, y_center = f / 100;
+  float x_init = ev.anchorX[md[2]] / 100, y_init = ev.anchorY[md[2]] / 100, z_init = ev.anchorZ[md[2]] / 100,
+        rt_init = ev.anchorRt[md[2]] / 100;
+  if (moduleType3 == 1) {
+    x_init = ev.anchorX[md[1]] / 100; y_init = ev.anchorY[md[1]] / 100;
+    z_init = ev.anchorZ[md[1]] / 100; rt_init = ev.anchorRt[md[1]] / 100;
+  }
+  int charge = 0;
+  float slope3c = (y3 - y_center) / (x3 - x_center);
+  float slope1c = (y1 - y_center) / (x1 - x_center);
+  if ((y3 - y_center) > 0 && (y1 - y_center) > 0) {
+    if (slope1c > 0 && slope3c < 0) charge = -1;
+    else if (slope1c < 0 && slope3c > 0) charge = 1;
+    else if (slope3c > slope1c) charge = -1;
+    else if (slope3c < slope1c) charge = 1;
+  } else if ((y3 - y_center) < 0 && (y1 - y_center) < 0) {
+    if (slope1c < 0 && slope3c > 0) charge = 1;
+    else if (slope1c > 0 && slope3c < 0) charge = -1;
+    else if (slope3c > slope1c) charge = -1;
+    else if (slope3c < slope1c) charge = 1;
+  } else if ((y3 - y_center) < 0 && (y1 - y_center) > 0) {
+    if ((x3 - x_center) > 0 && (x1 - x_center) > 0) charge = 1;
+    else if ((x3 - x_center) < 0 && (x1 - x_center) < 0) charge = -1;
+  } else if ((y3 - y_center) > 0 && (y1 - y_center) < 0) {
+    if ((x3 - x_center) > 0 && (x1 - x_center) > 0) charge = -1;
+    else if ((x3 - x_center) < 0 && (x1 - x_center) < 0) charge = 1;
+  }
+  float pseudo_phi = fatan((y_init - y_center) / (x_init - x_center));
+  float Pt = inner_pt, Px = Pt * std::fabs(fsin(pseudo_phi));
+  float Py = Pt * std::fabs(std::cos((double)pseudo_phi));  // double cos: see the note at the top
+  if (x_init > x_center && y_init > y_center) {
+    if (charge == 1) Py = -Py;
+    if (charge == -1) Px = -Px;
+  }
+  if (x_init < x_center && y_init > y_center) {
+    if (charge == -1) { Px = -Px; Py = -Py; }
+  }
+  if (x_init < x_center && y_init < y_center) {
+    if (charge == 1) Px = -Px;
+    if (charge == -1) Py = -Py;
+  }
+  if (x_init > x_center && y_init < y_center) {
+    if (charge == 1) { Px = -Px; Py = -Py; }
+  }
+  if (moduleType3 == 0) {
+    if (x4 < x3 && x3 < x2) Px = -std::fabs(Px);
+    if (x4 > x3 && x3 > x2) Px = std::fabs(Px);
+    if (y4 < y3 && y3 < y2) Py = -std::fabs(Py);
+    if (y4 > y3 && y3 > y2) Py = std::fabs(Py);
+  } else if (moduleType3 == 1) {
+    if (x3 < x2 && x2 < x1) Px = -std::fabs(Px);
+    if (x3 > x2 && x2 > x1) Px = std::fabs(Px);
+    if (y3 < y2 && y2 < y1) Py = -std::fabs(Py);
+    if (y3 > y2 && y2 > y1) Py = std::fabs(Py);
+  }
+  float AO = fsqrt((x1 - x_center) * (x1 - x_center) + (y1 - y_center) * (y1 - y_center));
+  float BO = fsqrt((x_init - x_center) * (x_init - x_center) + (y_init - y_center) * (y_init - y_center));
+  float AB = fsqrt((x1 - x_init) * (x1 - x_init) + (y1 - y_init) * (y1 - y_init));
+  float dPhi = facos((AO * AO + BO * BO - AB * AB) / (2 * AO * BO));
+  float ds = innerRadius / 100 * dPhi;
+  float Pz = (z_init - z1) / ds * Pt;
+  float p = fsqrt(Px * Px + Py * Py + Pz * Pz);
+  float Bz = magnetic_field;
+  float a = -0.299792 * Bz * charge;
+  float zsi = 0, rtsi = 0;
+  int layeri = 0, moduleTypei = 0;
+  rzChiSquared = 0;
+  for (size_t i = 2; i < 6; i++) {
+    if (i == 2) { zsi = z2; rtsi = rt2; layeri = layer2; moduleTypei = moduleType2; }
+    else if (i == 3) { zsi = z3; rtsi = rt3; layeri = layer3; moduleTypei = moduleType3; }
+    else if (i == 4) { zsi = z4; rtsi = rt4; layeri = layer4; moduleTypei = moduleType4; }
+    else { zsi = z5; rtsi = rt5; layeri = layer5; moduleTypei = moduleType5; }
+    if (moduleType3 == 0) { if (i == 3) continue; }
+    else { if (i == 2) continue; }
+    float diffr = 0, diffz = 0;
+    float rou = a / p;
+    float s = (zsi - z_init) * p / Pz;
+    float x = x_init + Px / a * fsin(rou * s) - Py / a * (1 - fcos(rou * s));
+    float y = y_init + Py / a * fsin(rou * s) + Px / a * (1 - fcos(rou * s));
+    diffr = (rtsi - fsqrt(x * x + y * y)) * 100;
+    if (layeri <= 6) {
+      float paraA = rt_init * rt_init + 2 * (Px * Px + Py * Py) / (a * a) + 2 * (y_init * Px - x_init * Py) / a - rtsi * rtsi;
+      float paraB = 2 * (x_init * Px + y_init * Py) / a;
+      float paraC = 2 * (y_init * Px - x_init * Py) / a + 2 * (Px * Px + Py * Py) / (a * a);
+      float A = paraB * paraB + paraC * paraC;
+      float B = 2 * paraA * paraB;
+      float C = paraA * paraA - paraC * paraC;
+      float sol1 = (-B + fsqrt(B * B - 4 * A * C)) / (2 * A);
+      float sol2 = (-B - fsqrt(B * B - 4 * A * C)) / (2 * A);
+      float solz1 = fasin(sol1) / rou * Pz / p + z_init;
+      float solz2 = fasin(sol2) / rou * Pz / p + z_init;
+      float diffz1 = (solz1 - zsi) * 100;
+      float diffz2 = (solz2 - zsi) * 100;
+      if (std::isnan(diffz1)) diffz = diffz2;
+      else if (std::isnan(diffz2)) diffz = diffz1;
+      else diffz = (std::fabs(diffz1) < std::fabs(diffz2)) ? diffz1 : diffz2;
+    }
+    residual = (layeri > 6) ? diffr : diffz;
+    error = moduleTypei == 0 ? 0.15f : 5.0f;
+    float drdz = 0;
+    short side = 0, subdets = 0;
+    if (i == 2) { drdz = std::fabs(G.drdzs[mod[1]]); side = G.sides[mod[1]]; subdets = G.subdets[mod[1]]; }
+    if (i == 3) { drdz = std::fabs(G.drdzs[mod[2]]); side = G.sides[mod[2]]; subdets = G.subdets[mod[2]]; }
+    if (i == 2 || i == 3) {
+      residual = (layeri <= 6 && ((side == Center) || (drdz < 1))) ? diffz : diffr;
+      float projection_missing = 1;
+      if (drdz < 1)
+        projection_missing = ((subdets == Endcap) || (side == Center)) ? 1.f : 1 / fsqrt(1 + drdz * drdz);
+      if (drdz > 1)
+        projection_missing = ((subdets == Endcap) || (side == Center)) ? 1.f : drdz / fsqrt(1 + drdz * drdz);
+      error = error * projection_missing;
+    }
+    rzChiSquared += 12 * (residual * residual) / (error * error);
+  }
+  if (inner_pt > 100 || std::isnan(rzChiSquared)) {
+    float slope;
+    if (moduleType1 == 0 && moduleType2 == 0 && moduleType3 == 1) slope = (z2 - z1) / (rt2 - rt1);
+    else slope = (z3 - z1) / (rt3 - rt1);
+    float r4 = (layer4 <= 6) ? ((z4 - z1) - slope * (rt4 - rt1)) : ((rt4 - rt1) - (z4 - z1) / slope);
+    float r5 = (layer4 <= 6) ? ((z5 - z1) - slope * (rt5 - rt1)) : ((rt5 - rt1) - (z5 - z1) / slope);
+    r4 = (moduleType4 == 0) ? r4 / 0.15f : r4 / 5.0f;
+    r5 = (moduleType5 == 0) ? r5 / 0.15f : r5 / 5.0f;
+    r4 = r4 * 100;
+    r5 = r5 * 100;
+    rzChiSquared = 12 * (r4 * r4 + r5 * r5);
+    return rzChiSquared < 4.677f;
+  }
+  TightCutFlag = false;
+  const float c = rzChiSquared;
+  if (layer1 == 7 && layer2 == 8 && layer3 == 9 && layer4 == 10 && layer5 == 11) {
+    if (c < 94.470f) TightCutFlag = true;
+    return true;
+  } else if (layer1 == 7 && layer2 == 8 && layer3 == 9 && layer4 == 10 && layer5 == 16) {
+    if (c < 22.099f) TightCutFlag = true;
+    return c < 37.956f;
+  } else if (layer1 == 7 && layer2 == 8 && layer3 == 9 && layer4 == 15 && layer5 == 16) {
+    if (c < 7.992f) TightCutFlag = true;
+    return c < 11.622f;
+  } else if (layer1 == 1 && layer2 == 7 && layer3 == 8 && layer4 == 9) {
+    if (layer5 == 10) {
+      if (c < 111.390f) TightCutFlag = true;
+      return true;
+    }
+    if (layer5 == 15) {
+      if (c < 18.351f) TightCutFlag = true;
+      return c < 37.941f;
+    }
+  } else if (layer1 == 1 && layer2 == 2 && layer3 == 7) {
+    if (layer4 == 8 && layer5 == 9) {
+      if (c < 116.148f) TightCutFlag = true;
+      return true;
+    }
+    if (layer4 == 8 && layer5 == 14) {
+      if (c < 19.352f) TightCutFlag = true;
+      return c < 52.561f;
+    } else if (layer4 == 13 && layer5 == 14) {
+      if (c < 10.392f) TightCutFlag = true;
+      return c < 13.76f;
+    }
+  } else if (layer1 == 1 && layer2 == 2 && layer3 == 3) {
+    if (layer4 == 7 && layer5 == 8) {
+      if (c < 27.824f) TightCutFlag = true;
+      return c < 44.247f;
+    } else if (layer4 == 7 && layer5 == 13) {
+      if (c < 18.145f) TightCutFlag = true;
+      return c < 33.752f;
+    } else if (layer4 == 12 && layer5 == 13) {
+      if (c < 13.308f) TightCutFlag = true;
+      return c < 21.213f;
+    } else if (layer4 == 4 && layer5 == 5) {
+      if (c < 15.627f) TightCutFlag = true;
+      return c < 29.035f;
+    } else if (layer4 == 4 && layer5 == 12) {
+      if (c < 14.64f) TightCutFlag = true;
+      return c < 23.037f;
+    }
+  } else if (layer1 == 2 && layer2 == 7 && layer3 == 8) {
+    if (layer4 == 9 && layer5 == 15) {
+      if (c < 24.662f) TightCutFlag = true;
+      return c < 41.036f;
+    } else if (layer4 == 14 && layer5 == 15) {
+      if (c < 8.866f) TightCutFlag = true;
+      return c < 14.092f;
+    }
+  } else if (layer1 == 2 && layer2 == 3 && layer3 == 7) {
+    if (layer4 == 8 && layer5 == 14) {
+      if (c < 23.730f) TightCutFlag = true;
+      return c < 23.748f;
+    }
+    if (layer4 == 13 && layer5 == 14) {
+      if (c < 10.772f) TightCutFlag = true;
+      return c < 17.945f;
+    }
+  } else if (layer1 == 2 && layer2 == 3 && layer3 == 4) {
+    if (layer4 == 5 && layer5 == 6) {
+      if (c < 6.065f) TightCutFlag = true;
+      return c < 8.803f;
+    } else if (layer4 == 5 && layer5 == 12) {
+      if (c < 5.693f) TightCutFlag = true;
+      return c < 7.930f;
+    } else if (layer4 == 12 && layer5 == 13) {
+      if (c < 5.473f) TightCutFlag = true;
+      return c < 7.626f;
+    }
+  }
+  return true;
+}
+
+// SDL/Quintuplet.h:753-782
+void computeErrorInRadius(const float* x1v, const float* y1v, const float* x2v, const float* y2v, const float* x3v,
+                          const float* y3v, float& rmin, float& rmax) {
+  float g, f;
+  rmin = SDL_INF;
+  rmax = 0.f;
+  for (size_t i = 0; i < 3; i++)
+    for (size_t j = 0; j < 3; j++)
+      for (size_t k = 0; k < 3; k++) {
+        float r = computeRadiusFromThreeAnchorHits(x1v[i], y1v[i], x2v[j], y2v[j], x3v[k], y3v[k], g, f);
+        rmax = fmaxf_(r, rmax);
+        rmin = fminf_(r, rmin);
+      }
+}
+
+// The nine matchRadii* functions of SDL/Quintuplet.h:785-1134 share one shape:
+// inverse-radius windows (1 +- bound)/R for the inner and bridge radii, the
+// "high pT" bounds when innerRadius > 2 GeV, and an overlap test in which some
+// categories widen the bridge (and inner) window by the 2S brute-force range.
+struct RadiiBounds {
+  float inner, bridge, innerHi, bridgeHi;
+  bool hasHigh, bridge2S, inner2S;
+};
+bool matchRadii(const RadiiBounds& b, float innerRadius, float bridgeRadius, float innerMin2S, float innerMax2S,
+                float bridgeMin2S, float bridgeMax2S) {
+  float ib = b.inner, bb = b.bridge;
+  if (b.hasHigh && innerRadius > 2.0f / (2.f * k2Rinv1GeVf)) { ib = b.innerHi; bb = b.bridgeHi; }
+  float innerInvMax = (1.f + ib) / innerRadius;
+  float innerInvMin = fmaxf_(0.f, (1.f - ib) / innerRadius);
+  float bridgeInvMax = (1.f + bb) / bridgeRadius;
+  float bridgeInvMin = fmaxf_(0.f, (1.f - bb) / bridgeRadius);
+  if (b.inner2S) {
+    // the 1.0 there is a double literal; fmin/fmax in double then rounded to float
+    innerInvMin = (float)std::fmin((double)innerInvMin, 1.0 / innerMax2S);
+    innerInvMax = (float)std::fmax((double)innerInvMax, 1.0 / innerMin2S);
+    bridgeInvMin = (float)std::fmin((double)bridgeInvMin, 1.0 / bridgeMax2S);
+    bridgeInvMax = (float)std::fmax((double)bridgeInvMax, 1.0 / bridgeMin2S);
+  } else if (b.bridge2S) {
+    bridgeInvMin = fminf_(bridgeInvMin, 1.0f / bridgeMax2S);
+    bridgeInvMax = fmaxf_(bridgeInvMax, 1.0f / bridgeMin2S);
+  }
+  return checkIntervalOverlap(innerInvMin, innerInvMax, bridgeInvMin, bridgeInvMax);
+}
+//                                           inner    bridge   innerHi  bridgeHi hasHigh bridge2S inner2S
+const RadiiBounds kBBBBB{0.1512f, 0.1781f, 0.4449f, 0.4033f, true, false, false};    // :822-853
+const RadiiBounds kBBBBE{0.1781f, 0.2167f, 0.4750f, 0.3903f, true, false, false};    // :855-892
+const RadiiBounds kBBBEE12378{0.178f, 0.507f, 0, 0, false, true, false};           // :785-820
+const RadiiBounds kBBBEE23478{0.2097f, 0.8557f, 0, 0, false, true, false};         // :937-971
+const RadiiBounds kBBBEE34578{0.066f, 0.617f, 0, 0, false, true, false};           // :973-1007
+const RadiiBounds kBBEEE{0.6376f, 2.1381f, 12.9173f, 5.1700f, true, true, false};    // :1009-1050
+const RadiiBounds kBEEEE{1.9382f, 3.7280f, 23.2713f, 21.7980f, true, true, true};   // :1052-1092
+const RadiiBounds kEEEEE{1.9382f, 2.2091f, 22.5226f, 21.0966f, true, true, true};   // :1094-1134
+
+// SDL/Quintuplet.h:1136-1221
+void computeSigmasForRegression(const Geometry& G, const uint16_t* mods, float* delta1, float* delta2, float* slopes,
+                                bool* isFlat, unsigned nPoints = 5, bool anchorHits = true) {
+  float inv1 = 0.01f / 0.009f;
+  float inv2 = 0.15f / 0.009f;
+  float inv3 = 2.4f / 0.009f;
+  for (size_t i = 0; i < nPoints; i++) {
+    int type = G.moduleType[mods[i]];
+    short subdet = G.subdets[mods[i]], side = G.sides[mods[i]];
+    const float drdz = G.drdzs[mods[i]];
+    slopes[i] = G.dxdys[mods[i]];
+    if (subdet == Barrel && type == PS && side == Center) {
+      delta1[i] = inv1; delta2[i] = inv1; slopes[i] = -999.f; isFlat[i] = true;
+    } else if (subdet == Barrel && type == TwoS) {
+      delta1[i] = 1.f; delta2[i] = 1.f; slopes[i] = -999.f; isFlat[i] = true;
+    } else if (subdet == Barrel && type == PS && side != Center) {
+      delta1[i] = inv1; isFlat[i] = false;
+      delta2[i] = anchorHits ? (inv2 * drdz / fsqrt(1 + drdz * drdz)) : (inv3 * drdz / fsqrt(1 + drdz * drdz));
+    } else if (subdet == Endcap && type == PS) {
+      delta1[i] = inv1; isFlat[i] = false;
+      delta2[i] = anchorHits ? inv2 : inv3;
+    } else if (subdet == Endcap && type == TwoS) {
+      delta1[i] = 1.f; delta2[i] = 500.f * inv1; isFlat[i] = false;
+    }
+  }
+}
+
+float angleM_of(float slope, float x, float y) {  // shared by :1254-1267 and :1337-1350
+  float absArctanSlope = (slope != SDL_INF) ? std::fabs(fatan(slope)) : 0.5f * float(M_PI);
+  if (x > 0 && y > 0) return 0.5f * float(M_PI) - absArctanSlope;
+  if (x < 0 && y > 0) return absArctanSlope + 0.5f * float(M_PI);
+  if (x < 0 && y < 0) return -(absArctanSlope + 0.5f * float(M_PI));
+  if (x > 0 && y < 0) return -(0.5f * float(M_PI) - absArctanSlope);
+  return 0;
+}
+
+// SDL/Quintuplet.h:1223-1320
+float computeRadiusUsingRegression(unsigned nPoints, const float* xs, const float* ys, const float* delta1,
+                                   const float* delta2, const float* slopes, const bool* isFlat, float& g, float& f,
+                                   float* sigmas, float& chiSquared) {
+  float sX1Sq = 0.f, sX2Sq = 0.f, sX1X2 = 0.f, sX1y = 0.f, sX2y = 0.f, sY = 0.f, sX1 = 0.f, sX2 = 0.f, sOne = 0.f;
+  for (size_t i = 0; i < nPoints; i++) {
+    float angleM = angleM_of(slopes[i], xs[i], ys[i]);
+    float xPrime, yPrime;
+    if (!isFlat[i]) {
+      xPrime = xs[i] * fcos(angleM) + ys[i] * fsin(angleM);
+      yPrime = ys[i] * fcos(angleM) - xs[i] * fsin(angleM);
+    } else {
+      xPrime = xs[i];
+      yPrime = ys[i];
+    }
+    sigmas[i] = 2 * fsqrt((xPrime * delta1[i]) * (xPrime * delta1[i]) + (yPrime * delta2[i]) * (yPrime * delta2[i]));
+    sX1Sq += (xs[i] * xs[i]) / (sigmas[i] * sigmas[i]);
+    sX2Sq += (ys[i] * ys[i]) / (sigmas[i] * sigmas[i]);
+    sX1X2 += (xs[i] * ys[i]) / (sigmas[i] * sigmas[i]);
+    sX1y += (xs[i] * (xs[i] * xs[i] + ys[i] * ys[i])) / (sigmas[i] * sigmas[i]);
+    sX2y += (ys[i] * (xs[i] * xs[i] + ys[i] * ys[i])) / (sigmas[i] * sigmas[i]);
+    sY += (xs[i] * xs[i] + ys[i] * ys[i]) / (sigmas[i] * sigmas[i]);
+    sX1 += xs[i] / (sigmas[i] * sigmas[i]);
+    sX2 += ys[i] / (sigmas[i] * sigmas[i]);
+    sOne += 1.0f / (sigmas[i] * sigmas[i]);
+  }
+  float denominator = (sX1X2 - sX1 * sX2) * (sX1X2 - sX1 * sX2) - (sX1Sq - sX1 * sX1) * (sX2Sq - sX2 * sX2);
+  float twoG = ((sX2y - sX2 * sY) * (sX1X2 - sX1 * sX2) - (sX1y - sX1 * sY) * (sX2Sq - sX2 * sX2)) / denominator;
+  float twoF = ((sX1y - sX1 * sY) * (sX1X2 - sX1 * sX2) - (sX2y - sX2 * sY) * (sX1Sq - sX1 * sX1)) / denominator;
+  float c = -(sY - twoG * sX1 - twoF * sX2) / sOne;
+  g = 0.5f * twoG;
+  f = 0.5f * twoF;
+  if (g * g + f * f - c < 0) {
+    chiSquared = -1;
+    return -1;
+  }
+  float radius = fsqrt(g * g + f * f - c);
+  chiSquared = 0.f;
+  for (size_t i = 0; i < nPoints; i++) {
+    chiSquared += (xs[i] * xs[i] + ys[i] * ys[i] - twoG * xs[i] - twoF * ys[i] + c) *
+                  (xs[i] * xs[i] + ys[i] * ys[i] - twoG * xs[i] - twoF * ys[i] + c) / (sigmas[i] * sigmas[i]);
+  }
+  return radius;
+}
+
+// SDL/Quintuplet.h:1322-1367
+float computeChiSquared(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
+                        const float* slopes, const bool* isFlat, float g, float f, float radius) {
+  float c = g * g + f * f - radius * radius;
+  float chiSquared = 0.f;
+  for (size_t i = 0; i < nPoints; i++) {
+    float angleM = angleM_of(slopes[i], xs[i], ys[i]);
+    float xPrime, yPrime;
+    if (!isFlat[i]) {
+      xPrime = xs[i] * fcos(angleM) + ys[i] * fsin(angleM);
+      yPrime = ys[i] * fcos(angleM) - xs[i] * fsin(angleM);
+    } else {
+      xPrime = xs[i];
+      yPrime = ys[i];
+    }
+    float sigma = 2 * fsqrt((xPrime * delta1[i]) * (xPrime * delta1[i]) + (yPrime * delta2[i]) * (yPrime * delta2[i]));
+    chiSquared += (xs[i] * xs[i] + ys[i] * ys[i] - 2 * g * xs[i] - 2 * f * ys[i] + c) *
+                  (xs[i] * xs[i] + ys[i] * ys[i] - 2 * g * xs[i] - 2 * f * ys[i] + c) / (sigma * sigma);
+  }
+  return chiSquared;
+}
+
+}  // namespace
+
+// SDL/Quintuplet.h:1370-1464
+void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, float& pt_beta, float sdIn_dr,
+                              float sdOut_dr, float dr, float lIn) {
+  if (lIn == 0) {
+    betaOut += copysignf_(fasin(fminf_(sdOut_dr * k2Rinv1GeVf / std::fabs(pt_beta), sinAlphaMax)), betaOut);
+    return;
+  }
+  if (betaIn * betaOut > 0.f &&
+      (std::fabs(pt_beta) < 4.f * pt_betaMax || (lIn >= 11 && std::fabs(pt_beta) < 8.f * pt_betaMax))) {
+    const float betaInUpd = betaIn + copysignf_(fasin(fminf_(sdIn_dr * k2Rinv1GeVf / std::fabs(pt_beta), sinAlphaMax)), betaIn);
+    const float betaOutUpd =
+        betaOut + copysignf_(fasin(fminf_(sdOut_dr * k2Rinv1GeVf / std::fabs(pt_beta), sinAlphaMax)), betaOut);
+    betaAv = 0.5f * (betaInUpd + betaOutUpd);
+    const float pt_beta_inv = 1.f / std::fabs(dr * k2Rinv1GeVf / fsin(betaAv));
+    betaIn += copysignf_(fasin(fminf_(sdIn_dr * k2Rinv1GeVf * pt_beta_inv, sinAlphaMax)), betaIn);
+    betaOut += copysignf_(fasin(fminf_(sdOut_dr * k2Rinv1GeVf * pt_beta_inv, sinAlphaMax)), betaOut);
+    betaAv = 0.5f * (betaIn + betaOut);
+    pt_beta = dr * k2Rinv1GeVf / fsin(betaAv);
+  } else if (lIn < 11 && std::fabs(betaOut) < 0.2f * std::fabs(betaIn) && std::fabs(pt_beta) < 12.f * pt_betaMax) {
+    const float pt_betaIn = dr * k2Rinv1GeVf / fsin(betaIn);
+    const float betaInUpd =
+        betaIn + copysignf_(fasin(fminf_(sdIn_dr * k2Rinv1GeVf / std::fabs(pt_betaIn), sinAlphaMax)), betaIn);
+    const float betaOutUpd =
+        betaOut + copysignf_(fasin(fminf_(sdOut_dr * k2Rinv1GeVf / std::fabs(pt_betaIn), sinAlphaMax)), betaIn);
+    betaAv = (std::fabs(betaOut) > 0.2f * std::fabs(betaIn)) ? (0.5f * (betaInUpd + betaOutUpd)) : betaInUpd;
+    pt_beta = dr * k2Rinv1GeVf / fsin(betaAv);
+    betaIn += copysignf_(fasin(fminf_(sdIn_dr * k2Rinv1GeVf / std::fabs(pt_beta), sinAlphaMax)), betaIn);
+    betaOut += copysignf_(fasin(fminf_(sdOut_dr * k2Rinv1GeVf / std::fabs(pt_beta), sinAlphaMax)), betaIn);
+    betaAv = 0.5f * (betaIn + betaOut);
+    pt_beta = dr * k2Rinv1GeVf / fsin(betaAv);
+  }
+}
+
+namespace {
+
+struct T4In {  // the four mini-doublets and two segments of a T4-style test
+  uint16_t m1, m2, m3, m4;
+  unsigned seg1, seg3, md1, md2, md3, md4;
+};
+
+// SDL/Quintuplet.h:1467-1742 (barrel inner segment; also used for BBBE)
+bool algoBBBB(const Event& ev, const T4In& q) {
+  const Geometry& G = *ev.g;
+  const float ptCut = ev.cfg.ptCut;
+  bool isPS_InLo = G.moduleType[q.m1] == PS, isPS_OutLo = G.moduleType[q.m3] == PS;
+  float rt_InLo = ev.anchorRt[q.md1], rt_InOut = ev.anchorRt[q.md2], rt_OutLo = ev.anchorRt[q.md3];
+  float z_InLo = ev.anchorZ[q.md1], z_InOut = ev.anchorZ[q.md2], z_OutLo = ev.anchorZ[q.md3];
+  float alpha1GeV_OutLo = fasin(fminf_(rt_OutLo * k2Rinv1GeVf / ptCut, sinAlphaMax));
+  float rtRatio = rt_OutLo / rt_InLo;
+  float dzDrtScale = ftan(alpha1GeV_OutLo) / alpha1GeV_OutLo;
+  float zpitch_InLo = isPS_InLo ? pixelPSZpitch : strip2SZpitch;
+  float zpitch_OutLo = isPS_OutLo ? pixelPSZpitch : strip2SZpitch;
+  float zHi = z_InLo + (z_InLo + deltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpitch_InLo + zpitch_OutLo);
+  float zLo = z_InLo + (z_InLo - deltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpitch_InLo + zpitch_OutLo);
+  if (!((z_OutLo >= zLo) && (z_OutLo <= zHi))) return false;
+  float drt_OutLo_InLo = rt_OutLo - rt_InLo;
+  float r3_InLo = fsqrt(z_InLo * z_InLo + rt_InLo * rt_InLo);
+  float drt_InSeg = rt_InOut - rt_InLo;
+  float dz_InSeg = z_InOut - z_InLo;
+  float dr3_InSeg = fsqrt(rt_InOut * rt_InOut + z_InOut * z_InOut) - fsqrt(rt_InLo * rt_InLo + z_InLo * z_InLo);
+  float coshEta = dr3_InSeg / drt_InSeg;
+  float dzErr = (zpitch_InLo + zpitch_OutLo) * (zpitch_InLo + zpitch_OutLo) * 2.f;
+  float sdlThetaMulsF = 0.015f * fsqrt(0.1f + 0.2f * (rt_OutLo - rt_InLo) / 50.f) * fsqrt(r3_InLo / rt_InLo);
+  float sdlMuls = sdlThetaMulsF * 3.f / ptCut * 4.f;
+  dzErr += sdlMuls * sdlMuls * drt_OutLo_InLo * drt_OutLo_InLo / 3.f * coshEta * coshEta;
+  dzErr = fsqrt(dzErr);
+  const float dzMean = dz_InSeg / drt_InSeg * drt_OutLo_InLo;
+  const float zWindow = dzErr / drt_InSeg * drt_OutLo_InLo + (zpitch_InLo + zpitch_OutLo);
+  float zLoPointed = z_InLo + dzMean * (z_InLo > 0.f ? 1.f : dzDrtScale) - zWindow;
+  float zHiPointed = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
+  if (!((z_OutLo >= zLoPointed) && (z_OutLo <= zHiPointed))) return false;
+  float sdlPVoff = 0.1f / rt_OutLo;
+  float sdlCut = alpha1GeV_OutLo + fsqrt(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff);
+  float deltaPhiPos = phi_mpi_pi(ev.anchorPhi[q.md4] - ev.anchorPhi[q.md2]);
+  if (!(std::fabs(deltaPhiPos) <= sdlCut)) return false;
+  float midPointX = 0.5f * (ev.anchorX[q.md1] + ev.anchorX[q.md3]);
+  float midPointY = 0.5f * (ev.anchorY[q.md1] + ev.anchorY[q.md3]);
+  float diffX = ev.anchorX[q.md3] - ev.anchorX[q.md1];
+  float diffY = ev.anchorY[q.md3] - ev.anchorY[q.md1];
+  float dPhi = deltaPhi(midPointX, midPointY, diffX, diffY);
+  if (!(std::fabs(dPhi) <= sdlCut)) return false;
+  float alpha_InLo = ev.segDPhiChanges[q.seg1];
+  float alpha_OutLo = ev.segDPhiChanges[q.seg3];
+  bool isEC_lastLayer = G.subdets[q.m4] == Endcap && G.moduleType[q.m4] == TwoS;
+  float alpha_OutUp = phi_mpi_pi(phi(ev.anchorX[q.md4] - ev.anchorX[q.md3], ev.anchorY[q.md4] - ev.anchorY[q.md3]) - ev.anchorPhi[q.md4]);
+  float alpha_OutUp_highEdge = alpha_OutUp, alpha_OutUp_lowEdge = alpha_OutUp;
+  float tl_axis_x = ev.anchorX[q.md4] - ev.anchorX[q.md1];
+  float tl_axis_y = ev.anchorY[q.md4] - ev.anchorY[q.md1];
+  float tl_axis_highEdge_x = tl_axis_x, tl_axis_highEdge_y = tl_axis_y;
+  float tl_axis_lowEdge_x = tl_axis_x, tl_axis_lowEdge_y = tl_axis_y;
+  float betaIn = alpha_InLo - phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md1]);
+  float betaInRHmin = betaIn, betaInRHmax = betaIn;
+  float betaOut = -alpha_OutUp + phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md4]);
+  float betaOutRHmin = betaOut, betaOutRHmax = betaOut;
+  if (isEC_lastLayer) {
+    alpha_OutUp_highEdge = phi_mpi_pi(phi(ev.anchorHighEdgeX[q.md4] - ev.anchorX[q.md3], ev.anchorHighEdgeY[q.md4] - ev.anchorY[q.md3]) - ev.anchorHighEdgePhi[q.md4]);
+    alpha_OutUp_lowEdge = phi_mpi_pi(phi(ev.anchorLowEdgeX[q.md4] - ev.anchorX[q.md3], ev.anchorLowEdgeY[q.md4] - ev.anchorY[q.md3]) - ev.anchorLowEdgePhi[q.md4]);
+    tl_axis_highEdge_x = ev.anchorHighEdgeX[q.md4] - ev.anchorX[q.md1];
+    tl_axis_highEdge_y = ev.anchorHighEdgeY[q.md4] - ev.anchorY[q.md1];
+    tl_axis_lowEdge_x = ev.anchorLowEdgeX[q.md4] - ev.anchorX[q.md1];
+    tl_axis_lowEdge_y = ev.anchorLowEdgeY[q.md4] - ev.anchorY[q.md1];
+    betaOutRHmin = -alpha_OutUp_highEdge + phi_mpi_pi(phi(tl_axis_highEdge_x, tl_axis_highEdge_y) - ev.anchorHighEdgePhi[q.md4]);
+    betaOutRHmax = -alpha_OutUp_lowEdge + phi_mpi_pi(phi(tl_axis_lowEdge_x, tl_axis_lowEdge_y) - ev.anchorLowEdgePhi[q.md4]);
+  }
+  float drt_tl_axis = fsqrt(tl_axis_x * tl_axis_x + tl_axis_y * tl_axis_y);
+  float corrF = 1.f;
+  float dxIn = ev.anchorX[q.md2] - ev.anchorX[q.md1], dyIn = ev.anchorY[q.md2] - ev.anchorY[q.md1];
+  const float rt_InSeg = fsqrt(dxIn * dxIn + dyIn * dyIn);
+  float betaInCut = fasin(fminf_((-rt_InSeg * corrF + drt_tl_axis) * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / drt_InSeg);
+  if (!(std::fabs(betaInRHmin) < betaInCut)) return false;
+  float betaAv = 0.5f * (betaIn + betaOut);
+  float pt_beta = drt_tl_axis * k2Rinv1GeVf / fsin(betaAv);
+  int lIn = 5;
+  int lOut = isEC_lastLayer ? 11 : 5;
+  float dxOut = ev.anchorX[q.md4] - ev.anchorX[q.md3], dyOut = ev.anchorY[q.md4] - ev.anchorY[q.md3];
+  float sdOut_dr = fsqrt(dxOut * dxOut + dyOut * dyOut);
+  float sdOut_d = ev.anchorRt[q.md4] - ev.anchorRt[q.md3];
+  runDeltaBetaIterationsT5(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, lIn);
+  const float betaInMMSF = (std::fabs(betaInRHmin + betaInRHmax) > 0) ? (2.f * betaIn / std::fabs(betaInRHmin + betaInRHmax)) : 0.f;
+  const float betaOutMMSF = (std::fabs(betaOutRHmin + betaOutRHmax) > 0) ? (2.f * betaOut / std::fabs(betaOutRHmin + betaOutRHmax)) : 0.f;
+  betaInRHmin *= betaInMMSF; betaInRHmax *= betaInMMSF;
+  betaOutRHmin *= betaOutMMSF; betaOutRHmax *= betaOutMMSF;
+  const float dBetaMuls = sdlThetaMulsF * 4.f / fminf_(std::fabs(pt_beta), pt_betaMax);
+  const float alphaInAbsReg = fmaxf_(std::fabs(alpha_InLo), fasin(fminf_(rt_InLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float alphaOutAbsReg = fmaxf_(std::fabs(alpha_OutLo), fasin(fminf_(rt_OutLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float dBetaInLum = lIn < 11 ? 0.0f : std::fabs(alphaInAbsReg * deltaZLum / z_InLo);
+  const float dBetaOutLum = lOut < 11 ? 0.0f : std::fabs(alphaOutAbsReg * deltaZLum / z_OutLo);
+  const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+  const float sinDPhi = fsin(dPhi);
+  const float dBetaRIn2 = 0;
+  float dBetaROut = 0;
+  if (isEC_lastLayer) {
+    dBetaROut = (fsqrt(ev.anchorHighEdgeX[q.md4] * ev.anchorHighEdgeX[q.md4] + ev.anchorHighEdgeY[q.md4] * ev.anchorHighEdgeY[q.md4]) -
+                 fsqrt(ev.anchorLowEdgeX[q.md4] * ev.anchorLowEdgeX[q.md4] + ev.anchorLowEdgeY[q.md4] * ev.anchorLowEdgeY[q.md4])) *
+                sinDPhi / drt_tl_axis;
+  }
+  const float dBetaROut2 = dBetaROut * dBetaROut;
+  float betaOutCut = fasin(fminf_(drt_tl_axis * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / sdOut_d) +
+                     fsqrt(dBetaLum2 + dBetaMuls * dBetaMuls);
+  if (!(std::fabs(betaOut) < betaOutCut)) return false;
+  float dBetaRes = 0.02f / fminf_(sdOut_d, drt_InSeg);
+  float spread = std::fabs(betaInRHmin - betaInRHmax) + std::fabs(betaOutRHmin - betaOutRHmax);
+  float dBetaCut2 = (dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + dBetaRIn2 + dBetaROut2 + 0.25f * spread * spread);
+  float dBeta = betaIn - betaOut;
+  return dBeta * dBeta <= dBetaCut2;
+}
+
+// SDL/Quintuplet.h:1744-2028 (barrel-endcap transition; also used for BEEE)
+bool algoBBEE(const Event& ev, const T4In& q) {
+  const Geometry& G = *ev.g;
+  const float ptCut = ev.cfg.ptCut;
+  bool isPS_InLo = G.moduleType[q.m1] == PS, isPS_OutLo = G.moduleType[q.m3] == PS;
+  float rt_InLo = ev.anchorRt[q.md1], rt_InOut = ev.anchorRt[q.md2], rt_OutLo = ev.anchorRt[q.md3];
+  float z_InLo = ev.anchorZ[q.md1], z_InOut = ev.anchorZ[q.md2], z_OutLo = ev.anchorZ[q.md3];
+  float alpha1GeV_OutLo = fasin(fminf_(rt_OutLo * k2Rinv1GeVf / ptCut, sinAlphaMax));
+  float dzDrtScale = ftan(alpha1GeV_OutLo) / alpha1GeV_OutLo;
+  float zpitch_InLo = isPS_InLo ? pixelPSZpitch : strip2SZpitch;
+  float zpitch_OutLo = isPS_OutLo ? pixelPSZpitch : strip2SZpitch;
+  float zGeom = zpitch_InLo + zpitch_OutLo;
+  if (!(z_InLo * z_OutLo > 0)) return false;
+  float dLum = copysignf_(deltaZLum, z_InLo);
+  bool isOutSgInnerMDPS = G.moduleType[q.m3] == PS;
+  float rtGeom1 = isOutSgInnerMDPS ? pixelPSZpitch : strip2SZpitch;
+  float zGeom1 = copysignf_(zGeom, z_InLo);
+  float rtLo = rt_InLo * (1.f + (z_OutLo - z_InLo - zGeom1) / (z_InLo + zGeom1 + dLum) / dzDrtScale) - rtGeom1;
+  float rtOut = rt_OutLo;
+  if (!(rtOut >= rtLo)) return false;
+  float zInForHi = z_InLo - zGeom1 - dLum;
+  if (zInForHi * z_InLo < 0) zInForHi = copysignf_(0.1f, z_InLo);
+  float rtHi = rt_InLo * (1.f + (z_OutLo - z_InLo + zGeom1) / zInForHi) + rtGeom1;
+  if (!((rt_OutLo >= rtLo) && (rt_OutLo <= rtHi))) return false;
+  float rIn = fsqrt(z_InLo * z_InLo + rt_InLo * rt_InLo);
+  const float dzSDIn = z_InOut - z_InLo;
+  const float kZ = (z_OutLo - z_InLo) / dzSDIn;
+  const float sdlThetaMulsF = 0.015f * fsqrt(0.1f + 0.2f * (rt_OutLo - rt_InLo) / 50.f) * fsqrt(rIn / rt_InLo);
+  const float sdlMuls = sdlThetaMulsF * 3.f / ptCut * 4.f;
+  if (!((kZ >= 0) && (rtOut >= rtLo) && (rtOut <= rtHi))) return false;
+  const float sdlPVoff = 0.1f / rt_OutLo;
+  float sdlCut = alpha1GeV_OutLo + fsqrt(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff);
+  float deltaPhiPos = phi_mpi_pi(ev.anchorPhi[q.md4] - ev.anchorPhi[q.md2]);
+  if (!(std::fabs(deltaPhiPos) <= sdlCut)) return false;
+  float midPointX = 0.5f * (ev.anchorX[q.md1] + ev.anchorX[q.md3]);
+  float midPointY = 0.5f * (ev.anchorY[q.md1] + ev.anchorY[q.md3]);
+  float diffX = ev.anchorX[q.md3] - ev.anchorX[q.md1];
+  float diffY = ev.anchorY[q.md3] - ev.anchorY[q.md1];
+  float dPhi = deltaPhi(midPointX, midPointY, diffX, diffY);
+  if (!(std::fabs(dPhi) <= sdlCut)) return false;
+  float sdIn_alpha = ev.segDPhiChanges[q.seg1];
+  float sdIn_alpha_min = ev.segDPhiChangeMins[q.seg1];
+  float sdIn_alpha_max = ev.segDPhiChangeMaxs[q.seg1];
+  float sdOut_alpha = sdIn_alpha;
+  float sdOut_alphaOut = phi_mpi_pi(phi(ev.anchorX[q.md4] - ev.anchorX[q.md3], ev.anchorY[q.md4] - ev.anchorY[q.md3]) - ev.anchorPhi[q.md4]);
+  float sdOut_alphaOut_min = phi_mpi_pi(ev.segDPhiChangeMins[q.seg3] - ev.segDPhiMins[q.seg3]);
+  float sdOut_alphaOut_max = phi_mpi_pi(ev.segDPhiChangeMaxs[q.seg3] - ev.segDPhiMaxs[q.seg3]);
+  float tl_axis_x = ev.anchorX[q.md4] - ev.anchorX[q.md1];
+  float tl_axis_y = ev.anchorY[q.md4] - ev.anchorY[q.md1];
+  float betaIn = sdIn_alpha - phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md1]);
+  float betaInRHmin = betaIn, betaInRHmax = betaIn;
+  float betaOut = -sdOut_alphaOut + phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md4]);
+  float betaOutRHmin = betaOut, betaOutRHmax = betaOut;
+  bool isEC_secondLayer = (G.subdets[q.m2] == Endcap) && (G.moduleType[q.m2] == TwoS);
+  if (isEC_secondLayer) {
+    betaInRHmin = betaIn - sdIn_alpha_min + sdIn_alpha;
+    betaInRHmax = betaIn - sdIn_alpha_max + sdIn_alpha;
+  }
+  betaOutRHmin = betaOut - sdOut_alphaOut_min + sdOut_alphaOut;
+  betaOutRHmax = betaOut - sdOut_alphaOut_max + sdOut_alphaOut;
+  if (std::fabs(betaOutRHmin) > std::fabs(betaOutRHmax)) std::swap(betaOutRHmin, betaOutRHmax);
+  if (std::fabs(betaInRHmin) > std::fabs(betaInRHmax)) std::swap(betaInRHmin, betaInRHmax);
+  float dxIn = ev.anchorX[q.md2] - ev.anchorX[q.md1], dyIn = ev.anchorY[q.md2] - ev.anchorY[q.md1];
+  float sdIn_dr = fsqrt(dxIn * dxIn + dyIn * dyIn);
+  float sdIn_d = rt_InOut - rt_InLo;
+  float dr = fsqrt(tl_axis_x * tl_axis_x + tl_axis_y * tl_axis_y);
+  const float corrF = 1.f;
+  float betaInCut = fasin(fminf_((-sdIn_dr * corrF + dr) * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / sdIn_d);
+  if (!(std::fabs(betaInRHmin) < betaInCut)) return false;
+  float betaAv = 0.5f * (betaIn + betaOut);
+  float pt_beta = dr * k2Rinv1GeVf / fsin(betaAv);
+  float lIn = 5;
+  float lOut = 11;
+  float dxOut = ev.anchorX[q.md4] - ev.anchorX[q.md3], dyOut = ev.anchorY[q.md4] - ev.anchorY[q.md3];
+  float sdOut_dr = fsqrt(dxOut * dxOut + dyOut * dyOut);
+  float sdOut_d = ev.anchorRt[q.md4] - ev.anchorRt[q.md3];
+  runDeltaBetaIterationsT5(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn);
+  const float betaInMMSF = (std::fabs(betaInRHmin + betaInRHmax) > 0) ? (2.f * betaIn / std::fabs(betaInRHmin + betaInRHmax)) : 0.;
+  const float betaOutMMSF = (std::fabs(betaOutRHmin + betaOutRHmax) > 0) ? (2.f * betaOut / std::fabs(betaOutRHmin + betaOutRHmax)) : 0.;
+  betaInRHmin *= betaInMMSF; betaInRHmax *= betaInMMSF;
+  betaOutRHmin *= betaOutMMSF; betaOutRHmax *= betaOutMMSF;
+  const float dBetaMuls = sdlThetaMulsF * 4.f / fminf_(std::fabs(pt_beta), pt_betaMax);
+  const float alphaInAbsReg = fmaxf_(std::fabs(sdIn_alpha), fasin(fminf_(rt_InLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float alphaOutAbsReg = fmaxf_(std::fabs(sdOut_alpha), fasin(fminf_(rt_OutLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float dBetaInLum = lIn < 11 ? 0.0f : std::fabs(alphaInAbsReg * deltaZLum / z_InLo);
+  const float dBetaOutLum = lOut < 11 ? 0.0f : std::fabs(alphaOutAbsReg * deltaZLum / z_OutLo);
+  const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+  const float sinDPhi = fsin(dPhi);
+  const float dBetaRIn2 = 0;
+  float dBetaROut = 0;
+  if (G.moduleType[q.m4] == TwoS) {
+    dBetaROut = (fsqrt(ev.anchorHighEdgeX[q.md4] * ev.anchorHighEdgeX[q.md4] + ev.anchorHighEdgeY[q.md4] * ev.anchorHighEdgeY[q.md4]) -
+                 fsqrt(ev.anchorLowEdgeX[q.md4] * ev.anchorLowEdgeX[q.md4] + ev.anchorLowEdgeY[q.md4] * ev.anchorLowEdgeY[q.md4])) *
+                sinDPhi / dr;
+  }
+  const float dBetaROut2 = dBetaROut * dBetaROut;
+  float betaOutCut = fasin(fminf_(dr * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / sdOut_d) + fsqrt(dBetaLum2 + dBetaMuls * dBetaMuls);
+  if (!(std::fabs(betaOut) < betaOutCut)) return false;
+  float dBetaRes = 0.02f / fminf_(sdOut_d, sdIn_d);
+  float spread = std::fabs(betaInRHmin - betaInRHmax) + std::fabs(betaOutRHmin - betaOutRHmax);
+  float dBetaCut2 = (dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + dBetaRIn2 + dBetaROut2 + 0.25f * spread * spread);
+  float dBeta = betaIn - betaOut;
+  return dBeta * dBeta <= dBetaCut2;
+}
+
+// SDL/Quintuplet.h:2030-2298 (endcap only)
+bool algoEEEE(const Event& ev, const T4In& q) {
+  const Geometry& G = *ev.g;
+  const float ptCut = ev.cfg.ptCut;
+  bool isPS_InLo = G.moduleType[q.m1] == PS, isPS_OutLo = G.moduleType[q.m3] == PS;
+  float rt_InLo = ev.anchorRt[q.md1], rt_InOut = ev.anchorRt[q.md2], rt_OutLo = ev.anchorRt[q.md3];
+  float z_InLo = ev.anchorZ[q.md1], z_InOut = ev.anchorZ[q.md2], z_OutLo = ev.anchorZ[q.md3];
+  float alpha1GeV_OutLo = fasin(fminf_(rt_OutLo * k2Rinv1GeVf / ptCut, sinAlphaMax));
+  float dzDrtScale = ftan(alpha1GeV_OutLo) / alpha1GeV_OutLo;
+  (void)isPS_OutLo;
+  if (!((z_InLo * z_OutLo) > 0)) return false;
+  float dLum = copysignf_(deltaZLum, z_InLo);
+  bool isOutSgInnerMDPS = G.moduleType[q.m3] == PS;
+  bool isInSgInnerMDPS = G.moduleType[q.m1] == PS;
+  float rtGeom = (isInSgInnerMDPS && isOutSgInnerMDPS)   ? 2.f * pixelPSZpitch
+                 : (isInSgInnerMDPS || isOutSgInnerMDPS) ? pixelPSZpitch + strip2SZpitch
+                                                         : 2.f * strip2SZpitch;
+  float dz = z_OutLo - z_InLo;
+  float rtLo = rt_InLo * (1.f + dz / (z_InLo + dLum) / dzDrtScale) - rtGeom;
+  float rtOut = rt_OutLo;
+  float rtHi = rt_InLo * (1.f + dz / (z_InLo - dLum)) + rtGeom;
+  if (!((rtOut >= rtLo) && (rtOut <= rtHi))) return false;
+  bool isInSgOuterMDPS = G.moduleType[q.m2] == PS;
+  const float drtSDIn = rt_InOut - rt_InLo;
+  const float dzSDIn = z_InOut - z_InLo;
+  const float dr3SDIn = fsqrt(rt_InOut * rt_InOut + z_InOut * z_InOut) - fsqrt(rt_InLo * rt_InLo + z_InLo * z_InLo);
+  float coshEta = dr3SDIn / drtSDIn;
+  float dzOutInAbs = std::fabs(z_OutLo - z_InLo);
+  float multDzDr = dzOutInAbs * coshEta / (coshEta * coshEta - 1.f);
+  float kZ = (z_OutLo - z_InLo) / dzSDIn;
+  float sdlThetaMulsF = 0.015f * fsqrt(0.1f + 0.2f * (rt_OutLo - rt_InLo) / 50.f);
+  float sdlMuls = sdlThetaMulsF * 3.f / ptCut * 4.f;
+  float drtErr = fsqrt(pixelPSZpitch * pixelPSZpitch * 2.f / (dzSDIn * dzSDIn) * (dzOutInAbs * dzOutInAbs) +
+                       sdlMuls * sdlMuls * multDzDr * multDzDr / 3.f * coshEta * coshEta);
+  float drtMean = drtSDIn * dzOutInAbs / std::fabs(dzSDIn);
+  float rtWindow = drtErr + rtGeom;
+  float rtLo_point = rt_InLo + drtMean / dzDrtScale - rtWindow;
+  float rtHi_point = rt_InLo + drtMean + rtWindow;
+  if (isInSgInnerMDPS && isInSgOuterMDPS) {
+    if (!(kZ >= 0 && rtOut >= rtLo_point && rtOut <= rtHi_point)) return false;
+  }
+  float sdlPVoff = 0.1f / rtOut;
+  float sdlCut = alpha1GeV_OutLo + fsqrt(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff);
+  float deltaPhiPos = phi_mpi_pi(ev.anchorPhi[q.md4] - ev.anchorPhi[q.md2]);
+  if (!(std::fabs(deltaPhiPos) <= sdlCut)) return false;
+  float midPointX = 0.5f * (ev.anchorX[q.md1] + ev.anchorX[q.md3]);
+  float midPointY = 0.5f * (ev.anchorY[q.md1] + ev.anchorY[q.md3]);
+  float diffX = ev.anchorX[q.md3] - ev.anchorX[q.md1];
+  float diffY = ev.anchorY[q.md3] - ev.anchorY[q.md1];
+  float dPhi = deltaPhi(midPointX, midPointY, diffX, diffY);
+  if (!(std::fabs(dPhi) <= sdlCut)) return false;
+  float sdIn_alpha = ev.segDPhiChanges[q.seg1];
+  float sdOut_alpha = sdIn_alpha;
+  float sdOut_dPhiPos = phi_mpi_pi(ev.anchorPhi[q.md4] - ev.anchorPhi[q.md3]);
+  float sdOut_dPhiChange = ev.segDPhiChanges[q.seg3];
+  float sdOut_dPhiChange_min = ev.segDPhiChangeMins[q.seg3];
+  float sdOut_dPhiChange_max = ev.segDPhiChangeMaxs[q.seg3];
+  float sdOut_alphaOutRHmin = phi_mpi_pi(sdOut_dPhiChange_min - sdOut_dPhiPos);
+  float sdOut_alphaOutRHmax = phi_mpi_pi(sdOut_dPhiChange_max - sdOut_dPhiPos);
+  float sdOut_alphaOut = phi_mpi_pi(sdOut_dPhiChange - sdOut_dPhiPos);
+  float tl_axis_x = ev.anchorX[q.md4] - ev.anchorX[q.md1];
+  float tl_axis_y = ev.anchorY[q.md4] - ev.anchorY[q.md1];
+  float betaIn = sdIn_alpha - phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md1]);
+  float sdIn_alphaRHmin = ev.segDPhiChangeMins[q.seg1];
+  float sdIn_alphaRHmax = ev.segDPhiChangeMaxs[q.seg1];
+  float betaInRHmin = betaIn + sdIn_alphaRHmin - sdIn_alpha;
+  float betaInRHmax = betaIn + sdIn_alphaRHmax - sdIn_alpha;
+  float betaOut = -sdOut_alphaOut + phi_mpi_pi(phi(tl_axis_x, tl_axis_y) - ev.anchorPhi[q.md4]);
+  float betaOutRHmin = betaOut - sdOut_alphaOutRHmin + sdOut_alphaOut;
+  float betaOutRHmax = betaOut - sdOut_alphaOutRHmax + sdOut_alphaOut;
+  if (std::fabs(betaOutRHmin) > std::fabs(betaOutRHmax)) std::swap(betaOutRHmin, betaOutRHmax);
+  if (std::fabs(betaInRHmin) > std::fabs(betaInRHmax)) std::swap(betaInRHmin, betaInRHmax);
+  float dxIn = ev.anchorX[q.md2] - ev.anchorX[q.md1], dyIn = ev.anchorY[q.md2] - ev.anchorY[q.md1];
+  float sdIn_dr = fsqrt(dxIn * dxIn + dyIn * dyIn);
+  float sdIn_d = rt_InOut - rt_InLo;
+  float dr = fsqrt(tl_axis_x * tl_axis_x + tl_axis_y * tl_axis_y);
+  const float corrF = 1.f;
+  float betaInCut = fasin(fminf_((-sdIn_dr * corrF + dr) * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / sdIn_d);
+  if (!(std::fabs(betaInRHmin) < betaInCut)) return false;
+  float betaAv = 0.5f * (betaIn + betaOut);
+  float pt_beta = dr * k2Rinv1GeVf / fsin(betaAv);
+  int lIn = 11;
+  int lOut = 13;
+  float dxOut = ev.anchorX[q.md4] - ev.anchorX[q.md3], dyOut = ev.anchorY[q.md4] - ev.anchorY[q.md3];
+  float sdOut_dr = fsqrt(dxOut * dxOut + dyOut * dyOut);
+  float sdOut_d = ev.anchorRt[q.md4] - ev.anchorRt[q.md3];
+  runDeltaBetaIterationsT5(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn);
+  const float betaInMMSF = (std::fabs(betaInRHmin + betaInRHmax) > 0) ? (2.f * betaIn / std::fabs(betaInRHmin + betaInRHmax)) : 0.;
+  const float betaOutMMSF = (std::fabs(betaOutRHmin + betaOutRHmax) > 0) ? (2.f * betaOut / std::fabs(betaOutRHmin + betaOutRHmax)) : 0.;
+  betaInRHmin *= betaInMMSF; betaInRHmax *= betaInMMSF;
+  betaOutRHmin *= betaOutMMSF; betaOutRHmax *= betaOutMMSF;
+  const float dBetaMuls = sdlThetaMulsF * 4.f / fminf_(std::fabs(pt_beta), pt_betaMax);
+  const float alphaInAbsReg = fmaxf_(std::fabs(sdIn_alpha), fasin(fminf_(rt_InLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float alphaOutAbsReg = fmaxf_(std::fabs(sdOut_alpha), fasin(fminf_(rt_OutLo * k2Rinv1GeVf / 3.0f, sinAlphaMax)));
+  const float dBetaInLum = lIn < 11 ? 0.0f : std::fabs(alphaInAbsReg * deltaZLum / z_InLo);
+  const float dBetaOutLum = lOut < 11 ? 0.0f : std::fabs(alphaOutAbsReg * deltaZLum / z_OutLo);
+  const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+  const float dBetaRIn2 = 0;
+  float dBetaROut2 = 0;
+  float betaOutCut = fasin(fminf_(dr * k2Rinv1GeVf / ptCut, sinAlphaMax)) + (0.02f / sdOut_d) + fsqrt(dBetaLum2 + dBetaMuls * dBetaMuls);
+  if (!(std::fabs(betaOut) < betaOutCut)) return false;
+  float dBetaRes = 0.02f / fminf_(sdOut_d, sdIn_d);
+  float spread = std::fabs(betaInRHmin - betaInRHmax) + std::fabs(betaOutRHmin - betaOutRHmax);
+  float dBetaCut2 = (dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + dBetaRIn2 + dBetaROut2 + 0.25f * spread * spread);
+  float dBeta = betaIn - betaOut;
+  return dBeta * dBeta <= dBetaCut2;
+}
+
+// SDL/Quintuplet.h:2299-2505
+bool algoSelector(const Event& ev, const T4In& q) {
+  const Geometry& G = *ev.g;
+  short s1 = G.subdets[q.m1], s2 = G.subdets[q.m2], s3 = G.subdets[q.m3], s4 = G.subdets[q.m4];
+  if (s1 == Barrel && s2 == Barrel && s3 == Barrel && s4 == Barrel) return algoBBBB(ev, q);
+  if (s1 == Barrel && s2 == Barrel && s3 == Endcap && s4 == Endcap) return algoBBEE(ev, q);
+  if (s1 == Barrel && s2 == Barrel && s3 == Barrel && s4 == Endcap) return algoBBBB(ev, q);
+  if (s1 == Barrel && s2 == Endcap && s3 == Endcap && s4 == Endcap) return algoBBEE(ev, q);
+  if (s1 == Endcap && s2 == Endcap && s3 == Endcap && s4 == Endcap) return algoEEEE(ev, q);
+  return false;
+}
+
+}  // namespace
+
+// SDL/Quintuplet.h:2508-2986 (plain mode: rz chi2 + r-phi chi2)
+bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned innerT3, unsigned outerT3, T5Out& o) {
+  const Geometry& G = *ev.g;
+  unsigned seg1 = ev.t3SegmentIndices[2 * innerT3], seg2 = ev.t3SegmentIndices[2 * innerT3 + 1];
+  unsigned seg3 = ev.t3SegmentIndices[2 * outerT3], seg4 = ev.t3SegmentIndices[2 * outerT3 + 1];
+  if (ev.segMdIndices[2 * seg2 + 1] != ev.segMdIndices[2 * seg3]) return false;
+  unsigned md[5] = {ev.segMdIndices[2 * seg1], ev.segMdIndices[2 * seg2], ev.segMdIndices[2 * seg2 + 1],
+                    ev.segMdIndices[2 * seg3 + 1], ev.segMdIndices[2 * seg4 + 1]};
+  if (!algoSelector(ev, T4In{mods[0], mods[1], mods[2], mods[3], seg1, seg3, md[0], md[1], md[2], md[3]})) return false;
+  if (!algoSelector(ev, T4In{mods[0], mods[1], mods[3], mods[4], seg1, seg4, md[0], md[1], md[3], md[4]})) return false;
+  float x[5], y[5];
+  for (int k = 0; k < 5; k++) { x[k] = ev.anchorX[md[k]]; y[k] = ev.anchorY[md[k]]; }
+  auto is2SEndcap = [&](int k) { return G.subdets[mods[k]] == Endcap && G.moduleType[mods[k]] == TwoS; };
+  float x1v[3] = {x[0], x[0], x[0]}, y1v[3] = {y[0], y[0], y[0]};
+  float x2v[3] = {x[1], x[1], x[1]}, y2v[3] = {y[1], y[1], y[1]};
+  float x3v[3] = {x[2], x[2], x[2]}, y3v[3] = {y[2], y[2], y[2]};
+  auto edges = [&](int k, float* xv, float* yv) {
+    if (is2SEndcap(k)) {
+      xv[1] = ev.anchorLowEdgeX[md[k]]; xv[2] = ev.anchorHighEdgeX[md[k]];
+      yv[1] = ev.anchorLowEdgeY[md[k]]; yv[2] = ev.anchorHighEdgeY[md[k]];
+    }
+  };
+  edges(0, x1v, y1v); edges(1, x2v, y2v); edges(2, x3v, y3v);
+  float innerMin2S, innerMax2S, bridgeMin2S, bridgeMax2S, outerMin2S, outerMax2S;
+  computeErrorInRadius(x1v, y1v, x2v, y2v, x3v, y3v, innerMin2S, innerMax2S);
+  for (int i = 0; i < 3; i++) { x1v[i] = x[3]; y1v[i] = y[3]; }
+  edges(3, x1v, y1v);
+  computeErrorInRadius(x2v, y2v, x3v, y3v, x1v, y1v, bridgeMin2S, bridgeMax2S);
+  for (int i = 0; i < 3; i++) { x2v[i] = x[4]; y2v[i] = y[4]; }
+  edges(4, x2v, y2v);
+  computeErrorInRadius(x3v, y3v, x1v, y1v, x2v, y2v, outerMin2S, outerMax2S);
+  (void)outerMin2S; (void)outerMax2S;
+  float g, f;
+  o.outerRadius = ev.t3CircleRadius[outerT3];
+  o.bridgeRadius = computeRadiusFromThreeAnchorHits(x[1], y[1], x[2], y[2], x[3], y[3], g, f);
+  o.innerRadius = ev.t3CircleRadius[innerT3];
+  g = ev.t3CircleCenterX[innerT3];
+  f = ev.t3CircleCenterY[innerT3];
+  float inner_pt = 2 * k2Rinv1GeVf * o.innerRadius;
+  o.tightCutFlag = false;
+  if (!passT5RZConstraint(ev, md, mods, o.rzChiSquared, inner_pt, o.innerRadius, g, f, o.tightCutFlag)) return false;
+  bool pass = o.innerRadius >= 0.95f * ev.cfg.ptCut / (2.f * k2Rinv1GeVf);
+  short s[5];
+  for (int k = 0; k < 5; k++) s[k] = G.subdets[mods[k]];
+  const RadiiBounds* b;
+  if (s[0] == Barrel && s[1] == Barrel && s[2] == Barrel && s[3] == Barrel && s[4] == Barrel) b = &kBBBBB;
+  else if (s[0] == Barrel && s[1] == Barrel && s[2] == Barrel && s[3] == Barrel && s[4] == Endcap) b = &kBBBBE;
+  else if (s[0] == Barrel && s[1] == Barrel && s[2] == Barrel && s[3] == Endcap && s[4] == Endcap)
+    b = G.layers[mods[0]] == 1 ? &kBBBEE12378 : (G.layers[mods[0]] == 2 ? &kBBBEE23478 : &kBBBEE34578);
+  else if (s[0] == Barrel && s[1] == Barrel && s[2] == Endcap && s[3] == Endcap && s[4] == Endcap) b = &kBBEEE;
+  else if (s[0] == Barrel && s[1] == Endcap && s[2] == Endcap && s[3] == Endcap && s[4] == Endcap) b = &kBEEEE;
+  else b = &kEEEEE;
+  pass = pass && matchRadii(*b, o.innerRadius, o.bridgeRadius, innerMin2S, innerMax2S, bridgeMin2S, bridgeMax2S);
+  if (!pass) return false;
+  float delta1[5], delta2[5], slopes[5], sigmas[5];
+  bool isFlat[5];
+  computeSigmasForRegression(G, mods, delta1, delta2, slopes, isFlat);
+  o.regressionRadius = computeRadiusUsingRegression(5, x, y, delta1, delta2, slopes, isFlat, o.regressionG,
+                                                    o.regressionF, sigmas, o.chiSquared);
+  if (ev.cfg.t5Mode == 0) {  // USE_RPHICHI2: SDL/Quintuplet.h:2943-2956
+    if (o.regressionRadius < 5.0f / (2.f * k2Rinv1GeVf)) {
+      if (!passChiSquaredConstraint(G.sdlLayers[mods[0]], G.sdlLayers[mods[1]], G.sdlLayers[mods[2]],
+                                    G.sdlLayers[mods[3]], G.sdlLayers[mods[4]], o.chiSquared))
+        return false;
+    }
+  }
+  float nd1[5], nd2[5], nsl[5];
+  float nx[5], ny[5];
+  for (int k = 0; k < 5; k++) { nx[k] = ev.outerX[md[k]]; ny[k] = ev.outerY[md[k]]; }
+  computeSigmasForRegression(G, mods, nd1, nd2, nsl, isFlat, 5, false);
+  o.nonAnchorChiSquared = computeChiSquared(5, nx, ny, nd1, nd2, nsl, isFlat, o.regressionG, o.regressionF, o.regressionRadius);
+  for (int k = 0; k < 5; k++) o.md[k] = md[k];
+  return true;
+}
+
+// SDL/Event.cc:903-991, SDL/Quintuplet.h:2988-3235, SDL/Kernels.h:202-249
+void event_create_quintuplets(Event& ev) {
+  const Geometry& G = *ev.g;
+  const unsigned nL = G.nLowerModules;
+  // createEligibleModulesListForQuintupletsGPU: SDL/Quintuplet.h:3117-3213 (module order)
+  int nElig = 0, total = 0;
+  ev.quintupletModuleIndices.assign(nL, -1);
+  ev.quintupletModuleOccupancy.assign(nL, 0);
+  for (unsigned i = 0; i < nL; i++) {
+    if (ev.nTriplets[i] == 0) continue;
+    if (G.subdets[i] == Barrel && G.layers[i] >= 3) continue;
+    if (G.subdets[i] == Endcap && G.layers[i] > 1) continue;
+    int en;
+    short rings = G.rings[i], layers = G.layers[i], subdets = G.subdets[i];
+    float e = std::fabs(G.eta[i]);
+    int cat;
+    if (layers <= 3 && subdets == 5) cat = 0;
+    else if (layers >= 4 && subdets == 5) cat = 1;
+    else if (layers <= 2 && subdets == 4 && rings >= 11) cat = 2;
+    else if (layers >= 3 && subdets == 4 && rings >= 8) cat = 2;
+    else if (layers <= 2 && subdets == 4 && rings <= 10) cat = 3;
+    else if (layers >= 3 && subdets == 4 && rings <= 7) cat = 3;
+    else cat = -1;
+    if (e < 0.75) en = 0;
+    else if (e > 0.75 && e < 1.5) en = 1;
+    else if (e > 1.5 && e < 2.25) en = 2;
+    else if (e > 2.25 && e < 3) en = 3;
+    else en = -1;
+    int occ;
+    if (cat == 0 && en == 0) occ = 336;
+    else if (cat == 0 && en == 1) occ = 414;
+    else if (cat == 0 && en == 2) occ = 231;
+    else if (cat == 0 && en == 3) occ = 146;
+    else if (cat == 3 && en == 1) occ = 0;
+    else if (cat == 3 && en == 2) occ = 191;
+    else if (cat == 3 && en == 3) occ = 106;
+    else occ = 0;
+    ev.quintupletModuleIndices[i] = total;
+    ev.indicesOfEligibleT5Modules[nElig++] = i;
+    ev.quintupletModuleOccupancy[i] = occ;
+    total += occ;
+  }
+  ev.nEligibleT5Modules = nElig;
+  ev.nTotalQuints = total;
+  const unsigned n = total;
+  ev.t5Capacity = n;
+  ev.nQuintuplets.assign(nL, 0); ev.totOccupancyQuintuplets.assign(nL, 0);
+  ev.t5TripletIndices.assign(2 * n, 0); ev.t5LowerModuleIndices.assign(5 * n, 0);
+  ev.t5LogicalLayers.assign(5 * n, 0); ev.t5HitIndices.assign(10 * n, 0);
+  ev.t5Layer.assign(n, 0); ev.t5IsDup.assign(n, 0); ev.t5TightCutFlag.assign(n, 0); ev.t5PartOfPT5.assign(n, 0);
+  for (auto* v : {&ev.t5InnerRadius, &ev.t5BridgeRadius, &ev.t5OuterRadius, &ev.t5Pt, &ev.t5Eta, &ev.t5Phi,
+                  &ev.t5ScoreRphiSum, &ev.t5RegressionRadius, &ev.t5RegressionG, &ev.t5RegressionF,
+                  &ev.t5RzChiSquared, &ev.t5ChiSquared, &ev.t5NonAnchorChiSquared})
+    v->assign(n, 0.f);
+
+  // createQuintupletsInGPUv2: SDL/Quintuplet.h:2988-3115
+  for (int it = 0; it < nElig; it++) {
+    uint16_t m1 = ev.indicesOfEligibleT5Modules[it];
+    int layer = G.layers[m1];
+    short adj;
+    if (layer == 1) adj = 1;
+    else if (layer == 2) adj = 0;
+    else continue;
+    unsigned nInner = ev.nTriplets[m1];
+    for (unsigned a = 0; a < nInner; a++) {
+      unsigned innerT3 = ev.tripletModuleIndices[m1] + a;
+      uint16_t m2 = ev.t3LowerModuleIndices[3 * innerT3 + 1];
+      uint16_t m3 = ev.t3LowerModuleIndices[3 * innerT3 + 2];
+      unsigned nOuter = ev.nTriplets[m3];
+      for (unsigned b = 0; b < nOuter; b++) {
+        unsigned outerT3 = ev.tripletModuleIndices[m3] + b;
+        uint16_t m4 = ev.t3LowerModuleIndices[3 * outerT3 + 1];
+        uint16_t m5 = ev.t3LowerModuleIndices[3 * outerT3 + 2];
+        uint16_t mods[5] = {m1, m2, m3, m4, m5};
+        T5Out o;
+        if (!runQuintupletDefaultAlgo(ev, mods, innerT3, outerT3, o)) continue;
+        unsigned occ = ev.totOccupancyQuintuplets[m1]++;
+        if ((int)occ >= ev.quintupletModuleOccupancy[m1]) continue;
+        unsigned k = ev.nQuintuplets[m1]++;
+        if (ev.quintupletModuleIndices[m1] == -1) continue;
+        unsigned q = ev.quintupletModuleIndices[m1] + k;
+        unsigned mdAdj = ev.segMdIndices[2 * ev.t3SegmentIndices[2 * innerT3 + adj]];
+        float phi = ev.anchorPhi[mdAdj];
+        float eta = ev.anchorEta[mdAdj];
+        float pt = (o.innerRadius + o.outerRadius) * 3.8f * 1.602f / (2 * 100 * 5.39f);
+        float scores = o.chiSquared + o.nonAnchorChiSquared;
+        // addQuintupletToMemory: SDL/Quintuplet.h:150-215
+        ev.t5TripletIndices[2 * q] = innerT3; ev.t5TripletIndices[2 * q + 1] = outerT3;
+        for (int j = 0; j < 5; j++) ev.t5LowerModuleIndices[5 * q + j] = mods[j];
+        ev.t5InnerRadius[q] = o.innerRadius; ev.t5OuterRadius[q] = o.outerRadius; ev.t5Pt[q] = pt;
+        ev.t5Eta[q] = eta; ev.t5Phi[q] = phi; ev.t5ScoreRphiSum[q] = scores; ev.t5Layer[q] = layer;
+        ev.t5IsDup[q] = 0; ev.t5TightCutFlag[q] = o.tightCutFlag; ev.t5RegressionRadius[q] = o.regressionRadius;
+        ev.t5RegressionG[q] = o.regressionG; ev.t5RegressionF[q] = o.regressionF;
+        ev.t5LogicalLayers[5 * q + 0] = ev.t3LogicalLayers[3 * innerT3];
+        ev.t5LogicalLayers[5 * q + 1] = ev.t3LogicalLayers[3 * innerT3 + 1];
+        ev.t5LogicalLayers[5 * q + 2] = ev.t3LogicalLayers[3 * innerT3 + 2];
+        ev.t5LogicalLayers[5 * q + 3] = ev.t3LogicalLayers[3 * outerT3 + 1];
+        ev.t5LogicalLayers[5 * q + 4] = ev.t3LogicalLayers[3 * outerT3 + 2];
+        for (int j = 0; j < 6; j++) ev.t5HitIndices[10 * q + j] = ev.t3HitIndices[6 * innerT3 + j];
+        for (int j = 0; j < 4; j++) ev.t5HitIndices[10 * q + 6 + j] = ev.t3HitIndices[6 * outerT3 + 2 + j];
+        ev.t5BridgeRadius[q] = o.bridgeRadius; ev.t5RzChiSquared[q] = o.rzChiSquared;
+        ev.t5ChiSquared[q] = o.chiSquared; ev.t5NonAnchorChiSquared[q] = o.nonAnchorChiSquared;
+        ev.t3PartOfT5[innerT3] = 1;
+        ev.t3PartOfT5[outerT3] = 1;
+      }
+    }
+  }
+  // removeDupQuintupletsInGPUAfterBuild: SDL/Kernels.h:202-249 (decisions independent of order)
+  for (unsigned lowmod = 0; lowmod < nL; lowmod++) {
+    unsigned nq = ev.nQuintuplets[lowmod];
+    int base = ev.quintupletModuleIndices[lowmod];
+    for (unsigned i1 = 0; i1 < nq; i1++) {
+      unsigned ix = base + i1;
+      float eta1 = ev.t5Eta[ix], phi1 = ev.t5Phi[ix], s1 = ev.t5ScoreRphiSum[ix];
+      for (unsigned j1 = i1 + 1; j1 < nq; j1++) {
+        unsigned jx = base + j1;
+        float dEta = std::fabs(eta1 - ev.t5Eta[jx]);
+        float dPhi = calculate_dPhi(phi1, ev.t5Phi[jx]);
+        float s2 = ev.t5ScoreRphiSum[jx];
+        if (dEta > 0.1f) continue;
+        if (std::fabs(dPhi) > 0.1f) continue;
+        int nMatched = 0;
+        for (int a = 0; a < 10; a++)
+          for (int b = 0; b < 10; b++)
+            if (ev.t5HitIndices[10 * ix + a] == ev.t5HitIndices[10 * jx + b]) { nMatched++; break; }
+        if (nMatched >= 7) {
+          if (s1 >= s2) ev.t5IsDup[ix] |= 1;
+          else ev.t5IsDup[jx] |= 1;
+        }
+      }
+    }
+  }
+  // addQuintupletRangesToEventExplicit: SDL/Quintuplet.h:3215-3235
+  for (unsigned i = 0; i < nL; i++) {
+    if (ev.nQuintuplets[i] == 0 || ev.quintupletModuleIndices[i] == -1) {
+      ev.quintupletRanges[i * 2] = -1; ev.quintupletRanges[i * 2 + 1] = -1;
+    } else {
+      ev.quintupletRanges[i * 2] = ev.quintupletModuleIndices[i];
+      ev.quintupletRanges[i * 2 + 1] = ev.quintupletModuleIndices[i] + ev.nQuintuplets[i] - 1;
+    }
+  }
+}
+
+}  // namespace orc

@@ -97,3 +97,24 @@ def test_staged_device_path_matches(engine, oracle_geometry):
     from oracle import orc
     ref = orc.Event(oracle_geometry).run_lst(ev, "t3")
     compare(e, ref, "t3")
+
+
+def test_kernel_timing_does_not_disturb_runs(engine):
+    """Per-kernel timing returns positive durations and leaves the event usable."""
+    import ctypes as C
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    _, es = engine
+    e = lst.Event(es)
+    L = lst.lib()
+    L.lst_event_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.lst_event_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    ev = synth.generate_event("pu", 4, n_pu=50)
+    lst._check(L.lst_event_set_timing(e.h, 1))
+    e.run_lst(ev, "t3")
+    ms = np.zeros(16, np.float32)
+    lst._check(L.lst_event_kernel_times(e.h, ms.ctypes.data_as(C.c_void_p), 16))
+    assert np.all(ms[:5] > 0) and np.all(ms[5:] == 0)
+    e.run_lst(ev, "t3")  # must not report a stale runtime error
+    lst._check(L.lst_event_set_timing(e.h, 0))
+    e.run_lst(ev, "t3")

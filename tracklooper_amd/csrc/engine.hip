@@ -92,7 +92,12 @@ struct lst_event {
   // timing of the main kernel of each stage
   bool timing = false;
   lst::KernelTimer timers[16];
-  lst::KernelTimer timer(int k) const { return timing ? timers[k] : lst::KernelTimer{}; }
+  uint32_t recorded = 0;  // timers recorded since timing was switched on
+  lst::KernelTimer timer(int k) {
+    if (!timing) return lst::KernelTimer{};
+    recorded |= 1u << k;
+    return timers[k];
+  }
   // device-resident prepared inputs (throughput path)
   struct Staged {
     char* mem = nullptr;
@@ -531,6 +536,7 @@ lst_status lst_event_set_timing(lst_event* ev, int on) {
       HIP_OK(hipEventCreate(&t.stop));
     }
   ev->timing = on != 0;
+  ev->recorded = 0;
   return LST_OK;
 }
 
@@ -540,9 +546,9 @@ lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n) {
   HIP_OK(hipStreamSynchronize(ev->stream));
   for (int k = 0; k < n && k < 16; k++) {
     ms[k] = 0.f;
-    if (hipEventQuery(ev->timers[k].stop) == hipSuccess &&
-        hipEventElapsedTime(&ms[k], ev->timers[k].start, ev->timers[k].stop) != hipSuccess)
-      ms[k] = 0.f;
+    // only timers recorded by a stage run hold a valid interval; querying the
+    // others would leave an error behind for the next hipGetLastError
+    if ((ev->recorded >> k) & 1u) HIP_OK(hipEventElapsedTime(&ms[k], ev->timers[k].start, ev->timers[k].stop));
   }
   return LST_OK;
 }
