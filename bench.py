@@ -20,9 +20,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC set"
-STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets"]
-KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets"]
-LAST_STAGE = 4  # triplets: the stages built so far
+STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets"]
+KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets", "k_quintuplets"]
+ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5"]
+LAST_STAGE = 5  # quintuplets: the stages built so far
 HBM_PEAK_GBS = 8000.0
 
 
@@ -41,18 +42,20 @@ def _input_struct(lst, ev):
 def algorithmic_bytes(counts: dict) -> dict:
     """Minimal HBM bytes per launch of each builder kernel: inputs read once, outputs written once.
 
-    Record sizes are the fields each kernel reads/writes (see DESIGN.md):
+    Record sizes are the fields each kernel reads/writes:
       hits kernel  : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
       MD builder   : read x,y,z,rt of the module-pair hits (16 B/hit), write 134 B/MD
       LS builder   : read 44 B of each mini-doublet, write 44 B/segment
       T3 builder   : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
+      T5 builder   : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
     """
-    h, md, ls, t3 = counts["hits"], counts["mds"], counts["segs"], counts["t3"]
+    h, md, ls, t3, t5 = counts["hits"], counts["mds"], counts["segs"], counts["t3"], counts["t5"]
     return {
         "k_hits": 46 * h,
         "k_mini_doublets": 16 * h + 134 * md,
         "k_segments": 44 * md + 44 * ls,
         "k_triplets": 22 * ls + 20 * md + 60 * t3,
+        "k_quintuplets": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
     }
 
 
@@ -62,7 +65,7 @@ def cpu_baseline(events, budget_s: float = 20.0) -> dict:
     from tracklooper_amd import synth
     g = orc.Geometry(synth.geometry_dir())
     n, t0 = 0, time.perf_counter()
-    stage = ["hits", "pls", "md", "ls", "t3"][LAST_STAGE]
+    stage = ORACLE_STAGES[LAST_STAGE]
     while time.perf_counter() - t0 < budget_s and n < 4 * len(events):
         orc.Event(g).run_lst(events[n % len(events)], stage)
         n += 1
@@ -138,7 +141,8 @@ def main():
         counts.append({"hits": int(lst.lib().lst_event_count(ev.h, b"hits")),
                        "mds": int(ev.field("mds.nMDs").sum()),
                        "segs": int(ev.field("segs.nSegments").sum()),
-                       "t3": int(ev.field("t3.nTriplets").sum())})
+                       "t3": int(ev.field("t3.nTriplets").sum()),
+                       "t5": int(ev.field("t5.nQuintuplets").sum())})
     lst._check(L.lst_event_set_timing(ev.h, 0))
     avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
     per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}
@@ -163,10 +167,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"per-event path, one PU{args.n_pu}-like synthetic event per step "
-                        f"({args.events} seeded events cycled), stages hits..triplets",
+                        f"({args.events} seeded events cycled), stages hits..{STAGE_NAMES[LAST_STAGE]}",
             "geometry": "OT800_IT615_pt0.8",
             "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],
             "mean_segments": mean_counts["segs"], "mean_triplets": mean_counts["t3"],
+            "mean_quintuplets": mean_counts["t5"],
             "gpu_ms_per_step": gpu_ms / args.steps,
             "kernel_ms": per_kernel,
         },

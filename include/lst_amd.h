@@ -83,6 +83,10 @@ lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments*
 lst_status lst_event_create_mini_doublets(lst_event* ev);        /* createMiniDoublets */
 lst_status lst_event_create_segments(lst_event* ev);             /* createSegmentsWithModuleMap */
 lst_status lst_event_create_triplets(lst_event* ev);             /* createTriplets */
+/* createQuintuplets (SDL/Event.cc:903-991): eligible-module list, T5 builder,
+ * after-build duplicate flags.  Needs an event created with
+ * LST_T5_PLAIN_CHI2; LST_E_INPUT otherwise. */
+lst_status lst_event_create_quintuplets(lst_event* ev);
 
 /* Replaces SDL::Event::resetEvent (SDL/Event.cc:45-149); keeps the arenas. */
 lst_status lst_event_reset(lst_event* ev);
@@ -118,7 +122,7 @@ typedef struct {
 
 /* Replaces SDL::LST<Acc>::run (SDL/LST.cc:12-161) up to and including
  * `last_stage` (0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,
- * 4 triplets): prepareInput on the host (SDL/LST.cc:175-365), then the stage
+ * 4 triplets, 5 quintuplets): prepareInput on the host (SDL/LST.cc:175-365), then the stage
  * sequence on the device. */
 lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage);
 
@@ -133,7 +137,8 @@ lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 /* Per-kernel device timing of the builder kernels (hipEvents on the event's
  * stream).  After a run with timing on, lst_event_kernel_times writes the
  * durations in ms of: 0 hits, 1 pixel segments, 2 mini-doublets,
- * 3 segments, 4 triplets (n <= 16 entries; synchronises the stream). */
+ * 3 segments, 4 triplets, 5 quintuplets (n <= 16 entries; synchronises
+ * the stream). */
 lst_status lst_event_set_timing(lst_event* ev, int on);
 lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n);
 /* The hipStream_t the event runs on. */

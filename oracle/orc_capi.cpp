@@ -111,6 +111,7 @@ int orc_run(void* hp, int upto) {
   if (upto > 2) event_create_mini_doublets(ev);
   if (upto > 3) event_create_segments(ev);
   if (upto > 4) event_create_triplets(ev);
+  if (upto > 5) event_create_quintuplets(ev);
   return 0;
 }
 
@@ -124,6 +125,9 @@ unsigned orc_event_scalar(void* hp, const char* key) {
   if (!std::strcmp(key, "mdCapacity")) return ev.mdCapacity;
   if (!std::strcmp(key, "segCapacity")) return ev.segCapacity;
   if (!std::strcmp(key, "t3Capacity")) return ev.t3Capacity;
+  if (!std::strcmp(key, "t5Capacity")) return ev.t5Capacity;
+  if (!std::strcmp(key, "nEligibleT5Modules")) return ev.nEligibleT5Modules;
+  if (!std::strcmp(key, "nTotalQuints")) return ev.nTotalQuints;
   return 0;
 }
 
@@ -195,6 +199,18 @@ int orc_event_field(void* hp, const char* key, const void** p, int* t, long* n) 
   F("t3.betaIn", ev.t3BetaIn); F("t3.circleRadius", ev.t3CircleRadius); F("t3.circleCenterX", ev.t3CircleCenterX);
   F("t3.circleCenterY", ev.t3CircleCenterY); F("t3.partOfPT5", ev.t3PartOfPT5); F("t3.partOfT5", ev.t3PartOfT5);
   F("t3.partOfPT3", ev.t3PartOfPT3);
+  // quintuplets
+  F("t5.nQuintuplets", ev.nQuintuplets); F("t5.totOccupancyQuintuplets", ev.totOccupancyQuintuplets);
+  F("t5.tripletIndices", ev.t5TripletIndices); F("t5.lowerModuleIndices", ev.t5LowerModuleIndices);
+  F("t5.hitIndices", ev.t5HitIndices); F("t5.logicalLayers", ev.t5LogicalLayers); F("t5.layer", ev.t5Layer);
+  F("t5.isDup", ev.t5IsDup); F("t5.TightCutFlag", ev.t5TightCutFlag); F("t5.partOfPT5", ev.t5PartOfPT5);
+  F("t5.innerRadius", ev.t5InnerRadius); F("t5.bridgeRadius", ev.t5BridgeRadius);
+  F("t5.outerRadius", ev.t5OuterRadius); F("t5.pt", ev.t5Pt); F("t5.eta", ev.t5Eta); F("t5.phi", ev.t5Phi);
+  F("t5.score_rphisum", ev.t5ScoreRphiSum); F("t5.regressionRadius", ev.t5RegressionRadius);
+  F("t5.regressionG", ev.t5RegressionG); F("t5.regressionF", ev.t5RegressionF);
+  F("t5.rzChiSquared", ev.t5RzChiSquared); F("t5.chiSquared", ev.t5ChiSquared);
+  F("t5.nonAnchorChiSquared", ev.t5NonAnchorChiSquared);
+  F("ranges.indicesOfEligibleT5Modules", ev.indicesOfEligibleT5Modules);
   return 0;
 }
 

@@ -75,6 +75,14 @@ struct Event {
   std::vector<uint16_t> t3LowerModuleIndices;
   std::vector<uint8_t> t3LogicalLayers, t3PartOfPT5, t3PartOfT5, t3PartOfPT3;
   std::vector<float> t3BetaIn, t3CircleRadius, t3CircleCenterX, t3CircleCenterY;
+  // quintuplets
+  unsigned t5Capacity = 0;
+  std::vector<unsigned> nQuintuplets, totOccupancyQuintuplets, t5TripletIndices, t5HitIndices;
+  std::vector<uint16_t> t5LowerModuleIndices;
+  std::vector<uint8_t> t5LogicalLayers, t5Layer, t5TightCutFlag, t5PartOfPT5;
+  std::vector<int8_t> t5IsDup;
+  std::vector<float> t5InnerRadius, t5BridgeRadius, t5OuterRadius, t5Pt, t5Eta, t5Phi, t5ScoreRphiSum,
+      t5RegressionRadius, t5RegressionG, t5RegressionF, t5RzChiSquared, t5ChiSquared, t5NonAnchorChiSquared;
 };
 
 bool event_add_hits(Event& ev, const Input& in);
@@ -82,6 +90,7 @@ void event_add_pixel_segments(Event& ev, const Input& in);
 void event_create_mini_doublets(Event& ev);
 void event_create_segments(Event& ev);
 void event_create_triplets(Event& ev);
+void event_create_quintuplets(Event& ev);
 float computeRadiusFromThreeAnchorHits(float x1, float y1, float x2, float y2, float x3, float y3, float& g, float& f);
 
 }  // namespace orc

@@ -1,0 +1,15 @@
+// ORACLE — test infrastructure only (see orc_common.h).
+#pragma once
+#include "orc_event.h"
+
+namespace orc {
+struct T5Out {
+  float innerRadius = 0, bridgeRadius = 0, outerRadius = 0, regressionG = 0, regressionF = 0, regressionRadius = 0,
+        rzChiSquared = 0, chiSquared = 0, nonAnchorChiSquared = 0;
+  bool tightCutFlag = false;
+  unsigned md[5] = {0, 0, 0, 0, 0};
+};
+bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned innerT3, unsigned outerT3, T5Out& o);
+void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, float& pt_beta, float sdIn_dr,
+                              float sdOut_dr, float dr, float lIn);
+}  // namespace orc

@@ -26,6 +26,12 @@ PLS_FIELDS = {"seedIdx": 1, "pLSHitsIdxs": 4, "charge": 1, "superbin": 1, "pixel
               "circleCenterX": 1, "circleCenterY": 1, "circleRadius": 1}
 T3_FIELDS = {"segmentIndices": 2, "lowerModuleIndices": 3, "logicalLayers": 3, "hitIndices": 6, "betaIn": 1,
              "circleRadius": 1, "circleCenterX": 1, "circleCenterY": 1}
+T5_FIELDS = {"tripletIndices": 2, "lowerModuleIndices": 5, "logicalLayers": 5, "hitIndices": 10, "layer": 1,
+             "isDup": 1, "TightCutFlag": 1, "partOfPT5": 1, "innerRadius": 1, "bridgeRadius": 1, "outerRadius": 1,
+             "pt": 1, "eta": 1, "phi": 1, "score_rphisum": 1, "regressionRadius": 1, "regressionG": 1,
+             "regressionF": 1, "rzChiSquared": 1, "chiSquared": 1, "nonAnchorChiSquared": 1}
+STAGE_ORDER = ["hits", "pls", "md", "ls", "t3", "t5"]
+DEEPEST = STAGE_ORDER[-1]  # the last stage the engine implements
 
 
 class Report:
@@ -67,10 +73,9 @@ def slots(start, count, mult=1):
 
 
 def compare(prod, orc, upto: str, report: Report | None = None, n_lower: int = 13200) -> Report:
-    """Compare every stage up to `upto` ("hits", "pls", "md", "ls", "t3")."""
+    """Compare every stage up to `upto` (one of STAGE_ORDER)."""
     r = report or Report()
-    order = ["hits", "pls", "md", "ls", "t3"]
-    last = order.index(upto)
+    last = STAGE_ORDER.index(upto)
     n_hits = orc.scalar("nHits")
     n_pls = orc.scalar("nPLS")
     for f in HIT_FIELDS:
@@ -122,4 +127,20 @@ def compare(prod, orc, upto: str, report: Report | None = None, n_lower: int = 1
         for f, m in T3_FIELDS.items():
             sl = slots(start, n, m)
             r.check("t3." + f, prod.field("t3." + f)[sl], orc.field("t3." + f)[sl])
+    if last >= 5:
+        sl = slots(orc.field("ranges.tripletModuleIndices"), orc.field("t3.nTriplets"))
+        r.check("t3.partOfT5", prod.field("t3.partOfT5")[sl], orc.field("t3.partOfT5")[sl])
+        n = orc.field("t5.nQuintuplets")
+        r.check("t5.nQuintuplets", prod.field("t5.nQuintuplets"), n)
+        r.check("t5.totOccupancyQuintuplets", prod.field("t5.totOccupancyQuintuplets"),
+                orc.field("t5.totOccupancyQuintuplets"))
+        for f in ("quintupletModuleIndices", "quintupletModuleOccupancy", "quintupletRanges"):
+            r.check("ranges." + f, prod.field("ranges." + f), orc.field("ranges." + f))
+        n_elig = orc.scalar("nEligibleT5Modules")
+        r.check("ranges.indicesOfEligibleT5Modules", prod.field("ranges.indicesOfEligibleT5Modules")[:n_elig],
+                orc.field("ranges.indicesOfEligibleT5Modules")[:n_elig])
+        start = np.maximum(orc.field("ranges.quintupletModuleIndices"), 0)
+        for f, m in T5_FIELDS.items():
+            sl = slots(start, n, m)
+            r.check("t5." + f, prod.field("t5." + f)[sl], orc.field("t5." + f)[sl])
     return r

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from tests.parity import Report, compare
+from tests.parity import DEEPEST, Report, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -24,24 +24,26 @@ def _run_both(engine, oracle_geometry, ev, upto):
 
 
 @pytest.mark.parametrize("index", range(5))
-def test_muon_gun_through_triplets(engine, oracle_geometry, index):
+def test_muon_gun_all_stages(engine, oracle_geometry, index):
     from tracklooper_amd import synth
     ev = synth.generate_event("muon", index)
-    prod, ref = _run_both(engine, oracle_geometry, ev, "t3")
-    rep = compare(prod, ref, "t3")
+    prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST)
+    rep = compare(prod, ref, DEEPEST)
     assert ref.field("t3.nTriplets").sum() > 0
+    assert ref.field("t5.nQuintuplets").sum() > 0
     print("\n".join(rep.lines) or "bit-identical")
 
 
 @pytest.mark.parametrize("n_pu", [50, 200])
-def test_pileup_through_triplets(engine, oracle_geometry, n_pu):
+def test_pileup_all_stages(engine, oracle_geometry, n_pu):
     from tracklooper_amd import synth
     ev = synth.generate_event("pu", 0, n_pu=n_pu)
-    prod, ref = _run_both(engine, oracle_geometry, ev, "t3")
-    rep = compare(prod, ref, "t3")
+    prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST)
+    rep = compare(prod, ref, DEEPEST)
     n_t3 = ref.field("t3.nTriplets").sum()
-    assert n_t3 > 1000
-    print(f"T3={n_t3}; floats compared {rep.float_compared}, not bit-identical {rep.float_inexact}")
+    n_t5 = ref.field("t5.nQuintuplets").sum()
+    assert n_t3 > 1000 and n_t5 > 100
+    print(f"T3={n_t3} T5={n_t5}; floats compared {rep.float_compared}, not bit-identical {rep.float_inexact}")
     print("\n".join(rep.lines))
 
 
@@ -51,9 +53,10 @@ def test_empty_event(engine, oracle_geometry):
     ev = synth.generate_event("muon", 0)
     ev = {k: (v[:0] if k.startswith(("ph2_", "see_")) else v) for k, v in ev.items()}
     ev["see_hitIdx_off"] = np.zeros(1, np.uint32)
-    prod, ref = _run_both(engine, oracle_geometry, ev, "t3")
-    compare(prod, ref, "t3")
+    prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST)
+    compare(prod, ref, DEEPEST)
     assert prod.getNumberOfMiniDoublets() == 0 and prod.getNumberOfTriplets() == 0
+    assert prod.getNumberOfQuintuplets() == 0
 
 
 def test_repeat_runs_identical(engine, oracle_geometry):
@@ -66,15 +69,18 @@ def test_repeat_runs_identical(engine, oracle_geometry):
     ev1 = synth.generate_event("muon", 7)
     from tests.parity import slots
 
-    def t3_hits():
+    def hits():
         sl = slots(e.field("ranges.tripletModuleIndices"), e.field("t3.nTriplets"), 6)
-        return e.field("t3.hitIndices")[sl]
+        q = slots(np.maximum(e.field("ranges.quintupletModuleIndices"), 0), e.field("t5.nQuintuplets"), 10)
+        return e.field("t3.hitIndices")[sl], e.field("t5.hitIndices")[q], e.field("t5.isDup")[q[::10] // 10]
 
-    e.run_lst(ev0, "t3")
-    a = t3_hits()
-    e.run_lst(ev1, "t3")
-    e.run_lst(ev0, "t3")
-    assert len(a) > 0 and np.array_equal(a, t3_hits())
+    e.run_lst(ev0, DEEPEST)
+    a = hits()
+    e.run_lst(ev1, DEEPEST)
+    e.run_lst(ev0, DEEPEST)
+    b = hits()
+    assert len(a[0]) > 0 and len(a[1]) > 0
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
 def test_staged_device_path_matches(engine, oracle_geometry):
@@ -92,11 +98,11 @@ def test_staged_device_path_matches(engine, oracle_geometry):
     L.lst_event_run_staged.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
     inp, keep = sys_path_bench._input_struct(lst, ev)
     lst._check(L.lst_event_stage_input(e.h, C.byref(inp), 3))
-    lst._check(L.lst_event_run_staged(e.h, 3, 4))
+    lst._check(L.lst_event_run_staged(e.h, 3, lst.STAGES[DEEPEST]))
     e.synchronize()
     from oracle import orc
-    ref = orc.Event(oracle_geometry).run_lst(ev, "t3")
-    compare(e, ref, "t3")
+    ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
+    compare(e, ref, DEEPEST)
 
 
 def test_kernel_timing_does_not_disturb_runs(engine):
@@ -111,10 +117,11 @@ def test_kernel_timing_does_not_disturb_runs(engine):
     L.lst_event_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     ev = synth.generate_event("pu", 4, n_pu=50)
     lst._check(L.lst_event_set_timing(e.h, 1))
-    e.run_lst(ev, "t3")
+    e.run_lst(ev, DEEPEST)
+    n_timed = lst.STAGES[DEEPEST] + 1
     ms = np.zeros(16, np.float32)
     lst._check(L.lst_event_kernel_times(e.h, ms.ctypes.data_as(C.c_void_p), 16))
-    assert np.all(ms[:5] > 0) and np.all(ms[5:] == 0)
-    e.run_lst(ev, "t3")  # must not report a stale runtime error
+    assert np.all(ms[:n_timed] > 0) and np.all(ms[n_timed:] == 0)
+    e.run_lst(ev, DEEPEST)  # must not report a stale runtime error
     lst._check(L.lst_event_set_timing(e.h, 0))
-    e.run_lst(ev, "t3")
+    e.run_lst(ev, DEEPEST)

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "liblst_amd.so")
 DEFAULT_GEOMETRY = os.path.join(HERE, "data", "OT800_IT615_pt0.8")
 
 _DT = {0: np.float32, 1: np.uint32, 2: np.int32, 3: np.uint16, 4: np.int16, 5: np.uint8, 6: np.int8}
-STAGES = {"hits": 0, "pls": 1, "md": 2, "ls": 3, "t3": 4}
+STAGES = {"hits": 0, "pls": 1, "md": 2, "ls": 3, "t3": 4, "t5": 5}
 LST_T5_PLAIN_CHI2, LST_T5_DNN = 0, 1
 _lib = None
 
@@ -65,7 +65,8 @@ def lib():
         L.lst_event_destroy.argtypes = [P]
         L.lst_event_add_hits.argtypes = [P, P, P, P, P, P, C.c_uint32, C.c_int]
         L.lst_event_add_pixel_segments.argtypes = [P, C.POINTER(_PixelSegments), C.c_int]
-        for n in ("create_mini_doublets", "create_segments", "create_triplets", "reset", "synchronize"):
+        for n in ("create_mini_doublets", "create_segments", "create_triplets", "create_quintuplets", "reset",
+                  "synchronize"):
             getattr(L, "lst_event_" + n).argtypes = [P]
         L.lst_event_count.argtypes = [P, C.c_char_p]
         L.lst_event_count.restype = C.c_uint32
@@ -144,6 +145,9 @@ class Event:
     def createTriplets(self):
         _check(lib().lst_event_create_triplets(self.h))
 
+    def createQuintuplets(self):
+        _check(lib().lst_event_create_quintuplets(self.h))
+
     def resetEvent(self):
         _check(lib().lst_event_reset(self.h))
 
@@ -158,6 +162,9 @@ class Event:
 
     def getNumberOfTriplets(self):
         return lib().lst_event_count(self.h, b"triplets")
+
+    def getNumberOfQuintuplets(self):
+        return lib().lst_event_count(self.h, b"quintuplets")
 
     def getNumberOfPixelSegments(self):
         return lib().lst_event_count(self.h, b"pixelSegments")

@@ -23,6 +23,7 @@ SOURCES = [
     "es.cpp",
     "prepare.cpp",
     "stages_hits_md_ls_t3.hip",
+    "stages_t5.hip",
     "engine.hip",
 ]
 

@@ -82,7 +82,7 @@ struct lst_event {
   int stage = -1;  // last completed stage
   lst::EventView v{};
   // capacities
-  uint32_t hitCap = 0, mdCap = 0, segCap = 0, plsCap = lst::kNMaxPixelSegments, t3Cap = 0;
+  uint32_t hitCap = 0, mdCap = 0, segCap = 0, plsCap = lst::kNMaxPixelSegments, t3Cap = 0, t5Cap = 0;
   char* arena = nullptr;
   size_t arenaBytes = 0;
   uint32_t* plsHitIdx[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -133,6 +133,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   mdCap = g.nTotalMDCap + lst::kNMaxPixelMDs;
   segCap = g.nTotalSegCap + lst::kNMaxPixelSegments;
   t3Cap = (uint32_t)g.t3CapBound;
+  t5Cap = (uint32_t)g.t5CapBound;
   Carve c;
   lst::EventView& e = v;
 #define CARVE(TY, NM, MU) c.take(e.H.NM, (size_t)hitCap * MU);
@@ -150,6 +151,9 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #define CARVE(TY, NM, MU) c.take(e.T.NM, (size_t)(t3Cap ? t3Cap : 1) * MU);
   LST_T3_FIELDS(CARVE)
 #undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.Q.NM, (size_t)(t5Cap ? t5Cap : 1) * MU);
+  LST_T5_FIELDS(CARVE)
+#undef CARVE
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);
   c.take(e.R.mdRanges, 2 * nMod); c.take(e.R.segmentRanges, 2 * nMod); c.take(e.R.tripletRanges, 2 * nMod);
@@ -158,6 +162,9 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.R.nMDs, nL + 1); c.take(e.R.totOccupancyMDs, nL + 1);
   c.take(e.R.nSegments, nL + 1); c.take(e.R.totOccupancySegments, nL + 1);
   c.take(e.R.nTriplets, nL); c.take(e.R.totOccupancyTriplets, nL);
+  c.take(e.R.quintupletModuleIndices, nL); c.take(e.R.quintupletModuleOccupancy, nL);
+  c.take(e.R.nQuintuplets, nL); c.take(e.R.totOccupancyQuintuplets, nL);
+  c.take(e.R.indicesOfEligibleT5Modules, nL);
   c.take(e.R.scalars, lst::kNScalars);
   for (int k = 0; k < 4; k++) c.take(plsHitIdx[k], plsCap);
   c.take(plsDPhiChange, plsCap);
@@ -170,6 +177,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   e.segModuleIndices = g.segIndices;
   e.segModuleOccupancy = g.segOccupancy;
   e.t3OccIfAny = g.t3OccIfAny;
+  e.t5OccIfAny = g.t5OccIfAny;
+  e.nT5SeedModules = g.nT5SeedModules;
   e.pixelModule = g.pixelModuleIndex;
   e.ptCut = es->ptCut;
 
@@ -211,6 +220,12 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     auto* cur = &e.T;
     LST_T3_FIELDS(FIELD)
   }
+  {
+    std::string prefix = "t5.";
+    size_t capacity = t5Cap;
+    auto* cur = &e.Q;
+    LST_T5_FIELDS(FIELD)
+  }
 #undef FIELD
 #undef REG
   add("hits.hitRanges", e.R.hitRanges, 2 * nMod);
@@ -234,6 +249,11 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   add("segs.totOccupancySegments", e.R.totOccupancySegments, nL + 1);
   add("t3.nTriplets", e.R.nTriplets, nL);
   add("t3.totOccupancyTriplets", e.R.totOccupancyTriplets, nL);
+  add("ranges.quintupletModuleIndices", e.R.quintupletModuleIndices, nL);
+  add("ranges.quintupletModuleOccupancy", e.R.quintupletModuleOccupancy, nL);
+  add("ranges.indicesOfEligibleT5Modules", e.R.indicesOfEligibleT5Modules, nL);
+  add("t5.nQuintuplets", e.R.nQuintuplets, nL);
+  add("t5.totOccupancyQuintuplets", e.R.totOccupancyQuintuplets, nL);
   add("scalars", e.R.scalars, lst::kNScalars);
   return LST_OK;
 }
@@ -408,6 +428,15 @@ lst_status lst_event_create_triplets(lst_event* ev) {
   return LST_OK;
 }
 
+lst_status lst_event_create_quintuplets(lst_event* ev) {
+  if (!ev || ev->stage != 4) return fail(LST_E_ORDER, "createQuintuplets needs triplets");
+  if (ev->t5Mode != LST_T5_PLAIN_CHI2) return fail(LST_E_INPUT, "T5 DNN selection is not available in this build");
+  lst::launch_quintuplets(ev->v, ev->stream, ev->timer(5));
+  HIP_OK(hipGetLastError());
+  ev->stage = 5;
+  return LST_OK;
+}
+
 lst_status lst_event_reset(lst_event* ev) {
   if (!ev) return fail(LST_E_INPUT, "null event");
   ev->stage = -1;
@@ -432,6 +461,7 @@ uint32_t lst_event_count(lst_event* ev, const char* what) {
   if (!std::strcmp(what, "miniDoublets")) key = "mds.nMDs";
   else if (!std::strcmp(what, "segments")) key = "segs.nSegments";
   else if (!std::strcmp(what, "triplets")) key = "t3.nTriplets";
+  else if (!std::strcmp(what, "quintuplets")) key = "t5.nQuintuplets";
   if (!key) return 0;
   const FieldRef* f = ev->field(key);
   std::vector<uint32_t> h(n);
@@ -469,7 +499,8 @@ lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
   if ((warn & 0xff) != LST_OK || last_stage < 2) return warn;
   if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
   if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_triplets(ev)) != LST_OK) return s;
+  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_quintuplets(ev)) != LST_OK) return s;
   return warn;
 }
 
@@ -525,7 +556,8 @@ lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
   if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s;
   if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
   if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
-  return lst_event_create_triplets(ev);
+  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s;
+  return lst_event_create_quintuplets(ev);
 }
 
 lst_status lst_event_set_timing(lst_event* ev, int on) {

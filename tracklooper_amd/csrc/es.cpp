@@ -250,6 +250,7 @@ bool load_host_geometry(const std::string& dir, uint32_t stride, HostGeometry& g
   g.mdOccupancy.assign(nLower + 1, 0);
   g.segOccupancy.assign(nLower + 1, 0);
   g.t3OccIfAny.assign(nLower, 0);
+  g.t5OccIfAny.assign(nLower, -1);
   for (uint32_t i = 0; i < nLower; i++) {
     int cat, eb;
     category(g.layers[i], g.rings[i], g.subdets[i], std::fabs(g.eta[i]), cat, eb);
@@ -273,6 +274,15 @@ bool load_host_geometry(const std::string& dir, uint32_t stride, HostGeometry& g
     else if (cat == 1) t3 = eb == 0 ? 755 : eb == 1 ? 347 : 0;
     else if (cat == 3) t3 = eb == 1 ? 38 : eb == 2 ? 46 : eb == 3 ? 39 : 0;
     g.t3OccIfAny[i] = t3;
+    // T5 seed modules and their table (SDL/Quintuplet.h:3129-3196): barrel
+    // layers 1-2 and endcap disk 1 qualify once they hold a triplet
+    const bool seed = (g.subdets[i] == 5 && g.layers[i] < 3) || (g.subdets[i] == 4 && g.layers[i] <= 1);
+    if (seed) {
+      int t5 = 0;
+      if (cat == 0) t5 = eb == 0 ? 336 : eb == 1 ? 414 : eb == 2 ? 231 : eb == 3 ? 146 : 0;
+      else if (cat == 3) t5 = eb == 2 ? 191 : eb == 3 ? 106 : 0;
+      g.t5OccIfAny[i] = t5;
+    }
   }
   // the pixel module's MD capacity is N_MAX_PIXEL_MD_PER_MODULES (SDL/Event.cc:398-407)
   g.mdOccupancy[nLower] = kNMaxPixelMDs;
@@ -314,7 +324,8 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
          o_gp = c.add(h.geoMapPhi), o_pi0 = c.add(h.pixIndex[0]), o_pi1 = c.add(h.pixIndex[1]),
          o_pi2 = c.add(h.pixIndex[2]), o_ps0 = c.add(h.pixSizes[0]), o_ps1 = c.add(h.pixSizes[1]),
          o_ps2 = c.add(h.pixSizes[2]), o_mdo = c.add(h.mdOccupancy), o_sgo = c.add(h.segOccupancy),
-         o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny);
+         o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny),
+         o_t5o = c.add(h.t5OccIfAny);
   char* base = nullptr;
   if (hipMalloc(&base, c.total) != hipSuccess) { err = "hipMalloc failed for geometry"; return false; }
   for (auto& it : c.items)
@@ -346,12 +357,20 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   d.mdOccupancy = (const int32_t*)P(o_mdo); d.segOccupancy = (const int32_t*)P(o_sgo);
   d.mdIndices = (const int32_t*)P(o_mdi); d.segIndices = (const int32_t*)P(o_sgi);
   d.t3OccIfAny = (const int32_t*)P(o_t3o);
+  d.t5OccIfAny = (const int32_t*)P(o_t5o);
   d.pixelModuleIndex = h.pixelModuleIndex;
   d.nPixels = h.nPixels;
   d.nTotalMDCap = h.mdIndices[h.nLowerModules];
   d.nTotalSegCap = h.segIndices[h.nLowerModules];
   d.t3CapBound = 0;
   for (auto v2 : h.t3OccIfAny) d.t3CapBound += v2;
+  d.t5CapBound = 0;
+  d.nT5SeedModules = 0;
+  for (auto v2 : h.t5OccIfAny)
+    if (v2 >= 0) {
+      d.t5CapBound += v2;
+      d.nT5SeedModules++;
+    }
   return true;
 }
 

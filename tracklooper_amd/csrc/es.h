@@ -28,6 +28,9 @@ struct HostGeometry {
   std::vector<int32_t> mdOccupancy, segOccupancy;      // [nLower + 1]
   std::vector<int32_t> mdIndices, segIndices;          // exclusive scans, [nLower + 1]
   std::vector<int32_t> t3OccIfAny;                     // [nLower]
+  // T5 capacity of a module that holds triplets; -1 where the module can never
+  // seed a T5 (barrel layers 3-6, endcap disks 2-5)
+  std::vector<int32_t> t5OccIfAny;                     // [nLower]
 };
 
 // Loads the reference file formats (SDL/ModuleMethods.h:191-226,
@@ -49,8 +52,11 @@ struct DeviceGeometry {
   const int32_t* mdIndices = nullptr;
   const int32_t* segIndices = nullptr;
   const int32_t* t3OccIfAny = nullptr;
+  const int32_t* t5OccIfAny = nullptr;
   int32_t nTotalMDCap = 0, nTotalSegCap = 0;  // outer-tracker totals (pixel excluded)
   int64_t t3CapBound = 0;                      // T3 capacity if every module has segments
+  int64_t t5CapBound = 0;                      // T5 capacity if every T5 seed module has triplets
+  uint32_t nT5SeedModules = 0;                 // modules with t5OccIfAny >= 0
 };
 
 bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err);

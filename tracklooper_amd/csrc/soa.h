@@ -2,7 +2,7 @@
 // drives the struct members, the arena carving and the by-name getters, so the
 // three can never disagree.  Names follow the reference structs
 // (SDL/Hit.h:13-56, SDL/MiniDoublet.h:16-53, SDL/Segment.h:17-50,
-// SDL/Triplet.h:17-31) so tests compare fields by name against the oracle.
+// SDL/Triplet.h:17-31, SDL/Quintuplet.h:20-48) so tests compare fields by name against the oracle.
 #pragma once
 #include <cstdint>
 
@@ -43,6 +43,14 @@
   X(uint32_t, hitIndices, 6) X(float, betaIn, 1) X(float, circleRadius, 1) X(float, circleCenterX, 1)      \
   X(float, circleCenterY, 1) X(uint8_t, partOfPT5, 1) X(uint8_t, partOfT5, 1) X(uint8_t, partOfPT3, 1)
 
+#define LST_T5_FIELDS(X)                                                                                   \
+  X(uint32_t, tripletIndices, 2) X(uint16_t, lowerModuleIndices, 5) X(uint8_t, logicalLayers, 5)           \
+  X(uint32_t, hitIndices, 10) X(uint8_t, layer, 1) X(int8_t, isDup, 1) X(uint8_t, TightCutFlag, 1)        \
+  X(uint8_t, partOfPT5, 1) X(float, innerRadius, 1) X(float, bridgeRadius, 1) X(float, outerRadius, 1)     \
+  X(float, pt, 1) X(float, eta, 1) X(float, phi, 1) X(float, score_rphisum, 1) X(float, regressionRadius, 1) \
+  X(float, regressionG, 1) X(float, regressionF, 1) X(float, rzChiSquared, 1) X(float, chiSquared, 1)      \
+  X(float, nonAnchorChiSquared, 1)
+
 #define LST_DECL_FIELD(T, N, M) T* N;
 
 namespace lst {
@@ -51,6 +59,7 @@ struct MDsSoA { LST_MD_FIELDS(LST_DECL_FIELD) };
 struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
+struct T5SoA { LST_T5_FIELDS(LST_DECL_FIELD) };
 
 // Per-module counters and ranges (SDL/Module.h:21-85, SDL/Hit.h:28-32).
 struct RangesSoA {
@@ -71,7 +80,12 @@ struct RangesSoA {
   uint32_t* totOccupancySegments;
   uint32_t* nTriplets;       // [nLower]
   uint32_t* totOccupancyTriplets;
+  int32_t* quintupletModuleIndices;    // [nLower], -1 where no T5 can start
+  int32_t* quintupletModuleOccupancy;  // [nLower]
+  uint32_t* nQuintuplets;              // [nLower]
+  uint32_t* totOccupancyQuintuplets;   // [nLower]
+  uint16_t* indicesOfEligibleT5Modules;  // [nLower]
   uint32_t* scalars;         // [16] device-side totals (see Scalar)
 };
-enum Scalar : int { kNTotalTrips = 0, kBadHits = 1, kNScalars = 16 };
+enum Scalar : int { kNTotalTrips = 0, kBadHits = 1, kNEligibleT5 = 2, kNTotalQuints = 3, kNScalars = 16 };
 }  // namespace lst

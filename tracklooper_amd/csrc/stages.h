@@ -13,12 +13,15 @@ struct EventView {  // everything a stage kernel needs, by value
   SegsSoA S;
   PLSSoA P;
   T3SoA T;
+  T5SoA Q;
   RangesSoA R;
   const int32_t* mdModuleIndices;    // [nLower + 1], geometry-only
   const int32_t* mdModuleOccupancy;  // [nLower + 1]
   const int32_t* segModuleIndices;   // [nLower + 1]
   const int32_t* segModuleOccupancy; // [nLower + 1]
   const int32_t* t3OccIfAny;         // [nLower] T3 capacity of a module that has segments
+  const int32_t* t5OccIfAny;         // [nLower] T5 capacity of a seed module, -1 if none
+  uint32_t nT5SeedModules;           // geometry bound on the eligible T5 module count
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };
@@ -26,8 +29,8 @@ struct EventView {  // everything a stage kernel needs, by value
 // Optional hipEvent pair recorded around the stage's main kernel (timing).
 struct KernelTimer {
   hipEvent_t start = nullptr, stop = nullptr;
-  void begin(hipStream_t s) const { if (start) hipEventRecord(start, s); }
-  void end(hipStream_t s) const { if (stop) hipEventRecord(stop, s); }
+  void begin(hipStream_t s) const { if (start) (void)hipEventRecord(start, s); }
+  void end(hipStream_t s) const { if (stop) (void)hipEventRecord(stop, s); }
 };
 
 void launch_hits(const EventView& v, hipStream_t s, const KernelTimer& t = {});
@@ -36,5 +39,6 @@ void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const 
 void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 
 }  // namespace lst

@@ -1,0 +1,316 @@
+// Quintuplet stage: eligible-module scan, builder, after-build duplicate
+// removal and ranges (SDL/Event.cc:903-991).
+//
+// The builder runs one 256-lane workgroup per eligible module.  Candidates
+// (inner T3 a of the module, outer T3 b on the inner T3's third module) are
+// linearised in the reference's serial order (a-major,
+// SDL/Quintuplet.h:2988-3115), evaluated 256 at a time and ranked with a
+// workgroup scan, so T5 slots equal the reference CPU slots and the per-module
+// capacity keeps exactly the first `cap` passing candidates.
+#include "algo_t5.h"
+#include "stages.h"
+#include "wave.h"
+
+namespace lst {
+namespace {
+
+constexpr int kT5Block = 256;
+constexpr int kMaxInnerT3 = 1024;   // > the largest T3 module capacity (755)
+constexpr int kMaxT5PerModule = 416;  // >= the largest T5 module capacity (414)
+
+// createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
+// seed modules holding a triplet, their capacities and the exclusive scan of
+// the capacities in module order.
+__global__ __launch_bounds__(1024) void k_t5_eligible(EventView v) {
+  __shared__ unsigned lds[2][17];
+  unsigned carryOcc = 0, carryElig = 0;
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t b = 0; b < nL; b += 1024) {
+    const uint32_t m = b + threadIdx.x;
+    const int occIf = m < nL ? v.t5OccIfAny[m] : -1;
+    const bool elig = occIf >= 0 && v.R.nTriplets[m] != 0;
+    const unsigned occ = elig ? (unsigned)occIf : 0u;
+    unsigned totOcc, totElig;
+    const unsigned exOcc = block_exclusive_scan<1024>(occ, lds[0], totOcc);
+    const unsigned exElig = block_exclusive_scan<1024>(elig ? 1u : 0u, lds[1], totElig);
+    if (m < nL) {
+      v.R.quintupletModuleIndices[m] = elig ? (int)(carryOcc + exOcc) : -1;
+      v.R.quintupletModuleOccupancy[m] = (int)occ;
+      v.R.nQuintuplets[m] = 0;
+      v.R.totOccupancyQuintuplets[m] = 0;
+      if (elig) v.R.indicesOfEligibleT5Modules[carryElig + exElig] = (uint16_t)m;
+    }
+    carryOcc += totOcc;
+    carryElig += totElig;
+  }
+  if (threadIdx.x == 0) {
+    v.R.scalars[kNEligibleT5] = carryElig;
+    v.R.scalars[kNTotalQuints] = carryOcc;
+  }
+}
+
+// brute-force radius range when any of the three points is an endcap 2S
+// mini-doublet; with three plain points the 27 evaluations coincide
+__device__ __forceinline__ void radius_range_of(const ModLite& ma, const MDPoint& a, const ModLite& mb,
+                                                const MDPoint& b, const ModLite& mc, const MDPoint& c, float& rmin,
+                                                float& rmax) {
+  const bool ea = is_endcap_2s(ma), eb = is_endcap_2s(mb), ec = is_endcap_2s(mc);
+  if (!(ea || eb || ec)) {
+    float g, f;
+    const float r = circle_radius3(a.x, a.y, b.x, b.y, c.x, c.y, g, f);
+    rmin = m_min(r, kSdlInf);
+    rmax = m_max(r, 0.f);
+    return;
+  }
+  const float xa[3] = {a.x, ea ? a.lx : a.x, ea ? a.hx : a.x}, ya[3] = {a.y, ea ? a.ly : a.y, ea ? a.hy : a.y};
+  const float xb[3] = {b.x, eb ? b.lx : b.x, eb ? b.hx : b.x}, yb[3] = {b.y, eb ? b.ly : b.y, eb ? b.hy : b.y};
+  const float xc[3] = {c.x, ec ? c.lx : c.x, ec ? c.hx : c.x}, yc[3] = {c.y, ec ? c.ly : c.y, ec ? c.hy : c.y};
+  radius_range(xa, ya, xb, yb, xc, yc, rmin, rmax);
+}
+
+// subdetector pattern of the five modules -> radius-matching category
+__device__ __forceinline__ int radii_category(const ModLite* m) {
+  const bool B0 = m[0].subdet == kBarrel, B1 = m[1].subdet == kBarrel, B2 = m[2].subdet == kBarrel,
+             B3 = m[3].subdet == kBarrel, B4 = m[4].subdet == kBarrel;
+  const bool E1 = m[1].subdet == kEndcap, E2 = m[2].subdet == kEndcap, E3 = m[3].subdet == kEndcap,
+             E4 = m[4].subdet == kEndcap;
+  if (B0 && B1 && B2 && B3 && B4) return 0;
+  if (B0 && B1 && B2 && B3 && E4) return 1;
+  if (B0 && B1 && B2 && E3 && E4) return m[0].layer == 1 ? 2 : (m[0].layer == 2 ? 3 : 4);
+  if (B0 && B1 && E2 && E3 && E4) return 5;
+  if (B0 && E1 && E2 && E3 && E4) return 6;
+  return 7;
+}
+
+// runQuintupletDefaultAlgo, plain chi2 mode (SDL/Quintuplet.h:2508-2986)
+__device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t innerT3, uint32_t outerT3,
+                             uint32_t* md, T5Result& r) {
+  const uint32_t seg1 = v.T.segmentIndices[2 * innerT3], seg2 = v.T.segmentIndices[2 * innerT3 + 1];
+  const uint32_t seg3 = v.T.segmentIndices[2 * outerT3], seg4 = v.T.segmentIndices[2 * outerT3 + 1];
+  if (v.S.mdIndices[2 * seg2 + 1] != v.S.mdIndices[2 * seg3]) return false;
+  md[0] = v.S.mdIndices[2 * seg1];
+  md[1] = v.S.mdIndices[2 * seg2];
+  md[2] = v.S.mdIndices[2 * seg2 + 1];
+  md[3] = v.S.mdIndices[2 * seg3 + 1];
+  md[4] = v.S.mdIndices[2 * seg4 + 1];
+  ModLite m[5];
+  MDPoint p[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    m[k] = load_lite(v.M, mods[k]);
+    p[k] = load_point(v.D, md[k]);
+  }
+  const SegAlpha a1 = load_alpha(v.S, seg1);
+  {
+    T4Args q{m[0], m[1], m[2], m[3], p[0], p[1], p[2], p[3], a1, load_alpha(v.S, seg3)};
+    if (!t4_select(q, v.ptCut)) return false;
+  }
+  {
+    T4Args q{m[0], m[1], m[3], m[4], p[0], p[1], p[3], p[4], a1, load_alpha(v.S, seg4)};
+    if (!t4_select(q, v.ptCut)) return false;
+  }
+  float g, f;
+  r.outerR = v.T.circleRadius[outerT3];
+  r.bridgeR = circle_radius3(p[1].x, p[1].y, p[2].x, p[2].y, p[3].x, p[3].y, g, f);
+  r.innerR = v.T.circleRadius[innerT3];
+  g = v.T.circleCenterX[innerT3];
+  f = v.T.circleCenterY[innerT3];
+  const float innerPt = 2 * k2Rinv1GeVf * r.innerR;
+  r.tight = false;
+  if (!t5_rz_chi2(m, p, innerPt, r.innerR, g, f, r.rzChi2, r.tight)) return false;
+  if (!(r.innerR >= 0.95f * v.ptCut / (2.f * k2Rinv1GeVf))) return false;
+  const int cat = radii_category(m);
+  float iMin = 0, iMax = 0, bMin = 0, bMax = 0;
+  if (cat >= 6) radius_range_of(m[0], p[0], m[1], p[1], m[2], p[2], iMin, iMax);
+  if (cat >= 2) radius_range_of(m[1], p[1], m[2], p[2], m[3], p[3], bMin, bMax);
+  if (!radii_compatible(cat, r.innerR, r.bridgeR, iMin, iMax, bMin, bMax)) return false;
+  float x[5], y[5], d1[5], d2[5], sl[5];
+  bool flat[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    x[k] = p[k].x;
+    y[k] = p[k].y;
+  }
+  regression_sigmas(m, true, d1, d2, sl, flat);
+  r.regR = regression_fit(x, y, d1, d2, sl, flat, r.regG, r.regF, r.chi2);
+  if (r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
+      !t5_rphi_chi2_ok(m[0].sdl, m[1].sdl, m[2].sdl, m[3].sdl, m[4].sdl, r.chi2))
+    return false;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    x[k] = v.D.outerX[md[k]];
+    y[k] = v.D.outerY[md[k]];
+  }
+  regression_sigmas(m, false, d1, d2, sl, flat);
+  r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
+  return true;
+}
+
+__global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
+  __shared__ uint32_t pre[kMaxInnerT3 + 1];
+  __shared__ unsigned lds[kT5Block / 64 + 1];
+  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
+  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const int layer = v.M.layers[m1];
+  const int cap = v.R.quintupletModuleOccupancy[m1];
+  const uint32_t nIn = v.R.nTriplets[m1];
+  unsigned count = 0;
+  if ((layer == 1 || layer == 2) && nIn <= (uint32_t)kMaxInnerT3) {
+    const int adj = layer == 1 ? 1 : 0;
+    const uint32_t t3Base = v.R.tripletModuleIndices[m1];
+    // pre[a] = number of candidates before inner triplet a
+    uint32_t run = 0;
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kT5Block) {
+      const uint32_t a = a0 + threadIdx.x;
+      const uint32_t cnt = a < nIn ? v.R.nTriplets[v.T.lowerModuleIndices[3 * (t3Base + a) + 2]] : 0;
+      unsigned tot;
+      const unsigned ex = block_exclusive_scan<kT5Block>(cnt, lds, tot);
+      if (a < nIn) pre[a + 1] = run + ex + cnt;
+      run += tot;
+    }
+    if (threadIdx.x == 0) pre[0] = 0;
+    __syncthreads();
+    const uint32_t total = run;
+    const uint32_t t5Base = (uint32_t)v.R.quintupletModuleIndices[m1];
+    for (uint32_t c0 = 0; c0 < total; c0 += kT5Block) {
+      const uint32_t L = c0 + threadIdx.x;
+      bool pass = false;
+      uint32_t innerT3 = 0, outerT3 = 0, md[5];
+      uint16_t mods[5];
+      T5Result r;
+      if (L < total) {
+        uint32_t lo = 0, hi = nIn;  // largest a with pre[a] <= L
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (pre[mid] <= L) lo = mid;
+          else hi = mid;
+        }
+        innerT3 = t3Base + lo;
+        mods[0] = m1;
+        mods[1] = v.T.lowerModuleIndices[3 * innerT3 + 1];
+        mods[2] = v.T.lowerModuleIndices[3 * innerT3 + 2];
+        outerT3 = (uint32_t)v.R.tripletModuleIndices[mods[2]] + (L - pre[lo]);
+        mods[3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
+        mods[4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
+        pass = t5_candidate(v, mods, innerT3, outerT3, md, r);
+      }
+      unsigned n;
+      const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, n);
+      if (pass && (int)rank < cap) {
+        const uint32_t q = t5Base + rank;
+        // addQuintupletToMemory (SDL/Quintuplet.h:150-215)
+        const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * innerT3 + adj]];
+        v.Q.tripletIndices[2 * q] = innerT3;
+        v.Q.tripletIndices[2 * q + 1] = outerT3;
+#pragma unroll
+        for (int j = 0; j < 5; j++) v.Q.lowerModuleIndices[5 * q + j] = mods[j];
+        v.Q.innerRadius[q] = r.innerR;
+        v.Q.outerRadius[q] = r.outerR;
+        v.Q.bridgeRadius[q] = r.bridgeR;
+        v.Q.pt[q] = (r.innerR + r.outerR) * 3.8f * 1.602f / (2 * 100 * 5.39f);
+        v.Q.eta[q] = v.D.anchorEta[mdAdj];
+        v.Q.phi[q] = v.D.anchorPhi[mdAdj];
+        v.Q.score_rphisum[q] = r.chi2 + r.nonAnchorChi2;
+        v.Q.layer[q] = (uint8_t)layer;
+        v.Q.isDup[q] = 0;
+        v.Q.TightCutFlag[q] = r.tight;
+        v.Q.partOfPT5[q] = 0;
+        v.Q.regressionRadius[q] = r.regR;
+        v.Q.regressionG[q] = r.regG;
+        v.Q.regressionF[q] = r.regF;
+        v.Q.rzChiSquared[q] = r.rzChi2;
+        v.Q.chiSquared[q] = r.chi2;
+        v.Q.nonAnchorChiSquared[q] = r.nonAnchorChi2;
+        v.Q.logicalLayers[5 * q + 0] = v.T.logicalLayers[3 * innerT3];
+        v.Q.logicalLayers[5 * q + 1] = v.T.logicalLayers[3 * innerT3 + 1];
+        v.Q.logicalLayers[5 * q + 2] = v.T.logicalLayers[3 * innerT3 + 2];
+        v.Q.logicalLayers[5 * q + 3] = v.T.logicalLayers[3 * outerT3 + 1];
+        v.Q.logicalLayers[5 * q + 4] = v.T.logicalLayers[3 * outerT3 + 2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) v.Q.hitIndices[10 * q + j] = v.T.hitIndices[6 * innerT3 + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v.Q.hitIndices[10 * q + 6 + j] = v.T.hitIndices[6 * outerT3 + 2 + j];
+        v.T.partOfT5[innerT3] = 1;
+        v.T.partOfT5[outerT3] = 1;
+      }
+      count += n;
+    }
+  }
+  if (threadIdx.x == 0) {
+    v.R.totOccupancyQuintuplets[m1] = count;
+    v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249).  For a close
+// pair (i < j) sharing >= 7 hits the reference flags i when score_i >= score_j
+// and j otherwise; each lane here owns one T5 and tests every partner, so the
+// flags are written without atomics and do not depend on evaluation order.
+__global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
+  __shared__ float sEta[kMaxT5PerModule], sPhi[kMaxT5PerModule], sScore[kMaxT5PerModule];
+  __shared__ uint32_t sHits[kMaxT5PerModule * 10];
+  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
+  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const uint32_t nq = v.R.nQuintuplets[m];
+  if (nq < 2 || nq > (uint32_t)kMaxT5PerModule) return;
+  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
+  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
+    sEta[i] = v.Q.eta[base + i];
+    sPhi[i] = v.Q.phi[base + i];
+    sScore[i] = v.Q.score_rphisum[base + i];
+  }
+  for (uint32_t i = threadIdx.x; i < nq * 10; i += kT5Block) sHits[i] = v.Q.hitIndices[10 * base + i];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
+    bool dup = false;
+    for (uint32_t j = 0; j < nq && !dup; j++) {
+      if (j == i) continue;
+      const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+      const float dEta = m_abs(sEta[lo] - sEta[hi]);
+      const float dPhi = wrap_dphi(sPhi[lo], sPhi[hi]);
+      if (dEta > 0.1f) continue;
+      if (m_abs(dPhi) > 0.1f) continue;
+      int matched = 0;
+      for (int a = 0; a < 10; a++) {
+        const uint32_t h = sHits[10 * lo + a];
+        for (int b = 0; b < 10; b++)
+          if (h == sHits[10 * hi + b]) {
+            matched++;
+            break;
+          }
+      }
+      if (matched >= 7) dup = (sScore[lo] >= sScore[hi]) ? (i == lo) : (i == hi);
+    }
+    if (dup) v.Q.isDup[base + i] = 1;
+  }
+}
+
+// addQuintupletRangesToEventExplicit (SDL/Quintuplet.h:3215-3235)
+__global__ void k_t5_ranges(EventView v) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+    const uint32_t n = v.R.nQuintuplets[m];
+    const int base = v.R.quintupletModuleIndices[m];
+    if (n == 0 || base == -1) {
+      v.R.quintupletRanges[2 * m] = -1;
+      v.R.quintupletRanges[2 * m + 1] = -1;
+    } else {
+      v.R.quintupletRanges[2 * m] = base;
+      v.R.quintupletRanges[2 * m + 1] = base + (int)n - 1;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+  const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
+  hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
+  t.begin(s);
+  hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  t.end(s);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  const unsigned g = (v.M.nLowerModules + 255) / 256;
+  hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
+}
+
+}  // namespace lst
