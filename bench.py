@@ -25,6 +25,11 @@ KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "
 ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5"]
 LAST_STAGE = 5  # quintuplets: the stages built so far
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
+# benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
+# Re-measured whenever a kernel changes; None for kernels not yet profiled.
+MEASURED_TRAFFIC = {"k_quintuplets": 55815210, "k_segments": 58299420, "k_triplets": 41180815,
+                    "k_mini_doublets": 36042844, "k_hits": 10494688, "k_t5_dup_after_build": 1343613}
 
 
 def _input_struct(lst, ev):
@@ -83,6 +88,8 @@ def main():
     ap.add_argument("--n-pu", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (reference for FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (each box has one MI355X); --gpus {args.gpus} is not supported.",
@@ -91,6 +98,15 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_calibration:
+        x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 1 GiB device-to-device 3 times", flush=True)
+        return
     import tracklooper_amd as lst
     from tracklooper_amd import synth
 
@@ -176,7 +192,7 @@ def main():
             "kernel_ms": per_kernel,
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dom),
                      "algorithmic_bytes": dom_bytes},
     }
     if not args.no_cpu_baseline:

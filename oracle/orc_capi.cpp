@@ -102,7 +102,8 @@ int orc_prepare(void* hp, unsigned nSeeds, const float* px, const float* py, con
 }
 
 // Stage sequence of SDL::LST::run (SDL/LST.cc:34-160); runs stages [0, upto).
-// 0 hits, 1 pixel segments, 2 MD, 3 LS, 4 T3.
+// 0 hits, 1 pixel segments, 2 MD, 3 LS, 4 T3, 5 T5, 6 pLS cleaning, 7 pT5,
+// 8 pT3, 9 track candidates and the output lists.
 int orc_run(void* hp, int upto) {
   auto* h = static_cast<Handle*>(hp);
   Event& ev = h->ev;
@@ -112,6 +113,13 @@ int orc_run(void* hp, int upto) {
   if (upto > 3) event_create_segments(ev);
   if (upto > 4) event_create_triplets(ev);
   if (upto > 5) event_create_quintuplets(ev);
+  if (upto > 6) event_pixel_segment_cleaning(ev);
+  if (upto > 7) event_create_pixel_quintuplets(ev);
+  if (upto > 8) event_create_pixel_triplets(ev);
+  if (upto > 9) {
+    event_create_track_candidates(ev);
+    event_get_output(ev);
+  }
   return 0;
 }
 
@@ -128,6 +136,15 @@ unsigned orc_event_scalar(void* hp, const char* key) {
   if (!std::strcmp(key, "t5Capacity")) return ev.t5Capacity;
   if (!std::strcmp(key, "nEligibleT5Modules")) return ev.nEligibleT5Modules;
   if (!std::strcmp(key, "nTotalQuints")) return ev.nTotalQuints;
+  if (!std::strcmp(key, "nPixelQuintuplets")) return ev.nPixelQuintuplets;
+  if (!std::strcmp(key, "totOccupancyPixelQuintuplets")) return ev.totOccupancyPixelQuintuplets;
+  if (!std::strcmp(key, "nPixelTriplets")) return ev.nPixelTriplets;
+  if (!std::strcmp(key, "totOccupancyPixelTriplets")) return ev.totOccupancyPixelTriplets;
+  if (!std::strcmp(key, "nTrackCandidates")) return ev.nTrackCandidates;
+  if (!std::strcmp(key, "nTrackCandidatespT3")) return ev.nTrackCandidatespT3;
+  if (!std::strcmp(key, "nTrackCandidatespT5")) return ev.nTrackCandidatespT5;
+  if (!std::strcmp(key, "nTrackCandidatespLS")) return ev.nTrackCandidatespLS;
+  if (!std::strcmp(key, "nTrackCandidatesT5")) return ev.nTrackCandidatesT5;
   return 0;
 }
 
@@ -211,6 +228,31 @@ int orc_event_field(void* hp, const char* key, const void** p, int* t, long* n) 
   F("t5.rzChiSquared", ev.t5RzChiSquared); F("t5.chiSquared", ev.t5ChiSquared);
   F("t5.nonAnchorChiSquared", ev.t5NonAnchorChiSquared);
   F("ranges.indicesOfEligibleT5Modules", ev.indicesOfEligibleT5Modules);
+  // pixel quintuplets
+  F("pt5.pixelIndices", ev.pt5PixelIndices); F("pt5.T5Indices", ev.pt5T5Indices); F("pt5.isDup", ev.pt5IsDup);
+  F("pt5.score", ev.pt5Score); F("pt5.eta", ev.pt5Eta); F("pt5.phi", ev.pt5Phi);
+  F("pt5.logicalLayers", ev.pt5LogicalLayers); F("pt5.hitIndices", ev.pt5HitIndices);
+  F("pt5.lowerModuleIndices", ev.pt5LowerModuleIndices); F("pt5.pixelRadius", ev.pt5PixelRadius);
+  F("pt5.quintupletRadius", ev.pt5QuintupletRadius); F("pt5.centerX", ev.pt5CenterX); F("pt5.centerY", ev.pt5CenterY);
+  F("pt5.rzChiSquared", ev.pt5RzChiSquared); F("pt5.rPhiChiSquared", ev.pt5RPhiChiSquared);
+  F("pt5.rPhiChiSquaredInwards", ev.pt5RPhiChiSquaredInwards);
+  // pixel triplets
+  F("pt3.pixelSegmentIndices", ev.pt3PixelSegmentIndices); F("pt3.tripletIndices", ev.pt3TripletIndices);
+  F("pt3.pixelRadius", ev.pt3PixelRadius); F("pt3.tripletRadius", ev.pt3TripletRadius); F("pt3.pt", ev.pt3Pt);
+  F("pt3.eta", ev.pt3Eta); F("pt3.phi", ev.pt3Phi); F("pt3.eta_pix", ev.pt3EtaPix); F("pt3.phi_pix", ev.pt3PhiPix);
+  F("pt3.score", ev.pt3Score); F("pt3.isDup", ev.pt3IsDup); F("pt3.logicalLayers", ev.pt3LogicalLayers);
+  F("pt3.hitIndices", ev.pt3HitIndices); F("pt3.lowerModuleIndices", ev.pt3LowerModuleIndices);
+  F("pt3.centerX", ev.pt3CenterX); F("pt3.centerY", ev.pt3CenterY); F("pt3.rzChiSquared", ev.pt3RzChiSquared);
+  F("pt3.rPhiChiSquared", ev.pt3RPhiChiSquared); F("pt3.rPhiChiSquaredInwards", ev.pt3RPhiChiSquaredInwards);
+  // track candidates
+  F("tc.trackCandidateType", ev.tcType); F("tc.directObjectIndices", ev.tcDirectObjectIndices);
+  F("tc.objectIndices", ev.tcObjectIndices); F("tc.hitIndices", ev.tcHitIndices);
+  F("tc.pixelSeedIndex", ev.tcPixelSeedIndex); F("tc.logicalLayers", ev.tcLogicalLayers);
+  F("tc.lowerModuleIndices", ev.tcLowerModuleIndices); F("tc.centerX", ev.tcCenterX); F("tc.centerY", ev.tcCenterY);
+  F("tc.radius", ev.tcRadius);
+  // LST output lists
+  F("out.hitIdxs", ev.outHitIdxs); F("out.hitOffsets", ev.outHitOffsets); F("out.len", ev.outLen);
+  F("out.seedIdx", ev.outSeedIdx); F("out.trackCandidateType", ev.outType);
   return 0;
 }
 

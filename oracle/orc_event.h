@@ -83,6 +83,33 @@ struct Event {
   std::vector<int8_t> t5IsDup;
   std::vector<float> t5InnerRadius, t5BridgeRadius, t5OuterRadius, t5Pt, t5Eta, t5Phi, t5ScoreRphiSum,
       t5RegressionRadius, t5RegressionG, t5RegressionF, t5RzChiSquared, t5ChiSquared, t5NonAnchorChiSquared;
+  // pixel quintuplets (SDL/PixelTriplet.h:1871-1913), capacity N_MAX_PIXEL_QUINTUPLETS
+  unsigned nPixelQuintuplets = 0, totOccupancyPixelQuintuplets = 0;
+  std::vector<unsigned> pt5PixelIndices, pt5T5Indices, pt5HitIndices;
+  std::vector<uint8_t> pt5IsDup, pt5LogicalLayers;
+  std::vector<uint16_t> pt5LowerModuleIndices;
+  std::vector<float> pt5Score, pt5Eta, pt5Phi, pt5PixelRadius, pt5QuintupletRadius, pt5CenterX, pt5CenterY,
+      pt5RzChiSquared, pt5RPhiChiSquared, pt5RPhiChiSquaredInwards;
+  // pixel triplets (SDL/PixelTriplet.h:20-75), capacity N_MAX_PIXEL_TRIPLETS
+  unsigned nPixelTriplets = 0, totOccupancyPixelTriplets = 0;
+  std::vector<unsigned> pt3PixelSegmentIndices, pt3TripletIndices, pt3HitIndices;
+  std::vector<uint8_t> pt3IsDup, pt3LogicalLayers;
+  std::vector<uint16_t> pt3LowerModuleIndices;
+  std::vector<float> pt3PixelRadius, pt3TripletRadius, pt3Pt, pt3Eta, pt3Phi, pt3EtaPix, pt3PhiPix, pt3Score,
+      pt3CenterX, pt3CenterY, pt3RzChiSquared, pt3RPhiChiSquared, pt3RPhiChiSquaredInwards;
+  // track candidates (SDL/TrackCandidate.h:20-111), capacity 31000
+  unsigned nTrackCandidates = 0, nTrackCandidatespT3 = 0, nTrackCandidatespT5 = 0, nTrackCandidatespLS = 0,
+           nTrackCandidatesT5 = 0;
+  std::vector<int16_t> tcType;
+  std::vector<unsigned> tcDirectObjectIndices, tcObjectIndices, tcHitIndices;
+  std::vector<int> tcPixelSeedIndex;
+  std::vector<uint8_t> tcLogicalLayers;
+  std::vector<uint16_t> tcLowerModuleIndices;
+  std::vector<float> tcCenterX, tcCenterY, tcRadius;
+  // LST::getOutput (SDL/LST.cc:366-420): ragged hit lists, flattened
+  std::vector<unsigned> outHitIdxs, outHitOffsets, outLen;
+  std::vector<int> outSeedIdx;
+  std::vector<int16_t> outType;
 };
 
 bool event_add_hits(Event& ev, const Input& in);
@@ -91,6 +118,13 @@ void event_create_mini_doublets(Event& ev);
 void event_create_segments(Event& ev);
 void event_create_triplets(Event& ev);
 void event_create_quintuplets(Event& ev);
+void event_pixel_segment_cleaning(Event& ev);
+void event_create_pixel_quintuplets(Event& ev);
+void event_create_pixel_triplets(Event& ev);
+void event_create_track_candidates(Event& ev);
+void event_get_output(Event& ev);
+void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, float& pt_beta, float sdIn_dr,
+                              float sdOut_dr, float dr, float lIn);
 float computeRadiusFromThreeAnchorHits(float x1, float y1, float x2, float y2, float x3, float y3, float& g, float& f);
 
 }  // namespace orc

@@ -435,6 +435,18 @@ float computeChiSquared(unsigned nPoints, const float* xs, const float* ys, cons
 
 }  // namespace
 
+// The pixel-triplet and pixel-quintuplet r-phi chi-squared use the same
+// per-point uncertainties and chi-squared sum (SDL/PixelTriplet.h:371-417,
+// 419-487, 2219-2350), with 3 or 5 points.
+void sigmasForRegression(const Geometry& G, const uint16_t* mods, float* delta1, float* delta2, float* slopes,
+                         bool* isFlat, unsigned nPoints, bool anchorHits) {
+  computeSigmasForRegression(G, mods, delta1, delta2, slopes, isFlat, nPoints, anchorHits);
+}
+float chiSquaredOfCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
+                         const float* slopes, const bool* isFlat, float g, float f, float radius) {
+  return computeChiSquared(nPoints, xs, ys, delta1, delta2, slopes, isFlat, g, f, radius);
+}
+
 // SDL/Quintuplet.h:1370-1464
 void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, float& pt_beta, float sdIn_dr,
                               float sdOut_dr, float dr, float lIn) {

@@ -12,4 +12,8 @@ struct T5Out {
 bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned innerT3, unsigned outerT3, T5Out& o);
 void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, float& pt_beta, float sdIn_dr,
                               float sdOut_dr, float dr, float lIn);
+void sigmasForRegression(const Geometry& G, const uint16_t* mods, float* delta1, float* delta2, float* slopes,
+                         bool* isFlat, unsigned nPoints, bool anchorHits);
+float chiSquaredOfCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
+                         const float* slopes, const bool* isFlat, float g, float f, float radius);
 }  // namespace orc
