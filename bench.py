@@ -20,10 +20,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC set"
-STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets"]
-KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets", "k_quintuplets"]
-ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5"]
-LAST_STAGE = 5  # quintuplets: the stages built so far
+STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets",
+               "pixel_segment_cleaning", "pixel_quintuplets", "pixel_triplets", "track_candidates"]
+# the kernel each stage timer brackets
+KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets", "k_quintuplets",
+                "k_pls_dup", "k_pixel_quintuplets", "k_pixel_triplets", "k_t5_dup_before_tc"]
+ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
+LAST_STAGE = 9  # the full chain up to track candidates
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
@@ -53,14 +56,23 @@ def algorithmic_bytes(counts: dict) -> dict:
       LS builder   : read 44 B of each mini-doublet, write 44 B/segment
       T3 builder   : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
       T5 builder   : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
+      pLS cleaning : read eta, phi, score, isQuad and 4 hit indices (29 B/pLS), write 1 B/pLS
+      pT5 builder  : read 60 B/pLS and 70 B/quintuplet, write 140 B/pixel quintuplet
+      pT3 builder  : read 60 B/pLS and 53 B/triplet, write 120 B/pixel triplet
+      T5 cleaning  : read eta, phi, score, flags and 10 hit indices (54 B/quintuplet), write 1 B/quintuplet
     """
     h, md, ls, t3, t5 = counts["hits"], counts["mds"], counts["segs"], counts["t3"], counts["t5"]
+    pls, pt5, pt3 = counts["pls"], counts["pt5"], counts["pt3"]
     return {
         "k_hits": 46 * h,
         "k_mini_doublets": 16 * h + 134 * md,
         "k_segments": 44 * md + 44 * ls,
         "k_triplets": 22 * ls + 20 * md + 60 * t3,
         "k_quintuplets": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
+        "k_pls_dup": 30 * pls,
+        "k_pixel_quintuplets": 60 * pls + 70 * t5 + 140 * pt5,
+        "k_pixel_triplets": 60 * pls + 53 * t3 + 120 * pt3,
+        "k_t5_dup_before_tc": 55 * t5,
     }
 
 
@@ -158,7 +170,11 @@ def main():
                        "mds": int(ev.field("mds.nMDs").sum()),
                        "segs": int(ev.field("segs.nSegments").sum()),
                        "t3": int(ev.field("t3.nTriplets").sum()),
-                       "t5": int(ev.field("t5.nQuintuplets").sum())})
+                       "t5": int(ev.field("t5.nQuintuplets").sum()),
+                       "pls": int(lst.lib().lst_event_count(ev.h, b"pixelSegments")),
+                       "pt5": int(lst.lib().lst_event_count(ev.h, b"pixelQuintuplets")),
+                       "pt3": int(lst.lib().lst_event_count(ev.h, b"pixelTriplets")),
+                       "tc": int(lst.lib().lst_event_count(ev.h, b"trackCandidates"))})
     lst._check(L.lst_event_set_timing(ev.h, 0))
     avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
     per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}
@@ -187,7 +203,8 @@ def main():
             "geometry": "OT800_IT615_pt0.8",
             "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],
             "mean_segments": mean_counts["segs"], "mean_triplets": mean_counts["t3"],
-            "mean_quintuplets": mean_counts["t5"],
+            "mean_quintuplets": mean_counts["t5"], "mean_pixel_quintuplets": mean_counts["pt5"],
+            "mean_pixel_triplets": mean_counts["pt3"], "mean_track_candidates": mean_counts["tc"],
             "gpu_ms_per_step": gpu_ms / args.steps,
             "kernel_ms": per_kernel,
         },

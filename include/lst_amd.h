@@ -87,6 +87,26 @@ lst_status lst_event_create_triplets(lst_event* ev);             /* createTriple
  * after-build duplicate flags.  Needs an event created with
  * LST_T5_PLAIN_CHI2; LST_E_INPUT otherwise. */
 lst_status lst_event_create_quintuplets(lst_event* ev);
+/* pixelLineSegmentCleaning (SDL/Event.cc:993-1007): first checkHitspLS pass. */
+lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev);
+/* createPixelQuintuplets (SDL/Event.cc:1009-1146): pT5 builder, pT5 duplicate
+ * flags, pT5 track candidates. */
+lst_status lst_event_create_pixel_quintuplets(lst_event* ev);
+/* createPixelTriplets (SDL/Event.cc:784-901): pT3 builder and duplicate flags. */
+lst_status lst_event_create_pixel_triplets(lst_event* ev);
+/* createTrackCandidates (SDL/Event.cc:620-782): cross cleaning, pT3/T5/pLS
+ * track candidates, second checkHitspLS pass. */
+lst_status lst_event_create_track_candidates(lst_event* ev);
+
+/* Replaces SDL::LST<Acc>::getOutput / getHitIdxs (SDL/LST.cc:366-420) after
+ * createTrackCandidates: *n_tc receives the number of track candidates; when
+ * any array is given (each may be NULL) and capacity >= *n_tc, hit_idxs
+ * receives 14 entries per candidate (the first len[i] are the candidate's
+ * ntuple hit indices, the reference's out_tc_hitIdxs_), len the list lengths,
+ * seed_idx the pixel seed index (-1 for T5 candidates) and type the candidate
+ * type (4 T5, 5 pT3, 7 pT5, 8 pLS).  LST_E_CAPACITY if capacity is too small. */
+lst_status lst_event_get_output(lst_event* ev, uint32_t* n_tc, uint32_t* hit_idxs, uint32_t* len, int32_t* seed_idx,
+                                int16_t* type, uint32_t capacity);
 
 /* Replaces SDL::Event::resetEvent (SDL/Event.cc:45-149); keeps the arenas. */
 lst_status lst_event_reset(lst_event* ev);
@@ -94,7 +114,9 @@ lst_status lst_event_reset(lst_event* ev);
 lst_status lst_event_synchronize(lst_event* ev);
 
 /* Replaces SDL::Event::getNumberOf* (SDL/Event.h:152-183): "hits", "pixelSegments",
- * "miniDoublets", "segments", "triplets" (outer-tracker totals). */
+ * "miniDoublets", "segments", "triplets", "quintuplets" (outer-tracker totals),
+ * "pixelQuintuplets", "pixelTriplets", "trackCandidates", "pT5TrackCandidates",
+ * "pT3TrackCandidates", "T5TrackCandidates", "pLSTrackCandidates". */
 uint32_t lst_event_count(lst_event* ev, const char* what);
 
 /* Replaces the host-copy getters SDL::Event::get{Hits,MiniDoublets,Segments,
@@ -122,7 +144,8 @@ typedef struct {
 
 /* Replaces SDL::LST<Acc>::run (SDL/LST.cc:12-161) up to and including
  * `last_stage` (0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,
- * 4 triplets, 5 quintuplets): prepareInput on the host (SDL/LST.cc:175-365), then the stage
+ * 4 triplets, 5 quintuplets, 6 pLS cleaning, 7 pixel quintuplets, 8 pixel
+ * triplets, 9 track candidates): prepareInput on the host (SDL/LST.cc:175-365), then the stage
  * sequence on the device. */
 lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage);
 
@@ -137,8 +160,9 @@ lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 /* Per-kernel device timing of the builder kernels (hipEvents on the event's
  * stream).  After a run with timing on, lst_event_kernel_times writes the
  * durations in ms of: 0 hits, 1 pixel segments, 2 mini-doublets,
- * 3 segments, 4 triplets, 5 quintuplets (n <= 16 entries; synchronises
- * the stream). */
+ * 3 segments, 4 triplets, 5 quintuplets, 6 pLS cleaning, 7 pixel
+ * quintuplets, 8 pixel triplets, 9 T5 duplicate removal before the track
+ * candidates (n <= 16 entries; synchronises the stream). */
 lst_status lst_event_set_timing(lst_event* ev, int on);
 lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n);
 /* The hipStream_t the event runs on. */

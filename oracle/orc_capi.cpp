@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see orc_common.h).
 // C entry points used by oracle/orc.py (ctypes).  Not part of the product.
 #include "orc_prepare.h"
+#include "orc_t5.h"
 #include <cstring>
 #include <string>
 
@@ -267,5 +268,19 @@ float orc_eta(float x, float y, float z) { return orc::eta(x, y, z); }
 int orc_binary_search(const unsigned* data, unsigned v, unsigned n) { return orc::binary_search(data, v, n); }
 float orc_circle_radius(float x1, float y1, float x2, float y2, float x3, float y3, float* g, float* f) {
   return orc::computeRadiusFromThreeAnchorHits(x1, y1, x2, y2, x3, y3, *g, *f);
+}
+// weighted algebraic circle fit of the T5 (SDL/Quintuplet.h:1223-1320) and the
+// chi-squared of points against a circle (:1322-1367), n <= 16
+float orc_regression_circle(unsigned n, const float* xs, const float* ys, const float* d1, const float* d2,
+                            const float* slopes, const unsigned char* flat, float* g, float* f, float* chi2) {
+  bool fl[16];
+  for (unsigned i = 0; i < n && i < 16; i++) fl[i] = flat[i] != 0;
+  return orc::regressionCircle(n, xs, ys, d1, d2, slopes, fl, *g, *f, *chi2);
+}
+float orc_circle_chi2(unsigned n, const float* xs, const float* ys, const float* d1, const float* d2,
+                      const float* slopes, const unsigned char* flat, float g, float f, float r) {
+  bool fl[16];
+  for (unsigned i = 0; i < n && i < 16; i++) fl[i] = flat[i] != 0;
+  return orc::chiSquaredOfCircle(n, xs, ys, d1, d2, slopes, fl, g, f, r);
 }
 }

@@ -442,6 +442,11 @@ void sigmasForRegression(const Geometry& G, const uint16_t* mods, float* delta1,
                          bool* isFlat, unsigned nPoints, bool anchorHits) {
   computeSigmasForRegression(G, mods, delta1, delta2, slopes, isFlat, nPoints, anchorHits);
 }
+float regressionCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
+                       const float* slopes, const bool* isFlat, float& g, float& f, float& chiSquared) {
+  float sigmas[16];
+  return computeRadiusUsingRegression(nPoints, xs, ys, delta1, delta2, slopes, isFlat, g, f, sigmas, chiSquared);
+}
 float chiSquaredOfCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
                          const float* slopes, const bool* isFlat, float g, float f, float radius) {
   return computeChiSquared(nPoints, xs, ys, delta1, delta2, slopes, isFlat, g, f, radius);

@@ -14,6 +14,8 @@ void runDeltaBetaIterationsT5(float& betaIn, float& betaOut, float& betaAv, floa
                               float sdOut_dr, float dr, float lIn);
 void sigmasForRegression(const Geometry& G, const uint16_t* mods, float* delta1, float* delta2, float* slopes,
                          bool* isFlat, unsigned nPoints, bool anchorHits);
+float regressionCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
+                       const float* slopes, const bool* isFlat, float& g, float& f, float& chiSquared);
 float chiSquaredOfCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
                          const float* slopes, const bool* isFlat, float g, float f, float radius);
 }  // namespace orc

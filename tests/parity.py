@@ -30,7 +30,21 @@ T5_FIELDS = {"tripletIndices": 2, "lowerModuleIndices": 5, "logicalLayers": 5, "
              "isDup": 1, "TightCutFlag": 1, "partOfPT5": 1, "innerRadius": 1, "bridgeRadius": 1, "outerRadius": 1,
              "pt": 1, "eta": 1, "phi": 1, "score_rphisum": 1, "regressionRadius": 1, "regressionG": 1,
              "regressionF": 1, "rzChiSquared": 1, "chiSquared": 1, "nonAnchorChiSquared": 1}
-STAGE_ORDER = ["hits", "pls", "md", "ls", "t3", "t5"]
+PT5_FIELDS = {"pixelIndices": 1, "T5Indices": 1, "isDup": 1, "score": 1, "eta": 1, "phi": 1, "logicalLayers": 7,
+              "hitIndices": 14, "lowerModuleIndices": 7, "pixelRadius": 1, "quintupletRadius": 1, "centerX": 1,
+              "centerY": 1, "rzChiSquared": 1, "rPhiChiSquared": 1, "rPhiChiSquaredInwards": 1}
+PT3_FIELDS = {"pixelSegmentIndices": 1, "tripletIndices": 1, "pixelRadius": 1, "tripletRadius": 1, "pt": 1, "eta": 1,
+              "phi": 1, "eta_pix": 1, "phi_pix": 1, "score": 1, "isDup": 1, "logicalLayers": 5, "hitIndices": 10,
+              "lowerModuleIndices": 5, "centerX": 1, "centerY": 1, "rzChiSquared": 1, "rPhiChiSquared": 1,
+              "rPhiChiSquaredInwards": 1}
+TC_FIELDS = {"trackCandidateType": 1, "directObjectIndices": 1, "objectIndices": 2, "logicalLayers": 7,
+             "hitIndices": 14, "pixelSeedIndex": 1, "lowerModuleIndices": 7}
+TC_FLOAT_FIELDS = ["centerX", "centerY", "radius"]  # not written by the reference for pLS candidates
+# device scalar slots (tracklooper_amd/csrc/soa.h, enum Scalar)
+SCALAR_SLOTS = {"nPixelQuintuplets": 4, "totOccupancyPixelQuintuplets": 5, "nPixelTriplets": 6,
+                "totOccupancyPixelTriplets": 7, "nTrackCandidates": 8, "nTrackCandidatespT3": 9,
+                "nTrackCandidatespT5": 10, "nTrackCandidatespLS": 11, "nTrackCandidatesT5": 12}
+STAGE_ORDER = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
 DEEPEST = STAGE_ORDER[-1]  # the last stage the engine implements
 
 
@@ -141,6 +155,53 @@ def compare(prod, orc, upto: str, report: Report | None = None, n_lower: int = 1
                 orc.field("ranges.indicesOfEligibleT5Modules")[:n_elig])
         start = np.maximum(orc.field("ranges.quintupletModuleIndices"), 0)
         for f, m in T5_FIELDS.items():
+            if f in ("isDup", "partOfPT5") and last > 5:
+                continue  # later stages update these; compared below
             sl = slots(start, n, m)
             r.check("t5." + f, prod.field("t5." + f)[sl], orc.field("t5." + f)[sl])
+    if last >= 6:
+        r.check("segs.isDup(pass 1)" if last == 6 else "segs.isDup", prod.field("segs.isDup")[:n_pls],
+                orc.field("segs.isDup")[:n_pls])
+    scal = prod.field("scalars") if last >= 7 else None
+    if last >= 7:
+        for k in ("nPixelQuintuplets", "totOccupancyPixelQuintuplets"):
+            r.check(k, scal[SCALAR_SLOTS[k]:SCALAR_SLOTS[k] + 1], np.array([orc.scalar(k)], np.uint32))
+        n5 = orc.scalar("nPixelQuintuplets")
+        for f, m in PT5_FIELDS.items():
+            r.check("pt5." + f, prod.field("pt5." + f)[:n5 * m], orc.field("pt5." + f)[:n5 * m])
+        t3sl = slots(orc.field("ranges.tripletModuleIndices"), orc.field("t3.nTriplets"))
+        r.check("t3.partOfPT5", prod.field("t3.partOfPT5")[t3sl], orc.field("t3.partOfPT5")[t3sl])
+        t5sl = slots(np.maximum(orc.field("ranges.quintupletModuleIndices"), 0), orc.field("t5.nQuintuplets"))
+        r.check("t5.partOfPT5", prod.field("t5.partOfPT5")[t5sl], orc.field("t5.partOfPT5")[t5sl])
+        r.check("segs.partOfPT5", prod.field("segs.partOfPT5")[:n_pls], orc.field("segs.partOfPT5")[:n_pls])
+    if last >= 8:
+        for k in ("nPixelTriplets", "totOccupancyPixelTriplets"):
+            r.check(k, scal[SCALAR_SLOTS[k]:SCALAR_SLOTS[k] + 1], np.array([orc.scalar(k)], np.uint32))
+        n3 = orc.scalar("nPixelTriplets")
+        for f, m in PT3_FIELDS.items():
+            r.check("pt3." + f, prod.field("pt3." + f)[:n3 * m], orc.field("pt3." + f)[:n3 * m])
+        r.check("t3.partOfPT3", prod.field("t3.partOfPT3")[t3sl], orc.field("t3.partOfPT3")[t3sl])
+    if last >= 7:
+        k_last = "nTrackCandidates"
+        ntc = orc.scalar(k_last)
+        for k in ("nTrackCandidates", "nTrackCandidatespT5", "nTrackCandidatespT3", "nTrackCandidatesT5",
+                  "nTrackCandidatespLS"):
+            r.check(k, scal[SCALAR_SLOTS[k]:SCALAR_SLOTS[k] + 1], np.array([orc.scalar(k)], np.uint32))
+        for f, m in TC_FIELDS.items():
+            r.check("tc." + f, prod.field("tc." + f)[:ntc * m], orc.field("tc." + f)[:ntc * m])
+        not_pls = np.nonzero(orc.field("tc.trackCandidateType")[:ntc] != 8)[0]
+        for f in TC_FLOAT_FIELDS:
+            r.check("tc." + f, prod.field("tc." + f)[not_pls], orc.field("tc." + f)[not_pls])
+    if last >= 9:
+        r.check("t5.isDup", prod.field("t5.isDup")[t5sl], orc.field("t5.isDup")[t5sl])
+        r.check("pt3.isDup(final)", prod.field("pt3.isDup")[:n3], orc.field("pt3.isDup")[:n3])
+        # LST::getOutput lists
+        ntc = orc.scalar("nTrackCandidates")
+        olen = orc.field("out.len")[:ntc]
+        r.check("out.len", prod.field("out.len")[:ntc], olen)
+        off = orc.field("out.hitOffsets")
+        flat = orc.field("out.hitIdxs")
+        ph = prod.field("out.hitIdxs")[:ntc * 14].reshape(-1, 14) if ntc else np.zeros((0, 14), np.uint32)
+        mask = np.arange(14)[None, :] < olen[:, None]
+        r.check("out.hitIdxs", ph[mask], flat[:off[ntc]] if ntc else flat[:0])
     return r

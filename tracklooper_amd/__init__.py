@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "liblst_amd.so")
 DEFAULT_GEOMETRY = os.path.join(HERE, "data", "OT800_IT615_pt0.8")
 
 _DT = {0: np.float32, 1: np.uint32, 2: np.int32, 3: np.uint16, 4: np.int16, 5: np.uint8, 6: np.int8}
-STAGES = {"hits": 0, "pls": 1, "md": 2, "ls": 3, "t3": 4, "t5": 5}
+STAGES = {"hits": 0, "pls": 1, "md": 2, "ls": 3, "t3": 4, "t5": 5, "plsclean": 6, "pt5": 7, "pt3": 8, "tc": 9}
 LST_T5_PLAIN_CHI2, LST_T5_DNN = 0, 1
 _lib = None
 
@@ -65,13 +65,15 @@ def lib():
         L.lst_event_destroy.argtypes = [P]
         L.lst_event_add_hits.argtypes = [P, P, P, P, P, P, C.c_uint32, C.c_int]
         L.lst_event_add_pixel_segments.argtypes = [P, C.POINTER(_PixelSegments), C.c_int]
-        for n in ("create_mini_doublets", "create_segments", "create_triplets", "create_quintuplets", "reset",
-                  "synchronize"):
+        for n in ("create_mini_doublets", "create_segments", "create_triplets", "create_quintuplets",
+                  "pixel_line_segment_cleaning", "create_pixel_quintuplets", "create_pixel_triplets",
+                  "create_track_candidates", "reset", "synchronize"):
             getattr(L, "lst_event_" + n).argtypes = [P]
         L.lst_event_count.argtypes = [P, C.c_char_p]
         L.lst_event_count.restype = C.c_uint32
         L.lst_event_get_field.argtypes = [P, C.c_char_p, P, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
         L.lst_run.argtypes = [P, C.POINTER(_Input), C.c_int]
+        L.lst_event_get_output.argtypes = [P, C.POINTER(C.c_uint32), P, P, P, P, C.c_uint32]
         _lib = L
     return _lib
 
@@ -148,6 +150,33 @@ class Event:
     def createQuintuplets(self):
         _check(lib().lst_event_create_quintuplets(self.h))
 
+    def pixelLineSegmentCleaning(self):
+        _check(lib().lst_event_pixel_line_segment_cleaning(self.h))
+
+    def createPixelQuintuplets(self):
+        _check(lib().lst_event_create_pixel_quintuplets(self.h))
+
+    def createPixelTriplets(self):
+        _check(lib().lst_event_create_pixel_triplets(self.h))
+
+    def createTrackCandidates(self):
+        _check(lib().lst_event_create_track_candidates(self.h))
+
+    def getOutput(self):
+        """LST::getOutput (SDL/LST.cc:366-391): per track candidate its ntuple hit
+        indices, their number, the pixel seed index and the candidate type."""
+        n = C.c_uint32(0)
+        _check(lib().lst_event_get_output(self.h, C.byref(n), None, None, None, None, 0))
+        k = n.value
+        hits = np.zeros((k, 14), np.uint32)
+        length = np.zeros(k, np.uint32)
+        seed = np.zeros(k, np.int32)
+        typ = np.zeros(k, np.int16)
+        if k:
+            _check(lib().lst_event_get_output(self.h, C.byref(n), _p(hits), _p(length), _p(seed), _p(typ), k))
+        return {"tc_hitIdxs": [list(hits[i, :length[i]]) for i in range(k)], "tc_len": length,
+                "tc_seedIdx": seed, "tc_trackCandidateType": typ}
+
     def resetEvent(self):
         _check(lib().lst_event_reset(self.h))
 
@@ -165,6 +194,9 @@ class Event:
 
     def getNumberOfQuintuplets(self):
         return lib().lst_event_count(self.h, b"quintuplets")
+
+    def getNumberOfTrackCandidates(self):
+        return lib().lst_event_count(self.h, b"trackCandidates")
 
     def getNumberOfPixelSegments(self):
         return lib().lst_event_count(self.h, b"pixelSegments")

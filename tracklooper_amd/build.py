@@ -24,6 +24,7 @@ SOURCES = [
     "prepare.cpp",
     "stages_hits_md_ls_t3.hip",
     "stages_t5.hip",
+    "stages_pixel.hip",
     "engine.hip",
 ]
 

@@ -83,6 +83,8 @@ struct lst_event {
   lst::EventView v{};
   // capacities
   uint32_t hitCap = 0, mdCap = 0, segCap = 0, plsCap = lst::kNMaxPixelSegments, t3Cap = 0, t5Cap = 0;
+  static constexpr uint32_t pt5Cap = lst::kNMaxPixelQuintuplets, pt3Cap = lst::kNMaxPixelTriplets,
+                            tcCap = lst::kNMaxPixelTrackCandidates + lst::kNMaxNonPixelTrackCandidates;
   char* arena = nullptr;
   size_t arenaBytes = 0;
   uint32_t* plsHitIdx[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -154,6 +156,19 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #define CARVE(TY, NM, MU) c.take(e.Q.NM, (size_t)(t5Cap ? t5Cap : 1) * MU);
   LST_T5_FIELDS(CARVE)
 #undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.P5.NM, (size_t)pt5Cap * MU);
+  LST_PT5_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.P3.NM, (size_t)pt3Cap * MU);
+  LST_PT3_FIELDS(CARVE)
+#undef CARVE
+#define CARVE(TY, NM, MU) c.take(e.C.NM, (size_t)tcCap * MU);
+  LST_TC_FIELDS(CARVE)
+#undef CARVE
+  c.take(e.W.connBase, plsCap); c.take(e.W.connSize, plsCap);
+  c.take(e.W.pt5Count, plsCap + 1); c.take(e.W.pt3Count, plsCap + 1);
+  c.take(e.W.t5TcCount, nL + 1);
+  c.take(e.W.outHits, (size_t)tcCap * 14); c.take(e.W.outLen, tcCap);
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);
   c.take(e.R.mdRanges, 2 * nMod); c.take(e.R.segmentRanges, 2 * nMod); c.take(e.R.tripletRanges, 2 * nMod);
@@ -179,6 +194,11 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   e.t3OccIfAny = g.t3OccIfAny;
   e.t5OccIfAny = g.t5OccIfAny;
   e.nT5SeedModules = g.nT5SeedModules;
+  for (int k = 0; k < 3; k++) {
+    e.pixIndex[k] = g.pixIndex[k];
+    e.pixSizes[k] = g.pixSizes[k];
+  }
+  e.tcPlsTriplets = 0;
   e.pixelModule = g.pixelModuleIndex;
   e.ptCut = es->ptCut;
 
@@ -226,6 +246,24 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     auto* cur = &e.Q;
     LST_T5_FIELDS(FIELD)
   }
+  {
+    std::string prefix = "pt5.";
+    size_t capacity = pt5Cap;
+    auto* cur = &e.P5;
+    LST_PT5_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "pt3.";
+    size_t capacity = pt3Cap;
+    auto* cur = &e.P3;
+    LST_PT3_FIELDS(FIELD)
+  }
+  {
+    std::string prefix = "tc.";
+    size_t capacity = tcCap;
+    auto* cur = &e.C;
+    LST_TC_FIELDS(FIELD)
+  }
 #undef FIELD
 #undef REG
   add("hits.hitRanges", e.R.hitRanges, 2 * nMod);
@@ -255,6 +293,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   add("t5.nQuintuplets", e.R.nQuintuplets, nL);
   add("t5.totOccupancyQuintuplets", e.R.totOccupancyQuintuplets, nL);
   add("scalars", e.R.scalars, lst::kNScalars);
+  add("out.hitIdxs", e.W.outHits, (size_t)tcCap * 14);
+  add("out.len", e.W.outLen, tcCap);
   return LST_OK;
 }
 
@@ -338,6 +378,11 @@ lst_status lst_es_create(const char* geom_dir, float pt_cut, uint32_t stride, ls
   if (!lst::load_host_geometry(geom_dir ? geom_dir : "", stride, es->host, err))
     return fail(err.rfind("cannot read", 0) == 0 ? LST_E_IO : LST_E_FORMAT, err);
   if (es->host.stride > 159) return fail(LST_E_FORMAT, "module fan-out above 159 is not supported");
+  for (int k = 0; k < 3; k++)
+    for (uint32_t sz : es->host.pixSizes[k])
+      if (sz > 64) return fail(LST_E_FORMAT, "a pixel superbin connects to more than 64 modules");
+  for (uint32_t m : es->host.connectedPixels)
+    if (m >= es->host.nLowerModules) return fail(LST_E_FORMAT, "pixel map names a module that is not a lower module");
   if (!lst::upload_geometry(es->host, es->dev, err)) return fail(LST_E_DEVICE, err);
   es->ptCut = pt_cut;
   *out = es.release();
@@ -437,6 +482,59 @@ lst_status lst_event_create_quintuplets(lst_event* ev) {
   return LST_OK;
 }
 
+lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev) {
+  if (!ev || ev->stage != 5) return fail(LST_E_ORDER, "pixelLineSegmentCleaning needs quintuplets");
+  lst::launch_pixel_segment_cleaning(ev->v, ev->stream, ev->timer(6));
+  HIP_OK(hipGetLastError());
+  ev->stage = 6;
+  return LST_OK;
+}
+
+lst_status lst_event_create_pixel_quintuplets(lst_event* ev) {
+  if (!ev || ev->stage != 6) return fail(LST_E_ORDER, "createPixelQuintuplets needs pixelLineSegmentCleaning");
+  lst::launch_pixel_quintuplets(ev->v, ev->stream, ev->timer(7));
+  HIP_OK(hipGetLastError());
+  ev->stage = 7;
+  return LST_OK;
+}
+
+lst_status lst_event_create_pixel_triplets(lst_event* ev) {
+  if (!ev || ev->stage != 7) return fail(LST_E_ORDER, "createPixelTriplets needs pixel quintuplets");
+  lst::launch_pixel_triplets(ev->v, ev->stream, ev->timer(8));
+  HIP_OK(hipGetLastError());
+  ev->stage = 8;
+  return LST_OK;
+}
+
+lst_status lst_event_create_track_candidates(lst_event* ev) {
+  if (!ev || ev->stage != 8) return fail(LST_E_ORDER, "createTrackCandidates needs pixel triplets");
+  lst::launch_track_candidates(ev->v, ev->stream, ev->timer(9));
+  HIP_OK(hipGetLastError());
+  ev->stage = 9;
+  return LST_OK;
+}
+
+lst_status lst_event_get_output(lst_event* ev, uint32_t* n_tc, uint32_t* hit_idxs, uint32_t* len, int32_t* seed_idx,
+                                int16_t* type, uint32_t capacity) {
+  if (!ev || !n_tc) return fail(LST_E_INPUT, "null argument");
+  if (ev->stage != 9) return fail(LST_E_ORDER, "getOutput needs track candidates");
+  uint32_t scal[lst::kNScalars];
+  HIP_OK(hipMemcpyAsync(scal, ev->v.R.scalars, sizeof(scal), hipMemcpyDeviceToHost, ev->stream));
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  const uint32_t n = scal[lst::kNTrackCandidates];
+  *n_tc = n;
+  if (!hit_idxs && !len && !seed_idx && !type) return LST_OK;
+  if (capacity < n) return fail(LST_E_CAPACITY, "output arrays hold fewer track candidates than were built");
+  if (n) {
+    if (hit_idxs) HIP_OK(hipMemcpyAsync(hit_idxs, ev->v.W.outHits, (size_t)n * 14 * 4, hipMemcpyDeviceToHost, ev->stream));
+    if (len) HIP_OK(hipMemcpyAsync(len, ev->v.W.outLen, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
+    if (seed_idx) HIP_OK(hipMemcpyAsync(seed_idx, ev->v.C.pixelSeedIndex, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
+    if (type) HIP_OK(hipMemcpyAsync(type, ev->v.C.trackCandidateType, (size_t)n * 2, hipMemcpyDeviceToHost, ev->stream));
+    HIP_OK(hipStreamSynchronize(ev->stream));
+  }
+  return LST_OK;
+}
+
 lst_status lst_event_reset(lst_event* ev) {
   if (!ev) return fail(LST_E_INPUT, "null event");
   ev->stage = -1;
@@ -462,6 +560,20 @@ uint32_t lst_event_count(lst_event* ev, const char* what) {
   else if (!std::strcmp(what, "segments")) key = "segs.nSegments";
   else if (!std::strcmp(what, "triplets")) key = "t3.nTriplets";
   else if (!std::strcmp(what, "quintuplets")) key = "t5.nQuintuplets";
+  else {
+    static const struct { const char* name; int idx; } sc[] = {
+        {"pixelQuintuplets", lst::kNPixelQuintuplets}, {"pixelTriplets", lst::kNPixelTriplets},
+        {"trackCandidates", lst::kNTrackCandidates}, {"pT3TrackCandidates", lst::kNTrackCandidatespT3},
+        {"pT5TrackCandidates", lst::kNTrackCandidatespT5}, {"pLSTrackCandidates", lst::kNTrackCandidatespLS},
+        {"T5TrackCandidates", lst::kNTrackCandidatesT5}};
+    for (const auto& e : sc)
+      if (!std::strcmp(what, e.name)) {
+        uint32_t val = 0;
+        if (hipMemcpyAsync(&val, ev->v.R.scalars + e.idx, 4, hipMemcpyDeviceToHost, ev->stream) != hipSuccess) return 0;
+        if (hipStreamSynchronize(ev->stream) != hipSuccess) return 0;
+        return val;
+      }
+  }
   if (!key) return 0;
   const FieldRef* f = ev->field(key);
   std::vector<uint32_t> h(n);
@@ -500,7 +612,11 @@ lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
   if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
   if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
   if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_quintuplets(ev)) != LST_OK) return s;
+  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s == LST_OK ? warn : s;
+  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_track_candidates(ev)) != LST_OK) return s;
   return warn;
 }
 
@@ -557,7 +673,11 @@ lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
   if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
   if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
   if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s;
-  return lst_event_create_quintuplets(ev);
+  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s;
+  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s;
+  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s;
+  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s;
+  return lst_event_create_track_candidates(ev);
 }
 
 lst_status lst_event_set_timing(lst_event* ev, int on) {

@@ -51,6 +51,29 @@
   X(float, regressionG, 1) X(float, regressionF, 1) X(float, rzChiSquared, 1) X(float, chiSquared, 1)      \
   X(float, nonAnchorChiSquared, 1)
 
+// pixel quintuplets (SDL/PixelTriplet.h:1871-1913)
+#define LST_PT5_FIELDS(X)                                                                                  \
+  X(uint32_t, pixelIndices, 1) X(uint32_t, T5Indices, 1) X(uint8_t, isDup, 1) X(float, score, 1)           \
+  X(float, eta, 1) X(float, phi, 1) X(uint8_t, logicalLayers, 7) X(uint32_t, hitIndices, 14)              \
+  X(uint16_t, lowerModuleIndices, 7) X(float, pixelRadius, 1) X(float, quintupletRadius, 1)                \
+  X(float, centerX, 1) X(float, centerY, 1) X(float, rzChiSquared, 1) X(float, rPhiChiSquared, 1)          \
+  X(float, rPhiChiSquaredInwards, 1)
+
+// pixel triplets (SDL/PixelTriplet.h:20-75; the never-written pixelRadiusError
+// and partOfPT5 buffers are not carried)
+#define LST_PT3_FIELDS(X)                                                                                  \
+  X(uint32_t, pixelSegmentIndices, 1) X(uint32_t, tripletIndices, 1) X(float, pixelRadius, 1)              \
+  X(float, tripletRadius, 1) X(float, pt, 1) X(float, eta, 1) X(float, phi, 1) X(float, eta_pix, 1)        \
+  X(float, phi_pix, 1) X(float, score, 1) X(uint8_t, isDup, 1) X(uint8_t, logicalLayers, 5)                \
+  X(uint32_t, hitIndices, 10) X(uint16_t, lowerModuleIndices, 5) X(float, centerX, 1) X(float, centerY, 1) \
+  X(float, rzChiSquared, 1) X(float, rPhiChiSquared, 1) X(float, rPhiChiSquaredInwards, 1)
+
+// track candidates (SDL/TrackCandidate.h:20-111)
+#define LST_TC_FIELDS(X)                                                                                   \
+  X(int16_t, trackCandidateType, 1) X(uint32_t, directObjectIndices, 1) X(uint32_t, objectIndices, 2)      \
+  X(uint8_t, logicalLayers, 7) X(uint32_t, hitIndices, 14) X(int32_t, pixelSeedIndex, 1)                  \
+  X(uint16_t, lowerModuleIndices, 7) X(float, centerX, 1) X(float, centerY, 1) X(float, radius, 1)
+
 #define LST_DECL_FIELD(T, N, M) T* N;
 
 namespace lst {
@@ -60,6 +83,20 @@ struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
 struct T5SoA { LST_T5_FIELDS(LST_DECL_FIELD) };
+struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
+struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };
+struct TCSoA { LST_TC_FIELDS(LST_DECL_FIELD) };
+
+// Per-pLS and per-module work arrays of the pixel stages.
+struct PixelWork {
+  uint32_t* connBase;     // [kNMaxPixelSegments] first connected-module slot in the pixel map
+  uint32_t* connSize;     // [kNMaxPixelSegments]
+  uint32_t* pt5Count;     // [kNMaxPixelSegments + 1] passing pT5 candidates per pLS, then offsets
+  uint32_t* pt3Count;     // [kNMaxPixelSegments + 1]
+  uint32_t* t5TcCount;    // [nLower + 1] T5 track candidates per eligible module, then offsets
+  uint32_t* outHits;      // [TC capacity * 14] output hit lists
+  uint32_t* outLen;       // [TC capacity]
+};
 
 // Per-module counters and ranges (SDL/Module.h:21-85, SDL/Hit.h:28-32).
 struct RangesSoA {
@@ -87,5 +124,20 @@ struct RangesSoA {
   uint16_t* indicesOfEligibleT5Modules;  // [nLower]
   uint32_t* scalars;         // [16] device-side totals (see Scalar)
 };
-enum Scalar : int { kNTotalTrips = 0, kBadHits = 1, kNEligibleT5 = 2, kNTotalQuints = 3, kNScalars = 16 };
+enum Scalar : int {
+  kNTotalTrips = 0,
+  kBadHits = 1,
+  kNEligibleT5 = 2,
+  kNTotalQuints = 3,
+  kNPixelQuintuplets = 4,
+  kTotOccPixelQuintuplets = 5,
+  kNPixelTriplets = 6,
+  kTotOccPixelTriplets = 7,
+  kNTrackCandidates = 8,
+  kNTrackCandidatespT3 = 9,
+  kNTrackCandidatespT5 = 10,
+  kNTrackCandidatespLS = 11,
+  kNTrackCandidatesT5 = 12,
+  kNScalars = 16
+};
 }  // namespace lst

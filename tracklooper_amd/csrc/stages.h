@@ -14,6 +14,10 @@ struct EventView {  // everything a stage kernel needs, by value
   PLSSoA P;
   T3SoA T;
   T5SoA Q;
+  PT5SoA P5;
+  PT3SoA P3;
+  TCSoA C;
+  PixelWork W;
   RangesSoA R;
   const int32_t* mdModuleIndices;    // [nLower + 1], geometry-only
   const int32_t* mdModuleOccupancy;  // [nLower + 1]
@@ -22,6 +26,9 @@ struct EventView {  // everything a stage kernel needs, by value
   const int32_t* t3OccIfAny;         // [nLower] T3 capacity of a module that has segments
   const int32_t* t5OccIfAny;         // [nLower] T5 capacity of a seed module, -1 if none
   uint32_t nT5SeedModules;           // geometry bound on the eligible T5 module count
+  const uint32_t* pixIndex[3];       // pixel map: connected-module start per superbin, by pLS type
+  const uint32_t* pixSizes[3];
+  int tcPlsTriplets;                 // TC_PLS_TRIPLETS: triplet pLS may become track candidates
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };
@@ -40,5 +47,9 @@ void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& 
 void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimer& t = {});
 
 }  // namespace lst
