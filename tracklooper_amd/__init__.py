@@ -45,16 +45,18 @@ class _Input(C.Structure):
 
 
 def library_path() -> str:
-    return LIB_PATH
+    """The in-tree build, or the file named by $LST_LIBRARY (an alternative build of the same sources)."""
+    return os.environ.get("LST_LIBRARY", LIB_PATH)
 
 
 def lib():
     """Load liblst_amd.so; raises if it has not been built (no fallback)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise LSTError(f"{LIB_PATH} is missing: run `python -m tracklooper_amd.build` (HIP, gfx950)")
-        L = C.CDLL(LIB_PATH)
+        path = library_path()
+        if not os.path.exists(path):
+            raise LSTError(f"{path} is missing: run `python -m tracklooper_amd.build` (HIP, gfx950)")
+        L = C.CDLL(path)
         P = C.c_void_p
         L.lst_last_error.restype = C.c_char_p
         L.lst_es_create.argtypes = [C.c_char_p, C.c_float, C.c_uint32, C.POINTER(P)]

@@ -169,6 +169,13 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5Count, plsCap + 1); c.take(e.W.pt3Count, plsCap + 1);
   c.take(e.W.t5TcCount, nL + 1);
   c.take(e.W.outHits, (size_t)tcCap * 14); c.take(e.W.outLen, tcCap);
+  c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
+  c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
+  c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3ResultMax);
+  c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
+  c.take(e.W5.resIdx, lst::kT5CandMax);
+  c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
+  c.take(e.W5.results, (size_t)lst::kT5ResultWords * lst::kT5ResultMax);
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);
   c.take(e.R.mdRanges, 2 * nMod); c.take(e.R.segmentRanges, 2 * nMod); c.take(e.R.tripletRanges, 2 * nMod);

@@ -87,6 +87,33 @@ struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
 struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };
 struct TCSoA { LST_TC_FIELDS(LST_DECL_FIELD) };
 
+// Work arrays of the flattened T3 builder (same scheme as the T5 one below).
+constexpr uint32_t kT3CandMax = 1u << 22;
+constexpr uint32_t kT3ResultMax = 1u << 20;
+constexpr int kT3ResultWords = 6;
+struct T3Work {
+  uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
+  uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
+  uint32_t* resIdx;   // [kT3CandMax]
+  uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check
+  uint32_t* results;  // [kT3ResultWords * kT3ResultMax]
+};
+
+// Work arrays of the flattened T5 builder: every (inner T3, outer T3)
+// candidate of the event gets a global index L (eligible module, then inner
+// T3, then outer T3: the reference's serial order).
+constexpr uint32_t kT5CandMax = 1u << 21;    // candidates per event handled by the flattened path
+constexpr uint32_t kT5ResultMax = 1u << 19;  // passing candidates per event parked by the flattened path
+struct T5Work {
+  uint32_t* candPre;  // [T3 capacity] per inner T3: candidates before it within its module
+  uint32_t* modOff;   // [nLower + 2] per eligible-list position: candidate count, then offsets
+  uint32_t* resIdx;   // [kT5CandMax] result slot of a passing candidate, ~0u otherwise
+  uint32_t* listA;    // [3 * kT5CandMax] (L, inner T3, outer T3) passing the shared-MD check
+  uint32_t* listB;    // [3 * kT5CandMax] ... and both T4-style tests
+  uint32_t* results;  // [kT5ResultWords * kT5ResultMax] selection results of passing candidates
+};
+constexpr int kT5ResultWords = 12;
+
 // Per-pLS and per-module work arrays of the pixel stages.
 struct PixelWork {
   uint32_t* connBase;     // [kNMaxPixelSegments] first connected-module slot in the pixel map
@@ -138,6 +165,13 @@ enum Scalar : int {
   kNTrackCandidatespT5 = 10,
   kNTrackCandidatespLS = 11,
   kNTrackCandidatesT5 = 12,
-  kNScalars = 16
+  kT5CandTotal = 13,
+  kT5ListA = 14,
+  kT5ListB = 15,
+  kT5Results = 16,
+  kT3CandTotal = 17,
+  kT3List = 18,
+  kT3Results = 19,
+  kNScalars = 32
 };
 }  // namespace lst

@@ -18,6 +18,8 @@ struct EventView {  // everything a stage kernel needs, by value
   PT3SoA P3;
   TCSoA C;
   PixelWork W;
+  T3Work W3;
+  T5Work W5;
   RangesSoA R;
   const int32_t* mdModuleIndices;    // [nLower + 1], geometry-only
   const int32_t* mdModuleOccupancy;  // [nLower + 1]

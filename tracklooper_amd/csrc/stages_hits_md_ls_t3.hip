@@ -353,8 +353,197 @@ __global__ __launch_bounds__(1024) void k_triplet_offsets(EventView v) {
   if (threadIdx.x == 0) v.R.scalars[kNTotalTrips] = carry;
 }
 
+// runTripletConstraintsAndAlgo (SDL/Triplet.h:796-872) for the segment pair
+// (inSeg on m1, outSeg), including the shared middle mini-doublet check
+__device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint32_t inSeg, uint32_t outSeg,
+                                          float& betaIn, float& radius, float& gx, float& gy) {
+  const uint32_t md1 = v.S.mdIndices[2 * inSeg];
+  const uint32_t md2 = v.S.mdIndices[2 * outSeg];
+  const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
+  if (v.S.mdIndices[2 * inSeg + 1] != md2) return false;  // SDL/Triplet.h:823-825
+  const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
+  T3Geo p;
+  p.x1 = v.D.anchorX[md1]; p.y1 = v.D.anchorY[md1]; p.z1 = v.D.anchorZ[md1]; p.r1 = v.D.anchorRt[md1];
+  p.phi1 = v.D.anchorPhi[md1];
+  p.x2 = v.D.anchorX[md2]; p.y2 = v.D.anchorY[md2]; p.z2 = v.D.anchorZ[md2]; p.r2 = v.D.anchorRt[md2];
+  p.x3 = v.D.anchorX[md3]; p.y3 = v.D.anchorY[md3]; p.z3 = v.D.anchorZ[md3]; p.r3 = v.D.anchorRt[md3];
+  if (!t3_rz(v.M.sdlLayers[m1], v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) return false;
+  const int16_t s1 = v.M.subdets[m1], s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
+  const bool psIn = v.M.moduleType[m1] == kPS, psOut = v.M.moduleType[m3] == kPS;
+  const float alphaIn = v.S.dPhiChanges[inSeg];
+  bool pass = false;
+  if (s1 == kBarrel && s2 == kBarrel && s3 == kBarrel) pass = t3_bbb(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
+  else if (s1 == kBarrel && s3 == kEndcap) pass = t3_bbe(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
+  else if (s1 == kEndcap && s2 == kEndcap && s3 == kEndcap)
+    pass = t3_eee(psIn, psOut, v.ptCut, p, alphaIn, v.S.dPhiChangeMins[inSeg], v.S.dPhiChangeMaxs[inSeg], betaIn);
+  if (pass) radius = circle_radius3(p.x1, p.y1, p.x2, p.y2, p.x3, p.y3, gx, gy);
+  return pass;
+}
+
+// addTripletToMemory (SDL/Triplet.h:153-239)
+__device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_t m1, uint32_t inSeg, uint32_t outSeg,
+                                         float betaIn, float radius, float gx, float gy) {
+  const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
+  const uint32_t md1 = v.S.mdIndices[2 * inSeg], mdB = v.S.mdIndices[2 * inSeg + 1];
+  const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
+  v.T.segmentIndices[2 * t] = inSeg; v.T.segmentIndices[2 * t + 1] = outSeg;
+  v.T.lowerModuleIndices[3 * t] = m1; v.T.lowerModuleIndices[3 * t + 1] = m2;
+  v.T.lowerModuleIndices[3 * t + 2] = m3;
+  v.T.betaIn[t] = betaIn; v.T.circleRadius[t] = radius; v.T.circleCenterX[t] = gx; v.T.circleCenterY[t] = gy;
+  v.T.logicalLayers[3 * t] = v.M.layers[m1] + (v.M.subdets[m1] == kEndcap) * 6;
+  v.T.logicalLayers[3 * t + 1] = v.M.layers[m2] + (v.M.subdets[m2] == kEndcap) * 6;
+  v.T.logicalLayers[3 * t + 2] = v.M.layers[m3] + (v.M.subdets[m3] == kEndcap) * 6;
+  v.T.hitIndices[6 * t + 0] = v.D.anchorHitIndices[md1]; v.T.hitIndices[6 * t + 1] = v.D.outerHitIndices[md1];
+  v.T.hitIndices[6 * t + 2] = v.D.anchorHitIndices[mdB]; v.T.hitIndices[6 * t + 3] = v.D.outerHitIndices[mdB];
+  v.T.hitIndices[6 * t + 4] = v.D.anchorHitIndices[md3]; v.T.hitIndices[6 * t + 5] = v.D.outerHitIndices[md3];
+  v.T.partOfPT5[t] = 0; v.T.partOfT5[t] = 0; v.T.partOfPT3[t] = 0;
+}
+
+// ----------------------------------------------------- flattened T3 builder
+// Candidates are numbered in the reference's serial order (module m1, inner
+// segment a, outer segment b on the inner segment's outer module): module m1
+// owns [modOff[m1], modOff[m1+1]), inner segment a owns
+// [candPre[a], candPre[a] + nSegments(m2(a))) within it.
+__device__ __forceinline__ bool t3_flat_mode(const EventView& v) {
+  return v.R.scalars[kT3CandTotal] <= kT3CandMax;
+}
+__device__ __forceinline__ bool t3_flat_done(const EventView& v) {
+  return t3_flat_mode(v) && v.R.scalars[kT3Results] <= kT3ResultMax;
+}
+
+__global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
+  __shared__ unsigned lds[5];
+  const uint32_t m1 = blockIdx.x;
+  const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+  const uint32_t segBase = v.segModuleIndices[m1];
+  uint32_t run = 0;
+  for (uint32_t a0 = 0; a0 < nIn; a0 += 256) {
+    const uint32_t a = a0 + threadIdx.x;
+    const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<256>(cnt, lds, tot);
+    if (a < nIn) v.W3.candPre[segBase + a] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) v.W3.modOff[m1] = run;
+}
+
+__global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t n = v.M.nLowerModules;
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < n; b += 1024) {
+    const uint32_t i = b + threadIdx.x;
+    const unsigned c = i < n ? v.W3.modOff[i] : 0u;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (i < n) v.W3.modOff[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.W3.modOff[n] = carry;
+    v.R.scalars[kT3CandTotal] = carry;
+    v.R.scalars[kT3List] = 0;
+    v.R.scalars[kT3Results] = 0;
+  }
+}
+
+// stage 1: shared middle mini-doublet over all candidates
+__global__ __launch_bounds__(256) void k_t3_filter_md(EventView v) {
+  if (!t3_flat_mode(v)) return;
+  const uint32_t total = v.R.scalars[kT3CandTotal];
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
+    const uint32_t L = L0 + threadIdx.x;
+    if (L >= total) continue;
+    v.W3.resIdx[L] = ~0u;
+    uint32_t lo = 0, hi = nL;  // largest m with modOff[m] <= L
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (v.W3.modOff[mid] <= L) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t m1 = lo, l = L - v.W3.modOff[lo];
+    const uint32_t segBase = v.segModuleIndices[m1];
+    uint32_t a0 = 0, a1 = v.R.nSegments[m1];  // largest a with candPre[a] <= l
+    while (a1 - a0 > 1) {
+      const uint32_t mid = (a0 + a1) >> 1;
+      if (v.W3.candPre[segBase + mid] <= l) a0 = mid;
+      else a1 = mid;
+    }
+    const uint32_t inSeg = segBase + a0;
+    const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg];
+    const uint32_t outSeg = v.segModuleIndices[m2] + (l - v.W3.candPre[inSeg]);
+    if (v.S.mdIndices[2 * inSeg + 1] == v.S.mdIndices[2 * outSeg]) {
+      const uint64_t mask = __ballot(1);
+      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      const uint32_t k = base + lanes_below(mask);
+      v.W3.list[3 * k] = L;
+      v.W3.list[3 * k + 1] = m1;
+      v.W3.list[3 * k + 2] = (inSeg - segBase) | ((l - v.W3.candPre[inSeg]) << 16);
+    }
+  }
+}
+
+// stage 2: the triplet selection; results parked per candidate
+__global__ __launch_bounds__(256) void k_t3_select(EventView v) {
+  if (!t3_flat_mode(v)) return;
+  const uint32_t n = v.R.scalars[kT3List];
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    if (i >= n) continue;
+    const uint32_t L = v.W3.list[3 * i], m1 = v.W3.list[3 * i + 1], ab = v.W3.list[3 * i + 2];
+    const uint32_t inSeg = v.segModuleIndices[m1] + (ab & 0xFFFFu);
+    const uint32_t outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (ab >> 16);
+    float betaIn = 0, radius = 0, gx = 0, gy = 0;
+    if (t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy)) {
+      const uint32_t k = atomicAdd(&v.R.scalars[kT3Results], 1u);
+      if (k >= kT3ResultMax) continue;  // overflow: the per-module builder redoes the stage
+      uint32_t* w = &v.W3.results[kT3ResultWords * k];
+      w[0] = inSeg; w[1] = outSeg;
+      w[2] = __float_as_uint(betaIn); w[3] = __float_as_uint(radius);
+      w[4] = __float_as_uint(gx); w[5] = __float_as_uint(gy);
+      v.W3.resIdx[L] = k;
+    }
+  }
+}
+
+// stage 3: per module, passing candidates in candidate order take the slots
+__global__ __launch_bounds__(256) void k_t3_place(EventView v) {
+  __shared__ unsigned lds[5];
+  if (!t3_flat_done(v)) return;
+  const uint32_t m1 = blockIdx.x;
+  const uint32_t off = v.W3.modOff[m1], n = v.W3.modOff[m1 + 1] - off;
+  const int cap = v.R.tripletModuleOccupancy[m1];
+  const uint32_t t3Base = v.R.tripletModuleIndices[m1];
+  unsigned count = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 256) {
+    const uint32_t l = c0 + threadIdx.x;
+    const uint32_t k = l < n ? v.W3.resIdx[off + l] : ~0u;
+    const bool pass = k != ~0u;
+    unsigned tot;
+    const unsigned rank = count + block_exclusive_scan<256>(pass ? 1u : 0u, lds, tot);
+    if (pass && (int)rank < cap) {
+      const uint32_t* w = &v.W3.results[kT3ResultWords * k];
+      t3_store(v, t3Base + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
+               __uint_as_float(w[4]), __uint_as_float(w[5]));
+    }
+    count += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.R.totOccupancyTriplets[m1] = count;
+    v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// Per-module builder, used when an event exceeds the flattened path's lists:
+// one wave per lower module walks its candidates in order.
 __global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
   __shared__ uint32_t pre[1025];
+  if (t3_flat_done(v)) return;
   const uint32_t m1 = blockIdx.x;
   const unsigned lane = threadIdx.x;
   const uint32_t nIn = v.R.nSegments[m1];
@@ -373,67 +562,26 @@ __global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
     if (lane == 0) pre[0] = 0;
     __syncthreads();
     const uint32_t total = run;
-    const bool psIn = v.M.moduleType[m1] == kPS;
-    const int l1 = v.M.sdlLayers[m1];
-    const int16_t s1 = v.M.subdets[m1];
     const uint32_t t3Base = v.R.tripletModuleIndices[m1];
     for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
       const uint32_t L = c0 + lane;
       bool pass = false;
       float betaIn = 0, radius = 0, gx = 0, gy = 0;
-      uint32_t inSeg = 0, outSeg = 0, md1 = 0, md2 = 0, md3 = 0;
-      uint16_t m2 = 0, m3 = 0;
+      uint32_t inSeg = 0, outSeg = 0;
       if (L < total) {
-        // largest a with pre[a] <= L
-        uint32_t lo = 0, hi = nIn;
+        uint32_t lo = 0, hi = nIn;  // largest a with pre[a] <= L
         while (hi - lo > 1) {
           const uint32_t mid = (lo + hi) >> 1;
           if (pre[mid] <= L) lo = mid;
           else hi = mid;
         }
         inSeg = segBase + lo;
-        m2 = v.S.outerLowerModuleIndices[inSeg];
-        outSeg = v.segModuleIndices[m2] + (L - pre[lo]);
-        m3 = v.S.outerLowerModuleIndices[outSeg];
-        md1 = v.S.mdIndices[2 * inSeg];
-        md2 = v.S.mdIndices[2 * outSeg];
-        md3 = v.S.mdIndices[2 * outSeg + 1];
-        if (v.S.mdIndices[2 * inSeg + 1] == md2) {  // shared middle mini-doublet (SDL/Triplet.h:823-825)
-          T3Geo p;
-          p.x1 = v.D.anchorX[md1]; p.y1 = v.D.anchorY[md1]; p.z1 = v.D.anchorZ[md1]; p.r1 = v.D.anchorRt[md1];
-          p.phi1 = v.D.anchorPhi[md1];
-          p.x2 = v.D.anchorX[md2]; p.y2 = v.D.anchorY[md2]; p.z2 = v.D.anchorZ[md2]; p.r2 = v.D.anchorRt[md2];
-          p.x3 = v.D.anchorX[md3]; p.y3 = v.D.anchorY[md3]; p.z3 = v.D.anchorZ[md3]; p.r3 = v.D.anchorRt[md3];
-          if (t3_rz(l1, v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) {
-            const int16_t s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
-            const bool psOut = v.M.moduleType[m3] == kPS;
-            const float alphaIn = v.S.dPhiChanges[inSeg];
-            if (s1 == kBarrel && s2 == kBarrel && s3 == kBarrel) pass = t3_bbb(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
-            else if (s1 == kBarrel && s3 == kEndcap) pass = t3_bbe(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
-            else if (s1 == kEndcap && s2 == kEndcap && s3 == kEndcap)
-              pass = t3_eee(psIn, psOut, v.ptCut, p, alphaIn, v.S.dPhiChangeMins[inSeg], v.S.dPhiChangeMaxs[inSeg],
-                            betaIn);
-            if (pass) radius = circle_radius3(p.x1, p.y1, p.x2, p.y2, p.x3, p.y3, gx, gy);
-          }
-        }
+        outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (L - pre[lo]);
+        pass = t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy);
       }
       unsigned n;
       const unsigned rank = count + wave_rank(pass, n);
-      if (pass && (int)rank < cap) {
-        const uint32_t t = t3Base + rank;
-        v.T.segmentIndices[2 * t] = inSeg; v.T.segmentIndices[2 * t + 1] = outSeg;
-        v.T.lowerModuleIndices[3 * t] = m1; v.T.lowerModuleIndices[3 * t + 1] = m2;
-        v.T.lowerModuleIndices[3 * t + 2] = m3;
-        v.T.betaIn[t] = betaIn; v.T.circleRadius[t] = radius; v.T.circleCenterX[t] = gx; v.T.circleCenterY[t] = gy;
-        v.T.logicalLayers[3 * t] = v.M.layers[m1] + (s1 == kEndcap) * 6;
-        v.T.logicalLayers[3 * t + 1] = v.M.layers[m2] + (v.M.subdets[m2] == kEndcap) * 6;
-        v.T.logicalLayers[3 * t + 2] = v.M.layers[m3] + (v.M.subdets[m3] == kEndcap) * 6;
-        const uint32_t mdB = v.S.mdIndices[2 * inSeg + 1];
-        v.T.hitIndices[6 * t + 0] = v.D.anchorHitIndices[md1]; v.T.hitIndices[6 * t + 1] = v.D.outerHitIndices[md1];
-        v.T.hitIndices[6 * t + 2] = v.D.anchorHitIndices[mdB]; v.T.hitIndices[6 * t + 3] = v.D.outerHitIndices[mdB];
-        v.T.hitIndices[6 * t + 4] = v.D.anchorHitIndices[md3]; v.T.hitIndices[6 * t + 5] = v.D.outerHitIndices[md3];
-        v.T.partOfPT5[t] = 0; v.T.partOfT5[t] = 0; v.T.partOfPT3[t] = 0;
-      }
+      if (pass && (int)rank < cap) t3_store(v, t3Base + rank, m1, inSeg, outSeg, betaIn, radius, gx, gy);
       count += n;
     }
   }
@@ -494,9 +642,14 @@ void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t) {
 
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_cand_count, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_filter_md, dim3(2048), dim3(256), 0, s, v);
   t.begin(s);
-  hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
   t.end(s);
+  hipLaunchKernelGGL(k_t3_place, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 

@@ -1,12 +1,16 @@
 // Quintuplet stage: eligible-module scan, builder, after-build duplicate
 // removal and ranges (SDL/Event.cc:903-991).
 //
-// The builder runs one 256-lane workgroup per eligible module.  Candidates
-// (inner T3 a of the module, outer T3 b on the inner T3's third module) are
-// linearised in the reference's serial order (a-major,
-// SDL/Quintuplet.h:2988-3115), evaluated 256 at a time and ranked with a
-// workgroup scan, so T5 slots equal the reference CPU slots and the per-module
-// capacity keeps exactly the first `cap` passing candidates.
+// Candidates (inner T3 a of an eligible module, outer T3 b on the inner T3's
+// third module) are numbered in the reference's serial order (module, then a,
+// then b; SDL/Quintuplet.h:2988-3115).  The flattened builder spreads them over
+// the whole chip: a shared-MD filter and the two T4-style tests compact the
+// survivors into lists (order within a list does not matter), the final
+// selection parks each passing candidate's results, and one work-group per
+// module then gives passing candidates their slots in candidate order, so T5
+// slots equal the reference CPU slots and the per-module capacity keeps exactly
+// the first `cap` passing candidates.  Events with more than kT5CandMax
+// candidates take the per-module builder instead (same order, same results).
 #include "algo_t5.h"
 #include "stages.h"
 #include "wave.h"
@@ -82,32 +86,63 @@ __device__ __forceinline__ int radii_category(const ModLite* m) {
   return 7;
 }
 
-// runQuintupletDefaultAlgo, plain chi2 mode (SDL/Quintuplet.h:2508-2986)
-__device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t innerT3, uint32_t outerT3,
-                             uint32_t* md, T5Result& r) {
-  const uint32_t seg1 = v.T.segmentIndices[2 * innerT3], seg2 = v.T.segmentIndices[2 * innerT3 + 1];
-  const uint32_t seg3 = v.T.segmentIndices[2 * outerT3], seg4 = v.T.segmentIndices[2 * outerT3 + 1];
-  if (v.S.mdIndices[2 * seg2 + 1] != v.S.mdIndices[2 * seg3]) return false;
-  md[0] = v.S.mdIndices[2 * seg1];
-  md[1] = v.S.mdIndices[2 * seg2];
-  md[2] = v.S.mdIndices[2 * seg2 + 1];
-  md[3] = v.S.mdIndices[2 * seg3 + 1];
-  md[4] = v.S.mdIndices[2 * seg4 + 1];
+// runQuintupletDefaultAlgo, plain chi2 mode (SDL/Quintuplet.h:2508-2986), in
+// three parts so that the flattened builder can compact survivors between them:
+// the shared mini-doublet check, the two T4-style tests, and the rest.
+__device__ __forceinline__ bool t5_shared_md(const EventView& v, uint32_t innerT3, uint32_t outerT3) {
+  const uint32_t seg2 = v.T.segmentIndices[2 * innerT3 + 1], seg3 = v.T.segmentIndices[2 * outerT3];
+  return v.S.mdIndices[2 * seg2 + 1] == v.S.mdIndices[2 * seg3];
+}
+
+struct T5Cand {  // the five modules and mini-doublets of a candidate
+  uint16_t mods[5];
+  uint32_t md[5];
+  uint32_t seg1, seg3, seg4;
+};
+__device__ __forceinline__ T5Cand t5_cand(const EventView& v, uint32_t innerT3, uint32_t outerT3) {
+  T5Cand c;
+  c.seg1 = v.T.segmentIndices[2 * innerT3];
+  const uint32_t seg2 = v.T.segmentIndices[2 * innerT3 + 1];
+  c.seg3 = v.T.segmentIndices[2 * outerT3];
+  c.seg4 = v.T.segmentIndices[2 * outerT3 + 1];
+  c.mods[0] = v.T.lowerModuleIndices[3 * innerT3];
+  c.mods[1] = v.T.lowerModuleIndices[3 * innerT3 + 1];
+  c.mods[2] = v.T.lowerModuleIndices[3 * innerT3 + 2];
+  c.mods[3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
+  c.mods[4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
+  c.md[0] = v.S.mdIndices[2 * c.seg1];
+  c.md[1] = v.S.mdIndices[2 * seg2];
+  c.md[2] = v.S.mdIndices[2 * seg2 + 1];
+  c.md[3] = v.S.mdIndices[2 * c.seg3 + 1];
+  c.md[4] = v.S.mdIndices[2 * c.seg4 + 1];
+  return c;
+}
+
+__device__ __forceinline__ bool t5_t4_tests(const EventView& v, const T5Cand& c) {
   ModLite m[5];
   MDPoint p[5];
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    m[k] = load_lite(v.M, mods[k]);
-    p[k] = load_point(v.D, md[k]);
+    m[k] = load_lite(v.M, c.mods[k]);
+    p[k] = load_point(v.D, c.md[k]);
   }
-  const SegAlpha a1 = load_alpha(v.S, seg1);
+  const SegAlpha a1 = load_alpha(v.S, c.seg1);
   {
-    T4Args q{m[0], m[1], m[2], m[3], p[0], p[1], p[2], p[3], a1, load_alpha(v.S, seg3)};
+    T4Args q{m[0], m[1], m[2], m[3], p[0], p[1], p[2], p[3], a1, load_alpha(v.S, c.seg3)};
     if (!t4_select(q, v.ptCut)) return false;
   }
-  {
-    T4Args q{m[0], m[1], m[3], m[4], p[0], p[1], p[3], p[4], a1, load_alpha(v.S, seg4)};
-    if (!t4_select(q, v.ptCut)) return false;
+  T4Args q{m[0], m[1], m[3], m[4], p[0], p[1], p[3], p[4], a1, load_alpha(v.S, c.seg4)};
+  return t4_select(q, v.ptCut);
+}
+
+__device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uint32_t innerT3, uint32_t outerT3,
+                                        T5Result& r) {
+  ModLite m[5];
+  MDPoint p[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    m[k] = load_lite(v.M, c.mods[k]);
+    p[k] = load_point(v.D, c.md[k]);
   }
   float g, f;
   r.outerR = v.T.circleRadius[outerT3];
@@ -138,17 +173,253 @@ __device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t 
     return false;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    x[k] = v.D.outerX[md[k]];
-    y[k] = v.D.outerY[md[k]];
+    x[k] = v.D.outerX[c.md[k]];
+    y[k] = v.D.outerY[c.md[k]];
   }
   regression_sigmas(m, false, d1, d2, sl, flat);
   r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
   return true;
 }
 
+__device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t innerT3, uint32_t outerT3,
+                             uint32_t* md, T5Result& r) {
+  if (!t5_shared_md(v, innerT3, outerT3)) return false;
+  const T5Cand c = t5_cand(v, innerT3, outerT3);
+#pragma unroll
+  for (int k = 0; k < 5; k++) md[k] = c.md[k];
+  if (!t5_t4_tests(v, c)) return false;
+  return t5_rest(v, c, innerT3, outerT3, r);
+}
+
+// addQuintupletToMemory (SDL/Quintuplet.h:150-215) plus the T3 flags
+__device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_t innerT3, uint32_t outerT3,
+                                         const T5Result& r, int layer) {
+  const int adj = layer == 1 ? 1 : 0;
+  const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * innerT3 + adj]];
+  v.Q.tripletIndices[2 * q] = innerT3;
+  v.Q.tripletIndices[2 * q + 1] = outerT3;
+  v.Q.lowerModuleIndices[5 * q + 0] = v.T.lowerModuleIndices[3 * innerT3];
+  v.Q.lowerModuleIndices[5 * q + 1] = v.T.lowerModuleIndices[3 * innerT3 + 1];
+  v.Q.lowerModuleIndices[5 * q + 2] = v.T.lowerModuleIndices[3 * innerT3 + 2];
+  v.Q.lowerModuleIndices[5 * q + 3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
+  v.Q.lowerModuleIndices[5 * q + 4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
+  v.Q.innerRadius[q] = r.innerR;
+  v.Q.outerRadius[q] = r.outerR;
+  v.Q.bridgeRadius[q] = r.bridgeR;
+  v.Q.pt[q] = (r.innerR + r.outerR) * 3.8f * 1.602f / (2 * 100 * 5.39f);
+  v.Q.eta[q] = v.D.anchorEta[mdAdj];
+  v.Q.phi[q] = v.D.anchorPhi[mdAdj];
+  v.Q.score_rphisum[q] = r.chi2 + r.nonAnchorChi2;
+  v.Q.layer[q] = (uint8_t)layer;
+  v.Q.isDup[q] = 0;
+  v.Q.TightCutFlag[q] = r.tight;
+  v.Q.partOfPT5[q] = 0;
+  v.Q.regressionRadius[q] = r.regR;
+  v.Q.regressionG[q] = r.regG;
+  v.Q.regressionF[q] = r.regF;
+  v.Q.rzChiSquared[q] = r.rzChi2;
+  v.Q.chiSquared[q] = r.chi2;
+  v.Q.nonAnchorChiSquared[q] = r.nonAnchorChi2;
+  v.Q.logicalLayers[5 * q + 0] = v.T.logicalLayers[3 * innerT3];
+  v.Q.logicalLayers[5 * q + 1] = v.T.logicalLayers[3 * innerT3 + 1];
+  v.Q.logicalLayers[5 * q + 2] = v.T.logicalLayers[3 * innerT3 + 2];
+  v.Q.logicalLayers[5 * q + 3] = v.T.logicalLayers[3 * outerT3 + 1];
+  v.Q.logicalLayers[5 * q + 4] = v.T.logicalLayers[3 * outerT3 + 2];
+#pragma unroll
+  for (int j = 0; j < 6; j++) v.Q.hitIndices[10 * q + j] = v.T.hitIndices[6 * innerT3 + j];
+#pragma unroll
+  for (int j = 0; j < 4; j++) v.Q.hitIndices[10 * q + 6 + j] = v.T.hitIndices[6 * outerT3 + 2 + j];
+  v.T.partOfT5[innerT3] = 1;
+  v.T.partOfT5[outerT3] = 1;
+}
+
+// ------------------------------------------------------ flattened T5 builder
+// Candidates of module b (eligible-list position) are [modOff[b], modOff[b+1]);
+// within a module, inner T3 a owns [candPre[a], candPre[a] + nTriplets(m3(a))).
+
+// per eligible module: candidate count and per-inner-T3 exclusive prefix
+__global__ __launch_bounds__(kT5Block) void k_t5_cand_count(EventView v) {
+  __shared__ unsigned lds[kT5Block / 64 + 1];
+  const uint32_t nElig = v.R.scalars[kNEligibleT5];
+  if (blockIdx.x >= nElig) return;
+  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const int layer = v.M.layers[m1];
+  const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
+  const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
+  uint32_t run = 0;
+  for (uint32_t a0 = 0; a0 < nIn; a0 += kT5Block) {
+    const uint32_t a = a0 + threadIdx.x;
+    const uint32_t cnt = a < nIn ? v.R.nTriplets[v.T.lowerModuleIndices[3 * (t3Base + a) + 2]] : 0;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<kT5Block>(cnt, lds, tot);
+    if (a < nIn) v.W5.candPre[t3Base + a] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) v.W5.modOff[blockIdx.x] = run;
+}
+
+// exclusive scan of the per-module counts (count = nEligible, read on the device)
+__global__ __launch_bounds__(1024) void k_t5_cand_scan(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t n = v.R.scalars[kNEligibleT5];
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < n; b += 1024) {
+    const uint32_t i = b + threadIdx.x;
+    const unsigned c = i < n ? v.W5.modOff[i] : 0u;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (i < n) v.W5.modOff[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.W5.modOff[n] = carry;
+    v.R.scalars[kT5CandTotal] = carry;
+    v.R.scalars[kT5ListA] = 0;
+    v.R.scalars[kT5ListB] = 0;
+    v.R.scalars[kT5Results] = 0;
+  }
+}
+
+__device__ __forceinline__ bool t5_flat_mode(const EventView& v) {
+  return v.R.scalars[kT5CandTotal] <= kT5CandMax;
+}
+// the flattened path completed: every candidate fitted and every result was parked
+__device__ __forceinline__ bool t5_flat_done(const EventView& v) {
+  return t5_flat_mode(v) && v.R.scalars[kT5Results] <= kT5ResultMax;
+}
+
+// candidate L -> (inner T3, outer T3)
+__device__ __forceinline__ void t5_decode(const EventView& v, uint32_t L, uint32_t nElig, uint32_t& innerT3,
+                                          uint32_t& outerT3) {
+  uint32_t lo = 0, hi = nElig;  // largest b with modOff[b] <= L
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (v.W5.modOff[mid] <= L) lo = mid;
+    else hi = mid;
+  }
+  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[lo];
+  const uint32_t l = L - v.W5.modOff[lo];
+  const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
+  uint32_t a0 = 0, a1 = v.R.nTriplets[m1];  // largest a with candPre[a] <= l
+  while (a1 - a0 > 1) {
+    const uint32_t mid = (a0 + a1) >> 1;
+    if (v.W5.candPre[t3Base + mid] <= l) a0 = mid;
+    else a1 = mid;
+  }
+  innerT3 = t3Base + a0;
+  const uint16_t m3 = v.T.lowerModuleIndices[3 * innerT3 + 2];
+  outerT3 = (uint32_t)v.R.tripletModuleIndices[m3] + (l - v.W5.candPre[innerT3]);
+}
+
+__device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uint32_t L, uint32_t a, uint32_t b) {
+  // wave-aggregated append: one atomic per wave
+  const uint64_t mask = __ballot(1);
+  const unsigned leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  const uint32_t k = base + lanes_below(mask);
+  list[3 * k] = L;
+  list[3 * k + 1] = a;
+  list[3 * k + 2] = b;
+}
+
+// stage 1: shared mini-doublet check over all candidates
+__global__ __launch_bounds__(256) void k_t5_filter_md(EventView v) {
+  if (!t5_flat_mode(v)) return;
+  const uint32_t total = v.R.scalars[kT5CandTotal];
+  const uint32_t nElig = v.R.scalars[kNEligibleT5];
+  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
+    const uint32_t L = L0 + threadIdx.x;
+    if (L < total) {
+      v.W5.resIdx[L] = ~0u;
+      uint32_t inner, outer;
+      t5_decode(v, L, nElig, inner, outer);
+      if (t5_shared_md(v, inner, outer)) list_push(v.W5.listA, &v.R.scalars[kT5ListA], L, inner, outer);
+    }
+  }
+}
+
+// stage 2: the two T4-style tests
+__global__ __launch_bounds__(256) void k_t5_filter_t4(EventView v) {
+  if (!t5_flat_mode(v)) return;
+  const uint32_t n = v.R.scalars[kT5ListA];
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    if (i < n) {
+      const uint32_t L = v.W5.listA[3 * i], inner = v.W5.listA[3 * i + 1], outer = v.W5.listA[3 * i + 2];
+      const T5Cand c = t5_cand(v, inner, outer);
+      if (t5_t4_tests(v, c)) list_push(v.W5.listB, &v.R.scalars[kT5ListB], L, inner, outer);
+    }
+  }
+}
+
+// stage 3: radii, chi2 cuts and the regression; results parked per candidate
+__global__ __launch_bounds__(256) void k_t5_final(EventView v) {
+  if (!t5_flat_mode(v)) return;
+  const uint32_t n = v.R.scalars[kT5ListB];
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    if (i < n) {
+      const uint32_t L = v.W5.listB[3 * i], inner = v.W5.listB[3 * i + 1], outer = v.W5.listB[3 * i + 2];
+      const T5Cand c = t5_cand(v, inner, outer);
+      T5Result r;
+      if (t5_rest(v, c, inner, outer, r)) {
+        const uint32_t k = atomicAdd(&v.R.scalars[kT5Results], 1u);
+        if (k >= kT5ResultMax) continue;  // overflow: the per-module builder redoes the stage
+        uint32_t* w = &v.W5.results[kT5ResultWords * k];
+        w[0] = inner; w[1] = outer;
+        w[2] = __float_as_uint(r.innerR); w[3] = __float_as_uint(r.bridgeR); w[4] = __float_as_uint(r.outerR);
+        w[5] = __float_as_uint(r.regR); w[6] = __float_as_uint(r.regG); w[7] = __float_as_uint(r.regF);
+        w[8] = __float_as_uint(r.rzChi2); w[9] = __float_as_uint(r.chi2); w[10] = __float_as_uint(r.nonAnchorChi2);
+        w[11] = r.tight ? 1u : 0u;
+        v.W5.resIdx[L] = k;
+      }
+    }
+  }
+}
+
+// stage 4: per module, passing candidates in candidate order take the slots
+__global__ __launch_bounds__(kT5Block) void k_t5_place(EventView v) {
+  __shared__ unsigned lds[kT5Block / 64 + 1];
+  if (!t5_flat_done(v)) return;
+  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
+  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const int layer = v.M.layers[m1];
+  const int cap = v.R.quintupletModuleOccupancy[m1];
+  const uint32_t off = v.W5.modOff[blockIdx.x], n = v.W5.modOff[blockIdx.x + 1] - off;
+  const uint32_t t5Base = (uint32_t)v.R.quintupletModuleIndices[m1];
+  unsigned count = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += kT5Block) {
+    const uint32_t l = c0 + threadIdx.x;
+    const uint32_t k = l < n ? v.W5.resIdx[off + l] : ~0u;
+    const bool pass = k != ~0u;
+    unsigned tot;
+    const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, tot);
+    if (pass && (int)rank < cap) {
+      const uint32_t* w = &v.W5.results[kT5ResultWords * k];
+      T5Result r;
+      r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
+      r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
+      r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
+      r.tight = w[11] != 0;
+      t5_store(v, t5Base + rank, w[0], w[1], r, layer);
+    }
+    count += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.R.totOccupancyQuintuplets[m1] = count;
+    v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
+  }
+}
+
+// Per-module builder, used when an event has more than kT5CandMax candidates
+// or more than kT5ResultMax passing ones:
+// one 256-lane work-group per eligible module walks its candidates in order.
 __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
   __shared__ uint32_t pre[kMaxInnerT3 + 1];
   __shared__ unsigned lds[kT5Block / 64 + 1];
+  if (t5_flat_done(v)) return;
   if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
   const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
   const int layer = v.M.layers[m1];
@@ -156,9 +427,7 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
   const uint32_t nIn = v.R.nTriplets[m1];
   unsigned count = 0;
   if ((layer == 1 || layer == 2) && nIn <= (uint32_t)kMaxInnerT3) {
-    const int adj = layer == 1 ? 1 : 0;
     const uint32_t t3Base = v.R.tripletModuleIndices[m1];
-    // pre[a] = number of candidates before inner triplet a
     uint32_t run = 0;
     for (uint32_t a0 = 0; a0 < nIn; a0 += kT5Block) {
       const uint32_t a = a0 + threadIdx.x;
@@ -196,43 +465,7 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
       }
       unsigned n;
       const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, n);
-      if (pass && (int)rank < cap) {
-        const uint32_t q = t5Base + rank;
-        // addQuintupletToMemory (SDL/Quintuplet.h:150-215)
-        const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * innerT3 + adj]];
-        v.Q.tripletIndices[2 * q] = innerT3;
-        v.Q.tripletIndices[2 * q + 1] = outerT3;
-#pragma unroll
-        for (int j = 0; j < 5; j++) v.Q.lowerModuleIndices[5 * q + j] = mods[j];
-        v.Q.innerRadius[q] = r.innerR;
-        v.Q.outerRadius[q] = r.outerR;
-        v.Q.bridgeRadius[q] = r.bridgeR;
-        v.Q.pt[q] = (r.innerR + r.outerR) * 3.8f * 1.602f / (2 * 100 * 5.39f);
-        v.Q.eta[q] = v.D.anchorEta[mdAdj];
-        v.Q.phi[q] = v.D.anchorPhi[mdAdj];
-        v.Q.score_rphisum[q] = r.chi2 + r.nonAnchorChi2;
-        v.Q.layer[q] = (uint8_t)layer;
-        v.Q.isDup[q] = 0;
-        v.Q.TightCutFlag[q] = r.tight;
-        v.Q.partOfPT5[q] = 0;
-        v.Q.regressionRadius[q] = r.regR;
-        v.Q.regressionG[q] = r.regG;
-        v.Q.regressionF[q] = r.regF;
-        v.Q.rzChiSquared[q] = r.rzChi2;
-        v.Q.chiSquared[q] = r.chi2;
-        v.Q.nonAnchorChiSquared[q] = r.nonAnchorChi2;
-        v.Q.logicalLayers[5 * q + 0] = v.T.logicalLayers[3 * innerT3];
-        v.Q.logicalLayers[5 * q + 1] = v.T.logicalLayers[3 * innerT3 + 1];
-        v.Q.logicalLayers[5 * q + 2] = v.T.logicalLayers[3 * innerT3 + 2];
-        v.Q.logicalLayers[5 * q + 3] = v.T.logicalLayers[3 * outerT3 + 1];
-        v.Q.logicalLayers[5 * q + 4] = v.T.logicalLayers[3 * outerT3 + 2];
-#pragma unroll
-        for (int j = 0; j < 6; j++) v.Q.hitIndices[10 * q + j] = v.T.hitIndices[6 * innerT3 + j];
-#pragma unroll
-        for (int j = 0; j < 4; j++) v.Q.hitIndices[10 * q + 6 + j] = v.T.hitIndices[6 * outerT3 + 2 + j];
-        v.T.partOfT5[innerT3] = 1;
-        v.T.partOfT5[outerT3] = 1;
-      }
+      if (pass && (int)rank < cap) t5_store(v, t5Base + rank, innerT3, outerT3, r, layer);
       count += n;
     }
   }
@@ -263,8 +496,12 @@ __global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
     bool dup = false;
+    const float si = sScore[i];
     for (uint32_t j = 0; j < nq && !dup; j++) {
       if (j == i) continue;
+      // the pair can only flag i if the score order points at i
+      const bool targetsI = i < j ? (si >= sScore[j]) : !(sScore[j] >= si);
+      if (!targetsI) continue;
       const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
       const float dEta = m_abs(sEta[lo] - sEta[hi]);
       const float dPhi = wrap_dphi(sPhi[lo], sPhi[hi]);
@@ -279,7 +516,7 @@ __global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
             break;
           }
       }
-      if (matched >= 7) dup = (sScore[lo] >= sScore[hi]) ? (i == lo) : (i == hi);
+      dup = matched >= 7;
     }
     if (dup) v.Q.isDup[base + i] = 1;
   }
@@ -304,10 +541,17 @@ __global__ void k_t5_ranges(EventView v) {
 
 void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
+  const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t5_cand_count, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t5_filter_md, dim3(flatGrid), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
   t.begin(s);
-  hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_t5_final, dim3(flatGrid), dim3(256), 0, s, v);
   t.end(s);
+  hipLaunchKernelGGL(k_t5_place, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
   hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks), dim3(kT5Block), 0, s, v);
   const unsigned g = (v.M.nLowerModules + 255) / 256;
   hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
