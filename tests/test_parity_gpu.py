@@ -146,3 +146,18 @@ def test_output_lists_match(engine, oracle_geometry):
     assert np.array_equal(out["tc_len"], ref.field("out.len")[:n])
     assert np.array_equal(out["tc_seedIdx"], ref.field("out.seedIdx")[:n])
     assert np.array_equal(out["tc_trackCandidateType"], ref.field("out.trackCandidateType")[:n])
+
+
+def test_per_module_builders_match(oracle_geometry, geom_dir, monkeypatch):
+    """The per-module T3/T5 builders (taken when an event overflows the
+    flattened path's lists) give the same objects as the oracle."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    from oracle import orc
+    monkeypatch.setenv("LST_PER_MODULE_BUILDERS", "1")
+    es = lst.LSTESData(geom_dir)
+    ev = synth.generate_event("pu", 7, n_pu=100)
+    prod = lst.Event(es).run_lst(ev, DEEPEST)
+    prod.synchronize()
+    ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
+    compare(prod, ref, DEEPEST)

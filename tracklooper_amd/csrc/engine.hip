@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -206,6 +207,12 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     e.pixSizes[k] = g.pixSizes[k];
   }
   e.tcPlsTriplets = 0;
+  // LST_PER_MODULE_BUILDERS=1 selects the per-module T3/T5 builders (same
+  // results; they are the path taken when an event overflows the flattened one)
+  {
+    const char* pm = std::getenv("LST_PER_MODULE_BUILDERS");
+    e.perModuleBuilders = (pm && pm[0] == '1') ? 1 : 0;
+  }
   e.pixelModule = g.pixelModuleIndex;
   e.ptCut = es->ptCut;
 

@@ -31,6 +31,7 @@ struct EventView {  // everything a stage kernel needs, by value
   const uint32_t* pixIndex[3];       // pixel map: connected-module start per superbin, by pLS type
   const uint32_t* pixSizes[3];
   int tcPlsTriplets;                 // TC_PLS_TRIPLETS: triplet pLS may become track candidates
+  int perModuleBuilders;             // run the per-module T3/T5 builders instead of the flattened ones
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };

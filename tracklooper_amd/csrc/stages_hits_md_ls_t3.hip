@@ -405,7 +405,7 @@ __device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_
 // owns [modOff[m1], modOff[m1+1]), inner segment a owns
 // [candPre[a], candPre[a] + nSegments(m2(a))) within it.
 __device__ __forceinline__ bool t3_flat_mode(const EventView& v) {
-  return v.R.scalars[kT3CandTotal] <= kT3CandMax;
+  return !v.perModuleBuilders && v.R.scalars[kT3CandTotal] <= kT3CandMax;
 }
 __device__ __forceinline__ bool t3_flat_done(const EventView& v) {
   return t3_flat_mode(v) && v.R.scalars[kT3Results] <= kT3ResultMax;

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(1024) void k_t5_cand_scan(EventView v) {
 }
 
 __device__ __forceinline__ bool t5_flat_mode(const EventView& v) {
-  return v.R.scalars[kT5CandTotal] <= kT5CandMax;
+  return !v.perModuleBuilders && v.R.scalars[kT5CandTotal] <= kT5CandMax;
 }
 // the flattened path completed: every candidate fitted and every result was parked
 __device__ __forceinline__ bool t5_flat_done(const EventView& v) {
