@@ -22,17 +22,18 @@ sys.path.insert(0, ROOT)
 METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC set"
 STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets",
                "pixel_segment_cleaning", "pixel_quintuplets", "pixel_triplets", "track_candidates"]
-# the kernel each stage timer brackets
-KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_triplets", "k_quintuplets",
-                "k_pls_dup", "k_pixel_quintuplets", "k_pixel_triplets", "k_t5_dup_before_tc"]
+# the kernel each timer slot brackets (lst_event_kernel_times)
+KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_t3_select", "k_t5_final",
+                "k_pls_pairs<false>", "k_pixel_quintuplets<false>", "k_pixel_triplets<false>", "k_t5_tc_pairs",
+                "k_pt5_pairs", "k_pt3_pairs", "k_cross_pls_pairs", "k_pls_pairs<true>", "k_t5_dup_after_build",
+                "k_t3_filter_md"]
 ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
 LAST_STAGE = 9  # the full chain up to track candidates
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
-MEASURED_TRAFFIC = {"k_quintuplets": 55815210, "k_segments": 58299420, "k_triplets": 41180815,
-                    "k_mini_doublets": 36042844, "k_hits": 10494688, "k_t5_dup_after_build": 1343613}
+MEASURED_TRAFFIC = {"k_segments": 58299420, "k_mini_doublets": 36042844}
 
 
 def _input_struct(lst, ev):
@@ -48,31 +49,44 @@ def _input_struct(lst, ev):
 
 
 def algorithmic_bytes(counts: dict) -> dict:
-    """Minimal HBM bytes per launch of each builder kernel: inputs read once, outputs written once.
+    """Minimal HBM bytes per launch of each timed kernel: inputs read once, outputs written once.
 
     Record sizes are the fields each kernel reads/writes:
-      hits kernel  : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
-      MD builder   : read x,y,z,rt of the module-pair hits (16 B/hit), write 134 B/MD
-      LS builder   : read 44 B of each mini-doublet, write 44 B/segment
-      T3 builder   : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
-      T5 builder   : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
-      pLS cleaning : read eta, phi, score, isQuad and 4 hit indices (29 B/pLS), write 1 B/pLS
-      pT5 builder  : read 60 B/pLS and 70 B/quintuplet, write 140 B/pixel quintuplet
-      pT3 builder  : read 60 B/pLS and 53 B/triplet, write 120 B/pixel triplet
-      T5 cleaning  : read eta, phi, score, flags and 10 hit indices (54 B/quintuplet), write 1 B/quintuplet
+      k_hits             : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
+      k_pixel_segments   : read 21 floats and 4 hit indices (100 B/pLS), write 2 MDs + 1 segment + pLS (400 B/pLS)
+      k_mini_doublets    : read x,y,z,rt of the module-pair hits (16 B/hit), write 134 B/MD
+      k_segments         : read 44 B of each mini-doublet, write 44 B/segment
+      k_t3_filter_md     : per candidate pair 18 B (prefix, module, two MD indices, result index), 12 B/survivor
+      k_t3_select        : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
+      k_t5_final         : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
+      k_pls_pairs        : read eta, phi, score, isQuad, flag and 4 hit indices (30 B/pLS), write 1 B/pLS
+      k_pixel_quintuplets: read 60 B/pLS and 70 B/quintuplet, write a 4 B count per pLS
+      k_pixel_triplets   : read 60 B/pLS and 53 B/triplet, write a 4 B count per pLS
+      k_pt5_pairs        : read score and 14 hit indices (60 B/pT5), write 1 B/pT5
+      k_pt3_pairs        : read score, layer and 10 hit indices (45 B/pT3), write 1 B/pT3
+      k_t5_tc_pairs,
+      k_t5_dup_after_build: read eta, phi, score, flags and 10 hit indices (54 B/quintuplet), write 1 B
+      k_cross_pls_pairs  : read 50 B/pLS (direction, flags, hit indices through the MDs), 25 B/track candidate
     """
     h, md, ls, t3, t5 = counts["hits"], counts["mds"], counts["segs"], counts["t3"], counts["t5"]
-    pls, pt5, pt3 = counts["pls"], counts["pt5"], counts["pt3"]
+    pls, pt5, pt3, tc = counts["pls"], counts["pt5"], counts["pt3"], counts["tc"]
     return {
         "k_hits": 46 * h,
+        "k_pixel_segments": 500 * pls,
         "k_mini_doublets": 16 * h + 134 * md,
         "k_segments": 44 * md + 44 * ls,
-        "k_triplets": 22 * ls + 20 * md + 60 * t3,
-        "k_quintuplets": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
-        "k_pls_dup": 30 * pls,
-        "k_pixel_quintuplets": 60 * pls + 70 * t5 + 140 * pt5,
-        "k_pixel_triplets": 60 * pls + 53 * t3 + 120 * pt3,
-        "k_t5_dup_before_tc": 55 * t5,
+        "k_t3_filter_md": 18 * counts.get("t3cand", 0) + 12 * counts.get("t3surv", 0),
+        "k_t3_select": 22 * ls + 20 * md + 60 * t3,
+        "k_t5_final": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
+        "k_pls_pairs<false>": 31 * pls,
+        "k_pls_pairs<true>": 31 * pls,
+        "k_pixel_quintuplets<false>": 64 * pls + 70 * t5,
+        "k_pixel_triplets<false>": 64 * pls + 53 * t3,
+        "k_t5_tc_pairs": 55 * t5,
+        "k_pt5_pairs": 61 * pt5,
+        "k_pt3_pairs": 46 * pt3,
+        "k_cross_pls_pairs": 50 * pls + 25 * tc,
+        "k_t5_dup_after_build": 55 * t5,
     }
 
 
@@ -174,7 +188,9 @@ def main():
                        "pls": int(lst.lib().lst_event_count(ev.h, b"pixelSegments")),
                        "pt5": int(lst.lib().lst_event_count(ev.h, b"pixelQuintuplets")),
                        "pt3": int(lst.lib().lst_event_count(ev.h, b"pixelTriplets")),
-                       "tc": int(lst.lib().lst_event_count(ev.h, b"trackCandidates"))})
+                       "tc": int(lst.lib().lst_event_count(ev.h, b"trackCandidates")),
+                       "t3cand": int(lst.lib().lst_event_count(ev.h, b"tripletCandidates")),
+                       "t3surv": int(lst.lib().lst_event_count(ev.h, b"tripletSharedMDCandidates"))})
     lst._check(L.lst_event_set_timing(ev.h, 0))
     avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
     per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}

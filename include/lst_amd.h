@@ -116,7 +116,10 @@ lst_status lst_event_synchronize(lst_event* ev);
 /* Replaces SDL::Event::getNumberOf* (SDL/Event.h:152-183): "hits", "pixelSegments",
  * "miniDoublets", "segments", "triplets", "quintuplets" (outer-tracker totals),
  * "pixelQuintuplets", "pixelTriplets", "trackCandidates", "pT5TrackCandidates",
- * "pT3TrackCandidates", "T5TrackCandidates", "pLSTrackCandidates". */
+ * "pT3TrackCandidates", "T5TrackCandidates", "pLSTrackCandidates".  Diagnostics
+ * with no counterpart in the reference: "tripletCandidates" (segment pairs the
+ * triplet builder enumerates), "tripletSharedMDCandidates" (those sharing the
+ * middle mini-doublet), "quintupletCandidates" (triplet pairs enumerated). */
 uint32_t lst_event_count(lst_event* ev, const char* what);
 
 /* Replaces the host-copy getters SDL::Event::get{Hits,MiniDoublets,Segments,
@@ -157,12 +160,16 @@ lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t sl
  * (asynchronous on the event's stream; same results as lst_run). */
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 
-/* Per-kernel device timing of the builder kernels (hipEvents on the event's
- * stream).  After a run with timing on, lst_event_kernel_times writes the
- * durations in ms of: 0 hits, 1 pixel segments, 2 mini-doublets,
- * 3 segments, 4 triplets, 5 quintuplets, 6 pLS cleaning, 7 pixel
- * quintuplets, 8 pixel triplets, 9 T5 duplicate removal before the track
- * candidates (n <= 16 entries; synchronises the stream). */
+/* Per-kernel device timing (hipEvents on the event's stream).  After a run
+ * with timing on, lst_event_kernel_times writes the durations in ms of the
+ * last launch of: 0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,
+ * 4 triplet selection, 5 quintuplet selection, 6 pLS cleaning (first pass),
+ * 7 pixel-quintuplet candidates, 8 pixel-triplet candidates, 9 T5 duplicate
+ * removal before the track candidates, 10 pT5 duplicate removal, 11 pT3
+ * duplicate removal, 12 pLS cross cleaning, 13 pLS cleaning (second pass),
+ * 14 T5 duplicate removal after build, 15 triplet shared-MD filter, 16 T5
+ * DNN (LST_T5_DNN mode only); 0 for a kernel the run did not reach
+ * (n <= 17 entries; synchronises the stream). */
 lst_status lst_event_set_timing(lst_event* ev, int on);
 lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n);
 /* The hipStream_t the event runs on. */

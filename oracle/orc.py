@@ -47,6 +47,10 @@ def lib():
         L.orc_event_field.argtypes = L.orc_geometry_field.argtypes
         L.orc_event_scalar.argtypes = [C.c_void_p, C.c_char_p]
         L.orc_event_scalar.restype = C.c_uint
+        L.orc_t5dnn_forward.argtypes = [C.c_void_p]
+        L.orc_t5dnn_forward.restype = C.c_float
+        L.orc_t5dnn_features.argtypes = [C.c_void_p, C.c_uint, C.c_void_p]
+        L.orc_t5dnn_features.restype = C.c_int
         _lib = L
     return _lib
 
@@ -110,6 +114,13 @@ class Event:
 
     def scalar(self, name):
         return lib().orc_event_scalar(self.h, name.encode())
+
+    def t5dnn_features(self, t5: int) -> np.ndarray:
+        """The 38 DNN inputs of stored quintuplet slot t5 (SDL/NeuralNetwork.h:36-114)."""
+        out = np.zeros(38, np.float32)
+        if lib().orc_t5dnn_features(self.h, t5, out.ctypes.data_as(C.c_void_p)) != 0:
+            raise IndexError(t5)
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

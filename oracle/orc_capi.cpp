@@ -277,6 +277,22 @@ float orc_regression_circle(unsigned n, const float* xs, const float* ys, const 
   for (unsigned i = 0; i < n && i < 16; i++) fl[i] = flat[i] != 0;
   return orc::regressionCircle(n, xs, ys, d1, d2, slopes, fl, *g, *f, *chi2);
 }
+// T5 DNN forward pass on 38 inputs (SDL/NeuralNetwork.h:116-162)
+float orc_t5dnn_forward(const float* in) { return orc::t5dnnForward(in); }
+// the 38 DNN inputs of stored quintuplet slot t5 (SDL/NeuralNetwork.h:36-114); -1 if out of range
+int orc_t5dnn_features(void* hp, unsigned t5, float* out) {
+  const orc::Event& ev = static_cast<Handle*>(hp)->ev;
+  if (2 * (size_t)t5 + 1 >= ev.t5TripletIndices.size()) return -1;
+  const unsigned in = ev.t5TripletIndices[2 * t5], ou = ev.t5TripletIndices[2 * t5 + 1];
+  const unsigned s1 = ev.t3SegmentIndices[2 * in], s2 = ev.t3SegmentIndices[2 * in + 1];
+  const unsigned s3 = ev.t3SegmentIndices[2 * ou], s4 = ev.t3SegmentIndices[2 * ou + 1];
+  const unsigned md[5] = {ev.segMdIndices[2 * s1], ev.segMdIndices[2 * s2], ev.segMdIndices[2 * s2 + 1],
+                          ev.segMdIndices[2 * s3 + 1], ev.segMdIndices[2 * s4 + 1]};
+  uint16_t mods[5];
+  for (int k = 0; k < 5; k++) mods[k] = ev.t5LowerModuleIndices[5 * t5 + k];
+  orc::t5dnnFeatures(ev, md, mods, in, ev.t5InnerRadius[t5], ev.t5OuterRadius[t5], ev.t5BridgeRadius[t5], out);
+  return 0;
+}
 float orc_circle_chi2(unsigned n, const float* xs, const float* ys, const float* d1, const float* d2,
                       const float* slopes, const unsigned char* flat, float g, float f, float r) {
   bool fl[16];

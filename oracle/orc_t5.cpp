@@ -826,7 +826,88 @@ bool algoSelector(const Event& ev, const T4In& q) {
 
 }  // namespace
 
-// SDL/Quintuplet.h:2508-2986 (plain mode: rz chi2 + r-phi chi2)
+// ---- T5 DNN (USE_T5_DNN): SDL/NeuralNetwork.h:21-165 ------------------------
+// Parameters: the float tables of SDL/NeuralNetworkWeights.h, kept as data
+// under tracklooper_amd/data/t5dnn (row-major wgtT_k[in][out]).
+namespace {
+const float kDnnBias0[32] = {
+#include "../tracklooper_amd/data/t5dnn/bias_0.inc"
+};
+const float kDnnW0[38 * 32] = {
+#include "../tracklooper_amd/data/t5dnn/wgtT_0.inc"
+};
+const float kDnnBias2[32] = {
+#include "../tracklooper_amd/data/t5dnn/bias_2.inc"
+};
+const float kDnnW2[32 * 32] = {
+#include "../tracklooper_amd/data/t5dnn/wgtT_2.inc"
+};
+const float kDnnBias4[1] = {
+#include "../tracklooper_amd/data/t5dnn/bias_4.inc"
+};
+const float kDnnW4[32] = {
+#include "../tracklooper_amd/data/t5dnn/wgtT_4.inc"
+};
+constexpr float kLSTWP2 = 0.6177366f;       // SDL/Constants.h:145
+constexpr float kMagneticField = 3.8112f;   // SDL/Constants.h:135
+}  // namespace
+
+// Linear(38,32) ReLU Linear(32,32) ReLU Linear(32,1) sigmoid; each output is
+// accumulated over the inputs in index order, then the bias is added
+// (SDL/NeuralNetwork.h:116-162)
+float t5dnnForward(const float in[38]) {
+  float h1[32], h2[32];
+  for (int c = 0; c < 32; c++) {
+    float acc = 0;
+    for (int i = 0; i < 38; i++) acc += in[i] * kDnnW0[32 * i + c];
+    acc += kDnnBias0[c];
+    h1[c] = acc > 0.f ? acc : 0.f;
+  }
+  for (int c = 0; c < 32; c++) {
+    float acc = 0;
+    for (int i = 0; i < 32; i++) acc += h1[i] * kDnnW2[32 * i + c];
+    acc += kDnnBias2[c];
+    h2[c] = acc > 0.f ? acc : 0.f;
+  }
+  float out = 0;
+  for (int i = 0; i < 32; i++) out += h2[i] * kDnnW4[i];
+  out += kDnnBias4[0];
+  return fexp(out) / (fexp(out) + 1);  // NaN once exp overflows, as in the reference
+}
+
+// the 38 inputs (SDL/NeuralNetwork.h:36-114): per anchor hit eta, phi, z, r
+// and layer (+6 in the endcap) for the inner T3 hits 1-3 and the outer T3 hits
+// 3-5, log10 of the T3/T5 transverse momenta, T5 eta/phi and log10 radii
+void t5dnnFeatures(const Event& ev, const unsigned md[5], const uint16_t mods[5], unsigned innerT3, float innerRadius,
+                   float outerRadius, float bridgeRadius, float out[38]) {
+  const Geometry& G = *ev.g;
+  auto hitBlock = [&](int k, float* o) {
+    const float x = ev.anchorX[md[k]], y = ev.anchorY[md[k]];
+    o[0] = ev.anchorEta[md[k]];
+    o[1] = ev.anchorPhi[md[k]];
+    o[2] = ev.anchorZ[md[k]];
+    o[3] = fsqrt(x * x + y * y);
+    o[4] = float(G.layers[mods[k]] + 6 * (G.subdets[mods[k]] == Endcap));
+  };
+  const int adj = G.layers[mods[0]] == 1 ? 1 : 0;
+  const unsigned mdEtaPhi = ev.segMdIndices[2 * ev.t3SegmentIndices[2 * innerT3 + adj]];
+  out[0] = temp_log10(2 * k2Rinv1GeVf * innerRadius);
+  hitBlock(0, out + 1);
+  hitBlock(1, out + 6);
+  hitBlock(2, out + 11);
+  out[16] = temp_log10(2 * k2Rinv1GeVf * outerRadius);
+  hitBlock(2, out + 17);
+  hitBlock(3, out + 22);
+  hitBlock(4, out + 27);
+  out[32] = temp_log10((innerRadius + outerRadius) * kMagneticField * 1.602f / (2 * 100 * 5.39f));
+  out[33] = ev.anchorEta[mdEtaPhi];
+  out[34] = ev.anchorPhi[mdEtaPhi];
+  out[35] = temp_log10(innerRadius);
+  out[36] = temp_log10(bridgeRadius);
+  out[37] = temp_log10(outerRadius);
+}
+
+// SDL/Quintuplet.h:2508-2986: rz chi2 and, by mode, the r-phi chi2 cut (0) or the T5 DNN (1)
 bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned innerT3, unsigned outerT3, T5Out& o) {
   const Geometry& G = *ev.g;
   unsigned seg1 = ev.t3SegmentIndices[2 * innerT3], seg2 = ev.t3SegmentIndices[2 * innerT3 + 1];
@@ -885,6 +966,13 @@ bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned 
   computeSigmasForRegression(G, mods, delta1, delta2, slopes, isFlat);
   o.regressionRadius = computeRadiusUsingRegression(5, x, y, delta1, delta2, slopes, isFlat, o.regressionG,
                                                     o.regressionF, sigmas, o.chiSquared);
+  if (ev.cfg.t5Mode == 1) {  // USE_T5_DNN: SDL/Quintuplet.h:2921-2941
+    float in[38];
+    t5dnnFeatures(ev, md, mods, innerT3, o.innerRadius, o.outerRadius, o.bridgeRadius, in);
+    const float score = t5dnnForward(in);
+    o.tightCutFlag = o.tightCutFlag && score > kLSTWP2;
+    if (!(score > kLSTWP2)) return false;
+  }
   if (ev.cfg.t5Mode == 0) {  // USE_RPHICHI2: SDL/Quintuplet.h:2943-2956
     if (o.regressionRadius < 5.0f / (2.f * k2Rinv1GeVf)) {
       if (!passChiSquaredConstraint(G.sdlLayers[mods[0]], G.sdlLayers[mods[1]], G.sdlLayers[mods[2]],

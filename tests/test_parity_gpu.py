@@ -14,12 +14,12 @@ def engine(geom_dir):
     return lst, es
 
 
-def _run_both(engine, oracle_geometry, ev, upto):
+def _run_both(engine, oracle_geometry, ev, upto, t5_mode=0):
     from oracle import orc
     lst, es = engine
-    prod = lst.Event(es).run_lst(ev, upto)
+    prod = lst.Event(es, t5_mode=t5_mode).run_lst(ev, upto)
     prod.synchronize()
-    ref = orc.Event(oracle_geometry).run_lst(ev, upto)
+    ref = orc.Event(oracle_geometry, t5_mode=t5_mode).run_lst(ev, upto)
     return prod, ref
 
 
@@ -122,10 +122,14 @@ def test_kernel_timing_does_not_disturb_runs(engine):
     ev = synth.generate_event("pu", 4, n_pu=50)
     lst._check(L.lst_event_set_timing(e.h, 1))
     e.run_lst(ev, DEEPEST)
-    n_timed = lst.STAGES[DEEPEST] + 1
     ms = np.zeros(16, np.float32)
     lst._check(L.lst_event_kernel_times(e.h, ms.ctypes.data_as(C.c_void_p), 16))
-    assert np.all(ms[:n_timed] > 0) and np.all(ms[n_timed:] == 0)
+    assert np.all(ms > 0)  # the full chain brackets all 16 timed kernels
+    e2 = lst.Event(es)
+    lst._check(L.lst_event_set_timing(e2.h, 1))
+    e2.run_lst(ev, "md")  # hits, pixel segments and mini-doublets only
+    lst._check(L.lst_event_kernel_times(e2.h, ms.ctypes.data_as(C.c_void_p), 16))
+    assert np.all(ms[:3] > 0) and np.all(ms[3:] == 0)
     e.run_lst(ev, DEEPEST)  # must not report a stale runtime error
     lst._check(L.lst_event_set_timing(e.h, 0))
     e.run_lst(ev, DEEPEST)
@@ -160,4 +164,28 @@ def test_per_module_builders_match(oracle_geometry, geom_dir, monkeypatch):
     prod = lst.Event(es).run_lst(ev, DEEPEST)
     prod.synchronize()
     ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
+    compare(prod, ref, DEEPEST)
+
+
+@pytest.mark.parametrize("case", [("muon", 0, 0), ("muon", 1, 0), ("pu", 0, 50), ("pu", 1, 200)])
+def test_t5_dnn_mode_all_stages(engine, oracle_geometry, case):
+    """DNN selection of the quintuplets (USE_RZCHI2 + USE_T5_DNN) through the whole chain."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    kind, index, n_pu = case
+    ev = synth.generate_event(kind, index, n_pu=n_pu) if kind == "pu" else synth.generate_event(kind, index)
+    prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST, t5_mode=lst.LST_T5_DNN)
+    rep = compare(prod, ref, DEEPEST)
+    assert ref.field("t5.nQuintuplets").sum() > 0
+    print(f"DNN mode T5={ref.field('t5.nQuintuplets').sum()} TC={ref.scalar('nTrackCandidates')}; "
+          f"floats compared {rep.float_compared}, not bit-identical {rep.float_inexact}")
+
+
+def test_t5_dnn_mode_per_module_builder(engine, oracle_geometry, monkeypatch):
+    """The per-module quintuplet builder evaluates the DNN inline: same results."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    monkeypatch.setenv("LST_PER_MODULE_BUILDERS", "1")
+    ev = synth.generate_event("pu", 2, n_pu=50)
+    prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST, t5_mode=lst.LST_T5_DNN)
     compare(prod, ref, DEEPEST)

@@ -25,6 +25,7 @@ SOURCES = [
     "stages_hits_md_ls_t3.hip",
     "stages_t5.hip",
     "stages_pixel.hip",
+    "stages_clean.hip",
     "engine.hip",
 ]
 

@@ -92,14 +92,13 @@ struct lst_event {
   float* plsDPhiChange = nullptr;
   std::vector<std::pair<std::string, FieldRef>> fields;
   lst::PreparedEvent prepared;
-  // timing of the main kernel of each stage
+  // timing of the hot kernels (slots of lst::TimerId)
   bool timing = false;
-  lst::KernelTimer timers[16];
-  uint32_t recorded = 0;  // timers recorded since timing was switched on
-  lst::KernelTimer timer(int k) {
-    if (!timing) return lst::KernelTimer{};
-    recorded |= 1u << k;
-    return timers[k];
+  lst::KernelTimers timers;
+  uint32_t recorded = 0;  // timer slots recorded since timing was switched on
+  const lst::KernelTimers& tm() {
+    timers.recorded = timing ? &recorded : nullptr;
+    return timers;
   }
   // device-resident prepared inputs (throughput path)
   struct Staged {
@@ -170,6 +169,9 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5Count, plsCap + 1); c.take(e.W.pt3Count, plsCap + 1);
   c.take(e.W.t5TcCount, nL + 1);
   c.take(e.W.outHits, (size_t)tcCap * 14); c.take(e.W.outLen, tcCap);
+  c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1);
+  c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
+  c.take(e.W.tcPix, 4 * (size_t)tcCap);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
   c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3ResultMax);
@@ -202,6 +204,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   e.t3OccIfAny = g.t3OccIfAny;
   e.t5OccIfAny = g.t5OccIfAny;
   e.nT5SeedModules = g.nT5SeedModules;
+  e.t5Cap = t5Cap ? t5Cap : 1;
   for (int k = 0; k < 3; k++) {
     e.pixIndex[k] = g.pixIndex[k];
     e.pixSizes[k] = g.pixSizes[k];
@@ -213,6 +216,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     const char* pm = std::getenv("LST_PER_MODULE_BUILDERS");
     e.perModuleBuilders = (pm && pm[0] == '1') ? 1 : 0;
   }
+  e.t5Dnn = t5Mode == LST_T5_DNN ? 1 : 0;
   e.pixelModule = g.pixelModuleIndex;
   e.ptCut = es->ptCut;
 
@@ -332,7 +336,7 @@ lst_status lst_event::add_hits(const float* x, const float* y, const float* z, c
   }
   v.nHits = n;
   v.nPLS = 0;
-  lst::launch_hits(v, stream, timer(0));
+  lst::launch_hits(v, stream, tm());
   HIP_OK(hipGetLastError());
   stage = 0;
   return LST_OK;
@@ -375,7 +379,7 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
   }
   v.nPLS = n;
   const uint32_t* hi[4] = {plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]};
-  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream, timer(1));
+  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream, tm());
   HIP_OK(hipGetLastError());
   stage = 1;
   return warn;
@@ -423,6 +427,7 @@ uint32_t lst_es_scalar(const lst_es* es, const char* name) {
 lst_status lst_event_create(lst_es* es, void* hip_stream, int t5_mode, lst_event** out) {
   *out = nullptr;
   if (!es) return fail(LST_E_INPUT, "null event setup");
+  if (t5_mode != LST_T5_PLAIN_CHI2 && t5_mode != LST_T5_DNN) return fail(LST_E_INPUT, "unknown T5 selection mode");
   auto ev = std::make_unique<lst_event>();
   ev->es = es;
   ev->t5Mode = t5_mode;
@@ -444,9 +449,9 @@ void lst_event_destroy(lst_event* ev) {
   if (ev->arena) hipFree(ev->arena);
   for (auto& st : ev->staged)
     if (st.mem) hipFree(st.mem);
-  for (auto& t : ev->timers) {
-    if (t.start) hipEventDestroy(t.start);
-    if (t.stop) hipEventDestroy(t.stop);
+  for (int k = 0; k < lst::kNTimers; k++) {
+    if (ev->timers.start[k]) hipEventDestroy(ev->timers.start[k]);
+    if (ev->timers.stop[k]) hipEventDestroy(ev->timers.stop[k]);
   }
   if (ev->ownStream) hipStreamDestroy(ev->stream);
   delete ev;
@@ -465,7 +470,7 @@ lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments*
 
 lst_status lst_event_create_mini_doublets(lst_event* ev) {
   if (!ev || ev->stage != 1) return fail(LST_E_ORDER, "createMiniDoublets needs hits and pixel segments");
-  lst::launch_mini_doublets(ev->v, ev->stream, ev->timer(2));
+  lst::launch_mini_doublets(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 2;
   return LST_OK;
@@ -473,7 +478,7 @@ lst_status lst_event_create_mini_doublets(lst_event* ev) {
 
 lst_status lst_event_create_segments(lst_event* ev) {
   if (!ev || ev->stage != 2) return fail(LST_E_ORDER, "createSegmentsWithModuleMap needs mini-doublets");
-  lst::launch_segments(ev->v, ev->stream, ev->timer(3));
+  lst::launch_segments(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 3;
   return LST_OK;
@@ -481,7 +486,7 @@ lst_status lst_event_create_segments(lst_event* ev) {
 
 lst_status lst_event_create_triplets(lst_event* ev) {
   if (!ev || ev->stage != 3) return fail(LST_E_ORDER, "createTriplets needs segments");
-  lst::launch_triplets(ev->v, ev->stream, ev->timer(4));
+  lst::launch_triplets(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 4;
   return LST_OK;
@@ -489,8 +494,7 @@ lst_status lst_event_create_triplets(lst_event* ev) {
 
 lst_status lst_event_create_quintuplets(lst_event* ev) {
   if (!ev || ev->stage != 4) return fail(LST_E_ORDER, "createQuintuplets needs triplets");
-  if (ev->t5Mode != LST_T5_PLAIN_CHI2) return fail(LST_E_INPUT, "T5 DNN selection is not available in this build");
-  lst::launch_quintuplets(ev->v, ev->stream, ev->timer(5));
+  lst::launch_quintuplets(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 5;
   return LST_OK;
@@ -498,7 +502,7 @@ lst_status lst_event_create_quintuplets(lst_event* ev) {
 
 lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev) {
   if (!ev || ev->stage != 5) return fail(LST_E_ORDER, "pixelLineSegmentCleaning needs quintuplets");
-  lst::launch_pixel_segment_cleaning(ev->v, ev->stream, ev->timer(6));
+  lst::launch_pixel_segment_cleaning(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 6;
   return LST_OK;
@@ -506,7 +510,7 @@ lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev) {
 
 lst_status lst_event_create_pixel_quintuplets(lst_event* ev) {
   if (!ev || ev->stage != 6) return fail(LST_E_ORDER, "createPixelQuintuplets needs pixelLineSegmentCleaning");
-  lst::launch_pixel_quintuplets(ev->v, ev->stream, ev->timer(7));
+  lst::launch_pixel_quintuplets(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 7;
   return LST_OK;
@@ -514,7 +518,7 @@ lst_status lst_event_create_pixel_quintuplets(lst_event* ev) {
 
 lst_status lst_event_create_pixel_triplets(lst_event* ev) {
   if (!ev || ev->stage != 7) return fail(LST_E_ORDER, "createPixelTriplets needs pixel quintuplets");
-  lst::launch_pixel_triplets(ev->v, ev->stream, ev->timer(8));
+  lst::launch_pixel_triplets(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 8;
   return LST_OK;
@@ -522,7 +526,7 @@ lst_status lst_event_create_pixel_triplets(lst_event* ev) {
 
 lst_status lst_event_create_track_candidates(lst_event* ev) {
   if (!ev || ev->stage != 8) return fail(LST_E_ORDER, "createTrackCandidates needs pixel triplets");
-  lst::launch_track_candidates(ev->v, ev->stream, ev->timer(9));
+  lst::launch_track_candidates(ev->v, ev->stream, ev->tm());
   HIP_OK(hipGetLastError());
   ev->stage = 9;
   return LST_OK;
@@ -579,7 +583,8 @@ uint32_t lst_event_count(lst_event* ev, const char* what) {
         {"pixelQuintuplets", lst::kNPixelQuintuplets}, {"pixelTriplets", lst::kNPixelTriplets},
         {"trackCandidates", lst::kNTrackCandidates}, {"pT3TrackCandidates", lst::kNTrackCandidatespT3},
         {"pT5TrackCandidates", lst::kNTrackCandidatespT5}, {"pLSTrackCandidates", lst::kNTrackCandidatespLS},
-        {"T5TrackCandidates", lst::kNTrackCandidatesT5}};
+        {"T5TrackCandidates", lst::kNTrackCandidatesT5}, {"tripletCandidates", lst::kT3CandTotal},
+        {"tripletSharedMDCandidates", lst::kT3List}, {"quintupletCandidates", lst::kT5CandTotal}};
     for (const auto& e : sc)
       if (!std::strcmp(what, e.name)) {
         uint32_t val = 0;
@@ -696,10 +701,10 @@ lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
 
 lst_status lst_event_set_timing(lst_event* ev, int on) {
   if (!ev) return fail(LST_E_INPUT, "null event");
-  if (on && !ev->timers[0].start)
-    for (auto& t : ev->timers) {
-      HIP_OK(hipEventCreate(&t.start));
-      HIP_OK(hipEventCreate(&t.stop));
+  if (on && !ev->timers.start[0])
+    for (int k = 0; k < lst::kNTimers; k++) {
+      HIP_OK(hipEventCreate(&ev->timers.start[k]));
+      HIP_OK(hipEventCreate(&ev->timers.stop[k]));
     }
   ev->timing = on != 0;
   ev->recorded = 0;
@@ -708,13 +713,13 @@ lst_status lst_event_set_timing(lst_event* ev, int on) {
 
 lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n) {
   if (!ev || !ms) return fail(LST_E_INPUT, "null argument");
-  if (!ev->timers[0].start) return fail(LST_E_ORDER, "timing was never enabled");
+  if (!ev->timers.start[0]) return fail(LST_E_ORDER, "timing was never enabled");
   HIP_OK(hipStreamSynchronize(ev->stream));
-  for (int k = 0; k < n && k < 16; k++) {
+  for (int k = 0; k < n && k < lst::kNTimers; k++) {
     ms[k] = 0.f;
     // only timers recorded by a stage run hold a valid interval; querying the
     // others would leave an error behind for the next hipGetLastError
-    if ((ev->recorded >> k) & 1u) HIP_OK(hipEventElapsedTime(&ms[k], ev->timers[k].start, ev->timers[k].stop));
+    if ((ev->recorded >> k) & 1u) HIP_OK(hipEventElapsedTime(&ms[k], ev->timers.start[k], ev->timers.stop[k]));
   }
   return LST_OK;
 }

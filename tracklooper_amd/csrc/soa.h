@@ -123,6 +123,12 @@ struct PixelWork {
   uint32_t* t5TcCount;    // [nLower + 1] T5 track candidates per eligible module, then offsets
   uint32_t* outHits;      // [TC capacity * 14] output hit lists
   uint32_t* outLen;       // [TC capacity]
+  uint32_t* t5ListOff;    // [nLower] first list position of each eligible T5 module
+  uint32_t* t5List;       // [T5 capacity] T5 slots of the eligible modules
+  float* tcEta;           // [TC capacity] direction a pLS is compared with in crossCleanpLS
+  float* tcPhi;           // [TC capacity]
+  uint8_t* tcKind;        // [TC capacity] 0: not compared, 1: T5, 2: pT3 or pT5
+  int32_t* tcPix;         // [TC capacity * 4] ntuple indices of the pLS hits of a pT3/pT5
 };
 
 // Per-module counters and ranges (SDL/Module.h:21-85, SDL/Hit.h:28-32).
@@ -172,6 +178,7 @@ enum Scalar : int {
   kT3CandTotal = 17,
   kT3List = 18,
   kT3Results = 19,
+  kT5Listed = 20,
   kNScalars = 32
 };
 }  // namespace lst

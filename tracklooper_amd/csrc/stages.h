@@ -28,31 +28,73 @@ struct EventView {  // everything a stage kernel needs, by value
   const int32_t* t3OccIfAny;         // [nLower] T3 capacity of a module that has segments
   const int32_t* t5OccIfAny;         // [nLower] T5 capacity of a seed module, -1 if none
   uint32_t nT5SeedModules;           // geometry bound on the eligible T5 module count
+  uint32_t t5Cap;                    // T5 arena slots
   const uint32_t* pixIndex[3];       // pixel map: connected-module start per superbin, by pLS type
   const uint32_t* pixSizes[3];
   int tcPlsTriplets;                 // TC_PLS_TRIPLETS: triplet pLS may become track candidates
   int perModuleBuilders;             // run the per-module T3/T5 builders instead of the flattened ones
+  int t5Dnn;                         // T5 selection by the DNN (USE_T5_DNN) instead of the r-phi chi2 cut
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };
 
-// Optional hipEvent pair recorded around the stage's main kernel (timing).
-struct KernelTimer {
-  hipEvent_t start = nullptr, stop = nullptr;
-  void begin(hipStream_t s) const { if (start) (void)hipEventRecord(start, s); }
-  void end(hipStream_t s) const { if (stop) (void)hipEventRecord(stop, s); }
+// Optional hipEvent pairs recorded around the hot kernels (timing).  Slot k
+// brackets one kernel; `recorded` (null: timing off) collects the slots used.
+enum TimerId : int {
+  kTmHits = 0,          // k_hits
+  kTmPixelSegments,     // k_pixel_segments
+  kTmMiniDoublets,      // k_mini_doublets
+  kTmSegments,          // k_segments
+  kTmT3Select,          // k_t3_select
+  kTmT5Final,           // k_t5_final
+  kTmPlsDup,            // k_pls_pairs<false> (pLS cleaning, first pass)
+  kTmPT5Build,          // k_pixel_quintuplets<false>
+  kTmPT3Build,          // k_pixel_triplets<false>
+  kTmT5DupTC,           // k_t5_tc_pairs
+  kTmPT5Dup,            // k_pt5_pairs
+  kTmPT3Dup,            // k_pt3_pairs
+  kTmCrossPLS,          // k_cross_pls_pairs
+  kTmPlsDup2,           // k_pls_pairs<true> (second pass)
+  kTmT5DupAfter,        // k_t5_dup_after_build
+  kTmT3Filter,          // k_t3_filter_md
+  kTmT5Dnn,             // k_t5_dnn (DNN mode only)
+  kNTimers
 };
+struct KernelTimers {
+  hipEvent_t start[kNTimers] = {}, stop[kNTimers] = {};
+  uint32_t* recorded = nullptr;
+  void begin(hipStream_t s, int k) const {
+    if (!recorded) return;
+    (void)hipEventRecord(start[k], s);
+    *recorded |= 1u << k;
+  }
+  void end(hipStream_t s, int k) const {
+    if (recorded) (void)hipEventRecord(stop[k], s);
+  }
+};
+inline const KernelTimers& no_timers() {
+  static const KernelTimers off{};
+  return off;
+}
 
-void launch_hits(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
 void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
-                           const KernelTimer& t = {});
-void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimer& t = {});
-void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimer& t = {});
+                           const KernelTimers& t = no_timers());
+void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+// all-pairs duplicate removal and cross cleaning (stages_clean.hip)
+void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t);
+void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
+void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
+void launch_cross_pt3(const EventView& v, hipStream_t s);
+void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t);
+void launch_cross_t5(const EventView& v, hipStream_t s);
+void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t);
 
 }  // namespace lst

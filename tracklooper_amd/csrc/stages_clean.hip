@@ -1,0 +1,549 @@
+// Duplicate removal and cross cleaning as all-pairs kernels.
+//
+// Every rule of this kind in the reference sets a flag on an object x when
+// some partner y satisfies a predicate on the pair (pLS cleaning,
+// SDL/Kernels.h:384-473; pT5 and pT3 duplicates, :322-381; T5 duplicates
+// before the track candidates, :250-320; crossCleanpT3/T5/pLS,
+// SDL/TrackCandidate.h:204-379).  The predicates read only pair values and
+// flags written by earlier kernels, so the flag is an OR over partners and
+// the pairs can be visited in any order.  Each kernel tiles the (x, y) plane:
+// a 256-lane work-group owns 256 x's, stages 256 y's at a time in LDS and
+// strides over the y tiles with gridDim.y; flags are set with stores of a
+// constant or 32-bit atomic ORs, never read-modify-write races.
+#include "algo_pixel.h"
+#include "stages.h"
+#include "wave.h"
+
+namespace lst {
+namespace {
+
+constexpr int kTile = 256;
+
+// OR `bits` into byte k of an int8/uint8 flag array (256-byte aligned base)
+__device__ __forceinline__ void or_byte(void* base, uint32_t k, uint32_t bits) {
+  unsigned int* word = reinterpret_cast<unsigned int*>(static_cast<char*>(base) + (k & ~3u));
+  atomicOr(word, bits << (8 * (k & 3u)));
+}
+
+// The y range [0, ny) is cut into gridDim.y equal shares, one per blockIdx.y,
+// so that small object counts still spread over many work-groups.
+struct YShare {
+  uint32_t b, e;
+};
+__device__ __forceinline__ YShare y_share(uint32_t ny) {
+  const uint32_t chunk = (ny + gridDim.y - 1) / gridDim.y;
+  const uint32_t b = blockIdx.y * chunk;
+  const uint32_t e = b + chunk < ny ? b + chunk : ny;
+  return {b < e ? b : e, e};
+}
+__device__ __forceinline__ uint32_t tile_len(const YShare& r, uint32_t y0) {
+  return r.e - y0 < (uint32_t)kTile ? r.e - y0 : (uint32_t)kTile;
+}
+
+// x tiles cover nxCap (blocks past the device-side count exit at once); the y
+// shares are sized so that about 2048 work-groups hold work at nx = nxTyp
+inline dim3 pair_grid(uint32_t nxCap, uint32_t nxTyp, uint32_t nyCap) {
+  unsigned gx = (nxCap + kTile - 1) / kTile;
+  gx = gx == 0 ? 1 : gx;
+  unsigned ax = (nxTyp + kTile - 1) / kTile;
+  ax = ax == 0 ? 1 : ax;
+  unsigned gy = (2048 + ax - 1) / ax;
+  gy = gy > 256 ? 256 : gy;
+  gy = gy > nyCap ? (nyCap ? nyCap : 1) : gy;
+  return dim3(gx, gy);
+}
+
+// ------------------------------------------------------------ pLS cleaning
+// checkHitspLS: for a pair (lo < hi) within 0.1 in eta the removal target is
+// the triplet seed over the quadruplet, else the higher score (ties: lo); the
+// first pass flags it (bit 1) when >= 3 of lo's hits are in hi, the second
+// pass (quadruplets without bit 1 only) flags it (bit 2) on any shared hit or
+// dR2 < 1e-5.  x is the candidate target k, y the partner j.
+template <bool kSecond>
+__global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
+  __shared__ uint32_t sHits[4 * kTile];
+  __shared__ uint8_t sQuad[kTile], sOk[kTile];
+  const uint32_t n = v.nPLS;
+  const YShare ys = y_share(n);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t k = x0 + threadIdx.x;
+    bool active = k < n;
+    float etaK = 0, phiK = 0, scoreK = 0;
+    int quadK = 0;
+    uint32_t hk[4] = {0, 0, 0, 0};
+    if (active) {
+      quadK = v.P.isQuad[k] ? 1 : 0;
+      if (kSecond && (!quadK || (v.P.isDup[k] & 1))) active = false;
+      etaK = v.P.eta[k]; phiK = v.P.phi[k]; scoreK = v.P.score[k];
+#pragma unroll
+      for (int a = 0; a < 4; a++) hk[a] = v.P.pLSHitsIdxs[4 * k + a];
+    }
+    bool flag = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t jl = y0 + threadIdx.x;
+      if (jl < ys.e) {
+        sEta[threadIdx.x] = v.P.eta[jl]; sPhi[threadIdx.x] = v.P.phi[jl]; sScore[threadIdx.x] = v.P.score[jl];
+        sQuad[threadIdx.x] = v.P.isQuad[jl] ? 1 : 0;
+        sOk[threadIdx.x] = kSecond ? (v.P.isQuad[jl] && !(v.P.isDup[jl] & 1)) : 1;
+#pragma unroll
+        for (int a = 0; a < 4; a++) sHits[4 * threadIdx.x + a] = v.P.pLSHitsIdxs[4 * jl + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || flag) continue;
+      for (uint32_t t = 0; t < m && !flag; t++) {
+        const uint32_t j = y0 + t;
+        if (j == k) continue;
+        const bool kLo = k < j;
+        const float etaLo = kLo ? etaK : sEta[t], etaHi = kLo ? sEta[t] : etaK;
+        if (m_abs(etaHi - etaLo) > 0.1f) continue;
+        if (!sOk[t]) continue;
+        const int quadDiff = kLo ? quadK - (int)sQuad[t] : (int)sQuad[t] - quadK;
+        const float scoreDiff = kLo ? scoreK - sScore[t] : sScore[t] - scoreK;
+        // removal target: hi when quadDiff > 0 or (== 0 and scoreDiff < 0), else lo
+        const bool rmHi = quadDiff > 0 || (quadDiff == 0 && scoreDiff < 0);
+        if (rmHi == kLo) continue;  // the target is the partner, not k
+        int matched = 0;
+        for (int a = 0; a < 4; a++) {
+          const uint32_t h = kLo ? hk[a] : sHits[4 * t + a];
+          bool hit = false;
+          for (int b = 0; b < 4; b++)
+            if (h == (kLo ? sHits[4 * t + b] : hk[b])) { hit = true; break; }
+          if (hit) {
+            matched++;
+            if (kSecond) break;
+          }
+        }
+        if (!kSecond) {
+          flag = matched >= 3;
+        } else {
+          const float dEta = m_abs(etaLo - etaHi);
+          const float dPhi = kLo ? wrap_dphi(phiK, sPhi[t]) : wrap_dphi(sPhi[t], phiK);
+          flag = matched >= 1 || dEta * dEta + dPhi * dPhi < 1e-5f;
+        }
+      }
+    }
+    if (flag) or_byte(v.P.isDup, k, kSecond ? 2u : 1u);
+  }
+}
+
+// ------------------------------------------------------------ pT5 duplicates
+// removeDupPixelQuintupletsInGPUFromMap: ix is dropped when a partner shares
+// >= 7 of its 14 hits and scores lower (ties: the higher index is dropped)
+__global__ __launch_bounds__(kTile) void k_pt5_pairs(EventView v) {
+  __shared__ float sScore[kTile];
+  __shared__ uint32_t sHits[14 * kTile];
+  const uint32_t n = v.R.scalars[kNPixelQuintuplets];
+  const YShare ys = y_share(n);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t ix = x0 + threadIdx.x;
+    const bool active = ix < n;
+    uint32_t h[14];
+    float s1 = 0;
+    if (active) {
+#pragma unroll
+      for (int a = 0; a < 14; a++) h[a] = v.P5.hitIndices[14 * ix + a];
+      s1 = v.P5.score[ix];
+    }
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t jl = y0 + threadIdx.x;
+      if (jl < ys.e) {
+        sScore[threadIdx.x] = v.P5.score[jl];
+        for (int a = 0; a < 14; a++) sHits[threadIdx.x * 14 + a] = v.P5.hitIndices[14 * jl + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const uint32_t jx = y0 + t;
+        if (jx == ix) continue;
+        const float s2 = sScore[t];
+        if (!(s1 > s2 || (s1 == s2 && ix > jx))) continue;
+        int matched = 0;
+        for (int a = 0; a < 14 && matched < 7 && matched + (14 - a) >= 7; a++)
+          for (int b = 0; b < 14; b++)
+            if (h[a] == sHits[t * 14 + b]) {
+              matched++;
+              break;
+            }
+        dup = matched >= 7;
+      }
+    }
+    if (dup) v.P5.isDup[ix] = 1;
+  }
+}
+
+// ------------------------------------------------------------ pT3 duplicates
+// removeDupPixelTripletsInGPUFromMap with checkHitspT3: ix is dropped when a
+// partner shares >= 5 hits (pixel + outer) and reaches a lower third layer, or
+// the same layer with a lower score (ties: the lower index is dropped)
+__global__ __launch_bounds__(kTile) void k_pt3_pairs(EventView v) {
+  __shared__ float sScore[kTile];
+  __shared__ uint8_t sLayer[kTile];
+  __shared__ uint32_t sHits[10 * kTile];
+  const uint32_t n = v.R.scalars[kNPixelTriplets];
+  const YShare ys = y_share(n);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t ix = x0 + threadIdx.x;
+    const bool active = ix < n;
+    uint32_t h[10];
+    uint8_t li = 0;
+    float si = 0;
+    if (active) {
+#pragma unroll
+      for (int a = 0; a < 10; a++) h[a] = v.P3.hitIndices[10 * ix + a];
+      li = v.P3.logicalLayers[5 * ix + 2];
+      si = v.P3.score[ix];
+    }
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t jl = y0 + threadIdx.x;
+      if (jl < ys.e) {
+        sScore[threadIdx.x] = v.P3.score[jl];
+        sLayer[threadIdx.x] = v.P3.logicalLayers[5 * jl + 2];
+        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.P3.hitIndices[10 * jl + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const uint32_t jx = y0 + t;
+        if (jx == ix) continue;
+        const uint8_t lj = sLayer[t];
+        const float sj = sScore[t];
+        if (!(lj < li || (li == lj && si > sj) || (li == lj && si == sj && ix < jx))) continue;
+        int matched = 0;
+        for (int a = 0; a < 4; a++)
+          for (int b = 0; b < 4; b++)
+            if (h[a] == sHits[t * 10 + b]) {
+              matched++;
+              break;
+            }
+        for (int a = 4; a < 10; a++)
+          for (int b = 4; b < 10; b++)
+            if (h[a] == sHits[t * 10 + b]) {
+              matched++;
+              break;
+            }
+        dup = matched >= 5;
+      }
+    }
+    if (dup) v.P3.isDup[ix] = 1;
+  }
+}
+
+// ---------------------------------------------------------- crossCleanpT3
+// a pT3 whose pLS direction is within dR2 < 1e-5 of a pT5's pLS is dropped
+__global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile];
+  const uint32_t n = v.R.scalars[kNPixelTriplets], n5 = v.R.scalars[kNPixelQuintuplets];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const YShare ys = y_share(n5);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t p = x0 + threadIdx.x;
+    const bool active = p < n && v.P3.isDup[p] == 0;
+    const float eta1 = active ? v.P3.eta_pix[p] : 0.f, phi1 = active ? v.P3.phi_pix[p] : 0.f;
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t ql = y0 + threadIdx.x;
+      if (ql < ys.e) {
+        const uint32_t j = v.P5.pixelIndices[ql] - plsBase;
+        sEta[threadIdx.x] = v.P.eta[j];
+        sPhi[threadIdx.x] = v.P.phi[j];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const float dEta = m_abs(eta1 - sEta[t]);
+        const float dPhi = wrap_dphi(phi1, sPhi[t]);
+        dup = dEta * dEta + dPhi * dPhi < 1e-5f;
+      }
+    }
+    if (dup) v.P3.isDup[p] = 1;
+  }
+}
+
+// --------------------------------------------------- T5 list in slot order
+__global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t n = v.R.scalars[kNEligibleT5];
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < n; b += 1024) {
+    const uint32_t i = b + threadIdx.x;
+    const unsigned c = i < n ? v.R.nQuintuplets[v.R.indicesOfEligibleT5Modules[i]] : 0u;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (i < n) v.W.t5ListOff[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) v.R.scalars[kT5Listed] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
+  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
+  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const uint32_t nq = v.R.nQuintuplets[m];
+  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
+  const uint32_t off = v.W.t5ListOff[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) v.W.t5List[off + k] = base + k;
+}
+
+// ---------------------------------------------------- T5 duplicates before TC
+// Among T5s not in a pT5 and not flagged after build, a close pair
+// (|dEta|, |dPhi| <= 0.1, and dR2 < 0.001 or >= 5 shared hits) flags the
+// higher score (ties: the lower index) with bit 2.
+__global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
+  __shared__ uint32_t sSlot[kTile];
+  __shared__ uint8_t sOk[kTile];
+  __shared__ uint32_t sHits[10 * kTile];
+  const uint32_t n = v.R.scalars[kT5Listed];
+  const YShare ys = y_share(n);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t xi = x0 + threadIdx.x;
+    uint32_t x = 0;
+    bool active = xi < n;
+    float etaX = 0, phiX = 0, sX = 0;
+    uint32_t hx[10];
+    if (active) {
+      x = v.W.t5List[xi];
+      active = !v.Q.partOfPT5[x] && !(v.Q.isDup[x] & 1);
+      etaX = v.Q.eta[x]; phiX = v.Q.phi[x]; sX = v.Q.score_rphisum[x];
+#pragma unroll
+      for (int a = 0; a < 10; a++) hx[a] = v.Q.hitIndices[10 * x + a];
+    }
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t yl = y0 + threadIdx.x;
+      if (yl < ys.e) {
+        const uint32_t y = v.W.t5List[yl];
+        sSlot[threadIdx.x] = y;
+        sEta[threadIdx.x] = v.Q.eta[y]; sPhi[threadIdx.x] = v.Q.phi[y]; sScore[threadIdx.x] = v.Q.score_rphisum[y];
+        sOk[threadIdx.x] = !v.Q.partOfPT5[y] && !(v.Q.isDup[y] & 1);
+        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.Q.hitIndices[10 * y + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const uint32_t y = sSlot[t];
+        if (y == x || !sOk[t]) continue;
+        const float sY = sScore[t];
+        if (!(sX > sY || (!(sX < sY) && x < y))) continue;
+        const bool xLo = x < y;
+        const float dEta = m_abs(xLo ? etaX - sEta[t] : sEta[t] - etaX);
+        const float dPhi = xLo ? wrap_dphi(phiX, sPhi[t]) : wrap_dphi(sPhi[t], phiX);
+        if (dEta > 0.1f) continue;
+        if (m_abs(dPhi) > 0.1f) continue;
+        const float dR2 = dEta * dEta + dPhi * dPhi;
+        if (dR2 < 0.001f) {
+          dup = true;
+          break;
+        }
+        int matched = 0;
+        for (int a = 0; a < 10; a++) {
+          const uint32_t h = xLo ? hx[a] : sHits[t * 10 + a];
+          for (int b = 0; b < 10; b++)
+            if (h == (xLo ? sHits[t * 10 + b] : hx[b])) {
+              matched++;
+              break;
+            }
+        }
+        dup = matched >= 5;
+      }
+    }
+    if (dup) or_byte(v.Q.isDup, x, 2u);
+  }
+}
+
+// --------------------------------------------------------------- crossCleanT5
+// a T5 (not flagged, not in a pT5) within dR2 < 1e-3 of any pT5 or pT3 is dropped
+__global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile];
+  const uint32_t n = v.R.scalars[kT5Listed];
+  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets], n3 = v.R.scalars[kNPixelTriplets];
+  const uint32_t ny = n5 + n3;
+  const YShare ys = y_share(ny);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t xi = x0 + threadIdx.x;
+    uint32_t q = 0;
+    bool active = xi < n;
+    float eta1 = 0, phi1 = 0;
+    if (active) {
+      q = v.W.t5List[xi];
+      active = v.Q.isDup[q] == 0 && v.Q.partOfPT5[q] == 0;
+      eta1 = v.Q.eta[q];
+      phi1 = v.Q.phi[q];
+    }
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t j = y0 + threadIdx.x;
+      if (j < ys.e) {
+        sEta[threadIdx.x] = j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5];
+        sPhi[threadIdx.x] = j < n5 ? v.P5.phi[j] : v.P3.phi[j - n5];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const float dEta = m_abs(eta1 - sEta[t]);
+        const float dPhi = wrap_dphi(phi1, sPhi[t]);
+        dup = dEta * dEta + dPhi * dPhi < 1e-3f;
+      }
+    }
+    if (dup) v.Q.isDup[q] = 1;
+  }
+}
+
+// -------------------------------------------------------------- crossCleanpLS
+// per track candidate: the direction a pLS is compared with, the dR2 bound,
+// and for pT3/pT5 the ntuple indices of its pLS's four hits (checkPixelHits,
+// SDL/TrackCandidate.h:166-202)
+__device__ __forceinline__ void pls_ntuple_hits(const EventView& v, uint32_t seg, int32_t* p) {
+  const uint32_t a = v.S.mdIndices[2 * seg], b = v.S.mdIndices[2 * seg + 1];
+  p[0] = (int32_t)v.H.idxs[v.D.anchorHitIndices[a]];
+  p[1] = (int32_t)v.H.idxs[v.D.anchorHitIndices[b]];
+  p[2] = (int32_t)v.H.idxs[v.D.outerHitIndices[a]];
+  p[3] = (int32_t)v.H.idxs[v.D.outerHitIndices[b]];
+}
+
+__global__ void k_tc_prep(EventView v) {
+  const uint32_t nTC = v.R.scalars[kNTrackCandidates];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nTC; t += gridDim.x * blockDim.x) {
+    const int16_t type = v.C.trackCandidateType[t];
+    const uint32_t inner = v.C.objectIndices[2 * t];
+    float eta = 0, phi = 0;
+    uint8_t kind = 0;
+    int32_t p[4] = {-1, -1, -1, -1};
+    if (type == 4) {
+      eta = v.Q.eta[inner]; phi = v.Q.phi[inner]; kind = 1;
+    } else if (type == 5) {
+      eta = v.P3.eta_pix[inner]; phi = v.P3.phi_pix[inner]; kind = 2;
+      pls_ntuple_hits(v, v.P3.pixelSegmentIndices[inner], p);
+    } else if (type == 7) {
+      eta = v.P.eta[inner - plsBase]; phi = v.P.phi[inner - plsBase]; kind = 2;
+      pls_ntuple_hits(v, inner, p);
+    }
+    v.W.tcEta[t] = eta;
+    v.W.tcPhi[t] = phi;
+    v.W.tcKind[t] = kind;
+    for (int a = 0; a < 4; a++) v.W.tcPix[4 * t + a] = p[a];
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile];
+  __shared__ uint8_t sKind[kTile];
+  __shared__ int32_t sPix[4 * kTile];
+  const uint32_t n = v.nPLS;
+  const uint32_t nTC = v.R.scalars[kNTrackCandidates];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const YShare ys = y_share(nTC);
+  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t i = x0 + threadIdx.x;
+    const bool active = i < n && v.P.isQuad[i] && v.P.isDup[i] == 0;
+    float eta1 = 0, phi1 = 0;
+    int32_t p1[4] = {-1, -1, -1, -1};
+    if (active) {
+      eta1 = v.P.eta[i];
+      phi1 = v.P.phi[i];
+      pls_ntuple_hits(v, plsBase + i, p1);
+    }
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      const uint32_t t = y0 + threadIdx.x;
+      if (t < ys.e) {
+        sEta[threadIdx.x] = v.W.tcEta[t];
+        sPhi[threadIdx.x] = v.W.tcPhi[t];
+        sKind[threadIdx.x] = v.W.tcKind[t];
+        for (int a = 0; a < 4; a++) sPix[4 * threadIdx.x + a] = v.W.tcPix[4 * t + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t2 = 0; t2 < m && !dup; t2++) {
+        const uint8_t kind = sKind[t2];
+        if (kind == 0) continue;
+        const float dEta = m_abs(eta1 - sEta[t2]);
+        const float dPhi = wrap_dphi(phi1, sPhi[t2]);
+        const float dR2 = dEta * dEta + dPhi * dPhi;
+        if (kind == 1) {
+          dup = dR2 < 1e-3f;
+          continue;
+        }
+        dup = dR2 < 0.000001f;
+        for (int a = 0; a < 4 && !dup; a++) {
+          if (p1[a] == -1) continue;
+          for (int b = 0; b < 4; b++)
+            if (p1[a] == sPix[4 * t2 + b]) {
+              dup = true;
+              break;
+            }
+        }
+      }
+    }
+    if (dup) v.P.isDup[i] = 1;
+  }
+}
+
+}  // namespace
+
+void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t) {
+  const dim3 g = pair_grid(v.nPLS, v.nPLS, v.nPLS);
+  const int tm = second ? kTmPlsDup2 : kTmPlsDup;
+  t.begin(s, tm);
+  if (second) hipLaunchKernelGGL(k_pls_pairs<true>, g, dim3(kTile), 0, s, v);
+  else hipLaunchKernelGGL(k_pls_pairs<false>, g, dim3(kTile), 0, s, v);
+  t.end(s, tm);
+}
+
+void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  t.begin(s, kTmPT5Dup);
+  hipLaunchKernelGGL(k_pt5_pairs, pair_grid(kNMaxPixelQuintuplets, 4096, kNMaxPixelQuintuplets), dim3(kTile), 0, s, v);
+  t.end(s, kTmPT5Dup);
+}
+
+void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  t.begin(s, kTmPT3Dup);
+  hipLaunchKernelGGL(k_pt3_pairs, pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelTriplets), dim3(kTile), 0, s, v);
+  t.end(s, kTmPT3Dup);
+}
+
+void launch_cross_pt3(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_cross_pt3_pairs, pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelQuintuplets), dim3(kTile), 0, s, v);
+}
+
+void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  const unsigned nElig = v.nT5SeedModules ? v.nT5SeedModules : 1;
+  hipLaunchKernelGGL(k_t5_list_offsets, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t5_list_fill, dim3(nElig), dim3(256), 0, s, v);
+  t.begin(s, kTmT5DupTC);
+  hipLaunchKernelGGL(k_t5_tc_pairs, pair_grid(v.t5Cap, 8192, v.t5Cap), dim3(kTile), 0, s, v);
+  t.end(s, kTmT5DupTC);
+}
+
+void launch_cross_t5(const EventView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_cross_t5_pairs, pair_grid(v.t5Cap, 8192, kNMaxPixelQuintuplets + kNMaxPixelTriplets), dim3(kTile),
+                     0, s, v);
+}
+
+void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  hipLaunchKernelGGL(k_tc_prep, dim3(128), dim3(256), 0, s, v);
+  t.begin(s, kTmCrossPLS);
+  hipLaunchKernelGGL(k_cross_pls_pairs, pair_grid(v.nPLS, v.nPLS, kNMaxPixelTrackCandidates + kNMaxNonPixelTrackCandidates),
+                     dim3(kTile), 0, s, v);
+  t.end(s, kTmCrossPLS);
+}
+
+}  // namespace lst

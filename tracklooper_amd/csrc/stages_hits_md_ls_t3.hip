@@ -1,12 +1,14 @@
 // Stage kernels: hits, pixel line segments, mini-doublets, line segments,
 // triplets.
 //
-// Builders run one wave64 per lower module (the unit that owns an output
-// range).  Candidates of the module are linearised in the reference's serial
-// loop order (SDL/MiniDoublet.h:911-925, SDL/Segment.h:810-838,
-// SDL/Triplet.h:886-913), evaluated 64 at a time, and the passing ones are
-// ranked with ballot+mbcnt, so object slots equal the reference CPU slots and
-// per-module capacity truncation keeps exactly the first `cap` objects.
+// Builders run one work-group per lower module (the unit that owns an output
+// range; one wave, which measured faster than four).  Candidates of the module
+// are linearised in the reference's serial loop order (SDL/MiniDoublet.h:911-925,
+// SDL/Segment.h:810-838, SDL/Triplet.h:886-913), evaluated one work-group width
+// at a time, and the passing ones are ranked with ballot+mbcnt, so object
+// slots equal the reference CPU slots and per-module capacity truncation keeps
+// exactly the first `cap` objects.  Triplets use a flattened variant that
+// spreads every module's candidates over the whole chip.
 #include <climits>
 
 #include "algo_md_ls_t3.h"
@@ -17,6 +19,17 @@ namespace lst {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kBuildBlock = kWave;  // lanes per module for the MD and LS builders
+
+// rank of a passing lane among the passing lanes of the work-group (ordered)
+__device__ __forceinline__ unsigned build_rank(bool pass, unsigned* lds, unsigned& n) {
+  if constexpr (kBuildBlock == kWave) {
+    (void)lds;
+    return wave_rank(pass, n);
+  } else {
+    return build_rank(pass, lds, n);
+  }
+}
 
 __device__ __forceinline__ int find_module(const ModulesView& M, uint32_t det) {
   // binary search over the sorted detIds (SDL/Hit.h:172-192)
@@ -79,10 +92,24 @@ __global__ void k_hits(EventView v) {
         hx = x + 2.5f * c; lx = x - 2.5f * c;
         hy = y + 2.5f * s; ly = y - 2.5f * s;
       }
-      atomicMin(&v.R.hitRanges[2 * mod], (int)i);
-      atomicMax(&v.R.hitRanges[2 * mod + 1], (int)i);
     }
     v.H.highEdgeXs[i] = hx; v.H.highEdgeYs[i] = hy; v.H.lowEdgeXs[i] = lx; v.H.lowEdgeYs[i] = ly;
+    // hit range of each module: the lanes of a wave hold consecutive hits, so
+    // the lanes that share a module give its smallest index at the lowest
+    // lane and its largest at the highest; one atomic pair per such group
+    const unsigned lane = threadIdx.x & 63u;
+    unsigned long long pending = __ballot(mod >= 0);
+    while (pending) {
+      const int leader = __ffsll(pending) - 1;
+      const int lm = __shfl(mod, leader, 64);
+      const unsigned long long grp = __ballot(mod == lm);
+      if ((int)lane == leader) {
+        const int top = 63 - __clzll(grp);
+        atomicMin(&v.R.hitRanges[2 * lm], (int)i);
+        atomicMax(&v.R.hitRanges[2 * lm + 1], (int)(i - lane + (unsigned)top));
+      }
+      pending &= ~grp;
+    }
   }
 }
 
@@ -195,8 +222,9 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
 }
 
 // --------------------------------------------------------------- mini-doublets
-__global__ __launch_bounds__(kWave) void k_mini_doublets(EventView v) {
+__global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
   __shared__ float sx[256], sy[256], sz[256], srt[256];
+  __shared__ unsigned lds[kBuildBlock / 64 + 1];
   const uint32_t m = blockIdx.x;
   const unsigned lane = threadIdx.x;
   const int loBase = v.R.hitRangesLower[m];
@@ -205,10 +233,10 @@ __global__ __launch_bounds__(kWave) void k_mini_doublets(EventView v) {
   if (loBase != -1) {
     const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
     const int upBase = v.R.hitRangesUpper[m];
-    for (int i = lane; i < nLo; i += kWave) {
+    for (int i = lane; i < nLo; i += kBuildBlock) {
       sx[i] = v.H.xs[loBase + i]; sy[i] = v.H.ys[loBase + i]; sz[i] = v.H.zs[loBase + i]; srt[i] = v.H.rts[loBase + i];
     }
-    for (int i = lane; i < nUp; i += kWave) {
+    for (int i = lane; i < nUp; i += kBuildBlock) {
       const int j = 128 + i;
       sx[j] = v.H.xs[upBase + i]; sy[j] = v.H.ys[upBase + i]; sz[j] = v.H.zs[upBase + i]; srt[j] = v.H.rts[upBase + i];
     }
@@ -217,7 +245,7 @@ __global__ __launch_bounds__(kWave) void k_mini_doublets(EventView v) {
     const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     const uint32_t base = v.mdModuleIndices[m];
     const int limit = nLo * nUp;
-    for (int c0 = 0; c0 < limit; c0 += kWave) {
+    for (int c0 = 0; c0 < limit; c0 += kBuildBlock) {
       const int k = c0 + lane;
       bool pass = false;
       MDOut o;
@@ -231,7 +259,7 @@ __global__ __launch_bounds__(kWave) void k_mini_doublets(EventView v) {
                    : md_endcap(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], o);
       }
       unsigned n;
-      const unsigned rank = count + wave_rank(pass, n);
+      const unsigned rank = count + build_rank(pass, lds, n);
       if (pass && (int)rank < cap) store_md(v, loBase + li, upBase + ui, m, anchorIsUpper, o, base + rank);
       count += n;
     }
@@ -254,10 +282,11 @@ __global__ void k_md_ranges(EventView v) {
 }
 
 // --------------------------------------------------------------- line segments
-__global__ __launch_bounds__(kWave) void k_segments(EventView v) {
+__global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
   __shared__ uint32_t pre[160];
   __shared__ uint16_t outMod[160];
   __shared__ uint32_t outCnt[160];
+  __shared__ unsigned lds[kBuildBlock / 64 + 1];
   const uint32_t m = blockIdx.x;
   const unsigned lane = threadIdx.x;
   const uint32_t nIn = v.R.nMDs[m];
@@ -265,7 +294,7 @@ __global__ __launch_bounds__(kWave) void k_segments(EventView v) {
   unsigned count = 0;
   if (nIn != 0) {
     const uint32_t nConn = v.M.nConnectedModules[m];
-    for (uint32_t k = lane; k < nConn; k += kWave) {
+    for (uint32_t k = lane; k < nConn; k += kBuildBlock) {
       const uint16_t om = v.M.moduleMap[(size_t)m * v.M.stride + k];
       outMod[k] = om;
       outCnt[k] = v.R.nMDs[om];
@@ -281,15 +310,19 @@ __global__ __launch_bounds__(kWave) void k_segments(EventView v) {
     const ModCtx ci = load_mod(v.M, m);
     const uint32_t inBase = v.R.mdRanges[2 * m];
     const uint32_t segBase = v.segModuleIndices[m];
-    for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+    for (uint32_t c0 = 0; c0 < total; c0 += kBuildBlock) {
       const uint32_t L = c0 + lane;
       bool pass = false;
       SegOut o{0, 0, 0, 0, 0, 0};
       uint32_t inMD = 0, outMD = 0;
       uint16_t om = 0;
       if (L < total) {
-        uint32_t k = 0;
-        while (k + 1 < nConn && pre[k + 1] <= L) k++;
+        uint32_t k = 0, hiK = nConn;  // largest k with pre[k] <= L
+        while (hiK - k > 1) {
+          const uint32_t mid = (k + hiK) >> 1;
+          if (pre[mid] <= L) k = mid;
+          else hiK = mid;
+        }
         om = outMod[k];
         const uint32_t local = L - pre[k];
         const uint32_t nOut = outCnt[k];
@@ -301,7 +334,7 @@ __global__ __launch_bounds__(kWave) void k_segments(EventView v) {
                                                               : seg_endcap(ci, co, v.ptCut, a, b, o);
       }
       unsigned n;
-      const unsigned rank = count + wave_rank(pass, n);
+      const unsigned rank = count + build_rank(pass, lds, n);
       if (pass && (int)rank < cap) {
         const uint32_t s = segBase + rank;
         v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
@@ -609,45 +642,47 @@ inline unsigned grid_for(uint32_t n, unsigned block) {
 
 }  // namespace
 
-void launch_hits(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_event_init, dim3(grid_for(2 * v.M.nModules, 256)), dim3(256), 0, s, v);
-  t.begin(s);
+  t.begin(s, kTmHits);
   hipLaunchKernelGGL(k_hits, dim3(grid_for(v.nHits, 256)), dim3(256), 0, s, v);
-  t.end(s);
+  t.end(s, kTmHits);
   hipLaunchKernelGGL(k_module_ranges, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_hit_ranges_fix, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
 }
 
 void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
-                           const KernelTimer& t) {
+                           const KernelTimers& t) {
   PixelInputs in{hitIdx[0], hitIdx[1], hitIdx[2], hitIdx[3], dPhiChange};
-  t.begin(s);
+  t.begin(s, kTmPixelSegments);
   hipLaunchKernelGGL(k_pixel_segments, dim3(grid_for(v.nPLS, 256)), dim3(256), 0, s, v, in);
-  t.end(s);
+  t.end(s, kTmPixelSegments);
 }
 
-void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimer& t) {
-  t.begin(s);
-  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
-  t.end(s);
+void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  t.begin(s, kTmMiniDoublets);
+  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+  t.end(s, kTmMiniDoublets);
   hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
-void launch_segments(const EventView& v, hipStream_t s, const KernelTimer& t) {
-  t.begin(s);
-  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
-  t.end(s);
+void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  t.begin(s, kTmSegments);
+  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+  t.end(s, kTmSegments);
   hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
-void launch_triplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_count, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  t.begin(s, kTmT3Filter);
   hipLaunchKernelGGL(k_t3_filter_md, dim3(2048), dim3(256), 0, s, v);
-  t.begin(s);
+  t.end(s, kTmT3Filter);
+  t.begin(s, kTmT3Select);
   hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
-  t.end(s);
+  t.end(s, kTmT3Select);
   hipLaunchKernelGGL(k_t3_place, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);

@@ -9,10 +9,8 @@
 // capacity.  The builders here run one wave per pLS over that pLS's
 // candidates in the same order: a counting pass, a scan over pLS, then a
 // writing pass that places every passing candidate at its serial slot.  TC
-// additions rank their flags with workgroup scans.  Every duplicate-removal
-// kernel gives each object its own lane that tests all partners; the flag an
-// object gets depends only on pair values and on flags written by earlier
-// kernels, so the results do not depend on execution order.
+// additions rank their flags with workgroup scans.  Duplicate removal and
+// cross cleaning run as all-pairs kernels in stages_clean.hip.
 #include "algo_pixel.h"
 #include "stages.h"
 #include "wave.h"
@@ -44,58 +42,6 @@ __global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t* counts, uint32_
   if (threadIdx.x == 0) {
     counts[n] = carry;
     if (total) *total = carry;
-  }
-}
-
-// --------------------------------------------------------------- pLS cleaning
-// checkHitspLS (SDL/Kernels.h:384-473).  For a pair (lo < hi) within 0.1 in eta
-// the reference drops the triplet seed over the quadruplet, else the higher
-// score (ties: lo), when lo shares >= 3 of its hits with hi (first pass), or
-// any hit or dR2 < 1e-5 (second pass, quadruplets not flagged by the first
-// pass only).  Lane k collects the pairs whose removal target is k.
-template <bool kSecond>
-__global__ void k_pls_dup(EventView v) {
-  const uint32_t n = v.nPLS;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    if (kSecond && (!v.P.isQuad[k] || (v.P.isDup[k] & 1))) continue;
-    const float etaK = v.P.eta[k];
-    bool flag = false;
-    for (uint32_t j = 0; j < n && !flag; j++) {
-      if (j == k) continue;
-      const float etaJ = v.P.eta[j];
-      const uint32_t lo = k < j ? k : j, hi = k < j ? j : k;
-      const float etaLo = k < j ? etaK : etaJ, etaHi = k < j ? etaJ : etaK;
-      if (m_abs(etaHi - etaLo) > 0.1f) continue;
-      if (kSecond && (!v.P.isQuad[j] || (v.P.isDup[j] & 1))) continue;
-      const int quadDiff = (int)v.P.isQuad[lo] - (int)v.P.isQuad[hi];
-      const float scoreDiff = v.P.score[lo] - v.P.score[hi];
-      uint32_t rm;
-      if (quadDiff > 0) rm = hi;
-      else if (quadDiff < 0) rm = lo;
-      else if (scoreDiff < 0) rm = hi;
-      else if (scoreDiff > 0) rm = lo;
-      else rm = lo;
-      if (rm != k) continue;
-      int matched = 0;
-      for (int a = 0; a < 4; a++) {
-        const uint32_t h = v.P.pLSHitsIdxs[4 * lo + a];
-        bool m = false;
-        for (int b = 0; b < 4; b++)
-          if (h == v.P.pLSHitsIdxs[4 * hi + b]) { m = true; break; }
-        if (m) {
-          matched++;
-          if (kSecond) break;
-        }
-      }
-      if (!kSecond) {
-        flag = matched >= 3;
-      } else {
-        const float dEta = m_abs(etaLo - etaHi);
-        const float dPhi = wrap_dphi(v.P.phi[lo], v.P.phi[hi]);
-        flag = matched >= 1 || dEta * dEta + dPhi * dPhi < 1e-5f;
-      }
-    }
-    if (flag) v.P.isDup[k] = (int8_t)(v.P.isDup[k] | (kSecond ? 2 : 1));
   }
 }
 
@@ -203,35 +149,6 @@ __global__ __launch_bounds__(kWave) void k_pixel_quintuplets(EventView v) {
 __global__ void k_pt5_totals(EventView v) {
   const uint32_t tot = v.R.scalars[kTotOccPixelQuintuplets];
   v.R.scalars[kNPixelQuintuplets] = tot < kNMaxPixelQuintuplets ? tot : kNMaxPixelQuintuplets;
-}
-
-// removeDupPixelQuintupletsInGPUFromMap (SDL/Kernels.h:350-381): ix is dropped
-// when a partner shares >= 7 of its 14 hits and scores lower (ties: the
-// higher index is dropped)
-__global__ void k_pt5_dup(EventView v) {
-  const uint32_t n = v.R.scalars[kNPixelQuintuplets];
-  for (uint32_t ix = blockIdx.x * blockDim.x + threadIdx.x; ix < n; ix += gridDim.x * blockDim.x) {
-    uint32_t h[14];
-#pragma unroll
-    for (int a = 0; a < 14; a++) h[a] = v.P5.hitIndices[14 * ix + a];
-    const float s1 = v.P5.score[ix];
-    bool dup = false;
-    for (uint32_t jx = 0; jx < n && !dup; jx++) {
-      if (jx == ix) continue;
-      const float s2 = v.P5.score[jx];
-      if (!(s1 > s2 || (s1 == s2 && ix > jx))) continue;  // the flag needs this ordering
-      int matched = 0;
-      for (int a = 0; a < 14 && matched < 7 && matched + (14 - a) >= 7; a++) {
-        for (int b = 0; b < 14; b++)
-          if (h[a] == v.P5.hitIndices[14 * jx + b]) {
-            matched++;
-            break;
-          }
-      }
-      dup = matched >= 7;
-    }
-    if (dup) v.P5.isDup[ix] = 1;
-  }
 }
 
 // addTrackCandidateToMemory (SDL/TrackCandidate.h:126-164), with the entries
@@ -380,61 +297,7 @@ __global__ void k_pt3_totals(EventView v) {
   v.R.scalars[kNPixelTriplets] = tot < kNMaxPixelTriplets ? tot : kNMaxPixelTriplets;
 }
 
-// removeDupPixelTripletsInGPUFromMap (SDL/Kernels.h:322-348) with checkHitspT3
-// (:137-198): ix is dropped when a partner shares >= 5 hits and reaches a
-// lower third layer, or the same layer with a lower score (ties: lower index)
-__global__ void k_pt3_dup(EventView v) {
-  const uint32_t n = v.R.scalars[kNPixelTriplets];
-  for (uint32_t ix = blockIdx.x * blockDim.x + threadIdx.x; ix < n; ix += gridDim.x * blockDim.x) {
-    uint32_t h[10];
-#pragma unroll
-    for (int a = 0; a < 10; a++) h[a] = v.P3.hitIndices[10 * ix + a];
-    const uint8_t li = v.P3.logicalLayers[5 * ix + 2];
-    const float si = v.P3.score[ix];
-    bool dup = false;
-    for (uint32_t jx = 0; jx < n && !dup; jx++) {
-      if (jx == ix) continue;
-      const uint8_t lj = v.P3.logicalLayers[5 * jx + 2];
-      const float sj = v.P3.score[jx];
-      if (!(lj < li || (li == lj && si > sj) || (li == lj && si == sj && ix < jx))) continue;
-      int matched = 0;
-      for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++)
-          if (h[a] == v.P3.hitIndices[10 * jx + b]) {
-            matched++;
-            break;
-          }
-      for (int a = 4; a < 10; a++)
-        for (int b = 4; b < 10; b++)
-          if (h[a] == v.P3.hitIndices[10 * jx + b]) {
-            matched++;
-            break;
-          }
-      dup = matched >= 5;
-    }
-    if (dup) v.P3.isDup[ix] = 1;
-  }
-}
-
 // ---------------------------------------------------------- track candidates
-// crossCleanpT3 (SDL/TrackCandidate.h:204-241)
-__global__ void k_tc_cross_pt3(EventView v) {
-  const uint32_t n = v.R.scalars[kNPixelTriplets], n5 = v.R.scalars[kNPixelQuintuplets];
-  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    if (v.P3.isDup[p]) continue;
-    const float eta1 = v.P3.eta_pix[p], phi1 = v.P3.phi_pix[p];
-    bool dup = false;
-    for (uint32_t q = 0; q < n5 && !dup; q++) {
-      const uint32_t j = v.P5.pixelIndices[q] - plsBase;
-      const float dEta = m_abs(eta1 - v.P.eta[j]);
-      const float dPhi = wrap_dphi(phi1, v.P.phi[j]);
-      dup = dEta * dEta + dPhi * dPhi < 1e-5f;
-    }
-    if (dup) v.P3.isDup[p] = 1;
-  }
-}
-
 // addpT3asTrackCandidatesInGPU (SDL/TrackCandidate.h:381-428)
 __global__ __launch_bounds__(1024) void k_tc_add_pt3(EventView v) {
   __shared__ unsigned lds[17];
@@ -460,79 +323,6 @@ __global__ __launch_bounds__(1024) void k_tc_add_pt3(EventView v) {
     const unsigned added = carry < room ? carry : room;
     v.R.scalars[kNTrackCandidates] = base + added;
     v.R.scalars[kNTrackCandidatespT3] = added;
-  }
-}
-
-// removeDupQuintupletsInGPUBeforeTC (SDL/Kernels.h:250-320).  Among T5s not in
-// a pT5 and not flagged after build, a close pair (|dEta|, |dPhi| <= 0.1 and
-// dR2 < 0.001 or >= 5 shared hits) flags the higher score (ties: the lower
-// index) with bit 2.  One workgroup per eligible module, one lane per T5.
-__global__ __launch_bounds__(256) void k_t5_dup_before_tc(EventView v) {
-  const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  if (blockIdx.x >= nElig) return;
-  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const uint32_t n1 = v.R.nQuintuplets[m1];
-  const uint32_t b1 = (uint32_t)v.R.quintupletModuleIndices[m1];
-  for (uint32_t i1 = threadIdx.x; i1 < n1; i1 += blockDim.x) {
-    const uint32_t x = b1 + i1;
-    if (v.Q.partOfPT5[x] || (v.Q.isDup[x] & 1)) continue;
-    const float etaX = v.Q.eta[x], phiX = v.Q.phi[x], sX = v.Q.score_rphisum[x];
-    bool dup = false;
-    for (uint32_t e = 0; e < nElig && !dup; e++) {
-      const uint16_t m2 = v.R.indicesOfEligibleT5Modules[e];
-      const uint32_t n2 = v.R.nQuintuplets[m2];
-      const uint32_t b2 = (uint32_t)v.R.quintupletModuleIndices[m2];
-      for (uint32_t j1 = 0; j1 < n2 && !dup; j1++) {
-        const uint32_t y = b2 + j1;
-        if (y == x) continue;
-        if (v.Q.partOfPT5[y] || (v.Q.isDup[y] & 1)) continue;
-        const float sY = v.Q.score_rphisum[y];
-        const bool loses = sX > sY || (!(sX < sY) && x < y);
-        if (!loses) continue;
-        const float etaY = v.Q.eta[y], phiY = v.Q.phi[y];
-        const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
-        const float dEta = m_abs(x < y ? etaX - etaY : etaY - etaX);
-        const float dPhi = x < y ? wrap_dphi(phiX, phiY) : wrap_dphi(phiY, phiX);
-        if (dEta > 0.1f) continue;
-        if (m_abs(dPhi) > 0.1f) continue;
-        const float dR2 = dEta * dEta + dPhi * dPhi;
-        int matched = 0;
-        for (int a = 0; a < 10; a++) {
-          const uint32_t h = v.Q.hitIndices[10 * lo + a];
-          for (int b = 0; b < 10; b++)
-            if (h == v.Q.hitIndices[10 * hi + b]) {
-              matched++;
-              break;
-            }
-        }
-        dup = dR2 < 0.001f || matched >= 5;
-      }
-    }
-    if (dup) v.Q.isDup[x] = (int8_t)(v.Q.isDup[x] | 2);
-  }
-}
-
-// crossCleanT5 with Crossclean_T5 (SDL/TrackCandidate.h:243-294)
-__global__ __launch_bounds__(256) void k_tc_cross_t5(EventView v) {
-  const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  if (blockIdx.x >= nElig) return;
-  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets], n3 = v.R.scalars[kNPixelTriplets];
-  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const uint32_t nq = v.R.nQuintuplets[m];
-  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
-  for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) {
-    const uint32_t q = base + k;
-    if (v.Q.isDup[q] || v.Q.partOfPT5[q]) continue;
-    const float eta1 = v.Q.eta[q], phi1 = v.Q.phi[q];
-    bool dup = false;
-    for (uint32_t j = 0; j < n5 + n3 && !dup; j++) {
-      const float eta2 = j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5];
-      const float phi2 = j < n5 ? v.P5.phi[j] : v.P3.phi[j - n5];
-      const float dEta = m_abs(eta1 - eta2);
-      const float dPhi = wrap_dphi(phi1, phi2);
-      dup = dEta * dEta + dPhi * dPhi < 1e-3f;
-    }
-    if (dup) v.Q.isDup[q] = 1;
   }
 }
 
@@ -595,63 +385,6 @@ __global__ __launch_bounds__(256) void k_tc_t5_write(EventView v) {
              &v.Q.hitIndices[10 * q], -1, v.Q.regressionG[q], v.Q.regressionF[q], v.Q.regressionRadius[q], q);
     }
     carry += tot;
-  }
-}
-
-// checkPixelHits (SDL/TrackCandidate.h:166-202)
-__device__ __forceinline__ int pixel_hits_matched(const EventView& v, uint32_t ix, uint32_t jx) {
-  int p1[4], p2[4];
-  const uint32_t a1 = v.S.mdIndices[2 * ix], b1 = v.S.mdIndices[2 * ix + 1];
-  const uint32_t a2 = v.S.mdIndices[2 * jx], b2 = v.S.mdIndices[2 * jx + 1];
-  p1[0] = (int)v.H.idxs[v.D.anchorHitIndices[a1]];
-  p1[1] = (int)v.H.idxs[v.D.anchorHitIndices[b1]];
-  p1[2] = (int)v.H.idxs[v.D.outerHitIndices[a1]];
-  p1[3] = (int)v.H.idxs[v.D.outerHitIndices[b1]];
-  p2[0] = (int)v.H.idxs[v.D.anchorHitIndices[a2]];
-  p2[1] = (int)v.H.idxs[v.D.anchorHitIndices[b2]];
-  p2[2] = (int)v.H.idxs[v.D.outerHitIndices[a2]];
-  p2[3] = (int)v.H.idxs[v.D.outerHitIndices[b2]];
-  int n = 0;
-  for (int i = 0; i < 4; i++) {
-    if (p1[i] == -1) continue;
-    bool m = false;
-    for (int j = 0; j < 4; j++) {
-      if (p2[j] == -1) continue;
-      if (p1[i] == p2[j]) { m = true; break; }
-    }
-    if (m) n++;
-  }
-  return n;
-}
-
-// crossCleanpLS (SDL/TrackCandidate.h:296-379)
-__global__ void k_tc_cross_pls(EventView v) {
-  const uint32_t nTC = v.R.scalars[kNTrackCandidates];
-  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v.nPLS; i += gridDim.x * blockDim.x) {
-    if (!v.P.isQuad[i] || v.P.isDup[i]) continue;
-    const float eta1 = v.P.eta[i], phi1 = v.P.phi[i];
-    bool dup = false;
-    for (uint32_t t = 0; t < nTC && !dup; t++) {
-      const int16_t type = v.C.trackCandidateType[t];
-      const uint32_t inner = v.C.objectIndices[2 * t];
-      if (type == 4) {
-        const float dEta = m_abs(eta1 - v.Q.eta[inner]);
-        const float dPhi = wrap_dphi(phi1, v.Q.phi[inner]);
-        dup = dEta * dEta + dPhi * dPhi < 1e-3f;
-      } else if (type == 5) {
-        const uint32_t plsIdx = v.P3.pixelSegmentIndices[inner];
-        const float dEta = m_abs(eta1 - v.P3.eta_pix[inner]);
-        const float dPhi = wrap_dphi(phi1, v.P3.phi_pix[inner]);
-        dup = pixel_hits_matched(v, plsBase + i, plsIdx) > 0 || dEta * dEta + dPhi * dPhi < 0.000001f;
-      } else if (type == 7) {
-        const uint32_t plsIdx = inner;
-        const float dEta = m_abs(eta1 - v.P.eta[plsIdx - plsBase]);
-        const float dPhi = wrap_dphi(phi1, v.P.phi[plsIdx - plsBase]);
-        dup = pixel_hits_matched(v, plsBase + i, plsIdx) > 0 || dEta * dEta + dPhi * dPhi < 0.000001f;
-      }
-    }
-    if (dup) v.P.isDup[i] = 1;
   }
 }
 
@@ -723,51 +456,47 @@ __global__ void k_tc_output(EventView v) {
 
 }  // namespace
 
-void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimer& t) {
-  t.begin(s);
-  hipLaunchKernelGGL(k_pls_dup<false>, dim3(grid_of(v.nPLS, 256)), dim3(256), 0, s, v);
-  t.end(s);
+void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  launch_pls_dup(v, s, false, t);
 }
 
-void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   const unsigned nP = v.nPLS ? v.nPLS : 1;
   hipLaunchKernelGGL(k_pls_connected, dim3(grid_of(v.nPLS, 256)), dim3(256), 0, s, v);
-  t.begin(s);
+  t.begin(s, kTmPT5Build);
   hipLaunchKernelGGL(k_pixel_quintuplets<false>, dim3(nP), dim3(kWave), 0, s, v);
-  t.end(s);
+  t.end(s, kTmPT5Build);
   hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, v.W.pt5Count, v.nPLS,
                      v.R.scalars + kTotOccPixelQuintuplets);
   hipLaunchKernelGGL(k_pt5_totals, dim3(1), dim3(1), 0, s, v);
   hipLaunchKernelGGL(k_pixel_quintuplets<true>, dim3(nP), dim3(kWave), 0, s, v);
-  hipLaunchKernelGGL(k_pt5_dup, dim3(grid_of(kNMaxPixelQuintuplets, 256)), dim3(256), 0, s, v);
+  launch_pt5_dup(v, s, t);
   hipLaunchKernelGGL(k_tc_add_pt5, dim3(1), dim3(1024), 0, s, v);
 }
 
-void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   const unsigned nP = v.nPLS ? v.nPLS : 1;
-  t.begin(s);
+  t.begin(s, kTmPT3Build);
   hipLaunchKernelGGL(k_pixel_triplets<false>, dim3(nP), dim3(kWave), 0, s, v);
-  t.end(s);
+  t.end(s, kTmPT3Build);
   hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, v.W.pt3Count, v.nPLS,
                      v.R.scalars + kTotOccPixelTriplets);
   hipLaunchKernelGGL(k_pt3_totals, dim3(1), dim3(1), 0, s, v);
   hipLaunchKernelGGL(k_pixel_triplets<true>, dim3(nP), dim3(kWave), 0, s, v);
-  hipLaunchKernelGGL(k_pt3_dup, dim3(grid_of(kNMaxPixelTriplets, 256)), dim3(256), 0, s, v);
+  launch_pt3_dup(v, s, t);
 }
 
-void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimers& t) {
   const unsigned nElig = v.nT5SeedModules ? v.nT5SeedModules : 1;
-  hipLaunchKernelGGL(k_tc_cross_pt3, dim3(grid_of(kNMaxPixelTriplets, 256)), dim3(256), 0, s, v);
+  launch_cross_pt3(v, s);
   hipLaunchKernelGGL(k_tc_add_pt3, dim3(1), dim3(1024), 0, s, v);
-  t.begin(s);
-  hipLaunchKernelGGL(k_t5_dup_before_tc, dim3(nElig), dim3(256), 0, s, v);
-  t.end(s);
-  hipLaunchKernelGGL(k_tc_cross_t5, dim3(nElig), dim3(256), 0, s, v);
+  launch_t5_dup_before_tc(v, s, t);
+  launch_cross_t5(v, s);
   hipLaunchKernelGGL(k_tc_t5_count, dim3(nElig), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_tc_t5_scan, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_tc_t5_write, dim3(nElig), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_pls_dup<true>, dim3(grid_of(v.nPLS, 256)), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_tc_cross_pls, dim3(grid_of(v.nPLS, 256)), dim3(256), 0, s, v);
+  launch_pls_dup(v, s, true, t);
+  launch_cross_pls(v, s, t);
   hipLaunchKernelGGL(k_tc_add_pls, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_tc_output, dim3(grid_of(kTCCap, 256)), dim3(256), 0, s, v);
 }

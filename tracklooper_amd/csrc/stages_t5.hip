@@ -21,6 +21,7 @@ namespace {
 constexpr int kT5Block = 256;
 constexpr int kMaxInnerT3 = 1024;   // > the largest T3 module capacity (755)
 constexpr int kMaxT5PerModule = 416;  // >= the largest T5 module capacity (414)
+constexpr int kAfterBuildSplit = 8;   // partner shares per module in the after-build duplicate check
 
 // createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
 // seed modules holding a triplet, their capacities and the exclusive scan of
@@ -168,7 +169,9 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
   }
   regression_sigmas(m, true, d1, d2, sl, flat);
   r.regR = regression_fit(x, y, d1, d2, sl, flat, r.regG, r.regF, r.chi2);
-  if (r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
+  // r-phi chi2 cut of the plain mode (USE_RPHICHI2, SDL/Quintuplet.h:2943-2956);
+  // the DNN mode applies the T5 DNN instead (k_t5_dnn / t5dnn_score)
+  if (!v.t5Dnn && r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
       !t5_rphi_chi2_ok(m[0].sdl, m[1].sdl, m[2].sdl, m[3].sdl, m[4].sdl, r.chi2))
     return false;
 #pragma unroll
@@ -179,6 +182,85 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
   regression_sigmas(m, false, d1, d2, sl, flat);
   r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
   return true;
+}
+
+// ------------------------------------------------------------------ T5 DNN
+// T5DNN::runInference (SDL/NeuralNetwork.h:21-165), the cut of the DNN mode
+// (USE_T5_DNN, SDL/Quintuplet.h:2921-2941).  Parameters: the float tables of
+// SDL/NeuralNetworkWeights.h, kept as data (row-major wgtT_k[in][out]).  Every
+// lane evaluates its own candidate; the loops unroll, so the weights are
+// uniform scalar loads and each output accumulates its inputs in index order
+// (no FMA contraction), as the serial reference does.
+__constant__ float cDnnBias0[32] = {
+#include "../data/t5dnn/bias_0.inc"
+};
+__constant__ float cDnnW0[38 * 32] = {
+#include "../data/t5dnn/wgtT_0.inc"
+};
+__constant__ float cDnnBias2[32] = {
+#include "../data/t5dnn/bias_2.inc"
+};
+__constant__ float cDnnW2[32 * 32] = {
+#include "../data/t5dnn/wgtT_2.inc"
+};
+__constant__ float cDnnBias4[1] = {
+#include "../data/t5dnn/bias_4.inc"
+};
+__constant__ float cDnnW4[32] = {
+#include "../data/t5dnn/wgtT_4.inc"
+};
+constexpr float kT5DnnWP = 0.6177366f;     // T5DNN::LSTWP2 (SDL/Constants.h:145)
+constexpr float kMagneticField = 3.8112f;  // SDL/Constants.h:135
+
+__device__ __forceinline__ float dnn_log10(float x) { return m_log(x) / 2.302585093f; }
+
+__device__ __forceinline__ float t5dnn_score(const EventView& v, const T5Cand& c, uint32_t innerT3, float innerR,
+                                          float bridgeR, float outerR) {
+  float in[38];
+  const int hitOf[6] = {0, 1, 2, 2, 3, 4};  // inner T3 hits 1-3, outer T3 hits 3-5
+#pragma unroll
+  for (int b = 0; b < 6; b++) {
+    const uint32_t md = c.md[hitOf[b]];
+    const uint16_t mod = c.mods[hitOf[b]];
+    const float x = v.D.anchorX[md], y = v.D.anchorY[md];
+    float* o = in + (b < 3 ? 1 + 5 * b : 17 + 5 * (b - 3));
+    o[0] = v.D.anchorEta[md];
+    o[1] = v.D.anchorPhi[md];
+    o[2] = v.D.anchorZ[md];
+    o[3] = m_sqrt(x * x + y * y);
+    o[4] = float(v.M.layers[mod] + 6 * (v.M.subdets[mod] == kEndcap));
+  }
+  const int adj = v.M.layers[c.mods[0]] == 1 ? 1 : 0;
+  const uint32_t mdEtaPhi = v.S.mdIndices[2 * v.T.segmentIndices[2 * innerT3 + adj]];
+  in[0] = dnn_log10(2 * k2Rinv1GeVf * innerR);
+  in[16] = dnn_log10(2 * k2Rinv1GeVf * outerR);
+  in[32] = dnn_log10((innerR + outerR) * kMagneticField * 1.602f / (2 * 100 * 5.39f));
+  in[33] = v.D.anchorEta[mdEtaPhi];
+  in[34] = v.D.anchorPhi[mdEtaPhi];
+  in[35] = dnn_log10(innerR);
+  in[36] = dnn_log10(bridgeR);
+  in[37] = dnn_log10(outerR);
+  float h1[32];
+#pragma unroll
+  for (int o = 0; o < 32; o++) {
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 38; i++) acc += in[i] * cDnnW0[32 * i + o];
+    acc += cDnnBias0[o];
+    h1[o] = acc > 0.f ? acc : 0.f;
+  }
+  float out = 0.f;
+#pragma unroll
+  for (int o = 0; o < 32; o++) {
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 32; i++) acc += h1[i] * cDnnW2[32 * i + o];
+    acc += cDnnBias2[o];
+    const float h2 = acc > 0.f ? acc : 0.f;
+    out += h2 * cDnnW4[o];
+  }
+  out += cDnnBias4[0];
+  return m_exp(out) / (m_exp(out) + 1);  // NaN once exp overflows, as in the reference
 }
 
 __device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t innerT3, uint32_t outerT3,
@@ -379,6 +461,22 @@ __global__ __launch_bounds__(256) void k_t5_final(EventView v) {
   }
 }
 
+// stage 3b (DNN mode): the T5 DNN over the parked results; a result that
+// fails gets its inner T3 index set to ~0 and is not placed.  The cut cannot
+// change the tight flag of a candidate that passes it.
+__global__ __launch_bounds__(256) void k_t5_dnn(EventView v) {
+  if (!v.t5Dnn || !t5_flat_mode(v)) return;
+  const uint32_t nRes = v.R.scalars[kT5Results];
+  const uint32_t n = nRes < kT5ResultMax ? nRes : kT5ResultMax;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t* w = &v.W5.results[kT5ResultWords * k];
+    const uint32_t inner = w[0], outer = w[1];
+    const float score = t5dnn_score(v, t5_cand(v, inner, outer), inner, __uint_as_float(w[2]), __uint_as_float(w[3]),
+                                    __uint_as_float(w[4]));
+    if (!(score > kT5DnnWP)) w[0] = ~0u;
+  }
+}
+
 // stage 4: per module, passing candidates in candidate order take the slots
 __global__ __launch_bounds__(kT5Block) void k_t5_place(EventView v) {
   __shared__ unsigned lds[kT5Block / 64 + 1];
@@ -393,7 +491,7 @@ __global__ __launch_bounds__(kT5Block) void k_t5_place(EventView v) {
   for (uint32_t c0 = 0; c0 < n; c0 += kT5Block) {
     const uint32_t l = c0 + threadIdx.x;
     const uint32_t k = l < n ? v.W5.resIdx[off + l] : ~0u;
-    const bool pass = k != ~0u;
+    const bool pass = k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
     unsigned tot;
     const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, tot);
     if (pass && (int)rank < cap) {
@@ -462,6 +560,8 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
         mods[3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
         mods[4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
         pass = t5_candidate(v, mods, innerT3, outerT3, md, r);
+        if (pass && v.t5Dnn)
+          pass = t5dnn_score(v, t5_cand(v, innerT3, outerT3), innerT3, r.innerR, r.bridgeR, r.outerR) > kT5DnnWP;
       }
       unsigned n;
       const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, n);
@@ -477,8 +577,9 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
 
 // removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249).  For a close
 // pair (i < j) sharing >= 7 hits the reference flags i when score_i >= score_j
-// and j otherwise; each lane here owns one T5 and tests every partner, so the
-// flags are written without atomics and do not depend on evaluation order.
+// and j otherwise.  A module's partners are split into kAfterBuildSplit equal
+// shares (blockIdx.y); each lane owns one T5 and tests the partners of its
+// share, so a flag is a store of 1 that does not depend on evaluation order.
 __global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
   __shared__ float sEta[kMaxT5PerModule], sPhi[kMaxT5PerModule], sScore[kMaxT5PerModule];
   __shared__ uint32_t sHits[kMaxT5PerModule * 10];
@@ -494,10 +595,12 @@ __global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
   }
   for (uint32_t i = threadIdx.x; i < nq * 10; i += kT5Block) sHits[i] = v.Q.hitIndices[10 * base + i];
   __syncthreads();
+  const uint32_t share = (nq + gridDim.y - 1) / gridDim.y;
+  const uint32_t jb = blockIdx.y * share, je = jb + share < nq ? jb + share : nq;
   for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
     bool dup = false;
     const float si = sScore[i];
-    for (uint32_t j = 0; j < nq && !dup; j++) {
+    for (uint32_t j = jb; j < je && !dup; j++) {
       if (j == i) continue;
       // the pair can only flag i if the score order points at i
       const bool targetsI = i < j ? (si >= sScore[j]) : !(sScore[j] >= si);
@@ -539,7 +642,7 @@ __global__ void k_t5_ranges(EventView v) {
 
 }  // namespace
 
-void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t) {
+void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
@@ -547,12 +650,19 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimer& t)
   hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_md, dim3(flatGrid), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
-  t.begin(s);
+  t.begin(s, kTmT5Final);
   hipLaunchKernelGGL(k_t5_final, dim3(flatGrid), dim3(256), 0, s, v);
-  t.end(s);
+  t.end(s, kTmT5Final);
+  if (v.t5Dnn) {
+    t.begin(s, kTmT5Dnn);
+    hipLaunchKernelGGL(k_t5_dnn, dim3(flatGrid), dim3(256), 0, s, v);
+    t.end(s, kTmT5Dnn);
+  }
   hipLaunchKernelGGL(k_t5_place, dim3(nBlocks), dim3(kT5Block), 0, s, v);
   hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  t.begin(s, kTmT5DupAfter);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks, kAfterBuildSplit), dim3(kT5Block), 0, s, v);
+  t.end(s, kTmT5DupAfter);
   const unsigned g = (v.M.nLowerModules + 255) / 256;
   hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
 }
