@@ -160,6 +160,39 @@ lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t sl
  * (asynchronous on the event's stream; same results as lst_run). */
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 
+/* ---- Batched engine.  The reference runs one SDL::Event per queue with one
+ * OpenMP thread per queue (bin/sdl.cc:389-419) and LST::run one event at a
+ * time (SDL/LST.cc:12-161); a batch keeps `n_contexts` event contexts (device
+ * arena + stream each) on one GPU and runs a batch of events round robin, so
+ * the contexts' kernels overlap.  Per event the results equal lst_run's. */
+typedef struct lst_batch lst_batch;
+
+/* Output lists of one event, as LST::hits()/len()/seedIdx()/trackCandidateType()
+ * (SDL/LST.cc:366-420).  The caller sets `capacity` and the arrays (any may be
+ * NULL); n_tc is set to the event's track-candidate count.  When n_tc exceeds
+ * capacity the first `capacity` entries are written and the call returns
+ * LST_OK | LST_W_TRUNCATED. */
+typedef struct {
+  uint32_t capacity;  /* in: entries the arrays hold */
+  uint32_t n_tc;      /* out */
+  uint32_t* hit_idxs; /* [capacity][14], input hit indices, entries past len are 0 */
+  uint32_t* len;      /* [capacity] */
+  int32_t* seed_idx;  /* [capacity] */
+  int16_t* type;      /* [capacity] 4 T5, 5 pT3, 7 pT5, 8 pLS */
+} lst_output;
+
+lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_batch** out);
+void lst_batch_destroy(lst_batch* b);
+uint32_t lst_batch_contexts(const lst_batch* b);
+/* n_ev events from host arrays (prepareInput on the host per event);
+ * out: NULL or n_ev output descriptors. */
+lst_status lst_run_batch(lst_batch* b, const lst_input* events, uint32_t n_ev, lst_output* out);
+/* Throughput path: prepare n_ev events once and keep them in HBM (slots 0..n_ev-1) ... */
+lst_status lst_batch_stage(lst_batch* b, const lst_input* events, uint32_t n_ev);
+/* ... then run n_ev events from slot `first` on with no host preparation;
+ * event i uses slot (first + i) modulo the number of staged slots. */
+lst_status lst_batch_run_staged(lst_batch* b, uint32_t first, uint32_t n_ev, lst_output* out);
+
 /* Per-kernel device timing (hipEvents on the event's stream).  After a run
  * with timing on, lst_event_kernel_times writes the durations in ms of the
  * last launch of: 0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,

@@ -90,6 +90,20 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _make_input(ev: dict):
+    """lst_input over an event dict's see_* / ph2_* arrays (kept alive by the caller)."""
+    a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
+    inp = _Input(
+        *[_p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
+                             "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ",
+                             "see_stateTrajGlbPx", "see_stateTrajGlbPy", "see_stateTrajGlbPz", "see_q",
+                             "see_hitIdx", "see_hitIdx_off")],
+        len(a["see_px"]),
+        *[_p(a[k]) for k in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")],
+        len(a["ph2_x"]))
+    return inp, list(a.values())
+
+
 class LSTESData:
     """Geometry resident in HBM (``SDL::loadAndFillESDevice``, SDL/LSTESData.cc:87-113)."""
 
@@ -214,22 +228,100 @@ class Event:
     # --- SDL::LST::run ----------------------------------------------------------
     def run_lst(self, ev: dict, upto: str = "t3"):
         """``SDL::LST::run`` (SDL/LST.cc:12-161) on one synthetic/real event dict."""
-        a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
-        self._keep = list(a.values())
-        inp = _Input(
-            *[_p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
-                                 "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ",
-                                 "see_stateTrajGlbPx", "see_stateTrajGlbPy", "see_stateTrajGlbPz", "see_q",
-                                 "see_hitIdx", "see_hitIdx_off")],
-            len(a["see_px"]),
-            *[_p(a[k]) for k in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")],
-            len(a["ph2_x"]))
+        inp, self._keep = _make_input(ev)
         _check(lib().lst_run(self.h, C.byref(inp), STAGES[upto]))
         return self
 
     def close(self):
         if self.h:
             lib().lst_event_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Output(C.Structure):
+    _fields_ = [("capacity", C.c_uint32), ("n_tc", C.c_uint32), ("hit_idxs", C.c_void_p), ("len", C.c_void_p),
+                ("seed_idx", C.c_void_p), ("type", C.c_void_p)]
+
+
+class Batch:
+    """Batched engine: ``n_contexts`` event contexts on one GPU run a list of
+    events round robin (lst_batch_* in include/lst_amd.h).  Per event the
+    output lists equal ``Event.run_lst(ev, "tc").getOutput()``."""
+
+    def __init__(self, es: LSTESData, n_contexts: int = 4, t5_mode: int = LST_T5_PLAIN_CHI2):
+        L = lib()
+        P = C.c_void_p
+        L.lst_batch_create.argtypes = [P, C.c_uint32, C.c_int, C.POINTER(P)]
+        L.lst_batch_destroy.argtypes = [P]
+        L.lst_run_batch.argtypes = [P, P, C.c_uint32, P]
+        L.lst_batch_stage.argtypes = [P, P, C.c_uint32]
+        L.lst_batch_run_staged.argtypes = [P, C.c_uint32, C.c_uint32, P]
+        self.es = es
+        self.h = C.c_void_p()
+        _check(L.lst_batch_create(es.h, n_contexts, t5_mode, C.byref(self.h)))
+
+    @staticmethod
+    def _inputs(events):
+        arr = (_Input * len(events))()
+        keep = []
+        for i, ev in enumerate(events):
+            arr[i], k = _make_input(ev)
+            keep.append(k)
+        return arr, keep
+
+    @staticmethod
+    def _outputs(n, capacity):
+        bufs = [{"hits": np.zeros((capacity, 14), np.uint32), "len": np.zeros(capacity, np.uint32),
+                 "seed": np.zeros(capacity, np.int32), "type": np.zeros(capacity, np.int16)} for _ in range(n)]
+        outs = (_Output * n)()
+        for o, b in zip(outs, bufs):
+            o.capacity = capacity
+            o.hit_idxs, o.len, o.seed_idx, o.type = (b["hits"].ctypes.data, b["len"].ctypes.data,
+                                                    b["seed"].ctypes.data, b["type"].ctypes.data)
+        return outs, bufs
+
+    @staticmethod
+    def _collect(outs, bufs):
+        res = []
+        for o, b in zip(outs, bufs):
+            k = min(o.n_tc, o.capacity)
+            res.append({"n_tc": o.n_tc,
+                        "tc_hitIdxs": [list(b["hits"][i, :b["len"][i]]) for i in range(k)],
+                        "tc_len": b["len"][:k].copy(), "tc_seedIdx": b["seed"][:k].copy(),
+                        "tc_trackCandidateType": b["type"][:k].copy()})
+        return res
+
+    def run(self, events, capacity: int = 8192):
+        """lst_run_batch on host event dicts; returns one output dict per event."""
+        arr, keep = self._inputs(events)
+        outs, bufs = self._outputs(len(events), capacity)
+        _check(lib().lst_run_batch(self.h, arr, len(events), outs))
+        return self._collect(outs, bufs)
+
+    def stage(self, events):
+        """Prepare events once and keep them in HBM (slots 0..len-1)."""
+        arr, keep = self._inputs(events)
+        _check(lib().lst_batch_stage(self.h, arr, len(events)))
+
+    def run_staged(self, first: int, n: int, capacity: int | None = 8192):
+        """Run n events from staged slot `first` on (slots cycle modulo the staged
+        count); capacity None skips the output copies."""
+        if capacity is None:
+            _check(lib().lst_batch_run_staged(self.h, first, n, None))
+            return None
+        outs, bufs = self._outputs(n, capacity)
+        _check(lib().lst_batch_run_staged(self.h, first, n, outs))
+        return self._collect(outs, bufs)
+
+    def close(self):
+        if self.h:
+            lib().lst_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

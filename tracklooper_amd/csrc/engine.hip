@@ -385,6 +385,62 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
   return warn;
 }
 
+// prepareInput on the host, then the prepared arrays into one device allocation
+static lst_status stage_input(const lst_input& in, lst_event::Staged& st) {
+  lst::PreparedEvent p;
+  lst::prepare_event(in, p);
+  if (st.mem) {
+    HIP_OK(hipFree(st.mem));
+    st = lst_event::Staged();
+  }
+  const uint32_t nh = p.x.size();
+  const uint32_t np = std::min<uint32_t>(p.ptIn.size(), lst::kNMaxPixelSegments);
+  Carve c;
+  c.take(st.x, nh); c.take(st.y, nh); c.take(st.z, nh); c.take(st.det, nh); c.take(st.idx, nh);
+  for (int k = 0; k < 4; k++) c.take(st.h[k], np);
+  c.take(st.seed, np); c.take(st.dphi, np); c.take(st.pt, np); c.take(st.pterr, np); c.take(st.px, np);
+  c.take(st.py, np); c.take(st.pz, np); c.take(st.eta, np); c.take(st.etaerr, np); c.take(st.phi, np);
+  c.take(st.charge, np); c.take(st.superbin, np); c.take(st.ptype, np); c.take(st.quad, np);
+  HIP_OK(hipMalloc(&st.mem, c.off ? c.off : 256));
+  st.bytes = c.off;
+  c.bind(st.mem);
+  st.nHits = nh;
+  st.nPLS = np;
+  auto up = [&](void* d, const void* h, size_t b) { return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess; };
+  HIP_OK(up(st.x, p.x.data(), nh * 4)); HIP_OK(up(st.y, p.y.data(), nh * 4)); HIP_OK(up(st.z, p.z.data(), nh * 4));
+  HIP_OK(up(st.det, p.detId.data(), nh * 4)); HIP_OK(up(st.idx, p.idx.data(), nh * 4));
+  HIP_OK(up(st.h[0], p.h0.data(), np * 4)); HIP_OK(up(st.h[1], p.h1.data(), np * 4));
+  HIP_OK(up(st.h[2], p.h2.data(), np * 4)); HIP_OK(up(st.h[3], p.h3.data(), np * 4));
+  HIP_OK(up(st.seed, p.seedIdx.data(), np * 4)); HIP_OK(up(st.dphi, p.dPhiChange.data(), np * 4));
+  HIP_OK(up(st.pt, p.ptIn.data(), np * 4)); HIP_OK(up(st.pterr, p.ptErr.data(), np * 4));
+  HIP_OK(up(st.px, p.px.data(), np * 4)); HIP_OK(up(st.py, p.py.data(), np * 4)); HIP_OK(up(st.pz, p.pz.data(), np * 4));
+  HIP_OK(up(st.eta, p.eta.data(), np * 4)); HIP_OK(up(st.etaerr, p.etaErr.data(), np * 4));
+  HIP_OK(up(st.phi, p.phi.data(), np * 4)); HIP_OK(up(st.charge, p.charge.data(), np * 4));
+  HIP_OK(up(st.superbin, p.superbin.data(), np * 4)); HIP_OK(up(st.ptype, p.pixelType.data(), np));
+  HIP_OK(up(st.quad, p.isQuad.data(), np));
+  return LST_OK;
+}
+
+// the chain on inputs already in device memory, enqueued on the event's stream
+static lst_status run_staged(lst_event* ev, const lst_event::Staged& st, int last_stage) {
+  lst_status s = ev->add_hits(st.x, st.y, st.z, st.det, st.idx, st.nHits, true);
+  if (s != LST_OK || last_stage < 1) return s;
+  lst_pixel_segments p;
+  p.hit_indices0 = st.h[0]; p.hit_indices1 = st.h[1]; p.hit_indices2 = st.h[2]; p.hit_indices3 = st.h[3];
+  p.delta_phi_change = st.dphi; p.pt_in = st.pt; p.pt_err = st.pterr; p.px = st.px; p.py = st.py; p.pz = st.pz;
+  p.eta = st.eta; p.eta_err = st.etaerr; p.phi = st.phi; p.charge = st.charge; p.seed_idx = st.seed;
+  p.superbin = st.superbin; p.pixel_type = st.ptype; p.is_quad = st.quad; p.n_pls = st.nPLS;
+  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s;
+  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
+  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
+  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s;
+  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s;
+  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s;
+  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s;
+  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s;
+  return lst_event_create_track_candidates(ev);
+}
+
 extern "C" {
 
 const char* lst_last_error(void) { return g_err.c_str(); }
@@ -641,62 +697,17 @@ lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
 
 lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t slot) {
   if (!ev || !in) return fail(LST_E_INPUT, "null argument");
-  lst::PreparedEvent p;
-  lst::prepare_event(*in, p);
   if (ev->staged.size() <= slot) ev->staged.resize(slot + 1);
   lst_event::Staged& st = ev->staged[slot];
-  if (st.mem) {
-    HIP_OK(hipStreamSynchronize(ev->stream));
-    HIP_OK(hipFree(st.mem));
-    st = lst_event::Staged();
-  }
-  const uint32_t nh = p.x.size();
-  const uint32_t np = std::min<uint32_t>(p.ptIn.size(), lst::kNMaxPixelSegments);
-  Carve c;
-  c.take(st.x, nh); c.take(st.y, nh); c.take(st.z, nh); c.take(st.det, nh); c.take(st.idx, nh);
-  for (int k = 0; k < 4; k++) c.take(st.h[k], np);
-  c.take(st.seed, np); c.take(st.dphi, np); c.take(st.pt, np); c.take(st.pterr, np); c.take(st.px, np);
-  c.take(st.py, np); c.take(st.pz, np); c.take(st.eta, np); c.take(st.etaerr, np); c.take(st.phi, np);
-  c.take(st.charge, np); c.take(st.superbin, np); c.take(st.ptype, np); c.take(st.quad, np);
-  HIP_OK(hipMalloc(&st.mem, c.off ? c.off : 256));
-  st.bytes = c.off;
-  c.bind(st.mem);
-  st.nHits = nh;
-  st.nPLS = np;
-  auto up = [&](void* d, const void* h, size_t b) { return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess; };
-  HIP_OK(up(st.x, p.x.data(), nh * 4)); HIP_OK(up(st.y, p.y.data(), nh * 4)); HIP_OK(up(st.z, p.z.data(), nh * 4));
-  HIP_OK(up(st.det, p.detId.data(), nh * 4)); HIP_OK(up(st.idx, p.idx.data(), nh * 4));
-  HIP_OK(up(st.h[0], p.h0.data(), np * 4)); HIP_OK(up(st.h[1], p.h1.data(), np * 4));
-  HIP_OK(up(st.h[2], p.h2.data(), np * 4)); HIP_OK(up(st.h[3], p.h3.data(), np * 4));
-  HIP_OK(up(st.seed, p.seedIdx.data(), np * 4)); HIP_OK(up(st.dphi, p.dPhiChange.data(), np * 4));
-  HIP_OK(up(st.pt, p.ptIn.data(), np * 4)); HIP_OK(up(st.pterr, p.ptErr.data(), np * 4));
-  HIP_OK(up(st.px, p.px.data(), np * 4)); HIP_OK(up(st.py, p.py.data(), np * 4)); HIP_OK(up(st.pz, p.pz.data(), np * 4));
-  HIP_OK(up(st.eta, p.eta.data(), np * 4)); HIP_OK(up(st.etaerr, p.etaErr.data(), np * 4));
-  HIP_OK(up(st.phi, p.phi.data(), np * 4)); HIP_OK(up(st.charge, p.charge.data(), np * 4));
-  HIP_OK(up(st.superbin, p.superbin.data(), np * 4)); HIP_OK(up(st.ptype, p.pixelType.data(), np));
-  HIP_OK(up(st.quad, p.isQuad.data(), np));
-  return ev->allocate(nh);
+  if (st.mem) HIP_OK(hipStreamSynchronize(ev->stream));
+  lst_status s = stage_input(*in, st);
+  if (s != LST_OK) return s;
+  return ev->allocate(st.nHits);
 }
 
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
   if (!ev || slot >= ev->staged.size() || !ev->staged[slot].mem) return fail(LST_E_INPUT, "input slot not staged");
-  const lst_event::Staged& st = ev->staged[slot];
-  lst_status s = ev->add_hits(st.x, st.y, st.z, st.det, st.idx, st.nHits, true);
-  if (s != LST_OK || last_stage < 1) return s;
-  lst_pixel_segments p;
-  p.hit_indices0 = st.h[0]; p.hit_indices1 = st.h[1]; p.hit_indices2 = st.h[2]; p.hit_indices3 = st.h[3];
-  p.delta_phi_change = st.dphi; p.pt_in = st.pt; p.pt_err = st.pterr; p.px = st.px; p.py = st.py; p.pz = st.pz;
-  p.eta = st.eta; p.eta_err = st.etaerr; p.phi = st.phi; p.charge = st.charge; p.seed_idx = st.seed;
-  p.superbin = st.superbin; p.pixel_type = st.ptype; p.is_quad = st.quad; p.n_pls = st.nPLS;
-  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s;
-  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
-  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
-  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s;
-  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s;
-  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s;
-  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s;
-  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s;
-  return lst_event_create_track_candidates(ev);
+  return run_staged(ev, ev->staged[slot], last_stage);
 }
 
 lst_status lst_event_set_timing(lst_event* ev, int on) {
@@ -725,5 +736,167 @@ lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n) {
 }
 
 void* lst_event_stream(lst_event* ev) { return ev ? (void*)ev->stream : nullptr; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Several event contexts (arena + stream each) run a batch of events round
+// robin; the host waits on a context only when it hands it the next event, so
+// the other contexts keep the GPU busy meanwhile.  Outputs travel through a
+// pinned buffer per context.
+struct lst_batch {
+  lst_es* es = nullptr;
+  std::vector<lst_event*> ctx;
+  struct Out {
+    char* pinned = nullptr;  // n_tc, then hits [cap*14], len [cap], seed [cap], type [cap]
+    uint32_t cap = 0;
+    int64_t pending = -1;    // batch position whose output is in flight
+    lst_output* dst = nullptr;
+  };
+  std::vector<Out> outs;
+  std::vector<lst_event::Staged> staged;
+};
+
+namespace {
+constexpr uint32_t kTCCap = lst::kNMaxPixelTrackCandidates + lst::kNMaxNonPixelTrackCandidates;
+
+size_t out_bytes(uint32_t cap) { return 256 + (size_t)cap * (14 * 4 + 4 + 4 + 2); }
+
+// enqueue the output copies of context c's event into its pinned buffer
+lst_status enqueue_output(lst_batch* b, uint32_t c, lst_output* dst) {
+  lst_batch::Out& o = b->outs[c];
+  lst_event* ev = b->ctx[c];
+  const uint32_t cap = std::min(dst->capacity, kTCCap);
+  if (!o.pinned || o.cap < cap) {
+    if (o.pinned) HIP_OK(hipHostFree(o.pinned));
+    o.pinned = nullptr;
+    HIP_OK(hipHostMalloc((void**)&o.pinned, out_bytes(cap), hipHostMallocDefault));
+    o.cap = cap;
+  }
+  char* p = o.pinned;
+  HIP_OK(hipMemcpyAsync(p, ev->v.R.scalars + lst::kNTrackCandidates, 4, hipMemcpyDeviceToHost, ev->stream));
+  p += 256;
+  if (cap) {
+    HIP_OK(hipMemcpyAsync(p, ev->v.W.outHits, (size_t)cap * 56, hipMemcpyDeviceToHost, ev->stream));
+    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 56, ev->v.W.outLen, (size_t)cap * 4, hipMemcpyDeviceToHost, ev->stream));
+    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 60, ev->v.C.pixelSeedIndex, (size_t)cap * 4, hipMemcpyDeviceToHost,
+                          ev->stream));
+    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 64, ev->v.C.trackCandidateType, (size_t)cap * 2, hipMemcpyDeviceToHost,
+                          ev->stream));
+  }
+  o.dst = dst;
+  return LST_OK;
+}
+
+// wait for context c and hand its finished event's output to the caller
+lst_status finish_context(lst_batch* b, uint32_t c, lst_status& warn) {
+  lst_batch::Out& o = b->outs[c];
+  lst_event* ev = b->ctx[c];
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  if (o.pending >= 0 && o.dst) {
+    lst_output* d = o.dst;
+    const uint32_t cap = std::min(d->capacity, kTCCap);
+    uint32_t n;
+    std::memcpy(&n, o.pinned, 4);
+    d->n_tc = n;
+    const uint32_t m = std::min(n, cap);
+    if (n > d->capacity) warn = LST_W_TRUNCATED;
+    const char* p = o.pinned + 256;
+    if (d->hit_idxs) std::memcpy(d->hit_idxs, p, (size_t)m * 56);
+    if (d->len) std::memcpy(d->len, p + (size_t)cap * 56, (size_t)m * 4);
+    if (d->seed_idx) std::memcpy(d->seed_idx, p + (size_t)cap * 60, (size_t)m * 4);
+    if (d->type) std::memcpy(d->type, p + (size_t)cap * 64, (size_t)m * 2);
+  }
+  o.pending = -1;
+  o.dst = nullptr;
+  return LST_OK;
+}
+
+// run events [0, n) through the contexts; `launch(ctx, i)` enqueues event i
+template <class F>
+lst_status run_round_robin(lst_batch* b, uint32_t n, lst_output* out, F launch) {
+  const uint32_t K = (uint32_t)b->ctx.size();
+  lst_status warn = LST_OK, s;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t c = i % K;
+    if (b->outs[c].pending >= 0 && (s = finish_context(b, c, warn)) != LST_OK) return s;
+    s = launch(b->ctx[c], i);
+    if ((s & 0xff) != LST_OK) return s;
+    if (s != LST_OK) warn = s;
+    if (out && (s = enqueue_output(b, c, &out[i])) != LST_OK) return s;
+    b->outs[c].pending = i;
+  }
+  for (uint32_t c = 0; c < K; c++)
+    if ((s = finish_context(b, c, warn)) != LST_OK) return s;
+  return warn;
+}
+}  // namespace
+
+extern "C" {
+
+lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_batch** out) {
+  if (!out) return fail(LST_E_INPUT, "null argument");
+  *out = nullptr;
+  if (!es) return fail(LST_E_INPUT, "null event setup");
+  if (n_contexts == 0 || n_contexts > 64) return fail(LST_E_INPUT, "a batch needs 1..64 event contexts");
+  auto b = std::make_unique<lst_batch>();
+  b->es = es;
+  for (uint32_t c = 0; c < n_contexts; c++) {
+    lst_event* ev = nullptr;
+    lst_status s = lst_event_create(es, nullptr, t5_mode, &ev);
+    if (s != LST_OK) {
+      lst_batch_destroy(b.release());
+      return s;
+    }
+    b->ctx.push_back(ev);
+  }
+  b->outs.resize(n_contexts);
+  *out = b.release();
+  return LST_OK;
+}
+
+void lst_batch_destroy(lst_batch* b) {
+  if (!b) return;
+  for (lst_event* ev : b->ctx) lst_event_destroy(ev);
+  for (auto& o : b->outs)
+    if (o.pinned) (void)hipHostFree(o.pinned);
+  for (auto& st : b->staged)
+    if (st.mem) (void)hipFree(st.mem);
+  delete b;
+}
+
+lst_status lst_run_batch(lst_batch* b, const lst_input* events, uint32_t n_ev, lst_output* out) {
+  if (!b || (n_ev && !events)) return fail(LST_E_INPUT, "null argument");
+  return run_round_robin(b, n_ev, out, [&](lst_event* ev, uint32_t i) { return lst_run(ev, &events[i], 9); });
+}
+
+lst_status lst_batch_stage(lst_batch* b, const lst_input* events, uint32_t n_ev) {
+  if (!b || (n_ev && !events)) return fail(LST_E_INPUT, "null argument");
+  for (lst_event* ev : b->ctx) HIP_OK(hipStreamSynchronize(ev->stream));
+  if (b->staged.size() < n_ev) b->staged.resize(n_ev);
+  uint32_t maxHits = 0;
+  for (uint32_t i = 0; i < n_ev; i++) {
+    lst_status s = stage_input(events[i], b->staged[i]);
+    if (s != LST_OK) return s;
+    maxHits = std::max(maxHits, b->staged[i].nHits);
+  }
+  for (lst_event* ev : b->ctx) {  // size every arena for the largest staged event up front
+    lst_status s = ev->allocate(maxHits);
+    if (s != LST_OK) return s;
+  }
+  return LST_OK;
+}
+
+lst_status lst_batch_run_staged(lst_batch* b, uint32_t first, uint32_t n_ev, lst_output* out) {
+  if (!b) return fail(LST_E_INPUT, "null batch");
+  const size_t ns = b->staged.size();
+  if (ns == 0 || first >= ns) return fail(LST_E_INPUT, "batch slots not staged");
+  for (const auto& st : b->staged)
+    if (!st.mem) return fail(LST_E_INPUT, "batch slot not staged");
+  return run_round_robin(b, n_ev, out,
+                         [&](lst_event* ev, uint32_t i) { return run_staged(ev, b->staged[(first + i) % ns], 9); });
+}
+
+uint32_t lst_batch_contexts(const lst_batch* b) { return b ? (uint32_t)b->ctx.size() : 0; }
 
 }  // extern "C"
