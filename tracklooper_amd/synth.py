@@ -263,6 +263,7 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
         pt = np.exp(rng.uniform(np.log(0.5), np.log(50.0), nt))
         vz = rng.normal(0.0, 5.0, nt)
         noise_frac = 0.0
+        hard = np.ones(nt, bool)
     elif kind == "pu":
         npu = 200 if n_pu is None else int(n_pu)
         nsoft = 35 * npu
@@ -271,6 +272,7 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
         vtx = rng.normal(0.0, 5.0, npu + 1)
         vz = np.concatenate([vtx[rng.integers(0, npu, nsoft)], np.full(200, vtx[npu])])
         noise_frac = 0.01
+        hard = np.concatenate([np.zeros(nsoft, bool), np.ones(200, bool)])  # the 200 tracks of the signal vertex
     else:
         raise ValueError(kind)
     eta = rng.uniform(-2.4, 2.4, nt)
@@ -333,8 +335,17 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
         see_hitIdx=np.array([x for s in seeds for x in s["hits"]], np.int32),
         see_hitIdx_off=hit_off,
         truth_hit_trk=trk.astype(np.int64), truth_seed_trk=np.array([s["trk"] for s in seeds], np.int64),
-        truth_pt=h.pt, truth_eta=h.eta, truth_phi=h.phi0, truth_vz=h.vz, truth_q=h.q,
+        truth_pix_trk=np.array([trk for trk in _pixel_hit_tracks(seeds, pix_id)], np.int64),
+        truth_pt=h.pt, truth_eta=h.eta, truth_phi=h.phi0, truth_vz=h.vz, truth_q=h.q, truth_is_hard=hard,
     )
+
+
+def _pixel_hit_tracks(seeds, n_pix):
+    """Track of every pixel hit (pixel hits are created per seeded helix)."""
+    trk = np.full(n_pix, -1, np.int64)
+    for s in seeds:
+        trk[s["hits"]] = s["trk"]
+    return trk
 
 
 _GEO = {}
