@@ -217,6 +217,8 @@ def main():
     if not tc_match:
         raise SystemExit("batched track candidates differ from the per-event ones")
     batch.close()
+    from tracklooper_amd import metrics
+    physics = metrics.summarize([metrics.event_metrics(o, e) for o, e in zip(per_event_out, events)])
 
     avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
     per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}
@@ -248,6 +250,7 @@ def main():
             "per_event_ms_per_step": ms_per_step,
             "batched_over_per_event": (args.steps / wall_batch) / (args.steps / wall),
             "tc_sets_match": tc_match,
+            "physics": {k: physics[k] for k in ("sim_tracks", "efficiency", "fake_rate", "duplicate_rate")},
             "contexts": args.contexts,
             "geometry": "OT800_IT615_pt0.8",
             "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],

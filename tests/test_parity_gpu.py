@@ -189,3 +189,50 @@ def test_t5_dnn_mode_per_module_builder(engine, oracle_geometry, monkeypatch):
     ev = synth.generate_event("pu", 2, n_pu=50)
     prod, ref = _run_both(engine, oracle_geometry, ev, DEEPEST, t5_mode=lst.LST_T5_DNN)
     compare(prod, ref, DEEPEST)
+
+
+def test_physics_metrics_match_oracle(engine, oracle_geometry):
+    """Efficiency / fake / duplicate flags computed on the engine's output equal the oracle's."""
+    from tracklooper_amd import metrics, synth
+    ev = synth.generate_event("pu", 5, n_pu=50)
+    prod, ref = _run_both(engine, oracle_geometry, ev, "tc")
+    flat, off, typ = ref.field("out.hitIdxs"), ref.field("out.hitOffsets"), ref.field("out.trackCandidateType")
+    n = ref.scalar("nTrackCandidates")
+    ref_out = {"tc_hitIdxs": [list(flat[off[i]:off[i + 1]]) for i in range(n)], "tc_trackCandidateType": typ[:n]}
+    a = metrics.event_metrics(prod.getOutput(), ev)
+    b = metrics.event_metrics(ref_out, ev)
+    np.testing.assert_array_equal(a["tc_is_fake"], b["tc_is_fake"])
+    np.testing.assert_array_equal(a["tc_is_duplicate"], b["tc_is_duplicate"])
+    np.testing.assert_array_equal(a["sim_tc_matched"], b["sim_tc_matched"])
+    s = metrics.summarize([a])
+    assert s["efficiency"] > 0.9 and s["fake_rate"] < 0.05
+    print(s)
+
+
+@pytest.fixture(scope="module")
+def pt06(geom_dir):
+    """The pt0.6 geometry (module-map fan-out 68) with pT cut 0.6: the DNN configuration."""
+    import tracklooper_amd as lst
+    from oracle import orc
+    from tracklooper_amd import synth
+    d = synth.geometry_dir("OT800_IT615_pt0.6")
+    return lst.LSTESData(d, pt_cut=0.6), orc.Geometry(d, stride=68)
+
+
+@pytest.mark.parametrize("case", [("muon", 0, 1), ("pu", 0, 0), ("pu", 1, 1)])
+def test_pt06_geometry_all_stages(pt06, case):
+    import tracklooper_amd as lst
+    from oracle import orc
+    from tracklooper_amd import synth
+    es, geom = pt06
+    kind, index, mode = case
+    g = "OT800_IT615_pt0.6"
+    ev = synth.generate_event(kind, index, geom_name=g) if kind == "muon" else \
+        synth.generate_event(kind, index, n_pu=50, geom_name=g)
+    prod = lst.Event(es, t5_mode=mode).run_lst(ev, DEEPEST)
+    prod.synchronize()
+    ref = orc.Event(geom, pt_cut=0.6, t5_mode=mode).run_lst(ev, DEEPEST)
+    rep = compare(prod, ref, DEEPEST)
+    assert ref.field("t5.nQuintuplets").sum() > 0 and ref.scalar("nTrackCandidates") > 0
+    print(f"pt0.6 mode {mode}: T5={ref.field('t5.nQuintuplets').sum()} TC={ref.scalar('nTrackCandidates')}; "
+          f"floats compared {rep.float_compared}, not bit-identical {rep.float_inexact}")

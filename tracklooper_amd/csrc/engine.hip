@@ -53,6 +53,34 @@ struct FieldRef {
   size_t count;
 };
 
+// Input copies of one stage: from device memory, the 4-byte aligned buffers go
+// in one multi-copy launch (others and host sources take hipMemcpyAsync).
+struct CopyBatch {
+  bool onDevice;
+  hipStream_t stream;
+  lst::CopyList list;
+  hipError_t err = hipSuccess;
+  CopyBatch(bool d, hipStream_t s) : onDevice(d), stream(s) {}
+  void add(void* dst, const void* src, size_t bytes) {
+    if (!bytes || err != hipSuccess) return;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 3u) == 0;
+    if (onDevice && aligned && bytes < (1ull << 32) && list.n < lst::kMaxCopies) {
+      list.src[list.n] = src;
+      list.dst[list.n] = dst;
+      list.bytes[list.n] = (uint32_t)bytes;
+      list.n++;
+    } else {
+      err = hipMemcpyAsync(dst, src, bytes, onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream);
+    }
+  }
+  hipError_t flush() {
+    if (err != hipSuccess) return err;
+    lst::launch_copy_multi(list, stream);
+    list.n = 0;
+    return hipGetLastError();
+  }
+};
+
 // Bump allocator over one device allocation; 256-byte aligned sub-buffers.
 struct Carve {
   size_t off = 0;
@@ -172,9 +200,21 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1);
   c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
   c.take(e.W.tcPix, 4 * (size_t)tcCap);
+  c.take(e.W.plsBinned, plsCap); c.take(e.W.plsBinOff, lst::kEtaBinOffsets);
+  c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
+  c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
+  c.take(e.W.pt3Mask, lst::kPixMaskWords); c.take(e.W.pt3MaskBase, plsCap);
+  c.take(e.W.pt5ParkAt, lst::kPixMaskWords); c.take(e.W.pt5Park, (size_t)lst::kPT5ParkWords * lst::kPixParkMax);
+  c.take(e.W.pt3ParkAt, lst::kPixMaskWords); c.take(e.W.pt3Park, (size_t)lst::kPT3ParkWords * lst::kPixParkMax);
+  c.take(e.W.pt5HashHead, lst::kPT5HashBuckets); c.take(e.W.pt5HashNext, 14 * (size_t)pt5Cap);
+  c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax);
+  c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDParkMax);
+  c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
+  c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSParkMax);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
   c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3ResultMax);
+  c.take(e.W3.mdHead, mdCap); c.take(e.W3.segNext, segCap);
   c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
   c.take(e.W5.resIdx, lst::kT5CandMax);
   c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
@@ -326,16 +366,19 @@ lst_status lst_event::add_hits(const float* x, const float* y, const float* z, c
                                const uint32_t* idx, uint32_t n, bool onDevice) {
   lst_status s = allocate(n);
   if (s != LST_OK) return s;
-  const hipMemcpyKind k = onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   if (n) {
-    HIP_OK(hipMemcpyAsync(v.H.xs, x, n * sizeof(float), k, stream));
-    HIP_OK(hipMemcpyAsync(v.H.ys, y, n * sizeof(float), k, stream));
-    HIP_OK(hipMemcpyAsync(v.H.zs, z, n * sizeof(float), k, stream));
-    HIP_OK(hipMemcpyAsync(v.H.detid, det, n * sizeof(uint32_t), k, stream));
-    HIP_OK(hipMemcpyAsync(v.H.idxs, idx, n * sizeof(uint32_t), k, stream));
+    CopyBatch cb(onDevice, stream);
+    cb.add(v.H.xs, x, n * sizeof(float));
+    cb.add(v.H.ys, y, n * sizeof(float));
+    cb.add(v.H.zs, z, n * sizeof(float));
+    cb.add(v.H.detid, det, n * sizeof(uint32_t));
+    cb.add(v.H.idxs, idx, n * sizeof(uint32_t));
+    HIP_OK(cb.flush());
   }
   v.nHits = n;
   v.nPLS = 0;
+  lst::launch_set_scalar(v.R.scalars + lst::kEvNHits, n, stream);
+  lst::launch_set_scalar(v.R.scalars + lst::kEvNPLS, 0, stream);
   lst::launch_hits(v, stream, tm());
   HIP_OK(hipGetLastError());
   stage = 0;
@@ -350,7 +393,6 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
     n = lst::kNMaxPixelSegments;
     warn = LST_W_TRUNCATED;
   }
-  const hipMemcpyKind k = onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   if (!onDevice) {  // pseudo-hit indices must address hits of this event
     const uint32_t* hi[4] = {p.hit_indices0, p.hit_indices1, p.hit_indices2, p.hit_indices3};
     for (int q = 0; q < 4; q++)
@@ -358,26 +400,29 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
         if (hi[q][t] >= v.nHits) return fail(LST_E_INPUT, "pixel segment hit index out of range");
   }
   if (n) {
-    HIP_OK(hipMemcpyAsync(plsHitIdx[0], p.hit_indices0, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(plsHitIdx[1], p.hit_indices1, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(plsHitIdx[2], p.hit_indices2, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(plsHitIdx[3], p.hit_indices3, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(plsDPhiChange, p.delta_phi_change, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.ptIn, p.pt_in, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.ptErr, p.pt_err, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.px, p.px, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.py, p.py, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.pz, p.pz, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.eta, p.eta, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.etaErr, p.eta_err, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.phi, p.phi, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.charge, p.charge, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.seedIdx, p.seed_idx, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.superbin, p.superbin, n * 4, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.pixelType, p.pixel_type, n, k, stream));
-    HIP_OK(hipMemcpyAsync(v.P.isQuad, p.is_quad, n, k, stream));
+    CopyBatch cb(onDevice, stream);
+    cb.add(plsHitIdx[0], p.hit_indices0, n * 4);
+    cb.add(plsHitIdx[1], p.hit_indices1, n * 4);
+    cb.add(plsHitIdx[2], p.hit_indices2, n * 4);
+    cb.add(plsHitIdx[3], p.hit_indices3, n * 4);
+    cb.add(plsDPhiChange, p.delta_phi_change, n * 4);
+    cb.add(v.P.ptIn, p.pt_in, n * 4);
+    cb.add(v.P.ptErr, p.pt_err, n * 4);
+    cb.add(v.P.px, p.px, n * 4);
+    cb.add(v.P.py, p.py, n * 4);
+    cb.add(v.P.pz, p.pz, n * 4);
+    cb.add(v.P.eta, p.eta, n * 4);
+    cb.add(v.P.etaErr, p.eta_err, n * 4);
+    cb.add(v.P.phi, p.phi, n * 4);
+    cb.add(v.P.charge, p.charge, n * 4);
+    cb.add(v.P.seedIdx, p.seed_idx, n * 4);
+    cb.add(v.P.superbin, p.superbin, n * 4);
+    cb.add(v.P.pixelType, p.pixel_type, n);
+    cb.add(v.P.isQuad, p.is_quad, n);
+    HIP_OK(cb.flush());
   }
   v.nPLS = n;
+  lst::launch_set_scalar(v.R.scalars + lst::kEvNPLS, n, stream);
   const uint32_t* hi[4] = {plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]};
   lst::launch_pixel_segments(v, hi, plsDPhiChange, stream, tm());
   HIP_OK(hipGetLastError());

@@ -87,6 +87,32 @@ struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
 struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };
 struct TCSoA { LST_TC_FIELDS(LST_DECL_FIELD) };
 
+// Work arrays of the flattened mini-doublet builder: every (lower hit, upper
+// hit) pair of every module gets a global index L (module, then lower hit,
+// then upper hit: the reference's serial order); passing pairs park their
+// MDOut, and a wave per module places them in L order.
+constexpr uint32_t kMDCandMax = 1u << 22;
+constexpr uint32_t kMDParkMax = 1u << 20;
+constexpr int kMDParkWords = 9;
+struct MDWork {
+  uint32_t* candOff;  // [nLower + 1] first candidate of each module, then the total
+  uint32_t* resIdx;   // [kMDCandMax] park slot of a passing candidate, ~0: failed
+  uint32_t* park;     // [kMDParkWords * kMDParkMax]
+};
+
+// Work arrays of the flattened line-segment builder: candidates of module m
+// are its (connected module k, inner MD, outer MD) triples in that order;
+// connPre holds, per module, the first candidate of each connected module.
+constexpr uint32_t kLSCandMax = 1u << 23;
+constexpr uint32_t kLSParkMax = 1u << 21;
+constexpr int kLSParkWords = 6;
+struct LSWork {
+  uint32_t* connPre;  // [nLower * (stride + 1)] per module: candidate prefix over its connected modules
+  uint32_t* candOff;  // [nLower + 1]
+  uint32_t* resIdx;   // [kLSCandMax]
+  uint32_t* park;     // [kLSParkWords * kLSParkMax] SegOut of passing candidates
+};
+
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).
 constexpr uint32_t kT3CandMax = 1u << 22;
 constexpr uint32_t kT3ResultMax = 1u << 20;
@@ -95,8 +121,10 @@ struct T3Work {
   uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
   uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
   uint32_t* resIdx;   // [kT3CandMax]
-  uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check
+  uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check, any order
   uint32_t* results;  // [kT3ResultWords * kT3ResultMax]
+  uint32_t* mdHead;   // [MD capacity] per outer-tracker MD: last linked segment starting there, ~0: none
+  uint32_t* segNext;  // [segment capacity] next segment with the same inner MD
 };
 
 // Work arrays of the flattened T5 builder: every (inner T3, outer T3)
@@ -129,7 +157,29 @@ struct PixelWork {
   float* tcPhi;           // [TC capacity]
   uint8_t* tcKind;        // [TC capacity] 0: not compared, 1: T5, 2: pT3 or pT5
   int32_t* tcPix;         // [TC capacity * 4] ntuple indices of the pLS hits of a pT3/pT5
+  uint32_t* plsBinned;    // [pLS capacity] pLS indices listed eta bin by eta bin
+  uint32_t* plsBinOff;    // [kEtaBinOffsets] first list position of each eta bin, then the end
+  uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
+  uint32_t* t5BinOff;     // [kEtaBinOffsets]
+  unsigned long long* pt5Mask;  // [kPixMaskWords] pass mask per 64-candidate chunk of each pLS (pT5)
+  uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none recorded
+  unsigned long long* pt3Mask;  // [kPixMaskWords] (pT3)
+  uint32_t* pt3MaskBase;        // [pLS capacity]
+  uint32_t* pt5ParkAt;          // [kPixMaskWords] per chunk: first parked result of its passing lanes, ~0: none
+  uint32_t* pt5Park;            // [kPT5ParkWords * kPixParkMax] t5 index + PT5Result of passing candidates
+  uint32_t* pt3ParkAt;          // [kPixMaskWords]
+  uint32_t* pt3Park;            // [kPT3ParkWords * kPixParkMax] t3 index + PT3Result
+  uint32_t* pt5HashHead;        // [kPT5HashBuckets] last (pT5, position) entry of each hit bucket, ~0: none
+  uint32_t* pt5HashNext;        // [14 * pT5 capacity] next entry of the same bucket
 };
+constexpr uint32_t kPT5HashBuckets = 1u << 15;
+constexpr uint32_t kPixMaskWords = 1u << 20;  // 64 Mi candidates per event; beyond: evaluated twice
+constexpr uint32_t kPixParkMax = 1u << 18;    // passing candidates parked per event; beyond: evaluated twice
+constexpr uint32_t kPT5ParkWords = 8;
+constexpr uint32_t kPT3ParkWords = 6;
+constexpr int kEtaBins = 64;                    // finite eta bins of width 1/8 over [-4, 4)
+constexpr int kNanEtaBin = kEtaBins;            // the bin of NaN eta
+constexpr int kEtaBinOffsets = kEtaBins + 2;    // starts of bins 0..kNanEtaBin, then the end
 
 // Per-module counters and ranges (SDL/Module.h:21-85, SDL/Hit.h:28-32).
 struct RangesSoA {
@@ -179,6 +229,16 @@ enum Scalar : int {
   kT3List = 18,
   kT3Results = 19,
   kT5Listed = 20,
+  kPT5MaskWords = 21,
+  kPT3MaskWords = 22,
+  kPT5Parked = 23,
+  kPT3Parked = 24,
+  kMDCandTotal = 25,
+  kMDParked = 26,
+  kLSCandTotal = 27,
+  kLSParked = 28,
+  kEvNHits = 29,  // hits of the event (set before the hit kernels run)
+  kEvNPLS = 30,   // pixel line segments of the event
   kNScalars = 32
 };
 }  // namespace lst

@@ -18,6 +18,8 @@ struct EventView {  // everything a stage kernel needs, by value
   PT3SoA P3;
   TCSoA C;
   PixelWork W;
+  MDWork W2;
+  LSWork WL;
   T3Work W3;
   T5Work W5;
   RangesSoA R;
@@ -76,6 +78,18 @@ inline const KernelTimers& no_timers() {
   static const KernelTimers off{};
   return off;
 }
+
+// Device-to-device copy of up to kMaxCopies 4-byte aligned buffers in one launch.
+constexpr int kMaxCopies = 24;
+struct CopyList {
+  const void* src[kMaxCopies];
+  void* dst[kMaxCopies];
+  uint32_t bytes[kMaxCopies];
+  int n = 0;
+};
+void launch_copy_multi(const CopyList& c, hipStream_t s);
+// writes one device word (the event's hit / pLS counts) in stream order
+void launch_set_scalar(uint32_t* p, uint32_t value, hipStream_t s);
 
 void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
 void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,

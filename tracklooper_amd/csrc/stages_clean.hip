@@ -18,6 +18,8 @@ namespace lst {
 namespace {
 
 constexpr int kTile = 256;
+constexpr int kBandShares = 16;  // y shares per x tile in the eta-banded kernels
+constexpr unsigned kPlsTiles = 64;  // x-tile work-groups of the pLS kernels (they stride over the tiles)
 
 // OR `bits` into byte k of an int8/uint8 flag array (256-byte aligned base)
 __device__ __forceinline__ void or_byte(void* base, uint32_t k, uint32_t bits) {
@@ -53,6 +55,67 @@ inline dim3 pair_grid(uint32_t nxCap, uint32_t nxTyp, uint32_t nyCap) {
   return dim3(gx, gy);
 }
 
+// ------------------------------------------------------------- eta bands
+// The T5 and pLS rules flag a pair only when |d eta| <= 0.1 or when d eta is
+// NaN (the test `> 0.1` is then false).  Objects are binned by eta (bins of
+// width 1/8 > 0.1 over [-4, 4), the ends clamped, NaN in a bin of its own) and
+// listed bin by bin; an x tile whose bins span [bmin, bmax] then only meets
+// the y's of bins bmin-1 .. bmax+1 plus the NaN bin (every y when the tile
+// holds a NaN x).  That is a superset of the pairs that can flag, so the flags
+// are unchanged.
+__device__ __forceinline__ int eta_bin(float eta) {
+  if (eta != eta) return kNanEtaBin;
+  if (!(eta > -4.f)) return 0;
+  if (!(eta < 4.f)) return kEtaBins - 1;
+  const int b = (int)((eta + 4.f) * 8.f);
+  return b < kEtaBins ? b : kEtaBins - 1;
+}
+
+// one 1024-lane work-group: histogram, offsets, scatter (order inside a bin is free)
+template <class IdOf, class EtaOf>
+__device__ void bin_by_eta(uint32_t n, IdOf idOf, EtaOf etaOf, uint32_t* binned, uint32_t* off) {
+  __shared__ unsigned cnt[kEtaBinOffsets], cur[kEtaBinOffsets];
+  for (int b = threadIdx.x; b < kEtaBinOffsets; b += blockDim.x) cnt[b] = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[eta_bin(etaOf(idOf(i)))], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned run = 0;
+    for (int b = 0; b <= kNanEtaBin; b++) {
+      off[b] = run;
+      cur[b] = run;
+      run += cnt[b];
+    }
+    off[kNanEtaBin + 1] = run;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t id = idOf(i);
+    binned[atomicAdd(&cur[eta_bin(etaOf(id))], 1u)] = id;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_bin_pls(EventView v) {
+  bin_by_eta(
+      v.R.scalars[kEvNPLS], [](uint32_t i) { return i; }, [&](uint32_t i) { return v.P.eta[i]; }, v.W.plsBinned,
+      v.W.plsBinOff);
+}
+
+// y positions a tile meets: [b1, b1 + n1) then [b2, b2 + n2) of the binned list
+struct Band {
+  uint32_t b1, n1, b2, n2;
+};
+__device__ __forceinline__ Band tile_band(const uint32_t* off, int bmin, int bmax) {
+  const uint32_t end = off[kNanEtaBin + 1];
+  if (bmax == kNanEtaBin) return {0, end, 0, 0};
+  const uint32_t b1 = off[bmin > 0 ? bmin - 1 : 0];
+  const uint32_t e1 = off[bmax + 2 < kNanEtaBin ? bmax + 2 : kNanEtaBin];
+  return {b1, e1 - b1, off[kNanEtaBin], end - off[kNanEtaBin]};
+}
+__device__ __forceinline__ uint32_t band_at(const Band& B, uint32_t k) {
+  return k < B.n1 ? B.b1 + k : B.b2 + (k - B.n1);
+}
+
 // ------------------------------------------------------------ pLS cleaning
 // checkHitspLS: for a pair (lo < hi) within 0.1 in eta the removal target is
 // the triplet seed over the quadruplet, else the higher score (ties: lo); the
@@ -62,17 +125,22 @@ inline dim3 pair_grid(uint32_t nxCap, uint32_t nxTyp, uint32_t nyCap) {
 template <bool kSecond>
 __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
   __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
-  __shared__ uint32_t sHits[4 * kTile];
+  __shared__ uint32_t sHits[4 * kTile], sId[kTile];
   __shared__ uint8_t sQuad[kTile], sOk[kTile];
-  const uint32_t n = v.nPLS;
-  const YShare ys = y_share(n);
+  const uint32_t n = v.R.scalars[kEvNPLS];
   for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
-    const uint32_t k = x0 + threadIdx.x;
-    bool active = k < n;
+    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
+    const Band B = tile_band(v.W.plsBinOff, eta_bin(v.P.eta[v.W.plsBinned[x0]]),
+                             eta_bin(v.P.eta[v.W.plsBinned[xl]]));
+    const YShare ys = y_share(B.n1 + B.n2);
+    const uint32_t xi = x0 + threadIdx.x;
+    bool active = xi < n;
+    uint32_t k = 0;
     float etaK = 0, phiK = 0, scoreK = 0;
     int quadK = 0;
     uint32_t hk[4] = {0, 0, 0, 0};
     if (active) {
+      k = v.W.plsBinned[xi];
       quadK = v.P.isQuad[k] ? 1 : 0;
       if (kSecond && (!quadK || (v.P.isDup[k] & 1))) active = false;
       etaK = v.P.eta[k]; phiK = v.P.phi[k]; scoreK = v.P.score[k];
@@ -82,8 +150,9 @@ __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
     bool flag = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      const uint32_t jl = y0 + threadIdx.x;
-      if (jl < ys.e) {
+      if (y0 + threadIdx.x < ys.e) {
+        const uint32_t jl = v.W.plsBinned[band_at(B, y0 + threadIdx.x)];
+        sId[threadIdx.x] = jl;
         sEta[threadIdx.x] = v.P.eta[jl]; sPhi[threadIdx.x] = v.P.phi[jl]; sScore[threadIdx.x] = v.P.score[jl];
         sQuad[threadIdx.x] = v.P.isQuad[jl] ? 1 : 0;
         sOk[threadIdx.x] = kSecond ? (v.P.isQuad[jl] && !(v.P.isDup[jl] & 1)) : 1;
@@ -94,7 +163,7 @@ __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
       const uint32_t m = tile_len(ys, y0);
       if (!active || flag) continue;
       for (uint32_t t = 0; t < m && !flag; t++) {
-        const uint32_t j = y0 + t;
+        const uint32_t j = sId[t];
         if (j == k) continue;
         const bool kLo = k < j;
         const float etaLo = kLo ? etaK : sEta[t], etaHi = kLo ? sEta[t] : etaK;
@@ -131,45 +200,52 @@ __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
 
 // ------------------------------------------------------------ pT5 duplicates
 // removeDupPixelQuintupletsInGPUFromMap: ix is dropped when a partner shares
-// >= 7 of its 14 hits and scores lower (ties: the higher index is dropped)
-__global__ __launch_bounds__(kTile) void k_pt5_pairs(EventView v) {
-  __shared__ float sScore[kTile];
-  __shared__ uint32_t sHits[14 * kTile];
+// >= 7 of its 14 hits and scores lower (ties: the higher index is dropped).
+// At least 7 of ix's 14 positions must hold a hit of the partner, so one of its
+// first 8 positions does: every partner that can flag ix shares a hit with
+// those 8 positions.  A hash index from hit value to (pT5, position) entries
+// gives those partners; each is then checked with the exact rule.
+constexpr uint32_t kPT5HashMask = kPT5HashBuckets - 1;
+__device__ __forceinline__ uint32_t hit_bucket(uint32_t h) { return (h * 0x9E3779B1u) >> 17 & kPT5HashMask; }
+
+__global__ __launch_bounds__(256) void k_pt5_hash_reset(EventView v) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < kPT5HashBuckets; b += gridDim.x * blockDim.x)
+    v.W.pt5HashHead[b] = ~0u;
+}
+
+__global__ __launch_bounds__(256) void k_pt5_hash_link(EventView v) {
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
-  const YShare ys = y_share(n);
-  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
-    const uint32_t ix = x0 + threadIdx.x;
-    const bool active = ix < n;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 14 * n; e += gridDim.x * blockDim.x)
+    v.W.pt5HashNext[e] = atomicExch(&v.W.pt5HashHead[hit_bucket(v.P5.hitIndices[e])], e);
+}
+
+__global__ __launch_bounds__(256) void k_pt5_pairs(EventView v) {
+  const uint32_t n = v.R.scalars[kNPixelQuintuplets];
+  for (uint32_t ix = blockIdx.x * blockDim.x + threadIdx.x; ix < n; ix += gridDim.x * blockDim.x) {
     uint32_t h[14];
-    float s1 = 0;
-    if (active) {
 #pragma unroll
-      for (int a = 0; a < 14; a++) h[a] = v.P5.hitIndices[14 * ix + a];
-      s1 = v.P5.score[ix];
-    }
+    for (int a = 0; a < 14; a++) h[a] = v.P5.hitIndices[14 * ix + a];
+    const float s1 = v.P5.score[ix];
     bool dup = false;
-    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
-      __syncthreads();
-      const uint32_t jl = y0 + threadIdx.x;
-      if (jl < ys.e) {
-        sScore[threadIdx.x] = v.P5.score[jl];
-        for (int a = 0; a < 14; a++) sHits[threadIdx.x * 14 + a] = v.P5.hitIndices[14 * jl + a];
-      }
-      __syncthreads();
-      const uint32_t m = tile_len(ys, y0);
-      if (!active || dup) continue;
-      for (uint32_t t = 0; t < m && !dup; t++) {
-        const uint32_t jx = y0 + t;
+    for (int a0 = 0; a0 < 8 && !dup; a0++) {
+      const uint32_t key = h[a0];
+      for (uint32_t e = v.W.pt5HashHead[hit_bucket(key)]; e != ~0u && !dup; e = v.W.pt5HashNext[e]) {
+        if (v.P5.hitIndices[e] != key) continue;
+        const uint32_t jx = e / 14;
         if (jx == ix) continue;
-        const float s2 = sScore[t];
+        const float s2 = v.P5.score[jx];
         if (!(s1 > s2 || (s1 == s2 && ix > jx))) continue;
+        uint32_t g[14];
+#pragma unroll
+        for (int b = 0; b < 14; b++) g[b] = v.P5.hitIndices[14 * jx + b];
         int matched = 0;
-        for (int a = 0; a < 14 && matched < 7 && matched + (14 - a) >= 7; a++)
-          for (int b = 0; b < 14; b++)
-            if (h[a] == sHits[t * 14 + b]) {
-              matched++;
-              break;
-            }
+#pragma unroll
+        for (int a = 0; a < 14; a++) {
+          bool hit = false;
+#pragma unroll
+          for (int b = 0; b < 14; b++) hit = hit || h[a] == g[b];
+          matched += hit ? 1 : 0;
+        }
         dup = matched >= 7;
       }
     }
@@ -287,33 +363,44 @@ __global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
 }
 
 __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
-  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
-  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const uint32_t nq = v.R.nQuintuplets[m];
-  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
-  const uint32_t off = v.W.t5ListOff[blockIdx.x];
-  for (uint32_t k = threadIdx.x; k < nq; k += blockDim.x) v.W.t5List[off + k] = base + k;
+  const uint32_t nElig = v.R.scalars[kNEligibleT5];
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nWaves = (gridDim.x * blockDim.x) / 64;
+  for (uint32_t e = wave; e < nElig; e += nWaves) {  // one wave per eligible module
+    const uint16_t m = v.R.indicesOfEligibleT5Modules[e];
+    const uint32_t nq = v.R.nQuintuplets[m];
+    const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
+    const uint32_t off = v.W.t5ListOff[e];
+    for (uint32_t k = threadIdx.x & 63u; k < nq; k += 64) v.W.t5List[off + k] = base + k;
+  }
 }
 
 // ---------------------------------------------------- T5 duplicates before TC
 // Among T5s not in a pT5 and not flagged after build, a close pair
 // (|dEta|, |dPhi| <= 0.1, and dR2 < 0.001 or >= 5 shared hits) flags the
 // higher score (ties: the lower index) with bit 2.
+__global__ __launch_bounds__(1024) void k_bin_t5(EventView v) {
+  bin_by_eta(
+      v.R.scalars[kT5Listed], [&](uint32_t i) { return v.W.t5List[i]; }, [&](uint32_t q) { return v.Q.eta[q]; },
+      v.W.t5Binned, v.W.t5BinOff);
+}
+
 __global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
   __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
   __shared__ uint32_t sSlot[kTile];
   __shared__ uint8_t sOk[kTile];
   __shared__ uint32_t sHits[10 * kTile];
   const uint32_t n = v.R.scalars[kT5Listed];
-  const YShare ys = y_share(n);
   for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
+    const Band B = tile_band(v.W.t5BinOff, eta_bin(v.Q.eta[v.W.t5Binned[x0]]), eta_bin(v.Q.eta[v.W.t5Binned[xl]]));
+    const YShare ys = y_share(B.n1 + B.n2);
     const uint32_t xi = x0 + threadIdx.x;
     uint32_t x = 0;
     bool active = xi < n;
     float etaX = 0, phiX = 0, sX = 0;
     uint32_t hx[10];
     if (active) {
-      x = v.W.t5List[xi];
+      x = v.W.t5Binned[xi];
       active = !v.Q.partOfPT5[x] && !(v.Q.isDup[x] & 1);
       etaX = v.Q.eta[x]; phiX = v.Q.phi[x]; sX = v.Q.score_rphisum[x];
 #pragma unroll
@@ -322,9 +409,8 @@ __global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
     bool dup = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      const uint32_t yl = y0 + threadIdx.x;
-      if (yl < ys.e) {
-        const uint32_t y = v.W.t5List[yl];
+      if (y0 + threadIdx.x < ys.e) {
+        const uint32_t y = v.W.t5Binned[band_at(B, y0 + threadIdx.x)];
         sSlot[threadIdx.x] = y;
         sEta[threadIdx.x] = v.Q.eta[y]; sPhi[threadIdx.x] = v.Q.phi[y]; sScore[threadIdx.x] = v.Q.score_rphisum[y];
         sOk[threadIdx.x] = !v.Q.partOfPT5[y] && !(v.Q.isDup[y] & 1);
@@ -445,7 +531,7 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
   __shared__ float sEta[kTile], sPhi[kTile];
   __shared__ uint8_t sKind[kTile];
   __shared__ int32_t sPix[4 * kTile];
-  const uint32_t n = v.nPLS;
+  const uint32_t n = v.R.scalars[kEvNPLS];
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
   const YShare ys = y_share(nTC);
@@ -500,8 +586,9 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
 }  // namespace
 
 void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t) {
-  const dim3 g = pair_grid(v.nPLS, v.nPLS, v.nPLS);
+  const dim3 g(kPlsTiles, kBandShares);  // x tiles stride over the pLS
   const int tm = second ? kTmPlsDup2 : kTmPlsDup;
+  if (!second) hipLaunchKernelGGL(k_bin_pls, dim3(1), dim3(1024), 0, s, v);  // the eta order is kept for pass 2
   t.begin(s, tm);
   if (second) hipLaunchKernelGGL(k_pls_pairs<true>, g, dim3(kTile), 0, s, v);
   else hipLaunchKernelGGL(k_pls_pairs<false>, g, dim3(kTile), 0, s, v);
@@ -509,8 +596,10 @@ void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const Kernel
 }
 
 void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  hipLaunchKernelGGL(k_pt5_hash_reset, dim3(kPT5HashBuckets / 256), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_hash_link, dim3(14 * kNMaxPixelQuintuplets / 256 + 1), dim3(256), 0, s, v);
   t.begin(s, kTmPT5Dup);
-  hipLaunchKernelGGL(k_pt5_pairs, pair_grid(kNMaxPixelQuintuplets, 4096, kNMaxPixelQuintuplets), dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_pairs, dim3(kNMaxPixelQuintuplets / 64 + 1), dim3(64), 0, s, v);
   t.end(s, kTmPT5Dup);
 }
 
@@ -525,11 +614,12 @@ void launch_cross_pt3(const EventView& v, hipStream_t s) {
 }
 
 void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  const unsigned nElig = v.nT5SeedModules ? v.nT5SeedModules : 1;
   hipLaunchKernelGGL(k_t5_list_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_list_fill, dim3(nElig), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_list_fill, dim3(256), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT5DupTC);
-  hipLaunchKernelGGL(k_t5_tc_pairs, pair_grid(v.t5Cap, 8192, v.t5Cap), dim3(kTile), 0, s, v);
+  const unsigned gx = (v.t5Cap + kTile - 1) / kTile;  // x tiles stride over the listed T5s
+  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(gx < 128 ? gx : 128, kBandShares), dim3(kTile), 0, s, v);
   t.end(s, kTmT5DupTC);
 }
 
@@ -541,8 +631,7 @@ void launch_cross_t5(const EventView& v, hipStream_t s) {
 void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_tc_prep, dim3(128), dim3(256), 0, s, v);
   t.begin(s, kTmCrossPLS);
-  hipLaunchKernelGGL(k_cross_pls_pairs, pair_grid(v.nPLS, v.nPLS, kNMaxPixelTrackCandidates + kNMaxNonPixelTrackCandidates),
-                     dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_cross_pls_pairs, dim3(kPlsTiles, 32), dim3(kTile), 0, s, v);
   t.end(s, kTmCrossPLS);
 }
 

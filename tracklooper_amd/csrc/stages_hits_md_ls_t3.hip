@@ -66,12 +66,15 @@ __global__ void k_event_init(EventView v) {
     v.R.tripletRanges[i] = -1;
     v.R.quintupletRanges[i] = -1;
   }
-  if (blockIdx.x == 0 && threadIdx.x < kNScalars) v.R.scalars[threadIdx.x] = 0;
+  // every scalar starts at 0 except the event's hit and pLS counts, set before this kernel
+  if (blockIdx.x == 0 && threadIdx.x < kNScalars && threadIdx.x != kEvNHits && threadIdx.x != kEvNPLS)
+    v.R.scalars[threadIdx.x] = 0;
 }
 
 // SDL/Hit.h:217-270
 __global__ void k_hits(EventView v) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v.nHits; i += gridDim.x * blockDim.x) {
+  const uint32_t nHits = v.R.scalars[kEvNHits];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nHits; i += gridDim.x * blockDim.x) {
     const float x = v.H.xs[i], y = v.H.ys[i], z = v.H.zs[i];
     const uint32_t det = v.H.detid[i];
     const float rt = m_sqrt(x * x + y * y);
@@ -176,12 +179,13 @@ struct PixelInputs {
 
 __global__ void k_pixel_segments(EventView v, PixelInputs in) {
   const uint32_t pix = v.pixelModule;
+  const uint32_t nPLS = v.R.scalars[kEvNPLS];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    v.R.nSegments[pix] = v.nPLS; v.R.totOccupancySegments[pix] = v.nPLS;
-    v.R.nMDs[pix] = 2 * v.nPLS; v.R.totOccupancyMDs[pix] = 2 * v.nPLS;
+    v.R.nSegments[pix] = nPLS; v.R.totOccupancySegments[pix] = nPLS;
+    v.R.nMDs[pix] = 2 * nPLS; v.R.totOccupancyMDs[pix] = 2 * nPLS;
   }
   const MDOut zero{0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < v.nPLS; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nPLS; t += gridDim.x * blockDim.x) {
     const uint32_t innerMD = v.mdModuleIndices[pix] + 2 * t, outerMD = innerMD + 1;
     const uint32_t seg = v.segModuleIndices[pix] + t;
     const uint32_t h0 = in.h0[t], h1 = in.h1[t], h2 = in.h2[t], h3 = in.h3[t];
@@ -222,9 +226,130 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
 }
 
 // --------------------------------------------------------------- mini-doublets
+// Flattened builder (the path every event takes unless it has more than
+// kMDCandMax hit pairs or kMDParkMax passing ones): candidates of module m are
+// [candOff[m], candOff[m+1]), pair (lower li, upper ui) at li * nUpper + ui.
+__global__ __launch_bounds__(1024) void k_md_cand_scan(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t nL = v.M.nLowerModules;
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < nL; b += 1024) {
+    const uint32_t m = b + threadIdx.x;
+    unsigned c = 0;
+    if (m < nL && v.R.hitRangesLower[m] != -1) {  // the counts are int8 as in the reference (<= 127 hits)
+      const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+      c = nLo > 0 && nUp > 0 ? (unsigned)(nLo * nUp) : 0u;
+    }
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (m < nL) v.W2.candOff[m] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.W2.candOff[nL] = carry;
+    v.R.scalars[kMDCandTotal] = carry;
+    v.R.scalars[kMDParked] = 0;
+  }
+}
+
+__device__ __forceinline__ bool md_flat_mode(const EventView& v) {
+  return !v.perModuleBuilders && v.R.scalars[kMDCandTotal] <= kMDCandMax;
+}
+__device__ __forceinline__ bool md_flat_done(const EventView& v) {
+  return md_flat_mode(v) && v.R.scalars[kMDParked] <= kMDParkMax;
+}
+
+__global__ __launch_bounds__(256) void k_md_eval(EventView v) {
+  if (!md_flat_mode(v)) return;
+  const uint32_t total = v.R.scalars[kMDCandTotal];
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
+    const uint32_t L = L0 + threadIdx.x;
+    bool pass = false;
+    MDOut o;
+    if (L < total) {
+      uint32_t lo = 0, hi = nL;  // largest m with candOff[m] <= L
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v.W2.candOff[mid] <= L) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t m = lo, l = L - v.W2.candOff[m];
+      const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m];
+      const uint32_t li = l / nUp, ui = l - li * nUp;
+      const uint32_t a = (uint32_t)v.R.hitRangesLower[m] + li, b = (uint32_t)v.R.hitRangesUpper[m] + ui;
+      const ModCtx c = load_mod(v.M, m);
+      const float xL = v.H.xs[a], yL = v.H.ys[a], zL = v.H.zs[a], rtL = v.H.rts[a];
+      const float xU = v.H.xs[b], yU = v.H.ys[b], zU = v.H.zs[b], rtU = v.H.rts[b];
+      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, o)
+                                 : md_endcap(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, o);
+      v.W2.resIdx[L] = ~0u;
+    }
+    if (pass) {  // wave-aggregated park
+      const uint64_t mask = __ballot(1);
+      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kMDParked], (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      const uint32_t k = base + lanes_below(mask);
+      if (k < kMDParkMax) {
+        float* w = reinterpret_cast<float*>(&v.W2.park[kMDParkWords * k]);
+        w[0] = o.dz; w[1] = o.dphi; w[2] = o.dphichange; w[3] = o.sx; w[4] = o.sy; w[5] = o.sz;
+        w[6] = o.nsdz; w[7] = o.nsdphi; w[8] = o.nsdphichange;
+        v.W2.resIdx[L] = k;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t md_wave_index() {
+  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
+// one wave per module: passing pairs take their slots in L order
+__global__ __launch_bounds__(256) void k_md_place(EventView v) {
+  if (!md_flat_done(v)) return;
+  const unsigned lane = lane_id();
+  const uint32_t nWaves = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += nWaves) {
+    const int loBase = v.R.hitRangesLower[m];
+    const int cap = v.mdModuleOccupancy[m];
+    unsigned count = 0;
+    const uint32_t off = v.W2.candOff[m], n = v.W2.candOff[m + 1] - off;
+    if (loBase != -1 && n != 0) {
+      const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m];
+      const uint32_t upBase = (uint32_t)v.R.hitRangesUpper[m];
+      const ModCtx c = load_mod(v.M, m);
+      const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
+      const uint32_t base = v.mdModuleIndices[m];
+      for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
+        const uint32_t l = c0 + lane;
+        const uint32_t k = l < n ? v.W2.resIdx[off + l] : ~0u;
+        const bool pass = k != ~0u;
+        unsigned tot;
+        const unsigned rank = count + wave_rank(pass, tot);
+        if (pass && (int)rank < cap) {
+          const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
+          const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
+          const uint32_t li = l / nUp, ui = l - li * nUp;
+          store_md(v, (uint32_t)loBase + li, upBase + ui, m, anchorIsUpper, o, base + rank);
+        }
+        count += tot;
+      }
+    }
+    if (lane == 0) {
+      v.R.totOccupancyMDs[m] = count;
+      v.R.nMDs[m] = min(count, (unsigned)max(cap, 0));
+    }
+  }
+}
+
+// Per-module builder, used when an event exceeds the flattened path's arrays:
+// one wave per lower module walks its pairs in order.
 __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
   __shared__ float sx[256], sy[256], sz[256], srt[256];
   __shared__ unsigned lds[kBuildBlock / 64 + 1];
+  if (md_flat_done(v)) return;
   const uint32_t m = blockIdx.x;
   const unsigned lane = threadIdx.x;
   const int loBase = v.R.hitRangesLower[m];
@@ -282,6 +407,153 @@ __global__ void k_md_ranges(EventView v) {
 }
 
 // --------------------------------------------------------------- line segments
+// Flattened builder, as for the mini-doublets.  Per module, connPre[k] is the
+// first candidate of connected module k (nIn * nMD(k) candidates each, inner
+// MD major), and candOff the module's first global candidate.
+__global__ __launch_bounds__(256) void k_ls_cand_count(EventView v) {
+  const unsigned lane = lane_id();
+  const uint32_t S = v.M.stride + 1;
+  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+    const uint32_t nIn = v.R.nMDs[m];
+    const uint32_t nConn = nIn != 0 ? v.M.nConnectedModules[m] : 0;
+    uint32_t run = 0;
+    for (uint32_t k0 = 0; k0 < nConn; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const uint32_t c = k < nConn ? nIn * v.R.nMDs[v.M.moduleMap[(size_t)m * v.M.stride + k]] : 0;
+      const unsigned incl = wave_inclusive_scan(c);
+      if (k < nConn) v.WL.connPre[(size_t)m * S + k] = run + incl - c;
+      run += __shfl(incl, kWave - 1, kWave);
+    }
+    if (lane == 0) {
+      v.WL.connPre[(size_t)m * S + nConn] = run;
+      v.WL.candOff[m] = run;
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_ls_cand_scan(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t nL = v.M.nLowerModules;
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < nL; b += 1024) {
+    const uint32_t m = b + threadIdx.x;
+    const unsigned c = m < nL ? v.WL.candOff[m] : 0u;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (m < nL) v.WL.candOff[m] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    v.WL.candOff[nL] = carry;
+    v.R.scalars[kLSCandTotal] = carry;
+    v.R.scalars[kLSParked] = 0;
+  }
+}
+
+__device__ __forceinline__ bool ls_flat_mode(const EventView& v) {
+  return !v.perModuleBuilders && v.R.scalars[kLSCandTotal] <= kLSCandMax;
+}
+__device__ __forceinline__ bool ls_flat_done(const EventView& v) {
+  return ls_flat_mode(v) && v.R.scalars[kLSParked] <= kLSParkMax;
+}
+
+// module-local candidate l -> (outer module, inner MD, outer MD)
+__device__ __forceinline__ void ls_decode(const EventView& v, uint32_t m, uint32_t l, uint16_t& om, uint32_t& inMD,
+                                          uint32_t& outMD) {
+  const uint32_t S = v.M.stride + 1;
+  const uint32_t* pre = &v.WL.connPre[(size_t)m * S];
+  uint32_t k = 0, hiK = v.M.nConnectedModules[m];  // largest k with pre[k] <= l
+  while (hiK - k > 1) {
+    const uint32_t mid = (k + hiK) >> 1;
+    if (pre[mid] <= l) k = mid;
+    else hiK = mid;
+  }
+  om = v.M.moduleMap[(size_t)m * v.M.stride + k];
+  const uint32_t local = l - pre[k];
+  const uint32_t nOut = v.R.nMDs[om];
+  inMD = v.R.mdRanges[2 * m] + local / nOut;
+  outMD = v.R.mdRanges[2 * om] + local % nOut;
+}
+
+__global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
+  if (!ls_flat_mode(v)) return;
+  const uint32_t total = v.R.scalars[kLSCandTotal];
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
+    const uint32_t L = L0 + threadIdx.x;
+    bool pass = false;
+    SegOut o{0, 0, 0, 0, 0, 0};
+    if (L < total) {
+      uint32_t lo = 0, hi = nL;  // largest m with candOff[m] <= L
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v.WL.candOff[mid] <= L) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t m = lo;
+      uint16_t om;
+      uint32_t inMD, outMD;
+      ls_decode(v, m, L - v.WL.candOff[m], om, inMD, outMD);
+      const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
+      const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
+      pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, o)
+                                                            : seg_endcap(ci, co, v.ptCut, a, b, o);
+      v.WL.resIdx[L] = ~0u;
+    }
+    if (pass) {  // wave-aggregated park
+      const uint64_t mask = __ballot(1);
+      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kLSParked], (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      const uint32_t k = base + lanes_below(mask);
+      if (k < kLSParkMax) {
+        float* w = reinterpret_cast<float*>(&v.WL.park[kLSParkWords * k]);
+        w[0] = o.dPhi; w[1] = o.dPhiMin; w[2] = o.dPhiMax;
+        w[3] = o.dPhiChange; w[4] = o.dPhiChangeMin; w[5] = o.dPhiChangeMax;
+        v.WL.resIdx[L] = k;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ls_place(EventView v) {
+  if (!ls_flat_done(v)) return;
+  const unsigned lane = lane_id();
+  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+    const int cap = v.segModuleOccupancy[m];
+    const uint32_t off = v.WL.candOff[m], n = v.WL.candOff[m + 1] - off;
+    const uint32_t segBase = v.segModuleIndices[m];
+    unsigned count = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
+      const uint32_t l = c0 + lane;
+      const uint32_t k = l < n ? v.WL.resIdx[off + l] : ~0u;
+      const bool pass = k != ~0u;
+      unsigned tot;
+      const unsigned rank = count + wave_rank(pass, tot);
+      if (pass && (int)rank < cap) {
+        uint16_t om;
+        uint32_t inMD, outMD;
+        ls_decode(v, m, l, om, inMD, outMD);
+        const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
+        const uint32_t s = segBase + rank;
+        v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
+        v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
+        v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
+        v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
+        v.S.dPhis[s] = w[0]; v.S.dPhiMins[s] = w[1]; v.S.dPhiMaxs[s] = w[2];
+        v.S.dPhiChanges[s] = w[3]; v.S.dPhiChangeMins[s] = w[4]; v.S.dPhiChangeMaxs[s] = w[5];
+      }
+      count += tot;
+    }
+    if (lane == 0) {
+      v.R.totOccupancySegments[m] = count;
+      v.R.nSegments[m] = min(count, (unsigned)max(cap, 0));
+    }
+  }
+}
+
+// Per-module builder, used when an event exceeds the flattened path's arrays.
 __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
   __shared__ uint32_t pre[160];
   __shared__ uint16_t outMod[160];
@@ -292,6 +564,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
   const uint32_t nIn = v.R.nMDs[m];
   const int cap = v.segModuleOccupancy[m];
   unsigned count = 0;
+  if (ls_flat_done(v)) return;
   if (nIn != 0) {
     const uint32_t nConn = v.M.nConnectedModules[m];
     for (uint32_t k = lane; k < nConn; k += kBuildBlock) {
@@ -444,21 +717,30 @@ __device__ __forceinline__ bool t3_flat_done(const EventView& v) {
   return t3_flat_mode(v) && v.R.scalars[kT3Results] <= kT3ResultMax;
 }
 
+// The flattened T3 kernels below run one wave per lower module at a time and
+// stride over the modules, so a fixed grid that fills the chip covers every
+// module without dispatching a work-group per (mostly small) module.
+__device__ __forceinline__ uint32_t wave_index() {
+  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+__device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / kWave; }
+constexpr unsigned kModuleWaveBlocks = 1024;  // x 4 waves
+
 __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
-  __shared__ unsigned lds[5];
-  const uint32_t m1 = blockIdx.x;
-  const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
-  const uint32_t segBase = v.segModuleIndices[m1];
-  uint32_t run = 0;
-  for (uint32_t a0 = 0; a0 < nIn; a0 += 256) {
-    const uint32_t a = a0 + threadIdx.x;
-    const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<256>(cnt, lds, tot);
-    if (a < nIn) v.W3.candPre[segBase + a] = run + ex;
-    run += tot;
+  const unsigned lane = lane_id();
+  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
+    const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+    const uint32_t segBase = v.segModuleIndices[m1];
+    uint32_t run = 0;
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
+      const uint32_t a = a0 + lane;
+      const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
+      const unsigned incl = wave_inclusive_scan(cnt);
+      if (a < nIn) v.W3.candPre[segBase + a] = run + incl - cnt;
+      run += __shfl(incl, kWave - 1, kWave);
+    }
+    if (lane == 0) v.W3.modOff[m1] = run;
   }
-  if (threadIdx.x == 0) v.W3.modOff[m1] = run;
 }
 
 __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
@@ -481,43 +763,54 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
   }
 }
 
-// stage 1: shared middle mini-doublet over all candidates
+// stage 1: the shared middle mini-doublet.  A candidate (inner segment s,
+// outer segment o of s's outer module) passes when o starts at s's outer
+// mini-doublet, so the survivors of s are the segments linked from that
+// mini-doublet: per outer-tracker MD a list of the segments starting there
+// (one atomicExch per segment; list order is free, the placement goes by L).
+__global__ __launch_bounds__(256) void k_t3_link_reset(EventView v) {
+  if (!t3_flat_mode(v)) return;
+  const uint32_t n = (uint32_t)v.mdModuleIndices[v.M.nLowerModules];  // outer-tracker MD slots
+  for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x) v.W3.mdHead[d] = ~0u;
+  const uint32_t total = v.R.scalars[kT3CandTotal];  // every candidate starts unplaced
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
+    v.W3.resIdx[L] = ~0u;
+}
+
+__global__ __launch_bounds__(256) void k_t3_link(EventView v) {
+  if (!t3_flat_mode(v)) return;
+  for (uint32_t m = wave_index(); m < v.M.nLowerModules; m += wave_count()) {
+    const uint32_t n = v.R.nSegments[m], base = v.segModuleIndices[m];
+    for (uint32_t a = lane_id(); a < n; a += kWave) {
+      const uint32_t o = base + a;
+      v.W3.segNext[o] = atomicExch(&v.W3.mdHead[v.S.mdIndices[2 * o]], o);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_t3_filter_md(EventView v) {
   if (!t3_flat_mode(v)) return;
-  const uint32_t total = v.R.scalars[kT3CandTotal];
-  const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
-    const uint32_t L = L0 + threadIdx.x;
-    if (L >= total) continue;
-    v.W3.resIdx[L] = ~0u;
-    uint32_t lo = 0, hi = nL;  // largest m with modOff[m] <= L
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (v.W3.modOff[mid] <= L) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t m1 = lo, l = L - v.W3.modOff[lo];
-    const uint32_t segBase = v.segModuleIndices[m1];
-    uint32_t a0 = 0, a1 = v.R.nSegments[m1];  // largest a with candPre[a] <= l
-    while (a1 - a0 > 1) {
-      const uint32_t mid = (a0 + a1) >> 1;
-      if (v.W3.candPre[segBase + mid] <= l) a0 = mid;
-      else a1 = mid;
-    }
-    const uint32_t inSeg = segBase + a0;
-    const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg];
-    const uint32_t outSeg = v.segModuleIndices[m2] + (l - v.W3.candPre[inSeg]);
-    if (v.S.mdIndices[2 * inSeg + 1] == v.S.mdIndices[2 * outSeg]) {
+  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
+  const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+  const uint32_t segBase = v.segModuleIndices[m1];
+  const uint32_t modOff = v.W3.modOff[m1];
+  for (uint32_t a = lane_id(); a < nIn; a += kWave) {
+    const uint32_t inSeg = segBase + a;
+    const uint32_t base2 = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]];
+    const uint32_t L0 = modOff + v.W3.candPre[inSeg];
+    for (uint32_t o = v.W3.mdHead[v.S.mdIndices[2 * inSeg + 1]]; o != ~0u; o = v.W3.segNext[o]) {
+      const uint32_t l2 = o - base2;
       const uint64_t mask = __ballot(1);
       const unsigned leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      const uint32_t k = base + lanes_below(mask);
-      v.W3.list[3 * k] = L;
+      uint32_t b = 0;
+      if (lane_id() == leader) b = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
+      b = __shfl(b, leader, 64);
+      const uint32_t k = b + lanes_below(mask);
+      v.W3.list[3 * k] = L0 + l2;
       v.W3.list[3 * k + 1] = m1;
-      v.W3.list[3 * k + 2] = (inSeg - segBase) | ((l - v.W3.candPre[inSeg]) << 16);
+      v.W3.list[3 * k + 2] = a | (l2 << 16);
     }
+  }
   }
 }
 
@@ -546,29 +839,30 @@ __global__ __launch_bounds__(256) void k_t3_select(EventView v) {
 
 // stage 3: per module, passing candidates in candidate order take the slots
 __global__ __launch_bounds__(256) void k_t3_place(EventView v) {
-  __shared__ unsigned lds[5];
   if (!t3_flat_done(v)) return;
-  const uint32_t m1 = blockIdx.x;
-  const uint32_t off = v.W3.modOff[m1], n = v.W3.modOff[m1 + 1] - off;
-  const int cap = v.R.tripletModuleOccupancy[m1];
-  const uint32_t t3Base = v.R.tripletModuleIndices[m1];
-  unsigned count = 0;
-  for (uint32_t c0 = 0; c0 < n; c0 += 256) {
-    const uint32_t l = c0 + threadIdx.x;
-    const uint32_t k = l < n ? v.W3.resIdx[off + l] : ~0u;
-    const bool pass = k != ~0u;
-    unsigned tot;
-    const unsigned rank = count + block_exclusive_scan<256>(pass ? 1u : 0u, lds, tot);
-    if (pass && (int)rank < cap) {
-      const uint32_t* w = &v.W3.results[kT3ResultWords * k];
-      t3_store(v, t3Base + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
-               __uint_as_float(w[4]), __uint_as_float(w[5]));
+  const unsigned lane = lane_id();
+  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
+    const uint32_t off = v.W3.modOff[m1], n = v.W3.modOff[m1 + 1] - off;
+    const int cap = v.R.tripletModuleOccupancy[m1];
+    const uint32_t t3Base = v.R.tripletModuleIndices[m1];
+    unsigned count = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
+      const uint32_t l = c0 + lane;
+      const uint32_t k = l < n ? v.W3.resIdx[off + l] : ~0u;
+      const bool pass = k != ~0u;
+      unsigned tot;
+      const unsigned rank = count + wave_rank(pass, tot);
+      if (pass && (int)rank < cap) {
+        const uint32_t* w = &v.W3.results[kT3ResultWords * k];
+        t3_store(v, t3Base + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
+                 __uint_as_float(w[4]), __uint_as_float(w[5]));
+      }
+      count += tot;
     }
-    count += tot;
-  }
-  if (threadIdx.x == 0) {
-    v.R.totOccupancyTriplets[m1] = count;
-    v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
+    if (lane == 0) {
+      v.R.totOccupancyTriplets[m1] = count;
+      v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
+    }
   }
 }
 
@@ -640,12 +934,39 @@ inline unsigned grid_for(uint32_t n, unsigned block) {
   return g == 0 ? 1 : (g > 4096 ? 4096 : g);
 }
 
+// one row of blocks per buffer; 4-byte words, then the 0-3 tail bytes
+__global__ __launch_bounds__(256) void k_copy_multi(CopyList c) {
+  const int e = blockIdx.y;
+  if (e >= c.n) return;
+  const uint32_t words = c.bytes[e] >> 2;
+  const uint32_t* src = static_cast<const uint32_t*>(c.src[e]);
+  uint32_t* dst = static_cast<uint32_t*>(c.dst[e]);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x < (c.bytes[e] & 3u))
+    static_cast<char*>(c.dst[e])[4 * words + threadIdx.x] = static_cast<const char*>(c.src[e])[4 * words + threadIdx.x];
+}
+
 }  // namespace
+
+__global__ void k_set_scalar(uint32_t* p, uint32_t value) { *p = value; }
+
+void launch_set_scalar(uint32_t* p, uint32_t value, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, p, value);
+}
+
+void launch_copy_multi(const CopyList& c, hipStream_t s) {
+  if (c.n <= 0) return;
+  uint32_t maxBytes = 0;
+  for (int e = 0; e < c.n; e++) maxBytes = c.bytes[e] > maxBytes ? c.bytes[e] : maxBytes;
+  unsigned gx = (maxBytes / 4 + 255) / 256;
+  gx = gx == 0 ? 1 : (gx > 64 ? 64 : gx);
+  hipLaunchKernelGGL(k_copy_multi, dim3(gx, c.n), dim3(256), 0, s, c);
+}
 
 void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_event_init, dim3(grid_for(2 * v.M.nModules, 256)), dim3(256), 0, s, v);
   t.begin(s, kTmHits);
-  hipLaunchKernelGGL(k_hits, dim3(grid_for(v.nHits, 256)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_hits, dim3(1024), dim3(256), 0, s, v);
   t.end(s, kTmHits);
   hipLaunchKernelGGL(k_module_ranges, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_hit_ranges_fix, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
@@ -655,35 +976,44 @@ void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const 
                            const KernelTimers& t) {
   PixelInputs in{hitIdx[0], hitIdx[1], hitIdx[2], hitIdx[3], dPhiChange};
   t.begin(s, kTmPixelSegments);
-  hipLaunchKernelGGL(k_pixel_segments, dim3(grid_for(v.nPLS, 256)), dim3(256), 0, s, v, in);
+  hipLaunchKernelGGL(k_pixel_segments, dim3(196), dim3(256), 0, s, v, in);
   t.end(s, kTmPixelSegments);
 }
 
 void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  hipLaunchKernelGGL(k_md_cand_scan, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmMiniDoublets);
-  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+  hipLaunchKernelGGL(k_md_eval, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmMiniDoublets);
+  hipLaunchKernelGGL(k_md_place, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
   hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
 void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  hipLaunchKernelGGL(k_ls_cand_count, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_ls_cand_scan, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmSegments);
-  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+  hipLaunchKernelGGL(k_ls_eval, dim3(4096), dim3(256), 0, s, v);
   t.end(s, kTmSegments);
+  hipLaunchKernelGGL(k_ls_place, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
   hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_cand_count, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_link_reset, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_link, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.begin(s, kTmT3Filter);
-  hipLaunchKernelGGL(k_t3_filter_md, dim3(2048), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_filter_md, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.end(s, kTmT3Filter);
   t.begin(s, kTmT3Select);
   hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmT3Select);
-  hipLaunchKernelGGL(k_t3_place, dim3(v.M.nLowerModules), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_place, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }

@@ -21,7 +21,6 @@ namespace {
 constexpr int kT5Block = 256;
 constexpr int kMaxInnerT3 = 1024;   // > the largest T3 module capacity (755)
 constexpr int kMaxT5PerModule = 416;  // >= the largest T5 module capacity (414)
-constexpr int kAfterBuildSplit = 8;   // partner shares per module in the after-build duplicate check
 
 // createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
 // seed modules holding a triplet, their capacities and the exclusive scan of
@@ -319,25 +318,34 @@ __device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_
 // Candidates of module b (eligible-list position) are [modOff[b], modOff[b+1]);
 // within a module, inner T3 a owns [candPre[a], candPre[a] + nTriplets(m3(a))).
 
+// The per-module kernels of the flattened builder run one wave per eligible
+// module at a time and stride over the modules (a fixed grid of
+// kModuleWaveBlocks work-groups), so small modules do not each dispatch one.
+constexpr unsigned kModuleWaveBlocks = 512;  // x 4 waves
+__device__ __forceinline__ uint32_t wave_index() {
+  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+}
+__device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / 64; }
+
 // per eligible module: candidate count and per-inner-T3 exclusive prefix
-__global__ __launch_bounds__(kT5Block) void k_t5_cand_count(EventView v) {
-  __shared__ unsigned lds[kT5Block / 64 + 1];
+__global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  if (blockIdx.x >= nElig) return;
-  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const int layer = v.M.layers[m1];
-  const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
-  const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-  uint32_t run = 0;
-  for (uint32_t a0 = 0; a0 < nIn; a0 += kT5Block) {
-    const uint32_t a = a0 + threadIdx.x;
-    const uint32_t cnt = a < nIn ? v.R.nTriplets[v.T.lowerModuleIndices[3 * (t3Base + a) + 2]] : 0;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<kT5Block>(cnt, lds, tot);
-    if (a < nIn) v.W5.candPre[t3Base + a] = run + ex;
-    run += tot;
+  const unsigned lane = lane_id();
+  for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
+    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
+    const int layer = v.M.layers[m1];
+    const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
+    const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
+    uint32_t run = 0;
+    for (uint32_t a0 = 0; a0 < nIn; a0 += 64) {
+      const uint32_t a = a0 + lane;
+      const uint32_t cnt = a < nIn ? v.R.nTriplets[v.T.lowerModuleIndices[3 * (t3Base + a) + 2]] : 0;
+      const unsigned incl = wave_inclusive_scan(cnt);
+      if (a < nIn) v.W5.candPre[t3Base + a] = run + incl - cnt;
+      run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) v.W5.modOff[e] = run;
   }
-  if (threadIdx.x == 0) v.W5.modOff[blockIdx.x] = run;
 }
 
 // exclusive scan of the per-module counts (count = nEligible, read on the device)
@@ -478,36 +486,38 @@ __global__ __launch_bounds__(256) void k_t5_dnn(EventView v) {
 }
 
 // stage 4: per module, passing candidates in candidate order take the slots
-__global__ __launch_bounds__(kT5Block) void k_t5_place(EventView v) {
-  __shared__ unsigned lds[kT5Block / 64 + 1];
+__global__ __launch_bounds__(256) void k_t5_place(EventView v) {
   if (!t5_flat_done(v)) return;
-  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
-  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const int layer = v.M.layers[m1];
-  const int cap = v.R.quintupletModuleOccupancy[m1];
-  const uint32_t off = v.W5.modOff[blockIdx.x], n = v.W5.modOff[blockIdx.x + 1] - off;
-  const uint32_t t5Base = (uint32_t)v.R.quintupletModuleIndices[m1];
-  unsigned count = 0;
-  for (uint32_t c0 = 0; c0 < n; c0 += kT5Block) {
-    const uint32_t l = c0 + threadIdx.x;
-    const uint32_t k = l < n ? v.W5.resIdx[off + l] : ~0u;
-    const bool pass = k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
-    unsigned tot;
-    const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, tot);
-    if (pass && (int)rank < cap) {
-      const uint32_t* w = &v.W5.results[kT5ResultWords * k];
-      T5Result r;
-      r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
-      r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
-      r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
-      r.tight = w[11] != 0;
-      t5_store(v, t5Base + rank, w[0], w[1], r, layer);
+  const uint32_t nElig = v.R.scalars[kNEligibleT5];
+  const unsigned lane = lane_id();
+  for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
+    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
+    const int layer = v.M.layers[m1];
+    const int cap = v.R.quintupletModuleOccupancy[m1];
+    const uint32_t off = v.W5.modOff[e], n = v.W5.modOff[e + 1] - off;
+    const uint32_t t5Base = (uint32_t)v.R.quintupletModuleIndices[m1];
+    unsigned count = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+      const uint32_t l = c0 + lane;
+      const uint32_t k = l < n ? v.W5.resIdx[off + l] : ~0u;
+      const bool pass = k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
+      unsigned tot;
+      const unsigned rank = count + wave_rank(pass, tot);
+      if (pass && (int)rank < cap) {
+        const uint32_t* w = &v.W5.results[kT5ResultWords * k];
+        T5Result r;
+        r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
+        r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
+        r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
+        r.tight = w[11] != 0;
+        t5_store(v, t5Base + rank, w[0], w[1], r, layer);
+      }
+      count += tot;
     }
-    count += tot;
-  }
-  if (threadIdx.x == 0) {
-    v.R.totOccupancyQuintuplets[m1] = count;
-    v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
+    if (lane == 0) {
+      v.R.totOccupancyQuintuplets[m1] = count;
+      v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
+    }
   }
 }
 
@@ -577,51 +587,58 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
 
 // removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249).  For a close
 // pair (i < j) sharing >= 7 hits the reference flags i when score_i >= score_j
-// and j otherwise.  A module's partners are split into kAfterBuildSplit equal
-// shares (blockIdx.y); each lane owns one T5 and tests the partners of its
-// share, so a flag is a store of 1 that does not depend on evaluation order.
-__global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
-  __shared__ float sEta[kMaxT5PerModule], sPhi[kMaxT5PerModule], sScore[kMaxT5PerModule];
-  __shared__ uint32_t sHits[kMaxT5PerModule * 10];
-  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
-  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const uint32_t nq = v.R.nQuintuplets[m];
-  if (nq < 2 || nq > (uint32_t)kMaxT5PerModule) return;
-  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
-  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
-    sEta[i] = v.Q.eta[base + i];
-    sPhi[i] = v.Q.phi[base + i];
-    sScore[i] = v.Q.score_rphisum[base + i];
-  }
-  for (uint32_t i = threadIdx.x; i < nq * 10; i += kT5Block) sHits[i] = v.Q.hitIndices[10 * base + i];
-  __syncthreads();
-  const uint32_t share = (nq + gridDim.y - 1) / gridDim.y;
-  const uint32_t jb = blockIdx.y * share, je = jb + share < nq ? jb + share : nq;
-  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
+// and j otherwise.  Work items are (eligible module, chunk of 64 of its T5s);
+// a wave takes one item, each lane owns one T5 and tests every partner of the
+// module, so a flag is a store of 1 that does not depend on evaluation order.
+// Partner records are read at wave-uniform addresses (scalar loads).
+constexpr uint32_t kT5Chunks = (kMaxT5PerModule + 63) / 64;
+
+__global__ __launch_bounds__(256) void k_t5_dup_after_build(EventView v) {
+  const uint32_t nElig = v.R.scalars[kNEligibleT5];
+  const unsigned lane = lane_id();
+  for (uint32_t it = wave_index(); it < nElig * kT5Chunks; it += wave_count()) {
+    const uint32_t e = it / kT5Chunks, cx = it % kT5Chunks;
+    const uint16_t m = v.R.indicesOfEligibleT5Modules[e];
+    const uint32_t nq = v.R.nQuintuplets[m];
+    if (nq < 2 || cx * 64 >= nq) continue;
+    const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
+    const uint32_t i = cx * 64 + lane;
+    const bool active = i < nq;
+    float etaI = 0, phiI = 0, si = 0;
+    uint32_t hi[10];
+    if (active) {
+      etaI = v.Q.eta[base + i];
+      phiI = v.Q.phi[base + i];
+      si = v.Q.score_rphisum[base + i];
+#pragma unroll
+      for (int a = 0; a < 10; a++) hi[a] = v.Q.hitIndices[10 * (base + i) + a];
+    }
     bool dup = false;
-    const float si = sScore[i];
-    for (uint32_t j = jb; j < je && !dup; j++) {
-      if (j == i) continue;
+    for (uint32_t j = 0; j < nq; j++) {
+      const float sj = v.Q.score_rphisum[base + j];
+      if (!active || dup || j == i) continue;
       // the pair can only flag i if the score order points at i
-      const bool targetsI = i < j ? (si >= sScore[j]) : !(sScore[j] >= si);
+      const bool targetsI = i < j ? (si >= sj) : !(sj >= si);
       if (!targetsI) continue;
-      const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-      const float dEta = m_abs(sEta[lo] - sEta[hi]);
-      const float dPhi = wrap_dphi(sPhi[lo], sPhi[hi]);
+      const float etaJ = v.Q.eta[base + j], phiJ = v.Q.phi[base + j];
+      const float dEta = m_abs(i < j ? etaI - etaJ : etaJ - etaI);
+      const float dPhi = i < j ? wrap_dphi(phiI, phiJ) : wrap_dphi(phiJ, phiI);
       if (dEta > 0.1f) continue;
       if (m_abs(dPhi) > 0.1f) continue;
+      uint32_t hj[10];
+#pragma unroll
+      for (int b = 0; b < 10; b++) hj[b] = v.Q.hitIndices[10 * (base + j) + b];
       int matched = 0;
+#pragma unroll
       for (int a = 0; a < 10; a++) {
-        const uint32_t h = sHits[10 * lo + a];
-        for (int b = 0; b < 10; b++)
-          if (h == sHits[10 * hi + b]) {
-            matched++;
-            break;
-          }
+        bool hit = false;
+#pragma unroll
+        for (int b = 0; b < 10; b++) hit = hit || hi[a] == hj[b];
+        matched += hit ? 1 : 0;
       }
       dup = matched >= 7;
     }
-    if (dup) v.Q.isDup[base + i] = 1;
+    if (active && dup) v.Q.isDup[base + i] = 1;
   }
 }
 
@@ -646,7 +663,7 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_cand_count, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_t5_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_md, dim3(flatGrid), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
@@ -658,10 +675,10 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
     hipLaunchKernelGGL(k_t5_dnn, dim3(flatGrid), dim3(256), 0, s, v);
     t.end(s, kTmT5Dnn);
   }
-  hipLaunchKernelGGL(k_t5_place, dim3(nBlocks), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_t5_place, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
   t.begin(s, kTmT5DupAfter);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks, kAfterBuildSplit), dim3(kT5Block), 0, s, v);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.end(s, kTmT5DupAfter);
   const unsigned g = (v.M.nLowerModules + 255) / 256;
   hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
