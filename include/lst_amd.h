@@ -160,6 +160,37 @@ lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t sl
  * (asynchronous on the event's stream; same results as lst_run). */
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 
+/* T5 DNN training rows (no counterpart in the reference, whose DNN score is
+ * not stored; the rows are the candidates at its DNN call site,
+ * SDL/Quintuplet.h:2921-2936).  lst_event_record_t5dnn(ev, capacity) on an
+ * LST_T5_DNN event makes later createQuintuplets calls record, per candidate
+ * that reaches the DNN, its 38 inputs (SDL/NeuralNetwork.h:36-114), its score,
+ * its (inner, outer) triplet pair and its 10 hit indices in input numbering;
+ * capacity 0 stops recording.  lst_event_get_t5dnn_record copies up to
+ * `capacity` rows (any array may be NULL) and sets *n to the rows recorded;
+ * row order is arbitrary.  LST_W_TRUNCATED if rows were dropped;
+ * LST_E_CAPACITY if the event took the per-module quintuplet builder. */
+lst_status lst_event_record_t5dnn(lst_event* ev, uint32_t capacity);
+lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* features, float* scores, uint32_t* t3_pairs,
+                                      uint32_t* hit_idxs, uint32_t capacity);
+
+/* T5 DNN parameters and training (no counterpart in the reference, which
+ * compiles its trained weights in, SDL/NeuralNetworkWeights.h).  Parameters
+ * are packed as bias_0[32], wgtT_0[38][32], bias_2[32], wgtT_2[32][32],
+ * bias_4[1], wgtT_4[32][1] (LST_T5DNN_NPARAMS floats).
+ * lst_t5dnn_set_weights / get_weights: the parameters every event's DNN mode
+ * uses (process-wide, on the current device; synchronises it).
+ * lst_t5dnn_forward / backward: one training step's kernels on device arrays
+ * (x [n][38]; activations h1, h2 [n][32]; logit, dlogit [n]; work d1, d2
+ * [n][32]; grad [LST_T5DNN_NPARAMS], overwritten), enqueued on `stream`. */
+#define LST_T5DNN_NPARAMS 2337
+lst_status lst_t5dnn_set_weights(const float* params);
+lst_status lst_t5dnn_get_weights(float* params);
+lst_status lst_t5dnn_forward(const float* x, const float* params, float* h1, float* h2, float* logit, uint32_t n,
+                             void* stream);
+lst_status lst_t5dnn_backward(const float* x, const float* params, const float* h1, const float* h2,
+                              const float* dlogit, float* d2, float* d1, float* grad, uint32_t n, void* stream);
+
 /* ---- Batched engine.  The reference runs one SDL::Event per queue with one
  * OpenMP thread per queue (bin/sdl.cc:389-419) and LST::run one event at a
  * time (SDL/LST.cc:12-161); a batch keeps `n_contexts` event contexts (device

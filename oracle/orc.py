@@ -51,6 +51,7 @@ def lib():
         L.orc_t5dnn_forward.restype = C.c_float
         L.orc_t5dnn_features.argtypes = [C.c_void_p, C.c_uint, C.c_void_p]
         L.orc_t5dnn_features.restype = C.c_int
+        L.orc_t5dnn_set_weights.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

@@ -217,6 +217,8 @@ int orc_event_field(void* hp, const char* key, const void** p, int* t, long* n) 
   F("t3.betaIn", ev.t3BetaIn); F("t3.circleRadius", ev.t3CircleRadius); F("t3.circleCenterX", ev.t3CircleCenterX);
   F("t3.circleCenterY", ev.t3CircleCenterY); F("t3.partOfPT5", ev.t3PartOfPT5); F("t3.partOfT5", ev.t3PartOfT5);
   F("t3.partOfPT3", ev.t3PartOfPT3);
+  F("dnn.features", ev.dnnFeatures); F("dnn.score", ev.dnnScore); F("dnn.t3Pair", ev.dnnT3Pair);
+  F("dnn.hitIdxs", ev.dnnHitIdxs);
   // quintuplets
   F("t5.nQuintuplets", ev.nQuintuplets); F("t5.totOccupancyQuintuplets", ev.totOccupancyQuintuplets);
   F("t5.tripletIndices", ev.t5TripletIndices); F("t5.lowerModuleIndices", ev.t5LowerModuleIndices);
@@ -279,6 +281,8 @@ float orc_regression_circle(unsigned n, const float* xs, const float* ys, const 
 }
 // T5 DNN forward pass on 38 inputs (SDL/NeuralNetwork.h:116-162)
 float orc_t5dnn_forward(const float* in) { return orc::t5dnnForward(in); }
+// replace the DNN parameters (packed as in the engine's lst_t5dnn_set_weights)
+void orc_t5dnn_set_weights(const float* p) { orc::t5dnnSetWeights(p); }
 // the 38 DNN inputs of stored quintuplet slot t5 (SDL/NeuralNetwork.h:36-114); -1 if out of range
 int orc_t5dnn_features(void* hp, unsigned t5, float* out) {
   const orc::Event& ev = static_cast<Handle*>(hp)->ev;

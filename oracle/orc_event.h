@@ -75,6 +75,10 @@ struct Event {
   std::vector<uint16_t> t3LowerModuleIndices;
   std::vector<uint8_t> t3LogicalLayers, t3PartOfPT5, t3PartOfT5, t3PartOfPT3;
   std::vector<float> t3BetaIn, t3CircleRadius, t3CircleCenterX, t3CircleCenterY;
+  // every T5 candidate at the DNN call site in DNN mode (SDL/Quintuplet.h:2921-2936), in serial
+  // order: 38 inputs, score, (inner, outer) T3 pair, 10 hit indices in input numbering
+  mutable std::vector<float> dnnFeatures, dnnScore;
+  mutable std::vector<unsigned> dnnT3Pair, dnnHitIdxs;
   // quintuplets
   unsigned t5Capacity = 0;
   std::vector<unsigned> nQuintuplets, totOccupancyQuintuplets, t5TripletIndices, t5HitIndices;

@@ -19,6 +19,7 @@ float regressionCircle(unsigned nPoints, const float* xs, const float* ys, const
 float chiSquaredOfCircle(unsigned nPoints, const float* xs, const float* ys, const float* delta1, const float* delta2,
                          const float* slopes, const bool* isFlat, float g, float f, float radius);
 float t5dnnForward(const float in[38]);
+void t5dnnSetWeights(const float* packed);
 void t5dnnFeatures(const Event& ev, const unsigned md[5], const uint16_t mods[5], unsigned innerT3, float innerRadius,
                    float outerRadius, float bridgeRadius, float out[38]);
 }  // namespace orc

@@ -88,3 +88,16 @@ def test_batch_dnn_mode_matches_per_event(es, events):
     res = lst.Batch(es, n_contexts=2, t5_mode=lst.LST_T5_DNN).run(sub)
     for r, p in zip(res, want):
         _same(r, p)
+
+
+@pytest.mark.parametrize("graphs", ["1", "0"])
+def test_batch_one_context_replays_match(es, events, per_event, monkeypatch, graphs):
+    """One context runs every staged event in turn (its captured chain replayed per event, or, with
+    LST_BATCH_GRAPHS=0, every kernel launched); each replay equals the per-event result."""
+    import tracklooper_amd as lst
+    monkeypatch.setenv("LST_BATCH_GRAPHS", graphs)
+    b = lst.Batch(es, n_contexts=1)
+    b.stage(events)
+    res = b.run_staged(0, 2 * len(events))
+    for i, r in enumerate(res):
+        _same(r, per_event[i % len(events)])

@@ -236,3 +236,25 @@ def test_pt06_geometry_all_stages(pt06, case):
     assert ref.field("t5.nQuintuplets").sum() > 0 and ref.scalar("nTrackCandidates") > 0
     print(f"pt0.6 mode {mode}: T5={ref.field('t5.nQuintuplets').sum()} TC={ref.scalar('nTrackCandidates')}; "
           f"floats compared {rep.float_compared}, not bit-identical {rep.float_inexact}")
+
+
+def test_t5dnn_record_matches_oracle(engine, oracle_geometry):
+    """Training rows of the T5 DNN (candidates at its call site): inputs, score, triplet pair and
+    hit indices equal the oracle's, row for row (rows sorted by triplet pair)."""
+    import tracklooper_amd as lst
+    from oracle import orc
+    from tracklooper_amd import synth
+    _, es = engine
+    ev = synth.generate_event("pu", 4, n_pu=50)
+    e = lst.Event(es, t5_mode=lst.LST_T5_DNN).record_t5dnn().run_lst(ev, "t5")
+    rec = e.t5dnn_record()
+    ref = orc.Event(oracle_geometry, t5_mode=1).run_lst(ev, "t5")
+    rpair = ref.field("dnn.t3Pair").reshape(-1, 2)
+    assert len(rpair) > 500 and len(rec["t3_pair"]) == len(rpair)
+    a = np.lexsort((rec["t3_pair"][:, 1], rec["t3_pair"][:, 0]))
+    b = np.lexsort((rpair[:, 1], rpair[:, 0]))
+    np.testing.assert_array_equal(rec["t3_pair"][a], rpair[b])
+    np.testing.assert_array_equal(rec["features"][a].view(np.uint32),
+                                  ref.field("dnn.features").reshape(-1, 38)[b].view(np.uint32))
+    np.testing.assert_array_equal(rec["score"][a].view(np.uint32), ref.field("dnn.score")[b].view(np.uint32))
+    np.testing.assert_array_equal(rec["hit_idxs"][a], ref.field("dnn.hitIdxs").reshape(-1, 10)[b])

@@ -56,6 +56,13 @@ def lib():
         path = library_path()
         if not os.path.exists(path):
             raise LSTError(f"{path} is missing: run `python -m tracklooper_amd.build` (HIP, gfx950)")
+        # PyTorch ships its own HIP runtime.  Loading it first lets the library
+        # bind to that same runtime, so torch tensors, streams and events and the
+        # library's device memory belong to one runtime in the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(path)
         P = C.c_void_p
         L.lst_last_error.restype = C.c_char_p
@@ -192,6 +199,27 @@ class Event:
             _check(lib().lst_event_get_output(self.h, C.byref(n), _p(hits), _p(length), _p(seed), _p(typ), k))
         return {"tc_hitIdxs": [list(hits[i, :length[i]]) for i in range(k)], "tc_len": length,
                 "tc_seedIdx": seed, "tc_trackCandidateType": typ}
+
+    def record_t5dnn(self, capacity: int = 1 << 19):
+        """Record the T5 DNN's inputs for every candidate at its call site (DNN mode)."""
+        lib().lst_event_record_t5dnn.argtypes = [C.c_void_p, C.c_uint32]
+        _check(lib().lst_event_record_t5dnn(self.h, capacity))
+        return self
+
+    def t5dnn_record(self) -> dict:
+        """The recorded rows: features [n, 38], score [n], t3_pair [n, 2], hit_idxs [n, 10]."""
+        L = lib()
+        L.lst_event_get_t5dnn_record.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)] + [C.c_void_p] * 4 + [C.c_uint32]
+        n = C.c_uint32(0)
+        _check(L.lst_event_get_t5dnn_record(self.h, C.byref(n), None, None, None, None, 0))
+        k = n.value
+        f = np.zeros((k, 38), np.float32)
+        sc = np.zeros(k, np.float32)
+        pr = np.zeros((k, 2), np.uint32)
+        hi = np.zeros((k, 10), np.uint32)
+        if k:
+            _check(L.lst_event_get_t5dnn_record(self.h, C.byref(n), _p(f), _p(sc), _p(pr), _p(hi), k))
+        return {"features": f, "score": sc, "t3_pair": pr, "hit_idxs": hi}
 
     def resetEvent(self):
         _check(lib().lst_event_reset(self.h))

@@ -26,6 +26,7 @@ SOURCES = [
     "stages_t5.hip",
     "stages_pixel.hip",
     "stages_clean.hip",
+    "t5dnn_train.hip",
     "engine.hip",
 ]
 

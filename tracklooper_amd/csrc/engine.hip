@@ -128,11 +128,19 @@ struct lst_event {
     timers.recorded = timing ? &recorded : nullptr;
     return timers;
   }
+  // captured launch sequence of the whole chain on described device inputs
+  // (batched engine): recaptured whenever the arena is reallocated
+  void* dnnRecordMem = nullptr;        // T5 DNN training rows (lst_event_record_t5dnn)
+  lst::InputDesc* descDev = nullptr;   // device copy of the current event's descriptor
+  lst::InputDesc* descHost = nullptr;  // pinned staging of that descriptor
+  hipGraphExec_t graph = nullptr;
+  uint32_t arenaGen = 0, graphGen = 0;
   // device-resident prepared inputs (throughput path)
   struct Staged {
     char* mem = nullptr;
     size_t bytes = 0;
     uint32_t nHits = 0, nPLS = 0;
+    bool truncated = false;  // the event had more pixel segments than kNMaxPixelSegments
     float *x = nullptr, *y = nullptr, *z = nullptr;
     uint32_t *det = nullptr, *idx = nullptr;
     uint32_t *h[4] = {nullptr, nullptr, nullptr, nullptr}, *seed = nullptr;
@@ -207,6 +215,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5ParkAt, lst::kPixMaskWords); c.take(e.W.pt5Park, (size_t)lst::kPT5ParkWords * lst::kPixParkMax);
   c.take(e.W.pt3ParkAt, lst::kPixMaskWords); c.take(e.W.pt3Park, (size_t)lst::kPT3ParkWords * lst::kPixParkMax);
   c.take(e.W.pt5HashHead, lst::kPT5HashBuckets); c.take(e.W.pt5HashNext, 14 * (size_t)pt5Cap);
+  c.take(e.FS.pre, lst::kScanCandMax); c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
   c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax);
   c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDParkMax);
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
@@ -214,7 +223,6 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
   c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3ResultMax);
-  c.take(e.W3.mdHead, mdCap); c.take(e.W3.segNext, segCap);
   c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
   c.take(e.W5.resIdx, lst::kT5CandMax);
   c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
@@ -236,6 +244,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   arenaBytes = c.off;
   HIP_OK(hipMalloc(&arena, arenaBytes));
   c.bind(arena);
+  arenaGen++;
   e.M = g.view;
   e.mdModuleIndices = g.mdIndices;
   e.mdModuleOccupancy = g.mdOccupancy;
@@ -440,6 +449,7 @@ static lst_status stage_input(const lst_input& in, lst_event::Staged& st) {
   }
   const uint32_t nh = p.x.size();
   const uint32_t np = std::min<uint32_t>(p.ptIn.size(), lst::kNMaxPixelSegments);
+  st.truncated = p.ptIn.size() > lst::kNMaxPixelSegments;  // SDL/Event.cc:237-244
   Carve c;
   c.take(st.x, nh); c.take(st.y, nh); c.take(st.z, nh); c.take(st.det, nh); c.take(st.idx, nh);
   for (int k = 0; k < 4; k++) c.take(st.h[k], np);
@@ -475,15 +485,70 @@ static lst_status run_staged(lst_event* ev, const lst_event::Staged& st, int las
   p.delta_phi_change = st.dphi; p.pt_in = st.pt; p.pt_err = st.pterr; p.px = st.px; p.py = st.py; p.pz = st.pz;
   p.eta = st.eta; p.eta_err = st.etaerr; p.phi = st.phi; p.charge = st.charge; p.seed_idx = st.seed;
   p.superbin = st.superbin; p.pixel_type = st.ptype; p.is_quad = st.quad; p.n_pls = st.nPLS;
-  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s;
-  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s;
-  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s;
-  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s;
-  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s;
-  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s;
-  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s;
-  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s;
-  return lst_event_create_track_candidates(ev);
+  const lst_status warn = st.truncated ? (lst_status)LST_W_TRUNCATED : LST_OK;
+  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s == LST_OK ? warn : s;
+  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s == LST_OK ? warn : s;
+  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s == LST_OK ? warn : s;
+  s = lst_event_create_track_candidates(ev);
+  return s == LST_OK ? warn : s;
+}
+
+// the whole chain on the inputs ev->descDev describes: kernel launches only,
+// no host reads, so that it can be captured once and replayed per event
+static void enqueue_described_chain(lst_event* ev) {
+  const lst::EventView& v = ev->v;
+  const hipStream_t s = ev->stream;
+  lst::launch_stage_in(v, ev->descDev, ev->plsHitIdx, ev->plsDPhiChange, s);
+  lst::launch_hits(v, s);
+  const uint32_t* hi[4] = {ev->plsHitIdx[0], ev->plsHitIdx[1], ev->plsHitIdx[2], ev->plsHitIdx[3]};
+  lst::launch_pixel_segments(v, hi, ev->plsDPhiChange, s);
+  lst::launch_mini_doublets(v, s);
+  lst::launch_segments(v, s);
+  lst::launch_triplets(v, s);
+  lst::launch_quintuplets(v, s);
+  lst::launch_pixel_segment_cleaning(v, s);
+  lst::launch_pixel_quintuplets(v, s);
+  lst::launch_pixel_triplets(v, s);
+  lst::launch_track_candidates(v, s);
+}
+
+// run a staged event through the context's captured chain (captured on first use)
+static lst_status run_staged_graph(lst_event* ev, const lst_event::Staged& st) {
+  lst_status s = ev->allocate(st.nHits);
+  if (s != LST_OK) return s;
+  if (!ev->descDev) {
+    HIP_OK(hipMalloc(&ev->descDev, sizeof(lst::InputDesc)));
+    HIP_OK(hipHostMalloc((void**)&ev->descHost, sizeof(lst::InputDesc), hipHostMallocDefault));
+  }
+  if (!ev->graph || ev->graphGen != ev->arenaGen) {
+    if (ev->graph) HIP_OK(hipGraphExecDestroy(ev->graph));
+    ev->graph = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(ev->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_described_chain(ev);
+    HIP_OK(hipStreamEndCapture(ev->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&ev->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(LST_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    ev->graphGen = ev->arenaGen;
+  }
+  lst::InputDesc& d = *ev->descHost;  // the stream is idle here (the batch waited for this context)
+  d.x = st.x; d.y = st.y; d.z = st.z; d.det = st.det; d.idx = st.idx; d.nHits = st.nHits;
+  for (int k = 0; k < 4; k++) d.hitIdx[k] = st.h[k];
+  d.dPhiChange = st.dphi; d.ptIn = st.pt; d.ptErr = st.pterr; d.px = st.px; d.py = st.py; d.pz = st.pz;
+  d.eta = st.eta; d.etaErr = st.etaerr; d.phi = st.phi; d.charge = st.charge; d.superbin = st.superbin;
+  d.seedIdx = st.seed; d.pixelType = st.ptype; d.isQuad = st.quad; d.nPLS = st.nPLS;
+  HIP_OK(hipMemcpyAsync(ev->descDev, ev->descHost, sizeof(lst::InputDesc), hipMemcpyHostToDevice, ev->stream));
+  HIP_OK(hipGraphLaunch(ev->graph, ev->stream));
+  ev->v.nHits = st.nHits;
+  ev->v.nPLS = st.nPLS;
+  ev->stage = 9;
+  return st.truncated ? (lst_status)LST_W_TRUNCATED : LST_OK;
 }
 
 extern "C" {
@@ -554,6 +619,10 @@ void lst_event_destroy(lst_event* ev) {
     if (ev->timers.start[k]) hipEventDestroy(ev->timers.start[k]);
     if (ev->timers.stop[k]) hipEventDestroy(ev->timers.stop[k]);
   }
+  if (ev->dnnRecordMem) (void)hipFree(ev->dnnRecordMem);
+  if (ev->graph) (void)hipGraphExecDestroy(ev->graph);
+  if (ev->descDev) (void)hipFree(ev->descDev);
+  if (ev->descHost) (void)hipHostFree(ev->descHost);
   if (ev->ownStream) hipStreamDestroy(ev->stream);
   delete ev;
 }
@@ -782,6 +851,50 @@ lst_status lst_event_kernel_times(lst_event* ev, float* ms, int n) {
 
 void* lst_event_stream(lst_event* ev) { return ev ? (void*)ev->stream : nullptr; }
 
+lst_status lst_event_record_t5dnn(lst_event* ev, uint32_t capacity) {
+  if (!ev) return fail(LST_E_INPUT, "null event");
+  if (ev->t5Mode != LST_T5_DNN) return fail(LST_E_INPUT, "the T5 DNN record needs an event in LST_T5_DNN mode");
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  if (ev->dnnRecordMem) HIP_OK(hipFree(ev->dnnRecordMem));
+  ev->dnnRecordMem = nullptr;
+  ev->v.dnnDump = lst::T5DnnDump{};
+  if (capacity == 0) return LST_OK;
+  if (capacity > lst::kT5ResultMax) capacity = lst::kT5ResultMax;
+  Carve c;
+  lst::T5DnnDump& d = ev->v.dnnDump;
+  c.take(d.features, 38 * (size_t)capacity); c.take(d.score, capacity);
+  c.take(d.t3Pair, 2 * (size_t)capacity); c.take(d.hitIdxs, 10 * (size_t)capacity);
+  HIP_OK(hipMalloc(&ev->dnnRecordMem, c.off));
+  c.bind(static_cast<char*>(ev->dnnRecordMem));
+  d.capacity = capacity;
+  return LST_OK;
+}
+
+lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* features, float* scores, uint32_t* t3_pairs,
+                                      uint32_t* hit_idxs, uint32_t capacity) {
+  if (!ev || !n) return fail(LST_E_INPUT, "null argument");
+  const lst::T5DnnDump& d = ev->v.dnnDump;
+  if (!d.features) return fail(LST_E_ORDER, "no T5 DNN record was requested");
+  if (ev->stage < 5) return fail(LST_E_ORDER, "the T5 DNN record needs createQuintuplets");
+  uint32_t scal[lst::kNScalars];
+  HIP_OK(hipMemcpyAsync(scal, ev->v.R.scalars, sizeof(scal), hipMemcpyDeviceToHost, ev->stream));
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  if (ev->v.perModuleBuilders || scal[lst::kT5CandTotal] > lst::kT5CandMax || scal[lst::kT5Results] > lst::kT5ResultMax)
+    return fail(LST_E_CAPACITY, "the event took the per-module quintuplet builder, which records no DNN rows");
+  const uint32_t rows = scal[lst::kT5Results];
+  const uint32_t kept = std::min(rows, d.capacity);
+  *n = kept;
+  const uint32_t m = std::min(kept, capacity);
+  if (m) {
+    if (features) HIP_OK(hipMemcpyAsync(features, d.features, (size_t)m * 38 * 4, hipMemcpyDeviceToHost, ev->stream));
+    if (scores) HIP_OK(hipMemcpyAsync(scores, d.score, (size_t)m * 4, hipMemcpyDeviceToHost, ev->stream));
+    if (t3_pairs) HIP_OK(hipMemcpyAsync(t3_pairs, d.t3Pair, (size_t)m * 8, hipMemcpyDeviceToHost, ev->stream));
+    if (hit_idxs) HIP_OK(hipMemcpyAsync(hit_idxs, d.hitIdxs, (size_t)m * 40, hipMemcpyDeviceToHost, ev->stream));
+    HIP_OK(hipStreamSynchronize(ev->stream));
+  }
+  return rows > d.capacity || kept > capacity ? (lst_status)LST_W_TRUNCATED : LST_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- batches
@@ -800,6 +913,7 @@ struct lst_batch {
   };
   std::vector<Out> outs;
   std::vector<lst_event::Staged> staged;
+  bool graphs = true;  // staged events run through each context's captured chain
 };
 
 namespace {
@@ -896,6 +1010,8 @@ lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_ba
     b->ctx.push_back(ev);
   }
   b->outs.resize(n_contexts);
+  const char* g = std::getenv("LST_BATCH_GRAPHS");  // "0": launch every kernel instead
+  b->graphs = !(g && g[0] == '0');
   *out = b.release();
   return LST_OK;
 }
@@ -938,8 +1054,10 @@ lst_status lst_batch_run_staged(lst_batch* b, uint32_t first, uint32_t n_ev, lst
   if (ns == 0 || first >= ns) return fail(LST_E_INPUT, "batch slots not staged");
   for (const auto& st : b->staged)
     if (!st.mem) return fail(LST_E_INPUT, "batch slot not staged");
-  return run_round_robin(b, n_ev, out,
-                         [&](lst_event* ev, uint32_t i) { return run_staged(ev, b->staged[(first + i) % ns], 9); });
+  return run_round_robin(b, n_ev, out, [&](lst_event* ev, uint32_t i) {
+    const lst_event::Staged& st = b->staged[(first + i) % ns];
+    return b->graphs ? run_staged_graph(ev, st) : run_staged(ev, st, 9);
+  });
 }
 
 uint32_t lst_batch_contexts(const lst_batch* b) { return b ? (uint32_t)b->ctx.size() : 0; }

@@ -87,6 +87,13 @@ struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
 struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };
 struct TCSoA { LST_TC_FIELDS(LST_DECL_FIELD) };
 
+// Pass-flag prefix shared by the flattened builders (one builder at a time).
+constexpr uint32_t kScanCandMax = 1u << 23;  // >= every builder's candidate maximum
+struct FlatScan {
+  uint32_t* pre;   // [kScanCandMax] exclusive prefix of the pass flags inside each scan block
+  uint32_t* sums;  // [kScanCandMax / 4096 + 1] exclusive prefix of the block totals, then the total
+};
+
 // Work arrays of the flattened mini-doublet builder: every (lower hit, upper
 // hit) pair of every module gets a global index L (module, then lower hit,
 // then upper hit: the reference's serial order); passing pairs park their
@@ -123,8 +130,6 @@ struct T3Work {
   uint32_t* resIdx;   // [kT3CandMax]
   uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check, any order
   uint32_t* results;  // [kT3ResultWords * kT3ResultMax]
-  uint32_t* mdHead;   // [MD capacity] per outer-tracker MD: last linked segment starting there, ~0: none
-  uint32_t* segNext;  // [segment capacity] next segment with the same inner MD
 };
 
 // Work arrays of the flattened T5 builder: every (inner T3, outer T3)
@@ -241,4 +246,9 @@ enum Scalar : int {
   kEvNPLS = 30,   // pixel line segments of the event
   kNScalars = 32
 };
+}  // namespace lst
+namespace lst {
+static_assert(kScanCandMax >= kMDCandMax && kScanCandMax >= kLSCandMax && kScanCandMax >= kT3CandMax &&
+                  kScanCandMax >= kT5CandMax,
+              "the shared pass-flag scan must cover every flattened builder");
 }  // namespace lst

@@ -6,6 +6,17 @@
 
 namespace lst {
 
+// Optional record of every T5 candidate at the DNN call site (DNN mode,
+// flattened builder): the 38 inputs, the score, the (inner, outer) T3 pair
+// and the 10 hit indices in input numbering.  Row k is parked result k.
+struct T5DnnDump {
+  float* features = nullptr;  // [capacity * 38]; null: no record
+  float* score = nullptr;     // [capacity]
+  uint32_t* t3Pair = nullptr; // [capacity * 2]
+  uint32_t* hitIdxs = nullptr;// [capacity * 10]
+  uint32_t capacity = 0;
+};
+
 struct EventView {  // everything a stage kernel needs, by value
   ModulesView M;
   HitsSoA H;
@@ -18,6 +29,7 @@ struct EventView {  // everything a stage kernel needs, by value
   PT3SoA P3;
   TCSoA C;
   PixelWork W;
+  FlatScan FS;
   MDWork W2;
   LSWork WL;
   T3Work W3;
@@ -36,6 +48,7 @@ struct EventView {  // everything a stage kernel needs, by value
   int tcPlsTriplets;                 // TC_PLS_TRIPLETS: triplet pLS may become track candidates
   int perModuleBuilders;             // run the per-module T3/T5 builders instead of the flattened ones
   int t5Dnn;                         // T5 selection by the DNN (USE_T5_DNN) instead of the r-phi chi2 cut
+  T5DnnDump dnnDump;
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };
@@ -90,6 +103,24 @@ struct CopyList {
 void launch_copy_multi(const CopyList& c, hipStream_t s);
 // writes one device word (the event's hit / pLS counts) in stream order
 void launch_set_scalar(uint32_t* p, uint32_t value, hipStream_t s);
+
+// An event's prepared inputs in device memory (hits, then pixel line
+// segments), described in device memory so that one captured launch sequence
+// can read whichever event the descriptor names.
+struct InputDesc {
+  const float *x, *y, *z;
+  const uint32_t *det, *idx;
+  uint32_t nHits;
+  const uint32_t* hitIdx[4];
+  const float *dPhiChange, *ptIn, *ptErr, *px, *py, *pz, *eta, *etaErr, *phi;
+  const int32_t *charge, *superbin;
+  const uint32_t* seedIdx;
+  const int8_t *pixelType, *isQuad;
+  uint32_t nPLS;
+};
+// copies the described inputs into the event arena and sets the event's counts
+void launch_stage_in(const EventView& v, const InputDesc* d, uint32_t* const plsHitIdx[4], float* plsDPhiChange,
+                     hipStream_t s);
 
 void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
 void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,

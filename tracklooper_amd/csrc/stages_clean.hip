@@ -18,8 +18,6 @@ namespace lst {
 namespace {
 
 constexpr int kTile = 256;
-constexpr int kBandShares = 16;  // y shares per x tile in the eta-banded kernels
-constexpr unsigned kPlsTiles = 64;  // x-tile work-groups of the pLS kernels (they stride over the tiles)
 
 // OR `bits` into byte k of an int8/uint8 flag array (256-byte aligned base)
 __device__ __forceinline__ void or_byte(void* base, uint32_t k, uint32_t bits) {
@@ -35,6 +33,25 @@ struct YShare {
 __device__ __forceinline__ YShare y_share(uint32_t ny) {
   const uint32_t chunk = (ny + gridDim.y - 1) / gridDim.y;
   const uint32_t b = blockIdx.y * chunk;
+  const uint32_t e = b + chunk < ny ? b + chunk : ny;
+  return {b < e ? b : e, e};
+}
+// Work items (x tile, y share) for a fixed 1-D grid whose blocks stride over
+// the items: the share count comes from the device-side x count, so that
+// about kPairItems items exist whatever the event's size.
+constexpr uint32_t kPairItems = 2048, kMaxShares = 256;
+struct PairItems {
+  uint32_t shares, items;
+};
+__device__ __forceinline__ PairItems pair_items(uint32_t nx) {
+  const uint32_t tiles = (nx + kTile - 1) / kTile;
+  uint32_t sh = tiles ? kPairItems / tiles : 1;
+  sh = sh < 1 ? 1 : (sh > kMaxShares ? kMaxShares : sh);
+  return {sh, tiles * sh};
+}
+__device__ __forceinline__ YShare share_of(uint32_t ny, uint32_t sh, uint32_t shares) {
+  const uint32_t chunk = (ny + shares - 1) / shares;
+  const uint32_t b = sh * chunk;
   const uint32_t e = b + chunk < ny ? b + chunk : ny;
   return {b < e ? b : e, e};
 }
@@ -128,11 +145,13 @@ __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
   __shared__ uint32_t sHits[4 * kTile], sId[kTile];
   __shared__ uint8_t sQuad[kTile], sOk[kTile];
   const uint32_t n = v.R.scalars[kEvNPLS];
-  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+  const PairItems items = pair_items(n);
+  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
+    const uint32_t x0 = (it / items.shares) * kTile;
     const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
     const Band B = tile_band(v.W.plsBinOff, eta_bin(v.P.eta[v.W.plsBinned[x0]]),
                              eta_bin(v.P.eta[v.W.plsBinned[xl]]));
-    const YShare ys = y_share(B.n1 + B.n2);
+    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
     const uint32_t xi = x0 + threadIdx.x;
     bool active = xi < n;
     uint32_t k = 0;
@@ -390,10 +409,12 @@ __global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
   __shared__ uint8_t sOk[kTile];
   __shared__ uint32_t sHits[10 * kTile];
   const uint32_t n = v.R.scalars[kT5Listed];
-  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+  const PairItems items = pair_items(n);
+  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
+    const uint32_t x0 = (it / items.shares) * kTile;
     const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
     const Band B = tile_band(v.W.t5BinOff, eta_bin(v.Q.eta[v.W.t5Binned[x0]]), eta_bin(v.Q.eta[v.W.t5Binned[xl]]));
-    const YShare ys = y_share(B.n1 + B.n2);
+    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
     const uint32_t xi = x0 + threadIdx.x;
     uint32_t x = 0;
     bool active = xi < n;
@@ -534,8 +555,10 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
   const uint32_t n = v.R.scalars[kEvNPLS];
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const YShare ys = y_share(nTC);
-  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+  const PairItems items = pair_items(n);
+  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
+    const uint32_t x0 = (it / items.shares) * kTile;
+    const YShare ys = share_of(nTC, it % items.shares, items.shares);
     const uint32_t i = x0 + threadIdx.x;
     const bool active = i < n && v.P.isQuad[i] && v.P.isDup[i] == 0;
     float eta1 = 0, phi1 = 0;
@@ -586,7 +609,7 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
 }  // namespace
 
 void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t) {
-  const dim3 g(kPlsTiles, kBandShares);  // x tiles stride over the pLS
+  const dim3 g(kPairItems);  // work-groups stride over the (x tile, y share) items
   const int tm = second ? kTmPlsDup2 : kTmPlsDup;
   if (!second) hipLaunchKernelGGL(k_bin_pls, dim3(1), dim3(1024), 0, s, v);  // the eta order is kept for pass 2
   t.begin(s, tm);
@@ -618,8 +641,7 @@ void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTime
   hipLaunchKernelGGL(k_t5_list_fill, dim3(256), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT5DupTC);
-  const unsigned gx = (v.t5Cap + kTile - 1) / kTile;  // x tiles stride over the listed T5s
-  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(gx < 128 ? gx : 128, kBandShares), dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
   t.end(s, kTmT5DupTC);
 }
 
@@ -631,7 +653,7 @@ void launch_cross_t5(const EventView& v, hipStream_t s) {
 void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_tc_prep, dim3(128), dim3(256), 0, s, v);
   t.begin(s, kTmCrossPLS);
-  hipLaunchKernelGGL(k_cross_pls_pairs, dim3(kPlsTiles, 32), dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_cross_pls_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
   t.end(s, kTmCrossPLS);
 }
 

@@ -12,6 +12,7 @@
 #include <climits>
 
 #include "algo_md_ls_t3.h"
+#include "flat_scan.h"
 #include "stages.h"
 #include "wave.h"
 
@@ -306,41 +307,49 @@ __device__ __forceinline__ uint32_t md_wave_index() {
   return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
-// one wave per module: passing pairs take their slots in L order
+// slots: passing pairs in L order (pass-flag scan), then per-module counts
+__global__ __launch_bounds__(1024) void k_md_scan_blocks(EventView v) {
+  if (!md_flat_done(v)) return;
+  scan_flag_blocks(
+      v.R.scalars[kMDCandTotal], [&](uint32_t L) { return v.W2.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+}
+__global__ __launch_bounds__(1024) void k_md_scan_sums(EventView v) {
+  if (!md_flat_done(v)) return;
+  scan_flag_sums(v.R.scalars[kMDCandTotal], v.FS.sums);
+}
+
+__device__ __forceinline__ uint32_t module_of(const uint32_t* off, uint32_t n, uint32_t L) {
+  uint32_t lo = 0, hi = n;  // largest m with off[m] <= L
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= L) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void k_md_place(EventView v) {
   if (!md_flat_done(v)) return;
-  const unsigned lane = lane_id();
-  const uint32_t nWaves = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += nWaves) {
-    const int loBase = v.R.hitRangesLower[m];
-    const int cap = v.mdModuleOccupancy[m];
-    unsigned count = 0;
-    const uint32_t off = v.W2.candOff[m], n = v.W2.candOff[m + 1] - off;
-    if (loBase != -1 && n != 0) {
-      const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m];
-      const uint32_t upBase = (uint32_t)v.R.hitRangesUpper[m];
-      const ModCtx c = load_mod(v.M, m);
-      const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
-      const uint32_t base = v.mdModuleIndices[m];
-      for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
-        const uint32_t l = c0 + lane;
-        const uint32_t k = l < n ? v.W2.resIdx[off + l] : ~0u;
-        const bool pass = k != ~0u;
-        unsigned tot;
-        const unsigned rank = count + wave_rank(pass, tot);
-        if (pass && (int)rank < cap) {
-          const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
-          const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
-          const uint32_t li = l / nUp, ui = l - li * nUp;
-          store_md(v, (uint32_t)loBase + li, upBase + ui, m, anchorIsUpper, o, base + rank);
-        }
-        count += tot;
-      }
-    }
-    if (lane == 0) {
-      v.R.totOccupancyMDs[m] = count;
-      v.R.nMDs[m] = min(count, (unsigned)max(cap, 0));
-    }
+  const uint32_t total = v.R.scalars[kMDCandTotal], nL = v.M.nLowerModules;
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    const uint32_t k = v.W2.resIdx[L];
+    if (k == ~0u) continue;
+    const uint32_t m = module_of(v.W2.candOff, nL, L);
+    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W2.candOff[m], total);
+    if ((int)rank >= v.mdModuleOccupancy[m]) continue;
+    const ModCtx c = load_mod(v.M, m);
+    const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
+    const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m], l = L - v.W2.candOff[m];
+    const uint32_t li = l / nUp, ui = l - li * nUp;
+    const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
+    const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
+    store_md(v, (uint32_t)v.R.hitRangesLower[m] + li, (uint32_t)v.R.hitRangesUpper[m] + ui, m, anchorIsUpper, o,
+             v.mdModuleIndices[m] + rank);
+  }
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+    const uint32_t count = flag_rank(v.FS, v.W2.candOff[m + 1], total) - flag_rank(v.FS, v.W2.candOff[m], total);
+    v.R.totOccupancyMDs[m] = count;
+    v.R.nMDs[m] = min(count, (unsigned)max(v.mdModuleOccupancy[m], 0));
   }
 }
 
@@ -517,39 +526,41 @@ __global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_ls_scan_blocks(EventView v) {
+  if (!ls_flat_done(v)) return;
+  scan_flag_blocks(
+      v.R.scalars[kLSCandTotal], [&](uint32_t L) { return v.WL.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+}
+__global__ __launch_bounds__(1024) void k_ls_scan_sums(EventView v) {
+  if (!ls_flat_done(v)) return;
+  scan_flag_sums(v.R.scalars[kLSCandTotal], v.FS.sums);
+}
+
 __global__ __launch_bounds__(256) void k_ls_place(EventView v) {
   if (!ls_flat_done(v)) return;
-  const unsigned lane = lane_id();
-  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
-    const int cap = v.segModuleOccupancy[m];
-    const uint32_t off = v.WL.candOff[m], n = v.WL.candOff[m + 1] - off;
-    const uint32_t segBase = v.segModuleIndices[m];
-    unsigned count = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
-      const uint32_t l = c0 + lane;
-      const uint32_t k = l < n ? v.WL.resIdx[off + l] : ~0u;
-      const bool pass = k != ~0u;
-      unsigned tot;
-      const unsigned rank = count + wave_rank(pass, tot);
-      if (pass && (int)rank < cap) {
-        uint16_t om;
-        uint32_t inMD, outMD;
-        ls_decode(v, m, l, om, inMD, outMD);
-        const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
-        const uint32_t s = segBase + rank;
-        v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
-        v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
-        v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
-        v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
-        v.S.dPhis[s] = w[0]; v.S.dPhiMins[s] = w[1]; v.S.dPhiMaxs[s] = w[2];
-        v.S.dPhiChanges[s] = w[3]; v.S.dPhiChangeMins[s] = w[4]; v.S.dPhiChangeMaxs[s] = w[5];
-      }
-      count += tot;
-    }
-    if (lane == 0) {
-      v.R.totOccupancySegments[m] = count;
-      v.R.nSegments[m] = min(count, (unsigned)max(cap, 0));
-    }
+  const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    const uint32_t k = v.WL.resIdx[L];
+    if (k == ~0u) continue;
+    const uint32_t m = module_of(v.WL.candOff, nL, L);
+    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.WL.candOff[m], total);
+    if ((int)rank >= v.segModuleOccupancy[m]) continue;
+    uint16_t om;
+    uint32_t inMD, outMD;
+    ls_decode(v, m, L - v.WL.candOff[m], om, inMD, outMD);
+    const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
+    const uint32_t s = v.segModuleIndices[m] + rank;
+    v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
+    v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
+    v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
+    v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
+    v.S.dPhis[s] = w[0]; v.S.dPhiMins[s] = w[1]; v.S.dPhiMaxs[s] = w[2];
+    v.S.dPhiChanges[s] = w[3]; v.S.dPhiChangeMins[s] = w[4]; v.S.dPhiChangeMaxs[s] = w[5];
+  }
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+    const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
+    v.R.totOccupancySegments[m] = count;
+    v.R.nSegments[m] = min(count, (unsigned)max(v.segModuleOccupancy[m], 0));
   }
 }
 
@@ -763,54 +774,45 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
   }
 }
 
-// stage 1: the shared middle mini-doublet.  A candidate (inner segment s,
-// outer segment o of s's outer module) passes when o starts at s's outer
-// mini-doublet, so the survivors of s are the segments linked from that
-// mini-doublet: per outer-tracker MD a list of the segments starting there
-// (one atomicExch per segment; list order is free, the placement goes by L).
-__global__ __launch_bounds__(256) void k_t3_link_reset(EventView v) {
-  if (!t3_flat_mode(v)) return;
-  const uint32_t n = (uint32_t)v.mdModuleIndices[v.M.nLowerModules];  // outer-tracker MD slots
-  for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x) v.W3.mdHead[d] = ~0u;
-  const uint32_t total = v.R.scalars[kT3CandTotal];  // every candidate starts unplaced
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
-    v.W3.resIdx[L] = ~0u;
-}
-
-__global__ __launch_bounds__(256) void k_t3_link(EventView v) {
-  if (!t3_flat_mode(v)) return;
-  for (uint32_t m = wave_index(); m < v.M.nLowerModules; m += wave_count()) {
-    const uint32_t n = v.R.nSegments[m], base = v.segModuleIndices[m];
-    for (uint32_t a = lane_id(); a < n; a += kWave) {
-      const uint32_t o = base + a;
-      v.W3.segNext[o] = atomicExch(&v.W3.mdHead[v.S.mdIndices[2 * o]], o);
-    }
-  }
-}
-
+// stage 1: shared middle mini-doublet over all candidates (each candidate
+// decoded by binary searches over the module offsets and the module's
+// per-inner-segment prefix); survivors appended to a list, order free
 __global__ __launch_bounds__(256) void k_t3_filter_md(EventView v) {
   if (!t3_flat_mode(v)) return;
-  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
-  const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
-  const uint32_t segBase = v.segModuleIndices[m1];
-  const uint32_t modOff = v.W3.modOff[m1];
-  for (uint32_t a = lane_id(); a < nIn; a += kWave) {
-    const uint32_t inSeg = segBase + a;
-    const uint32_t base2 = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]];
-    const uint32_t L0 = modOff + v.W3.candPre[inSeg];
-    for (uint32_t o = v.W3.mdHead[v.S.mdIndices[2 * inSeg + 1]]; o != ~0u; o = v.W3.segNext[o]) {
-      const uint32_t l2 = o - base2;
+  const uint32_t total = v.R.scalars[kT3CandTotal];
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
+    const uint32_t L = L0 + threadIdx.x;
+    if (L >= total) continue;
+    v.W3.resIdx[L] = ~0u;
+    uint32_t lo = 0, hi = nL;  // largest m with modOff[m] <= L
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (v.W3.modOff[mid] <= L) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t m1 = lo, l = L - v.W3.modOff[lo];
+    const uint32_t segBase = v.segModuleIndices[m1];
+    uint32_t a0 = 0, a1 = v.R.nSegments[m1];  // largest a with candPre[a] <= l
+    while (a1 - a0 > 1) {
+      const uint32_t mid = (a0 + a1) >> 1;
+      if (v.W3.candPre[segBase + mid] <= l) a0 = mid;
+      else a1 = mid;
+    }
+    const uint32_t inSeg = segBase + a0;
+    const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg];
+    const uint32_t outSeg = v.segModuleIndices[m2] + (l - v.W3.candPre[inSeg]);
+    if (v.S.mdIndices[2 * inSeg + 1] == v.S.mdIndices[2 * outSeg]) {
       const uint64_t mask = __ballot(1);
       const unsigned leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t b = 0;
-      if (lane_id() == leader) b = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
-      b = __shfl(b, leader, 64);
-      const uint32_t k = b + lanes_below(mask);
-      v.W3.list[3 * k] = L0 + l2;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      const uint32_t k = base + lanes_below(mask);
+      v.W3.list[3 * k] = L;
       v.W3.list[3 * k + 1] = m1;
-      v.W3.list[3 * k + 2] = a | (l2 << 16);
+      v.W3.list[3 * k + 2] = (inSeg - segBase) | ((l - v.W3.candPre[inSeg]) << 16);
     }
-  }
   }
 }
 
@@ -838,31 +840,34 @@ __global__ __launch_bounds__(256) void k_t3_select(EventView v) {
 }
 
 // stage 3: per module, passing candidates in candidate order take the slots
+__global__ __launch_bounds__(1024) void k_t3_scan_blocks(EventView v) {
+  if (!t3_flat_done(v)) return;
+  scan_flag_blocks(
+      v.R.scalars[kT3CandTotal], [&](uint32_t L) { return v.W3.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+}
+__global__ __launch_bounds__(1024) void k_t3_scan_sums(EventView v) {
+  if (!t3_flat_done(v)) return;
+  scan_flag_sums(v.R.scalars[kT3CandTotal], v.FS.sums);
+}
+
+// passing candidates take their slots in L order; then per-module counts
 __global__ __launch_bounds__(256) void k_t3_place(EventView v) {
   if (!t3_flat_done(v)) return;
-  const unsigned lane = lane_id();
-  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
-    const uint32_t off = v.W3.modOff[m1], n = v.W3.modOff[m1 + 1] - off;
-    const int cap = v.R.tripletModuleOccupancy[m1];
-    const uint32_t t3Base = v.R.tripletModuleIndices[m1];
-    unsigned count = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += kWave) {
-      const uint32_t l = c0 + lane;
-      const uint32_t k = l < n ? v.W3.resIdx[off + l] : ~0u;
-      const bool pass = k != ~0u;
-      unsigned tot;
-      const unsigned rank = count + wave_rank(pass, tot);
-      if (pass && (int)rank < cap) {
-        const uint32_t* w = &v.W3.results[kT3ResultWords * k];
-        t3_store(v, t3Base + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
-                 __uint_as_float(w[4]), __uint_as_float(w[5]));
-      }
-      count += tot;
-    }
-    if (lane == 0) {
-      v.R.totOccupancyTriplets[m1] = count;
-      v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
-    }
+  const uint32_t total = v.R.scalars[kT3CandTotal], nL = v.M.nLowerModules;
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    const uint32_t k = v.W3.resIdx[L];
+    if (k == ~0u) continue;
+    const uint32_t m1 = module_of(v.W3.modOff, nL, L);
+    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W3.modOff[m1], total);
+    if ((int)rank >= v.R.tripletModuleOccupancy[m1]) continue;
+    const uint32_t* w = &v.W3.results[kT3ResultWords * k];
+    t3_store(v, v.R.tripletModuleIndices[m1] + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
+             __uint_as_float(w[4]), __uint_as_float(w[5]));
+  }
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+    const uint32_t count = flag_rank(v.FS, v.W3.modOff[m + 1], total) - flag_rank(v.FS, v.W3.modOff[m], total);
+    v.R.totOccupancyTriplets[m] = count;
+    v.R.nTriplets[m] = min(count, (unsigned)max(v.R.tripletModuleOccupancy[m], 0));
   }
 }
 
@@ -950,6 +955,40 @@ __global__ __launch_bounds__(256) void k_copy_multi(CopyList c) {
 
 __global__ void k_set_scalar(uint32_t* p, uint32_t value) { *p = value; }
 
+struct PlsArena {
+  uint32_t* hitIdx[4];
+  float* dPhiChange;
+};
+
+__global__ __launch_bounds__(256) void k_stage_in(EventView v, const InputDesc* __restrict__ d, PlsArena a) {
+  const uint32_t nh = d->nHits, np = d->nPLS;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    v.R.scalars[kEvNHits] = nh;
+    v.R.scalars[kEvNPLS] = np;
+  }
+  const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (uint32_t i = t0; i < nh; i += stride) {
+    v.H.xs[i] = d->x[i]; v.H.ys[i] = d->y[i]; v.H.zs[i] = d->z[i];
+    v.H.detid[i] = d->det[i]; v.H.idxs[i] = d->idx[i];
+  }
+  for (uint32_t i = t0; i < np; i += stride) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.hitIdx[k][i] = d->hitIdx[k][i];
+    a.dPhiChange[i] = d->dPhiChange[i];
+    v.P.ptIn[i] = d->ptIn[i]; v.P.ptErr[i] = d->ptErr[i];
+    v.P.px[i] = d->px[i]; v.P.py[i] = d->py[i]; v.P.pz[i] = d->pz[i];
+    v.P.eta[i] = d->eta[i]; v.P.etaErr[i] = d->etaErr[i]; v.P.phi[i] = d->phi[i];
+    v.P.charge[i] = d->charge[i]; v.P.seedIdx[i] = d->seedIdx[i]; v.P.superbin[i] = d->superbin[i];
+    v.P.pixelType[i] = d->pixelType[i]; v.P.isQuad[i] = d->isQuad[i];
+  }
+}
+
+void launch_stage_in(const EventView& v, const InputDesc* d, uint32_t* const plsHitIdx[4], float* plsDPhiChange,
+                     hipStream_t s) {
+  const PlsArena a{{plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]}, plsDPhiChange};
+  hipLaunchKernelGGL(k_stage_in, dim3(512), dim3(256), 0, s, v, d, a);
+}
+
 void launch_set_scalar(uint32_t* p, uint32_t value, hipStream_t s) {
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, p, value);
 }
@@ -985,7 +1024,9 @@ void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers&
   t.begin(s, kTmMiniDoublets);
   hipLaunchKernelGGL(k_md_eval, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmMiniDoublets);
-  hipLaunchKernelGGL(k_md_place, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_md_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_md_scan_sums, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_md_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
   hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
@@ -996,7 +1037,9 @@ void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
   t.begin(s, kTmSegments);
   hipLaunchKernelGGL(k_ls_eval, dim3(4096), dim3(256), 0, s, v);
   t.end(s, kTmSegments);
-  hipLaunchKernelGGL(k_ls_place, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_ls_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_ls_scan_sums, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_ls_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
   hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
@@ -1005,15 +1048,15 @@ void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_link_reset, dim3(1024), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t3_link, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.begin(s, kTmT3Filter);
-  hipLaunchKernelGGL(k_t3_filter_md, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_filter_md, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmT3Filter);
   t.begin(s, kTmT3Select);
   hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmT3Select);
-  hipLaunchKernelGGL(k_t3_place, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_scan_sums, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }

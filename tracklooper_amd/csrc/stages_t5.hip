@@ -11,7 +11,9 @@
 // slots equal the reference CPU slots and the per-module capacity keeps exactly
 // the first `cap` passing candidates.  Events with more than kT5CandMax
 // candidates take the per-module builder instead (same order, same results).
+#include "../../include/lst_amd.h"
 #include "algo_t5.h"
+#include "flat_scan.h"
 #include "stages.h"
 #include "wave.h"
 
@@ -21,6 +23,7 @@ namespace {
 constexpr int kT5Block = 256;
 constexpr int kMaxInnerT3 = 1024;   // > the largest T3 module capacity (755)
 constexpr int kMaxT5PerModule = 416;  // >= the largest T5 module capacity (414)
+constexpr int kAfterBuildSplit = 8;   // partner shares per module in the after-build duplicate check
 
 // createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
 // seed modules holding a triplet, their capacities and the exclusive scan of
@@ -213,9 +216,9 @@ constexpr float kMagneticField = 3.8112f;  // SDL/Constants.h:135
 
 __device__ __forceinline__ float dnn_log10(float x) { return m_log(x) / 2.302585093f; }
 
-__device__ __forceinline__ float t5dnn_score(const EventView& v, const T5Cand& c, uint32_t innerT3, float innerR,
-                                          float bridgeR, float outerR) {
-  float in[38];
+// the 38 inputs (SDL/NeuralNetwork.h:36-114)
+__device__ __forceinline__ void t5dnn_features(const EventView& v, const T5Cand& c, uint32_t innerT3, float innerR,
+                                               float bridgeR, float outerR, float* in) {
   const int hitOf[6] = {0, 1, 2, 2, 3, 4};  // inner T3 hits 1-3, outer T3 hits 3-5
 #pragma unroll
   for (int b = 0; b < 6; b++) {
@@ -239,6 +242,10 @@ __device__ __forceinline__ float t5dnn_score(const EventView& v, const T5Cand& c
   in[35] = dnn_log10(innerR);
   in[36] = dnn_log10(bridgeR);
   in[37] = dnn_log10(outerR);
+}
+
+// Linear(38,32) ReLU Linear(32,32) ReLU Linear(32,1) sigmoid (SDL/NeuralNetwork.h:116-162)
+__device__ __forceinline__ float t5dnn_forward(const float* in) {
   float h1[32];
 #pragma unroll
   for (int o = 0; o < 32; o++) {
@@ -260,6 +267,13 @@ __device__ __forceinline__ float t5dnn_score(const EventView& v, const T5Cand& c
   }
   out += cDnnBias4[0];
   return m_exp(out) / (m_exp(out) + 1);  // NaN once exp overflows, as in the reference
+}
+
+__device__ __forceinline__ float t5dnn_score(const EventView& v, const T5Cand& c, uint32_t innerT3, float innerR,
+                                             float bridgeR, float outerR) {
+  float in[38];
+  t5dnn_features(v, c, innerT3, innerR, bridgeR, outerR, in);
+  return t5dnn_forward(in);
 }
 
 __device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t innerT3, uint32_t outerT3,
@@ -479,45 +493,65 @@ __global__ __launch_bounds__(256) void k_t5_dnn(EventView v) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     uint32_t* w = &v.W5.results[kT5ResultWords * k];
     const uint32_t inner = w[0], outer = w[1];
-    const float score = t5dnn_score(v, t5_cand(v, inner, outer), inner, __uint_as_float(w[2]), __uint_as_float(w[3]),
-                                    __uint_as_float(w[4]));
+    float in[38];
+    t5dnn_features(v, t5_cand(v, inner, outer), inner, __uint_as_float(w[2]), __uint_as_float(w[3]),
+                   __uint_as_float(w[4]), in);
+    const float score = t5dnn_forward(in);
+    if (v.dnnDump.features && k < v.dnnDump.capacity) {  // training rows: every candidate at the DNN call site
+      for (int a = 0; a < 38; a++) v.dnnDump.features[38 * (size_t)k + a] = in[a];
+      v.dnnDump.score[k] = score;
+      v.dnnDump.t3Pair[2 * k] = inner;
+      v.dnnDump.t3Pair[2 * k + 1] = outer;
+      for (int a = 0; a < 6; a++) v.dnnDump.hitIdxs[10 * (size_t)k + a] = v.H.idxs[v.T.hitIndices[6 * inner + a]];
+      for (int a = 0; a < 4; a++)
+        v.dnnDump.hitIdxs[10 * (size_t)k + 6 + a] = v.H.idxs[v.T.hitIndices[6 * outer + 2 + a]];
+    }
     if (!(score > kT5DnnWP)) w[0] = ~0u;
   }
 }
 
-// stage 4: per module, passing candidates in candidate order take the slots
+// stage 4: passing candidates take their slots in candidate order (pass-flag
+// scan; a result the DNN rejected does not pass), then per-module counts
+__device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t L) {
+  const uint32_t k = v.W5.resIdx[L];
+  return k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
+}
+__global__ __launch_bounds__(1024) void k_t5_scan_blocks(EventView v) {
+  if (!t5_flat_done(v)) return;
+  scan_flag_blocks(v.R.scalars[kT5CandTotal], [&](uint32_t L) { return t5_passed(v, L); }, v.FS.pre, v.FS.sums);
+}
+__global__ __launch_bounds__(1024) void k_t5_scan_sums(EventView v) {
+  if (!t5_flat_done(v)) return;
+  scan_flag_sums(v.R.scalars[kT5CandTotal], v.FS.sums);
+}
+
 __global__ __launch_bounds__(256) void k_t5_place(EventView v) {
   if (!t5_flat_done(v)) return;
-  const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  const unsigned lane = lane_id();
-  for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
+  const uint32_t total = v.R.scalars[kT5CandTotal], nElig = v.R.scalars[kNEligibleT5];
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    if (!t5_passed(v, L)) continue;
+    uint32_t lo = 0, hi = nElig;  // largest e with modOff[e] <= L
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (v.W5.modOff[mid] <= L) lo = mid;
+      else hi = mid;
+    }
+    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[lo];
+    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W5.modOff[lo], total);
+    if ((int)rank >= v.R.quintupletModuleOccupancy[m1]) continue;
+    const uint32_t* w = &v.W5.results[kT5ResultWords * v.W5.resIdx[L]];
+    T5Result r;
+    r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
+    r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
+    r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
+    r.tight = w[11] != 0;
+    t5_store(v, (uint32_t)v.R.quintupletModuleIndices[m1] + rank, w[0], w[1], r, v.M.layers[m1]);
+  }
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nElig; e += gridDim.x * blockDim.x) {
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
-    const int layer = v.M.layers[m1];
-    const int cap = v.R.quintupletModuleOccupancy[m1];
-    const uint32_t off = v.W5.modOff[e], n = v.W5.modOff[e + 1] - off;
-    const uint32_t t5Base = (uint32_t)v.R.quintupletModuleIndices[m1];
-    unsigned count = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-      const uint32_t l = c0 + lane;
-      const uint32_t k = l < n ? v.W5.resIdx[off + l] : ~0u;
-      const bool pass = k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
-      unsigned tot;
-      const unsigned rank = count + wave_rank(pass, tot);
-      if (pass && (int)rank < cap) {
-        const uint32_t* w = &v.W5.results[kT5ResultWords * k];
-        T5Result r;
-        r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
-        r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
-        r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
-        r.tight = w[11] != 0;
-        t5_store(v, t5Base + rank, w[0], w[1], r, layer);
-      }
-      count += tot;
-    }
-    if (lane == 0) {
-      v.R.totOccupancyQuintuplets[m1] = count;
-      v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
-    }
+    const uint32_t count = flag_rank(v.FS, v.W5.modOff[e + 1], total) - flag_rank(v.FS, v.W5.modOff[e], total);
+    v.R.totOccupancyQuintuplets[m1] = count;
+    v.R.nQuintuplets[m1] = min(count, (unsigned)max(v.R.quintupletModuleOccupancy[m1], 0));
   }
 }
 
@@ -587,58 +621,51 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
 
 // removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249).  For a close
 // pair (i < j) sharing >= 7 hits the reference flags i when score_i >= score_j
-// and j otherwise.  Work items are (eligible module, chunk of 64 of its T5s);
-// a wave takes one item, each lane owns one T5 and tests every partner of the
-// module, so a flag is a store of 1 that does not depend on evaluation order.
-// Partner records are read at wave-uniform addresses (scalar loads).
-constexpr uint32_t kT5Chunks = (kMaxT5PerModule + 63) / 64;
-
-__global__ __launch_bounds__(256) void k_t5_dup_after_build(EventView v) {
-  const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  const unsigned lane = lane_id();
-  for (uint32_t it = wave_index(); it < nElig * kT5Chunks; it += wave_count()) {
-    const uint32_t e = it / kT5Chunks, cx = it % kT5Chunks;
-    const uint16_t m = v.R.indicesOfEligibleT5Modules[e];
-    const uint32_t nq = v.R.nQuintuplets[m];
-    if (nq < 2 || cx * 64 >= nq) continue;
-    const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
-    const uint32_t i = cx * 64 + lane;
-    const bool active = i < nq;
-    float etaI = 0, phiI = 0, si = 0;
-    uint32_t hi[10];
-    if (active) {
-      etaI = v.Q.eta[base + i];
-      phiI = v.Q.phi[base + i];
-      si = v.Q.score_rphisum[base + i];
-#pragma unroll
-      for (int a = 0; a < 10; a++) hi[a] = v.Q.hitIndices[10 * (base + i) + a];
-    }
+// and j otherwise.  A module's partners are split into kAfterBuildSplit equal
+// shares (blockIdx.y); each lane owns one T5 and tests the partners of its
+// share, so a flag is a store of 1 that does not depend on evaluation order.
+__global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
+  __shared__ float sEta[kMaxT5PerModule], sPhi[kMaxT5PerModule], sScore[kMaxT5PerModule];
+  __shared__ uint32_t sHits[kMaxT5PerModule * 10];
+  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
+  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
+  const uint32_t nq = v.R.nQuintuplets[m];
+  if (nq < 2 || nq > (uint32_t)kMaxT5PerModule) return;
+  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
+  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
+    sEta[i] = v.Q.eta[base + i];
+    sPhi[i] = v.Q.phi[base + i];
+    sScore[i] = v.Q.score_rphisum[base + i];
+  }
+  for (uint32_t i = threadIdx.x; i < nq * 10; i += kT5Block) sHits[i] = v.Q.hitIndices[10 * base + i];
+  __syncthreads();
+  const uint32_t share = (nq + gridDim.y - 1) / gridDim.y;
+  const uint32_t jb = blockIdx.y * share, je = jb + share < nq ? jb + share : nq;
+  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
     bool dup = false;
-    for (uint32_t j = 0; j < nq; j++) {
-      const float sj = v.Q.score_rphisum[base + j];
-      if (!active || dup || j == i) continue;
+    const float si = sScore[i];
+    for (uint32_t j = jb; j < je && !dup; j++) {
+      if (j == i) continue;
       // the pair can only flag i if the score order points at i
-      const bool targetsI = i < j ? (si >= sj) : !(sj >= si);
+      const bool targetsI = i < j ? (si >= sScore[j]) : !(sScore[j] >= si);
       if (!targetsI) continue;
-      const float etaJ = v.Q.eta[base + j], phiJ = v.Q.phi[base + j];
-      const float dEta = m_abs(i < j ? etaI - etaJ : etaJ - etaI);
-      const float dPhi = i < j ? wrap_dphi(phiI, phiJ) : wrap_dphi(phiJ, phiI);
+      const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+      const float dEta = m_abs(sEta[lo] - sEta[hi]);
+      const float dPhi = wrap_dphi(sPhi[lo], sPhi[hi]);
       if (dEta > 0.1f) continue;
       if (m_abs(dPhi) > 0.1f) continue;
-      uint32_t hj[10];
-#pragma unroll
-      for (int b = 0; b < 10; b++) hj[b] = v.Q.hitIndices[10 * (base + j) + b];
       int matched = 0;
-#pragma unroll
       for (int a = 0; a < 10; a++) {
-        bool hit = false;
-#pragma unroll
-        for (int b = 0; b < 10; b++) hit = hit || hi[a] == hj[b];
-        matched += hit ? 1 : 0;
+        const uint32_t h = sHits[10 * lo + a];
+        for (int b = 0; b < 10; b++)
+          if (h == sHits[10 * hi + b]) {
+            matched++;
+            break;
+          }
       }
       dup = matched >= 7;
     }
-    if (active && dup) v.Q.isDup[base + i] = 1;
+    if (dup) v.Q.isDup[base + i] = 1;
   }
 }
 
@@ -675,13 +702,49 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
     hipLaunchKernelGGL(k_t5_dnn, dim3(flatGrid), dim3(256), 0, s, v);
     t.end(s, kTmT5Dnn);
   }
-  hipLaunchKernelGGL(k_t5_place, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t5_scan_sums, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t5_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
   t.begin(s, kTmT5DupAfter);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks, kAfterBuildSplit), dim3(kT5Block), 0, s, v);
   t.end(s, kTmT5DupAfter);
   const unsigned g = (v.M.nLowerModules + 255) / 256;
   hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
 }
 
 }  // namespace lst
+// process-wide T5 DNN parameters (packed: bias_0, wgtT_0, bias_2, wgtT_2, bias_4, wgtT_4)
+extern "C" lst_status lst_t5dnn_set_weights(const float* params) {
+  if (!params) return LST_E_INPUT;
+  const float* p = params;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnBias0), p, 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnW0), p, 38 * 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 38 * 32;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnBias2), p, 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnW2), p, 32 * 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32 * 32;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnBias4), p, 4) != hipSuccess) return LST_E_DEVICE;
+  p += 1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnW4), p, 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? LST_OK : LST_E_DEVICE;
+}
+
+extern "C" lst_status lst_t5dnn_get_weights(float* params) {
+  if (!params) return LST_E_INPUT;
+  float* p = params;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnBias0), 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnW0), 38 * 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 38 * 32;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnBias2), 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnW2), 32 * 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  p += 32 * 32;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnBias4), 4) != hipSuccess) return LST_E_DEVICE;
+  p += 1;
+  if (hipMemcpyFromSymbol(p, HIP_SYMBOL(lst::cDnnW4), 32 * 4) != hipSuccess) return LST_E_DEVICE;
+  return LST_OK;
+}
