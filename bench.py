@@ -23,8 +23,8 @@ METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC
 STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets",
                "pixel_segment_cleaning", "pixel_quintuplets", "pixel_triplets", "track_candidates"]
 # the kernel each timer slot brackets (lst_event_kernel_times)
-KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_mini_doublets", "k_segments", "k_t3_select", "k_t5_final",
-                "k_pls_pairs<false>", "k_pixel_quintuplets<false>", "k_pixel_triplets<false>", "k_t5_tc_pairs",
+KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_md_eval", "k_ls_eval", "k_t3_select", "k_t5_final",
+                "k_pls_pairs<false>", "k_pix_stage_a<PT5Kind>", "k_pix_stage_a<PT3Kind>", "k_t5_tc_pairs",
                 "k_pt5_pairs", "k_pt3_pairs", "k_cross_pls_pairs", "k_pls_pairs<true>", "k_t5_dup_after_build",
                 "k_t3_filter_md", "k_t5_dnn"]
 ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
-MEASURED_TRAFFIC = {"k_segments": 58299420, "k_mini_doublets": 36042844}
+MEASURED_TRAFFIC = {}
 
 
 def _input_struct(lst, ev):
@@ -54,14 +54,16 @@ def algorithmic_bytes(counts: dict) -> dict:
     Record sizes are the fields each kernel reads/writes:
       k_hits             : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
       k_pixel_segments   : read 21 floats and 4 hit indices (100 B/pLS), write 2 MDs + 1 segment + pLS (400 B/pLS)
-      k_mini_doublets    : read x,y,z,rt of the module-pair hits (16 B/hit), write 134 B/MD
-      k_segments         : read 44 B of each mini-doublet, write 44 B/segment
+      k_md_eval          : read x,y,z,rt of the module-pair hits (16 B/hit), write a 4 B result index per
+                           candidate pair and 36 B of parked results per mini-doublet
+      k_ls_eval          : read 44 B of each mini-doublet, write a 4 B result index per candidate and
+                           24 B of parked results per segment
       k_t3_filter_md     : per candidate pair 18 B (prefix, module, two MD indices, result index), 12 B/survivor
       k_t3_select        : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
       k_t5_final         : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
       k_pls_pairs        : read eta, phi, score, isQuad, flag and 4 hit indices (30 B/pLS), write 1 B/pLS
-      k_pixel_quintuplets: read 60 B/pLS and 70 B/quintuplet, write a 4 B count per pLS
-      k_pixel_triplets   : read 60 B/pLS and 53 B/triplet, write a 4 B count per pLS
+      k_pix_stage_a<PT5Kind>: read 64 B/pLS and the 128 B record of each quintuplet, write 40 B per survivor
+      k_pix_stage_a<PT3Kind>: read 64 B/pLS and the 64 B record of each triplet, write 40 B per survivor
       k_pt5_pairs        : read score and 14 hit indices (60 B/pT5), write 1 B/pT5
       k_pt3_pairs        : read score, layer and 10 hit indices (45 B/pT3), write 1 B/pT3
       k_t5_tc_pairs,
@@ -73,15 +75,15 @@ def algorithmic_bytes(counts: dict) -> dict:
     return {
         "k_hits": 46 * h,
         "k_pixel_segments": 500 * pls,
-        "k_mini_doublets": 16 * h + 134 * md,
-        "k_segments": 44 * md + 44 * ls,
+        "k_md_eval": 16 * h + 4 * counts.get("mdcand", 0) + 36 * md,
+        "k_ls_eval": 44 * md + 4 * counts.get("lscand", 0) + 24 * ls,
         "k_t3_filter_md": 18 * counts.get("t3cand", 0) + 12 * counts.get("t3surv", 0),
         "k_t3_select": 22 * ls + 20 * md + 60 * t3,
         "k_t5_final": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
         "k_pls_pairs<false>": 31 * pls,
         "k_pls_pairs<true>": 31 * pls,
-        "k_pixel_quintuplets<false>": 64 * pls + 70 * t5,
-        "k_pixel_triplets<false>": 64 * pls + 53 * t3,
+        "k_pix_stage_a<PT5Kind>": 64 * pls + 128 * t5 + 40 * counts.get("pt5surv", 0),
+        "k_pix_stage_a<PT3Kind>": 64 * pls + 64 * t3 + 40 * counts.get("pt3surv", 0),
         "k_t5_tc_pairs": 55 * t5,
         "k_pt5_pairs": 61 * pt5,
         "k_pt3_pairs": 46 * pt3,
@@ -192,7 +194,11 @@ def main():
                        "pt3": int(lst.lib().lst_event_count(ev.h, b"pixelTriplets")),
                        "tc": int(lst.lib().lst_event_count(ev.h, b"trackCandidates")),
                        "t3cand": int(lst.lib().lst_event_count(ev.h, b"tripletCandidates")),
-                       "t3surv": int(lst.lib().lst_event_count(ev.h, b"tripletSharedMDCandidates"))})
+                       "t3surv": int(lst.lib().lst_event_count(ev.h, b"tripletSharedMDCandidates")),
+                       "mdcand": int(lst.lib().lst_event_count(ev.h, b"miniDoubletCandidates")),
+                       "lscand": int(lst.lib().lst_event_count(ev.h, b"segmentCandidates")),
+                       "pt5surv": int(lst.lib().lst_event_count(ev.h, b"pixelQuintupletSurvivors")),
+                       "pt3surv": int(lst.lib().lst_event_count(ev.h, b"pixelTripletSurvivors"))})
     lst._check(L.lst_event_set_timing(ev.h, 0))
     per_event_out = []
     for i in range(len(events)):

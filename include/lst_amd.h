@@ -119,7 +119,10 @@ lst_status lst_event_synchronize(lst_event* ev);
  * "pT3TrackCandidates", "T5TrackCandidates", "pLSTrackCandidates".  Diagnostics
  * with no counterpart in the reference: "tripletCandidates" (segment pairs the
  * triplet builder enumerates), "tripletSharedMDCandidates" (those sharing the
- * middle mini-doublet), "quintupletCandidates" (triplet pairs enumerated). */
+ * middle mini-doublet), "quintupletCandidates" (triplet pairs enumerated),
+ * "miniDoubletCandidates" (hit pairs enumerated), "segmentCandidates"
+ * (mini-doublet pairs enumerated), "pixelQuintupletSurvivors" and
+ * "pixelTripletSurvivors" (pLS-object pairs passing the cheap cuts). */
 uint32_t lst_event_count(lst_event* ev, const char* what);
 
 /* Replaces the host-copy getters SDL::Event::get{Hits,MiniDoublets,Segments,

@@ -155,15 +155,23 @@ __device__ __forceinline__ bool pixel_tracklet(const ModulesView& M, const MDsSo
   return dBeta * dBeta <= cut2;
 }
 
+// radius-compatibility class of a T3 (SDL/PixelTriplet.h:615-734): 3 when its
+// lower module is in the endcap, else 2 when its middle one is, else 1 when
+// its upper one is, else 0
+__device__ __forceinline__ int pt3_radius_kind(const ModulesView& M, unsigned lo, unsigned mid, unsigned up) {
+  if (M.subdets[lo] == kEndcap) return 3;
+  if (M.subdets[mid] == kEndcap) return 2;
+  if (M.subdets[up] == kEndcap) return 1;
+  return 0;
+}
+
 // SDL/PixelTriplet.h:615-734
-__device__ __forceinline__ bool pt3_radius_ok(const ModulesView& M, float pixR, float pixRErr, float t3R, unsigned lo,
-                                              unsigned mid, unsigned up) {
+__device__ __forceinline__ bool pt3_radius_ok(int kind, float pixR, float pixRErr, float t3R) {
   float tb, pb, tbHi, pbHi;
-  int kind;
-  if (M.subdets[lo] == kEndcap) { kind = 3; tb = 1.7006f; pb = 0.26367f; pbHi = 2.286f; tbHi = 2.436f; }
-  else if (M.subdets[mid] == kEndcap) { kind = 2; tb = 1.59294f; pb = 0.255181f; pbHi = 2.2091f; tbHi = 2.3548f; }
-  else if (M.subdets[up] == kEndcap) { kind = 1; tb = 0.45972f; pb = 0.19644f; pbHi = 0.6805f; tbHi = 0.8557f; }
-  else { kind = 0; tb = 0.15624f; pb = 0.17235f; pbHi = 0.6375f; tbHi = 0.6588f; }
+  if (kind == 3) { tb = 1.7006f; pb = 0.26367f; pbHi = 2.286f; tbHi = 2.436f; }
+  else if (kind == 2) { tb = 1.59294f; pb = 0.255181f; pbHi = 2.2091f; tbHi = 2.3548f; }
+  else if (kind == 1) { tb = 0.45972f; pb = 0.19644f; pbHi = 0.6805f; tbHi = 0.8557f; }
+  else { tb = 0.15624f; pb = 0.17235f; pbHi = 0.6375f; tbHi = 0.6588f; }
   if (pixR > 2.0f * kR1GeVf) { pb = pbHi; tb = tbHi; }
   const float tMax = (1 + tb) / t3R;
   const float tMin = m_max((1 - tb) / t3R, 0.0f);
@@ -285,14 +293,23 @@ __device__ __forceinline__ void pixel_sigmas(const ModulesView& M, const uint16_
   }
 }
 
-__device__ __forceinline__ float points_chi2(int n, const float* x, const float* y, const float* d1, const float* d2,
-                                             const float* sl, const bool* flat, float g, float f, float r) {
+// r-phi uncertainty of each point (SDL/PixelTriplet.h:419-487, 2218-2387)
+__device__ __forceinline__ void point_sigmas(const ModulesView& M, const uint16_t* mods, int n, const float* x,
+                                             const float* y, float* sig) {
+  float d1[5], d2[5], sl[5];
+  bool flat[5];
+  pixel_sigmas(M, mods, n, d1, d2, sl, flat);
+  for (int i = 0; i < n; i++) sig[i] = point_sigma(x[i], y[i], d1[i], d2[i], sl[i], flat[i]);
+}
+
+// chi2 of points against the circle (g, f, r), sigma per point given
+__device__ __forceinline__ float points_chi2(int n, const float* x, const float* y, const float* sig, float g,
+                                             float f, float r) {
   const float c = g * g + f * f - r * r;
   float chi2 = 0.f;
   for (int i = 0; i < n; i++) {
-    const float s = point_sigma(x[i], y[i], d1[i], d2[i], sl[i], flat[i]);
     const float t = x[i] * x[i] + y[i] * y[i] - 2 * g * x[i] - 2 * f * y[i] + c;
-    chi2 += t * t / (s * s);
+    chi2 += t * t / (sig[i] * sig[i]);
   }
   return chi2;
 }
@@ -301,10 +318,56 @@ struct PT3Result {
   float pixR, t3R, rz, rphi, rphiIn;
 };
 
-// runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024); `pls` is the global
-// segment index of the pLS, `arr` its pLS array index
-__device__ __forceinline__ bool pt3_select(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3,
-                                           bool chi2Cuts, PT3Result& r) {
+// Per-T3 quantities of the pixel-triplet selection that do not depend on the
+// pLS, gathered once per T3 of a pixel-connected module (kT3RecWords words):
+//   [0,3) x and [3,6) y of the anchor points of its three mini-doublets,
+//   [6,9) their r-phi sigmas, [9] circle radius, [10] circle centre x,
+//   [11] centre y, [12] packed: bits 0-1 radius class, bits 2-16 the pixel-
+//   object layer numbers of its three modules (5 bits each), bit 17 middle
+//   module is 2S
+struct T3Rec {
+  float x[3], y[3], sig[3], r, g, f;
+  uint32_t packed;
+};
+__device__ __forceinline__ int rec_layer(uint32_t packed, int shift, int k) { return (packed >> (shift + 5 * k)) & 31; }
+
+__device__ __forceinline__ void make_t3_rec(const EventView& v, uint32_t t3, float* out) {
+  const ModulesView& M = v.M;
+  const uint16_t mods[3] = {v.T.lowerModuleIndices[3 * t3], v.T.lowerModuleIndices[3 * t3 + 1],
+                            v.T.lowerModuleIndices[3 * t3 + 2]};
+  const uint32_t segIn = v.T.segmentIndices[2 * t3], segOut = v.T.segmentIndices[2 * t3 + 1];
+  const uint32_t md[3] = {v.S.mdIndices[2 * segIn], v.S.mdIndices[2 * segIn + 1], v.S.mdIndices[2 * segOut + 1]};
+  float x[3], y[3], sig[3];
+  for (int k = 0; k < 3; k++) {
+    x[k] = v.D.anchorX[md[k]];
+    y[k] = v.D.anchorY[md[k]];
+  }
+  point_sigmas(M, mods, 3, x, y, sig);
+  uint32_t packed = (uint32_t)pt3_radius_kind(M, mods[0], mods[1], mods[2]);
+  for (int k = 0; k < 3; k++) packed |= (uint32_t)pix_layer(M, mods[k]) << (2 + 5 * k);
+  if (M.moduleType[mods[1]] == kTwoS) packed |= 1u << 17;
+  float4* o = reinterpret_cast<float4*>(out);
+  o[0] = make_float4(x[0], x[1], x[2], y[0]);
+  o[1] = make_float4(y[1], y[2], sig[0], sig[1]);
+  o[2] = make_float4(sig[2], v.T.circleRadius[t3], v.T.circleCenterX[t3], v.T.circleCenterY[t3]);
+  o[3] = make_float4(__uint_as_float(packed), 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ T3Rec load_t3_rec(const float* rec) {
+  const float4 a = *(const float4*)rec, b = *(const float4*)(rec + 4), c = *(const float4*)(rec + 8),
+               d = *(const float4*)(rec + 12);
+  T3Rec r;
+  r.x[0] = a.x; r.x[1] = a.y; r.x[2] = a.z;
+  r.y[0] = a.w; r.y[1] = b.x; r.y[2] = b.y;
+  r.sig[0] = b.z; r.sig[1] = b.w; r.sig[2] = c.x;
+  r.r = c.y; r.g = c.z; r.f = c.w;
+  r.packed = __float_as_uint(d.x);
+  return r;
+}
+
+// the two pixel-tracklet tests of runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:
+// 826-1024): the pLS against the inner and against the outer segment of T3 t3
+__device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3) {
   const ModulesView& M = v.M;
   const uint16_t lo = v.T.lowerModuleIndices[3 * t3], mid = v.T.lowerModuleIndices[3 * t3 + 1],
                  up = v.T.lowerModuleIndices[3 * t3 + 2];
@@ -316,38 +379,31 @@ __device__ __forceinline__ bool pt3_select(const EventView& v, uint32_t pls, uin
   const uint32_t md3 = v.S.mdIndices[2 * segOut], md4 = v.S.mdIndices[2 * segOut + 1];
   if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md1, md2, lo, mid, alphaPix, v.S.dPhiChanges[segIn], v.ptCut))
     return false;
-  if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut))
-    return false;
-  r.pixR = s.ptIn * kR1GeVf;
-  const float pixRErr = s.ptErr * kR1GeVf;
-  const uint32_t mdC = v.S.mdIndices[2 * segOut + 1];
-  float xs[3] = {v.D.anchorX[md1], v.D.anchorX[md2], v.D.anchorX[mdC]};
-  float ys[3] = {v.D.anchorY[md1], v.D.anchorY[md2], v.D.anchorY[mdC]};
-  r.t3R = v.T.circleRadius[t3];
-  const float g = v.T.circleCenterX[t3], f = v.T.circleCenterY[t3];
-  if (!pt3_radius_ok(M, r.pixR, pixRErr, r.t3R, lo, mid, up)) return false;
-  const uint16_t mods[3] = {lo, mid, up};
-  const int l1 = pix_layer(M, lo), l2 = pix_layer(M, mid), l3 = pix_layer(M, up);
-  const bool cuts = chi2Cuts && s.ptIn < 5.0f;
-  if (cuts) {
-    const float rts[3] = {v.D.anchorRt[md1], v.D.anchorRt[md2], v.D.anchorRt[mdC]};
-    const float zs[3] = {v.D.anchorZ[md1], v.D.anchorZ[md2], v.D.anchorZ[mdC]};
-    r.rz = pt3_rz_chi2(M, mods, v.D.anchorX[pixOut], v.D.anchorY[pixOut], v.D.anchorZ[pixOut], v.D.anchorRt[pixOut],
-                       rts, zs, s.px, s.py, s.pz, s.charge);
-    if (!pt3_rz_ok(l1, l2, l3, r.rz)) return false;
-  } else {
-    r.rz = -1;
-  }
-  {
-    float d1[3], d2[3], sl[3];
-    bool flat[3];
-    pixel_sigmas(M, mods, 3, d1, d2, sl, flat);
-    r.rphi = points_chi2(3, xs, ys, d1, d2, sl, flat, s.g, s.f, s.radius);
-  }
+  return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut);
+}
+
+// runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024) is the AND of pure
+// cuts, evaluated here in two parts: pt3_cheap (radius compatibility, the
+// r-phi and inward r-phi chi2 cuts) and pt3_costly (the two pixel-tracklet
+// tests and the r-z chi2 cut), which need double-precision transcendentals.
+// Either order gives the same pass/fail and the same values.  `pls` is the
+// global segment index of the pLS, `arr` its pLS array index.
+__device__ __forceinline__ bool pt3_cheap(const EventView& v, uint32_t pls, uint32_t arr, const T3Rec& t,
+                                          bool chi2Cuts, PT3Result& r) {
+  const float ptIn = v.P.ptIn[arr];
+  r.pixR = ptIn * kR1GeVf;
+  const float pixRErr = v.P.ptErr[arr] * kR1GeVf;
+  r.t3R = t.r;
+  r.rz = -1;
+  if (!pt3_radius_ok(t.packed & 3, r.pixR, pixRErr, r.t3R)) return false;
+  const int l1 = rec_layer(t.packed, 2, 0), l2 = rec_layer(t.packed, 2, 1), l3 = rec_layer(t.packed, 2, 2);
+  const bool cuts = chi2Cuts && ptIn < 5.0f;
+  r.rphi = points_chi2(3, t.x, t.y, t.sig, v.P.circleCenterX[arr], v.P.circleCenterY[arr], v.P.circleRadius[arr]);
   if (cuts && !pt3_rphi_ok(l1, l2, l3, r.rphi)) return false;
   {
+    const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
     const float xa = v.D.anchorX[pixIn], ya = v.D.anchorY[pixIn], xb = v.D.anchorX[pixOut], yb = v.D.anchorY[pixOut];
-    const float rr = r.t3R;
+    const float g = t.g, f = t.f, rr = r.t3R;
     float res = (xa - g) * (xa - g) + (ya - f) * (ya - f) - rr * rr;
     float c2 = res * res;
     res = (xb - g) * (xb - g) + (yb - f) * (yb - f) - rr * rr;
@@ -355,8 +411,26 @@ __device__ __forceinline__ bool pt3_select(const EventView& v, uint32_t pls, uin
     c2 *= 0.5f;
     r.rphiIn = c2;
   }
-  if (cuts && !pt3_rphi_in_ok(l1, l2, l3, r.rphiIn)) return false;
-  return true;
+  return !(cuts && !pt3_rphi_in_ok(l1, l2, l3, r.rphiIn));
+}
+
+__device__ __forceinline__ bool pt3_costly(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3, bool chi2Cuts,
+                                           PT3Result& r) {
+  if (!pt3_tracklets(v, pls, arr, t3)) return false;
+  const PixelSeed s = load_seed(v.P, arr);
+  if (!(chi2Cuts && s.ptIn < 5.0f)) return true;
+  const ModulesView& M = v.M;
+  const uint16_t mods[3] = {v.T.lowerModuleIndices[3 * t3], v.T.lowerModuleIndices[3 * t3 + 1],
+                            v.T.lowerModuleIndices[3 * t3 + 2]};
+  const uint32_t segIn = v.T.segmentIndices[2 * t3], segOut = v.T.segmentIndices[2 * t3 + 1];
+  const uint32_t md1 = v.S.mdIndices[2 * segIn], md2 = v.S.mdIndices[2 * segIn + 1],
+                 mdC = v.S.mdIndices[2 * segOut + 1];
+  const uint32_t pixOut = v.S.mdIndices[2 * pls + 1];
+  const float rts[3] = {v.D.anchorRt[md1], v.D.anchorRt[md2], v.D.anchorRt[mdC]};
+  const float zs[3] = {v.D.anchorZ[md1], v.D.anchorZ[md2], v.D.anchorZ[mdC]};
+  r.rz = pt3_rz_chi2(M, mods, v.D.anchorX[pixOut], v.D.anchorY[pixOut], v.D.anchorZ[pixOut], v.D.anchorRt[pixOut], rts,
+                     zs, s.px, s.py, s.pz, s.charge);
+  return pt3_rz_ok(pix_layer(M, mods[0]), pix_layer(M, mods[1]), pix_layer(M, mods[2]), r.rz);
 }
 
 __device__ __forceinline__ bool pt5_rz_ok(int l1, int l2, int l3, int l4, int l5, float c) {  // :2038-2126
@@ -465,26 +539,70 @@ struct PT5Result {
   float rz, rphi, rphiIn, pixR, t5R, cx, cy;
 };
 
-// runPixelQuintupletDefaultAlgo (SDL/PixelTriplet.h:2480-2634)
-__device__ __forceinline__ bool pt5_select(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t5, PT5Result& r) {
+// Per-T5 quantities of the pixel-quintuplet selection that do not depend on
+// the pLS, gathered once per T5 of a pixel-connected module (kT5RecWords words):
+//   [0,5) x, [5,10) y, [10,15) z, [15,20) rt of the anchor points of its five
+//   mini-doublets, [20,25) their r-phi sigmas, [25,30) their r-z errors,
+//   [30] circle radius of the inner T3, [31] packed: bits 0-4 barrel flag of
+//   each module, bits 5-6 radius class of the inner T3, bits 7-31 the pixel-
+//   object layer numbers of the five modules (5 bits each)
+
+__device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, float* out) {
   const ModulesView& M = v.M;
   const uint32_t innerT3 = v.Q.tripletIndices[2 * t5], outerT3 = v.Q.tripletIndices[2 * t5 + 1];
-  {
-    PT3Result tmp;
-    if (!pt3_select(v, pls, arr, innerT3, false, tmp)) return false;
-  }
   const uint32_t s1 = v.T.segmentIndices[2 * innerT3], s2 = v.T.segmentIndices[2 * innerT3 + 1];
   const uint32_t s3 = v.T.segmentIndices[2 * outerT3], s4 = v.T.segmentIndices[2 * outerT3 + 1];
-  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
   const uint32_t md[5] = {v.S.mdIndices[2 * s1], v.S.mdIndices[2 * s2], v.S.mdIndices[2 * s2 + 1],
                           v.S.mdIndices[2 * s3 + 1], v.S.mdIndices[2 * s4 + 1]};
   uint16_t mods[5];
-  int l[5];
+  float w[kT5RecWords];
+  uint32_t packed = 0;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     mods[k] = v.Q.lowerModuleIndices[5 * t5 + k];
-    l[k] = pix_layer(M, mods[k]);
+    const unsigned m = mods[k];
+    w[k] = v.D.anchorX[md[k]];
+    w[5 + k] = v.D.anchorY[md[k]];
+    w[10 + k] = v.D.anchorZ[md[k]];
+    w[15 + k] = v.D.anchorRt[md[k]];
+    // r-z error of the point (computePT5RZChiSquared, SDL/PixelTriplet.h:2636-2677)
+    const int type = M.moduleType[m], side = M.sides[m], sub = M.subdets[m];
+    const float drdz = M.drdzs[m];
+    float error = type == 0 ? 0.15f : 5.0f;
+    if (type == 0 && sub == kBarrel && side != kCenter) error /= m_sqrt(1.f + drdz * drdz);
+    w[25 + k] = error;
+    if (sub == kBarrel) packed |= 1u << k;
+    packed |= (uint32_t)pix_layer(M, m) << (7 + 5 * k);
   }
+  packed |= (uint32_t)pt3_radius_kind(M, mods[0], mods[1], mods[2]) << 5;
+  point_sigmas(M, mods, 5, w, w + 5, w + 20);
+  w[30] = v.T.circleRadius[innerT3];
+  w[31] = __uint_as_float(packed);
+  float4* o = reinterpret_cast<float4*>(out);
+#pragma unroll
+  for (int k = 0; k < kT5RecWords / 4; k++) o[k] = make_float4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
+// runPixelQuintupletDefaultAlgo (SDL/PixelTriplet.h:2480-2634), the AND of pure
+// cuts, in two parts as for the pixel triplets: pt5_cheap (the inner T3's
+// radius compatibility, the r-z, r-phi and inward r-phi chi2 cuts, which set
+// every value of the result) and pt5_costly (the pixel-tracklet tests of the
+// inner T3).  `rec` is the T5's kT5RecWords record.
+__device__ __forceinline__ bool pt5_cheap(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t5, const float* rec,
+                                          PT5Result& r) {
+  float w[kT5RecWords];
+#pragma unroll
+  for (int k = 0; k < kT5RecWords; k += 4) {
+    const float4 q = *(const float4*)(rec + k);
+    w[k] = q.x; w[k + 1] = q.y; w[k + 2] = q.z; w[k + 3] = q.w;
+  }
+  const uint32_t packed = __float_as_uint(w[31]);
+  // inner T3 (runPixelTripletDefaultAlgo without chi2 cuts): radius compatibility
+  if (!pt3_radius_ok((packed >> 5) & 3, v.P.ptIn[arr] * kR1GeVf, v.P.ptErr[arr] * kR1GeVf, w[30])) return false;
+  int l[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) l[k] = rec_layer(packed, 7, k);
+  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
   const float zP0 = v.D.anchorZ[pixIn], zP1 = v.D.anchorZ[pixOut];
   const float rP0 = v.D.anchorRt[pixIn], rP1 = v.D.anchorRt[pixOut];
   r.rz = 0;
@@ -494,33 +612,18 @@ __device__ __forceinline__ bool pt5_select(const EventView& v, uint32_t pls, uin
   if (r.pixR < 5.0f * kR1GeVf) {  // computePT5RZChiSquared (:2636-2677)
     const float slope = (zP1 - zP0) / (rP1 - rP0);
     float rmse = 0;
+#pragma unroll
     for (int i = 0; i < 5; i++) {
-      const unsigned m = mods[i];
-      const int type = M.moduleType[m], side = M.sides[m], sub = M.subdets[m];
-      const float zs = v.D.anchorZ[md[i]], rts = v.D.anchorRt[md[i]];
-      const float residual = sub == kBarrel ? (zs - zP0) - slope * (rts - rP0) : (rts - rP0) - (zs - zP0) / slope;
-      const float drdz = M.drdzs[m];
-      float error = type == 0 ? 0.15f : 5.0f;
-      if (type == 0 && sub == kBarrel && side != kCenter) error /= m_sqrt(1.f + drdz * drdz);
+      const float zs = w[10 + i], rts = w[15 + i], error = w[25 + i];
+      const float residual = ((packed >> i) & 1) ? (zs - zP0) - slope * (rts - rP0) : (rts - rP0) - (zs - zP0) / slope;
       rmse += (residual * residual) / (error * error);
     }
     r.rz = m_sqrt(0.2f * rmse);
     if (!pt5_rz_ok(l[0], l[1], l[2], l[3], l[4], r.rz)) return false;
   }
-  float xs[5], ys[5];
-#pragma unroll
-  for (int k = 0; k < 5; k++) {
-    xs[k] = v.D.anchorX[md[k]];
-    ys[k] = v.D.anchorY[md[k]];
-  }
   const float t5g = v.Q.regressionG[t5], t5f = v.Q.regressionF[t5];
   r.t5R = v.Q.regressionRadius[t5];
-  {
-    float d1[5], d2[5], sl[5];
-    bool flat[5];
-    pixel_sigmas(M, mods, 5, d1, d2, sl, flat);
-    r.rphi = points_chi2(5, xs, ys, d1, d2, sl, flat, r.cx, r.cy, r.pixR);
-  }
+  r.rphi = points_chi2(5, w, w + 5, w + 20, r.cx, r.cy, r.pixR);
   if (r.pixR < 5.0f * kR1GeVf && !pt5_rphi_ok(l[0], l[1], l[2], l[3], l[4], r.rphi)) return false;
   {
     const float xa = v.D.anchorX[pixIn], ya = v.D.anchorY[pixIn], xb = v.D.anchorX[pixOut], yb = v.D.anchorY[pixOut];
@@ -537,6 +640,10 @@ __device__ __forceinline__ bool pt5_select(const EventView& v, uint32_t pls, uin
   r.cx = (r.cx + t5g) / 2;
   r.cy = (r.cy + t5f) / 2;
   return true;
+}
+
+__device__ __forceinline__ bool pt5_costly(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t5) {
+  return pt3_tracklets(v, pls, arr, v.Q.tripletIndices[2 * t5]);
 }
 
 }  // namespace lst

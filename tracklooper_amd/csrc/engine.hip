@@ -212,9 +212,13 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
   c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
   c.take(e.W.pt3Mask, lst::kPixMaskWords); c.take(e.W.pt3MaskBase, plsCap);
-  c.take(e.W.pt5ParkAt, lst::kPixMaskWords); c.take(e.W.pt5Park, (size_t)lst::kPT5ParkWords * lst::kPixParkMax);
-  c.take(e.W.pt3ParkAt, lst::kPixMaskWords); c.take(e.W.pt3Park, (size_t)lst::kPT3ParkWords * lst::kPixParkMax);
-  c.take(e.W.pt5HashHead, lst::kPT5HashBuckets); c.take(e.W.pt5HashNext, 14 * (size_t)pt5Cap);
+  c.take(e.W.pixWords, plsCap); c.take(e.W.pixFull, plsCap);
+  c.take(e.W.pixSurv, (size_t)lst::kSurvWords * lst::kPixSurvMax);
+  c.take(e.W.pixParkIdx, (size_t)64 * lst::kPixMaskWords);
+  c.take(e.W.t3ListOff, nL); c.take(e.W.t3List, t3Cap ? t3Cap : 1);
+  c.take(e.W.t5Rec, (size_t)lst::kT5RecWords * (t5Cap ? t5Cap : 1));
+  c.take(e.W.t3Rec, (size_t)lst::kT3RecWords * (t3Cap ? t3Cap : 1));
+  c.take(e.W.pt5BucketOff, lst::kPT5HashBuckets + 1); c.take(e.W.pt5BucketEnt, 14 * (size_t)pt5Cap);
   c.take(e.FS.pre, lst::kScanCandMax); c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
   c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax);
   c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDParkMax);
@@ -754,7 +758,9 @@ uint32_t lst_event_count(lst_event* ev, const char* what) {
         {"trackCandidates", lst::kNTrackCandidates}, {"pT3TrackCandidates", lst::kNTrackCandidatespT3},
         {"pT5TrackCandidates", lst::kNTrackCandidatespT5}, {"pLSTrackCandidates", lst::kNTrackCandidatespLS},
         {"T5TrackCandidates", lst::kNTrackCandidatesT5}, {"tripletCandidates", lst::kT3CandTotal},
-        {"tripletSharedMDCandidates", lst::kT3List}, {"quintupletCandidates", lst::kT5CandTotal}};
+        {"tripletSharedMDCandidates", lst::kT3List}, {"quintupletCandidates", lst::kT5CandTotal},
+        {"miniDoubletCandidates", lst::kMDCandTotal}, {"segmentCandidates", lst::kLSCandTotal},
+        {"pixelQuintupletSurvivors", lst::kPT5Survivors}, {"pixelTripletSurvivors", lst::kPT3Survivors}};
     for (const auto& e : sc)
       if (!std::strcmp(what, e.name)) {
         uint32_t val = 0;

@@ -167,21 +167,27 @@ struct PixelWork {
   uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
   uint32_t* t5BinOff;     // [kEtaBinOffsets]
   unsigned long long* pt5Mask;  // [kPixMaskWords] pass mask per 64-candidate chunk of each pLS (pT5)
-  uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none recorded
+  uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none
   unsigned long long* pt3Mask;  // [kPixMaskWords] (pT3)
   uint32_t* pt3MaskBase;        // [pLS capacity]
-  uint32_t* pt5ParkAt;          // [kPixMaskWords] per chunk: first parked result of its passing lanes, ~0: none
-  uint32_t* pt5Park;            // [kPT5ParkWords * kPixParkMax] t5 index + PT5Result of passing candidates
-  uint32_t* pt3ParkAt;          // [kPixMaskWords]
-  uint32_t* pt3Park;            // [kPT3ParkWords * kPixParkMax] t3 index + PT3Result
-  uint32_t* pt5HashHead;        // [kPT5HashBuckets] last (pT5, position) entry of each hit bucket, ~0: none
-  uint32_t* pt5HashNext;        // [14 * pT5 capacity] next entry of the same bucket
+  uint32_t* pixWords;           // [pLS capacity] mask words of the pLS (the builder running)
+  uint8_t* pixFull;             // [pLS capacity] 1: the pLS's survivors did not all fit, evaluated in full
+  uint32_t* pixSurv;            // [kSurvWords * kPixSurvMax] survivors of the cheap cuts: pLS, mask
+                                //   position, object, partial result
+  uint32_t* pixParkIdx;         // [64 * kPixMaskWords] per mask position: the survivor of a passing candidate
+  uint32_t* t3ListOff;          // [nLower] first list position of each module's T3s (pixel-triplet modules)
+  uint32_t* t3List;             // [T3 capacity] T3 slots of those modules
+  float* t5Rec;                 // [kT5RecWords * T5 capacity] pLS-independent quantities per T5
+  float* t3Rec;                 // [kT3RecWords * T3 capacity] the same per T3
+  uint32_t* pt5BucketOff;       // [kPT5HashBuckets + 1] entry counts of the hit buckets, then first positions
+  uint32_t* pt5BucketEnt;       // [14 * pT5 capacity] (pT5, position) entries 14 * pT5 + position, by bucket
 };
 constexpr uint32_t kPT5HashBuckets = 1u << 15;
-constexpr uint32_t kPixMaskWords = 1u << 20;  // 64 Mi candidates per event; beyond: evaluated twice
-constexpr uint32_t kPixParkMax = 1u << 18;    // passing candidates parked per event; beyond: evaluated twice
-constexpr uint32_t kPT5ParkWords = 8;
-constexpr uint32_t kPT3ParkWords = 6;
+constexpr uint32_t kPixMaskWords = 1u << 16;  // 4 Mi candidates per builder and event; beyond: evaluated in full
+constexpr uint32_t kPixSurvMax = 1u << 18;    // survivors of the cheap cuts per builder; beyond: evaluated in full
+constexpr uint32_t kSurvWords = 10;           // pLS, mask position, object, up to 7 result words
+constexpr int kT5RecWords = 32;               // per-T5 record of the pixel-quintuplet selection
+constexpr int kT3RecWords = 16;               // per-T3 record of the pixel-triplet selection
 constexpr int kEtaBins = 64;                    // finite eta bins of width 1/8 over [-4, 4)
 constexpr int kNanEtaBin = kEtaBins;            // the bin of NaN eta
 constexpr int kEtaBinOffsets = kEtaBins + 2;    // starts of bins 0..kNanEtaBin, then the end
@@ -236,15 +242,17 @@ enum Scalar : int {
   kT5Listed = 20,
   kPT5MaskWords = 21,
   kPT3MaskWords = 22,
-  kPT5Parked = 23,
-  kPT3Parked = 24,
+  kPT5Survivors = 23,
+  kPT3Survivors = 24,
   kMDCandTotal = 25,
   kMDParked = 26,
   kLSCandTotal = 27,
   kLSParked = 28,
   kEvNHits = 29,  // hits of the event (set before the hit kernels run)
   kEvNPLS = 30,   // pixel line segments of the event
-  kNScalars = 32
+  kPixFull = 31,  // bit 0: some pLS evaluated in full by the pT5 builder, bit 1: by the pT3 builder
+  kT3Listed = 32,  // T3 slots listed for the pixel-triplet records
+  kNScalars = 40
 };
 }  // namespace lst
 namespace lst {

@@ -58,13 +58,13 @@ struct EventView {  // everything a stage kernel needs, by value
 enum TimerId : int {
   kTmHits = 0,          // k_hits
   kTmPixelSegments,     // k_pixel_segments
-  kTmMiniDoublets,      // k_mini_doublets
-  kTmSegments,          // k_segments
+  kTmMiniDoublets,      // k_md_eval
+  kTmSegments,          // k_ls_eval
   kTmT3Select,          // k_t3_select
   kTmT5Final,           // k_t5_final
   kTmPlsDup,            // k_pls_pairs<false> (pLS cleaning, first pass)
-  kTmPT5Build,          // k_pixel_quintuplets<false>
-  kTmPT3Build,          // k_pixel_triplets<false>
+  kTmPT5Build,          // k_pix_stage_a<PT5Kind> (pixel quintuplets, cheap cuts)
+  kTmPT3Build,          // k_pix_stage_a<PT3Kind> (pixel triplets, cheap cuts)
   kTmT5DupTC,           // k_t5_tc_pairs
   kTmPT5Dup,            // k_pt5_pairs
   kTmPT3Dup,            // k_pt3_pairs
@@ -138,6 +138,7 @@ void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const Kernel
 void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_cross_pt3(const EventView& v, hipStream_t s);
+void launch_t5_list(const EventView& v, hipStream_t s);
 void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_cross_t5(const EventView& v, hipStream_t s);
 void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t);

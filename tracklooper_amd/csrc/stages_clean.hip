@@ -18,6 +18,7 @@ namespace lst {
 namespace {
 
 constexpr int kTile = 256;
+constexpr int kWave = 64;
 
 // OR `bits` into byte k of an int8/uint8 flag array (256-byte aligned base)
 __device__ __forceinline__ void or_byte(void* base, uint32_t k, uint32_t bits) {
@@ -222,53 +223,115 @@ __global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
 // >= 7 of its 14 hits and scores lower (ties: the higher index is dropped).
 // At least 7 of ix's 14 positions must hold a hit of the partner, so one of its
 // first 8 positions does: every partner that can flag ix shares a hit with
-// those 8 positions.  A hash index from hit value to (pT5, position) entries
-// gives those partners; each is then checked with the exact rule.
+// those 8 positions.  The (pT5, position) entries are sorted by a hash of the
+// hit into contiguous buckets (count, scan, scatter); one wave per pT5 then
+// checks the entries of its 8 buckets in parallel with the exact rule.
 constexpr uint32_t kPT5HashMask = kPT5HashBuckets - 1;
 __device__ __forceinline__ uint32_t hit_bucket(uint32_t h) { return (h * 0x9E3779B1u) >> 17 & kPT5HashMask; }
 
-__global__ __launch_bounds__(256) void k_pt5_hash_reset(EventView v) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < kPT5HashBuckets; b += gridDim.x * blockDim.x)
-    v.W.pt5HashHead[b] = ~0u;
-}
-
-__global__ __launch_bounds__(256) void k_pt5_hash_link(EventView v) {
+__global__ __launch_bounds__(256) void k_pt5_bucket_count(EventView v) {
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 14 * n; e += gridDim.x * blockDim.x)
-    v.W.pt5HashNext[e] = atomicExch(&v.W.pt5HashHead[hit_bucket(v.P5.hitIndices[e])], e);
+    atomicAdd(&v.W.pt5BucketOff[hit_bucket(v.P5.hitIndices[e])], 1u);
+}
+
+// inclusive scan of the bucket counts in place (one 1024-lane work-group,
+// kPT5HashBuckets / 1024 consecutive buckets per lane); entry kPT5HashBuckets
+// gets the total
+__global__ __launch_bounds__(1024) void k_pt5_bucket_scan(EventView v) {
+  __shared__ unsigned lds[17];
+  constexpr uint32_t kPer = kPT5HashBuckets / 1024;
+  static_assert(kPer % 4 == 0, "buckets per lane");
+  uint4* c = reinterpret_cast<uint4*>(v.W.pt5BucketOff + kPer * threadIdx.x);
+  uint32_t x[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer / 4; k++) {
+    const uint4 q = c[k];
+    x[4 * k] = q.x; x[4 * k + 1] = q.y; x[4 * k + 2] = q.z; x[4 * k + 3] = q.w;
+  }
+  unsigned sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) sum += x[k];
+  unsigned tot;
+  unsigned run = block_exclusive_scan<1024>(sum, lds, tot);
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    run += x[k];
+    x[k] = run;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kPer / 4; k++) c[k] = make_uint4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+  if (threadIdx.x == 0) v.W.pt5BucketOff[kPT5HashBuckets] = tot;
+}
+
+// entry e to its bucket; afterwards pt5BucketOff[b] is the first position of
+// bucket b (the inclusive ends are counted down)
+__global__ __launch_bounds__(256) void k_pt5_bucket_fill(EventView v) {
+  const uint32_t n = v.R.scalars[kNPixelQuintuplets];
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 14 * n; e += gridDim.x * blockDim.x)
+    v.W.pt5BucketEnt[atomicSub(&v.W.pt5BucketOff[hit_bucket(v.P5.hitIndices[e])], 1u) - 1] = e;
 }
 
 __global__ __launch_bounds__(256) void k_pt5_pairs(EventView v) {
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
-  for (uint32_t ix = blockIdx.x * blockDim.x + threadIdx.x; ix < n; ix += gridDim.x * blockDim.x) {
+  const unsigned lane = threadIdx.x % kWave;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t ix = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; ix < n; ix += waves) {
+    const uint32_t mine = lane < 14 ? v.P5.hitIndices[14 * ix + lane] : 0u;
     uint32_t h[14];
 #pragma unroll
-    for (int a = 0; a < 14; a++) h[a] = v.P5.hitIndices[14 * ix + a];
+    for (int a = 0; a < 14; a++) h[a] = __shfl(mine, a, kWave);
     const float s1 = v.P5.score[ix];
-    bool dup = false;
-    for (int a0 = 0; a0 < 8 && !dup; a0++) {
-      const uint32_t key = h[a0];
-      for (uint32_t e = v.W.pt5HashHead[hit_bucket(key)]; e != ~0u && !dup; e = v.W.pt5HashNext[e]) {
-        if (v.P5.hitIndices[e] != key) continue;
-        const uint32_t jx = e / 14;
-        if (jx == ix) continue;
-        const float s2 = v.P5.score[jx];
-        if (!(s1 > s2 || (s1 == s2 && ix > jx))) continue;
-        uint32_t g[14];
-#pragma unroll
-        for (int b = 0; b < 14; b++) g[b] = v.P5.hitIndices[14 * jx + b];
-        int matched = 0;
-#pragma unroll
-        for (int a = 0; a < 14; a++) {
-          bool hit = false;
-#pragma unroll
-          for (int b = 0; b < 14; b++) hit = hit || h[a] == g[b];
-          matched += hit ? 1 : 0;
-        }
-        dup = matched >= 7;
-      }
+    // lanes 0-7: the bucket of hit position `lane`; candidates numbered over the 8 buckets
+    uint32_t bStart = 0, bSize = 0;
+    if (lane < 8) {
+      const uint32_t b = hit_bucket(mine);
+      bStart = v.W.pt5BucketOff[b];
+      bSize = v.W.pt5BucketOff[b + 1] - bStart;
     }
-    if (dup) v.P5.isDup[ix] = 1;
+    const uint32_t incl = wave_inclusive_scan(bSize);
+    uint32_t ends[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) ends[j] = __shfl(incl, j, kWave);
+    const uint32_t total = ends[7];
+    bool dup = false;
+    for (uint32_t c0 = 0; c0 < total && !dup; c0 += kWave) {
+      const uint32_t c = c0 + lane;
+      int k = 0;           // bucket of candidate c: the first with ends[k] > c
+      uint32_t before = 0;  // candidates in the buckets before it
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        k += ends[j] <= c ? 1 : 0;
+        before = ends[j] <= c ? ends[j] : before;
+      }
+      // every lane takes part in the shuffles (k < 8 for every lane)
+      const uint32_t start = __shfl(bStart, k, kWave);
+      const uint32_t key = __shfl(mine, k, kWave);
+      bool d = false;
+      if (c < total) {
+        const uint32_t e = v.W.pt5BucketEnt[start + (c - before)];
+        const uint32_t jx = e / 14;
+        if (v.P5.hitIndices[e] == key && jx != ix) {
+          const float s2 = v.P5.score[jx];
+          if (s1 > s2 || (s1 == s2 && ix > jx)) {
+            uint32_t g[14];
+#pragma unroll
+            for (int b = 0; b < 14; b++) g[b] = v.P5.hitIndices[14 * jx + b];
+            int matched = 0;
+#pragma unroll
+            for (int a = 0; a < 14; a++) {
+              bool hit = false;
+#pragma unroll
+              for (int b = 0; b < 14; b++) hit = hit || h[a] == g[b];
+              matched += hit ? 1 : 0;
+            }
+            d = matched >= 7;
+          }
+        }
+      }
+      dup = __ballot(d) != 0ull;
+    }
+    if (dup && lane == 0) v.P5.isDup[ix] = 1;
   }
 }
 
@@ -618,11 +681,13 @@ void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const Kernel
   t.end(s, tm);
 }
 
+// the bucket counts are cleared by k_pls_connected, earlier in the chain
 void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_pt5_hash_reset, dim3(kPT5HashBuckets / 256), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_pt5_hash_link, dim3(14 * kNMaxPixelQuintuplets / 256 + 1), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_bucket_count, dim3(256), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_bucket_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_bucket_fill, dim3(256), dim3(256), 0, s, v);
   t.begin(s, kTmPT5Dup);
-  hipLaunchKernelGGL(k_pt5_pairs, dim3(kNMaxPixelQuintuplets / 64 + 1), dim3(64), 0, s, v);
+  hipLaunchKernelGGL(k_pt5_pairs, dim3(1024), dim3(256), 0, s, v);
   t.end(s, kTmPT5Dup);
 }
 
@@ -636,9 +701,14 @@ void launch_cross_pt3(const EventView& v, hipStream_t s) {
   hipLaunchKernelGGL(k_cross_pt3_pairs, pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelQuintuplets), dim3(kTile), 0, s, v);
 }
 
-void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
+// the T5 slots of the eligible modules, module by module; built once the T5s
+// are final (the pixel-quintuplet stage) and read by the later stages
+void launch_t5_list(const EventView& v, hipStream_t s) {
   hipLaunchKernelGGL(k_t5_list_offsets, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t5_list_fill, dim3(256), dim3(256), 0, s, v);
+}
+
+void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT5DupTC);
   hipLaunchKernelGGL(k_t5_tc_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);

@@ -20,7 +20,8 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kMaxConn = 64;  // >= the largest connected-module list of a superbin (40)
-constexpr unsigned kPixelWaves = 4096;  // one-wave work-groups striding over the pLS
+constexpr unsigned kPixelWaves = 4096;     // one-wave work-groups striding over the pLS
+constexpr unsigned kPixelFullWaves = 256;  // the same for the overflow kernels, which normally return at once
 constexpr unsigned kTCCap = kNMaxPixelTrackCandidates + kNMaxNonPixelTrackCandidates;
 
 inline unsigned grid_of(uint32_t n, unsigned block) {
@@ -52,9 +53,12 @@ __global__ void k_pls_connected(EventView v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     v.R.scalars[kPT5MaskWords] = 0;
     v.R.scalars[kPT3MaskWords] = 0;
-    v.R.scalars[kPT5Parked] = 0;
-    v.R.scalars[kPT3Parked] = 0;
+    v.R.scalars[kPT5Survivors] = 0;
+    v.R.scalars[kPT3Survivors] = 0;
+    v.R.scalars[kPixFull] = 0;
   }
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= kPT5HashBuckets; b += gridDim.x * blockDim.x)
+    v.W.pt5BucketOff[b] = 0;  // hit-bucket counts of the pT5 duplicate search
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nPLS; i += gridDim.x * blockDim.x) {
     const int sb = v.P.superbin[i];
@@ -69,154 +73,410 @@ __global__ void k_pls_connected(EventView v) {
   }
 }
 
-// ---------------------------------------------------------- pixel quintuplets
-// one wave per pLS (striding over the pLS); candidates (connected module k, T5 b) in serial order
-template <bool kWrite>
-__global__ __launch_bounds__(kWave) void k_pixel_quintuplets(EventView v) {
+// ------------------------------------------------------ pixel-object builders
+// Pixel quintuplets (SDL/Event.cc:1009-1146) and pixel triplets
+// (SDL/Event.cc:784-901) loop over the pLS, their connected modules and the
+// objects of each module, and take slots in that serial order.  Candidate
+// (pLS i, position L) is the L-th object over pLS i's connected modules.
+//   records  per T5 / T3, the quantities of the selection that do not
+//            depend on the pLS (one lane per listed object)
+//   stage A  one wave per pLS: the cheap cuts; each survivor (pLS, mask
+//            position, object, partial result) is appended to a list; the
+//            pLS's pass-mask words are cleared
+//   stage B  one lane per survivor: the costly cuts; a pass sets its bit in
+//            the pLS's mask, records where its result is, and counts the pLS
+//   full     pLS whose survivors did not all fit (flagged by stage A) are
+//            counted by evaluating every candidate (no work otherwise)
+//   scan     per-pLS counts to slot offsets
+//   write    one wave per pLS: the passing candidates, in mask order, to
+//            their slots; the flagged pLS are evaluated again instead
+
+// the connected-module prefix of pLS i in LDS: pre[k] candidates before
+// module k, objBase[k] the module's first object; returns the total
+__device__ __forceinline__ uint32_t conn_total(uint32_t incl, uint32_t size, unsigned lane, uint32_t* pre) {
+  if (lane < size) pre[lane + 1] = incl;
+  if (lane == 0) pre[0] = 0;
+  __syncthreads();
+  return __shfl(incl, kWave - 1, kWave);
+}
+template <class K>
+__device__ __forceinline__ uint32_t conn_prefix(const EventView& v, uint32_t i, uint32_t* pre, uint32_t* objBase,
+                                                uint32_t& size) {
+  const unsigned lane = threadIdx.x;
+  size = v.W.connSize[i];
+  const uint32_t base = v.W.connBase[i];
+  unsigned cnt = 0;
+  uint32_t ob = 0;
+  if (lane < size) {
+    const uint32_t m = v.M.connectedPixels[base + lane];
+    if (m < v.M.nLowerModules && K::module_ok(v, m)) {
+      cnt = K::n_obj(v, m);
+      ob = K::obj_base(v, m);
+    }
+  }
+  const unsigned incl = wave_inclusive_scan(cnt);
+  if (lane < size) objBase[lane] = ob;
+  return conn_total(incl, size, lane, pre);
+}
+// the object of candidate L: module k is the largest with pre[k] <= L
+__device__ __forceinline__ uint32_t candidate_object(const uint32_t* pre, const uint32_t* objBase, uint32_t size,
+                                                     uint32_t L) {
+  uint32_t lo = 0, hi = size;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= L) lo = mid;
+    else hi = mid;
+  }
+  return objBase[lo] + (L - pre[lo]);
+}
+
+struct PT5Kind {
+  using Res = PT5Result;
+  static constexpr uint32_t kBit = 1, kCap = kNMaxPixelQuintuplets;
+  static constexpr int kMaskWordsScalar = kPT5MaskWords, kSurvScalar = kPT5Survivors;
+  __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt5Mask; }
+  __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt5MaskBase; }
+  __host__ __device__ static uint32_t* count(const EventView& v) { return v.W.pt5Count; }
+  __device__ static bool pls_ok(const EventView& v, uint32_t i) { return v.P.isDup[i] == 0; }
+  __device__ static bool module_ok(const EventView& v, uint32_t m) { return v.M.moduleType[m] != kTwoS; }
+  __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nQuintuplets[m]; }
+  __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.quintupletModuleIndices[m]; }
+  __device__ static bool cheap(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
+    if (v.Q.isDup[t] != 0) return false;
+    return pt5_cheap(v, pls, i, t, v.W.t5Rec + (size_t)kT5RecWords * t, r);
+  }
+  __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res&) {
+    return pt5_costly(v, pls, i, t);
+  }
+  __device__ static void pack(const Res& r, uint32_t* w) {
+    w[0] = __float_as_uint(r.rz); w[1] = __float_as_uint(r.rphi); w[2] = __float_as_uint(r.rphiIn);
+    w[3] = __float_as_uint(r.pixR); w[4] = __float_as_uint(r.t5R); w[5] = __float_as_uint(r.cx);
+    w[6] = __float_as_uint(r.cy);
+  }
+  __device__ static void unpack(const uint32_t* w, Res& r) {
+    r.rz = __uint_as_float(w[0]); r.rphi = __uint_as_float(w[1]); r.rphiIn = __uint_as_float(w[2]);
+    r.pixR = __uint_as_float(w[3]); r.t5R = __uint_as_float(w[4]); r.cx = __uint_as_float(w[5]);
+    r.cy = __uint_as_float(w[6]);
+  }
+  // addPixelQuintupletToMemory (SDL/PixelTriplet.h:1961-2036)
+  __device__ static void write(const EventView& v, uint32_t q, uint32_t pls, uint32_t i, uint32_t t5, const Res& r) {
+    v.P5.pixelIndices[q] = pls;
+    v.P5.T5Indices[q] = t5;
+    v.P5.isDup[q] = 0;
+    v.P5.score[q] = r.rphi;
+    v.P5.eta[q] = v.Q.eta[t5];
+    v.P5.phi[q] = v.Q.phi[t5];
+    v.P5.pixelRadius[q] = r.pixR;
+    v.P5.quintupletRadius[q] = r.t5R;
+    v.P5.centerX[q] = r.cx;
+    v.P5.centerY[q] = r.cy;
+    v.P5.logicalLayers[7 * q] = 0;
+    v.P5.logicalLayers[7 * q + 1] = 0;
+    v.P5.lowerModuleIndices[7 * q] = v.S.innerLowerModuleIndices[pls];
+    v.P5.lowerModuleIndices[7 * q + 1] = v.S.outerLowerModuleIndices[pls];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      v.P5.logicalLayers[7 * q + 2 + j] = v.Q.logicalLayers[5 * t5 + j];
+      v.P5.lowerModuleIndices[7 * q + 2 + j] = v.Q.lowerModuleIndices[5 * t5 + j];
+    }
+    const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
+    v.P5.hitIndices[14 * q] = v.D.anchorHitIndices[pi];
+    v.P5.hitIndices[14 * q + 1] = v.D.outerHitIndices[pi];
+    v.P5.hitIndices[14 * q + 2] = v.D.anchorHitIndices[po];
+    v.P5.hitIndices[14 * q + 3] = v.D.outerHitIndices[po];
+#pragma unroll
+    for (int j = 0; j < 10; j++) v.P5.hitIndices[14 * q + 4 + j] = v.Q.hitIndices[10 * t5 + j];
+    v.P5.rzChiSquared[q] = r.rz;
+    v.P5.rPhiChiSquared[q] = r.rphi;
+    v.P5.rPhiChiSquaredInwards[q] = r.rphiIn;
+    v.T.partOfPT5[v.Q.tripletIndices[2 * t5]] = 1;
+    v.T.partOfPT5[v.Q.tripletIndices[2 * t5 + 1]] = 1;
+    v.P.partOfPT5[i] = 1;
+    v.Q.partOfPT5[t5] = 1;
+  }
+};
+
+struct PT3Kind {
+  using Res = PT3Result;
+  static constexpr uint32_t kBit = 2, kCap = kNMaxPixelTriplets;
+  static constexpr int kMaskWordsScalar = kPT3MaskWords, kSurvScalar = kPT3Survivors;
+  __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt3Mask; }
+  __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt3MaskBase; }
+  __host__ __device__ static uint32_t* count(const EventView& v) { return v.W.pt3Count; }
+  __device__ static bool pls_ok(const EventView& v, uint32_t i) { return v.P.isDup[i] == 0 && v.P.partOfPT5[i] == 0; }
+  __device__ static bool module_ok(const EventView& v, uint32_t m) {
+    return v.M.moduleType[m] != kTwoS && (v.M.layers[m] == 1 || v.M.layers[m] == 2);
+  }
+  __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nTriplets[m]; }
+  __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.tripletModuleIndices[m]; }
+  __device__ static bool cheap(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
+    const T3Rec rec = load_t3_rec(v.W.t3Rec + (size_t)kT3RecWords * t);
+    if ((rec.packed >> 17) & 1) return false;  // middle module 2S
+    if (v.T.partOfPT5[t] != 0) return false;
+    return pt3_cheap(v, pls, i, rec, true, r);
+  }
+  __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
+    return pt3_costly(v, pls, i, t, true, r);
+  }
+  __device__ static void pack(const Res& r, uint32_t* w) {
+    w[0] = __float_as_uint(r.pixR); w[1] = __float_as_uint(r.t3R); w[2] = __float_as_uint(r.rz);
+    w[3] = __float_as_uint(r.rphi); w[4] = __float_as_uint(r.rphiIn);
+  }
+  __device__ static void unpack(const uint32_t* w, Res& r) {
+    r.pixR = __uint_as_float(w[0]); r.t3R = __uint_as_float(w[1]); r.rz = __uint_as_float(w[2]);
+    r.rphi = __uint_as_float(w[3]); r.rphiIn = __uint_as_float(w[4]);
+  }
+  // addPixelTripletToMemory (SDL/PixelTriplet.h:134-204)
+  __device__ static void write(const EventView& v, uint32_t q, uint32_t pls, uint32_t i, uint32_t t3, const Res& r) {
+    const uint32_t m = v.T.lowerModuleIndices[3 * t3];
+    const int adj = v.M.layers[m] == 1 ? 1 : 0;
+    const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * t3] + adj];
+    v.P3.pixelSegmentIndices[q] = pls;
+    v.P3.tripletIndices[q] = t3;
+    v.P3.pixelRadius[q] = r.pixR;
+    v.P3.tripletRadius[q] = r.t3R;
+    v.P3.pt[q] = v.P.ptIn[i];
+    v.P3.eta[q] = v.D.anchorEta[mdAdj];
+    v.P3.phi[q] = v.D.anchorPhi[mdAdj];
+    v.P3.eta_pix[q] = v.P.eta[i];
+    v.P3.phi_pix[q] = v.P.phi[i];
+    v.P3.isDup[q] = 0;
+    v.P3.score[q] = r.rphi + r.rphiIn;
+    v.P3.centerX[q] = 0;
+    v.P3.centerY[q] = 0;
+    v.P3.logicalLayers[5 * q] = 0;
+    v.P3.logicalLayers[5 * q + 1] = 0;
+    v.P3.lowerModuleIndices[5 * q] = v.S.innerLowerModuleIndices[pls];
+    v.P3.lowerModuleIndices[5 * q + 1] = v.S.outerLowerModuleIndices[pls];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      v.P3.logicalLayers[5 * q + 2 + j] = v.T.logicalLayers[3 * t3 + j];
+      v.P3.lowerModuleIndices[5 * q + 2 + j] = v.T.lowerModuleIndices[3 * t3 + j];
+    }
+    const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
+    v.P3.hitIndices[10 * q] = v.D.anchorHitIndices[pi];
+    v.P3.hitIndices[10 * q + 1] = v.D.outerHitIndices[pi];
+    v.P3.hitIndices[10 * q + 2] = v.D.anchorHitIndices[po];
+    v.P3.hitIndices[10 * q + 3] = v.D.outerHitIndices[po];
+#pragma unroll
+    for (int j = 0; j < 6; j++) v.P3.hitIndices[10 * q + 4 + j] = v.T.hitIndices[6 * t3 + j];
+    v.P3.rzChiSquared[q] = r.rz;
+    v.P3.rPhiChiSquared[q] = r.rphi;
+    v.P3.rPhiChiSquaredInwards[q] = r.rphiIn;
+    v.T.partOfPT3[t3] = 1;
+  }
+};
+
+// T3 slots of the modules the pixel-triplet builder connects to, module by
+// module (offsets: one 1024-lane work-group; fill: one wave per module)
+__global__ __launch_bounds__(1024) void k_t3_list_offsets(EventView v) {
+  __shared__ unsigned lds[17];
+  const uint32_t nL = v.M.nLowerModules;
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < nL; b += 1024) {
+    const uint32_t m = b + threadIdx.x;
+    const unsigned c = m < nL && PT3Kind::module_ok(v, m) ? PT3Kind::n_obj(v, m) : 0u;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
+    if (m < nL) v.W.t3ListOff[m] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) v.R.scalars[kT3Listed] = carry;
+}
+__global__ __launch_bounds__(256) void k_t3_list_fill(EventView v) {
+  const uint32_t nL = v.M.nLowerModules;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; m < nL; m += waves) {
+    if (!PT3Kind::module_ok(v, m)) continue;
+    const uint32_t n = PT3Kind::n_obj(v, m);
+    const uint32_t base = n ? PT3Kind::obj_base(v, m) : 0u, off = v.W.t3ListOff[m];
+    for (uint32_t k = threadIdx.x % kWave; k < n; k += kWave) v.W.t3List[off + k] = base + k;
+  }
+}
+
+// the records: one lane per listed T5 (those of modules that are not 2S),
+// then one per listed T3
+__global__ __launch_bounds__(256) void k_pix_records(EventView v) {
+  const uint32_t n5 = v.R.scalars[kT5Listed], n3 = v.R.scalars[kT3Listed];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n5 + n3; k += gridDim.x * blockDim.x) {
+    if (k < n5) {
+      const uint32_t t5 = v.W.t5List[k];
+      if (PT5Kind::module_ok(v, v.Q.lowerModuleIndices[5 * t5]))
+        make_t5_rec(v, t5, v.W.t5Rec + (size_t)kT5RecWords * t5);
+    } else {
+      const uint32_t t3 = v.W.t3List[k - n5];
+      make_t3_rec(v, t3, v.W.t3Rec + (size_t)kT3RecWords * t3);
+    }
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
   __shared__ uint32_t pre[kMaxConn + 1];
+  __shared__ uint32_t objBase[kMaxConn];
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const unsigned lane = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
     __syncthreads();  // the previous pLS's prefix in LDS is no longer read
-    const unsigned lane = threadIdx.x;
-    const uint32_t nL = v.M.nLowerModules;
-    unsigned count = 0;
-    const uint32_t size = v.W.connSize[i];
-    if (v.P.isDup[i] == 0 && size <= (uint32_t)kMaxConn) {
-      const uint32_t base = v.W.connBase[i];
-      unsigned cnt = 0;
-      if (lane < size) {
-        const uint32_t m = v.M.connectedPixels[base + lane];
-        if (m < nL && v.M.moduleType[m] != kTwoS) cnt = v.R.nQuintuplets[m];
-      }
-      const unsigned incl = wave_inclusive_scan(cnt);
-      if (lane < size) pre[lane + 1] = incl;
-      if (lane == 0) pre[0] = 0;
-      const uint32_t total = __shfl(incl, kWave - 1, kWave);
-      __syncthreads();
-      const uint32_t pls = (uint32_t)v.segModuleIndices[nL] + i;
-      const uint32_t slotBase = kWrite ? v.W.pt5Count[i] : 0;
-      // pass masks: the counting pass records one 64-bit mask per chunk, and the
-      // writing pass evaluates only the lanes those masks mark
-      uint32_t maskBase = ~0u;
-      if (!kWrite) {
-        const uint32_t words = (total + kWave - 1) / kWave;
+    uint32_t total = 0, size = 0, words = 0, maskBase = ~0u;
+    bool full = false;
+    if (K::pls_ok(v, i) && v.W.connSize[i] <= (uint32_t)kMaxConn) {
+      total = conn_prefix<K>(v, i, pre, objBase, size);
+      words = (total + kWave - 1) / kWave;
+      if (words) {
         uint32_t b = 0;
-        if (lane == 0 && words) b = atomicAdd(&v.R.scalars[kPT5MaskWords], words);
+        if (lane == 0) b = atomicAdd(&v.R.scalars[K::kMaskWordsScalar], words);
         b = __shfl(b, 0, kWave);
-        if (words && b + words <= kPixMaskWords) maskBase = b;
-        if (lane == 0) v.W.pt5MaskBase[i] = maskBase;
-      } else {
-        maskBase = v.W.pt5MaskBase[i];
-      }
-      const uint32_t maskWords = (total + kWave - 1) / kWave;
-      uint32_t maskHeld = ~0u, parkReg = ~0u;
-      unsigned long long maskReg = 0ull;
-      for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
-        const uint32_t L = c0 + lane, w = c0 / kWave;
-        bool pass = false;
-        uint32_t t5 = 0;
-        PT5Result r;
-        bool eval = L < total;
-        if (kWrite && maskBase != ~0u) {
-          // each lane holds one chunk's mask and park position; 64 chunks per refill
-          const uint32_t wb = w & ~(uint32_t)(kWave - 1);
-          if (wb != maskHeld) {
-            maskHeld = wb;
-            maskReg = wb + lane < maskWords ? v.W.pt5Mask[maskBase + wb + lane] : 0ull;
-            parkReg = wb + lane < maskWords ? v.W.pt5ParkAt[maskBase + wb + lane] : ~0u;
-          }
-          const unsigned long long mk = __shfl(maskReg, (int)(w - wb), kWave);
-          if (mk == 0ull) continue;  // no passing candidate in this chunk
-          const uint32_t pb = __shfl(parkReg, (int)(w - wb), kWave);
-          eval = eval && ((mk >> lane) & 1ull);
-          if (pb != ~0u) {  // results parked by the counting pass: no second evaluation
-            if (eval) {
-              const uint32_t* rec = &v.W.pt5Park[kPT5ParkWords * (pb + lanes_below(mk))];
-              t5 = rec[0];
-              r.rz = __uint_as_float(rec[1]); r.rphi = __uint_as_float(rec[2]); r.rphiIn = __uint_as_float(rec[3]);
-              r.pixR = __uint_as_float(rec[4]); r.t5R = __uint_as_float(rec[5]);
-              r.cx = __uint_as_float(rec[6]); r.cy = __uint_as_float(rec[7]);
-              pass = true;
-            }
-            eval = false;
-          }
-        }
-        if (eval) {
-          uint32_t lo = 0, hi = size;  // largest k with pre[k] <= L
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (pre[mid] <= L) lo = mid;
-            else hi = mid;
-          }
-          const uint32_t m = v.M.connectedPixels[base + lo];
-          t5 = (uint32_t)v.R.quintupletModuleIndices[m] + (L - pre[lo]);
-          if (v.Q.isDup[t5] == 0) pass = pt5_select(v, pls, i, t5, r);
-        }
-        unsigned n;
-        const unsigned rank = count + wave_rank(pass, n);
-        const unsigned long long passMask = __ballot(pass);  // every lane of the wave takes part
-        if (!kWrite && maskBase != ~0u) {
-          uint32_t pb = ~0u;
-          if (passMask) {
-            if (lane == 0) {
-              pb = atomicAdd(&v.R.scalars[kPT5Parked], (uint32_t)__popcll(passMask));
-              if (pb + (uint32_t)__popcll(passMask) > kPixParkMax) pb = ~0u;
-            }
-            pb = __shfl(pb, 0, kWave);
-          }
-          if (lane == 0) {
-            v.W.pt5Mask[maskBase + w] = passMask;
-            v.W.pt5ParkAt[maskBase + w] = pb;
-          }
-          if (pass && pb != ~0u) {
-            uint32_t* rec = &v.W.pt5Park[kPT5ParkWords * (pb + lanes_below(passMask))];
-            rec[0] = t5;
-            rec[1] = __float_as_uint(r.rz); rec[2] = __float_as_uint(r.rphi); rec[3] = __float_as_uint(r.rphiIn);
-            rec[4] = __float_as_uint(r.pixR); rec[5] = __float_as_uint(r.t5R);
-            rec[6] = __float_as_uint(r.cx); rec[7] = __float_as_uint(r.cy);
-          }
-        }
-        if (kWrite && pass && slotBase + rank < kNMaxPixelQuintuplets) {
-          // addPixelQuintupletToMemory (SDL/PixelTriplet.h:1961-2036)
-          const uint32_t q = slotBase + rank;
-          v.P5.pixelIndices[q] = pls;
-          v.P5.T5Indices[q] = t5;
-          v.P5.isDup[q] = 0;
-          v.P5.score[q] = r.rphi;
-          v.P5.eta[q] = v.Q.eta[t5];
-          v.P5.phi[q] = v.Q.phi[t5];
-          v.P5.pixelRadius[q] = r.pixR;
-          v.P5.quintupletRadius[q] = r.t5R;
-          v.P5.centerX[q] = r.cx;
-          v.P5.centerY[q] = r.cy;
-          v.P5.logicalLayers[7 * q] = 0;
-          v.P5.logicalLayers[7 * q + 1] = 0;
-          v.P5.lowerModuleIndices[7 * q] = v.S.innerLowerModuleIndices[pls];
-          v.P5.lowerModuleIndices[7 * q + 1] = v.S.outerLowerModuleIndices[pls];
-  #pragma unroll
-          for (int j = 0; j < 5; j++) {
-            v.P5.logicalLayers[7 * q + 2 + j] = v.Q.logicalLayers[5 * t5 + j];
-            v.P5.lowerModuleIndices[7 * q + 2 + j] = v.Q.lowerModuleIndices[5 * t5 + j];
-          }
-          const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
-          v.P5.hitIndices[14 * q] = v.D.anchorHitIndices[pi];
-          v.P5.hitIndices[14 * q + 1] = v.D.outerHitIndices[pi];
-          v.P5.hitIndices[14 * q + 2] = v.D.anchorHitIndices[po];
-          v.P5.hitIndices[14 * q + 3] = v.D.outerHitIndices[po];
-  #pragma unroll
-          for (int j = 0; j < 10; j++) v.P5.hitIndices[14 * q + 4 + j] = v.Q.hitIndices[10 * t5 + j];
-          v.P5.rzChiSquared[q] = r.rz;
-          v.P5.rPhiChiSquared[q] = r.rphi;
-          v.P5.rPhiChiSquaredInwards[q] = r.rphiIn;
-          v.T.partOfPT5[v.Q.tripletIndices[2 * t5]] = 1;
-          v.T.partOfPT5[v.Q.tripletIndices[2 * t5 + 1]] = 1;
-          v.P.partOfPT5[i] = 1;
-          v.Q.partOfPT5[t5] = 1;
-        }
-        count += n;
+        if (b + words <= kPixMaskWords) maskBase = b;
+        else full = true;
       }
     }
-    if (!kWrite && lane == 0) v.W.pt5Count[i] = count;
+    if (maskBase != ~0u) {
+      const uint32_t pls = plsBase + i;
+      for (uint32_t w = 0; w < words; w++) {
+        const uint32_t L = w * kWave + lane;
+        typename K::Res r;
+        uint32_t t = 0;
+        bool surv = false;
+        if (L < total) {
+          t = candidate_object(pre, objBase, size, L);
+          surv = K::cheap(v, pls, i, t, r);
+        }
+        if (lane == 0) K::mask(v)[maskBase + w] = 0ull;
+        const unsigned long long sm = __ballot(surv);  // every lane of the wave takes part
+        if (sm) {
+          const uint32_t n = (uint32_t)__popcll(sm);
+          uint32_t sb = 0;
+          if (lane == 0) sb = atomicAdd(&v.R.scalars[K::kSurvScalar], n);
+          sb = __shfl(sb, 0, kWave);
+          const uint32_t s = sb + lanes_below(sm);
+          if (surv && s < kPixSurvMax) {
+            uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+            rec[0] = i;
+            rec[1] = (maskBase + w) * kWave + lane;
+            rec[2] = t;
+            K::pack(r, rec + 3);
+          }
+          if (sb + n > kPixSurvMax) full = true;
+        }
+      }
+    }
+    if (lane == 0) {
+      K::mask_base(v)[i] = maskBase;
+      v.W.pixWords[i] = words;
+      K::count(v)[i] = 0;
+      v.W.pixFull[i] = full ? 1 : 0;
+      if (full) atomicOr(&v.R.scalars[kPixFull], K::kBit);
+    }
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
+  const uint32_t nS = v.R.scalars[K::kSurvScalar] < kPixSurvMax ? v.R.scalars[K::kSurvScalar] : kPixSurvMax;
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nS; s += gridDim.x * blockDim.x) {
+    uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+    const uint32_t i = rec[0], slot = rec[1], t = rec[2];
+    typename K::Res r;
+    K::unpack(rec + 3, r);
+    if (K::costly(v, plsBase + i, i, t, r)) {
+      K::pack(r, rec + 3);
+      atomicOr(&K::mask(v)[slot / kWave], 1ull << (slot % kWave));
+      v.W.pixParkIdx[slot] = s;
+      atomicAdd(&K::count(v)[i], 1u);
+    }
+  }
+}
+
+// the pLS flagged by stage A, counted by evaluating every candidate
+template <class K>
+__global__ __launch_bounds__(kWave) void k_pix_full(EventView v) {
+  if ((v.R.scalars[kPixFull] & K::kBit) == 0) return;
+  __shared__ uint32_t pre[kMaxConn + 1];
+  __shared__ uint32_t objBase[kMaxConn];
+  const uint32_t nPLS = v.R.scalars[kEvNPLS];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const unsigned lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
+    if (!v.W.pixFull[i]) continue;
+    __syncthreads();
+    uint32_t size;
+    const uint32_t total = conn_prefix<K>(v, i, pre, objBase, size);
+    unsigned count = 0;
+    for (uint32_t L0 = 0; L0 < total; L0 += kWave) {
+      const uint32_t L = L0 + lane;
+      bool pass = false;
+      if (L < total) {
+        const uint32_t t = candidate_object(pre, objBase, size, L);
+        typename K::Res r;
+        pass = K::cheap(v, plsBase + i, i, t, r) && K::costly(v, plsBase + i, i, t, r);
+      }
+      count += (unsigned)__popcll(__ballot(pass));
+    }
+    if (lane == 0) K::count(v)[i] = count;
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(kWave) void k_pix_write(EventView v) {
+  const uint32_t nPLS = v.R.scalars[kEvNPLS];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const unsigned lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
+    const uint32_t maskBase = K::mask_base(v)[i];
+    if (maskBase == ~0u || v.W.pixFull[i]) continue;
+    const uint32_t words = v.W.pixWords[i], slotBase = K::count(v)[i], pls = plsBase + i;
+    unsigned done = 0;
+    for (uint32_t w = 0; w < words; w++) {
+      const unsigned long long mk = K::mask(v)[maskBase + w];
+      if (mk == 0ull) continue;
+      if ((mk >> lane) & 1ull) {
+        const uint32_t q = slotBase + done + lanes_below(mk);
+        if (q < K::kCap) {
+          const uint32_t s = v.W.pixParkIdx[(maskBase + w) * kWave + lane];
+          const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+          typename K::Res r;
+          K::unpack(rec + 3, r);
+          K::write(v, q, pls, i, rec[2], r);
+        }
+      }
+      done += (unsigned)__popcll(mk);
+    }
+  }
+}
+
+// the flagged pLS: every candidate evaluated again and written to its slot
+template <class K>
+__global__ __launch_bounds__(kWave) void k_pix_full_write(EventView v) {
+  if ((v.R.scalars[kPixFull] & K::kBit) == 0) return;
+  __shared__ uint32_t pre[kMaxConn + 1];
+  __shared__ uint32_t objBase[kMaxConn];
+  const uint32_t nPLS = v.R.scalars[kEvNPLS];
+  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
+  const unsigned lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
+    if (!v.W.pixFull[i]) continue;
+    __syncthreads();
+    uint32_t size;
+    const uint32_t total = conn_prefix<K>(v, i, pre, objBase, size);
+    const uint32_t slotBase = K::count(v)[i], pls = plsBase + i;
+    unsigned done = 0;
+    for (uint32_t L0 = 0; L0 < total; L0 += kWave) {
+      const uint32_t L = L0 + lane;
+      bool pass = false;
+      uint32_t t = 0;
+      typename K::Res r;
+      if (L < total) {
+        t = candidate_object(pre, objBase, size, L);
+        pass = K::cheap(v, pls, i, t, r) && K::costly(v, pls, i, t, r);
+      }
+      unsigned n;
+      const unsigned rank = done + wave_rank(pass, n);
+      if (pass && slotBase + rank < K::kCap) K::write(v, slotBase + rank, pls, i, t, r);
+      done += n;
+    }
   }
 }
 
@@ -224,6 +484,30 @@ __global__ void k_pt5_totals(EventView v) {
   const uint32_t tot = v.R.scalars[kTotOccPixelQuintuplets];
   v.R.scalars[kNPixelQuintuplets] = tot < kNMaxPixelQuintuplets ? tot : kNMaxPixelQuintuplets;
 }
+
+// Flag compaction over items [0, n) by 256-lane work-groups: work-group b
+// takes items [256 b, 256 b + 256) and counts the flagged items before them
+// itself, so the work-groups need no scan between them.  Returns this lane's
+// rank among the flagged items; `last` is set in the work-group holding item
+// n - 1 (work-group 0 when n is 0), whose `total` is then the flagged total.
+constexpr unsigned kCompactBlock = 256;
+template <class Flag>
+__device__ __forceinline__ unsigned block_compact(uint32_t n, Flag flag, uint32_t item, bool& flagged, bool& last,
+                                                  unsigned& total) {
+  __shared__ unsigned lds[kCompactBlock / 64 + 1];
+  const uint32_t b0 = blockIdx.x * kCompactBlock;
+  unsigned c = 0;
+  for (uint32_t q = threadIdx.x; q < b0 && q < n; q += kCompactBlock) c += flag(q) ? 1u : 0u;
+  unsigned before;
+  block_exclusive_scan<kCompactBlock>(c, lds, before);
+  flagged = item < n && flag(item);
+  unsigned tot;
+  const unsigned rank = before + block_exclusive_scan<kCompactBlock>(flagged ? 1u : 0u, lds, tot);
+  last = (b0 < n && n <= b0 + kCompactBlock) || (n == 0 && blockIdx.x == 0);
+  total = before + tot;
+  return rank;
+}
+inline unsigned compact_grid(uint32_t cap) { return (cap + kCompactBlock - 1) / kCompactBlock; }
 
 // addTrackCandidateToMemory (SDL/TrackCandidate.h:126-164), with the entries
 // past the object's size cleared (the reference's buffers start zeroed)
@@ -247,183 +531,27 @@ __device__ __forceinline__ void add_tc(const TCSoA& C, uint32_t idx, int16_t typ
 }
 
 // addpT5asTrackCandidateInGPU (SDL/TrackCandidate.h:532-580): the first TCs
-__global__ __launch_bounds__(1024) void k_tc_add_pt5(EventView v) {
-  __shared__ unsigned lds[17];
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt5(EventView v) {
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
+  if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t q = b + threadIdx.x;
-    const bool add = q < n && v.P5.isDup[q] == 0;
-    unsigned tot;
-    const unsigned slot = carry + block_exclusive_scan<1024>(add ? 1u : 0u, lds, tot);
-    if (add && slot < kNMaxPixelTrackCandidates) {
-      const uint32_t pls = v.P5.pixelIndices[q];
-      const float radius = 0.5f * (v.P5.pixelRadius[q] + v.P5.quintupletRadius[q]);
-      add_tc(v.C, slot, 7, pls, v.P5.T5Indices[q], &v.P5.logicalLayers[7 * q], &v.P5.lowerModuleIndices[7 * q],
-             &v.P5.hitIndices[14 * q], (int)v.P.seedIdx[pls - plsBase], v.P5.centerX[q], v.P5.centerY[q], radius, q);
-    }
-    carry += tot;
+  const uint32_t q = blockIdx.x * kCompactBlock + threadIdx.x;
+  bool add, last;
+  unsigned total;
+  const unsigned slot = block_compact(n, [&](uint32_t k) { return v.P5.isDup[k] == 0; }, q, add, last, total);
+  if (add && slot < kNMaxPixelTrackCandidates) {
+    const uint32_t pls = v.P5.pixelIndices[q];
+    const float radius = 0.5f * (v.P5.pixelRadius[q] + v.P5.quintupletRadius[q]);
+    add_tc(v.C, slot, 7, pls, v.P5.T5Indices[q], &v.P5.logicalLayers[7 * q], &v.P5.lowerModuleIndices[7 * q],
+           &v.P5.hitIndices[14 * q], (int)v.P.seedIdx[pls - plsBase], v.P5.centerX[q], v.P5.centerY[q], radius, q);
   }
-  if (threadIdx.x == 0) {
-    const unsigned added = carry < kNMaxPixelTrackCandidates ? carry : kNMaxPixelTrackCandidates;
+  if (last && threadIdx.x == 0) {
+    const unsigned added = total < kNMaxPixelTrackCandidates ? total : kNMaxPixelTrackCandidates;
     v.R.scalars[kNTrackCandidates] = added;
     v.R.scalars[kNTrackCandidatespT5] = added;
     v.R.scalars[kNTrackCandidatespT3] = 0;
     v.R.scalars[kNTrackCandidatespLS] = 0;
     v.R.scalars[kNTrackCandidatesT5] = 0;
-  }
-}
-
-// ------------------------------------------------------------- pixel triplets
-// one wave per pLS (striding over the pLS); candidates (connected module k, T3 b) in serial order
-template <bool kWrite>
-__global__ __launch_bounds__(kWave) void k_pixel_triplets(EventView v) {
-  __shared__ uint32_t pre[kMaxConn + 1];
-  const uint32_t nPLS = v.R.scalars[kEvNPLS];
-  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
-    __syncthreads();  // the previous pLS's prefix in LDS is no longer read
-    const unsigned lane = threadIdx.x;
-    const uint32_t nL = v.M.nLowerModules;
-    unsigned count = 0;
-    const uint32_t size = v.W.connSize[i];
-    if (v.P.isDup[i] == 0 && v.P.partOfPT5[i] == 0 && size <= (uint32_t)kMaxConn) {
-      const uint32_t base = v.W.connBase[i];
-      unsigned cnt = 0;
-      if (lane < size) {
-        const uint32_t m = v.M.connectedPixels[base + lane];
-        if (m < nL && v.M.moduleType[m] != kTwoS && (v.M.layers[m] == 1 || v.M.layers[m] == 2))
-          cnt = v.R.nTriplets[m];
-      }
-      const unsigned incl = wave_inclusive_scan(cnt);
-      if (lane < size) pre[lane + 1] = incl;
-      if (lane == 0) pre[0] = 0;
-      const uint32_t total = __shfl(incl, kWave - 1, kWave);
-      __syncthreads();
-      const uint32_t plsBase = (uint32_t)v.segModuleIndices[nL];
-      const uint32_t pls = plsBase + i;
-      const uint32_t slotBase = kWrite ? v.W.pt3Count[i] : 0;
-      uint32_t maskBase = ~0u;  // pass masks, as for the pixel quintuplets
-      if (!kWrite) {
-        const uint32_t words = (total + kWave - 1) / kWave;
-        uint32_t b = 0;
-        if (lane == 0 && words) b = atomicAdd(&v.R.scalars[kPT3MaskWords], words);
-        b = __shfl(b, 0, kWave);
-        if (words && b + words <= kPixMaskWords) maskBase = b;
-        if (lane == 0) v.W.pt3MaskBase[i] = maskBase;
-      } else {
-        maskBase = v.W.pt3MaskBase[i];
-      }
-      const uint32_t maskWords = (total + kWave - 1) / kWave;
-      uint32_t maskHeld = ~0u, parkReg = ~0u;
-      unsigned long long maskReg = 0ull;
-      for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
-        const uint32_t L = c0 + lane, w = c0 / kWave;
-        bool pass = false;
-        uint32_t t3 = 0, m = 0;
-        PT3Result r;
-        bool eval = L < total;
-        if (kWrite && maskBase != ~0u) {
-          const uint32_t wb = w & ~(uint32_t)(kWave - 1);
-          if (wb != maskHeld) {
-            maskHeld = wb;
-            maskReg = wb + lane < maskWords ? v.W.pt3Mask[maskBase + wb + lane] : 0ull;
-            parkReg = wb + lane < maskWords ? v.W.pt3ParkAt[maskBase + wb + lane] : ~0u;
-          }
-          const unsigned long long mk = __shfl(maskReg, (int)(w - wb), kWave);
-          if (mk == 0ull) continue;
-          const uint32_t pb = __shfl(parkReg, (int)(w - wb), kWave);
-          eval = eval && ((mk >> lane) & 1ull);
-          if (pb != ~0u) {
-            if (eval) {
-              const uint32_t* rec = &v.W.pt3Park[kPT3ParkWords * (pb + lanes_below(mk))];
-              t3 = rec[0];
-              m = v.T.lowerModuleIndices[3 * t3];
-              r.pixR = __uint_as_float(rec[1]); r.t3R = __uint_as_float(rec[2]); r.rz = __uint_as_float(rec[3]);
-              r.rphi = __uint_as_float(rec[4]); r.rphiIn = __uint_as_float(rec[5]);
-              pass = true;
-            }
-            eval = false;
-          }
-        }
-        if (eval) {
-          uint32_t lo = 0, hi = size;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (pre[mid] <= L) lo = mid;
-            else hi = mid;
-          }
-          m = v.M.connectedPixels[base + lo];
-          t3 = (uint32_t)v.R.tripletModuleIndices[m] + (L - pre[lo]);
-          if (v.M.moduleType[v.T.lowerModuleIndices[3 * t3 + 1]] != kTwoS && v.T.partOfPT5[t3] == 0)
-            pass = pt3_select(v, pls, i, t3, true, r);
-        }
-        unsigned n;
-        const unsigned rank = count + wave_rank(pass, n);
-        const unsigned long long passMask = __ballot(pass);  // every lane of the wave takes part
-        if (!kWrite && maskBase != ~0u) {
-          uint32_t pb = ~0u;
-          if (passMask) {
-            if (lane == 0) {
-              pb = atomicAdd(&v.R.scalars[kPT3Parked], (uint32_t)__popcll(passMask));
-              if (pb + (uint32_t)__popcll(passMask) > kPixParkMax) pb = ~0u;
-            }
-            pb = __shfl(pb, 0, kWave);
-          }
-          if (lane == 0) {
-            v.W.pt3Mask[maskBase + w] = passMask;
-            v.W.pt3ParkAt[maskBase + w] = pb;
-          }
-          if (pass && pb != ~0u) {
-            uint32_t* rec = &v.W.pt3Park[kPT3ParkWords * (pb + lanes_below(passMask))];
-            rec[0] = t3;
-            rec[1] = __float_as_uint(r.pixR); rec[2] = __float_as_uint(r.t3R); rec[3] = __float_as_uint(r.rz);
-            rec[4] = __float_as_uint(r.rphi); rec[5] = __float_as_uint(r.rphiIn);
-          }
-        }
-        if (kWrite && pass && slotBase + rank < kNMaxPixelTriplets) {
-          // addPixelTripletToMemory (SDL/PixelTriplet.h:134-204)
-          const uint32_t q = slotBase + rank;
-          const int adj = v.M.layers[m] == 1 ? 1 : 0;
-          const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * t3] + adj];
-          v.P3.pixelSegmentIndices[q] = pls;
-          v.P3.tripletIndices[q] = t3;
-          v.P3.pixelRadius[q] = r.pixR;
-          v.P3.tripletRadius[q] = r.t3R;
-          v.P3.pt[q] = v.P.ptIn[i];
-          v.P3.eta[q] = v.D.anchorEta[mdAdj];
-          v.P3.phi[q] = v.D.anchorPhi[mdAdj];
-          v.P3.eta_pix[q] = v.P.eta[i];
-          v.P3.phi_pix[q] = v.P.phi[i];
-          v.P3.isDup[q] = 0;
-          v.P3.score[q] = r.rphi + r.rphiIn;
-          v.P3.centerX[q] = 0;
-          v.P3.centerY[q] = 0;
-          v.P3.logicalLayers[5 * q] = 0;
-          v.P3.logicalLayers[5 * q + 1] = 0;
-          v.P3.lowerModuleIndices[5 * q] = v.S.innerLowerModuleIndices[pls];
-          v.P3.lowerModuleIndices[5 * q + 1] = v.S.outerLowerModuleIndices[pls];
-  #pragma unroll
-          for (int j = 0; j < 3; j++) {
-            v.P3.logicalLayers[5 * q + 2 + j] = v.T.logicalLayers[3 * t3 + j];
-            v.P3.lowerModuleIndices[5 * q + 2 + j] = v.T.lowerModuleIndices[3 * t3 + j];
-          }
-          const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
-          v.P3.hitIndices[10 * q] = v.D.anchorHitIndices[pi];
-          v.P3.hitIndices[10 * q + 1] = v.D.outerHitIndices[pi];
-          v.P3.hitIndices[10 * q + 2] = v.D.anchorHitIndices[po];
-          v.P3.hitIndices[10 * q + 3] = v.D.outerHitIndices[po];
-  #pragma unroll
-          for (int j = 0; j < 6; j++) v.P3.hitIndices[10 * q + 4 + j] = v.T.hitIndices[6 * t3 + j];
-          v.P3.rzChiSquared[q] = r.rz;
-          v.P3.rPhiChiSquared[q] = r.rphi;
-          v.P3.rPhiChiSquaredInwards[q] = r.rphiIn;
-          v.T.partOfPT3[t3] = 1;
-        }
-        count += n;
-      }
-    }
-    if (!kWrite && lane == 0) v.W.pt3Count[i] = count;
   }
 }
 
@@ -434,28 +562,24 @@ __global__ void k_pt3_totals(EventView v) {
 
 // ---------------------------------------------------------- track candidates
 // addpT3asTrackCandidatesInGPU (SDL/TrackCandidate.h:381-428)
-__global__ __launch_bounds__(1024) void k_tc_add_pt3(EventView v) {
-  __shared__ unsigned lds[17];
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt3(EventView v) {
   const uint32_t n = v.R.scalars[kNPixelTriplets];
+  if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t base = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t p = b + threadIdx.x;
-    const bool add = p < n && v.P3.isDup[p] == 0;
-    unsigned tot;
-    const unsigned slot = base + carry + block_exclusive_scan<1024>(add ? 1u : 0u, lds, tot);
-    if (add && slot < kNMaxPixelTrackCandidates) {
-      const float radius = 0.5f * (v.P3.pixelRadius[p] + v.P3.tripletRadius[p]);
-      const uint32_t pls = v.P3.pixelSegmentIndices[p];
-      add_tc(v.C, slot, 5, p, p, &v.P3.logicalLayers[5 * p], &v.P3.lowerModuleIndices[5 * p],
-             &v.P3.hitIndices[10 * p], (int)v.P.seedIdx[pls - plsBase], v.P3.centerX[p], v.P3.centerY[p], radius, p);
-    }
-    carry += tot;
+  const uint32_t p = blockIdx.x * kCompactBlock + threadIdx.x;
+  bool add, last;
+  unsigned total;
+  const unsigned slot = base + block_compact(n, [&](uint32_t k) { return v.P3.isDup[k] == 0; }, p, add, last, total);
+  if (add && slot < kNMaxPixelTrackCandidates) {
+    const float radius = 0.5f * (v.P3.pixelRadius[p] + v.P3.tripletRadius[p]);
+    const uint32_t pls = v.P3.pixelSegmentIndices[p];
+    add_tc(v.C, slot, 5, p, p, &v.P3.logicalLayers[5 * p], &v.P3.lowerModuleIndices[5 * p], &v.P3.hitIndices[10 * p],
+           (int)v.P.seedIdx[pls - plsBase], v.P3.centerX[p], v.P3.centerY[p], radius, p);
   }
-  if (threadIdx.x == 0) {
+  if (last && threadIdx.x == 0) {
     const unsigned room = base < kNMaxPixelTrackCandidates ? kNMaxPixelTrackCandidates - base : 0u;
-    const unsigned added = carry < room ? carry : room;
+    const unsigned added = total < room ? total : room;
     v.R.scalars[kNTrackCandidates] = base + added;
     v.R.scalars[kNTrackCandidatespT3] = added;
   }
@@ -535,44 +659,43 @@ __global__ __launch_bounds__(256) void k_tc_t5_write(EventView v) {
 
 // addpLSasTrackCandidateInGPU (SDL/TrackCandidate.h:484-530) with
 // addpLSTrackCandidateToMemory (:112-124): hits stored in (x, z, y, w) order
-__global__ __launch_bounds__(1024) void k_tc_add_pls(EventView v) {
-  __shared__ unsigned lds[17];
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pls(EventView v) {
   const uint32_t n = v.R.scalars[kEvNPLS];
+  if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t base = v.R.scalars[kNTrackCandidates];
   const uint32_t nT5 = v.R.scalars[kNTrackCandidatesT5];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    bool add = false;
-    if (i < n) add = v.tcPlsTriplets ? v.P.isDup[i] == 0 : (v.P.isQuad[i] && v.P.isDup[i] == 0);
-    unsigned tot;
-    const unsigned slot = base + carry + block_exclusive_scan<1024>(add ? 1u : 0u, lds, tot);
-    if (add && slot - nT5 < kNMaxPixelTrackCandidates) {
-      v.C.trackCandidateType[slot] = 8;
-      v.C.directObjectIndices[slot] = i;
-      v.C.pixelSeedIndex[slot] = (int)v.P.seedIdx[i];
-      v.C.objectIndices[2 * slot] = i;
-      v.C.objectIndices[2 * slot + 1] = i;
-      const uint32_t* h = &v.P.pLSHitsIdxs[4 * i];
-      v.C.hitIndices[14 * slot + 0] = h[0];
-      v.C.hitIndices[14 * slot + 1] = h[2];
-      v.C.hitIndices[14 * slot + 2] = h[1];
-      v.C.hitIndices[14 * slot + 3] = h[3];
-      for (int k = 4; k < 14; k++) v.C.hitIndices[14 * slot + k] = 0;
-      for (int k = 0; k < 7; k++) {
-        v.C.logicalLayers[7 * slot + k] = 0;
-        v.C.lowerModuleIndices[7 * slot + k] = 0;
-      }
-      v.C.centerX[slot] = 0;  // not written by the reference for pLS candidates
-      v.C.centerY[slot] = 0;
-      v.C.radius[slot] = 0;
+  const uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool triplets = v.tcPlsTriplets;
+  bool add, last;
+  unsigned total;
+  const unsigned slot =
+      base + block_compact(
+                 n, [&](uint32_t k) { return triplets ? v.P.isDup[k] == 0 : (v.P.isQuad[k] && v.P.isDup[k] == 0); }, i,
+                 add, last, total);
+  if (add && slot - nT5 < kNMaxPixelTrackCandidates) {
+    v.C.trackCandidateType[slot] = 8;
+    v.C.directObjectIndices[slot] = i;
+    v.C.pixelSeedIndex[slot] = (int)v.P.seedIdx[i];
+    v.C.objectIndices[2 * slot] = i;
+    v.C.objectIndices[2 * slot + 1] = i;
+    const uint32_t* h = &v.P.pLSHitsIdxs[4 * i];
+    v.C.hitIndices[14 * slot + 0] = h[0];
+    v.C.hitIndices[14 * slot + 1] = h[2];
+    v.C.hitIndices[14 * slot + 2] = h[1];
+    v.C.hitIndices[14 * slot + 3] = h[3];
+    for (int k = 4; k < 14; k++) v.C.hitIndices[14 * slot + k] = 0;
+    for (int k = 0; k < 7; k++) {
+      v.C.logicalLayers[7 * slot + k] = 0;
+      v.C.lowerModuleIndices[7 * slot + k] = 0;
     }
-    carry += tot;
+    v.C.centerX[slot] = 0;  // not written by the reference for pLS candidates
+    v.C.centerY[slot] = 0;
+    v.C.radius[slot] = 0;
   }
-  if (threadIdx.x == 0) {
+  if (last && threadIdx.x == 0) {
     const unsigned limit = nT5 + kNMaxPixelTrackCandidates;
     const unsigned room = base < limit ? limit - base : 0u;
-    const unsigned added = carry < room ? carry : room;
+    const unsigned added = total < room ? total : room;
     v.R.scalars[kNTrackCandidates] = base + added;
     v.R.scalars[kNTrackCandidatespLS] = added;
   }
@@ -605,33 +728,39 @@ void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const Kern
   launch_pls_dup(v, s, false, t);
 }
 
+template <class K>
+void launch_pixel_builder(const EventView& v, hipStream_t s, const KernelTimers& t, int timer, uint32_t* countScalar,
+                          void (*totals)(EventView)) {
+  t.begin(s, timer);
+  hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  t.end(s, timer);
+  hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_pix_full<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar);
+  hipLaunchKernelGGL(totals, dim3(1), dim3(1), 0, s, v);
+  hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_pix_full_write<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
+}
+
 void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_pls_connected, dim3(grid_of(kNMaxPixelSegments, 256)), dim3(256), 0, s, v);
-  t.begin(s, kTmPT5Build);
-  hipLaunchKernelGGL(k_pixel_quintuplets<false>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
-  t.end(s, kTmPT5Build);
-  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, v.W.pt5Count, v.R.scalars + kEvNPLS,
-                     v.R.scalars + kTotOccPixelQuintuplets);
-  hipLaunchKernelGGL(k_pt5_totals, dim3(1), dim3(1), 0, s, v);
-  hipLaunchKernelGGL(k_pixel_quintuplets<true>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  launch_t5_list(v, s);
+  hipLaunchKernelGGL(k_t3_list_offsets, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_list_fill, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_pix_records, dim3(256), dim3(256), 0, s, v);
+  launch_pixel_builder<PT5Kind>(v, s, t, kTmPT5Build, v.R.scalars + kTotOccPixelQuintuplets, k_pt5_totals);
   launch_pt5_dup(v, s, t);
-  hipLaunchKernelGGL(k_tc_add_pt5, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_tc_add_pt5, dim3(compact_grid(kNMaxPixelQuintuplets)), dim3(kCompactBlock), 0, s, v);
 }
 
 void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  t.begin(s, kTmPT3Build);
-  hipLaunchKernelGGL(k_pixel_triplets<false>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
-  t.end(s, kTmPT3Build);
-  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, v.W.pt3Count, v.R.scalars + kEvNPLS,
-                     v.R.scalars + kTotOccPixelTriplets);
-  hipLaunchKernelGGL(k_pt3_totals, dim3(1), dim3(1), 0, s, v);
-  hipLaunchKernelGGL(k_pixel_triplets<true>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  launch_pixel_builder<PT3Kind>(v, s, t, kTmPT3Build, v.R.scalars + kTotOccPixelTriplets, k_pt3_totals);
   launch_pt3_dup(v, s, t);
 }
 
 void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimers& t) {
   launch_cross_pt3(v, s);
-  hipLaunchKernelGGL(k_tc_add_pt3, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_tc_add_pt3, dim3(compact_grid(kNMaxPixelTriplets)), dim3(kCompactBlock), 0, s, v);
   launch_t5_dup_before_tc(v, s, t);
   launch_cross_t5(v, s);
   hipLaunchKernelGGL(k_tc_t5_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
@@ -639,7 +768,7 @@ void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTime
   hipLaunchKernelGGL(k_tc_t5_write, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   launch_pls_dup(v, s, true, t);
   launch_cross_pls(v, s, t);
-  hipLaunchKernelGGL(k_tc_add_pls, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_tc_add_pls, dim3(compact_grid(kNMaxPixelSegments)), dim3(kCompactBlock), 0, s, v);
   hipLaunchKernelGGL(k_tc_output, dim3(grid_of(kTCCap, 256)), dim3(256), 0, s, v);
 }
 
