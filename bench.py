@@ -26,7 +26,7 @@ STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets"
 KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_md_eval", "k_ls_eval", "k_t3_select", "k_t5_final",
                 "k_pls_pairs<false>", "k_pix_stage_a<PT5Kind>", "k_pix_stage_a<PT3Kind>", "k_t5_tc_pairs",
                 "k_pt5_pairs", "k_pt3_pairs", "k_cross_pls_pairs", "k_pls_pairs<true>", "k_t5_dup_after_build",
-                "k_t3_filter_md", "k_t5_dnn"]
+                "k_t3_enum", "k_t5_dnn"]
 ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
 LAST_STAGE = 9  # the full chain up to track candidates
 HBM_PEAK_GBS = 8000.0
@@ -58,7 +58,8 @@ def algorithmic_bytes(counts: dict) -> dict:
                            candidate pair and 36 B of parked results per mini-doublet
       k_ls_eval          : read 44 B of each mini-doublet, write a 4 B result index per candidate and
                            24 B of parked results per segment
-      k_t3_filter_md     : per candidate pair 18 B (prefix, module, two MD indices, result index), 12 B/survivor
+      k_t3_enum          : per inner segment 20 B (outer mini-doublet, its segment range, prefix, outer module),
+                           per shared-MD candidate 4 B read (listed segment) and 12 B written
       k_t3_select        : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
       k_t5_final         : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
       k_pls_pairs        : read eta, phi, score, isQuad, flag and 4 hit indices (30 B/pLS), write 1 B/pLS
@@ -77,7 +78,7 @@ def algorithmic_bytes(counts: dict) -> dict:
         "k_pixel_segments": 500 * pls,
         "k_md_eval": 16 * h + 4 * counts.get("mdcand", 0) + 36 * md,
         "k_ls_eval": 44 * md + 4 * counts.get("lscand", 0) + 24 * ls,
-        "k_t3_filter_md": 18 * counts.get("t3cand", 0) + 12 * counts.get("t3surv", 0),
+        "k_t3_enum": 20 * ls + 16 * counts.get("t3surv", 0),
         "k_t3_select": 22 * ls + 20 * md + 60 * t3,
         "k_t5_final": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
         "k_pls_pairs<false>": 31 * pls,

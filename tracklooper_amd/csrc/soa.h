@@ -105,6 +105,7 @@ struct MDWork {
   uint32_t* candOff;  // [nLower + 1] first candidate of each module, then the total
   uint32_t* resIdx;   // [kMDCandMax] park slot of a passing candidate, ~0: failed
   uint32_t* park;     // [kMDParkWords * kMDParkMax]
+  uint4* cand;        // [kMDCandMax] candidate L: (lower hit, upper hit, module, 0)
 };
 
 // Work arrays of the flattened line-segment builder: candidates of module m
@@ -118,6 +119,7 @@ struct LSWork {
   uint32_t* candOff;  // [nLower + 1]
   uint32_t* resIdx;   // [kLSCandMax]
   uint32_t* park;     // [kLSParkWords * kLSParkMax] SegOut of passing candidates
+  uint4* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD, module, outer module)
 };
 
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).
@@ -130,6 +132,11 @@ struct T3Work {
   uint32_t* resIdx;   // [kT3CandMax]
   uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check, any order
   uint32_t* results;  // [kT3ResultWords * kT3ResultMax]
+  uint32_t* survPre;  // [segment capacity] per inner segment: shared-MD candidates before it within its module
+  uint32_t* survOff;  // [nLower + 1] per lower module: shared-MD candidate count, then offsets
+  uint32_t* segByMD;  // [segment capacity] each module's segments grouped by inner mini-doublet
+  uint32_t* mdSegOff; // [MD capacity] per mini-doublet: first position of its segments in segByMD
+  uint32_t* mdSegEnd; // [MD capacity] per mini-doublet: end of its segments in segByMD
 };
 
 // Work arrays of the flattened T5 builder: every (inner T3, outer T3)

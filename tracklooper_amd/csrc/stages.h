@@ -71,7 +71,7 @@ enum TimerId : int {
   kTmCrossPLS,          // k_cross_pls_pairs
   kTmPlsDup2,           // k_pls_pairs<true> (second pass)
   kTmT5DupAfter,        // k_t5_dup_after_build
-  kTmT3Filter,          // k_t3_filter_md
+  kTmT3Filter,          // k_t3_enum (triplet candidates sharing the middle mini-doublet)
   kTmT5Dnn,             // k_t5_dnn (DNN mode only)
   kNTimers
 };

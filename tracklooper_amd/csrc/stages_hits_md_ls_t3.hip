@@ -260,25 +260,33 @@ __device__ __forceinline__ bool md_flat_done(const EventView& v) {
   return md_flat_mode(v) && v.R.scalars[kMDParked] <= kMDParkMax;
 }
 
+__device__ __forceinline__ uint32_t md_wave_index() {
+  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
+// the candidates written out, L by L (one wave per module)
+__global__ __launch_bounds__(256) void k_md_cand_fill(EventView v) {
+  if (!md_flat_mode(v)) return;
+  const unsigned lane = lane_id();
+  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+    const uint32_t n = v.W2.candOff[m + 1] - v.W2.candOff[m];
+    if (n == 0) continue;
+    const uint32_t off = v.W2.candOff[m], nUp = (uint32_t)v.R.hitRangesnUpper[m];
+    const uint32_t lo = (uint32_t)v.R.hitRangesLower[m], up = (uint32_t)v.R.hitRangesUpper[m];
+    for (uint32_t l = lane; l < n; l += kWave) v.W2.cand[off + l] = make_uint4(lo + l / nUp, up + l % nUp, m, 0u);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_md_eval(EventView v) {
   if (!md_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kMDCandTotal];
-  const uint32_t nL = v.M.nLowerModules;
   for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
     const uint32_t L = L0 + threadIdx.x;
     bool pass = false;
     MDOut o;
     if (L < total) {
-      uint32_t lo = 0, hi = nL;  // largest m with candOff[m] <= L
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (v.W2.candOff[mid] <= L) lo = mid;
-        else hi = mid;
-      }
-      const uint32_t m = lo, l = L - v.W2.candOff[m];
-      const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m];
-      const uint32_t li = l / nUp, ui = l - li * nUp;
-      const uint32_t a = (uint32_t)v.R.hitRangesLower[m] + li, b = (uint32_t)v.R.hitRangesUpper[m] + ui;
+      const uint4 cd = v.W2.cand[L];
+      const uint32_t a = cd.x, b = cd.y, m = cd.z;
       const ModCtx c = load_mod(v.M, m);
       const float xL = v.H.xs[a], yL = v.H.ys[a], zL = v.H.zs[a], rtL = v.H.rts[a];
       const float xU = v.H.xs[b], yU = v.H.ys[b], zU = v.H.zs[b], rtU = v.H.rts[b];
@@ -303,9 +311,6 @@ __global__ __launch_bounds__(256) void k_md_eval(EventView v) {
   }
 }
 
-__device__ __forceinline__ uint32_t md_wave_index() {
-  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-}
 
 // slots: passing pairs in L order (pass-flag scan), then per-module counts
 __global__ __launch_bounds__(1024) void k_md_scan_blocks(EventView v) {
@@ -334,17 +339,15 @@ __global__ __launch_bounds__(256) void k_md_place(EventView v) {
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
     const uint32_t k = v.W2.resIdx[L];
     if (k == ~0u) continue;
-    const uint32_t m = module_of(v.W2.candOff, nL, L);
+    const uint4 cd = v.W2.cand[L];
+    const uint32_t m = cd.z;
     const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W2.candOff[m], total);
     if ((int)rank >= v.mdModuleOccupancy[m]) continue;
     const ModCtx c = load_mod(v.M, m);
     const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
-    const uint32_t nUp = (uint32_t)v.R.hitRangesnUpper[m], l = L - v.W2.candOff[m];
-    const uint32_t li = l / nUp, ui = l - li * nUp;
     const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
     const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
-    store_md(v, (uint32_t)v.R.hitRangesLower[m] + li, (uint32_t)v.R.hitRangesUpper[m] + ui, m, anchorIsUpper, o,
-             v.mdModuleIndices[m] + rank);
+    store_md(v, cd.x, cd.y, m, anchorIsUpper, o, v.mdModuleIndices[m] + rank);
   }
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
     const uint32_t count = flag_rank(v.FS, v.W2.candOff[m + 1], total) - flag_rank(v.FS, v.W2.candOff[m], total);
@@ -466,43 +469,48 @@ __device__ __forceinline__ bool ls_flat_done(const EventView& v) {
   return ls_flat_mode(v) && v.R.scalars[kLSParked] <= kLSParkMax;
 }
 
-// module-local candidate l -> (outer module, inner MD, outer MD)
-__device__ __forceinline__ void ls_decode(const EventView& v, uint32_t m, uint32_t l, uint16_t& om, uint32_t& inMD,
-                                          uint32_t& outMD) {
+
+// the candidates written out, L by L (one wave per module; the connected
+// modules' values are loaded by the lanes at once, then walked with shuffles)
+__global__ __launch_bounds__(256) void k_ls_cand_fill(EventView v) {
+  if (!ls_flat_mode(v)) return;
+  const unsigned lane = lane_id();
   const uint32_t S = v.M.stride + 1;
-  const uint32_t* pre = &v.WL.connPre[(size_t)m * S];
-  uint32_t k = 0, hiK = v.M.nConnectedModules[m];  // largest k with pre[k] <= l
-  while (hiK - k > 1) {
-    const uint32_t mid = (k + hiK) >> 1;
-    if (pre[mid] <= l) k = mid;
-    else hiK = mid;
+  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+    const uint32_t nIn = v.R.nMDs[m];
+    const uint32_t nConn = nIn != 0 ? v.M.nConnectedModules[m] : 0;
+    if (nConn == 0) continue;
+    const uint32_t base = v.WL.candOff[m], inBase = (uint32_t)v.R.mdRanges[2 * m];
+    for (uint32_t k0 = 0; k0 < nConn; k0 += kWave) {
+      uint32_t om = 0, nOut = 0, outBase = 0, off = 0;
+      if (k0 + lane < nConn) {
+        om = v.M.moduleMap[(size_t)m * v.M.stride + k0 + lane];
+        nOut = v.R.nMDs[om];
+        outBase = nOut ? (uint32_t)v.R.mdRanges[2 * om] : 0u;
+        off = base + v.WL.connPre[(size_t)m * S + k0 + lane];
+      }
+      const uint32_t nk = nConn - k0 < (uint32_t)kWave ? nConn - k0 : (uint32_t)kWave;
+      for (uint32_t kk = 0; kk < nk; kk++) {
+        const uint32_t gOm = __shfl(om, kk, kWave), gOut = __shfl(nOut, kk, kWave);
+        const uint32_t gBase = __shfl(outBase, kk, kWave), gOff = __shfl(off, kk, kWave);
+        const uint32_t n = nIn * gOut;
+        for (uint32_t l = lane; l < n; l += kWave)
+          v.WL.cand[gOff + l] = make_uint4(inBase + l / gOut, gBase + l % gOut, m, gOm);
+      }
+    }
   }
-  om = v.M.moduleMap[(size_t)m * v.M.stride + k];
-  const uint32_t local = l - pre[k];
-  const uint32_t nOut = v.R.nMDs[om];
-  inMD = v.R.mdRanges[2 * m] + local / nOut;
-  outMD = v.R.mdRanges[2 * om] + local % nOut;
 }
 
 __global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
   if (!ls_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kLSCandTotal];
-  const uint32_t nL = v.M.nLowerModules;
   for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
     const uint32_t L = L0 + threadIdx.x;
     bool pass = false;
     SegOut o{0, 0, 0, 0, 0, 0};
     if (L < total) {
-      uint32_t lo = 0, hi = nL;  // largest m with candOff[m] <= L
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (v.WL.candOff[mid] <= L) lo = mid;
-        else hi = mid;
-      }
-      const uint32_t m = lo;
-      uint16_t om;
-      uint32_t inMD, outMD;
-      ls_decode(v, m, L - v.WL.candOff[m], om, inMD, outMD);
+      const uint4 c = v.WL.cand[L];
+      const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
       const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
       pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, o)
@@ -542,12 +550,10 @@ __global__ __launch_bounds__(256) void k_ls_place(EventView v) {
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
     const uint32_t k = v.WL.resIdx[L];
     if (k == ~0u) continue;
-    const uint32_t m = module_of(v.WL.candOff, nL, L);
+    const uint4 c = v.WL.cand[L];
+    const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
     const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.WL.candOff[m], total);
     if ((int)rank >= v.segModuleOccupancy[m]) continue;
-    uint16_t om;
-    uint32_t inMD, outMD;
-    ls_decode(v, m, L - v.WL.candOff[m], om, inMD, outMD);
     const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
     const uint32_t s = v.segModuleIndices[m] + rank;
     v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
@@ -737,81 +743,129 @@ __device__ __forceinline__ uint32_t wave_index() {
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / kWave; }
 constexpr unsigned kModuleWaveBlocks = 1024;  // x 4 waves
 
+// per inner segment: its candidates (the outer module's segments, which
+// number them in serial order) and its shared-MD candidates (the segments
+// listed under its outer mini-doublet); per module, both totals
 __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
   const unsigned lane = lane_id();
   for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
     const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
     const uint32_t segBase = v.segModuleIndices[m1];
-    uint32_t run = 0;
+    uint32_t run = 0, srun = 0;
     for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
       const uint32_t a = a0 + lane;
-      const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
-      const unsigned incl = wave_inclusive_scan(cnt);
-      if (a < nIn) v.W3.candPre[segBase + a] = run + incl - cnt;
+      uint32_t cnt = 0, sc = 0;
+      if (a < nIn) {
+        const uint32_t inSeg = segBase + a, X = v.S.mdIndices[2 * inSeg + 1];
+        cnt = v.R.nSegments[v.S.outerLowerModuleIndices[inSeg]];
+        sc = v.W3.mdSegEnd[X] - v.W3.mdSegOff[X];
+      }
+      const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
+      if (a < nIn) {
+        v.W3.candPre[segBase + a] = run + incl - cnt;
+        v.W3.survPre[segBase + a] = srun + sincl - sc;
+      }
       run += __shfl(incl, kWave - 1, kWave);
+      srun += __shfl(sincl, kWave - 1, kWave);
     }
-    if (lane == 0) v.W3.modOff[m1] = run;
+    if (lane == 0) {
+      v.W3.modOff[m1] = run;
+      v.W3.survOff[m1] = srun;
+    }
   }
 }
 
 __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
   __shared__ unsigned lds[17];
   const uint32_t n = v.M.nLowerModules;
-  unsigned carry = 0;
+  unsigned carry = 0, scarry = 0;
   for (uint32_t b = 0; b < n; b += 1024) {
     const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? v.W3.modOff[i] : 0u;
-    unsigned tot;
+    const unsigned c = i < n ? v.W3.modOff[i] : 0u, sc = i < n ? v.W3.survOff[i] : 0u;
+    unsigned tot, stot;
     const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (i < n) v.W3.modOff[i] = carry + ex;
+    const unsigned sex = block_exclusive_scan<1024>(sc, lds, stot);
+    if (i < n) {
+      v.W3.modOff[i] = carry + ex;
+      v.W3.survOff[i] = scarry + sex;
+    }
     carry += tot;
+    scarry += stot;
   }
   if (threadIdx.x == 0) {
     v.W3.modOff[n] = carry;
+    v.W3.survOff[n] = scarry;
     v.R.scalars[kT3CandTotal] = carry;
-    v.R.scalars[kT3List] = 0;
+    v.R.scalars[kT3List] = scarry;
     v.R.scalars[kT3Results] = 0;
   }
 }
 
-// stage 1: shared middle mini-doublet over all candidates (each candidate
-// decoded by binary searches over the module offsets and the module's
-// per-inner-segment prefix); survivors appended to a list, order free
-__global__ __launch_bounds__(256) void k_t3_filter_md(EventView v) {
+// stage 1: the candidates whose inner segment's outer mini-doublet X is the
+// outer segment's inner one.  Per mini-doublet, the segments starting at it
+// are listed first (one wave per lower module: counts, offsets and a scatter
+// inside the module's own segment range, order free); the candidate count
+// pass also counts, per inner segment, the segments listed under X; then one
+// wave per module m1 walks its inner segments and writes, for each segment
+// listed under X, the candidate's serial number L and offsets at the
+// positions those counts give (list order free: L carries the serial order).
+__global__ __launch_bounds__(kWave) void k_md_seg_lists(EventView v) {
+  const unsigned lane = threadIdx.x;
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t m = blockIdx.x; m < nL; m += gridDim.x) {
+    const uint32_t nMD = v.R.nMDs[m];
+    if (nMD == 0) continue;
+    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], nS = v.R.nSegments[m];
+    const uint32_t segBase = (uint32_t)v.segModuleIndices[m];
+    __syncthreads();
+    for (uint32_t k = lane; k < nMD; k += kWave) v.W3.mdSegEnd[mdBase + k] = 0;
+    __syncthreads();
+    for (uint32_t k = lane; k < nS; k += kWave) atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * (segBase + k)]], 1u);
+    __syncthreads();
+    uint32_t run = segBase;
+    for (uint32_t k0 = 0; k0 < nMD; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const uint32_t c = k < nMD ? v.W3.mdSegEnd[mdBase + k] : 0u;
+      const uint32_t incl = wave_inclusive_scan(c);
+      if (k < nMD) {
+        v.W3.mdSegOff[mdBase + k] = run + incl - c;
+        v.W3.mdSegEnd[mdBase + k] = run + incl - c;  // cursor of the scatter, the end afterwards
+      }
+      run += __shfl(incl, kWave - 1, kWave);
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < nS; k += kWave) {
+      const uint32_t s = segBase + k;
+      v.W3.segByMD[atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * s]], 1u)] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_t3_res_init(EventView v) {
   if (!t3_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kT3CandTotal];
-  const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
-    const uint32_t L = L0 + threadIdx.x;
-    if (L >= total) continue;
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
     v.W3.resIdx[L] = ~0u;
-    uint32_t lo = 0, hi = nL;  // largest m with modOff[m] <= L
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (v.W3.modOff[mid] <= L) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t m1 = lo, l = L - v.W3.modOff[lo];
-    const uint32_t segBase = v.segModuleIndices[m1];
-    uint32_t a0 = 0, a1 = v.R.nSegments[m1];  // largest a with candPre[a] <= l
-    while (a1 - a0 > 1) {
-      const uint32_t mid = (a0 + a1) >> 1;
-      if (v.W3.candPre[segBase + mid] <= l) a0 = mid;
-      else a1 = mid;
-    }
-    const uint32_t inSeg = segBase + a0;
-    const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg];
-    const uint32_t outSeg = v.segModuleIndices[m2] + (l - v.W3.candPre[inSeg]);
-    if (v.S.mdIndices[2 * inSeg + 1] == v.S.mdIndices[2 * outSeg]) {
-      const uint64_t mask = __ballot(1);
-      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT3List], (uint32_t)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      const uint32_t k = base + lanes_below(mask);
-      v.W3.list[3 * k] = L;
-      v.W3.list[3 * k + 1] = m1;
-      v.W3.list[3 * k + 2] = (inSeg - segBase) | ((l - v.W3.candPre[inSeg]) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_t3_enum(EventView v) {
+  if (!t3_flat_mode(v)) return;
+  const unsigned lane = lane_id();
+  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
+    const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+    const uint32_t segBase = v.segModuleIndices[m1], mOff = v.W3.modOff[m1], sOff = v.W3.survOff[m1];
+    for (uint32_t a = lane; a < nIn; a += kWave) {
+      const uint32_t inSeg = segBase + a;
+      const uint32_t X = v.S.mdIndices[2 * inSeg + 1];
+      const uint32_t b = v.W3.mdSegOff[X], e = v.W3.mdSegEnd[X];
+      const uint32_t Lbase = mOff + v.W3.candPre[inSeg], k0 = sOff + v.W3.survPre[inSeg];
+      const uint32_t seg2Base = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]];
+      for (uint32_t q = b; q < e; q++) {
+        const uint32_t j = v.W3.segByMD[q] - seg2Base, k = k0 + (q - b);
+        v.W3.list[3 * k] = Lbase + j;
+        v.W3.list[3 * k + 1] = m1;
+        v.W3.list[3 * k + 2] = a | (j << 16);
+      }
     }
   }
 }
@@ -1021,6 +1075,7 @@ void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const 
 
 void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_md_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_md_cand_fill, dim3(1024), dim3(256), 0, s, v);
   t.begin(s, kTmMiniDoublets);
   hipLaunchKernelGGL(k_md_eval, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmMiniDoublets);
@@ -1034,6 +1089,7 @@ void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers&
 void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_ls_cand_count, dim3(1024), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_ls_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_ls_cand_fill, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.begin(s, kTmSegments);
   hipLaunchKernelGGL(k_ls_eval, dim3(4096), dim3(256), 0, s, v);
   t.end(s, kTmSegments);
@@ -1046,10 +1102,12 @@ void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
 
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_md_seg_lists, dim3(4096), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_t3_res_init, dim3(1024), dim3(256), 0, s, v);
   t.begin(s, kTmT3Filter);
-  hipLaunchKernelGGL(k_t3_filter_md, dim3(2048), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t3_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.end(s, kTmT3Filter);
   t.begin(s, kTmT3Select);
   hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
