@@ -212,6 +212,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
   c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
   c.take(e.W.pt3Mask, lst::kPixMaskWords); c.take(e.W.pt3MaskBase, plsCap);
+  c.take(e.W.pt5Shard, 2 * lst::kPixShards); c.take(e.W.pt3Shard, 2 * lst::kPixShards);
   c.take(e.W.pixWords, plsCap); c.take(e.W.pixFull, plsCap);
   c.take(e.W.pixSurv, (size_t)lst::kSurvWords * lst::kPixSurvMax);
   c.take(e.W.pixParkIdx, (size_t)64 * lst::kPixMaskWords);
@@ -221,18 +222,20 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5BucketOff, lst::kPT5HashBuckets + 1); c.take(e.W.pt5BucketEnt, 14 * (size_t)pt5Cap);
   c.take(e.FS.pre, lst::kScanCandMax); c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
   c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax);
-  c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDParkMax); c.take(e.W2.cand, lst::kMDCandMax);
+  c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDCandMax); c.take(e.W2.cand, lst::kMDCandMax);
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
-  c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSParkMax);
+  c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
   c.take(e.WL.cand, lst::kLSCandMax);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
   c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
-  c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3ResultMax);
+  c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3CandMax);
   c.take(e.W3.survPre, segCap); c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
   c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
   c.take(e.W5.resIdx, lst::kT5CandMax);
   c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
   c.take(e.W5.results, (size_t)lst::kT5ResultWords * lst::kT5ResultMax);
+  c.take(e.W5.survPre, t3Cap ? t3Cap : 1); c.take(e.W5.survOff, nL + 2); c.take(e.W5.t3ByMD, t3Cap ? t3Cap : 1);
+  c.take(e.W5.mdT3Off, mdCap); c.take(e.W5.mdT3End, mdCap);
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);
   c.take(e.R.mdRanges, 2 * nMod); c.take(e.R.segmentRanges, 2 * nMod); c.take(e.R.tripletRanges, 2 * nMod);

@@ -98,40 +98,37 @@ struct FlatScan {
 // hit) pair of every module gets a global index L (module, then lower hit,
 // then upper hit: the reference's serial order); passing pairs park their
 // MDOut, and a wave per module places them in L order.
-constexpr uint32_t kMDCandMax = 1u << 22;
-constexpr uint32_t kMDParkMax = 1u << 20;
+constexpr uint32_t kMDCandMax = 1u << 21;
 constexpr int kMDParkWords = 9;
 struct MDWork {
   uint32_t* candOff;  // [nLower + 1] first candidate of each module, then the total
   uint32_t* resIdx;   // [kMDCandMax] park slot of a passing candidate, ~0: failed
-  uint32_t* park;     // [kMDParkWords * kMDParkMax]
+  uint32_t* park;     // [kMDParkWords * kMDCandMax] MDOut of passing candidate L at L
   uint4* cand;        // [kMDCandMax] candidate L: (lower hit, upper hit, module, 0)
 };
 
 // Work arrays of the flattened line-segment builder: candidates of module m
 // are its (connected module k, inner MD, outer MD) triples in that order;
 // connPre holds, per module, the first candidate of each connected module.
-constexpr uint32_t kLSCandMax = 1u << 23;
-constexpr uint32_t kLSParkMax = 1u << 21;
+constexpr uint32_t kLSCandMax = 1u << 22;
 constexpr int kLSParkWords = 6;
 struct LSWork {
   uint32_t* connPre;  // [nLower * (stride + 1)] per module: candidate prefix over its connected modules
   uint32_t* candOff;  // [nLower + 1]
   uint32_t* resIdx;   // [kLSCandMax]
-  uint32_t* park;     // [kLSParkWords * kLSParkMax] SegOut of passing candidates
+  uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
   uint4* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD, module, outer module)
 };
 
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).
 constexpr uint32_t kT3CandMax = 1u << 22;
-constexpr uint32_t kT3ResultMax = 1u << 20;
 constexpr int kT3ResultWords = 6;
 struct T3Work {
   uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
   uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
   uint32_t* resIdx;   // [kT3CandMax]
   uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check, any order
-  uint32_t* results;  // [kT3ResultWords * kT3ResultMax]
+  uint32_t* results;  // [kT3ResultWords * kT3CandMax] selection results of list entry i at i
   uint32_t* survPre;  // [segment capacity] per inner segment: shared-MD candidates before it within its module
   uint32_t* survOff;  // [nLower + 1] per lower module: shared-MD candidate count, then offsets
   uint32_t* segByMD;  // [segment capacity] each module's segments grouped by inner mini-doublet
@@ -151,6 +148,11 @@ struct T5Work {
   uint32_t* listA;    // [3 * kT5CandMax] (L, inner T3, outer T3) passing the shared-MD check
   uint32_t* listB;    // [3 * kT5CandMax] ... and both T4-style tests
   uint32_t* results;  // [kT5ResultWords * kT5ResultMax] selection results of passing candidates
+  uint32_t* survPre;  // [T3 capacity] per inner T3: shared-MD candidates before it within its module
+  uint32_t* survOff;  // [nLower + 2] per eligible-list position: shared-MD candidate count, then offsets
+  uint32_t* t3ByMD;   // [T3 capacity] each module's T3s grouped by first mini-doublet
+  uint32_t* mdT3Off;  // [MD capacity] per mini-doublet: first position of the T3s starting at it in t3ByMD
+  uint32_t* mdT3End;  // [MD capacity] per mini-doublet: end of those T3s in t3ByMD
 };
 constexpr int kT5ResultWords = 12;
 
@@ -177,6 +179,8 @@ struct PixelWork {
   uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none
   unsigned long long* pt3Mask;  // [kPixMaskWords] (pT3)
   uint32_t* pt3MaskBase;        // [pLS capacity]
+  uint32_t* pt5Shard;           // [2 * kPixShards] per shard: mask words taken, survivors taken (pT5)
+  uint32_t* pt3Shard;           // [2 * kPixShards] (pT3)
   uint32_t* pixWords;           // [pLS capacity] mask words of the pLS (the builder running)
   uint8_t* pixFull;             // [pLS capacity] 1: the pLS's survivors did not all fit, evaluated in full
   uint32_t* pixSurv;            // [kSurvWords * kPixSurvMax] survivors of the cheap cuts: pLS, mask
@@ -193,6 +197,12 @@ constexpr uint32_t kPT5HashBuckets = 1u << 15;
 constexpr uint32_t kPixMaskWords = 1u << 16;  // 4 Mi candidates per builder and event; beyond: evaluated in full
 constexpr uint32_t kPixSurvMax = 1u << 18;    // survivors of the cheap cuts per builder; beyond: evaluated in full
 constexpr uint32_t kSurvWords = 10;           // pLS, mask position, object, up to 7 result words
+// pLS i takes its mask words and survivor records from shard i % kPixShards,
+// each shard owning an equal slice of both arrays, so that no counter is
+// shared by more than a few dozen waves
+constexpr uint32_t kPixShards = 64;
+constexpr uint32_t kShardWords = kPixMaskWords / kPixShards;
+constexpr uint32_t kShardSurv = kPixSurvMax / kPixShards;
 constexpr int kT5RecWords = 32;               // per-T5 record of the pixel-quintuplet selection
 constexpr int kT3RecWords = 16;               // per-T3 record of the pixel-triplet selection
 constexpr int kEtaBins = 64;                    // finite eta bins of width 1/8 over [-4, 4)
@@ -245,16 +255,11 @@ enum Scalar : int {
   kT5Results = 16,
   kT3CandTotal = 17,
   kT3List = 18,
-  kT3Results = 19,
   kT5Listed = 20,
-  kPT5MaskWords = 21,
-  kPT3MaskWords = 22,
   kPT5Survivors = 23,
   kPT3Survivors = 24,
   kMDCandTotal = 25,
-  kMDParked = 26,
   kLSCandTotal = 27,
-  kLSParked = 28,
   kEvNHits = 29,  // hits of the event (set before the hit kernels run)
   kEvNPLS = 30,   // pixel line segments of the event
   kPixFull = 31,  // bit 0: some pLS evaluated in full by the pT5 builder, bit 1: by the pT3 builder

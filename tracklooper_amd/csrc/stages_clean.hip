@@ -431,17 +431,10 @@ __global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(EventView v) {
 // --------------------------------------------------- T5 list in slot order
 __global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
   __shared__ unsigned lds[17];
-  const uint32_t n = v.R.scalars[kNEligibleT5];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? v.R.nQuintuplets[v.R.indicesOfEligibleT5Modules[i]] : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (i < n) v.W.t5ListOff[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) v.R.scalars[kT5Listed] = carry;
+  const unsigned total = scan_wide<kScanPer>(
+      v.R.scalars[kNEligibleT5], [&](uint32_t i) { return v.R.nQuintuplets[v.R.indicesOfEligibleT5Modules[i]]; },
+      [&](uint32_t i, unsigned p, unsigned) { v.W.t5ListOff[i] = p; }, lds);
+  if (threadIdx.x == 0) v.R.scalars[kT5Listed] = total;
 }
 
 __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {

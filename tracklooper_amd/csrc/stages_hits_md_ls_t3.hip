@@ -228,28 +228,23 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
 
 // --------------------------------------------------------------- mini-doublets
 // Flattened builder (the path every event takes unless it has more than
-// kMDCandMax hit pairs or kMDParkMax passing ones): candidates of module m are
-// [candOff[m], candOff[m+1]), pair (lower li, upper ui) at li * nUpper + ui.
+// kMDCandMax hit pairs): candidates of module m are [candOff[m], candOff[m+1]),
+// pair (lower li, upper ui) at li * nUpper + ui.  A passing candidate parks its
+// results at its own index L, so no slot is taken with an atomic.
 __global__ __launch_bounds__(1024) void k_md_cand_scan(EventView v) {
   __shared__ unsigned lds[17];
   const uint32_t nL = v.M.nLowerModules;
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < nL; b += 1024) {
-    const uint32_t m = b + threadIdx.x;
-    unsigned c = 0;
-    if (m < nL && v.R.hitRangesLower[m] != -1) {  // the counts are int8 as in the reference (<= 127 hits)
-      const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
-      c = nLo > 0 && nUp > 0 ? (unsigned)(nLo * nUp) : 0u;
-    }
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (m < nL) v.W2.candOff[m] = carry + ex;
-    carry += tot;
-  }
+  const unsigned total = scan_wide<kScanPer>(
+      nL,
+      [&](uint32_t m) {  // the counts are int8 as in the reference (<= 127 hits)
+        if (v.R.hitRangesLower[m] == -1) return 0u;
+        const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+        return nLo > 0 && nUp > 0 ? (unsigned)(nLo * nUp) : 0u;
+      },
+      [&](uint32_t m, unsigned p, unsigned) { v.W2.candOff[m] = p; }, lds);
   if (threadIdx.x == 0) {
-    v.W2.candOff[nL] = carry;
-    v.R.scalars[kMDCandTotal] = carry;
-    v.R.scalars[kMDParked] = 0;
+    v.W2.candOff[nL] = total;
+    v.R.scalars[kMDCandTotal] = total;
   }
 }
 
@@ -257,7 +252,7 @@ __device__ __forceinline__ bool md_flat_mode(const EventView& v) {
   return !v.perModuleBuilders && v.R.scalars[kMDCandTotal] <= kMDCandMax;
 }
 __device__ __forceinline__ bool md_flat_done(const EventView& v) {
-  return md_flat_mode(v) && v.R.scalars[kMDParked] <= kMDParkMax;
+  return md_flat_mode(v);
 }
 
 __device__ __forceinline__ uint32_t md_wave_index() {
@@ -294,19 +289,11 @@ __global__ __launch_bounds__(256) void k_md_eval(EventView v) {
                                  : md_endcap(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, o);
       v.W2.resIdx[L] = ~0u;
     }
-    if (pass) {  // wave-aggregated park
-      const uint64_t mask = __ballot(1);
-      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kMDParked], (uint32_t)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      const uint32_t k = base + lanes_below(mask);
-      if (k < kMDParkMax) {
-        float* w = reinterpret_cast<float*>(&v.W2.park[kMDParkWords * k]);
-        w[0] = o.dz; w[1] = o.dphi; w[2] = o.dphichange; w[3] = o.sx; w[4] = o.sy; w[5] = o.sz;
-        w[6] = o.nsdz; w[7] = o.nsdphi; w[8] = o.nsdphichange;
-        v.W2.resIdx[L] = k;
-      }
+    if (pass) {  // parked at the candidate's own index
+      float* w = reinterpret_cast<float*>(&v.W2.park[kMDParkWords * (size_t)L]);
+      w[0] = o.dz; w[1] = o.dphi; w[2] = o.dphichange; w[3] = o.sx; w[4] = o.sy; w[5] = o.sz;
+      w[6] = o.nsdz; w[7] = o.nsdphi; w[8] = o.nsdphichange;
+      v.W2.resIdx[L] = L;
     }
   }
 }
@@ -446,19 +433,12 @@ __global__ __launch_bounds__(256) void k_ls_cand_count(EventView v) {
 __global__ __launch_bounds__(1024) void k_ls_cand_scan(EventView v) {
   __shared__ unsigned lds[17];
   const uint32_t nL = v.M.nLowerModules;
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < nL; b += 1024) {
-    const uint32_t m = b + threadIdx.x;
-    const unsigned c = m < nL ? v.WL.candOff[m] : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (m < nL) v.WL.candOff[m] = carry + ex;
-    carry += tot;
-  }
+  const unsigned total = scan_wide<kScanPer>(
+      nL, [&](uint32_t m) { return v.WL.candOff[m]; }, [&](uint32_t m, unsigned p, unsigned) { v.WL.candOff[m] = p; },
+      lds);
   if (threadIdx.x == 0) {
-    v.WL.candOff[nL] = carry;
-    v.R.scalars[kLSCandTotal] = carry;
-    v.R.scalars[kLSParked] = 0;
+    v.WL.candOff[nL] = total;
+    v.R.scalars[kLSCandTotal] = total;
   }
 }
 
@@ -466,7 +446,7 @@ __device__ __forceinline__ bool ls_flat_mode(const EventView& v) {
   return !v.perModuleBuilders && v.R.scalars[kLSCandTotal] <= kLSCandMax;
 }
 __device__ __forceinline__ bool ls_flat_done(const EventView& v) {
-  return ls_flat_mode(v) && v.R.scalars[kLSParked] <= kLSParkMax;
+  return ls_flat_mode(v);
 }
 
 
@@ -517,19 +497,11 @@ __global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
                                                             : seg_endcap(ci, co, v.ptCut, a, b, o);
       v.WL.resIdx[L] = ~0u;
     }
-    if (pass) {  // wave-aggregated park
-      const uint64_t mask = __ballot(1);
-      const unsigned leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kLSParked], (uint32_t)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      const uint32_t k = base + lanes_below(mask);
-      if (k < kLSParkMax) {
-        float* w = reinterpret_cast<float*>(&v.WL.park[kLSParkWords * k]);
-        w[0] = o.dPhi; w[1] = o.dPhiMin; w[2] = o.dPhiMax;
-        w[3] = o.dPhiChange; w[4] = o.dPhiChangeMin; w[5] = o.dPhiChangeMax;
-        v.WL.resIdx[L] = k;
-      }
+    if (pass) {  // parked at the candidate's own index
+      float* w = reinterpret_cast<float*>(&v.WL.park[kLSParkWords * (size_t)L]);
+      w[0] = o.dPhi; w[1] = o.dPhiMin; w[2] = o.dPhiMax;
+      w[3] = o.dPhiChange; w[4] = o.dPhiChangeMin; w[5] = o.dPhiChangeMax;
+      v.WL.resIdx[L] = L;
     }
   }
 }
@@ -660,20 +632,14 @@ __global__ void k_segment_ranges(EventView v) {
 // (SDL/Triplet.h:1007-1102); exclusive scan in module order.
 __global__ __launch_bounds__(1024) void k_triplet_offsets(EventView v) {
   __shared__ unsigned lds[17];
-  unsigned carry = 0;
-  const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t b = 0; b < nL; b += 1024) {
-    const uint32_t m = b + threadIdx.x;
-    const int occ = (m < nL && v.R.nSegments[m] != 0) ? v.t3OccIfAny[m] : 0;
-    unsigned total;
-    const unsigned ex = block_exclusive_scan<1024>((unsigned)occ, lds, total);
-    if (m < nL) {
-      v.R.tripletModuleOccupancy[m] = occ;
-      v.R.tripletModuleIndices[m] = carry + ex;
-    }
-    carry += total;
-  }
-  if (threadIdx.x == 0) v.R.scalars[kNTotalTrips] = carry;
+  const unsigned total = scan_wide<kScanPer>(
+      v.M.nLowerModules, [&](uint32_t m) { return v.R.nSegments[m] != 0 ? (unsigned)v.t3OccIfAny[m] : 0u; },
+      [&](uint32_t m, unsigned p, unsigned occ) {
+        v.R.tripletModuleOccupancy[m] = (int)occ;
+        v.R.tripletModuleIndices[m] = (int)p;
+      },
+      lds);
+  if (threadIdx.x == 0) v.R.scalars[kNTotalTrips] = total;
 }
 
 // runTripletConstraintsAndAlgo (SDL/Triplet.h:796-872) for the segment pair
@@ -731,7 +697,7 @@ __device__ __forceinline__ bool t3_flat_mode(const EventView& v) {
   return !v.perModuleBuilders && v.R.scalars[kT3CandTotal] <= kT3CandMax;
 }
 __device__ __forceinline__ bool t3_flat_done(const EventView& v) {
-  return t3_flat_mode(v) && v.R.scalars[kT3Results] <= kT3ResultMax;
+  return t3_flat_mode(v);
 }
 
 // The flattened T3 kernels below run one wave per lower module at a time and
@@ -778,26 +744,17 @@ __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
 __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
   __shared__ unsigned lds[17];
   const uint32_t n = v.M.nLowerModules;
-  unsigned carry = 0, scarry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? v.W3.modOff[i] : 0u, sc = i < n ? v.W3.survOff[i] : 0u;
-    unsigned tot, stot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    const unsigned sex = block_exclusive_scan<1024>(sc, lds, stot);
-    if (i < n) {
-      v.W3.modOff[i] = carry + ex;
-      v.W3.survOff[i] = scarry + sex;
-    }
-    carry += tot;
-    scarry += stot;
-  }
+  const unsigned total = scan_wide<kScanPer>(
+      n, [&](uint32_t m) { return v.W3.modOff[m]; }, [&](uint32_t m, unsigned p, unsigned) { v.W3.modOff[m] = p; },
+      lds);
+  const unsigned surv = scan_wide<kScanPer>(
+      n, [&](uint32_t m) { return v.W3.survOff[m]; }, [&](uint32_t m, unsigned p, unsigned) { v.W3.survOff[m] = p; },
+      lds);
   if (threadIdx.x == 0) {
-    v.W3.modOff[n] = carry;
-    v.W3.survOff[n] = scarry;
-    v.R.scalars[kT3CandTotal] = carry;
-    v.R.scalars[kT3List] = scarry;
-    v.R.scalars[kT3Results] = 0;
+    v.W3.modOff[n] = total;
+    v.W3.survOff[n] = surv;
+    v.R.scalars[kT3CandTotal] = total;
+    v.R.scalars[kT3List] = surv;
   }
 }
 
@@ -882,9 +839,8 @@ __global__ __launch_bounds__(256) void k_t3_select(EventView v) {
     const uint32_t outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (ab >> 16);
     float betaIn = 0, radius = 0, gx = 0, gy = 0;
     if (t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy)) {
-      const uint32_t k = atomicAdd(&v.R.scalars[kT3Results], 1u);
-      if (k >= kT3ResultMax) continue;  // overflow: the per-module builder redoes the stage
-      uint32_t* w = &v.W3.results[kT3ResultWords * k];
+      const uint32_t k = i;  // the list position: no slot taken with an atomic
+      uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)k];
       w[0] = inSeg; w[1] = outSeg;
       w[2] = __float_as_uint(betaIn); w[3] = __float_as_uint(radius);
       w[4] = __float_as_uint(gx); w[5] = __float_as_uint(gy);

@@ -33,29 +33,24 @@ inline unsigned grid_of(uint32_t n, unsigned block) {
 __global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t* counts, const uint32_t* count, uint32_t* total) {
   __shared__ unsigned lds[17];
   const uint32_t n = *count;
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? counts[i] : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (i < n) counts[i] = carry + ex;
-    carry += tot;
-  }
+  const unsigned sum = scan_wide<kScanPer>(
+      n, [&](uint32_t i) { return counts[i]; }, [&](uint32_t i, unsigned p, unsigned) { counts[i] = p; }, lds);
   if (threadIdx.x == 0) {
-    counts[n] = carry;
-    if (total) *total = carry;
+    counts[n] = sum;
+    if (total) *total = sum;
   }
 }
 
 // connected-module range of every pLS in the pixel map (SDL/Event.cc:828-856)
 __global__ void k_pls_connected(EventView v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    v.R.scalars[kPT5MaskWords] = 0;
-    v.R.scalars[kPT3MaskWords] = 0;
     v.R.scalars[kPT5Survivors] = 0;
     v.R.scalars[kPT3Survivors] = 0;
     v.R.scalars[kPixFull] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2 * kPixShards) {
+    v.W.pt5Shard[threadIdx.x] = 0;
+    v.W.pt3Shard[threadIdx.x] = 0;
   }
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= kPT5HashBuckets; b += gridDim.x * blockDim.x)
     v.W.pt5BucketOff[b] = 0;  // hit-bucket counts of the pT5 duplicate search
@@ -133,8 +128,9 @@ __device__ __forceinline__ uint32_t candidate_object(const uint32_t* pre, const 
 struct PT5Kind {
   using Res = PT5Result;
   static constexpr uint32_t kBit = 1, kCap = kNMaxPixelQuintuplets;
-  static constexpr int kMaskWordsScalar = kPT5MaskWords, kSurvScalar = kPT5Survivors;
+  static constexpr int kSurvScalar = kPT5Survivors;
   __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt5Mask; }
+  __device__ static uint32_t* shard(const EventView& v) { return v.W.pt5Shard; }
   __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt5MaskBase; }
   __host__ __device__ static uint32_t* count(const EventView& v) { return v.W.pt5Count; }
   __device__ static bool pls_ok(const EventView& v, uint32_t i) { return v.P.isDup[i] == 0; }
@@ -199,8 +195,9 @@ struct PT5Kind {
 struct PT3Kind {
   using Res = PT3Result;
   static constexpr uint32_t kBit = 2, kCap = kNMaxPixelTriplets;
-  static constexpr int kMaskWordsScalar = kPT3MaskWords, kSurvScalar = kPT3Survivors;
+  static constexpr int kSurvScalar = kPT3Survivors;
   __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt3Mask; }
+  __device__ static uint32_t* shard(const EventView& v) { return v.W.pt3Shard; }
   __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt3MaskBase; }
   __host__ __device__ static uint32_t* count(const EventView& v) { return v.W.pt3Count; }
   __device__ static bool pls_ok(const EventView& v, uint32_t i) { return v.P.isDup[i] == 0 && v.P.partOfPT5[i] == 0; }
@@ -271,17 +268,10 @@ struct PT3Kind {
 // module (offsets: one 1024-lane work-group; fill: one wave per module)
 __global__ __launch_bounds__(1024) void k_t3_list_offsets(EventView v) {
   __shared__ unsigned lds[17];
-  const uint32_t nL = v.M.nLowerModules;
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < nL; b += 1024) {
-    const uint32_t m = b + threadIdx.x;
-    const unsigned c = m < nL && PT3Kind::module_ok(v, m) ? PT3Kind::n_obj(v, m) : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (m < nL) v.W.t3ListOff[m] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) v.R.scalars[kT3Listed] = carry;
+  const unsigned total = scan_wide<kScanPer>(
+      v.M.nLowerModules, [&](uint32_t m) { return PT3Kind::module_ok(v, m) ? PT3Kind::n_obj(v, m) : 0u; },
+      [&](uint32_t m, unsigned p, unsigned) { v.W.t3ListOff[m] = p; }, lds);
+  if (threadIdx.x == 0) v.R.scalars[kT3Listed] = total;
 }
 __global__ __launch_bounds__(256) void k_t3_list_fill(EventView v) {
   const uint32_t nL = v.M.nLowerModules;
@@ -326,9 +316,9 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
       words = (total + kWave - 1) / kWave;
       if (words) {
         uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&v.R.scalars[K::kMaskWordsScalar], words);
+        if (lane == 0) b = atomicAdd(&K::shard(v)[i % kPixShards], words);
         b = __shfl(b, 0, kWave);
-        if (b + words <= kPixMaskWords) maskBase = b;
+        if (b + words <= kShardWords) maskBase = (i % kPixShards) * kShardWords + b;
         else full = true;
       }
     }
@@ -348,17 +338,17 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
         if (sm) {
           const uint32_t n = (uint32_t)__popcll(sm);
           uint32_t sb = 0;
-          if (lane == 0) sb = atomicAdd(&v.R.scalars[K::kSurvScalar], n);
+          if (lane == 0) sb = atomicAdd(&K::shard(v)[kPixShards + i % kPixShards], n);
           sb = __shfl(sb, 0, kWave);
           const uint32_t s = sb + lanes_below(sm);
-          if (surv && s < kPixSurvMax) {
-            uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+          if (surv && s < kShardSurv) {
+            uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * ((i % kPixShards) * kShardSurv + s)];
             rec[0] = i;
             rec[1] = (maskBase + w) * kWave + lane;
             rec[2] = t;
             K::pack(r, rec + 3);
           }
-          if (sb + n > kPixSurvMax) full = true;
+          if (sb + n > kShardSurv) full = true;
         }
       }
     }
@@ -374,9 +364,24 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
 
 template <class K>
 __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
-  const uint32_t nS = v.R.scalars[K::kSurvScalar] < kPixSurvMax ? v.R.scalars[K::kSurvScalar] : kPixSurvMax;
+  __shared__ uint32_t pre[kPixShards + 1];  // survivors before each shard
+  const unsigned lane = threadIdx.x;
+  const uint32_t c = K::shard(v)[kPixShards + lane];
+  const uint32_t incl = wave_inclusive_scan(c < kShardSurv ? c : kShardSurv);
+  pre[lane + 1] = incl;
+  if (lane == 0) pre[0] = 0;
+  __syncthreads();
+  const uint32_t nS = pre[kPixShards];
+  if (blockIdx.x == 0 && lane == 0) v.R.scalars[K::kSurvScalar] = nS;
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nS; s += gridDim.x * blockDim.x) {
+  for (uint32_t g = blockIdx.x * blockDim.x + lane; g < nS; g += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = kPixShards;  // the shard holding survivor g: largest lo with pre[lo] <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t s = lo * kShardSurv + (g - pre[lo]);
     uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
     const uint32_t i = rec[0], slot = rec[1], t = rec[2];
     typename K::Res r;
@@ -617,18 +622,11 @@ __global__ __launch_bounds__(256) void k_tc_t5_count(EventView v) {
 
 __global__ __launch_bounds__(1024) void k_tc_t5_scan(EventView v) {
   __shared__ unsigned lds[17];
-  const uint32_t n = v.R.scalars[kNEligibleT5];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? v.W.t5TcCount[i] : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (i < n) v.W.t5TcCount[i] = carry + ex;
-    carry += tot;
-  }
+  const unsigned total = scan_wide<kScanPer>(
+      v.R.scalars[kNEligibleT5], [&](uint32_t i) { return v.W.t5TcCount[i]; },
+      [&](uint32_t i, unsigned p, unsigned) { v.W.t5TcCount[i] = p; }, lds);
   if (threadIdx.x == 0) {
-    const unsigned added = carry < kNMaxNonPixelTrackCandidates ? carry : kNMaxNonPixelTrackCandidates;
+    const unsigned added = total < kNMaxNonPixelTrackCandidates ? total : kNMaxNonPixelTrackCandidates;
     v.R.scalars[kNTrackCandidatesT5] = added;
     v.R.scalars[kNTrackCandidates] += added;
   }

@@ -29,30 +29,27 @@ constexpr int kAfterBuildSplit = 8;   // partner shares per module in the after-
 // seed modules holding a triplet, their capacities and the exclusive scan of
 // the capacities in module order.
 __global__ __launch_bounds__(1024) void k_t5_eligible(EventView v) {
-  __shared__ unsigned lds[2][17];
-  unsigned carryOcc = 0, carryElig = 0;
+  __shared__ unsigned lds[17];
   const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t b = 0; b < nL; b += 1024) {
-    const uint32_t m = b + threadIdx.x;
-    const int occIf = m < nL ? v.t5OccIfAny[m] : -1;
-    const bool elig = occIf >= 0 && v.R.nTriplets[m] != 0;
-    const unsigned occ = elig ? (unsigned)occIf : 0u;
-    unsigned totOcc, totElig;
-    const unsigned exOcc = block_exclusive_scan<1024>(occ, lds[0], totOcc);
-    const unsigned exElig = block_exclusive_scan<1024>(elig ? 1u : 0u, lds[1], totElig);
-    if (m < nL) {
-      v.R.quintupletModuleIndices[m] = elig ? (int)(carryOcc + exOcc) : -1;
-      v.R.quintupletModuleOccupancy[m] = (int)occ;
-      v.R.nQuintuplets[m] = 0;
-      v.R.totOccupancyQuintuplets[m] = 0;
-      if (elig) v.R.indicesOfEligibleT5Modules[carryElig + exElig] = (uint16_t)m;
-    }
-    carryOcc += totOcc;
-    carryElig += totElig;
-  }
+  const auto elig = [&](uint32_t m) { return v.t5OccIfAny[m] >= 0 && v.R.nTriplets[m] != 0; };
+  const unsigned totOcc = scan_wide<kScanPer>(
+      nL, [&](uint32_t m) { return elig(m) ? (unsigned)v.t5OccIfAny[m] : 0u; },
+      [&](uint32_t m, unsigned p, unsigned occ) {
+        v.R.quintupletModuleIndices[m] = elig(m) ? (int)p : -1;
+        v.R.quintupletModuleOccupancy[m] = (int)occ;
+        v.R.nQuintuplets[m] = 0;
+        v.R.totOccupancyQuintuplets[m] = 0;
+      },
+      lds);
+  const unsigned totElig = scan_wide<kScanPer>(
+      nL, [&](uint32_t m) { return elig(m) ? 1u : 0u; },
+      [&](uint32_t m, unsigned p, unsigned e) {
+        if (e) v.R.indicesOfEligibleT5Modules[p] = (uint16_t)m;
+      },
+      lds);
   if (threadIdx.x == 0) {
-    v.R.scalars[kNEligibleT5] = carryElig;
-    v.R.scalars[kNTotalQuints] = carryOcc;
+    v.R.scalars[kNEligibleT5] = totElig;
+    v.R.scalars[kNTotalQuints] = totOcc;
   }
 }
 
@@ -336,12 +333,57 @@ __device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_
 // module at a time and stride over the modules (a fixed grid of
 // kModuleWaveBlocks work-groups), so small modules do not each dispatch one.
 constexpr unsigned kModuleWaveBlocks = 512;  // x 4 waves
+constexpr int kWave = 64;
 __device__ __forceinline__ uint32_t wave_index() {
   return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
 }
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / 64; }
 
 // per eligible module: candidate count and per-inner-T3 exclusive prefix
+// Per mini-doublet, the T3s whose first mini-doublet it is (one wave per
+// lower module: counts, offsets and a scatter inside the module's own T3
+// range, order free).
+__global__ __launch_bounds__(kWave) void k_md_t3_lists(EventView v) {
+  const unsigned lane = threadIdx.x;
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t m = blockIdx.x; m < nL; m += gridDim.x) {
+    const uint32_t nMD = v.R.nMDs[m];
+    if (nMD == 0) continue;
+    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], nT = v.R.nTriplets[m];
+    const uint32_t t3Base = nT ? (uint32_t)v.R.tripletModuleIndices[m] : 0u;
+    __syncthreads();
+    for (uint32_t k = lane; k < nMD; k += kWave) v.W5.mdT3End[mdBase + k] = 0;
+    __syncthreads();
+    for (uint32_t k = lane; k < nT; k += kWave)
+      atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * (t3Base + k)]]], 1u);
+    __syncthreads();
+    uint32_t run = t3Base;
+    for (uint32_t k0 = 0; k0 < nMD; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const uint32_t c = k < nMD ? v.W5.mdT3End[mdBase + k] : 0u;
+      const uint32_t incl = wave_inclusive_scan(c);
+      if (k < nMD) {
+        v.W5.mdT3Off[mdBase + k] = run + incl - c;
+        v.W5.mdT3End[mdBase + k] = run + incl - c;  // cursor of the scatter, the end afterwards
+      }
+      run += __shfl(incl, kWave - 1, kWave);
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < nT; k += kWave) {
+      const uint32_t t = t3Base + k;
+      v.W5.t3ByMD[atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * t]]], 1u)] = t;
+    }
+  }
+}
+
+// the outer mini-doublet of a T3, which an outer T3 must start at
+__device__ __forceinline__ uint32_t t3_last_md(const EventView& v, uint32_t t3) {
+  return v.S.mdIndices[2 * v.T.segmentIndices[2 * t3 + 1] + 1];
+}
+
+// per inner T3: its candidates (the T3s of its third module, which number
+// them in serial order) and its shared-MD candidates (the T3s listed under its
+// last mini-doublet); per eligible module, both totals
 __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   const unsigned lane = lane_id();
@@ -350,15 +392,27 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
     const int layer = v.M.layers[m1];
     const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
     const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-    uint32_t run = 0;
+    uint32_t run = 0, srun = 0;
     for (uint32_t a0 = 0; a0 < nIn; a0 += 64) {
       const uint32_t a = a0 + lane;
-      const uint32_t cnt = a < nIn ? v.R.nTriplets[v.T.lowerModuleIndices[3 * (t3Base + a) + 2]] : 0;
-      const unsigned incl = wave_inclusive_scan(cnt);
-      if (a < nIn) v.W5.candPre[t3Base + a] = run + incl - cnt;
+      uint32_t cnt = 0, sc = 0;
+      if (a < nIn) {
+        const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
+        cnt = v.R.nTriplets[v.T.lowerModuleIndices[3 * inner + 2]];
+        sc = v.W5.mdT3End[X] - v.W5.mdT3Off[X];
+      }
+      const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
+      if (a < nIn) {
+        v.W5.candPre[t3Base + a] = run + incl - cnt;
+        v.W5.survPre[t3Base + a] = srun + sincl - sc;
+      }
       run += __shfl(incl, 63, 64);
+      srun += __shfl(sincl, 63, 64);
     }
-    if (lane == 0) v.W5.modOff[e] = run;
+    if (lane == 0) {
+      v.W5.modOff[e] = run;
+      v.W5.survOff[e] = srun;
+    }
   }
 }
 
@@ -366,19 +420,17 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
 __global__ __launch_bounds__(1024) void k_t5_cand_scan(EventView v) {
   __shared__ unsigned lds[17];
   const uint32_t n = v.R.scalars[kNEligibleT5];
-  unsigned carry = 0;
-  for (uint32_t b = 0; b < n; b += 1024) {
-    const uint32_t i = b + threadIdx.x;
-    const unsigned c = i < n ? v.W5.modOff[i] : 0u;
-    unsigned tot;
-    const unsigned ex = block_exclusive_scan<1024>(c, lds, tot);
-    if (i < n) v.W5.modOff[i] = carry + ex;
-    carry += tot;
-  }
+  const unsigned total = scan_wide<kScanPer>(
+      n, [&](uint32_t i) { return v.W5.modOff[i]; }, [&](uint32_t i, unsigned p, unsigned) { v.W5.modOff[i] = p; },
+      lds);
+  const unsigned surv = scan_wide<kScanPer>(
+      n, [&](uint32_t i) { return v.W5.survOff[i]; }, [&](uint32_t i, unsigned p, unsigned) { v.W5.survOff[i] = p; },
+      lds);
   if (threadIdx.x == 0) {
-    v.W5.modOff[n] = carry;
-    v.R.scalars[kT5CandTotal] = carry;
-    v.R.scalars[kT5ListA] = 0;
+    v.W5.modOff[n] = total;
+    v.W5.survOff[n] = surv;
+    v.R.scalars[kT5CandTotal] = total;
+    v.R.scalars[kT5ListA] = surv;
     v.R.scalars[kT5ListB] = 0;
     v.R.scalars[kT5Results] = 0;
   }
@@ -392,28 +444,6 @@ __device__ __forceinline__ bool t5_flat_done(const EventView& v) {
   return t5_flat_mode(v) && v.R.scalars[kT5Results] <= kT5ResultMax;
 }
 
-// candidate L -> (inner T3, outer T3)
-__device__ __forceinline__ void t5_decode(const EventView& v, uint32_t L, uint32_t nElig, uint32_t& innerT3,
-                                          uint32_t& outerT3) {
-  uint32_t lo = 0, hi = nElig;  // largest b with modOff[b] <= L
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (v.W5.modOff[mid] <= L) lo = mid;
-    else hi = mid;
-  }
-  const uint16_t m1 = v.R.indicesOfEligibleT5Modules[lo];
-  const uint32_t l = L - v.W5.modOff[lo];
-  const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-  uint32_t a0 = 0, a1 = v.R.nTriplets[m1];  // largest a with candPre[a] <= l
-  while (a1 - a0 > 1) {
-    const uint32_t mid = (a0 + a1) >> 1;
-    if (v.W5.candPre[t3Base + mid] <= l) a0 = mid;
-    else a1 = mid;
-  }
-  innerT3 = t3Base + a0;
-  const uint16_t m3 = v.T.lowerModuleIndices[3 * innerT3 + 2];
-  outerT3 = (uint32_t)v.R.tripletModuleIndices[m3] + (l - v.W5.candPre[innerT3]);
-}
 
 __device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uint32_t L, uint32_t a, uint32_t b) {
   // wave-aggregated append: one atomic per wave
@@ -428,18 +458,37 @@ __device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uin
   list[3 * k + 2] = b;
 }
 
-// stage 1: shared mini-doublet check over all candidates
-__global__ __launch_bounds__(256) void k_t5_filter_md(EventView v) {
+// stage 1: the candidates whose outer T3 starts at the inner T3's last
+// mini-doublet, written at the positions the count pass gives (one wave per
+// eligible module; list order free: L carries the serial order)
+__global__ __launch_bounds__(256) void k_t5_res_init(EventView v) {
   if (!t5_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kT5CandTotal];
+  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
+    v.W5.resIdx[L] = ~0u;
+}
+
+__global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
+  if (!t5_flat_mode(v)) return;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
-    const uint32_t L = L0 + threadIdx.x;
-    if (L < total) {
-      v.W5.resIdx[L] = ~0u;
-      uint32_t inner, outer;
-      t5_decode(v, L, nElig, inner, outer);
-      if (t5_shared_md(v, inner, outer)) list_push(v.W5.listA, &v.R.scalars[kT5ListA], L, inner, outer);
+  const unsigned lane = lane_id();
+  for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
+    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
+    const int layer = v.M.layers[m1];
+    const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
+    const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
+    const uint32_t mOff = v.W5.modOff[e], sOff = v.W5.survOff[e];
+    for (uint32_t a = lane; a < nIn; a += 64) {
+      const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
+      const uint32_t b = v.W5.mdT3Off[X], end = v.W5.mdT3End[X];
+      const uint32_t Lbase = mOff + v.W5.candPre[inner], k0 = sOff + v.W5.survPre[inner];
+      const uint32_t outBase = (uint32_t)v.R.tripletModuleIndices[v.T.lowerModuleIndices[3 * inner + 2]];
+      for (uint32_t q = b; q < end; q++) {
+        const uint32_t outer = v.W5.t3ByMD[q], k = k0 + (q - b);
+        v.W5.listA[3 * k] = Lbase + (outer - outBase);
+        v.W5.listA[3 * k + 1] = inner;
+        v.W5.listA[3 * k + 2] = outer;
+      }
     }
   }
 }
@@ -469,7 +518,12 @@ __global__ __launch_bounds__(256) void k_t5_final(EventView v) {
       const T5Cand c = t5_cand(v, inner, outer);
       T5Result r;
       if (t5_rest(v, c, inner, outer, r)) {
-        const uint32_t k = atomicAdd(&v.R.scalars[kT5Results], 1u);
+        // wave-aggregated slot: one atomic per wave
+        const uint64_t mask = __ballot(1);
+        const unsigned leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT5Results], (uint32_t)__popcll(mask));
+        const uint32_t k = __shfl(base, leader, 64) + lanes_below(mask);
         if (k >= kT5ResultMax) continue;  // overflow: the per-module builder redoes the stage
         uint32_t* w = &v.W5.results[kT5ResultWords * k];
         w[0] = inner; w[1] = outer;
@@ -690,9 +744,11 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_md_t3_lists, dim3(4096), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_filter_md, dim3(flatGrid), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_res_init, dim3(1024), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_t5_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
   t.begin(s, kTmT5Final);
   hipLaunchKernelGGL(k_t5_final, dim3(flatGrid), dim3(256), 0, s, v);

@@ -64,4 +64,49 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* l
   return r;
 }
 
+// Exclusive scan of n values by one 1024-lane work-group.  Wave w owns the
+// 64 * kPer consecutive values from 64 * kPer * w and loads them row by row
+// (coalesced, every load in flight before any use), scans the rows with a
+// running carry, and one work-group scan of the 16 wave totals joins the
+// waves (n beyond 1024 * kPer takes further rounds).  get(i) gives value i;
+// put(i, prefix, value) receives its exclusive prefix.  Returns the total.
+// Every lane must call it; lds needs 17 entries.
+template <int kPer, class Get, class Put>
+__device__ __forceinline__ unsigned scan_wide(uint32_t n, Get get, Put put, unsigned* lds) {
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  unsigned carry = 0;
+  for (uint32_t b = 0; b < n; b += 1024u * kPer) {
+    const uint32_t base = b + 64u * kPer * w + lane;
+    unsigned x[kPer], pre[kPer], run = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) x[k] = base + 64u * k < n ? get(base + 64u * k) : 0u;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const unsigned incl = wave_inclusive_scan(x[k]);
+      pre[k] = run + incl - x[k];
+      run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) lds[w] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned acc = 0;
+      for (int k = 0; k < 16; k++) {
+        const unsigned t = lds[k];
+        lds[k] = acc;
+        acc += t;
+      }
+      lds[16] = acc;
+    }
+    __syncthreads();
+    const unsigned off = carry + lds[w];
+#pragma unroll
+    for (int k = 0; k < kPer; k++)
+      if (base + 64u * k < n) put(base + 64u * k, off + pre[k], x[k]);
+    carry += lds[16];
+    __syncthreads();
+  }
+  return carry;
+}
+constexpr int kScanPer = 16;  // values per lane of scan_wide over module-sized arrays
+
 }  // namespace lst
