@@ -205,11 +205,12 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5Count, plsCap + 1); c.take(e.W.pt3Count, plsCap + 1);
   c.take(e.W.t5TcCount, nL + 1);
   c.take(e.W.outHits, (size_t)tcCap * 14); c.take(e.W.outLen, tcCap);
-  c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1);
+  c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1); c.take(e.W.t5ListMod, t5Cap ? t5Cap : 1);
   c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
-  c.take(e.W.tcPix, 4 * (size_t)tcCap);
   c.take(e.W.plsBinned, plsCap); c.take(e.W.plsBinOff, lst::kEtaBinOffsets);
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
+  c.take(e.W.pixBinned, pt5Cap + pt3Cap); c.take(e.W.pixBinOff, lst::kEtaBinOffsets);
+  c.take(e.W.tcBinned, tcCap); c.take(e.W.tcBinOff, lst::kEtaBinOffsets); c.take(e.W.tcHitSet, lst::kTCHitSet);
   c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
   c.take(e.W.pt3Mask, lst::kPixMaskWords); c.take(e.W.pt3MaskBase, plsCap);
   c.take(e.W.pt5Shard, 2 * lst::kPixShards); c.take(e.W.pt3Shard, 2 * lst::kPixShards);
@@ -227,14 +228,14 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
   c.take(e.WL.cand, lst::kLSCandMax);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
-  c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 3 * (size_t)lst::kT3CandMax);
+  c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 2 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3CandMax);
-  c.take(e.W3.survPre, segCap); c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
+  c.take(e.W3.survPre, segCap); c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.segTmp, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
   c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
   c.take(e.W5.resIdx, lst::kT5CandMax);
   c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
   c.take(e.W5.results, (size_t)lst::kT5ResultWords * lst::kT5ResultMax);
-  c.take(e.W5.survPre, t3Cap ? t3Cap : 1); c.take(e.W5.survOff, nL + 2); c.take(e.W5.t3ByMD, t3Cap ? t3Cap : 1);
+  c.take(e.W5.survPre, t3Cap ? t3Cap : 1); c.take(e.W5.survOff, nL + 2); c.take(e.W5.t3ByMD, t3Cap ? t3Cap : 1); c.take(e.W5.t3Tmp, t3Cap ? t3Cap : 1);
   c.take(e.W5.mdT3Off, mdCap); c.take(e.W5.mdT3End, mdCap);
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);

@@ -126,12 +126,14 @@ constexpr int kT3ResultWords = 6;
 struct T3Work {
   uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
   uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
-  uint32_t* resIdx;   // [kT3CandMax]
-  uint32_t* list;     // [3 * kT3CandMax] (L, module, inner/outer segment offsets) passing the shared-MD check, any order
+  uint32_t* resIdx;   // [kT3CandMax] per list entry: its own position when it passes, else ~0
+  uint32_t* list;     // [2 * kT3CandMax] (module, inner/outer segment offsets) of the candidates passing the
+                      //   shared-MD check, in serial order
   uint32_t* results;  // [kT3ResultWords * kT3CandMax] selection results of list entry i at i
   uint32_t* survPre;  // [segment capacity] per inner segment: shared-MD candidates before it within its module
   uint32_t* survOff;  // [nLower + 1] per lower module: shared-MD candidate count, then offsets
-  uint32_t* segByMD;  // [segment capacity] each module's segments grouped by inner mini-doublet
+  uint32_t* segByMD;  // [segment capacity] each module's segments grouped by inner mini-doublet, in slot order
+  uint32_t* segTmp;   // [segment capacity] the same grouping before the ordering
   uint32_t* mdSegOff; // [MD capacity] per mini-doublet: first position of its segments in segByMD
   uint32_t* mdSegEnd; // [MD capacity] per mini-doublet: end of its segments in segByMD
 };
@@ -144,13 +146,15 @@ constexpr uint32_t kT5ResultMax = 1u << 19;  // passing candidates per event par
 struct T5Work {
   uint32_t* candPre;  // [T3 capacity] per inner T3: candidates before it within its module
   uint32_t* modOff;   // [nLower + 2] per eligible-list position: candidate count, then offsets
-  uint32_t* resIdx;   // [kT5CandMax] result slot of a passing candidate, ~0u otherwise
-  uint32_t* listA;    // [3 * kT5CandMax] (L, inner T3, outer T3) passing the shared-MD check
-  uint32_t* listB;    // [3 * kT5CandMax] ... and both T4-style tests
+  uint32_t* resIdx;   // [kT5CandMax] per list-A entry: its result slot when it passes, ~0u otherwise
+  uint32_t* listA;    // [3 * kT5CandMax] (eligible-list position, inner T3, outer T3) of the candidates passing
+                      //   the shared-MD check, in serial order
+  uint32_t* listB;    // [3 * kT5CandMax] (list-A position, inner T3, outer T3) passing both T4-style tests, any order
   uint32_t* results;  // [kT5ResultWords * kT5ResultMax] selection results of passing candidates
   uint32_t* survPre;  // [T3 capacity] per inner T3: shared-MD candidates before it within its module
   uint32_t* survOff;  // [nLower + 2] per eligible-list position: shared-MD candidate count, then offsets
-  uint32_t* t3ByMD;   // [T3 capacity] each module's T3s grouped by first mini-doublet
+  uint32_t* t3ByMD;   // [T3 capacity] each module's T3s grouped by first mini-doublet, in slot order
+  uint32_t* t3Tmp;    // [T3 capacity] the same grouping before the ordering
   uint32_t* mdT3Off;  // [MD capacity] per mini-doublet: first position of the T3s starting at it in t3ByMD
   uint32_t* mdT3End;  // [MD capacity] per mini-doublet: end of those T3s in t3ByMD
 };
@@ -165,16 +169,21 @@ struct PixelWork {
   uint32_t* t5TcCount;    // [nLower + 1] T5 track candidates per eligible module, then offsets
   uint32_t* outHits;      // [TC capacity * 14] output hit lists
   uint32_t* outLen;       // [TC capacity]
-  uint32_t* t5ListOff;    // [nLower] first list position of each eligible T5 module
+  uint32_t* t5ListOff;    // [nLower + 1] first list position of each eligible T5 module, then the total
   uint32_t* t5List;       // [T5 capacity] T5 slots of the eligible modules
+  uint32_t* t5ListMod;    // [T5 capacity] eligible-list position of the module of each listed T5
   float* tcEta;           // [TC capacity] direction a pLS is compared with in crossCleanpLS
   float* tcPhi;           // [TC capacity]
   uint8_t* tcKind;        // [TC capacity] 0: not compared, 1: T5, 2: pT3 or pT5
-  int32_t* tcPix;         // [TC capacity * 4] ntuple indices of the pLS hits of a pT3/pT5
   uint32_t* plsBinned;    // [pLS capacity] pLS indices listed eta bin by eta bin
   uint32_t* plsBinOff;    // [kEtaBinOffsets] first list position of each eta bin, then the end
   uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
   uint32_t* t5BinOff;     // [kEtaBinOffsets]
+  uint32_t* tcBinned;     // [TC capacity] track candidates, eta bin by eta bin
+  uint32_t* tcBinOff;     // [kEtaBinOffsets]
+  int32_t* tcHitSet;      // [kTCHitSet] open-addressing set of the pLS hits of the pT3/pT5 candidates
+  uint32_t* pixBinned;    // [pT5 + pT3 capacity] pT5 j and pT3 (n5 + j), eta bin by eta bin
+  uint32_t* pixBinOff;    // [kEtaBinOffsets]
   unsigned long long* pt5Mask;  // [kPixMaskWords] pass mask per 64-candidate chunk of each pLS (pT5)
   uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none
   unsigned long long* pt3Mask;  // [kPixMaskWords] (pT3)
@@ -194,6 +203,7 @@ struct PixelWork {
   uint32_t* pt5BucketEnt;       // [14 * pT5 capacity] (pT5, position) entries 14 * pT5 + position, by bucket
 };
 constexpr uint32_t kPT5HashBuckets = 1u << 15;
+constexpr uint32_t kTCHitSet = 1u << 18;  // > 4 * pixel TC capacity (120000): the probes stay short
 constexpr uint32_t kPixMaskWords = 1u << 16;  // 4 Mi candidates per builder and event; beyond: evaluated in full
 constexpr uint32_t kPixSurvMax = 1u << 18;    // survivors of the cheap cuts per builder; beyond: evaluated in full
 constexpr uint32_t kSurvWords = 10;           // pLS, mask position, object, up to 7 result words

@@ -139,6 +139,7 @@ void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_cross_pt3(const EventView& v, hipStream_t s);
 void launch_t5_list(const EventView& v, hipStream_t s);
+void launch_t5_dup_after_build(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t);
 void launch_cross_t5(const EventView& v, hipStream_t s);
 void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t);

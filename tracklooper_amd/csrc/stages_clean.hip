@@ -431,10 +431,14 @@ __global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(EventView v) {
 // --------------------------------------------------- T5 list in slot order
 __global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
   __shared__ unsigned lds[17];
+  const uint32_t n = v.R.scalars[kNEligibleT5];
   const unsigned total = scan_wide<kScanPer>(
-      v.R.scalars[kNEligibleT5], [&](uint32_t i) { return v.R.nQuintuplets[v.R.indicesOfEligibleT5Modules[i]]; },
+      n, [&](uint32_t i) { return v.R.nQuintuplets[v.R.indicesOfEligibleT5Modules[i]]; },
       [&](uint32_t i, unsigned p, unsigned) { v.W.t5ListOff[i] = p; }, lds);
-  if (threadIdx.x == 0) v.R.scalars[kT5Listed] = total;
+  if (threadIdx.x == 0) {
+    v.W.t5ListOff[n] = total;
+    v.R.scalars[kT5Listed] = total;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
@@ -445,7 +449,81 @@ __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
     const uint32_t nq = v.R.nQuintuplets[m];
     const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
     const uint32_t off = v.W.t5ListOff[e];
-    for (uint32_t k = threadIdx.x & 63u; k < nq; k += 64) v.W.t5List[off + k] = base + k;
+    for (uint32_t k = threadIdx.x & 63u; k < nq; k += 64) {
+      v.W.t5List[off + k] = base + k;
+      v.W.t5ListMod[off + k] = e;
+    }
+  }
+}
+
+// ------------------------------------------------ T5 duplicates after build
+// removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249): within one
+// module, a close pair (i < j, |d eta| and |d phi| <= 0.1) sharing >= 7 hits
+// flags i when score_i >= score_j and j otherwise.  The T5 list runs module
+// by module, so an x tile only meets the T5s of the modules it spans (the y
+// range is split into shares as in the other pair kernels); a flag is a store
+// of 1 that does not depend on evaluation order.
+__global__ __launch_bounds__(kTile) void k_t5_dup_after_build(EventView v) {
+  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
+  __shared__ uint32_t sSlot[kTile], sMod[kTile];
+  __shared__ uint32_t sHits[10 * kTile];
+  const uint32_t n = v.R.scalars[kT5Listed];
+  const PairItems items = pair_items(n);
+  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
+    const uint32_t x0 = (it / items.shares) * kTile;
+    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
+    const uint32_t yb = v.W.t5ListOff[v.W.t5ListMod[x0]], ye = v.W.t5ListOff[v.W.t5ListMod[xl] + 1];
+    const YShare sh = share_of(ye - yb, it % items.shares, items.shares);
+    const YShare ys{yb + sh.b, yb + sh.e};
+    const uint32_t xi = x0 + threadIdx.x;
+    const bool active = xi < n;
+    uint32_t q = 0, mod = 0;
+    float ei = 0, pi = 0, si = 0;
+    uint32_t hx[10];
+    if (active) {
+      q = v.W.t5List[xi];
+      mod = v.W.t5ListMod[xi];
+      ei = v.Q.eta[q]; pi = v.Q.phi[q]; si = v.Q.score_rphisum[q];
+    }
+#pragma unroll
+    for (int a = 0; a < 10; a++) hx[a] = active ? v.Q.hitIndices[10 * q + a] : 0u;
+    bool dup = false;
+    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
+      __syncthreads();
+      if (y0 + threadIdx.x < ys.e) {
+        const uint32_t k = y0 + threadIdx.x, y = v.W.t5List[k];
+        sSlot[threadIdx.x] = y;
+        sMod[threadIdx.x] = v.W.t5ListMod[k];
+        sEta[threadIdx.x] = v.Q.eta[y]; sPhi[threadIdx.x] = v.Q.phi[y]; sScore[threadIdx.x] = v.Q.score_rphisum[y];
+        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.Q.hitIndices[10 * y + a];
+      }
+      __syncthreads();
+      const uint32_t m = tile_len(ys, y0);
+      if (!active || dup) continue;
+      for (uint32_t t = 0; t < m && !dup; t++) {
+        const uint32_t y = sSlot[t];
+        if (sMod[t] != mod || y == q) continue;
+        // the pair can only flag q if the score order points at it
+        const bool xLo = q < y;
+        const float sj = sScore[t];
+        if (!(xLo ? (si >= sj) : !(sj >= si))) continue;
+        const float dEta = m_abs(xLo ? ei - sEta[t] : sEta[t] - ei);
+        const float dPhi = xLo ? wrap_dphi(pi, sPhi[t]) : wrap_dphi(sPhi[t], pi);
+        if (dEta > 0.1f) continue;
+        if (m_abs(dPhi) > 0.1f) continue;
+        int matched = 0;
+        for (int a = 0; a < 10; a++) {
+          const uint32_t h = xLo ? hx[a] : sHits[t * 10 + a];
+          for (int b = 0; b < 10; b++)
+            if (h == (xLo ? sHits[t * 10 + b] : hx[b])) {
+              matched++;
+              break;
+            }
+        }
+        dup = matched >= 7;
+      }
+    }
+    if (dup) v.Q.isDup[q] = 1;
   }
 }
 
@@ -528,20 +606,33 @@ __global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
 }
 
 // --------------------------------------------------------------- crossCleanT5
-// a T5 (not flagged, not in a pT5) within dR2 < 1e-3 of any pT5 or pT3 is dropped
+// a T5 (not flagged, not in a pT5) within dR2 < 1e-3 of any pT5 or pT3 is
+// dropped.  dR2 < 1e-3 needs |d eta| < 0.032, and a NaN direction never flags,
+// so the pT5/pT3 are binned by eta and a T5 tile (the T5s in eta-bin order)
+// meets only the bins next to its own.
+__global__ __launch_bounds__(1024) void k_bin_pix(EventView v) {
+  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
+  bin_by_eta(
+      n5 + v.R.scalars[kNPixelTriplets], [](uint32_t i) { return i; },
+      [&](uint32_t j) { return j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5]; }, v.W.pixBinned, v.W.pixBinOff);
+}
+
 __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
   __shared__ float sEta[kTile], sPhi[kTile];
   const uint32_t n = v.R.scalars[kT5Listed];
-  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets], n3 = v.R.scalars[kNPixelTriplets];
-  const uint32_t ny = n5 + n3;
-  const YShare ys = y_share(ny);
-  for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
+  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
+  const PairItems items = pair_items(n);
+  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
+    const uint32_t x0 = (it / items.shares) * kTile;
+    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
+    const Band B = tile_band(v.W.pixBinOff, eta_bin(v.Q.eta[v.W.t5Binned[x0]]), eta_bin(v.Q.eta[v.W.t5Binned[xl]]));
+    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
     const uint32_t xi = x0 + threadIdx.x;
     uint32_t q = 0;
     bool active = xi < n;
     float eta1 = 0, phi1 = 0;
     if (active) {
-      q = v.W.t5List[xi];
+      q = v.W.t5Binned[xi];
       active = v.Q.isDup[q] == 0 && v.Q.partOfPT5[q] == 0;
       eta1 = v.Q.eta[q];
       phi1 = v.Q.phi[q];
@@ -549,8 +640,8 @@ __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
     bool dup = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      const uint32_t j = y0 + threadIdx.x;
-      if (j < ys.e) {
+      if (y0 + threadIdx.x < ys.e) {
+        const uint32_t j = v.W.pixBinned[band_at(B, y0 + threadIdx.x)];
         sEta[threadIdx.x] = j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5];
         sPhi[threadIdx.x] = j < n5 ? v.P5.phi[j] : v.P3.phi[j - n5];
       }
@@ -568,9 +659,14 @@ __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
 }
 
 // -------------------------------------------------------------- crossCleanpLS
-// per track candidate: the direction a pLS is compared with, the dR2 bound,
-// and for pT3/pT5 the ntuple indices of its pLS's four hits (checkPixelHits,
-// SDL/TrackCandidate.h:166-202)
+// A quadruplet pLS not yet flagged is dropped when a T5 candidate lies within
+// dR2 < 1e-3 of it, or a pT3/pT5 candidate within dR2 < 1e-6, or when one of
+// its four pixel hits is a pixel hit of a pT3/pT5 candidate (checkPixelHits,
+// SDL/TrackCandidate.h:166-202).  The two direction tests need |d eta| <
+// 0.032 and never pass for a NaN direction, so the candidates are binned by
+// eta and a pLS tile (the pLS in eta-bin order) meets only the bins next to
+// its own; the hit test is a lookup in the set of the candidates' pixel hits
+// (ntuple indices), done once per pLS.
 __device__ __forceinline__ void pls_ntuple_hits(const EventView& v, uint32_t seg, int32_t* p) {
   const uint32_t a = v.S.mdIndices[2 * seg], b = v.S.mdIndices[2 * seg + 1];
   p[0] = (int32_t)v.H.idxs[v.D.anchorHitIndices[a]];
@@ -578,7 +674,25 @@ __device__ __forceinline__ void pls_ntuple_hits(const EventView& v, uint32_t seg
   p[2] = (int32_t)v.H.idxs[v.D.outerHitIndices[a]];
   p[3] = (int32_t)v.H.idxs[v.D.outerHitIndices[b]];
 }
+constexpr int32_t kSetEmpty = INT32_MIN;  // never an ntuple index (those are >= -1); k_pls_connected clears the set
+__device__ __forceinline__ uint32_t set_slot(int32_t h) { return ((uint32_t)h * 0x9E3779B1u) >> (32 - 18); }
+static_assert(kTCHitSet == 1u << 18, "set_slot takes 18 bits");
+__device__ __forceinline__ void set_insert(int32_t* set, int32_t h) {
+  for (uint32_t s = set_slot(h);; s = (s + 1) & (kTCHitSet - 1)) {
+    const int32_t old = atomicCAS(&set[s], kSetEmpty, h);
+    if (old == kSetEmpty || old == h) return;
+  }
+}
+__device__ __forceinline__ bool set_has(const int32_t* set, int32_t h) {
+  for (uint32_t s = set_slot(h);; s = (s + 1) & (kTCHitSet - 1)) {
+    const int32_t k = set[s];
+    if (k == h) return true;
+    if (k == kSetEmpty) return false;
+  }
+}
 
+// per track candidate: the direction a pLS is compared with and its kind (0:
+// none, 1: T5, 2: pT3/pT5); the pT3/pT5 pixel hits go into the set
 __global__ void k_tc_prep(EventView v) {
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
@@ -600,39 +714,52 @@ __global__ void k_tc_prep(EventView v) {
     v.W.tcEta[t] = eta;
     v.W.tcPhi[t] = phi;
     v.W.tcKind[t] = kind;
-    for (int a = 0; a < 4; a++) v.W.tcPix[4 * t + a] = p[a];
+    if (kind == 2)
+      for (int a = 0; a < 4; a++)
+        if (p[a] != -1) set_insert(v.W.tcHitSet, p[a]);
   }
+}
+
+__global__ __launch_bounds__(1024) void k_bin_tc(EventView v) {
+  bin_by_eta(
+      v.R.scalars[kNTrackCandidates], [](uint32_t i) { return i; }, [&](uint32_t t) { return v.W.tcEta[t]; },
+      v.W.tcBinned, v.W.tcBinOff);
 }
 
 __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
   __shared__ float sEta[kTile], sPhi[kTile];
   __shared__ uint8_t sKind[kTile];
-  __shared__ int32_t sPix[4 * kTile];
   const uint32_t n = v.R.scalars[kEvNPLS];
-  const uint32_t nTC = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
   const PairItems items = pair_items(n);
   for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
     const uint32_t x0 = (it / items.shares) * kTile;
-    const YShare ys = share_of(nTC, it % items.shares, items.shares);
-    const uint32_t i = x0 + threadIdx.x;
-    const bool active = i < n && v.P.isQuad[i] && v.P.isDup[i] == 0;
+    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
+    const Band B = tile_band(v.W.tcBinOff, eta_bin(v.P.eta[v.W.plsBinned[x0]]), eta_bin(v.P.eta[v.W.plsBinned[xl]]));
+    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
+    const uint32_t xi = x0 + threadIdx.x;
+    uint32_t i = 0;
+    bool active = xi < n;
     float eta1 = 0, phi1 = 0;
-    int32_t p1[4] = {-1, -1, -1, -1};
+    bool dup = false;
     if (active) {
+      i = v.W.plsBinned[xi];
+      active = v.P.isQuad[i] && v.P.isDup[i] == 0;
       eta1 = v.P.eta[i];
       phi1 = v.P.phi[i];
-      pls_ntuple_hits(v, plsBase + i, p1);
+      if (active && it % items.shares == 0) {  // the hit test, once per pLS
+        int32_t p1[4];
+        pls_ntuple_hits(v, plsBase + i, p1);
+        for (int a = 0; a < 4 && !dup; a++) dup = p1[a] != -1 && set_has(v.W.tcHitSet, p1[a]);
+      }
     }
-    bool dup = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      const uint32_t t = y0 + threadIdx.x;
-      if (t < ys.e) {
+      if (y0 + threadIdx.x < ys.e) {
+        const uint32_t t = v.W.tcBinned[band_at(B, y0 + threadIdx.x)];
         sEta[threadIdx.x] = v.W.tcEta[t];
         sPhi[threadIdx.x] = v.W.tcPhi[t];
         sKind[threadIdx.x] = v.W.tcKind[t];
-        for (int a = 0; a < 4; a++) sPix[4 * threadIdx.x + a] = v.W.tcPix[4 * t + a];
       }
       __syncthreads();
       const uint32_t m = tile_len(ys, y0);
@@ -643,22 +770,10 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
         const float dEta = m_abs(eta1 - sEta[t2]);
         const float dPhi = wrap_dphi(phi1, sPhi[t2]);
         const float dR2 = dEta * dEta + dPhi * dPhi;
-        if (kind == 1) {
-          dup = dR2 < 1e-3f;
-          continue;
-        }
-        dup = dR2 < 0.000001f;
-        for (int a = 0; a < 4 && !dup; a++) {
-          if (p1[a] == -1) continue;
-          for (int b = 0; b < 4; b++)
-            if (p1[a] == sPix[4 * t2 + b]) {
-              dup = true;
-              break;
-            }
-        }
+        dup = kind == 1 ? dR2 < 1e-3f : dR2 < 0.000001f;
       }
     }
-    if (dup) v.P.isDup[i] = 1;
+    if (active && dup) v.P.isDup[i] = 1;
   }
 }
 
@@ -701,6 +816,12 @@ void launch_t5_list(const EventView& v, hipStream_t s) {
   hipLaunchKernelGGL(k_t5_list_fill, dim3(256), dim3(256), 0, s, v);
 }
 
+void launch_t5_dup_after_build(const EventView& v, hipStream_t s, const KernelTimers& t) {
+  t.begin(s, kTmT5DupAfter);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(kPairItems), dim3(kTile), 0, s, v);
+  t.end(s, kTmT5DupAfter);
+}
+
 void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT5DupTC);
@@ -709,12 +830,13 @@ void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTime
 }
 
 void launch_cross_t5(const EventView& v, hipStream_t s) {
-  hipLaunchKernelGGL(k_cross_t5_pairs, pair_grid(v.t5Cap, 8192, kNMaxPixelQuintuplets + kNMaxPixelTriplets), dim3(kTile),
-                     0, s, v);
+  hipLaunchKernelGGL(k_bin_pix, dim3(1), dim3(1024), 0, s, v);
+  hipLaunchKernelGGL(k_cross_t5_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
 }
 
 void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_tc_prep, dim3(128), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_bin_tc, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmCrossPLS);
   hipLaunchKernelGGL(k_cross_pls_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
   t.end(s, kTmCrossPLS);

@@ -226,6 +226,27 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
   }
 }
 
+// module ranges (SDL/MiniDoublet.h, Segment.h, Triplet.h addRangesToEventExplicit),
+// written where each module's object count is final; every range starts at -1
+__device__ __forceinline__ void md_range(const EventView& v, uint32_t m, uint32_t n) {
+  if (n != 0 && v.R.hitRanges[2 * m] != -1) {
+    v.R.mdRanges[2 * m] = v.mdModuleIndices[m];
+    v.R.mdRanges[2 * m + 1] = v.mdModuleIndices[m] + n - 1;
+  }
+}
+__device__ __forceinline__ void seg_range(const EventView& v, uint32_t m, uint32_t n) {
+  if (n != 0) {
+    v.R.segmentRanges[2 * m] = v.segModuleIndices[m];
+    v.R.segmentRanges[2 * m + 1] = v.segModuleIndices[m] + n - 1;
+  }
+}
+__device__ __forceinline__ void t3_range(const EventView& v, uint32_t m, uint32_t n) {
+  if (n != 0) {
+    v.R.tripletRanges[2 * m] = v.R.tripletModuleIndices[m];
+    v.R.tripletRanges[2 * m + 1] = v.R.tripletModuleIndices[m] + n - 1;
+  }
+}
+
 // --------------------------------------------------------------- mini-doublets
 // Flattened builder (the path every event takes unless it has more than
 // kMDCandMax hit pairs): candidates of module m are [candOff[m], candOff[m+1]),
@@ -340,6 +361,7 @@ __global__ __launch_bounds__(256) void k_md_place(EventView v) {
     const uint32_t count = flag_rank(v.FS, v.W2.candOff[m + 1], total) - flag_rank(v.FS, v.W2.candOff[m], total);
     v.R.totOccupancyMDs[m] = count;
     v.R.nMDs[m] = min(count, (unsigned)max(v.mdModuleOccupancy[m], 0));
+    md_range(v, m, v.R.nMDs[m]);
   }
 }
 
@@ -391,19 +413,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
   if (lane == 0) {
     v.R.totOccupancyMDs[m] = count;
     v.R.nMDs[m] = min(count, (unsigned)max(cap, 0));
+    md_range(v, m, v.R.nMDs[m]);
   }
 }
 
-// SDL/MiniDoublet.h:1084-1104
-__global__ void k_md_ranges(EventView v) {
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
-    const uint32_t n = v.R.nMDs[m];
-    if (n != 0 && v.R.hitRanges[2 * m] != -1) {
-      v.R.mdRanges[2 * m] = v.mdModuleIndices[m];
-      v.R.mdRanges[2 * m + 1] = v.mdModuleIndices[m] + n - 1;
-    }
-  }
-}
 
 // --------------------------------------------------------------- line segments
 // Flattened builder, as for the mini-doublets.  Per module, connPre[k] is the
@@ -539,6 +552,7 @@ __global__ __launch_bounds__(256) void k_ls_place(EventView v) {
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
     v.R.totOccupancySegments[m] = count;
     v.R.nSegments[m] = min(count, (unsigned)max(v.segModuleOccupancy[m], 0));
+    seg_range(v, m, v.R.nSegments[m]);
   }
 }
 
@@ -613,19 +627,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
   if (lane == 0) {
     v.R.totOccupancySegments[m] = count;
     v.R.nSegments[m] = min(count, (unsigned)max(cap, 0));
+    seg_range(v, m, v.R.nSegments[m]);
   }
 }
 
-// SDL/Segment.h:1010-1029
-__global__ void k_segment_ranges(EventView v) {
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
-    const uint32_t n = v.R.nSegments[m];
-    if (n != 0) {
-      v.R.segmentRanges[2 * m] = v.segModuleIndices[m];
-      v.R.segmentRanges[2 * m + 1] = v.segModuleIndices[m] + n - 1;
-    }
-  }
-}
 
 // -------------------------------------------------------------------- triplets
 // capacity offsets: a module with segments gets its tabulated capacity, others 0
@@ -760,12 +765,14 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
 
 // stage 1: the candidates whose inner segment's outer mini-doublet X is the
 // outer segment's inner one.  Per mini-doublet, the segments starting at it
-// are listed first (one wave per lower module: counts, offsets and a scatter
-// inside the module's own segment range, order free); the candidate count
-// pass also counts, per inner segment, the segments listed under X; then one
-// wave per module m1 walks its inner segments and writes, for each segment
-// listed under X, the candidate's serial number L and offsets at the
-// positions those counts give (list order free: L carries the serial order).
+// are listed first, in slot order (one wave per lower module: counts,
+// offsets, a scatter inside the module's own segment range, then each
+// segment placed at its rank within its short list); the candidate count
+// pass also counts, per inner segment,
+// the segments listed under X; then one wave per module m1 walks its inner
+// segments and writes, for each segment listed under X, the candidate at the
+// position those counts give.  The list is thereby in the reference's serial
+// order (module, inner segment, outer segment).
 __global__ __launch_bounds__(kWave) void k_md_seg_lists(EventView v) {
   const unsigned lane = threadIdx.x;
   const uint32_t nL = v.M.nLowerModules;
@@ -793,16 +800,19 @@ __global__ __launch_bounds__(kWave) void k_md_seg_lists(EventView v) {
     __syncthreads();
     for (uint32_t k = lane; k < nS; k += kWave) {
       const uint32_t s = segBase + k;
-      v.W3.segByMD[atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * s]], 1u)] = s;
+      v.W3.segTmp[atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * s]], 1u)] = s;
+    }
+    __syncthreads();
+    // slot order inside each list: a segment's position is the number of
+    // segments of its list with a lower slot (a list holds a few segments)
+    for (uint32_t k = lane; k < nS; k += kWave) {
+      const uint32_t s = segBase + k, X = v.S.mdIndices[2 * s];
+      const uint32_t b = v.W3.mdSegOff[X], e = v.W3.mdSegEnd[X];
+      uint32_t r = b;
+      for (uint32_t q = b; q < e; q++) r += v.W3.segTmp[q] < s ? 1u : 0u;
+      v.W3.segByMD[r] = s;
     }
   }
-}
-
-__global__ __launch_bounds__(256) void k_t3_res_init(EventView v) {
-  if (!t3_flat_mode(v)) return;
-  const uint32_t total = v.R.scalars[kT3CandTotal];
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
-    v.W3.resIdx[L] = ~0u;
 }
 
 __global__ __launch_bounds__(256) void k_t3_enum(EventView v) {
@@ -810,74 +820,75 @@ __global__ __launch_bounds__(256) void k_t3_enum(EventView v) {
   const unsigned lane = lane_id();
   for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
     const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
-    const uint32_t segBase = v.segModuleIndices[m1], mOff = v.W3.modOff[m1], sOff = v.W3.survOff[m1];
+    const uint32_t segBase = v.segModuleIndices[m1], sOff = v.W3.survOff[m1];
     for (uint32_t a = lane; a < nIn; a += kWave) {
       const uint32_t inSeg = segBase + a;
       const uint32_t X = v.S.mdIndices[2 * inSeg + 1];
       const uint32_t b = v.W3.mdSegOff[X], e = v.W3.mdSegEnd[X];
-      const uint32_t Lbase = mOff + v.W3.candPre[inSeg], k0 = sOff + v.W3.survPre[inSeg];
+      const uint32_t k0 = sOff + v.W3.survPre[inSeg];
       const uint32_t seg2Base = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]];
       for (uint32_t q = b; q < e; q++) {
         const uint32_t j = v.W3.segByMD[q] - seg2Base, k = k0 + (q - b);
-        v.W3.list[3 * k] = Lbase + j;
-        v.W3.list[3 * k + 1] = m1;
-        v.W3.list[3 * k + 2] = a | (j << 16);
+        v.W3.list[2 * k] = m1;
+        v.W3.list[2 * k + 1] = a | (j << 16);
       }
     }
   }
 }
 
-// stage 2: the triplet selection; results parked per candidate
+// stage 2: the triplet selection; results parked at the list position
 __global__ __launch_bounds__(256) void k_t3_select(EventView v) {
   if (!t3_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT3List];
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
     const uint32_t i = i0 + threadIdx.x;
     if (i >= n) continue;
-    const uint32_t L = v.W3.list[3 * i], m1 = v.W3.list[3 * i + 1], ab = v.W3.list[3 * i + 2];
+    const uint32_t m1 = v.W3.list[2 * i], ab = v.W3.list[2 * i + 1];
     const uint32_t inSeg = v.segModuleIndices[m1] + (ab & 0xFFFFu);
     const uint32_t outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (ab >> 16);
     float betaIn = 0, radius = 0, gx = 0, gy = 0;
-    if (t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy)) {
-      const uint32_t k = i;  // the list position: no slot taken with an atomic
-      uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)k];
+    const bool pass = t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy);
+    if (pass) {  // results at the list position: no slot taken with an atomic
+      uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)i];
       w[0] = inSeg; w[1] = outSeg;
       w[2] = __float_as_uint(betaIn); w[3] = __float_as_uint(radius);
       w[4] = __float_as_uint(gx); w[5] = __float_as_uint(gy);
-      v.W3.resIdx[L] = k;
     }
+    v.W3.resIdx[i] = pass ? i : ~0u;
   }
 }
 
-// stage 3: per module, passing candidates in candidate order take the slots
+// stage 3: the list is in serial order (modules, inner segments and each
+// mini-doublet's segments in order), so per module the passing entries take
+// the slots in list order: a pass-flag scan over the list positions
 __global__ __launch_bounds__(1024) void k_t3_scan_blocks(EventView v) {
   if (!t3_flat_done(v)) return;
   scan_flag_blocks(
-      v.R.scalars[kT3CandTotal], [&](uint32_t L) { return v.W3.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+      v.R.scalars[kT3List], [&](uint32_t k) { return v.W3.resIdx[k] != ~0u; }, v.FS.pre, v.FS.sums);
 }
 __global__ __launch_bounds__(1024) void k_t3_scan_sums(EventView v) {
   if (!t3_flat_done(v)) return;
-  scan_flag_sums(v.R.scalars[kT3CandTotal], v.FS.sums);
+  scan_flag_sums(v.R.scalars[kT3List], v.FS.sums);
 }
 
-// passing candidates take their slots in L order; then per-module counts
+// passing entries take their slots in list order; then per-module counts
 __global__ __launch_bounds__(256) void k_t3_place(EventView v) {
   if (!t3_flat_done(v)) return;
-  const uint32_t total = v.R.scalars[kT3CandTotal], nL = v.M.nLowerModules;
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
-    const uint32_t k = v.W3.resIdx[L];
-    if (k == ~0u) continue;
-    const uint32_t m1 = module_of(v.W3.modOff, nL, L);
-    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W3.modOff[m1], total);
+  const uint32_t total = v.R.scalars[kT3List], nL = v.M.nLowerModules;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    if (v.W3.resIdx[k] == ~0u) continue;
+    const uint32_t m1 = v.W3.list[2 * k];
+    const uint32_t rank = flag_rank(v.FS, k, total) - flag_rank(v.FS, v.W3.survOff[m1], total);
     if ((int)rank >= v.R.tripletModuleOccupancy[m1]) continue;
-    const uint32_t* w = &v.W3.results[kT3ResultWords * k];
+    const uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)k];
     t3_store(v, v.R.tripletModuleIndices[m1] + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
              __uint_as_float(w[4]), __uint_as_float(w[5]));
   }
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
-    const uint32_t count = flag_rank(v.FS, v.W3.modOff[m + 1], total) - flag_rank(v.FS, v.W3.modOff[m], total);
+    const uint32_t count = flag_rank(v.FS, v.W3.survOff[m + 1], total) - flag_rank(v.FS, v.W3.survOff[m], total);
     v.R.totOccupancyTriplets[m] = count;
     v.R.nTriplets[m] = min(count, (unsigned)max(v.R.tripletModuleOccupancy[m], 0));
+    t3_range(v, m, v.R.nTriplets[m]);
   }
 }
 
@@ -930,19 +941,10 @@ __global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
   if (lane == 0) {
     v.R.totOccupancyTriplets[m1] = count;
     v.R.nTriplets[m1] = min(count, (unsigned)max(cap, 0));
+    t3_range(v, m1, v.R.nTriplets[m1]);
   }
 }
 
-// SDL/Triplet.h:1104-1123
-__global__ void k_triplet_ranges(EventView v) {
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
-    const uint32_t n = v.R.nTriplets[m];
-    if (n != 0) {
-      v.R.tripletRanges[2 * m] = v.R.tripletModuleIndices[m];
-      v.R.tripletRanges[2 * m + 1] = v.R.tripletModuleIndices[m] + n - 1;
-    }
-  }
-}
 
 inline unsigned grid_for(uint32_t n, unsigned block) {
   unsigned g = (n + block - 1) / block;
@@ -1039,7 +1041,6 @@ void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers&
   hipLaunchKernelGGL(k_md_scan_sums, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_md_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
-  hipLaunchKernelGGL(k_md_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
 void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
@@ -1053,15 +1054,13 @@ void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_ls_scan_sums, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_ls_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
-  hipLaunchKernelGGL(k_segment_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_seg_lists, dim3(4096), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_md_seg_lists, dim3(8192), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_res_init, dim3(1024), dim3(256), 0, s, v);
   t.begin(s, kTmT3Filter);
   hipLaunchKernelGGL(k_t3_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   t.end(s, kTmT3Filter);
@@ -1072,7 +1071,6 @@ void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_t3_scan_sums, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t3_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
-  hipLaunchKernelGGL(k_triplet_ranges, dim3(grid_for(v.M.nLowerModules, 256)), dim3(256), 0, s, v);
 }
 
 }  // namespace lst

@@ -29,15 +29,18 @@ inline unsigned grid_of(uint32_t n, unsigned block) {
   return g == 0 ? 1 : (g > 8192 ? 8192 : g);
 }
 
-// exclusive scan of counts[0, *count) in place; counts[*count] and *total get the sum
-__global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t* counts, const uint32_t* count, uint32_t* total) {
+// exclusive scan of counts[0, *count) in place; counts[*count] and *total get
+// the sum, *kept the sum clamped to cap
+__global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t* counts, const uint32_t* count, uint32_t* total,
+                                                      uint32_t cap, uint32_t* kept) {
   __shared__ unsigned lds[17];
   const uint32_t n = *count;
   const unsigned sum = scan_wide<kScanPer>(
       n, [&](uint32_t i) { return counts[i]; }, [&](uint32_t i, unsigned p, unsigned) { counts[i] = p; }, lds);
   if (threadIdx.x == 0) {
     counts[n] = sum;
-    if (total) *total = sum;
+    *total = sum;
+    *kept = sum < cap ? sum : cap;
   }
 }
 
@@ -54,6 +57,8 @@ __global__ void k_pls_connected(EventView v) {
   }
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= kPT5HashBuckets; b += gridDim.x * blockDim.x)
     v.W.pt5BucketOff[b] = 0;  // hit-bucket counts of the pT5 duplicate search
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < kTCHitSet; b += gridDim.x * blockDim.x)
+    v.W.tcHitSet[b] = INT32_MIN;  // the empty set of the pLS cross cleaning
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nPLS; i += gridDim.x * blockDim.x) {
     const int sb = v.P.superbin[i];
@@ -79,9 +84,9 @@ __global__ void k_pls_connected(EventView v) {
 //            position, object, partial result) is appended to a list; the
 //            pLS's pass-mask words are cleared
 //   stage B  one lane per survivor: the costly cuts; a pass sets its bit in
-//            the pLS's mask, records where its result is, and counts the pLS
-//   full     pLS whose survivors did not all fit (flagged by stage A) are
-//            counted by evaluating every candidate (no work otherwise)
+//            the pLS's mask, records where its result is, and counts the pLS;
+//            pLS whose survivors did not all fit (flagged by stage A) are
+//            counted instead by evaluating every candidate
 //   scan     per-pLS counts to slot offsets
 //   write    one wave per pLS: the passing candidates, in mask order, to
 //            their slots; the flagged pLS are evaluated again instead
@@ -362,9 +367,14 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
   }
 }
 
+// stage B, and the counting of the pLS flagged by stage A: their survivors are
+// skipped here, and when any pLS is flagged the same waves then evaluate every
+// candidate of each flagged pLS (its count only; the write evaluates again)
 template <class K>
 __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
-  __shared__ uint32_t pre[kPixShards + 1];  // survivors before each shard
+  __shared__ uint32_t pre[kMaxConn + 1];  // survivors before each shard, then the connected-module prefix
+  __shared__ uint32_t objBase[kMaxConn];
+  static_assert(kMaxConn >= (int)kPixShards, "the shard prefix reuses the module prefix");
   const unsigned lane = threadIdx.x;
   const uint32_t c = K::shard(v)[kPixShards + lane];
   const uint32_t incl = wave_inclusive_scan(c < kShardSurv ? c : kShardSurv);
@@ -384,6 +394,7 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
     const uint32_t s = lo * kShardSurv + (g - pre[lo]);
     uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
     const uint32_t i = rec[0], slot = rec[1], t = rec[2];
+    if (v.W.pixFull[i]) continue;
     typename K::Res r;
     K::unpack(rec + 3, r);
     if (K::costly(v, plsBase + i, i, t, r)) {
@@ -393,17 +404,8 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
       atomicAdd(&K::count(v)[i], 1u);
     }
   }
-}
-
-// the pLS flagged by stage A, counted by evaluating every candidate
-template <class K>
-__global__ __launch_bounds__(kWave) void k_pix_full(EventView v) {
   if ((v.R.scalars[kPixFull] & K::kBit) == 0) return;
-  __shared__ uint32_t pre[kMaxConn + 1];
-  __shared__ uint32_t objBase[kMaxConn];
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
-  const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const unsigned lane = threadIdx.x;
   for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
     if (!v.W.pixFull[i]) continue;
     __syncthreads();
@@ -485,11 +487,6 @@ __global__ __launch_bounds__(kWave) void k_pix_full_write(EventView v) {
   }
 }
 
-__global__ void k_pt5_totals(EventView v) {
-  const uint32_t tot = v.R.scalars[kTotOccPixelQuintuplets];
-  v.R.scalars[kNPixelQuintuplets] = tot < kNMaxPixelQuintuplets ? tot : kNMaxPixelQuintuplets;
-}
-
 // Flag compaction over items [0, n) by 256-lane work-groups: work-group b
 // takes items [256 b, 256 b + 256) and counts the flagged items before them
 // itself, so the work-groups need no scan between them.  Returns this lane's
@@ -558,11 +555,6 @@ __global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt5(EventView v) {
     v.R.scalars[kNTrackCandidatespLS] = 0;
     v.R.scalars[kNTrackCandidatesT5] = 0;
   }
-}
-
-__global__ void k_pt3_totals(EventView v) {
-  const uint32_t tot = v.R.scalars[kTotOccPixelTriplets];
-  v.R.scalars[kNPixelTriplets] = tot < kNMaxPixelTriplets ? tot : kNMaxPixelTriplets;
 }
 
 // ---------------------------------------------------------- track candidates
@@ -728,31 +720,31 @@ void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const Kern
 
 template <class K>
 void launch_pixel_builder(const EventView& v, hipStream_t s, const KernelTimers& t, int timer, uint32_t* countScalar,
-                          void (*totals)(EventView)) {
+                          uint32_t* keptScalar) {
   t.begin(s, timer);
   hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
   t.end(s, timer);
   hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
-  hipLaunchKernelGGL(k_pix_full<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
-  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar);
-  hipLaunchKernelGGL(totals, dim3(1), dim3(1), 0, s, v);
+  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar,
+                     K::kCap, keptScalar);
   hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_pix_full_write<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
 }
 
 void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_pls_connected, dim3(grid_of(kNMaxPixelSegments, 256)), dim3(256), 0, s, v);
-  launch_t5_list(v, s);
   hipLaunchKernelGGL(k_t3_list_offsets, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t3_list_fill, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_pix_records, dim3(256), dim3(256), 0, s, v);
-  launch_pixel_builder<PT5Kind>(v, s, t, kTmPT5Build, v.R.scalars + kTotOccPixelQuintuplets, k_pt5_totals);
+  launch_pixel_builder<PT5Kind>(v, s, t, kTmPT5Build, v.R.scalars + kTotOccPixelQuintuplets,
+                                v.R.scalars + kNPixelQuintuplets);
   launch_pt5_dup(v, s, t);
   hipLaunchKernelGGL(k_tc_add_pt5, dim3(compact_grid(kNMaxPixelQuintuplets)), dim3(kCompactBlock), 0, s, v);
 }
 
 void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  launch_pixel_builder<PT3Kind>(v, s, t, kTmPT3Build, v.R.scalars + kTotOccPixelTriplets, k_pt3_totals);
+  launch_pixel_builder<PT3Kind>(v, s, t, kTmPT3Build, v.R.scalars + kTotOccPixelTriplets,
+                                v.R.scalars + kNPixelTriplets);
   launch_pt3_dup(v, s, t);
 }
 

@@ -23,7 +23,16 @@ namespace {
 constexpr int kT5Block = 256;
 constexpr int kMaxInnerT3 = 1024;   // > the largest T3 module capacity (755)
 constexpr int kMaxT5PerModule = 416;  // >= the largest T5 module capacity (414)
-constexpr int kAfterBuildSplit = 8;   // partner shares per module in the after-build duplicate check
+
+// addQuintupletRangesToEventExplicit (SDL/Quintuplet.h:3215-3235), written
+// where a module's T5 count is final; every range starts at -1
+__device__ __forceinline__ void t5_range(const EventView& v, uint32_t m, uint32_t n) {
+  const int base = v.R.quintupletModuleIndices[m];
+  if (n != 0 && base != -1) {
+    v.R.quintupletRanges[2 * m] = base;
+    v.R.quintupletRanges[2 * m + 1] = base + (int)n - 1;
+  }
+}
 
 // createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
 // seed modules holding a triplet, their capacities and the exclusive scan of
@@ -340,9 +349,9 @@ __device__ __forceinline__ uint32_t wave_index() {
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / 64; }
 
 // per eligible module: candidate count and per-inner-T3 exclusive prefix
-// Per mini-doublet, the T3s whose first mini-doublet it is (one wave per
-// lower module: counts, offsets and a scatter inside the module's own T3
-// range, order free).
+// Per mini-doublet, the T3s whose first mini-doublet it is, in slot order (one
+// wave per lower module: counts, offsets, a scatter inside the module's own T3
+// range, then each T3 placed at its rank within its short list).
 __global__ __launch_bounds__(kWave) void k_md_t3_lists(EventView v) {
   const unsigned lane = threadIdx.x;
   const uint32_t nL = v.M.nLowerModules;
@@ -371,7 +380,17 @@ __global__ __launch_bounds__(kWave) void k_md_t3_lists(EventView v) {
     __syncthreads();
     for (uint32_t k = lane; k < nT; k += kWave) {
       const uint32_t t = t3Base + k;
-      v.W5.t3ByMD[atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * t]]], 1u)] = t;
+      v.W5.t3Tmp[atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * t]]], 1u)] = t;
+    }
+    __syncthreads();
+    // slot order inside each list: a T3's position is the number of T3s of
+    // its list with a lower slot (a list holds a few T3s)
+    for (uint32_t k = lane; k < nT; k += kWave) {
+      const uint32_t t = t3Base + k, X = v.S.mdIndices[2 * v.T.segmentIndices[2 * t]];
+      const uint32_t b = v.W5.mdT3Off[X], e = v.W5.mdT3End[X];
+      uint32_t r = b;
+      for (uint32_t q = b; q < e; q++) r += v.W5.t3Tmp[q] < t ? 1u : 0u;
+      v.W5.t3ByMD[r] = t;
     }
   }
 }
@@ -460,14 +479,8 @@ __device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uin
 
 // stage 1: the candidates whose outer T3 starts at the inner T3's last
 // mini-doublet, written at the positions the count pass gives (one wave per
-// eligible module; list order free: L carries the serial order)
-__global__ __launch_bounds__(256) void k_t5_res_init(EventView v) {
-  if (!t5_flat_mode(v)) return;
-  const uint32_t total = v.R.scalars[kT5CandTotal];
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x)
-    v.W5.resIdx[L] = ~0u;
-}
-
+// eligible module).  With each mini-doublet's T3s in slot order, list A is in
+// the reference's serial order (module, inner T3, outer T3).
 __global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
   if (!t5_flat_mode(v)) return;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
@@ -477,17 +490,16 @@ __global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
     const int layer = v.M.layers[m1];
     const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
     const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-    const uint32_t mOff = v.W5.modOff[e], sOff = v.W5.survOff[e];
+    const uint32_t sOff = v.W5.survOff[e];
     for (uint32_t a = lane; a < nIn; a += 64) {
       const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
       const uint32_t b = v.W5.mdT3Off[X], end = v.W5.mdT3End[X];
-      const uint32_t Lbase = mOff + v.W5.candPre[inner], k0 = sOff + v.W5.survPre[inner];
-      const uint32_t outBase = (uint32_t)v.R.tripletModuleIndices[v.T.lowerModuleIndices[3 * inner + 2]];
+      const uint32_t k0 = sOff + v.W5.survPre[inner];
       for (uint32_t q = b; q < end; q++) {
-        const uint32_t outer = v.W5.t3ByMD[q], k = k0 + (q - b);
-        v.W5.listA[3 * k] = Lbase + (outer - outBase);
+        const uint32_t k = k0 + (q - b);
+        v.W5.listA[3 * k] = e;
         v.W5.listA[3 * k + 1] = inner;
-        v.W5.listA[3 * k + 2] = outer;
+        v.W5.listA[3 * k + 2] = v.W5.t3ByMD[q];
       }
     }
   }
@@ -500,9 +512,10 @@ __global__ __launch_bounds__(256) void k_t5_filter_t4(EventView v) {
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
     const uint32_t i = i0 + threadIdx.x;
     if (i < n) {
-      const uint32_t L = v.W5.listA[3 * i], inner = v.W5.listA[3 * i + 1], outer = v.W5.listA[3 * i + 2];
+      const uint32_t inner = v.W5.listA[3 * i + 1], outer = v.W5.listA[3 * i + 2];
       const T5Cand c = t5_cand(v, inner, outer);
-      if (t5_t4_tests(v, c)) list_push(v.W5.listB, &v.R.scalars[kT5ListB], L, inner, outer);
+      if (t5_t4_tests(v, c)) list_push(v.W5.listB, &v.R.scalars[kT5ListB], i, inner, outer);
+      else v.W5.resIdx[i] = ~0u;
     }
   }
 }
@@ -517,6 +530,7 @@ __global__ __launch_bounds__(256) void k_t5_final(EventView v) {
       const uint32_t L = v.W5.listB[3 * i], inner = v.W5.listB[3 * i + 1], outer = v.W5.listB[3 * i + 2];
       const T5Cand c = t5_cand(v, inner, outer);
       T5Result r;
+      v.W5.resIdx[L] = ~0u;  // L: the list-A position; set below when the candidate passes
       if (t5_rest(v, c, inner, outer, r)) {
         // wave-aggregated slot: one atomic per wave
         const uint64_t mask = __ballot(1);
@@ -566,34 +580,32 @@ __global__ __launch_bounds__(256) void k_t5_dnn(EventView v) {
 
 // stage 4: passing candidates take their slots in candidate order (pass-flag
 // scan; a result the DNN rejected does not pass), then per-module counts
-__device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t L) {
-  const uint32_t k = v.W5.resIdx[L];
-  return k != ~0u && v.W5.results[kT5ResultWords * k] != ~0u;
+__device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t k) {
+  const uint32_t r = v.W5.resIdx[k];
+  return r != ~0u && v.W5.results[kT5ResultWords * r] != ~0u;
 }
 __global__ __launch_bounds__(1024) void k_t5_scan_blocks(EventView v) {
   if (!t5_flat_done(v)) return;
-  scan_flag_blocks(v.R.scalars[kT5CandTotal], [&](uint32_t L) { return t5_passed(v, L); }, v.FS.pre, v.FS.sums);
+  scan_flag_blocks(v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); }, v.FS.pre, v.FS.sums);
 }
 __global__ __launch_bounds__(1024) void k_t5_scan_sums(EventView v) {
   if (!t5_flat_done(v)) return;
-  scan_flag_sums(v.R.scalars[kT5CandTotal], v.FS.sums);
+  scan_flag_sums(v.R.scalars[kT5ListA], v.FS.sums);
 }
 
+// list A is in serial order (eligible modules, inner T3s and each
+// mini-doublet's T3s in order), so per module the passing entries take the
+// slots in list order
 __global__ __launch_bounds__(256) void k_t5_place(EventView v) {
   if (!t5_flat_done(v)) return;
-  const uint32_t total = v.R.scalars[kT5CandTotal], nElig = v.R.scalars[kNEligibleT5];
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
-    if (!t5_passed(v, L)) continue;
-    uint32_t lo = 0, hi = nElig;  // largest e with modOff[e] <= L
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (v.W5.modOff[mid] <= L) lo = mid;
-      else hi = mid;
-    }
-    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[lo];
-    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W5.modOff[lo], total);
+  const uint32_t total = v.R.scalars[kT5ListA], nElig = v.R.scalars[kNEligibleT5];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+    if (!t5_passed(v, k)) continue;
+    const uint32_t e = v.W5.listA[3 * k];
+    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
+    const uint32_t rank = flag_rank(v.FS, k, total) - flag_rank(v.FS, v.W5.survOff[e], total);
     if ((int)rank >= v.R.quintupletModuleOccupancy[m1]) continue;
-    const uint32_t* w = &v.W5.results[kT5ResultWords * v.W5.resIdx[L]];
+    const uint32_t* w = &v.W5.results[kT5ResultWords * v.W5.resIdx[k]];
     T5Result r;
     r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
     r.regR = __uint_as_float(w[5]); r.regG = __uint_as_float(w[6]); r.regF = __uint_as_float(w[7]);
@@ -603,9 +615,10 @@ __global__ __launch_bounds__(256) void k_t5_place(EventView v) {
   }
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nElig; e += gridDim.x * blockDim.x) {
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
-    const uint32_t count = flag_rank(v.FS, v.W5.modOff[e + 1], total) - flag_rank(v.FS, v.W5.modOff[e], total);
+    const uint32_t count = flag_rank(v.FS, v.W5.survOff[e + 1], total) - flag_rank(v.FS, v.W5.survOff[e], total);
     v.R.totOccupancyQuintuplets[m1] = count;
     v.R.nQuintuplets[m1] = min(count, (unsigned)max(v.R.quintupletModuleOccupancy[m1], 0));
+    t5_range(v, m1, v.R.nQuintuplets[m1]);
   }
 }
 
@@ -670,73 +683,10 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
   if (threadIdx.x == 0) {
     v.R.totOccupancyQuintuplets[m1] = count;
     v.R.nQuintuplets[m1] = min(count, (unsigned)max(cap, 0));
+    t5_range(v, m1, v.R.nQuintuplets[m1]);
   }
 }
 
-// removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249).  For a close
-// pair (i < j) sharing >= 7 hits the reference flags i when score_i >= score_j
-// and j otherwise.  A module's partners are split into kAfterBuildSplit equal
-// shares (blockIdx.y); each lane owns one T5 and tests the partners of its
-// share, so a flag is a store of 1 that does not depend on evaluation order.
-__global__ __launch_bounds__(kT5Block) void k_t5_dup_after_build(EventView v) {
-  __shared__ float sEta[kMaxT5PerModule], sPhi[kMaxT5PerModule], sScore[kMaxT5PerModule];
-  __shared__ uint32_t sHits[kMaxT5PerModule * 10];
-  if (blockIdx.x >= v.R.scalars[kNEligibleT5]) return;
-  const uint16_t m = v.R.indicesOfEligibleT5Modules[blockIdx.x];
-  const uint32_t nq = v.R.nQuintuplets[m];
-  if (nq < 2 || nq > (uint32_t)kMaxT5PerModule) return;
-  const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
-  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
-    sEta[i] = v.Q.eta[base + i];
-    sPhi[i] = v.Q.phi[base + i];
-    sScore[i] = v.Q.score_rphisum[base + i];
-  }
-  for (uint32_t i = threadIdx.x; i < nq * 10; i += kT5Block) sHits[i] = v.Q.hitIndices[10 * base + i];
-  __syncthreads();
-  const uint32_t share = (nq + gridDim.y - 1) / gridDim.y;
-  const uint32_t jb = blockIdx.y * share, je = jb + share < nq ? jb + share : nq;
-  for (uint32_t i = threadIdx.x; i < nq; i += kT5Block) {
-    bool dup = false;
-    const float si = sScore[i];
-    for (uint32_t j = jb; j < je && !dup; j++) {
-      if (j == i) continue;
-      // the pair can only flag i if the score order points at i
-      const bool targetsI = i < j ? (si >= sScore[j]) : !(sScore[j] >= si);
-      if (!targetsI) continue;
-      const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-      const float dEta = m_abs(sEta[lo] - sEta[hi]);
-      const float dPhi = wrap_dphi(sPhi[lo], sPhi[hi]);
-      if (dEta > 0.1f) continue;
-      if (m_abs(dPhi) > 0.1f) continue;
-      int matched = 0;
-      for (int a = 0; a < 10; a++) {
-        const uint32_t h = sHits[10 * lo + a];
-        for (int b = 0; b < 10; b++)
-          if (h == sHits[10 * hi + b]) {
-            matched++;
-            break;
-          }
-      }
-      dup = matched >= 7;
-    }
-    if (dup) v.Q.isDup[base + i] = 1;
-  }
-}
-
-// addQuintupletRangesToEventExplicit (SDL/Quintuplet.h:3215-3235)
-__global__ void k_t5_ranges(EventView v) {
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
-    const uint32_t n = v.R.nQuintuplets[m];
-    const int base = v.R.quintupletModuleIndices[m];
-    if (n == 0 || base == -1) {
-      v.R.quintupletRanges[2 * m] = -1;
-      v.R.quintupletRanges[2 * m + 1] = -1;
-    } else {
-      v.R.quintupletRanges[2 * m] = base;
-      v.R.quintupletRanges[2 * m + 1] = base + (int)n - 1;
-    }
-  }
-}
 
 }  // namespace
 
@@ -744,10 +694,9 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_t3_lists, dim3(4096), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_md_t3_lists, dim3(8192), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_res_init, dim3(1024), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
   t.begin(s, kTmT5Final);
@@ -762,11 +711,8 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
   hipLaunchKernelGGL(k_t5_scan_sums, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t5_place, dim3(2048), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
-  t.begin(s, kTmT5DupAfter);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(nBlocks, kAfterBuildSplit), dim3(kT5Block), 0, s, v);
-  t.end(s, kTmT5DupAfter);
-  const unsigned g = (v.M.nLowerModules + 255) / 256;
-  hipLaunchKernelGGL(k_t5_ranges, dim3(g ? g : 1), dim3(256), 0, s, v);
+  launch_t5_list(v, s);
+  launch_t5_dup_after_build(v, s, t);
 }
 
 }  // namespace lst
