@@ -33,7 +33,24 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    "k_hits": 6910022,
+    "k_pixel_segments": 2126231,
+    "k_md_eval": 10781172,
+    "k_ls_eval": 55795064,
+    "k_t3_select": 26924365,
+    "k_t5_final": 28225495,
+    "k_pls_pairs<false>": 914619,
+    "k_pix_stage_a<PT5Kind>": 13681276,
+    "k_pix_stage_a<PT3Kind>": 8786794,
+    "k_t5_tc_pairs": 12200941,
+    "k_pt5_pairs": 3444092,
+    "k_pt3_pairs": 130185,
+    "k_cross_pls_pairs": 1844375,
+    "k_pls_pairs<true>": 12103997,
+    "k_t5_dup_after_build": 9619777,
+    "k_t3_enum": 11414917,
+}
 
 
 def _input_struct(lst, ev):
@@ -199,7 +216,8 @@ def main():
                        "mdcand": int(lst.lib().lst_event_count(ev.h, b"miniDoubletCandidates")),
                        "lscand": int(lst.lib().lst_event_count(ev.h, b"segmentCandidates")),
                        "pt5surv": int(lst.lib().lst_event_count(ev.h, b"pixelQuintupletSurvivors")),
-                       "pt3surv": int(lst.lib().lst_event_count(ev.h, b"pixelTripletSurvivors"))})
+                       "pt3surv": int(lst.lib().lst_event_count(ev.h, b"pixelTripletSurvivors")),
+                       "arena_mib": int(lst.lib().lst_event_count(ev.h, b"arenaMiB"))})
     lst._check(L.lst_event_set_timing(ev.h, 0))
     per_event_out = []
     for i in range(len(events)):
@@ -259,6 +277,7 @@ def main():
             "tc_sets_match": tc_match,
             "physics": {k: physics[k] for k in ("sim_tracks", "efficiency", "fake_rate", "duplicate_rate")},
             "contexts": args.contexts,
+            "device_mib_per_context": counts[0]["arena_mib"],
             "geometry": "OT800_IT615_pt0.8",
             "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],
             "mean_segments": mean_counts["segs"], "mean_triplets": mean_counts["t3"],

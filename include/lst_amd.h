@@ -122,7 +122,8 @@ lst_status lst_event_synchronize(lst_event* ev);
  * middle mini-doublet), "quintupletCandidates" (triplet pairs enumerated),
  * "miniDoubletCandidates" (hit pairs enumerated), "segmentCandidates"
  * (mini-doublet pairs enumerated), "pixelQuintupletSurvivors" and
- * "pixelTripletSurvivors" (pLS-object pairs passing the cheap cuts). */
+ * "pixelTripletSurvivors" (pLS-object pairs passing the cheap cuts), and
+ * "arenaMiB" (device memory of the event handle, MiB). */
 uint32_t lst_event_count(lst_event* ev, const char* what);
 
 /* Replaces the host-copy getters SDL::Event::get{Hits,MiniDoublets,Segments,

@@ -751,6 +751,7 @@ uint32_t lst_event_count(lst_event* ev, const char* what) {
   if (!ev || !what) return 0;
   if (!std::strcmp(what, "hits")) return ev->v.nHits;
   if (!std::strcmp(what, "pixelSegments")) return ev->v.nPLS;
+  if (!std::strcmp(what, "arenaMiB")) return (uint32_t)(ev->arenaBytes >> 20);
   const uint32_t nL = ev->v.M.nLowerModules;
   const char* key = nullptr;
   uint32_t n = nL;
