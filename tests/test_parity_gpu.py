@@ -167,6 +167,22 @@ def test_per_module_builders_match(oracle_geometry, geom_dir, monkeypatch):
     compare(prod, ref, DEEPEST)
 
 
+def test_list_by_count_match(oracle_geometry, geom_dir, monkeypatch):
+    """The per-mini-doublet segment and T3 lists built by direct counting (the
+    path of modules too large for the LDS counting sort) give the same objects
+    as the oracle."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
+    from oracle import orc
+    monkeypatch.setenv("LST_LIST_BY_COUNT", "1")
+    es = lst.LSTESData(geom_dir)
+    ev = synth.generate_event("pu", 8, n_pu=200)
+    prod = lst.Event(es).run_lst(ev, DEEPEST)
+    prod.synchronize()
+    ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
+    compare(prod, ref, DEEPEST)
+
+
 @pytest.mark.parametrize("case", [("muon", 0, 0), ("muon", 1, 0), ("pu", 0, 50), ("pu", 1, 200)])
 def test_t5_dnn_mode_all_stages(engine, oracle_geometry, case):
     """DNN selection of the quintuplets (USE_RZCHI2 + USE_T5_DNN) through the whole chain."""

@@ -209,6 +209,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
   c.take(e.W.plsBinned, plsCap); c.take(e.W.plsBinOff, lst::kEtaBinOffsets);
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
+  c.take(e.W.t5BinRec, t5Cap ? t5Cap : 1); c.take(e.W.t5ListRec, t5Cap ? t5Cap : 1);
   c.take(e.W.pixBinned, pt5Cap + pt3Cap); c.take(e.W.pixBinOff, lst::kEtaBinOffsets);
   c.take(e.W.tcBinned, tcCap); c.take(e.W.tcBinOff, lst::kEtaBinOffsets); c.take(e.W.tcHitSet, lst::kTCHitSet);
   c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
@@ -274,6 +275,10 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   {
     const char* pm = std::getenv("LST_PER_MODULE_BUILDERS");
     e.perModuleBuilders = (pm && pm[0] == '1') ? 1 : 0;
+    // LST_LIST_BY_COUNT=1: every module's per-mini-doublet lists take the
+    // path of the modules too large for the LDS counting sort (same lists)
+    const char* lc = std::getenv("LST_LIST_BY_COUNT");
+    e.listByCount = (lc && lc[0] == '1') ? 1 : 0;
   }
   e.t5Dnn = t5Mode == LST_T5_DNN ? 1 : 0;
   e.pixelModule = g.pixelModuleIndex;

@@ -172,6 +172,7 @@ struct PixelWork {
   uint32_t* t5ListOff;    // [nLower + 1] first list position of each eligible T5 module, then the total
   uint32_t* t5List;       // [T5 capacity] T5 slots of the eligible modules
   uint32_t* t5ListMod;    // [T5 capacity] eligible-list position of the module of each listed T5
+  float4* t5ListRec;      // [T5 capacity] (eta, phi, score, slot bits) of each listed T5
   float* tcEta;           // [TC capacity] direction a pLS is compared with in crossCleanpLS
   float* tcPhi;           // [TC capacity]
   uint8_t* tcKind;        // [TC capacity] 0: not compared, 1: T5, 2: pT3 or pT5
@@ -179,6 +180,7 @@ struct PixelWork {
   uint32_t* plsBinOff;    // [kEtaBinOffsets] first list position of each eta bin, then the end
   uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
   uint32_t* t5BinOff;     // [kEtaBinOffsets]
+  float4* t5BinRec;       // [T5 capacity] the same records, eta bin by eta bin
   uint32_t* tcBinned;     // [TC capacity] track candidates, eta bin by eta bin
   uint32_t* tcBinOff;     // [kEtaBinOffsets]
   int32_t* tcHitSet;      // [kTCHitSet] open-addressing set of the pLS hits of the pT3/pT5 candidates

@@ -6,9 +6,11 @@
 // before the track candidates, :250-320; crossCleanpT3/T5/pLS,
 // SDL/TrackCandidate.h:204-379).  The predicates read only pair values and
 // flags written by earlier kernels, so the flag is an OR over partners and
-// the pairs can be visited in any order.  Each kernel tiles the (x, y) plane:
-// a 256-lane work-group owns 256 x's, stages 256 y's at a time in LDS and
-// strides over the y tiles with gridDim.y; flags are set with stores of a
+// the pairs can be visited in any order.  Where every x meets many y's the
+// kernel tiles the (x, y) plane: a 256-lane work-group owns 256 x's, stages
+// 256 y's at a time in LDS and strides over the y tiles; where an x meets a
+// short range of its own (the T5 rules: its module, or its eta band) one wave
+// per x runs its lanes over that range.  Flags are set with stores of a
 // constant or 32-bit atomic ORs, never read-modify-write races.
 #include "algo_pixel.h"
 #include "stages.h"
@@ -89,34 +91,72 @@ __device__ __forceinline__ int eta_bin(float eta) {
   return b < kEtaBins ? b : kEtaBins - 1;
 }
 
-// one 1024-lane work-group: histogram, offsets, scatter (order inside a bin is free)
-template <class IdOf, class EtaOf>
-__device__ void bin_by_eta(uint32_t n, IdOf idOf, EtaOf etaOf, uint32_t* binned, uint32_t* off) {
+// One 1024-lane work-group: histogram, offsets, scatter (order inside a bin is
+// free).  Item i is load(key(i)) (a struct with a member `eta`); store(position,
+// item) writes it at its binned position.  Up to 1024 * kBinPer items are
+// loaded once and kept in registers for the scatter: each lane issues all its
+// key loads, then all its item loads (indices clamped to n - 1, so that no
+// branch splits them); a larger count loads every item twice.
+constexpr int kBinPer = 16;
+struct SameKey {
+  __device__ uint32_t operator()(uint32_t i) const { return i; }
+};
+template <class Load, class Store, class Key = SameKey>
+__device__ void bin_by_eta(uint32_t n, Load load, Store store, uint32_t* off, Key key = Key()) {
   __shared__ unsigned cnt[kEtaBinOffsets], cur[kEtaBinOffsets];
-  for (int b = threadIdx.x; b < kEtaBinOffsets; b += blockDim.x) cnt[b] = 0;
+  using Item = decltype(load(0u));
+  if (threadIdx.x < kEtaBinOffsets) cnt[threadIdx.x] = 0;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[eta_bin(etaOf(idOf(i)))], 1u);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned run = 0;
-    for (int b = 0; b <= kNanEtaBin; b++) {
-      off[b] = run;
-      cur[b] = run;
-      run += cnt[b];
+  const bool fits = n <= 1024u * kBinPer;
+  Item it[kBinPer];
+  int bin[kBinPer];
+  if (fits && n != 0) {
+    uint32_t k[kBinPer];
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) {
+      const uint32_t i = j * 1024u + threadIdx.x;
+      k[j] = key(i < n ? i : n - 1);
     }
-    off[kNanEtaBin + 1] = run;
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) it[j] = load(k[j]);
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) {
+      bin[j] = j * 1024u + threadIdx.x < n ? eta_bin(it[j].eta) : -1;
+      if (bin[j] >= 0) atomicAdd(&cnt[bin[j]], 1u);
+    }
+  } else if (!fits) {
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) atomicAdd(&cnt[eta_bin(load(key(i)).eta)], 1u);
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const uint32_t id = idOf(i);
-    binned[atomicAdd(&cur[eta_bin(etaOf(id))], 1u)] = id;
+  if (threadIdx.x < 64) {  // exclusive scan of the kNanEtaBin + 1 counts, two bins per lane
+    const uint32_t b0 = 2 * threadIdx.x, b1 = b0 + 1;
+    const unsigned c0 = b0 <= (uint32_t)kNanEtaBin ? cnt[b0] : 0u, c1 = b1 <= (uint32_t)kNanEtaBin ? cnt[b1] : 0u;
+    const unsigned incl = wave_inclusive_scan(c0 + c1), ex = incl - c0 - c1;
+    if (b0 <= (uint32_t)kNanEtaBin) off[b0] = cur[b0] = ex;
+    if (b1 <= (uint32_t)kNanEtaBin) off[b1] = cur[b1] = ex + c0;
+    if (threadIdx.x == 63) off[kNanEtaBin + 1] = incl;
+  }
+  __syncthreads();
+  if (fits && n != 0) {
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++)
+      if (bin[j] >= 0) store(atomicAdd(&cur[bin[j]], 1u), it[j]);
+  } else if (!fits) {
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+      const Item x = load(key(i));
+      store(atomicAdd(&cur[eta_bin(x.eta)], 1u), x);
+    }
   }
 }
+struct BinItem {
+  uint32_t id;
+  float eta;
+};
 
 __global__ __launch_bounds__(1024) void k_bin_pls(EventView v) {
   bin_by_eta(
-      v.R.scalars[kEvNPLS], [](uint32_t i) { return i; }, [&](uint32_t i) { return v.P.eta[i]; }, v.W.plsBinned,
-      v.W.plsBinOff);
+      v.R.scalars[kEvNPLS], [&](uint32_t i) { return BinItem{i, v.P.eta[i]}; },
+      [&](uint32_t pos, const BinItem& t) { v.W.plsBinned[pos] = t.id; }, v.W.plsBinOff);
 }
 
 // y positions a tile meets: [b1, b1 + n1) then [b2, b2 + n2) of the binned list
@@ -450,8 +490,10 @@ __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
     const uint32_t base = (uint32_t)v.R.quintupletModuleIndices[m];
     const uint32_t off = v.W.t5ListOff[e];
     for (uint32_t k = threadIdx.x & 63u; k < nq; k += 64) {
-      v.W.t5List[off + k] = base + k;
+      const uint32_t q = base + k;
+      v.W.t5List[off + k] = q;
       v.W.t5ListMod[off + k] = e;
+      v.W.t5ListRec[off + k] = make_float4(v.Q.eta[q], v.Q.phi[q], v.Q.score_rphisum[q], __uint_as_float(q));
     }
   }
 }
@@ -459,71 +501,51 @@ __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
 // ------------------------------------------------ T5 duplicates after build
 // removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249): within one
 // module, a close pair (i < j, |d eta| and |d phi| <= 0.1) sharing >= 7 hits
-// flags i when score_i >= score_j and j otherwise.  The T5 list runs module
-// by module, so an x tile only meets the T5s of the modules it spans (the y
-// range is split into shares as in the other pair kernels); a flag is a store
-// of 1 that does not depend on evaluation order.
-__global__ __launch_bounds__(kTile) void k_t5_dup_after_build(EventView v) {
-  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
-  __shared__ uint32_t sSlot[kTile], sMod[kTile];
-  __shared__ uint32_t sHits[10 * kTile];
+// flags i when score_i >= score_j and j otherwise.  One wave per listed T5 q
+// runs its lanes over the T5s of q's module (contiguous slots, so the loads
+// coalesce) and flags q when any partner points the rule at it; a flag is a
+// store of 1 that does not depend on evaluation order.
+__global__ __launch_bounds__(256) void k_t5_dup_after_build(EventView v) {
   const uint32_t n = v.R.scalars[kT5Listed];
-  const PairItems items = pair_items(n);
-  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
-    const uint32_t x0 = (it / items.shares) * kTile;
-    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
-    const uint32_t yb = v.W.t5ListOff[v.W.t5ListMod[x0]], ye = v.W.t5ListOff[v.W.t5ListMod[xl] + 1];
-    const YShare sh = share_of(ye - yb, it % items.shares, items.shares);
-    const YShare ys{yb + sh.b, yb + sh.e};
-    const uint32_t xi = x0 + threadIdx.x;
-    const bool active = xi < n;
-    uint32_t q = 0, mod = 0;
-    float ei = 0, pi = 0, si = 0;
-    uint32_t hx[10];
-    if (active) {
-      q = v.W.t5List[xi];
-      mod = v.W.t5ListMod[xi];
-      ei = v.Q.eta[q]; pi = v.Q.phi[q]; si = v.Q.score_rphisum[q];
-    }
-#pragma unroll
-    for (int a = 0; a < 10; a++) hx[a] = active ? v.Q.hitIndices[10 * q + a] : 0u;
+  const unsigned lane = threadIdx.x % kWave;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n; i += waves) {
+    const uint32_t q = v.W.t5List[i], mod = v.W.t5ListMod[i];
+    const uint32_t yb = v.W.t5ListOff[mod], ye = v.W.t5ListOff[mod + 1];
+    const float ei = v.Q.eta[q], pi = v.Q.phi[q], si = v.Q.score_rphisum[q];
     bool dup = false;
-    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
-      __syncthreads();
-      if (y0 + threadIdx.x < ys.e) {
-        const uint32_t k = y0 + threadIdx.x, y = v.W.t5List[k];
-        sSlot[threadIdx.x] = y;
-        sMod[threadIdx.x] = v.W.t5ListMod[k];
-        sEta[threadIdx.x] = v.Q.eta[y]; sPhi[threadIdx.x] = v.Q.phi[y]; sScore[threadIdx.x] = v.Q.score_rphisum[y];
-        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.Q.hitIndices[10 * y + a];
-      }
-      __syncthreads();
-      const uint32_t m = tile_len(ys, y0);
-      if (!active || dup) continue;
-      for (uint32_t t = 0; t < m && !dup; t++) {
-        const uint32_t y = sSlot[t];
-        if (sMod[t] != mod || y == q) continue;
-        // the pair can only flag q if the score order points at it
+    for (uint32_t k0 = yb; k0 < ye && !dup; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      bool d = false;
+      if (k < ye) {
+        const uint32_t y = q + k - i;  // the module's T5s are consecutive slots in list order
         const bool xLo = q < y;
-        const float sj = sScore[t];
-        if (!(xLo ? (si >= sj) : !(sj >= si))) continue;
-        const float dEta = m_abs(xLo ? ei - sEta[t] : sEta[t] - ei);
-        const float dPhi = xLo ? wrap_dphi(pi, sPhi[t]) : wrap_dphi(sPhi[t], pi);
-        if (dEta > 0.1f) continue;
-        if (m_abs(dPhi) > 0.1f) continue;
-        int matched = 0;
-        for (int a = 0; a < 10; a++) {
-          const uint32_t h = xLo ? hx[a] : sHits[t * 10 + a];
-          for (int b = 0; b < 10; b++)
-            if (h == (xLo ? sHits[t * 10 + b] : hx[b])) {
-              matched++;
-              break;
-            }
+        const float sj = v.Q.score_rphisum[y];
+        const float ey = v.Q.eta[y], py = v.Q.phi[y];
+        const float dEta = m_abs(xLo ? ei - ey : ey - ei);
+        const float dPhi = xLo ? wrap_dphi(pi, py) : wrap_dphi(py, pi);
+        // the pair can only flag q if the score order points at it
+        if (y != q && (xLo ? (si >= sj) : !(sj >= si)) && !(dEta > 0.1f) && !(m_abs(dPhi) > 0.1f)) {
+          uint32_t hx[10], hy[10];
+#pragma unroll
+          for (int a = 0; a < 10; a++) {
+            hx[a] = v.Q.hitIndices[10 * q + a];
+            hy[a] = v.Q.hitIndices[10 * y + a];
+          }
+          int matched = 0;
+#pragma unroll
+          for (int a = 0; a < 10; a++) {
+            bool hit = false;
+#pragma unroll
+            for (int b = 0; b < 10; b++) hit = hit || (xLo ? hx[a] == hy[b] : hy[a] == hx[b]);
+            matched += hit ? 1 : 0;
+          }
+          d = matched >= 7;
         }
-        dup = matched >= 7;
       }
+      dup = __ballot(d) != 0ull;
     }
-    if (dup) v.Q.isDup[q] = 1;
+    if (dup && lane == 0) v.Q.isDup[q] = 1;
   }
 }
 
@@ -531,77 +553,108 @@ __global__ __launch_bounds__(kTile) void k_t5_dup_after_build(EventView v) {
 // Among T5s not in a pT5 and not flagged after build, a close pair
 // (|dEta|, |dPhi| <= 0.1, and dR2 < 0.001 or >= 5 shared hits) flags the
 // higher score (ties: the lower index) with bit 2.
+struct T5BinItem {
+  float4 rec;  // (eta, phi, score, slot bits)
+  float eta;
+};
 __global__ __launch_bounds__(1024) void k_bin_t5(EventView v) {
   bin_by_eta(
-      v.R.scalars[kT5Listed], [&](uint32_t i) { return v.W.t5List[i]; }, [&](uint32_t q) { return v.Q.eta[q]; },
-      v.W.t5Binned, v.W.t5BinOff);
+      v.R.scalars[kT5Listed],
+      [&](uint32_t i) {
+        const float4 r = v.W.t5ListRec[i];
+        return T5BinItem{r, r.x};
+      },
+      [&](uint32_t pos, const T5BinItem& t) {
+        v.W.t5Binned[pos] = __float_as_uint(t.rec.w);
+        v.W.t5BinRec[pos] = t.rec;
+      },
+      v.W.t5BinOff);
 }
 
-__global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
-  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
-  __shared__ uint32_t sSlot[kTile];
-  __shared__ uint8_t sOk[kTile];
-  __shared__ uint32_t sHits[10 * kTile];
+// One wave per binned T5 x (not in a pT5, not a duplicate after build); its
+// lanes run over the binned records of x's eta bin, the bins next to it and
+// the NaN bin (every record when x's eta is NaN), kT5TcUnroll records per lane
+// per pass so that their loads overlap.  The partners that pass the direction
+// and score cuts are queued in LDS (with a bit for dR2 < 0.001) and checked
+// 64 at a time: their pT5 and duplicate flags, then the 100-compare hit count
+// unless the direction alone decides, so that this work runs with every lane
+// busy instead of once per pass for a few lanes.
+constexpr uint32_t kT5TcUnroll = 4, kT5TcQueue = 512, kNearBit = 0x80000000u;
+__global__ __launch_bounds__(256) void k_t5_tc_pairs(EventView v) {
+  __shared__ uint32_t sQueue[256 / kWave][kT5TcQueue];
   const uint32_t n = v.R.scalars[kT5Listed];
-  const PairItems items = pair_items(n);
-  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
-    const uint32_t x0 = (it / items.shares) * kTile;
-    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
-    const Band B = tile_band(v.W.t5BinOff, eta_bin(v.Q.eta[v.W.t5Binned[x0]]), eta_bin(v.Q.eta[v.W.t5Binned[xl]]));
-    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
-    const uint32_t xi = x0 + threadIdx.x;
-    uint32_t x = 0;
-    bool active = xi < n;
-    float etaX = 0, phiX = 0, sX = 0;
-    uint32_t hx[10];
-    if (active) {
-      x = v.W.t5Binned[xi];
-      active = !v.Q.partOfPT5[x] && !(v.Q.isDup[x] & 1);
-      etaX = v.Q.eta[x]; phiX = v.Q.phi[x]; sX = v.Q.score_rphisum[x];
-#pragma unroll
-      for (int a = 0; a < 10; a++) hx[a] = v.Q.hitIndices[10 * x + a];
-    }
+  const unsigned lane = threadIdx.x % kWave;
+  uint32_t* queue = sQueue[threadIdx.x / kWave];
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t first = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  for (uint32_t xi = first; xi < n; xi += waves) {
+    const float4 rx = v.W.t5BinRec[xi];
+    const uint32_t x = __float_as_uint(rx.w);
+    if (((unsigned)v.Q.partOfPT5[x] | ((unsigned)v.Q.isDup[x] & 1u)) != 0u) continue;
+    const float etaX = rx.x, phiX = rx.y, sX = rx.z;
+    const int bx = eta_bin(etaX);
+    const Band B = tile_band(v.W.t5BinOff, bx, bx);
+    const uint32_t ny = B.n1 + B.n2;
     bool dup = false;
-    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
-      __syncthreads();
-      if (y0 + threadIdx.x < ys.e) {
-        const uint32_t y = v.W.t5Binned[band_at(B, y0 + threadIdx.x)];
-        sSlot[threadIdx.x] = y;
-        sEta[threadIdx.x] = v.Q.eta[y]; sPhi[threadIdx.x] = v.Q.phi[y]; sScore[threadIdx.x] = v.Q.score_rphisum[y];
-        sOk[threadIdx.x] = !v.Q.partOfPT5[y] && !(v.Q.isDup[y] & 1);
-        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.Q.hitIndices[10 * y + a];
+    uint32_t nq = 0;  // queued partners (the same in every lane)
+    for (uint32_t k0 = 0; k0 < ny && !dup; k0 += kWave * kT5TcUnroll) {
+      float4 ry[kT5TcUnroll];
+#pragma unroll
+      for (uint32_t u = 0; u < kT5TcUnroll; u++) {
+        const uint32_t k = k0 + u * kWave + lane;
+        ry[u] = v.W.t5BinRec[band_at(B, k < ny ? k : ny - 1)];
       }
-      __syncthreads();
-      const uint32_t m = tile_len(ys, y0);
-      if (!active || dup) continue;
-      for (uint32_t t = 0; t < m && !dup; t++) {
-        const uint32_t y = sSlot[t];
-        if (y == x || !sOk[t]) continue;
-        const float sY = sScore[t];
-        if (!(sX > sY || (!(sX < sY) && x < y))) continue;
+#pragma unroll
+      for (uint32_t u = 0; u < kT5TcUnroll; u++) {
+        const uint32_t y = __float_as_uint(ry[u].w);
+        const float sY = ry[u].z;
         const bool xLo = x < y;
-        const float dEta = m_abs(xLo ? etaX - sEta[t] : sEta[t] - etaX);
-        const float dPhi = xLo ? wrap_dphi(phiX, sPhi[t]) : wrap_dphi(sPhi[t], phiX);
-        if (dEta > 0.1f) continue;
-        if (m_abs(dPhi) > 0.1f) continue;
-        const float dR2 = dEta * dEta + dPhi * dPhi;
-        if (dR2 < 0.001f) {
-          dup = true;
-          break;
+        const float dEta = m_abs(xLo ? etaX - ry[u].x : ry[u].x - etaX);
+        const float dPhi = xLo ? wrap_dphi(phiX, ry[u].y) : wrap_dphi(ry[u].y, phiX);
+        const bool close = k0 + u * kWave + lane < ny && y != x && (sX > sY || (!(sX < sY) && x < y)) &&
+                           !(dEta > 0.1f) && !(m_abs(dPhi) > 0.1f);
+        const bool near = dEta * dEta + dPhi * dPhi < 0.001f;
+        const unsigned long long m = __ballot(close);
+        if (close) queue[nq + lanes_below(m)] = y | (near ? kNearBit : 0u);
+        nq += (uint32_t)__popcll(m);
+      }
+      const bool last = k0 + kWave * kT5TcUnroll >= ny;
+      if (nq != 0 && (last || nq > kT5TcQueue - kWave * kT5TcUnroll)) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        bool d = false;
+        for (uint32_t c0 = 0; c0 < nq; c0 += kWave) {
+          if (c0 + lane >= nq) continue;
+          const uint32_t e = queue[c0 + lane], y = e & ~kNearBit;
+          if (((unsigned)v.Q.partOfPT5[y] | ((unsigned)v.Q.isDup[y] & 1u)) != 0u) continue;
+          if (e & kNearBit) {
+            d = true;
+            continue;
+          }
+          const bool xLo = x < y;
+          uint32_t hx[10], hy[10];
+#pragma unroll
+          for (int a = 0; a < 10; a++) {
+            hx[a] = v.Q.hitIndices[10 * x + a];
+            hy[a] = v.Q.hitIndices[10 * y + a];
+          }
+          int matched = 0;
+#pragma unroll
+          for (int a = 0; a < 10; a++) {
+            bool hit = false;
+#pragma unroll
+            for (int b = 0; b < 10; b++) hit = hit || (xLo ? hx[a] == hy[b] : hy[a] == hx[b]);
+            matched += hit ? 1 : 0;
+          }
+          d = d || matched >= 5;
         }
-        int matched = 0;
-        for (int a = 0; a < 10; a++) {
-          const uint32_t h = xLo ? hx[a] : sHits[t * 10 + a];
-          for (int b = 0; b < 10; b++)
-            if (h == (xLo ? sHits[t * 10 + b] : hx[b])) {
-              matched++;
-              break;
-            }
-        }
-        dup = matched >= 5;
+        dup = __ballot(d) != 0ull;
+        nq = 0;
+        __builtin_amdgcn_wave_barrier();
       }
     }
-    if (dup) or_byte(v.Q.isDup, x, 2u);
+    if (dup && lane == 0) or_byte(v.Q.isDup, x, 2u);
   }
 }
 
@@ -613,8 +666,9 @@ __global__ __launch_bounds__(kTile) void k_t5_tc_pairs(EventView v) {
 __global__ __launch_bounds__(1024) void k_bin_pix(EventView v) {
   const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
   bin_by_eta(
-      n5 + v.R.scalars[kNPixelTriplets], [](uint32_t i) { return i; },
-      [&](uint32_t j) { return j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5]; }, v.W.pixBinned, v.W.pixBinOff);
+      n5 + v.R.scalars[kNPixelTriplets],
+      [&](uint32_t j) { return BinItem{j, j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5]}; },
+      [&](uint32_t pos, const BinItem& t) { v.W.pixBinned[pos] = t.id; }, v.W.pixBinOff);
 }
 
 __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
@@ -722,8 +776,8 @@ __global__ void k_tc_prep(EventView v) {
 
 __global__ __launch_bounds__(1024) void k_bin_tc(EventView v) {
   bin_by_eta(
-      v.R.scalars[kNTrackCandidates], [](uint32_t i) { return i; }, [&](uint32_t t) { return v.W.tcEta[t]; },
-      v.W.tcBinned, v.W.tcBinOff);
+      v.R.scalars[kNTrackCandidates], [&](uint32_t t) { return BinItem{t, v.W.tcEta[t]}; },
+      [&](uint32_t pos, const BinItem& t) { v.W.tcBinned[pos] = t.id; }, v.W.tcBinOff);
 }
 
 __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
@@ -818,14 +872,14 @@ void launch_t5_list(const EventView& v, hipStream_t s) {
 
 void launch_t5_dup_after_build(const EventView& v, hipStream_t s, const KernelTimers& t) {
   t.begin(s, kTmT5DupAfter);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(kPairItems), dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmT5DupAfter);
 }
 
 void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT5DupTC);
-  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
+  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(2048), dim3(256), 0, s, v);
   t.end(s, kTmT5DupTC);
 }
 

@@ -258,9 +258,8 @@ __global__ __launch_bounds__(1024) void k_md_cand_scan(EventView v) {
   const unsigned total = scan_wide<kScanPer>(
       nL,
       [&](uint32_t m) {  // the counts are int8 as in the reference (<= 127 hits)
-        if (v.R.hitRangesLower[m] == -1) return 0u;
-        const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
-        return nLo > 0 && nUp > 0 ? (unsigned)(nLo * nUp) : 0u;
+        const int lo = v.R.hitRangesLower[m], nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+        return (lo != -1) & (nLo > 0) & (nUp > 0) ? (unsigned)(nLo * nUp) : 0u;
       },
       [&](uint32_t m, unsigned p, unsigned) { v.W2.candOff[m] = p; }, lds);
   if (threadIdx.x == 0) {
@@ -434,7 +433,7 @@ __global__ __launch_bounds__(256) void k_ls_cand_count(EventView v) {
       const uint32_t c = k < nConn ? nIn * v.R.nMDs[v.M.moduleMap[(size_t)m * v.M.stride + k]] : 0;
       const unsigned incl = wave_inclusive_scan(c);
       if (k < nConn) v.WL.connPre[(size_t)m * S + k] = run + incl - c;
-      run += __shfl(incl, kWave - 1, kWave);
+      run += wave_last(incl);
     }
     if (lane == 0) {
       v.WL.connPre[(size_t)m * S + nConn] = run;
@@ -638,7 +637,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
 __global__ __launch_bounds__(1024) void k_triplet_offsets(EventView v) {
   __shared__ unsigned lds[17];
   const unsigned total = scan_wide<kScanPer>(
-      v.M.nLowerModules, [&](uint32_t m) { return v.R.nSegments[m] != 0 ? (unsigned)v.t3OccIfAny[m] : 0u; },
+      v.M.nLowerModules,
+      [&](uint32_t m) {
+        const unsigned ns = v.R.nSegments[m], occ = (unsigned)v.t3OccIfAny[m];
+        return ns != 0 ? occ : 0u;
+      },
       [&](uint32_t m, unsigned p, unsigned occ) {
         v.R.tripletModuleOccupancy[m] = (int)occ;
         v.R.tripletModuleIndices[m] = (int)p;
@@ -736,8 +739,8 @@ __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
         v.W3.candPre[segBase + a] = run + incl - cnt;
         v.W3.survPre[segBase + a] = srun + sincl - sc;
       }
-      run += __shfl(incl, kWave - 1, kWave);
-      srun += __shfl(sincl, kWave - 1, kWave);
+      run += wave_last(incl);
+      srun += wave_last(sincl);
     }
     if (lane == 0) {
       v.W3.modOff[m1] = run;
@@ -747,14 +750,16 @@ __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
 }
 
 __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
-  __shared__ unsigned lds[17];
+  __shared__ unsigned lds[34];
   const uint32_t n = v.M.nLowerModules;
-  const unsigned total = scan_wide<kScanPer>(
-      n, [&](uint32_t m) { return v.W3.modOff[m]; }, [&](uint32_t m, unsigned p, unsigned) { v.W3.modOff[m] = p; },
+  const UVec<2> tot = scan_wide_n<kScanPer, 2>(
+      n, [&](uint32_t m) { return UVec<2>{{v.W3.modOff[m], v.W3.survOff[m]}}; },
+      [&](uint32_t m, const UVec<2>& p, const UVec<2>&) {
+        v.W3.modOff[m] = p.v[0];
+        v.W3.survOff[m] = p.v[1];
+      },
       lds);
-  const unsigned surv = scan_wide<kScanPer>(
-      n, [&](uint32_t m) { return v.W3.survOff[m]; }, [&](uint32_t m, unsigned p, unsigned) { v.W3.survOff[m] = p; },
-      lds);
+  const unsigned total = tot.v[0], surv = tot.v[1];
   if (threadIdx.x == 0) {
     v.W3.modOff[n] = total;
     v.W3.survOff[n] = surv;
@@ -773,45 +778,17 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
 // segments and writes, for each segment listed under X, the candidate at the
 // position those counts give.  The list is thereby in the reference's serial
 // order (module, inner segment, outer segment).
-__global__ __launch_bounds__(kWave) void k_md_seg_lists(EventView v) {
-  const unsigned lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_md_seg_lists(EventView v) {
+  __shared__ __align__(16) ListLds sLists[4];
+  ListLds& sX = sLists[threadIdx.x / kWave];
   const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t m = blockIdx.x; m < nL; m += gridDim.x) {
-    const uint32_t nMD = v.R.nMDs[m];
+  for (uint32_t m = wave_index(); m < nL; m += wave_count()) {
+    const uint32_t nMD = v.R.nMDs[m], nS = v.R.nSegments[m];
+    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], segBase = (uint32_t)v.segModuleIndices[m];
     if (nMD == 0) continue;
-    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], nS = v.R.nSegments[m];
-    const uint32_t segBase = (uint32_t)v.segModuleIndices[m];
-    __syncthreads();
-    for (uint32_t k = lane; k < nMD; k += kWave) v.W3.mdSegEnd[mdBase + k] = 0;
-    __syncthreads();
-    for (uint32_t k = lane; k < nS; k += kWave) atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * (segBase + k)]], 1u);
-    __syncthreads();
-    uint32_t run = segBase;
-    for (uint32_t k0 = 0; k0 < nMD; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      const uint32_t c = k < nMD ? v.W3.mdSegEnd[mdBase + k] : 0u;
-      const uint32_t incl = wave_inclusive_scan(c);
-      if (k < nMD) {
-        v.W3.mdSegOff[mdBase + k] = run + incl - c;
-        v.W3.mdSegEnd[mdBase + k] = run + incl - c;  // cursor of the scatter, the end afterwards
-      }
-      run += __shfl(incl, kWave - 1, kWave);
-    }
-    __syncthreads();
-    for (uint32_t k = lane; k < nS; k += kWave) {
-      const uint32_t s = segBase + k;
-      v.W3.segTmp[atomicAdd(&v.W3.mdSegEnd[v.S.mdIndices[2 * s]], 1u)] = s;
-    }
-    __syncthreads();
-    // slot order inside each list: a segment's position is the number of
-    // segments of its list with a lower slot (a list holds a few segments)
-    for (uint32_t k = lane; k < nS; k += kWave) {
-      const uint32_t s = segBase + k, X = v.S.mdIndices[2 * s];
-      const uint32_t b = v.W3.mdSegOff[X], e = v.W3.mdSegEnd[X];
-      uint32_t r = b;
-      for (uint32_t q = b; q < e; q++) r += v.W3.segTmp[q] < s ? 1u : 0u;
-      v.W3.segByMD[r] = s;
-    }
+    md_lists(
+        nMD, mdBase, nS, segBase, [&](uint32_t k) { return v.S.mdIndices[2 * (segBase + k)]; }, v.W3.mdSegOff,
+        v.W3.mdSegEnd, v.W3.segByMD, v.W3.segTmp, sX, v.listByCount);
   }
 }
 
@@ -910,7 +887,7 @@ __global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
       const uint32_t cnt = a < nIn ? v.R.nSegments[v.S.outerLowerModuleIndices[segBase + a]] : 0;
       const uint32_t incl = wave_inclusive_scan(cnt);
       if (a < nIn) pre[a + 1] = run + incl;
-      run += __shfl(incl, 63, 64);
+      run += wave_last(incl);
     }
     if (lane == 0) pre[0] = 0;
     __syncthreads();
@@ -1058,7 +1035,7 @@ void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
 
 void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
   hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_seg_lists, dim3(8192), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_md_seg_lists, dim3((v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
   t.begin(s, kTmT3Filter);

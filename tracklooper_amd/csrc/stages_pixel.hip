@@ -97,7 +97,7 @@ __device__ __forceinline__ uint32_t conn_total(uint32_t incl, uint32_t size, uns
   if (lane < size) pre[lane + 1] = incl;
   if (lane == 0) pre[0] = 0;
   __syncthreads();
-  return __shfl(incl, kWave - 1, kWave);
+  return wave_last(incl);
 }
 template <class K>
 __device__ __forceinline__ uint32_t conn_prefix(const EventView& v, uint32_t i, uint32_t* pre, uint32_t* objBase,
@@ -206,8 +206,10 @@ struct PT3Kind {
   __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt3MaskBase; }
   __host__ __device__ static uint32_t* count(const EventView& v) { return v.W.pt3Count; }
   __device__ static bool pls_ok(const EventView& v, uint32_t i) { return v.P.isDup[i] == 0 && v.P.partOfPT5[i] == 0; }
-  __device__ static bool module_ok(const EventView& v, uint32_t m) {
-    return v.M.moduleType[m] != kTwoS && (v.M.layers[m] == 1 || v.M.layers[m] == 2);
+  __device__ static bool module_ok(const EventView& v, uint32_t m) {  // both loads issued, no branch
+    const auto type = v.M.moduleType[m];
+    const auto layer = v.M.layers[m];
+    return (type != kTwoS) & ((layer == 1) | (layer == 2));
   }
   __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nTriplets[m]; }
   __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.tripletModuleIndices[m]; }
@@ -274,7 +276,12 @@ struct PT3Kind {
 __global__ __launch_bounds__(1024) void k_t3_list_offsets(EventView v) {
   __shared__ unsigned lds[17];
   const unsigned total = scan_wide<kScanPer>(
-      v.M.nLowerModules, [&](uint32_t m) { return PT3Kind::module_ok(v, m) ? PT3Kind::n_obj(v, m) : 0u; },
+      v.M.nLowerModules,
+      [&](uint32_t m) {
+        const bool ok = PT3Kind::module_ok(v, m);
+        const uint32_t n = PT3Kind::n_obj(v, m);
+        return ok ? n : 0u;
+      },
       [&](uint32_t m, unsigned p, unsigned) { v.W.t3ListOff[m] = p; }, lds);
   if (threadIdx.x == 0) v.R.scalars[kT3Listed] = total;
 }

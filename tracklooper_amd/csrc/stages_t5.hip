@@ -38,24 +38,27 @@ __device__ __forceinline__ void t5_range(const EventView& v, uint32_t m, uint32_
 // seed modules holding a triplet, their capacities and the exclusive scan of
 // the capacities in module order.
 __global__ __launch_bounds__(1024) void k_t5_eligible(EventView v) {
-  __shared__ unsigned lds[17];
+  __shared__ unsigned lds[34];
   const uint32_t nL = v.M.nLowerModules;
-  const auto elig = [&](uint32_t m) { return v.t5OccIfAny[m] >= 0 && v.R.nTriplets[m] != 0; };
-  const unsigned totOcc = scan_wide<kScanPer>(
-      nL, [&](uint32_t m) { return elig(m) ? (unsigned)v.t5OccIfAny[m] : 0u; },
-      [&](uint32_t m, unsigned p, unsigned occ) {
-        v.R.quintupletModuleIndices[m] = elig(m) ? (int)p : -1;
-        v.R.quintupletModuleOccupancy[m] = (int)occ;
+  // the capacities and the eligibility flags scanned together; both loads
+  // issued with no branch, and the puts use the scanned values only
+  const UVec<2> tot = scan_wide_n<kScanPer, 2>(
+      nL,
+      [&](uint32_t m) {
+        const int occ = v.t5OccIfAny[m];
+        const auto nt = v.R.nTriplets[m];
+        const bool e = (occ >= 0) & (nt != 0);
+        return UVec<2>{{e ? (unsigned)occ : 0u, e ? 1u : 0u}};
+      },
+      [&](uint32_t m, const UVec<2>& p, const UVec<2>& x) {
+        v.R.quintupletModuleIndices[m] = x.v[1] ? (int)p.v[0] : -1;
+        v.R.quintupletModuleOccupancy[m] = (int)x.v[0];
         v.R.nQuintuplets[m] = 0;
         v.R.totOccupancyQuintuplets[m] = 0;
+        if (x.v[1]) v.R.indicesOfEligibleT5Modules[p.v[1]] = (uint16_t)m;
       },
       lds);
-  const unsigned totElig = scan_wide<kScanPer>(
-      nL, [&](uint32_t m) { return elig(m) ? 1u : 0u; },
-      [&](uint32_t m, unsigned p, unsigned e) {
-        if (e) v.R.indicesOfEligibleT5Modules[p] = (uint16_t)m;
-      },
-      lds);
+  const unsigned totOcc = tot.v[0], totElig = tot.v[1];
   if (threadIdx.x == 0) {
     v.R.scalars[kNEligibleT5] = totElig;
     v.R.scalars[kNTotalQuints] = totOcc;
@@ -352,46 +355,18 @@ __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.
 // Per mini-doublet, the T3s whose first mini-doublet it is, in slot order (one
 // wave per lower module: counts, offsets, a scatter inside the module's own T3
 // range, then each T3 placed at its rank within its short list).
-__global__ __launch_bounds__(kWave) void k_md_t3_lists(EventView v) {
-  const unsigned lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_md_t3_lists(EventView v) {
+  __shared__ __align__(16) ListLds sLists[4];
+  ListLds& sX = sLists[threadIdx.x / kWave];
   const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t m = blockIdx.x; m < nL; m += gridDim.x) {
-    const uint32_t nMD = v.R.nMDs[m];
-    if (nMD == 0) continue;
-    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], nT = v.R.nTriplets[m];
+  for (uint32_t m = wave_index(); m < nL; m += wave_count()) {
+    const uint32_t nMD = v.R.nMDs[m], nT = v.R.nTriplets[m];
+    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m];
     const uint32_t t3Base = nT ? (uint32_t)v.R.tripletModuleIndices[m] : 0u;
-    __syncthreads();
-    for (uint32_t k = lane; k < nMD; k += kWave) v.W5.mdT3End[mdBase + k] = 0;
-    __syncthreads();
-    for (uint32_t k = lane; k < nT; k += kWave)
-      atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * (t3Base + k)]]], 1u);
-    __syncthreads();
-    uint32_t run = t3Base;
-    for (uint32_t k0 = 0; k0 < nMD; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      const uint32_t c = k < nMD ? v.W5.mdT3End[mdBase + k] : 0u;
-      const uint32_t incl = wave_inclusive_scan(c);
-      if (k < nMD) {
-        v.W5.mdT3Off[mdBase + k] = run + incl - c;
-        v.W5.mdT3End[mdBase + k] = run + incl - c;  // cursor of the scatter, the end afterwards
-      }
-      run += __shfl(incl, kWave - 1, kWave);
-    }
-    __syncthreads();
-    for (uint32_t k = lane; k < nT; k += kWave) {
-      const uint32_t t = t3Base + k;
-      v.W5.t3Tmp[atomicAdd(&v.W5.mdT3End[v.S.mdIndices[2 * v.T.segmentIndices[2 * t]]], 1u)] = t;
-    }
-    __syncthreads();
-    // slot order inside each list: a T3's position is the number of T3s of
-    // its list with a lower slot (a list holds a few T3s)
-    for (uint32_t k = lane; k < nT; k += kWave) {
-      const uint32_t t = t3Base + k, X = v.S.mdIndices[2 * v.T.segmentIndices[2 * t]];
-      const uint32_t b = v.W5.mdT3Off[X], e = v.W5.mdT3End[X];
-      uint32_t r = b;
-      for (uint32_t q = b; q < e; q++) r += v.W5.t3Tmp[q] < t ? 1u : 0u;
-      v.W5.t3ByMD[r] = t;
-    }
+    if (nMD == 0) continue;
+    md_lists(
+        nMD, mdBase, nT, t3Base, [&](uint32_t k) { return v.S.mdIndices[2 * v.T.segmentIndices[2 * (t3Base + k)]]; },
+        v.W5.mdT3Off, v.W5.mdT3End, v.W5.t3ByMD, v.W5.t3Tmp, sX, v.listByCount);
   }
 }
 
@@ -425,8 +400,8 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
         v.W5.candPre[t3Base + a] = run + incl - cnt;
         v.W5.survPre[t3Base + a] = srun + sincl - sc;
       }
-      run += __shfl(incl, 63, 64);
-      srun += __shfl(sincl, 63, 64);
+      run += wave_last(incl);
+      srun += wave_last(sincl);
     }
     if (lane == 0) {
       v.W5.modOff[e] = run;
@@ -437,14 +412,16 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
 
 // exclusive scan of the per-module counts (count = nEligible, read on the device)
 __global__ __launch_bounds__(1024) void k_t5_cand_scan(EventView v) {
-  __shared__ unsigned lds[17];
+  __shared__ unsigned lds[34];
   const uint32_t n = v.R.scalars[kNEligibleT5];
-  const unsigned total = scan_wide<kScanPer>(
-      n, [&](uint32_t i) { return v.W5.modOff[i]; }, [&](uint32_t i, unsigned p, unsigned) { v.W5.modOff[i] = p; },
+  const UVec<2> tot = scan_wide_n<kScanPer, 2>(
+      n, [&](uint32_t i) { return UVec<2>{{v.W5.modOff[i], v.W5.survOff[i]}}; },
+      [&](uint32_t i, const UVec<2>& p, const UVec<2>&) {
+        v.W5.modOff[i] = p.v[0];
+        v.W5.survOff[i] = p.v[1];
+      },
       lds);
-  const unsigned surv = scan_wide<kScanPer>(
-      n, [&](uint32_t i) { return v.W5.survOff[i]; }, [&](uint32_t i, unsigned p, unsigned) { v.W5.survOff[i] = p; },
-      lds);
+  const unsigned total = tot.v[0], surv = tot.v[1];
   if (threadIdx.x == 0) {
     v.W5.modOff[n] = total;
     v.W5.survOff[n] = surv;
@@ -694,7 +671,7 @@ void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_t3_lists, dim3(8192), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_md_t3_lists, dim3((v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
   hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
   hipLaunchKernelGGL(k_t5_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);

@@ -22,22 +22,23 @@ __device__ __forceinline__ unsigned wave_rank(bool pass, unsigned& count) {
   return lanes_below(m);
 }
 
-// inclusive prefix sum across the 64 lanes of a wave
+// Inclusive prefix sum across the 64 lanes of a wave (every lane active):
+// a Hillis-Steele scan inside each row of 16 lanes with DPP row shifts by 1,
+// 2, 4 and 8 (a lane whose source is outside its row adds 0), then the row
+// totals carried over with the DPP row broadcasts of lane 15 (into rows 1
+// and 3) and lane 31 (into rows 2 and 3).  Register moves only, no LDS.
 __device__ __forceinline__ unsigned wave_inclusive_scan(unsigned v) {
-  const unsigned lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    unsigned t = __shfl_up(v, d, 64);
-    if (lane >= (unsigned)d) v += t;
-  }
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
 }
 
-__device__ __forceinline__ unsigned wave_sum(unsigned v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
+// value of lane 63 (every lane active)
+__device__ __forceinline__ unsigned wave_last(unsigned v) { return (unsigned)__builtin_amdgcn_readlane((int)v, 63); }
 
 // Exclusive prefix sum of a workgroup of BLOCK threads (BLOCK multiple of 64);
 // `lds` must hold BLOCK/64 + 1 words.  Returns the exclusive prefix and the
@@ -64,49 +65,193 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* l
   return r;
 }
 
-// Exclusive scan of n values by one 1024-lane work-group.  Wave w owns the
-// 64 * kPer consecutive values from 64 * kPer * w and loads them row by row
-// (coalesced, every load in flight before any use), scans the rows with a
-// running carry, and one work-group scan of the 16 wave totals joins the
-// waves (n beyond 1024 * kPer takes further rounds).  get(i) gives value i;
-// put(i, prefix, value) receives its exclusive prefix.  Returns the total.
-// Every lane must call it; lds needs 17 entries.
-template <int kPer, class Get, class Put>
-__device__ __forceinline__ unsigned scan_wide(uint32_t n, Get get, Put put, unsigned* lds) {
+// kN unsigned values scanned side by side
+template <int kN>
+struct UVec {
+  unsigned v[kN];
+};
+
+// kN exclusive scans of n values each by one 1024-lane work-group.  Wave w
+// owns the 64 * kPer consecutive values from 64 * kPer * w and loads them row
+// by row (coalesced, every load in flight before any use), scans the rows with
+// a running carry, and wave 0 scans the 16 wave totals to join the waves (n
+// beyond 1024 * kPer takes further rounds); the kN scans run interleaved.
+// get(i) gives the values of i (a UVec<kN>); put(i, prefixes, values)
+// receives their exclusive prefixes.  Returns the totals.  Every lane must
+// call it; lds needs 17 * kN entries.
+template <int kPer, int kN, class Get, class Put>
+__device__ __forceinline__ UVec<kN> scan_wide_n(uint32_t n, Get get, Put put, unsigned* lds) {
   const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  unsigned carry = 0;
+  UVec<kN> carry;
+#pragma unroll
+  for (int c = 0; c < kN; c++) carry.v[c] = 0;
   for (uint32_t b = 0; b < n; b += 1024u * kPer) {
     const uint32_t base = b + 64u * kPer * w + lane;
-    unsigned x[kPer], pre[kPer], run = 0;
+    unsigned x[kN][kPer], pre[kN][kPer], run[kN];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) x[k] = base + 64u * k < n ? get(base + 64u * k) : 0u;
+    for (int k = 0; k < kPer; k++) {  // get() of a clamped index, so that no branch splits the loads
+      const uint32_t i = base + 64u * k;
+      const UVec<kN> g = get(i < n ? i : n - 1);
+#pragma unroll
+      for (int c = 0; c < kN; c++) x[c][k] = i < n ? g.v[c] : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < kN; c++) run[c] = 0;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const unsigned incl = wave_inclusive_scan(x[k]);
-      pre[k] = run + incl - x[k];
-      run += __shfl(incl, 63, 64);
-    }
-    if (lane == 0) lds[w] = run;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned acc = 0;
-      for (int k = 0; k < 16; k++) {
-        const unsigned t = lds[k];
-        lds[k] = acc;
-        acc += t;
-      }
-      lds[16] = acc;
-    }
-    __syncthreads();
-    const unsigned off = carry + lds[w];
 #pragma unroll
-    for (int k = 0; k < kPer; k++)
-      if (base + 64u * k < n) put(base + 64u * k, off + pre[k], x[k]);
-    carry += lds[16];
+      for (int c = 0; c < kN; c++) {
+        const unsigned incl = wave_inclusive_scan(x[c][k]);
+        pre[c][k] = run[c] + incl - x[c][k];
+        run[c] += wave_last(incl);
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < kN; c++) lds[17 * c + w] = run[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+#pragma unroll
+      for (int c = 0; c < kN; c++) {
+        const unsigned t = lane < 16 ? lds[17 * c + lane] : 0u;
+        const unsigned inc = wave_inclusive_scan(t);
+        if (lane < 16) lds[17 * c + lane] = inc - t;
+        if (lane == 15) lds[17 * c + 16] = inc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = base + 64u * k;
+      if (i < n) {
+        UVec<kN> p, val;
+#pragma unroll
+        for (int c = 0; c < kN; c++) {
+          p.v[c] = carry.v[c] + lds[17 * c + w] + pre[c][k];
+          val.v[c] = x[c][k];
+        }
+        put(i, p, val);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kN; c++) carry.v[c] += lds[17 * c + 16];
     __syncthreads();
   }
   return carry;
 }
+
+// One exclusive scan (scan_wide_n with kN = 1): get(i) gives value i,
+// put(i, prefix, value) receives its exclusive prefix; returns the total.
+// lds needs 17 entries.
+template <int kPer, class Get, class Put>
+__device__ __forceinline__ unsigned scan_wide(uint32_t n, Get get, Put put, unsigned* lds) {
+  return scan_wide_n<kPer, 1>(
+             n, [&](uint32_t i) { return UVec<1>{{get(i)}}; },
+             [&](uint32_t i, const UVec<1>& p, const UVec<1>& x) { put(i, p.v[0], x.v[0]); }, lds)
+      .v[0];
+}
 constexpr int kScanPer = 16;  // values per lane of scan_wide over module-sized arrays
+
+// orders this wave's LDS writes before its later LDS reads of other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per mini-doublet of one module, the module's objects (segments or T3s: the
+// slots base .. base + n - 1) that start at it, as the range off[X] .. end[X]
+// of byMD: lists in mini-doublet order, slot order inside a list.  One wave
+// per module, with no work-group barrier (other waves of the work-group work
+// on other modules); first(k) is the global index of the mini-doublet object
+// base + k starts at, one of the module's nMD from mdBase.  Up to kListMDs
+// mini-doublets and kListCap objects: a counting sort in the wave's LDS
+// (counts, offsets, scatter, then each object ranked by slot inside its short
+// list).  A larger module counts positions directly (objects with a lower
+// mini-doublet plus those of the same one with a lower slot) over its keys
+// staged kListCap at a time, with partial counts kept in tmp, off and end
+// (byCount sends every module there).
+constexpr uint32_t kListCap = 512, kListMDs = 128;
+struct ListLds {
+  uint32_t x[kListCap + 4];  // mini-doublet of each object, relative to mdBase
+  uint32_t tmp[kListCap];    // objects grouped by mini-doublet
+  uint32_t cnt[kListMDs];    // counts, then scatter cursors (list ends afterwards)
+  uint32_t off[kListMDs];    // list starts, relative to base
+};
+template <class First>
+__device__ void md_lists(uint32_t nMD, uint32_t mdBase, uint32_t n, uint32_t base, First first, uint32_t* off,
+                         uint32_t* end, uint32_t* byMD, uint32_t* tmp, ListLds& L, bool byCount) {
+  const unsigned lane = threadIdx.x & 63u;
+  if (nMD <= kListMDs && n <= kListCap && !byCount) {
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += 64) L.x[k] = first(k) - mdBase;
+    for (uint32_t k = lane; k < nMD; k += 64) L.cnt[k] = 0;
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += 64) atomicAdd(&L.cnt[L.x[k]], 1u);
+    wave_lds_sync();
+    {
+      const uint32_t b0 = 2 * lane, b1 = b0 + 1;
+      const uint32_t c0 = b0 < nMD ? L.cnt[b0] : 0u, c1 = b1 < nMD ? L.cnt[b1] : 0u;
+      const uint32_t incl = wave_inclusive_scan(c0 + c1), ex = incl - c0 - c1;
+      wave_lds_sync();
+      if (b0 < nMD) {
+        L.off[b0] = L.cnt[b0] = ex;
+        off[mdBase + b0] = base + ex;
+        end[mdBase + b0] = base + ex + c0;
+      }
+      if (b1 < nMD) {
+        L.off[b1] = L.cnt[b1] = ex + c0;
+        off[mdBase + b1] = base + ex + c0;
+        end[mdBase + b1] = base + incl;
+      }
+    }
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += 64) L.tmp[atomicAdd(&L.cnt[L.x[k]], 1u)] = k;
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += 64) {
+      const uint32_t X = L.x[k], b = L.off[X], e = L.cnt[X];
+      uint32_t r = b;
+      for (uint32_t q = b; q < e; q++) r += L.tmp[q] < k ? 1u : 0u;
+      byMD[base + r] = base + k;
+    }
+    return;
+  }
+  const uint4* s4 = reinterpret_cast<const uint4*>(L.x);
+  for (uint32_t c0 = 0; c0 == 0 || c0 < n; c0 += kListCap) {
+    const uint32_t cn = n - c0 < kListCap ? n - c0 : kListCap;
+    wave_lds_sync();
+    for (uint32_t k = lane; k < cn + 4; k += 64) L.x[k] = k < cn ? first(c0 + k) - mdBase : ~0u;  // ~0: none
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += 64) {
+      const uint32_t X = first(k) - mdBase;
+      uint32_t r = 0;
+      for (uint32_t q = 0; q < cn; q += 4) {
+        const uint4 y = s4[q / 4];
+        const uint32_t g = c0 + q;
+        r += (y.x < X) + (y.y < X) + (y.z < X) + (y.w < X);
+        r += ((y.x == X) & (g < k)) + ((y.y == X) & (g + 1 < k)) + ((y.z == X) & (g + 2 < k)) +
+             ((y.w == X) & (g + 3 < k));
+      }
+      tmp[base + k] = (c0 ? tmp[base + k] : 0u) + r;
+    }
+    for (uint32_t k = lane; k < nMD; k += 64) {
+      uint32_t lo = 0, eq = 0;
+      for (uint32_t q = 0; q < cn; q += 4) {
+        const uint4 y = s4[q / 4];
+        lo += (y.x < k) + (y.y < k) + (y.z < k) + (y.w < k);
+        eq += (y.x == k) + (y.y == k) + (y.z == k) + (y.w == k);
+      }
+      off[mdBase + k] = (c0 ? off[mdBase + k] : 0u) + lo;
+      end[mdBase + k] = (c0 ? end[mdBase + k] : 0u) + eq;
+    }
+  }
+  for (uint32_t k = lane; k < n; k += 64) byMD[base + tmp[base + k]] = base + k;
+  for (uint32_t k = lane; k < nMD; k += 64) {
+    const uint32_t lo = off[mdBase + k];
+    off[mdBase + k] = base + lo;
+    end[mdBase + k] = base + lo + end[mdBase + k];
+  }
+}
 
 }  // namespace lst
