@@ -208,6 +208,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1); c.take(e.W.t5ListMod, t5Cap ? t5Cap : 1);
   c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
   c.take(e.W.plsBinned, plsCap); c.take(e.W.plsBinOff, lst::kEtaBinOffsets);
+  c.take(e.W.plsBinRec, plsCap); c.take(e.W.plsBinHits, plsCap);
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
   c.take(e.W.t5BinRec, t5Cap ? t5Cap : 1); c.take(e.W.t5ListRec, t5Cap ? t5Cap : 1);
   c.take(e.W.pixBinned, pt5Cap + pt3Cap); c.take(e.W.pixBinOff, lst::kEtaBinOffsets);

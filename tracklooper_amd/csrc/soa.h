@@ -178,6 +178,8 @@ struct PixelWork {
   uint8_t* tcKind;        // [TC capacity] 0: not compared, 1: T5, 2: pT3 or pT5
   uint32_t* plsBinned;    // [pLS capacity] pLS indices listed eta bin by eta bin
   uint32_t* plsBinOff;    // [kEtaBinOffsets] first list position of each eta bin, then the end
+  float4* plsBinRec;      // [pLS capacity] (eta, phi, score, index | isQuad << 31) of each binned pLS
+  uint4* plsBinHits;      // [pLS capacity] its four hits
   uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
   uint32_t* t5BinOff;     // [kEtaBinOffsets]
   float4* t5BinRec;       // [T5 capacity] the same records, eta bin by eta bin

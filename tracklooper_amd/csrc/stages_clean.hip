@@ -97,11 +97,10 @@ __device__ __forceinline__ int eta_bin(float eta) {
 // loaded once and kept in registers for the scatter: each lane issues all its
 // key loads, then all its item loads (indices clamped to n - 1, so that no
 // branch splits them); a larger count loads every item twice.
-constexpr int kBinPer = 16;
 struct SameKey {
   __device__ uint32_t operator()(uint32_t i) const { return i; }
 };
-template <class Load, class Store, class Key = SameKey>
+template <int kBinPer = 16, class Load, class Store, class Key = SameKey>
 __device__ void bin_by_eta(uint32_t n, Load load, Store store, uint32_t* off, Key key = Key()) {
   __shared__ unsigned cnt[kEtaBinOffsets], cur[kEtaBinOffsets];
   using Item = decltype(load(0u));
@@ -153,10 +152,26 @@ struct BinItem {
   float eta;
 };
 
+struct PlsBinItem {
+  float4 rec;  // (eta, phi, score, index | isQuad << 31)
+  uint4 hits;
+  float eta;
+};
 __global__ __launch_bounds__(1024) void k_bin_pls(EventView v) {
-  bin_by_eta(
-      v.R.scalars[kEvNPLS], [&](uint32_t i) { return BinItem{i, v.P.eta[i]}; },
-      [&](uint32_t pos, const BinItem& t) { v.W.plsBinned[pos] = t.id; }, v.W.plsBinOff);
+  bin_by_eta<8>(
+      v.R.scalars[kEvNPLS],
+      [&](uint32_t i) {
+        const float eta = v.P.eta[i];
+        const uint32_t quad = v.P.isQuad[i] ? 1u : 0u;
+        const uint4 h = reinterpret_cast<const uint4*>(v.P.pLSHitsIdxs)[i];
+        return PlsBinItem{make_float4(eta, v.P.phi[i], v.P.score[i], __uint_as_float(i | quad << 31)), h, eta};
+      },
+      [&](uint32_t pos, const PlsBinItem& t) {
+        v.W.plsBinned[pos] = __float_as_uint(t.rec.w) & 0x7fffffffu;
+        v.W.plsBinRec[pos] = t.rec;
+        v.W.plsBinHits[pos] = t.hits;
+      },
+      v.W.plsBinOff);
 }
 
 // y positions a tile meets: [b1, b1 + n1) then [b2, b2 + n2) of the binned list
@@ -179,82 +194,73 @@ __device__ __forceinline__ uint32_t band_at(const Band& B, uint32_t k) {
 // the triplet seed over the quadruplet, else the higher score (ties: lo); the
 // first pass flags it (bit 1) when >= 3 of lo's hits are in hi, the second
 // pass (quadruplets without bit 1 only) flags it (bit 2) on any shared hit or
-// dR2 < 1e-5.  x is the candidate target k, y the partner j.
+// dR2 < 1e-5.  x is the candidate target k, y the partner j.  One wave per
+// binned pLS k; its lanes run over the binned records (direction, score,
+// quadruplet flag and hits, kPlsUnroll per lane per pass) of k's eta bin, the
+// bins next to it and the NaN bin.
+constexpr uint32_t kPlsUnroll = 4, kQuadBit = 0x80000000u;
 template <bool kSecond>
-__global__ __launch_bounds__(kTile) void k_pls_pairs(EventView v) {
-  __shared__ float sEta[kTile], sPhi[kTile], sScore[kTile];
-  __shared__ uint32_t sHits[4 * kTile], sId[kTile];
-  __shared__ uint8_t sQuad[kTile], sOk[kTile];
+__global__ __launch_bounds__(256) void k_pls_pairs(EventView v) {
   const uint32_t n = v.R.scalars[kEvNPLS];
-  const PairItems items = pair_items(n);
-  for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
-    const uint32_t x0 = (it / items.shares) * kTile;
-    const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
-    const Band B = tile_band(v.W.plsBinOff, eta_bin(v.P.eta[v.W.plsBinned[x0]]),
-                             eta_bin(v.P.eta[v.W.plsBinned[xl]]));
-    const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
-    const uint32_t xi = x0 + threadIdx.x;
-    bool active = xi < n;
-    uint32_t k = 0;
-    float etaK = 0, phiK = 0, scoreK = 0;
-    int quadK = 0;
-    uint32_t hk[4] = {0, 0, 0, 0};
-    if (active) {
-      k = v.W.plsBinned[xi];
-      quadK = v.P.isQuad[k] ? 1 : 0;
-      if (kSecond && (!quadK || (v.P.isDup[k] & 1))) active = false;
-      etaK = v.P.eta[k]; phiK = v.P.phi[k]; scoreK = v.P.score[k];
-#pragma unroll
-      for (int a = 0; a < 4; a++) hk[a] = v.P.pLSHitsIdxs[4 * k + a];
-    }
+  const unsigned lane = threadIdx.x % kWave;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t first = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  for (uint32_t xi = first; xi < n; xi += waves) {
+    const float4 ax = v.W.plsBinRec[xi];
+    const uint4 hx = v.W.plsBinHits[xi];
+    const uint32_t k = __float_as_uint(ax.w) & ~kQuadBit;
+    const int quadK = (__float_as_uint(ax.w) & kQuadBit) ? 1 : 0;
+    if (kSecond && (!quadK || (v.P.isDup[k] & 1))) continue;
+    const float etaK = ax.x, phiK = ax.y, scoreK = ax.z;
+    const int bx = eta_bin(etaK);
+    const Band B = tile_band(v.W.plsBinOff, bx, bx);
+    const uint32_t ny = B.n1 + B.n2;
+    const uint32_t hk[4] = {hx.x, hx.y, hx.z, hx.w};
     bool flag = false;
-    for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
-      __syncthreads();
-      if (y0 + threadIdx.x < ys.e) {
-        const uint32_t jl = v.W.plsBinned[band_at(B, y0 + threadIdx.x)];
-        sId[threadIdx.x] = jl;
-        sEta[threadIdx.x] = v.P.eta[jl]; sPhi[threadIdx.x] = v.P.phi[jl]; sScore[threadIdx.x] = v.P.score[jl];
-        sQuad[threadIdx.x] = v.P.isQuad[jl] ? 1 : 0;
-        sOk[threadIdx.x] = kSecond ? (v.P.isQuad[jl] && !(v.P.isDup[jl] & 1)) : 1;
+    for (uint32_t k0 = 0; k0 < ny && !flag; k0 += kWave * kPlsUnroll) {
+      float4 ay[kPlsUnroll];
+      uint4 hy[kPlsUnroll];
 #pragma unroll
-        for (int a = 0; a < 4; a++) sHits[4 * threadIdx.x + a] = v.P.pLSHitsIdxs[4 * jl + a];
+      for (uint32_t u = 0; u < kPlsUnroll; u++) {
+        const uint32_t kk = k0 + u * kWave + lane;
+        const uint32_t at = band_at(B, kk < ny ? kk : ny - 1);
+        ay[u] = v.W.plsBinRec[at];
+        hy[u] = v.W.plsBinHits[at];
       }
-      __syncthreads();
-      const uint32_t m = tile_len(ys, y0);
-      if (!active || flag) continue;
-      for (uint32_t t = 0; t < m && !flag; t++) {
-        const uint32_t j = sId[t];
-        if (j == k) continue;
+      bool f = false;
+#pragma unroll
+      for (uint32_t u = 0; u < kPlsUnroll; u++) {
+        const uint32_t j = __float_as_uint(ay[u].w) & ~kQuadBit;
+        const int quadJ = (__float_as_uint(ay[u].w) & kQuadBit) ? 1 : 0;
         const bool kLo = k < j;
-        const float etaLo = kLo ? etaK : sEta[t], etaHi = kLo ? sEta[t] : etaK;
-        if (m_abs(etaHi - etaLo) > 0.1f) continue;
-        if (!sOk[t]) continue;
-        const int quadDiff = kLo ? quadK - (int)sQuad[t] : (int)sQuad[t] - quadK;
-        const float scoreDiff = kLo ? scoreK - sScore[t] : sScore[t] - scoreK;
+        const float etaLo = kLo ? etaK : ay[u].x, etaHi = kLo ? ay[u].x : etaK;
+        const int quadDiff = kLo ? quadK - quadJ : quadJ - quadK;
+        const float scoreDiff = kLo ? scoreK - ay[u].z : ay[u].z - scoreK;
         // removal target: hi when quadDiff > 0 or (== 0 and scoreDiff < 0), else lo
         const bool rmHi = quadDiff > 0 || (quadDiff == 0 && scoreDiff < 0);
-        if (rmHi == kLo) continue;  // the target is the partner, not k
+        const bool cand = k0 + u * kWave + lane < ny && j != k && !(m_abs(etaHi - etaLo) > 0.1f) &&
+                          (!kSecond || quadJ) && rmHi != kLo;
+        if (!cand) continue;
+        const uint32_t hj[4] = {hy[u].x, hy[u].y, hy[u].z, hy[u].w};
         int matched = 0;
+#pragma unroll
         for (int a = 0; a < 4; a++) {
-          const uint32_t h = kLo ? hk[a] : sHits[4 * t + a];
           bool hit = false;
-          for (int b = 0; b < 4; b++)
-            if (h == (kLo ? sHits[4 * t + b] : hk[b])) { hit = true; break; }
-          if (hit) {
-            matched++;
-            if (kSecond) break;
-          }
+#pragma unroll
+          for (int b = 0; b < 4; b++) hit = hit || (kLo ? hk[a] == hj[b] : hj[a] == hk[b]);
+          matched += hit ? 1 : 0;
         }
         if (!kSecond) {
-          flag = matched >= 3;
-        } else {
+          f = f || matched >= 3;
+        } else if (!(v.P.isDup[j] & 1)) {
           const float dEta = m_abs(etaLo - etaHi);
-          const float dPhi = kLo ? wrap_dphi(phiK, sPhi[t]) : wrap_dphi(sPhi[t], phiK);
-          flag = matched >= 1 || dEta * dEta + dPhi * dPhi < 1e-5f;
+          const float dPhi = kLo ? wrap_dphi(phiK, ay[u].y) : wrap_dphi(ay[u].y, phiK);
+          f = f || matched >= 1 || dEta * dEta + dPhi * dPhi < 1e-5f;
         }
       }
+      flag = __ballot(f) != 0ull;
     }
-    if (flag) or_byte(v.P.isDup, k, kSecond ? 2u : 1u);
+    if (flag && lane == 0) or_byte(v.P.isDup, k, kSecond ? 2u : 1u);
   }
 }
 
@@ -834,12 +840,12 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
 }  // namespace
 
 void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t) {
-  const dim3 g(kPairItems);  // work-groups stride over the (x tile, y share) items
+  const dim3 g(1024);  // 4096 waves stride over the binned pLS
   const int tm = second ? kTmPlsDup2 : kTmPlsDup;
   if (!second) hipLaunchKernelGGL(k_bin_pls, dim3(1), dim3(1024), 0, s, v);  // the eta order is kept for pass 2
   t.begin(s, tm);
-  if (second) hipLaunchKernelGGL(k_pls_pairs<true>, g, dim3(kTile), 0, s, v);
-  else hipLaunchKernelGGL(k_pls_pairs<false>, g, dim3(kTile), 0, s, v);
+  if (second) hipLaunchKernelGGL(k_pls_pairs<true>, g, dim3(256), 0, s, v);
+  else hipLaunchKernelGGL(k_pls_pairs<false>, g, dim3(256), 0, s, v);
   t.end(s, tm);
 }
 

@@ -21,6 +21,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kMaxConn = 64;  // >= the largest connected-module list of a superbin (40)
 constexpr unsigned kPixelWaves = 4096;     // one-wave work-groups striding over the pLS
+constexpr unsigned kPixWG = 256;           // stage A: the waves of a work-group share one pLS
 constexpr unsigned kPixelFullWaves = 256;  // the same for the overflow kernels, which normally return at once
 constexpr unsigned kTCCap = kNMaxPixelTrackCandidates + kNMaxNonPixelTrackCandidates;
 
@@ -117,6 +118,34 @@ __device__ __forceinline__ uint32_t conn_prefix(const EventView& v, uint32_t i, 
   const unsigned incl = wave_inclusive_scan(cnt);
   if (lane < size) objBase[lane] = ob;
   return conn_total(incl, size, lane, pre);
+}
+// conn_prefix for a work-group of several waves: wave 0 fills the LDS, every
+// lane reads the total back
+template <class K>
+__device__ __forceinline__ uint32_t conn_prefix_wg(const EventView& v, uint32_t i, uint32_t* pre, uint32_t* objBase,
+                                                   uint32_t& size) {
+  const unsigned lane = threadIdx.x % kWave;
+  size = v.W.connSize[i];
+  if (threadIdx.x < kWave) {
+    const uint32_t base = v.W.connBase[i];
+    unsigned cnt = 0;
+    uint32_t ob = 0;
+    if (lane < size) {
+      const uint32_t m = v.M.connectedPixels[base + lane];
+      if (m < v.M.nLowerModules && K::module_ok(v, m)) {
+        cnt = K::n_obj(v, m);
+        ob = K::obj_base(v, m);
+      }
+    }
+    const unsigned incl = wave_inclusive_scan(cnt);
+    if (lane < size) {
+      objBase[lane] = ob;
+      pre[lane + 1] = incl;
+    }
+    if (lane == 0) pre[0] = 0;
+  }
+  __syncthreads();
+  return pre[size];
 }
 // the object of candidate L: module k is the largest with pre[k] <= L
 __device__ __forceinline__ uint32_t candidate_object(const uint32_t* pre, const uint32_t* objBase, uint32_t size,
@@ -313,30 +342,35 @@ __global__ __launch_bounds__(256) void k_pix_records(EventView v) {
 }
 
 template <class K>
-__global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
+__global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
   __shared__ uint32_t pre[kMaxConn + 1];
   __shared__ uint32_t objBase[kMaxConn];
+  __shared__ uint32_t sMaskBase, sFull;
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const unsigned lane = threadIdx.x;
+  const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
     __syncthreads();  // the previous pLS's prefix in LDS is no longer read
-    uint32_t total = 0, size = 0, words = 0, maskBase = ~0u;
-    bool full = false;
-    if (K::pls_ok(v, i) && v.W.connSize[i] <= (uint32_t)kMaxConn) {
-      total = conn_prefix<K>(v, i, pre, objBase, size);
-      words = (total + kWave - 1) / kWave;
+    uint32_t total = 0, size = 0;
+    const bool plsOk = K::pls_ok(v, i), connOk = v.W.connSize[i] <= (uint32_t)kMaxConn;
+    if (plsOk & connOk) total = conn_prefix_wg<K>(v, i, pre, objBase, size);  // the same branch in every lane
+    const uint32_t words = (total + kWave - 1) / kWave;
+    if (threadIdx.x == 0) {
+      uint32_t mb = ~0u, full = 0;
       if (words) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&K::shard(v)[i % kPixShards], words);
-        b = __shfl(b, 0, kWave);
-        if (b + words <= kShardWords) maskBase = (i % kPixShards) * kShardWords + b;
-        else full = true;
+        const uint32_t b = atomicAdd(&K::shard(v)[i % kPixShards], words);
+        if (b + words <= kShardWords) mb = (i % kPixShards) * kShardWords + b;
+        else full = 1;
       }
+      sMaskBase = mb;
+      sFull = full;
     }
+    __syncthreads();
+    const uint32_t maskBase = sMaskBase;
+    bool full = false;
     if (maskBase != ~0u) {
       const uint32_t pls = plsBase + i;
-      for (uint32_t w = 0; w < words; w++) {
+      for (uint32_t w = wave; w < words; w += kPixWG / kWave) {
         const uint32_t L = w * kWave + lane;
         typename K::Res r;
         uint32_t t = 0;
@@ -364,7 +398,8 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_a(EventView v) {
         }
       }
     }
-    if (lane == 0) {
+    full = __syncthreads_or(full) || sFull;
+    if (threadIdx.x == 0) {
       K::mask_base(v)[i] = maskBase;
       v.W.pixWords[i] = words;
       K::count(v)[i] = 0;
@@ -433,30 +468,47 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
   }
 }
 
+// One work-group per pLS: its mask words (kPixWG at a time) are staged in
+// LDS with the exclusive prefix of their pass counts, then each wave takes
+// every fourth word and its lanes write the word's passing candidates (the
+// lanes of the set bits) to their slots, in candidate order.
 template <class K>
-__global__ __launch_bounds__(kWave) void k_pix_write(EventView v) {
+__global__ __launch_bounds__(kPixWG) void k_pix_write(EventView v) {
+  __shared__ unsigned long long sMask[kPixWG];
+  __shared__ uint32_t sPre[kPixWG];
+  __shared__ unsigned lds[kPixWG / kWave + 1];
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const unsigned lane = threadIdx.x;
+  const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
     const uint32_t maskBase = K::mask_base(v)[i];
-    if (maskBase == ~0u || v.W.pixFull[i]) continue;
-    const uint32_t words = v.W.pixWords[i], slotBase = K::count(v)[i], pls = plsBase + i;
-    unsigned done = 0;
-    for (uint32_t w = 0; w < words; w++) {
-      const unsigned long long mk = K::mask(v)[maskBase + w];
-      if (mk == 0ull) continue;
-      if ((mk >> lane) & 1ull) {
-        const uint32_t q = slotBase + done + lanes_below(mk);
+    const uint8_t full = v.W.pixFull[i];
+    if (maskBase == ~0u || full) continue;  // the same in every lane
+    const uint32_t words = v.W.pixWords[i], pls = plsBase + i;
+    uint32_t before = K::count(v)[i];
+    for (uint32_t w0 = 0; w0 < words; w0 += kPixWG) {
+      const uint32_t w = w0 + threadIdx.x;
+      const unsigned long long mk = w < words ? K::mask(v)[maskBase + w] : 0ull;
+      unsigned chunk;
+      const unsigned pre = block_exclusive_scan<kPixWG>((unsigned)__popcll(mk), lds, chunk);
+      sMask[threadIdx.x] = mk;
+      sPre[threadIdx.x] = pre;
+      __syncthreads();
+      const uint32_t nw = words - w0 < kPixWG ? words - w0 : kPixWG;
+      for (uint32_t k = wave; k < nw; k += kPixWG / kWave) {
+        const unsigned long long m = sMask[k];
+        if (((m >> lane) & 1ull) == 0) continue;
+        const uint32_t q = before + sPre[k] + lanes_below(m);
         if (q < K::kCap) {
-          const uint32_t s = v.W.pixParkIdx[(maskBase + w) * kWave + lane];
+          const uint32_t s = v.W.pixParkIdx[(maskBase + w0 + k) * kWave + lane];
           const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
           typename K::Res r;
           K::unpack(rec + 3, r);
           K::write(v, q, pls, i, rec[2], r);
         }
       }
-      done += (unsigned)__popcll(mk);
+      before += chunk;
+      __syncthreads();
     }
   }
 }
@@ -729,12 +781,12 @@ template <class K>
 void launch_pixel_builder(const EventView& v, hipStream_t s, const KernelTimers& t, int timer, uint32_t* countScalar,
                           uint32_t* keptScalar) {
   t.begin(s, timer);
-  hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
   t.end(s, timer);
   hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
   hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar,
                      K::kCap, keptScalar);
-  hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
   hipLaunchKernelGGL(k_pix_full_write<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
 }
 
