@@ -195,6 +195,74 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
   return true;
 }
 
+// The flattened builder gives each candidate two lanes (an even lane and the
+// odd one after it), which evaluate the two halves of the tests side by side;
+// the candidate passes when both halves pass.  The per-module builder runs
+// the same tests in one lane (t5_t4_tests, t5_rest).
+
+// half `part` of t5_t4_tests: part 0 the T4 of modules 0-3, part 1 the T4 of
+// modules 0, 1, 3, 4
+__device__ __forceinline__ bool t5_t4_half(const EventView& v, const T5Cand& c, uint32_t part) {
+  const uint16_t mc = part ? c.mods[3] : c.mods[2], md = part ? c.mods[4] : c.mods[3];
+  const uint32_t pc = part ? c.md[3] : c.md[2], pd = part ? c.md[4] : c.md[3];
+  const T4Args q{load_lite(v.M, c.mods[0]), load_lite(v.M, c.mods[1]), load_lite(v.M, mc), load_lite(v.M, md),
+                 load_point(v.D, c.md[0]),  load_point(v.D, c.md[1]),  load_point(v.D, pc),  load_point(v.D, pd),
+                 load_alpha(v.S, c.seg1),   load_alpha(v.S, part ? c.seg4 : c.seg3)};
+  return t4_select(q, v.ptCut);
+}
+
+// half `part` of t5_rest: part 0 the radii, the r-z chi2 and the radius
+// compatibility (filling innerR, bridgeR, outerR, rzChi2, tight); part 1 the
+// regression, the r-phi chi2 cut and the non-anchor chi2 (filling regR, regG,
+// regF, chi2, nonAnchorChi2)
+__device__ __forceinline__ bool t5_rest_half(const EventView& v, const T5Cand& c, uint32_t innerT3,
+                                             uint32_t outerT3, uint32_t part, T5Result& r) {
+  ModLite m[5];
+  MDPoint p[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    m[k] = load_lite(v.M, c.mods[k]);
+    p[k] = load_point(v.D, c.md[k]);
+  }
+  if (part == 0) {
+    float g, f;
+    r.outerR = v.T.circleRadius[outerT3];
+    r.bridgeR = circle_radius3(p[1].x, p[1].y, p[2].x, p[2].y, p[3].x, p[3].y, g, f);
+    r.innerR = v.T.circleRadius[innerT3];
+    g = v.T.circleCenterX[innerT3];
+    f = v.T.circleCenterY[innerT3];
+    const float innerPt = 2 * k2Rinv1GeVf * r.innerR;
+    r.tight = false;
+    if (!t5_rz_chi2(m, p, innerPt, r.innerR, g, f, r.rzChi2, r.tight)) return false;
+    if (!(r.innerR >= 0.95f * v.ptCut / (2.f * k2Rinv1GeVf))) return false;
+    const int cat = radii_category(m);
+    float iMin = 0, iMax = 0, bMin = 0, bMax = 0;
+    if (cat >= 6) radius_range_of(m[0], p[0], m[1], p[1], m[2], p[2], iMin, iMax);
+    if (cat >= 2) radius_range_of(m[1], p[1], m[2], p[2], m[3], p[3], bMin, bMax);
+    return radii_compatible(cat, r.innerR, r.bridgeR, iMin, iMax, bMin, bMax);
+  }
+  float x[5], y[5], d1[5], d2[5], sl[5];
+  bool flat[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    x[k] = p[k].x;
+    y[k] = p[k].y;
+  }
+  regression_sigmas(m, true, d1, d2, sl, flat);
+  r.regR = regression_fit(x, y, d1, d2, sl, flat, r.regG, r.regF, r.chi2);
+  if (!v.t5Dnn && r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
+      !t5_rphi_chi2_ok(m[0].sdl, m[1].sdl, m[2].sdl, m[3].sdl, m[4].sdl, r.chi2))
+    return false;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    x[k] = v.D.outerX[c.md[k]];
+    y[k] = v.D.outerY[c.md[k]];
+  }
+  regression_sigmas(m, false, d1, d2, sl, flat);
+  r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
+  return true;
+}
+
 // ------------------------------------------------------------------ T5 DNN
 // T5DNN::runInference (SDL/NeuralNetwork.h:21-165), the cut of the DNN mode
 // (USE_T5_DNN, SDL/Quintuplet.h:2921-2941).  Parameters: the float tables of
@@ -486,45 +554,76 @@ __global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
 __global__ __launch_bounds__(256) void k_t5_filter_t4(EventView v) {
   if (!t5_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT5ListA];
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-    const uint32_t i = i0 + threadIdx.x;
+  for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < 2 * n; g0 += gridDim.x * blockDim.x) {
+    const uint32_t g = g0 + threadIdx.x, i = g >> 1, part = g & 1;
+    uint32_t inner = 0, outer = 0;
+    bool pass = false;
     if (i < n) {
-      const uint32_t inner = v.W5.listA[3 * i + 1], outer = v.W5.listA[3 * i + 2];
-      const T5Cand c = t5_cand(v, inner, outer);
-      if (t5_t4_tests(v, c)) list_push(v.W5.listB, &v.R.scalars[kT5ListB], i, inner, outer);
+      inner = v.W5.listA[3 * i + 1];
+      outer = v.W5.listA[3 * i + 2];
+      pass = t5_t4_half(v, t5_cand(v, inner, outer), part);
+    }
+    const int other = __shfl_xor((int)pass, 1, kWave);  // every lane of the wave takes part
+    const bool both = pass && other != 0;
+    if (i < n && part == 0) {
+      if (both) list_push(v.W5.listB, &v.R.scalars[kT5ListB], i, inner, outer);
       else v.W5.resIdx[i] = ~0u;
     }
   }
 }
 
-// stage 3: radii, chi2 cuts and the regression; results parked per candidate
+// stage 3: radii, chi2 cuts and the regression; results parked per candidate.
+// The two halves run in different waves (so that neither waits for the
+// other's branch): waves 2p and 2p + 1 of a work-group take halves 0 and 1 of
+// the same 64 candidates, and the odd wave hands its results over in LDS.
 __global__ __launch_bounds__(256) void k_t5_final(EventView v) {
+  __shared__ float sHalf[2][6][kWave];  // per wave pair: pass, regR, regG, regF, chi2, nonAnchorChi2
   if (!t5_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT5ListB];
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-    const uint32_t i = i0 + threadIdx.x;
+  const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, part = wave & 1, pair = wave >> 1;
+  for (uint32_t c0 = blockIdx.x * 2 * kWave; c0 < n; c0 += gridDim.x * 2 * kWave) {
+    const uint32_t i = c0 + pair * kWave + lane;
+    uint32_t L = 0, inner = 0, outer = 0;
+    bool pass = false;
+    T5Result r{};
     if (i < n) {
-      const uint32_t L = v.W5.listB[3 * i], inner = v.W5.listB[3 * i + 1], outer = v.W5.listB[3 * i + 2];
-      const T5Cand c = t5_cand(v, inner, outer);
-      T5Result r;
+      L = v.W5.listB[3 * i];
+      inner = v.W5.listB[3 * i + 1];
+      outer = v.W5.listB[3 * i + 2];
+      pass = t5_rest_half(v, t5_cand(v, inner, outer), inner, outer, part, r);
+    }
+    if (part == 1) {
+      float* h = &sHalf[pair][0][lane];
+      h[0] = pass ? 1.f : 0.f;
+      h[kWave] = r.regR; h[2 * kWave] = r.regG; h[3 * kWave] = r.regF;
+      h[4 * kWave] = r.chi2; h[5 * kWave] = r.nonAnchorChi2;
+    }
+    __syncthreads();
+    if (part == 0 && i < n) {
+      const float* h = &sHalf[pair][0][lane];
       v.W5.resIdx[L] = ~0u;  // L: the list-A position; set below when the candidate passes
-      if (t5_rest(v, c, inner, outer, r)) {
+      if (pass && h[0] != 0.f) {
+        r.regR = h[kWave]; r.regG = h[2 * kWave]; r.regF = h[3 * kWave];
+        r.chi2 = h[4 * kWave]; r.nonAnchorChi2 = h[5 * kWave];
         // wave-aggregated slot: one atomic per wave
         const uint64_t mask = __ballot(1);
         const unsigned leader = __ffsll((unsigned long long)mask) - 1;
         uint32_t base = 0;
         if (lane_id() == leader) base = atomicAdd(&v.R.scalars[kT5Results], (uint32_t)__popcll(mask));
         const uint32_t k = __shfl(base, leader, 64) + lanes_below(mask);
-        if (k >= kT5ResultMax) continue;  // overflow: the per-module builder redoes the stage
-        uint32_t* w = &v.W5.results[kT5ResultWords * k];
-        w[0] = inner; w[1] = outer;
-        w[2] = __float_as_uint(r.innerR); w[3] = __float_as_uint(r.bridgeR); w[4] = __float_as_uint(r.outerR);
-        w[5] = __float_as_uint(r.regR); w[6] = __float_as_uint(r.regG); w[7] = __float_as_uint(r.regF);
-        w[8] = __float_as_uint(r.rzChi2); w[9] = __float_as_uint(r.chi2); w[10] = __float_as_uint(r.nonAnchorChi2);
-        w[11] = r.tight ? 1u : 0u;
-        v.W5.resIdx[L] = k;
+        if (k < kT5ResultMax) {  // overflow: the per-module builder redoes the stage
+          uint32_t* w = &v.W5.results[kT5ResultWords * k];
+          w[0] = inner; w[1] = outer;
+          w[2] = __float_as_uint(r.innerR); w[3] = __float_as_uint(r.bridgeR); w[4] = __float_as_uint(r.outerR);
+          w[5] = __float_as_uint(r.regR); w[6] = __float_as_uint(r.regG); w[7] = __float_as_uint(r.regF);
+          w[8] = __float_as_uint(r.rzChi2); w[9] = __float_as_uint(r.chi2);
+          w[10] = __float_as_uint(r.nonAnchorChi2);
+          w[11] = r.tight ? 1u : 0u;
+          v.W5.resIdx[L] = k;
+        }
       }
     }
+    __syncthreads();  // the LDS halves are read before the next candidates overwrite them
   }
 }
 
