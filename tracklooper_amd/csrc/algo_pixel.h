@@ -382,6 +382,20 @@ __device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, 
   return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut);
 }
 
+// one of the two pixel-tracklet tests of pt3_tracklets: part 0 the inner
+// segment of the T3, part 1 the outer one
+__device__ __forceinline__ bool pt3_tracklet_half(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3,
+                                                  uint32_t part) {
+  const ModulesView& M = v.M;
+  const uint16_t mA = v.T.lowerModuleIndices[3 * t3 + part], mB = v.T.lowerModuleIndices[3 * t3 + 1 + part];
+  const uint32_t seg = v.T.segmentIndices[2 * t3 + part];
+  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
+  const PixelSeed s = load_seed(v.P, arr);
+  const float alphaPix = v.S.dPhiChanges[pls];
+  const uint32_t mdA = v.S.mdIndices[2 * seg], mdB = v.S.mdIndices[2 * seg + 1];
+  return pixel_tracklet(M, v.D, s, pixIn, pixOut, mdA, mdB, mA, mB, alphaPix, v.S.dPhiChanges[seg], v.ptCut);
+}
+
 // runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024) is the AND of pure
 // cuts, evaluated here in two parts: pt3_cheap (radius compatibility, the
 // r-phi and inward r-phi chi2 cuts) and pt3_costly (the two pixel-tracklet
@@ -417,6 +431,28 @@ __device__ __forceinline__ bool pt3_cheap(const EventView& v, uint32_t pls, uint
 __device__ __forceinline__ bool pt3_costly(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3, bool chi2Cuts,
                                            PT3Result& r) {
   if (!pt3_tracklets(v, pls, arr, t3)) return false;
+  const PixelSeed s = load_seed(v.P, arr);
+  if (!(chi2Cuts && s.ptIn < 5.0f)) return true;
+  const ModulesView& M = v.M;
+  const uint16_t mods[3] = {v.T.lowerModuleIndices[3 * t3], v.T.lowerModuleIndices[3 * t3 + 1],
+                            v.T.lowerModuleIndices[3 * t3 + 2]};
+  const uint32_t segIn = v.T.segmentIndices[2 * t3], segOut = v.T.segmentIndices[2 * t3 + 1];
+  const uint32_t md1 = v.S.mdIndices[2 * segIn], md2 = v.S.mdIndices[2 * segIn + 1],
+                 mdC = v.S.mdIndices[2 * segOut + 1];
+  const uint32_t pixOut = v.S.mdIndices[2 * pls + 1];
+  const float rts[3] = {v.D.anchorRt[md1], v.D.anchorRt[md2], v.D.anchorRt[mdC]};
+  const float zs[3] = {v.D.anchorZ[md1], v.D.anchorZ[md2], v.D.anchorZ[mdC]};
+  r.rz = pt3_rz_chi2(M, mods, v.D.anchorX[pixOut], v.D.anchorY[pixOut], v.D.anchorZ[pixOut], v.D.anchorRt[pixOut], rts,
+                     zs, s.px, s.py, s.pz, s.charge);
+  return pt3_rz_ok(pix_layer(M, mods[0]), pix_layer(M, mods[1]), pix_layer(M, mods[2]), r.rz);
+}
+
+// pt3_costly in two parts that can run side by side: part 0 the first
+// pixel-tracklet test, part 1 the second and the r-z chi2 cut (filling r.rz)
+__device__ __forceinline__ bool pt3_costly_half(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3,
+                                                bool chi2Cuts, PT3Result& r, uint32_t part) {
+  if (!pt3_tracklet_half(v, pls, arr, t3, part)) return false;
+  if (part == 0) return true;
   const PixelSeed s = load_seed(v.P, arr);
   if (!(chi2Cuts && s.ptIn < 5.0f)) return true;
   const ModulesView& M = v.M;

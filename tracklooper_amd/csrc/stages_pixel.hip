@@ -178,6 +178,12 @@ struct PT5Kind {
   __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res&) {
     return pt5_costly(v, pls, i, t);
   }
+  // costly in two parts (the two pixel-tracklet tests); no result value changes
+  __device__ static bool costly_half(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res&, uint32_t part) {
+    return pt3_tracklet_half(v, pls, i, v.Q.tripletIndices[2 * t], part);
+  }
+  __device__ static float half_out(const Res&) { return 0.f; }
+  __device__ static void half_in(Res&, float) {}
   __device__ static void pack(const Res& r, uint32_t* w) {
     w[0] = __float_as_uint(r.rz); w[1] = __float_as_uint(r.rphi); w[2] = __float_as_uint(r.rphiIn);
     w[3] = __float_as_uint(r.pixR); w[4] = __float_as_uint(r.t5R); w[5] = __float_as_uint(r.cx);
@@ -251,6 +257,14 @@ struct PT3Kind {
   __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
     return pt3_costly(v, pls, i, t, true, r);
   }
+  // costly in two parts: the first pixel-tracklet test; the second and the
+  // r-z chi2 (whose value part 1 hands over)
+  __device__ static bool costly_half(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r,
+                                     uint32_t part) {
+    return pt3_costly_half(v, pls, i, t, true, r, part);
+  }
+  __device__ static float half_out(const Res& r) { return r.rz; }
+  __device__ static void half_in(Res& r, float rz) { r.rz = rz; }
   __device__ static void pack(const Res& r, uint32_t* w) {
     w[0] = __float_as_uint(r.pixR); w[1] = __float_as_uint(r.t3R); w[2] = __float_as_uint(r.rz);
     w[3] = __float_as_uint(r.rphi); w[4] = __float_as_uint(r.rphiIn);
@@ -413,48 +427,71 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
 // skipped here, and when any pLS is flagged the same waves then evaluate every
 // candidate of each flagged pLS (its count only; the write evaluates again)
 template <class K>
-__global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
+__global__ __launch_bounds__(kPixWG) void k_pix_stage_b(EventView v) {
   __shared__ uint32_t pre[kMaxConn + 1];  // survivors before each shard, then the connected-module prefix
   __shared__ uint32_t objBase[kMaxConn];
+  __shared__ uint32_t sPass[kPixWG / kWave / 2][kWave];
+  __shared__ float sOut[kPixWG / kWave / 2][kWave];
+  __shared__ uint32_t sCount[kPixWG / kWave];
   static_assert(kMaxConn >= (int)kPixShards, "the shard prefix reuses the module prefix");
-  const unsigned lane = threadIdx.x;
-  const uint32_t c = K::shard(v)[kPixShards + lane];
-  const uint32_t incl = wave_inclusive_scan(c < kShardSurv ? c : kShardSurv);
-  pre[lane + 1] = incl;
-  if (lane == 0) pre[0] = 0;
+  const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, part = wave & 1, pair = wave >> 1;
+  if (wave == 0) {
+    const uint32_t c = K::shard(v)[kPixShards + lane];
+    const uint32_t incl = wave_inclusive_scan(c < kShardSurv ? c : kShardSurv);
+    pre[lane + 1] = incl;
+    if (lane == 0) pre[0] = 0;
+  }
   __syncthreads();
   const uint32_t nS = pre[kPixShards];
-  if (blockIdx.x == 0 && lane == 0) v.R.scalars[K::kSurvScalar] = nS;
+  if (blockIdx.x == 0 && threadIdx.x == 0) v.R.scalars[K::kSurvScalar] = nS;
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  for (uint32_t g = blockIdx.x * blockDim.x + lane; g < nS; g += gridDim.x * blockDim.x) {
-    uint32_t lo = 0, hi = kPixShards;  // the shard holding survivor g: largest lo with pre[lo] <= g
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (pre[mid] <= g) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t s = lo * kShardSurv + (g - pre[lo]);
-    uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
-    const uint32_t i = rec[0], slot = rec[1], t = rec[2];
-    if (v.W.pixFull[i]) continue;
+  // waves 2p and 2p + 1 take the two parts of the costly tests of the same
+  // 64 survivors; the odd wave hands its outcome over in LDS
+  constexpr uint32_t kPerPass = kPixWG / 2;
+  for (uint32_t g0 = blockIdx.x * kPerPass; g0 < nS; g0 += gridDim.x * kPerPass) {
+    const uint32_t g = g0 + pair * kWave + lane;
+    bool act = g < nS, pass = false;
+    uint32_t s = 0, i = 0, slot = 0, t = 0;
     typename K::Res r;
-    K::unpack(rec + 3, r);
-    if (K::costly(v, plsBase + i, i, t, r)) {
-      K::pack(r, rec + 3);
+    if (act) {
+      uint32_t lo = 0, hi = kPixShards;  // the shard holding survivor g: largest lo with pre[lo] <= g
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= g) lo = mid;
+        else hi = mid;
+      }
+      s = lo * kShardSurv + (g - pre[lo]);
+      const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+      i = rec[0]; slot = rec[1]; t = rec[2];
+      act = v.W.pixFull[i] == 0;
+      if (act) {
+        K::unpack(rec + 3, r);
+        pass = K::costly_half(v, plsBase + i, i, t, r, part);
+      }
+    }
+    if (part == 1) {
+      sPass[pair][lane] = pass ? 1u : 0u;
+      sOut[pair][lane] = pass ? K::half_out(r) : 0.f;
+    }
+    __syncthreads();
+    if (part == 0 && pass && sPass[pair][lane]) {
+      K::half_in(r, sOut[pair][lane]);
+      K::pack(r, &v.W.pixSurv[(size_t)kSurvWords * s + 3]);
       atomicOr(&K::mask(v)[slot / kWave], 1ull << (slot % kWave));
       v.W.pixParkIdx[slot] = s;
       atomicAdd(&K::count(v)[i], 1u);
     }
+    __syncthreads();  // the LDS outcomes are read before the next survivors overwrite them
   }
   if ((v.R.scalars[kPixFull] & K::kBit) == 0) return;
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
-    if (!v.W.pixFull[i]) continue;
+    if (!v.W.pixFull[i]) continue;  // the same in every lane
     __syncthreads();
     uint32_t size;
-    const uint32_t total = conn_prefix<K>(v, i, pre, objBase, size);
+    const uint32_t total = conn_prefix_wg<K>(v, i, pre, objBase, size);
     unsigned count = 0;
-    for (uint32_t L0 = 0; L0 < total; L0 += kWave) {
+    for (uint32_t L0 = wave * kWave; L0 < total; L0 += kPixWG) {
       const uint32_t L = L0 + lane;
       bool pass = false;
       if (L < total) {
@@ -464,7 +501,13 @@ __global__ __launch_bounds__(kWave) void k_pix_stage_b(EventView v) {
       }
       count += (unsigned)__popcll(__ballot(pass));
     }
-    if (lane == 0) K::count(v)[i] = count;
+    if (lane == 0) sCount[wave] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned c = 0;
+      for (uint32_t w = 0; w < kPixWG / kWave; w++) c += sCount[w];
+      K::count(v)[i] = c;
+    }
   }
 }
 
@@ -783,7 +826,7 @@ void launch_pixel_builder(const EventView& v, hipStream_t s, const KernelTimers&
   t.begin(s, timer);
   hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
   t.end(s, timer);
-  hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kWave), 0, s, v);
+  hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
   hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar,
                      K::kCap, keptScalar);
   hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
