@@ -205,6 +205,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5Count, plsCap + 1); c.take(e.W.pt3Count, plsCap + 1);
   c.take(e.W.t5TcCount, nL + 1);
   c.take(e.W.outHits, (size_t)tcCap * 14); c.take(e.W.outLen, tcCap);
+  c.take(e.W.outPacked, lst::kOutHeaderWords + (size_t)tcCap * lst::kOutRecWords);
   c.take(e.W.t5ListOff, nL + 1); c.take(e.W.t5List, t5Cap ? t5Cap : 1); c.take(e.W.t5ListMod, t5Cap ? t5Cap : 1);
   c.take(e.W.tcEta, tcCap); c.take(e.W.tcPhi, tcCap); c.take(e.W.tcKind, tcCap);
   c.take(e.W.plsBinned, plsCap); c.take(e.W.plsBinOff, lst::kEtaBinOffsets);
@@ -938,7 +939,7 @@ struct lst_batch {
 namespace {
 constexpr uint32_t kTCCap = lst::kNMaxPixelTrackCandidates + lst::kNMaxNonPixelTrackCandidates;
 
-size_t out_bytes(uint32_t cap) { return 256 + (size_t)cap * (14 * 4 + 4 + 4 + 2); }
+size_t out_bytes(uint32_t cap) { return 4 * (lst::kOutHeaderWords + (size_t)cap * lst::kOutRecWords); }
 
 // enqueue the output copies of context c's event into its pinned buffer
 lst_status enqueue_output(lst_batch* b, uint32_t c, lst_output* dst) {
@@ -951,17 +952,8 @@ lst_status enqueue_output(lst_batch* b, uint32_t c, lst_output* dst) {
     HIP_OK(hipHostMalloc((void**)&o.pinned, out_bytes(cap), hipHostMallocDefault));
     o.cap = cap;
   }
-  char* p = o.pinned;
-  HIP_OK(hipMemcpyAsync(p, ev->v.R.scalars + lst::kNTrackCandidates, 4, hipMemcpyDeviceToHost, ev->stream));
-  p += 256;
-  if (cap) {
-    HIP_OK(hipMemcpyAsync(p, ev->v.W.outHits, (size_t)cap * 56, hipMemcpyDeviceToHost, ev->stream));
-    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 56, ev->v.W.outLen, (size_t)cap * 4, hipMemcpyDeviceToHost, ev->stream));
-    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 60, ev->v.C.pixelSeedIndex, (size_t)cap * 4, hipMemcpyDeviceToHost,
-                          ev->stream));
-    HIP_OK(hipMemcpyAsync(p + (size_t)cap * 64, ev->v.C.trackCandidateType, (size_t)cap * 2, hipMemcpyDeviceToHost,
-                          ev->stream));
-  }
+  // the packed block k_tc_output wrote: count, then one record per track candidate
+  HIP_OK(hipMemcpyAsync(o.pinned, ev->v.W.outPacked, out_bytes(cap), hipMemcpyDeviceToHost, ev->stream));
   o.dst = dst;
   return LST_OK;
 }
@@ -979,11 +971,13 @@ lst_status finish_context(lst_batch* b, uint32_t c, lst_status& warn) {
     d->n_tc = n;
     const uint32_t m = std::min(n, cap);
     if (n > d->capacity) warn = LST_W_TRUNCATED;
-    const char* p = o.pinned + 256;
-    if (d->hit_idxs) std::memcpy(d->hit_idxs, p, (size_t)m * 56);
-    if (d->len) std::memcpy(d->len, p + (size_t)cap * 56, (size_t)m * 4);
-    if (d->seed_idx) std::memcpy(d->seed_idx, p + (size_t)cap * 60, (size_t)m * 4);
-    if (d->type) std::memcpy(d->type, p + (size_t)cap * 64, (size_t)m * 2);
+    const uint32_t* rec = reinterpret_cast<const uint32_t*>(o.pinned) + lst::kOutHeaderWords;
+    for (uint32_t t = 0; t < m; t++, rec += lst::kOutRecWords) {
+      if (d->hit_idxs) std::memcpy(d->hit_idxs + 14 * (size_t)t, rec, 56);
+      if (d->len) d->len[t] = rec[14];
+      if (d->seed_idx) d->seed_idx[t] = (int32_t)rec[15];
+      if (d->type) d->type[t] = (int16_t)(int32_t)rec[16];
+    }
   }
   o.pending = -1;
   o.dst = nullptr;

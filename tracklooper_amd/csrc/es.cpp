@@ -211,6 +211,21 @@ bool load_host_geometry(const std::string& dir, uint32_t stride, HostGeometry& g
   std::vector<std::pair<uint32_t, float>> ep(ecPhi.begin(), ecPhi.end());
   std::sort(ep.begin(), ep.end());
   for (auto& e : ep) { g.geoMapDetId.push_back(e.first); g.geoMapPhi.push_back(e.second); }
+  // the hit kernel's detId lookup: one table probe instead of two binary searches
+  g.detHashBits = 10;
+  while ((1u << g.detHashBits) < 4 * nMod) g.detHashBits++;
+  if (g.geoMapDetId.size() >= 0xFFFF) { err = "endcap orientation map too large"; return false; }
+  g.detHash.assign(2u << g.detHashBits, kNoDetId);
+  for (uint32_t k = 0; k < nMod; k++) {
+    const uint32_t det = dets[k];
+    if (det == kNoDetId) { err = "detId 0xFFFFFFFF is reserved"; return false; }
+    const auto it = std::lower_bound(g.geoMapDetId.begin(), g.geoMapDetId.end(), det);
+    const uint32_t ec = (it != g.geoMapDetId.end() && *it == det) ? (uint32_t)(it - g.geoMapDetId.begin()) + 1 : 0;
+    uint32_t s = det_hash_slot(det, g.detHashBits);
+    while (g.detHash[2 * s] != kNoDetId) s = (s + 1) & ((1u << g.detHashBits) - 1);
+    g.detHash[2 * s] = det;
+    g.detHash[2 * s + 1] = (uint32_t)g.mapIdx[k] | ec << 16;
+  }
 
   // pLS superbin maps (SDL/LSTESData.cc:57-74, SDL/ModuleMethods.h:33-112).
   // kind 0 uses keys superbin + 45000 of the unsigned maps, kinds 1/2 the
@@ -325,7 +340,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
          o_pi2 = c.add(h.pixIndex[2]), o_ps0 = c.add(h.pixSizes[0]), o_ps1 = c.add(h.pixSizes[1]),
          o_ps2 = c.add(h.pixSizes[2]), o_mdo = c.add(h.mdOccupancy), o_sgo = c.add(h.segOccupancy),
          o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny),
-         o_t5o = c.add(h.t5OccIfAny);
+         o_t5o = c.add(h.t5OccIfAny), o_dh = c.add(h.detHash);
   char* base = nullptr;
   if (hipMalloc(&base, c.total) != hipSuccess) { err = "hipMalloc failed for geometry"; return false; }
   for (auto& it : c.items)
@@ -351,6 +366,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   v.connectedPixels = (const uint32_t*)P(o_cp); v.geoMapDetId = (const uint32_t*)P(o_gd);
   v.geoMapPhi = (const float*)P(o_gp);
   v.nModules = h.nModules; v.nLowerModules = h.nLowerModules; v.stride = h.stride; v.nEndCapMap = h.geoMapDetId.size();
+  v.detHash = (const uint2*)P(o_dh); v.detHashBits = h.detHashBits;
   d.pixIndex[0] = (const uint32_t*)P(o_pi0); d.pixIndex[1] = (const uint32_t*)P(o_pi1);
   d.pixIndex[2] = (const uint32_t*)P(o_pi2); d.pixSizes[0] = (const uint32_t*)P(o_ps0);
   d.pixSizes[1] = (const uint32_t*)P(o_ps1); d.pixSizes[2] = (const uint32_t*)P(o_ps2);

@@ -14,6 +14,10 @@ struct HostGeometry {
   uint32_t nModules = 0, nLowerModules = 0, nPixels = 0, stride = 0;
   uint16_t pixelModuleIndex = 0;
   std::vector<uint32_t> detIds, mapdetId, connectedPixels, geoMapDetId;
+  // open-addressing table detId -> (module index | (endcap-map index + 1) << 16),
+  // 2 words per slot, 2^detHashBits slots, empty slots hold kNoDetId
+  std::vector<uint32_t> detHash;
+  uint32_t detHashBits = 0;
   std::vector<uint16_t> moduleMap, mapIdx, nConnectedModules, partnerModuleIndices;
   std::vector<float> drdzs, dxdys, eta, r, geoMapPhi;
   std::vector<int16_t> layers, rings, modules, rods, subdets, sides;

@@ -64,7 +64,14 @@ struct ModulesView {
   const uint32_t* connectedPixels;
   const uint32_t* geoMapDetId;  // endcap 2S orientation map, sorted
   const float* geoMapPhi;
-  uint32_t nModules, nLowerModules, stride, nEndCapMap;
+  // open-addressing table of the detIds (linear probing, never more than a
+  // quarter full): slot (detId, module index | (endcap-map index + 1) << 16)
+  const uint2* detHash;
+  uint32_t nModules, nLowerModules, stride, nEndCapMap, detHashBits;
 };
+constexpr uint32_t kNoDetId = 0xFFFFFFFFu;  // an empty slot of the detId table
+__host__ __device__ inline uint32_t det_hash_slot(uint32_t det, uint32_t bits) {
+  return (det * 0x9E3779B1u) >> (32 - bits);
+}
 
 }  // namespace lst

@@ -169,6 +169,8 @@ struct PixelWork {
   uint32_t* t5TcCount;    // [nLower + 1] T5 track candidates per eligible module, then offsets
   uint32_t* outHits;      // [TC capacity * 14] output hit lists
   uint32_t* outLen;       // [TC capacity]
+  uint32_t* outPacked;    // [kOutHeaderWords + TC capacity * kOutRecWords] the count, then per track
+                          // candidate its 14 hit slots, length, seed index and type: one copy to the host
   uint32_t* t5ListOff;    // [nLower + 1] first list position of each eligible T5 module, then the total
   uint32_t* t5List;       // [T5 capacity] T5 slots of the eligible modules
   uint32_t* t5ListMod;    // [T5 capacity] eligible-list position of the module of each listed T5
@@ -219,6 +221,7 @@ constexpr uint32_t kShardWords = kPixMaskWords / kPixShards;
 constexpr uint32_t kShardSurv = kPixSurvMax / kPixShards;
 constexpr int kT5RecWords = 32;               // per-T5 record of the pixel-quintuplet selection
 constexpr int kT3RecWords = 16;               // per-T3 record of the pixel-triplet selection
+constexpr uint32_t kOutHeaderWords = 64, kOutRecWords = 17;  // packed output: 256-byte header, 68-byte records
 constexpr int kEtaBins = 64;                    // finite eta bins of width 1/8 over [-4, 4)
 constexpr int kNanEtaBin = kEtaBins;            // the bin of NaN eta
 constexpr int kEtaBinOffsets = kEtaBins + 2;    // starts of bins 0..kNanEtaBin, then the end

@@ -32,29 +32,17 @@ __device__ __forceinline__ unsigned build_rank(bool pass, unsigned* lds, unsigne
   }
 }
 
-__device__ __forceinline__ int find_module(const ModulesView& M, uint32_t det) {
-  // binary search over the sorted detIds (SDL/Hit.h:172-192)
-  int lo = 0, hi = (int)M.nModules - 1;
-  while (lo <= hi) {
-    int mid = (lo + hi) >> 1;
-    uint32_t t = M.mapdetId[mid];
-    if (t == det) return M.mapIdx[mid];
-    if (t > det) hi = mid - 1;
-    else lo = mid + 1;
+// the detId table entry of det, ~0u when det is no module (the table is at
+// most a quarter full, so a probe run ends at an empty slot)
+__device__ __forceinline__ uint32_t det_lookup(const ModulesView& M, uint32_t det) {
+  const uint32_t mask = (1u << M.detHashBits) - 1;
+  uint32_t s = det_hash_slot(det, M.detHashBits);
+  for (uint32_t n = 0; n <= mask; n++, s = (s + 1) & mask) {
+    const uint2 e = M.detHash[s];
+    if (e.x == det) return e.y;
+    if (e.x == kNoDetId) break;
   }
-  return -1;
-}
-
-__device__ __forceinline__ int find_endcap(const ModulesView& M, uint32_t det) {
-  int lo = 0, hi = (int)M.nEndCapMap - 1;
-  while (lo <= hi) {
-    int mid = (lo + hi) >> 1;
-    uint32_t t = M.geoMapDetId[mid];
-    if (t == det) return mid;
-    if (t > det) hi = mid - 1;
-    else lo = mid + 1;
-  }
-  return -1;
+  return ~0u;
 }
 
 // ------------------------------------------------------------------------ hits
@@ -83,14 +71,17 @@ __global__ void k_hits(EventView v) {
     v.H.phis[i] = phi_of(x, y);
     v.H.etas[i] = ((z > 0) - (z < 0)) * m_acosh(m_sqrt(x * x + y * y + z * z) / rt);
     float hx = 0.f, hy = 0.f, lx = 0.f, ly = 0.f;
-    const int mod = find_module(v.M, det);
+    // module index and endcap-map entry (SDL/Hit.h:172-192 and the endcap
+    // geometry map) from one table probe
+    const uint32_t entry = det_lookup(v.M, det);
+    const int mod = entry == ~0u ? -1 : (int)(entry & 0xFFFFu);
     if (mod < 0) {
       v.H.moduleIndices[i] = 0xFFFF;
       atomicOr(&v.R.scalars[kBadHits], 1u);
     } else {
       v.H.moduleIndices[i] = mod;
       if (v.M.subdets[mod] == kEndcap && v.M.moduleType[mod] == kTwoS) {
-        const int e = find_endcap(v.M, det);
+        const int e = (int)(entry >> 16) - 1;
         const float p = e >= 0 ? v.M.geoMapPhi[e] : 0.f;
         const float c = m_cos(p), s = m_sin(p);
         hx = x + 2.5f * c; lx = x - 2.5f * c;

@@ -795,22 +795,38 @@ __global__ __launch_bounds__(kCompactBlock) void k_tc_add_pls(EventView v) {
 
 // LST::getHitIdxs (SDL/LST.cc:393-420): ntuple hit indices of each TC, the
 // repeated fourth hit of a triplet seed dropped
+// the output hit lists (outHits, outLen), and the same with the seed index and
+// type as one packed block (outPacked) that a batch copies to the host at once
 __global__ void k_tc_output(EventView v) {
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
+  if (blockIdx.x == 0 && threadIdx.x == 0) v.W.outPacked[0] = nTC;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nTC; t += gridDim.x * blockDim.x) {
     const int16_t type = v.C.trackCandidateType[t];
     const int maxN = type == 7 ? 14 : (type == 5 || type == 4) ? 10 : type == 8 ? 4 : 0;
+    uint32_t hits[14];
     uint32_t len = 0, last = 0;
+#pragma unroll
+    for (int i = 0; i < 14; i++) hits[i] = 0;
     for (int i = 0; i < maxN; i++) {
       const uint32_t h = v.C.hitIndices[14 * t + i];
       const uint32_t idx = type == 8 ? h : v.H.idxs[h];
       if (type != 4 && len == 3 && last == idx) continue;
-      v.W.outHits[14 * t + len] = idx;
+#pragma unroll
+      for (int k = 0; k < 14; k++)  // register write at a run-time position without a scratch array
+        if ((uint32_t)k == len) hits[k] = idx;
       last = idx;
       len++;
     }
-    for (uint32_t i = len; i < 14; i++) v.W.outHits[14 * t + i] = 0;
+    uint32_t* rec = v.W.outPacked + kOutHeaderWords + (size_t)kOutRecWords * t;
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      v.W.outHits[14 * t + i] = hits[i];
+      rec[i] = hits[i];
+    }
     v.W.outLen[t] = len;
+    rec[14] = len;
+    rec[15] = (uint32_t)v.C.pixelSeedIndex[t];
+    rec[16] = (uint32_t)(int32_t)type;
   }
 }
 
