@@ -40,22 +40,22 @@ HBM_PEAK_GBS = 8000.0
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
 MEASURED_TRAFFIC = {
-    "k_hits": 6910022,
-    "k_pixel_segments": 2126231,
-    "k_md_eval": 10781172,
-    "k_ls_eval": 55795064,
-    "k_t3_select": 26924365,
-    "k_t5_final": 28225495,
-    "k_pls_pairs<false>": 914619,
-    "k_pix_stage_a<PT5Kind>": 13681276,
-    "k_pix_stage_a<PT3Kind>": 8786794,
-    "k_t5_tc_pairs": 12200941,
-    "k_pt5_pairs": 3444092,
-    "k_pt3_pairs": 130185,
-    "k_cross_pls_pairs": 1844375,
-    "k_pls_pairs<true>": 12103997,
-    "k_t5_dup_after_build": 9619777,
-    "k_t3_enum": 11414917,
+    "k_hits": 8886013,
+    "k_pixel_segments": 2126388,
+    "k_md_eval": 10787449,
+    "k_ls_eval": 55819306,
+    "k_t3_select": 26946944,
+    "k_t5_final": 30591466,
+    "k_pls_pairs<false>": 1040702,
+    "k_pix_stage_a<PT5Kind>": 13680911,
+    "k_pix_stage_a<PT3Kind>": 8830522,
+    "k_t5_tc_pairs": 3107656,
+    "k_pt5_pairs": 3444473,
+    "k_pt3_pairs": 130123,
+    "k_cross_pls_pairs": 1848638,
+    "k_pls_pairs<true>": 1066252,
+    "k_t5_dup_after_build": 2989914,
+    "k_t3_enum": 11421991,
 }
 
 
