@@ -56,9 +56,12 @@ __device__ __forceinline__ float muls_scale(const ModCtx& c, float ptCut) {
 }
 
 // ----------------------------------------------------------------- mini-doublets
-// dphi threshold: SDL/MiniDoublet.h:339-401
-__device__ __forceinline__ float md_dphi_cut(const ModCtx& c, float ptCut, float rt, float dPhi, float dz) {
-  const float slope = m_asin(m_min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+// dphi threshold: SDL/MiniDoublet.h:339-401.  slope = asin(min(rt * k2Rinv1GeVf /
+// ptCut, kSinAlphaMax)) of the hit on the pixel side, computed once per hit
+__device__ __forceinline__ float md_hit_slope(float rt, float ptCut) {
+  return m_asin(m_min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+}
+__device__ __forceinline__ float md_dphi_cut(const ModCtx& c, float ptCut, float slope, float dPhi, float dz) {
   const float rNom = c.subdet == kBarrel ? c_rminBarrel[c.layer - 1] : c_rminEndcap[c.layer - 1];
   const float pvOff = 0.1f / rNom;
   const float muls = muls_scale(c, ptCut);
@@ -111,13 +114,13 @@ struct MDOut {
 
 // SDL/MiniDoublet.h:645-772
 __device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, float xL, float yL, float zL, float rtL,
-                                          float xU, float yU, float zU, float rtU, MDOut& o) {
+                                          float xU, float yU, float zU, float rtU, float slope, MDOut& o) {
   o.dz = zL - zU;
   const float dzCut = c.type == kPS ? 2.f : 10.f;
   const float sign = ((o.dz > 0) - (o.dz < 0)) * ((zL > 0) - (zL < 0));
   const float crosser = (m_abs(o.dz) > 2) * sign;
   if (!(m_abs(o.dz) < dzCut && crosser <= 0)) return false;
-  const float cut = md_dphi_cut(c, ptCut, c.layerType == kPixelLayer ? rtL : rtU, 0.f, 0.f);
+  const float cut = md_dphi_cut(c, ptCut, slope, 0.f, 0.f);
   float shiftedRt = 0.f;
   if (c.side != kCenter) {
     float xn, yn, zn;
@@ -154,7 +157,7 @@ __device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, float xL
 
 // SDL/MiniDoublet.h:774-894 (o.dz holds drt, as the reference stores it)
 __device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, float xL, float yL, float zL, float rtL,
-                                          float xU, float yU, float zU, float rtU, MDOut& o) {
+                                          float xU, float yU, float zU, float rtU, float slope, MDOut& o) {
   float dz = zL - zU;
   if (!(m_abs(dz) < 1.f)) return false;
   o.dz = rtL - rtU;
@@ -171,7 +174,7 @@ __device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, float xL
   }
   o.nsdphi = delta_phi(xL, yL, xU, yU);
   if (c.type == kPS) dz = c.layerType == kPixelLayer ? zL - zn : zU - zn;
-  const float cut = md_dphi_cut(c, ptCut, c.layerType == kPixelLayer ? rtL : rtU, o.dphi, dz);
+  const float cut = md_dphi_cut(c, ptCut, slope, o.dphi, dz);
   if (!(m_abs(o.dphi) < cut)) return false;
   const float frac = m_abs(dz) / m_abs(zL);
   o.dphichange = o.dphi / frac * (1.f + frac);
@@ -230,12 +233,15 @@ __device__ __forceinline__ bool seg_dalpha_pass(const ModCtx& ci, const ModCtx& 
 }
 
 // SDL/Segment.h:454-558
+// bSlope: (asin(min(b.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)), the tan of
+// it), a function of the outer mini-doublet's anchor hit only, computed once
+// per MD from the hit's slope (md_hit_slope, the same formula)
 __device__ __forceinline__ bool seg_barrel(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                           const MDRef& b, SegOut& o) {
+                                           const MDRef& b, float2 bSlope, SegOut& o) {
   const float muls = muls_scale(ci, ptCut);
-  const float slope = m_asin(m_min(b.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+  const float slope = bSlope.x;
   const float pvOff = 0.1f / b.rt;
-  const float scale = m_tan(slope) / slope;
+  const float scale = bSlope.y / slope;
   const float zGeom = ci.layer <= 2 ? 2.f * kPixelPSZpitch : 2.f * kStrip2SZpitch;
   const float zLo = a.z + (a.z - kDeltaZLum) * (b.rt / a.rt - 1.f) * (a.z > 0.f ? 1.f : scale) - zGeom;
   const float zHi = a.z + (a.z + kDeltaZLum) * (b.rt / a.rt - 1.f) * (a.z < 0.f ? 1.f : scale) + zGeom;
@@ -250,8 +256,8 @@ __device__ __forceinline__ bool seg_barrel(const ModCtx& ci, const ModCtx& co, f
 
 // SDL/Segment.h:560-704
 __device__ __forceinline__ bool seg_endcap(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                           const MDRef& b, SegOut& o) {
-  const float slope = m_asin(m_min(b.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+                                           const MDRef& b, float2 bSlope, SegOut& o) {
+  const float slope = bSlope.x;
   const float r2S = 60.f;
   const float rtGeom = (a.rt < r2S && b.rt < r2S)   ? (2.f * kPixelPSZpitch)
                        : (a.rt < r2S || b.rt < r2S) ? (kPixelPSZpitch + kStrip2SZpitch)
@@ -259,7 +265,7 @@ __device__ __forceinline__ bool seg_endcap(const ModCtx& ci, const ModCtx& co, f
   if (!(a.z * b.z >= 0)) return false;
   const float dz = b.z - a.z;
   const float dLum = sign_copy(kDeltaZLum, a.z);
-  const float scale = slope / m_tan(slope);
+  const float scale = slope / bSlope.y;
   const float rtLo = m_max(a.rt * (1.f + dz / (a.z + dLum) * scale) - rtGeom, a.rt - 0.5f * rtGeom);
   const float rtHi = a.rt * (b.z - dLum) / (a.z - dLum) + rtGeom;
   if (!(b.rt >= rtLo && b.rt <= rtHi)) return false;

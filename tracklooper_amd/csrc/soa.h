@@ -105,6 +105,7 @@ struct MDWork {
   uint32_t* resIdx;   // [kMDCandMax] park slot of a passing candidate, ~0: failed
   uint32_t* park;     // [kMDParkWords * kMDCandMax] MDOut of passing candidate L at L
   uint4* cand;        // [kMDCandMax] candidate L: (lower hit, upper hit, module, 0)
+  float* hitSlope;    // [hit capacity] per hit: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax))
 };
 
 // Work arrays of the flattened line-segment builder: candidates of module m
@@ -118,6 +119,7 @@ struct LSWork {
   uint32_t* resIdx;   // [kLSCandMax]
   uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
   uint4* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD, module, outer module)
+  float2* mdSlope;    // [MD capacity] per MD: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
 };
 
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).

@@ -68,6 +68,7 @@ __global__ void k_hits(EventView v) {
     const uint32_t det = v.H.detid[i];
     const float rt = m_sqrt(x * x + y * y);
     v.H.rts[i] = rt;
+    v.W2.hitSlope[i] = md_hit_slope(rt, v.ptCut);
     v.H.phis[i] = phi_of(x, y);
     v.H.etas[i] = ((z > 0) - (z < 0)) * m_acosh(m_sqrt(x * x + y * y + z * z) / rt);
     float hx = 0.f, hy = 0.f, lx = 0.f, ly = 0.f;
@@ -150,6 +151,10 @@ __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, 
   const float ahx = H.highEdgeXs[a], ahy = H.highEdgeYs[a], alx = H.lowEdgeXs[a], aly = H.lowEdgeYs[a];
   D.anchorX[idx] = H.xs[a]; D.anchorY[idx] = H.ys[a]; D.anchorZ[idx] = H.zs[a];
   D.anchorRt[idx] = H.rts[a]; D.anchorPhi[idx] = H.phis[a]; D.anchorEta[idx] = H.etas[a];
+  {  // the segment slope of this MD as the outer one: the anchor hit's slope (same formula) and its tan
+    const float sl = v.W2.hitSlope[a];
+    v.WL.mdSlope[idx] = make_float2(sl, m_tan(sl));
+  }
   D.anchorHighEdgeX[idx] = ahx; D.anchorHighEdgeY[idx] = ahy; D.anchorLowEdgeX[idx] = alx; D.anchorLowEdgeY[idx] = aly;
   D.anchorHighEdgePhi[idx] = m_atan2(ahy, ahx);
   D.anchorLowEdgePhi[idx] = m_atan2(aly, alx);
@@ -296,8 +301,9 @@ __global__ __launch_bounds__(256) void k_md_eval(EventView v) {
       const ModCtx c = load_mod(v.M, m);
       const float xL = v.H.xs[a], yL = v.H.ys[a], zL = v.H.zs[a], rtL = v.H.rts[a];
       const float xU = v.H.xs[b], yU = v.H.ys[b], zU = v.H.zs[b], rtU = v.H.rts[b];
-      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, o)
-                                 : md_endcap(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, o);
+      const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? a : b];  // the rt of the pixel-side hit
+      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, slope, o)
+                                 : md_endcap(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, slope, o);
       v.W2.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
@@ -390,9 +396,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
         li = k / nUp;
         ui = k - li * nUp;
         const int u = 128 + ui;
+        const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? loBase + li : upBase + ui];
         pass = c.subdet == kBarrel
-                   ? md_barrel(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], o)
-                   : md_endcap(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], o);
+                   ? md_barrel(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], slope, o)
+                   : md_endcap(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], slope, o);
       }
       unsigned n;
       const unsigned rank = count + build_rank(pass, lds, n);
@@ -496,8 +503,9 @@ __global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
       const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
       const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
-      pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, o)
-                                                            : seg_endcap(ci, co, v.ptCut, a, b, o);
+      const float2 bs = v.WL.mdSlope[outMD];
+      pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, bs, o)
+                                                            : seg_endcap(ci, co, v.ptCut, a, b, bs, o);
       v.WL.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
@@ -596,8 +604,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
         outMD = v.R.mdRanges[2 * om] + local % nOut;
         const ModCtx co = load_mod(v.M, om);
         const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
-        pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, o)
-                                                              : seg_endcap(ci, co, v.ptCut, a, b, o);
+        const float2 bs = v.WL.mdSlope[outMD];
+        pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, bs, o)
+                                                              : seg_endcap(ci, co, v.ptCut, a, b, bs, o);
       }
       unsigned n;
       const unsigned rank = count + build_rank(pass, lds, n);
