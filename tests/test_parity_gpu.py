@@ -167,14 +167,15 @@ def test_per_module_builders_match(oracle_geometry, geom_dir, monkeypatch):
     compare(prod, ref, DEEPEST)
 
 
-def test_list_by_count_match(oracle_geometry, geom_dir, monkeypatch):
+def test_large_paths_match(oracle_geometry, geom_dir, monkeypatch):
     """The per-mini-doublet segment and T3 lists built by direct counting (the
-    path of modules too large for the LDS counting sort) give the same objects
-    as the oracle."""
+    path of modules too large for the LDS counting sort) and the eta binnings
+    that load their items twice (the path of counts too large for registers)
+    give the same objects as the oracle."""
     import tracklooper_amd as lst
     from tracklooper_amd import synth
     from oracle import orc
-    monkeypatch.setenv("LST_LIST_BY_COUNT", "1")
+    monkeypatch.setenv("LST_LARGE_PATHS", "1")
     es = lst.LSTESData(geom_dir)
     ev = synth.generate_event("pu", 8, n_pu=200)
     prod = lst.Event(es).run_lst(ev, DEEPEST)

@@ -278,10 +278,11 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   {
     const char* pm = std::getenv("LST_PER_MODULE_BUILDERS");
     e.perModuleBuilders = (pm && pm[0] == '1') ? 1 : 0;
-    // LST_LIST_BY_COUNT=1: every module's per-mini-doublet lists take the
-    // path of the modules too large for the LDS counting sort (same lists)
-    const char* lc = std::getenv("LST_LIST_BY_COUNT");
-    e.listByCount = (lc && lc[0] == '1') ? 1 : 0;
+    // LST_LARGE_PATHS=1: the per-mini-doublet lists take the path of modules
+    // too large for the LDS counting sort, and the eta binnings the path of
+    // counts too large to keep in registers (same results)
+    const char* lc = std::getenv("LST_LARGE_PATHS");
+    e.largePaths = (lc && lc[0] == '1') ? 1 : 0;
   }
   e.t5Dnn = t5Mode == LST_T5_DNN ? 1 : 0;
   e.pixelModule = g.pixelModuleIndex;

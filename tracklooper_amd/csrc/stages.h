@@ -47,7 +47,7 @@ struct EventView {  // everything a stage kernel needs, by value
   const uint32_t* pixSizes[3];
   int tcPlsTriplets;                 // TC_PLS_TRIPLETS: triplet pLS may become track candidates
   int perModuleBuilders;             // run the per-module T3/T5 builders instead of the flattened ones
-  int listByCount;                   // build every per-mini-doublet list by direct counting (large-module path)
+  int largePaths;                    // per-MD lists and eta binning take the paths of oversized modules/events
   int t5Dnn;                         // T5 selection by the DNN (USE_T5_DNN) instead of the r-phi chi2 cut
   T5DnnDump dnnDump;
   uint32_t nHits, nPLS, pixelModule;

@@ -96,17 +96,17 @@ __device__ __forceinline__ int eta_bin(float eta) {
 // item) writes it at its binned position.  Up to 1024 * kBinPer items are
 // loaded once and kept in registers for the scatter: each lane issues all its
 // key loads, then all its item loads (indices clamped to n - 1, so that no
-// branch splits them); a larger count loads every item twice.
+// branch splits them); a larger count (or twoPass) loads every item twice.
 struct SameKey {
   __device__ uint32_t operator()(uint32_t i) const { return i; }
 };
 template <int kBinPer = 16, class Load, class Store, class Key = SameKey>
-__device__ void bin_by_eta(uint32_t n, Load load, Store store, uint32_t* off, Key key = Key()) {
+__device__ void bin_by_eta(uint32_t n, bool twoPass, Load load, Store store, uint32_t* off, Key key = Key()) {
   __shared__ unsigned cnt[kEtaBinOffsets], cur[kEtaBinOffsets];
   using Item = decltype(load(0u));
   if (threadIdx.x < kEtaBinOffsets) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const bool fits = n <= 1024u * kBinPer;
+  const bool fits = n <= 1024u * kBinPer && !twoPass;
   Item it[kBinPer];
   int bin[kBinPer];
   if (fits && n != 0) {
@@ -159,7 +159,7 @@ struct PlsBinItem {
 };
 __global__ __launch_bounds__(1024) void k_bin_pls(EventView v) {
   bin_by_eta<8>(
-      v.R.scalars[kEvNPLS],
+      v.R.scalars[kEvNPLS], v.largePaths != 0,
       [&](uint32_t i) {
         const float eta = v.P.eta[i];
         const uint32_t quad = v.P.isQuad[i] ? 1u : 0u;
@@ -565,7 +565,7 @@ struct T5BinItem {
 };
 __global__ __launch_bounds__(1024) void k_bin_t5(EventView v) {
   bin_by_eta(
-      v.R.scalars[kT5Listed],
+      v.R.scalars[kT5Listed], v.largePaths != 0,
       [&](uint32_t i) {
         const float4 r = v.W.t5ListRec[i];
         return T5BinItem{r, r.x};
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(256) void k_t5_tc_pairs(EventView v) {
 __global__ __launch_bounds__(1024) void k_bin_pix(EventView v) {
   const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
   bin_by_eta(
-      n5 + v.R.scalars[kNPixelTriplets],
+      n5 + v.R.scalars[kNPixelTriplets], v.largePaths != 0,
       [&](uint32_t j) { return BinItem{j, j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5]}; },
       [&](uint32_t pos, const BinItem& t) { v.W.pixBinned[pos] = t.id; }, v.W.pixBinOff);
 }
@@ -782,7 +782,7 @@ __global__ void k_tc_prep(EventView v) {
 
 __global__ __launch_bounds__(1024) void k_bin_tc(EventView v) {
   bin_by_eta(
-      v.R.scalars[kNTrackCandidates], [&](uint32_t t) { return BinItem{t, v.W.tcEta[t]}; },
+      v.R.scalars[kNTrackCandidates], v.largePaths != 0, [&](uint32_t t) { return BinItem{t, v.W.tcEta[t]}; },
       [&](uint32_t pos, const BinItem& t) { v.W.tcBinned[pos] = t.id; }, v.W.tcBinOff);
 }
 

@@ -788,7 +788,7 @@ __global__ __launch_bounds__(256) void k_md_seg_lists(EventView v) {
     if (nMD == 0) continue;
     md_lists(
         nMD, mdBase, nS, segBase, [&](uint32_t k) { return v.S.mdIndices[2 * (segBase + k)]; }, v.W3.mdSegOff,
-        v.W3.mdSegEnd, v.W3.segByMD, v.W3.segTmp, sX, v.listByCount);
+        v.W3.mdSegEnd, v.W3.segByMD, v.W3.segTmp, sX, v.largePaths);
   }
 }
 

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_md_t3_lists(EventView v) {
     if (nMD == 0) continue;
     md_lists(
         nMD, mdBase, nT, t3Base, [&](uint32_t k) { return v.S.mdIndices[2 * v.T.segmentIndices[2 * (t3Base + k)]]; },
-        v.W5.mdT3Off, v.W5.mdT3End, v.W5.t3ByMD, v.W5.t3Tmp, sX, v.listByCount);
+        v.W5.mdT3Off, v.W5.mdT3End, v.W5.t3ByMD, v.W5.t3Tmp, sX, v.largePaths);
   }
 }
 
