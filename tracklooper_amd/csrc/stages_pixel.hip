@@ -803,18 +803,29 @@ __global__ void k_tc_output(EventView v) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nTC; t += gridDim.x * blockDim.x) {
     const int16_t type = v.C.trackCandidateType[t];
     const int maxN = type == 7 ? 14 : (type == 5 || type == 4) ? 10 : type == 8 ? 4 : 0;
-    uint32_t hits[14];
+    // the hit slots, then their ntuple indices, loaded before any use with no
+    // branch between the loads (an unused slot reads slot 0 and hit 0, both in
+    // bounds, and its value is not used), then compacted
+    uint32_t h[14], idx[14], hits[14];
+#pragma unroll
+    for (int i = 0; i < 14; i++) h[i] = v.C.hitIndices[14 * t + (i < maxN ? i : 0)];
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      const bool direct = type == 8 || i >= maxN;
+      const uint32_t g = v.H.idxs[direct ? 0u : h[i]];
+      idx[i] = direct ? h[i] : g;
+    }
     uint32_t len = 0, last = 0;
 #pragma unroll
     for (int i = 0; i < 14; i++) hits[i] = 0;
-    for (int i = 0; i < maxN; i++) {
-      const uint32_t h = v.C.hitIndices[14 * t + i];
-      const uint32_t idx = type == 8 ? h : v.H.idxs[h];
-      if (type != 4 && len == 3 && last == idx) continue;
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      if (i >= maxN) break;
+      if (type != 4 && len == 3 && last == idx[i]) continue;
 #pragma unroll
       for (int k = 0; k < 14; k++)  // register write at a run-time position without a scratch array
-        if ((uint32_t)k == len) hits[k] = idx;
-      last = idx;
+        if ((uint32_t)k == len) hits[k] = idx[i];
+      last = idx[i];
       len++;
     }
     uint32_t* rec = v.W.outPacked + kOutHeaderWords + (size_t)kOutRecWords * t;
