@@ -341,16 +341,21 @@ __global__ __launch_bounds__(256) void k_md_place(EventView v) {
   if (!md_flat_done(v)) return;
   const uint32_t total = v.R.scalars[kMDCandTotal], nL = v.M.nLowerModules;
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    // the loads that only need L, then (for a passing candidate) those that
+    // need its module and park slot, each group issued before any use
     const uint32_t k = v.W2.resIdx[L];
-    if (k == ~0u) continue;
     const uint4 cd = v.W2.cand[L];
+    const uint32_t rankL = v.FS.pre[L] + v.FS.sums[L / kScanBlockItems];  // flag_rank for L < total
+    if (k == ~0u) continue;
     const uint32_t m = cd.z;
-    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.W2.candOff[m], total);
-    if ((int)rank >= v.mdModuleOccupancy[m]) continue;
+    const uint32_t off = v.W2.candOff[m];
+    const int occ = v.mdModuleOccupancy[m];
     const ModCtx c = load_mod(v.M, m);
-    const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
     const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
+    const uint32_t rank = rankL - flag_rank(v.FS, off, total);
+    if ((int)rank >= occ) continue;
+    const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     store_md(v, cd.x, cd.y, m, anchorIsUpper, o, v.mdModuleIndices[m] + rank);
   }
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
@@ -531,20 +536,28 @@ __global__ __launch_bounds__(256) void k_ls_place(EventView v) {
   if (!ls_flat_done(v)) return;
   const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
+    // the loads that only need L, then those of a passing candidate, each
+    // group issued before any use
     const uint32_t k = v.WL.resIdx[L];
-    if (k == ~0u) continue;
     const uint4 c = v.WL.cand[L];
+    const uint32_t rankL = v.FS.pre[L] + v.FS.sums[L / kScanBlockItems];  // flag_rank for L < total
+    if (k == ~0u) continue;
     const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
-    const uint32_t rank = flag_rank(v.FS, L, total) - flag_rank(v.FS, v.WL.candOff[m], total);
-    if ((int)rank >= v.segModuleOccupancy[m]) continue;
+    const uint32_t off = v.WL.candOff[m];
+    const int occ = v.segModuleOccupancy[m];
+    const uint32_t segBase = v.segModuleIndices[m];
+    const uint32_t hitIn = v.D.anchorHitIndices[inMD], hitOut = v.D.anchorHitIndices[outMD];
     const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
-    const uint32_t s = v.segModuleIndices[m] + rank;
+    const float w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
+    const uint32_t rank = rankL - flag_rank(v.FS, off, total);
+    if ((int)rank >= occ) continue;
+    const uint32_t s = segBase + rank;
     v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
     v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
-    v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
-    v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
-    v.S.dPhis[s] = w[0]; v.S.dPhiMins[s] = w[1]; v.S.dPhiMaxs[s] = w[2];
-    v.S.dPhiChanges[s] = w[3]; v.S.dPhiChangeMins[s] = w[4]; v.S.dPhiChangeMaxs[s] = w[5];
+    v.S.innerMiniDoubletAnchorHitIndices[s] = hitIn;
+    v.S.outerMiniDoubletAnchorHitIndices[s] = hitOut;
+    v.S.dPhis[s] = w0; v.S.dPhiMins[s] = w1; v.S.dPhiMaxs[s] = w2;
+    v.S.dPhiChanges[s] = w3; v.S.dPhiChangeMins[s] = w4; v.S.dPhiChangeMaxs[s] = w5;
   }
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
