@@ -40,22 +40,22 @@ HBM_PEAK_GBS = 8000.0
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
 MEASURED_TRAFFIC = {
-    "k_hits": 8886013,
-    "k_pixel_segments": 2126388,
-    "k_md_eval": 10787449,
-    "k_ls_eval": 55819306,
-    "k_t3_select": 26946944,
-    "k_t5_final": 30591466,
-    "k_pls_pairs<false>": 1040702,
-    "k_pix_stage_a<PT5Kind>": 13680911,
-    "k_pix_stage_a<PT3Kind>": 8830522,
-    "k_t5_tc_pairs": 3107656,
-    "k_pt5_pairs": 3444473,
+    "k_hits": 9298959,
+    "k_pixel_segments": 2270225,
+    "k_md_eval": 11194577,
+    "k_ls_eval": 59011672,
+    "k_t3_select": 26938071,
+    "k_t5_final": 30703883,
+    "k_pls_pairs<false>": 1040703,
+    "k_pix_stage_a<PT5Kind>": 13693140,
+    "k_pix_stage_a<PT3Kind>": 8830565,
+    "k_t5_tc_pairs": 3106291,
+    "k_pt5_pairs": 3444467,
     "k_pt3_pairs": 130123,
-    "k_cross_pls_pairs": 1848638,
-    "k_pls_pairs<true>": 1066252,
-    "k_t5_dup_after_build": 2989914,
-    "k_t3_enum": 11421991,
+    "k_cross_pls_pairs": 1849519,
+    "k_pls_pairs<true>": 1066281,
+    "k_t5_dup_after_build": 2989941,
+    "k_t3_enum": 11423055,
 }
 
 
