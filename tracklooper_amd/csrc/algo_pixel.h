@@ -402,21 +402,35 @@ __device__ __forceinline__ bool pt3_tracklet_half(const EventView& v, uint32_t p
 // tests and the r-z chi2 cut), which need double-precision transcendentals.
 // Either order gives the same pass/fail and the same values.  `pls` is the
 // global segment index of the pLS, `arr` its pLS array index.
-__device__ __forceinline__ bool pt3_cheap(const EventView& v, uint32_t pls, uint32_t arr, const T3Rec& t,
-                                          bool chi2Cuts, PT3Result& r) {
-  const float ptIn = v.P.ptIn[arr];
+// the pLS values the cheap cuts read, loaded once per pLS by the builders
+// that evaluate many candidates of one pLS
+struct PlsCtx {
+  float ptIn, ptErr, cx, cy, pixR;        // seed pT and its error, the pLS circle
+  float xa, ya, za, ra, xb, yb, zb, rb;   // anchor x, y, z, rt of its inner and outer mini-doublet
+};
+__device__ __forceinline__ PlsCtx load_pls_ctx(const EventView& v, uint32_t pls, uint32_t arr) {
+  PlsCtx p;
+  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
+  p.ptIn = v.P.ptIn[arr]; p.ptErr = v.P.ptErr[arr];
+  p.cx = v.P.circleCenterX[arr]; p.cy = v.P.circleCenterY[arr]; p.pixR = v.P.circleRadius[arr];
+  p.xa = v.D.anchorX[pixIn]; p.ya = v.D.anchorY[pixIn]; p.za = v.D.anchorZ[pixIn]; p.ra = v.D.anchorRt[pixIn];
+  p.xb = v.D.anchorX[pixOut]; p.yb = v.D.anchorY[pixOut]; p.zb = v.D.anchorZ[pixOut]; p.rb = v.D.anchorRt[pixOut];
+  return p;
+}
+
+__device__ __forceinline__ bool pt3_cheap(const PlsCtx& p, const T3Rec& t, bool chi2Cuts, PT3Result& r) {
+  const float ptIn = p.ptIn;
   r.pixR = ptIn * kR1GeVf;
-  const float pixRErr = v.P.ptErr[arr] * kR1GeVf;
+  const float pixRErr = p.ptErr * kR1GeVf;
   r.t3R = t.r;
   r.rz = -1;
   if (!pt3_radius_ok(t.packed & 3, r.pixR, pixRErr, r.t3R)) return false;
   const int l1 = rec_layer(t.packed, 2, 0), l2 = rec_layer(t.packed, 2, 1), l3 = rec_layer(t.packed, 2, 2);
   const bool cuts = chi2Cuts && ptIn < 5.0f;
-  r.rphi = points_chi2(3, t.x, t.y, t.sig, v.P.circleCenterX[arr], v.P.circleCenterY[arr], v.P.circleRadius[arr]);
+  r.rphi = points_chi2(3, t.x, t.y, t.sig, p.cx, p.cy, p.pixR);
   if (cuts && !pt3_rphi_ok(l1, l2, l3, r.rphi)) return false;
   {
-    const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
-    const float xa = v.D.anchorX[pixIn], ya = v.D.anchorY[pixIn], xb = v.D.anchorX[pixOut], yb = v.D.anchorY[pixOut];
+    const float xa = p.xa, ya = p.ya, xb = p.xb, yb = p.yb;
     const float g = t.g, f = t.f, rr = r.t3R;
     float res = (xa - g) * (xa - g) + (ya - f) * (ya - f) - rr * rr;
     float c2 = res * res;
@@ -624,7 +638,7 @@ __device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, flo
 // radius compatibility, the r-z, r-phi and inward r-phi chi2 cuts, which set
 // every value of the result) and pt5_costly (the pixel-tracklet tests of the
 // inner T3).  `rec` is the T5's kT5RecWords record.
-__device__ __forceinline__ bool pt5_cheap(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t5, const float* rec,
+__device__ __forceinline__ bool pt5_cheap(const EventView& v, const PlsCtx& p, uint32_t t5, const float* rec,
                                           PT5Result& r) {
   float w[kT5RecWords];
 #pragma unroll
@@ -634,17 +648,16 @@ __device__ __forceinline__ bool pt5_cheap(const EventView& v, uint32_t pls, uint
   }
   const uint32_t packed = __float_as_uint(w[31]);
   // inner T3 (runPixelTripletDefaultAlgo without chi2 cuts): radius compatibility
-  if (!pt3_radius_ok((packed >> 5) & 3, v.P.ptIn[arr] * kR1GeVf, v.P.ptErr[arr] * kR1GeVf, w[30])) return false;
+  if (!pt3_radius_ok((packed >> 5) & 3, p.ptIn * kR1GeVf, p.ptErr * kR1GeVf, w[30])) return false;
   int l[5];
 #pragma unroll
   for (int k = 0; k < 5; k++) l[k] = rec_layer(packed, 7, k);
-  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
-  const float zP0 = v.D.anchorZ[pixIn], zP1 = v.D.anchorZ[pixOut];
-  const float rP0 = v.D.anchorRt[pixIn], rP1 = v.D.anchorRt[pixOut];
+  const float zP0 = p.za, zP1 = p.zb;
+  const float rP0 = p.ra, rP1 = p.rb;
   r.rz = 0;
-  r.cx = v.P.circleCenterX[arr];
-  r.cy = v.P.circleCenterY[arr];
-  r.pixR = v.P.circleRadius[arr];
+  r.cx = p.cx;
+  r.cy = p.cy;
+  r.pixR = p.pixR;
   if (r.pixR < 5.0f * kR1GeVf) {  // computePT5RZChiSquared (:2636-2677)
     const float slope = (zP1 - zP0) / (rP1 - rP0);
     float rmse = 0;
@@ -662,7 +675,7 @@ __device__ __forceinline__ bool pt5_cheap(const EventView& v, uint32_t pls, uint
   r.rphi = points_chi2(5, w, w + 5, w + 20, r.cx, r.cy, r.pixR);
   if (r.pixR < 5.0f * kR1GeVf && !pt5_rphi_ok(l[0], l[1], l[2], l[3], l[4], r.rphi)) return false;
   {
-    const float xa = v.D.anchorX[pixIn], ya = v.D.anchorY[pixIn], xb = v.D.anchorX[pixOut], yb = v.D.anchorY[pixOut];
+    const float xa = p.xa, ya = p.ya, xb = p.xb, yb = p.yb;
     const float rr = r.t5R;
     float c2 = 0;
     float res = (xa - t5g) * (xa - t5g) + (ya - t5f) * (ya - t5f) - rr * rr;

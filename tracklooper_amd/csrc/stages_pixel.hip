@@ -171,9 +171,9 @@ struct PT5Kind {
   __device__ static bool module_ok(const EventView& v, uint32_t m) { return v.M.moduleType[m] != kTwoS; }
   __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nQuintuplets[m]; }
   __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.quintupletModuleIndices[m]; }
-  __device__ static bool cheap(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
+  __device__ static bool cheap(const EventView& v, const PlsCtx& p, uint32_t t, Res& r) {
     if (v.Q.isDup[t] != 0) return false;
-    return pt5_cheap(v, pls, i, t, v.W.t5Rec + (size_t)kT5RecWords * t, r);
+    return pt5_cheap(v, p, t, v.W.t5Rec + (size_t)kT5RecWords * t, r);
   }
   __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res&) {
     return pt5_costly(v, pls, i, t);
@@ -248,11 +248,11 @@ struct PT3Kind {
   }
   __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nTriplets[m]; }
   __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.tripletModuleIndices[m]; }
-  __device__ static bool cheap(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
+  __device__ static bool cheap(const EventView& v, const PlsCtx& p, uint32_t t, Res& r) {
     const T3Rec rec = load_t3_rec(v.W.t3Rec + (size_t)kT3RecWords * t);
     if ((rec.packed >> 17) & 1) return false;  // middle module 2S
     if (v.T.partOfPT5[t] != 0) return false;
-    return pt3_cheap(v, pls, i, rec, true, r);
+    return pt3_cheap(p, rec, true, r);
   }
   __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
     return pt3_costly(v, pls, i, t, true, r);
@@ -384,6 +384,7 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
     bool full = false;
     if (maskBase != ~0u) {
       const uint32_t pls = plsBase + i;
+      const PlsCtx pc = load_pls_ctx(v, pls, i);  // once per pLS, not per candidate
       for (uint32_t w = wave; w < words; w += kPixWG / kWave) {
         const uint32_t L = w * kWave + lane;
         typename K::Res r;
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
         bool surv = false;
         if (L < total) {
           t = candidate_object(pre, objBase, size, L);
-          surv = K::cheap(v, pls, i, t, r);
+          surv = K::cheap(v, pc, t, r);
         }
         if (lane == 0) K::mask(v)[maskBase + w] = 0ull;
         const unsigned long long sm = __ballot(surv);  // every lane of the wave takes part
@@ -491,13 +492,14 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_b(EventView v) {
     uint32_t size;
     const uint32_t total = conn_prefix_wg<K>(v, i, pre, objBase, size);
     unsigned count = 0;
+    const PlsCtx pc = load_pls_ctx(v, plsBase + i, i);
     for (uint32_t L0 = wave * kWave; L0 < total; L0 += kPixWG) {
       const uint32_t L = L0 + lane;
       bool pass = false;
       if (L < total) {
         const uint32_t t = candidate_object(pre, objBase, size, L);
         typename K::Res r;
-        pass = K::cheap(v, plsBase + i, i, t, r) && K::costly(v, plsBase + i, i, t, r);
+        pass = K::cheap(v, pc, t, r) && K::costly(v, plsBase + i, i, t, r);
       }
       count += (unsigned)__popcll(__ballot(pass));
     }
@@ -571,6 +573,7 @@ __global__ __launch_bounds__(kWave) void k_pix_full_write(EventView v) {
     uint32_t size;
     const uint32_t total = conn_prefix<K>(v, i, pre, objBase, size);
     const uint32_t slotBase = K::count(v)[i], pls = plsBase + i;
+    const PlsCtx pc = load_pls_ctx(v, pls, i);
     unsigned done = 0;
     for (uint32_t L0 = 0; L0 < total; L0 += kWave) {
       const uint32_t L = L0 + lane;
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(kWave) void k_pix_full_write(EventView v) {
       typename K::Res r;
       if (L < total) {
         t = candidate_object(pre, objBase, size, L);
-        pass = K::cheap(v, pls, i, t, r) && K::costly(v, pls, i, t, r);
+        pass = K::cheap(v, pc, t, r) && K::costly(v, pls, i, t, r);
       }
       unsigned n;
       const unsigned rank = done + wave_rank(pass, n);
