@@ -292,6 +292,7 @@ __device__ __forceinline__ bool seg_endcap(const ModCtx& ci, const ModCtx& co, f
 // ------------------------------------------------------------------------ triplets
 struct T3Geo {  // anchor coordinates of the three MDs
   float x1, y1, z1, r1, phi1, x2, y2, z2, r2, x3, y3, z3, r3;
+  float2 slope3;  // asin(min(r3 * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan, kept per MD
 };
 
 // SDL/Triplet.h:241-298
@@ -312,9 +313,9 @@ __device__ __forceinline__ bool t3_rz(int l1, int l2, int l3, const T3Geo& p) {
 // SDL/Triplet.h:300-402 (barrel-barrel-barrel)
 __device__ __forceinline__ bool t3_bbb(bool psIn, bool psOut, float ptCut, const T3Geo& p, float alphaIn,
                                        float& betaIn) {
-  const float a1 = m_asin(m_min(p.r3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+  const float a1 = p.slope3.x;
   const float ratio = p.r3 / p.r1;
-  const float scale = m_tan(a1) / a1;
+  const float scale = p.slope3.y / a1;
   const float zpIn = psIn ? kPixelPSZpitch : kStrip2SZpitch;
   const float zpOut = psOut ? kPixelPSZpitch : kStrip2SZpitch;
   const float zHi = p.z1 + (p.z1 + kDeltaZLum) * (ratio - 1.f) * (p.z1 < 0.f ? 1.f : scale) + (zpIn + zpOut);
@@ -347,8 +348,8 @@ __device__ __forceinline__ bool t3_bbb(bool psIn, bool psOut, float ptCut, const
 // SDL/Triplet.h:404-526 (barrel-barrel-endcap and barrel-endcap-endcap)
 __device__ __forceinline__ bool t3_bbe(bool psIn, bool psOut, float ptCut, const T3Geo& p, float alphaIn,
                                        float& betaIn) {
-  const float a1 = m_asin(m_min(p.r3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-  const float scale = m_tan(a1) / a1;
+  const float a1 = p.slope3.x;
+  const float scale = p.slope3.y / a1;
   const float zGeom = (psIn ? kPixelPSZpitch : kStrip2SZpitch) + (psOut ? kPixelPSZpitch : kStrip2SZpitch);
   if (!(p.z1 * p.z3 > 0)) return false;
   const float dLum = sign_copy(kDeltaZLum, p.z1);
@@ -372,8 +373,8 @@ __device__ __forceinline__ bool t3_bbe(bool psIn, bool psOut, float ptCut, const
 // SDL/Triplet.h:528-663 (endcap-endcap-endcap)
 __device__ __forceinline__ bool t3_eee(bool psIn, bool psOut, float ptCut, const T3Geo& p, float alphaIn,
                                        float alphaMin, float alphaMax, float& betaIn) {
-  const float a1 = m_asin(m_min(p.r3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-  const float scale = m_tan(a1) / a1;
+  const float a1 = p.slope3.x;
+  const float scale = p.slope3.y / a1;
   if (!(p.z1 * p.z3 > 0)) return false;
   const float dLum = sign_copy(kDeltaZLum, p.z1);
   const float rtGeom = (psIn && psOut) ? 2.f * kPixelPSZpitch

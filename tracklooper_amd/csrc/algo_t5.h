@@ -77,6 +77,7 @@ struct T4Args {
   ModLite m1, m2, m3, m4;  // modules of the four mini-doublets
   MDPoint a, b, c, d;      // the four mini-doublets (inner segment a-b, outer segment c-d)
   SegAlpha s1, s3;         // inner and outer segment
+  float2 cSlope;           // asin(min(c.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan, kept per MD
 };
 
 __device__ __forceinline__ float planar_dist(float x1, float y1, float x2, float y2) {
@@ -130,9 +131,9 @@ __device__ __forceinline__ bool phi_windows(const T4Args& q, float cut, float& d
 __device__ __forceinline__ bool t4_bbbb(const T4Args& q, float ptCut) {
   const float zpIn = q.m1.type == kPS ? kPixelPSZpitch : kStrip2SZpitch;
   const float zpOut = q.m3.type == kPS ? kPixelPSZpitch : kStrip2SZpitch;
-  const float a1 = m_asin(m_min(q.c.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+  const float a1 = q.cSlope.x;
   const float ratio = q.c.rt / q.a.rt;
-  const float scale = m_tan(a1) / a1;
+  const float scale = q.cSlope.y / a1;
   const float zHi = q.a.z + (q.a.z + kDeltaZLum) * (ratio - 1.f) * (q.a.z < 0.f ? 1.f : scale) + (zpIn + zpOut);
   const float zLo = q.a.z + (q.a.z - kDeltaZLum) * (ratio - 1.f) * (q.a.z > 0.f ? 1.f : scale) - (zpIn + zpOut);
   if (!(q.c.z >= zLo && q.c.z <= zHi)) return false;
@@ -193,8 +194,8 @@ __device__ __forceinline__ bool t4_bbbb(const T4Args& q, float ptCut) {
 __device__ __forceinline__ bool t4_bbee(const T4Args& q, float ptCut) {
   const float zpIn = q.m1.type == kPS ? kPixelPSZpitch : kStrip2SZpitch;
   const float zpOut = q.m3.type == kPS ? kPixelPSZpitch : kStrip2SZpitch;
-  const float a1 = m_asin(m_min(q.c.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-  const float scale = m_tan(a1) / a1;
+  const float a1 = q.cSlope.x;
+  const float scale = q.cSlope.y / a1;
   if (!(q.a.z * q.c.z > 0)) return false;
   const float dLum = sign_copy(kDeltaZLum, q.a.z);
   const float rtGeom1 = q.m3.type == kPS ? kPixelPSZpitch : kStrip2SZpitch;
@@ -252,8 +253,8 @@ __device__ __forceinline__ bool t4_bbee(const T4Args& q, float ptCut) {
 
 // SDL/Quintuplet.h:2030-2298 (endcap only)
 __device__ __forceinline__ bool t4_eeee(const T4Args& q, float ptCut) {
-  const float a1 = m_asin(m_min(q.c.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-  const float scale = m_tan(a1) / a1;
+  const float a1 = q.cSlope.x;
+  const float scale = q.cSlope.y / a1;
   if (!((q.a.z * q.c.z) > 0)) return false;
   const float dLum = sign_copy(kDeltaZLum, q.a.z);
   const bool psOutLo = q.m3.type == kPS, psInLo = q.m1.type == kPS;

@@ -677,6 +677,7 @@ __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint3
   p.phi1 = v.D.anchorPhi[md1];
   p.x2 = v.D.anchorX[md2]; p.y2 = v.D.anchorY[md2]; p.z2 = v.D.anchorZ[md2]; p.r2 = v.D.anchorRt[md2];
   p.x3 = v.D.anchorX[md3]; p.y3 = v.D.anchorY[md3]; p.z3 = v.D.anchorZ[md3]; p.r3 = v.D.anchorRt[md3];
+  p.slope3 = v.WL.mdSlope[md3];
   if (!t3_rz(v.M.sdlLayers[m1], v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) return false;
   const int16_t s1 = v.M.subdets[m1], s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
   const bool psIn = v.M.moduleType[m1] == kPS, psOut = v.M.moduleType[m3] == kPS;
