@@ -40,22 +40,22 @@ HBM_PEAK_GBS = 8000.0
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
 MEASURED_TRAFFIC = {
-    "k_hits": 9298959,
-    "k_pixel_segments": 2270225,
-    "k_md_eval": 11194577,
-    "k_ls_eval": 59011672,
-    "k_t3_select": 26938071,
-    "k_t5_final": 30703883,
+    "k_hits": 9291658,
+    "k_pixel_segments": 2275345,
+    "k_md_eval": 11193543,
+    "k_ls_eval": 59015460,
+    "k_t3_select": 29453981,
+    "k_t5_final": 30588061,
     "k_pls_pairs<false>": 1040703,
-    "k_pix_stage_a<PT5Kind>": 13693140,
-    "k_pix_stage_a<PT3Kind>": 8830565,
-    "k_t5_tc_pairs": 3106291,
-    "k_pt5_pairs": 3444467,
+    "k_pix_stage_a<PT5Kind>": 13754184,
+    "k_pix_stage_a<PT3Kind>": 9032682,
+    "k_t5_tc_pairs": 3107492,
+    "k_pt5_pairs": 3444474,
     "k_pt3_pairs": 130123,
-    "k_cross_pls_pairs": 1849519,
-    "k_pls_pairs<true>": 1066281,
-    "k_t5_dup_after_build": 2989941,
-    "k_t3_enum": 11423055,
+    "k_cross_pls_pairs": 1850339,
+    "k_pls_pairs<true>": 1066260,
+    "k_t5_dup_after_build": 2989936,
+    "k_t3_enum": 11423063,
 }
 
 
