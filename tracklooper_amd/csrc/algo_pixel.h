@@ -26,10 +26,12 @@ __device__ __forceinline__ PixelSeed load_seed(const PLSSoA& P, unsigned i) {
 
 // pixel segment (anchor points a, b) against an outer-tracker segment (c, d)
 // on modules mIn (lower) and mOut (upper); alphaIn/alphaOut are the two
-// segments' dPhiChange
+// segments' dPhiChange; slopeC is c's slope asin(min(rt * k2Rinv1GeVf / ptCut,
+// kSinAlphaMax)) and its tan, kept per mini-doublet
 __device__ __forceinline__ bool pixel_tracklet(const ModulesView& M, const MDsSoA& D, const PixelSeed& s,
                                                uint32_t mdA, uint32_t mdB, uint32_t mdC, uint32_t mdD, unsigned mIn,
-                                               unsigned mOut, float alphaIn, float alphaOut, float ptCut) {
+                                               unsigned mOut, float alphaIn, float alphaOut, float ptCut,
+                                               float2 slopeC) {
   const int16_t subIn = M.subdets[mIn], subOut = M.subdets[mOut];
   const bool bb = subIn == kBarrel && (subOut == kBarrel || subOut == kEndcap);
   const bool ee = subIn == kEndcap && subOut == kEndcap;
@@ -44,9 +46,9 @@ __device__ __forceinline__ bool pixel_tracklet(const ModulesView& M, const MDsSo
   float thetaMuls, dPhi, drtInSeg;
   if (bb) {  // runTripletDefaultAlgoPPBB (:1264-1553)
     if (!(m_abs(delta_phi(x2, y2, x3, y3)) <= 0.5f * kPi)) return false;
-    const float a1 = m_asin(m_min(rt3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+    const float a1 = slopeC.x;
     const float ratio = rt3 / rt2;
-    const float scale = m_tan(a1) / a1;
+    const float scale = slopeC.y / a1;
     const float zpIn = 0.05f, zpInOut = 0.05f;
     const float zGeom = zpIn + zpitchOut;
     const float zHi = z2 + (z2 + kDeltaZLum) * (ratio - 1.f) * (z2 < 0.f ? 1.f : scale) + (zpInOut + zpitchOut);
@@ -78,8 +80,8 @@ __device__ __forceinline__ bool pixel_tracklet(const ModulesView& M, const MDsSo
     if (!(z2 * z3 > 0)) return false;
     const float zpIn = 0.05f;
     const float zGeom = zpIn + zpitchOut;
-    const float slope = m_asin(m_min(rt3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-    const float scale = m_tan(slope) / slope;
+    const float slope = slopeC.x;
+    const float scale = slopeC.y / slope;
     const float dLum = sign_copy(kDeltaZLum, z2);
     const float rtGeom1 = psOut ? kPixelPSZpitch : kStrip2SZpitch;
     const float zGeom1 = sign_copy(zGeom, z2);
@@ -106,7 +108,7 @@ __device__ __forceinline__ bool pixel_tracklet(const ModulesView& M, const MDsSo
     const float drtMean =
         drtDzIn * dzAbs * (1.f - drOutIn * drOutIn * 4 * k2Rinv1GeVf * k2Rinv1GeVf / ptIn / ptIn / 24.f);
     if (!((rt3 >= rt2 + drtMean - rtWin) && (rt3 <= rt2 + drtMean + rtWin))) return false;
-    const float a1 = m_asin(m_min(rt3 * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+    const float a1 = slopeC.x;
     const float pvOff = 0.1f / rt3;
     const float cut = a1 + m_sqrt(muls * muls + pvOff * pvOff);
     dPhi = delta_phi(0.5f * (x1 + x3), 0.5f * (y1 + y3), x3 - x1, y3 - y1);
@@ -377,9 +379,11 @@ __device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, 
   const float alphaPix = v.S.dPhiChanges[pls];
   const uint32_t md1 = v.S.mdIndices[2 * segIn], md2 = v.S.mdIndices[2 * segIn + 1];
   const uint32_t md3 = v.S.mdIndices[2 * segOut], md4 = v.S.mdIndices[2 * segOut + 1];
-  if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md1, md2, lo, mid, alphaPix, v.S.dPhiChanges[segIn], v.ptCut))
+  if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md1, md2, lo, mid, alphaPix, v.S.dPhiChanges[segIn], v.ptCut,
+                      v.WL.mdSlope[md1]))
     return false;
-  return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut);
+  return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut,
+                        v.WL.mdSlope[md3]);
 }
 
 // one of the two pixel-tracklet tests of pt3_tracklets: part 0 the inner
@@ -393,7 +397,8 @@ __device__ __forceinline__ bool pt3_tracklet_half(const EventView& v, uint32_t p
   const PixelSeed s = load_seed(v.P, arr);
   const float alphaPix = v.S.dPhiChanges[pls];
   const uint32_t mdA = v.S.mdIndices[2 * seg], mdB = v.S.mdIndices[2 * seg + 1];
-  return pixel_tracklet(M, v.D, s, pixIn, pixOut, mdA, mdB, mA, mB, alphaPix, v.S.dPhiChanges[seg], v.ptCut);
+  return pixel_tracklet(M, v.D, s, pixIn, pixOut, mdA, mdB, mA, mB, alphaPix, v.S.dPhiChanges[seg], v.ptCut,
+                        v.WL.mdSlope[mdA]);
 }
 
 // runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024) is the AND of pure
