@@ -40,22 +40,22 @@ HBM_PEAK_GBS = 8000.0
 # benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
 # Re-measured whenever a kernel changes; None for kernels not yet profiled.
 MEASURED_TRAFFIC = {
-    "k_hits": 9291658,
-    "k_pixel_segments": 2275345,
-    "k_md_eval": 11193543,
-    "k_ls_eval": 59015460,
-    "k_t3_select": 29453981,
-    "k_t5_final": 30588061,
-    "k_pls_pairs<false>": 1040703,
-    "k_pix_stage_a<PT5Kind>": 13754184,
-    "k_pix_stage_a<PT3Kind>": 9032682,
-    "k_t5_tc_pairs": 3107492,
-    "k_pt5_pairs": 3444474,
+    "k_hits": 9291438,
+    "k_pixel_segments": 2275347,
+    "k_md_eval": 11193552,
+    "k_ls_eval": 59016072,
+    "k_t3_select": 29454083,
+    "k_t5_final": 30587625,
+    "k_pls_pairs<false>": 1040702,
+    "k_pix_stage_a<PT5Kind>": 13754338,
+    "k_pix_stage_a<PT3Kind>": 9036923,
+    "k_t5_tc_pairs": 3106630,
+    "k_pt5_pairs": 3444484,
     "k_pt3_pairs": 130123,
-    "k_cross_pls_pairs": 1850339,
-    "k_pls_pairs<true>": 1066260,
-    "k_t5_dup_after_build": 2989936,
-    "k_t3_enum": 11423063,
+    "k_cross_pls_pairs": 1850537,
+    "k_pls_pairs<true>": 1066195,
+    "k_t5_dup_after_build": 2989959,
+    "k_t3_enum": 11423061,
 }
 
 
