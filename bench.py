@@ -1,78 +1,55 @@
-"""Throughput benchmark of the MI355X LST engine.
+"""Throughput benchmark of the MI355X LST engine on uneven batches.
 
-One step = the LST chain on one synthetic PU200-like event (about 7000
-helices with pT = 0.4 + Exp(0.5) GeV plus 200 hard tracks, 1% noise hits,
-OT800_IT615_pt0.8 geometry), inputs already resident in HBM (prepared once on
-the host and staged; generation and staging are outside the timed region).
-Prints ONE JSON line (see the `result` dict) with the roofline of the
-dominant kernel and the oracle's CPU throughput on a bounded sample.
+One step = one launch set of the batched engine over the 64 events of an
+uneven batch (pileup drawn U{50..250} per event, 35 charged helices per
+interaction with pT = 0.4 + Exp(0.5) GeV, 200 hard tracks, 1% noise hits,
+OT800_IT615_pt0.8 geometry).  The events are prepared on the host once
+(pinned; the reference preloads its inputs the same way, bin/sdl.cc:352-384);
+inside the timed region each step copies its events' inputs to the device
+(as the reference's timed loop does, code/core/trkCore.cc:877-879), runs the
+whole chain, and copies the track-candidate lists back to the host.
+
+The same events also run through the plain per-event path (one event handle,
+one event at a time, the same host-to-device copy and output copy per event),
+and the two track-candidate sets are compared event by event.  Prints ONE
+JSON line with the roofline of the dominant kernel of the batched launch set
+and the oracle's CPU throughput on all host cores and on one.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The batched engine runs one HIP stream per event context.  HIP gives a
-# process 4 hardware queues unless told otherwise and the default stream holds
-# one of them, so a fourth context would share a queue (and serialize) with
-# another; 8 queues give every context its own.  Read when HIP initializes.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "PU200-like events/sec on 1 MI355X, batched vs per-event, at matched TC set"
-STAGE_NAMES = ["hits", "pixel_segments", "mini_doublets", "segments", "triplets", "quintuplets",
-               "pixel_segment_cleaning", "pixel_quintuplets", "pixel_triplets", "track_candidates"]
-# the kernel each timer slot brackets (lst_event_kernel_times)
+# the kernel each per-event timer slot brackets (lst_event_kernel_times)
 KERNEL_NAMES = ["k_hits", "k_pixel_segments", "k_md_eval", "k_ls_eval", "k_t3_select", "k_t5_final",
                 "k_pls_pairs<false>", "k_pix_stage_a<PT5Kind>", "k_pix_stage_a<PT3Kind>", "k_t5_tc_pairs",
                 "k_pt5_pairs", "k_pt3_pairs", "k_cross_pls_pairs", "k_pls_pairs<true>", "k_t5_dup_after_build",
                 "k_t3_enum", "k_t5_dnn"]
-ORACLE_STAGES = ["hits", "pls", "md", "ls", "t3", "t5", "plsclean", "pt5", "pt3", "tc"]
-LAST_STAGE = 9  # the full chain up to track candidates
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this
-# benchmark, scaled by a 1 GiB copy calibration (profiles/pmc_traffic_pu200_chain.md).
-# Re-measured whenever a kernel changes; None for kernels not yet profiled.
-MEASURED_TRAFFIC = {
-    "k_hits": 9291438,
-    "k_pixel_segments": 2275347,
-    "k_md_eval": 11193552,
-    "k_ls_eval": 59016072,
-    "k_t3_select": 29454083,
-    "k_t5_final": 30587625,
-    "k_pls_pairs<false>": 1040702,
-    "k_pix_stage_a<PT5Kind>": 13754338,
-    "k_pix_stage_a<PT3Kind>": 9036923,
-    "k_t5_tc_pairs": 3106630,
-    "k_pt5_pairs": 3444484,
-    "k_pt3_pairs": 130123,
-    "k_cross_pls_pairs": 1850537,
-    "k_pls_pairs<true>": 1066195,
-    "k_t5_dup_after_build": 2989959,
-    "k_t3_enum": 11423061,
-}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_c3_batch.md")
 
 
-def _input_struct(lst, ev):
-    import numpy as np
-    a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
-    p = lambda x: x.ctypes.data_as(C.c_void_p)
-    inp = lst._Input(
-        *[p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
-                            "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ", "see_stateTrajGlbPx",
-                            "see_stateTrajGlbPy", "see_stateTrajGlbPz", "see_q", "see_hitIdx", "see_hitIdx_off")],
-        len(a["see_px"]), *[p(a[k]) for k in ("ph2_detId", "ph2_x", "ph2_y", "ph2_z")], len(a["ph2_x"]))
-    return inp, a
+def event_counts(lst, ev) -> dict:
+    L = lst.lib()
+    c = lambda k: int(L.lst_event_count(ev.h, k.encode()))
+    return {"hits": c("hits"), "mds": int(ev.field("mds.nMDs").sum()), "segs": int(ev.field("segs.nSegments").sum()),
+            "t3": int(ev.field("t3.nTriplets").sum()), "t5": int(ev.field("t5.nQuintuplets").sum()),
+            "pls": c("pixelSegments"), "pt5": c("pixelQuintuplets"), "pt3": c("pixelTriplets"),
+            "tc": c("trackCandidates"), "t3surv": c("tripletSharedMDCandidates"), "mdcand": c("miniDoubletCandidates"),
+            "lscand": c("segmentCandidates"), "pt5surv": c("pixelQuintupletSurvivors"),
+            "pt3surv": c("pixelTripletSurvivors"), "arena_mib": c("arenaMiB")}
 
 
-def algorithmic_bytes(counts: dict) -> dict:
-    """Minimal HBM bytes per launch of each timed kernel: inputs read once, outputs written once.
+def algorithmic_bytes(n: dict) -> dict:
+    """Minimal HBM bytes of each kernel for one event: inputs read once, outputs written once.
 
     Record sizes are the fields each kernel reads/writes:
       k_hits             : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
@@ -82,7 +59,7 @@ def algorithmic_bytes(counts: dict) -> dict:
       k_ls_eval          : read 44 B of each mini-doublet, write a 4 B result index per candidate and
                            24 B of parked results per segment
       k_t3_enum          : per inner segment 20 B (outer mini-doublet, its segment range, prefix, outer module),
-                           per shared-MD candidate 4 B read (listed segment) and 12 B written
+                           per shared-MD candidate 16 B
       k_t3_select        : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
       k_t5_final         : read 53 B/triplet, 28 B/segment, 52 B/mini-doublet, write 119 B/quintuplet
       k_pls_pairs        : read eta, phi, score, isQuad, flag and 4 hit indices (30 B/pLS), write 1 B/pLS
@@ -93,57 +70,95 @@ def algorithmic_bytes(counts: dict) -> dict:
       k_t5_tc_pairs,
       k_t5_dup_after_build: read eta, phi, score, flags and 10 hit indices (54 B/quintuplet), write 1 B
       k_cross_pls_pairs  : read 50 B/pLS (direction, flags, hit indices through the MDs), 25 B/track candidate
+      k_out_copy         : read and write the 68 B output record of each track candidate
     """
-    h, md, ls, t3, t5 = counts["hits"], counts["mds"], counts["segs"], counts["t3"], counts["t5"]
-    pls, pt5, pt3, tc = counts["pls"], counts["pt5"], counts["pt3"], counts["tc"]
+    h, md, ls, t3, t5 = n["hits"], n["mds"], n["segs"], n["t3"], n["t5"]
+    pls, pt5, pt3, tc = n["pls"], n["pt5"], n["pt3"], n["tc"]
     return {
         "k_hits": 46 * h,
         "k_pixel_segments": 500 * pls,
-        "k_md_eval": 16 * h + 4 * counts.get("mdcand", 0) + 36 * md,
-        "k_ls_eval": 44 * md + 4 * counts.get("lscand", 0) + 24 * ls,
-        "k_t3_enum": 20 * ls + 16 * counts.get("t3surv", 0),
+        "k_md_eval": 16 * h + 4 * n["mdcand"] + 36 * md,
+        "k_ls_eval": 44 * md + 4 * n["lscand"] + 24 * ls,
+        "k_t3_enum": 20 * ls + 16 * n["t3surv"],
         "k_t3_select": 22 * ls + 20 * md + 60 * t3,
         "k_t5_final": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
-        "k_pls_pairs<false>": 31 * pls,
-        "k_pls_pairs<true>": 31 * pls,
-        "k_pix_stage_a<PT5Kind>": 64 * pls + 128 * t5 + 40 * counts.get("pt5surv", 0),
-        "k_pix_stage_a<PT3Kind>": 64 * pls + 64 * t3 + 40 * counts.get("pt3surv", 0),
+        "k_pls_pairs": 31 * pls,
+        "k_pix_stage_a<PT5Kind>": 64 * pls + 128 * t5 + 40 * n["pt5surv"],
+        "k_pix_stage_a<PT3Kind>": 64 * pls + 64 * t3 + 40 * n["pt3surv"],
         "k_t5_tc_pairs": 55 * t5,
         "k_pt5_pairs": 61 * pt5,
-        "k_pt3_pairs": 46 * pt3,
+        "k_pt3_pairs (y shares)": 46 * pt3,
         "k_cross_pls_pairs": 50 * pls + 25 * tc,
         "k_t5_dup_after_build": 55 * t5,
+        "k_out_copy": 136 * tc,
     }
 
 
-def cpu_baseline(events, budget_s: float = 20.0) -> dict:
-    """The oracle (serial C++ restatement, 1 thread) on a bounded sample of the same events."""
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (calibrated; see the file), or None when the kernel is not in it."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        m = re.match(r"\|\s*`?([^|`]+?)`?\s*\|\s*([0-9.]+)\s*\|\s*([0-9.]+)\s*\|\s*([0-9.]+)\s*\|", line)
+        if m and m.group(1).strip() == kernel:
+            return float(m.group(4)) * 1e6  # total MB per dispatch
+    return None
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(events, budget_s: float, threads: int) -> dict:
+    """The oracle (serial C++ restatement, one event per thread as the reference runs one event per OpenMP
+    thread, bin/sdl.cc:405-419) over the same events for about budget_s seconds."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import orc
     from tracklooper_amd import synth
     g = orc.Geometry(synth.geometry_dir())
-    n, t0 = 0, time.perf_counter()
-    stage = ORACLE_STAGES[LAST_STAGE]
-    while time.perf_counter() - t0 < budget_s and n < 4 * len(events):
-        orc.Event(g).run_lst(events[n % len(events)], stage)
-        n += 1
+    t0 = time.perf_counter()
+    done = [0] * threads
+
+    def worker(w):
+        i = w
+        while time.perf_counter() - t0 < budget_s:
+            orc.Event(g).run_lst(events[i % len(events)], "tc")  # ctypes releases the GIL during the call
+            done[w] += 1
+            i += threads
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(worker, range(threads)))
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": f"{n} PU200-like events through the same stages, serial oracle, {dt:.1f} s"}
+    return {"value": sum(done) / dt, "events": sum(done), "seconds": dt, "threads": threads}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--events", type=int, default=4, help="distinct seeded events cycled through")
-    ap.add_argument("--n-pu", type=int, default=200)
-    ap.add_argument("--contexts", type=int, default=4, help="event contexts of the batched engine")
+    ap.add_argument("--steps", type=int, default=30, help="timed launch sets (64 events each)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch-events", type=int, default=64, help="events per launch set")
+    ap.add_argument("--batches", type=int, default=2, help="distinct uneven batches cycled through")
+    ap.add_argument("--per-event-steps", type=int, default=2, help="batches the per-event path is timed on")
     ap.add_argument("--capacity", type=int, default=8192, help="track candidates copied back per event")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline configuration")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the host cores available (at most 16)")
+    ap.add_argument("--workers", type=int, default=8, help="processes generating the events")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (reference for FETCH_SIZE/WRITE_SIZE scaling)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="stage, tune and run --steps launch sets, nothing else (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (each box has one MI355X); --gpus {args.gpus} is not supported.",
@@ -151,6 +166,17 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    from tracklooper_amd import synth
+
+    if not args.copy_calibration:  # events are generated before the process touches the GPU (fork pool)
+        t_gen = time.perf_counter()
+        E = args.batch_events
+        events = []
+        for b in range(args.batches):
+            events += synth.c3_batch(b, E, workers=args.workers)
+        t_gen = time.perf_counter() - t_gen
+        print(f"generated {len(events)} events in {t_gen:.1f} s", file=sys.stderr, flush=True)
+
     import torch
     if args.copy_calibration:
         x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -162,116 +188,115 @@ def main():
         print("copied 1 GiB device-to-device 3 times", flush=True)
         return
     import tracklooper_amd as lst
-    from tracklooper_amd import synth
-
-    t_gen = time.perf_counter()
-    events = [synth.generate_event("pu", i, n_pu=args.n_pu) for i in range(args.events)]
-    t_gen = time.perf_counter() - t_gen
-    es = lst.LSTESData(synth.geometry_dir())
     L = lst.lib()
+    es = lst.LSTESData(synth.geometry_dir())
+    n_ev = len(events)
 
-    # batched engine (measured first, while its contexts' streams are the only ones besides the default
-    # stream): contexts run the events round robin, outputs copied to the host
-    batch = lst.Batch(es, n_contexts=args.contexts)
+    # ---- batched engine: launch sets of E events
+    batch = lst.Batch(es, slots=E)
     batch.stage(events)
-    outs, bufs = lst.Batch._outputs(args.steps, args.capacity)
-    for b in bufs:  # host output buffers mapped before the timed region
-        for a in b.values():
-            a.fill(0)
-    # untimed: graph capture in every context and the clocks' ramp after the host-side work above
-    batch_warmup = max(args.warmup, len(events), 8 * args.contexts)
-    warm_outs, warm_bufs = lst.Batch._outputs(batch_warmup, args.capacity)
-    lst._check(L.lst_batch_run_staged(batch.h, 0, batch_warmup, warm_outs))
+    tune = {"default_ms": None, "tuned_ms": None}
+    if not args.no_autotune:
+        d_ms, t_ms, _ = batch.autotune(0, repeats=2)
+        tune = {"default_ms": d_ms, "tuned_ms": t_ms}
+        print(f"autotune: summed kernel time {d_ms:.2f} ms at the default work-group counts, {t_ms:.2f} ms tuned",
+              file=sys.stderr, flush=True)
+    outs, bufs = lst.Batch._outputs(E, args.capacity)
+    for w in range(args.warmup):
+        lst._check(L.lst_batch_run_staged(batch.h, (w % args.batches) * E, E, outs))
     torch.cuda.synchronize()
+    if args.profile_only:
+        for s in range(args.steps):
+            lst._check(L.lst_batch_run_staged(batch.h, (s % args.batches) * E, E, outs))
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_only": True, "steps": args.steps}), flush=True)
+        return
     t0 = time.perf_counter()
-    lst._check(L.lst_batch_run_staged(batch.h, 0, args.steps, outs))
+    for s in range(args.steps):  # each call: inputs to the device, the chain, outputs to the host
+        lst._check(L.lst_batch_run_staged(batch.h, (s % args.batches) * E, E, outs))
     torch.cuda.synchronize()
     wall_batch = time.perf_counter() - t0
+    # outputs of every event for the comparison with the per-event path
+    batched = []
+    for b in range(args.batches):
+        batched += batch.run_staged(b * E, E, args.capacity)
+    # per-kernel device time of one launch set (no graph), for the roofline
+    kms = np.zeros(L.lst_tune_count(), np.float32)
+    L.lst_batch_kernel_times.argtypes = [lst.C.c_void_p, lst.C.c_uint32, lst.C.c_void_p, lst.C.c_uint32]
+    lst._check(L.lst_batch_kernel_times(batch.h, 0, kms.ctypes.data_as(lst.C.c_void_p), len(kms)))
+    batch_kernel_ms = {L.lst_tune_name(k).decode(): float(kms[k]) for k in range(len(kms)) if kms[k] > 0}
+    tuning = batch.tuning()
     batch.close()
 
-    # per-event engine: one context, every kernel launched for each event
+    # ---- plain per-event path: one handle, one event at a time
+    import ctypes as C
     ev = lst.Event(es)
     L.lst_event_stage_input.argtypes = [C.c_void_p, C.POINTER(lst._Input), C.c_uint32]
     L.lst_event_run_staged.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
     L.lst_event_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.lst_event_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
-    L.lst_event_stream.argtypes = [C.c_void_p]
-    L.lst_event_stream.restype = C.c_void_p
     for i, e in enumerate(events):
-        inp, keep = _input_struct(lst, e)
+        inp, keep = lst._make_input(e)
         lst._check(L.lst_event_stage_input(ev.h, C.byref(inp), i))
-    stream = torch.cuda.ExternalStream(L.lst_event_stream(ev.h))
-    torch.cuda.synchronize()
+    n_tc = C.c_uint32(0)
+    cap = args.capacity
+    hits = np.zeros((cap, 14), np.uint32)
+    ln = np.zeros(cap, np.uint32)
+    sd = np.zeros(cap, np.int32)
+    ty = np.zeros(cap, np.int16)
 
-    def step(i):
-        lst._check(L.lst_event_run_staged(ev.h, i % len(events), LAST_STAGE))
+    def one_event(i):  # inputs to the device, the chain, the output lists to the host
+        lst._check(L.lst_event_run_staged(ev.h, i, 9))
+        lst._check(L.lst_event_get_output(ev.h, C.byref(n_tc), lst._p(hits), lst._p(ln), lst._p(sd), lst._p(ty),
+                                          cap))
 
-    for i in range(args.warmup):
-        step(i)
-    ev.synchronize()
+    for i in range(min(n_ev, 8)):
+        one_event(i)
     torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n_per = args.per_event_steps * E
     t0 = time.perf_counter()
-    start.record(stream)
-    for i in range(args.steps):
-        step(i)
-    stop.record(stream)
-    ev.synchronize()
+    for i in range(n_per):
+        one_event(i % n_ev)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(stop)
-    ms_per_step = wall * 1e3 / args.steps
+    wall_per = time.perf_counter() - t0
 
-    # per-kernel device time of the main kernel of every stage, and object counts per event
+    # outputs and object counts of every event; per-kernel times of the per-event path
+    per_event_out, counts = [], []
     lst._check(L.lst_event_set_timing(ev.h, 1))
-    ktimes = np.zeros((len(events), len(KERNEL_NAMES)), np.float32)
-    counts = []
-    for i in range(len(events)):
-        step(i)
+    ktimes = np.zeros((n_ev, len(KERNEL_NAMES)), np.float32)
+    for i in range(n_ev):
+        lst._check(L.lst_event_run_staged(ev.h, i, 9))
         lst._check(L.lst_event_kernel_times(ev.h, ktimes[i].ctypes.data_as(C.c_void_p), len(KERNEL_NAMES)))
-        counts.append({"hits": int(lst.lib().lst_event_count(ev.h, b"hits")),
-                       "mds": int(ev.field("mds.nMDs").sum()),
-                       "segs": int(ev.field("segs.nSegments").sum()),
-                       "t3": int(ev.field("t3.nTriplets").sum()),
-                       "t5": int(ev.field("t5.nQuintuplets").sum()),
-                       "pls": int(lst.lib().lst_event_count(ev.h, b"pixelSegments")),
-                       "pt5": int(lst.lib().lst_event_count(ev.h, b"pixelQuintuplets")),
-                       "pt3": int(lst.lib().lst_event_count(ev.h, b"pixelTriplets")),
-                       "tc": int(lst.lib().lst_event_count(ev.h, b"trackCandidates")),
-                       "t3cand": int(lst.lib().lst_event_count(ev.h, b"tripletCandidates")),
-                       "t3surv": int(lst.lib().lst_event_count(ev.h, b"tripletSharedMDCandidates")),
-                       "mdcand": int(lst.lib().lst_event_count(ev.h, b"miniDoubletCandidates")),
-                       "lscand": int(lst.lib().lst_event_count(ev.h, b"segmentCandidates")),
-                       "pt5surv": int(lst.lib().lst_event_count(ev.h, b"pixelQuintupletSurvivors")),
-                       "pt3surv": int(lst.lib().lst_event_count(ev.h, b"pixelTripletSurvivors")),
-                       "arena_mib": int(lst.lib().lst_event_count(ev.h, b"arenaMiB"))})
-    lst._check(L.lst_event_set_timing(ev.h, 0))
-    per_event_out = []
-    for i in range(len(events)):
-        step(i)
         per_event_out.append(ev.getOutput())
+        counts.append(event_counts(lst, ev))
+    lst._check(L.lst_event_set_timing(ev.h, 0))
 
-    batched = lst.Batch._collect(outs, bufs)
-    tc_match = all(b["n_tc"] == len(per_event_out[i % len(events)]["tc_hitIdxs"]) and
-                   b["tc_hitIdxs"] == per_event_out[i % len(events)]["tc_hitIdxs"] and
-                   np.array_equal(b["tc_trackCandidateType"], per_event_out[i % len(events)]["tc_trackCandidateType"])
-                   for i, b in enumerate(batched))
+    tc_match = all(b["n_tc"] == len(p["tc_hitIdxs"]) and b["tc_hitIdxs"] == p["tc_hitIdxs"] and
+                   np.array_equal(b["tc_trackCandidateType"], p["tc_trackCandidateType"]) and
+                   np.array_equal(b["tc_seedIdx"], p["tc_seedIdx"])
+                   for b, p in zip(batched, per_event_out))
     if not tc_match:
         raise SystemExit("batched track candidates differ from the per-event ones")
     from tracklooper_amd import metrics
     physics = metrics.summarize([metrics.event_metrics(o, e) for o, e in zip(per_event_out, events)])
 
-    avg_ms = ktimes.mean(0)[: len(KERNEL_NAMES)]
-    per_kernel = {KERNEL_NAMES[k]: float(avg_ms[k]) for k in range(len(KERNEL_NAMES))}
-    dom = max(per_kernel, key=per_kernel.get)
-    mean_counts = {k: float(np.mean([c[k] for c in counts])) for k in counts[0]}
-    abytes = algorithmic_bytes(mean_counts)
-    dom_bytes = abytes.get(dom, 0.0)
-    achieved = dom_bytes / (per_kernel[dom] * 1e-3) / 1e9 if per_kernel[dom] > 0 else 0.0
+    # roofline of the dominant kernel of the batched launch set (first batch)
+    first = counts[:E]
+    abytes = {}
+    for c in first:
+        for k, b in algorithmic_bytes(c).items():
+            abytes[k] = abytes.get(k, 0) + b
+    dom = max(batch_kernel_ms, key=batch_kernel_ms.get)
+    dom_bytes = abytes.get(dom)
+    achieved = dom_bytes / (batch_kernel_ms[dom] * 1e-3) / 1e9 if dom_bytes else None
+    traffic = measured_traffic(dom)
 
+    value = args.steps * E / wall_batch
+    per_event_rate = n_per / wall_per
+    pus = np.concatenate([synth.c3_pileups(b, E) for b in range(args.batches)])
     result = {
         "metric": METRIC,
-        "value": args.steps / wall_batch,
+        "value": value,
         "unit": "events/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -283,33 +308,45 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched engine, {args.contexts} event contexts round robin, one PU{args.n_pu}-like "
-                        f"synthetic event per step ({args.events} seeded events cycled, inputs resident in HBM, "
-                        f"output lists copied to the host), stages hits..{STAGE_NAMES[LAST_STAGE]}",
-            "batched_events_per_s": args.steps / wall_batch,
-            "per_event_events_per_s": args.steps / wall,
-            "per_event_ms_per_step": ms_per_step,
-            "batched_over_per_event": (args.steps / wall_batch) / (args.steps / wall),
+            "workload": f"uneven batches of {E} synthetic events, pileup U{{50..250}} per event "
+                        f"(mean {pus.mean():.0f}), OT800_IT615_pt0.8, plain chi2 T5 selection; one step = one "
+                        f"launch set of {E} events: inputs host-to-device, hits..track candidates, output lists "
+                        f"device-to-host; {args.batches} batches cycled",
+            "batched_events_per_s": value,
+            "per_event_events_per_s": per_event_rate,
+            "per_event_ms_per_event": wall_per * 1e3 / n_per,
+            "batched_over_per_event": value / per_event_rate,
             "tc_sets_match": tc_match,
+            "events_compared": n_ev,
             "physics": {k: physics[k] for k in ("sim_tracks", "efficiency", "fake_rate", "duplicate_rate")},
-            "contexts": args.contexts,
-            "batch_warmup_events": batch_warmup,
-            "gpu_max_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
-            "device_mib_per_context": counts[0]["arena_mib"],
+            "slots": E,
+            "autotune_summed_kernel_ms": tune,
+            "tuned_grid": {k: v for k, v in tuning.items() if v},
+            "device_mib_per_slot": counts[0]["arena_mib"],
             "geometry": "OT800_IT615_pt0.8",
-            "mean_hits": mean_counts["hits"], "mean_mini_doublets": mean_counts["mds"],
-            "mean_segments": mean_counts["segs"], "mean_triplets": mean_counts["t3"],
-            "mean_quintuplets": mean_counts["t5"], "mean_pixel_quintuplets": mean_counts["pt5"],
-            "mean_pixel_triplets": mean_counts["pt3"], "mean_track_candidates": mean_counts["tc"],
-            "per_event_gpu_ms_per_step": gpu_ms / args.steps,
-            "kernel_ms": per_kernel,
+            "mean_hits": float(np.mean([c["hits"] for c in counts])),
+            "min_hits": int(np.min([c["hits"] for c in counts])),
+            "max_hits": int(np.max([c["hits"] for c in counts])),
+            "mean_track_candidates": float(np.mean([c["tc"] for c in counts])),
+            "batch_kernel_ms": batch_kernel_ms,
+            "batch_kernel_ms_sum": float(sum(batch_kernel_ms.values())),
+            "per_event_kernel_ms": {KERNEL_NAMES[k]: float(ktimes[:, k].mean()) for k in range(len(KERNEL_NAMES))},
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dom),
-                     "algorithmic_bytes": dom_bytes},
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "algorithmic_bytes": dom_bytes, "kernel_ms": batch_kernel_ms[dom],
+                     "events_per_launch": E},
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(events, args.cpu_budget)
+        threads = args.cpu_threads or min(len(os.sched_getaffinity(0)), 16)
+        one = cpu_baseline(events[:E], args.cpu_budget, 1)
+        many = cpu_baseline(events[:E], args.cpu_budget, threads) if threads > 1 else one
+        result["cpu_baseline"] = {
+            "value": many["value"], "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"{many['events']} events of the first uneven batch in {many['seconds']:.1f} s, one event per "
+                      f"thread, {threads} threads ({cpu_model()}); 1 thread: {one['value']:.3f} events/s "
+                      f"({one['events']} events in {one['seconds']:.1f} s)",
+            "one_thread_value": one["value"], "cpu_model": cpu_model()}
     print(json.dumps(result), flush=True)
 
 

@@ -23,13 +23,17 @@ typedef struct lst_es lst_es;        /* geometry / event setup (LSTESDeviceData)
 typedef struct lst_event lst_event;  /* one event's device state (SDL::Event)    */
 
 /* Status codes.  LST_W_TRUNCATED is a warning bit OR-ed into LST_OK when a
- * capacity truncated objects exactly as the reference would (the reference
- * only prints a banner, SDL/Event.cc:237-244,759-781). */
+ * capacity truncated objects exactly as the reference would: more than 50000
+ * pixel segments, or a per-module mini-doublet / segment / triplet /
+ * quintuplet capacity, or the pT5 / pT3 capacity (the reference only prints a
+ * banner, SDL/Event.cc:237-244,759-781, SDL/MiniDoublet.h:960-965). */
 typedef enum {
   LST_OK = 0,
   LST_E_IO = 1,        /* geometry file missing or unreadable (reference: exit(2), SDL/LSTESData.cc:31-38) */
   LST_E_FORMAT = 2,    /* malformed geometry file (reference: std::runtime_error, SDL/ModuleConnectionMap.cc:14-46) */
-  LST_E_INPUT = 3,     /* bad input, e.g. a detId absent from the geometry (reference: indexes mapIdx[-1], SDL/Hit.h:245-246) */
+  LST_E_INPUT = 3,     /* bad input: a detId absent from the geometry (reference: indexes mapIdx[-1],
+                          SDL/Hit.h:245-246), more than 127 hits on a module (its int8 counts wrap,
+                          SDL/Hit.h:31-32), a module's hits not contiguous (SDL/Hit.h:261-267) */
   LST_E_ORDER = 4,     /* stage called out of order */
   LST_E_DEVICE = 5,    /* HIP runtime error */
   LST_E_CAPACITY = 6,  /* output buffer too small */
@@ -84,8 +88,9 @@ lst_status lst_event_create_mini_doublets(lst_event* ev);        /* createMiniDo
 lst_status lst_event_create_segments(lst_event* ev);             /* createSegmentsWithModuleMap */
 lst_status lst_event_create_triplets(lst_event* ev);             /* createTriplets */
 /* createQuintuplets (SDL/Event.cc:903-991): eligible-module list, T5 builder,
- * after-build duplicate flags.  Needs an event created with
- * LST_T5_PLAIN_CHI2; LST_E_INPUT otherwise. */
+ * after-build duplicate flags.  The T5 selection is the event's mode: the r-phi
+ * chi2 cut (LST_T5_PLAIN_CHI2) or the T5 DNN at its working point
+ * (LST_T5_DNN). */
 lst_status lst_event_create_quintuplets(lst_event* ev);
 /* pixelLineSegmentCleaning (SDL/Event.cc:993-1007): first checkHitspLS pass. */
 lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev);
@@ -99,7 +104,11 @@ lst_status lst_event_create_pixel_triplets(lst_event* ev);
 lst_status lst_event_create_track_candidates(lst_event* ev);
 
 /* Replaces SDL::LST<Acc>::getOutput / getHitIdxs (SDL/LST.cc:366-420) after
- * createTrackCandidates: *n_tc receives the number of track candidates; when
+ * createTrackCandidates (waits for the event's stream).  LST_E_INPUT if the
+ * device-side hit checks found bad input (hits given as device pointers are
+ * checked there; host hits are checked by lst_run and lst_event_stage_input
+ * before any upload), LST_W_TRUNCATED if a capacity truncated objects.
+ * *n_tc receives the number of track candidates; when
  * any array is given (each may be NULL) and capacity >= *n_tc, hit_idxs
  * receives 14 entries per candidate (the first len[i] are the candidate's
  * ntuple hit indices, the reference's out_tc_hitIdxs_), len the list lengths,
@@ -110,7 +119,8 @@ lst_status lst_event_get_output(lst_event* ev, uint32_t* n_tc, uint32_t* hit_idx
 
 /* Replaces SDL::Event::resetEvent (SDL/Event.cc:45-149); keeps the arenas. */
 lst_status lst_event_reset(lst_event* ev);
-/* Waits for the event's stream. */
+/* Waits for the event's stream; then the status of the device-side checks
+ * (as lst_event_get_output). */
 lst_status lst_event_synchronize(lst_event* ev);
 
 /* Replaces SDL::Event::getNumberOf* (SDL/Event.h:152-183): "hits", "pixelSegments",
@@ -152,15 +162,16 @@ typedef struct {
 /* Replaces SDL::LST<Acc>::run (SDL/LST.cc:12-161) up to and including
  * `last_stage` (0 hits, 1 pixel segments, 2 mini-doublets, 3 segments,
  * 4 triplets, 5 quintuplets, 6 pLS cleaning, 7 pixel quintuplets, 8 pixel
- * triplets, 9 track candidates): prepareInput on the host (SDL/LST.cc:175-365), then the stage
- * sequence on the device. */
+ * triplets, 9 track candidates): prepareInput on the host (SDL/LST.cc:175-365),
+ * the hit checks of LST_E_INPUT, then the stage sequence on the device. */
 lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage);
 
-/* Throughput path (no reference counterpart; the reference preloads inputs on
- * the host, bin/sdl.cc:352-384): prepare an event on the host once and keep it
- * resident in HBM in input slot `slot` of the event handle ... */
+/* Throughput path (the reference preloads its inputs on the host,
+ * bin/sdl.cc:352-384, and its timed loop copies them to the device,
+ * code/core/trkCore.cc:877-879): prepare an event on the host once (with the
+ * hit checks) and keep it in pinned host memory in input slot `slot` ... */
 lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t slot);
-/* ... then run the chain on it with no host work and no host round trip
+/* ... then copy it to the device in one transfer and run the chain on it
  * (asynchronous on the event's stream; same results as lst_run). */
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage);
 
@@ -197,9 +208,14 @@ lst_status lst_t5dnn_backward(const float* x, const float* params, const float* 
 
 /* ---- Batched engine.  The reference runs one SDL::Event per queue with one
  * OpenMP thread per queue (bin/sdl.cc:389-419) and LST::run one event at a
- * time (SDL/LST.cc:12-161); a batch keeps `n_contexts` event contexts (device
- * arena + stream each) on one GPU and runs a batch of events round robin, so
- * the contexts' kernels overlap.  Per event the results equal lst_run's. */
+ * time (SDL/LST.cc:12-161).  A batch holds `n_slots` event arenas on one GPU
+ * and runs up to n_slots events in one launch set: each kernel of the stage
+ * sequence is launched once for all of them, and the sequence is one captured
+ * HIP graph.  Inputs are prepared on the host and copied to the device inside
+ * the run, overlapped with the previous launch set; outputs come back to
+ * pinned host memory.  An event whose candidates overflow the flattened
+ * builders' work arrays is run again by a per-event handle (per-module
+ * builders).  Per event the results equal lst_run's. */
 typedef struct lst_batch lst_batch;
 
 /* Output lists of one event, as LST::hits()/len()/seedIdx()/trackCandidateType()
@@ -214,19 +230,41 @@ typedef struct {
   uint32_t* len;      /* [capacity] */
   int32_t* seed_idx;  /* [capacity] */
   int16_t* type;      /* [capacity] 4 T5, 5 pT3, 7 pT5, 8 pLS */
+  int32_t status;     /* out: the event's lst_status (LST_E_INPUT, LST_W_TRUNCATED, ...) */
 } lst_output;
 
-lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_batch** out);
+/* n_slots: events per launch set, 1..128; each slot holds one event arena. */
+lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch** out);
 void lst_batch_destroy(lst_batch* b);
 uint32_t lst_batch_contexts(const lst_batch* b);
-/* n_ev events from host arrays (prepareInput on the host per event);
- * out: NULL or n_ev output descriptors. */
+/* n_ev events from host arrays (prepareInput on the host per event, then
+ * launch sets of n_slots events); out: NULL or n_ev output descriptors.
+ * Returns the first event error (each event's own status is in out[i].status),
+ * with LST_W_TRUNCATED if any event was truncated. */
 lst_status lst_run_batch(lst_batch* b, const lst_input* events, uint32_t n_ev, lst_output* out);
-/* Throughput path: prepare n_ev events once and keep them in HBM (slots 0..n_ev-1) ... */
+/* Throughput path: prepare n_ev events once and keep them in pinned host
+ * memory (staged events 0..n_ev-1) ... */
 lst_status lst_batch_stage(lst_batch* b, const lst_input* events, uint32_t n_ev);
-/* ... then run n_ev events from slot `first` on with no host preparation;
- * event i uses slot (first + i) modulo the number of staged slots. */
+/* ... then run n_ev events from staged event `first` on (inputs copied to the
+ * device inside the run, no host preparation); event i is staged event
+ * (first + i) modulo the number staged. */
 lst_status lst_batch_run_staged(lst_batch* b, uint32_t first, uint32_t n_ev, lst_output* out);
+/* Autotuner (no reference counterpart): on one launch set of staged events
+ * from `first` on, times every grid-stride kernel of the sequence (the
+ * lst_tune_count() kernels named by lst_tune_name) at each of a list of
+ * work-group counts, `repeats` times, and keeps the fastest count per kernel
+ * for the batch's later runs; *default_ms / *tuned_ms (may be NULL) get the
+ * summed kernel times at the default and the chosen counts. */
+lst_status lst_batch_autotune(lst_batch* b, uint32_t first, uint32_t repeats, float* default_ms, float* tuned_ms);
+/* Device time in ms of each tunable kernel (n entries, in lst_tune_name
+ * order; 0 for a kernel the sequence does not run) over one launch set of
+ * staged events from `first` on, launched without the graph (synchronises). */
+lst_status lst_batch_kernel_times(lst_batch* b, uint32_t first, float* ms, uint32_t n);
+/* The chosen work-group counts (0: the kernel's default), n entries. */
+lst_status lst_batch_get_tuning(const lst_batch* b, uint32_t* grid, uint32_t n);
+lst_status lst_batch_set_tuning(lst_batch* b, const uint32_t* grid, uint32_t n);
+uint32_t lst_tune_count(void);
+const char* lst_tune_name(uint32_t k);
 
 /* Per-kernel device timing (hipEvents on the event's stream).  After a run
  * with timing on, lst_event_kernel_times writes the durations in ms of the

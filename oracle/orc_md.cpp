@@ -468,6 +468,8 @@ void event_create_mini_doublets(Event& ev) {
     int limit = nUp * nLo;
     for (int h = 0; h < limit; h++) {
       int li = h / nUp, ui = h % nUp;
+      if (ui >= nUp) continue;  // SDL/MiniDoublet.h:919-923: counts wrapped negative skip every pair
+      if (li >= nLo) continue;
       unsigned lh = loBase + li, uh = upBase + ui;
       float xL = ev.xs[lh], yL = ev.ys[lh], zL = ev.zs[lh], rL = ev.rts[lh];
       float xU = ev.xs[uh], yU = ev.ys[uh], zU = ev.zs[uh], rU = ev.rts[uh];

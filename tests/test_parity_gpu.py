@@ -88,25 +88,56 @@ def test_repeat_runs_identical(engine, oracle_geometry):
 
 
 def test_staged_device_path_matches(engine, oracle_geometry):
-    """The HBM-resident throughput path gives exactly the lst_run results."""
+    """The throughput path (prepared once into pinned memory, copied to the device at each run)
+    gives exactly the oracle's results, for a slot reused with another event."""
     import ctypes as C
     import tracklooper_amd as lst
     from tracklooper_amd import synth
-    from tests.parity import compare
-    sys_path_bench = __import__("bench")
+    from oracle import orc
     _, es = engine
-    ev = synth.generate_event("pu", 2, n_pu=100)
     e = lst.Event(es)
     L = lst.lib()
     L.lst_event_stage_input.argtypes = [C.c_void_p, C.POINTER(lst._Input), C.c_uint32]
     L.lst_event_run_staged.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
-    inp, keep = sys_path_bench._input_struct(lst, ev)
-    lst._check(L.lst_event_stage_input(e.h, C.byref(inp), 3))
-    lst._check(L.lst_event_run_staged(e.h, 3, lst.STAGES[DEEPEST]))
-    e.synchronize()
+    for ev in (synth.generate_event("muon", 3), synth.generate_event("pu", 2, n_pu=100)):
+        inp, keep = lst._make_input(ev)
+        lst._check(L.lst_event_stage_input(e.h, C.byref(inp), 3))
+        lst._check(L.lst_event_run_staged(e.h, 3, lst.STAGES[DEEPEST]))
+        e.synchronize()
+        ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
+        compare(e, ref, DEEPEST)
+
+
+def test_stage_by_stage_event_api(engine, oracle_geometry):
+    """SDL::Event driven method by method (addHitToEvent ... createTrackCandidates) on the prepared
+    inputs of a PU200 event gives the oracle's objects (SDL/Event.h:115-150)."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import synth
     from oracle import orc
+    _, es = engine
+    ev = synth.generate_event("pu", 9, n_pu=200)
     ref = orc.Event(oracle_geometry).run_lst(ev, DEEPEST)
-    compare(e, ref, DEEPEST)
+    f = ref.field
+    e = lst.Event(es)
+    e.addHitToEvent(f("in_x"), f("in_y"), f("in_z"), f("in_detId"), f("in_idx"))
+    e.addPixelSegmentToEvent(f("in_hitIdx0"), f("in_hitIdx1"), f("in_hitIdx2"), f("in_hitIdx3"), f("in_dPhiChange"),
+                             f("in_ptIn"), f("in_ptErr"), f("in_px"), f("in_py"), f("in_pz"), f("in_eta"),
+                             f("in_etaErr"), f("in_phi"), f("in_charge"), f("in_seedIdx"), f("in_superbin"),
+                             f("in_pixelType"), f("in_isQuad"))
+    with pytest.raises(lst.LSTError):  # out of order: triplets before segments
+        e.createTriplets()
+    e.createMiniDoublets()
+    e.createSegmentsWithModuleMap()
+    e.createTriplets()
+    e.createQuintuplets()
+    e.pixelLineSegmentCleaning()
+    e.createPixelQuintuplets()
+    e.createPixelTriplets()
+    e.createTrackCandidates()
+    e.synchronize()
+    rep = compare(e, ref, DEEPEST)
+    assert e.getNumberOfTrackCandidates() == ref.scalar("nTrackCandidates") > 1000
+    print(f"stage by stage: TC={e.getNumberOfTrackCandidates()}; floats compared {rep.float_compared}")
 
 
 def test_kernel_timing_does_not_disturb_runs(engine):
@@ -236,16 +267,19 @@ def pt06(geom_dir):
     return lst.LSTESData(d, pt_cut=0.6), orc.Geometry(d, stride=68)
 
 
-@pytest.mark.parametrize("case", [("muon", 0, 1), ("pu", 0, 0), ("pu", 1, 1)])
+@pytest.mark.parametrize("case", [("muon", 0, 1, 0), ("pu", 0, 0, 50), ("pu", 1, 1, 50), ("pu", 2, 0, 200),
+                                  ("pu", 2, 1, 200), ("pu", 3, 1, 250)])
 def test_pt06_geometry_all_stages(pt06, case):
+    """The pt0.6 configuration (denser combinatorics; at PU200-250 the per-module capacities and the
+    flattened builders' work arrays are under the most pressure), plain and DNN T5 selection."""
     import tracklooper_amd as lst
     from oracle import orc
     from tracklooper_amd import synth
     es, geom = pt06
-    kind, index, mode = case
+    kind, index, mode, n_pu = case
     g = "OT800_IT615_pt0.6"
     ev = synth.generate_event(kind, index, geom_name=g) if kind == "muon" else \
-        synth.generate_event(kind, index, n_pu=50, geom_name=g)
+        synth.generate_event(kind, index, n_pu=n_pu, geom_name=g)
     prod = lst.Event(es, t5_mode=mode).run_lst(ev, DEEPEST)
     prod.synchronize()
     ref = orc.Event(geom, pt_cut=0.6, t5_mode=mode).run_lst(ev, DEEPEST)

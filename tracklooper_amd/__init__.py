@@ -97,9 +97,20 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+# C types of the lst_input arrays (include/lst_amd.h)
+_INPUT_TYPES = {"see_q": np.int32, "see_hitIdx": np.int32, "see_hitIdx_off": np.uint32, "ph2_detId": np.uint32}
+
+
 def _make_input(ev: dict):
-    """lst_input over an event dict's see_* / ph2_* arrays (kept alive by the caller)."""
-    a = {k: np.ascontiguousarray(v) for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
+    """lst_input over an event dict's see_* / ph2_* arrays (kept alive by the caller); every array is
+    converted to the C type the struct names (float32 unless listed in _INPUT_TYPES)."""
+    a = {k: np.ascontiguousarray(v, dtype=_INPUT_TYPES.get(k, np.float32))
+         for k, v in ev.items() if k.startswith(("see_", "ph2_"))}
+    n_seeds = len(a["see_px"])
+    if len(a["see_hitIdx_off"]) != n_seeds + 1:
+        raise LSTError(f"see_hitIdx_off has {len(a['see_hitIdx_off'])} entries, {n_seeds + 1} expected")
+    if n_seeds and (a["see_hitIdx_off"][-1] > len(a["see_hitIdx"]) or np.any(np.diff(a["see_hitIdx_off"].astype(np.int64)) < 0)):
+        raise LSTError("see_hitIdx_off is not a non-decreasing list of offsets into see_hitIdx")
     inp = _Input(
         *[_p(a[k]) for k in ("see_px", "see_py", "see_pz", "see_dxy", "see_dz", "see_ptErr", "see_etaErr",
                              "see_stateTrajGlbX", "see_stateTrajGlbY", "see_stateTrajGlbZ",
@@ -274,15 +285,16 @@ class Event:
 
 class _Output(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("n_tc", C.c_uint32), ("hit_idxs", C.c_void_p), ("len", C.c_void_p),
-                ("seed_idx", C.c_void_p), ("type", C.c_void_p)]
+                ("seed_idx", C.c_void_p), ("type", C.c_void_p), ("status", C.c_int32)]
 
 
 class Batch:
-    """Batched engine: ``n_contexts`` event contexts on one GPU run a list of
-    events round robin (lst_batch_* in include/lst_amd.h).  Per event the
-    output lists equal ``Event.run_lst(ev, "tc").getOutput()``."""
+    """Batched engine (lst_batch_* in include/lst_amd.h): up to ``slots`` events
+    run in one launch set on one GPU, each kernel launched once for all of them,
+    inputs copied to the device inside the run.  Per event the output lists
+    equal ``Event.run_lst(ev, "tc").getOutput()``."""
 
-    def __init__(self, es: LSTESData, n_contexts: int = 4, t5_mode: int = LST_T5_PLAIN_CHI2):
+    def __init__(self, es: LSTESData, slots: int = 64, t5_mode: int = LST_T5_PLAIN_CHI2):
         L = lib()
         P = C.c_void_p
         L.lst_batch_create.argtypes = [P, C.c_uint32, C.c_int, C.POINTER(P)]
@@ -290,9 +302,16 @@ class Batch:
         L.lst_run_batch.argtypes = [P, P, C.c_uint32, P]
         L.lst_batch_stage.argtypes = [P, P, C.c_uint32]
         L.lst_batch_run_staged.argtypes = [P, C.c_uint32, C.c_uint32, P]
+        L.lst_batch_autotune.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.lst_batch_get_tuning.argtypes = [P, P, C.c_uint32]
+        L.lst_batch_set_tuning.argtypes = [P, P, C.c_uint32]
+        L.lst_tune_count.restype = C.c_uint32
+        L.lst_tune_name.argtypes = [C.c_uint32]
+        L.lst_tune_name.restype = C.c_char_p
         self.es = es
+        self.slots = slots
         self.h = C.c_void_p()
-        _check(L.lst_batch_create(es.h, n_contexts, t5_mode, C.byref(self.h)))
+        _check(L.lst_batch_create(es.h, slots, t5_mode, C.byref(self.h)))
 
     @staticmethod
     def _inputs(events):
@@ -319,7 +338,7 @@ class Batch:
         res = []
         for o, b in zip(outs, bufs):
             k = min(o.n_tc, o.capacity)
-            res.append({"n_tc": o.n_tc,
+            res.append({"n_tc": o.n_tc, "status": o.status,
                         "tc_hitIdxs": [list(b["hits"][i, :b["len"][i]]) for i in range(k)],
                         "tc_len": b["len"][:k].copy(), "tc_seedIdx": b["seed"][:k].copy(),
                         "tc_trackCandidateType": b["type"][:k].copy()})
@@ -333,19 +352,41 @@ class Batch:
         return self._collect(outs, bufs)
 
     def stage(self, events):
-        """Prepare events once and keep them in HBM (slots 0..len-1)."""
+        """Prepare events once and keep them in pinned host memory (staged events 0..len-1)."""
         arr, keep = self._inputs(events)
         _check(lib().lst_batch_stage(self.h, arr, len(events)))
 
     def run_staged(self, first: int, n: int, capacity: int | None = 8192):
-        """Run n events from staged slot `first` on (slots cycle modulo the staged
-        count); capacity None skips the output copies."""
+        """Run n staged events from `first` on (cycling modulo the staged count); capacity None
+        skips the output lists (the output headers still come back)."""
         if capacity is None:
             _check(lib().lst_batch_run_staged(self.h, first, n, None))
             return None
         outs, bufs = self._outputs(n, capacity)
         _check(lib().lst_batch_run_staged(self.h, first, n, outs))
         return self._collect(outs, bufs)
+
+    def autotune(self, first: int = 0, repeats: int = 2):
+        """Choose each grid-stride kernel's work-group count on one launch set of staged events;
+        returns (summed kernel ms at the defaults, at the chosen counts, {kernel: count})."""
+        L = lib()
+        d, t = C.c_float(), C.c_float()
+        _check(L.lst_batch_autotune(self.h, first, repeats, C.byref(d), C.byref(t)))
+        return d.value, t.value, self.tuning()
+
+    def tuning(self) -> dict:
+        L = lib()
+        n = L.lst_tune_count()
+        g = np.zeros(n, np.uint32)
+        _check(L.lst_batch_get_tuning(self.h, _p(g), n))
+        return {L.lst_tune_name(k).decode(): int(g[k]) for k in range(n)}
+
+    def set_tuning(self, table: dict):
+        L = lib()
+        n = L.lst_tune_count()
+        names = [L.lst_tune_name(k).decode() for k in range(n)]
+        g = np.array([int(table.get(nm, 0)) for nm in names], np.uint32)
+        _check(L.lst_batch_set_tuning(self.h, _p(g), n))
 
     def close(self):
         if self.h:

@@ -103,6 +103,70 @@ struct lst_es {
   float ptCut = 0.8f;
 };
 
+// An event prepared on the host (prepareInput, SDL/LST.cc:175-365) and kept
+// in one pinned block, so that its inputs cross to the device in one copy per
+// event: x, y, z, detId, idx per hit, then the pLS arrays.
+struct PreparedInput {
+  char* host = nullptr;  // pinned
+  size_t bytes = 0;
+  uint32_t nHits = 0, nPLS = 0;
+  bool truncated = false;  // the event had more pixel segments than kNMaxPixelSegments (SDL/Event.cc:237-244)
+  enum { X, Y, Z, DET, IDX, H0, H1, H2, H3, SEED, DPHI, PT, PTERR, PX, PY, PZ, ETA, ETAERR, PHI, CHARGE, SUPERBIN,
+         PTYPE, QUAD, NF };
+  size_t off[NF] = {};
+  void release() {
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    bytes = 0;
+  }
+  // the descriptor of this event's arrays at device address `base` (or at the host block itself)
+  lst::InputDesc desc(const char* base) const {
+    lst::InputDesc d{};
+    auto at = [&](int f) { return base + off[f]; };
+    d.x = (const float*)at(X); d.y = (const float*)at(Y); d.z = (const float*)at(Z);
+    d.det = (const uint32_t*)at(DET); d.idx = (const uint32_t*)at(IDX); d.nHits = nHits;
+    for (int k = 0; k < 4; k++) d.hitIdx[k] = (const uint32_t*)at(H0 + k);
+    d.dPhiChange = (const float*)at(DPHI); d.ptIn = (const float*)at(PT); d.ptErr = (const float*)at(PTERR);
+    d.px = (const float*)at(PX); d.py = (const float*)at(PY); d.pz = (const float*)at(PZ);
+    d.eta = (const float*)at(ETA); d.etaErr = (const float*)at(ETAERR); d.phi = (const float*)at(PHI);
+    d.charge = (const int32_t*)at(CHARGE); d.superbin = (const int32_t*)at(SUPERBIN);
+    d.seedIdx = (const uint32_t*)at(SEED); d.pixelType = (const int8_t*)at(PTYPE); d.isQuad = (const int8_t*)at(QUAD);
+    d.nPLS = nPLS;
+    d.outHost = nullptr;
+    d.outCap = 0;
+    return d;
+  }
+};
+
+// prepareInput on the host, then the prepared arrays into one pinned block
+static lst_status prepare_input(const lst_input& in, PreparedInput& p) {
+  lst::PreparedEvent e;
+  lst::prepare_event(in, e);
+  p.release();
+  const uint32_t nh = (uint32_t)e.x.size();
+  const uint32_t np = std::min<uint32_t>((uint32_t)e.ptIn.size(), lst::kNMaxPixelSegments);
+  p.truncated = e.ptIn.size() > lst::kNMaxPixelSegments;
+  p.nHits = nh;
+  p.nPLS = np;
+  const void* src[PreparedInput::NF] = {e.x.data(), e.y.data(), e.z.data(), e.detId.data(), e.idx.data(),
+                                        e.h0.data(), e.h1.data(), e.h2.data(), e.h3.data(), e.seedIdx.data(),
+                                        e.dPhiChange.data(), e.ptIn.data(), e.ptErr.data(), e.px.data(), e.py.data(),
+                                        e.pz.data(), e.eta.data(), e.etaErr.data(), e.phi.data(), e.charge.data(),
+                                        e.superbin.data(), e.pixelType.data(), e.isQuad.data()};
+  size_t len[PreparedInput::NF];
+  size_t off = 0;
+  for (int f = 0; f < PreparedInput::NF; f++) {
+    len[f] = f < PreparedInput::H0 ? (size_t)nh * 4 : f < PreparedInput::PTYPE ? (size_t)np * 4 : (size_t)np;
+    p.off[f] = off;
+    off += (len[f] + 255) & ~size_t(255);
+  }
+  p.bytes = off ? off : 256;
+  HIP_OK(hipHostMalloc((void**)&p.host, p.bytes, hipHostMallocDefault));
+  for (int f = 0; f < PreparedInput::NF; f++)
+    if (len[f]) std::memcpy(p.host + p.off[f], src[f], len[f]);
+  return LST_OK;
+}
+
 struct lst_event {
   lst_es* es = nullptr;
   hipStream_t stream = nullptr;
@@ -110,14 +174,13 @@ struct lst_event {
   int t5Mode = 0;
   int stage = -1;  // last completed stage
   lst::EventView v{};
+  lst::EventView* vDev = nullptr;  // device copy of v (kernels read their view from device memory)
   // capacities
   uint32_t hitCap = 0, mdCap = 0, segCap = 0, plsCap = lst::kNMaxPixelSegments, t3Cap = 0, t5Cap = 0;
   static constexpr uint32_t pt5Cap = lst::kNMaxPixelQuintuplets, pt3Cap = lst::kNMaxPixelTriplets,
                             tcCap = lst::kNMaxPixelTrackCandidates + lst::kNMaxNonPixelTrackCandidates;
   char* arena = nullptr;
   size_t arenaBytes = 0;
-  uint32_t* plsHitIdx[4] = {nullptr, nullptr, nullptr, nullptr};
-  float* plsDPhiChange = nullptr;
   std::vector<std::pair<std::string, FieldRef>> fields;
   lst::PreparedEvent prepared;
   // timing of the hot kernels (slots of lst::TimerId)
@@ -128,30 +191,19 @@ struct lst_event {
     timers.recorded = timing ? &recorded : nullptr;
     return timers;
   }
-  // captured launch sequence of the whole chain on described device inputs
-  // (batched engine): recaptured whenever the arena is reallocated
+  // the launch context of this handle's one event
+  lst::Launch launch() { return lst::Launch{v, vDev, 1, stream, tm()}; }
   void* dnnRecordMem = nullptr;        // T5 DNN training rows (lst_event_record_t5dnn)
-  lst::InputDesc* descDev = nullptr;   // device copy of the current event's descriptor
-  lst::InputDesc* descHost = nullptr;  // pinned staging of that descriptor
-  hipGraphExec_t graph = nullptr;
-  uint32_t arenaGen = 0, graphGen = 0;
-  // device-resident prepared inputs (throughput path)
-  struct Staged {
-    char* mem = nullptr;
-    size_t bytes = 0;
-    uint32_t nHits = 0, nPLS = 0;
-    bool truncated = false;  // the event had more pixel segments than kNMaxPixelSegments
-    float *x = nullptr, *y = nullptr, *z = nullptr;
-    uint32_t *det = nullptr, *idx = nullptr;
-    uint32_t *h[4] = {nullptr, nullptr, nullptr, nullptr}, *seed = nullptr;
-    float *dphi = nullptr, *pt = nullptr, *pterr = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr,
-          *eta = nullptr, *etaerr = nullptr, *phi = nullptr;
-    int32_t *charge = nullptr, *superbin = nullptr;
-    int8_t *ptype = nullptr, *quad = nullptr;
-  };
-  std::vector<Staged> staged;
+  // prepared inputs kept in pinned host memory (throughput path): copied to the
+  // device, described by descDev, at each run
+  std::vector<PreparedInput> staged;
+  char* inDev = nullptr;               // device copy of the running event's prepared block
+  size_t inDevBytes = 0;
+  lst::InputDesc* descDev = nullptr;   // device descriptor of that block
+  lst::InputDesc* descHost = nullptr;  // pinned staging of the descriptor
 
   lst_status allocate(uint32_t nHitsNeeded);
+  lst_status upload_view();
   lst_status add_hits(const float* x, const float* y, const float* z, const uint32_t* det, const uint32_t* idx,
                       uint32_t n, bool onDevice);
   lst_status add_pixel_segments(const lst_pixel_segments& p, bool onDevice);
@@ -253,12 +305,12 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.R.nQuintuplets, nL); c.take(e.R.totOccupancyQuintuplets, nL);
   c.take(e.R.indicesOfEligibleT5Modules, nL);
   c.take(e.R.scalars, lst::kNScalars);
-  for (int k = 0; k < 4; k++) c.take(plsHitIdx[k], plsCap);
-  c.take(plsDPhiChange, plsCap);
+  for (int k = 0; k < 4; k++) c.take(e.plsInHitIdx[k], plsCap);
+  c.take(e.plsInDPhiChange, plsCap);
   arenaBytes = c.off;
   HIP_OK(hipMalloc(&arena, arenaBytes));
   c.bind(arena);
-  arenaGen++;
+
   e.M = g.view;
   e.mdModuleIndices = g.mdIndices;
   e.mdModuleOccupancy = g.mdOccupancy;
@@ -287,6 +339,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   e.t5Dnn = t5Mode == LST_T5_DNN ? 1 : 0;
   e.pixelModule = g.pixelModuleIndex;
   e.ptCut = es->ptCut;
+  e.outCap = tcCap;
 
   // by-name field table (host getters)
   fields.clear();
@@ -381,6 +434,14 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   add("scalars", e.R.scalars, lst::kNScalars);
   add("out.hitIdxs", e.W.outHits, (size_t)tcCap * 14);
   add("out.len", e.W.outLen, tcCap);
+  return upload_view();
+}
+
+// the device copy of the view, which every kernel of this handle reads
+lst_status lst_event::upload_view() {
+  if (!vDev) HIP_OK(hipMalloc(&vDev, sizeof(lst::EventView)));
+  HIP_OK(hipStreamSynchronize(stream));
+  HIP_OK(hipMemcpy(vDev, &v, sizeof(lst::EventView), hipMemcpyHostToDevice));
   return LST_OK;
 }
 
@@ -407,7 +468,7 @@ lst_status lst_event::add_hits(const float* x, const float* y, const float* z, c
   v.nPLS = 0;
   lst::launch_set_scalar(v.R.scalars + lst::kEvNHits, n, stream);
   lst::launch_set_scalar(v.R.scalars + lst::kEvNPLS, 0, stream);
-  lst::launch_hits(v, stream, tm());
+  lst::launch_hits(launch());
   HIP_OK(hipGetLastError());
   stage = 0;
   return LST_OK;
@@ -429,11 +490,11 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
   }
   if (n) {
     CopyBatch cb(onDevice, stream);
-    cb.add(plsHitIdx[0], p.hit_indices0, n * 4);
-    cb.add(plsHitIdx[1], p.hit_indices1, n * 4);
-    cb.add(plsHitIdx[2], p.hit_indices2, n * 4);
-    cb.add(plsHitIdx[3], p.hit_indices3, n * 4);
-    cb.add(plsDPhiChange, p.delta_phi_change, n * 4);
+    cb.add(v.plsInHitIdx[0], p.hit_indices0, n * 4);
+    cb.add(v.plsInHitIdx[1], p.hit_indices1, n * 4);
+    cb.add(v.plsInHitIdx[2], p.hit_indices2, n * 4);
+    cb.add(v.plsInHitIdx[3], p.hit_indices3, n * 4);
+    cb.add(v.plsInDPhiChange, p.delta_phi_change, n * 4);
     cb.add(v.P.ptIn, p.pt_in, n * 4);
     cb.add(v.P.ptErr, p.pt_err, n * 4);
     cb.add(v.P.px, p.px, n * 4);
@@ -451,61 +512,38 @@ lst_status lst_event::add_pixel_segments(const lst_pixel_segments& p, bool onDev
   }
   v.nPLS = n;
   lst::launch_set_scalar(v.R.scalars + lst::kEvNPLS, n, stream);
-  const uint32_t* hi[4] = {plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]};
-  lst::launch_pixel_segments(v, hi, plsDPhiChange, stream, tm());
+  lst::launch_pixel_segments(launch());
   HIP_OK(hipGetLastError());
   stage = 1;
   return warn;
 }
 
-// prepareInput on the host, then the prepared arrays into one device allocation
-static lst_status stage_input(const lst_input& in, lst_event::Staged& st) {
-  lst::PreparedEvent p;
-  lst::prepare_event(in, p);
-  if (st.mem) {
-    HIP_OK(hipFree(st.mem));
-    st = lst_event::Staged();
+// Host-side checks of the hits the reference's hit kernels assume (SDL/Hit.h:194-270): every detId is a
+// module of the geometry, and each module's hits are contiguous and at most 127 (its int8 counts).
+static lst_status check_hits(const lst::HostGeometry& g, const uint32_t* det, uint32_t n) {
+  uint32_t prev = ~0u, run = 0;
+  std::vector<uint8_t> seen;
+  for (uint32_t i = 0; i < n; i++) {
+    if (det[i] == prev) {
+      if (++run > 127 && det[i] != 1u)
+        return fail(LST_E_INPUT, "more than 127 hits on module " + std::to_string(det[i]));
+      continue;
+    }
+    const int m = g.module_of(det[i]);
+    if (m < 0) return fail(LST_E_INPUT, "hit " + std::to_string(i) + " has detId " + std::to_string(det[i]) +
+                                            ", which is no module of the geometry");
+    if (seen.empty()) seen.assign(g.nModules, 0);
+    if (seen[m]) return fail(LST_E_INPUT, "the hits of module " + std::to_string(det[i]) + " are not contiguous");
+    seen[m] = 1;
+    prev = det[i];
+    run = 1;
   }
-  const uint32_t nh = p.x.size();
-  const uint32_t np = std::min<uint32_t>(p.ptIn.size(), lst::kNMaxPixelSegments);
-  st.truncated = p.ptIn.size() > lst::kNMaxPixelSegments;  // SDL/Event.cc:237-244
-  Carve c;
-  c.take(st.x, nh); c.take(st.y, nh); c.take(st.z, nh); c.take(st.det, nh); c.take(st.idx, nh);
-  for (int k = 0; k < 4; k++) c.take(st.h[k], np);
-  c.take(st.seed, np); c.take(st.dphi, np); c.take(st.pt, np); c.take(st.pterr, np); c.take(st.px, np);
-  c.take(st.py, np); c.take(st.pz, np); c.take(st.eta, np); c.take(st.etaerr, np); c.take(st.phi, np);
-  c.take(st.charge, np); c.take(st.superbin, np); c.take(st.ptype, np); c.take(st.quad, np);
-  HIP_OK(hipMalloc(&st.mem, c.off ? c.off : 256));
-  st.bytes = c.off;
-  c.bind(st.mem);
-  st.nHits = nh;
-  st.nPLS = np;
-  auto up = [&](void* d, const void* h, size_t b) { return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess; };
-  HIP_OK(up(st.x, p.x.data(), nh * 4)); HIP_OK(up(st.y, p.y.data(), nh * 4)); HIP_OK(up(st.z, p.z.data(), nh * 4));
-  HIP_OK(up(st.det, p.detId.data(), nh * 4)); HIP_OK(up(st.idx, p.idx.data(), nh * 4));
-  HIP_OK(up(st.h[0], p.h0.data(), np * 4)); HIP_OK(up(st.h[1], p.h1.data(), np * 4));
-  HIP_OK(up(st.h[2], p.h2.data(), np * 4)); HIP_OK(up(st.h[3], p.h3.data(), np * 4));
-  HIP_OK(up(st.seed, p.seedIdx.data(), np * 4)); HIP_OK(up(st.dphi, p.dPhiChange.data(), np * 4));
-  HIP_OK(up(st.pt, p.ptIn.data(), np * 4)); HIP_OK(up(st.pterr, p.ptErr.data(), np * 4));
-  HIP_OK(up(st.px, p.px.data(), np * 4)); HIP_OK(up(st.py, p.py.data(), np * 4)); HIP_OK(up(st.pz, p.pz.data(), np * 4));
-  HIP_OK(up(st.eta, p.eta.data(), np * 4)); HIP_OK(up(st.etaerr, p.etaErr.data(), np * 4));
-  HIP_OK(up(st.phi, p.phi.data(), np * 4)); HIP_OK(up(st.charge, p.charge.data(), np * 4));
-  HIP_OK(up(st.superbin, p.superbin.data(), np * 4)); HIP_OK(up(st.ptype, p.pixelType.data(), np));
-  HIP_OK(up(st.quad, p.isQuad.data(), np));
   return LST_OK;
 }
 
-// the chain on inputs already in device memory, enqueued on the event's stream
-static lst_status run_staged(lst_event* ev, const lst_event::Staged& st, int last_stage) {
-  lst_status s = ev->add_hits(st.x, st.y, st.z, st.det, st.idx, st.nHits, true);
-  if (s != LST_OK || last_stage < 1) return s;
-  lst_pixel_segments p;
-  p.hit_indices0 = st.h[0]; p.hit_indices1 = st.h[1]; p.hit_indices2 = st.h[2]; p.hit_indices3 = st.h[3];
-  p.delta_phi_change = st.dphi; p.pt_in = st.pt; p.pt_err = st.pterr; p.px = st.px; p.py = st.py; p.pz = st.pz;
-  p.eta = st.eta; p.eta_err = st.etaerr; p.phi = st.phi; p.charge = st.charge; p.seed_idx = st.seed;
-  p.superbin = st.superbin; p.pixel_type = st.ptype; p.is_quad = st.quad; p.n_pls = st.nPLS;
-  const lst_status warn = st.truncated ? (lst_status)LST_W_TRUNCATED : LST_OK;
-  if ((s = ev->add_pixel_segments(p, true)) != LST_OK || last_stage < 2) return s == LST_OK ? warn : s;
+// the stage sequence of one event, launched stage by stage as the SDL::Event methods would
+static lst_status run_stages(lst_event* ev, int last_stage, lst_status warn) {
+  lst_status s = LST_OK;
   if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
   if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
   if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s == LST_OK ? warn : s;
@@ -517,57 +555,54 @@ static lst_status run_staged(lst_event* ev, const lst_event::Staged& st, int las
   return s == LST_OK ? warn : s;
 }
 
-// the whole chain on the inputs ev->descDev describes: kernel launches only,
-// no host reads, so that it can be captured once and replayed per event
-static void enqueue_described_chain(lst_event* ev) {
-  const lst::EventView& v = ev->v;
-  const hipStream_t s = ev->stream;
-  lst::launch_stage_in(v, ev->descDev, ev->plsHitIdx, ev->plsDPhiChange, s);
-  lst::launch_hits(v, s);
-  const uint32_t* hi[4] = {ev->plsHitIdx[0], ev->plsHitIdx[1], ev->plsHitIdx[2], ev->plsHitIdx[3]};
-  lst::launch_pixel_segments(v, hi, ev->plsDPhiChange, s);
-  lst::launch_mini_doublets(v, s);
-  lst::launch_segments(v, s);
-  lst::launch_triplets(v, s);
-  lst::launch_quintuplets(v, s);
-  lst::launch_pixel_segment_cleaning(v, s);
-  lst::launch_pixel_quintuplets(v, s);
-  lst::launch_pixel_triplets(v, s);
-  lst::launch_track_candidates(v, s);
-}
-
-// run a staged event through the context's captured chain (captured on first use)
-static lst_status run_staged_graph(lst_event* ev, const lst_event::Staged& st) {
-  lst_status s = ev->allocate(st.nHits);
+// a prepared event through the handle: its block copied to the device in one
+// transfer, spread into the arena by one kernel, then the stages
+static lst_status run_prepared(lst_event* ev, const PreparedInput& p, int last_stage) {
+  lst_status s = ev->allocate(p.nHits);
   if (s != LST_OK) return s;
   if (!ev->descDev) {
     HIP_OK(hipMalloc(&ev->descDev, sizeof(lst::InputDesc)));
     HIP_OK(hipHostMalloc((void**)&ev->descHost, sizeof(lst::InputDesc), hipHostMallocDefault));
   }
-  if (!ev->graph || ev->graphGen != ev->arenaGen) {
-    if (ev->graph) HIP_OK(hipGraphExecDestroy(ev->graph));
-    ev->graph = nullptr;
-    hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(ev->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_described_chain(ev);
-    HIP_OK(hipStreamEndCapture(ev->stream, &g));
-    const hipError_t e = hipGraphInstantiate(&ev->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(LST_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    ev->graphGen = ev->arenaGen;
+  if (ev->inDevBytes < p.bytes) {
+    HIP_OK(hipStreamSynchronize(ev->stream));
+    if (ev->inDev) HIP_OK(hipFree(ev->inDev));
+    ev->inDev = nullptr;
+    HIP_OK(hipMalloc(&ev->inDev, p.bytes));
+    ev->inDevBytes = p.bytes;
   }
-  lst::InputDesc& d = *ev->descHost;  // the stream is idle here (the batch waited for this context)
-  d.x = st.x; d.y = st.y; d.z = st.z; d.det = st.det; d.idx = st.idx; d.nHits = st.nHits;
-  for (int k = 0; k < 4; k++) d.hitIdx[k] = st.h[k];
-  d.dPhiChange = st.dphi; d.ptIn = st.pt; d.ptErr = st.pterr; d.px = st.px; d.py = st.py; d.pz = st.pz;
-  d.eta = st.eta; d.etaErr = st.etaerr; d.phi = st.phi; d.charge = st.charge; d.superbin = st.superbin;
-  d.seedIdx = st.seed; d.pixelType = st.ptype; d.isQuad = st.quad; d.nPLS = st.nPLS;
+  HIP_OK(hipStreamSynchronize(ev->stream));  // the pinned descriptor of the previous run has been read
+  *ev->descHost = p.desc(ev->inDev);
+  HIP_OK(hipMemcpyAsync(ev->inDev, p.host, p.bytes, hipMemcpyHostToDevice, ev->stream));
   HIP_OK(hipMemcpyAsync(ev->descDev, ev->descHost, sizeof(lst::InputDesc), hipMemcpyHostToDevice, ev->stream));
-  HIP_OK(hipGraphLaunch(ev->graph, ev->stream));
-  ev->v.nHits = st.nHits;
-  ev->v.nPLS = st.nPLS;
-  ev->stage = 9;
-  return st.truncated ? (lst_status)LST_W_TRUNCATED : LST_OK;
+  const lst::Launch L = ev->launch();
+  lst::launch_stage_in(L, ev->descDev);
+  lst::launch_hits(L);
+  ev->v.nHits = p.nHits;
+  ev->v.nPLS = 0;
+  ev->stage = 0;
+  HIP_OK(hipGetLastError());
+  if (last_stage < 1) return LST_OK;
+  lst::launch_pixel_segments(L);
+  ev->v.nPLS = p.nPLS;
+  ev->stage = 1;
+  HIP_OK(hipGetLastError());
+  const lst_status warn = p.truncated ? (lst_status)LST_W_TRUNCATED : LST_OK;
+  if (last_stage < 2) return warn;
+  return run_stages(ev, last_stage, warn);
+}
+
+// the device-side flags the hit kernels and the last kernel leave in the
+// event's scalars, as a status: bad input is an error, truncation a warning
+static lst_status device_status(lst_event* ev) {
+  uint32_t scal[lst::kNScalars];
+  HIP_OK(hipMemcpyAsync(scal, ev->v.R.scalars, sizeof(scal), hipMemcpyDeviceToHost, ev->stream));
+  HIP_OK(hipStreamSynchronize(ev->stream));
+  const uint32_t bad = scal[lst::kBadHits];
+  if (bad & lst::kBadDetId) return fail(LST_E_INPUT, "a hit's detId is no module of the geometry");
+  if (bad & lst::kBadModuleHits) return fail(LST_E_INPUT, "more than 127 hits on one module");
+  if (bad & lst::kBadNonContiguous) return fail(LST_E_INPUT, "the hits of a module are not contiguous");
+  return scal[lst::kTruncated] ? (lst_status)LST_W_TRUNCATED : LST_OK;
 }
 
 extern "C" {
@@ -630,19 +665,19 @@ lst_status lst_event_create(lst_es* es, void* hip_stream, int t5_mode, lst_event
 
 void lst_event_destroy(lst_event* ev) {
   if (!ev) return;
-  if (ev->stream) hipStreamSynchronize(ev->stream);
-  if (ev->arena) hipFree(ev->arena);
-  for (auto& st : ev->staged)
-    if (st.mem) hipFree(st.mem);
+  if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+  if (ev->arena) (void)hipFree(ev->arena);
+  if (ev->vDev) (void)hipFree(ev->vDev);
+  if (ev->inDev) (void)hipFree(ev->inDev);
+  for (auto& st : ev->staged) st.release();
   for (int k = 0; k < lst::kNTimers; k++) {
-    if (ev->timers.start[k]) hipEventDestroy(ev->timers.start[k]);
-    if (ev->timers.stop[k]) hipEventDestroy(ev->timers.stop[k]);
+    if (ev->timers.start[k]) (void)hipEventDestroy(ev->timers.start[k]);
+    if (ev->timers.stop[k]) (void)hipEventDestroy(ev->timers.stop[k]);
   }
   if (ev->dnnRecordMem) (void)hipFree(ev->dnnRecordMem);
-  if (ev->graph) (void)hipGraphExecDestroy(ev->graph);
   if (ev->descDev) (void)hipFree(ev->descDev);
   if (ev->descHost) (void)hipHostFree(ev->descHost);
-  if (ev->ownStream) hipStreamDestroy(ev->stream);
+  if (ev->ownStream) (void)hipStreamDestroy(ev->stream);
   delete ev;
 }
 
@@ -659,7 +694,7 @@ lst_status lst_event_add_pixel_segments(lst_event* ev, const lst_pixel_segments*
 
 lst_status lst_event_create_mini_doublets(lst_event* ev) {
   if (!ev || ev->stage != 1) return fail(LST_E_ORDER, "createMiniDoublets needs hits and pixel segments");
-  lst::launch_mini_doublets(ev->v, ev->stream, ev->tm());
+  lst::launch_mini_doublets(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 2;
   return LST_OK;
@@ -667,7 +702,7 @@ lst_status lst_event_create_mini_doublets(lst_event* ev) {
 
 lst_status lst_event_create_segments(lst_event* ev) {
   if (!ev || ev->stage != 2) return fail(LST_E_ORDER, "createSegmentsWithModuleMap needs mini-doublets");
-  lst::launch_segments(ev->v, ev->stream, ev->tm());
+  lst::launch_segments(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 3;
   return LST_OK;
@@ -675,7 +710,7 @@ lst_status lst_event_create_segments(lst_event* ev) {
 
 lst_status lst_event_create_triplets(lst_event* ev) {
   if (!ev || ev->stage != 3) return fail(LST_E_ORDER, "createTriplets needs segments");
-  lst::launch_triplets(ev->v, ev->stream, ev->tm());
+  lst::launch_triplets(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 4;
   return LST_OK;
@@ -683,7 +718,7 @@ lst_status lst_event_create_triplets(lst_event* ev) {
 
 lst_status lst_event_create_quintuplets(lst_event* ev) {
   if (!ev || ev->stage != 4) return fail(LST_E_ORDER, "createQuintuplets needs triplets");
-  lst::launch_quintuplets(ev->v, ev->stream, ev->tm());
+  lst::launch_quintuplets(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 5;
   return LST_OK;
@@ -691,7 +726,7 @@ lst_status lst_event_create_quintuplets(lst_event* ev) {
 
 lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev) {
   if (!ev || ev->stage != 5) return fail(LST_E_ORDER, "pixelLineSegmentCleaning needs quintuplets");
-  lst::launch_pixel_segment_cleaning(ev->v, ev->stream, ev->tm());
+  lst::launch_pixel_segment_cleaning(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 6;
   return LST_OK;
@@ -699,7 +734,7 @@ lst_status lst_event_pixel_line_segment_cleaning(lst_event* ev) {
 
 lst_status lst_event_create_pixel_quintuplets(lst_event* ev) {
   if (!ev || ev->stage != 6) return fail(LST_E_ORDER, "createPixelQuintuplets needs pixelLineSegmentCleaning");
-  lst::launch_pixel_quintuplets(ev->v, ev->stream, ev->tm());
+  lst::launch_pixel_quintuplets(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 7;
   return LST_OK;
@@ -707,7 +742,7 @@ lst_status lst_event_create_pixel_quintuplets(lst_event* ev) {
 
 lst_status lst_event_create_pixel_triplets(lst_event* ev) {
   if (!ev || ev->stage != 7) return fail(LST_E_ORDER, "createPixelTriplets needs pixel quintuplets");
-  lst::launch_pixel_triplets(ev->v, ev->stream, ev->tm());
+  lst::launch_pixel_triplets(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 8;
   return LST_OK;
@@ -715,7 +750,7 @@ lst_status lst_event_create_pixel_triplets(lst_event* ev) {
 
 lst_status lst_event_create_track_candidates(lst_event* ev) {
   if (!ev || ev->stage != 8) return fail(LST_E_ORDER, "createTrackCandidates needs pixel triplets");
-  lst::launch_track_candidates(ev->v, ev->stream, ev->tm());
+  lst::launch_track_candidates(ev->launch());
   HIP_OK(hipGetLastError());
   ev->stage = 9;
   return LST_OK;
@@ -725,12 +760,14 @@ lst_status lst_event_get_output(lst_event* ev, uint32_t* n_tc, uint32_t* hit_idx
                                 int16_t* type, uint32_t capacity) {
   if (!ev || !n_tc) return fail(LST_E_INPUT, "null argument");
   if (ev->stage != 9) return fail(LST_E_ORDER, "getOutput needs track candidates");
+  const lst_status st = device_status(ev);
+  if ((st & 0xff) != LST_OK) return st;
   uint32_t scal[lst::kNScalars];
   HIP_OK(hipMemcpyAsync(scal, ev->v.R.scalars, sizeof(scal), hipMemcpyDeviceToHost, ev->stream));
   HIP_OK(hipStreamSynchronize(ev->stream));
   const uint32_t n = scal[lst::kNTrackCandidates];
   *n_tc = n;
-  if (!hit_idxs && !len && !seed_idx && !type) return LST_OK;
+  if (!hit_idxs && !len && !seed_idx && !type) return st;
   if (capacity < n) return fail(LST_E_CAPACITY, "output arrays hold fewer track candidates than were built");
   if (n) {
     if (hit_idxs) HIP_OK(hipMemcpyAsync(hit_idxs, ev->v.W.outHits, (size_t)n * 14 * 4, hipMemcpyDeviceToHost, ev->stream));
@@ -739,7 +776,7 @@ lst_status lst_event_get_output(lst_event* ev, uint32_t* n_tc, uint32_t* hit_idx
     if (type) HIP_OK(hipMemcpyAsync(type, ev->v.C.trackCandidateType, (size_t)n * 2, hipMemcpyDeviceToHost, ev->stream));
     HIP_OK(hipStreamSynchronize(ev->stream));
   }
-  return LST_OK;
+  return st;
 }
 
 lst_status lst_event_reset(lst_event* ev) {
@@ -753,7 +790,8 @@ lst_status lst_event_reset(lst_event* ev) {
 lst_status lst_event_synchronize(lst_event* ev) {
   if (!ev) return fail(LST_E_INPUT, "null event");
   HIP_OK(hipStreamSynchronize(ev->stream));
-  return LST_OK;
+  if (ev->stage < 0) return LST_OK;
+  return device_status(ev);
 }
 
 uint32_t lst_event_count(lst_event* ev, const char* what) {
@@ -812,38 +850,35 @@ lst_status lst_event_get_field(lst_event* ev, const char* name, void* dst, size_
 
 lst_status lst_run(lst_event* ev, const lst_input* in, int last_stage) {
   if (!ev || !in) return fail(LST_E_INPUT, "null argument");
+  if (in->n_hits && (!in->ph2_detId || !in->ph2_x || !in->ph2_y || !in->ph2_z))
+    return fail(LST_E_INPUT, "null hit arrays");
   lst::prepare_event(*in, ev->prepared);
   const auto& p = ev->prepared;
-  lst_status s = ev->add_hits(p.x.data(), p.y.data(), p.z.data(), p.detId.data(), p.idx.data(),
-                              (uint32_t)p.x.size(), false);
+  lst_status s = check_hits(ev->es->host, p.detId.data(), (uint32_t)p.x.size());
+  if (s != LST_OK) return s;
+  s = ev->add_hits(p.x.data(), p.y.data(), p.z.data(), p.detId.data(), p.idx.data(), (uint32_t)p.x.size(), false);
   if (s != LST_OK || last_stage < 1) return s;
   const lst_pixel_segments pv = p.view();
   lst_status warn = ev->add_pixel_segments(pv, false);
   if ((warn & 0xff) != LST_OK || last_stage < 2) return warn;
-  if ((s = lst_event_create_mini_doublets(ev)) != LST_OK || last_stage < 3) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_segments(ev)) != LST_OK || last_stage < 4) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_triplets(ev)) != LST_OK || last_stage < 5) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_quintuplets(ev)) != LST_OK || last_stage < 6) return s == LST_OK ? warn : s;
-  if ((s = lst_event_pixel_line_segment_cleaning(ev)) != LST_OK || last_stage < 7) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_pixel_quintuplets(ev)) != LST_OK || last_stage < 8) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_pixel_triplets(ev)) != LST_OK || last_stage < 9) return s == LST_OK ? warn : s;
-  if ((s = lst_event_create_track_candidates(ev)) != LST_OK) return s;
-  return warn;
+  return run_stages(ev, last_stage, warn);
 }
 
 lst_status lst_event_stage_input(lst_event* ev, const lst_input* in, uint32_t slot) {
   if (!ev || !in) return fail(LST_E_INPUT, "null argument");
   if (ev->staged.size() <= slot) ev->staged.resize(slot + 1);
-  lst_event::Staged& st = ev->staged[slot];
-  if (st.mem) HIP_OK(hipStreamSynchronize(ev->stream));
-  lst_status s = stage_input(*in, st);
+  PreparedInput& st = ev->staged[slot];
+  HIP_OK(hipStreamSynchronize(ev->stream));  // a run may still be copying this slot's block
+  lst_status s = prepare_input(*in, st);
   if (s != LST_OK) return s;
+  if ((s = check_hits(ev->es->host, (const uint32_t*)(st.host + st.off[PreparedInput::DET]), st.nHits)) != LST_OK)
+    return s;
   return ev->allocate(st.nHits);
 }
 
 lst_status lst_event_run_staged(lst_event* ev, uint32_t slot, int last_stage) {
-  if (!ev || slot >= ev->staged.size() || !ev->staged[slot].mem) return fail(LST_E_INPUT, "input slot not staged");
-  return run_staged(ev, ev->staged[slot], last_stage);
+  if (!ev || slot >= ev->staged.size() || !ev->staged[slot].host) return fail(LST_E_INPUT, "input slot not staged");
+  return run_prepared(ev, ev->staged[slot], last_stage);
 }
 
 lst_status lst_event_set_timing(lst_event* ev, int on) {
@@ -880,7 +915,7 @@ lst_status lst_event_record_t5dnn(lst_event* ev, uint32_t capacity) {
   if (ev->dnnRecordMem) HIP_OK(hipFree(ev->dnnRecordMem));
   ev->dnnRecordMem = nullptr;
   ev->v.dnnDump = lst::T5DnnDump{};
-  if (capacity == 0) return LST_OK;
+  if (capacity == 0) return ev->upload_view();
   if (capacity > lst::kT5ResultMax) capacity = lst::kT5ResultMax;
   Carve c;
   lst::T5DnnDump& d = ev->v.dnnDump;
@@ -889,7 +924,7 @@ lst_status lst_event_record_t5dnn(lst_event* ev, uint32_t capacity) {
   HIP_OK(hipMalloc(&ev->dnnRecordMem, c.off));
   c.bind(static_cast<char*>(ev->dnnRecordMem));
   d.capacity = capacity;
-  return LST_OK;
+  return ev->upload_view();
 }
 
 lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* features, float* scores, uint32_t* t3_pairs,
@@ -920,112 +955,319 @@ lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* feature
 }  // extern "C"
 
 // ---------------------------------------------------------------- batches
-// Several event contexts (arena + stream each) run a batch of events round
-// robin; the host waits on a context only when it hands it the next event, so
-// the other contexts keep the GPU busy meanwhile.  Outputs travel through a
-// pinned buffer per context.
+// A batch runs up to `slots` events in one launch set: every kernel of the
+// stage sequence is launched once for all of them (gridDim.z = slots), each
+// event in its own arena, and the sequence is one captured graph.  Inputs are
+// prepared on the host once and kept pinned; each launch set copies its
+// events' blocks to the device on a copy stream while the previous set
+// computes (two input buffers), and its last kernel copies each event's packed
+// output to pinned host memory.  An event whose candidates overflow a
+// flattened builder's work arrays (a flag in its output header) is run again
+// by a per-event handle, which has the per-module builders.
 struct lst_batch {
   lst_es* es = nullptr;
-  std::vector<lst_event*> ctx;
-  struct Out {
-    char* pinned = nullptr;  // n_tc, then hits [cap*14], len [cap], seed [cap], type [cap]
-    uint32_t cap = 0;
-    int64_t pending = -1;    // batch position whose output is in flight
-    lst_output* dst = nullptr;
-  };
-  std::vector<Out> outs;
-  std::vector<lst_event::Staged> staged;
-  bool graphs = true;  // staged events run through each context's captured chain
+  int t5Mode = 0;
+  uint32_t slots = 0;
+  std::vector<lst_event*> ctx;          // one arena per slot
+  lst_event* rerun = nullptr;           // per-event handle for events that overflow the flattened builders
+  hipStream_t stream = nullptr, copyStream = nullptr;
+  lst::EventView* viewsDev = nullptr;   // [slots]
+  char* inDev[2] = {nullptr, nullptr};  // device input blocks of a launch set, slots x slotBytes each
+  size_t slotBytes = 0;
+  lst::InputDesc* descDev[2] = {nullptr, nullptr};   // [slots]
+  lst::InputDesc* descHost[2] = {nullptr, nullptr};  // pinned [slots]
+  uint32_t* outHost[2] = {nullptr, nullptr};         // pinned, slots x out_words()
+  uint32_t outCapAlloc = 0;             // records per event outHost holds
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool graphs = true;
+  hipEvent_t copyDone[2] = {}, computeDone[2] = {};
+  std::vector<PreparedInput> staged;
+  uint32_t grid[lst::kNTune] = {};      // chosen work-group counts (0: the defaults)
+  size_t out_words(uint32_t cap) const {
+    return ((lst::kOutHeaderWords + (size_t)cap * lst::kOutRecWords) + 63) & ~size_t(63);
+  }
 };
 
 namespace {
 constexpr uint32_t kTCCap = lst::kNMaxPixelTrackCandidates + lst::kNMaxNonPixelTrackCandidates;
 
-size_t out_bytes(uint32_t cap) { return 4 * (lst::kOutHeaderWords + (size_t)cap * lst::kOutRecWords); }
+void drop_graphs(lst_batch* b) {
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  b->graph[0] = b->graph[1] = nullptr;
+}
 
-// enqueue the output copies of context c's event into its pinned buffer
-lst_status enqueue_output(lst_batch* b, uint32_t c, lst_output* dst) {
-  lst_batch::Out& o = b->outs[c];
-  lst_event* ev = b->ctx[c];
-  const uint32_t cap = std::min(dst->capacity, kTCCap);
-  if (!o.pinned || o.cap < cap) {
-    if (o.pinned) HIP_OK(hipHostFree(o.pinned));
-    o.pinned = nullptr;
-    HIP_OK(hipHostMalloc((void**)&o.pinned, out_bytes(cap), hipHostMallocDefault));
-    o.cap = cap;
+// every slot's arena sized for nHits, and the device view array
+lst_status batch_arenas(lst_batch* b, uint32_t nHits) {
+  bool changed = !b->viewsDev;
+  for (lst_event* ev : b->ctx) {
+    const char* before = ev->arena;
+    lst_status s = ev->allocate(nHits);
+    if (s != LST_OK) return s;
+    changed |= ev->arena != before;
   }
-  // the packed block k_tc_output wrote: count, then one record per track candidate
-  HIP_OK(hipMemcpyAsync(o.pinned, ev->v.W.outPacked, out_bytes(cap), hipMemcpyDeviceToHost, ev->stream));
-  o.dst = dst;
+  if (!changed) return LST_OK;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  if (!b->viewsDev) HIP_OK(hipMalloc(&b->viewsDev, sizeof(lst::EventView) * b->slots));
+  for (uint32_t i = 0; i < b->slots; i++)
+    HIP_OK(hipMemcpy(b->viewsDev + i, &b->ctx[i]->v, sizeof(lst::EventView), hipMemcpyHostToDevice));
+  drop_graphs(b);
   return LST_OK;
 }
 
-// wait for context c and hand its finished event's output to the caller
-lst_status finish_context(lst_batch* b, uint32_t c, lst_status& warn) {
-  lst_batch::Out& o = b->outs[c];
-  lst_event* ev = b->ctx[c];
-  HIP_OK(hipStreamSynchronize(ev->stream));
-  if (o.pending >= 0 && o.dst) {
-    lst_output* d = o.dst;
-    const uint32_t cap = std::min(d->capacity, kTCCap);
-    uint32_t n;
-    std::memcpy(&n, o.pinned, 4);
-    d->n_tc = n;
-    const uint32_t m = std::min(n, cap);
-    if (n > d->capacity) warn = LST_W_TRUNCATED;
-    const uint32_t* rec = reinterpret_cast<const uint32_t*>(o.pinned) + lst::kOutHeaderWords;
-    for (uint32_t t = 0; t < m; t++, rec += lst::kOutRecWords) {
-      if (d->hit_idxs) std::memcpy(d->hit_idxs + 14 * (size_t)t, rec, 56);
-      if (d->len) d->len[t] = rec[14];
-      if (d->seed_idx) d->seed_idx[t] = (int32_t)rec[15];
-      if (d->type) d->type[t] = (int16_t)(int32_t)rec[16];
+// input and output buffers for events of up to `bytes` prepared bytes and `cap` output records
+lst_status batch_io(lst_batch* b, size_t bytes, uint32_t cap) {
+  HIP_OK(hipStreamSynchronize(b->stream));
+  HIP_OK(hipStreamSynchronize(b->copyStream));
+  bytes = (bytes + 255) & ~size_t(255);
+  if (bytes > b->slotBytes) {
+    for (auto& p : b->inDev)
+      if (p) HIP_OK(hipFree(p));
+    for (auto& p : b->inDev) HIP_OK(hipMalloc(&p, bytes * b->slots));
+    b->slotBytes = bytes;
+  }
+  if (!b->descDev[0])
+    for (int k = 0; k < 2; k++) {
+      HIP_OK(hipMalloc(&b->descDev[k], sizeof(lst::InputDesc) * b->slots));
+      HIP_OK(hipHostMalloc((void**)&b->descHost[k], sizeof(lst::InputDesc) * b->slots, hipHostMallocDefault));
+      HIP_OK(hipEventCreateWithFlags(&b->copyDone[k], hipEventDisableTiming));
+      HIP_OK(hipEventCreateWithFlags(&b->computeDone[k], hipEventDisableTiming));
+    }
+  if (!b->outHost[0] || cap > b->outCapAlloc) {
+    for (auto& p : b->outHost)
+      if (p) HIP_OK(hipHostFree(p));
+    for (auto& p : b->outHost)
+      HIP_OK(hipHostMalloc((void**)&p, 4 * b->out_words(cap) * b->slots, hipHostMallocDefault));
+    b->outCapAlloc = cap;
+  }
+  return LST_OK;
+}
+
+// the stage sequence of one launch set reading the descriptors of buffer `buf`
+void enqueue_set(lst_batch* b, int buf, const hipEvent_t* tuneEvents = nullptr, uint32_t* tuneRecorded = nullptr) {
+  lst::Launch L{b->ctx[0]->v, b->viewsDev, b->slots, b->stream, lst::no_timers()};
+  L.grid = b->grid;
+  L.fallbacks = false;  // overflowing events are run again by the per-event handle
+  L.tuneEvents = tuneEvents;
+  L.tuneRecorded = tuneRecorded;
+  lst::launch_stage_in(L, b->descDev[buf]);
+  lst::launch_hits(L);
+  lst::launch_pixel_segments(L);
+  lst::launch_mini_doublets(L);
+  lst::launch_segments(L);
+  lst::launch_triplets(L);
+  lst::launch_quintuplets(L);
+  lst::launch_pixel_segment_cleaning(L);
+  lst::launch_pixel_quintuplets(L);
+  lst::launch_pixel_triplets(L);
+  lst::launch_track_candidates(L);
+  lst::launch_out_copy(L, b->descDev[buf]);
+}
+
+lst_status capture(lst_batch* b) {
+  for (int buf = 0; buf < 2; buf++) {
+    if (b->graph[buf]) continue;
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_set(b, buf);
+    HIP_OK(hipStreamEndCapture(b->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&b->graph[buf], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(LST_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return LST_OK;
+}
+
+// the inputs of events ids[0..n) into buffer buf (copy stream); the host may
+// rewrite descHost[buf] because the set that last used buf has been consumed
+lst_status issue_inputs(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, uint32_t outCap) {
+  const size_t ow = b->out_words(outCap);
+  for (uint32_t i = 0; i < b->slots; i++) {
+    lst::InputDesc& d = b->descHost[buf][i];
+    if (i < n) {
+      const PreparedInput& p = b->staged[ids[i]];
+      char* dst = b->inDev[buf] + (size_t)i * b->slotBytes;
+      d = p.desc(dst);
+      HIP_OK(hipMemcpyAsync(dst, p.host, p.bytes, hipMemcpyHostToDevice, b->copyStream));
+    } else {
+      d = lst::InputDesc{};  // an empty event
+    }
+    d.outHost = b->outHost[buf] + ow * i;
+    d.outCap = outCap;
+  }
+  HIP_OK(hipMemcpyAsync(b->descDev[buf], b->descHost[buf], sizeof(lst::InputDesc) * b->slots,
+                        hipMemcpyHostToDevice, b->copyStream));
+  HIP_OK(hipEventRecord(b->copyDone[buf], b->copyStream));
+  return LST_OK;
+}
+
+lst_status launch_set(lst_batch* b, int buf) {
+  HIP_OK(hipStreamWaitEvent(b->stream, b->copyDone[buf], 0));
+  if (b->graphs) {
+    lst_status s = capture(b);
+    if (s != LST_OK) return s;
+    HIP_OK(hipGraphLaunch(b->graph[buf], b->stream));
+  } else {
+    enqueue_set(b, buf);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipEventRecord(b->computeDone[buf], b->stream));
+  return LST_OK;
+}
+
+// one event's output lists from a packed block (or the per-event handle) into
+// the caller's descriptor; returns the event's status
+lst_status copy_output(const uint32_t* blk, lst_output* d) {
+  const uint32_t n = blk[lst::kOutNTC];
+  d->n_tc = n;
+  const uint32_t m = std::min(n, d->capacity);
+  const uint32_t* rec = blk + lst::kOutHeaderWords;
+  for (uint32_t t = 0; t < m; t++, rec += lst::kOutRecWords) {
+    if (d->hit_idxs) std::memcpy(d->hit_idxs + 14 * (size_t)t, rec, 56);
+    if (d->len) d->len[t] = rec[14];
+    if (d->seed_idx) d->seed_idx[t] = (int32_t)rec[15];
+    if (d->type) d->type[t] = (int16_t)(int32_t)rec[16];
+  }
+  return n > d->capacity || blk[lst::kOutTruncated] ? (lst_status)LST_W_TRUNCATED : LST_OK;
+}
+
+lst_status rerun_event(lst_batch* b, const PreparedInput& p, lst_output* d) {
+  if (!b->rerun) {
+    lst_status s = lst_event_create(b->es, nullptr, b->t5Mode, &b->rerun);
+    if (s != LST_OK) return s;
+  }
+  lst_event* ev = b->rerun;
+  lst_status s = run_prepared(ev, p, 9);
+  if ((s & 0xff) != LST_OK) return s;
+  if (!d) return lst_event_synchronize(ev);
+  uint32_t n = 0;
+  lst_status st = lst_event_get_output(ev, &n, nullptr, nullptr, nullptr, nullptr, 0);
+  if ((st & 0xff) != LST_OK) return st;
+  std::vector<uint32_t> hits((size_t)n * 14), len(n);
+  std::vector<int32_t> seed(n);
+  std::vector<int16_t> type(n);
+  if (n) {
+    st = lst_event_get_output(ev, &n, hits.data(), len.data(), seed.data(), type.data(), n);
+    if ((st & 0xff) != LST_OK) return st;
+  }
+  d->n_tc = n;
+  const uint32_t m = std::min(n, d->capacity);
+  for (uint32_t t = 0; t < m; t++) {
+    if (d->hit_idxs) std::memcpy(d->hit_idxs + 14 * (size_t)t, hits.data() + 14 * (size_t)t, 56);
+    if (d->len) d->len[t] = len[t];
+    if (d->seed_idx) d->seed_idx[t] = seed[t];
+    if (d->type) d->type[t] = type[t];
+  }
+  return n > d->capacity ? (lst_status)(st | LST_W_TRUNCATED) : st;
+}
+
+// waits for the set in buffer buf and hands its outputs over
+lst_status consume(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, lst_output* out, uint32_t outCap,
+                   lst_status& agg) {
+  HIP_OK(hipEventSynchronize(b->computeDone[buf]));
+  const size_t ow = b->out_words(outCap);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* blk = b->outHost[buf] + ow * i;
+    lst_output* d = out ? &out[i] : nullptr;
+    const PreparedInput& p = b->staged[ids[i]];
+    lst_status s;
+    if (blk[lst::kOutBadHits]) {
+      const uint32_t bad = blk[lst::kOutBadHits];
+      s = fail(LST_E_INPUT, "event " + std::to_string(ids[i]) +
+                                (bad & lst::kBadDetId ? ": a hit's detId is no module of the geometry"
+                                 : bad & lst::kBadModuleHits ? ": more than 127 hits on one module"
+                                                              : ": the hits of a module are not contiguous"));
+      if (d) d->n_tc = 0;
+    } else if (blk[lst::kOutOverflow]) {
+      s = rerun_event(b, p, d);
+    } else {
+      s = d ? copy_output(blk, d) : (blk[lst::kOutTruncated] ? (lst_status)LST_W_TRUNCATED : LST_OK);
+    }
+    if (p.truncated) s = (lst_status)(s | LST_W_TRUNCATED);
+    if (d) d->status = (int32_t)s;
+    if ((s & 0xff) != LST_OK && (agg & 0xff) == LST_OK) agg = (lst_status)((s & 0xff) | (agg & LST_W_TRUNCATED));
+    else agg = (lst_status)(agg | (s & LST_W_TRUNCATED));
+  }
+  return LST_OK;
+}
+
+// events ids[0..n) through launch sets of b->slots events, two sets in flight
+lst_status run_ids(lst_batch* b, const std::vector<uint32_t>& ids, lst_output* out) {
+  uint32_t outCap = 0;
+  size_t maxBytes = 0;
+  uint32_t maxHits = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    if (out) outCap = std::max(outCap, std::min(out[i].capacity, kTCCap));
+    maxBytes = std::max(maxBytes, b->staged[ids[i]].bytes);
+    maxHits = std::max(maxHits, b->staged[ids[i]].nHits);
+  }
+  lst_status s = batch_arenas(b, maxHits);
+  if (s != LST_OK) return s;
+  if ((s = batch_io(b, maxBytes, outCap)) != LST_OK) return s;
+  const uint32_t S = b->slots, nSets = (uint32_t)((ids.size() + S - 1) / S);
+  lst_status agg = LST_OK;
+  auto set_n = [&](uint32_t k) { return std::min<uint32_t>(S, (uint32_t)ids.size() - k * S); };
+  for (uint32_t k = 0; k < nSets; k++) {
+    const int buf = (int)(k & 1);
+    if ((s = issue_inputs(b, buf, ids.data() + (size_t)k * S, set_n(k), outCap)) != LST_OK) return s;
+    if ((s = launch_set(b, buf)) != LST_OK) return s;
+    if (k > 0) {
+      const uint32_t j = k - 1;
+      if ((s = consume(b, buf ^ 1, ids.data() + (size_t)j * S, set_n(j), out ? out + (size_t)j * S : nullptr, outCap,
+                       agg)) != LST_OK)
+        return s;
     }
   }
-  o.pending = -1;
-  o.dst = nullptr;
-  return LST_OK;
+  if (nSets > 0) {
+    const uint32_t j = nSets - 1;
+    if ((s = consume(b, (int)(j & 1), ids.data() + (size_t)j * S, set_n(j), out ? out + (size_t)j * S : nullptr,
+                     outCap, agg)) != LST_OK)
+      return s;
+  }
+  return agg;
 }
 
-// run events [0, n) through the contexts; `launch(ctx, i)` enqueues event i
-template <class F>
-lst_status run_round_robin(lst_batch* b, uint32_t n, lst_output* out, F launch) {
-  const uint32_t K = (uint32_t)b->ctx.size();
-  lst_status warn = LST_OK, s;
-  for (uint32_t i = 0; i < n; i++) {
-    const uint32_t c = i % K;
-    if (b->outs[c].pending >= 0 && (s = finish_context(b, c, warn)) != LST_OK) return s;
-    s = launch(b->ctx[c], i);
-    if ((s & 0xff) != LST_OK) return s;
-    if (s != LST_OK) warn = s;
-    if (out && (s = enqueue_output(b, c, &out[i])) != LST_OK) return s;
-    b->outs[c].pending = i;
+lst_status stage_events(lst_batch* b, const lst_input* events, uint32_t n_ev) {
+  HIP_OK(hipStreamSynchronize(b->stream));
+  HIP_OK(hipStreamSynchronize(b->copyStream));
+  for (auto& p : b->staged) p.release();
+  b->staged.assign(n_ev, PreparedInput());
+  for (uint32_t i = 0; i < n_ev; i++) {
+    lst_status s = prepare_input(events[i], b->staged[i]);
+    if (s != LST_OK) return s;
   }
-  for (uint32_t c = 0; c < K; c++)
-    if ((s = finish_context(b, c, warn)) != LST_OK) return s;
-  return warn;
+  return LST_OK;
 }
 }  // namespace
 
 extern "C" {
 
-lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_batch** out) {
+const char* lst_tune_name(uint32_t k) { return k < lst::kNTune ? lst::kTuneNames[k] : nullptr; }
+uint32_t lst_tune_count(void) { return lst::kNTune; }
+
+lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch** out) {
   if (!out) return fail(LST_E_INPUT, "null argument");
   *out = nullptr;
   if (!es) return fail(LST_E_INPUT, "null event setup");
-  if (n_contexts == 0 || n_contexts > 64) return fail(LST_E_INPUT, "a batch needs 1..64 event contexts");
+  if (n_slots == 0 || n_slots > lst::kMaxBatchSlots) return fail(LST_E_INPUT, "a batch holds 1..128 events per launch set");
+  if (t5_mode != LST_T5_PLAIN_CHI2 && t5_mode != LST_T5_DNN) return fail(LST_E_INPUT, "unknown T5 selection mode");
   auto b = std::make_unique<lst_batch>();
   b->es = es;
-  for (uint32_t c = 0; c < n_contexts; c++) {
+  b->t5Mode = t5_mode;
+  b->slots = n_slots;
+  HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&b->copyStream, hipStreamNonBlocking));
+  for (uint32_t c = 0; c < n_slots; c++) {
     lst_event* ev = nullptr;
-    lst_status s = lst_event_create(es, nullptr, t5_mode, &ev);
+    lst_status s = lst_event_create(es, b->stream, t5_mode, &ev);
     if (s != LST_OK) {
       lst_batch_destroy(b.release());
       return s;
     }
+    ev->v.noFallback = 1;  // uploaded with the batch's view array
     b->ctx.push_back(ev);
   }
-  b->outs.resize(n_contexts);
-  const char* g = std::getenv("LST_BATCH_GRAPHS");  // "0": launch every kernel instead
+  const char* g = std::getenv("LST_BATCH_GRAPHS");  // "0": launch every kernel instead of the captured graph
   b->graphs = !(g && g[0] == '0');
   *out = b.release();
   return LST_OK;
@@ -1033,48 +1275,201 @@ lst_status lst_batch_create(lst_es* es, uint32_t n_contexts, int t5_mode, lst_ba
 
 void lst_batch_destroy(lst_batch* b) {
   if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->copyStream) (void)hipStreamSynchronize(b->copyStream);
+  drop_graphs(b);
   for (lst_event* ev : b->ctx) lst_event_destroy(ev);
-  for (auto& o : b->outs)
-    if (o.pinned) (void)hipHostFree(o.pinned);
-  for (auto& st : b->staged)
-    if (st.mem) (void)hipFree(st.mem);
+  if (b->rerun) lst_event_destroy(b->rerun);
+  for (auto& p : b->staged) p.release();
+  for (int k = 0; k < 2; k++) {
+    if (b->inDev[k]) (void)hipFree(b->inDev[k]);
+    if (b->descDev[k]) (void)hipFree(b->descDev[k]);
+    if (b->descHost[k]) (void)hipHostFree(b->descHost[k]);
+    if (b->outHost[k]) (void)hipHostFree(b->outHost[k]);
+    if (b->copyDone[k]) (void)hipEventDestroy(b->copyDone[k]);
+    if (b->computeDone[k]) (void)hipEventDestroy(b->computeDone[k]);
+  }
+  if (b->viewsDev) (void)hipFree(b->viewsDev);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->copyStream) (void)hipStreamDestroy(b->copyStream);
   delete b;
 }
 
+uint32_t lst_batch_contexts(const lst_batch* b) { return b ? b->slots : 0; }
+
 lst_status lst_run_batch(lst_batch* b, const lst_input* events, uint32_t n_ev, lst_output* out) {
   if (!b || (n_ev && !events)) return fail(LST_E_INPUT, "null argument");
-  return run_round_robin(b, n_ev, out, [&](lst_event* ev, uint32_t i) { return lst_run(ev, &events[i], 9); });
+  lst_status s = stage_events(b, events, n_ev);
+  if (s != LST_OK) return s;
+  std::vector<uint32_t> ids(n_ev);
+  for (uint32_t i = 0; i < n_ev; i++) ids[i] = i;
+  return run_ids(b, ids, out);
 }
 
 lst_status lst_batch_stage(lst_batch* b, const lst_input* events, uint32_t n_ev) {
   if (!b || (n_ev && !events)) return fail(LST_E_INPUT, "null argument");
-  for (lst_event* ev : b->ctx) HIP_OK(hipStreamSynchronize(ev->stream));
-  if (b->staged.size() < n_ev) b->staged.resize(n_ev);
+  lst_status s = stage_events(b, events, n_ev);
+  if (s != LST_OK) return s;
   uint32_t maxHits = 0;
-  for (uint32_t i = 0; i < n_ev; i++) {
-    lst_status s = stage_input(events[i], b->staged[i]);
-    if (s != LST_OK) return s;
-    maxHits = std::max(maxHits, b->staged[i].nHits);
+  size_t maxBytes = 0;
+  for (const auto& p : b->staged) {
+    maxHits = std::max(maxHits, p.nHits);
+    maxBytes = std::max(maxBytes, p.bytes);
   }
-  for (lst_event* ev : b->ctx) {  // size every arena for the largest staged event up front
-    lst_status s = ev->allocate(maxHits);
-    if (s != LST_OK) return s;
-  }
-  return LST_OK;
+  if ((s = batch_arenas(b, maxHits)) != LST_OK) return s;
+  return batch_io(b, maxBytes, b->outCapAlloc);
 }
 
 lst_status lst_batch_run_staged(lst_batch* b, uint32_t first, uint32_t n_ev, lst_output* out) {
   if (!b) return fail(LST_E_INPUT, "null batch");
   const size_t ns = b->staged.size();
   if (ns == 0 || first >= ns) return fail(LST_E_INPUT, "batch slots not staged");
-  for (const auto& st : b->staged)
-    if (!st.mem) return fail(LST_E_INPUT, "batch slot not staged");
-  return run_round_robin(b, n_ev, out, [&](lst_event* ev, uint32_t i) {
-    const lst_event::Staged& st = b->staged[(first + i) % ns];
-    return b->graphs ? run_staged_graph(ev, st) : run_staged(ev, st, 9);
-  });
+  std::vector<uint32_t> ids(n_ev);
+  for (uint32_t i = 0; i < n_ev; i++) ids[i] = (uint32_t)((first + i) % ns);
+  return run_ids(b, ids, out);
 }
 
-uint32_t lst_batch_contexts(const lst_batch* b) { return b ? (uint32_t)b->ctx.size() : 0; }
+lst_status lst_batch_autotune(lst_batch* b, uint32_t first, uint32_t repeats, float* default_ms, float* tuned_ms) {
+  if (!b) return fail(LST_E_INPUT, "null batch");
+  const size_t ns = b->staged.size();
+  if (ns == 0 || first >= ns) return fail(LST_E_INPUT, "batch slots not staged");
+  if (repeats == 0) repeats = 1;
+  std::vector<uint32_t> ids(std::min<size_t>(b->slots, ns));
+  for (size_t i = 0; i < ids.size(); i++) ids[i] = (uint32_t)((first + i) % ns);
+  uint32_t maxHits = 0;
+  size_t maxBytes = 0;
+  for (uint32_t id : ids) {
+    maxHits = std::max(maxHits, b->staged[id].nHits);
+    maxBytes = std::max(maxBytes, b->staged[id].bytes);
+  }
+  lst_status s = batch_arenas(b, maxHits);
+  if (s != LST_OK) return s;
+  if ((s = batch_io(b, maxBytes, b->outCapAlloc)) != LST_OK) return s;
+  std::vector<hipEvent_t> ev(2 * lst::kNTune);
+  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  // every tunable kernel at once takes candidate count c (0: its default); the
+  // kernels run one after the other on one stream, so each one's time depends
+  // on its own count only
+  static const uint32_t cand[] = {0, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096};
+  constexpr int nc = sizeof(cand) / sizeof(cand[0]);
+  std::vector<float> best((size_t)lst::kNTune * nc, 1e30f);
+  uint32_t saved[lst::kNTune];
+  std::memcpy(saved, b->grid, sizeof(saved));
+  uint32_t recordedAll[2] = {0, 0};
+  lst_status st = LST_OK;
+  for (int c = 0; c < nc && st == LST_OK; c++) {
+    for (int k = 0; k < lst::kNTune; k++)  // the pair kernels' tuned count is of y shares, at most 256
+      b->grid[k] = (k == lst::kTnPT3Pairs || k == lst::kTnCrossPT3) ? std::min(cand[c], 256u) : cand[c];
+    for (uint32_t r = 0; r < repeats && st == LST_OK; r++) {
+      if ((st = issue_inputs(b, 0, ids.data(), (uint32_t)ids.size(), 0)) != LST_OK) break;
+      if (hipStreamWaitEvent(b->stream, b->copyDone[0], 0) != hipSuccess) {
+        st = fail(LST_E_DEVICE, "hipStreamWaitEvent failed");
+        break;
+      }
+      uint32_t recorded[2] = {0, 0};
+      enqueue_set(b, 0, ev.data(), recorded);
+      if (hipStreamSynchronize(b->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+        st = fail(LST_E_DEVICE, "autotune run failed");
+        break;
+      }
+      for (int k = 0; k < lst::kNTune; k++) {
+        if (!((recorded[k / 32] >> (k % 32)) & 1u)) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) continue;
+        float& bk = best[(size_t)k * nc + c];
+        bk = std::min(bk, ms);
+      }
+      recordedAll[0] |= recorded[0];
+      recordedAll[1] |= recorded[1];
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (st != LST_OK) {
+    std::memcpy(b->grid, saved, sizeof(saved));
+    return st;
+  }
+  float tDefault = 0.f, tTuned = 0.f;
+  for (int k = 0; k < lst::kNTune; k++) {
+    b->grid[k] = 0;
+    if (!((recordedAll[k / 32] >> (k % 32)) & 1u)) continue;
+    int bc = 0;
+    for (int c = 1; c < nc; c++)
+      if (best[(size_t)k * nc + c] < 0.97f * best[(size_t)k * nc + bc]) bc = c;  // keep the default within 3 %
+    b->grid[k] = cand[bc];
+    tDefault += best[(size_t)k * nc];
+    tTuned += best[(size_t)k * nc + bc];
+  }
+  if (default_ms) *default_ms = tDefault;
+  if (tuned_ms) *tuned_ms = tTuned;
+  drop_graphs(b);
+  return LST_OK;
+}
+
+lst_status lst_batch_kernel_times(lst_batch* b, uint32_t first, float* ms, uint32_t n) {
+  if (!b || !ms) return fail(LST_E_INPUT, "null argument");
+  const size_t ns = b->staged.size();
+  if (ns == 0 || first >= ns) return fail(LST_E_INPUT, "batch slots not staged");
+  std::vector<uint32_t> ids(std::min<size_t>(b->slots, ns));
+  for (size_t i = 0; i < ids.size(); i++) ids[i] = (uint32_t)((first + i) % ns);
+  uint32_t maxHits = 0;
+  size_t maxBytes = 0;
+  for (uint32_t id : ids) {
+    maxHits = std::max(maxHits, b->staged[id].nHits);
+    maxBytes = std::max(maxBytes, b->staged[id].bytes);
+  }
+  lst_status s = batch_arenas(b, maxHits);
+  if (s != LST_OK) return s;
+  if ((s = batch_io(b, maxBytes, b->outCapAlloc)) != LST_OK) return s;
+  std::vector<hipEvent_t> ev(2 * lst::kNTune);
+  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  uint32_t recorded[2] = {0, 0};
+  s = issue_inputs(b, 0, ids.data(), (uint32_t)ids.size(), 0);
+  if (s == LST_OK) {
+    if (hipStreamWaitEvent(b->stream, b->copyDone[0], 0) != hipSuccess) s = fail(LST_E_DEVICE, "hipStreamWaitEvent");
+  }
+  if (s == LST_OK) {
+    enqueue_set(b, 0, ev.data(), recorded);
+    if (hipStreamSynchronize(b->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+      s = fail(LST_E_DEVICE, "timed launch set failed");
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    ms[k] = 0.f;
+    if (s == LST_OK && k < (uint32_t)lst::kNTune && ((recorded[k / 32] >> (k % 32)) & 1u))
+      (void)hipEventElapsedTime(&ms[k], ev[2 * k], ev[2 * k + 1]);
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return s;
+}
+
+lst_status lst_batch_get_tuning(const lst_batch* b, uint32_t* grid, uint32_t n) {
+  if (!b || !grid) return fail(LST_E_INPUT, "null argument");
+  for (uint32_t k = 0; k < n && k < (uint32_t)lst::kNTune; k++) grid[k] = b->grid[k];
+  return LST_OK;
+}
+
+lst_status lst_batch_set_tuning(lst_batch* b, const uint32_t* grid, uint32_t n) {
+  if (!b || !grid) return fail(LST_E_INPUT, "null argument");
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (uint32_t k = 0; k < (uint32_t)lst::kNTune; k++) b->grid[k] = k < n ? grid[k] : 0u;
+  drop_graphs(b);
+  return LST_OK;
+}
 
 }  // extern "C"
+
+namespace lst {
+const char* const kTuneNames[kNTune] = {
+    "k_stage_in", "k_event_init", "k_hits", "k_module_ranges", "k_hit_ranges_fix", "k_pixel_segments",
+    "k_md_cand_fill", "k_md_eval", "k_md_scan_blocks", "k_md_place",
+    "k_ls_cand_count", "k_ls_cand_fill", "k_ls_eval", "k_ls_scan_blocks", "k_ls_place",
+    "k_md_seg_lists", "k_t3_cand_count", "k_t3_enum", "k_t3_select", "k_t3_scan_blocks", "k_t3_place",
+    "k_md_t3_lists", "k_t5_cand_count", "k_t5_enum", "k_t5_filter_t4", "k_t5_final", "k_t5_dnn",
+    "k_t5_scan_blocks", "k_t5_place", "k_t5_list_fill", "k_t5_dup_after_build", "k_pls_pairs",
+    "k_pls_connected", "k_t3_list_fill", "k_pix_records",
+    "k_pix_stage_a<PT5Kind>", "k_pix_stage_b<PT5Kind>", "k_pix_write<PT5Kind>", "k_pix_full_write<PT5Kind>",
+    "k_pt5_bucket_count", "k_pt5_bucket_fill", "k_pt5_pairs",
+    "k_pix_stage_a<PT3Kind>", "k_pix_stage_b<PT3Kind>", "k_pix_write<PT3Kind>", "k_pix_full_write<PT3Kind>",
+    "k_pt3_pairs (y shares)", "k_cross_pt3_pairs (y shares)",
+    "k_t5_tc_pairs", "k_cross_t5_pairs", "k_tc_t5_count", "k_tc_t5_write", "k_tc_prep", "k_cross_pls_pairs",
+    "k_tc_output", "k_out_copy"};
+}  // namespace lst

@@ -346,7 +346,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   for (auto& it : c.items)
     if (it.second.second &&
         hipMemcpy(base + it.first, it.second.first, it.second.second, hipMemcpyHostToDevice) != hipSuccess) {
-      hipFree(base);
+      (void)hipFree(base);
       err = "geometry upload failed";
       return false;
     }
@@ -391,7 +391,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
 }
 
 void free_geometry(DeviceGeometry& d) {
-  if (d.arena) hipFree(d.arena);
+  if (d.arena) (void)hipFree(d.arena);
   d = DeviceGeometry();
 }
 

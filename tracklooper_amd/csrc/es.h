@@ -18,6 +18,16 @@ struct HostGeometry {
   // 2 words per slot, 2^detHashBits slots, empty slots hold kNoDetId
   std::vector<uint32_t> detHash;
   uint32_t detHashBits = 0;
+  // module index of a detId, -1 if the geometry has no such module
+  int module_of(uint32_t det) const {
+    const uint32_t mask = (1u << detHashBits) - 1;
+    uint32_t s = det_hash_slot(det, detHashBits);
+    for (uint32_t n = 0; n <= mask; n++, s = (s + 1) & mask) {
+      if (detHash[2 * s] == det) return (int)(detHash[2 * s + 1] & 0xFFFFu);
+      if (detHash[2 * s] == kNoDetId) break;
+    }
+    return -1;
+  }
   std::vector<uint16_t> moduleMap, mapIdx, nConnectedModules, partnerModuleIndices;
   std::vector<float> drdzs, dxdys, eta, r, geoMapPhi;
   std::vector<int16_t> layers, rings, modules, rods, subdets, sides;

@@ -283,8 +283,24 @@ enum Scalar : int {
   kEvNPLS = 30,   // pixel line segments of the event
   kPixFull = 31,  // bit 0: some pLS evaluated in full by the pT5 builder, bit 1: by the pT3 builder
   kT3Listed = 32,  // T3 slots listed for the pixel-triplet records
+  kTruncated = 33,  // bits of TruncBit: objects dropped at a capacity, as the reference drops them
   kNScalars = 40
 };
+// kBadHits bits: input the reference cannot represent (it indexes out of bounds or wraps a count there)
+enum BadHitBit : uint32_t {
+  kBadDetId = 1,         // a hit's detId is no module of the geometry (SDL/Hit.h:245-246 would index mapIdx[-1])
+  kBadModuleHits = 2,    // more than 127 hits on one module (the int8 counts of SDL/Hit.h:31-32 wrap)
+  kBadNonContiguous = 4  // a module's hits are not contiguous in the input (the ranges of SDL/Hit.h:261-267 assume so)
+};
+// kTruncated bits
+enum TruncBit : uint32_t {
+  kTruncMD = 1, kTruncLS = 2, kTruncT3 = 4, kTruncT5 = 8, kTruncPT5 = 16, kTruncPT3 = 32
+};
+// kOutOverflow bits: the flattened builders' work arrays could not hold the event's candidates, so the stage
+// needs the per-module builder (run by the per-event path)
+enum OverflowBit : uint32_t { kOvfMD = 1, kOvfLS = 2, kOvfT3 = 4, kOvfT5 = 8, kOvfPerModule = 16 };
+// words of the packed output header
+enum OutHeader : int { kOutNTC = 0, kOutOverflow = 1, kOutTruncated = 2, kOutBadHits = 3 };
 }  // namespace lst
 namespace lst {
 static_assert(kScanCandMax >= kMDCandMax && kScanCandMax >= kLSCandMax && kScanCandMax >= kT3CandMax &&

@@ -49,10 +49,37 @@ struct EventView {  // everything a stage kernel needs, by value
   int perModuleBuilders;             // run the per-module T3/T5 builders instead of the flattened ones
   int largePaths;                    // per-MD lists and eta binning take the paths of oversized modules/events
   int t5Dnn;                         // T5 selection by the DNN (USE_T5_DNN) instead of the r-phi chi2 cut
+  int noFallback;                    // a batch slot: no per-module builder runs after the flattened ones
   T5DnnDump dnnDump;
+  uint32_t* plsInHitIdx[4];          // [pLS capacity] the four pseudo-hit indices of each pLS, as added
+  float* plsInDPhiChange;            // [pLS capacity] its deltaPhiChange, as added
+  uint32_t outCap;                   // track-candidate records W.outPacked holds
   uint32_t nHits, nPLS, pixelModule;
   float ptCut;
 };
+
+// Kernels read their event's view from a device array: view blockIdx.z.  One
+// launch sequence therefore serves one event (gridDim.z = 1) or a batch of
+// events, each with its own arena, in the same launches (gridDim.z = events).
+#define LST_VIEWS const EventView* __restrict__ vs_
+#define LST_BIND const EventView& v = vs_[blockIdx.z]
+
+// Grid-stride kernels whose work-group count a batch may choose (autotuner).
+// Every kernel listed here is correct for any gridDim.x >= 1.
+enum TuneId : int {
+  kTnStageIn = 0, kTnEventInit, kTnHits, kTnModuleRanges, kTnHitRangesFix, kTnPixelSegments,
+  kTnMdCandFill, kTnMdEval, kTnMdScanBlocks, kTnMdPlace,
+  kTnLsCandCount, kTnLsCandFill, kTnLsEval, kTnLsScanBlocks, kTnLsPlace,
+  kTnMdSegLists, kTnT3CandCount, kTnT3Enum, kTnT3Select, kTnT3ScanBlocks, kTnT3Place,
+  kTnMdT3Lists, kTnT5CandCount, kTnT5Enum, kTnT5FilterT4, kTnT5Final, kTnT5Dnn, kTnT5ScanBlocks, kTnT5Place,
+  kTnT5ListFill, kTnT5DupAfter, kTnPlsPairs, kTnPlsConnected, kTnT3ListFill, kTnPixRecords,
+  kTnPT5StageA, kTnPT5StageB, kTnPT5Write, kTnPT5FullWrite, kTnPT5BucketCount, kTnPT5BucketFill, kTnPT5Pairs,
+  kTnPT3StageA, kTnPT3StageB, kTnPT3Write, kTnPT3FullWrite, kTnPT3Pairs, kTnCrossPT3,
+  kTnT5TcPairs, kTnCrossT5, kTnTcT5Count, kTnTcT5Write, kTnTcPrep, kTnCrossPls, kTnTcOutput, kTnOutCopy,
+  kNTune
+};
+extern const char* const kTuneNames[kNTune];
+constexpr uint32_t kMaxBatchSlots = 128;  // events of one batch launch set
 
 // Optional hipEvent pairs recorded around the hot kernels (timing).  Slot k
 // brackets one kernel; `recorded` (null: timing off) collects the slots used.
@@ -93,6 +120,39 @@ inline const KernelTimers& no_timers() {
   return off;
 }
 
+// One enqueue of (part of) the stage sequence.
+struct Launch {
+  const EventView& v;            // host copy of the first event's view: geometry sizes and mode flags, which
+                                 //   every event of a launch shares
+  const EventView* dv;           // device array of nEv views
+  uint32_t nEv;                  // events of the launch (gridDim.z)
+  hipStream_t s;
+  const KernelTimers& t;         // per-kernel timers (one-event launches)
+  const uint32_t* grid = nullptr;  // [kNTune] chosen work-group counts (0: default), null: defaults
+  bool fallbacks = true;         // launch the per-module builders that take events overflowing the flattened ones
+  const hipEvent_t* tuneEvents = nullptr;  // [2 kNTune] recorded before / after each tunable kernel (autotuner)
+  uint32_t* tuneRecorded = nullptr;        // bit k of word k / 32: kernel k was bracketed
+  uint32_t gx(int id, uint32_t dflt) const { return grid && grid[id] ? grid[id] : dflt; }
+  void tune_begin(int id) const {
+    if (!tuneEvents) return;
+    (void)hipEventRecord(tuneEvents[2 * id], s);
+    tuneRecorded[id / 32] |= 1u << (id % 32);
+  }
+  void tune_end(int id) const {
+    if (tuneEvents) (void)hipEventRecord(tuneEvents[2 * id + 1], s);
+  }
+};
+// launch kernel K over every event of L; gridDim = (GX, GY, events)
+#define LST_GO(L, K, GX, GY, BLOCK) hipLaunchKernelGGL(K, dim3((GX), (GY), (L).nEv), dim3(BLOCK), 0, (L).s, (L).dv)
+// the same for a grid-stride kernel whose work-group count the launch may override
+#define LST_GO_T(L, ID, K, GX, BLOCK)                                                              \
+  do {                                                                                             \
+    const uint32_t gx_ = (L).gx(ID, (GX));                                                         \
+    (L).tune_begin(ID);                                                                            \
+    hipLaunchKernelGGL(K, dim3(gx_, 1, (L).nEv), dim3(BLOCK), 0, (L).s, (L).dv);                   \
+    (L).tune_end(ID);                                                                              \
+  } while (0)
+
 // Device-to-device copy of up to kMaxCopies 4-byte aligned buffers in one launch.
 constexpr int kMaxCopies = 24;
 struct CopyList {
@@ -118,31 +178,35 @@ struct InputDesc {
   const uint32_t* seedIdx;
   const int8_t *pixelType, *isQuad;
   uint32_t nPLS;
+  uint32_t* outHost;  // where k_out_copy puts the packed output block (pinned host memory), null: nowhere
+  uint32_t outCap;    // track-candidate records outHost holds
 };
-// copies the described inputs into the event arena and sets the event's counts
-void launch_stage_in(const EventView& v, const InputDesc* d, uint32_t* const plsHitIdx[4], float* plsDPhiChange,
-                     hipStream_t s);
+// copies the described inputs into the event arenas (descriptor blockIdx.z of
+// the device array d) and sets the events' counts
+void launch_stage_in(const Launch& L, const InputDesc* d);
 
-void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
-                           const KernelTimers& t = no_timers());
-void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
-void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimers& t = no_timers());
+void launch_hits(const Launch& L);
+void launch_pixel_segments(const Launch& L);
+void launch_mini_doublets(const Launch& L);
+void launch_segments(const Launch& L);
+void launch_triplets(const Launch& L);
+void launch_quintuplets(const Launch& L);
+void launch_pixel_segment_cleaning(const Launch& L);
+void launch_pixel_quintuplets(const Launch& L);
+void launch_pixel_triplets(const Launch& L);
+void launch_track_candidates(const Launch& L);
+// copies each event's packed output block (header and the records it holds room for) to the
+// destination its descriptor names
+void launch_out_copy(const Launch& L, const InputDesc* d);
 // all-pairs duplicate removal and cross cleaning (stages_clean.hip)
-void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t);
-void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
-void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t);
-void launch_cross_pt3(const EventView& v, hipStream_t s);
-void launch_t5_list(const EventView& v, hipStream_t s);
-void launch_t5_dup_after_build(const EventView& v, hipStream_t s, const KernelTimers& t);
-void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t);
-void launch_cross_t5(const EventView& v, hipStream_t s);
-void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t);
+void launch_pls_dup(const Launch& L, bool second);
+void launch_pt5_dup(const Launch& L);
+void launch_pt3_dup(const Launch& L);
+void launch_cross_pt3(const Launch& L);
+void launch_t5_list(const Launch& L);
+void launch_t5_dup_after_build(const Launch& L);
+void launch_t5_dup_before_tc(const Launch& L);
+void launch_cross_t5(const Launch& L);
+void launch_cross_pls(const Launch& L);
 
 }  // namespace lst

@@ -157,7 +157,8 @@ struct PlsBinItem {
   uint4 hits;
   float eta;
 };
-__global__ __launch_bounds__(1024) void k_bin_pls(EventView v) {
+__global__ __launch_bounds__(1024) void k_bin_pls(LST_VIEWS) {
+  LST_BIND;
   bin_by_eta<8>(
       v.R.scalars[kEvNPLS], v.largePaths != 0,
       [&](uint32_t i) {
@@ -200,7 +201,8 @@ __device__ __forceinline__ uint32_t band_at(const Band& B, uint32_t k) {
 // bins next to it and the NaN bin.
 constexpr uint32_t kPlsUnroll = 4, kQuadBit = 0x80000000u;
 template <bool kSecond>
-__global__ __launch_bounds__(256) void k_pls_pairs(EventView v) {
+__global__ __launch_bounds__(256) void k_pls_pairs(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kEvNPLS];
   const unsigned lane = threadIdx.x % kWave;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
@@ -275,7 +277,8 @@ __global__ __launch_bounds__(256) void k_pls_pairs(EventView v) {
 constexpr uint32_t kPT5HashMask = kPT5HashBuckets - 1;
 __device__ __forceinline__ uint32_t hit_bucket(uint32_t h) { return (h * 0x9E3779B1u) >> 17 & kPT5HashMask; }
 
-__global__ __launch_bounds__(256) void k_pt5_bucket_count(EventView v) {
+__global__ __launch_bounds__(256) void k_pt5_bucket_count(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 14 * n; e += gridDim.x * blockDim.x)
     atomicAdd(&v.W.pt5BucketOff[hit_bucket(v.P5.hitIndices[e])], 1u);
@@ -284,7 +287,8 @@ __global__ __launch_bounds__(256) void k_pt5_bucket_count(EventView v) {
 // inclusive scan of the bucket counts in place (one 1024-lane work-group,
 // kPT5HashBuckets / 1024 consecutive buckets per lane); entry kPT5HashBuckets
 // gets the total
-__global__ __launch_bounds__(1024) void k_pt5_bucket_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_pt5_bucket_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   constexpr uint32_t kPer = kPT5HashBuckets / 1024;
   static_assert(kPer % 4 == 0, "buckets per lane");
@@ -312,13 +316,15 @@ __global__ __launch_bounds__(1024) void k_pt5_bucket_scan(EventView v) {
 
 // entry e to its bucket; afterwards pt5BucketOff[b] is the first position of
 // bucket b (the inclusive ends are counted down)
-__global__ __launch_bounds__(256) void k_pt5_bucket_fill(EventView v) {
+__global__ __launch_bounds__(256) void k_pt5_bucket_fill(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 14 * n; e += gridDim.x * blockDim.x)
     v.W.pt5BucketEnt[atomicSub(&v.W.pt5BucketOff[hit_bucket(v.P5.hitIndices[e])], 1u) - 1] = e;
 }
 
-__global__ __launch_bounds__(256) void k_pt5_pairs(EventView v) {
+__global__ __launch_bounds__(256) void k_pt5_pairs(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
   const unsigned lane = threadIdx.x % kWave;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
@@ -385,7 +391,8 @@ __global__ __launch_bounds__(256) void k_pt5_pairs(EventView v) {
 // removeDupPixelTripletsInGPUFromMap with checkHitspT3: ix is dropped when a
 // partner shares >= 5 hits (pixel + outer) and reaches a lower third layer, or
 // the same layer with a lower score (ties: the lower index is dropped)
-__global__ __launch_bounds__(kTile) void k_pt3_pairs(EventView v) {
+__global__ __launch_bounds__(kTile) void k_pt3_pairs(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sScore[kTile];
   __shared__ uint8_t sLayer[kTile];
   __shared__ uint32_t sHits[10 * kTile];
@@ -443,7 +450,8 @@ __global__ __launch_bounds__(kTile) void k_pt3_pairs(EventView v) {
 
 // ---------------------------------------------------------- crossCleanpT3
 // a pT3 whose pLS direction is within dR2 < 1e-5 of a pT5's pLS is dropped
-__global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(EventView v) {
+__global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sEta[kTile], sPhi[kTile];
   const uint32_t n = v.R.scalars[kNPixelTriplets], n5 = v.R.scalars[kNPixelQuintuplets];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
@@ -475,7 +483,8 @@ __global__ __launch_bounds__(kTile) void k_cross_pt3_pairs(EventView v) {
 }
 
 // --------------------------------------------------- T5 list in slot order
-__global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
+__global__ __launch_bounds__(1024) void k_t5_list_offsets(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const uint32_t n = v.R.scalars[kNEligibleT5];
   const unsigned total = scan_wide<kScanPer>(
@@ -487,7 +496,8 @@ __global__ __launch_bounds__(1024) void k_t5_list_offsets(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_list_fill(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nWaves = (gridDim.x * blockDim.x) / 64;
   for (uint32_t e = wave; e < nElig; e += nWaves) {  // one wave per eligible module
@@ -511,7 +521,8 @@ __global__ __launch_bounds__(256) void k_t5_list_fill(EventView v) {
 // runs its lanes over the T5s of q's module (contiguous slots, so the loads
 // coalesce) and flags q when any partner points the rule at it; a flag is a
 // store of 1 that does not depend on evaluation order.
-__global__ __launch_bounds__(256) void k_t5_dup_after_build(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_dup_after_build(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kT5Listed];
   const unsigned lane = threadIdx.x % kWave;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
@@ -563,7 +574,8 @@ struct T5BinItem {
   float4 rec;  // (eta, phi, score, slot bits)
   float eta;
 };
-__global__ __launch_bounds__(1024) void k_bin_t5(EventView v) {
+__global__ __launch_bounds__(1024) void k_bin_t5(LST_VIEWS) {
+  LST_BIND;
   bin_by_eta(
       v.R.scalars[kT5Listed], v.largePaths != 0,
       [&](uint32_t i) {
@@ -586,7 +598,8 @@ __global__ __launch_bounds__(1024) void k_bin_t5(EventView v) {
 // unless the direction alone decides, so that this work runs with every lane
 // busy instead of once per pass for a few lanes.
 constexpr uint32_t kT5TcUnroll = 4, kT5TcQueue = 512, kNearBit = 0x80000000u;
-__global__ __launch_bounds__(256) void k_t5_tc_pairs(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_tc_pairs(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t sQueue[256 / kWave][kT5TcQueue];
   const uint32_t n = v.R.scalars[kT5Listed];
   const unsigned lane = threadIdx.x % kWave;
@@ -669,7 +682,8 @@ __global__ __launch_bounds__(256) void k_t5_tc_pairs(EventView v) {
 // dropped.  dR2 < 1e-3 needs |d eta| < 0.032, and a NaN direction never flags,
 // so the pT5/pT3 are binned by eta and a T5 tile (the T5s in eta-bin order)
 // meets only the bins next to its own.
-__global__ __launch_bounds__(1024) void k_bin_pix(EventView v) {
+__global__ __launch_bounds__(1024) void k_bin_pix(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
   bin_by_eta(
       n5 + v.R.scalars[kNPixelTriplets], v.largePaths != 0,
@@ -677,7 +691,8 @@ __global__ __launch_bounds__(1024) void k_bin_pix(EventView v) {
       [&](uint32_t pos, const BinItem& t) { v.W.pixBinned[pos] = t.id; }, v.W.pixBinOff);
 }
 
-__global__ __launch_bounds__(kTile) void k_cross_t5_pairs(EventView v) {
+__global__ __launch_bounds__(kTile) void k_cross_t5_pairs(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sEta[kTile], sPhi[kTile];
   const uint32_t n = v.R.scalars[kT5Listed];
   const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
@@ -753,7 +768,20 @@ __device__ __forceinline__ bool set_has(const int32_t* set, int32_t h) {
 
 // per track candidate: the direction a pLS is compared with and its kind (0:
 // none, 1: T5, 2: pT3/pT5); the pT3/pT5 pixel hits go into the set
-__global__ void k_tc_prep(EventView v) {
+__global__ void k_tc_prep(LST_VIEWS) {
+  LST_BIND;
+  // modules whose objects a capacity truncated (the per-module counts are final here); one OR per wave
+  uint32_t trunc = 0;
+  const uint32_t nL = v.M.nLowerModules;
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+    const uint32_t a = v.R.totOccupancyMDs[m], b = v.R.nMDs[m], c = v.R.totOccupancySegments[m],
+                   d = v.R.nSegments[m], e = v.R.totOccupancyTriplets[m], f = v.R.nTriplets[m],
+                   g = v.R.totOccupancyQuintuplets[m], h = v.R.nQuintuplets[m];
+    trunc |= (a > b ? (uint32_t)kTruncMD : 0u) | (c > d ? (uint32_t)kTruncLS : 0u) |
+             (e > f ? (uint32_t)kTruncT3 : 0u) | (g > h ? (uint32_t)kTruncT5 : 0u);
+  }
+  trunc = wave_or(trunc);
+  if (trunc && (threadIdx.x & 63) == 0) atomicOr(&v.R.scalars[kTruncated], trunc);
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nTC; t += gridDim.x * blockDim.x) {
@@ -780,13 +808,15 @@ __global__ void k_tc_prep(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_bin_tc(EventView v) {
+__global__ __launch_bounds__(1024) void k_bin_tc(LST_VIEWS) {
+  LST_BIND;
   bin_by_eta(
       v.R.scalars[kNTrackCandidates], v.largePaths != 0, [&](uint32_t t) { return BinItem{t, v.W.tcEta[t]}; },
       [&](uint32_t pos, const BinItem& t) { v.W.tcBinned[pos] = t.id; }, v.W.tcBinOff);
 }
 
-__global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
+__global__ __launch_bounds__(kTile) void k_cross_pls_pairs(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sEta[kTile], sPhi[kTile];
   __shared__ uint8_t sKind[kTile];
   const uint32_t n = v.R.scalars[kEvNPLS];
@@ -839,67 +869,72 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(EventView v) {
 
 }  // namespace
 
-void launch_pls_dup(const EventView& v, hipStream_t s, bool second, const KernelTimers& t) {
-  const dim3 g(1024);  // 4096 waves stride over the binned pLS
+void launch_pls_dup(const Launch& L, bool second) {
   const int tm = second ? kTmPlsDup2 : kTmPlsDup;
-  if (!second) hipLaunchKernelGGL(k_bin_pls, dim3(1), dim3(1024), 0, s, v);  // the eta order is kept for pass 2
-  t.begin(s, tm);
-  if (second) hipLaunchKernelGGL(k_pls_pairs<true>, g, dim3(256), 0, s, v);
-  else hipLaunchKernelGGL(k_pls_pairs<false>, g, dim3(256), 0, s, v);
-  t.end(s, tm);
+  if (!second) LST_GO(L, k_bin_pls, 1, 1, 1024);  // the eta order is kept for pass 2
+  L.t.begin(L.s, tm);
+  if (second) LST_GO_T(L, kTnPlsPairs, k_pls_pairs<true>, 1024, 256);  // 4096 waves stride over the binned pLS
+  else LST_GO_T(L, kTnPlsPairs, k_pls_pairs<false>, 1024, 256);
+  L.t.end(L.s, tm);
 }
 
 // the bucket counts are cleared by k_pls_connected, earlier in the chain
-void launch_pt5_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_pt5_bucket_count, dim3(256), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_pt5_bucket_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_pt5_bucket_fill, dim3(256), dim3(256), 0, s, v);
-  t.begin(s, kTmPT5Dup);
-  hipLaunchKernelGGL(k_pt5_pairs, dim3(1024), dim3(256), 0, s, v);
-  t.end(s, kTmPT5Dup);
+void launch_pt5_dup(const Launch& L) {
+  LST_GO_T(L, kTnPT5BucketCount, k_pt5_bucket_count, 256, 256);
+  LST_GO(L, k_pt5_bucket_scan, 1, 1, 1024);
+  LST_GO_T(L, kTnPT5BucketFill, k_pt5_bucket_fill, 256, 256);
+  L.t.begin(L.s, kTmPT5Dup);
+  LST_GO_T(L, kTnPT5Pairs, k_pt5_pairs, 1024, 256);
+  L.t.end(L.s, kTmPT5Dup);
 }
 
-void launch_pt3_dup(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  t.begin(s, kTmPT3Dup);
-  hipLaunchKernelGGL(k_pt3_pairs, pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelTriplets), dim3(kTile), 0, s, v);
-  t.end(s, kTmPT3Dup);
+void launch_pt3_dup(const Launch& L) {
+  const dim3 g = pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelTriplets);
+  L.t.begin(L.s, kTmPT3Dup);
+  L.tune_begin(kTnPT3Pairs);
+  LST_GO(L, k_pt3_pairs, g.x, L.gx(kTnPT3Pairs, g.y), kTile);  // the tuned count is of y shares
+  L.tune_end(kTnPT3Pairs);
+  L.t.end(L.s, kTmPT3Dup);
 }
 
-void launch_cross_pt3(const EventView& v, hipStream_t s) {
-  hipLaunchKernelGGL(k_cross_pt3_pairs, pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelQuintuplets), dim3(kTile), 0, s, v);
+void launch_cross_pt3(const Launch& L) {
+  const dim3 g = pair_grid(kNMaxPixelTriplets, 512, kNMaxPixelQuintuplets);
+  L.tune_begin(kTnCrossPT3);
+  LST_GO(L, k_cross_pt3_pairs, g.x, L.gx(kTnCrossPT3, g.y), kTile);  // the tuned count is of y shares
+  L.tune_end(kTnCrossPT3);
 }
 
 // the T5 slots of the eligible modules, module by module; built once the T5s
 // are final (the pixel-quintuplet stage) and read by the later stages
-void launch_t5_list(const EventView& v, hipStream_t s) {
-  hipLaunchKernelGGL(k_t5_list_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_list_fill, dim3(256), dim3(256), 0, s, v);
+void launch_t5_list(const Launch& L) {
+  LST_GO(L, k_t5_list_offsets, 1, 1, 1024);
+  LST_GO_T(L, kTnT5ListFill, k_t5_list_fill, 256, 256);
 }
 
-void launch_t5_dup_after_build(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  t.begin(s, kTmT5DupAfter);
-  hipLaunchKernelGGL(k_t5_dup_after_build, dim3(2048), dim3(256), 0, s, v);
-  t.end(s, kTmT5DupAfter);
+void launch_t5_dup_after_build(const Launch& L) {
+  L.t.begin(L.s, kTmT5DupAfter);
+  LST_GO_T(L, kTnT5DupAfter, k_t5_dup_after_build, 2048, 256);
+  L.t.end(L.s, kTmT5DupAfter);
 }
 
-void launch_t5_dup_before_tc(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_bin_t5, dim3(1), dim3(1024), 0, s, v);
-  t.begin(s, kTmT5DupTC);
-  hipLaunchKernelGGL(k_t5_tc_pairs, dim3(2048), dim3(256), 0, s, v);
-  t.end(s, kTmT5DupTC);
+void launch_t5_dup_before_tc(const Launch& L) {
+  LST_GO(L, k_bin_t5, 1, 1, 1024);
+  L.t.begin(L.s, kTmT5DupTC);
+  LST_GO_T(L, kTnT5TcPairs, k_t5_tc_pairs, 2048, 256);
+  L.t.end(L.s, kTmT5DupTC);
 }
 
-void launch_cross_t5(const EventView& v, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_pix, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_cross_t5_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
+void launch_cross_t5(const Launch& L) {
+  LST_GO(L, k_bin_pix, 1, 1, 1024);
+  LST_GO_T(L, kTnCrossT5, k_cross_t5_pairs, kPairItems, kTile);
 }
 
-void launch_cross_pls(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_tc_prep, dim3(128), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_bin_tc, dim3(1), dim3(1024), 0, s, v);
-  t.begin(s, kTmCrossPLS);
-  hipLaunchKernelGGL(k_cross_pls_pairs, dim3(kPairItems), dim3(kTile), 0, s, v);
-  t.end(s, kTmCrossPLS);
+void launch_cross_pls(const Launch& L) {
+  LST_GO_T(L, kTnTcPrep, k_tc_prep, 128, 256);
+  LST_GO(L, k_bin_tc, 1, 1, 1024);
+  L.t.begin(L.s, kTmCrossPLS);
+  LST_GO_T(L, kTnCrossPls, k_cross_pls_pairs, kPairItems, kTile);
+  L.t.end(L.s, kTmCrossPLS);
 }
 
 }  // namespace lst

@@ -46,10 +46,12 @@ __device__ __forceinline__ uint32_t det_lookup(const ModulesView& M, uint32_t de
 }
 
 // ------------------------------------------------------------------------ hits
-__global__ void k_event_init(EventView v) {
+__global__ void k_event_init(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n2 = 2 * v.M.nModules;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
     v.R.hitRanges[i] = (i & 1) ? -1 : INT_MAX;
+    if (i < v.M.nModules) v.R.hitRangesUpper[i] = 0;  // per-module hit counts of k_hits
     v.R.mdRanges[i] = -1;
     v.R.segmentRanges[i] = -1;
     v.R.tripletRanges[i] = -1;
@@ -61,7 +63,8 @@ __global__ void k_event_init(EventView v) {
 }
 
 // SDL/Hit.h:217-270
-__global__ void k_hits(EventView v) {
+__global__ void k_hits(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nHits = v.R.scalars[kEvNHits];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nHits; i += gridDim.x * blockDim.x) {
     const float x = v.H.xs[i], y = v.H.ys[i], z = v.H.zs[i];
@@ -78,7 +81,7 @@ __global__ void k_hits(EventView v) {
     const int mod = entry == ~0u ? -1 : (int)(entry & 0xFFFFu);
     if (mod < 0) {
       v.H.moduleIndices[i] = 0xFFFF;
-      atomicOr(&v.R.scalars[kBadHits], 1u);
+      atomicOr(&v.R.scalars[kBadHits], (uint32_t)kBadDetId);
     } else {
       v.H.moduleIndices[i] = mod;
       if (v.M.subdets[mod] == kEndcap && v.M.moduleType[mod] == kTwoS) {
@@ -103,6 +106,7 @@ __global__ void k_hits(EventView v) {
         const int top = 63 - __clzll(grp);
         atomicMin(&v.R.hitRanges[2 * lm], (int)i);
         atomicMax(&v.R.hitRanges[2 * lm + 1], (int)(i - lane + (unsigned)top));
+        atomicAdd(&v.R.hitRangesUpper[lm], (int)__popcll(grp));  // hit count, checked by k_module_ranges
       }
       pending &= ~grp;
     }
@@ -110,9 +114,19 @@ __global__ void k_hits(EventView v) {
 }
 
 // SDL/Hit.h:194-215
-__global__ void k_module_ranges(EventView v) {
+__global__ void k_module_ranges(LST_VIEWS) {
+  LST_BIND;
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nModules; m += gridDim.x * blockDim.x) {
     const int hi = v.R.hitRanges[2 * m + 1];
+    // the hit count k_hits kept in hitRangesUpper[m] (read before this lane overwrites it; no other lane
+    // reads it): the reference's ranges need at most 127 hits per outer-tracker module, all contiguous
+    const int count = v.R.hitRangesUpper[m];
+    if (hi != -1) {
+      const int span = hi - v.R.hitRanges[2 * m] + 1;
+      uint32_t bad = count > 127 && m != v.pixelModule ? (uint32_t)kBadModuleHits : 0u;  // pLS pseudo-hits
+      bad |= count != span ? (uint32_t)kBadNonContiguous : 0u;
+      if (bad) atomicOr(&v.R.scalars[kBadHits], bad);
+    }
     int32_t lower = -1, upper = -1;
     int8_t nlo = -1, nup = -1;
     if (m < v.M.nLowerModules) {
@@ -129,7 +143,8 @@ __global__ void k_module_ranges(EventView v) {
     v.R.hitRangesnLower[m] = nlo; v.R.hitRangesnUpper[m] = nup;
   }
 }
-__global__ void k_hit_ranges_fix(EventView v) {
+__global__ void k_hit_ranges_fix(LST_VIEWS) {
+  LST_BIND;
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nModules; m += gridDim.x * blockDim.x)
     if (v.R.hitRanges[2 * m + 1] == -1) v.R.hitRanges[2 * m] = -1;
 }
@@ -166,15 +181,8 @@ __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, 
 
 // ------------------------------------------------------------- pixel segments
 // SDL/Segment.h:1031-1117 and addPixelSegmentToMemory SDL/Segment.h:392-452
-struct PixelInputs {
-  const uint32_t* h0;
-  const uint32_t* h1;
-  const uint32_t* h2;
-  const uint32_t* h3;
-  const float* dPhiChange;
-};
-
-__global__ void k_pixel_segments(EventView v, PixelInputs in) {
+__global__ void k_pixel_segments(LST_VIEWS) {
+  LST_BIND;
   const uint32_t pix = v.pixelModule;
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -185,7 +193,8 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nPLS; t += gridDim.x * blockDim.x) {
     const uint32_t innerMD = v.mdModuleIndices[pix] + 2 * t, outerMD = innerMD + 1;
     const uint32_t seg = v.segModuleIndices[pix] + t;
-    const uint32_t h0 = in.h0[t], h1 = in.h1[t], h2 = in.h2[t], h3 = in.h3[t];
+    const uint32_t h0 = v.plsInHitIdx[0][t], h1 = v.plsInHitIdx[1][t], h2 = v.plsInHitIdx[2][t],
+                   h3 = v.plsInHitIdx[3][t];
     store_md(v, h0, h1, pix, false, zero, innerMD);
     store_md(v, h2, h3, pix, false, zero, outerMD);
     // line through the PCA pseudo-hit with slope sinh(eta) and its residual at r3LH
@@ -196,7 +205,7 @@ __global__ void k_pixel_segments(EventView v, PixelInputs in) {
     v.S.mdIndices[2 * seg] = innerMD; v.S.mdIndices[2 * seg + 1] = outerMD;
     v.S.innerLowerModuleIndices[seg] = pix; v.S.outerLowerModuleIndices[seg] = pix;
     v.S.innerMiniDoubletAnchorHitIndices[seg] = h0; v.S.outerMiniDoubletAnchorHitIndices[seg] = h2;
-    v.S.dPhiChanges[seg] = in.dPhiChange[t];
+    v.S.dPhiChanges[seg] = v.plsInDPhiChange[t];
     v.P.isDup[t] = 0;
     v.P.partOfPT5[t] = 0;
     v.P.score[t] = score;
@@ -248,14 +257,15 @@ __device__ __forceinline__ void t3_range(const EventView& v, uint32_t m, uint32_
 // kMDCandMax hit pairs): candidates of module m are [candOff[m], candOff[m+1]),
 // pair (lower li, upper ui) at li * nUpper + ui.  A passing candidate parks its
 // results at its own index L, so no slot is taken with an atomic.
-__global__ __launch_bounds__(1024) void k_md_cand_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_md_cand_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const uint32_t nL = v.M.nLowerModules;
   const unsigned total = scan_wide<kScanPer>(
       nL,
       [&](uint32_t m) {  // the counts are int8 as in the reference (<= 127 hits)
         const int lo = v.R.hitRangesLower[m], nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
-        return (lo != -1) & (nLo > 0) & (nUp > 0) ? (unsigned)(nLo * nUp) : 0u;
+        return ((lo != -1) & (nLo > 0) & (nUp > 0)) ? (unsigned)(nLo * nUp) : 0u;
       },
       [&](uint32_t m, unsigned p, unsigned) { v.W2.candOff[m] = p; }, lds);
   if (threadIdx.x == 0) {
@@ -276,7 +286,8 @@ __device__ __forceinline__ uint32_t md_wave_index() {
 }
 
 // the candidates written out, L by L (one wave per module)
-__global__ __launch_bounds__(256) void k_md_cand_fill(EventView v) {
+__global__ __launch_bounds__(256) void k_md_cand_fill(LST_VIEWS) {
+  LST_BIND;
   if (!md_flat_mode(v)) return;
   const unsigned lane = lane_id();
   for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
@@ -288,7 +299,8 @@ __global__ __launch_bounds__(256) void k_md_cand_fill(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_md_eval(EventView v) {
+__global__ __launch_bounds__(256) void k_md_eval(LST_VIEWS) {
+  LST_BIND;
   if (!md_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kMDCandTotal];
   for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
@@ -317,12 +329,14 @@ __global__ __launch_bounds__(256) void k_md_eval(EventView v) {
 
 
 // slots: passing pairs in L order (pass-flag scan), then per-module counts
-__global__ __launch_bounds__(1024) void k_md_scan_blocks(EventView v) {
+__global__ __launch_bounds__(1024) void k_md_scan_blocks(LST_VIEWS) {
+  LST_BIND;
   if (!md_flat_done(v)) return;
   scan_flag_blocks(
       v.R.scalars[kMDCandTotal], [&](uint32_t L) { return v.W2.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
 }
-__global__ __launch_bounds__(1024) void k_md_scan_sums(EventView v) {
+__global__ __launch_bounds__(1024) void k_md_scan_sums(LST_VIEWS) {
+  LST_BIND;
   if (!md_flat_done(v)) return;
   scan_flag_sums(v.R.scalars[kMDCandTotal], v.FS.sums);
 }
@@ -337,8 +351,19 @@ __device__ __forceinline__ uint32_t module_of(const uint32_t* off, uint32_t n, u
   return lo;
 }
 
-__global__ __launch_bounds__(256) void k_md_place(EventView v) {
-  if (!md_flat_done(v)) return;
+__global__ __launch_bounds__(256) void k_md_place(LST_VIEWS) {
+  LST_BIND;
+  if (!md_flat_done(v)) {
+    // no per-module builder follows in a batch launch set: the stage is left empty (the event's overflow
+    // flag sends it to the per-event path)
+    if (v.noFallback)
+      for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+        v.R.totOccupancyMDs[m] = 0;
+        v.R.nMDs[m] = 0;
+        md_range(v, m, 0);
+      }
+    return;
+  }
   const uint32_t total = v.R.scalars[kMDCandTotal], nL = v.M.nLowerModules;
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
     // the loads that only need L, then (for a passing candidate) those that
@@ -368,7 +393,8 @@ __global__ __launch_bounds__(256) void k_md_place(EventView v) {
 
 // Per-module builder, used when an event exceeds the flattened path's arrays:
 // one wave per lower module walks its pairs in order.
-__global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
+__global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sx[256], sy[256], sz[256], srt[256];
   __shared__ unsigned lds[kBuildBlock / 64 + 1];
   if (md_flat_done(v)) return;
@@ -377,8 +403,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
   const int loBase = v.R.hitRangesLower[m];
   const int cap = v.mdModuleOccupancy[m];
   unsigned count = 0;
-  if (loBase != -1) {
-    const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+  const int nLo = v.R.hitRangesnLower[m], nUp = v.R.hitRangesnUpper[m];
+  // counts wrapped negative (more than 127 hits, an input error) build nothing, as the reference's
+  // index checks skip every pair then (SDL/MiniDoublet.h:919-923)
+  if (loBase != -1 && nLo > 0 && nUp > 0) {
     const int upBase = v.R.hitRangesUpper[m];
     for (int i = lane; i < nLo; i += kBuildBlock) {
       sx[i] = v.H.xs[loBase + i]; sy[i] = v.H.ys[loBase + i]; sz[i] = v.H.zs[loBase + i]; srt[i] = v.H.rts[loBase + i];
@@ -424,7 +452,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(EventView v) {
 // Flattened builder, as for the mini-doublets.  Per module, connPre[k] is the
 // first candidate of connected module k (nIn * nMD(k) candidates each, inner
 // MD major), and candOff the module's first global candidate.
-__global__ __launch_bounds__(256) void k_ls_cand_count(EventView v) {
+__global__ __launch_bounds__(256) void k_ls_cand_count(LST_VIEWS) {
+  LST_BIND;
   const unsigned lane = lane_id();
   const uint32_t S = v.M.stride + 1;
   for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
@@ -445,7 +474,8 @@ __global__ __launch_bounds__(256) void k_ls_cand_count(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_ls_cand_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_ls_cand_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const uint32_t nL = v.M.nLowerModules;
   const unsigned total = scan_wide<kScanPer>(
@@ -467,7 +497,8 @@ __device__ __forceinline__ bool ls_flat_done(const EventView& v) {
 
 // the candidates written out, L by L (one wave per module; the connected
 // modules' values are loaded by the lanes at once, then walked with shuffles)
-__global__ __launch_bounds__(256) void k_ls_cand_fill(EventView v) {
+__global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
+  LST_BIND;
   if (!ls_flat_mode(v)) return;
   const unsigned lane = lane_id();
   const uint32_t S = v.M.stride + 1;
@@ -496,7 +527,8 @@ __global__ __launch_bounds__(256) void k_ls_cand_fill(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
+__global__ __launch_bounds__(256) void k_ls_eval(LST_VIEWS) {
+  LST_BIND;
   if (!ls_flat_mode(v)) return;
   const uint32_t total = v.R.scalars[kLSCandTotal];
   for (uint32_t L0 = blockIdx.x * blockDim.x; L0 < total; L0 += gridDim.x * blockDim.x) {
@@ -522,18 +554,31 @@ __global__ __launch_bounds__(256) void k_ls_eval(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_ls_scan_blocks(EventView v) {
+__global__ __launch_bounds__(1024) void k_ls_scan_blocks(LST_VIEWS) {
+  LST_BIND;
   if (!ls_flat_done(v)) return;
   scan_flag_blocks(
       v.R.scalars[kLSCandTotal], [&](uint32_t L) { return v.WL.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
 }
-__global__ __launch_bounds__(1024) void k_ls_scan_sums(EventView v) {
+__global__ __launch_bounds__(1024) void k_ls_scan_sums(LST_VIEWS) {
+  LST_BIND;
   if (!ls_flat_done(v)) return;
   scan_flag_sums(v.R.scalars[kLSCandTotal], v.FS.sums);
 }
 
-__global__ __launch_bounds__(256) void k_ls_place(EventView v) {
-  if (!ls_flat_done(v)) return;
+__global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
+  LST_BIND;
+  if (!ls_flat_done(v)) {
+    // no per-module builder follows in a batch launch set: the stage is left empty (the event's overflow
+    // flag sends it to the per-event path)
+    if (v.noFallback)
+      for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+        v.R.totOccupancySegments[m] = 0;
+        v.R.nSegments[m] = 0;
+        seg_range(v, m, 0);
+      }
+    return;
+  }
   const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
   for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
     // the loads that only need L, then those of a passing candidate, each
@@ -568,7 +613,8 @@ __global__ __launch_bounds__(256) void k_ls_place(EventView v) {
 }
 
 // Per-module builder, used when an event exceeds the flattened path's arrays.
-__global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
+__global__ __launch_bounds__(kBuildBlock) void k_segments(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t pre[160];
   __shared__ uint16_t outMod[160];
   __shared__ uint32_t outCnt[160];
@@ -647,7 +693,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(EventView v) {
 // -------------------------------------------------------------------- triplets
 // capacity offsets: a module with segments gets its tabulated capacity, others 0
 // (SDL/Triplet.h:1007-1102); exclusive scan in module order.
-__global__ __launch_bounds__(1024) void k_triplet_offsets(EventView v) {
+__global__ __launch_bounds__(1024) void k_triplet_offsets(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const unsigned total = scan_wide<kScanPer>(
       v.M.nLowerModules,
@@ -734,7 +781,8 @@ constexpr unsigned kModuleWaveBlocks = 1024;  // x 4 waves
 // per inner segment: its candidates (the outer module's segments, which
 // number them in serial order) and its shared-MD candidates (the segments
 // listed under its outer mini-doublet); per module, both totals
-__global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
+__global__ __launch_bounds__(256) void k_t3_cand_count(LST_VIEWS) {
+  LST_BIND;
   const unsigned lane = lane_id();
   for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
     const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
@@ -763,7 +811,8 @@ __global__ __launch_bounds__(256) void k_t3_cand_count(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_t3_cand_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[34];
   const uint32_t n = v.M.nLowerModules;
   const UVec<2> tot = scan_wide_n<kScanPer, 2>(
@@ -792,7 +841,8 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(EventView v) {
 // segments and writes, for each segment listed under X, the candidate at the
 // position those counts give.  The list is thereby in the reference's serial
 // order (module, inner segment, outer segment).
-__global__ __launch_bounds__(256) void k_md_seg_lists(EventView v) {
+__global__ __launch_bounds__(256) void k_md_seg_lists(LST_VIEWS) {
+  LST_BIND;
   __shared__ __align__(16) ListLds sLists[4];
   ListLds& sX = sLists[threadIdx.x / kWave];
   const uint32_t nL = v.M.nLowerModules;
@@ -806,7 +856,8 @@ __global__ __launch_bounds__(256) void k_md_seg_lists(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_t3_enum(EventView v) {
+__global__ __launch_bounds__(256) void k_t3_enum(LST_VIEWS) {
+  LST_BIND;
   if (!t3_flat_mode(v)) return;
   const unsigned lane = lane_id();
   for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
@@ -828,7 +879,8 @@ __global__ __launch_bounds__(256) void k_t3_enum(EventView v) {
 }
 
 // stage 2: the triplet selection; results parked at the list position
-__global__ __launch_bounds__(256) void k_t3_select(EventView v) {
+__global__ __launch_bounds__(256) void k_t3_select(LST_VIEWS) {
+  LST_BIND;
   if (!t3_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT3List];
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
@@ -852,19 +904,32 @@ __global__ __launch_bounds__(256) void k_t3_select(EventView v) {
 // stage 3: the list is in serial order (modules, inner segments and each
 // mini-doublet's segments in order), so per module the passing entries take
 // the slots in list order: a pass-flag scan over the list positions
-__global__ __launch_bounds__(1024) void k_t3_scan_blocks(EventView v) {
+__global__ __launch_bounds__(1024) void k_t3_scan_blocks(LST_VIEWS) {
+  LST_BIND;
   if (!t3_flat_done(v)) return;
   scan_flag_blocks(
       v.R.scalars[kT3List], [&](uint32_t k) { return v.W3.resIdx[k] != ~0u; }, v.FS.pre, v.FS.sums);
 }
-__global__ __launch_bounds__(1024) void k_t3_scan_sums(EventView v) {
+__global__ __launch_bounds__(1024) void k_t3_scan_sums(LST_VIEWS) {
+  LST_BIND;
   if (!t3_flat_done(v)) return;
   scan_flag_sums(v.R.scalars[kT3List], v.FS.sums);
 }
 
 // passing entries take their slots in list order; then per-module counts
-__global__ __launch_bounds__(256) void k_t3_place(EventView v) {
-  if (!t3_flat_done(v)) return;
+__global__ __launch_bounds__(256) void k_t3_place(LST_VIEWS) {
+  LST_BIND;
+  if (!t3_flat_done(v)) {
+    // no per-module builder follows in a batch launch set: the stage is left empty (the event's overflow
+    // flag sends it to the per-event path)
+    if (v.noFallback)
+      for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < v.M.nLowerModules; m += gridDim.x * blockDim.x) {
+        v.R.totOccupancyTriplets[m] = 0;
+        v.R.nTriplets[m] = 0;
+        t3_range(v, m, 0);
+      }
+    return;
+  }
   const uint32_t total = v.R.scalars[kT3List], nL = v.M.nLowerModules;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
     if (v.W3.resIdx[k] == ~0u) continue;
@@ -885,7 +950,8 @@ __global__ __launch_bounds__(256) void k_t3_place(EventView v) {
 
 // Per-module builder, used when an event exceeds the flattened path's lists:
 // one wave per lower module walks its candidates in order.
-__global__ __launch_bounds__(kWave) void k_triplets(EventView v) {
+__global__ __launch_bounds__(kWave) void k_triplets(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t pre[1025];
   if (t3_flat_done(v)) return;
   const uint32_t m1 = blockIdx.x;
@@ -958,12 +1024,9 @@ __global__ __launch_bounds__(256) void k_copy_multi(CopyList c) {
 
 __global__ void k_set_scalar(uint32_t* p, uint32_t value) { *p = value; }
 
-struct PlsArena {
-  uint32_t* hitIdx[4];
-  float* dPhiChange;
-};
-
-__global__ __launch_bounds__(256) void k_stage_in(EventView v, const InputDesc* __restrict__ d, PlsArena a) {
+__global__ __launch_bounds__(256) void k_stage_in(LST_VIEWS, const InputDesc* __restrict__ ds) {
+  LST_BIND;
+  const InputDesc* d = ds + blockIdx.z;
   const uint32_t nh = d->nHits, np = d->nPLS;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     v.R.scalars[kEvNHits] = nh;
@@ -976,8 +1039,8 @@ __global__ __launch_bounds__(256) void k_stage_in(EventView v, const InputDesc* 
   }
   for (uint32_t i = t0; i < np; i += stride) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) a.hitIdx[k][i] = d->hitIdx[k][i];
-    a.dPhiChange[i] = d->dPhiChange[i];
+    for (int k = 0; k < 4; k++) v.plsInHitIdx[k][i] = d->hitIdx[k][i];
+    v.plsInDPhiChange[i] = d->dPhiChange[i];
     v.P.ptIn[i] = d->ptIn[i]; v.P.ptErr[i] = d->ptErr[i];
     v.P.px[i] = d->px[i]; v.P.py[i] = d->py[i]; v.P.pz[i] = d->pz[i];
     v.P.eta[i] = d->eta[i]; v.P.etaErr[i] = d->etaErr[i]; v.P.phi[i] = d->phi[i];
@@ -986,10 +1049,10 @@ __global__ __launch_bounds__(256) void k_stage_in(EventView v, const InputDesc* 
   }
 }
 
-void launch_stage_in(const EventView& v, const InputDesc* d, uint32_t* const plsHitIdx[4], float* plsDPhiChange,
-                     hipStream_t s) {
-  const PlsArena a{{plsHitIdx[0], plsHitIdx[1], plsHitIdx[2], plsHitIdx[3]}, plsDPhiChange};
-  hipLaunchKernelGGL(k_stage_in, dim3(512), dim3(256), 0, s, v, d, a);
+void launch_stage_in(const Launch& L, const InputDesc* d) {
+  L.tune_begin(kTnStageIn);
+  hipLaunchKernelGGL(k_stage_in, dim3(L.gx(kTnStageIn, 512), 1, L.nEv), dim3(256), 0, L.s, L.dv, d);
+  L.tune_end(kTnStageIn);
 }
 
 void launch_set_scalar(uint32_t* p, uint32_t value, hipStream_t s) {
@@ -1005,63 +1068,65 @@ void launch_copy_multi(const CopyList& c, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_multi, dim3(gx, c.n), dim3(256), 0, s, c);
 }
 
-void launch_hits(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_event_init, dim3(grid_for(2 * v.M.nModules, 256)), dim3(256), 0, s, v);
-  t.begin(s, kTmHits);
-  hipLaunchKernelGGL(k_hits, dim3(1024), dim3(256), 0, s, v);
-  t.end(s, kTmHits);
-  hipLaunchKernelGGL(k_module_ranges, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_hit_ranges_fix, dim3(grid_for(v.M.nModules, 256)), dim3(256), 0, s, v);
+void launch_hits(const Launch& L) {
+  const EventView& v = L.v;
+  LST_GO_T(L, kTnEventInit, k_event_init, grid_for(2 * v.M.nModules, 256), 256);
+  L.t.begin(L.s, kTmHits);
+  LST_GO_T(L, kTnHits, k_hits, 1024, 256);
+  L.t.end(L.s, kTmHits);
+  LST_GO_T(L, kTnModuleRanges, k_module_ranges, grid_for(v.M.nModules, 256), 256);
+  LST_GO_T(L, kTnHitRangesFix, k_hit_ranges_fix, grid_for(v.M.nModules, 256), 256);
 }
 
-void launch_pixel_segments(const EventView& v, const uint32_t* hitIdx[4], const float* dPhiChange, hipStream_t s,
-                           const KernelTimers& t) {
-  PixelInputs in{hitIdx[0], hitIdx[1], hitIdx[2], hitIdx[3], dPhiChange};
-  t.begin(s, kTmPixelSegments);
-  hipLaunchKernelGGL(k_pixel_segments, dim3(196), dim3(256), 0, s, v, in);
-  t.end(s, kTmPixelSegments);
+void launch_pixel_segments(const Launch& L) {
+  L.t.begin(L.s, kTmPixelSegments);
+  LST_GO_T(L, kTnPixelSegments, k_pixel_segments, 196, 256);
+  L.t.end(L.s, kTmPixelSegments);
 }
 
-void launch_mini_doublets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_md_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_cand_fill, dim3(1024), dim3(256), 0, s, v);
-  t.begin(s, kTmMiniDoublets);
-  hipLaunchKernelGGL(k_md_eval, dim3(2048), dim3(256), 0, s, v);
-  t.end(s, kTmMiniDoublets);
-  hipLaunchKernelGGL(k_md_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_scan_sums, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_place, dim3(2048), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_mini_doublets, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+void launch_mini_doublets(const Launch& L) {
+  const EventView& v = L.v;
+  LST_GO(L, k_md_cand_scan, 1, 1, 1024);
+  LST_GO_T(L, kTnMdCandFill, k_md_cand_fill, 1024, 256);
+  L.t.begin(L.s, kTmMiniDoublets);
+  LST_GO_T(L, kTnMdEval, k_md_eval, 2048, 256);
+  L.t.end(L.s, kTmMiniDoublets);
+  LST_GO_T(L, kTnMdScanBlocks, k_md_scan_blocks, kScanGrid, 1024);
+  LST_GO(L, k_md_scan_sums, 1, 1, 1024);
+  LST_GO_T(L, kTnMdPlace, k_md_place, 2048, 256);
+  if (L.fallbacks) LST_GO(L, k_mini_doublets, v.M.nLowerModules, 1, kBuildBlock);
 }
 
-void launch_segments(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_ls_cand_count, dim3(1024), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_ls_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_ls_cand_fill, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  t.begin(s, kTmSegments);
-  hipLaunchKernelGGL(k_ls_eval, dim3(4096), dim3(256), 0, s, v);
-  t.end(s, kTmSegments);
-  hipLaunchKernelGGL(k_ls_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_ls_scan_sums, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_ls_place, dim3(2048), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_segments, dim3(v.M.nLowerModules), dim3(kBuildBlock), 0, s, v);
+void launch_segments(const Launch& L) {
+  const EventView& v = L.v;
+  LST_GO_T(L, kTnLsCandCount, k_ls_cand_count, 1024, 256);
+  LST_GO(L, k_ls_cand_scan, 1, 1, 1024);
+  LST_GO_T(L, kTnLsCandFill, k_ls_cand_fill, kModuleWaveBlocks, 256);
+  L.t.begin(L.s, kTmSegments);
+  LST_GO_T(L, kTnLsEval, k_ls_eval, 4096, 256);
+  L.t.end(L.s, kTmSegments);
+  LST_GO_T(L, kTnLsScanBlocks, k_ls_scan_blocks, kScanGrid, 1024);
+  LST_GO(L, k_ls_scan_sums, 1, 1, 1024);
+  LST_GO_T(L, kTnLsPlace, k_ls_place, 2048, 256);
+  if (L.fallbacks) LST_GO(L, k_segments, v.M.nLowerModules, 1, kBuildBlock);
 }
 
-void launch_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_triplet_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_seg_lists, dim3((v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t3_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t3_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  t.begin(s, kTmT3Filter);
-  hipLaunchKernelGGL(k_t3_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  t.end(s, kTmT3Filter);
-  t.begin(s, kTmT3Select);
-  hipLaunchKernelGGL(k_t3_select, dim3(2048), dim3(256), 0, s, v);
-  t.end(s, kTmT3Select);
-  hipLaunchKernelGGL(k_t3_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_scan_sums, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_place, dim3(2048), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_triplets, dim3(v.M.nLowerModules), dim3(kWave), 0, s, v);
+void launch_triplets(const Launch& L) {
+  const EventView& v = L.v;
+  LST_GO(L, k_triplet_offsets, 1, 1, 1024);
+  LST_GO_T(L, kTnMdSegLists, k_md_seg_lists, (v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1, 256);
+  LST_GO_T(L, kTnT3CandCount, k_t3_cand_count, kModuleWaveBlocks, 256);
+  LST_GO(L, k_t3_cand_scan, 1, 1, 1024);
+  L.t.begin(L.s, kTmT3Filter);
+  LST_GO_T(L, kTnT3Enum, k_t3_enum, kModuleWaveBlocks, 256);
+  L.t.end(L.s, kTmT3Filter);
+  L.t.begin(L.s, kTmT3Select);
+  LST_GO_T(L, kTnT3Select, k_t3_select, 2048, 256);
+  L.t.end(L.s, kTmT3Select);
+  LST_GO_T(L, kTnT3ScanBlocks, k_t3_scan_blocks, kScanGrid, 1024);
+  LST_GO(L, k_t3_scan_sums, 1, 1, 1024);
+  LST_GO_T(L, kTnT3Place, k_t3_place, 2048, 256);
+  if (L.fallbacks) LST_GO(L, k_triplets, v.M.nLowerModules, 1, kWave);
 }
 
 }  // namespace lst

@@ -30,23 +30,9 @@ inline unsigned grid_of(uint32_t n, unsigned block) {
   return g == 0 ? 1 : (g > 8192 ? 8192 : g);
 }
 
-// exclusive scan of counts[0, *count) in place; counts[*count] and *total get
-// the sum, *kept the sum clamped to cap
-__global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t* counts, const uint32_t* count, uint32_t* total,
-                                                      uint32_t cap, uint32_t* kept) {
-  __shared__ unsigned lds[17];
-  const uint32_t n = *count;
-  const unsigned sum = scan_wide<kScanPer>(
-      n, [&](uint32_t i) { return counts[i]; }, [&](uint32_t i, unsigned p, unsigned) { counts[i] = p; }, lds);
-  if (threadIdx.x == 0) {
-    counts[n] = sum;
-    *total = sum;
-    *kept = sum < cap ? sum : cap;
-  }
-}
-
 // connected-module range of every pLS in the pixel map (SDL/Event.cc:828-856)
-__global__ void k_pls_connected(EventView v) {
+__global__ void k_pls_connected(LST_VIEWS) {
+  LST_BIND;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     v.R.scalars[kPT5Survivors] = 0;
     v.R.scalars[kPT3Survivors] = 0;
@@ -163,6 +149,8 @@ struct PT5Kind {
   using Res = PT5Result;
   static constexpr uint32_t kBit = 1, kCap = kNMaxPixelQuintuplets;
   static constexpr int kSurvScalar = kPT5Survivors;
+  static constexpr int kTotScalar = kTotOccPixelQuintuplets, kKeptScalar = kNPixelQuintuplets;
+  static constexpr int kTnA = kTnPT5StageA, kTnB = kTnPT5StageB, kTnWrite = kTnPT5Write, kTnFull = kTnPT5FullWrite;
   __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt5Mask; }
   __device__ static uint32_t* shard(const EventView& v) { return v.W.pt5Shard; }
   __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt5MaskBase; }
@@ -236,6 +224,8 @@ struct PT3Kind {
   using Res = PT3Result;
   static constexpr uint32_t kBit = 2, kCap = kNMaxPixelTriplets;
   static constexpr int kSurvScalar = kPT3Survivors;
+  static constexpr int kTotScalar = kTotOccPixelTriplets, kKeptScalar = kNPixelTriplets;
+  static constexpr int kTnA = kTnPT3StageA, kTnB = kTnPT3StageB, kTnWrite = kTnPT3Write, kTnFull = kTnPT3FullWrite;
   __host__ __device__ static unsigned long long* mask(const EventView& v) { return v.W.pt3Mask; }
   __device__ static uint32_t* shard(const EventView& v) { return v.W.pt3Shard; }
   __host__ __device__ static uint32_t* mask_base(const EventView& v) { return v.W.pt3MaskBase; }
@@ -316,7 +306,8 @@ struct PT3Kind {
 
 // T3 slots of the modules the pixel-triplet builder connects to, module by
 // module (offsets: one 1024-lane work-group; fill: one wave per module)
-__global__ __launch_bounds__(1024) void k_t3_list_offsets(EventView v) {
+__global__ __launch_bounds__(1024) void k_t3_list_offsets(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const unsigned total = scan_wide<kScanPer>(
       v.M.nLowerModules,
@@ -328,7 +319,8 @@ __global__ __launch_bounds__(1024) void k_t3_list_offsets(EventView v) {
       [&](uint32_t m, unsigned p, unsigned) { v.W.t3ListOff[m] = p; }, lds);
   if (threadIdx.x == 0) v.R.scalars[kT3Listed] = total;
 }
-__global__ __launch_bounds__(256) void k_t3_list_fill(EventView v) {
+__global__ __launch_bounds__(256) void k_t3_list_fill(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nL = v.M.nLowerModules;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; m < nL; m += waves) {
@@ -341,7 +333,8 @@ __global__ __launch_bounds__(256) void k_t3_list_fill(EventView v) {
 
 // the records: one lane per listed T5 (those of modules that are not 2S),
 // then one per listed T3
-__global__ __launch_bounds__(256) void k_pix_records(EventView v) {
+__global__ __launch_bounds__(256) void k_pix_records(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n5 = v.R.scalars[kT5Listed], n3 = v.R.scalars[kT3Listed];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n5 + n3; k += gridDim.x * blockDim.x) {
     if (k < n5) {
@@ -355,8 +348,27 @@ __global__ __launch_bounds__(256) void k_pix_records(EventView v) {
   }
 }
 
+// exclusive scan of the per-pLS pass counts of builder K in place; the entry
+// after the last pLS and the total scalar get the sum, the kept scalar the sum
+// clamped to the capacity
 template <class K>
-__global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
+__global__ __launch_bounds__(1024) void k_scan_inplace(LST_VIEWS) {
+  LST_BIND;
+  __shared__ unsigned lds[17];
+  uint32_t* counts = K::count(v);
+  const uint32_t n = v.R.scalars[kEvNPLS];
+  const unsigned sum = scan_wide<kScanPer>(
+      n, [&](uint32_t i) { return counts[i]; }, [&](uint32_t i, unsigned p, unsigned) { counts[i] = p; }, lds);
+  if (threadIdx.x == 0) {
+    counts[n] = sum;
+    v.R.scalars[K::kTotScalar] = sum;
+    v.R.scalars[K::kKeptScalar] = sum < K::kCap ? sum : K::kCap;
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(kPixWG) void k_pix_stage_a(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t pre[kMaxConn + 1];
   __shared__ uint32_t objBase[kMaxConn];
   __shared__ uint32_t sMaskBase, sFull;
@@ -428,7 +440,8 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(EventView v) {
 // skipped here, and when any pLS is flagged the same waves then evaluate every
 // candidate of each flagged pLS (its count only; the write evaluates again)
 template <class K>
-__global__ __launch_bounds__(kPixWG) void k_pix_stage_b(EventView v) {
+__global__ __launch_bounds__(kPixWG) void k_pix_stage_b(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t pre[kMaxConn + 1];  // survivors before each shard, then the connected-module prefix
   __shared__ uint32_t objBase[kMaxConn];
   __shared__ uint32_t sPass[kPixWG / kWave / 2][kWave];
@@ -518,7 +531,8 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_b(EventView v) {
 // every fourth word and its lanes write the word's passing candidates (the
 // lanes of the set bits) to their slots, in candidate order.
 template <class K>
-__global__ __launch_bounds__(kPixWG) void k_pix_write(EventView v) {
+__global__ __launch_bounds__(kPixWG) void k_pix_write(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned long long sMask[kPixWG];
   __shared__ uint32_t sPre[kPixWG];
   __shared__ unsigned lds[kPixWG / kWave + 1];
@@ -560,7 +574,8 @@ __global__ __launch_bounds__(kPixWG) void k_pix_write(EventView v) {
 
 // the flagged pLS: every candidate evaluated again and written to its slot
 template <class K>
-__global__ __launch_bounds__(kWave) void k_pix_full_write(EventView v) {
+__global__ __launch_bounds__(kWave) void k_pix_full_write(LST_VIEWS) {
+  LST_BIND;
   if ((v.R.scalars[kPixFull] & K::kBit) == 0) return;
   __shared__ uint32_t pre[kMaxConn + 1];
   __shared__ uint32_t objBase[kMaxConn];
@@ -638,7 +653,8 @@ __device__ __forceinline__ void add_tc(const TCSoA& C, uint32_t idx, int16_t typ
 }
 
 // addpT5asTrackCandidateInGPU (SDL/TrackCandidate.h:532-580): the first TCs
-__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt5(EventView v) {
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt5(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kNPixelQuintuplets];
   if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
@@ -664,7 +680,8 @@ __global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt5(EventView v) {
 
 // ---------------------------------------------------------- track candidates
 // addpT3asTrackCandidatesInGPU (SDL/TrackCandidate.h:381-428)
-__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt3(EventView v) {
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pt3(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kNPixelTriplets];
   if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t base = v.R.scalars[kNTrackCandidates];
@@ -700,7 +717,8 @@ __device__ __forceinline__ uint32_t wave_index() {
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / kWave; }
 constexpr unsigned kModuleWaveBlocks = 512;  // x 4 waves
 
-__global__ __launch_bounds__(256) void k_tc_t5_count(EventView v) {
+__global__ __launch_bounds__(256) void k_tc_t5_count(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
     const uint16_t m = v.R.indicesOfEligibleT5Modules[e];
@@ -717,7 +735,8 @@ __global__ __launch_bounds__(256) void k_tc_t5_count(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_tc_t5_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_tc_t5_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[17];
   const unsigned total = scan_wide<kScanPer>(
       v.R.scalars[kNEligibleT5], [&](uint32_t i) { return v.W.t5TcCount[i]; },
@@ -729,7 +748,8 @@ __global__ __launch_bounds__(1024) void k_tc_t5_scan(EventView v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_tc_t5_write(EventView v) {
+__global__ __launch_bounds__(256) void k_tc_t5_write(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   const uint32_t tcBase = v.R.scalars[kNTrackCandidatespT5] + v.R.scalars[kNTrackCandidatespT3];
   for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
@@ -754,7 +774,8 @@ __global__ __launch_bounds__(256) void k_tc_t5_write(EventView v) {
 
 // addpLSasTrackCandidateInGPU (SDL/TrackCandidate.h:484-530) with
 // addpLSTrackCandidateToMemory (:112-124): hits stored in (x, z, y, w) order
-__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pls(EventView v) {
+__global__ __launch_bounds__(kCompactBlock) void k_tc_add_pls(LST_VIEWS) {
+  LST_BIND;
   const uint32_t n = v.R.scalars[kEvNPLS];
   if (blockIdx.x * kCompactBlock >= n && blockIdx.x != 0) return;
   const uint32_t base = v.R.scalars[kNTrackCandidates];
@@ -800,9 +821,26 @@ __global__ __launch_bounds__(kCompactBlock) void k_tc_add_pls(EventView v) {
 // repeated fourth hit of a triplet seed dropped
 // the output hit lists (outHits, outLen), and the same with the seed index and
 // type as one packed block (outPacked) that a batch copies to the host at once
-__global__ void k_tc_output(EventView v) {
+__global__ void k_tc_output(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nTC = v.R.scalars[kNTrackCandidates];
-  if (blockIdx.x == 0 && threadIdx.x == 0) v.W.outPacked[0] = nTC;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // header: the count, then which flattened builders overflowed, which capacities truncated and which
+    // inputs were bad (the truncation bits are final: k_tc_prep ran before)
+    const uint32_t* sc = v.R.scalars;
+    uint32_t ovf = v.perModuleBuilders ? (uint32_t)kOvfPerModule : 0u;
+    ovf |= sc[kMDCandTotal] > kMDCandMax ? (uint32_t)kOvfMD : 0u;
+    ovf |= sc[kLSCandTotal] > kLSCandMax ? (uint32_t)kOvfLS : 0u;
+    ovf |= sc[kT3CandTotal] > kT3CandMax ? (uint32_t)kOvfT3 : 0u;
+    ovf |= (sc[kT5CandTotal] > kT5CandMax || sc[kT5Results] > kT5ResultMax) ? (uint32_t)kOvfT5 : 0u;
+    uint32_t trunc = sc[kTruncated];
+    trunc |= sc[kTotOccPixelQuintuplets] > sc[kNPixelQuintuplets] ? (uint32_t)kTruncPT5 : 0u;
+    trunc |= sc[kTotOccPixelTriplets] > sc[kNPixelTriplets] ? (uint32_t)kTruncPT3 : 0u;
+    v.W.outPacked[kOutNTC] = nTC;
+    v.W.outPacked[kOutOverflow] = ovf;
+    v.W.outPacked[kOutTruncated] = trunc;
+    v.W.outPacked[kOutBadHits] = sc[kBadHits];
+  }
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nTC; t += gridDim.x * blockDim.x) {
     const int16_t type = v.C.trackCandidateType[t];
     const int maxN = type == 7 ? 14 : (type == 5 || type == 4) ? 10 : type == 8 ? 4 : 0;
@@ -831,67 +869,84 @@ __global__ void k_tc_output(EventView v) {
       last = idx[i];
       len++;
     }
-    uint32_t* rec = v.W.outPacked + kOutHeaderWords + (size_t)kOutRecWords * t;
 #pragma unroll
-    for (int i = 0; i < 14; i++) {
-      v.W.outHits[14 * t + i] = hits[i];
-      rec[i] = hits[i];
-    }
+    for (int i = 0; i < 14; i++) v.W.outHits[14 * t + i] = hits[i];
     v.W.outLen[t] = len;
-    rec[14] = len;
-    rec[15] = (uint32_t)v.C.pixelSeedIndex[t];
-    rec[16] = (uint32_t)(int32_t)type;
+    if (t < v.outCap) {  // the packed block (possibly host memory) holds outCap records
+      uint32_t* rec = v.W.outPacked + kOutHeaderWords + (size_t)kOutRecWords * t;
+#pragma unroll
+      for (int i = 0; i < 14; i++) rec[i] = hits[i];
+      rec[14] = len;
+      rec[15] = (uint32_t)v.C.pixelSeedIndex[t];
+      rec[16] = (uint32_t)(int32_t)type;
+    }
   }
+}
+
+// the packed output block of each event to its descriptor's destination,
+// 16 bytes per lane (the block and the destination are 256-byte aligned)
+__global__ __launch_bounds__(256) void k_out_copy(LST_VIEWS, const InputDesc* __restrict__ ds) {
+  LST_BIND;
+  const InputDesc& d = ds[blockIdx.z];
+  uint32_t* dst = d.outHost;
+  if (!dst) return;
+  const uint32_t nTC = v.W.outPacked[kOutNTC];
+  const uint32_t words = kOutHeaderWords + (nTC < d.outCap ? nTC : d.outCap) * kOutRecWords;
+  const uint4* src4 = reinterpret_cast<const uint4*>(v.W.outPacked);
+  uint4* dst4 = reinterpret_cast<uint4*>(dst);
+  const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (uint32_t i = t0; i < words / 4; i += stride) dst4[i] = src4[i];
+  if (t0 < (words & 3u)) dst[(words & ~3u) + t0] = v.W.outPacked[(words & ~3u) + t0];
 }
 
 }  // namespace
 
-void launch_pixel_segment_cleaning(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  launch_pls_dup(v, s, false, t);
-}
+void launch_pixel_segment_cleaning(const Launch& L) { launch_pls_dup(L, false); }
 
 template <class K>
-void launch_pixel_builder(const EventView& v, hipStream_t s, const KernelTimers& t, int timer, uint32_t* countScalar,
-                          uint32_t* keptScalar) {
-  t.begin(s, timer);
-  hipLaunchKernelGGL(k_pix_stage_a<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
-  t.end(s, timer);
-  hipLaunchKernelGGL(k_pix_stage_b<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
-  hipLaunchKernelGGL(k_scan_inplace, dim3(1), dim3(1024), 0, s, K::count(v), v.R.scalars + kEvNPLS, countScalar,
-                     K::kCap, keptScalar);
-  hipLaunchKernelGGL(k_pix_write<K>, dim3(kPixelWaves), dim3(kPixWG), 0, s, v);
-  hipLaunchKernelGGL(k_pix_full_write<K>, dim3(kPixelFullWaves), dim3(kWave), 0, s, v);
+void launch_pixel_builder(const Launch& L, int timer) {
+  L.t.begin(L.s, timer);
+  LST_GO_T(L, K::kTnA, k_pix_stage_a<K>, kPixelWaves, kPixWG);
+  L.t.end(L.s, timer);
+  LST_GO_T(L, K::kTnB, k_pix_stage_b<K>, kPixelWaves, kPixWG);
+  LST_GO(L, k_scan_inplace<K>, 1, 1, 1024);
+  LST_GO_T(L, K::kTnWrite, k_pix_write<K>, kPixelWaves, kPixWG);
+  LST_GO_T(L, K::kTnFull, k_pix_full_write<K>, kPixelFullWaves, kWave);
 }
 
-void launch_pixel_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  hipLaunchKernelGGL(k_pls_connected, dim3(grid_of(kNMaxPixelSegments, 256)), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t3_list_offsets, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t3_list_fill, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_pix_records, dim3(256), dim3(256), 0, s, v);
-  launch_pixel_builder<PT5Kind>(v, s, t, kTmPT5Build, v.R.scalars + kTotOccPixelQuintuplets,
-                                v.R.scalars + kNPixelQuintuplets);
-  launch_pt5_dup(v, s, t);
-  hipLaunchKernelGGL(k_tc_add_pt5, dim3(compact_grid(kNMaxPixelQuintuplets)), dim3(kCompactBlock), 0, s, v);
+void launch_pixel_quintuplets(const Launch& L) {
+  LST_GO_T(L, kTnPlsConnected, k_pls_connected, grid_of(kNMaxPixelSegments, 256), 256);
+  LST_GO(L, k_t3_list_offsets, 1, 1, 1024);
+  LST_GO_T(L, kTnT3ListFill, k_t3_list_fill, kModuleWaveBlocks, 256);
+  LST_GO_T(L, kTnPixRecords, k_pix_records, 256, 256);
+  launch_pixel_builder<PT5Kind>(L, kTmPT5Build);
+  launch_pt5_dup(L);
+  LST_GO(L, k_tc_add_pt5, compact_grid(kNMaxPixelQuintuplets), 1, kCompactBlock);
 }
 
-void launch_pixel_triplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  launch_pixel_builder<PT3Kind>(v, s, t, kTmPT3Build, v.R.scalars + kTotOccPixelTriplets,
-                                v.R.scalars + kNPixelTriplets);
-  launch_pt3_dup(v, s, t);
+void launch_pixel_triplets(const Launch& L) {
+  launch_pixel_builder<PT3Kind>(L, kTmPT3Build);
+  launch_pt3_dup(L);
 }
 
-void launch_track_candidates(const EventView& v, hipStream_t s, const KernelTimers& t) {
-  launch_cross_pt3(v, s);
-  hipLaunchKernelGGL(k_tc_add_pt3, dim3(compact_grid(kNMaxPixelTriplets)), dim3(kCompactBlock), 0, s, v);
-  launch_t5_dup_before_tc(v, s, t);
-  launch_cross_t5(v, s);
-  hipLaunchKernelGGL(k_tc_t5_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_tc_t5_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_tc_t5_write, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  launch_pls_dup(v, s, true, t);
-  launch_cross_pls(v, s, t);
-  hipLaunchKernelGGL(k_tc_add_pls, dim3(compact_grid(kNMaxPixelSegments)), dim3(kCompactBlock), 0, s, v);
-  hipLaunchKernelGGL(k_tc_output, dim3(grid_of(kTCCap, 256)), dim3(256), 0, s, v);
+void launch_track_candidates(const Launch& L) {
+  launch_cross_pt3(L);
+  LST_GO(L, k_tc_add_pt3, compact_grid(kNMaxPixelTriplets), 1, kCompactBlock);
+  launch_t5_dup_before_tc(L);
+  launch_cross_t5(L);
+  LST_GO_T(L, kTnTcT5Count, k_tc_t5_count, kModuleWaveBlocks, 256);
+  LST_GO(L, k_tc_t5_scan, 1, 1, 1024);
+  LST_GO_T(L, kTnTcT5Write, k_tc_t5_write, kModuleWaveBlocks, 256);
+  launch_pls_dup(L, true);
+  launch_cross_pls(L);
+  LST_GO(L, k_tc_add_pls, compact_grid(kNMaxPixelSegments), 1, kCompactBlock);
+  LST_GO_T(L, kTnTcOutput, k_tc_output, grid_of(kTCCap, 256), 256);
+}
+
+void launch_out_copy(const Launch& L, const InputDesc* d) {
+  L.tune_begin(kTnOutCopy);
+  hipLaunchKernelGGL(k_out_copy, dim3(L.gx(kTnOutCopy, 64), 1, L.nEv), dim3(256), 0, L.s, L.dv, d);
+  L.tune_end(kTnOutCopy);
 }
 
 }  // namespace lst

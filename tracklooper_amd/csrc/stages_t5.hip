@@ -37,7 +37,8 @@ __device__ __forceinline__ void t5_range(const EventView& v, uint32_t m, uint32_
 // createEligibleModulesListForQuintupletsGPU (SDL/Quintuplet.h:3117-3213):
 // seed modules holding a triplet, their capacities and the exclusive scan of
 // the capacities in module order.
-__global__ __launch_bounds__(1024) void k_t5_eligible(EventView v) {
+__global__ __launch_bounds__(1024) void k_t5_eligible(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[34];
   const uint32_t nL = v.M.nLowerModules;
   // the capacities and the eligibility flags scanned together; both loads
@@ -423,7 +424,8 @@ __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.
 // Per mini-doublet, the T3s whose first mini-doublet it is, in slot order (one
 // wave per lower module: counts, offsets, a scatter inside the module's own T3
 // range, then each T3 placed at its rank within its short list).
-__global__ __launch_bounds__(256) void k_md_t3_lists(EventView v) {
+__global__ __launch_bounds__(256) void k_md_t3_lists(LST_VIEWS) {
+  LST_BIND;
   __shared__ __align__(16) ListLds sLists[4];
   ListLds& sX = sLists[threadIdx.x / kWave];
   const uint32_t nL = v.M.nLowerModules;
@@ -446,7 +448,8 @@ __device__ __forceinline__ uint32_t t3_last_md(const EventView& v, uint32_t t3) 
 // per inner T3: its candidates (the T3s of its third module, which number
 // them in serial order) and its shared-MD candidates (the T3s listed under its
 // last mini-doublet); per eligible module, both totals
-__global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_cand_count(LST_VIEWS) {
+  LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   const unsigned lane = lane_id();
   for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
@@ -479,7 +482,8 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(EventView v) {
 }
 
 // exclusive scan of the per-module counts (count = nEligible, read on the device)
-__global__ __launch_bounds__(1024) void k_t5_cand_scan(EventView v) {
+__global__ __launch_bounds__(1024) void k_t5_cand_scan(LST_VIEWS) {
+  LST_BIND;
   __shared__ unsigned lds[34];
   const uint32_t n = v.R.scalars[kNEligibleT5];
   const UVec<2> tot = scan_wide_n<kScanPer, 2>(
@@ -526,7 +530,8 @@ __device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uin
 // mini-doublet, written at the positions the count pass gives (one wave per
 // eligible module).  With each mini-doublet's T3s in slot order, list A is in
 // the reference's serial order (module, inner T3, outer T3).
-__global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_enum(LST_VIEWS) {
+  LST_BIND;
   if (!t5_flat_mode(v)) return;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
   const unsigned lane = lane_id();
@@ -551,7 +556,8 @@ __global__ __launch_bounds__(256) void k_t5_enum(EventView v) {
 }
 
 // stage 2: the two T4-style tests
-__global__ __launch_bounds__(256) void k_t5_filter_t4(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_filter_t4(LST_VIEWS) {
+  LST_BIND;
   if (!t5_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT5ListA];
   for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < 2 * n; g0 += gridDim.x * blockDim.x) {
@@ -576,7 +582,8 @@ __global__ __launch_bounds__(256) void k_t5_filter_t4(EventView v) {
 // The two halves run in different waves (so that neither waits for the
 // other's branch): waves 2p and 2p + 1 of a work-group take halves 0 and 1 of
 // the same 64 candidates, and the odd wave hands its results over in LDS.
-__global__ __launch_bounds__(256) void k_t5_final(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_final(LST_VIEWS) {
+  LST_BIND;
   __shared__ float sHalf[2][6][kWave];  // per wave pair: pass, regR, regG, regF, chi2, nonAnchorChi2
   if (!t5_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT5ListB];
@@ -630,7 +637,8 @@ __global__ __launch_bounds__(256) void k_t5_final(EventView v) {
 // stage 3b (DNN mode): the T5 DNN over the parked results; a result that
 // fails gets its inner T3 index set to ~0 and is not placed.  The cut cannot
 // change the tight flag of a candidate that passes it.
-__global__ __launch_bounds__(256) void k_t5_dnn(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_dnn(LST_VIEWS) {
+  LST_BIND;
   if (!v.t5Dnn || !t5_flat_mode(v)) return;
   const uint32_t nRes = v.R.scalars[kT5Results];
   const uint32_t n = nRes < kT5ResultMax ? nRes : kT5ResultMax;
@@ -660,11 +668,13 @@ __device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t k) {
   const uint32_t r = v.W5.resIdx[k];
   return r != ~0u && v.W5.results[kT5ResultWords * r] != ~0u;
 }
-__global__ __launch_bounds__(1024) void k_t5_scan_blocks(EventView v) {
+__global__ __launch_bounds__(1024) void k_t5_scan_blocks(LST_VIEWS) {
+  LST_BIND;
   if (!t5_flat_done(v)) return;
   scan_flag_blocks(v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); }, v.FS.pre, v.FS.sums);
 }
-__global__ __launch_bounds__(1024) void k_t5_scan_sums(EventView v) {
+__global__ __launch_bounds__(1024) void k_t5_scan_sums(LST_VIEWS) {
+  LST_BIND;
   if (!t5_flat_done(v)) return;
   scan_flag_sums(v.R.scalars[kT5ListA], v.FS.sums);
 }
@@ -672,7 +682,8 @@ __global__ __launch_bounds__(1024) void k_t5_scan_sums(EventView v) {
 // list A is in serial order (eligible modules, inner T3s and each
 // mini-doublet's T3s in order), so per module the passing entries take the
 // slots in list order
-__global__ __launch_bounds__(256) void k_t5_place(EventView v) {
+__global__ __launch_bounds__(256) void k_t5_place(LST_VIEWS) {
+  LST_BIND;
   if (!t5_flat_done(v)) return;
   const uint32_t total = v.R.scalars[kT5ListA], nElig = v.R.scalars[kNEligibleT5];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
@@ -701,7 +712,8 @@ __global__ __launch_bounds__(256) void k_t5_place(EventView v) {
 // Per-module builder, used when an event has more than kT5CandMax candidates
 // or more than kT5ResultMax passing ones:
 // one 256-lane work-group per eligible module walks its candidates in order.
-__global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
+__global__ __launch_bounds__(kT5Block) void k_quintuplets(LST_VIEWS) {
+  LST_BIND;
   __shared__ uint32_t pre[kMaxInnerT3 + 1];
   __shared__ unsigned lds[kT5Block / 64 + 1];
   if (t5_flat_done(v)) return;
@@ -766,29 +778,30 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(EventView v) {
 
 }  // namespace
 
-void launch_quintuplets(const EventView& v, hipStream_t s, const KernelTimers& t) {
+void launch_quintuplets(const Launch& L) {
+  const EventView& v = L.v;
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
-  hipLaunchKernelGGL(k_t5_eligible, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_md_t3_lists, dim3((v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t5_cand_count, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t5_cand_scan, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_enum, dim3(kModuleWaveBlocks), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_t5_filter_t4, dim3(flatGrid), dim3(256), 0, s, v);
-  t.begin(s, kTmT5Final);
-  hipLaunchKernelGGL(k_t5_final, dim3(flatGrid), dim3(256), 0, s, v);
-  t.end(s, kTmT5Final);
+  LST_GO(L, k_t5_eligible, 1, 1, 1024);
+  LST_GO_T(L, kTnMdT3Lists, k_md_t3_lists, (v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1, 256);
+  LST_GO_T(L, kTnT5CandCount, k_t5_cand_count, kModuleWaveBlocks, 256);
+  LST_GO(L, k_t5_cand_scan, 1, 1, 1024);
+  LST_GO_T(L, kTnT5Enum, k_t5_enum, kModuleWaveBlocks, 256);
+  LST_GO_T(L, kTnT5FilterT4, k_t5_filter_t4, flatGrid, 256);
+  L.t.begin(L.s, kTmT5Final);
+  LST_GO_T(L, kTnT5Final, k_t5_final, flatGrid, 256);
+  L.t.end(L.s, kTmT5Final);
   if (v.t5Dnn) {
-    t.begin(s, kTmT5Dnn);
-    hipLaunchKernelGGL(k_t5_dnn, dim3(flatGrid), dim3(256), 0, s, v);
-    t.end(s, kTmT5Dnn);
+    L.t.begin(L.s, kTmT5Dnn);
+    LST_GO_T(L, kTnT5Dnn, k_t5_dnn, flatGrid, 256);
+    L.t.end(L.s, kTmT5Dnn);
   }
-  hipLaunchKernelGGL(k_t5_scan_blocks, dim3(kScanGrid), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_scan_sums, dim3(1), dim3(1024), 0, s, v);
-  hipLaunchKernelGGL(k_t5_place, dim3(2048), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_quintuplets, dim3(nBlocks), dim3(kT5Block), 0, s, v);
-  launch_t5_list(v, s);
-  launch_t5_dup_after_build(v, s, t);
+  LST_GO_T(L, kTnT5ScanBlocks, k_t5_scan_blocks, kScanGrid, 1024);
+  LST_GO(L, k_t5_scan_sums, 1, 1, 1024);
+  LST_GO_T(L, kTnT5Place, k_t5_place, 2048, 256);
+  if (L.fallbacks) LST_GO(L, k_quintuplets, nBlocks, 1, kT5Block);
+  launch_t5_list(L);
+  launch_t5_dup_after_build(L);
 }
 
 }  // namespace lst

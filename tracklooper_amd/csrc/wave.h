@@ -40,6 +40,12 @@ __device__ __forceinline__ unsigned wave_inclusive_scan(unsigned v) {
 // value of lane 63 (every lane active)
 __device__ __forceinline__ unsigned wave_last(unsigned v) { return (unsigned)__builtin_amdgcn_readlane((int)v, 63); }
 
+// bitwise OR over the 64 lanes of the wave (every lane must take part)
+__device__ __forceinline__ unsigned wave_or(unsigned x) {
+  for (int off = 32; off > 0; off >>= 1) x |= (unsigned)__shfl_xor((int)x, off, 64);
+  return x;
+}
+
 // Exclusive prefix sum of a workgroup of BLOCK threads (BLOCK multiple of 64);
 // `lds` must hold BLOCK/64 + 1 words.  Returns the exclusive prefix and the
 // group total through `total`.

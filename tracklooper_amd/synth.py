@@ -248,12 +248,15 @@ def _pixel_crossings(h: Helices):
 
 
 def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id: int | None = None,
-                   geom: SynthGeometry | None = None, geom_name: str = "OT800_IT615_pt0.8") -> dict:
+                   geom: SynthGeometry | None = None, geom_name: str = "OT800_IT615_pt0.8",
+                   max_module_hits: int = 127) -> dict:
     """Generate one event.
 
     kind: "muon" (10 muons, pT log-U[0.5,50] GeV) or "pu" (n_pu x 35 charged
     helices with pT = 0.4 + Exp(0.5) GeV plus 200 hard log-U[1,100] GeV, 1%
     noise hits).  Returns a dict with the ``LST::run`` arguments and truth.
+    max_module_hits: hits kept per sensor (127 keeps the input valid for the
+    reference's int8 counts; a larger value makes inputs the engine rejects).
     """
     geo = geom or _geometry_cache(geom_name)
     cid = config_id if config_id is not None else {"muon": 1, "pu": 2}[kind]
@@ -292,7 +295,7 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
     # at most 127 hits per sensor (hit counts are int8 in SDL/Hit.h:31-32)
     _, first, counts = np.unique(det, return_index=True, return_counts=True)
     rank = np.arange(len(det)) - np.repeat(first, counts)
-    keep = rank < 127
+    keep = rank < max_module_hits
     det, pos, trk = det[keep], pos[keep], trk[keep]
 
     # pixel seeds
@@ -355,3 +358,34 @@ def _geometry_cache(name):
     if name not in _GEO:
         _GEO[name] = SynthGeometry(geometry_dir(name))
     return _GEO[name]
+
+
+# ---------------------------------------------------------------- uneven batches
+C3_CONFIG = 3  # config id of the uneven-pileup batches
+
+
+def c3_pileups(batch: int, n: int = 64) -> np.ndarray:
+    """Pileup of each event of uneven batch `batch`: U{50..250}, keyed by (config 3, batch)."""
+    return np.random.default_rng([C3_CONFIG, batch, 0xC3]).integers(50, 251, n)
+
+
+def _gen_one(spec):
+    kind, index, n_pu, cid, geom_name = spec
+    return generate_event(kind, index, n_pu=n_pu, config_id=cid, geom_name=geom_name)
+
+
+def generate_many(specs, workers: int = 1) -> list:
+    """generate_event over (kind, index, n_pu, config_id, geom_name) tuples, in order.  workers > 1
+    forks a process pool: call it before the process starts using the GPU."""
+    if workers <= 1 or len(specs) < 2:
+        return [_gen_one(s) for s in specs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(workers, len(specs))) as pool:
+        return pool.map(_gen_one, specs, chunksize=1)
+
+
+def c3_batch(batch: int, n: int = 64, geom_name: str = "OT800_IT615_pt0.8", workers: int = 1) -> list:
+    """The n events of uneven batch `batch`: pileup U{50..250} each (35 charged helices per
+    interaction, the PU spectra of generate_event), seeded by (config 3, batch, position)."""
+    pus = c3_pileups(batch, n)
+    return generate_many([("pu", batch * n + i, int(pu), C3_CONFIG, geom_name) for i, pu in enumerate(pus)], workers)
