@@ -211,7 +211,7 @@ void event_add_pixel_segments(Event& ev, const Input& in) {
   ev.indicesOfEligibleT5Modules.assign(nL, 0);
 
   unsigned size = in.ptIn.size();
-  if (size > kNMaxPixelSegments) size = kNMaxPixelSegments;  // Event.cc:237-244
+  if (size > kNMaxPixelSegments) size = kNMaxPixelSegments;  // SDL/Event.cc:237-244
   const uint16_t pix = g.pixelModuleIndex;
   ev.miniDoubletModuleOccupancy[nL] = kNMaxPixelMDs;
   md_array_ranges(ev);

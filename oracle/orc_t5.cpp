@@ -1155,3 +1155,9 @@ void event_create_quintuplets(Event& ev) {
 }
 
 }  // namespace orc
+
+// The T5 r-phi chi-squared table (see orc_pixel_cut_table).
+extern "C" int orc_t5_cut_table(const int* l, float value) {
+  return orc::passChiSquaredConstraint(l[0], l[1], l[2], l[3], l[4], value);
+}
+

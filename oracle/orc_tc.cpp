@@ -607,7 +607,7 @@ bool runPixelQuintupletDefaultAlgo(const Event& ev, unsigned pls, unsigned t5, u
   }
   if (o.pixelRadius < 5.0f * kR1GeVf && !passPT5RPhiChiSquaredCuts(l[0], l[1], l[2], l[3], l[4], o.rPhiChiSquared))
     return false;
-  {
+  {  // computePT5RPhiChiSquaredInwards (SDL/PixelTriplet.h:2374-2387)
     const float xPix[2] = {ev.anchorX[pixIn], ev.anchorX[pixOut]};
     const float yPix[2] = {ev.anchorY[pixIn], ev.anchorY[pixOut]};
     const float r = o.quintupletRadius;
@@ -774,7 +774,7 @@ void event_create_pixel_quintuplets(Event& ev) {
       }
     }
   }
-  // addpT5asTrackCandidateInGPU
+  // addpT5asTrackCandidateInGPU (SDL/TrackCandidate.h:531-581)
   for (unsigned q = 0; q < np; q++) {
     if (ev.pt5IsDup[q]) continue;
     const unsigned idx = ev.nTrackCandidates++;
@@ -1078,3 +1078,19 @@ void event_get_output(Event& ev) {
 }
 
 }  // namespace orc
+
+// The chi-squared cut tables by name, for the tests that probe them against
+// the thresholds extracted from the reference: -1 for an unknown name.
+extern "C" int orc_pixel_cut_table(const char* name, const int* l, float value) {
+  using namespace orc;
+  const std::string n(name);
+  if (n == "passPT3RZChiSquaredCuts") return passPT3RZChiSquaredCuts(l[0], l[1], l[2], value);
+  if (n == "passPT3RPhiChiSquaredCuts") return passPT3RPhiChiSquaredCuts(l[0], l[1], l[2], value);
+  if (n == "passPT3RPhiChiSquaredInwardsCuts") return passPT3RPhiChiSquaredInwardsCuts(l[0], l[1], l[2], value);
+  if (n == "passPT5RZChiSquaredCuts") return passPT5RZChiSquaredCuts(l[0], l[1], l[2], l[3], l[4], value);
+  if (n == "passPT5RPhiChiSquaredCuts") return passPT5RPhiChiSquaredCuts(l[0], l[1], l[2], l[3], l[4], value);
+  if (n == "passPT5RPhiChiSquaredInwardsCuts")
+    return passPT5RPhiChiSquaredInwardsCuts(l[0], l[1], l[2], l[3], l[4], value);
+  return -1;
+}
+

@@ -10,7 +10,8 @@ both a C/C++ initializer fragment (`-0x1.206b7ep+2f,` per line, row-major,
 Each decimal literal is rounded to float32 exactly (round-to-nearest-even on
 the exact decimal value), as a C++ compiler does for a `1.234f` literal.
 
-Usage: python extract_weights.py /path/to/SDL/NeuralNetworkWeights.h
+Usage: python tests/golden/extract_t5dnn_weights.py /path/to/SDL/NeuralNetworkWeights.h
+(writes tracklooper_amd/data/t5dnn/*.inc)
 """
 from __future__ import annotations
 
@@ -21,6 +22,8 @@ import sys
 from fractions import Fraction
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# the parameter files the engine and the oracle include
+OUT_DIR = os.path.join(os.path.dirname(os.path.dirname(HERE)), "tracklooper_amd", "data", "t5dnn")
 TENSORS = {"bias_0": 32, "wgtT_0": 38 * 32, "bias_2": 32, "wgtT_2": 32 * 32, "bias_4": 1, "wgtT_4": 32}
 
 
@@ -71,7 +74,7 @@ def main(argv):
     with open(argv[1]) as f:
         tensors = parse(f.read())
     for name, vals in tensors.items():
-        with open(os.path.join(HERE, name + ".inc"), "w") as f:
+        with open(os.path.join(OUT_DIR, name + ".inc"), "w") as f:
             for v in vals:
                 f.write(float.hex(v) + "f,\n")
     print("wrote", ", ".join(f"{k}[{len(v)}]" for k, v in tensors.items()))
