@@ -71,6 +71,16 @@ def algorithmic_bytes(n: dict) -> dict:
       k_t5_dup_after_build: read eta, phi, score, flags and 10 hit indices (54 B/quintuplet), write 1 B
       k_cross_pls_pairs  : read 50 B/pLS (direction, flags, hit indices through the MDs), 25 B/track candidate
       k_out_copy         : read and write the 68 B output record of each track candidate
+      k_md_cand_fill     : write the 16 B record of each candidate hit pair
+      k_md_place         : per mini-doublet read its candidate record (16 B) and parked results (36 B) and the
+                           derived fields of its two hits (84 B), write the 134 B mini-doublet
+      k_ls_cand_fill     : write the 16 B record of each candidate mini-doublet pair, read 8 B per mini-doublet
+      k_ls_place         : per segment read its candidate record (16 B), parked results (24 B) and two anchor
+                           hit indices (8 B), write the 44 B segment
+      k_t3_place         : per triplet read its list entry (8 B) and results (24 B), write the 60 B triplet
+      k_t5_place         : per quintuplet read its list entry (12 B) and results (48 B), write the 119 B quintuplet
+      k_pix_write<PT5Kind>: per pLS its mask words (8 B) and per pixel quintuplet its parked result (40 B) and
+                           the 150 B record written
     """
     h, md, ls, t3, t5 = n["hits"], n["mds"], n["segs"], n["t3"], n["t5"]
     pls, pt5, pt3, tc = n["pls"], n["pt5"], n["pt3"], n["tc"]
@@ -91,6 +101,13 @@ def algorithmic_bytes(n: dict) -> dict:
         "k_cross_pls_pairs": 50 * pls + 25 * tc,
         "k_t5_dup_after_build": 55 * t5,
         "k_out_copy": 136 * tc,
+        "k_md_cand_fill": 16 * n["mdcand"],
+        "k_md_place": 270 * md,
+        "k_ls_cand_fill": 16 * n["lscand"] + 8 * md,
+        "k_ls_place": 92 * ls,
+        "k_t3_place": 92 * t3,
+        "k_t5_place": 179 * t5,
+        "k_pix_write<PT5Kind>": 8 * pls + 190 * pt5,
     }
 
 
@@ -201,19 +218,29 @@ def main():
         tune = {"default_ms": d_ms, "tuned_ms": t_ms}
         print(f"autotune: summed kernel time {d_ms:.2f} ms at the default work-group counts, {t_ms:.2f} ms tuned",
               file=sys.stderr, flush=True)
-    outs, bufs = lst.Batch._outputs(E, args.capacity)
-    for w in range(args.warmup):
-        lst._check(L.lst_batch_run_staged(batch.h, (w % args.batches) * E, E, outs))
+    # output descriptors of a run of K launch sets: event i of the run writes into buffer i % E (the sets are
+    # consumed one after the other, so the buffers hold the last set's lists at the end)
+    outs1, bufs = lst.Batch._outputs(E, args.capacity)
+
+    def outputs_for(n_sets):
+        outs = (lst._Output * (n_sets * E))()
+        for i in range(n_sets * E):
+            outs[i] = outs1[i % E]
+        return outs
+
+    warm_outs, timed_outs = outputs_for(args.warmup), outputs_for(args.steps)
+    n_staged = args.batches * E
+    # one call runs all the steps: launch set k + 1's inputs cross to the device while set k computes
+    if args.warmup:
+        lst._check(L.lst_batch_run_staged(batch.h, 0, args.warmup * E, warm_outs))
     torch.cuda.synchronize()
     if args.profile_only:
-        for s in range(args.steps):
-            lst._check(L.lst_batch_run_staged(batch.h, (s % args.batches) * E, E, outs))
+        lst._check(L.lst_batch_run_staged(batch.h, 0, args.steps * E, timed_outs))
         torch.cuda.synchronize()
         print(json.dumps({"profile_only": True, "steps": args.steps}), flush=True)
         return
     t0 = time.perf_counter()
-    for s in range(args.steps):  # each call: inputs to the device, the chain, outputs to the host
-        lst._check(L.lst_batch_run_staged(batch.h, (s % args.batches) * E, E, outs))
+    lst._check(L.lst_batch_run_staged(batch.h, (args.warmup * E) % n_staged, args.steps * E, timed_outs))
     torch.cuda.synchronize()
     wall_batch = time.perf_counter() - t0
     # outputs of every event for the comparison with the per-event path
@@ -311,7 +338,8 @@ def main():
             "workload": f"uneven batches of {E} synthetic events, pileup U{{50..250}} per event "
                         f"(mean {pus.mean():.0f}), OT800_IT615_pt0.8, plain chi2 T5 selection; one step = one "
                         f"launch set of {E} events: inputs host-to-device, hits..track candidates, output lists "
-                        f"device-to-host; {args.batches} batches cycled",
+                        f"device-to-host; {args.batches} batches cycled; the {args.steps} steps run in one call, "
+                        f"each set's input copies overlapping the previous set's kernels",
             "batched_events_per_s": value,
             "per_event_events_per_s": per_event_rate,
             "per_event_ms_per_event": wall_per * 1e3 / n_per,

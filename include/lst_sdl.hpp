@@ -123,6 +123,8 @@ class Event {
   // waits for the event's stream; LSTError on bad input found by the device-side checks
   void wait() { detail::check(lst_event_synchronize(ev_), "wait"); }
 
+  // the hits added, pixel pseudo-hits included (the reference's per-layer hit counters behind this getter
+  // are reset but never filled, SDL/Event.cc:20-60,1264-1272)
   unsigned int getNumberOfHits() { return lst_event_count(ev_, "hits"); }
   unsigned int getNumberOfMiniDoublets() { return lst_event_count(ev_, "miniDoublets"); }
   unsigned int getNumberOfSegments() { return lst_event_count(ev_, "segments"); }

@@ -24,7 +24,8 @@ constexpr unsigned kScanGrid = 512;         // work-groups of the block pass (th
 // pass 1: per block of kScanBlockItems candidates, the in-block exclusive
 // prefix of the flags, and the block total
 template <class Flag>
-__device__ __forceinline__ void scan_flag_blocks(uint32_t total, Flag flag, uint32_t* pre, uint32_t* sums) {
+__device__ __forceinline__ void scan_flag_blocks(uint32_t total, Flag flag, uint32_t* pre, uint32_t* sums,
+                                                 uint32_t* pass) {
   __shared__ unsigned lds[17];
   const uint32_t nBlocks = (total + kScanBlockItems - 1) / kScanBlockItems;
   for (uint32_t b = blockIdx.x; b < nBlocks; b += gridDim.x) {
@@ -40,6 +41,7 @@ __device__ __forceinline__ void scan_flag_blocks(uint32_t total, Flag flag, uint
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       if (L0 + k < total) pre[L0 + k] = run;
+      if (f[k]) pass[b * kScanBlockItems + run] = L0 + k;  // its place among the block's passing candidates
       run += f[k];
     }
     if (threadIdx.x == 0) sums[b] = tot;
@@ -61,6 +63,17 @@ __device__ __forceinline__ void scan_flag_sums(uint32_t total, uint32_t* sums) {
     carry += tot;
   }
   if (threadIdx.x == 0) sums[nBlocks] = carry;
+}
+
+// fn(L, rank(L)) for every passing candidate, work-group b taking scan blocks
+// b, b + gridDim.x, ... and its lanes the block's passing candidates
+template <class Fn>
+__device__ __forceinline__ void for_each_passing(const FlatScan& s, uint32_t total, Fn fn) {
+  const uint32_t nBlocks = (total + kScanBlockItems - 1) / kScanBlockItems;
+  for (uint32_t b = blockIdx.x; b < nBlocks; b += gridDim.x) {
+    const uint32_t first = s.sums[b], cnt = s.sums[b + 1] - first;
+    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) fn(s.pass[b * kScanBlockItems + j], first + j);
+  }
 }
 
 // number of passing candidates before L (L <= total)

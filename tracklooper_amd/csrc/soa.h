@@ -92,6 +92,7 @@ constexpr uint32_t kScanCandMax = 1u << 23;  // >= every builder's candidate max
 struct FlatScan {
   uint32_t* pre;   // [kScanCandMax] exclusive prefix of the pass flags inside each scan block
   uint32_t* sums;  // [kScanCandMax / 4096 + 1] exclusive prefix of the block totals, then the total
+  uint32_t* pass;  // [kScanCandMax] per scan block, its passing candidates in order (block b from b * 4096)
 };
 
 // Work arrays of the flattened mini-doublet builder: every (lower hit, upper

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(1024) void k_md_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!md_flat_done(v)) return;
   scan_flag_blocks(
-      v.R.scalars[kMDCandTotal], [&](uint32_t L) { return v.W2.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+      v.R.scalars[kMDCandTotal], [&](uint32_t L) { return v.W2.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
 }
 __global__ __launch_bounds__(1024) void k_md_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -365,24 +365,20 @@ __global__ __launch_bounds__(256) void k_md_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kMDCandTotal], nL = v.M.nLowerModules;
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
-    // the loads that only need L, then (for a passing candidate) those that
-    // need its module and park slot, each group issued before any use
-    const uint32_t k = v.W2.resIdx[L];
+  for_each_passing(v.FS, total, [&](uint32_t L, uint32_t rankL) {
+    // the loads that need L, then those that need its module, each group issued before any use
     const uint4 cd = v.W2.cand[L];
-    const uint32_t rankL = v.FS.pre[L] + v.FS.sums[L / kScanBlockItems];  // flag_rank for L < total
-    if (k == ~0u) continue;
+    const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * (size_t)L]);
+    const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
     const uint32_t m = cd.z;
     const uint32_t off = v.W2.candOff[m];
     const int occ = v.mdModuleOccupancy[m];
     const ModCtx c = load_mod(v.M, m);
-    const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * k]);
-    const MDOut o{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
     const uint32_t rank = rankL - flag_rank(v.FS, off, total);
-    if ((int)rank >= occ) continue;
+    if ((int)rank >= occ) return;
     const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     store_md(v, cd.x, cd.y, m, anchorIsUpper, o, v.mdModuleIndices[m] + rank);
-  }
+  });
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
     const uint32_t count = flag_rank(v.FS, v.W2.candOff[m + 1], total) - flag_rank(v.FS, v.W2.candOff[m], total);
     v.R.totOccupancyMDs[m] = count;
@@ -558,7 +554,7 @@ __global__ __launch_bounds__(1024) void k_ls_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!ls_flat_done(v)) return;
   scan_flag_blocks(
-      v.R.scalars[kLSCandTotal], [&](uint32_t L) { return v.WL.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums);
+      v.R.scalars[kLSCandTotal], [&](uint32_t L) { return v.WL.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
 }
 __global__ __launch_bounds__(1024) void k_ls_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -580,22 +576,18 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
-  for (uint32_t L = blockIdx.x * blockDim.x + threadIdx.x; L < total; L += gridDim.x * blockDim.x) {
-    // the loads that only need L, then those of a passing candidate, each
-    // group issued before any use
-    const uint32_t k = v.WL.resIdx[L];
+  for_each_passing(v.FS, total, [&](uint32_t L, uint32_t rankL) {
+    // the loads that need L, then those that need its mini-doublets and module, each group issued before any use
     const uint4 c = v.WL.cand[L];
-    const uint32_t rankL = v.FS.pre[L] + v.FS.sums[L / kScanBlockItems];  // flag_rank for L < total
-    if (k == ~0u) continue;
+    const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * (size_t)L]);
+    const float w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
     const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
     const uint32_t off = v.WL.candOff[m];
     const int occ = v.segModuleOccupancy[m];
     const uint32_t segBase = v.segModuleIndices[m];
     const uint32_t hitIn = v.D.anchorHitIndices[inMD], hitOut = v.D.anchorHitIndices[outMD];
-    const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * k]);
-    const float w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
     const uint32_t rank = rankL - flag_rank(v.FS, off, total);
-    if ((int)rank >= occ) continue;
+    if ((int)rank >= occ) return;
     const uint32_t s = segBase + rank;
     v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
     v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
@@ -603,7 +595,7 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
     v.S.outerMiniDoubletAnchorHitIndices[s] = hitOut;
     v.S.dPhis[s] = w0; v.S.dPhiMins[s] = w1; v.S.dPhiMaxs[s] = w2;
     v.S.dPhiChanges[s] = w3; v.S.dPhiChangeMins[s] = w4; v.S.dPhiChangeMaxs[s] = w5;
-  }
+  });
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
     v.R.totOccupancySegments[m] = count;
@@ -908,7 +900,7 @@ __global__ __launch_bounds__(1024) void k_t3_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!t3_flat_done(v)) return;
   scan_flag_blocks(
-      v.R.scalars[kT3List], [&](uint32_t k) { return v.W3.resIdx[k] != ~0u; }, v.FS.pre, v.FS.sums);
+      v.R.scalars[kT3List], [&](uint32_t k) { return v.W3.resIdx[k] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
 }
 __global__ __launch_bounds__(1024) void k_t3_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -931,15 +923,15 @@ __global__ __launch_bounds__(256) void k_t3_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kT3List], nL = v.M.nLowerModules;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
-    if (v.W3.resIdx[k] == ~0u) continue;
+  for_each_passing(v.FS, total, [&](uint32_t k, uint32_t rankK) {
     const uint32_t m1 = v.W3.list[2 * k];
-    const uint32_t rank = flag_rank(v.FS, k, total) - flag_rank(v.FS, v.W3.survOff[m1], total);
-    if ((int)rank >= v.R.tripletModuleOccupancy[m1]) continue;
     const uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)k];
-    t3_store(v, v.R.tripletModuleIndices[m1] + rank, m1, w[0], w[1], __uint_as_float(w[2]), __uint_as_float(w[3]),
-             __uint_as_float(w[4]), __uint_as_float(w[5]));
-  }
+    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
+    const uint32_t rank = rankK - flag_rank(v.FS, v.W3.survOff[m1], total);
+    if ((int)rank >= v.R.tripletModuleOccupancy[m1]) return;
+    t3_store(v, v.R.tripletModuleIndices[m1] + rank, m1, w0, w1, __uint_as_float(w2), __uint_as_float(w3),
+             __uint_as_float(w4), __uint_as_float(w5));
+  });
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
     const uint32_t count = flag_rank(v.FS, v.W3.survOff[m + 1], total) - flag_rank(v.FS, v.W3.survOff[m], total);
     v.R.totOccupancyTriplets[m] = count;

@@ -671,7 +671,7 @@ __device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t k) {
 __global__ __launch_bounds__(1024) void k_t5_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!t5_flat_done(v)) return;
-  scan_flag_blocks(v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); }, v.FS.pre, v.FS.sums);
+  scan_flag_blocks(v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); }, v.FS.pre, v.FS.sums, v.FS.pass);
 }
 __global__ __launch_bounds__(1024) void k_t5_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -686,12 +686,11 @@ __global__ __launch_bounds__(256) void k_t5_place(LST_VIEWS) {
   LST_BIND;
   if (!t5_flat_done(v)) return;
   const uint32_t total = v.R.scalars[kT5ListA], nElig = v.R.scalars[kNEligibleT5];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
-    if (!t5_passed(v, k)) continue;
+  for_each_passing(v.FS, total, [&](uint32_t k, uint32_t rankK) {
     const uint32_t e = v.W5.listA[3 * k];
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
-    const uint32_t rank = flag_rank(v.FS, k, total) - flag_rank(v.FS, v.W5.survOff[e], total);
-    if ((int)rank >= v.R.quintupletModuleOccupancy[m1]) continue;
+    const uint32_t rank = rankK - flag_rank(v.FS, v.W5.survOff[e], total);
+    if ((int)rank >= v.R.quintupletModuleOccupancy[m1]) return;
     const uint32_t* w = &v.W5.results[kT5ResultWords * v.W5.resIdx[k]];
     T5Result r;
     r.innerR = __uint_as_float(w[2]); r.bridgeR = __uint_as_float(w[3]); r.outerR = __uint_as_float(w[4]);
@@ -699,7 +698,7 @@ __global__ __launch_bounds__(256) void k_t5_place(LST_VIEWS) {
     r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
     r.tight = w[11] != 0;
     t5_store(v, (uint32_t)v.R.quintupletModuleIndices[m1] + rank, w[0], w[1], r, v.M.layers[m1]);
-  }
+  });
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nElig; e += gridDim.x * blockDim.x) {
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
     const uint32_t count = flag_rank(v.FS, v.W5.survOff[e + 1], total) - flag_rank(v.FS, v.W5.survOff[e], total);
