@@ -201,10 +201,29 @@ lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* feature
 #define LST_T5DNN_NPARAMS 2337
 lst_status lst_t5dnn_set_weights(const float* params);
 lst_status lst_t5dnn_get_weights(float* params);
+/* The score a T5 candidate must exceed in LST_T5_DNN mode (process-wide, on
+ * the current device; default the reference's LSTWP2 = 0.6177366,
+ * SDL/Constants.h:145, which SDL/Quintuplet.h:2937-2938 applies), for
+ * retrained parameters calibrated to another working point. */
+lst_status lst_t5dnn_set_working_point(float wp);
+float lst_t5dnn_get_working_point(void);
 lst_status lst_t5dnn_forward(const float* x, const float* params, float* h1, float* h2, float* logit, uint32_t n,
                              void* stream);
 lst_status lst_t5dnn_backward(const float* x, const float* params, const float* h1, const float* h2,
                               const float* dlogit, float* d2, float* d1, float* grad, uint32_t n, void* stream);
+/* One training step on the matrix cores: forward, binary cross-entropy of the
+ * logits against labels y [n] (0 or 1), and backward, every GEMM in BF16 with
+ * FP32 accumulation (the 32 -> 1 output layer and the element-wise steps in
+ * FP32).  grad [LST_T5DNN_NPARAMS] receives the gradient of the mean loss,
+ * *loss (device pointer, may be NULL) the mean loss.  Enqueued on `stream`;
+ * uses a workspace the library allocates per device (kept for the process);
+ * deterministic (partials of 64-sample chunks summed in chunk order). */
+lst_status lst_t5dnn_train_step_bf16(const float* x, const float* y, const float* params, float* grad, float* loss,
+                                     uint32_t n, void* stream);
+/* Adam (Kingma and Ba 2015) on the packed parameters in place: m, v the
+ * moment estimates (zero before step 1), step = 1, 2, ... */
+lst_status lst_t5dnn_adam(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,
+                          float beta2, float eps, void* stream);
 
 /* ---- Batched engine.  The reference runs one SDL::Event per queue with one
  * OpenMP thread per queue (bin/sdl.cc:389-419) and LST::run one event at a

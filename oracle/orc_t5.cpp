@@ -851,6 +851,7 @@ float kDnnW4[32] = {
 #include "../tracklooper_amd/data/t5dnn/wgtT_4.inc"
 };
 constexpr float kLSTWP2 = 0.6177366f;       // SDL/Constants.h:145
+float gDnnWP = kLSTWP2;                      // the working point in use (LSTWP2 unless set)
 constexpr float kMagneticField = 3.8112f;   // SDL/Constants.h:135
 }  // namespace
 
@@ -988,8 +989,8 @@ bool runQuintupletDefaultAlgo(const Event& ev, const uint16_t mods[5], unsigned 
     ev.dnnT3Pair.push_back(outerT3);
     for (int a = 0; a < 6; a++) ev.dnnHitIdxs.push_back(ev.idxs[ev.t3HitIndices[6 * innerT3 + a]]);
     for (int a = 0; a < 4; a++) ev.dnnHitIdxs.push_back(ev.idxs[ev.t3HitIndices[6 * outerT3 + 2 + a]]);
-    o.tightCutFlag = o.tightCutFlag && score > kLSTWP2;
-    if (!(score > kLSTWP2)) return false;
+    o.tightCutFlag = o.tightCutFlag && score > gDnnWP;
+    if (!(score > gDnnWP)) return false;
   }
   if (ev.cfg.t5Mode == 0) {  // USE_RPHICHI2: SDL/Quintuplet.h:2943-2956
     if (o.regressionRadius < 5.0f / (2.f * k2Rinv1GeVf)) {
@@ -1161,3 +1162,4 @@ extern "C" int orc_t5_cut_table(const int* l, float value) {
   return orc::passChiSquaredConstraint(l[0], l[1], l[2], l[3], l[4], value);
 }
 
+extern "C" void orc_t5dnn_set_working_point(float wp) { orc::gDnnWP = wp; }

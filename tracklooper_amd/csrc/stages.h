@@ -79,6 +79,7 @@ enum TuneId : int {
   kNTune
 };
 extern const char* const kTuneNames[kNTune];
+extern const uint32_t kBatchGrid[kNTune];  // batch launch-set defaults (0: the one-event default)
 constexpr uint32_t kMaxBatchSlots = 128;  // events of one batch launch set
 
 // Optional hipEvent pairs recorded around the hot kernels (timing).  Slot k
@@ -132,7 +133,10 @@ struct Launch {
   bool fallbacks = true;         // launch the per-module builders that take events overflowing the flattened ones
   const hipEvent_t* tuneEvents = nullptr;  // [2 kNTune] recorded before / after each tunable kernel (autotuner)
   uint32_t* tuneRecorded = nullptr;        // bit k of word k / 32: kernel k was bracketed
-  uint32_t gx(int id, uint32_t dflt) const { return grid && grid[id] ? grid[id] : dflt; }
+  uint32_t gx(int id, uint32_t dflt) const {
+    if (grid && grid[id]) return grid[id];
+    return nEv > 1 && kBatchGrid[id] ? kBatchGrid[id] : dflt;
+  }
   void tune_begin(int id) const {
     if (!tuneEvents) return;
     (void)hipEventRecord(tuneEvents[2 * id], s);

@@ -289,7 +289,7 @@ __constant__ float cDnnBias4[1] = {
 __constant__ float cDnnW4[32] = {
 #include "../data/t5dnn/wgtT_4.inc"
 };
-constexpr float kT5DnnWP = 0.6177366f;     // T5DNN::LSTWP2 (SDL/Constants.h:145)
+__constant__ float cDnnWP = 0.6177366f;    // working point, by default T5DNN::LSTWP2 (SDL/Constants.h:145)
 constexpr float kMagneticField = 3.8112f;  // SDL/Constants.h:135
 
 __device__ __forceinline__ float dnn_log10(float x) { return m_log(x) / 2.302585093f; }
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(256) void k_t5_dnn(LST_VIEWS) {
       for (int a = 0; a < 4; a++)
         v.dnnDump.hitIdxs[10 * (size_t)k + 6 + a] = v.H.idxs[v.T.hitIndices[6 * outer + 2 + a]];
     }
-    if (!(score > kT5DnnWP)) w[0] = ~0u;
+    if (!(score > cDnnWP)) w[0] = ~0u;
   }
 }
 
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(kT5Block) void k_quintuplets(LST_VIEWS) {
         mods[4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
         pass = t5_candidate(v, mods, innerT3, outerT3, md, r);
         if (pass && v.t5Dnn)
-          pass = t5dnn_score(v, t5_cand(v, innerT3, outerT3), innerT3, r.innerR, r.bridgeR, r.outerR) > kT5DnnWP;
+          pass = t5dnn_score(v, t5_cand(v, innerT3, outerT3), innerT3, r.innerR, r.bridgeR, r.outerR) > cDnnWP;
       }
       unsigned n;
       const unsigned rank = count + block_exclusive_scan<kT5Block>(pass ? 1u : 0u, lds, n);
@@ -820,6 +820,16 @@ extern "C" lst_status lst_t5dnn_set_weights(const float* params) {
   p += 1;
   if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnW4), p, 32 * 4) != hipSuccess) return LST_E_DEVICE;
   return hipDeviceSynchronize() == hipSuccess ? LST_OK : LST_E_DEVICE;
+}
+
+extern "C" lst_status lst_t5dnn_set_working_point(float wp) {
+  if (hipMemcpyToSymbol(HIP_SYMBOL(lst::cDnnWP), &wp, sizeof(float)) != hipSuccess) return LST_E_DEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? LST_OK : LST_E_DEVICE;
+}
+
+extern "C" float lst_t5dnn_get_working_point(void) {
+  float wp = 0.f;
+  return hipMemcpyFromSymbol(&wp, HIP_SYMBOL(lst::cDnnWP), sizeof(float)) == hipSuccess ? wp : -1.f;
 }
 
 extern "C" lst_status lst_t5dnn_get_weights(float* params) {

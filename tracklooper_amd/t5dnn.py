@@ -9,9 +9,18 @@ chain's parameter files (tracklooper_amd/data/t5dnn/*.inc).
 
 Training rows are the engine's record of every T5 candidate that reaches the
 DNN (``Event.record_t5dnn``); a row is signal when all 10 of its hits belong
-to one simulated track.  The forward and backward passes run in the library's
-kernels (``lst_t5dnn_forward`` / ``lst_t5dnn_backward``) behind a
-``torch.autograd.Function``; the optimiser and the loss are plain PyTorch.
+to one simulated track.  Two training paths run on the library's kernels:
+
+* FP32: forward and backward (``lst_t5dnn_forward`` / ``lst_t5dnn_backward``)
+  behind a ``torch.autograd.Function``, with PyTorch's loss and Adam;
+* BF16 on the matrix cores (``train_bf16``): one fused kernel per step for
+  forward, loss and backward (``lst_t5dnn_train_step_bf16``, GEMMs in BF16 with
+  FP32 accumulation) and the library's Adam (``lst_t5dnn_adam``); Python only
+  draws the minibatches.
+
+``evaluate_selection`` runs the chain with the plain chi-squared cut and with
+the network (a given parameter set and working point) on the same events and
+compares efficiency, fake rate and duplicate rate (``tracklooper_amd.metrics``).
 """
 from __future__ import annotations
 
@@ -83,7 +92,21 @@ def _lib():
     L.lst_t5dnn_backward.argtypes = [P] * 8 + [C.c_uint32, P]
     L.lst_t5dnn_set_weights.argtypes = [P]
     L.lst_t5dnn_get_weights.argtypes = [P]
+    L.lst_t5dnn_train_step_bf16.argtypes = [P] * 5 + [C.c_uint32, P]
+    L.lst_t5dnn_adam.argtypes = [P] * 4 + [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, P]
+    L.lst_t5dnn_set_working_point.argtypes = [C.c_float]
+    L.lst_t5dnn_get_working_point.restype = C.c_float
     return lst, L
+
+
+def set_chain_working_point(wp: float) -> None:
+    """The score a T5 candidate must exceed in the chain's DNN mode (process-wide)."""
+    lst, L = _lib()
+    lst._check(L.lst_t5dnn_set_working_point(float(wp)))
+
+
+def get_chain_working_point() -> float:
+    return float(_lib()[1].lst_t5dnn_get_working_point())
 
 
 def set_chain_weights(params: np.ndarray) -> None:
@@ -178,24 +201,185 @@ def train(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 10, b
     return P.detach().cpu().numpy().astype(np.float32), losses
 
 
+def train_step_bf16(X, Y, P, G, loss=None):
+    """One fused forward / loss / backward on the matrix cores: G <- d(mean BCE)/dP, loss <- mean BCE."""
+    import torch
+    lst, L = _lib()
+    s = torch.cuda.current_stream(X.device).cuda_stream
+    lst._check(L.lst_t5dnn_train_step_bf16(X.data_ptr(), Y.data_ptr(), P.data_ptr(), G.data_ptr(),
+                                           None if loss is None else loss.data_ptr(), X.shape[0], s))
+
+
+def train_bf16(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 10, batch: int = 8192,
+               lr: float = 1e-3, seed: int = 0, device: str = "cuda") -> tuple[np.ndarray, list[float]]:
+    """Adam on the mean binary cross-entropy, every step one fused BF16 MFMA kernel plus the library's Adam;
+    returns the parameters and the mean loss of each epoch."""
+    import torch
+    lst, L = _lib()
+    g = torch.Generator().manual_seed(seed)
+    X = torch.as_tensor(x, dtype=torch.float32, device=device)
+    Y = torch.as_tensor(y, dtype=torch.float32, device=device)
+    P = torch.as_tensor(params0, dtype=torch.float32, device=device).clone()
+    G, M, V = torch.zeros_like(P), torch.zeros_like(P), torch.zeros_like(P)
+    step_loss = torch.zeros(1, dtype=torch.float32, device=device)
+    s = torch.cuda.current_stream(X.device).cuda_stream
+    losses, step = [], 0
+    for _ in range(epochs):
+        perm = torch.randperm(X.shape[0], generator=g).to(device)
+        tot = torch.zeros(1, dtype=torch.float64, device=device)
+        for i in range(0, X.shape[0], batch):
+            idx = perm[i:i + batch]
+            xb, yb = X[idx].contiguous(), Y[idx].contiguous()
+            train_step_bf16(xb, yb, P, G, step_loss)
+            step += 1
+            lst._check(L.lst_t5dnn_adam(P.data_ptr(), G.data_ptr(), M.data_ptr(), V.data_ptr(), step, lr, 0.9, 0.999,
+                                        1e-8, s))
+            tot += step_loss.double() * idx.numel()
+        losses.append(float(tot) / X.shape[0])
+    return P.cpu().numpy().astype(np.float32), losses
+
+
+def gradient_errors(x: np.ndarray, y: np.ndarray, params: np.ndarray, n: int = 8192) -> dict:
+    """Relative errors of the FP32 and the BF16 MFMA gradients (and losses) of the mean BCE against float64
+    autograd on the first n rows: max |g - g64| / max |g64| and ||g - g64|| / ||g64||."""
+    import torch
+    n = min(n, len(y))
+    X = torch.as_tensor(x[:n], dtype=torch.float32, device="cuda")
+    Y = torch.as_tensor(y[:n], dtype=torch.float32, device="cuda")
+    P = torch.as_tensor(params, dtype=torch.float32, device="cuda")
+    P64 = torch.as_tensor(params, dtype=torch.float64).requires_grad_(True)
+    l64 = torch.nn.functional.binary_cross_entropy_with_logits(reference_logits(X.cpu().double(), P64),
+                                                               Y.cpu().double())
+    l64.backward()
+    g64 = P64.grad.numpy()
+    P32 = P.clone().requires_grad_(True)
+    l32 = torch.nn.functional.binary_cross_entropy_with_logits(mlp(X, P32), Y)
+    l32.backward()
+    g32 = P32.grad.cpu().double().numpy()
+    G = torch.zeros_like(P)
+    lb = torch.zeros(1, dtype=torch.float32, device="cuda")
+    train_step_bf16(X, Y, P, G, lb)
+    gb = G.cpu().double().numpy()
+    scale, norm = np.abs(g64).max(), np.linalg.norm(g64)
+    out = {"rows": n}
+    for name, gr, lo in (("fp32", g32, float(l32.detach())), ("bf16_mfma", gb, float(lb))):
+        out[name] = {"grad_max_rel": float(np.abs(gr - g64).max() / scale),
+                     "grad_l2_rel": float(np.linalg.norm(gr - g64) / norm),
+                     "loss_rel": abs(lo - float(l64)) / abs(float(l64))}
+    return out
+
+
+def collect_rows(es, events, capacity: int = 1 << 19):
+    """Training rows (features, labels) of many events, one event handle reused."""
+    import tracklooper_amd as lst
+    e = lst.Event(es, t5_mode=lst.LST_T5_DNN).record_t5dnn(capacity)
+    xs, ys = [], []
+    for ev in events:
+        e.run_lst(ev, "t5")
+        rec = e.t5dnn_record()
+        xs.append(rec["features"])
+        ys.append(labels(rec["hit_idxs"], ev["truth_hit_trk"]))
+    e.close()
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def scores(x: np.ndarray, params: np.ndarray, batch: int = 1 << 20) -> np.ndarray:
+    """Sigmoid scores of rows with the FP32 forward kernel (the chain's arithmetic)."""
+    import torch
+    P = torch.as_tensor(params, dtype=torch.float32, device="cuda")
+    out = []
+    for i in range(0, len(x), batch):
+        X = torch.as_tensor(x[i:i + batch], dtype=torch.float32, device="cuda")
+        out.append(torch.sigmoid(mlp(X, P)).cpu().numpy())
+    return np.concatenate(out) if out else np.zeros(0, np.float32)
+
+
+def selection_metrics(es, events, t5_mode: int, params: np.ndarray | None = None, wp: float | None = None,
+                      slots: int = 64) -> dict:
+    """Efficiency, fake and duplicate rates of the chain on events (batched engine) in a T5 selection mode;
+    in DNN mode with the given parameters and working point (restored afterwards)."""
+    import tracklooper_amd as lst
+    from tracklooper_amd import metrics
+    old_p, old_wp = get_chain_weights(), get_chain_working_point()
+    try:
+        if params is not None:
+            set_chain_weights(params)
+        if wp is not None:
+            set_chain_working_point(wp)
+        b = lst.Batch(es, slots=min(slots, len(events)), t5_mode=t5_mode)
+        outs = b.run(events)
+        b.close()
+    finally:
+        set_chain_weights(old_p)
+        set_chain_working_point(old_wp)
+    return metrics.summarize([metrics.event_metrics(o, ev) for o, ev in zip(outs, events)])
+
+
+def evaluate_selection(es, events, params: np.ndarray, wp: float) -> dict:
+    """The plain chi-squared cut, the shipped network at LSTWP2, and `params` at `wp`, on the same events."""
+    keys = ("efficiency", "fake_rate", "duplicate_rate", "track_candidates", "sim_tracks")
+    res = {"plain_chi2": selection_metrics(es, events, 0),
+           "shipped_dnn_lstwp2": selection_metrics(es, events, 1, load_params(), float(LSTWP2)),
+           "trained_dnn": selection_metrics(es, events, 1, params, wp)}
+    return {k: {q: v[q] for q in keys} | {"by_type": v["by_type"]} for k, v in res.items()}
+
+
+def choose_working_point(es, events, params: np.ndarray, max_eff_loss: float = 0.0025,
+                         grid=tuple(np.round(np.arange(0.05, 0.96, 0.05), 2))) -> tuple[float, list]:
+    """The highest threshold (fewest fakes) whose efficiency on these events is at most `max_eff_loss` below
+    the plain chi-squared cut's; returns it and the scan."""
+    plain = selection_metrics(es, events, 0)
+    scan, best = [], float(grid[0])
+    for wp in grid:
+        m = selection_metrics(es, events, 1, params, float(wp))
+        scan.append({"wp": float(wp), "efficiency": m["efficiency"], "fake_rate": m["fake_rate"]})
+        if m["efficiency"] >= plain["efficiency"] - max_eff_loss:
+            best = float(wp)
+    return best, [{"plain_efficiency": plain["efficiency"], "plain_fake_rate": plain["fake_rate"]}] + scan
+
+
 def main(argv=None):
-    """Collect rows from synthetic PU events, train from the reference parameters, write them."""
+    """Collect rows from synthetic PU200 events, train the T5 DNN (BF16 MFMA steps, or FP32 with --fp32) from
+    the shipped parameters, report the gradient errors against float64, write the parameter files."""
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--events", type=int, default=8)
+    ap.add_argument("--min-rows", type=int, default=0, help="add events until this many rows are collected")
     ap.add_argument("--n-pu", type=int, default=200)
+    ap.add_argument("--geometry", default="OT800_IT615_pt0.8")
     ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--fp32", action="store_true", help="the FP32 autograd path instead of the BF16 MFMA steps")
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", default="t5dnn_trained", help="directory for the parameter files")
     args = ap.parse_args(argv)
-    import tracklooper_amd as lst
+    import time
     from tracklooper_amd import synth
-    es = lst.LSTESData(synth.geometry_dir())
-    events = [synth.generate_event("pu", 1000 + i, n_pu=args.n_pu) for i in range(args.events)]
-    x, y = collect(es, events)
-    p, losses = train(x, y, load_params(), epochs=args.epochs, lr=args.lr)
+    pt = 0.6 if "pt0.6" in args.geometry else 0.8
+    specs = lambda lo, hi: [("pu", 5000 + i, args.n_pu, 2, args.geometry) for i in range(lo, hi)]
+    events = synth.generate_many(specs(0, args.events), args.workers)  # before the GPU is used (fork)
+    import tracklooper_amd as lst
+    es = lst.LSTESData(synth.geometry_dir(args.geometry), pt_cut=pt)
+    x, y = collect_rows(es, events)
+    n_ev = len(events)
+    while len(y) < args.min_rows:  # more events until enough rows (a pool cannot fork now: serial)
+        more = synth.generate_many(specs(n_ev, n_ev + 8), 1)
+        n_ev += len(more)
+        xm, ym = collect_rows(es, more)
+        x, y = np.concatenate([x, xm]), np.concatenate([y, ym])
+    p0 = load_params()
+    errors = gradient_errors(x, y, p0)
+    t0 = time.perf_counter()
+    if args.fp32:
+        p, losses = train(x, y, p0, epochs=args.epochs, batch=args.batch, lr=args.lr)
+    else:
+        p, losses = train_bf16(x, y, p0, epochs=args.epochs, batch=args.batch, lr=args.lr)
+    dt = time.perf_counter() - t0
     write_params(p, args.out)
-    print(json.dumps({"rows": int(len(y)), "signal_fraction": float(y.mean()), "losses": losses,
-                      "out": args.out}), flush=True)
+    print(json.dumps({"events": n_ev, "rows": int(len(y)), "signal_fraction": float(y.mean()),
+                      "precision": "fp32" if args.fp32 else "bf16_mfma", "batch": args.batch, "epochs": args.epochs,
+                      "train_seconds": dt, "steps_per_s": args.epochs * -(-len(y) // args.batch) / dt,
+                      "losses": losses, "gradient_errors_vs_float64": errors, "out": args.out}), flush=True)
 
 
 if __name__ == "__main__":
