@@ -1463,11 +1463,11 @@ namespace lst {
 // (0: the kernel's one-event default), chosen by lst_batch_autotune on 64
 // uneven PU50-250 events on an MI355X; the autotuner refines them per batch.
 const uint32_t kBatchGrid[kNTune] = {
-    128, 16, 128, 32, 32, 0, 32, 128, 16, 0, 32, 0,
-    128, 64, 0, 128, 32, 128, 128, 16, 0, 128, 32, 128,
-    64, 128, 0, 16, 16, 32, 256, 256, 16, 32, 0, 256,
+    128, 16, 128, 32, 32, 0, 32, 128, 16, 128, 32, 0,
+    128, 64, 128, 128, 32, 128, 128, 16, 128, 128, 32, 128,
+    64, 128, 0, 16, 128, 32, 256, 256, 16, 32, 0, 256,
     64, 512, 16, 0, 2048, 128, 256, 16, 512, 16, 32, 32,
-    256, 256, 32, 0, 16, 256, 16, 16,
+    256, 256, 32, 0, 16, 256, 16, 256,
 };
 const char* const kTuneNames[kNTune] = {
     "k_stage_in", "k_event_init", "k_hits", "k_module_ranges", "k_hit_ranges_fix", "k_pixel_segments",

@@ -65,14 +65,21 @@ __device__ __forceinline__ void scan_flag_sums(uint32_t total, uint32_t* sums) {
   if (threadIdx.x == 0) sums[nBlocks] = carry;
 }
 
-// fn(L, rank(L)) for every passing candidate, work-group b taking scan blocks
-// b, b + gridDim.x, ... and its lanes the block's passing candidates
+// fn(L, rank(L)) for every passing candidate: the lanes take the passing
+// ranks r = 0, 1, ... grid-stride, each finding its scan block (the last b with
+// sums[b] <= r, never an empty block) by binary search over the block totals
 template <class Fn>
 __device__ __forceinline__ void for_each_passing(const FlatScan& s, uint32_t total, Fn fn) {
   const uint32_t nBlocks = (total + kScanBlockItems - 1) / kScanBlockItems;
-  for (uint32_t b = blockIdx.x; b < nBlocks; b += gridDim.x) {
-    const uint32_t first = s.sums[b], cnt = s.sums[b + 1] - first;
-    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) fn(s.pass[b * kScanBlockItems + j], first + j);
+  const uint32_t nPass = s.sums[nBlocks];
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nPass; r += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nBlocks;  // sums[lo] <= r < sums[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s.sums[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    fn(s.pass[lo * kScanBlockItems + (r - s.sums[lo])], r);
   }
 }
 
