@@ -220,6 +220,14 @@ lst_status lst_t5dnn_backward(const float* x, const float* params, const float* 
  * deterministic (partials of 64-sample chunks summed in chunk order). */
 lst_status lst_t5dnn_train_step_bf16(const float* x, const float* y, const float* params, float* grad, float* loss,
                                      uint32_t n, void* stream);
+/* lst_t5dnn_train_step_bf16 with every GEMM operand split into two BF16 terms
+ * (hi = BF16(v), lo = BF16(v - hi)) and three MFMAs per product (lo*hi +
+ * hi*lo + hi*hi, Henry, Tang and Heinecke, ARITH 2019): GEMMs of about FP32
+ * accuracy on the matrix cores, for inputs such as the raw T5 features
+ * (z up to 270 cm) whose BF16 rounding is visible in the loss.  Same arguments
+ * and workspace as lst_t5dnn_train_step_bf16. */
+lst_status lst_t5dnn_train_step_bf16x3(const float* x, const float* y, const float* params, float* grad, float* loss,
+                                       uint32_t n, void* stream);
 /* Adam (Kingma and Ba 2015) on the packed parameters in place: m, v the
  * moment estimates (zero before step 1), step = 1, 2, ... */
 lst_status lst_t5dnn_adam(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,

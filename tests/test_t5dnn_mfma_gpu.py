@@ -1,4 +1,5 @@
-"""The BF16 MFMA training step of the T5 DNN (lst_t5dnn_train_step_bf16):
+"""The MFMA training steps of the T5 DNN (lst_t5dnn_train_step_bf16 and the split-operand
+lst_t5dnn_train_step_bf16x3):
 gradients against float64 autograd with the error stated per precision,
 determinism, ragged sizes, a training run, and the chain's settable working
 point against the oracle (GPU)."""
@@ -27,33 +28,41 @@ def bf16(a):
     return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def emulated_bf16_step(x, y, p):
-    """The BF16 step's arithmetic in float64, rounding to bfloat16 exactly where the kernel does (inputs,
-    weights of the two hidden layers, H1 as layer-2 and dW2 operand, dH2 and dH1): pins the kernel's GEMM
-    operand layouts, masks and sums independently of the rounding error itself."""
+def emulated_bf16_step(x, y, p, split=False):
+    """The MFMA step's arithmetic in float64, rounding to bfloat16 exactly where the kernel does: every GEMM
+    operand as bf16(v) (plain), or as the pair bf16(v), bf16(v - bf16(v)) with the three products
+    lo*hi + hi*lo + hi*hi (split).  Pins the kernel's GEMM operand layouts, masks and sums independently of
+    the rounding error itself."""
     from tracklooper_amd import t5dnn
     o, P = 0, {}
     for name, k in t5dnn.TENSORS:
         P[name] = np.asarray(p[o:o + k], np.float64)
         o += k
     n = len(y)
+
+    def M(a, b):
+        ah, bh = bf16(a), bf16(b)
+        r = ah @ bh
+        if split:
+            r = r + bf16(a - ah) @ bh + ah @ bf16(b - bh)
+        return r
+
+    carried = (lambda v: bf16(v) + bf16(v - bf16(v))) if split else bf16  # the value a GEMM operand carries
     W0, W2, W4 = P["wgtT_0"].reshape(38, 32), P["wgtT_2"].reshape(32, 32), P["wgtT_4"]
-    Xb, W0b, W2b = bf16(x), bf16(W0), bf16(W2)
-    H1 = np.maximum(Xb @ W0b + P["bias_0"], 0)
-    A1 = bf16(H1)
-    H2 = np.maximum(A1 @ W2b + P["bias_2"], 0)
+    H1 = np.maximum(M(x, W0) + P["bias_0"], 0)
+    H2 = np.maximum(M(H1, W2) + P["bias_2"], 0)
     z = H2 @ W4 + P["bias_4"][0]
     loss = np.mean(np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
     dl = (1 / (1 + np.exp(-z)) - y) / n
     dH2 = (H2 > 0) * dl[:, None] * W4[None, :]
-    dH2b = bf16(dH2)
-    dH1b = bf16((H1 > 0) * (dH2b @ W2b.T))
-    g = np.concatenate([dH1b.sum(0), (Xb.T @ dH1b).ravel(), dH2.sum(0), (A1.T @ dH2b).ravel(), [dl.sum()],
-                        H2.T @ dl])
+    dH1 = (H1 > 0) * M(dH2, W2.T)
+    g = np.concatenate([carried(dH1).sum(0), M(np.asarray(x, np.float64).T, dH1).ravel(), dH2.sum(0),
+                        M(H1.T, dH2).ravel(), [dl.sum()], H2.T @ dl])
     return g, loss
 
 
-def test_bf16_step_matches_its_float64_emulation(rows):
+@pytest.mark.parametrize("split,tol", [(False, 2e-3), (True, 1e-4)])
+def test_bf16_step_matches_its_float64_emulation(rows, split, tol):
     """The kernel against the same arithmetic in float64 with the same bfloat16 roundings: only the FP32
     accumulation order differs."""
     import torch
@@ -63,22 +72,25 @@ def test_bf16_step_matches_its_float64_emulation(rows):
     p = t5dnn.load_params() + rng.normal(0, 0.01, t5dnn.N_PARAMS).astype(np.float32)
     for xs, ys in ((x[:8192], y[:8192]), (rng.normal(0, 1, (3000, 38)).astype(np.float32),
                                           (rng.uniform(size=3000) < 0.3).astype(np.float32))):
-        g_ref, loss_ref = emulated_bf16_step(xs, ys, p)
+        g_ref, loss_ref = emulated_bf16_step(xs, ys, p, split)
         P = torch.as_tensor(p, device="cuda")
         G = torch.zeros_like(P)
         L = torch.zeros(1, device="cuda")
-        t5dnn.train_step_bf16(torch.as_tensor(xs, device="cuda"), torch.as_tensor(ys, device="cuda"), P, G, L)
+        t5dnn.train_step_bf16(torch.as_tensor(xs, device="cuda"), torch.as_tensor(ys, device="cuda"), P, G, L,
+                              split)
         g = G.cpu().double().numpy()
         rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
-        print(f"kernel vs emulation: gradient L2 rel {rel:.2e}, loss {float(L):.7f} vs {loss_ref:.7f}")
-        assert rel < 2e-3
+        print(f"split={split} kernel vs emulation: gradient L2 rel {rel:.2e}, loss {float(L):.7f} vs {loss_ref:.7f}")
+        assert rel < tol
         assert abs(float(L) - loss_ref) < 1e-5 * max(1.0, loss_ref)
 
 
 def test_bf16_and_fp32_gradients_against_float64(rows):
-    """The error of each precision against float64 autograd, stated: the FP32 kernels agree to a few 1e-7;
-    the BF16 GEMM operands (8-bit mantissa) give a loss within 1e-3 and a gradient within about 10 % in L2
-    (sums over 8192 samples of terms of both signs, each rounded to 3 significant digits)."""
+    """The error of each precision against float64 autograd, stated: the FP32 kernels agree to a few 1e-7.
+    Plain BF16 GEMM operands (8-bit mantissa) on the raw T5 features (z up to 270 cm, radii up to 110 cm)
+    move the logits by about 0.1 and give a loss within a few 1e-3 and a gradient within about 10-25 % in L2
+    (observed 0.08-0.14 on PU200 rows); the split operands (16-bit mantissa, three MFMAs per product) bring
+    both to about 1e-5 (gradient below 1e-3, loss below 5e-4 asserted)."""
     from tracklooper_amd import t5dnn
     _, _, x, y = rows
     assert len(y) > 8192 and 0.02 < y.mean() < 0.98
@@ -87,10 +99,12 @@ def test_bf16_and_fp32_gradients_against_float64(rows):
     err = t5dnn.gradient_errors(x, y, p, n=8192)
     print(err)
     assert err["fp32"]["grad_l2_rel"] < 1e-5 and err["fp32"]["loss_rel"] < 1e-5
-    assert err["bf16_mfma"]["grad_l2_rel"] < 0.15 and err["bf16_mfma"]["loss_rel"] < 1e-3
+    assert err["bf16_mfma"]["grad_l2_rel"] < 0.3 and err["bf16_mfma"]["loss_rel"] < 0.05
+    assert err["bf16x3_mfma"]["grad_l2_rel"] < 1e-3 and err["bf16x3_mfma"]["loss_rel"] < 5e-4
 
 
-def test_bf16_step_deterministic_ragged_and_empty(rows):
+@pytest.mark.parametrize("split", [False, True])
+def test_bf16_step_deterministic_ragged_and_empty(rows, split):
     import torch
     from tracklooper_amd import t5dnn
     _, _, x, y = rows
@@ -100,17 +114,17 @@ def test_bf16_step_deterministic_ragged_and_empty(rows):
         Y = torch.as_tensor(y[:n], device="cuda")
         g1, g2 = torch.zeros_like(P), torch.zeros_like(P)
         l1, l2 = torch.zeros(1, device="cuda"), torch.zeros(1, device="cuda")
-        t5dnn.train_step_bf16(X, Y, P, g1, l1)
-        t5dnn.train_step_bf16(X, Y, P, g2, l2)
+        t5dnn.train_step_bf16(X, Y, P, g1, l1, split)
+        t5dnn.train_step_bf16(X, Y, P, g2, l2, split)
         assert torch.equal(g1, g2) and torch.equal(l1, l2), n
         assert bool(torch.isfinite(g1).all()) and float(l1) > 0
     # the gradient of n rows equals the mean of the per-chunk gradients weighted by rows (chunks of 64)
     X = torch.as_tensor(x[:128], device="cuda")
     Y = torch.as_tensor(y[:128], device="cuda")
     ga, gb, g = torch.zeros_like(P), torch.zeros_like(P), torch.zeros_like(P)
-    t5dnn.train_step_bf16(X[:64].contiguous(), Y[:64].contiguous(), P, ga)
-    t5dnn.train_step_bf16(X[64:].contiguous(), Y[64:].contiguous(), P, gb)
-    t5dnn.train_step_bf16(X, Y, P, g)
+    t5dnn.train_step_bf16(X[:64].contiguous(), Y[:64].contiguous(), P, ga, None, split)
+    t5dnn.train_step_bf16(X[64:].contiguous(), Y[64:].contiguous(), P, gb, None, split)
+    t5dnn.train_step_bf16(X, Y, P, g, None, split)
     torch.testing.assert_close(g, 0.5 * (ga + gb), rtol=1e-5, atol=1e-7)
 
 

@@ -15,7 +15,9 @@
 //    cross-entropy and backward fused per chunk of 64 samples, every GEMM a
 //    v_mfma_f32_16x16x16_bf16 with BF16 operands and FP32 accumulation (the
 //    32 -> 1 output layer and the element-wise steps in FP32), the chunk
-//    partials summed in chunk order as above.
+//    partials summed in chunk order as above.  lst_t5dnn_train_step_bf16x3 is
+//    the same step with every operand split into two BF16 terms and three
+//    MFMAs per product, which brings the GEMMs to about FP32 accuracy.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -149,6 +151,7 @@ __device__ __forceinline__ short f2bf(float x) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (short)(u >> 16);
 }
+__device__ __forceinline__ float bf2f(short s) { return __uint_as_float(((uint32_t)(uint16_t)s) << 16); }
 
 // D = A B + C for 16x16 tiles.  Lane l holds A[l % 16][4 (l / 16) + e],
 // B[4 (l / 16) + e][l % 16] and D[4 (l / 16) + e][l % 16], e = 0..3.
@@ -156,36 +159,60 @@ __device__ __forceinline__ f32x4 mma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// An operand of kT BF16 terms: kT = 1 is the value rounded to BF16; kT = 2
+// adds the BF16 rounding of the remainder (hi + lo carries 16 mantissa bits).
+// The product of two split operands takes three MFMAs, lo*hi + hi*lo + hi*hi
+// (the lo*lo term is below FP32 rounding) -- the "bf16x3" scheme of Henry,
+// Tang and Heinecke, "Leveraging the bfloat16 Artificial Intelligence Datatype
+// For Higher-Precision Computations" (ARITH 2019).
+template <int kT>
+__device__ __forceinline__ void put_split(short* hi, short* lo, float v) {
+  const short h = f2bf(v);
+  *hi = h;
+  if constexpr (kT == 2) *lo = f2bf(v - bf2f(h));
+}
+template <int kT>
+__device__ __forceinline__ f32x4 mma_split(const bf16x4 (&a)[kT], const bf16x4 (&b)[kT], f32x4 c) {
+  if constexpr (kT == 2) {
+    c = mma16(a[1], b[0], c);
+    c = mma16(a[0], b[1], c);
+  }
+  return mma16(a[0], b[0], c);
+}
+
 constexpr int kK1 = 48;              // the 38 inputs padded to three K steps of 16
 constexpr int kStepWaves = 4;        // one 16-sample tile per wave
 constexpr int kStepRows = 16 * kStepWaves;
 
 // One chunk of kStepRows samples per work-group: forward, loss, backward, and the chunk's gradient partial
-// (of the mean loss over n samples) at part[c * kPart], its loss sum at part[c * kPart + kNP].
-__global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict__ x, const float* __restrict__ y,
-                                                         const float* __restrict__ p, float* __restrict__ part,
-                                                         uint32_t n, float invN) {
-  __shared__ short sW0[kK1][kH];         // layer 1 B operand [in][out], inputs 38..47 zero
-  __shared__ short sW2[kH][kH];          // layer 2 B operand [in][out]
-  __shared__ short sW2t[kH][kH];         // dH1 = dH2 W2^T B operand [out][in]
+// (of the mean loss over n samples) at part[c * kPart], its loss sum at part[c * kPart + kNP].  Every GEMM
+// operand has kT BF16 terms (put_split); the 32 -> 1 output layer and the element-wise steps are FP32.
+template <int kT>
+__global__ __launch_bounds__(256) void k_t5dnn_step(const float* __restrict__ x, const float* __restrict__ y,
+                                                    const float* __restrict__ p, float* __restrict__ part,
+                                                    uint32_t n, float invN) {
+  __shared__ short sW0[kT][kK1][kH];         // layer 1 B operand [in][out], inputs 38..47 zero
+  __shared__ short sW2[kT][kH][kH];          // layer 2 B operand [in][out]
+  __shared__ short sW2t[kT][kH][kH];         // dH1 = dH2 W2^T B operand [out][in]
   __shared__ float sB0[kH], sB2[kH], sW4[kH];
-  __shared__ short sX[kStepWaves][16][kK1];
+  __shared__ short sX[kT][kStepWaves][16][kK1];
   __shared__ float sH1[kStepWaves][16][kH], sH2[kStepWaves][16][kH];
-  __shared__ short sA[kStepWaves][16][kH];   // H1 in BF16 (layer 2 A), then dH2 in BF16
-  __shared__ short sD1[kStepWaves][16][kH];  // dH1 in BF16
+  __shared__ short sA[kT][kStepWaves][16][kH];   // H1 (layer 2 A), then dH2
+  __shared__ short sD1[kT][kStepWaves][16][kH];  // dH1
   __shared__ float sDl[kStepWaves][16], sLoss[kStepWaves][16];
-  __shared__ float sP[kStepWaves][kPart];    // per-wave partials, summed in wave order at the end
+  __shared__ float sP[kStepWaves][kPart];        // per-wave partials, summed in wave order at the end
+  constexpr int L = kT - 1;                      // index of the lo term (the hi term when kT = 1)
 
   const int w = threadIdx.x / 64, l = threadIdx.x & 63, m = l & 15, kg = l >> 4;
   for (int t = threadIdx.x; t < kK1 * kH; t += blockDim.x) {
     const int i = t / kH, o = t % kH;
-    sW0[i][o] = i < kIn ? f2bf(p[kW0 + kH * i + o]) : (short)0;
+    put_split<kT>(&sW0[0][i][o], &sW0[L][i][o], i < kIn ? p[kW0 + kH * i + o] : 0.f);
   }
   for (int t = threadIdx.x; t < kH * kH; t += blockDim.x) {
     const int i = t / kH, o = t % kH;
-    const short v = f2bf(p[kW2 + kH * i + o]);
-    sW2[i][o] = v;
-    sW2t[o][i] = v;
+    const float v = p[kW2 + kH * i + o];
+    put_split<kT>(&sW2[0][i][o], &sW2[L][i][o], v);
+    put_split<kT>(&sW2t[0][o][i], &sW2t[L][o][i], v);
   }
   if (threadIdx.x < kH) {
     sB0[threadIdx.x] = p[kB0 + threadIdx.x];
@@ -197,7 +224,7 @@ __global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict
   for (int t = l; t < 16 * kK1; t += 64) {
     const int r = t / kK1, i = t % kK1;
     const uint32_t g = r0 + r;
-    sX[w][r][i] = (i < kIn && g < n) ? f2bf(x[(size_t)kIn * g + i]) : (short)0;
+    put_split<kT>(&sX[0][w][r][i], &sX[L][w][r][i], (i < kIn && g < n) ? x[(size_t)kIn * g + i] : 0.f);
   }
   __syncthreads();
 
@@ -205,19 +232,20 @@ __global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict
   for (int nh = 0; nh < 2; nh++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int ks = 0; ks < 3; ks++) {
-      bf16x4 a, b;
-      for (int e = 0; e < 4; e++) {
-        a[e] = sX[w][m][16 * ks + 4 * kg + e];
-        b[e] = sW0[16 * ks + 4 * kg + e][16 * nh + m];
-      }
-      acc = mma16(a, b, acc);
+      bf16x4 a[kT], b[kT];
+      for (int q = 0; q < kT; q++)
+        for (int e = 0; e < 4; e++) {
+          a[q][e] = sX[q][w][m][16 * ks + 4 * kg + e];
+          b[q][e] = sW0[q][16 * ks + 4 * kg + e][16 * nh + m];
+        }
+      acc = mma_split<kT>(a, b, acc);
     }
     for (int e = 0; e < 4; e++) {
       const int r = 4 * kg + e, c = 16 * nh + m;
       float h = acc[e] + sB0[c];
       h = h > 0.f ? h : 0.f;
       sH1[w][r][c] = h;
-      sA[w][r][c] = f2bf(h);
+      put_split<kT>(&sA[0][w][r][c], &sA[L][w][r][c], h);
     }
   }
   __syncthreads();
@@ -225,12 +253,13 @@ __global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict
   for (int nh = 0; nh < 2; nh++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int ks = 0; ks < 2; ks++) {
-      bf16x4 a, b;
-      for (int e = 0; e < 4; e++) {
-        a[e] = sA[w][m][16 * ks + 4 * kg + e];
-        b[e] = sW2[16 * ks + 4 * kg + e][16 * nh + m];
-      }
-      acc = mma16(a, b, acc);
+      bf16x4 a[kT], b[kT];
+      for (int q = 0; q < kT; q++)
+        for (int e = 0; e < 4; e++) {
+          a[q][e] = sA[q][w][m][16 * ks + 4 * kg + e];
+          b[q][e] = sW2[q][16 * ks + 4 * kg + e][16 * nh + m];
+        }
+      acc = mma_split<kT>(a, b, acc);
     }
     for (int e = 0; e < 4; e++) {
       const int r = 4 * kg + e, c = 16 * nh + m;
@@ -256,10 +285,10 @@ __global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict
     sLoss[w][l] = loss;
   }
   __syncthreads();
-  // dH2 = dlogit W4^T masked by H2 > 0 (BF16 for the GEMMs); output-layer and layer-2 bias partials
+  // dH2 = dlogit W4^T masked by H2 > 0 (split for the GEMMs); output-layer and layer-2 bias partials
   for (int t = l; t < 16 * kH; t += 64) {
     const int r = t / kH, k = t % kH;
-    sA[w][r][k] = f2bf(sH2[w][r][k] > 0.f ? sDl[w][r] * sW4[k] : 0.f);
+    put_split<kT>(&sA[0][w][r][k], &sA[L][w][r][k], sH2[w][r][k] > 0.f ? sDl[w][r] * sW4[k] : 0.f);
   }
   float* P = sP[w];
   if (l < kH) {  // column l: sums over the tile's 16 samples in order
@@ -286,43 +315,53 @@ __global__ __launch_bounds__(256) void k_t5dnn_step_bf16(const float* __restrict
   for (int nh = 0; nh < 2; nh++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int ks = 0; ks < 2; ks++) {
-      bf16x4 a, b;
-      for (int e = 0; e < 4; e++) {
-        a[e] = sA[w][m][16 * ks + 4 * kg + e];
-        b[e] = sW2t[16 * ks + 4 * kg + e][16 * nh + m];
-      }
-      acc = mma16(a, b, acc);
+      bf16x4 a[kT], b[kT];
+      for (int q = 0; q < kT; q++)
+        for (int e = 0; e < 4; e++) {
+          a[q][e] = sA[q][w][m][16 * ks + 4 * kg + e];
+          b[q][e] = sW2t[q][16 * ks + 4 * kg + e][16 * nh + m];
+        }
+      acc = mma_split<kT>(a, b, acc);
     }
     for (int e = 0; e < 4; e++) {
       const int r = 4 * kg + e, c = 16 * nh + m;
-      sD1[w][r][c] = f2bf(sH1[w][r][c] > 0.f ? acc[e] : 0.f);
+      put_split<kT>(&sD1[0][w][r][c], &sD1[L][w][r][c], sH1[w][r][c] > 0.f ? acc[e] : 0.f);
     }
   }
   __syncthreads();
-  if (l < kH) {  // layer-1 bias partial: column sums of dH1 (the BF16 values the GEMM uses)
+  if (l < kH) {  // layer-1 bias partial: column sums of dH1 (the split values the GEMM uses)
     float db0 = 0.f;
-    for (int r = 0; r < 16; r++) db0 += __uint_as_float(((uint32_t)(uint16_t)sD1[w][r][l]) << 16);
+    for (int r = 0; r < 16; r++) {
+      float d = bf2f(sD1[0][w][r][l]);
+      if constexpr (kT == 2) d += bf2f(sD1[1][w][r][l]);
+      db0 += d;
+    }
     P[kB0 + l] = db0;
   }
   // dW2 = H1^T dH2 (in x out) and dW0 = X^T dH1 (inputs x hidden), K = the tile's 16 samples
   for (int mh = 0; mh < 2; mh++)
     for (int nh = 0; nh < 2; nh++) {
-      bf16x4 a, b;
+      bf16x4 a[kT], b[kT];
       for (int e = 0; e < 4; e++) {
-        a[e] = f2bf(sH1[w][4 * kg + e][16 * mh + m]);
-        b[e] = sA[w][4 * kg + e][16 * nh + m];
+        short hl[2];
+        put_split<kT>(&hl[0], &hl[1], sH1[w][4 * kg + e][16 * mh + m]);
+        for (int q = 0; q < kT; q++) {
+          a[q][e] = hl[q];
+          b[q][e] = sA[q][w][4 * kg + e][16 * nh + m];
+        }
       }
-      const f32x4 acc = mma16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
+      const f32x4 acc = mma_split<kT>(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
       for (int e = 0; e < 4; e++) P[kW2 + kH * (16 * mh + 4 * kg + e) + 16 * nh + m] = acc[e];
     }
   for (int mh = 0; mh < 3; mh++)
     for (int nh = 0; nh < 2; nh++) {
-      bf16x4 a, b;
-      for (int e = 0; e < 4; e++) {
-        a[e] = sX[w][4 * kg + e][16 * mh + m];
-        b[e] = sD1[w][4 * kg + e][16 * nh + m];
-      }
-      const f32x4 acc = mma16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
+      bf16x4 a[kT], b[kT];
+      for (int q = 0; q < kT; q++)
+        for (int e = 0; e < 4; e++) {
+          a[q][e] = sX[q][w][4 * kg + e][16 * mh + m];
+          b[q][e] = sD1[q][w][4 * kg + e][16 * nh + m];
+        }
+      const f32x4 acc = mma_split<kT>(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
       for (int e = 0; e < 4; e++) {
         const int i = 16 * mh + 4 * kg + e;
         if (i < kIn) P[kW0 + kH * i + 16 * nh + m] = acc[e];
@@ -382,10 +421,14 @@ float* workspace_for(size_t floats) {
 }
 
 lst_status step_bf16(const float* x, const float* y, const float* params, float* grad, float* loss, uint32_t n,
-                     float* part, hipStream_t s) {
+                     float* part, hipStream_t s, int terms) {
   const uint32_t chunks = (n + kStepRows - 1) / kStepRows;
   if (chunks == 0) return LST_OK;
-  hipLaunchKernelGGL(k_t5dnn_step_bf16, dim3(chunks), dim3(256), 0, s, x, y, params, part, n, 1.f / (float)n);
+  if (terms == 2) {
+    hipLaunchKernelGGL(k_t5dnn_step<2>, dim3(chunks), dim3(256), 0, s, x, y, params, part, n, 1.f / (float)n);
+  } else {
+    hipLaunchKernelGGL(k_t5dnn_step<1>, dim3(chunks), dim3(256), 0, s, x, y, params, part, n, 1.f / (float)n);
+  }
   hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 255) / 256), dim3(256), 0, s, part, chunks, (uint32_t)kPart,
                      (uint32_t)kNP, grad);
   if (loss) {
@@ -430,7 +473,16 @@ lst_status lst_t5dnn_train_step_bf16(const float* x, const float* y, const float
   const uint32_t chunks = (n + kStepRows - 1) / kStepRows;
   float* part = workspace_for((size_t)(chunks ? chunks : 1) * kPart);
   if (!part) return LST_E_DEVICE;
-  return step_bf16(x, y, params, grad, loss, n, part, static_cast<hipStream_t>(stream));
+  return step_bf16(x, y, params, grad, loss, n, part, static_cast<hipStream_t>(stream), 1);
+}
+
+lst_status lst_t5dnn_train_step_bf16x3(const float* x, const float* y, const float* params, float* grad, float* loss,
+                                       uint32_t n, void* stream) {
+  if (!x || !y || !params || !grad) return LST_E_INPUT;
+  const uint32_t chunks = (n + kStepRows - 1) / kStepRows;
+  float* part = workspace_for((size_t)(chunks ? chunks : 1) * kPart);
+  if (!part) return LST_E_DEVICE;
+  return step_bf16(x, y, params, grad, loss, n, part, static_cast<hipStream_t>(stream), 2);
 }
 
 lst_status lst_t5dnn_adam(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,

@@ -375,12 +375,16 @@ def _gen_one(spec):
 
 
 def generate_many(specs, workers: int = 1) -> list:
-    """generate_event over (kind, index, n_pu, config_id, geom_name) tuples, in order.  workers > 1
-    forks a process pool: call it before the process starts using the GPU."""
+    """generate_event over (kind, index, n_pu, config_id, geom_name) tuples, in order.  workers > 1 runs a
+    process pool: forked while the process has not used the GPU, spawned (fresh interpreters that never
+    touch it) once it has."""
     if workers <= 1 or len(specs) < 2:
         return [_gen_one(s) for s in specs]
     import multiprocessing as mp
-    with mp.get_context("fork").Pool(min(workers, len(specs))) as pool:
+    import sys
+    torch, pkg = sys.modules.get("torch"), sys.modules.get("tracklooper_amd")
+    gpu_used = (torch is not None and torch.cuda.is_initialized()) or getattr(pkg, "_lib", None) is not None
+    with mp.get_context("spawn" if gpu_used else "fork").Pool(min(workers, len(specs))) as pool:
         return pool.map(_gen_one, specs, chunksize=1)
 
 

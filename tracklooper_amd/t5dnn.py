@@ -93,6 +93,7 @@ def _lib():
     L.lst_t5dnn_set_weights.argtypes = [P]
     L.lst_t5dnn_get_weights.argtypes = [P]
     L.lst_t5dnn_train_step_bf16.argtypes = [P] * 5 + [C.c_uint32, P]
+    L.lst_t5dnn_train_step_bf16x3.argtypes = [P] * 5 + [C.c_uint32, P]
     L.lst_t5dnn_adam.argtypes = [P] * 4 + [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, P]
     L.lst_t5dnn_set_working_point.argtypes = [C.c_float]
     L.lst_t5dnn_get_working_point.restype = C.c_float
@@ -201,19 +202,22 @@ def train(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 10, b
     return P.detach().cpu().numpy().astype(np.float32), losses
 
 
-def train_step_bf16(X, Y, P, G, loss=None):
-    """One fused forward / loss / backward on the matrix cores: G <- d(mean BCE)/dP, loss <- mean BCE."""
+def train_step_bf16(X, Y, P, G, loss=None, split: bool = False):
+    """One fused forward / loss / backward on the matrix cores: G <- d(mean BCE)/dP, loss <- mean BCE.
+    split: every GEMM operand as two BF16 terms, three MFMAs per product (about FP32 accuracy)."""
     import torch
     lst, L = _lib()
     s = torch.cuda.current_stream(X.device).cuda_stream
-    lst._check(L.lst_t5dnn_train_step_bf16(X.data_ptr(), Y.data_ptr(), P.data_ptr(), G.data_ptr(),
-                                           None if loss is None else loss.data_ptr(), X.shape[0], s))
+    fn = L.lst_t5dnn_train_step_bf16x3 if split else L.lst_t5dnn_train_step_bf16
+    lst._check(fn(X.data_ptr(), Y.data_ptr(), P.data_ptr(), G.data_ptr(), None if loss is None else loss.data_ptr(),
+                  X.shape[0], s))
 
 
 def train_bf16(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 10, batch: int = 8192,
-               lr: float = 1e-3, seed: int = 0, device: str = "cuda") -> tuple[np.ndarray, list[float]]:
-    """Adam on the mean binary cross-entropy, every step one fused BF16 MFMA kernel plus the library's Adam;
-    returns the parameters and the mean loss of each epoch."""
+               lr: float = 1e-3, seed: int = 0, device: str = "cuda", split: bool = True
+               ) -> tuple[np.ndarray, list[float]]:
+    """Adam on the mean binary cross-entropy, every step one fused MFMA kernel (split BF16 operands, or plain
+    BF16 with split=False) plus the library's Adam; returns the parameters and the mean loss of each epoch."""
     import torch
     lst, L = _lib()
     g = torch.Generator().manual_seed(seed)
@@ -230,7 +234,7 @@ def train_bf16(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 
         for i in range(0, X.shape[0], batch):
             idx = perm[i:i + batch]
             xb, yb = X[idx].contiguous(), Y[idx].contiguous()
-            train_step_bf16(xb, yb, P, G, step_loss)
+            train_step_bf16(xb, yb, P, G, step_loss, split)
             step += 1
             lst._check(L.lst_t5dnn_adam(P.data_ptr(), G.data_ptr(), M.data_ptr(), V.data_ptr(), step, lr, 0.9, 0.999,
                                         1e-8, s))
@@ -240,8 +244,8 @@ def train_bf16(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 
 
 
 def gradient_errors(x: np.ndarray, y: np.ndarray, params: np.ndarray, n: int = 8192) -> dict:
-    """Relative errors of the FP32 and the BF16 MFMA gradients (and losses) of the mean BCE against float64
-    autograd on the first n rows: max |g - g64| / max |g64| and ||g - g64|| / ||g64||."""
+    """Relative errors of the FP32, the BF16 MFMA and the split-BF16 MFMA gradients (and losses) of the mean
+    BCE against float64 autograd on the first n rows: max |g - g64| / max |g64| and ||g - g64|| / ||g64||."""
     import torch
     n = min(n, len(y))
     X = torch.as_tensor(x[:n], dtype=torch.float32, device="cuda")
@@ -256,13 +260,15 @@ def gradient_errors(x: np.ndarray, y: np.ndarray, params: np.ndarray, n: int = 8
     l32 = torch.nn.functional.binary_cross_entropy_with_logits(mlp(X, P32), Y)
     l32.backward()
     g32 = P32.grad.cpu().double().numpy()
-    G = torch.zeros_like(P)
-    lb = torch.zeros(1, dtype=torch.float32, device="cuda")
-    train_step_bf16(X, Y, P, G, lb)
-    gb = G.cpu().double().numpy()
+    res = [("fp32", g32, float(l32.detach()))]
+    for name, split in (("bf16_mfma", False), ("bf16x3_mfma", True)):
+        G = torch.zeros_like(P)
+        lb = torch.zeros(1, dtype=torch.float32, device="cuda")
+        train_step_bf16(X, Y, P, G, lb, split)
+        res.append((name, G.cpu().double().numpy(), float(lb)))
     scale, norm = np.abs(g64).max(), np.linalg.norm(g64)
     out = {"rows": n}
-    for name, gr, lo in (("fp32", g32, float(l32.detach())), ("bf16_mfma", gb, float(lb))):
+    for name, gr, lo in res:
         out[name] = {"grad_max_rel": float(np.abs(gr - g64).max() / scale),
                      "grad_l2_rel": float(np.linalg.norm(gr - g64) / norm),
                      "loss_rel": abs(lo - float(l64)) / abs(float(l64))}
@@ -339,8 +345,10 @@ def choose_working_point(es, events, params: np.ndarray, max_eff_loss: float = 0
 
 
 def main(argv=None):
-    """Collect rows from synthetic PU200 events, train the T5 DNN (BF16 MFMA steps, or FP32 with --fp32) from
-    the shipped parameters, report the gradient errors against float64, write the parameter files."""
+    """Collect rows from synthetic PU200 events, train the T5 DNN (MFMA steps, or FP32 autograd) from
+    the shipped parameters, report the gradient errors against float64, write the parameter files.  With
+    --val-events and --eval-events: choose the trained network's working point on the validation events, then
+    compare the plain chi-squared cut, the shipped network and the trained one on the evaluation events."""
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--events", type=int, default=8)
     ap.add_argument("--min-rows", type=int, default=0, help="add events until this many rows are collected")
@@ -349,15 +357,23 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--fp32", action="store_true", help="the FP32 autograd path instead of the BF16 MFMA steps")
+    ap.add_argument("--precision", choices=("bf16x3", "bf16", "fp32"), default="bf16x3",
+                    help="MFMA steps with split BF16 operands, with plain BF16 operands, or the FP32 autograd path")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--out", default="t5dnn_trained", help="directory for the parameter files")
+    ap.add_argument("--val-events", type=int, default=0, help="events for the working-point choice")
+    ap.add_argument("--eval-events", type=int, default=0, help="events for the selection comparison")
     args = ap.parse_args(argv)
     import time
     from tracklooper_amd import synth
     pt = 0.6 if "pt0.6" in args.geometry else 0.8
     specs = lambda lo, hi: [("pu", 5000 + i, args.n_pu, 2, args.geometry) for i in range(lo, hi)]
-    events = synth.generate_many(specs(0, args.events), args.workers)  # before the GPU is used (fork)
+    # every event before the GPU is used (the pool forks); validation and evaluation seeds follow the training ones
+    n_val, n_eval = (args.val_events, args.eval_events) if args.eval_events else (0, 0)
+    held = synth.generate_many(specs(100000, 100000 + n_val + n_eval), args.workers) if n_eval else []
+    val_events, eval_events = held[:n_val], held[n_val:]
+    events = synth.generate_many(specs(0, args.events), args.workers)
+    print(f"generated {len(events)} training and {len(held)} held-out events", file=sys.stderr, flush=True)
     import tracklooper_amd as lst
     es = lst.LSTESData(synth.geometry_dir(args.geometry), pt_cut=pt)
     x, y = collect_rows(es, events)
@@ -367,19 +383,28 @@ def main(argv=None):
         n_ev += len(more)
         xm, ym = collect_rows(es, more)
         x, y = np.concatenate([x, xm]), np.concatenate([y, ym])
+    print(f"{len(y)} rows from {n_ev} events", file=sys.stderr, flush=True)
     p0 = load_params()
     errors = gradient_errors(x, y, p0)
     t0 = time.perf_counter()
-    if args.fp32:
+    if args.precision == "fp32":
         p, losses = train(x, y, p0, epochs=args.epochs, batch=args.batch, lr=args.lr)
     else:
-        p, losses = train_bf16(x, y, p0, epochs=args.epochs, batch=args.batch, lr=args.lr)
+        p, losses = train_bf16(x, y, p0, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                               split=args.precision == "bf16x3")
     dt = time.perf_counter() - t0
     write_params(p, args.out)
-    print(json.dumps({"events": n_ev, "rows": int(len(y)), "signal_fraction": float(y.mean()),
-                      "precision": "fp32" if args.fp32 else "bf16_mfma", "batch": args.batch, "epochs": args.epochs,
-                      "train_seconds": dt, "steps_per_s": args.epochs * -(-len(y) // args.batch) / dt,
-                      "losses": losses, "gradient_errors_vs_float64": errors, "out": args.out}), flush=True)
+    res = {"events": n_ev, "rows": int(len(y)), "signal_fraction": float(y.mean()),
+           "precision": args.precision, "batch": args.batch, "epochs": args.epochs,
+           "train_seconds": dt, "steps_per_s": args.epochs * -(-len(y) // args.batch) / dt,
+           "losses": losses, "gradient_errors_vs_float64": errors, "out": args.out}
+    print(json.dumps(res), flush=True)
+    if eval_events:
+        wp, scan = choose_working_point(es, val_events or eval_events, p)
+        print(f"working point {wp}", file=sys.stderr, flush=True)
+        res = {"geometry": args.geometry, "val_events": len(val_events), "eval_events": len(eval_events),
+               "working_point": wp, "wp_scan": scan, "selection": evaluate_selection(es, eval_events, p, wp)}
+        print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":
