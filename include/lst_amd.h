@@ -228,6 +228,18 @@ lst_status lst_t5dnn_train_step_bf16(const float* x, const float* y, const float
  * and workspace as lst_t5dnn_train_step_bf16. */
 lst_status lst_t5dnn_train_step_bf16x3(const float* x, const float* y, const float* params, float* grad, float* loss,
                                        uint32_t n, void* stream);
+/* lst_t5dnn_train_step_bf16 without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training loops, harnesses that forbid hipMalloc).  The
+ * chunk partials live in a fixed per-device array of the library
+ * (LST_T5DNN_NOALLOC_MAX_ROWS / 64 chunks, about 4.8 MB, held until the
+ * process ends); n above LST_T5DNN_NOALLOC_MAX_ROWS returns LST_E_CAPACITY.
+ * Same kernels and bit-identical results as lst_t5dnn_train_step_bf16.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other threads) race. */
+#define LST_T5DNN_NOALLOC_MAX_ROWS 32768u
+lst_status lst_t5dnn_train_step_bf16_noalloc(const float* x, const float* y, const float* params, float* grad,
+                                             float* loss, uint32_t n, void* stream);
 /* Adam (Kingma and Ba 2015) on the packed parameters in place: m, v the
  * moment estimates (zero before step 1), step = 1, 2, ... */
 lst_status lst_t5dnn_adam(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,

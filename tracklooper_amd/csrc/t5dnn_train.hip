@@ -420,6 +420,34 @@ float* workspace_for(size_t floats) {
   return w.ptr[dev];
 }
 
+// Scratch of the no-allocation BF16 step: the chunk partials of up to
+// kScratchChunks chunks (kScratchChunks * kStepRows samples per call).  Host
+// code uses the device address from hipGetSymbolAddress, never the name.
+constexpr uint32_t kScratchChunks = 512;
+static __device__ __align__(256) float scratch[kScratchChunks * kPart];
+static_assert(kScratchChunks * kStepRows == LST_T5DNN_NOALLOC_MAX_ROWS, "scratch holds the declared maximum");
+struct ScratchAddr {
+  std::mutex mu;
+  float* ptr[kMaxDevices] = {};
+};
+ScratchAddr& scratch_addr() {
+  static ScratchAddr s;
+  return s;
+}
+// this device's copy of `scratch` (looked up once per device, with the device current)
+float* scratch_for_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  ScratchAddr& s = scratch_addr();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (!s.ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    s.ptr[dev] = static_cast<float*>(p);
+  }
+  return s.ptr[dev];
+}
+
 lst_status step_bf16(const float* x, const float* y, const float* params, float* grad, float* loss, uint32_t n,
                      float* part, hipStream_t s, int terms) {
   const uint32_t chunks = (n + kStepRows - 1) / kStepRows;
@@ -483,6 +511,15 @@ lst_status lst_t5dnn_train_step_bf16x3(const float* x, const float* y, const flo
   float* part = workspace_for((size_t)(chunks ? chunks : 1) * kPart);
   if (!part) return LST_E_DEVICE;
   return step_bf16(x, y, params, grad, loss, n, part, static_cast<hipStream_t>(stream), 2);
+}
+
+lst_status lst_t5dnn_train_step_bf16_noalloc(const float* x, const float* y, const float* params, float* grad,
+                                             float* loss, uint32_t n, void* stream) {
+  if (!x || !y || !params || !grad) return LST_E_INPUT;
+  if (n > LST_T5DNN_NOALLOC_MAX_ROWS) return LST_E_CAPACITY;
+  float* part = scratch_for_device();
+  if (!part) return LST_E_DEVICE;
+  return step_bf16(x, y, params, grad, loss, n, part, static_cast<hipStream_t>(stream), 1);
 }
 
 lst_status lst_t5dnn_adam(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,
