@@ -157,8 +157,9 @@ def _plane_alpha(h: Helices, idx, a0, c, n, iters=6):
     return a
 
 
-def _ot_hits(geo: SynthGeometry, h: Helices, rng):
-    """Return hit arrays (sensor row, position, track id)."""
+def _ot_hits(geo: SynthGeometry, h: Helices, rng, exact: bool = False):
+    """Return hit arrays (sensor row, position, track id); exact: the helix crossing points themselves
+    (no smearing, no strip-centre quantisation)."""
     rows, pos, trk = [], [], []
     ntrk = len(h.pt)
     for kind, ref, members, tree in geo.groups:
@@ -205,6 +206,8 @@ def _ot_hits(geo: SynthGeometry, h: Helices, rng):
             strip_len = np.where(t == 25, 5.0, 2.5)
             du_q = np.where(t == 23, np.round(du / 0.15) * 0.15, np.sign(du) * strip_len / 2.0)
             dv_s = dv + rng.normal(0.0, 20e-4, len(dv))
+            if exact:
+                du_q, dv_s = du, dv
             p = c + du_q[:, None] * geo.u[m] + dv_s[:, None] * geo.v[m]
             rows.append(sensor_rows)
             pos.append(p)
@@ -249,7 +252,7 @@ def _pixel_crossings(h: Helices):
 
 def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id: int | None = None,
                    geom: SynthGeometry | None = None, geom_name: str = "OT800_IT615_pt0.8",
-                   max_module_hits: int = 127) -> dict:
+                   max_module_hits: int = 127, exact: bool = False) -> dict:
     """Generate one event.
 
     kind: "muon" (10 muons, pT log-U[0.5,50] GeV) or "pu" (n_pu x 35 charged
@@ -257,6 +260,8 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
     noise hits).  Returns a dict with the ``LST::run`` arguments and truth.
     max_module_hits: hits kept per sensor (127 keeps the input valid for the
     reference's int8 counts; a larger value makes inputs the engine rejects).
+    exact: hits at the exact helix crossings and unsmeared seeds (for checks
+    that hold on perfect helices).
     """
     geo = geom or _geometry_cache(geom_name)
     cid = config_id if config_id is not None else {"muon": 1, "pu": 2}[kind]
@@ -283,7 +288,7 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
     q = rng.choice(np.array([-1.0, 1.0]), nt)
     h = Helices(q, pt, eta, phi0, vz)
 
-    rows, pos, trk = _ot_hits(geo, h, rng)
+    rows, pos, trk = _ot_hits(geo, h, rng, exact)
     if noise_frac > 0:
         nrow, npos = _noise_hits(geo, int(round(noise_frac * len(rows))), rng)
         rows = np.concatenate([rows, nrow])
@@ -316,10 +321,10 @@ def generate_event(kind: str, index: int, *, n_pu: int | None = None, config_id:
             a_last = A[i, nsh - 1]
             r3 = h.pos(np.array([a_last]), [i])[0]
             p3 = h.mom(np.array([a_last]), [i])[0]
-            sm = 1.0 + rng.normal(0.0, 0.01)
+            sm = 1.0 + (0.0 if exact else 1.0) * rng.normal(0.0, 0.01)
             seeds.append(dict(
                 px=h.pt[i] * np.cos(h.phi0[i]) * sm, py=h.pt[i] * np.sin(h.phi0[i]) * sm, pz=h.pt[i] * h.sh[i] * sm,
-                dxy=rng.normal(0.0, 0.002), dz=h.vz[i] + rng.normal(0.0, 0.005),
+                dxy=rng.normal(0.0, 0.002) * (1.0 - exact), dz=h.vz[i] + rng.normal(0.0, 0.005) * (1.0 - exact),
                 ptErr=0.015 * h.pt[i] + 0.005, etaErr=0.002,
                 gx=r3[0], gy=r3[1], gz=r3[2], gpx=p3[0] * sm, gpy=p3[1] * sm, gpz=p3[2] * sm,
                 q=int(h.q[i]), hits=ids[:nsh], trk=int(i)))
