@@ -217,7 +217,7 @@ lst_status lst_t5dnn_backward(const float* x, const float* params, const float* 
  * FP32).  grad [LST_T5DNN_NPARAMS] receives the gradient of the mean loss,
  * *loss (device pointer, may be NULL) the mean loss.  Enqueued on `stream`;
  * uses a workspace the library allocates per device (kept for the process);
- * deterministic (partials of 64-sample chunks summed in chunk order). */
+ * deterministic (partials of 64-sample chunks summed in a fixed order). */
 lst_status lst_t5dnn_train_step_bf16(const float* x, const float* y, const float* params, float* grad, float* loss,
                                      uint32_t n, void* stream);
 /* lst_t5dnn_train_step_bf16 with every GEMM operand split into two BF16 terms

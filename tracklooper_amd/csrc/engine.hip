@@ -51,7 +51,19 @@ struct FieldRef {
   int dtype;
   size_t elem;
   size_t count;
+  size_t stride = 0;  // bytes between elements when they are not contiguous (record fields)
 };
+
+// dst[i] = the elem-byte field of record i (records `stride` bytes apart), i < count; elem is 2, 4 or 8
+__global__ void k_gather_field(const char* __restrict__ src, uint32_t stride, uint32_t elem, size_t count,
+                               char* __restrict__ dst) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const char* p = src + i * stride;
+    if (elem == 8) reinterpret_cast<uint2*>(dst)[i] = *reinterpret_cast<const uint2*>(p);
+    else if (elem == 4) reinterpret_cast<uint32_t*>(dst)[i] = *reinterpret_cast<const uint32_t*>(p);
+    else reinterpret_cast<uint16_t*>(dst)[i] = *reinterpret_cast<const uint16_t*>(p);
+  }
+}
 
 // Input copies of one stage: from device memory, the 4-byte aligned buffers go
 // in one multi-copy launch (others and host sources take hipMemcpyAsync).
@@ -194,6 +206,8 @@ struct lst_event {
   // the launch context of this handle's one event
   lst::Launch launch() { return lst::Launch{v, vDev, 1, stream, tm()}; }
   void* dnnRecordMem = nullptr;        // T5 DNN training rows (lst_event_record_t5dnn)
+  char* gatherMem = nullptr;           // a record field gathered contiguous for lst_event_get_field
+  size_t gatherBytes = 0;
   // prepared inputs kept in pinned host memory (throughput path): copied to the
   // device, described by descDev, at each run
   std::vector<PreparedInput> staged;
@@ -229,9 +243,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #define CARVE(TY, NM, MU) c.take(e.H.NM, (size_t)hitCap * MU);
   LST_HIT_FIELDS(CARVE)
 #undef CARVE
-#define CARVE(TY, NM, MU) c.take(e.D.NM, (size_t)mdCap * MU);
-  LST_MD_FIELDS(CARVE)
-#undef CARVE
+  char* mdRec = nullptr;  // the mini-doublet records (fields set after binding)
+  c.take(mdRec, (size_t)mdCap * lst::kMDRecBytes);
 #define CARVE(TY, NM, MU) c.take(e.S.NM, (size_t)segCap * MU);
   LST_SEG_FIELDS(CARVE)
 #undef CARVE
@@ -281,7 +294,6 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax); c.take(e.W2.hitSlope, hitCap);
   c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDCandMax); c.take(e.W2.cand, lst::kMDCandMax);
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
-  c.take(e.WL.mdSlope, mdCap);
   c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
   c.take(e.WL.cand, lst::kLSCandMax);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
@@ -311,6 +323,14 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   arenaBytes = c.off;
   HIP_OK(hipMalloc(&arena, arenaBytes));
   c.bind(arena);
+  {
+    uint32_t k = 0;
+#define SLOT(TY, NM, MU) e.D.NM.p = mdRec + 4 * k++;
+    LST_MD_FIELDS(SLOT)
+#undef SLOT
+    if (k != lst::kMDFields) return fail(LST_E_DEVICE, "mini-doublet record layout");
+    e.WL.mdSlope.p = mdRec + 4 * lst::kMDSlopeWord;
+  }
 
   e.M = g.view;
   e.mdModuleIndices = g.mdIndices;
@@ -356,11 +376,14 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     auto* cur = &e.H;
     LST_HIT_FIELDS(FIELD)
   }
+  auto addRec = [&](const std::string& name, auto f, size_t n) {
+    using T = std::remove_reference_t<decltype(f[0])>;
+    fields.push_back({name, FieldRef{(void*)f.p, dtype_of<T>(), sizeof(T), n, lst::kMDRecBytes}});
+  };
   {
-    std::string prefix = "mds.";
-    size_t capacity = mdCap;
-    auto* cur = &e.D;
-    LST_MD_FIELDS(FIELD)
+#define MDFIELD(TY, NM, MU) addRec("mds." #NM, e.D.NM, mdCap);
+    LST_MD_FIELDS(MDFIELD)
+#undef MDFIELD
   }
   {
     std::string prefix = "segs.";
@@ -676,6 +699,7 @@ void lst_event_destroy(lst_event* ev) {
     if (ev->timers.stop[k]) (void)hipEventDestroy(ev->timers.stop[k]);
   }
   if (ev->dnnRecordMem) (void)hipFree(ev->dnnRecordMem);
+  if (ev->gatherMem) (void)hipFree(ev->gatherMem);
   if (ev->descDev) (void)hipFree(ev->descDev);
   if (ev->descHost) (void)hipHostFree(ev->descHost);
   if (ev->ownStream) (void)hipStreamDestroy(ev->stream);
@@ -844,7 +868,22 @@ lst_status lst_event_get_field(lst_event* ev, const char* name, void* dst, size_
   if (!dst) return LST_OK;
   const size_t bytes = f->count * f->elem;
   if (dst_bytes < bytes) return fail(LST_E_CAPACITY, "destination too small");
-  HIP_OK(hipMemcpyAsync(dst, f->ptr, bytes, hipMemcpyDeviceToHost, ev->stream));
+  const void* src = f->ptr;
+  if (f->stride) {  // a record field: gathered contiguous on the device first (a debug getter, not the hot path)
+    if (ev->gatherBytes < bytes) {
+      HIP_OK(hipStreamSynchronize(ev->stream));
+      if (ev->gatherMem) HIP_OK(hipFree(ev->gatherMem));
+      ev->gatherMem = nullptr;
+      ev->gatherBytes = 0;
+      HIP_OK(hipMalloc(&ev->gatherMem, bytes));
+      ev->gatherBytes = bytes;
+    }
+    hipLaunchKernelGGL(k_gather_field, dim3(1024), dim3(256), 0, ev->stream, static_cast<const char*>(f->ptr),
+                       (uint32_t)f->stride, (uint32_t)f->elem, f->count, ev->gatherMem);
+    HIP_OK(hipGetLastError());
+    src = ev->gatherMem;
+  }
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ev->stream));
   HIP_OK(hipStreamSynchronize(ev->stream));
   return LST_OK;
 }

@@ -78,7 +78,24 @@
 
 namespace lst {
 struct HitsSoA { LST_HIT_FIELDS(LST_DECL_FIELD) };
-struct MDsSoA { LST_MD_FIELDS(LST_DECL_FIELD) };
+// Mini-doublets are stored as records: the fields of one mini-doublet side by
+// side in 4-byte slots (field k at byte 4k, the per-MD slope pair after the
+// last field), kMDRecBytes per mini-doublet.  Storing or gathering one
+// mini-doublet then touches two cache lines and one page instead of one line
+// and one page per field; D.<field>[i] addresses field <field> of record i.
+constexpr uint32_t kMDFields = 34;
+constexpr uint32_t kMDSlopeWord = kMDFields;  // float2 slope pair (words 34, 35)
+constexpr uint32_t kMDRecBytes = 4 * (kMDSlopeWord + 2);
+static_assert(kMDRecBytes % 16 == 0, "records stay 16-byte aligned");
+template <class T>
+struct MDField {
+  char* p;  // this field's slot in record 0
+  __host__ __device__ __forceinline__ T& operator[](size_t i) const {
+    return *reinterpret_cast<T*>(p + i * kMDRecBytes);
+  }
+};
+#define LST_DECL_MD_FIELD(T, N, M) MDField<T> N;
+struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
 struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
@@ -120,7 +137,7 @@ struct LSWork {
   uint32_t* resIdx;   // [kLSCandMax]
   uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
   uint4* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD, module, outer module)
-  float2* mdSlope;    // [MD capacity] per MD: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
+  MDField<float2> mdSlope;  // in the MD record: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
 };
 
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).

@@ -9,13 +9,13 @@
 //    function of the Python driver): forward and activation gradients one
 //    thread per sample, accumulated in index order as the in-chain inference
 //    does; weight gradients per chunk of samples staged in LDS, then the chunk
-//    partials summed in chunk order, so a gradient does not depend on
-//    scheduling.
+//    partials summed in a fixed order (k_sum_chunks), so a gradient does not
+//    depend on scheduling.
 //  * BF16 on the matrix cores (lst_t5dnn_train_step_bf16): forward, binary
 //    cross-entropy and backward fused per chunk of 64 samples, every GEMM a
 //    v_mfma_f32_16x16x16_bf16 with BF16 operands and FP32 accumulation (the
 //    32 -> 1 output layer and the element-wise steps in FP32), the chunk
-//    partials summed in chunk order as above.  lst_t5dnn_train_step_bf16x3 is
+//    partials summed in the fixed order as above.  lst_t5dnn_train_step_bf16x3 is
 //    the same step with every operand split into two BF16 terms and three
 //    MFMAs per product, which brings the GEMMs to about FP32 accuracy.
 #include <hip/hip_runtime.h>
@@ -131,14 +131,36 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad_partial(const float* __restri
   }
 }
 
-// out[q] = sum over chunks c = 0, 1, ... of part[c * stride + q], q < width
+// Sum of the chunk partials: column q < width of out = sum over chunks c of
+// part[c * stride + q], columns q >= n1 going to out2[q - n1] (when out2 is
+// given).  A work-group takes 64 columns; wave w adds the chunks of its quarter
+// [w * per, (w + 1) * per) in chunk order, then the four wave sums are added in
+// wave order: a fixed order, so the result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part, uint32_t chunks, uint32_t stride,
-                                                    uint32_t width, float* __restrict__ out) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= width) return;
+                                                    uint32_t width, float* __restrict__ out, uint32_t n1,
+                                                    float* __restrict__ out2) {
+  __shared__ float sum[4][64];
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, q = blockIdx.x * 64 + l;
+  const uint32_t per = (chunks + 3) / 4, c0 = w * per, c1 = c0 + per < chunks ? c0 + per : chunks;
   float acc = 0.f;
-  for (uint32_t c = 0; c < chunks; c++) acc += part[(size_t)c * stride + q];
-  out[q] = acc;
+  if (q < width) {
+    uint32_t c = c0;
+    for (; c + 8 <= c1; c += 8) {  // eight loads in flight, added in order
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = part[(size_t)(c + k) * stride + q];
+#pragma unroll
+      for (int k = 0; k < 8; k++) acc += v[k];
+    }
+    for (; c < c1; c++) acc += part[(size_t)c * stride + q];
+  }
+  sum[w][l] = acc;
+  __syncthreads();
+  if (w == 0 && q < width) {
+    const float t = ((sum[0][l] + sum[1][l]) + sum[2][l]) + sum[3][l];
+    if (q < n1) out[q] = t;
+    else if (out2) out2[q - n1] = t;
+  }
 }
 
 // ------------------------------------------------------------------ BF16 MFMA
@@ -457,11 +479,8 @@ lst_status step_bf16(const float* x, const float* y, const float* params, float*
   } else {
     hipLaunchKernelGGL(k_t5dnn_step<1>, dim3(chunks), dim3(256), 0, s, x, y, params, part, n, 1.f / (float)n);
   }
-  hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 255) / 256), dim3(256), 0, s, part, chunks, (uint32_t)kPart,
-                     (uint32_t)kNP, grad);
-  if (loss) {
-    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, s, part + kNP, chunks, (uint32_t)kPart, 1u, loss);
-  }
+  hipLaunchKernelGGL(k_sum_chunks, dim3((kPart + 63) / 64), dim3(256), 0, s, part, chunks, (uint32_t)kPart,
+                     (uint32_t)kPart, grad, (uint32_t)kNP, loss);
   return hipGetLastError() == hipSuccess ? LST_OK : LST_E_DEVICE;
 }
 
@@ -484,14 +503,15 @@ lst_status lst_t5dnn_backward(const float* x, const float* params, const float* 
   const uint32_t chunks = (n + kWRows - 1) / kWRows;
   hipLaunchKernelGGL(k_mlp_backward_act, dim3(grid_for(n)), dim3(256), 0, s, params, h1, h2, dlogit, d2, d1, n);
   if (chunks == 0) {
-    hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 255) / 256), dim3(256), 0, s, grad, 0u, 0u, (uint32_t)kNP, grad);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 63) / 64), dim3(256), 0, s, grad, 0u, 0u, (uint32_t)kNP, grad,
+                       (uint32_t)kNP, nullptr);
     return hipGetLastError() == hipSuccess ? LST_OK : LST_E_DEVICE;
   }
   float* part = workspace_for((size_t)chunks * kNP);
   if (!part) return LST_E_DEVICE;
   hipLaunchKernelGGL(k_mlp_wgrad_partial, dim3(chunks), dim3(256), 0, s, x, h1, h2, dlogit, d2, d1, part, n);
-  hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 255) / 256), dim3(256), 0, s, part, chunks, (uint32_t)kNP,
-                     (uint32_t)kNP, grad);
+  hipLaunchKernelGGL(k_sum_chunks, dim3((kNP + 63) / 64), dim3(256), 0, s, part, chunks, (uint32_t)kNP,
+                     (uint32_t)kNP, grad, (uint32_t)kNP, nullptr);
   return hipGetLastError() == hipSuccess ? LST_OK : LST_E_DEVICE;
 }
 

@@ -225,21 +225,22 @@ def train_bf16(x: np.ndarray, y: np.ndarray, params0: np.ndarray, epochs: int = 
     Y = torch.as_tensor(y, dtype=torch.float32, device=device)
     P = torch.as_tensor(params0, dtype=torch.float32, device=device).clone()
     G, M, V = torch.zeros_like(P), torch.zeros_like(P), torch.zeros_like(P)
-    step_loss = torch.zeros(1, dtype=torch.float32, device=device)
     s = torch.cuda.current_stream(X.device).cuda_stream
     losses, step = [], 0
+    n = X.shape[0]
     for _ in range(epochs):
-        perm = torch.randperm(X.shape[0], generator=g).to(device)
-        tot = torch.zeros(1, dtype=torch.float64, device=device)
-        for i in range(0, X.shape[0], batch):
-            idx = perm[i:i + batch]
-            xb, yb = X[idx].contiguous(), Y[idx].contiguous()
-            train_step_bf16(xb, yb, P, G, step_loss, split)
+        perm = torch.randperm(n, generator=g).to(device)
+        Xp, Yp = X[perm], Y[perm]  # one gather per epoch; the batches are then contiguous slices
+        batch_loss = torch.zeros(-(-n // batch), dtype=torch.float32, device=device)
+        for b, i in enumerate(range(0, n, batch)):
+            xb, yb = Xp[i:i + batch], Yp[i:i + batch]
+            train_step_bf16(xb, yb, P, G, batch_loss[b:], split)
             step += 1
             lst._check(L.lst_t5dnn_adam(P.data_ptr(), G.data_ptr(), M.data_ptr(), V.data_ptr(), step, lr, 0.9, 0.999,
                                         1e-8, s))
-            tot += step_loss.double() * idx.numel()
-        losses.append(float(tot) / X.shape[0])
+        rows = torch.full_like(batch_loss, float(batch), dtype=torch.float64)
+        rows[-1] = n - batch * (len(rows) - 1)
+        losses.append(float((batch_loss.double() * rows).sum()) / n)
     return P.cpu().numpy().astype(np.float32), losses
 
 
