@@ -21,7 +21,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kMaxConn = 64;  // >= the largest connected-module list of a superbin (40)
 constexpr unsigned kPixelWaves = 4096;     // one-wave work-groups striding over the pLS
-constexpr unsigned kPixWG = 256;           // stage A: the waves of a work-group share one pLS
+constexpr unsigned kPixWG = 256;           // stage A and the write: four waves per work-group, one pLS per wave
 constexpr unsigned kPixelFullWaves = 256;  // the same for the overflow kernels, which normally return at once
 constexpr unsigned kTCCap = kNMaxPixelTrackCandidates + kNMaxNonPixelTrackCandidates;
 
@@ -369,35 +369,54 @@ __global__ __launch_bounds__(1024) void k_scan_inplace(LST_VIEWS) {
 template <class K>
 __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(LST_VIEWS) {
   LST_BIND;
-  __shared__ uint32_t pre[kMaxConn + 1];
-  __shared__ uint32_t objBase[kMaxConn];
-  __shared__ uint32_t sMaskBase, sFull;
+  constexpr uint32_t kWavesPerWG = kPixWG / kWave;
+  __shared__ uint32_t preAll[kWavesPerWG][kMaxConn + 1];
+  __shared__ uint32_t objBaseAll[kWavesPerWG][kMaxConn];
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
   const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
-    __syncthreads();  // the previous pLS's prefix in LDS is no longer read
+  uint32_t* pre = preAll[wave];
+  uint32_t* objBase = objBaseAll[wave];
+  // one wave per pLS (no work-group barrier: the waves of a work-group take different pLS)
+  for (uint32_t i = blockIdx.x * kWavesPerWG + wave; i < nPLS; i += gridDim.x * kWavesPerWG) {
+    wave_lds_sync();  // the previous pLS's prefix in LDS is no longer read
     uint32_t total = 0, size = 0;
     const bool plsOk = K::pls_ok(v, i), connOk = v.W.connSize[i] <= (uint32_t)kMaxConn;
-    if (plsOk & connOk) total = conn_prefix_wg<K>(v, i, pre, objBase, size);  // the same branch in every lane
-    const uint32_t words = (total + kWave - 1) / kWave;
-    if (threadIdx.x == 0) {
-      uint32_t mb = ~0u, full = 0;
-      if (words) {
-        const uint32_t b = atomicAdd(&K::shard(v)[i % kPixShards], words);
-        if (b + words <= kShardWords) mb = (i % kPixShards) * kShardWords + b;
-        else full = 1;
+    if (plsOk & connOk) {  // the same branch in every lane
+      size = v.W.connSize[i];
+      const uint32_t base = v.W.connBase[i];
+      unsigned cnt = 0;
+      uint32_t ob = 0;
+      if (lane < size) {
+        const uint32_t m = v.M.connectedPixels[base + lane];
+        if (m < v.M.nLowerModules && K::module_ok(v, m)) {
+          cnt = K::n_obj(v, m);
+          ob = K::obj_base(v, m);
+        }
       }
-      sMaskBase = mb;
-      sFull = full;
+      const unsigned incl = wave_inclusive_scan(cnt);
+      if (lane < size) {
+        objBase[lane] = ob;
+        pre[lane + 1] = incl;
+      }
+      if (lane == 0) pre[0] = 0;
+      wave_lds_sync();
+      total = pre[size];
     }
-    __syncthreads();
-    const uint32_t maskBase = sMaskBase;
+    const uint32_t words = (total + kWave - 1) / kWave;
+    uint32_t maskBase = ~0u;
     bool full = false;
+    if (words) {
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&K::shard(v)[i % kPixShards], words);
+      b = (uint32_t)__shfl((int)b, 0, kWave);
+      if (b + words <= kShardWords) maskBase = (i % kPixShards) * kShardWords + b;
+      else full = true;
+    }
     if (maskBase != ~0u) {
       const uint32_t pls = plsBase + i;
       const PlsCtx pc = load_pls_ctx(v, pls, i);  // once per pLS, not per candidate
-      for (uint32_t w = wave; w < words; w += kPixWG / kWave) {
+      for (uint32_t w = 0; w < words; w++) {
         const uint32_t L = w * kWave + lane;
         typename K::Res r;
         uint32_t t = 0;
@@ -412,7 +431,7 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(LST_VIEWS) {
           const uint32_t n = (uint32_t)__popcll(sm);
           uint32_t sb = 0;
           if (lane == 0) sb = atomicAdd(&K::shard(v)[kPixShards + i % kPixShards], n);
-          sb = __shfl(sb, 0, kWave);
+          sb = (uint32_t)__shfl((int)sb, 0, kWave);
           const uint32_t s = sb + lanes_below(sm);
           if (surv && s < kShardSurv) {
             uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * ((i % kPixShards) * kShardSurv + s)];
@@ -421,12 +440,11 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_a(LST_VIEWS) {
             rec[2] = t;
             K::pack(r, rec + 3);
           }
-          if (sb + n > kShardSurv) full = true;
+          if (sb + n > kShardSurv) full = true;  // the same in every lane
         }
       }
     }
-    full = __syncthreads_or(full) || sFull;
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       K::mask_base(v)[i] = maskBase;
       v.W.pixWords[i] = words;
       K::count(v)[i] = 0;
@@ -526,38 +544,39 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_b(LST_VIEWS) {
   }
 }
 
-// One work-group per pLS: its mask words (kPixWG at a time) are staged in
-// LDS with the exclusive prefix of their pass counts, then each wave takes
-// every fourth word and its lanes write the word's passing candidates (the
-// lanes of the set bits) to their slots, in candidate order.
+// One wave per pLS: the lanes load 64 of its mask words at a time and scan
+// their pass counts in registers; then, for each non-zero word in order (found
+// from the ballot of non-zero words), the lanes of its set bits write their
+// passing candidates to their slots, in candidate order.  No barrier: the
+// waves of a work-group take different pLS.
 template <class K>
 __global__ __launch_bounds__(kPixWG) void k_pix_write(LST_VIEWS) {
   LST_BIND;
-  __shared__ unsigned long long sMask[kPixWG];
-  __shared__ uint32_t sPre[kPixWG];
-  __shared__ unsigned lds[kPixWG / kWave + 1];
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  for (uint32_t i = blockIdx.x; i < nPLS; i += gridDim.x) {
+  const unsigned lane = threadIdx.x % kWave;
+  constexpr uint32_t kWavesPerWG = kPixWG / kWave;
+  for (uint32_t i = blockIdx.x * kWavesPerWG + threadIdx.x / kWave; i < nPLS; i += gridDim.x * kWavesPerWG) {
     const uint32_t maskBase = K::mask_base(v)[i];
     const uint8_t full = v.W.pixFull[i];
-    if (maskBase == ~0u || full) continue;  // the same in every lane
+    if (maskBase == ~0u || full) continue;  // the same in every lane of the wave
     const uint32_t words = v.W.pixWords[i], pls = plsBase + i;
     uint32_t before = K::count(v)[i];
-    for (uint32_t w0 = 0; w0 < words; w0 += kPixWG) {
-      const uint32_t w = w0 + threadIdx.x;
+    for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
+      const uint32_t w = w0 + lane;
       const unsigned long long mk = w < words ? K::mask(v)[maskBase + w] : 0ull;
-      unsigned chunk;
-      const unsigned pre = block_exclusive_scan<kPixWG>((unsigned)__popcll(mk), lds, chunk);
-      sMask[threadIdx.x] = mk;
-      sPre[threadIdx.x] = pre;
-      __syncthreads();
-      const uint32_t nw = words - w0 < kPixWG ? words - w0 : kPixWG;
-      for (uint32_t k = wave; k < nw; k += kPixWG / kWave) {
-        const unsigned long long m = sMask[k];
+      const unsigned cnt = (unsigned)__popcll(mk);
+      const unsigned incl = wave_inclusive_scan(cnt);
+      const unsigned pre = incl - cnt;
+      unsigned long long nz = __ballot(mk != 0ull);
+      while (nz) {
+        const int k = __ffsll((long long)nz) - 1;
+        nz &= nz - 1;
+        const unsigned long long m = ((unsigned long long)(unsigned)__shfl((int)(mk >> 32), k, kWave) << 32) |
+                                     (unsigned)__shfl((int)(unsigned)mk, k, kWave);
+        const unsigned pk = (unsigned)__shfl((int)pre, k, kWave);
         if (((m >> lane) & 1ull) == 0) continue;
-        const uint32_t q = before + sPre[k] + lanes_below(m);
+        const uint32_t q = before + pk + lanes_below(m);
         if (q < K::kCap) {
           const uint32_t s = v.W.pixParkIdx[(maskBase + w0 + k) * kWave + lane];
           const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
@@ -566,8 +585,7 @@ __global__ __launch_bounds__(kPixWG) void k_pix_write(LST_VIEWS) {
           K::write(v, q, pls, i, rec[2], r);
         }
       }
-      before += chunk;
-      __syncthreads();
+      before += wave_last(incl);
     }
   }
 }
