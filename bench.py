@@ -313,6 +313,11 @@ def main():
     for c in first:
         for k, b in algorithmic_bytes(c).items():
             abytes[k] = abytes.get(k, 0) + b
+    # per launch set, averaged over the cycled batches (what a profile of the timed steps sees)
+    mean_bytes = {}
+    for c in counts:
+        for k, b in algorithmic_bytes(c).items():
+            mean_bytes[k] = mean_bytes.get(k, 0) + b * E / len(counts)
     dom = max(batch_kernel_ms, key=batch_kernel_ms.get)
     dom_bytes = abytes.get(dom)
     achieved = dom_bytes / (batch_kernel_ms[dom] * 1e-3) / 1e9 if dom_bytes else None
@@ -357,6 +362,7 @@ def main():
             "max_hits": int(np.max([c["hits"] for c in counts])),
             "mean_track_candidates": float(np.mean([c["tc"] for c in counts])),
             "batch_kernel_ms": batch_kernel_ms,
+            "batch_algorithmic_mb": {k: b / 1e6 for k, b in mean_bytes.items()},
             "batch_kernel_ms_sum": float(sum(batch_kernel_ms.values())),
             "per_event_kernel_ms": {KERNEL_NAMES[k]: float(ktimes[:, k].mean()) for k in range(len(KERNEL_NAMES))},
         },
