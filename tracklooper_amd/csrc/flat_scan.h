@@ -65,14 +65,23 @@ __device__ __forceinline__ void scan_flag_sums(uint32_t total, uint32_t* sums) {
   if (threadIdx.x == 0) sums[nBlocks] = carry;
 }
 
-// fn(L, rank(L)) for every passing candidate: the lanes take the passing
-// ranks r = 0, 1, ... grid-stride, each finding its scan block (the last b with
-// sums[b] <= r, never an empty block) by binary search over the block totals
-template <class Fn>
-__device__ __forceinline__ void for_each_passing(const FlatScan& s, uint32_t total, Fn fn) {
+// fn(L, rank(L)) for every passing candidate, then each(j) for j < nEach:
+// the lanes take the work items grid-stride, first the passing ranks
+// r = 0, 1, ... (each lane finds its rank's scan block, the last b with
+// sums[b] <= r, never an empty block, by binary search over the block totals),
+// then the nEach other items.  One index space, so a lane does one kind of
+// item per step and the two chains of dependent loads run side by side in
+// different lanes instead of one after the other in each.
+template <class Fn, class Each>
+__device__ __forceinline__ void for_each_passing(const FlatScan& s, uint32_t total, uint32_t nEach, Fn fn,
+                                                 Each each) {
   const uint32_t nBlocks = (total + kScanBlockItems - 1) / kScanBlockItems;
   const uint32_t nPass = s.sums[nBlocks];
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nPass; r += gridDim.x * blockDim.x) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nPass + nEach; r += gridDim.x * blockDim.x) {
+    if (r >= nPass) {
+      each(r - nPass);
+      continue;
+    }
     uint32_t lo = 0, hi = nBlocks;  // sums[lo] <= r < sums[hi]
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;

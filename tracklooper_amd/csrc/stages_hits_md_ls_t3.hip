@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_md_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kMDCandTotal], nL = v.M.nLowerModules;
-  for_each_passing(v.FS, total, [&](uint32_t L, uint32_t rankL) {
+  for_each_passing(v.FS, total, nL, [&](uint32_t L, uint32_t rankL) {
     // the loads that need L, then those that need its module, each group issued before any use
     const uint4 cd = v.W2.cand[L];
     const float* w = reinterpret_cast<const float*>(&v.W2.park[kMDParkWords * (size_t)L]);
@@ -378,13 +378,12 @@ __global__ __launch_bounds__(256) void k_md_place(LST_VIEWS) {
     if ((int)rank >= occ) return;
     const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     store_md(v, cd.x, cd.y, m, anchorIsUpper, o, v.mdModuleIndices[m] + rank);
-  });
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+  }, [&](uint32_t m) {  // then per module its counts and range
     const uint32_t count = flag_rank(v.FS, v.W2.candOff[m + 1], total) - flag_rank(v.FS, v.W2.candOff[m], total);
     v.R.totOccupancyMDs[m] = count;
     v.R.nMDs[m] = min(count, (unsigned)max(v.mdModuleOccupancy[m], 0));
     md_range(v, m, v.R.nMDs[m]);
-  }
+  });
 }
 
 // Per-module builder, used when an event exceeds the flattened path's arrays:
@@ -576,7 +575,7 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
-  for_each_passing(v.FS, total, [&](uint32_t L, uint32_t rankL) {
+  for_each_passing(v.FS, total, nL, [&](uint32_t L, uint32_t rankL) {
     // the loads that need L, then those that need its mini-doublets and module, each group issued before any use
     const uint4 c = v.WL.cand[L];
     const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * (size_t)L]);
@@ -595,13 +594,12 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
     v.S.outerMiniDoubletAnchorHitIndices[s] = hitOut;
     v.S.dPhis[s] = w0; v.S.dPhiMins[s] = w1; v.S.dPhiMaxs[s] = w2;
     v.S.dPhiChanges[s] = w3; v.S.dPhiChangeMins[s] = w4; v.S.dPhiChangeMaxs[s] = w5;
-  });
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+  }, [&](uint32_t m) {  // then per module its counts and range
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
     v.R.totOccupancySegments[m] = count;
     v.R.nSegments[m] = min(count, (unsigned)max(v.segModuleOccupancy[m], 0));
     seg_range(v, m, v.R.nSegments[m]);
-  }
+  });
 }
 
 // Per-module builder, used when an event exceeds the flattened path's arrays.
@@ -923,7 +921,7 @@ __global__ __launch_bounds__(256) void k_t3_place(LST_VIEWS) {
     return;
   }
   const uint32_t total = v.R.scalars[kT3List], nL = v.M.nLowerModules;
-  for_each_passing(v.FS, total, [&](uint32_t k, uint32_t rankK) {
+  for_each_passing(v.FS, total, nL, [&](uint32_t k, uint32_t rankK) {
     const uint32_t m1 = v.W3.list[2 * k];
     const uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)k];
     const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
@@ -931,13 +929,12 @@ __global__ __launch_bounds__(256) void k_t3_place(LST_VIEWS) {
     if ((int)rank >= v.R.tripletModuleOccupancy[m1]) return;
     t3_store(v, v.R.tripletModuleIndices[m1] + rank, m1, w0, w1, __uint_as_float(w2), __uint_as_float(w3),
              __uint_as_float(w4), __uint_as_float(w5));
-  });
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nL; m += gridDim.x * blockDim.x) {
+  }, [&](uint32_t m) {  // then per module its counts and range
     const uint32_t count = flag_rank(v.FS, v.W3.survOff[m + 1], total) - flag_rank(v.FS, v.W3.survOff[m], total);
     v.R.totOccupancyTriplets[m] = count;
     v.R.nTriplets[m] = min(count, (unsigned)max(v.R.tripletModuleOccupancy[m], 0));
     t3_range(v, m, v.R.nTriplets[m]);
-  }
+  });
 }
 
 // Per-module builder, used when an event exceeds the flattened path's lists:

@@ -686,7 +686,7 @@ __global__ __launch_bounds__(256) void k_t5_place(LST_VIEWS) {
   LST_BIND;
   if (!t5_flat_done(v)) return;
   const uint32_t total = v.R.scalars[kT5ListA], nElig = v.R.scalars[kNEligibleT5];
-  for_each_passing(v.FS, total, [&](uint32_t k, uint32_t rankK) {
+  for_each_passing(v.FS, total, nElig, [&](uint32_t k, uint32_t rankK) {
     const uint32_t e = v.W5.listA[3 * k];
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
     const uint32_t rank = rankK - flag_rank(v.FS, v.W5.survOff[e], total);
@@ -698,14 +698,13 @@ __global__ __launch_bounds__(256) void k_t5_place(LST_VIEWS) {
     r.rzChi2 = __uint_as_float(w[8]); r.chi2 = __uint_as_float(w[9]); r.nonAnchorChi2 = __uint_as_float(w[10]);
     r.tight = w[11] != 0;
     t5_store(v, (uint32_t)v.R.quintupletModuleIndices[m1] + rank, w[0], w[1], r, v.M.layers[m1]);
-  });
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nElig; e += gridDim.x * blockDim.x) {
+  }, [&](uint32_t e) {  // then per eligible module its counts and range
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
     const uint32_t count = flag_rank(v.FS, v.W5.survOff[e + 1], total) - flag_rank(v.FS, v.W5.survOff[e], total);
     v.R.totOccupancyQuintuplets[m1] = count;
     v.R.nQuintuplets[m1] = min(count, (unsigned)max(v.R.quintupletModuleOccupancy[m1], 0));
     t5_range(v, m1, v.R.nQuintuplets[m1]);
-  }
+  });
 }
 
 // Per-module builder, used when an event has more than kT5CandMax candidates
