@@ -96,6 +96,10 @@ struct MDField {
 };
 #define LST_DECL_MD_FIELD(T, N, M) MDField<T> N;
 struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
+// word of each field in the record (field order of LST_MD_FIELDS)
+#define LST_MD_WORD(T, N, M) kMDw_##N,
+enum MDWord : uint32_t { LST_MD_FIELDS(LST_MD_WORD) kMDWordCount };
+static_assert(kMDWordCount == kMDFields, "one word per mini-doublet field");
 struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };

@@ -153,30 +153,37 @@ __global__ void k_hit_ranges_fix(LST_VIEWS) {
 // SDL/MiniDoublet.h:193-262
 __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, uint32_t upperHit, uint16_t mod,
                                          bool anchorIsUpper, const MDOut& o, uint32_t idx) {
-  const MDsSoA& D = v.D;
   const HitsSoA& H = v.H;
   const uint32_t a = anchorIsUpper ? upperHit : lowerHit;
   const uint32_t b = anchorIsUpper ? lowerHit : upperHit;
-  D.moduleIndices[idx] = mod;
-  D.anchorHitIndices[idx] = a;
-  D.outerHitIndices[idx] = b;
-  D.dphichanges[idx] = o.dphichange; D.dphis[idx] = o.dphi; D.dzs[idx] = o.dz;
-  D.shiftedXs[idx] = o.sx; D.shiftedYs[idx] = o.sy; D.shiftedZs[idx] = o.sz;
-  D.noShiftedDzs[idx] = o.nsdz; D.noShiftedDphis[idx] = o.nsdphi; D.noShiftedDphiChanges[idx] = o.nsdphichange;
+  // the record is built in registers and written with 16-byte stores: one lane writes its record's
+  // 144 contiguous bytes in nine stores instead of 36 scattered 4-byte ones
+  uint32_t w[kMDRecBytes / 4];
+  const auto f = [&](MDWord k, float x) { w[k] = __float_as_uint(x); };
+  w[kMDw_anchorHitIndices] = a;
+  w[kMDw_outerHitIndices] = b;
+  w[kMDw_moduleIndices] = mod;  // the u16 field in the low half of its slot
+  f(kMDw_dphichanges, o.dphichange); f(kMDw_dphis, o.dphi); f(kMDw_dzs, o.dz);
+  f(kMDw_shiftedXs, o.sx); f(kMDw_shiftedYs, o.sy); f(kMDw_shiftedZs, o.sz);
+  f(kMDw_noShiftedDzs, o.nsdz); f(kMDw_noShiftedDphis, o.nsdphi); f(kMDw_noShiftedDphiChanges, o.nsdphichange);
   const float ahx = H.highEdgeXs[a], ahy = H.highEdgeYs[a], alx = H.lowEdgeXs[a], aly = H.lowEdgeYs[a];
-  D.anchorX[idx] = H.xs[a]; D.anchorY[idx] = H.ys[a]; D.anchorZ[idx] = H.zs[a];
-  D.anchorRt[idx] = H.rts[a]; D.anchorPhi[idx] = H.phis[a]; D.anchorEta[idx] = H.etas[a];
+  f(kMDw_anchorX, H.xs[a]); f(kMDw_anchorY, H.ys[a]); f(kMDw_anchorZ, H.zs[a]);
+  f(kMDw_anchorRt, H.rts[a]); f(kMDw_anchorPhi, H.phis[a]); f(kMDw_anchorEta, H.etas[a]);
+  f(kMDw_anchorHighEdgeX, ahx); f(kMDw_anchorHighEdgeY, ahy); f(kMDw_anchorLowEdgeX, alx); f(kMDw_anchorLowEdgeY, aly);
+  f(kMDw_anchorHighEdgePhi, m_atan2(ahy, ahx));
+  f(kMDw_anchorLowEdgePhi, m_atan2(aly, alx));
+  f(kMDw_outerX, H.xs[b]); f(kMDw_outerY, H.ys[b]); f(kMDw_outerZ, H.zs[b]);
+  f(kMDw_outerRt, H.rts[b]); f(kMDw_outerPhi, H.phis[b]); f(kMDw_outerEta, H.etas[b]);
+  f(kMDw_outerHighEdgeX, H.highEdgeXs[b]); f(kMDw_outerHighEdgeY, H.highEdgeYs[b]);
+  f(kMDw_outerLowEdgeX, H.lowEdgeXs[b]); f(kMDw_outerLowEdgeY, H.lowEdgeYs[b]);
   {  // the segment slope of this MD as the outer one: the anchor hit's slope (same formula) and its tan
     const float sl = v.W2.hitSlope[a];
-    v.WL.mdSlope[idx] = make_float2(sl, m_tan(sl));
+    w[kMDSlopeWord] = __float_as_uint(sl);
+    w[kMDSlopeWord + 1] = __float_as_uint(m_tan(sl));
   }
-  D.anchorHighEdgeX[idx] = ahx; D.anchorHighEdgeY[idx] = ahy; D.anchorLowEdgeX[idx] = alx; D.anchorLowEdgeY[idx] = aly;
-  D.anchorHighEdgePhi[idx] = m_atan2(ahy, ahx);
-  D.anchorLowEdgePhi[idx] = m_atan2(aly, alx);
-  D.outerX[idx] = H.xs[b]; D.outerY[idx] = H.ys[b]; D.outerZ[idx] = H.zs[b];
-  D.outerRt[idx] = H.rts[b]; D.outerPhi[idx] = H.phis[b]; D.outerEta[idx] = H.etas[b];
-  D.outerHighEdgeX[idx] = H.highEdgeXs[b]; D.outerHighEdgeY[idx] = H.highEdgeYs[b];
-  D.outerLowEdgeX[idx] = H.lowEdgeXs[b]; D.outerLowEdgeY[idx] = H.lowEdgeYs[b];
+  uint4* rec = reinterpret_cast<uint4*>(v.D.anchorHitIndices.p + (size_t)idx * kMDRecBytes);
+#pragma unroll
+  for (uint32_t q = 0; q < kMDRecBytes / 16; q++) rec[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
 }
 
 // ------------------------------------------------------------- pixel segments
