@@ -74,9 +74,9 @@ def algorithmic_bytes(n: dict) -> dict:
       k_md_cand_fill     : write the 16 B record of each candidate hit pair
       k_md_place         : per mini-doublet read its candidate record (16 B) and parked results (36 B) and the
                            derived fields of its two hits (84 B), write the 134 B mini-doublet
-      k_ls_cand_fill     : write the 16 B record of each candidate mini-doublet pair, read 8 B per mini-doublet
-      k_ls_place         : per segment read its candidate record (16 B), parked results (24 B) and two anchor
-                           hit indices (8 B), write the 44 B segment
+      k_ls_cand_fill     : write the 8 B record of each candidate mini-doublet pair, read 8 B per mini-doublet
+      k_ls_place         : per segment read its candidate record (8 B), parked results (24 B), the two
+                           mini-doublets' anchor hits and modules (12 B), write the 44 B segment
       k_t3_place         : per triplet read its list entry (8 B) and results (24 B), write the 60 B triplet
       k_t5_place         : per quintuplet read its list entry (12 B) and results (48 B), write the 119 B quintuplet
       k_pix_write<PT5Kind>: per pLS its mask words (8 B) and per pixel quintuplet its parked result (40 B) and
@@ -103,8 +103,8 @@ def algorithmic_bytes(n: dict) -> dict:
         "k_out_copy": 136 * tc,
         "k_md_cand_fill": 16 * n["mdcand"],
         "k_md_place": 270 * md,
-        "k_ls_cand_fill": 16 * n["lscand"] + 8 * md,
-        "k_ls_place": 92 * ls,
+        "k_ls_cand_fill": 8 * n["lscand"] + 8 * md,
+        "k_ls_place": 88 * ls,
         "k_t3_place": 92 * t3,
         "k_t5_place": 179 * t5,
         "k_pix_write<PT5Kind>": 8 * pls + 190 * pt5,

@@ -140,7 +140,7 @@ struct LSWork {
   uint32_t* candOff;  // [nLower + 1]
   uint32_t* resIdx;   // [kLSCandMax]
   uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
-  uint4* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD, module, outer module)
+  uint2* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD); the modules are in the MD records
   MDField<float2> mdSlope;  // in the MD record: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
 };
 

@@ -519,11 +519,11 @@ __global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
       }
       const uint32_t nk = nConn - k0 < (uint32_t)kWave ? nConn - k0 : (uint32_t)kWave;
       for (uint32_t kk = 0; kk < nk; kk++) {
-        const uint32_t gOm = __shfl(om, kk, kWave), gOut = __shfl(nOut, kk, kWave);
+        const uint32_t gOut = __shfl(nOut, kk, kWave);
         const uint32_t gBase = __shfl(outBase, kk, kWave), gOff = __shfl(off, kk, kWave);
         const uint32_t n = nIn * gOut;
         for (uint32_t l = lane; l < n; l += kWave)
-          v.WL.cand[gOff + l] = make_uint4(inBase + l / gOut, gBase + l % gOut, m, gOm);
+          v.WL.cand[gOff + l] = make_uint2(inBase + l / gOut, gBase + l % gOut);
       }
     }
   }
@@ -538,8 +538,8 @@ __global__ __launch_bounds__(256) void k_ls_eval(LST_VIEWS) {
     bool pass = false;
     SegOut o{0, 0, 0, 0, 0, 0};
     if (L < total) {
-      const uint4 c = v.WL.cand[L];
-      const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
+      const uint2 c = v.WL.cand[L];
+      const uint32_t inMD = c.x, outMD = c.y, m = v.D.moduleIndices[inMD], om = v.D.moduleIndices[outMD];
       const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
       const float2 bs = v.WL.mdSlope[outMD];
@@ -584,10 +584,10 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
   const uint32_t total = v.R.scalars[kLSCandTotal], nL = v.M.nLowerModules;
   for_each_passing(v.FS, total, nL, [&](uint32_t L, uint32_t rankL) {
     // the loads that need L, then those that need its mini-doublets and module, each group issued before any use
-    const uint4 c = v.WL.cand[L];
+    const uint2 c = v.WL.cand[L];
     const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * (size_t)L]);
     const float w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
-    const uint32_t inMD = c.x, outMD = c.y, m = c.z, om = c.w;
+    const uint32_t inMD = c.x, outMD = c.y, m = v.D.moduleIndices[inMD], om = v.D.moduleIndices[outMD];
     const uint32_t off = v.WL.candOff[m];
     const int occ = v.segModuleOccupancy[m];
     const uint32_t segBase = v.segModuleIndices[m];
