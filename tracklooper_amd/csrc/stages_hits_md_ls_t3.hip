@@ -497,8 +497,12 @@ __device__ __forceinline__ bool ls_flat_done(const EventView& v) {
 }
 
 
-// the candidates written out, L by L (one wave per module; the connected
-// modules' values are loaded by the lanes at once, then walked with shuffles)
+// the candidates written out, L by L: one wave per module, its connected
+// modules 64 at a time in the lanes (outer MD count and first MD, and the
+// exclusive prefix of their candidate counts); the lanes then take the
+// module's candidates of those connections in order, each finding its
+// connection by a binary search over the prefix (shuffles), so consecutive
+// lanes write consecutive records whatever the per-connection counts
 __global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
   LST_BIND;
   if (!ls_flat_mode(v)) return;
@@ -510,20 +514,33 @@ __global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
     if (nConn == 0) continue;
     const uint32_t base = v.WL.candOff[m], inBase = (uint32_t)v.R.mdRanges[2 * m];
     for (uint32_t k0 = 0; k0 < nConn; k0 += kWave) {
-      uint32_t om = 0, nOut = 0, outBase = 0, off = 0;
+      uint32_t nOut = 0, outBase = 0;
       if (k0 + lane < nConn) {
-        om = v.M.moduleMap[(size_t)m * v.M.stride + k0 + lane];
+        const uint32_t om = v.M.moduleMap[(size_t)m * v.M.stride + k0 + lane];
         nOut = v.R.nMDs[om];
         outBase = nOut ? (uint32_t)v.R.mdRanges[2 * om] : 0u;
-        off = base + v.WL.connPre[(size_t)m * S + k0 + lane];
       }
+      const uint32_t cnt = nIn * nOut;
+      const uint32_t incl = wave_inclusive_scan(cnt), pre = incl - cnt;  // candidates before connection k0 + lane
+      const uint32_t chunk = wave_last(incl);
       const uint32_t nk = nConn - k0 < (uint32_t)kWave ? nConn - k0 : (uint32_t)kWave;
-      for (uint32_t kk = 0; kk < nk; kk++) {
-        const uint32_t gOut = __shfl(nOut, kk, kWave);
-        const uint32_t gBase = __shfl(outBase, kk, kWave), gOff = __shfl(off, kk, kWave);
-        const uint32_t n = nIn * gOut;
-        for (uint32_t l = lane; l < n; l += kWave)
-          v.WL.cand[gOff + l] = make_uint2(inBase + l / gOut, gBase + l % gOut);
+      const uint32_t first = base + v.WL.connPre[(size_t)m * S + k0];  // L of the chunk's first candidate
+      for (uint32_t l0 = 0; l0 < chunk; l0 += kWave) {
+        const uint32_t l = l0 + lane;
+        // the connection k of candidate l: the largest k < nk with pre_k <= l (every lane takes part)
+        uint32_t k = 0;
+        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+          const uint32_t c = k + step;
+          const uint32_t pc = (uint32_t)__shfl((int)pre, (int)(c < nk ? c : nk - 1), kWave);
+          if (c < nk && pc <= l) k = c;
+        }
+        const uint32_t pk = (uint32_t)__shfl((int)pre, (int)k, kWave);
+        const uint32_t gOut = (uint32_t)__shfl((int)nOut, (int)k, kWave);
+        const uint32_t gBase = (uint32_t)__shfl((int)outBase, (int)k, kWave);
+        if (l < chunk) {
+          const uint32_t r = l - pk;
+          v.WL.cand[first + l] = make_uint2(inBase + r / gOut, gBase + r % gOut);
+        }
       }
     }
   }
