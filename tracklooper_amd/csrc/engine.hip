@@ -51,14 +51,16 @@ struct FieldRef {
   int dtype;
   size_t elem;
   size_t count;
-  size_t stride = 0;  // bytes between elements when they are not contiguous (record fields)
+  size_t stride = 0;  // bytes between records when the field is in records (0: a plain array)
+  size_t mult = 1;    // elements of the field per record (4 bytes apart)
 };
 
-// dst[i] = the elem-byte field of record i (records `stride` bytes apart), i < count; elem is 2, 4 or 8
-__global__ void k_gather_field(const char* __restrict__ src, uint32_t stride, uint32_t elem, size_t count,
-                               char* __restrict__ dst) {
+// dst[i] = element i of a record field: word i % mult of the field in record i / mult (records `stride`
+// bytes apart), i < count; elem is 2, 4 or 8 bytes
+__global__ void k_gather_field(const char* __restrict__ src, uint32_t stride, uint32_t elem, uint32_t mult,
+                               size_t count, char* __restrict__ dst) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
-    const char* p = src + i * stride;
+    const char* p = src + (i / mult) * stride + (i % mult) * 4;
     if (elem == 8) reinterpret_cast<uint2*>(dst)[i] = *reinterpret_cast<const uint2*>(p);
     else if (elem == 4) reinterpret_cast<uint32_t*>(dst)[i] = *reinterpret_cast<const uint32_t*>(p);
     else reinterpret_cast<uint16_t*>(dst)[i] = *reinterpret_cast<const uint16_t*>(p);
@@ -245,9 +247,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #undef CARVE
   char* mdRec = nullptr;  // the mini-doublet records (fields set after binding)
   c.take(mdRec, (size_t)mdCap * lst::kMDRecBytes);
-#define CARVE(TY, NM, MU) c.take(e.S.NM, (size_t)segCap * MU);
-  LST_SEG_FIELDS(CARVE)
-#undef CARVE
+  char* segRec = nullptr;  // the segment records (fields set after binding)
+  c.take(segRec, (size_t)segCap * lst::kSegRecBytes);
 #define CARVE(TY, NM, MU) c.take(e.P.NM, (size_t)plsCap * MU);
   LST_PLS_FIELDS(CARVE)
 #undef CARVE
@@ -330,6 +331,11 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #undef SLOT
     if (k != lst::kMDFields) return fail(LST_E_DEVICE, "mini-doublet record layout");
     e.WL.mdSlope.p = mdRec + 4 * lst::kMDSlopeWord;
+    uint32_t w = 0;
+#define SLOT(TY, NM, MU) e.S.NM.p = segRec + 4 * w; w += MU;
+    LST_SEG_FIELDS(SLOT)
+#undef SLOT
+    if (w != lst::kSegRecWords) return fail(LST_E_DEVICE, "segment record layout");
   }
 
   e.M = g.view;
@@ -376,20 +382,19 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     auto* cur = &e.H;
     LST_HIT_FIELDS(FIELD)
   }
-  auto addRec = [&](const std::string& name, auto f, size_t n) {
+  auto addRec = [&](const std::string& name, auto f, size_t n, size_t stride, size_t mult) {
     using T = std::remove_reference_t<decltype(f[0])>;
-    fields.push_back({name, FieldRef{(void*)f.p, dtype_of<T>(), sizeof(T), n, lst::kMDRecBytes}});
+    fields.push_back({name, FieldRef{(void*)f.p, dtype_of<T>(), sizeof(T), n, stride, mult}});
   };
   {
-#define MDFIELD(TY, NM, MU) addRec("mds." #NM, e.D.NM, mdCap);
+#define MDFIELD(TY, NM, MU) addRec("mds." #NM, e.D.NM, mdCap, lst::kMDRecBytes, 1);
     LST_MD_FIELDS(MDFIELD)
 #undef MDFIELD
   }
   {
-    std::string prefix = "segs.";
-    size_t capacity = segCap;
-    auto* cur = &e.S;
-    LST_SEG_FIELDS(FIELD)
+#define SEGFIELD(TY, NM, MU) addRec("segs." #NM, e.S.NM, (size_t)segCap * MU, lst::kSegRecBytes, MU);
+    LST_SEG_FIELDS(SEGFIELD)
+#undef SEGFIELD
   }
   {
     std::string prefix = "segs.";
@@ -879,7 +884,7 @@ lst_status lst_event_get_field(lst_event* ev, const char* name, void* dst, size_
       ev->gatherBytes = bytes;
     }
     hipLaunchKernelGGL(k_gather_field, dim3(1024), dim3(256), 0, ev->stream, static_cast<const char*>(f->ptr),
-                       (uint32_t)f->stride, (uint32_t)f->elem, f->count, ev->gatherMem);
+                       (uint32_t)f->stride, (uint32_t)f->elem, (uint32_t)f->mult, f->count, ev->gatherMem);
     HIP_OK(hipGetLastError());
     src = ev->gatherMem;
   }

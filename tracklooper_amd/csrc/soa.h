@@ -100,7 +100,35 @@ struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
 #define LST_MD_WORD(T, N, M) kMDw_##N,
 enum MDWord : uint32_t { LST_MD_FIELDS(LST_MD_WORD) kMDWordCount };
 static_assert(kMDWordCount == kMDFields, "one word per mini-doublet field");
-struct SegsSoA { LST_SEG_FIELDS(LST_DECL_FIELD) };
+// Segments are records too: 12 four-byte words per segment (the two MD
+// indices, then one word per other field, in LST_SEG_FIELDS order; the u16
+// module indices in the low half of their word), 48 bytes.  The triplet, T5
+// and pixel selections read two to six fields of a segment they reach
+// through an index: one cache line instead of one per field.  Element i of a
+// field with M entries per segment (mdIndices: M = 2) is word i % M of the
+// field in record i / M.
+template <class T, uint32_t M, uint32_t REC>
+struct RecField {
+  char* p;  // the field's first word in record 0
+  __host__ __device__ __forceinline__ T& operator[](size_t i) const {
+    return *reinterpret_cast<T*>(p + (i / M) * REC + (i % M) * 4);
+  }
+};
+#define LST_SEG_MULT(T, N, M) M,
+constexpr uint32_t kSegMult[] = {LST_SEG_FIELDS(LST_SEG_MULT)};
+#define LST_SEG_INDEX(T, N, M) kSegF_##N,
+enum SegFieldIndex : uint32_t { LST_SEG_FIELDS(LST_SEG_INDEX) kSegFieldCount };
+// first word of field number f in a segment record
+constexpr uint32_t seg_word(uint32_t f) {
+  uint32_t w = 0;
+  for (uint32_t i = 0; i < f; i++) w += kSegMult[i];
+  return w;
+}
+constexpr uint32_t kSegRecWords = seg_word(kSegFieldCount);
+constexpr uint32_t kSegRecBytes = 4 * kSegRecWords;
+static_assert(kSegRecBytes % 16 == 0, "segment records stay 16-byte aligned");
+#define LST_DECL_SEG_FIELD(T, N, M) RecField<T, M, kSegRecBytes> N;
+struct SegsSoA { LST_SEG_FIELDS(LST_DECL_SEG_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
 struct T5SoA { LST_T5_FIELDS(LST_DECL_FIELD) };

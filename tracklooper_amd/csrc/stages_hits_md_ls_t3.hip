@@ -186,6 +186,28 @@ __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, 
   for (uint32_t q = 0; q < kMDRecBytes / 16; q++) rec[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
 }
 
+// One segment record built in registers and written with three 16-byte stores.
+// d: dPhi, dPhiMin, dPhiMax, dPhiChange, dPhiChangeMin, dPhiChangeMax.
+__device__ __forceinline__ void store_seg(const EventView& v, uint32_t s, uint32_t inMD, uint32_t outMD, uint16_t m,
+                                          uint16_t om, uint32_t hitIn, uint32_t hitOut, const float (&d)[6]) {
+  uint32_t w[kSegRecWords];
+  w[seg_word(kSegF_mdIndices)] = inMD;
+  w[seg_word(kSegF_mdIndices) + 1] = outMD;
+  w[seg_word(kSegF_innerLowerModuleIndices)] = m;  // u16 fields in the low half of their word
+  w[seg_word(kSegF_outerLowerModuleIndices)] = om;
+  w[seg_word(kSegF_innerMiniDoubletAnchorHitIndices)] = hitIn;
+  w[seg_word(kSegF_outerMiniDoubletAnchorHitIndices)] = hitOut;
+  w[seg_word(kSegF_dPhis)] = __float_as_uint(d[0]);
+  w[seg_word(kSegF_dPhiMins)] = __float_as_uint(d[1]);
+  w[seg_word(kSegF_dPhiMaxs)] = __float_as_uint(d[2]);
+  w[seg_word(kSegF_dPhiChanges)] = __float_as_uint(d[3]);
+  w[seg_word(kSegF_dPhiChangeMins)] = __float_as_uint(d[4]);
+  w[seg_word(kSegF_dPhiChangeMaxs)] = __float_as_uint(d[5]);
+  uint4* rec = reinterpret_cast<uint4*>(v.S.mdIndices.p + (size_t)s * kSegRecBytes);
+#pragma unroll
+  for (uint32_t q = 0; q < kSegRecBytes / 16; q++) rec[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
 // ------------------------------------------------------------- pixel segments
 // SDL/Segment.h:1031-1117 and addPixelSegmentToMemory SDL/Segment.h:392-452
 __global__ void k_pixel_segments(LST_VIEWS) {
@@ -209,10 +231,10 @@ __global__ void k_pixel_segments(LST_VIEWS) {
     const float icpt = v.H.zs[h0] - slope * v.H.rts[h0];
     float score = (v.H.rts[h2] * slope + icpt) - v.H.zs[h2];
     score = score * score;
-    v.S.mdIndices[2 * seg] = innerMD; v.S.mdIndices[2 * seg + 1] = outerMD;
-    v.S.innerLowerModuleIndices[seg] = pix; v.S.outerLowerModuleIndices[seg] = pix;
-    v.S.innerMiniDoubletAnchorHitIndices[seg] = h0; v.S.outerMiniDoubletAnchorHitIndices[seg] = h2;
-    v.S.dPhiChanges[seg] = v.plsInDPhiChange[t];
+    {  // the reference writes no dPhi, dPhiMin/Max or dPhiChangeMin/Max for a pLS (SDL/Segment.h:392-411): 0 here
+      const float d[6] = {0.f, 0.f, 0.f, v.plsInDPhiChange[t], 0.f, 0.f};
+      store_seg(v, seg, innerMD, outerMD, pix, pix, h0, h2, d);
+    }
     v.P.isDup[t] = 0;
     v.P.partOfPT5[t] = 0;
     v.P.score[t] = score;
@@ -612,12 +634,8 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
     const uint32_t rank = rankL - flag_rank(v.FS, off, total);
     if ((int)rank >= occ) return;
     const uint32_t s = segBase + rank;
-    v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
-    v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
-    v.S.innerMiniDoubletAnchorHitIndices[s] = hitIn;
-    v.S.outerMiniDoubletAnchorHitIndices[s] = hitOut;
-    v.S.dPhis[s] = w0; v.S.dPhiMins[s] = w1; v.S.dPhiMaxs[s] = w2;
-    v.S.dPhiChanges[s] = w3; v.S.dPhiChangeMins[s] = w4; v.S.dPhiChangeMaxs[s] = w5;
+    const float d[6] = {w0, w1, w2, w3, w4, w5};
+    store_seg(v, s, inMD, outMD, m, om, hitIn, hitOut, d);
   }, [&](uint32_t m) {  // then per module its counts and range
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
     v.R.totOccupancySegments[m] = count;
@@ -685,13 +703,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(LST_VIEWS) {
       const unsigned rank = count + build_rank(pass, lds, n);
       if (pass && (int)rank < cap) {
         const uint32_t s = segBase + rank;
-        v.S.mdIndices[2 * s] = inMD; v.S.mdIndices[2 * s + 1] = outMD;
-        v.S.innerLowerModuleIndices[s] = m; v.S.outerLowerModuleIndices[s] = om;
-        v.S.innerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[inMD];
-        v.S.outerMiniDoubletAnchorHitIndices[s] = v.D.anchorHitIndices[outMD];
-        v.S.dPhis[s] = o.dPhi; v.S.dPhiMins[s] = o.dPhiMin; v.S.dPhiMaxs[s] = o.dPhiMax;
-        v.S.dPhiChanges[s] = o.dPhiChange; v.S.dPhiChangeMins[s] = o.dPhiChangeMin;
-        v.S.dPhiChangeMaxs[s] = o.dPhiChangeMax;
+        const float d[6] = {o.dPhi, o.dPhiMin, o.dPhiMax, o.dPhiChange, o.dPhiChangeMin, o.dPhiChangeMax};
+        store_seg(v, s, inMD, outMD, m, om, v.D.anchorHitIndices[inMD], v.D.anchorHitIndices[outMD], d);
       }
       count += n;
     }
