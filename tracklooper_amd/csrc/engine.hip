@@ -56,14 +56,15 @@ struct FieldRef {
 };
 
 // dst[i] = element i of a record field: word i % mult of the field in record i / mult (records `stride`
-// bytes apart), i < count; elem is 2, 4 or 8 bytes
+// bytes apart), i < count; elem is 1, 2, 4 or 8 bytes
 __global__ void k_gather_field(const char* __restrict__ src, uint32_t stride, uint32_t elem, uint32_t mult,
                                size_t count, char* __restrict__ dst) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
     const char* p = src + (i / mult) * stride + (i % mult) * 4;
     if (elem == 8) reinterpret_cast<uint2*>(dst)[i] = *reinterpret_cast<const uint2*>(p);
     else if (elem == 4) reinterpret_cast<uint32_t*>(dst)[i] = *reinterpret_cast<const uint32_t*>(p);
-    else reinterpret_cast<uint16_t*>(dst)[i] = *reinterpret_cast<const uint16_t*>(p);
+    else if (elem == 2) reinterpret_cast<uint16_t*>(dst)[i] = *reinterpret_cast<const uint16_t*>(p);
+    else reinterpret_cast<uint8_t*>(dst)[i] = *reinterpret_cast<const uint8_t*>(p);
   }
 }
 
@@ -252,9 +253,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #define CARVE(TY, NM, MU) c.take(e.P.NM, (size_t)plsCap * MU);
   LST_PLS_FIELDS(CARVE)
 #undef CARVE
-#define CARVE(TY, NM, MU) c.take(e.T.NM, (size_t)(t3Cap ? t3Cap : 1) * MU);
-  LST_T3_FIELDS(CARVE)
-#undef CARVE
+  char* t3Rec = nullptr;  // the triplet records (fields set after binding)
+  c.take(t3Rec, (size_t)(t3Cap ? t3Cap : 1) * lst::kT3ObjBytes);
 #define CARVE(TY, NM, MU) c.take(e.Q.NM, (size_t)(t5Cap ? t5Cap : 1) * MU);
   LST_T5_FIELDS(CARVE)
 #undef CARVE
@@ -336,6 +336,11 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     LST_SEG_FIELDS(SLOT)
 #undef SLOT
     if (w != lst::kSegRecWords) return fail(LST_E_DEVICE, "segment record layout");
+    w = 0;
+#define SLOT(TY, NM, MU) e.T.NM.p = t3Rec + 4 * w; w += MU;
+    LST_T3_FIELDS(SLOT)
+#undef SLOT
+    if (w > lst::kT3ObjWords) return fail(LST_E_DEVICE, "triplet record layout");
   }
 
   e.M = g.view;
@@ -403,10 +408,9 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     LST_PLS_FIELDS(FIELD)
   }
   {
-    std::string prefix = "t3.";
-    size_t capacity = t3Cap;
-    auto* cur = &e.T;
-    LST_T3_FIELDS(FIELD)
+#define T3FIELD(TY, NM, MU) addRec("t3." #NM, e.T.NM, (size_t)t3Cap * MU, lst::kT3ObjBytes, MU);
+    LST_T3_FIELDS(T3FIELD)
+#undef T3FIELD
   }
   {
     std::string prefix = "t5.";

@@ -130,7 +130,21 @@ static_assert(kSegRecBytes % 16 == 0, "segment records stay 16-byte aligned");
 #define LST_DECL_SEG_FIELD(T, N, M) RecField<T, M, kSegRecBytes> N;
 struct SegsSoA { LST_SEG_FIELDS(LST_DECL_SEG_FIELD) };
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
-struct T3SoA { LST_T3_FIELDS(LST_DECL_FIELD) };
+// Triplets as records as well: one 4-byte word per element (u16 and u8
+// fields in the low bytes of their words), 21 words padded to 24 (96 bytes).
+#define LST_T3_MULT(T, N, M) M,
+constexpr uint32_t kT3Mult[] = {LST_T3_FIELDS(LST_T3_MULT)};
+#define LST_T3_INDEX(T, N, M) kT3F_##N,
+enum T3FieldIndex : uint32_t { LST_T3_FIELDS(LST_T3_INDEX) kT3FieldCount };
+constexpr uint32_t t3_word(uint32_t f) {
+  uint32_t w = 0;
+  for (uint32_t i = 0; i < f; i++) w += kT3Mult[i];
+  return w;
+}
+constexpr uint32_t kT3ObjWords = (t3_word(kT3FieldCount) + 3) & ~3u;
+constexpr uint32_t kT3ObjBytes = 4 * kT3ObjWords;
+#define LST_DECL_T3_FIELD(T, N, M) RecField<T, M, kT3ObjBytes> N;
+struct T3SoA { LST_T3_FIELDS(LST_DECL_T3_FIELD) };
 struct T5SoA { LST_T5_FIELDS(LST_DECL_FIELD) };
 struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
 struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };

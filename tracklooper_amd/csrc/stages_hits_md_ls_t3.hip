@@ -771,17 +771,27 @@ __device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_
   const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
   const uint32_t md1 = v.S.mdIndices[2 * inSeg], mdB = v.S.mdIndices[2 * inSeg + 1];
   const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
-  v.T.segmentIndices[2 * t] = inSeg; v.T.segmentIndices[2 * t + 1] = outSeg;
-  v.T.lowerModuleIndices[3 * t] = m1; v.T.lowerModuleIndices[3 * t + 1] = m2;
-  v.T.lowerModuleIndices[3 * t + 2] = m3;
-  v.T.betaIn[t] = betaIn; v.T.circleRadius[t] = radius; v.T.circleCenterX[t] = gx; v.T.circleCenterY[t] = gy;
-  v.T.logicalLayers[3 * t] = v.M.layers[m1] + (v.M.subdets[m1] == kEndcap) * 6;
-  v.T.logicalLayers[3 * t + 1] = v.M.layers[m2] + (v.M.subdets[m2] == kEndcap) * 6;
-  v.T.logicalLayers[3 * t + 2] = v.M.layers[m3] + (v.M.subdets[m3] == kEndcap) * 6;
-  v.T.hitIndices[6 * t + 0] = v.D.anchorHitIndices[md1]; v.T.hitIndices[6 * t + 1] = v.D.outerHitIndices[md1];
-  v.T.hitIndices[6 * t + 2] = v.D.anchorHitIndices[mdB]; v.T.hitIndices[6 * t + 3] = v.D.outerHitIndices[mdB];
-  v.T.hitIndices[6 * t + 4] = v.D.anchorHitIndices[md3]; v.T.hitIndices[6 * t + 5] = v.D.outerHitIndices[md3];
-  v.T.partOfPT5[t] = 0; v.T.partOfT5[t] = 0; v.T.partOfPT3[t] = 0;
+  // the record built in registers (u16 and u8 fields in the low bytes of their words, flags 0) and
+  // written with 16-byte stores
+  uint32_t w[kT3ObjWords] = {};
+  w[t3_word(kT3F_segmentIndices)] = inSeg;
+  w[t3_word(kT3F_segmentIndices) + 1] = outSeg;
+  const uint16_t mods[3] = {(uint16_t)m1, m2, m3};
+  const uint32_t mds[3] = {md1, mdB, md3};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    w[t3_word(kT3F_lowerModuleIndices) + k] = mods[k];
+    w[t3_word(kT3F_logicalLayers) + k] = (uint8_t)(v.M.layers[mods[k]] + (v.M.subdets[mods[k]] == kEndcap) * 6);
+    w[t3_word(kT3F_hitIndices) + 2 * k] = v.D.anchorHitIndices[mds[k]];
+    w[t3_word(kT3F_hitIndices) + 2 * k + 1] = v.D.outerHitIndices[mds[k]];
+  }
+  w[t3_word(kT3F_betaIn)] = __float_as_uint(betaIn);
+  w[t3_word(kT3F_circleRadius)] = __float_as_uint(radius);
+  w[t3_word(kT3F_circleCenterX)] = __float_as_uint(gx);
+  w[t3_word(kT3F_circleCenterY)] = __float_as_uint(gy);
+  uint4* rec = reinterpret_cast<uint4*>(v.T.segmentIndices.p + (size_t)t * kT3ObjBytes);
+#pragma unroll
+  for (uint32_t q = 0; q < kT3ObjBytes / 16; q++) rec[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
 }
 
 // ----------------------------------------------------- flattened T3 builder
