@@ -11,6 +11,14 @@ the r-z windows (observed 95.9 % and 98.3 %).  A cut with a wrong sign, a
 swapped operand or a mistyped threshold loses whole classes of true
 combinations and fails these bounds.
 
+Pixel quintuplets (pLS of the track's seed with one of its true quintuplets on
+a module the pLS's superbin connects): 98.5 % built on exact helices and 98.3 %
+at resolution (asserted >= 97 %; the misses are forward, |eta| 1.5-2.3, where
+the synthetic pixel disks are a rough model).  Pixel triplets are built only
+for what the pT5 builder left (pLS not in a pT5, triplets not in one), a few
+dozen combinations biased to those the pT5 cuts rejected: 8 of 12 and 31 of 45
+(asserted >= 50 %, which a broken cut would miss by far).
+
 On exact helices the fitted quantities of the true quintuplets have closed
 forms: the three-hit circle radii (inner, bridge, outer) and the regression
 radius equal the helix radius pT / (0.3 B), and the r-phi and r-z chi-squared
@@ -34,12 +42,17 @@ def conn(oracle_geometry):
 def _run(oracle_geometry, conn, exact):
     from oracle import orc
     from tracklooper_amd import synth
-    tot, rows = {}, []
+    pmap = truth.PixelMap(oracle_geometry)
+    tot, rows = {"pt5": [0, 0], "pt3": [0, 0]}, []
     for i in range(N_EVENTS):
         ev = synth.generate_event("muon", 100 + i, exact=exact)
-        o = orc.Event(oracle_geometry).run_lst(ev, "t5")
+        o = orc.Event(oracle_geometry).run_lst(ev, "pt3")
         c = truth.truth_combinations(o, ev, conn)
         truth.add_counts(tot, c)
+        p = truth.pixel_truth(o, ev, conn, pmap, c)
+        for k in ("pt5", "pt3"):
+            tot[k][0] += p[k][0]
+            tot[k][1] += p[k][1]
         if c["t5_true"]:
             k = np.array([s for s, _ in c["t5_true"]])
             t = np.array([t for _, t in c["t5_true"]])
@@ -57,15 +70,16 @@ def exact_run(oracle_geometry, conn):
 def test_true_combinations_built_on_exact_helices(exact_run):
     tot, _ = exact_run
     print({k: f"{a}/{b}" for k, (a, b) in tot.items()})
+    floor = {"md": 0.99, "ls": 0.99, "t3": 0.99, "t5": 0.99, "pt5": 0.97, "pt3": 0.5}
     for stage, (built, allowed) in tot.items():
-        assert allowed > 300, stage
-        assert built / allowed >= 0.99, f"{stage}: {built} of {allowed} true combinations built"
+        assert allowed > (300 if stage != "pt3" else 5), stage
+        assert built / allowed >= floor[stage], f"{stage}: {built} of {allowed} true combinations built"
 
 
 def test_true_combinations_built_at_detector_resolution(oracle_geometry, conn):
     tot, _ = _run(oracle_geometry, conn, False)
     print({k: f"{a}/{b}" for k, (a, b) in tot.items()})
-    floor = {"md": 0.99, "ls": 0.99, "t3": 0.94, "t5": 0.96}
+    floor = {"md": 0.99, "ls": 0.99, "t3": 0.94, "t5": 0.96, "pt5": 0.97, "pt3": 0.5}
     for stage, (built, allowed) in tot.items():
         assert built / allowed >= floor[stage], f"{stage}: {built} of {allowed} true combinations built"
 

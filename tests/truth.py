@@ -13,7 +13,12 @@ built are counted:
   rejects outright (sdl layers 12-13-14, 13-14-15, 14-15-16,
   SDL/Triplet.h:241-298);
 * quintuplets: two of its built triplets sharing a mini-doublet
-  (SDL/Quintuplet.h:2988-3115).
+  (SDL/Quintuplet.h:2988-3115);
+* pixel quintuplets and triplets (`pixel_truth`): a pLS of the track's own
+  seed with one of the track's built quintuplets (triplets) on a module the
+  pLS's superbin connects, under the builders' own preconditions (pLS not a
+  duplicate, object not a duplicate / not already part of a pT5, PS module,
+  SDL/PixelTriplet.h:2679-2788 and :1026-1164).
 
 `run` is anything with ``field(name)`` returning the stage arrays under the
 names of the engine's field table (the oracle's Event or the engine's Event).
@@ -109,6 +114,7 @@ def truth_combinations(run, ev: dict, conn: Connectivity, pt_min: float = 1.0) -
                 if k is not None:
                     out["t3"][0] += 1
                     true_t3.setdefault(t, []).append(k)
+    out["t3_true"] = [(k, t) for t, ks in true_t3.items() for k in ks]
 
     qti = run.field("t5.tripletIndices")
     t5_of = {(int(qti[2 * k]), int(qti[2 * k + 1])): k
@@ -134,3 +140,80 @@ def add_counts(total: dict, one: dict) -> dict:
         total[k][0] += one[k][0]
         total[k][1] += one[k][1]
     return total
+
+
+class PixelMap:
+    """The superbin -> connected lower modules lists of a geometry, per pixel type (0 all, 1 pos, 2 neg),
+    laid out as the builders read them (SDL/Event.cc:813-816)."""
+
+    SIZE_SUPERBINS = 45000
+
+    def __init__(self, geom):
+        self.modules = geom.field("connectedPixels").astype(np.int64)
+        idx = [geom.field(n).astype(np.int64) for n in
+               ("connectedPixelsIndex", "connectedPixelsIndexPos", "connectedPixelsIndexNeg")]
+        size = [geom.field(n).astype(np.int64) for n in
+                ("connectedPixelsSizes", "connectedPixelsSizesPos", "connectedPixelsSizesNeg")]
+        last = self.SIZE_SUPERBINS - 1
+        off_pos = idx[0][last] + size[0][last]
+        off_neg = idx[1][last] + size[1][last] + off_pos
+        self.index = [idx[0], idx[1] + off_pos, idx[2] + off_neg]
+        self.size = size
+        self.module_type = geom.field("moduleType").astype(np.int64)
+        self.layers = geom.field("layers").astype(np.int64)
+
+    def connected(self, superbin: int, pixel_type: int) -> np.ndarray:
+        if superbin < 0 or superbin >= self.SIZE_SUPERBINS or not 0 <= pixel_type <= 2:
+            return np.zeros(0, np.int64)
+        b = self.index[pixel_type][superbin]
+        return self.modules[b:b + self.size[pixel_type][superbin]]
+
+
+PS_MODULE = 0  # moduleType of a PS module (SDL/Module.h)
+
+
+def pixel_truth(run, ev: dict, conn: Connectivity, pmap: PixelMap, comb: dict, pt_min: float = 1.0) -> dict:
+    """{"pt5"|"pt3": [built, allowed]}: run after the pixel triplets (its pLS duplicate flags are those of the
+    first cleaning pass); comb = truth_combinations of the same run."""
+    nL = conn.n_lower
+    n_pls = int(run.scalar("nPLS"))
+    pls_base = int(run.field("ranges.segmentModuleIndices")[nL])
+    seed_of = run.field("segs.seedIdx")[:n_pls].astype(np.int64)
+    trk_of_seed = ev["truth_seed_trk"]
+    pt = ev["truth_pt"]
+    is_dup = run.field("segs.isDup")[:n_pls]
+    part_pt5 = run.field("segs.partOfPT5")[:n_pls]
+    superbin = run.field("segs.superbin")[:n_pls].astype(np.int64)
+    ptype = run.field("segs.pixelType")[:n_pls].astype(np.int64)
+    t5_mod = run.field("t5.lowerModuleIndices").astype(np.int64)
+    t5_dup = run.field("t5.isDup")
+    t3_mod = run.field("t3.lowerModuleIndices").astype(np.int64)
+    t3_part5 = run.field("t3.partOfPT5")
+    n5, n3 = int(run.scalar("nPixelQuintuplets")), int(run.scalar("nPixelTriplets"))
+    pt5 = set(zip(run.field("pt5.pixelIndices")[:n5].tolist(), run.field("pt5.T5Indices")[:n5].tolist()))
+    pt3 = set(zip(run.field("pt3.pixelSegmentIndices")[:n3].tolist(), run.field("pt3.tripletIndices")[:n3].tolist()))
+    t5_by_trk, t3_by_trk = {}, {}
+    for k, t in comb["t5_true"]:
+        t5_by_trk.setdefault(t, []).append(k)
+    for k, t in comb["t3_true"]:
+        t3_by_trk.setdefault(t, []).append(k)
+    out = {"pt5": [0, 0], "pt3": [0, 0]}
+    for i in range(n_pls):
+        t = int(trk_of_seed[seed_of[i]])
+        if t < 0 or pt[t] <= pt_min or is_dup[i]:
+            continue
+        mods = set(int(m) for m in pmap.connected(int(superbin[i]), int(ptype[i]))
+                   if m < nL and pmap.module_type[m] == PS_MODULE)
+        for k in t5_by_trk.get(t, []):  # SDL/PixelTriplet.h:2679-2788
+            if int(t5_mod[5 * k]) in mods and not t5_dup[k]:
+                out["pt5"][1] += 1
+                out["pt5"][0] += (pls_base + i, k) in pt5
+        if part_pt5[i]:
+            continue
+        for k in t3_by_trk.get(t, []):  # SDL/PixelTriplet.h:1026-1164
+            m = int(t3_mod[3 * k])
+            if (m in mods and pmap.layers[m] in (1, 2) and pmap.module_type[int(t3_mod[3 * k + 1])] == PS_MODULE
+                    and not t3_part5[k]):
+                out["pt3"][1] += 1
+                out["pt3"][0] += (pls_base + i, k) in pt3
+    return out
