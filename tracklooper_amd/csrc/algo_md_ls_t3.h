@@ -75,14 +75,18 @@ __device__ __forceinline__ float md_dphi_cut(const ModCtx& c, float ptCut, float
   return slope + m_sqrt(muls * muls + pvOff * pvOff + lum * lum);
 }
 
-// strip-hit shift along the line of sight of the PS pixel hit: SDL/MiniDoublet.h:403-567
-__device__ __forceinline__ void md_shift(const ModCtx& c, float xL, float yL, float zL, float rtL, float xU, float yU,
-                                         float zU, float rtU, float& xn, float& yn, float& zn) {
-  float xp, yp, zp, rtp, xo, yo;
-  const bool lowerIsPixelSide =
-      c.type == kPS ? (c.subdet == kBarrel ? c.layerType != kPixelLayer : c.layerType == kPixelLayer) : true;
-  if (lowerIsPixelSide) { xo = xU; yo = yU; xp = xL; yp = yL; zp = zL; rtp = rtL; }
-  else { xo = xL; yo = yL; xp = xU; yp = yU; zp = zU; rtp = rtU; }
+// strip-hit shift along the line of sight of the PS pixel hit: SDL/MiniDoublet.h:403-567.
+// Everything but the last step depends only on the module pair and the hit on
+// its pixel side, so it is computed once per such hit (md_shift_point, in
+// k_hits) instead of once per hit pair; md_shift_pair finishes it for a pair.
+// The same expressions on the same floats: the results are those of the
+// reference's per-pair function.
+__device__ __forceinline__ bool md_lower_is_pixel_side(const ModCtx& c) {
+  return c.type == kPS ? (c.subdet == kBarrel ? c.layerType != kPixelLayer : c.layerType == kPixelLayer) : true;
+}
+__device__ __forceinline__ bool md_shifts(const ModCtx& c) { return c.subdet == kEndcap || c.side != kCenter; }
+// c: the lower module of the pair; (xp, yp, zp, rtp): its pixel-side hit
+__device__ __forceinline__ float4 md_shift_point(const ModCtx& c, float xp, float yp, float zp, float rtp) {
   const float slope = c.dxdy;
   const float angleA = m_abs(m_atan(rtp / zp));
   const float angleB = c.subdet == kEndcap ? kPi / 2.f : m_atan(c.drdz);
@@ -97,58 +101,78 @@ __device__ __forceinline__ void md_shift(const ModCtx& c, float xL, float yL, fl
   else angleM = 2.f * kPi - absAtanSlope;
   const float xa = xp + drprime * m_sin(angleM);
   const float ya = yp + drprime * m_cos(angleM);
+  const float absdz = m_abs(sep / m_sin(angleA + angleB) * m_cos(angleA));
+  const float abszn = c.layerType == kPixelLayer ? m_abs(zp) + absdz : m_abs(zp) - absdz;
+  return make_float4(xa, ya, abszn * ((zp > 0) ? 1 : -1), 0.f);
+}
+// the shifted strip hit from the pixel-side shift point sp and the other hit (xo, yo)
+__device__ __forceinline__ void md_shift_pair(const ModCtx& c, float4 sp, float xo, float yo, float& xn, float& yn) {
+  const float slope = c.dxdy, xa = sp.x, ya = sp.y;
   if (slope == kSdlInf) { xn = xa; yn = yo; }
   else if (slope == 0) { xn = xo; yn = ya; }
   else {
     xn = (slope * xa + (1.f / slope) * xo - ya + yo) / (slope + (1.f / slope));
     yn = (xn - xa) * slope + ya;
   }
-  const float absdz = m_abs(sep / m_sin(angleA + angleB) * m_cos(angleA));
-  const float abszn = c.layerType == kPixelLayer ? m_abs(zp) + absdz : m_abs(zp) - absdz;
-  zn = abszn * ((zp > 0) ? 1 : -1);
 }
 
 struct MDOut {
   float dz, dphi, dphichange, sx, sy, sz, nsdz, nsdphi, nsdphichange;
 };
 
+// A candidate hit pair: coordinates, the hits' phi (phi_of of their x, y, kept per hit) and, for a module
+// pair that shifts its strip hits, the shift point of the pixel-side hit.  A deltaPhi of two points whose
+// phi is known is the wrapped difference of those phis (the same floats the reference computes).
+struct MDPair {
+  float xL, yL, zL, rtL, phiL, xU, yU, zU, rtU, phiU;
+  float4 sp;
+};
+// deltaPhiChange(x1, y1, x2, y2) with phi1 = phi_of(x1, y1) known (SDL/Hit.h:140-149)
+__device__ __forceinline__ float md_dphi_change(float x1, float y1, float phi1, float x2, float y2) {
+  return phi_mpi_pi(phi_of(x2 - x1, y2 - y1) - phi1);
+}
+
 // SDL/MiniDoublet.h:645-772
-__device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, float xL, float yL, float zL, float rtL,
-                                          float xU, float yU, float zU, float rtU, float slope, MDOut& o) {
+__device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, const MDPair& p, float slope, MDOut& o) {
+  const float xL = p.xL, yL = p.yL, zL = p.zL, rtL = p.rtL, xU = p.xU, yU = p.yU, zU = p.zU, rtU = p.rtU;
   o.dz = zL - zU;
   const float dzCut = c.type == kPS ? 2.f : 10.f;
   const float sign = ((o.dz > 0) - (o.dz < 0)) * ((zL > 0) - (zL < 0));
   const float crosser = (m_abs(o.dz) > 2) * sign;
   if (!(m_abs(o.dz) < dzCut && crosser <= 0)) return false;
   const float cut = md_dphi_cut(c, ptCut, slope, 0.f, 0.f);
-  float shiftedRt = 0.f;
+  float shiftedRt = 0.f, phiS = 0.f;
   if (c.side != kCenter) {
-    float xn, yn, zn;
-    md_shift(c, xL, yL, zL, rtL, xU, yU, zU, rtU, xn, yn, zn);
+    float xn, yn;
+    const bool lowerPixelSide = md_lower_is_pixel_side(c);
+    md_shift_pair(c, p.sp, lowerPixelSide ? xU : xL, lowerPixelSide ? yU : yL, xn, yn);
     o.sx = xn; o.sy = yn;
     shiftedRt = m_sqrt(xn * xn + yn * yn);
+    phiS = phi_of(xn, yn);
     if (c.layerType == kPixelLayer) {
       o.sz = zU;
-      o.dphi = delta_phi(xL, yL, o.sx, o.sy);
+      o.dphi = phi_mpi_pi(phiS - p.phiL);  // deltaPhi(lower, shifted)
     } else {
       o.sz = zL;
-      o.dphi = delta_phi(o.sx, o.sy, xU, yU);
+      o.dphi = phi_mpi_pi(p.phiU - phiS);  // deltaPhi(shifted, upper)
     }
-    o.nsdphi = delta_phi(xL, yL, xU, yU);
+    o.nsdphi = phi_mpi_pi(p.phiU - p.phiL);
   } else {
     o.sx = 0; o.sy = 0; o.sz = 0;
-    o.dphi = delta_phi(xL, yL, xU, yU);
+    o.dphi = phi_mpi_pi(p.phiU - p.phiL);
     o.nsdphi = o.dphi;
   }
   if (!(m_abs(o.dphi) < cut)) return false;
   if (c.side != kCenter) {
     if (c.layerType != kPixelLayer)
-      o.dphichange = rtL < shiftedRt ? delta_phi_change(xL, yL, o.sx, o.sy) : delta_phi_change(o.sx, o.sy, xL, yL);
+      o.dphichange = rtL < shiftedRt ? md_dphi_change(xL, yL, p.phiL, o.sx, o.sy)
+                                     : md_dphi_change(o.sx, o.sy, phiS, xL, yL);
     else
-      o.dphichange = shiftedRt < rtU ? delta_phi_change(o.sx, o.sy, xU, yU) : delta_phi_change(xU, yU, o.sx, o.sy);
-    o.nsdphichange = rtL < rtU ? delta_phi_change(xL, yL, xU, yU) : delta_phi_change(xU, yU, xL, yL);
+      o.dphichange = shiftedRt < rtU ? md_dphi_change(o.sx, o.sy, phiS, xU, yU)
+                                     : md_dphi_change(xU, yU, p.phiU, o.sx, o.sy);
+    o.nsdphichange = rtL < rtU ? md_dphi_change(xL, yL, p.phiL, xU, yU) : md_dphi_change(xU, yU, p.phiU, xL, yL);
   } else {
-    o.dphichange = delta_phi_change(xL, yL, xU, yU);
+    o.dphichange = md_dphi_change(xL, yL, p.phiL, xU, yU);
     o.nsdphichange = o.dphichange;
   }
   o.nsdz = 0;
@@ -156,23 +180,26 @@ __device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, float xL
 }
 
 // SDL/MiniDoublet.h:774-894 (o.dz holds drt, as the reference stores it)
-__device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, float xL, float yL, float zL, float rtL,
-                                          float xU, float yU, float zU, float rtU, float slope, MDOut& o) {
+__device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, const MDPair& p, float slope, MDOut& o) {
+  const float xL = p.xL, yL = p.yL, zL = p.zL, rtL = p.rtL, xU = p.xU, yU = p.yU, zU = p.zU, rtU = p.rtU;
   float dz = zL - zU;
   if (!(m_abs(dz) < 1.f)) return false;
   o.dz = rtL - rtU;
   if (!(m_abs(o.dz) < (c.type == kPS ? 2.f : 10.f))) return false;
-  float xn, yn, zn;
-  md_shift(c, xL, yL, zL, rtL, xU, yU, zU, rtU, xn, yn, zn);
+  float xn, yn;
+  const bool lowerPixelSide = md_lower_is_pixel_side(c);
+  md_shift_pair(c, p.sp, lowerPixelSide ? xU : xL, lowerPixelSide ? yU : yL, xn, yn);
+  const float zn = p.sp.z;
   o.sx = xn; o.sy = yn;
+  const float phiS = phi_of(xn, yn);
   if (c.type == kPS && c.layerType != kPixelLayer) {
     o.sz = zL;
-    o.dphi = delta_phi(o.sx, o.sy, xU, yU);
+    o.dphi = phi_mpi_pi(p.phiU - phiS);  // deltaPhi(shifted, upper)
   } else {
     o.sz = zU;
-    o.dphi = delta_phi(xL, yL, xn, yn);
+    o.dphi = phi_mpi_pi(phiS - p.phiL);  // deltaPhi(lower, shifted)
   }
-  o.nsdphi = delta_phi(xL, yL, xU, yU);
+  o.nsdphi = phi_mpi_pi(p.phiU - p.phiL);
   if (c.type == kPS) dz = c.layerType == kPixelLayer ? zL - zn : zU - zn;
   const float cut = md_dphi_cut(c, ptCut, slope, o.dphi, dz);
   if (!(m_abs(o.dphi) < cut)) return false;

@@ -292,7 +292,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5BucketOff, lst::kPT5HashBuckets + 1); c.take(e.W.pt5BucketEnt, 14 * (size_t)pt5Cap);
   c.take(e.FS.pre, lst::kScanCandMax); c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
   c.take(e.FS.pass, lst::kScanCandMax);
-  c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax); c.take(e.W2.hitSlope, hitCap);
+  c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax); c.take(e.W2.hitSlope, hitCap); c.take(e.W2.shiftPoint, hitCap);
   c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDCandMax); c.take(e.W2.cand, lst::kMDCandMax);
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
   c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);

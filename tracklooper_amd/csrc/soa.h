@@ -170,6 +170,8 @@ struct MDWork {
   uint32_t* park;     // [kMDParkWords * kMDCandMax] MDOut of passing candidate L at L
   uint4* cand;        // [kMDCandMax] candidate L: (lower hit, upper hit, module, 0)
   float* hitSlope;    // [hit capacity] per hit: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax))
+  float4* shiftPoint; // [hit capacity] per pixel-side hit of a module pair whose strip hits are shifted:
+                      // the shift point (xa, ya) and the shifted z (md_shift_point)
 };
 
 // Work arrays of the flattened line-segment builder: candidates of module m

@@ -84,6 +84,14 @@ __global__ void k_hits(LST_VIEWS) {
       atomicOr(&v.R.scalars[kBadHits], (uint32_t)kBadDetId);
     } else {
       v.H.moduleIndices[i] = mod;
+      {  // the strip-hit shift point, when this hit is the pixel-side hit of a pair that shifts
+        const bool lower = v.M.isLower[mod] != 0;
+        const uint32_t ml = lower ? (uint32_t)mod : v.M.partnerModuleIndices[mod];
+        if (ml < v.M.nLowerModules) {
+          const ModCtx c = load_mod(v.M, ml);
+          if (md_shifts(c) && lower == md_lower_is_pixel_side(c)) v.W2.shiftPoint[i] = md_shift_point(c, x, y, z, rt);
+        }
+      }
       if (v.M.subdets[mod] == kEndcap && v.M.moduleType[mod] == kTwoS) {
         const int e = (int)(entry >> 16) - 1;
         const float p = e >= 0 ? v.M.geoMapPhi[e] : 0.f;
@@ -340,11 +348,11 @@ __global__ __launch_bounds__(256) void k_md_eval(LST_VIEWS) {
       const uint4 cd = v.W2.cand[L];
       const uint32_t a = cd.x, b = cd.y, m = cd.z;
       const ModCtx c = load_mod(v.M, m);
-      const float xL = v.H.xs[a], yL = v.H.ys[a], zL = v.H.zs[a], rtL = v.H.rts[a];
-      const float xU = v.H.xs[b], yU = v.H.ys[b], zU = v.H.zs[b], rtU = v.H.rts[b];
+      MDPair p{v.H.xs[a], v.H.ys[a], v.H.zs[a], v.H.rts[a], v.H.phis[a],
+               v.H.xs[b], v.H.ys[b], v.H.zs[b], v.H.rts[b], v.H.phis[b], make_float4(0.f, 0.f, 0.f, 0.f)};
+      if (md_shifts(c)) p.sp = v.W2.shiftPoint[md_lower_is_pixel_side(c) ? a : b];
       const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? a : b];  // the rt of the pixel-side hit
-      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, slope, o)
-                                 : md_endcap(c, v.ptCut, xL, yL, zL, rtL, xU, yU, zU, rtU, slope, o);
+      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, p, slope, o) : md_endcap(c, v.ptCut, p, slope, o);
       v.W2.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
@@ -454,9 +462,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(LST_VIEWS) {
         ui = k - li * nUp;
         const int u = 128 + ui;
         const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? loBase + li : upBase + ui];
-        pass = c.subdet == kBarrel
-                   ? md_barrel(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], slope, o)
-                   : md_endcap(c, v.ptCut, sx[li], sy[li], sz[li], srt[li], sx[u], sy[u], sz[u], srt[u], slope, o);
+        MDPair p{sx[li], sy[li], sz[li], srt[li], v.H.phis[loBase + li], sx[u], sy[u], sz[u], srt[u],
+                 v.H.phis[upBase + ui], make_float4(0.f, 0.f, 0.f, 0.f)};
+        if (md_shifts(c)) p.sp = v.W2.shiftPoint[md_lower_is_pixel_side(c) ? loBase + li : upBase + ui];
+        pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, p, slope, o) : md_endcap(c, v.ptCut, p, slope, o);
       }
       unsigned n;
       const unsigned rank = count + build_rank(pass, lds, n);
