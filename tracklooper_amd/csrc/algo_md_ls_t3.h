@@ -85,22 +85,18 @@ __device__ __forceinline__ bool md_lower_is_pixel_side(const ModCtx& c) {
   return c.type == kPS ? (c.subdet == kBarrel ? c.layerType != kPixelLayer : c.layerType == kPixelLayer) : true;
 }
 __device__ __forceinline__ bool md_shifts(const ModCtx& c) { return c.subdet == kEndcap || c.side != kCenter; }
-// c: the lower module of the pair; (xp, yp, zp, rtp): its pixel-side hit
-__device__ __forceinline__ float4 md_shift_point(const ModCtx& c, float xp, float yp, float zp, float rtp) {
-  const float slope = c.dxdy;
+// c: the lower module m of the pair; (xp, yp, zp, rtp): its pixel-side hit.  The module-only terms (angle
+// B, and sin / cos of the slope angle per quadrant of the hit) come from the geometry (es.cpp).
+__device__ __forceinline__ float4 md_shift_point(const ModulesView& M, uint32_t m, const ModCtx& c, float xp, float yp,
+                                                 float zp, float rtp) {
   const float angleA = m_abs(m_atan(rtp / zp));
-  const float angleB = c.subdet == kEndcap ? kPi / 2.f : m_atan(c.drdz);
+  const float angleB = M.shiftAngleB[m];
   float sep = gap_size(c);
   if (c.type == kPS && c.layerType != kPixelLayer) sep *= -1;
   const float drprime = (sep / m_sin(angleA + angleB)) * m_sin(angleA);
-  const float absAtanSlope = slope != kSdlInf ? m_abs(m_atan(slope)) : kPi / 2.f;
-  float angleM;
-  if (xp > 0 && yp > 0) angleM = absAtanSlope;
-  else if (xp > 0 && yp < 0) angleM = kPi - absAtanSlope;
-  else if (xp < 0 && yp < 0) angleM = kPi + absAtanSlope;
-  else angleM = 2.f * kPi - absAtanSlope;
-  const float xa = xp + drprime * m_sin(angleM);
-  const float ya = yp + drprime * m_cos(angleM);
+  const int q = (xp > 0 && yp > 0) ? 0 : (xp > 0 && yp < 0) ? 1 : (xp < 0 && yp < 0) ? 2 : 3;
+  const float xa = xp + drprime * M.shiftSinM[4 * m + q];
+  const float ya = yp + drprime * M.shiftCosM[4 * m + q];
   const float absdz = m_abs(sep / m_sin(angleA + angleB) * m_cos(angleA));
   const float abszn = c.layerType == kPixelLayer ? m_abs(zp) + absdz : m_abs(zp) - absdz;
   return make_float4(xa, ya, abszn * ((zp > 0) ? 1 : -1), 0.f);

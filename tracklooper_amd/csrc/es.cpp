@@ -1,5 +1,6 @@
 // Geometry loading and upload for the LST engine.
 #include "es.h"
+#include "lst_math.h"
 
 #include <hip/hip_runtime.h>
 
@@ -179,6 +180,20 @@ bool load_host_geometry(const std::string& dir, uint32_t stride, HostGeometry& g
     if (g.drdzs[i] == 0) g.drdzs[i] = g.drdzs[f->second];
     if (g.dxdys[i] == 0) g.dxdys[i] = g.dxdys[f->second];
   }
+  // the module-only terms of the strip-hit shift, with the float expressions of the per-hit step
+  g.shiftAngleB.assign(nMod, 0.f);
+  g.shiftSinM.assign(4 * (size_t)nMod, 0.f);
+  g.shiftCosM.assign(4 * (size_t)nMod, 0.f);
+  for (uint32_t i = 0; i < nMod; i++) {
+    g.shiftAngleB[i] = g.subdets[i] == kEndcap ? kPi / 2.f : m_atan(g.drdzs[i]);
+    const float slope = g.dxdys[i];
+    const float a = slope != kSdlInf ? m_abs(m_atan(slope)) : kPi / 2.f;
+    const float angleM[4] = {a, kPi - a, kPi + a, 2.f * kPi - a};  // x>0 y>0, x>0 y<0, x<0 y<0, otherwise
+    for (int q = 0; q < 4; q++) {
+      g.shiftSinM[4 * (size_t)i + q] = m_sin(angleM[q]);
+      g.shiftCosM[4 * (size_t)i + q] = m_cos(angleM[q]);
+    }
+  }
   auto idxOf = [&](uint32_t d, bool& ok) -> uint16_t {
     auto f = index.find(d);
     if (f == index.end()) { ok = false; return 0; }
@@ -340,7 +355,8 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
          o_pi2 = c.add(h.pixIndex[2]), o_ps0 = c.add(h.pixSizes[0]), o_ps1 = c.add(h.pixSizes[1]),
          o_ps2 = c.add(h.pixSizes[2]), o_mdo = c.add(h.mdOccupancy), o_sgo = c.add(h.segOccupancy),
          o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny),
-         o_t5o = c.add(h.t5OccIfAny), o_dh = c.add(h.detHash);
+         o_t5o = c.add(h.t5OccIfAny), o_dh = c.add(h.detHash), o_sab = c.add(h.shiftAngleB),
+         o_ssm = c.add(h.shiftSinM), o_scm = c.add(h.shiftCosM);
   char* base = nullptr;
   if (hipMalloc(&base, c.total) != hipSuccess) { err = "hipMalloc failed for geometry"; return false; }
   for (auto& it : c.items)
@@ -365,6 +381,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   v.moduleLayerType = (const uint8_t*)P(o_mlt); v.sdlLayers = (const int32_t*)P(o_sdl);
   v.connectedPixels = (const uint32_t*)P(o_cp); v.geoMapDetId = (const uint32_t*)P(o_gd);
   v.geoMapPhi = (const float*)P(o_gp);
+  v.shiftAngleB = (const float*)P(o_sab); v.shiftSinM = (const float*)P(o_ssm); v.shiftCosM = (const float*)P(o_scm);
   v.nModules = h.nModules; v.nLowerModules = h.nLowerModules; v.stride = h.stride; v.nEndCapMap = h.geoMapDetId.size();
   v.detHash = (const uint2*)P(o_dh); v.detHashBits = h.detHashBits;
   d.pixIndex[0] = (const uint32_t*)P(o_pi0); d.pixIndex[1] = (const uint32_t*)P(o_pi1);

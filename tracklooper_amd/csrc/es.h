@@ -33,6 +33,9 @@ struct HostGeometry {
   std::vector<int16_t> layers, rings, modules, rods, subdets, sides;
   std::vector<uint8_t> isInverted, isLower, isAnchor, moduleType, moduleLayerType;
   std::vector<int32_t> sdlLayers;
+  // per module, the module-only terms of the strip-hit shift (SDL/MiniDoublet.h:403-567): angle B (pi/2 in
+  // the endcap, else atan(drdz)) and, per quadrant of the pixel-side hit, sin and cos of the slope angle
+  std::vector<float> shiftAngleB, shiftSinM, shiftCosM;
   // pLS superbin -> connected-module lists (SDL/PixelMap.h:16-36); the pos/neg
   // index arrays already include the offsets the reference adds per event
   // (SDL/Event.cc:813-816), so a pLS only needs (superbin, pixelType).

@@ -61,6 +61,9 @@ struct ModulesView {
   const uint8_t* moduleType;
   const uint8_t* moduleLayerType;
   const int32_t* sdlLayers;
+  const float* shiftAngleB;  // [nModules] strip-hit shift terms of the module (es.cpp)
+  const float* shiftSinM;    // [4 nModules] per quadrant of the pixel-side hit
+  const float* shiftCosM;
   const uint32_t* connectedPixels;
   const uint32_t* geoMapDetId;  // endcap 2S orientation map, sorted
   const float* geoMapPhi;

@@ -89,7 +89,7 @@ __global__ void k_hits(LST_VIEWS) {
         const uint32_t ml = lower ? (uint32_t)mod : v.M.partnerModuleIndices[mod];
         if (ml < v.M.nLowerModules) {
           const ModCtx c = load_mod(v.M, ml);
-          if (md_shifts(c) && lower == md_lower_is_pixel_side(c)) v.W2.shiftPoint[i] = md_shift_point(c, x, y, z, rt);
+          if (md_shifts(c) && lower == md_lower_is_pixel_side(c)) v.W2.shiftPoint[i] = md_shift_point(v.M, ml, c, x, y, z, rt);
         }
       }
       if (v.M.subdets[mod] == kEndcap && v.M.moduleType[mod] == kTwoS) {
