@@ -389,8 +389,12 @@ def generate_many(specs, workers: int = 1) -> list:
     import sys
     torch, pkg = sys.modules.get("torch"), sys.modules.get("tracklooper_amd")
     gpu_used = (torch is not None and torch.cuda.is_initialized()) or getattr(pkg, "_lib", None) is not None
-    with mp.get_context("spawn" if gpu_used else "fork").Pool(min(workers, len(specs))) as pool:
+    pool = mp.get_context("spawn" if gpu_used else "fork").Pool(min(workers, len(specs)))
+    try:
         return pool.map(_gen_one, specs, chunksize=1)
+    finally:  # the workers exit on their own: no SIGTERM (which a profiler's signal handler would take)
+        pool.close()
+        pool.join()
 
 
 def c3_batch(batch: int, n: int = 64, geom_name: str = "OT800_IT615_pt0.8", workers: int = 1) -> list:
