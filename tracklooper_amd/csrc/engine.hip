@@ -148,6 +148,7 @@ struct PreparedInput {
     d.seedIdx = (const uint32_t*)at(SEED); d.pixelType = (const int8_t*)at(PTYPE); d.isQuad = (const int8_t*)at(QUAD);
     d.nPLS = nPLS;
     d.outHost = nullptr;
+    d.outDev = nullptr;
     d.outCap = 0;
     return d;
   }
@@ -1009,8 +1010,10 @@ lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* feature
 // event in its own arena, and the sequence is one captured graph.  Inputs are
 // prepared on the host once and kept pinned; each launch set copies its
 // events' blocks to the device on a copy stream while the previous set
-// computes (two input buffers), and its last kernel copies each event's packed
-// output to pinned host memory.  An event whose candidates overflow a
+// computes (two input buffers); its last kernel puts each event's packed
+// output block in a device block and the block's header in pinned host
+// memory, and the host copies the records back on an output stream while the
+// next set computes.  An event whose candidates overflow a
 // flattened builder's work arrays (a flag in its output header) is run again
 // by a per-event handle, which has the per-module builders.
 struct lst_batch {
@@ -1019,14 +1022,15 @@ struct lst_batch {
   uint32_t slots = 0;
   std::vector<lst_event*> ctx;          // one arena per slot
   lst_event* rerun = nullptr;           // per-event handle for events that overflow the flattened builders
-  hipStream_t stream = nullptr, copyStream = nullptr;
+  hipStream_t stream = nullptr, copyStream = nullptr, outStream = nullptr;
   lst::EventView* viewsDev = nullptr;   // [slots]
   char* inDev[2] = {nullptr, nullptr};  // device input blocks of a launch set, slots x slotBytes each
   size_t slotBytes = 0;
   lst::InputDesc* descDev[2] = {nullptr, nullptr};   // [slots]
   lst::InputDesc* descHost[2] = {nullptr, nullptr};  // pinned [slots]
   uint32_t* outHost[2] = {nullptr, nullptr};         // pinned, slots x out_words()
-  uint32_t outCapAlloc = 0;             // records per event outHost holds
+  uint32_t* outDev[2] = {nullptr, nullptr};          // device, slots x out_words()
+  uint32_t outCapAlloc = 0;             // records per event outHost and outDev hold
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graphs = true;
   hipEvent_t copyDone[2] = {}, computeDone[2] = {};
@@ -1085,8 +1089,11 @@ lst_status batch_io(lst_batch* b, size_t bytes, uint32_t cap) {
   if (!b->outHost[0] || cap > b->outCapAlloc) {
     for (auto& p : b->outHost)
       if (p) HIP_OK(hipHostFree(p));
+    for (auto& p : b->outDev)
+      if (p) HIP_OK(hipFree(p));
     for (auto& p : b->outHost)
       HIP_OK(hipHostMalloc((void**)&p, 4 * b->out_words(cap) * b->slots, hipHostMallocDefault));
+    for (auto& p : b->outDev) HIP_OK(hipMalloc(&p, 4 * b->out_words(cap) * b->slots));
     b->outCapAlloc = cap;
   }
   return LST_OK;
@@ -1159,6 +1166,7 @@ lst_status issue_inputs(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, 
       d = lst::InputDesc{};  // an empty event
     }
     d.outHost = b->outHost[buf] + ow * i;
+    d.outDev = b->outDev[buf] + ow * i;
     d.outCap = outCap;
   }
   HIP_OK(hipMemcpyAsync(b->descDev[buf], b->descHost[buf], sizeof(lst::InputDesc) * b->slots,
@@ -1232,6 +1240,20 @@ lst_status consume(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, lst_o
                    lst_status& agg) {
   HIP_OK(hipEventSynchronize(b->computeDone[buf]));
   const size_t ow = b->out_words(outCap);
+  // the records the callers take, device block to pinned block behind the header that is already there;
+  // the next launch set computes meanwhile, and the set after it (which rewrites outDev[buf]) is issued
+  // only once this returns
+  bool copied = false;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t* blk = b->outHost[buf] + ow * i;
+    if (!outs[i] || blk[lst::kOutBadHits] || blk[lst::kOutOverflow]) continue;
+    const size_t rec = (size_t)std::min(blk[lst::kOutNTC], outCap) * lst::kOutRecWords;
+    if (!rec) continue;
+    HIP_OK(hipMemcpyAsync(blk + lst::kOutHeaderWords, b->outDev[buf] + ow * i + lst::kOutHeaderWords, 4 * rec,
+                          hipMemcpyDeviceToHost, b->outStream));
+    copied = true;
+  }
+  if (copied) HIP_OK(hipStreamSynchronize(b->outStream));
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t* blk = b->outHost[buf] + ow * i;
     lst_output* d = outs[i];
@@ -1331,6 +1353,7 @@ lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch
   b->slots = n_slots;
   HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&b->copyStream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&b->outStream, hipStreamNonBlocking));
   for (uint32_t c = 0; c < n_slots; c++) {
     lst_event* ev = nullptr;
     lst_status s = lst_event_create(es, b->stream, t5_mode, &ev);
@@ -1351,6 +1374,7 @@ void lst_batch_destroy(lst_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->copyStream) (void)hipStreamSynchronize(b->copyStream);
+  if (b->outStream) (void)hipStreamSynchronize(b->outStream);
   drop_graphs(b);
   for (lst_event* ev : b->ctx) lst_event_destroy(ev);
   if (b->rerun) lst_event_destroy(b->rerun);
@@ -1360,12 +1384,14 @@ void lst_batch_destroy(lst_batch* b) {
     if (b->descDev[k]) (void)hipFree(b->descDev[k]);
     if (b->descHost[k]) (void)hipHostFree(b->descHost[k]);
     if (b->outHost[k]) (void)hipHostFree(b->outHost[k]);
+    if (b->outDev[k]) (void)hipFree(b->outDev[k]);
     if (b->copyDone[k]) (void)hipEventDestroy(b->copyDone[k]);
     if (b->computeDone[k]) (void)hipEventDestroy(b->computeDone[k]);
   }
   if (b->viewsDev) (void)hipFree(b->viewsDev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   if (b->copyStream) (void)hipStreamDestroy(b->copyStream);
+  if (b->outStream) (void)hipStreamDestroy(b->outStream);
   delete b;
 }
 

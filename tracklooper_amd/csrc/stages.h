@@ -182,8 +182,9 @@ struct InputDesc {
   const uint32_t* seedIdx;
   const int8_t *pixelType, *isQuad;
   uint32_t nPLS;
-  uint32_t* outHost;  // where k_out_copy puts the packed output block (pinned host memory), null: nowhere
-  uint32_t outCap;    // track-candidate records outHost holds
+  uint32_t* outHost;  // where k_out_copy puts the block's header (pinned host memory), null: no output
+  uint32_t* outDev;   // where k_out_copy puts the whole packed block (device memory, read back by the host)
+  uint32_t outCap;    // track-candidate records outDev holds
 };
 // copies the described inputs into the event arenas (descriptor blockIdx.z of
 // the device array d) and sets the events' counts
@@ -200,7 +201,7 @@ void launch_pixel_quintuplets(const Launch& L);
 void launch_pixel_triplets(const Launch& L);
 void launch_track_candidates(const Launch& L);
 // copies each event's packed output block (header and the records it holds room for) to the
-// destination its descriptor names
+// device block its descriptor names, and the header alone to its host block
 void launch_out_copy(const Launch& L, const InputDesc* d);
 // all-pairs duplicate removal and cross cleaning (stages_clean.hip)
 void launch_pls_dup(const Launch& L, bool second);

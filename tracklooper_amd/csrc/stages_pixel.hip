@@ -901,20 +901,28 @@ __global__ void k_tc_output(LST_VIEWS) {
   }
 }
 
-// the packed output block of each event to its descriptor's destination,
-// 16 bytes per lane (the block and the destination are 256-byte aligned)
+// the packed output block of each event to its descriptor's device block and
+// the block's header to its host block, 16 bytes per lane (the blocks are
+// 256-byte aligned).  The host reads the headers and copies the records it
+// needs from the device block on a copy engine, so no kernel of the launch set
+// waits on writes across the host link.
 __global__ __launch_bounds__(256) void k_out_copy(LST_VIEWS, const InputDesc* __restrict__ ds) {
   LST_BIND;
   const InputDesc& d = ds[blockIdx.z];
-  uint32_t* dst = d.outHost;
-  if (!dst) return;
+  if (!d.outHost) return;
   const uint32_t nTC = v.W.outPacked[kOutNTC];
   const uint32_t words = kOutHeaderWords + (nTC < d.outCap ? nTC : d.outCap) * kOutRecWords;
   const uint4* src4 = reinterpret_cast<const uint4*>(v.W.outPacked);
-  uint4* dst4 = reinterpret_cast<uint4*>(dst);
+  uint4* dev4 = reinterpret_cast<uint4*>(d.outDev);
+  uint4* host4 = reinterpret_cast<uint4*>(d.outHost);
+  static_assert(kOutHeaderWords % 4 == 0, "the header is whole 16-byte words");
   const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  for (uint32_t i = t0; i < words / 4; i += stride) dst4[i] = src4[i];
-  if (t0 < (words & 3u)) dst[(words & ~3u) + t0] = v.W.outPacked[(words & ~3u) + t0];
+  for (uint32_t i = t0; i < words / 4; i += stride) {
+    const uint4 x = src4[i];
+    dev4[i] = x;
+    if (i < kOutHeaderWords / 4) host4[i] = x;
+  }
+  if (t0 < (words & 3u)) d.outDev[(words & ~3u) + t0] = v.W.outPacked[(words & ~3u) + t0];
 }
 
 }  // namespace
