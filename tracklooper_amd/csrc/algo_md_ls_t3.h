@@ -6,53 +6,25 @@
 // SDL/Segment.h:177-796, triplets SDL/Triplet.h:241-872.
 #pragma once
 #include "lst_types.h"
+#include "module_terms.h"
 #include "soa.h"
 
 namespace lst {
 
-__constant__ float c_mulsBarrel[6] = {0.0052, 0.0038, 0.0034, 0.0034, 0.0032, 0.0034};
-__constant__ float c_mulsEndcap[5] = {0.006, 0.006, 0.006, 0.006, 0.006};
-__constant__ float c_rminBarrel[6] = {25.007152356, 37.2186993757, 52.3104270826, 68.6658656666, 85.9770373007,
-                                      108.301772384};
-__constant__ float c_rminEndcap[5] = {130.992832231, 154.813883559, 185.352604327, 221.635123002, 265.022076742};
-
-struct ModCtx {
-  int16_t subdet, side, layer, ring, rod;
-  uint8_t type, layerType;
-  float drdz, dxdy, partnerDrdz;
-};
-
+// A module's attributes and the module-only terms of the cuts come from the
+// geometry's per-module records (module_terms.h): one 16-byte load each.
 __device__ __forceinline__ ModCtx load_mod(const ModulesView& M, unsigned m) {
+  const float4 r = ld_global(M.modCtx + m);
   ModCtx c;
-  c.subdet = M.subdets[m]; c.side = M.sides[m]; c.layer = M.layers[m]; c.ring = M.rings[m]; c.rod = M.rods[m];
-  c.type = M.moduleType[m]; c.layerType = M.moduleLayerType[m];
-  c.drdz = M.drdzs[m]; c.dxdy = M.dxdys[m];
-  c.partnerDrdz = M.drdzs[M.partnerModuleIndices[m]];
+  unpack_mod(__float_as_uint(r.x), c);
+  c.drdz = r.y; c.dxdy = r.z; c.partnerDrdz = r.w;
   return c;
 }
-
-// Separation between the two sensors of a module (SDL/MiniDoublet.h:264-337,
-// same values as SDL/Segment.h:202-262).
-__device__ __forceinline__ bool tighter_tilted(const ModCtx& c) {
-  return c.subdet == kBarrel && ((c.side != kCenter && c.layer == 3) ||
-                                 (c.side == kNegZ && ((c.layer == 2 && c.rod > 5) || (c.layer == 1 && c.rod > 9))) ||
-                                 (c.side == kPosZ && ((c.layer == 2 && c.rod < 8) || (c.layer == 1 && c.rod < 4))));
-}
-__device__ __forceinline__ float gap_size(const ModCtx& c) {
-  if (c.subdet == kBarrel && c.side == kCenter) {
-    const float flat[6] = {0.26f, 0.16f, 0.16f, 0.18f, 0.18f, 0.18f};
-    return flat[c.layer - 1];
-  }
-  if (tighter_tilted(c)) return 0.26f;
-  if (c.subdet == kEndcap) {
-    const int ringLimit = c.layer <= 2 ? 10 : (c.layer <= 4 ? 8 : 9);
-    return (c.ring - 1) < ringLimit ? 0.4f : 0.18f;
-  }
-  return 0.4f;
-}
-
-__device__ __forceinline__ float muls_scale(const ModCtx& c, float ptCut) {
-  return c.subdet == kBarrel ? c_mulsBarrel[c.layer - 1] * 3.f / ptCut : c_mulsEndcap[c.layer - 1] * 3.f / ptCut;
+// {muls, segTilt, segRes, -} of module m
+__device__ __forceinline__ float4 seg_terms(const ModulesView& M, unsigned m) { return ld_global(M.modTerms + 2 * m); }
+// {mdBase, mdTilt2, mdRoot, -} of module m
+__device__ __forceinline__ float4 md_terms(const ModulesView& M, unsigned m) {
+  return ld_global(M.modTerms + 2 * m + 1);
 }
 
 // ----------------------------------------------------------------- mini-doublets
@@ -61,18 +33,12 @@ __device__ __forceinline__ float muls_scale(const ModCtx& c, float ptCut) {
 __device__ __forceinline__ float md_hit_slope(float rt, float ptCut) {
   return m_asin(m_min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
 }
-__device__ __forceinline__ float md_dphi_cut(const ModCtx& c, float ptCut, float slope, float dPhi, float dz) {
-  const float rNom = c.subdet == kBarrel ? c_rminBarrel[c.layer - 1] : c_rminEndcap[c.layer - 1];
-  const float pvOff = 0.1f / rNom;
-  const float muls = muls_scale(c, ptCut);
-  if (c.subdet == kBarrel && c.side == kCenter) return slope + m_sqrt(muls * muls + pvOff * pvOff);
-  if (c.subdet == kBarrel) {
-    const float drdz = (c.type == kPS && c.layerType == kStripLayer) ? c.drdz : c.partnerDrdz;
-    const float tilt = 0.5f * kPixelPSZpitch * drdz / m_sqrt(1.f + drdz * drdz) / gap_size(c);
-    return slope + m_sqrt(muls * muls + pvOff * pvOff + tilt * tilt * slope * slope);
-  }
+// mt = md_terms of the module: mdBase = muls^2 + pvOff^2, mdTilt2 = tilt^2, mdRoot = sqrt(mdBase)
+__device__ __forceinline__ float md_dphi_cut(const ModCtx& c, float4 mt, float slope, float dPhi, float dz) {
+  if (c.subdet == kBarrel && c.side == kCenter) return slope + mt.z;
+  if (c.subdet == kBarrel) return slope + m_sqrt(mt.x + mt.y * slope * slope);
   const float lum = m_abs(dPhi * kDeltaZLum / dz);
-  return slope + m_sqrt(muls * muls + pvOff * pvOff + lum * lum);
+  return slope + m_sqrt(mt.x + lum * lum);
 }
 
 // strip-hit shift along the line of sight of the PS pixel hit: SDL/MiniDoublet.h:403-567.
@@ -129,14 +95,14 @@ __device__ __forceinline__ float md_dphi_change(float x1, float y1, float phi1, 
 }
 
 // SDL/MiniDoublet.h:645-772
-__device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, const MDPair& p, float slope, MDOut& o) {
+__device__ __forceinline__ bool md_barrel(const ModCtx& c, float4 mt, const MDPair& p, float slope, MDOut& o) {
   const float xL = p.xL, yL = p.yL, zL = p.zL, rtL = p.rtL, xU = p.xU, yU = p.yU, zU = p.zU, rtU = p.rtU;
   o.dz = zL - zU;
   const float dzCut = c.type == kPS ? 2.f : 10.f;
   const float sign = ((o.dz > 0) - (o.dz < 0)) * ((zL > 0) - (zL < 0));
   const float crosser = (m_abs(o.dz) > 2) * sign;
   if (!(m_abs(o.dz) < dzCut && crosser <= 0)) return false;
-  const float cut = md_dphi_cut(c, ptCut, slope, 0.f, 0.f);
+  const float cut = md_dphi_cut(c, mt, slope, 0.f, 0.f);
   float shiftedRt = 0.f, phiS = 0.f;
   if (c.side != kCenter) {
     float xn, yn;
@@ -176,7 +142,7 @@ __device__ __forceinline__ bool md_barrel(const ModCtx& c, float ptCut, const MD
 }
 
 // SDL/MiniDoublet.h:774-894 (o.dz holds drt, as the reference stores it)
-__device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, const MDPair& p, float slope, MDOut& o) {
+__device__ __forceinline__ bool md_endcap(const ModCtx& c, float4 mt, const MDPair& p, float slope, MDOut& o) {
   const float xL = p.xL, yL = p.yL, zL = p.zL, rtL = p.rtL, xU = p.xU, yU = p.yU, zU = p.zU, rtU = p.rtU;
   float dz = zL - zU;
   if (!(m_abs(dz) < 1.f)) return false;
@@ -197,7 +163,7 @@ __device__ __forceinline__ bool md_endcap(const ModCtx& c, float ptCut, const MD
   }
   o.nsdphi = phi_mpi_pi(p.phiU - p.phiL);
   if (c.type == kPS) dz = c.layerType == kPixelLayer ? zL - zn : zU - zn;
-  const float cut = md_dphi_cut(c, ptCut, slope, o.dphi, dz);
+  const float cut = md_dphi_cut(c, mt, slope, o.dphi, dz);
   if (!(m_abs(o.dphi) < cut)) return false;
   const float frac = m_abs(dz) / m_abs(zL);
   o.dphichange = o.dphi / frac * (1.f + frac);
@@ -224,21 +190,20 @@ struct SegOut {
   float dPhi, dPhiMin, dPhiMax, dPhiChange, dPhiChangeMin, dPhiChangeMax;
 };
 
-// dalpha thresholds: SDL/Segment.h:264-356
-__device__ __forceinline__ void seg_dalpha_cuts(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                                const MDRef& b, float th[3]) {
-  const float muls = muls_scale(ci, ptCut);
+// dalpha thresholds: SDL/Segment.h:264-356.  ti, to: seg_terms of the inner
+// and outer module (muls, the tilt term that is 0 unless a tilted barrel
+// module, 0.02 / gap); the resolution of both ends uses the inner module's gap,
+// as the reference's does
+__device__ __forceinline__ void seg_dalpha_cuts(const ModCtx& ci, float4 ti, const ModCtx& co, float4 to, float ptCut,
+                                                const MDRef& a, const MDRef& b, float th[3]) {
+  const float muls = ti.x;
   const float dr = m_sqrt((b.y - a.y) * (b.y - a.y) + (b.x - a.x) * (b.x - a.x));
   const float bfield = m_asin(m_min(dr * k2Rinv1GeVf / ptCut, kSinAlphaMax));
-  const float gi = gap_size(ci), go = gap_size(co);
-  const bool tiltI = ci.subdet == kBarrel && ci.side != kCenter;
-  const bool tiltO = co.subdet == kBarrel && co.side != kCenter;
-  const float tI = tiltI ? (0.5f * kPixelPSZpitch * ci.drdz / m_sqrt(1.f + ci.drdz * ci.drdz) / gi) : 0;
-  const float tO = tiltO ? (0.5f * kPixelPSZpitch * co.drdz / m_sqrt(1.f + co.drdz * co.drdz) / go) : 0;
+  const float tI = ti.y, tO = to.y;
   const float lumI = ci.subdet == kBarrel ? tI * bfield : a.dphi * 15.0f / a.dz;
   const float lumO = co.subdet == kBarrel ? tO * bfield : b.dphi * 15.0f / b.dz;
-  const float resI = 0.02f / gi * (ci.subdet == kBarrel ? 1.0f : m_abs(a.z) / a.rt);
-  const float resO = 0.02f / gi * (co.subdet == kBarrel ? 1.0f : m_abs(b.z) / b.rt);
+  const float resI = ti.z * (ci.subdet == kBarrel ? 1.0f : m_abs(a.z) / a.rt);
+  const float resO = ti.z * (co.subdet == kBarrel ? 1.0f : m_abs(b.z) / b.rt);
   const float res = resI + resO;
   const float base2 = res * res + muls * muls;
   th[0] = (ci.subdet == kBarrel && ci.side == kCenter) ? bfield + m_sqrt(base2) : bfield + m_sqrt(base2 + lumI * lumI);
@@ -246,10 +211,10 @@ __device__ __forceinline__ void seg_dalpha_cuts(const ModCtx& ci, const ModCtx& 
   th[2] = bfield + m_sqrt(base2);
 }
 
-__device__ __forceinline__ bool seg_dalpha_pass(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                                const MDRef& b, float dPhiChange) {
+__device__ __forceinline__ bool seg_dalpha_pass(const ModCtx& ci, float4 ti, const ModCtx& co, float4 to, float ptCut,
+                                                const MDRef& a, const MDRef& b, float dPhiChange) {
   float th[3];
-  seg_dalpha_cuts(ci, co, ptCut, a, b, th);
+  seg_dalpha_cuts(ci, ti, co, to, ptCut, a, b, th);
   if (!(m_abs(a.dphichange - dPhiChange) < th[0])) return false;
   if (!(m_abs(b.dphichange - dPhiChange) < th[1])) return false;
   return m_abs(a.dphichange - b.dphichange) < th[2];
@@ -259,9 +224,9 @@ __device__ __forceinline__ bool seg_dalpha_pass(const ModCtx& ci, const ModCtx& 
 // bSlope: (asin(min(b.rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)), the tan of
 // it), a function of the outer mini-doublet's anchor hit only, computed once
 // per MD from the hit's slope (md_hit_slope, the same formula)
-__device__ __forceinline__ bool seg_barrel(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                           const MDRef& b, float2 bSlope, SegOut& o) {
-  const float muls = muls_scale(ci, ptCut);
+__device__ __forceinline__ bool seg_barrel(const ModCtx& ci, float4 ti, const ModCtx& co, float4 to, float ptCut,
+                                           const MDRef& a, const MDRef& b, float2 bSlope, SegOut& o) {
+  const float muls = ti.x;
   const float slope = bSlope.x;
   const float pvOff = 0.1f / b.rt;
   const float scale = bSlope.y / slope;
@@ -274,12 +239,12 @@ __device__ __forceinline__ bool seg_barrel(const ModCtx& ci, const ModCtx& co, f
   if (!(m_abs(o.dPhi) <= cut)) return false;
   o.dPhiChange = phi_mpi_pi(phi_of(b.x - a.x, b.y - a.y) - a.phi);
   if (!(m_abs(o.dPhiChange) <= cut)) return false;
-  return seg_dalpha_pass(ci, co, ptCut, a, b, o.dPhiChange);
+  return seg_dalpha_pass(ci, ti, co, to, ptCut, a, b, o.dPhiChange);
 }
 
 // SDL/Segment.h:560-704
-__device__ __forceinline__ bool seg_endcap(const ModCtx& ci, const ModCtx& co, float ptCut, const MDRef& a,
-                                           const MDRef& b, float2 bSlope, SegOut& o) {
+__device__ __forceinline__ bool seg_endcap(const ModCtx& ci, float4 ti, const ModCtx& co, float4 to, float ptCut,
+                                           const MDRef& a, const MDRef& b, float2 bSlope, SegOut& o) {
   const float slope = bSlope.x;
   const float r2S = 60.f;
   const float rtGeom = (a.rt < r2S && b.rt < r2S)   ? (2.f * kPixelPSZpitch)
@@ -309,7 +274,7 @@ __device__ __forceinline__ bool seg_endcap(const ModCtx& ci, const ModCtx& co, f
   o.dPhiChangeMin = o.dPhiMin / frac * (1.f + frac);
   o.dPhiChangeMax = o.dPhiMax / frac * (1.f + frac);
   if (!(m_abs(o.dPhiChange) <= slope)) return false;
-  return seg_dalpha_pass(ci, co, ptCut, a, b, o.dPhiChange);
+  return seg_dalpha_pass(ci, ti, co, to, ptCut, a, b, o.dPhiChange);
 }
 
 // ------------------------------------------------------------------------ triplets

@@ -655,6 +655,7 @@ lst_status lst_es_create(const char* geom_dir, float pt_cut, uint32_t stride, ls
       if (sz > 64) return fail(LST_E_FORMAT, "a pixel superbin connects to more than 64 modules");
   for (uint32_t m : es->host.connectedPixels)
     if (m >= es->host.nLowerModules) return fail(LST_E_FORMAT, "pixel map names a module that is not a lower module");
+  if (!lst::fill_module_terms(es->host, pt_cut, err)) return fail(LST_E_FORMAT, err);
   if (!lst::upload_geometry(es->host, es->dev, err)) return fail(LST_E_DEVICE, err);
   es->ptCut = pt_cut;
   *out = es.release();

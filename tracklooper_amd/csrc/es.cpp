@@ -1,6 +1,7 @@
 // Geometry loading and upload for the LST engine.
 #include "es.h"
 #include "lst_math.h"
+#include "module_terms.h"
 
 #include <hip/hip_runtime.h>
 
@@ -342,6 +343,31 @@ struct Carver {
 };
 }  // namespace
 
+bool fill_module_terms(HostGeometry& g, float ptCut, std::string& err) {
+  const uint32_t n = g.nModules;
+  g.modCtx.assign(4 * (size_t)n, 0.f);
+  g.modTerms.assign(8 * (size_t)n, 0.f);
+  for (uint32_t i = 0; i < n; i++) {
+    ModCtx c;
+    c.subdet = g.subdets[i]; c.side = g.sides[i]; c.layer = g.layers[i]; c.ring = g.rings[i]; c.rod = g.rods[i];
+    c.type = g.moduleType[i]; c.layerType = g.moduleLayerType[i];
+    c.drdz = g.drdzs[i]; c.dxdy = g.dxdys[i]; c.partnerDrdz = g.drdzs[g.partnerModuleIndices[i]];
+    if (!mod_packable(c)) {
+      err = "module " + std::to_string(i) + ": attributes outside the packed ranges";
+      return false;
+    }
+    const uint32_t w = pack_mod(c);
+    float* ctx = &g.modCtx[4 * (size_t)i];
+    std::memcpy(ctx, &w, 4);
+    ctx[1] = c.drdz; ctx[2] = c.dxdy; ctx[3] = c.partnerDrdz;
+    if ((c.subdet == kBarrel && c.layer >= 1 && c.layer <= 6) || (c.subdet == kEndcap && c.layer >= 1 && c.layer <= 5)) {
+      const ModTerms t = module_terms(c, ptCut);
+      std::memcpy(&g.modTerms[8 * (size_t)i], &t, sizeof(t));
+    }
+  }
+  return true;
+}
+
 bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err) {
   Carver c;
   size_t o_det = c.add(h.detIds), o_map = c.add(h.moduleMap), o_mdet = c.add(h.mapdetId), o_midx = c.add(h.mapIdx),
@@ -356,7 +382,8 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
          o_ps2 = c.add(h.pixSizes[2]), o_mdo = c.add(h.mdOccupancy), o_sgo = c.add(h.segOccupancy),
          o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny),
          o_t5o = c.add(h.t5OccIfAny), o_dh = c.add(h.detHash), o_sab = c.add(h.shiftAngleB),
-         o_ssm = c.add(h.shiftSinM), o_scm = c.add(h.shiftCosM);
+         o_ssm = c.add(h.shiftSinM), o_scm = c.add(h.shiftCosM), o_mcx = c.add(h.modCtx),
+         o_mtm = c.add(h.modTerms);
   char* base = nullptr;
   if (hipMalloc(&base, c.total) != hipSuccess) { err = "hipMalloc failed for geometry"; return false; }
   for (auto& it : c.items)
@@ -382,6 +409,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   v.connectedPixels = (const uint32_t*)P(o_cp); v.geoMapDetId = (const uint32_t*)P(o_gd);
   v.geoMapPhi = (const float*)P(o_gp);
   v.shiftAngleB = (const float*)P(o_sab); v.shiftSinM = (const float*)P(o_ssm); v.shiftCosM = (const float*)P(o_scm);
+  v.modCtx = (const float4*)P(o_mcx); v.modTerms = (const float4*)P(o_mtm);
   v.nModules = h.nModules; v.nLowerModules = h.nLowerModules; v.stride = h.stride; v.nEndCapMap = h.geoMapDetId.size();
   v.detHash = (const uint2*)P(o_dh); v.detHashBits = h.detHashBits;
   d.pixIndex[0] = (const uint32_t*)P(o_pi0); d.pixIndex[1] = (const uint32_t*)P(o_pi1);

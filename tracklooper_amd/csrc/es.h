@@ -36,6 +36,7 @@ struct HostGeometry {
   // per module, the module-only terms of the strip-hit shift (SDL/MiniDoublet.h:403-567): angle B (pi/2 in
   // the endcap, else atan(drdz)) and, per quadrant of the pixel-side hit, sin and cos of the slope angle
   std::vector<float> shiftAngleB, shiftSinM, shiftCosM;
+  std::vector<float> modCtx, modTerms;  // 4 and 8 floats per module (fill_module_terms)
   // pLS superbin -> connected-module lists (SDL/PixelMap.h:16-36); the pos/neg
   // index arrays already include the offsets the reference adds per event
   // (SDL/Event.cc:813-816), so a pLS only needs (superbin, pixelType).
@@ -76,6 +77,8 @@ struct DeviceGeometry {
   uint32_t nT5SeedModules = 0;                 // modules with t5OccIfAny >= 0
 };
 
+// the per-module context records and cut terms (module_terms.h) for a pT cut
+bool fill_module_terms(HostGeometry& g, float ptCut, std::string& err);
 bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err);
 void free_geometry(DeviceGeometry& d);
 

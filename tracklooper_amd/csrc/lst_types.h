@@ -64,6 +64,8 @@ struct ModulesView {
   const float* shiftAngleB;  // [nModules] strip-hit shift terms of the module (es.cpp)
   const float* shiftSinM;    // [4 nModules] per quadrant of the pixel-side hit
   const float* shiftCosM;
+  const float4* modCtx;    // [nModules] packed attributes, drdz, dxdy, partner drdz (module_terms.h)
+  const float4* modTerms;  // [2 nModules] module-only terms of the segment and mini-doublet cuts
   const uint32_t* connectedPixels;
   const uint32_t* geoMapDetId;  // endcap 2S orientation map, sorted
   const float* geoMapPhi;
@@ -72,6 +74,21 @@ struct ModulesView {
   const uint2* detHash;
   uint32_t nModules, nLowerModules, stride, nEndCapMap, detHashBits;
 };
+// a load issued as a global (not flat) memory load: p addresses device memory
+// (16-byte types are loaded as one native vector)
+template <class T>
+__device__ __forceinline__ T ld_global(const T* p) {
+  if constexpr (sizeof(T) == 16) {
+    typedef unsigned V4 __attribute__((ext_vector_type(4)));
+    const V4 x = *(const __attribute__((address_space(1))) V4*)p;
+    T r;
+    __builtin_memcpy(&r, &x, 16);
+    return r;
+  } else {
+    return *(const __attribute__((address_space(1))) T*)p;
+  }
+}
+
 constexpr uint32_t kNoDetId = 0xFFFFFFFFu;  // an empty slot of the detId table
 __host__ __device__ inline uint32_t det_hash_slot(uint32_t det, uint32_t bits) {
   return (det * 0x9E3779B1u) >> (32 - bits);

@@ -350,11 +350,12 @@ __global__ __launch_bounds__(256, 5) void k_md_eval(LST_VIEWS) {
       const uint4 cd = v.W2.cand[L];
       const uint32_t a = cd.x, b = cd.y, m = cd.z;
       const ModCtx c = load_mod(v.M, m);
+      const float4 mt = md_terms(v.M, m);
       MDPair p{v.H.xs[a], v.H.ys[a], v.H.zs[a], v.H.rts[a], v.H.phis[a],
                v.H.xs[b], v.H.ys[b], v.H.zs[b], v.H.rts[b], v.H.phis[b], make_float4(0.f, 0.f, 0.f, 0.f)};
       if (md_shifts(c)) p.sp = v.W2.shiftPoint[md_lower_is_pixel_side(c) ? a : b];
       const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? a : b];  // the rt of the pixel-side hit
-      pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, p, slope, o) : md_endcap(c, v.ptCut, p, slope, o);
+      pass = c.subdet == kBarrel ? md_barrel(c, mt, p, slope, o) : md_endcap(c, mt, p, slope, o);
       v.W2.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
@@ -451,6 +452,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(LST_VIEWS) {
     }
     __syncthreads();
     const ModCtx c = load_mod(v.M, m);
+    const float4 mt = md_terms(v.M, m);
     const bool anchorIsUpper = c.type == kPS && c.layerType == kStripLayer;
     const uint32_t base = v.mdModuleIndices[m];
     const int limit = nLo * nUp;
@@ -467,7 +469,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(LST_VIEWS) {
         MDPair p{sx[li], sy[li], sz[li], srt[li], v.H.phis[loBase + li], sx[u], sy[u], sz[u], srt[u],
                  v.H.phis[upBase + ui], make_float4(0.f, 0.f, 0.f, 0.f)};
         if (md_shifts(c)) p.sp = v.W2.shiftPoint[md_lower_is_pixel_side(c) ? loBase + li : upBase + ui];
-        pass = c.subdet == kBarrel ? md_barrel(c, v.ptCut, p, slope, o) : md_endcap(c, v.ptCut, p, slope, o);
+        pass = c.subdet == kBarrel ? md_barrel(c, mt, p, slope, o) : md_endcap(c, mt, p, slope, o);
       }
       unsigned n;
       const unsigned rank = count + build_rank(pass, lds, n);
@@ -579,7 +581,8 @@ __global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
   }
 }
 
-// four waves per SIMD (128 VGPRs instead of the compiler's 134, which allowed three; no spills)
+// four waves per SIMD (118 VGPRs with the module terms from the geometry; at most 128 kept as a bound,
+// since three waves were 9 % slower in a 64-event launch set; five would spill)
 __global__ __launch_bounds__(256, 4) void k_ls_eval(LST_VIEWS) {
   LST_BIND;
   if (!ls_flat_mode(v)) return;
@@ -592,10 +595,11 @@ __global__ __launch_bounds__(256, 4) void k_ls_eval(LST_VIEWS) {
       const uint2 c = v.WL.cand[L];
       const uint32_t inMD = c.x, outMD = c.y, m = v.D.moduleIndices[inMD], om = v.D.moduleIndices[outMD];
       const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
+      const float4 ti = seg_terms(v.M, m), to = seg_terms(v.M, om);
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
       const float2 bs = v.WL.mdSlope[outMD];
-      pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, bs, o)
-                                                            : seg_endcap(ci, co, v.ptCut, a, b, bs, o);
+      pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, ti, co, to, v.ptCut, a, b, bs, o)
+                                                            : seg_endcap(ci, ti, co, to, v.ptCut, a, b, bs, o);
       v.WL.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
@@ -685,6 +689,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(LST_VIEWS) {
     __syncthreads();
     const uint32_t total = pre[nConn];
     const ModCtx ci = load_mod(v.M, m);
+    const float4 ti = seg_terms(v.M, m);
     const uint32_t inBase = v.R.mdRanges[2 * m];
     const uint32_t segBase = v.segModuleIndices[m];
     for (uint32_t c0 = 0; c0 < total; c0 += kBuildBlock) {
@@ -706,10 +711,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(LST_VIEWS) {
         inMD = inBase + local / nOut;
         outMD = v.R.mdRanges[2 * om] + local % nOut;
         const ModCtx co = load_mod(v.M, om);
+        const float4 to = seg_terms(v.M, om);
         const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
         const float2 bs = v.WL.mdSlope[outMD];
-        pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, co, v.ptCut, a, b, bs, o)
-                                                              : seg_endcap(ci, co, v.ptCut, a, b, bs, o);
+        pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, ti, co, to, v.ptCut, a, b, bs, o)
+                                                              : seg_endcap(ci, ti, co, to, v.ptCut, a, b, bs, o);
       }
       unsigned n;
       const unsigned rank = count + build_rank(pass, lds, n);
