@@ -272,19 +272,18 @@ __device__ __forceinline__ float pt3_rz_chi2(const ModulesView& M, const uint16_
 
 // per-point uncertainty table shared by the pT3/pT5 r-phi chi2 (anchor hits)
 __device__ __forceinline__ void pixel_sigmas(const ModulesView& M, const uint16_t* mods, int n, float* d1, float* d2,
-                                             float* sl, bool* flat) {
+                                             bool* flat) {
   const float inv1 = 0.01f / 0.009f, inv2 = 0.15f / 0.009f;
   for (int i = 0; i < n; i++) {
     const unsigned m = mods[i];
     const int16_t sub = M.subdets[m], side = M.sides[m];
     const uint8_t type = M.moduleType[m];
     const float drdz = M.drdzs[m];
-    sl[i] = M.dxdys[m];
     d1[i] = 0; d2[i] = 0; flat[i] = false;
     if (sub == kBarrel && type == kPS && side == kCenter) {
-      d1[i] = inv1; d2[i] = inv1; sl[i] = -999.f; flat[i] = true;
+      d1[i] = inv1; d2[i] = inv1; flat[i] = true;
     } else if (sub == kBarrel && type == kTwoS) {
-      d1[i] = 1.f; d2[i] = 1.f; sl[i] = -999.f; flat[i] = true;
+      d1[i] = 1.f; d2[i] = 1.f; flat[i] = true;
     } else if (sub == kBarrel && type == kPS) {
       d1[i] = inv1; d2[i] = inv2 * drdz / m_sqrt(1 + drdz * drdz);
     } else if (sub == kEndcap && type == kPS) {
@@ -298,10 +297,10 @@ __device__ __forceinline__ void pixel_sigmas(const ModulesView& M, const uint16_
 // r-phi uncertainty of each point (SDL/PixelTriplet.h:419-487, 2218-2387)
 __device__ __forceinline__ void point_sigmas(const ModulesView& M, const uint16_t* mods, int n, const float* x,
                                              const float* y, float* sig) {
-  float d1[5], d2[5], sl[5];
+  float d1[5], d2[5];
   bool flat[5];
-  pixel_sigmas(M, mods, n, d1, d2, sl, flat);
-  for (int i = 0; i < n; i++) sig[i] = point_sigma(x[i], y[i], d1[i], d2[i], sl[i], flat[i]);
+  pixel_sigmas(M, mods, n, d1, d2, flat);
+  for (int i = 0; i < n; i++) sig[i] = point_sigma(M, mods[i], x[i], y[i], d1[i], d2[i], flat[i]);
 }
 
 // chi2 of points against the circle (g, f, r), sigma per point given

@@ -25,9 +25,13 @@ struct ModLite {  // the module attributes the T5 selection reads
   float drdz, dxdy;
 };
 __device__ __forceinline__ ModLite load_lite(const ModulesView& M, unsigned m) {
+  const float4 r = ld_global(M.modCtx + m);  // one 16-byte load (module_terms.h)
+  const uint32_t w = __float_as_uint(r.x);
+  ModCtx full;
+  unpack_mod(w, full);
   ModLite c;
-  c.subdet = M.subdets[m]; c.side = M.sides[m]; c.layer = M.layers[m]; c.type = M.moduleType[m];
-  c.sdl = M.sdlLayers[m]; c.drdz = M.drdzs[m]; c.dxdy = M.dxdys[m];
+  c.subdet = full.subdet; c.side = full.side; c.layer = full.layer; c.type = full.type;
+  c.sdl = unpack_sdl_layer(w); c.drdz = r.y; c.dxdy = r.z;
   return c;
 }
 __device__ __forceinline__ bool is_endcap_2s(const ModLite& c) { return c.subdet == kEndcap && c.type == kTwoS; }
@@ -540,16 +544,15 @@ __device__ __forceinline__ bool t5_rphi_chi2_ok(int l1, int l2, int l3, int l4, 
 
 // ---------------------------------------------------------- circle regression
 // per-point uncertainties (SDL/Quintuplet.h:1136-1221)
-__device__ __forceinline__ void regression_sigmas(const ModLite* mod, bool anchor, float* d1, float* d2, float* slope,
-                                                  bool* flat) {
+// (the slope of a non-flat module enters through its rotation table, point_sigma)
+__device__ __forceinline__ void regression_sigmas(const ModLite* mod, bool anchor, float* d1, float* d2, bool* flat) {
   const float inv1 = 0.01f / 0.009f, inv2 = 0.15f / 0.009f, inv3 = 2.4f / 0.009f;
   for (int i = 0; i < 5; i++) {
     const ModLite& m = mod[i];
-    slope[i] = m.dxdy;
     if (m.subdet == kBarrel && m.type == kPS && m.side == kCenter) {
-      d1[i] = inv1; d2[i] = inv1; slope[i] = -999.f; flat[i] = true;
+      d1[i] = inv1; d2[i] = inv1; flat[i] = true;
     } else if (m.subdet == kBarrel && m.type == kTwoS) {
-      d1[i] = 1.f; d2[i] = 1.f; slope[i] = -999.f; flat[i] = true;
+      d1[i] = 1.f; d2[i] = 1.f; flat[i] = true;
     } else if (m.subdet == kBarrel && m.type == kPS) {
       d1[i] = inv1; flat[i] = false;
       d2[i] = (anchor ? inv2 : inv3) * m.drdz / m_sqrt(1 + m.drdz * m.drdz);
@@ -562,29 +565,34 @@ __device__ __forceinline__ void regression_sigmas(const ModLite* mod, bool ancho
   }
 }
 
-__device__ __forceinline__ float point_sigma(float x, float y, float d1, float d2, float slope, bool flat) {
-  const float a = slope != kSdlInf ? m_abs(m_atan(slope)) : 0.5f * kPi;
-  float ang;
-  if (x > 0 && y > 0) ang = 0.5f * kPi - a;
-  else if (x < 0 && y > 0) ang = a + 0.5f * kPi;
-  else if (x < 0 && y < 0) ang = -(a + 0.5f * kPi);
-  else if (x > 0 && y < 0) ang = -(0.5f * kPi - a);
-  else ang = 0;
+// the r-phi uncertainty of point (x, y) on module m: the point is rotated by an angle of the module's
+// slope and the point's quadrant, whose cosine and sine the geometry holds (sigRot); a point on an axis
+// keeps angle 0 (cos 1, sin 0)
+__device__ __forceinline__ float point_sigma(const ModulesView& M, unsigned m, float x, float y, float d1, float d2,
+                                             bool flat) {
   float xp = x, yp = y;
   if (!flat) {
-    xp = x * m_cos(ang) + y * m_sin(ang);
-    yp = y * m_cos(ang) - x * m_sin(ang);
+    const int q = (x > 0 && y > 0) ? 0 : (x < 0 && y > 0) ? 1 : (x < 0 && y < 0) ? 2 : (x > 0 && y < 0) ? 3 : 4;
+    float c = 1.f, s = 0.f;
+    if (q < 4) {
+      const float2 r = ld_global(M.sigRot + 4 * m + q);
+      c = r.x;
+      s = r.y;
+    }
+    xp = x * c + y * s;
+    yp = y * c - x * s;
   }
   return 2 * m_sqrt((xp * d1) * (xp * d1) + (yp * d2) * (yp * d2));
 }
 
 // weighted algebraic circle fit and its chi2 (SDL/Quintuplet.h:1223-1320)
-__device__ __forceinline__ float regression_fit(const float* x, const float* y, const float* d1, const float* d2,
-                                                const float* slope, const bool* flat, float& g, float& f, float& chi2) {
+__device__ __forceinline__ float regression_fit(const ModulesView& M, const uint16_t* mods, const float* x,
+                                                const float* y, const float* d1, const float* d2, const bool* flat,
+                                                float& g, float& f, float& chi2) {
   float sxx = 0.f, syy = 0.f, sxy = 0.f, sxr = 0.f, syr = 0.f, sr = 0.f, sx = 0.f, sy = 0.f, s1 = 0.f;
   float sig[5];
   for (int i = 0; i < 5; i++) {
-    sig[i] = point_sigma(x[i], y[i], d1[i], d2[i], slope[i], flat[i]);
+    sig[i] = point_sigma(M, mods[i], x[i], y[i], d1[i], d2[i], flat[i]);
     const float w2 = sig[i] * sig[i];
     sxx += (x[i] * x[i]) / w2;
     syy += (y[i] * y[i]) / w2;
@@ -616,12 +624,13 @@ __device__ __forceinline__ float regression_fit(const float* x, const float* y, 
 }
 
 // chi2 of points against a given circle (SDL/Quintuplet.h:1322-1367)
-__device__ __forceinline__ float circle_chi2(const float* x, const float* y, const float* d1, const float* d2,
-                                             const float* slope, const bool* flat, float g, float f, float r) {
+__device__ __forceinline__ float circle_chi2(const ModulesView& M, const uint16_t* mods, const float* x, const float* y,
+                                             const float* d1, const float* d2, const bool* flat, float g, float f,
+                                             float r) {
   const float c = g * g + f * f - r * r;
   float chi2 = 0.f;
   for (int i = 0; i < 5; i++) {
-    const float s = point_sigma(x[i], y[i], d1[i], d2[i], slope[i], flat[i]);
+    const float s = point_sigma(M, mods[i], x[i], y[i], d1[i], d2[i], flat[i]);
     const float t = x[i] * x[i] + y[i] * y[i] - 2 * g * x[i] - 2 * f * y[i] + c;
     chi2 += t * t / (s * s);
   }

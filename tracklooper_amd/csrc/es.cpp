@@ -182,6 +182,19 @@ bool load_host_geometry(const std::string& dir, uint32_t stride, HostGeometry& g
     if (g.dxdys[i] == 0) g.dxdys[i] = g.dxdys[f->second];
   }
   // the module-only terms of the strip-hit shift, with the float expressions of the per-hit step
+  // the rotation of a point's r-phi uncertainty in the circle regressions (SDL/Quintuplet.h:1136-1221,
+  // SDL/PixelTriplet.h:419-487): by an angle of the module's slope and the quadrant of the point,
+  // q0 x>0 y>0, q1 x<0 y>0, q2 x<0 y<0, q3 x>0 y<0 (a point on an axis is not rotated)
+  g.sigRot.assign(8 * (size_t)nMod, 0.f);
+  for (uint32_t i = 0; i < nMod; i++) {
+    const float slope = g.dxdys[i];
+    const float a = slope != kSdlInf ? m_abs(m_atan(slope)) : 0.5f * kPi;
+    const float ang[4] = {0.5f * kPi - a, a + 0.5f * kPi, -(a + 0.5f * kPi), -(0.5f * kPi - a)};
+    for (int q = 0; q < 4; q++) {
+      g.sigRot[8 * (size_t)i + 2 * q] = m_cos(ang[q]);
+      g.sigRot[8 * (size_t)i + 2 * q + 1] = m_sin(ang[q]);
+    }
+  }
   g.shiftAngleB.assign(nMod, 0.f);
   g.shiftSinM.assign(4 * (size_t)nMod, 0.f);
   g.shiftCosM.assign(4 * (size_t)nMod, 0.f);
@@ -352,11 +365,11 @@ bool fill_module_terms(HostGeometry& g, float ptCut, std::string& err) {
     c.subdet = g.subdets[i]; c.side = g.sides[i]; c.layer = g.layers[i]; c.ring = g.rings[i]; c.rod = g.rods[i];
     c.type = g.moduleType[i]; c.layerType = g.moduleLayerType[i];
     c.drdz = g.drdzs[i]; c.dxdy = g.dxdys[i]; c.partnerDrdz = g.drdzs[g.partnerModuleIndices[i]];
-    if (!mod_packable(c)) {
+    if (!mod_packable(c, g.sdlLayers[i])) {
       err = "module " + std::to_string(i) + ": attributes outside the packed ranges";
       return false;
     }
-    const uint32_t w = pack_mod(c);
+    const uint32_t w = pack_mod(c, g.sdlLayers[i]);
     float* ctx = &g.modCtx[4 * (size_t)i];
     std::memcpy(ctx, &w, 4);
     ctx[1] = c.drdz; ctx[2] = c.dxdy; ctx[3] = c.partnerDrdz;
@@ -383,7 +396,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
          o_mdi = c.add(h.mdIndices), o_sgi = c.add(h.segIndices), o_t3o = c.add(h.t3OccIfAny),
          o_t5o = c.add(h.t5OccIfAny), o_dh = c.add(h.detHash), o_sab = c.add(h.shiftAngleB),
          o_ssm = c.add(h.shiftSinM), o_scm = c.add(h.shiftCosM), o_mcx = c.add(h.modCtx),
-         o_mtm = c.add(h.modTerms);
+         o_mtm = c.add(h.modTerms), o_sgr = c.add(h.sigRot);
   char* base = nullptr;
   if (hipMalloc(&base, c.total) != hipSuccess) { err = "hipMalloc failed for geometry"; return false; }
   for (auto& it : c.items)
@@ -409,7 +422,7 @@ bool upload_geometry(const HostGeometry& h, DeviceGeometry& d, std::string& err)
   v.connectedPixels = (const uint32_t*)P(o_cp); v.geoMapDetId = (const uint32_t*)P(o_gd);
   v.geoMapPhi = (const float*)P(o_gp);
   v.shiftAngleB = (const float*)P(o_sab); v.shiftSinM = (const float*)P(o_ssm); v.shiftCosM = (const float*)P(o_scm);
-  v.modCtx = (const float4*)P(o_mcx); v.modTerms = (const float4*)P(o_mtm);
+  v.modCtx = (const float4*)P(o_mcx); v.modTerms = (const float4*)P(o_mtm); v.sigRot = (const float2*)P(o_sgr);
   v.nModules = h.nModules; v.nLowerModules = h.nLowerModules; v.stride = h.stride; v.nEndCapMap = h.geoMapDetId.size();
   v.detHash = (const uint2*)P(o_dh); v.detHashBits = h.detHashBits;
   d.pixIndex[0] = (const uint32_t*)P(o_pi0); d.pixIndex[1] = (const uint32_t*)P(o_pi1);

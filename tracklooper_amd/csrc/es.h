@@ -37,6 +37,7 @@ struct HostGeometry {
   // the endcap, else atan(drdz)) and, per quadrant of the pixel-side hit, sin and cos of the slope angle
   std::vector<float> shiftAngleB, shiftSinM, shiftCosM;
   std::vector<float> modCtx, modTerms;  // 4 and 8 floats per module (fill_module_terms)
+  std::vector<float> sigRot;  // 8 floats per module: cos, sin of the point rotation per quadrant
   // pLS superbin -> connected-module lists (SDL/PixelMap.h:16-36); the pos/neg
   // index arrays already include the offsets the reference adds per event
   // (SDL/Event.cc:813-816), so a pLS only needs (superbin, pixelType).

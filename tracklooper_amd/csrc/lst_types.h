@@ -66,6 +66,7 @@ struct ModulesView {
   const float* shiftCosM;
   const float4* modCtx;    // [nModules] packed attributes, drdz, dxdy, partner drdz (module_terms.h)
   const float4* modTerms;  // [2 nModules] module-only terms of the segment and mini-doublet cuts
+  const float2* sigRot;    // [4 nModules] cos, sin of the r-phi regression's point rotation per quadrant
   const uint32_t* connectedPixels;
   const uint32_t* geoMapDetId;  // endcap 2S orientation map, sorted
   const float* geoMapPhi;
@@ -75,14 +76,14 @@ struct ModulesView {
   uint32_t nModules, nLowerModules, stride, nEndCapMap, detHashBits;
 };
 // a load issued as a global (not flat) memory load: p addresses device memory
-// (16-byte types are loaded as one native vector)
+// (vector types, 8 or 16 bytes, are loaded as one native vector)
 template <class T>
 __device__ __forceinline__ T ld_global(const T* p) {
-  if constexpr (sizeof(T) == 16) {
-    typedef unsigned V4 __attribute__((ext_vector_type(4)));
-    const V4 x = *(const __attribute__((address_space(1))) V4*)p;
+  if constexpr (sizeof(T) == 16 || sizeof(T) == 8) {
+    typedef unsigned V __attribute__((ext_vector_type(sizeof(T) / 4)));
+    const V x = *(const __attribute__((address_space(1))) V*)p;
     T r;
-    __builtin_memcpy(&r, &x, 16);
+    __builtin_memcpy(&r, &x, sizeof(T));
     return r;
   } else {
     return *(const __attribute__((address_space(1))) T*)p;

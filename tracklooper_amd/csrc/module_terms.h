@@ -2,7 +2,7 @@
 // cuts.  The same functions run on the host, where the geometry loader
 // evaluates them once per module into two device tables, and on the device.
 //
-// modCtx[m] (16 bytes): the integer attributes packed in one word, then drdz,
+// modCtx[m] (16 bytes): the integer attributes (and the SDL layer number) packed in one word, then drdz,
 // dxdy and the partner module's drdz.  A selection loads a module's context
 // with one 16-byte load instead of ten scattered ones.
 //
@@ -29,14 +29,15 @@ struct ModCtx {
 };
 
 // packed integer attributes: subdet [0,3) side [3,5) layer [5,9) ring [9,14) rod [14,22) type [22,24)
-// layerType [24,26); mod_packable says whether a module's values fit
-LST_HD bool mod_packable(const ModCtx& c) {
+// layerType [24,26) sdlLayer [26,31); mod_packable says whether a module's values fit
+LST_HD bool mod_packable(const ModCtx& c, int sdlLayer) {
   return c.subdet >= 0 && c.subdet < 8 && c.side >= 0 && c.side < 4 && c.layer >= 0 && c.layer < 16 &&
-         c.ring >= 0 && c.ring < 32 && c.rod >= 0 && c.rod < 256 && c.type < 4 && c.layerType < 4;
+         c.ring >= 0 && c.ring < 32 && c.rod >= 0 && c.rod < 256 && c.type < 4 && c.layerType < 4 && sdlLayer >= 0 &&
+         sdlLayer < 32;
 }
-LST_HD uint32_t pack_mod(const ModCtx& c) {
+LST_HD uint32_t pack_mod(const ModCtx& c, int sdlLayer) {
   return (uint32_t)c.subdet | (uint32_t)c.side << 3 | (uint32_t)c.layer << 5 | (uint32_t)c.ring << 9 |
-         (uint32_t)c.rod << 14 | (uint32_t)c.type << 22 | (uint32_t)c.layerType << 24;
+         (uint32_t)c.rod << 14 | (uint32_t)c.type << 22 | (uint32_t)c.layerType << 24 | (uint32_t)sdlLayer << 26;
 }
 LST_HD void unpack_mod(uint32_t w, ModCtx& c) {
   c.subdet = (int16_t)(w & 7u);
@@ -47,6 +48,7 @@ LST_HD void unpack_mod(uint32_t w, ModCtx& c) {
   c.type = (uint8_t)((w >> 22) & 3u);
   c.layerType = (uint8_t)((w >> 24) & 3u);
 }
+LST_HD int unpack_sdl_layer(uint32_t w) { return (int)((w >> 26) & 31u); }
 
 // Separation between the two sensors of a module (SDL/MiniDoublet.h:264-337,
 // same values as SDL/Segment.h:202-262).

@@ -172,15 +172,15 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
   if (cat >= 6) radius_range_of(m[0], p[0], m[1], p[1], m[2], p[2], iMin, iMax);
   if (cat >= 2) radius_range_of(m[1], p[1], m[2], p[2], m[3], p[3], bMin, bMax);
   if (!radii_compatible(cat, r.innerR, r.bridgeR, iMin, iMax, bMin, bMax)) return false;
-  float x[5], y[5], d1[5], d2[5], sl[5];
+  float x[5], y[5], d1[5], d2[5];
   bool flat[5];
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     x[k] = p[k].x;
     y[k] = p[k].y;
   }
-  regression_sigmas(m, true, d1, d2, sl, flat);
-  r.regR = regression_fit(x, y, d1, d2, sl, flat, r.regG, r.regF, r.chi2);
+  regression_sigmas(m, true, d1, d2, flat);
+  r.regR = regression_fit(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.chi2);
   // r-phi chi2 cut of the plain mode (USE_RPHICHI2, SDL/Quintuplet.h:2943-2956);
   // the DNN mode applies the T5 DNN instead (k_t5_dnn / t5dnn_score)
   if (!v.t5Dnn && r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
@@ -191,8 +191,8 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
     x[k] = v.D.outerX[c.md[k]];
     y[k] = v.D.outerY[c.md[k]];
   }
-  regression_sigmas(m, false, d1, d2, sl, flat);
-  r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
+  regression_sigmas(m, false, d1, d2, flat);
+  r.nonAnchorChi2 = circle_chi2(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.regR);
   return true;
 }
 
@@ -242,15 +242,15 @@ __device__ __forceinline__ bool t5_rest_half(const EventView& v, const T5Cand& c
     if (cat >= 2) radius_range_of(m[1], p[1], m[2], p[2], m[3], p[3], bMin, bMax);
     return radii_compatible(cat, r.innerR, r.bridgeR, iMin, iMax, bMin, bMax);
   }
-  float x[5], y[5], d1[5], d2[5], sl[5];
+  float x[5], y[5], d1[5], d2[5];
   bool flat[5];
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     x[k] = p[k].x;
     y[k] = p[k].y;
   }
-  regression_sigmas(m, true, d1, d2, sl, flat);
-  r.regR = regression_fit(x, y, d1, d2, sl, flat, r.regG, r.regF, r.chi2);
+  regression_sigmas(m, true, d1, d2, flat);
+  r.regR = regression_fit(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.chi2);
   if (!v.t5Dnn && r.regR < 5.0f / (2.f * k2Rinv1GeVf) &&
       !t5_rphi_chi2_ok(m[0].sdl, m[1].sdl, m[2].sdl, m[3].sdl, m[4].sdl, r.chi2))
     return false;
@@ -259,8 +259,8 @@ __device__ __forceinline__ bool t5_rest_half(const EventView& v, const T5Cand& c
     x[k] = v.D.outerX[c.md[k]];
     y[k] = v.D.outerY[c.md[k]];
   }
-  regression_sigmas(m, false, d1, d2, sl, flat);
-  r.nonAnchorChi2 = circle_chi2(x, y, d1, d2, sl, flat, r.regG, r.regF, r.regR);
+  regression_sigmas(m, false, d1, d2, flat);
+  r.nonAnchorChi2 = circle_chi2(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.regR);
   return true;
 }
 
