@@ -291,15 +291,16 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.t5Rec, (size_t)lst::kT5RecWords * (t5Cap ? t5Cap : 1));
   c.take(e.W.t3Rec, (size_t)lst::kT3RecWords * (t3Cap ? t3Cap : 1));
   c.take(e.W.pt5BucketOff, lst::kPT5HashBuckets + 1); c.take(e.W.pt5BucketEnt, 14 * (size_t)pt5Cap);
-  c.take(e.FS.pre, lst::kScanCandMax); c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
+  c.take(e.FS.mask, lst::kScanCandMax / 64); c.take(e.FS.pre, lst::kScanCandMax / 64);
+  c.take(e.FS.sums, lst::kScanCandMax / 4096 + 1);
   c.take(e.FS.pass, lst::kScanCandMax);
-  c.take(e.W2.candOff, nL + 1); c.take(e.W2.resIdx, lst::kMDCandMax); c.take(e.W2.hitSlope, hitCap); c.take(e.W2.shiftPoint, hitCap);
+  c.take(e.W2.candOff, nL + 1); c.take(e.W2.hitSlope, hitCap); c.take(e.W2.shiftPoint, hitCap);
   c.take(e.W2.park, (size_t)lst::kMDParkWords * lst::kMDCandMax); c.take(e.W2.cand, lst::kMDCandMax);
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
-  c.take(e.WL.resIdx, lst::kLSCandMax); c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
+  c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
   c.take(e.WL.cand, lst::kLSCandMax);
   c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
-  c.take(e.W3.resIdx, lst::kT3CandMax); c.take(e.W3.list, 2 * (size_t)lst::kT3CandMax);
+  c.take(e.W3.list, 2 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3CandMax);
   c.take(e.W3.survPre, segCap); c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.segTmp, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
   c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);

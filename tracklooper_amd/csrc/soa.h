@@ -153,7 +153,8 @@ struct TCSoA { LST_TC_FIELDS(LST_DECL_FIELD) };
 // Pass-flag prefix shared by the flattened builders (one builder at a time).
 constexpr uint32_t kScanCandMax = 1u << 23;  // >= every builder's candidate maximum
 struct FlatScan {
-  uint32_t* pre;   // [kScanCandMax] exclusive prefix of the pass flags inside each scan block
+  uint64_t* mask;  // [kScanCandMax / 64] pass flags, candidate L is bit L % 64 of word L / 64
+  uint32_t* pre;   // [kScanCandMax / 64] per flag word: passing candidates before it in its scan block
   uint32_t* sums;  // [kScanCandMax / 4096 + 1] exclusive prefix of the block totals, then the total
   uint32_t* pass;  // [kScanCandMax] per scan block, its passing candidates in order (block b from b * 4096)
 };
@@ -166,7 +167,6 @@ constexpr uint32_t kMDCandMax = 1u << 21;
 constexpr int kMDParkWords = 9;
 struct MDWork {
   uint32_t* candOff;  // [nLower + 1] first candidate of each module, then the total
-  uint32_t* resIdx;   // [kMDCandMax] park slot of a passing candidate, ~0: failed
   uint32_t* park;     // [kMDParkWords * kMDCandMax] MDOut of passing candidate L at L
   uint4* cand;        // [kMDCandMax] candidate L: (lower hit, upper hit, module, 0)
   float* hitSlope;    // [hit capacity] per hit: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax))
@@ -182,7 +182,6 @@ constexpr int kLSParkWords = 6;
 struct LSWork {
   uint32_t* connPre;  // [nLower * (stride + 1)] per module: candidate prefix over its connected modules
   uint32_t* candOff;  // [nLower + 1]
-  uint32_t* resIdx;   // [kLSCandMax]
   uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
   uint2* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD); the modules are in the MD records
   MDField<float2> mdSlope;  // in the MD record: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
@@ -194,7 +193,6 @@ constexpr int kT3ResultWords = 6;
 struct T3Work {
   uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
   uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
-  uint32_t* resIdx;   // [kT3CandMax] per list entry: its own position when it passes, else ~0
   uint32_t* list;     // [2 * kT3CandMax] (module, inner/outer segment offsets) of the candidates passing the
                       //   shared-MD check, in serial order
   uint32_t* results;  // [kT3ResultWords * kT3CandMax] selection results of list entry i at i

@@ -356,14 +356,13 @@ __global__ __launch_bounds__(256, 5) void k_md_eval(LST_VIEWS) {
       if (md_shifts(c)) p.sp = v.W2.shiftPoint[md_lower_is_pixel_side(c) ? a : b];
       const float slope = v.W2.hitSlope[c.layerType == kPixelLayer ? a : b];  // the rt of the pixel-side hit
       pass = c.subdet == kBarrel ? md_barrel(c, mt, p, slope, o) : md_endcap(c, mt, p, slope, o);
-      v.W2.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
       float* w = reinterpret_cast<float*>(&v.W2.park[kMDParkWords * (size_t)L]);
       w[0] = o.dz; w[1] = o.dphi; w[2] = o.dphichange; w[3] = o.sx; w[4] = o.sy; w[5] = o.sz;
       w[6] = o.nsdz; w[7] = o.nsdphi; w[8] = o.nsdphichange;
-      v.W2.resIdx[L] = L;
     }
+    store_flag_word(v.FS, L0 + (threadIdx.x & ~63u), total, pass);
   }
 }
 
@@ -372,8 +371,7 @@ __global__ __launch_bounds__(256, 5) void k_md_eval(LST_VIEWS) {
 __global__ __launch_bounds__(1024) void k_md_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!md_flat_done(v)) return;
-  scan_flag_blocks(
-      v.R.scalars[kMDCandTotal], [&](uint32_t L) { return v.W2.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
+  scan_mask_blocks(v.FS, v.R.scalars[kMDCandTotal]);
 }
 __global__ __launch_bounds__(1024) void k_md_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -600,22 +598,20 @@ __global__ __launch_bounds__(256, 4) void k_ls_eval(LST_VIEWS) {
       const float2 bs = v.WL.mdSlope[outMD];
       pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, ti, co, to, v.ptCut, a, b, bs, o)
                                                             : seg_endcap(ci, ti, co, to, v.ptCut, a, b, bs, o);
-      v.WL.resIdx[L] = ~0u;
     }
     if (pass) {  // parked at the candidate's own index
       float* w = reinterpret_cast<float*>(&v.WL.park[kLSParkWords * (size_t)L]);
       w[0] = o.dPhi; w[1] = o.dPhiMin; w[2] = o.dPhiMax;
       w[3] = o.dPhiChange; w[4] = o.dPhiChangeMin; w[5] = o.dPhiChangeMax;
-      v.WL.resIdx[L] = L;
     }
+    store_flag_word(v.FS, L0 + (threadIdx.x & ~63u), total, pass);
   }
 }
 
 __global__ __launch_bounds__(1024) void k_ls_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!ls_flat_done(v)) return;
-  scan_flag_blocks(
-      v.R.scalars[kLSCandTotal], [&](uint32_t L) { return v.WL.resIdx[L] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
+  scan_mask_blocks(v.FS, v.R.scalars[kLSCandTotal]);
 }
 __global__ __launch_bounds__(1024) void k_ls_scan_sums(LST_VIEWS) {
   LST_BIND;
@@ -934,25 +930,28 @@ __global__ __launch_bounds__(256) void k_t3_enum(LST_VIEWS) {
 }
 
 // stage 2: the triplet selection; results parked at the list position
-__global__ __launch_bounds__(256) void k_t3_select(LST_VIEWS) {
+// four waves per SIMD: at most 128 VGPRs (no spills; the compiler's own allocation can exceed 128)
+__global__ __launch_bounds__(256, 4) void k_t3_select(LST_VIEWS) {
   LST_BIND;
   if (!t3_flat_mode(v)) return;
   const uint32_t n = v.R.scalars[kT3List];
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
     const uint32_t i = i0 + threadIdx.x;
-    if (i >= n) continue;
-    const uint32_t m1 = v.W3.list[2 * i], ab = v.W3.list[2 * i + 1];
-    const uint32_t inSeg = v.segModuleIndices[m1] + (ab & 0xFFFFu);
-    const uint32_t outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (ab >> 16);
-    float betaIn = 0, radius = 0, gx = 0, gy = 0;
-    const bool pass = t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy);
-    if (pass) {  // results at the list position: no slot taken with an atomic
-      uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)i];
-      w[0] = inSeg; w[1] = outSeg;
-      w[2] = __float_as_uint(betaIn); w[3] = __float_as_uint(radius);
-      w[4] = __float_as_uint(gx); w[5] = __float_as_uint(gy);
+    bool pass = false;
+    if (i < n) {
+      const uint32_t m1 = v.W3.list[2 * i], ab = v.W3.list[2 * i + 1];
+      const uint32_t inSeg = v.segModuleIndices[m1] + (ab & 0xFFFFu);
+      const uint32_t outSeg = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]] + (ab >> 16);
+      float betaIn = 0, radius = 0, gx = 0, gy = 0;
+      pass = t3_select(v, m1, inSeg, outSeg, betaIn, radius, gx, gy);
+      if (pass) {  // results at the list position: no slot taken with an atomic
+        uint32_t* w = &v.W3.results[kT3ResultWords * (size_t)i];
+        w[0] = inSeg; w[1] = outSeg;
+        w[2] = __float_as_uint(betaIn); w[3] = __float_as_uint(radius);
+        w[4] = __float_as_uint(gx); w[5] = __float_as_uint(gy);
+      }
     }
-    v.W3.resIdx[i] = pass ? i : ~0u;
+    store_flag_word(v.FS, i0 + (threadIdx.x & ~63u), n, pass);
   }
 }
 
@@ -962,8 +961,7 @@ __global__ __launch_bounds__(256) void k_t3_select(LST_VIEWS) {
 __global__ __launch_bounds__(1024) void k_t3_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!t3_flat_done(v)) return;
-  scan_flag_blocks(
-      v.R.scalars[kT3List], [&](uint32_t k) { return v.W3.resIdx[k] != ~0u; }, v.FS.pre, v.FS.sums, v.FS.pass);
+  scan_mask_blocks(v.FS, v.R.scalars[kT3List]);
 }
 __global__ __launch_bounds__(1024) void k_t3_scan_sums(LST_VIEWS) {
   LST_BIND;

@@ -673,7 +673,7 @@ __device__ __forceinline__ bool t5_passed(const EventView& v, uint32_t k) {
 __global__ __launch_bounds__(1024) void k_t5_scan_blocks(LST_VIEWS) {
   LST_BIND;
   if (!t5_flat_done(v)) return;
-  scan_flag_blocks(v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); }, v.FS.pre, v.FS.sums, v.FS.pass);
+  scan_flag_blocks(v.FS, v.R.scalars[kT5ListA], [&](uint32_t k) { return t5_passed(v, k); });
 }
 __global__ __launch_bounds__(1024) void k_t5_scan_sums(LST_VIEWS) {
   LST_BIND;
