@@ -176,13 +176,16 @@ __device__ __forceinline__ bool md_endcap(const ModCtx& c, float4 mt, const MDPa
 struct MDRef {  // the MD fields a segment selection reads
   float x, y, z, rt, phi, dphichange, dphi, dz, hiEdgePhi, loEdgePhi;
   uint32_t anchorHit;
+  uint16_t mod;
 };
+// the first three 16-byte chunks of the record
 __device__ __forceinline__ MDRef load_md(const MDsSoA& D, unsigned i) {
+  const uint4 q0 = md_chunk(D, i, 0), q1 = md_chunk(D, i, 1), q2 = md_chunk(D, i, 2);
   MDRef r;
-  r.x = D.anchorX[i]; r.y = D.anchorY[i]; r.z = D.anchorZ[i]; r.rt = D.anchorRt[i]; r.phi = D.anchorPhi[i];
-  r.dphichange = D.dphichanges[i]; r.dphi = D.dphis[i]; r.dz = D.dzs[i];
-  r.hiEdgePhi = D.anchorHighEdgePhi[i]; r.loEdgePhi = D.anchorLowEdgePhi[i];
-  r.anchorHit = D.anchorHitIndices[i];
+  r.anchorHit = q0.x; r.mod = (uint16_t)q0.y; r.dphichange = __uint_as_float(q0.z); r.dz = __uint_as_float(q0.w);
+  r.x = __uint_as_float(q1.x); r.y = __uint_as_float(q1.y); r.z = __uint_as_float(q1.z); r.rt = __uint_as_float(q1.w);
+  r.dphi = __uint_as_float(q2.x); r.phi = __uint_as_float(q2.y);
+  r.hiEdgePhi = __uint_as_float(q2.z); r.loEdgePhi = __uint_as_float(q2.w);
   return r;
 }
 

@@ -10,11 +10,13 @@ namespace lst {
 struct MDPoint {  // anchor hit of a mini-doublet, plus its strip edges
   float x, y, z, rt, phi, hx, hy, lx, ly, hphi, lphi;
 };
+// chunks 1-3 of the record
 __device__ __forceinline__ MDPoint load_point(const MDsSoA& D, unsigned i) {
+  const uint4 q1 = md_chunk(D, i, 1), q2 = md_chunk(D, i, 2), q3 = md_chunk(D, i, 3);
   MDPoint p;
-  p.x = D.anchorX[i]; p.y = D.anchorY[i]; p.z = D.anchorZ[i]; p.rt = D.anchorRt[i]; p.phi = D.anchorPhi[i];
-  p.hx = D.anchorHighEdgeX[i]; p.hy = D.anchorHighEdgeY[i]; p.lx = D.anchorLowEdgeX[i]; p.ly = D.anchorLowEdgeY[i];
-  p.hphi = D.anchorHighEdgePhi[i]; p.lphi = D.anchorLowEdgePhi[i];
+  p.x = __uint_as_float(q1.x); p.y = __uint_as_float(q1.y); p.z = __uint_as_float(q1.z); p.rt = __uint_as_float(q1.w);
+  p.phi = __uint_as_float(q2.y); p.hphi = __uint_as_float(q2.z); p.lphi = __uint_as_float(q2.w);
+  p.hx = __uint_as_float(q3.x); p.hy = __uint_as_float(q3.y); p.lx = __uint_as_float(q3.z); p.ly = __uint_as_float(q3.w);
   return p;
 }
 

@@ -5,6 +5,7 @@
 // SDL/Triplet.h:17-31, SDL/Quintuplet.h:20-48) so tests compare fields by name against the oracle.
 #pragma once
 #include <cstdint>
+#include "lst_types.h"
 
 // X(type, name, per-object multiplicity)
 #define LST_HIT_FIELDS(X)                                                                                  \
@@ -12,17 +13,21 @@
   X(uint32_t, detid, 1) X(float, rts, 1) X(float, phis, 1) X(float, etas, 1) X(float, highEdgeXs, 1)      \
   X(float, highEdgeYs, 1) X(float, lowEdgeXs, 1) X(float, lowEdgeYs, 1)
 
+// Mini-doublet fields in record order (4-byte words).  The first four
+// 16-byte chunks hold what the selections read: chunk 0 the segment cut's
+// anchor hit, module, dphichange and dz; chunk 1 the anchor point (x, y, z,
+// rt) every later stage reads; chunk 2 dphi, phi and the strip-edge phis;
+// chunk 3 the strip-edge points of the T4/T5 tests.
 #define LST_MD_FIELDS(X)                                                                                   \
-  X(uint32_t, anchorHitIndices, 1) X(uint32_t, outerHitIndices, 1) X(uint16_t, moduleIndices, 1)           \
-  X(float, dphichanges, 1) X(float, dzs, 1) X(float, dphis, 1) X(float, shiftedXs, 1)                      \
+  X(uint32_t, anchorHitIndices, 1) X(uint16_t, moduleIndices, 1) X(float, dphichanges, 1) X(float, dzs, 1) \
+  X(float, anchorX, 1) X(float, anchorY, 1) X(float, anchorZ, 1) X(float, anchorRt, 1)                     \
+  X(float, dphis, 1) X(float, anchorPhi, 1) X(float, anchorHighEdgePhi, 1) X(float, anchorLowEdgePhi, 1)   \
+  X(float, anchorHighEdgeX, 1) X(float, anchorHighEdgeY, 1) X(float, anchorLowEdgeX, 1)                    \
+  X(float, anchorLowEdgeY, 1) X(uint32_t, outerHitIndices, 1) X(float, anchorEta, 1) X(float, shiftedXs, 1) \
   X(float, shiftedYs, 1) X(float, shiftedZs, 1) X(float, noShiftedDzs, 1) X(float, noShiftedDphis, 1)      \
-  X(float, noShiftedDphiChanges, 1) X(float, anchorX, 1) X(float, anchorY, 1) X(float, anchorZ, 1)         \
-  X(float, anchorRt, 1) X(float, anchorPhi, 1) X(float, anchorEta, 1) X(float, anchorHighEdgeX, 1)         \
-  X(float, anchorHighEdgeY, 1) X(float, anchorLowEdgeX, 1) X(float, anchorLowEdgeY, 1)                     \
-  X(float, anchorHighEdgePhi, 1) X(float, anchorLowEdgePhi, 1) X(float, outerX, 1) X(float, outerY, 1)     \
-  X(float, outerZ, 1) X(float, outerRt, 1) X(float, outerPhi, 1) X(float, outerEta, 1)                     \
-  X(float, outerHighEdgeX, 1) X(float, outerHighEdgeY, 1) X(float, outerLowEdgeX, 1)                       \
-  X(float, outerLowEdgeY, 1)
+  X(float, noShiftedDphiChanges, 1) X(float, outerX, 1) X(float, outerY, 1) X(float, outerZ, 1)            \
+  X(float, outerRt, 1) X(float, outerPhi, 1) X(float, outerEta, 1) X(float, outerHighEdgeX, 1)             \
+  X(float, outerHighEdgeY, 1) X(float, outerLowEdgeX, 1) X(float, outerLowEdgeY, 1)
 
 #define LST_SEG_FIELDS(X)                                                                                  \
   X(uint32_t, mdIndices, 2) X(uint16_t, innerLowerModuleIndices, 1) X(uint16_t, outerLowerModuleIndices, 1) \
@@ -100,6 +105,16 @@ struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
 #define LST_MD_WORD(T, N, M) kMDw_##N,
 enum MDWord : uint32_t { LST_MD_FIELDS(LST_MD_WORD) kMDWordCount };
 static_assert(kMDWordCount == kMDFields, "one word per mini-doublet field");
+static_assert(kMDw_anchorHitIndices == 0 && kMDw_moduleIndices == 1 && kMDw_dphichanges == 2 && kMDw_dzs == 3 &&
+                  kMDw_anchorX == 4 && kMDw_anchorY == 5 && kMDw_anchorZ == 6 && kMDw_anchorRt == 7 &&
+                  kMDw_dphis == 8 && kMDw_anchorPhi == 9 && kMDw_anchorHighEdgePhi == 10 &&
+                  kMDw_anchorLowEdgePhi == 11 && kMDw_anchorHighEdgeX == 12 && kMDw_anchorHighEdgeY == 13 &&
+                  kMDw_anchorLowEdgeX == 14 && kMDw_anchorLowEdgeY == 15,
+              "the chunks md_chunk readers expect");
+// 16-byte chunk q (words 4q .. 4q + 3) of mini-doublet record i, one global load
+__device__ __forceinline__ uint4 md_chunk(const MDsSoA& D, uint32_t i, uint32_t q) {
+  return ld_global(reinterpret_cast<const uint4*>(D.anchorHitIndices.p + (size_t)i * kMDRecBytes) + q);
+}
 // Segments are records too: 12 four-byte words per segment (the two MD
 // indices, then one word per other field, in LST_SEG_FIELDS order; the u16
 // module indices in the low half of their word), 48 bytes.  The triplet, T5

@@ -591,11 +591,12 @@ __global__ __launch_bounds__(256, 4) void k_ls_eval(LST_VIEWS) {
     SegOut o{0, 0, 0, 0, 0, 0};
     if (L < total) {
       const uint2 c = v.WL.cand[L];
-      const uint32_t inMD = c.x, outMD = c.y, m = v.D.moduleIndices[inMD], om = v.D.moduleIndices[outMD];
-      const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
-      const float4 ti = seg_terms(v.M, m), to = seg_terms(v.M, om);
+      const uint32_t inMD = c.x, outMD = c.y;
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
       const float2 bs = v.WL.mdSlope[outMD];
+      const uint32_t m = a.mod, om = b.mod;
+      const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
+      const float4 ti = seg_terms(v.M, m), to = seg_terms(v.M, om);
       pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, ti, co, to, v.ptCut, a, b, bs, o)
                                                             : seg_endcap(ci, ti, co, to, v.ptCut, a, b, bs, o);
     }
@@ -760,11 +761,12 @@ __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint3
   const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
   if (v.S.mdIndices[2 * inSeg + 1] != md2) return false;  // SDL/Triplet.h:823-825
   const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
-  T3Geo p;
-  p.x1 = v.D.anchorX[md1]; p.y1 = v.D.anchorY[md1]; p.z1 = v.D.anchorZ[md1]; p.r1 = v.D.anchorRt[md1];
+  T3Geo p;  // the anchor points: chunk 1 of each record, and the first one's phi from its chunk 2
+  const uint4 a1 = md_chunk(v.D, md1, 1), a2 = md_chunk(v.D, md2, 1), a3 = md_chunk(v.D, md3, 1);
+  p.x1 = __uint_as_float(a1.x); p.y1 = __uint_as_float(a1.y); p.z1 = __uint_as_float(a1.z); p.r1 = __uint_as_float(a1.w);
   p.phi1 = v.D.anchorPhi[md1];
-  p.x2 = v.D.anchorX[md2]; p.y2 = v.D.anchorY[md2]; p.z2 = v.D.anchorZ[md2]; p.r2 = v.D.anchorRt[md2];
-  p.x3 = v.D.anchorX[md3]; p.y3 = v.D.anchorY[md3]; p.z3 = v.D.anchorZ[md3]; p.r3 = v.D.anchorRt[md3];
+  p.x2 = __uint_as_float(a2.x); p.y2 = __uint_as_float(a2.y); p.z2 = __uint_as_float(a2.z); p.r2 = __uint_as_float(a2.w);
+  p.x3 = __uint_as_float(a3.x); p.y3 = __uint_as_float(a3.y); p.z3 = __uint_as_float(a3.z); p.r3 = __uint_as_float(a3.w);
   p.slope3 = v.WL.mdSlope[md3];
   if (!t3_rz(v.M.sdlLayers[m1], v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) return false;
   const int16_t s1 = v.M.subdets[m1], s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
