@@ -20,6 +20,10 @@ __device__ __forceinline__ ModCtx load_mod(const ModulesView& M, unsigned m) {
   c.drdz = r.y; c.dxdy = r.z; c.partnerDrdz = r.w;
   return c;
 }
+// the packed attribute word of module m (module_terms.h layout)
+__device__ __forceinline__ uint32_t mod_word(const ModulesView& M, unsigned m) {
+  return ld_global(reinterpret_cast<const uint32_t*>(M.modCtx + m));
+}
 // {muls, segTilt, segRes, -} of module m
 __device__ __forceinline__ float4 seg_terms(const ModulesView& M, unsigned m) { return ld_global(M.modTerms + 2 * m); }
 // {mdBase, mdTilt2, mdRoot, -} of module m

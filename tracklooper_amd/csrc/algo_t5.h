@@ -43,7 +43,9 @@ struct SegAlpha {
   float dPhiChange, dPhiChangeMin, dPhiChangeMax, dPhiMin, dPhiMax;
 };
 __device__ __forceinline__ SegAlpha load_alpha(const SegsSoA& S, unsigned s) {
-  return SegAlpha{S.dPhiChanges[s], S.dPhiChangeMins[s], S.dPhiChangeMaxs[s], S.dPhiMins[s], S.dPhiMaxs[s]};
+  const uint4 c1 = seg_chunk(S, s, 1), c2 = seg_chunk(S, s, 2);
+  return SegAlpha{__uint_as_float(c2.y), __uint_as_float(c2.z), __uint_as_float(c2.w), __uint_as_float(c1.w),
+                  __uint_as_float(c2.x)};
 }
 
 // beta refinement by the pT estimate: SDL/Quintuplet.h:1370-1464

@@ -144,6 +144,14 @@ constexpr uint32_t kSegRecBytes = 4 * kSegRecWords;
 static_assert(kSegRecBytes % 16 == 0, "segment records stay 16-byte aligned");
 #define LST_DECL_SEG_FIELD(T, N, M) RecField<T, M, kSegRecBytes> N;
 struct SegsSoA { LST_SEG_FIELDS(LST_DECL_SEG_FIELD) };
+static_assert(seg_word(kSegF_innerLowerModuleIndices) == 2 && seg_word(kSegF_outerLowerModuleIndices) == 3 &&
+                  seg_word(kSegF_dPhiMins) == 7 && seg_word(kSegF_dPhiMaxs) == 8 && seg_word(kSegF_dPhiChanges) == 9,
+              "the chunks seg_chunk readers expect");
+// 16-byte chunk q of segment record s: 0 {inner MD, outer MD, inner module, outer module}, 1 {anchor hits,
+// dPhi, dPhiMin}, 2 {dPhiMax, dPhiChange, dPhiChangeMin, dPhiChangeMax}; one global load
+__device__ __forceinline__ uint4 seg_chunk(const SegsSoA& S, uint32_t s, uint32_t q) {
+  return ld_global(reinterpret_cast<const uint4*>(S.mdIndices.p + (size_t)s * kSegRecBytes) + q);
+}
 struct PLSSoA { LST_PLS_FIELDS(LST_DECL_FIELD) };
 // Triplets as records as well: one 4-byte word per element (u16 and u8
 // fields in the low bytes of their words), 21 words padded to 24 (96 bytes).
@@ -160,6 +168,12 @@ constexpr uint32_t kT3ObjWords = (t3_word(kT3FieldCount) + 3) & ~3u;
 constexpr uint32_t kT3ObjBytes = 4 * kT3ObjWords;
 #define LST_DECL_T3_FIELD(T, N, M) RecField<T, M, kT3ObjBytes> N;
 struct T3SoA { LST_T3_FIELDS(LST_DECL_T3_FIELD) };
+static_assert(t3_word(kT3F_lowerModuleIndices) == 2, "the chunks t3_chunk readers expect");
+// 16-byte chunk q of triplet record t: 0 {inner segment, outer segment, module 0, module 1}, 1 {module 2,
+// logical layers}, ...; one global load
+__device__ __forceinline__ uint4 t3_chunk(const T3SoA& T, uint32_t t, uint32_t q) {
+  return ld_global(reinterpret_cast<const uint4*>(T.segmentIndices.p + (size_t)t * kT3ObjBytes) + q);
+}
 struct T5SoA { LST_T5_FIELDS(LST_DECL_FIELD) };
 struct PT5SoA { LST_PT5_FIELDS(LST_DECL_FIELD) };
 struct PT3SoA { LST_PT3_FIELDS(LST_DECL_FIELD) };

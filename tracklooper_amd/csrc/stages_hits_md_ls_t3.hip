@@ -756,11 +756,10 @@ __global__ __launch_bounds__(1024) void k_triplet_offsets(LST_VIEWS) {
 // (inSeg on m1, outSeg), including the shared middle mini-doublet check
 __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint32_t inSeg, uint32_t outSeg,
                                           float& betaIn, float& radius, float& gx, float& gy) {
-  const uint32_t md1 = v.S.mdIndices[2 * inSeg];
-  const uint32_t md2 = v.S.mdIndices[2 * outSeg];
-  const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
-  if (v.S.mdIndices[2 * inSeg + 1] != md2) return false;  // SDL/Triplet.h:823-825
-  const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
+  const uint4 si = seg_chunk(v.S, inSeg, 0), so = seg_chunk(v.S, outSeg, 0);
+  const uint32_t md1 = si.x, md2 = so.x, md3 = so.y;
+  if (si.y != md2) return false;  // SDL/Triplet.h:823-825
+  const uint16_t m2 = (uint16_t)si.w, m3 = (uint16_t)so.w;
   T3Geo p;  // the anchor points: chunk 1 of each record, and the first one's phi from its chunk 2
   const uint4 a1 = md_chunk(v.D, md1, 1), a2 = md_chunk(v.D, md2, 1), a3 = md_chunk(v.D, md3, 1);
   p.x1 = __uint_as_float(a1.x); p.y1 = __uint_as_float(a1.y); p.z1 = __uint_as_float(a1.z); p.r1 = __uint_as_float(a1.w);
@@ -768,15 +767,19 @@ __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint3
   p.x2 = __uint_as_float(a2.x); p.y2 = __uint_as_float(a2.y); p.z2 = __uint_as_float(a2.z); p.r2 = __uint_as_float(a2.w);
   p.x3 = __uint_as_float(a3.x); p.y3 = __uint_as_float(a3.y); p.z3 = __uint_as_float(a3.z); p.r3 = __uint_as_float(a3.w);
   p.slope3 = v.WL.mdSlope[md3];
-  if (!t3_rz(v.M.sdlLayers[m1], v.M.sdlLayers[m2], v.M.sdlLayers[m3], p)) return false;
-  const int16_t s1 = v.M.subdets[m1], s2 = v.M.subdets[m2], s3 = v.M.subdets[m3];
-  const bool psIn = v.M.moduleType[m1] == kPS, psOut = v.M.moduleType[m3] == kPS;
-  const float alphaIn = v.S.dPhiChanges[inSeg];
+  const uint32_t w1 = mod_word(v.M, m1), w2 = mod_word(v.M, m2), w3 = mod_word(v.M, m3);
+  const uint4 sa = seg_chunk(v.S, inSeg, 2);  // dPhiMax, dPhiChange, dPhiChangeMin, dPhiChangeMax
+  ModCtx c1, c2, c3;
+  unpack_mod(w1, c1); unpack_mod(w2, c2); unpack_mod(w3, c3);
+  if (!t3_rz(unpack_sdl_layer(w1), unpack_sdl_layer(w2), unpack_sdl_layer(w3), p)) return false;
+  const int16_t s1 = c1.subdet, s2 = c2.subdet, s3 = c3.subdet;
+  const bool psIn = c1.type == kPS, psOut = c3.type == kPS;
+  const float alphaIn = __uint_as_float(sa.y);
   bool pass = false;
   if (s1 == kBarrel && s2 == kBarrel && s3 == kBarrel) pass = t3_bbb(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
   else if (s1 == kBarrel && s3 == kEndcap) pass = t3_bbe(psIn, psOut, v.ptCut, p, alphaIn, betaIn);
   else if (s1 == kEndcap && s2 == kEndcap && s3 == kEndcap)
-    pass = t3_eee(psIn, psOut, v.ptCut, p, alphaIn, v.S.dPhiChangeMins[inSeg], v.S.dPhiChangeMaxs[inSeg], betaIn);
+    pass = t3_eee(psIn, psOut, v.ptCut, p, alphaIn, __uint_as_float(sa.z), __uint_as_float(sa.w), betaIn);
   if (pass) radius = circle_radius3(p.x1, p.y1, p.x2, p.y2, p.x3, p.y3, gx, gy);
   return pass;
 }

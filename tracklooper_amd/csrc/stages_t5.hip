@@ -113,21 +113,25 @@ struct T5Cand {  // the five modules and mini-doublets of a candidate
   uint32_t seg1, seg3, seg4;
 };
 __device__ __forceinline__ T5Cand t5_cand(const EventView& v, uint32_t innerT3, uint32_t outerT3) {
-  T5Cand c;
-  c.seg1 = v.T.segmentIndices[2 * innerT3];
-  const uint32_t seg2 = v.T.segmentIndices[2 * innerT3 + 1];
-  c.seg3 = v.T.segmentIndices[2 * outerT3];
-  c.seg4 = v.T.segmentIndices[2 * outerT3 + 1];
-  c.mods[0] = v.T.lowerModuleIndices[3 * innerT3];
-  c.mods[1] = v.T.lowerModuleIndices[3 * innerT3 + 1];
-  c.mods[2] = v.T.lowerModuleIndices[3 * innerT3 + 2];
-  c.mods[3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
-  c.mods[4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
-  c.md[0] = v.S.mdIndices[2 * c.seg1];
-  c.md[1] = v.S.mdIndices[2 * seg2];
-  c.md[2] = v.S.mdIndices[2 * seg2 + 1];
-  c.md[3] = v.S.mdIndices[2 * c.seg3 + 1];
-  c.md[4] = v.S.mdIndices[2 * c.seg4 + 1];
+  T5Cand c;  // the first two chunks of both triplet records, then the first chunk of the four segments
+  const uint4 i0 = t3_chunk(v.T, innerT3, 0), i1 = t3_chunk(v.T, innerT3, 1);
+  const uint4 o0 = t3_chunk(v.T, outerT3, 0), o1 = t3_chunk(v.T, outerT3, 1);
+  c.seg1 = i0.x;
+  const uint32_t seg2 = i0.y;
+  c.seg3 = o0.x;
+  c.seg4 = o0.y;
+  c.mods[0] = (uint16_t)i0.z;
+  c.mods[1] = (uint16_t)i0.w;
+  c.mods[2] = (uint16_t)i1.x;
+  c.mods[3] = (uint16_t)o0.w;
+  c.mods[4] = (uint16_t)o1.x;
+  const uint4 s1 = seg_chunk(v.S, c.seg1, 0), s2 = seg_chunk(v.S, seg2, 0);
+  const uint4 s3 = seg_chunk(v.S, c.seg3, 0), s4 = seg_chunk(v.S, c.seg4, 0);
+  c.md[0] = s1.x;
+  c.md[1] = s2.x;
+  c.md[2] = s2.y;
+  c.md[3] = s3.y;
+  c.md[4] = s4.y;
   return c;
 }
 
