@@ -299,15 +299,15 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.WL.connPre, (size_t)nL * (g.view.stride + 1)); c.take(e.WL.candOff, nL + 1);
   c.take(e.WL.park, (size_t)lst::kLSParkWords * lst::kLSCandMax);
   c.take(e.WL.cand, lst::kLSCandMax);
-  c.take(e.W3.candPre, segCap); c.take(e.W3.modOff, nL + 1);
+  c.take(e.W3.modOff, nL + 1);
   c.take(e.W3.list, 2 * (size_t)lst::kT3CandMax);
   c.take(e.W3.results, (size_t)lst::kT3ResultWords * lst::kT3CandMax);
-  c.take(e.W3.survPre, segCap); c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.segTmp, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
-  c.take(e.W5.candPre, t3Cap ? t3Cap : 1); c.take(e.W5.modOff, nL + 2);
+  c.take(e.W3.survOff, nL + 1); c.take(e.W3.segByMD, segCap); c.take(e.W3.segTmp, segCap); c.take(e.W3.mdSegOff, mdCap); c.take(e.W3.mdSegEnd, mdCap);
+  c.take(e.W5.modOff, nL + 2);
   c.take(e.W5.resIdx, lst::kT5CandMax);
   c.take(e.W5.listA, 3 * (size_t)lst::kT5CandMax); c.take(e.W5.listB, 3 * (size_t)lst::kT5CandMax);
   c.take(e.W5.results, (size_t)lst::kT5ResultWords * lst::kT5ResultMax);
-  c.take(e.W5.survPre, t3Cap ? t3Cap : 1); c.take(e.W5.survOff, nL + 2); c.take(e.W5.t3ByMD, t3Cap ? t3Cap : 1); c.take(e.W5.t3Tmp, t3Cap ? t3Cap : 1);
+  c.take(e.W5.survOff, nL + 2); c.take(e.W5.t3ByMD, t3Cap ? t3Cap : 1); c.take(e.W5.t3Tmp, t3Cap ? t3Cap : 1);
   c.take(e.W5.mdT3Off, mdCap); c.take(e.W5.mdT3End, mdCap);
   c.take(e.R.hitRanges, 2 * nMod); c.take(e.R.hitRangesLower, nMod); c.take(e.R.hitRangesUpper, nMod);
   c.take(e.R.hitRangesnLower, nMod); c.take(e.R.hitRangesnUpper, nMod);

@@ -220,12 +220,10 @@ struct LSWork {
 constexpr uint32_t kT3CandMax = 1u << 22;
 constexpr int kT3ResultWords = 6;
 struct T3Work {
-  uint32_t* candPre;  // [segment capacity] per inner segment: candidates before it within its module
   uint32_t* modOff;   // [nLower + 1] per lower module: candidate count, then offsets
   uint32_t* list;     // [2 * kT3CandMax] (module, inner/outer segment offsets) of the candidates passing the
                       //   shared-MD check, in serial order
   uint32_t* results;  // [kT3ResultWords * kT3CandMax] selection results of list entry i at i
-  uint32_t* survPre;  // [segment capacity] per inner segment: shared-MD candidates before it within its module
   uint32_t* survOff;  // [nLower + 1] per lower module: shared-MD candidate count, then offsets
   uint32_t* segByMD;  // [segment capacity] each module's segments grouped by inner mini-doublet, in slot order
   uint32_t* segTmp;   // [segment capacity] the same grouping before the ordering
@@ -239,14 +237,12 @@ struct T3Work {
 constexpr uint32_t kT5CandMax = 1u << 21;    // candidates per event handled by the flattened path
 constexpr uint32_t kT5ResultMax = 1u << 19;  // passing candidates per event parked by the flattened path
 struct T5Work {
-  uint32_t* candPre;  // [T3 capacity] per inner T3: candidates before it within its module
   uint32_t* modOff;   // [nLower + 2] per eligible-list position: candidate count, then offsets
   uint32_t* resIdx;   // [kT5CandMax] per list-A entry: its result slot when it passes, ~0u otherwise
   uint32_t* listA;    // [3 * kT5CandMax] (eligible-list position, inner T3, outer T3) of the candidates passing
                       //   the shared-MD check, in serial order
   uint32_t* listB;    // [3 * kT5CandMax] (list-A position, inner T3, outer T3) passing both T4-style tests, any order
   uint32_t* results;  // [kT5ResultWords * kT5ResultMax] selection results of passing candidates
-  uint32_t* survPre;  // [T3 capacity] per inner T3: shared-MD candidates before it within its module
   uint32_t* survOff;  // [nLower + 2] per eligible-list position: shared-MD candidate count, then offsets
   uint32_t* t3ByMD;   // [T3 capacity] each module's T3s grouped by first mini-doublet, in slot order
   uint32_t* t3Tmp;    // [T3 capacity] the same grouping before the ordering

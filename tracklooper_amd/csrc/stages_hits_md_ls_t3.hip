@@ -816,8 +816,8 @@ __device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_
 // ----------------------------------------------------- flattened T3 builder
 // Candidates are numbered in the reference's serial order (module m1, inner
 // segment a, outer segment b on the inner segment's outer module): module m1
-// owns [modOff[m1], modOff[m1+1]), inner segment a owns
-// [candPre[a], candPre[a] + nSegments(m2(a))) within it.
+// owns [modOff[m1], modOff[m1+1]); the shared-MD candidates that the list
+// keeps are [survOff[m1], survOff[m1+1]) in the same order.
 __device__ __forceinline__ bool t3_flat_mode(const EventView& v) {
   return !v.perModuleBuilders && v.R.scalars[kT3CandTotal] <= kT3CandMax;
 }
@@ -834,9 +834,9 @@ __device__ __forceinline__ uint32_t wave_index() {
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / kWave; }
 constexpr unsigned kModuleWaveBlocks = 1024;  // x 4 waves
 
-// per inner segment: its candidates (the outer module's segments, which
-// number them in serial order) and its shared-MD candidates (the segments
-// listed under its outer mini-doublet); per module, both totals
+// per module, the candidates of its inner segments (the outer modules'
+// segments) and their shared-MD candidates (the segments listed under each
+// inner segment's outer mini-doublet)
 __global__ __launch_bounds__(256) void k_t3_cand_count(LST_VIEWS) {
   LST_BIND;
   const unsigned lane = lane_id();
@@ -848,15 +848,11 @@ __global__ __launch_bounds__(256) void k_t3_cand_count(LST_VIEWS) {
       const uint32_t a = a0 + lane;
       uint32_t cnt = 0, sc = 0;
       if (a < nIn) {
-        const uint32_t inSeg = segBase + a, X = v.S.mdIndices[2 * inSeg + 1];
-        cnt = v.R.nSegments[v.S.outerLowerModuleIndices[inSeg]];
-        sc = v.W3.mdSegEnd[X] - v.W3.mdSegOff[X];
+        const uint4 s0 = seg_chunk(v.S, segBase + a, 0);  // outer MD X = s0.y, outer module = s0.w
+        cnt = v.R.nSegments[(uint16_t)s0.w];
+        sc = v.W3.mdSegEnd[s0.y] - v.W3.mdSegOff[s0.y];
       }
       const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
-      if (a < nIn) {
-        v.W3.candPre[segBase + a] = run + incl - cnt;
-        v.W3.survPre[segBase + a] = srun + sincl - sc;
-      }
       run += wave_last(incl);
       srun += wave_last(sincl);
     }
@@ -912,24 +908,51 @@ __global__ __launch_bounds__(256) void k_md_seg_lists(LST_VIEWS) {
   }
 }
 
+// One wave per module m1, its inner segments 64 at a time in the lanes (the
+// segments listed under each one's outer mini-doublet: range and count, and
+// the exclusive prefix of the counts); the lanes then take the chunk's
+// candidates in order, each finding its inner segment by a binary search over
+// the prefix (shuffles), so consecutive lanes write consecutive entries.
 __global__ __launch_bounds__(256) void k_t3_enum(LST_VIEWS) {
   LST_BIND;
   if (!t3_flat_mode(v)) return;
   const unsigned lane = lane_id();
+  uint2* list = reinterpret_cast<uint2*>(v.W3.list);
   for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
     const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
-    const uint32_t segBase = v.segModuleIndices[m1], sOff = v.W3.survOff[m1];
-    for (uint32_t a = lane; a < nIn; a += kWave) {
-      const uint32_t inSeg = segBase + a;
-      const uint32_t X = v.S.mdIndices[2 * inSeg + 1];
-      const uint32_t b = v.W3.mdSegOff[X], e = v.W3.mdSegEnd[X];
-      const uint32_t k0 = sOff + v.W3.survPre[inSeg];
-      const uint32_t seg2Base = v.segModuleIndices[v.S.outerLowerModuleIndices[inSeg]];
-      for (uint32_t q = b; q < e; q++) {
-        const uint32_t j = v.W3.segByMD[q] - seg2Base, k = k0 + (q - b);
-        v.W3.list[2 * k] = m1;
-        v.W3.list[2 * k + 1] = a | (j << 16);
+    if (nIn == 0) continue;
+    const uint32_t segBase = v.segModuleIndices[m1];
+    uint32_t run = v.W3.survOff[m1];  // list position of the chunk's first candidate
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
+      const uint32_t a = a0 + lane;
+      uint32_t cnt = 0, b = 0, seg2Base = 0;
+      if (a < nIn) {
+        const uint4 s0 = seg_chunk(v.S, segBase + a, 0);  // outer MD X = s0.y, outer module = s0.w
+        b = v.W3.mdSegOff[s0.y];
+        cnt = v.W3.mdSegEnd[s0.y] - b;
+        seg2Base = v.segModuleIndices[(uint16_t)s0.w];
       }
+      const uint32_t incl = wave_inclusive_scan(cnt), pre = incl - cnt;
+      const uint32_t chunk = wave_last(incl);
+      const uint32_t nk = nIn - a0 < (uint32_t)kWave ? nIn - a0 : (uint32_t)kWave;
+      for (uint32_t l0 = 0; l0 < chunk; l0 += kWave) {
+        const uint32_t l = l0 + lane;
+        // the inner segment k of candidate l: the largest k < nk with pre_k <= l (every lane takes part)
+        uint32_t k = 0;
+        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+          const uint32_t c = k + step;
+          const uint32_t pc = (uint32_t)__shfl((int)pre, (int)(c < nk ? c : nk - 1), kWave);
+          if (c < nk && pc <= l) k = c;
+        }
+        const uint32_t pk = (uint32_t)__shfl((int)pre, (int)k, kWave);
+        const uint32_t bk = (uint32_t)__shfl((int)b, (int)k, kWave);
+        const uint32_t sk = (uint32_t)__shfl((int)seg2Base, (int)k, kWave);
+        if (l < chunk) {
+          const uint32_t j = v.W3.segByMD[bk + (l - pk)] - sk;
+          list[run + l] = make_uint2(m1, (a0 + k) | (j << 16));
+        }
+      }
+      run += chunk;
     }
   }
 }

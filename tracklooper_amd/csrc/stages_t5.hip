@@ -411,8 +411,9 @@ __device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_
 }
 
 // ------------------------------------------------------ flattened T5 builder
-// Candidates of module b (eligible-list position) are [modOff[b], modOff[b+1]);
-// within a module, inner T3 a owns [candPre[a], candPre[a] + nTriplets(m3(a))).
+// Candidates of module b (eligible-list position) are [modOff[b], modOff[b+1]),
+// inner T3 by inner T3; the shared-MD candidates that list A keeps are
+// [survOff[b], survOff[b+1]) in the same order.
 
 // The per-module kernels of the flattened builder run one wave per eligible
 // module at a time and stride over the modules (a fixed grid of
@@ -446,12 +447,12 @@ __global__ __launch_bounds__(256) void k_md_t3_lists(LST_VIEWS) {
 
 // the outer mini-doublet of a T3, which an outer T3 must start at
 __device__ __forceinline__ uint32_t t3_last_md(const EventView& v, uint32_t t3) {
-  return v.S.mdIndices[2 * v.T.segmentIndices[2 * t3 + 1] + 1];
+  return seg_chunk(v.S, t3_chunk(v.T, t3, 0).y, 0).y;  // the outer segment's outer mini-doublet
 }
 
-// per inner T3: its candidates (the T3s of its third module, which number
-// them in serial order) and its shared-MD candidates (the T3s listed under its
-// last mini-doublet); per eligible module, both totals
+// per eligible module, the candidates of its inner T3s (the T3s of their
+// third modules) and their shared-MD candidates (the T3s listed under each
+// inner T3's last mini-doublet)
 __global__ __launch_bounds__(256) void k_t5_cand_count(LST_VIEWS) {
   LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
@@ -467,14 +468,10 @@ __global__ __launch_bounds__(256) void k_t5_cand_count(LST_VIEWS) {
       uint32_t cnt = 0, sc = 0;
       if (a < nIn) {
         const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
-        cnt = v.R.nTriplets[v.T.lowerModuleIndices[3 * inner + 2]];
+        cnt = v.R.nTriplets[(uint16_t)t3_chunk(v.T, inner, 1).x];  // the inner T3's third module
         sc = v.W5.mdT3End[X] - v.W5.mdT3Off[X];
       }
       const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
-      if (a < nIn) {
-        v.W5.candPre[t3Base + a] = run + incl - cnt;
-        v.W5.survPre[t3Base + a] = srun + sincl - sc;
-      }
       run += wave_last(incl);
       srun += wave_last(sincl);
     }
@@ -532,8 +529,11 @@ __device__ __forceinline__ void list_push(uint32_t* list, uint32_t* counter, uin
 
 // stage 1: the candidates whose outer T3 starts at the inner T3's last
 // mini-doublet, written at the positions the count pass gives (one wave per
-// eligible module).  With each mini-doublet's T3s in slot order, list A is in
-// the reference's serial order (module, inner T3, outer T3).
+// eligible module, its inner T3s 64 at a time in the lanes; the lanes then
+// take the chunk's candidates in order, each finding its inner T3 by a binary
+// search over the prefix of the counts, so consecutive lanes write
+// consecutive entries).  With each mini-doublet's T3s in slot order, list A is
+// in the reference's serial order (module, inner T3, outer T3).
 __global__ __launch_bounds__(256) void k_t5_enum(LST_VIEWS) {
   LST_BIND;
   if (!t5_flat_mode(v)) return;
@@ -543,18 +543,38 @@ __global__ __launch_bounds__(256) void k_t5_enum(LST_VIEWS) {
     const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
     const int layer = v.M.layers[m1];
     const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
+    if (nIn == 0) continue;
     const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-    const uint32_t sOff = v.W5.survOff[e];
-    for (uint32_t a = lane; a < nIn; a += 64) {
-      const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
-      const uint32_t b = v.W5.mdT3Off[X], end = v.W5.mdT3End[X];
-      const uint32_t k0 = sOff + v.W5.survPre[inner];
-      for (uint32_t q = b; q < end; q++) {
-        const uint32_t k = k0 + (q - b);
-        v.W5.listA[3 * k] = e;
-        v.W5.listA[3 * k + 1] = inner;
-        v.W5.listA[3 * k + 2] = v.W5.t3ByMD[q];
+    uint32_t run = v.W5.survOff[e];  // list position of the chunk's first candidate
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
+      const uint32_t a = a0 + lane;
+      uint32_t cnt = 0, b = 0;
+      if (a < nIn) {
+        const uint32_t X = t3_last_md(v, t3Base + a);
+        b = v.W5.mdT3Off[X];
+        cnt = v.W5.mdT3End[X] - b;
       }
+      const uint32_t incl = wave_inclusive_scan(cnt), pre = incl - cnt;
+      const uint32_t chunk = wave_last(incl);
+      const uint32_t nk = nIn - a0 < (uint32_t)kWave ? nIn - a0 : (uint32_t)kWave;
+      for (uint32_t l0 = 0; l0 < chunk; l0 += kWave) {
+        const uint32_t l = l0 + lane;
+        uint32_t k = 0;  // the largest k < nk with pre_k <= l (every lane takes part)
+        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+          const uint32_t c = k + step;
+          const uint32_t pc = (uint32_t)__shfl((int)pre, (int)(c < nk ? c : nk - 1), kWave);
+          if (c < nk && pc <= l) k = c;
+        }
+        const uint32_t pk = (uint32_t)__shfl((int)pre, (int)k, kWave);
+        const uint32_t bk = (uint32_t)__shfl((int)b, (int)k, kWave);
+        if (l < chunk) {
+          const uint32_t kk = run + l;
+          v.W5.listA[3 * kk] = e;
+          v.W5.listA[3 * kk + 1] = t3Base + a0 + k;
+          v.W5.listA[3 * kk + 2] = v.W5.t3ByMD[bk + (l - pk)];
+        }
+      }
+      run += chunk;
     }
   }
 }
