@@ -379,10 +379,10 @@ __device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, 
   const uint32_t md1 = v.S.mdIndices[2 * segIn], md2 = v.S.mdIndices[2 * segIn + 1];
   const uint32_t md3 = v.S.mdIndices[2 * segOut], md4 = v.S.mdIndices[2 * segOut + 1];
   if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md1, md2, lo, mid, alphaPix, v.S.dPhiChanges[segIn], v.ptCut,
-                      v.WL.mdSlope[md1]))
+                      md_slope(v.D, md1)))
     return false;
   return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut,
-                        v.WL.mdSlope[md3]);
+                        md_slope(v.D, md3));
 }
 
 // one of the two pixel-tracklet tests of pt3_tracklets: part 0 the inner
@@ -397,7 +397,7 @@ __device__ __forceinline__ bool pt3_tracklet_half(const EventView& v, uint32_t p
   const float alphaPix = v.S.dPhiChanges[pls];
   const uint32_t mdA = v.S.mdIndices[2 * seg], mdB = v.S.mdIndices[2 * seg + 1];
   return pixel_tracklet(M, v.D, s, pixIn, pixOut, mdA, mdB, mA, mB, alphaPix, v.S.dPhiChanges[seg], v.ptCut,
-                        v.WL.mdSlope[mdA]);
+                        md_slope(v.D, mdA));
 }
 
 // runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024) is the AND of pure

@@ -247,8 +247,10 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
 #define CARVE(TY, NM, MU) c.take(e.H.NM, (size_t)hitCap * MU);
   LST_HIT_FIELDS(CARVE)
 #undef CARVE
-  char* mdRec = nullptr;  // the mini-doublet records (fields set after binding)
-  c.take(mdRec, (size_t)mdCap * lst::kMDRecBytes);
+  char* mdHot = nullptr;  // the mini-doublet records, hot and cold parts (fields set after binding)
+  char* mdCold = nullptr;
+  c.take(mdHot, (size_t)mdCap * lst::kMDHotBytes);
+  c.take(mdCold, (size_t)mdCap * lst::kMDColdBytes);
   char* segRec = nullptr;  // the segment records (fields set after binding)
   c.take(segRec, (size_t)segCap * lst::kSegRecBytes);
 #define CARVE(TY, NM, MU) c.take(e.P.NM, (size_t)plsCap * MU);
@@ -328,11 +330,13 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.bind(arena);
   {
     uint32_t k = 0;
-#define SLOT(TY, NM, MU) e.D.NM.p = mdRec + 4 * k++;
+#define SLOT(TY, NM, MU) \
+  e.D.NM.p = (k < lst::kMDHotFields ? mdHot : mdCold) + 4 * lst::md_slot(k); \
+  k++;
     LST_MD_FIELDS(SLOT)
 #undef SLOT
     if (k != lst::kMDFields) return fail(LST_E_DEVICE, "mini-doublet record layout");
-    e.WL.mdSlope.p = mdRec + 4 * lst::kMDSlopeWord;
+    e.WL.mdSlope.p = mdHot + 4 * lst::kMDSlopeWord;
     uint32_t w = 0;
 #define SLOT(TY, NM, MU) e.S.NM.p = segRec + 4 * w; w += MU;
     LST_SEG_FIELDS(SLOT)
@@ -394,7 +398,7 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
     fields.push_back({name, FieldRef{(void*)f.p, dtype_of<T>(), sizeof(T), n, stride, mult}});
   };
   {
-#define MDFIELD(TY, NM, MU) addRec("mds." #NM, e.D.NM, mdCap, lst::kMDRecBytes, 1);
+#define MDFIELD(TY, NM, MU) addRec("mds." #NM, e.D.NM, mdCap, lst::md_rec_bytes(lst::kMDw_##NM), 1);
     LST_MD_FIELDS(MDFIELD)
 #undef MDFIELD
   }

@@ -83,37 +83,48 @@
 
 namespace lst {
 struct HitsSoA { LST_HIT_FIELDS(LST_DECL_FIELD) };
-// Mini-doublets are stored as records: the fields of one mini-doublet side by
-// side in 4-byte slots (field k at byte 4k, the per-MD slope pair after the
-// last field), kMDRecBytes per mini-doublet.  Storing or gathering one
-// mini-doublet then touches two cache lines and one page instead of one line
-// and one page per field; D.<field>[i] addresses field <field> of record i.
+// Mini-doublets are stored as records in two arrays: the hot record (80 bytes,
+// five 16-byte chunks) holds what the later selections read, the cold record
+// (64 bytes) the rest.  Hot words: fields 0-15 (words 0-15), the slope pair
+// (words 16-17), fields 16-17 (words 18-19); cold words: fields 18-33.  A
+// gather of a mini-doublet then reads whole chunks of one short record
+// instead of single words of a long one; D.<field>[i] addresses field
+// <field> of record i in its array.
 constexpr uint32_t kMDFields = 34;
-constexpr uint32_t kMDSlopeWord = kMDFields;  // float2 slope pair (words 34, 35)
-constexpr uint32_t kMDRecBytes = 4 * (kMDSlopeWord + 2);
-static_assert(kMDRecBytes % 16 == 0, "records stay 16-byte aligned");
-template <class T>
+constexpr uint32_t kMDHotFields = 18;                 // fields 0 .. 17 are in the hot record
+constexpr uint32_t kMDSlopeWord = 16;                 // float2 slope pair, hot words 16, 17
+constexpr uint32_t kMDHotBytes = 80, kMDColdBytes = 64;
+static_assert(4 * (kMDHotFields + 2) == kMDHotBytes && 4 * (kMDFields - kMDHotFields) == kMDColdBytes,
+              "every hot and cold word holds a field (or the slope pair)");
+// word of field k in its record, and its record's size
+constexpr uint32_t md_slot(uint32_t k) { return k < 16 ? k : k < kMDHotFields ? k + 2 : k - kMDHotFields; }
+constexpr uint32_t md_rec_bytes(uint32_t k) { return k < kMDHotFields ? kMDHotBytes : kMDColdBytes; }
+template <class T, uint32_t REC>
 struct MDField {
   char* p;  // this field's slot in record 0
   __host__ __device__ __forceinline__ T& operator[](size_t i) const {
-    return *reinterpret_cast<T*>(p + i * kMDRecBytes);
+    return *reinterpret_cast<T*>(p + i * REC);
   }
 };
-#define LST_DECL_MD_FIELD(T, N, M) MDField<T> N;
-struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
-// word of each field in the record (field order of LST_MD_FIELDS)
+// field number of each field (order of LST_MD_FIELDS)
 #define LST_MD_WORD(T, N, M) kMDw_##N,
 enum MDWord : uint32_t { LST_MD_FIELDS(LST_MD_WORD) kMDWordCount };
 static_assert(kMDWordCount == kMDFields, "one word per mini-doublet field");
+#define LST_DECL_MD_FIELD(T, N, M) MDField<T, md_rec_bytes(kMDw_##N)> N;
+struct MDsSoA { LST_MD_FIELDS(LST_DECL_MD_FIELD) };
 static_assert(kMDw_anchorHitIndices == 0 && kMDw_moduleIndices == 1 && kMDw_dphichanges == 2 && kMDw_dzs == 3 &&
                   kMDw_anchorX == 4 && kMDw_anchorY == 5 && kMDw_anchorZ == 6 && kMDw_anchorRt == 7 &&
                   kMDw_dphis == 8 && kMDw_anchorPhi == 9 && kMDw_anchorHighEdgePhi == 10 &&
                   kMDw_anchorLowEdgePhi == 11 && kMDw_anchorHighEdgeX == 12 && kMDw_anchorHighEdgeY == 13 &&
-                  kMDw_anchorLowEdgeX == 14 && kMDw_anchorLowEdgeY == 15,
-              "the chunks md_chunk readers expect");
-// 16-byte chunk q (words 4q .. 4q + 3) of mini-doublet record i, one global load
+                  kMDw_anchorLowEdgeX == 14 && kMDw_anchorLowEdgeY == 15 && kMDw_shiftedXs == kMDHotFields,
+              "the chunks md_chunk readers expect, and the first cold field");
+// the slope pair of mini-doublet record i (hot words 16, 17), one global load
+__device__ __forceinline__ float2 md_slope(const MDsSoA& D, uint32_t i) {
+  return ld_global(reinterpret_cast<const float2*>(D.anchorHitIndices.p + (size_t)i * kMDHotBytes + 4 * kMDSlopeWord));
+}
+// 16-byte chunk q (hot words 4q .. 4q + 3) of mini-doublet record i, one global load
 __device__ __forceinline__ uint4 md_chunk(const MDsSoA& D, uint32_t i, uint32_t q) {
-  return ld_global(reinterpret_cast<const uint4*>(D.anchorHitIndices.p + (size_t)i * kMDRecBytes) + q);
+  return ld_global(reinterpret_cast<const uint4*>(D.anchorHitIndices.p + (size_t)i * kMDHotBytes) + q);
 }
 // Segments are records too: 12 four-byte words per segment (the two MD
 // indices, then one word per other field, in LST_SEG_FIELDS order; the u16
@@ -213,7 +224,7 @@ struct LSWork {
   uint32_t* candOff;  // [nLower + 1]
   uint32_t* park;     // [kLSParkWords * kLSCandMax] SegOut of passing candidate L at L
   uint2* cand;        // [kLSCandMax] candidate L: (inner MD, outer MD); the modules are in the MD records
-  MDField<float2> mdSlope;  // in the MD record: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)) and its tan
+  MDField<float2, kMDHotBytes> mdSlope;  // in the hot MD record: asin(min(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax)), its tan
 };
 
 // Work arrays of the flattened T3 builder (same scheme as the T5 one below).

@@ -63,7 +63,7 @@ __global__ void k_event_init(LST_VIEWS) {
 }
 
 // SDL/Hit.h:217-270
-__global__ void k_hits(LST_VIEWS) {
+__global__ __launch_bounds__(256) void k_hits(LST_VIEWS) {
   LST_BIND;
   const uint32_t nHits = v.R.scalars[kEvNHits];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nHits; i += gridDim.x * blockDim.x) {
@@ -164,13 +164,14 @@ __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, 
   const HitsSoA& H = v.H;
   const uint32_t a = anchorIsUpper ? upperHit : lowerHit;
   const uint32_t b = anchorIsUpper ? lowerHit : upperHit;
-  // the record is built in registers and written with 16-byte stores: one lane writes its record's
-  // 144 contiguous bytes in nine stores instead of 36 scattered 4-byte ones
-  uint32_t w[kMDRecBytes / 4];
-  const auto f = [&](MDWord k, float x) { w[k] = __float_as_uint(x); };
-  w[kMDw_anchorHitIndices] = a;
-  w[kMDw_outerHitIndices] = b;
-  w[kMDw_moduleIndices] = mod;  // the u16 field in the low half of its slot
+  // the hot and cold records are built in registers and written with 16-byte stores: one lane writes
+  // its 80 + 64 contiguous bytes in nine stores instead of 36 scattered 4-byte ones
+  uint32_t hot[kMDHotBytes / 4], cold[kMDColdBytes / 4];
+  const auto u = [&](MDWord k, uint32_t x) { (k < kMDHotFields ? hot : cold)[md_slot(k)] = x; };
+  const auto f = [&](MDWord k, float x) { u(k, __float_as_uint(x)); };
+  u(kMDw_anchorHitIndices, a);
+  u(kMDw_outerHitIndices, b);
+  u(kMDw_moduleIndices, mod);  // the u16 field in the low half of its slot
   f(kMDw_dphichanges, o.dphichange); f(kMDw_dphis, o.dphi); f(kMDw_dzs, o.dz);
   f(kMDw_shiftedXs, o.sx); f(kMDw_shiftedYs, o.sy); f(kMDw_shiftedZs, o.sz);
   f(kMDw_noShiftedDzs, o.nsdz); f(kMDw_noShiftedDphis, o.nsdphi); f(kMDw_noShiftedDphiChanges, o.nsdphichange);
@@ -186,12 +187,16 @@ __device__ __forceinline__ void store_md(const EventView& v, uint32_t lowerHit, 
   f(kMDw_outerLowEdgeX, H.lowEdgeXs[b]); f(kMDw_outerLowEdgeY, H.lowEdgeYs[b]);
   {  // the segment slope of this MD as the outer one: the anchor hit's slope (same formula) and its tan
     const float sl = v.W2.hitSlope[a];
-    w[kMDSlopeWord] = __float_as_uint(sl);
-    w[kMDSlopeWord + 1] = __float_as_uint(m_tan(sl));
+    hot[kMDSlopeWord] = __float_as_uint(sl);
+    hot[kMDSlopeWord + 1] = __float_as_uint(m_tan(sl));
   }
-  uint4* rec = reinterpret_cast<uint4*>(v.D.anchorHitIndices.p + (size_t)idx * kMDRecBytes);
+  uint4* h = reinterpret_cast<uint4*>(v.D.anchorHitIndices.p + (size_t)idx * kMDHotBytes);
+  uint4* c = reinterpret_cast<uint4*>(v.D.shiftedXs.p + (size_t)idx * kMDColdBytes);
 #pragma unroll
-  for (uint32_t q = 0; q < kMDRecBytes / 16; q++) rec[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+  for (uint32_t q = 0; q < kMDHotBytes / 16; q++) h[q] = make_uint4(hot[4 * q], hot[4 * q + 1], hot[4 * q + 2], hot[4 * q + 3]);
+#pragma unroll
+  for (uint32_t q = 0; q < kMDColdBytes / 16; q++)
+    c[q] = make_uint4(cold[4 * q], cold[4 * q + 1], cold[4 * q + 2], cold[4 * q + 3]);
 }
 
 // One segment record built in registers and written with three 16-byte stores.
@@ -218,7 +223,7 @@ __device__ __forceinline__ void store_seg(const EventView& v, uint32_t s, uint32
 
 // ------------------------------------------------------------- pixel segments
 // SDL/Segment.h:1031-1117 and addPixelSegmentToMemory SDL/Segment.h:392-452
-__global__ void k_pixel_segments(LST_VIEWS) {
+__global__ __launch_bounds__(256) void k_pixel_segments(LST_VIEWS) {
   LST_BIND;
   const uint32_t pix = v.pixelModule;
   const uint32_t nPLS = v.R.scalars[kEvNPLS];
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(256, 4) void k_ls_eval(LST_VIEWS) {
       const uint2 c = v.WL.cand[L];
       const uint32_t inMD = c.x, outMD = c.y;
       const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
-      const float2 bs = v.WL.mdSlope[outMD];
+      const float2 bs = md_slope(v.D, outMD);
       const uint32_t m = a.mod, om = b.mod;
       const ModCtx ci = load_mod(v.M, m), co = load_mod(v.M, om);
       const float4 ti = seg_terms(v.M, m), to = seg_terms(v.M, om);
@@ -710,7 +715,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_segments(LST_VIEWS) {
         const ModCtx co = load_mod(v.M, om);
         const float4 to = seg_terms(v.M, om);
         const MDRef a = load_md(v.D, inMD), b = load_md(v.D, outMD);
-        const float2 bs = v.WL.mdSlope[outMD];
+        const float2 bs = md_slope(v.D, outMD);
         pass = (ci.subdet == kBarrel && co.subdet == kBarrel) ? seg_barrel(ci, ti, co, to, v.ptCut, a, b, bs, o)
                                                               : seg_endcap(ci, ti, co, to, v.ptCut, a, b, bs, o);
       }
@@ -766,7 +771,7 @@ __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint3
   p.phi1 = v.D.anchorPhi[md1];
   p.x2 = __uint_as_float(a2.x); p.y2 = __uint_as_float(a2.y); p.z2 = __uint_as_float(a2.z); p.r2 = __uint_as_float(a2.w);
   p.x3 = __uint_as_float(a3.x); p.y3 = __uint_as_float(a3.y); p.z3 = __uint_as_float(a3.z); p.r3 = __uint_as_float(a3.w);
-  p.slope3 = v.WL.mdSlope[md3];
+  p.slope3 = md_slope(v.D, md3);
   const uint32_t w1 = mod_word(v.M, m1), w2 = mod_word(v.M, m2), w3 = mod_word(v.M, m3);
   const uint4 sa = seg_chunk(v.S, inSeg, 2);  // dPhiMax, dPhiChange, dPhiChangeMin, dPhiChangeMax
   ModCtx c1, c2, c3;

@@ -145,10 +145,10 @@ __device__ __forceinline__ bool t5_t4_tests(const EventView& v, const T5Cand& c)
   }
   const SegAlpha a1 = load_alpha(v.S, c.seg1);
   {
-    T4Args q{m[0], m[1], m[2], m[3], p[0], p[1], p[2], p[3], a1, load_alpha(v.S, c.seg3), v.WL.mdSlope[c.md[2]]};
+    T4Args q{m[0], m[1], m[2], m[3], p[0], p[1], p[2], p[3], a1, load_alpha(v.S, c.seg3), md_slope(v.D, c.md[2])};
     if (!t4_select(q, v.ptCut)) return false;
   }
-  T4Args q{m[0], m[1], m[3], m[4], p[0], p[1], p[3], p[4], a1, load_alpha(v.S, c.seg4), v.WL.mdSlope[c.md[3]]};
+  T4Args q{m[0], m[1], m[3], m[4], p[0], p[1], p[3], p[4], a1, load_alpha(v.S, c.seg4), md_slope(v.D, c.md[3])};
   return t4_select(q, v.ptCut);
 }
 
@@ -212,7 +212,7 @@ __device__ __forceinline__ bool t5_t4_half(const EventView& v, const T5Cand& c, 
   const uint32_t pc = part ? c.md[3] : c.md[2], pd = part ? c.md[4] : c.md[3];
   const T4Args q{load_lite(v.M, c.mods[0]), load_lite(v.M, c.mods[1]), load_lite(v.M, mc), load_lite(v.M, md),
                  load_point(v.D, c.md[0]),  load_point(v.D, c.md[1]),  load_point(v.D, pc),  load_point(v.D, pd),
-                 load_alpha(v.S, c.seg1),   load_alpha(v.S, part ? c.seg4 : c.seg3), v.WL.mdSlope[pc]};
+                 load_alpha(v.S, c.seg1),   load_alpha(v.S, part ? c.seg4 : c.seg3), md_slope(v.D, pc)};
   return t4_select(q, v.ptCut);
 }
 
