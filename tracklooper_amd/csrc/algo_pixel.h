@@ -354,9 +354,7 @@ __device__ __forceinline__ void make_t3_rec(const EventView& v, uint32_t t3, flo
   o[3] = make_float4(__uint_as_float(packed), 0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ T3Rec load_t3_rec(const float* rec) {
-  const float4 a = *(const float4*)rec, b = *(const float4*)(rec + 4), c = *(const float4*)(rec + 8),
-               d = *(const float4*)(rec + 12);
+__device__ __forceinline__ T3Rec t3_rec_of(float4 a, float4 b, float4 c, float4 d) {
   T3Rec r;
   r.x[0] = a.x; r.x[1] = a.y; r.x[2] = a.z;
   r.y[0] = a.w; r.y[1] = b.x; r.y[2] = b.y;
@@ -364,6 +362,10 @@ __device__ __forceinline__ T3Rec load_t3_rec(const float* rec) {
   r.r = c.y; r.g = c.z; r.f = c.w;
   r.packed = __float_as_uint(d.x);
   return r;
+}
+__device__ __forceinline__ T3Rec load_t3_rec(const float* rec) {
+  const float4* q = reinterpret_cast<const float4*>(rec);
+  return t3_rec_of(ld_global(q), ld_global(q + 1), ld_global(q + 2), ld_global(q + 3));
 }
 
 // the two pixel-tracklet tests of runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:
@@ -644,15 +646,20 @@ __device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, flo
 // inner T3).  `rec` is the T5's kT5RecWords record.
 __device__ __forceinline__ bool pt5_cheap(const EventView& v, const PlsCtx& p, uint32_t t5, const float* rec,
                                           PT5Result& r) {
+  // the last chunk (inner radius, packed flags) first: the radius test ends most candidates, and only the
+  // others load the rest of the record
+  const float4* q4 = reinterpret_cast<const float4*>(rec);
+  const float4 last = ld_global(q4 + kT5RecWords / 4 - 1);
+  const uint32_t packed = __float_as_uint(last.w);
+  // inner T3 (runPixelTripletDefaultAlgo without chi2 cuts): radius compatibility
+  if (!pt3_radius_ok((packed >> 5) & 3, p.ptIn * kR1GeVf, p.ptErr * kR1GeVf, last.z)) return false;
   float w[kT5RecWords];
 #pragma unroll
-  for (int k = 0; k < kT5RecWords; k += 4) {
-    const float4 q = *(const float4*)(rec + k);
+  for (int k = 0; k < kT5RecWords - 4; k += 4) {
+    const float4 q = ld_global(q4 + k / 4);
     w[k] = q.x; w[k + 1] = q.y; w[k + 2] = q.z; w[k + 3] = q.w;
   }
-  const uint32_t packed = __float_as_uint(w[31]);
-  // inner T3 (runPixelTripletDefaultAlgo without chi2 cuts): radius compatibility
-  if (!pt3_radius_ok((packed >> 5) & 3, p.ptIn * kR1GeVf, p.ptErr * kR1GeVf, w[30])) return false;
+  w[kT5RecWords - 4] = last.x; w[kT5RecWords - 3] = last.y; w[kT5RecWords - 2] = last.z; w[kT5RecWords - 1] = last.w;
   int l[5];
 #pragma unroll
   for (int k = 0; k < 5; k++) l[k] = rec_layer(packed, 7, k);

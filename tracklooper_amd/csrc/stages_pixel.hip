@@ -238,11 +238,16 @@ struct PT3Kind {
   }
   __device__ static uint32_t n_obj(const EventView& v, uint32_t m) { return v.R.nTriplets[m]; }
   __device__ static uint32_t obj_base(const EventView& v, uint32_t m) { return (uint32_t)v.R.tripletModuleIndices[m]; }
+  // the record's last two chunks (radius, packed flags) first: most candidates end at the 2S-middle or the
+  // radius test, and only the others load the points
   __device__ static bool cheap(const EventView& v, const PlsCtx& p, uint32_t t, Res& r) {
-    const T3Rec rec = load_t3_rec(v.W.t3Rec + (size_t)kT3RecWords * t);
-    if ((rec.packed >> 17) & 1) return false;  // middle module 2S
+    const float4* q = reinterpret_cast<const float4*>(v.W.t3Rec + (size_t)kT3RecWords * t);
+    const float4 c = ld_global(q + 2), d = ld_global(q + 3);
+    const uint32_t packed = __float_as_uint(d.x);
+    if ((packed >> 17) & 1) return false;  // middle module 2S
+    if (!pt3_radius_ok(packed & 3, p.ptIn * kR1GeVf, p.ptErr * kR1GeVf, c.y)) return false;  // pt3_cheap's first cut
     if (v.T.partOfPT5[t] != 0) return false;
-    return pt3_cheap(p, rec, true, r);
+    return pt3_cheap(p, t3_rec_of(ld_global(q), ld_global(q + 1), c, d), true, r);
   }
   __device__ static bool costly(const EventView& v, uint32_t pls, uint32_t i, uint32_t t, Res& r) {
     return pt3_costly(v, pls, i, t, true, r);
