@@ -54,10 +54,11 @@ def algorithmic_bytes(n: dict) -> dict:
     Record sizes are the fields each kernel reads/writes:
       k_hits             : read x,y,z,detId (16 B/hit), write rt,phi,eta,module,4 edges (30 B/hit)
       k_pixel_segments   : read 21 floats and 4 hit indices (100 B/pLS), write 2 MDs + 1 segment + pLS (400 B/pLS)
-      k_md_eval          : read x,y,z,rt of the module-pair hits (16 B/hit), write a 4 B result index per
+      k_md_eval          : read x,y,z,rt of the module-pair hits (16 B/hit), write one pass bit per
                            candidate pair and 36 B of parked results per mini-doublet
-      k_ls_eval          : read 44 B of each mini-doublet, write a 4 B result index per candidate and
-                           24 B of parked results per segment
+      k_ls_eval          : read 52 B of each mini-doublet (anchor point and phi, dphi, dz, dphichange, edge
+                           phis, module, slope pair), write one pass bit per candidate and 24 B of parked
+                           results per segment
       k_t3_enum          : per inner segment 20 B (outer mini-doublet, its segment range, prefix, outer module),
                            per shared-MD candidate 16 B
       k_t3_select        : read 22 B/segment and 20 B/mini-doublet, write 60 B/triplet
@@ -87,8 +88,8 @@ def algorithmic_bytes(n: dict) -> dict:
     return {
         "k_hits": 46 * h,
         "k_pixel_segments": 500 * pls,
-        "k_md_eval": 16 * h + 4 * n["mdcand"] + 36 * md,
-        "k_ls_eval": 44 * md + 4 * n["lscand"] + 24 * ls,
+        "k_md_eval": 16 * h + n["mdcand"] / 8 + 36 * md,
+        "k_ls_eval": 52 * md + n["lscand"] / 8 + 24 * ls,
         "k_t3_enum": 20 * ls + 16 * n["t3surv"],
         "k_t3_select": 22 * ls + 20 * md + 60 * t3,
         "k_t5_final": 53 * t3 + 28 * ls + 52 * md + 119 * t5,
