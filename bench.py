@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--batches", type=int, default=2, help="distinct uneven batches cycled through")
     ap.add_argument("--per-event-steps", type=int, default=2, help="batches the per-event path is timed on")
     ap.add_argument("--capacity", type=int, default=8192, help="track candidates copied back per event")
+    ap.add_argument("--sets-in-flight", type=int, default=2, choices=(1, 2),
+                    help="launch sets the batch keeps in flight (2: each with its own arenas and stream)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline configuration")
@@ -219,6 +221,7 @@ def main():
         tune = {"default_ms": d_ms, "tuned_ms": t_ms}
         print(f"autotune: summed kernel time {d_ms:.2f} ms at the default work-group counts, {t_ms:.2f} ms tuned",
               file=sys.stderr, flush=True)
+    batch.set_sets_in_flight(args.sets_in_flight)  # the tuner timed one set alone
     # output descriptors of a run of K launch sets: event i of the run writes into buffer i % E (the sets are
     # consumed one after the other, so the buffers hold the last set's lists at the end)
     outs1, bufs = lst.Batch._outputs(E, args.capacity)
@@ -345,7 +348,9 @@ def main():
                         f"(mean {pus.mean():.0f}), OT800_IT615_pt0.8, plain chi2 T5 selection; one step = one "
                         f"launch set of {E} events: inputs host-to-device, hits..track candidates, output lists "
                         f"device-to-host; {args.batches} batches cycled; the {args.steps} steps run in one call, "
-                        f"each set's input copies overlapping the previous set's kernels",
+                        f"each set's input copies overlapping the previous set's kernels"
+                        + (", two launch sets in flight (each on its own arenas and stream)"
+                           if args.sets_in_flight == 2 else ""),
             "batched_events_per_s": value,
             "per_event_events_per_s": per_event_rate,
             "per_event_ms_per_event": wall_per * 1e3 / n_per,
@@ -354,6 +359,7 @@ def main():
             "events_compared": n_ev,
             "physics": {k: physics[k] for k in ("sim_tracks", "efficiency", "fake_rate", "duplicate_rate")},
             "slots": E,
+            "sets_in_flight": args.sets_in_flight,
             "autotune_summed_kernel_ms": tune,
             "tuned_grid": {k: v for k, v in tuning.items() if v},
             "device_mib_per_slot": counts[0]["arena_mib"],

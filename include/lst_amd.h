@@ -276,6 +276,14 @@ typedef struct {
 lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch** out);
 void lst_batch_destroy(lst_batch* b);
 uint32_t lst_batch_contexts(const lst_batch* b);
+/* Launch sets kept in flight (1, the default, or 2; no reference counterpart:
+ * the reference runs one event per stream and OpenMP thread,
+ * bin/sdl.cc:389-419).  With 2, each set has its own n_slots arenas, view
+ * array and compute stream, and a launch set runs beside the previous one,
+ * filling the GPU while the other's small kernels and tails drain; the device
+ * memory of the arenas doubles.  Synchronises the batch; the results are the
+ * same either way. */
+lst_status lst_batch_set_sets_in_flight(lst_batch* b, uint32_t sets);
 /* n_ev events from host arrays (prepareInput on the host per event, then
  * launch sets of n_slots events); out: NULL or n_ev output descriptors.
  * Returns the first event error (each event's own status is in out[i].status),

@@ -76,6 +76,26 @@ def test_batch_staged_inputs_match_per_event_twice(es, events, per_event):
     b.close()
 
 
+def test_batch_two_sets_in_flight_match_per_event(es, events, per_event):
+    """Two launch sets in flight (each with its own arenas and stream): 7 staged events in sets of 2 (both
+    sets used three times and more), host inputs, then back to one set in flight: same lists."""
+    import tracklooper_amd as lst
+    b = lst.Batch(es, slots=2)
+    b.set_sets_in_flight(2)
+    b.stage(events)
+    for _ in range(2):
+        res = b.run_staged(0, 3 * len(events))  # staged event i % 7, 21 events in 11 launch sets
+        for k, r in enumerate(res):
+            assert r["status"] == 0
+            _same(r, per_event[k % len(events)])
+    for r, p in zip(b.run(events), per_event):
+        _same(r, p)
+    b.set_sets_in_flight(1)
+    for r, p in zip(b.run(events), per_event):
+        _same(r, p)
+    b.close()
+
+
 def test_batch_without_graphs_matches(es, events, per_event, monkeypatch):
     """LST_BATCH_GRAPHS=0 launches the kernels one by one: same lists."""
     import tracklooper_amd as lst

@@ -39,6 +39,8 @@ def test_error_paths_without_device():
     L = lst.lib()
     assert L.lst_event_create_mini_doublets(None) == 4  # LST_E_ORDER
     assert L.lst_event_get_field(None, b"hits.xs", None, 0, None, None) == 3  # LST_E_INPUT
+    L.lst_batch_set_sets_in_flight.argtypes = [C.c_void_p, C.c_uint32]
+    assert L.lst_batch_set_sets_in_flight(None, 2) == 3  # LST_E_INPUT
     es = C.c_void_p()
     assert L.lst_es_create(b"/nonexistent/geometry", 0.8, 0, C.byref(es)) == 1  # LST_E_IO
     assert b"cannot read" in L.lst_last_error()

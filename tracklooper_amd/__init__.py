@@ -366,6 +366,12 @@ class Batch:
         _check(lib().lst_batch_run_staged(self.h, first, n, outs))
         return self._collect(outs, bufs)
 
+    def set_sets_in_flight(self, sets: int):
+        """lst_batch_set_sets_in_flight: 1 or 2 launch sets in flight (2: twice the arenas)."""
+        L = lib()
+        L.lst_batch_set_sets_in_flight.argtypes = [C.c_void_p, C.c_uint32]
+        _check(L.lst_batch_set_sets_in_flight(self.h, sets))
+
     def autotune(self, first: int = 0, repeats: int = 2):
         """Choose each grid-stride kernel's work-group count on one launch set of staged events;
         returns (summed kernel ms at the defaults, at the chosen counts, {kernel: count})."""

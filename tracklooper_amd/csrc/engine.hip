@@ -1015,35 +1015,53 @@ lst_status lst_event_get_t5dnn_record(lst_event* ev, uint32_t* n, float* feature
 // stage sequence is launched once for all of them (gridDim.z = slots), each
 // event in its own arena, and the sequence is one captured graph.  Inputs are
 // prepared on the host once and kept pinned; each launch set copies its
-// events' blocks to the device on a copy stream while the previous set
-// computes (two input buffers); its last kernel puts each event's packed
-// output block in a device block and the block's header in pinned host
-// memory, and the host copies the records back on an output stream while the
-// next set computes.  An event whose candidates overflow a
+// events' blocks to the device on a transfer stream while the previous set
+// computes; its last kernel puts each event's packed output block in a device
+// block and the block's header in pinned host memory, and the host copies the
+// records back on the transfer stream while the next set computes.  Launch set k
+// uses frame k % 3 (input blocks, descriptors, output blocks) and, with two
+// sets in flight, arena set k % 2 on its own compute stream; the host consumes
+// set k - (sets in flight) after launching set k.  An event whose candidates overflow a
 // flattened builder's work arrays (a flag in its output header) is run again
 // by a per-event handle, which has the per-module builders.
 struct lst_batch {
   lst_es* es = nullptr;
   int t5Mode = 0;
   uint32_t slots = 0;
-  std::vector<lst_event*> ctx;          // one arena per slot
+  uint32_t sets = 1;                    // launch sets in flight (1 or 2)
+  std::vector<lst_event*> ctx;          // one arena per slot, per set in flight (set s: ctx[s * slots ...])
   lst_event* rerun = nullptr;           // per-event handle for events that overflow the flattened builders
-  hipStream_t stream = nullptr, copyStream = nullptr, outStream = nullptr;
-  lst::EventView* viewsDev = nullptr;   // [slots]
-  char* inDev[2] = {nullptr, nullptr};  // device input blocks of a launch set, slots x slotBytes each
+  // per set in flight its compute stream and device view array (with one set, entry 1 is entry 0)
+  hipStream_t stream[2] = {nullptr, nullptr};
+  // input copies and output record copies (one stream: a process has few hardware queues, and a stream
+  // that shares one with a compute stream waits for that stream's graphs)
+  hipStream_t xferStream = nullptr;
+  lst::EventView* viewsDev[2] = {nullptr, nullptr};  // [slots] each
+  static constexpr int kFrames = 3;
+  // per frame: the launch set's device input blocks (slots x slotBytes), its descriptors (pinned), its output
+  // blocks (pinned headers and records, device blocks), and the events of its input copy and of its end
+  char* inDev[kFrames] = {};
   size_t slotBytes = 0;
-  lst::InputDesc* descDev[2] = {nullptr, nullptr};   // [slots]
-  lst::InputDesc* descHost[2] = {nullptr, nullptr};  // pinned [slots]
-  uint32_t* outHost[2] = {nullptr, nullptr};         // pinned, slots x out_words()
-  uint32_t* outDev[2] = {nullptr, nullptr};          // device, slots x out_words()
+  lst::InputDesc* descHost[kFrames] = {};
+  uint32_t* outHost[kFrames] = {};      // slots x out_words()
+  uint32_t* outDev[kFrames] = {};       // slots x out_words()
   uint32_t outCapAlloc = 0;             // records per event outHost and outDev hold
+  hipEvent_t copyDone[kFrames] = {}, computeDone[kFrames] = {}, outDone[kFrames] = {};
+  // per arena set: the descriptors its graph reads (copied in on its stream before each launch) and the graph
+  lst::InputDesc* descDev[2] = {nullptr, nullptr};  // [slots]
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graphs = true;
-  hipEvent_t copyDone[2] = {}, computeDone[2] = {};
   std::vector<PreparedInput> staged;
   uint32_t grid[lst::kNTune] = {};      // chosen work-group counts (0: the defaults)
   size_t out_words(uint32_t cap) const {
     return ((lst::kOutHeaderWords + (size_t)cap * lst::kOutRecWords) + 63) & ~size_t(63);
+  }
+  // the arena set of launch set k
+  uint32_t set_of(uint32_t k) const { return sets == 2 ? k % 2 : 0u; }
+  lst_status sync_compute() const {
+    for (uint32_t s = 0; s < sets; s++)
+      if (hipStreamSynchronize(stream[s]) != hipSuccess) return LST_E_DEVICE;
+    return LST_OK;
   }
 };
 
@@ -1056,9 +1074,10 @@ void drop_graphs(lst_batch* b) {
   b->graph[0] = b->graph[1] = nullptr;
 }
 
-// every slot's arena sized for nHits, and the device view array
+// every slot's arena (of every set in flight) sized for nHits, and the device view arrays
 lst_status batch_arenas(lst_batch* b, uint32_t nHits) {
-  bool changed = !b->viewsDev;
+  bool changed = false;
+  for (uint32_t s = 0; s < b->sets; s++) changed |= !b->viewsDev[s];
   for (lst_event* ev : b->ctx) {
     const char* before = ev->arena;
     lst_status s = ev->allocate(nHits);
@@ -1066,18 +1085,22 @@ lst_status batch_arenas(lst_batch* b, uint32_t nHits) {
     changed |= ev->arena != before;
   }
   if (!changed) return LST_OK;
-  HIP_OK(hipStreamSynchronize(b->stream));
-  if (!b->viewsDev) HIP_OK(hipMalloc(&b->viewsDev, sizeof(lst::EventView) * b->slots));
-  for (uint32_t i = 0; i < b->slots; i++)
-    HIP_OK(hipMemcpy(b->viewsDev + i, &b->ctx[i]->v, sizeof(lst::EventView), hipMemcpyHostToDevice));
+  if (b->sync_compute() != LST_OK) return fail(LST_E_DEVICE, "stream synchronize failed");
+  for (uint32_t s = 0; s < b->sets; s++) {
+    if (!b->viewsDev[s]) HIP_OK(hipMalloc(&b->viewsDev[s], sizeof(lst::EventView) * b->slots));
+    for (uint32_t i = 0; i < b->slots; i++)
+      HIP_OK(hipMemcpy(b->viewsDev[s] + i, &b->ctx[s * b->slots + i]->v, sizeof(lst::EventView),
+                       hipMemcpyHostToDevice));
+  }
+  if (b->sets == 1) b->viewsDev[1] = b->viewsDev[0];
   drop_graphs(b);
   return LST_OK;
 }
 
 // input and output buffers for events of up to `bytes` prepared bytes and `cap` output records
 lst_status batch_io(lst_batch* b, size_t bytes, uint32_t cap) {
-  HIP_OK(hipStreamSynchronize(b->stream));
-  HIP_OK(hipStreamSynchronize(b->copyStream));
+  if (b->sync_compute() != LST_OK) return fail(LST_E_DEVICE, "stream synchronize failed");
+  HIP_OK(hipStreamSynchronize(b->xferStream));
   bytes = (bytes + 255) & ~size_t(255);
   if (bytes > b->slotBytes) {
     for (auto& p : b->inDev)
@@ -1085,13 +1108,15 @@ lst_status batch_io(lst_batch* b, size_t bytes, uint32_t cap) {
     for (auto& p : b->inDev) HIP_OK(hipMalloc(&p, bytes * b->slots));
     b->slotBytes = bytes;
   }
-  if (!b->descDev[0])
-    for (int k = 0; k < 2; k++) {
-      HIP_OK(hipMalloc(&b->descDev[k], sizeof(lst::InputDesc) * b->slots));
+  if (!b->descHost[0])
+    for (int k = 0; k < lst_batch::kFrames; k++) {
       HIP_OK(hipHostMalloc((void**)&b->descHost[k], sizeof(lst::InputDesc) * b->slots, hipHostMallocDefault));
       HIP_OK(hipEventCreateWithFlags(&b->copyDone[k], hipEventDisableTiming));
       HIP_OK(hipEventCreateWithFlags(&b->computeDone[k], hipEventDisableTiming));
+      HIP_OK(hipEventCreateWithFlags(&b->outDone[k], hipEventDisableTiming));
     }
+  for (int k = 0; k < 2; k++)
+    if (!b->descDev[k]) HIP_OK(hipMalloc(&b->descDev[k], sizeof(lst::InputDesc) * b->slots));
   if (!b->outHost[0] || cap > b->outCapAlloc) {
     for (auto& p : b->outHost)
       if (p) HIP_OK(hipHostFree(p));
@@ -1105,14 +1130,14 @@ lst_status batch_io(lst_batch* b, size_t bytes, uint32_t cap) {
   return LST_OK;
 }
 
-// the stage sequence of one launch set reading the descriptors of buffer `buf`
-void enqueue_set(lst_batch* b, int buf, const hipEvent_t* tuneEvents = nullptr, uint32_t* tuneRecorded = nullptr) {
-  lst::Launch L{b->ctx[0]->v, b->viewsDev, b->slots, b->stream, lst::no_timers()};
+// the stage sequence of one launch set on arena set `set`, reading that set's descriptors
+void enqueue_set(lst_batch* b, uint32_t set, const hipEvent_t* tuneEvents = nullptr, uint32_t* tuneRecorded = nullptr) {
+  lst::Launch L{b->ctx[set * b->slots]->v, b->viewsDev[set], b->slots, b->stream[set], lst::no_timers()};
   L.grid = b->grid;
   L.fallbacks = false;  // overflowing events are run again by the per-event handle
   L.tuneEvents = tuneEvents;
   L.tuneRecorded = tuneRecorded;
-  lst::launch_stage_in(L, b->descDev[buf]);
+  lst::launch_stage_in(L, b->descDev[set]);
   lst::launch_hits(L);
   lst::launch_pixel_segments(L);
   lst::launch_mini_doublets(L);
@@ -1123,17 +1148,18 @@ void enqueue_set(lst_batch* b, int buf, const hipEvent_t* tuneEvents = nullptr, 
   lst::launch_pixel_quintuplets(L);
   lst::launch_pixel_triplets(L);
   lst::launch_track_candidates(L);
-  lst::launch_out_copy(L, b->descDev[buf]);
+  lst::launch_out_copy(L, b->descDev[set]);
 }
 
 lst_status capture(lst_batch* b) {
-  for (int buf = 0; buf < 2; buf++) {
-    if (b->graph[buf]) continue;
+  for (uint32_t set = 0; set < b->sets; set++) {
+    if (b->graph[set]) continue;
     hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_set(b, buf);
-    HIP_OK(hipStreamEndCapture(b->stream, &g));
-    const hipError_t e = hipGraphInstantiate(&b->graph[buf], g, nullptr, nullptr, 0);
+    hipStream_t st = b->stream[set];
+    HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_set(b, set);
+    HIP_OK(hipStreamEndCapture(st, &g));
+    const hipError_t e = hipGraphInstantiate(&b->graph[set], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(LST_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
@@ -1157,41 +1183,54 @@ std::vector<uint32_t> largest_first(const lst_batch* b, const uint32_t* ids, siz
   return pos;
 }
 
-// the inputs of events ids[0..n) into buffer buf (copy stream); the host may
-// rewrite descHost[buf] because the set that last used buf has been consumed
-lst_status issue_inputs(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, uint32_t outCap) {
+// the inputs of events ids[0..n) into frame f (transfer stream); the host may
+// rewrite descHost[f] and the copies may overwrite inDev[f] because the set
+// that last used frame f has been consumed
+lst_status issue_inputs(lst_batch* b, int f, const uint32_t* ids, uint32_t n, uint32_t outCap) {
   const size_t ow = b->out_words(outCap);
   for (uint32_t i = 0; i < b->slots; i++) {
-    lst::InputDesc& d = b->descHost[buf][i];
+    lst::InputDesc& d = b->descHost[f][i];
     if (i < n) {
       const PreparedInput& p = b->staged[ids[i]];
-      char* dst = b->inDev[buf] + (size_t)i * b->slotBytes;
+      char* dst = b->inDev[f] + (size_t)i * b->slotBytes;
       d = p.desc(dst);
-      HIP_OK(hipMemcpyAsync(dst, p.host, p.bytes, hipMemcpyHostToDevice, b->copyStream));
+      HIP_OK(hipMemcpyAsync(dst, p.host, p.bytes, hipMemcpyHostToDevice, b->xferStream));
     } else {
       d = lst::InputDesc{};  // an empty event
     }
-    d.outHost = b->outHost[buf] + ow * i;
-    d.outDev = b->outDev[buf] + ow * i;
+    d.outHost = b->outHost[f] + ow * i;
+    d.outDev = b->outDev[f] + ow * i;
     d.outCap = outCap;
   }
-  HIP_OK(hipMemcpyAsync(b->descDev[buf], b->descHost[buf], sizeof(lst::InputDesc) * b->slots,
-                        hipMemcpyHostToDevice, b->copyStream));
-  HIP_OK(hipEventRecord(b->copyDone[buf], b->copyStream));
+  HIP_OK(hipEventRecord(b->copyDone[f], b->xferStream));
   return LST_OK;
 }
 
-lst_status launch_set(lst_batch* b, int buf) {
-  HIP_OK(hipStreamWaitEvent(b->stream, b->copyDone[buf], 0));
+// frame f's descriptors into arena set `set`'s descriptor array, on that set's stream once frame f's
+// inputs are on the device (stream order: after the set's previous launch set, whose graph read them)
+lst_status stage_descriptors(lst_batch* b, uint32_t set, int f) {
+  hipStream_t st = b->stream[set];
+  HIP_OK(hipStreamWaitEvent(st, b->copyDone[f], 0));
+  HIP_OK(hipMemcpyAsync(b->descDev[set], b->descHost[f], sizeof(lst::InputDesc) * b->slots,
+                        hipMemcpyHostToDevice, st));
+  return LST_OK;
+}
+
+// launch set k (frame k % 3) on its arena set's stream; with two sets in flight it runs beside set k - 1
+lst_status launch_set(lst_batch* b, uint32_t k) {
+  const uint32_t set = b->set_of(k);
+  const int f = (int)(k % lst_batch::kFrames);
+  hipStream_t st = b->stream[set];
+  lst_status s = stage_descriptors(b, set, f);
+  if (s != LST_OK) return s;
   if (b->graphs) {
-    lst_status s = capture(b);
-    if (s != LST_OK) return s;
-    HIP_OK(hipGraphLaunch(b->graph[buf], b->stream));
+    if ((s = capture(b)) != LST_OK) return s;
+    HIP_OK(hipGraphLaunch(b->graph[set], st));
   } else {
-    enqueue_set(b, buf);
+    enqueue_set(b, set);
     HIP_OK(hipGetLastError());
   }
-  HIP_OK(hipEventRecord(b->computeDone[buf], b->stream));
+  HIP_OK(hipEventRecord(b->computeDone[f], st));
   return LST_OK;
 }
 
@@ -1241,27 +1280,31 @@ lst_status rerun_event(lst_batch* b, const PreparedInput& p, lst_output* d) {
   return n > d->capacity ? (lst_status)(st | LST_W_TRUNCATED) : st;
 }
 
-// waits for the set in buffer buf and hands its outputs over (slot i's to outs[i], which may be null)
-lst_status consume(lst_batch* b, int buf, const uint32_t* ids, uint32_t n, lst_output* const* outs, uint32_t outCap,
-                   lst_status& agg) {
-  HIP_OK(hipEventSynchronize(b->computeDone[buf]));
+// waits for the set of frame f and queues the copies of the records the callers take, device block to
+// pinned block behind the header that is already there (transfer stream, then the frame's outDone event)
+lst_status collect(lst_batch* b, int f, uint32_t n, lst_output* const* outs, uint32_t outCap) {
+  HIP_OK(hipEventSynchronize(b->computeDone[f]));
   const size_t ow = b->out_words(outCap);
-  // the records the callers take, device block to pinned block behind the header that is already there;
-  // the next launch set computes meanwhile, and the set after it (which rewrites outDev[buf]) is issued
-  // only once this returns
-  bool copied = false;
   for (uint32_t i = 0; i < n; i++) {
-    uint32_t* blk = b->outHost[buf] + ow * i;
+    uint32_t* blk = b->outHost[f] + ow * i;
     if (!outs[i] || blk[lst::kOutBadHits] || blk[lst::kOutOverflow]) continue;
     const size_t rec = (size_t)std::min(blk[lst::kOutNTC], outCap) * lst::kOutRecWords;
     if (!rec) continue;
-    HIP_OK(hipMemcpyAsync(blk + lst::kOutHeaderWords, b->outDev[buf] + ow * i + lst::kOutHeaderWords, 4 * rec,
-                          hipMemcpyDeviceToHost, b->outStream));
-    copied = true;
+    HIP_OK(hipMemcpyAsync(blk + lst::kOutHeaderWords, b->outDev[f] + ow * i + lst::kOutHeaderWords, 4 * rec,
+                          hipMemcpyDeviceToHost, b->xferStream));
   }
-  if (copied) HIP_OK(hipStreamSynchronize(b->outStream));
+  HIP_OK(hipEventRecord(b->outDone[f], b->xferStream));
+  return LST_OK;
+}
+
+// once frame f's record copies are done, hands the set's outputs over (slot i's to outs[i], which may be
+// null); the later launch sets compute meanwhile, and the next one to use frame f is issued only after this
+lst_status unpack(lst_batch* b, int f, const uint32_t* ids, uint32_t n, lst_output* const* outs, uint32_t outCap,
+                  lst_status& agg) {
+  HIP_OK(hipEventSynchronize(b->outDone[f]));
+  const size_t ow = b->out_words(outCap);
   for (uint32_t i = 0; i < n; i++) {
-    const uint32_t* blk = b->outHost[buf] + ow * i;
+    const uint32_t* blk = b->outHost[f] + ow * i;
     lst_output* d = outs[i];
     const PreparedInput& p = b->staged[ids[i]];
     lst_status s;
@@ -1309,29 +1352,33 @@ lst_status run_ids(lst_batch* b, const std::vector<uint32_t>& ids, lst_output* o
   }
   lst_status agg = LST_OK;
   auto set_n = [&](uint32_t k) { return std::min<uint32_t>(S, (uint32_t)ids.size() - k * S); };
+  auto collect_set = [&](uint32_t j) {
+    return collect(b, (int)(j % lst_batch::kFrames), set_n(j), sout.data() + (size_t)j * S, outCap);
+  };
+  auto unpack_set = [&](uint32_t j) {
+    return unpack(b, (int)(j % lst_batch::kFrames), sid.data() + (size_t)j * S, set_n(j), sout.data() + (size_t)j * S,
+                  outCap, agg);
+  };
+  // launch set k (its inputs queued first, so that they cross while the earlier sets compute), then hand
+  // over set k - d (d = sets in flight), whose outputs the host unpacks while the later sets compute; frame
+  // k % 3 was last used by set k - 3, handed over before this iteration
+  const uint32_t d = b->sets;
   for (uint32_t k = 0; k < nSets; k++) {
-    const int buf = (int)(k & 1);
-    if ((s = issue_inputs(b, buf, sid.data() + (size_t)k * S, set_n(k), outCap)) != LST_OK) return s;
-    if ((s = launch_set(b, buf)) != LST_OK) return s;
-    if (k > 0) {
-      const uint32_t j = k - 1;
-      if ((s = consume(b, buf ^ 1, sid.data() + (size_t)j * S, set_n(j), sout.data() + (size_t)j * S, outCap,
-                       agg)) != LST_OK)
-        return s;
-    }
-  }
-  if (nSets > 0) {
-    const uint32_t j = nSets - 1;
-    if ((s = consume(b, (int)(j & 1), sid.data() + (size_t)j * S, set_n(j), sout.data() + (size_t)j * S, outCap,
-                     agg)) != LST_OK)
+    if ((s = issue_inputs(b, (int)(k % lst_batch::kFrames), sid.data() + (size_t)k * S, set_n(k), outCap)) != LST_OK)
       return s;
+    if ((s = launch_set(b, k)) != LST_OK) return s;
+    if (k >= d && ((s = collect_set(k - d)) != LST_OK || (s = unpack_set(k - d)) != LST_OK)) return s;
+  }
+  for (uint32_t j = nSets > d ? nSets - d : 0; j < nSets; j++) {
+    if ((s = collect_set(j)) != LST_OK) return s;
+    if ((s = unpack_set(j)) != LST_OK) return s;
   }
   return agg;
 }
 
 lst_status stage_events(lst_batch* b, const lst_input* events, uint32_t n_ev) {
-  HIP_OK(hipStreamSynchronize(b->stream));
-  HIP_OK(hipStreamSynchronize(b->copyStream));
+  if (b->sync_compute() != LST_OK) return fail(LST_E_DEVICE, "stream synchronize failed");
+  HIP_OK(hipStreamSynchronize(b->xferStream));
   for (auto& p : b->staged) p.release();
   b->staged.assign(n_ev, PreparedInput());
   for (uint32_t i = 0; i < n_ev; i++) {
@@ -1357,12 +1404,12 @@ lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch
   b->es = es;
   b->t5Mode = t5_mode;
   b->slots = n_slots;
-  HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  HIP_OK(hipStreamCreateWithFlags(&b->copyStream, hipStreamNonBlocking));
-  HIP_OK(hipStreamCreateWithFlags(&b->outStream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&b->stream[0], hipStreamNonBlocking));
+  b->stream[1] = b->stream[0];
+  HIP_OK(hipStreamCreateWithFlags(&b->xferStream, hipStreamNonBlocking));
   for (uint32_t c = 0; c < n_slots; c++) {
     lst_event* ev = nullptr;
-    lst_status s = lst_event_create(es, b->stream, t5_mode, &ev);
+    lst_status s = lst_event_create(es, b->stream[0], t5_mode, &ev);
     if (s != LST_OK) {
       lst_batch_destroy(b.release());
       return s;
@@ -1378,30 +1425,69 @@ lst_status lst_batch_create(lst_es* es, uint32_t n_slots, int t5_mode, lst_batch
 
 void lst_batch_destroy(lst_batch* b) {
   if (!b) return;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->copyStream) (void)hipStreamSynchronize(b->copyStream);
-  if (b->outStream) (void)hipStreamSynchronize(b->outStream);
+  (void)b->sync_compute();
+  if (b->xferStream) (void)hipStreamSynchronize(b->xferStream);
   drop_graphs(b);
   for (lst_event* ev : b->ctx) lst_event_destroy(ev);
   if (b->rerun) lst_event_destroy(b->rerun);
   for (auto& p : b->staged) p.release();
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < lst_batch::kFrames; k++) {
     if (b->inDev[k]) (void)hipFree(b->inDev[k]);
-    if (b->descDev[k]) (void)hipFree(b->descDev[k]);
     if (b->descHost[k]) (void)hipHostFree(b->descHost[k]);
     if (b->outHost[k]) (void)hipHostFree(b->outHost[k]);
     if (b->outDev[k]) (void)hipFree(b->outDev[k]);
     if (b->copyDone[k]) (void)hipEventDestroy(b->copyDone[k]);
     if (b->computeDone[k]) (void)hipEventDestroy(b->computeDone[k]);
+    if (b->outDone[k]) (void)hipEventDestroy(b->outDone[k]);
   }
-  if (b->viewsDev) (void)hipFree(b->viewsDev);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
-  if (b->copyStream) (void)hipStreamDestroy(b->copyStream);
-  if (b->outStream) (void)hipStreamDestroy(b->outStream);
+  for (int k = 0; k < 2; k++)
+    if (b->descDev[k]) (void)hipFree(b->descDev[k]);
+  if (b->viewsDev[0]) (void)hipFree(b->viewsDev[0]);
+  if (b->viewsDev[1] && b->viewsDev[1] != b->viewsDev[0]) (void)hipFree(b->viewsDev[1]);
+  if (b->stream[1] && b->stream[1] != b->stream[0]) (void)hipStreamDestroy(b->stream[1]);
+  if (b->stream[0]) (void)hipStreamDestroy(b->stream[0]);
+  if (b->xferStream) (void)hipStreamDestroy(b->xferStream);
   delete b;
 }
 
 uint32_t lst_batch_contexts(const lst_batch* b) { return b ? b->slots : 0; }
+
+lst_status lst_batch_set_sets_in_flight(lst_batch* b, uint32_t sets) {
+  if (!b) return fail(LST_E_INPUT, "null batch");
+  if (sets != 1 && sets != 2) return fail(LST_E_INPUT, "a batch keeps 1 or 2 launch sets in flight");
+  if (sets == b->sets) return LST_OK;
+  if (b->sync_compute() != LST_OK) return fail(LST_E_DEVICE, "stream synchronize failed");
+  HIP_OK(hipStreamSynchronize(b->xferStream));
+  drop_graphs(b);
+  if (sets == 2) {  // a second set of contexts on a stream of its own; arenas come with the next run
+    hipStream_t st = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::vector<lst_event*> more;
+    for (uint32_t c = 0; c < b->slots; c++) {
+      lst_event* ev = nullptr;
+      const lst_status s = lst_event_create(b->es, st, b->t5Mode, &ev);
+      if (s != LST_OK) {
+        for (lst_event* e : more) lst_event_destroy(e);
+        (void)hipStreamDestroy(st);
+        return s;
+      }
+      ev->v.noFallback = 1;
+      more.push_back(ev);
+    }
+    b->ctx.insert(b->ctx.end(), more.begin(), more.end());
+    b->stream[1] = st;
+    b->viewsDev[1] = nullptr;
+  } else {
+    for (size_t c = b->slots; c < b->ctx.size(); c++) lst_event_destroy(b->ctx[c]);
+    b->ctx.resize(b->slots);
+    if (b->viewsDev[1] && b->viewsDev[1] != b->viewsDev[0]) (void)hipFree(b->viewsDev[1]);
+    b->viewsDev[1] = b->viewsDev[0];
+    (void)hipStreamDestroy(b->stream[1]);
+    b->stream[1] = b->stream[0];
+  }
+  b->sets = sets;
+  return LST_OK;
+}
 
 lst_status lst_run_batch(lst_batch* b, const lst_input* events, uint32_t n_ev, lst_output* out) {
   if (!b || (n_ev && !events)) return fail(LST_E_INPUT, "null argument");
@@ -1470,13 +1556,10 @@ lst_status lst_batch_autotune(lst_batch* b, uint32_t first, uint32_t repeats, fl
       b->grid[k] = (k == lst::kTnPT3Pairs || k == lst::kTnCrossPT3) ? std::min(cand[c], 256u) : cand[c];
     for (uint32_t r = 0; r < repeats && st == LST_OK; r++) {
       if ((st = issue_inputs(b, 0, sid.data(), (uint32_t)sid.size(), 0)) != LST_OK) break;
-      if (hipStreamWaitEvent(b->stream, b->copyDone[0], 0) != hipSuccess) {
-        st = fail(LST_E_DEVICE, "hipStreamWaitEvent failed");
-        break;
-      }
+      if ((st = stage_descriptors(b, 0, 0)) != LST_OK) break;
       uint32_t recorded[2] = {0, 0};
       enqueue_set(b, 0, ev.data(), recorded);
-      if (hipStreamSynchronize(b->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+      if (hipStreamSynchronize(b->stream[0]) != hipSuccess || hipGetLastError() != hipSuccess) {
         st = fail(LST_E_DEVICE, "autotune run failed");
         break;
       }
@@ -1534,12 +1617,10 @@ lst_status lst_batch_kernel_times(lst_batch* b, uint32_t first, float* ms, uint3
   std::vector<uint32_t> sid;  // the slot order of a run: largest event first
   for (uint32_t p : largest_first(b, ids.data(), ids.size(), b->slots)) sid.push_back(ids[p]);
   s = issue_inputs(b, 0, sid.data(), (uint32_t)sid.size(), 0);
-  if (s == LST_OK) {
-    if (hipStreamWaitEvent(b->stream, b->copyDone[0], 0) != hipSuccess) s = fail(LST_E_DEVICE, "hipStreamWaitEvent");
-  }
+  if (s == LST_OK) s = stage_descriptors(b, 0, 0);
   if (s == LST_OK) {
     enqueue_set(b, 0, ev.data(), recorded);
-    if (hipStreamSynchronize(b->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+    if (hipStreamSynchronize(b->stream[0]) != hipSuccess || hipGetLastError() != hipSuccess)
       s = fail(LST_E_DEVICE, "timed launch set failed");
   }
   for (uint32_t k = 0; k < n; k++) {
@@ -1559,7 +1640,7 @@ lst_status lst_batch_get_tuning(const lst_batch* b, uint32_t* grid, uint32_t n) 
 
 lst_status lst_batch_set_tuning(lst_batch* b, const uint32_t* grid, uint32_t n) {
   if (!b || !grid) return fail(LST_E_INPUT, "null argument");
-  HIP_OK(hipStreamSynchronize(b->stream));
+  if (b->sync_compute() != LST_OK) return fail(LST_E_DEVICE, "stream synchronize failed");
   for (uint32_t k = 0; k < (uint32_t)lst::kNTune; k++) b->grid[k] = k < n ? grid[k] : 0u;
   drop_graphs(b);
   return LST_OK;

@@ -527,18 +527,21 @@ __global__ __launch_bounds__(256) void k_t5_dup_after_build(LST_VIEWS) {
   const unsigned lane = threadIdx.x % kWave;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n; i += waves) {
-    const uint32_t q = v.W.t5List[i], mod = v.W.t5ListMod[i];
+    const uint32_t mod = v.W.t5ListMod[i];
     const uint32_t yb = v.W.t5ListOff[mod], ye = v.W.t5ListOff[mod + 1];
-    const float ei = v.Q.eta[q], pi = v.Q.phi[q], si = v.Q.score_rphisum[q];
+    const float4 rx = ld_global(v.W.t5ListRec + i);  // eta, phi, score, slot
+    const uint32_t q = __float_as_uint(rx.w);
+    const float ei = rx.x, pi = rx.y, si = rx.z;
     bool dup = false;
     for (uint32_t k0 = yb; k0 < ye && !dup; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool d = false;
       if (k < ye) {
-        const uint32_t y = q + k - i;  // the module's T5s are consecutive slots in list order
+        const float4 ry = ld_global(v.W.t5ListRec + k);
+        const uint32_t y = __float_as_uint(ry.w);  // the module's T5s are consecutive slots in list order
         const bool xLo = q < y;
-        const float sj = v.Q.score_rphisum[y];
-        const float ey = v.Q.eta[y], py = v.Q.phi[y];
+        const float sj = ry.z;
+        const float ey = ry.x, py = ry.y;
         const float dEta = m_abs(xLo ? ei - ey : ey - ei);
         const float dPhi = xLo ? wrap_dphi(pi, py) : wrap_dphi(py, pi);
         // the pair can only flag q if the score order points at it
