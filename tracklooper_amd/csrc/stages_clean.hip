@@ -517,31 +517,42 @@ __global__ __launch_bounds__(256) void k_t5_list_fill(LST_VIEWS) {
 // ------------------------------------------------ T5 duplicates after build
 // removeDupQuintupletsInGPUAfterBuild (SDL/Kernels.h:202-249): within one
 // module, a close pair (i < j, |d eta| and |d phi| <= 0.1) sharing >= 7 hits
-// flags i when score_i >= score_j and j otherwise.  One wave per listed T5 q
-// runs its lanes over the T5s of q's module (contiguous slots, so the loads
-// coalesce) and flags q when any partner points the rule at it; a flag is a
-// store of 1 that does not depend on evaluation order.
+// flags i when score_i >= score_j and j otherwise.  Each 16-lane group of a
+// wave takes one listed T5 q (four per wave: a module holds a few T5s, so a
+// whole wave per T5 left most lanes idle) and runs its lanes over the T5s of
+// q's module, 16 at a time (contiguous slots, so the loads coalesce); it
+// flags q when any partner points the rule at it, with a store of 1 that does
+// not depend on evaluation order.  The list records give eta, phi, score and
+// slot in one 16-byte load.
 __global__ __launch_bounds__(256) void k_t5_dup_after_build(LST_VIEWS) {
   LST_BIND;
+  constexpr unsigned kGroup = 16;
   const uint32_t n = v.R.scalars[kT5Listed];
-  const unsigned lane = threadIdx.x % kWave;
-  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n; i += waves) {
-    const uint32_t mod = v.W.t5ListMod[i];
-    const uint32_t yb = v.W.t5ListOff[mod], ye = v.W.t5ListOff[mod + 1];
-    const float4 rx = ld_global(v.W.t5ListRec + i);  // eta, phi, score, slot
-    const uint32_t q = __float_as_uint(rx.w);
-    const float ei = rx.x, pi = rx.y, si = rx.z;
+  const unsigned lane = threadIdx.x % kWave, g = lane / kGroup, gl = lane % kGroup;
+  const uint64_t gmask = 0xFFFFull << (kGroup * g);
+  const uint32_t groups = gridDim.x * (blockDim.x / kGroup);
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; i - g < n; i += groups) {
+    // the four groups of a wave take i - g .. i - g + 3; the loop bound is the wave's first one, so
+    // every lane of the wave runs the same iterations
+    const bool has = i < n;
+    uint32_t q = 0, yb = 0, len = 0;
+    float ei = 0.f, pi = 0.f, si = 0.f;
+    if (has) {
+      const uint32_t mod = v.W.t5ListMod[i];
+      yb = v.W.t5ListOff[mod];
+      len = v.W.t5ListOff[mod + 1] - yb;
+      const float4 rx = ld_global(v.W.t5ListRec + i);  // eta, phi, score, slot
+      q = __float_as_uint(rx.w);
+      ei = rx.x; pi = rx.y; si = rx.z;
+    }
     bool dup = false;
-    for (uint32_t k0 = yb; k0 < ye && !dup; k0 += kWave) {
-      const uint32_t k = k0 + lane;
+    for (uint32_t off = 0; __ballot(has && !dup && off < len) != 0ull; off += kGroup) {
       bool d = false;
-      if (k < ye) {
-        const float4 ry = ld_global(v.W.t5ListRec + k);
+      if (has && !dup && off + gl < len) {
+        const float4 ry = ld_global(v.W.t5ListRec + yb + off + gl);
         const uint32_t y = __float_as_uint(ry.w);  // the module's T5s are consecutive slots in list order
         const bool xLo = q < y;
-        const float sj = ry.z;
-        const float ey = ry.x, py = ry.y;
+        const float sj = ry.z, ey = ry.x, py = ry.y;
         const float dEta = m_abs(xLo ? ei - ey : ey - ei);
         const float dPhi = xLo ? wrap_dphi(pi, py) : wrap_dphi(py, pi);
         // the pair can only flag q if the score order points at it
@@ -563,9 +574,9 @@ __global__ __launch_bounds__(256) void k_t5_dup_after_build(LST_VIEWS) {
           d = matched >= 7;
         }
       }
-      dup = __ballot(d) != 0ull;
+      dup = dup || (__ballot(d) & gmask) != 0ull;
     }
-    if (dup && lane == 0) v.Q.isDup[q] = 1;
+    if (has && dup && gl == 0) v.Q.isDup[q] = 1;
   }
 }
 
