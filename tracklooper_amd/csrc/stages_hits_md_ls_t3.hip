@@ -841,29 +841,30 @@ constexpr unsigned kModuleWaveBlocks = 1024;  // x 4 waves
 
 // per module, the candidates of its inner segments (the outer modules'
 // segments) and their shared-MD candidates (the segments listed under each
-// inner segment's outer mini-doublet)
+// inner segment's outer mini-doublet).  One 16-lane group per module (four
+// modules per wave: most hold a few segments); the lanes stride over the inner
+// segments and the group sums their counts.
 __global__ __launch_bounds__(256) void k_t3_cand_count(LST_VIEWS) {
   LST_BIND;
-  const unsigned lane = lane_id();
-  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
-    const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
-    const uint32_t segBase = v.segModuleIndices[m1];
-    uint32_t run = 0, srun = 0;
-    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
-      const uint32_t a = a0 + lane;
-      uint32_t cnt = 0, sc = 0;
-      if (a < nIn) {
-        const uint4 s0 = seg_chunk(v.S, segBase + a, 0);  // outer MD X = s0.y, outer module = s0.w
-        cnt = v.R.nSegments[(uint16_t)s0.w];
-        sc = v.W3.mdSegEnd[s0.y] - v.W3.mdSegOff[s0.y];
-      }
-      const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
-      run += wave_last(incl);
-      srun += wave_last(sincl);
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
+  const uint32_t nL = v.M.nLowerModules, groups = gridDim.x * (blockDim.x / kRow);
+  // the wave's four groups take consecutive modules; the loop runs on the first one's index, so every lane
+  // of the wave takes the same iterations
+  for (uint32_t m1 = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; m1 - g < nL; m1 += groups) {
+    const bool has = m1 < nL;
+    const uint32_t nIn = has && v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+    const uint32_t segBase = has ? v.segModuleIndices[m1] : 0;
+    uint32_t cnt = 0, sc = 0;
+    for (uint32_t a = gl; a < nIn; a += kRow) {
+      const uint4 s0 = seg_chunk(v.S, segBase + a, 0);  // outer MD X = s0.y, outer module = s0.w
+      cnt += v.R.nSegments[(uint16_t)s0.w];
+      sc += v.W3.mdSegEnd[s0.y] - v.W3.mdSegOff[s0.y];
     }
-    if (lane == 0) {
-      v.W3.modOff[m1] = run;
-      v.W3.survOff[m1] = srun;
+    cnt = row_sum(cnt);
+    sc = row_sum(sc);
+    if (has && gl == 0) {
+      v.W3.modOff[m1] = cnt;
+      v.W3.survOff[m1] = sc;
     }
   }
 }
@@ -913,23 +914,27 @@ __global__ __launch_bounds__(256) void k_md_seg_lists(LST_VIEWS) {
   }
 }
 
-// One wave per module m1, its inner segments 64 at a time in the lanes (the
-// segments listed under each one's outer mini-doublet: range and count, and
-// the exclusive prefix of the counts); the lanes then take the chunk's
-// candidates in order, each finding its inner segment by a binary search over
-// the prefix (shuffles), so consecutive lanes write consecutive entries.
+// One 16-lane group per module m1 (four modules per wave), its inner segments
+// 16 at a time in the lanes (the segments listed under each one's outer
+// mini-doublet: range and count, and the exclusive prefix of the counts); the
+// lanes then take the chunk's candidates in order, each finding its inner
+// segment by a binary search over the prefix (row shuffles), so consecutive
+// lanes write consecutive entries.  Every group-level step runs with the
+// group's 16 lanes active (its loop bounds are the group's own).
 __global__ __launch_bounds__(256) void k_t3_enum(LST_VIEWS) {
   LST_BIND;
   if (!t3_flat_mode(v)) return;
-  const unsigned lane = lane_id();
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
   uint2* list = reinterpret_cast<uint2*>(v.W3.list);
-  for (uint32_t m1 = wave_index(); m1 < v.M.nLowerModules; m1 += wave_count()) {
-    const uint32_t nIn = v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
+  const uint32_t nL = v.M.nLowerModules, groups = gridDim.x * (blockDim.x / kRow);
+  for (uint32_t m1 = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; m1 - g < nL; m1 += groups) {
+    const bool has = m1 < nL;
+    const uint32_t nIn = has && v.M.nConnectedModules[m1] != 0 ? v.R.nSegments[m1] : 0;
     if (nIn == 0) continue;
     const uint32_t segBase = v.segModuleIndices[m1];
     uint32_t run = v.W3.survOff[m1];  // list position of the chunk's first candidate
-    for (uint32_t a0 = 0; a0 < nIn; a0 += kWave) {
-      const uint32_t a = a0 + lane;
+    for (uint32_t a0 = 0; a0 < nIn; a0 += kRow) {
+      const uint32_t a = a0 + gl;
       uint32_t cnt = 0, b = 0, seg2Base = 0;
       if (a < nIn) {
         const uint4 s0 = seg_chunk(v.S, segBase + a, 0);  // outer MD X = s0.y, outer module = s0.w
@@ -937,21 +942,18 @@ __global__ __launch_bounds__(256) void k_t3_enum(LST_VIEWS) {
         cnt = v.W3.mdSegEnd[s0.y] - b;
         seg2Base = v.segModuleIndices[(uint16_t)s0.w];
       }
-      const uint32_t incl = wave_inclusive_scan(cnt), pre = incl - cnt;
-      const uint32_t chunk = wave_last(incl);
-      const uint32_t nk = nIn - a0 < (uint32_t)kWave ? nIn - a0 : (uint32_t)kWave;
-      for (uint32_t l0 = 0; l0 < chunk; l0 += kWave) {
-        const uint32_t l = l0 + lane;
-        // the inner segment k of candidate l: the largest k < nk with pre_k <= l (every lane takes part)
-        uint32_t k = 0;
-        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+      const uint32_t incl = row_inclusive_scan(cnt), pre = incl - cnt;
+      const uint32_t chunk = row_last(incl);
+      const uint32_t nk = nIn - a0 < kRow ? nIn - a0 : kRow;
+      for (uint32_t l0 = 0; l0 < chunk; l0 += kRow) {
+        const uint32_t l = l0 + gl;
+        uint32_t k = 0;  // the largest k < nk with pre_k <= l
+        for (uint32_t step = kRow / 2; step >= 1; step >>= 1) {
           const uint32_t c = k + step;
-          const uint32_t pc = (uint32_t)__shfl((int)pre, (int)(c < nk ? c : nk - 1), kWave);
+          const uint32_t pc = row_shfl(pre, c < nk ? c : nk - 1);
           if (c < nk && pc <= l) k = c;
         }
-        const uint32_t pk = (uint32_t)__shfl((int)pre, (int)k, kWave);
-        const uint32_t bk = (uint32_t)__shfl((int)b, (int)k, kWave);
-        const uint32_t sk = (uint32_t)__shfl((int)seg2Base, (int)k, kWave);
+        const uint32_t pk = row_shfl(pre, k), bk = row_shfl(b, k), sk = row_shfl(seg2Base, k);
         if (l < chunk) {
           const uint32_t j = v.W3.segByMD[bk + (l - pk)] - sk;
           list[run + l] = make_uint2(m1, (a0 + k) | (j << 16));

@@ -452,32 +452,34 @@ __device__ __forceinline__ uint32_t t3_last_md(const EventView& v, uint32_t t3) 
 
 // per eligible module, the candidates of its inner T3s (the T3s of their
 // third modules) and their shared-MD candidates (the T3s listed under each
-// inner T3's last mini-doublet)
+// inner T3's last mini-doublet).  One 16-lane group per eligible module (four
+// per wave); the lanes stride over the inner T3s and the group sums.
 __global__ __launch_bounds__(256) void k_t5_cand_count(LST_VIEWS) {
   LST_BIND;
   const uint32_t nElig = v.R.scalars[kNEligibleT5];
-  const unsigned lane = lane_id();
-  for (uint32_t e = wave_index(); e < nElig; e += wave_count()) {
-    const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
-    const int layer = v.M.layers[m1];
-    const uint32_t nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
-    const uint32_t t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
-    uint32_t run = 0, srun = 0;
-    for (uint32_t a0 = 0; a0 < nIn; a0 += 64) {
-      const uint32_t a = a0 + lane;
-      uint32_t cnt = 0, sc = 0;
-      if (a < nIn) {
-        const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
-        cnt = v.R.nTriplets[(uint16_t)t3_chunk(v.T, inner, 1).x];  // the inner T3's third module
-        sc = v.W5.mdT3End[X] - v.W5.mdT3Off[X];
-      }
-      const unsigned incl = wave_inclusive_scan(cnt), sincl = wave_inclusive_scan(sc);
-      run += wave_last(incl);
-      srun += wave_last(sincl);
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
+  const uint32_t groups = gridDim.x * (blockDim.x / kRow);
+  // the loop runs on the wave's first group's index, so every lane of the wave takes the same iterations
+  for (uint32_t e = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; e - g < nElig; e += groups) {
+    const bool has = e < nElig;
+    uint32_t nIn = 0, t3Base = 0;
+    if (has) {
+      const uint16_t m1 = v.R.indicesOfEligibleT5Modules[e];
+      const int layer = v.M.layers[m1];
+      nIn = (layer == 1 || layer == 2) ? v.R.nTriplets[m1] : 0;
+      t3Base = (uint32_t)v.R.tripletModuleIndices[m1];
     }
-    if (lane == 0) {
-      v.W5.modOff[e] = run;
-      v.W5.survOff[e] = srun;
+    uint32_t cnt = 0, sc = 0;
+    for (uint32_t a = gl; a < nIn; a += kRow) {
+      const uint32_t inner = t3Base + a, X = t3_last_md(v, inner);
+      cnt += v.R.nTriplets[(uint16_t)t3_chunk(v.T, inner, 1).x];  // the inner T3's third module
+      sc += v.W5.mdT3End[X] - v.W5.mdT3Off[X];
+    }
+    cnt = row_sum(cnt);
+    sc = row_sum(sc);
+    if (has && gl == 0) {
+      v.W5.modOff[e] = cnt;
+      v.W5.survOff[e] = sc;
     }
   }
 }

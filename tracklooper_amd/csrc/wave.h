@@ -37,6 +37,30 @@ __device__ __forceinline__ unsigned wave_inclusive_scan(unsigned v) {
   return v;
 }
 
+// Row-of-16 primitives: a wave's four rows of 16 lanes work as independent
+// groups (each row's lanes all active).  Inclusive prefix sum inside each row
+// (the first four steps of the wave scan), the value of the row's last lane,
+// and the row's sum in every lane of the row.
+constexpr unsigned kRow = 16;
+__device__ __forceinline__ unsigned row_inclusive_scan(unsigned v) {
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  return v;
+}
+__device__ __forceinline__ unsigned row_last(unsigned v) {
+  return (unsigned)__shfl((int)v, (int)((__lane_id() & ~(kRow - 1)) | (kRow - 1)), 64);
+}
+__device__ __forceinline__ unsigned row_sum(unsigned v) {
+  for (unsigned off = kRow / 2; off > 0; off >>= 1) v += (unsigned)__shfl_xor((int)v, (int)off, 64);
+  return v;
+}
+// lane `k` of the calling lane's row (k < 16)
+__device__ __forceinline__ unsigned row_shfl(unsigned v, unsigned k) {
+  return (unsigned)__shfl((int)v, (int)((__lane_id() & ~(kRow - 1)) | k), 64);
+}
+
 // value of lane 63 (every lane active)
 __device__ __forceinline__ unsigned wave_last(unsigned v) { return (unsigned)__builtin_amdgcn_readlane((int)v, 63); }
 
