@@ -491,23 +491,25 @@ __global__ __launch_bounds__(kBuildBlock) void k_mini_doublets(LST_VIEWS) {
 // --------------------------------------------------------------- line segments
 // Flattened builder, as for the mini-doublets.  Per module, connPre[k] is the
 // first candidate of connected module k (nIn * nMD(k) candidates each, inner
-// MD major), and candOff the module's first global candidate.
+// MD major), and candOff the module's first global candidate.  One 16-lane
+// row per module (four modules per wave), its connections 16 at a time.
 __global__ __launch_bounds__(256) void k_ls_cand_count(LST_VIEWS) {
   LST_BIND;
-  const unsigned lane = lane_id();
-  const uint32_t S = v.M.stride + 1;
-  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
+  const uint32_t S = v.M.stride + 1, nL = v.M.nLowerModules, groups = gridDim.x * (blockDim.x / kRow);
+  for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; m - g < nL; m += groups) {
+    if (m >= nL) continue;
     const uint32_t nIn = v.R.nMDs[m];
     const uint32_t nConn = nIn != 0 ? v.M.nConnectedModules[m] : 0;
     uint32_t run = 0;
-    for (uint32_t k0 = 0; k0 < nConn; k0 += kWave) {
-      const uint32_t k = k0 + lane;
+    for (uint32_t k0 = 0; k0 < nConn; k0 += kRow) {  // the row's 16 lanes take each step together
+      const uint32_t k = k0 + gl;
       const uint32_t c = k < nConn ? nIn * v.R.nMDs[v.M.moduleMap[(size_t)m * v.M.stride + k]] : 0;
-      const unsigned incl = wave_inclusive_scan(c);
+      const unsigned incl = row_inclusive_scan(c);
       if (k < nConn) v.WL.connPre[(size_t)m * S + k] = run + incl - c;
-      run += wave_last(incl);
+      run += row_last(incl);
     }
-    if (lane == 0) {
+    if (gl == 0) {
       v.WL.connPre[(size_t)m * S + nConn] = run;
       v.WL.candOff[m] = run;
     }
@@ -535,46 +537,46 @@ __device__ __forceinline__ bool ls_flat_done(const EventView& v) {
 }
 
 
-// the candidates written out, L by L: one wave per module, its connected
-// modules 64 at a time in the lanes (outer MD count and first MD, and the
-// exclusive prefix of their candidate counts); the lanes then take the
-// module's candidates of those connections in order, each finding its
-// connection by a binary search over the prefix (shuffles), so consecutive
-// lanes write consecutive records whatever the per-connection counts
+// the candidates written out, L by L: one 16-lane row per module (four
+// modules per wave), its connected modules 16 at a time in the lanes (outer MD
+// count and first MD, and the exclusive prefix of their candidate counts); the
+// lanes then take the module's candidates of those connections in order, each
+// finding its connection by a binary search over the prefix (row shuffles), so
+// consecutive lanes write consecutive records whatever the per-connection
+// counts
 __global__ __launch_bounds__(256) void k_ls_cand_fill(LST_VIEWS) {
   LST_BIND;
   if (!ls_flat_mode(v)) return;
-  const unsigned lane = lane_id();
-  const uint32_t S = v.M.stride + 1;
-  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
+  const uint32_t S = v.M.stride + 1, nL = v.M.nLowerModules, groups = gridDim.x * (blockDim.x / kRow);
+  for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; m - g < nL; m += groups) {
+    if (m >= nL) continue;
     const uint32_t nIn = v.R.nMDs[m];
     const uint32_t nConn = nIn != 0 ? v.M.nConnectedModules[m] : 0;
     if (nConn == 0) continue;
     const uint32_t base = v.WL.candOff[m], inBase = (uint32_t)v.R.mdRanges[2 * m];
-    for (uint32_t k0 = 0; k0 < nConn; k0 += kWave) {
+    for (uint32_t k0 = 0; k0 < nConn; k0 += kRow) {  // the row's 16 lanes take each step together
       uint32_t nOut = 0, outBase = 0;
-      if (k0 + lane < nConn) {
-        const uint32_t om = v.M.moduleMap[(size_t)m * v.M.stride + k0 + lane];
+      if (k0 + gl < nConn) {
+        const uint32_t om = v.M.moduleMap[(size_t)m * v.M.stride + k0 + gl];
         nOut = v.R.nMDs[om];
         outBase = nOut ? (uint32_t)v.R.mdRanges[2 * om] : 0u;
       }
       const uint32_t cnt = nIn * nOut;
-      const uint32_t incl = wave_inclusive_scan(cnt), pre = incl - cnt;  // candidates before connection k0 + lane
-      const uint32_t chunk = wave_last(incl);
-      const uint32_t nk = nConn - k0 < (uint32_t)kWave ? nConn - k0 : (uint32_t)kWave;
+      const uint32_t incl = row_inclusive_scan(cnt), pre = incl - cnt;  // candidates before connection k0 + gl
+      const uint32_t chunk = row_last(incl);
+      const uint32_t nk = nConn - k0 < kRow ? nConn - k0 : kRow;
       const uint32_t first = base + v.WL.connPre[(size_t)m * S + k0];  // L of the chunk's first candidate
-      for (uint32_t l0 = 0; l0 < chunk; l0 += kWave) {
-        const uint32_t l = l0 + lane;
-        // the connection k of candidate l: the largest k < nk with pre_k <= l (every lane takes part)
+      for (uint32_t l0 = 0; l0 < chunk; l0 += kRow) {
+        const uint32_t l = l0 + gl;
+        // the connection k of candidate l: the largest k < nk with pre_k <= l
         uint32_t k = 0;
-        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+        for (uint32_t step = kRow / 2; step >= 1; step >>= 1) {
           const uint32_t c = k + step;
-          const uint32_t pc = (uint32_t)__shfl((int)pre, (int)(c < nk ? c : nk - 1), kWave);
+          const uint32_t pc = row_shfl(pre, c < nk ? c : nk - 1);
           if (c < nk && pc <= l) k = c;
         }
-        const uint32_t pk = (uint32_t)__shfl((int)pre, (int)k, kWave);
-        const uint32_t gOut = (uint32_t)__shfl((int)nOut, (int)k, kWave);
-        const uint32_t gBase = (uint32_t)__shfl((int)outBase, (int)k, kWave);
+        const uint32_t pk = row_shfl(pre, k), gOut = row_shfl(nOut, k), gBase = row_shfl(outBase, k);
         if (l < chunk) {
           const uint32_t r = l - pk;
           v.WL.cand[first + l] = make_uint2(inBase + r / gOut, gBase + r % gOut);
