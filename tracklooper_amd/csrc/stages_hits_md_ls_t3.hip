@@ -327,17 +327,19 @@ __device__ __forceinline__ uint32_t md_wave_index() {
   return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
-// the candidates written out, L by L (one wave per module)
+// the candidates written out, L by L: one 16-lane row per module (four modules per wave), the module's candidate hit pairs in order
 __global__ __launch_bounds__(256) void k_md_cand_fill(LST_VIEWS) {
   LST_BIND;
   if (!md_flat_mode(v)) return;
-  const unsigned lane = lane_id();
-  for (uint32_t m = md_wave_index(); m < v.M.nLowerModules; m += (gridDim.x * blockDim.x) / kWave) {
+  const unsigned g = lane_id() / kRow, gl = lane_id() % kRow;
+  const uint32_t nL = v.M.nLowerModules, groups = gridDim.x * (blockDim.x / kRow);
+  for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / kRow; m - g < nL; m += groups) {
+    if (m >= nL) continue;
     const uint32_t n = v.W2.candOff[m + 1] - v.W2.candOff[m];
     if (n == 0) continue;
     const uint32_t off = v.W2.candOff[m], nUp = (uint32_t)v.R.hitRangesnUpper[m];
     const uint32_t lo = (uint32_t)v.R.hitRangesLower[m], up = (uint32_t)v.R.hitRangesUpper[m];
-    for (uint32_t l = lane; l < n; l += kWave) v.W2.cand[off + l] = make_uint4(lo + l / nUp, up + l % nUp, m, 0u);
+    for (uint32_t l = gl; l < n; l += kRow) v.W2.cand[off + l] = make_uint4(lo + l / nUp, up + l % nUp, m, 0u);
   }
 }
 
@@ -893,8 +895,8 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(LST_VIEWS) {
 
 // stage 1: the candidates whose inner segment's outer mini-doublet X is the
 // outer segment's inner one.  Per mini-doublet, the segments starting at it
-// are listed first, in slot order (one wave per lower module: counts,
-// offsets, a scatter inside the module's own segment range, then each
+// are listed first, in slot order (four lower modules per wave, one per row
+// where they are small: counts, offsets, a scatter inside the module's own segment range, then each
 // segment placed at its rank within its short list); the candidate count
 // pass also counts, per inner segment,
 // the segments listed under X; then one wave per module m1 walks its inner
@@ -903,17 +905,17 @@ __global__ __launch_bounds__(1024) void k_t3_cand_scan(LST_VIEWS) {
 // order (module, inner segment, outer segment).
 __global__ __launch_bounds__(256) void k_md_seg_lists(LST_VIEWS) {
   LST_BIND;
-  __shared__ __align__(16) ListLds sLists[4];
-  ListLds& sX = sLists[threadIdx.x / kWave];
-  const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t m = wave_index(); m < nL; m += wave_count()) {
-    const uint32_t nMD = v.R.nMDs[m], nS = v.R.nSegments[m];
-    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m], segBase = (uint32_t)v.segModuleIndices[m];
-    if (nMD == 0) continue;
-    md_lists(
-        nMD, mdBase, nS, segBase, [&](uint32_t k) { return v.S.mdIndices[2 * (segBase + k)]; }, v.W3.mdSegOff,
-        v.W3.mdSegEnd, v.W3.segByMD, v.W3.segTmp, sX, v.largePaths);
-  }
+  __shared__ __align__(16) WaveLists sLists[4];
+  md_lists_all(
+      v.M.nLowerModules,
+      [&](uint32_t m, uint32_t& nMD, uint32_t& mdBase, uint32_t& n, uint32_t& base) {
+        nMD = v.R.nMDs[m];
+        n = v.R.nSegments[m];
+        mdBase = (uint32_t)v.mdModuleIndices[m];
+        base = (uint32_t)v.segModuleIndices[m];
+      },
+      [&](uint32_t s) { return seg_chunk(v.S, s, 0).x; }, v.W3.mdSegOff, v.W3.mdSegEnd, v.W3.segByMD, v.W3.segTmp,
+      sLists[threadIdx.x / kWave], v.largePaths != 0);
 }
 
 // One 16-lane group per module m1 (four modules per wave), its inner segments
@@ -1203,7 +1205,7 @@ void launch_segments(const Launch& L) {
 void launch_triplets(const Launch& L) {
   const EventView& v = L.v;
   LST_GO(L, k_triplet_offsets, 1, 1, 1024);
-  LST_GO_T(L, kTnMdSegLists, k_md_seg_lists, (v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1, 256);
+  LST_GO_T(L, kTnMdSegLists, k_md_seg_lists, (v.M.nLowerModules + 15) / 16 ? (v.M.nLowerModules + 15) / 16 : 1, 256);
   LST_GO_T(L, kTnT3CandCount, k_t3_cand_count, kModuleWaveBlocks, 256);
   LST_GO(L, k_t3_cand_scan, 1, 1, 1024);
   L.t.begin(L.s, kTmT3Filter);

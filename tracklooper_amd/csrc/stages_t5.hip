@@ -426,23 +426,22 @@ __device__ __forceinline__ uint32_t wave_index() {
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / 64; }
 
 // per eligible module: candidate count and per-inner-T3 exclusive prefix
-// Per mini-doublet, the T3s whose first mini-doublet it is, in slot order (one
-// wave per lower module: counts, offsets, a scatter inside the module's own T3
+// Per mini-doublet, the T3s whose first mini-doublet it is, in slot order (four
+// lower modules per wave, one per row where they are small: counts, offsets, a scatter inside the module's own T3
 // range, then each T3 placed at its rank within its short list).
 __global__ __launch_bounds__(256) void k_md_t3_lists(LST_VIEWS) {
   LST_BIND;
-  __shared__ __align__(16) ListLds sLists[4];
-  ListLds& sX = sLists[threadIdx.x / kWave];
-  const uint32_t nL = v.M.nLowerModules;
-  for (uint32_t m = wave_index(); m < nL; m += wave_count()) {
-    const uint32_t nMD = v.R.nMDs[m], nT = v.R.nTriplets[m];
-    const uint32_t mdBase = (uint32_t)v.mdModuleIndices[m];
-    const uint32_t t3Base = nT ? (uint32_t)v.R.tripletModuleIndices[m] : 0u;
-    if (nMD == 0) continue;
-    md_lists(
-        nMD, mdBase, nT, t3Base, [&](uint32_t k) { return v.S.mdIndices[2 * v.T.segmentIndices[2 * (t3Base + k)]]; },
-        v.W5.mdT3Off, v.W5.mdT3End, v.W5.t3ByMD, v.W5.t3Tmp, sX, v.largePaths);
-  }
+  __shared__ __align__(16) WaveLists sLists[4];
+  md_lists_all(
+      v.M.nLowerModules,
+      [&](uint32_t m, uint32_t& nMD, uint32_t& mdBase, uint32_t& n, uint32_t& base) {
+        nMD = v.R.nMDs[m];
+        n = v.R.nTriplets[m];
+        mdBase = (uint32_t)v.mdModuleIndices[m];
+        base = n ? (uint32_t)v.R.tripletModuleIndices[m] : 0u;
+      },
+      [&](uint32_t t) { return seg_chunk(v.S, t3_chunk(v.T, t, 0).x, 0).x; },  // the inner segment's inner MD
+      v.W5.mdT3Off, v.W5.mdT3End, v.W5.t3ByMD, v.W5.t3Tmp, sLists[threadIdx.x / kWave], v.largePaths != 0);
 }
 
 // the outer mini-doublet of a T3, which an outer T3 must start at
@@ -809,7 +808,7 @@ void launch_quintuplets(const Launch& L) {
   const unsigned nBlocks = v.nT5SeedModules ? v.nT5SeedModules : 1;
   const unsigned flatGrid = 2048;  // grid-stride over candidates / survivors
   LST_GO(L, k_t5_eligible, 1, 1, 1024);
-  LST_GO_T(L, kTnMdT3Lists, k_md_t3_lists, (v.M.nLowerModules + 3) / 4 ? (v.M.nLowerModules + 3) / 4 : 1, 256);
+  LST_GO_T(L, kTnMdT3Lists, k_md_t3_lists, (v.M.nLowerModules + 15) / 16 ? (v.M.nLowerModules + 15) / 16 : 1, 256);
   LST_GO_T(L, kTnT5CandCount, k_t5_cand_count, kModuleWaveBlocks, 256);
   LST_GO(L, k_t5_cand_scan, 1, 1, 1024);
   LST_GO_T(L, kTnT5Enum, k_t5_enum, kModuleWaveBlocks, 256);

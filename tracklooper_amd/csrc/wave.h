@@ -284,4 +284,86 @@ __device__ void md_lists(uint32_t nMD, uint32_t mdBase, uint32_t n, uint32_t bas
   }
 }
 
+// The counting sort of md_lists by one 16-lane row, for a module of at most
+// kRowListMDs mini-doublets and kRowListCap objects, in the row's quarter of
+// the wave's list LDS (the row's lanes all active).
+constexpr uint32_t kRowListCap = 128, kRowListMDs = 2 * kRow;
+struct RowLds {
+  uint32_t x[kRowListCap];    // mini-doublet of each object, relative to mdBase
+  uint32_t tmp[kRowListCap];  // objects grouped by mini-doublet
+  uint32_t cnt[kRowListMDs];  // counts, then scatter cursors (list ends afterwards)
+  uint32_t off[kRowListMDs];  // list starts, relative to base
+};
+union WaveLists {  // one wave's list LDS: the whole of it for one module, or a quarter per row
+  ListLds full;
+  RowLds rows[4];
+};
+template <class First>
+__device__ void md_lists_row(uint32_t nMD, uint32_t mdBase, uint32_t n, uint32_t base, First first, uint32_t* off,
+                             uint32_t* end, uint32_t* byMD, RowLds& L) {
+  const unsigned gl = __lane_id() % kRow;
+  wave_lds_sync();
+  for (uint32_t k = gl; k < n; k += kRow) L.x[k] = first(k) - mdBase;
+  for (uint32_t k = gl; k < nMD; k += kRow) L.cnt[k] = 0;
+  wave_lds_sync();
+  for (uint32_t k = gl; k < n; k += kRow) atomicAdd(&L.cnt[L.x[k]], 1u);
+  wave_lds_sync();
+  {
+    const uint32_t b0 = 2 * gl, b1 = b0 + 1;
+    const uint32_t c0 = b0 < nMD ? L.cnt[b0] : 0u, c1 = b1 < nMD ? L.cnt[b1] : 0u;
+    const uint32_t incl = row_inclusive_scan(c0 + c1), ex = incl - c0 - c1;
+    wave_lds_sync();
+    if (b0 < nMD) {
+      L.off[b0] = L.cnt[b0] = ex;
+      off[mdBase + b0] = base + ex;
+      end[mdBase + b0] = base + ex + c0;
+    }
+    if (b1 < nMD) {
+      L.off[b1] = L.cnt[b1] = ex + c0;
+      off[mdBase + b1] = base + ex + c0;
+      end[mdBase + b1] = base + incl;
+    }
+  }
+  wave_lds_sync();
+  for (uint32_t k = gl; k < n; k += kRow) L.tmp[atomicAdd(&L.cnt[L.x[k]], 1u)] = k;
+  wave_lds_sync();
+  for (uint32_t k = gl; k < n; k += kRow) {
+    const uint32_t X = L.x[k], b = L.off[X], e = L.cnt[X];
+    uint32_t r = b;
+    for (uint32_t q = b; q < e; q++) r += L.tmp[q] < k ? 1u : 0u;
+    byMD[base + r] = base + k;
+  }
+}
+
+// md_lists for every lower module, four modules per wave: when all four fit a
+// row (kRowListMDs, kRowListCap, byCount off; most modules hold a few
+// objects), each 16-lane row sorts one; otherwise the whole wave takes them
+// one after the other.  info(m, nMD, mdBase, n, base) describes module m;
+// firstOf(object) is the mini-doublet the object starts at.
+template <class Info, class FirstOf>
+__device__ void md_lists_all(uint32_t nL, Info info, FirstOf firstOf, uint32_t* off, uint32_t* end, uint32_t* byMD,
+                             uint32_t* tmp, WaveLists& W, bool byCount) {
+  const unsigned g = __lane_id() / kRow;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const uint32_t nWaves = gridDim.x * blockDim.x / 64;
+  for (uint32_t m0 = 4 * wave; m0 < nL; m0 += 4 * nWaves) {
+    const uint32_t m = m0 + g;
+    uint32_t nMD = 0, mdBase = 0, n = 0, base = 0;
+    if (m < nL) info(m, nMD, mdBase, n, base);
+    const bool fits = nMD <= kRowListMDs && n <= kRowListCap;
+    if (!byCount && __ballot(!fits) == 0ull) {  // the same branch in every lane
+      if (nMD != 0)
+        md_lists_row(nMD, mdBase, n, base, [&](uint32_t k) { return firstOf(base + k); }, off, end, byMD, W.rows[g]);
+    } else {
+      for (uint32_t r = 0; r < 4 && m0 + r < nL; r++) {
+        uint32_t nMDr = 0, mdBaser = 0, nr = 0, baser = 0;
+        info(m0 + r, nMDr, mdBaser, nr, baser);
+        if (nMDr == 0) continue;
+        md_lists(nMDr, mdBaser, nr, baser, [&](uint32_t k) { return firstOf(baser + k); }, off, end, byMD, tmp, W.full,
+                 byCount);
+      }
+    }
+  }
+}
+
 }  // namespace lst
