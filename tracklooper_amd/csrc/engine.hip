@@ -288,7 +288,6 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.pt5Shard, 2 * lst::kPixShards); c.take(e.W.pt3Shard, 2 * lst::kPixShards);
   c.take(e.W.pixWords, plsCap); c.take(e.W.pixFull, plsCap);
   c.take(e.W.pixSurv, (size_t)lst::kSurvWords * lst::kPixSurvMax);
-  c.take(e.W.pixParkIdx, (size_t)64 * lst::kPixMaskWords);
   c.take(e.W.t3ListOff, nL); c.take(e.W.t3List, t3Cap ? t3Cap : 1);
   c.take(e.W.t5Rec, (size_t)lst::kT5RecWords * (t5Cap ? t5Cap : 1));
   c.take(e.W.t3Rec, (size_t)lst::kT3RecWords * (t3Cap ? t3Cap : 1));

@@ -302,7 +302,6 @@ struct PixelWork {
   uint8_t* pixFull;             // [pLS capacity] 1: the pLS's survivors did not all fit, evaluated in full
   uint32_t* pixSurv;            // [kSurvWords * kPixSurvMax] survivors of the cheap cuts: pLS, mask
                                 //   position, object, partial result
-  uint32_t* pixParkIdx;         // [64 * kPixMaskWords] per mask position: the survivor of a passing candidate
   uint32_t* t3ListOff;          // [nLower] first list position of each module's T3s (pixel-triplet modules)
   uint32_t* t3List;             // [T3 capacity] T3 slots of those modules
   float* t5Rec;                 // [kT5RecWords * T5 capacity] pLS-independent quantities per T5

@@ -515,7 +515,6 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_b(LST_VIEWS) {
       K::half_in(r, sOut[pair][lane]);
       K::pack(r, &v.W.pixSurv[(size_t)kSurvWords * s + 3]);
       atomicOr(&K::mask(v)[slot / kWave], 1ull << (slot % kWave));
-      v.W.pixParkIdx[slot] = s;
       atomicAdd(&K::count(v)[i], 1u);
     }
     __syncthreads();  // the LDS outcomes are read before the next survivors overwrite them
@@ -549,48 +548,43 @@ __global__ __launch_bounds__(kPixWG) void k_pix_stage_b(LST_VIEWS) {
   }
 }
 
-// One wave per pLS: the lanes load 64 of its mask words at a time and scan
-// their pass counts in registers; then, for each non-zero word in order (found
-// from the ballot of non-zero words), the lanes of its set bits write their
-// passing candidates to their slots, in candidate order.  No barrier: the
-// waves of a work-group take different pLS.
+// One lane per survivor of stage A (the shards' lists, as stage B numbers
+// them): a survivor whose mask bit stage B set is written at its slot, the
+// pLS's first slot (count, scanned) plus the passing candidates before it in
+// the pLS's mask words.  The pLS flagged full are written by k_pix_full_write.
 template <class K>
 __global__ __launch_bounds__(kPixWG) void k_pix_write(LST_VIEWS) {
   LST_BIND;
-  const uint32_t nPLS = v.R.scalars[kEvNPLS];
+  __shared__ uint32_t pre[kPixShards + 1];  // survivors before each shard
+  if (threadIdx.x < kWave) {
+    const uint32_t c = K::shard(v)[kPixShards + threadIdx.x];
+    const uint32_t incl = wave_inclusive_scan(c < kShardSurv ? c : kShardSurv);
+    pre[threadIdx.x + 1] = incl;
+    if (threadIdx.x == 0) pre[0] = 0;
+  }
+  __syncthreads();
+  static_assert(kPixShards == kWave, "one lane per shard");
+  const uint32_t nS = pre[kPixShards];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
-  const unsigned lane = threadIdx.x % kWave;
-  constexpr uint32_t kWavesPerWG = kPixWG / kWave;
-  for (uint32_t i = blockIdx.x * kWavesPerWG + threadIdx.x / kWave; i < nPLS; i += gridDim.x * kWavesPerWG) {
-    const uint32_t maskBase = K::mask_base(v)[i];
-    const uint8_t full = v.W.pixFull[i];
-    if (maskBase == ~0u || full) continue;  // the same in every lane of the wave
-    const uint32_t words = v.W.pixWords[i], pls = plsBase + i;
-    uint32_t before = K::count(v)[i];
-    for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
-      const uint32_t w = w0 + lane;
-      const unsigned long long mk = w < words ? K::mask(v)[maskBase + w] : 0ull;
-      const unsigned cnt = (unsigned)__popcll(mk);
-      const unsigned incl = wave_inclusive_scan(cnt);
-      const unsigned pre = incl - cnt;
-      unsigned long long nz = __ballot(mk != 0ull);
-      while (nz) {
-        const int k = __ffsll((long long)nz) - 1;
-        nz &= nz - 1;
-        const unsigned long long m = ((unsigned long long)(unsigned)__shfl((int)(mk >> 32), k, kWave) << 32) |
-                                     (unsigned)__shfl((int)(unsigned)mk, k, kWave);
-        const unsigned pk = (unsigned)__shfl((int)pre, k, kWave);
-        if (((m >> lane) & 1ull) == 0) continue;
-        const uint32_t q = before + pk + lanes_below(m);
-        if (q < K::kCap) {
-          const uint32_t s = v.W.pixParkIdx[(maskBase + w0 + k) * kWave + lane];
-          const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
-          typename K::Res r;
-          K::unpack(rec + 3, r);
-          K::write(v, q, pls, i, rec[2], r);
-        }
-      }
-      before += wave_last(incl);
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nS; g += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = kPixShards;  // the shard holding survivor g: largest lo with pre[lo] <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t s = lo * kShardSurv + (g - pre[lo]);
+    const uint32_t* rec = &v.W.pixSurv[(size_t)kSurvWords * s];
+    const uint32_t i = rec[0], slot = rec[1], t = rec[2];
+    const unsigned long long* mask = K::mask(v);
+    const unsigned long long mk = mask[slot / kWave];
+    if (((mk >> (slot % kWave)) & 1ull) == 0 || v.W.pixFull[i]) continue;
+    uint32_t q = K::count(v)[i] + (uint32_t)__popcll(mk & ((1ull << (slot % kWave)) - 1ull));
+    for (uint32_t w = K::mask_base(v)[i]; w < slot / kWave; w++) q += (uint32_t)__popcll(mask[w]);
+    if (q < K::kCap) {
+      typename K::Res r;
+      K::unpack(rec + 3, r);
+      K::write(v, q, plsBase + i, i, t, r);
     }
   }
 }
