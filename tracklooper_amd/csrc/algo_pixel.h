@@ -334,14 +334,17 @@ __device__ __forceinline__ int rec_layer(uint32_t packed, int shift, int k) { re
 
 __device__ __forceinline__ void make_t3_rec(const EventView& v, uint32_t t3, float* out) {
   const ModulesView& M = v.M;
-  const uint16_t mods[3] = {v.T.lowerModuleIndices[3 * t3], v.T.lowerModuleIndices[3 * t3 + 1],
-                            v.T.lowerModuleIndices[3 * t3 + 2]};
-  const uint32_t segIn = v.T.segmentIndices[2 * t3], segOut = v.T.segmentIndices[2 * t3 + 1];
-  const uint32_t md[3] = {v.S.mdIndices[2 * segIn], v.S.mdIndices[2 * segIn + 1], v.S.mdIndices[2 * segOut + 1]};
+  // the record's chunks and the segments' and mini-doublets' first chunks, each one 16-byte load
+  const uint4 c0 = t3_chunk(v.T, t3, 0), c1 = t3_chunk(v.T, t3, 1), c3 = t3_chunk(v.T, t3, 3),
+              c4 = t3_chunk(v.T, t3, 4);
+  const uint16_t mods[3] = {(uint16_t)c0.z, (uint16_t)c0.w, (uint16_t)c1.x};
+  const uint4 sIn = seg_chunk(v.S, c0.x, 0), sOut = seg_chunk(v.S, c0.y, 0);
+  const uint32_t md[3] = {sIn.x, sIn.y, sOut.y};
   float x[3], y[3], sig[3];
   for (int k = 0; k < 3; k++) {
-    x[k] = v.D.anchorX[md[k]];
-    y[k] = v.D.anchorY[md[k]];
+    const uint4 p = md_chunk(v.D, md[k], 1);
+    x[k] = __uint_as_float(p.x);
+    y[k] = __uint_as_float(p.y);
   }
   point_sigmas(M, mods, 3, x, y, sig);
   uint32_t packed = (uint32_t)pt3_radius_kind(M, mods[0], mods[1], mods[2]);
@@ -350,7 +353,7 @@ __device__ __forceinline__ void make_t3_rec(const EventView& v, uint32_t t3, flo
   float4* o = reinterpret_cast<float4*>(out);
   o[0] = make_float4(x[0], x[1], x[2], y[0]);
   o[1] = make_float4(y[1], y[2], sig[0], sig[1]);
-  o[2] = make_float4(sig[2], v.T.circleRadius[t3], v.T.circleCenterX[t3], v.T.circleCenterY[t3]);
+  o[2] = make_float4(sig[2], __uint_as_float(c3.w), __uint_as_float(c4.x), __uint_as_float(c4.y));
   o[3] = make_float4(__uint_as_float(packed), 0.f, 0.f, 0.f);
 }
 
@@ -606,10 +609,10 @@ struct PT5Result {
 __device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, float* out) {
   const ModulesView& M = v.M;
   const uint32_t innerT3 = v.Q.tripletIndices[2 * t5], outerT3 = v.Q.tripletIndices[2 * t5 + 1];
-  const uint32_t s1 = v.T.segmentIndices[2 * innerT3], s2 = v.T.segmentIndices[2 * innerT3 + 1];
-  const uint32_t s3 = v.T.segmentIndices[2 * outerT3], s4 = v.T.segmentIndices[2 * outerT3 + 1];
-  const uint32_t md[5] = {v.S.mdIndices[2 * s1], v.S.mdIndices[2 * s2], v.S.mdIndices[2 * s2 + 1],
-                          v.S.mdIndices[2 * s3 + 1], v.S.mdIndices[2 * s4 + 1]};
+  const uint4 ti = t3_chunk(v.T, innerT3, 0), to = t3_chunk(v.T, outerT3, 0);
+  const uint4 g1 = seg_chunk(v.S, ti.x, 0), g2 = seg_chunk(v.S, ti.y, 0), g3 = seg_chunk(v.S, to.x, 0),
+              g4 = seg_chunk(v.S, to.y, 0);
+  const uint32_t md[5] = {g1.x, g2.x, g2.y, g3.y, g4.y};
   uint16_t mods[5];
   float w[kT5RecWords];
   uint32_t packed = 0;
@@ -617,10 +620,11 @@ __device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, flo
   for (int k = 0; k < 5; k++) {
     mods[k] = v.Q.lowerModuleIndices[5 * t5 + k];
     const unsigned m = mods[k];
-    w[k] = v.D.anchorX[md[k]];
-    w[5 + k] = v.D.anchorY[md[k]];
-    w[10 + k] = v.D.anchorZ[md[k]];
-    w[15 + k] = v.D.anchorRt[md[k]];
+    const uint4 p = md_chunk(v.D, md[k], 1);  // x, y, z, rt
+    w[k] = __uint_as_float(p.x);
+    w[5 + k] = __uint_as_float(p.y);
+    w[10 + k] = __uint_as_float(p.z);
+    w[15 + k] = __uint_as_float(p.w);
     // r-z error of the point (computePT5RZChiSquared, SDL/PixelTriplet.h:2636-2677)
     const int type = M.moduleType[m], side = M.sides[m], sub = M.subdets[m];
     const float drdz = M.drdzs[m];
@@ -632,7 +636,7 @@ __device__ __forceinline__ void make_t5_rec(const EventView& v, uint32_t t5, flo
   }
   packed |= (uint32_t)pt3_radius_kind(M, mods[0], mods[1], mods[2]) << 5;
   point_sigmas(M, mods, 5, w, w + 5, w + 20);
-  w[30] = v.T.circleRadius[innerT3];
+  w[30] = __uint_as_float(t3_chunk(v.T, innerT3, 3).w);  // circle radius
   w[31] = __uint_as_float(packed);
   float4* o = reinterpret_cast<float4*>(out);
 #pragma unroll

@@ -179,9 +179,12 @@ constexpr uint32_t kT3ObjWords = (t3_word(kT3FieldCount) + 3) & ~3u;
 constexpr uint32_t kT3ObjBytes = 4 * kT3ObjWords;
 #define LST_DECL_T3_FIELD(T, N, M) RecField<T, M, kT3ObjBytes> N;
 struct T3SoA { LST_T3_FIELDS(LST_DECL_T3_FIELD) };
-static_assert(t3_word(kT3F_lowerModuleIndices) == 2, "the chunks t3_chunk readers expect");
+static_assert(t3_word(kT3F_lowerModuleIndices) == 2 && t3_word(kT3F_circleRadius) == 15 &&
+                  t3_word(kT3F_circleCenterX) == 16 && t3_word(kT3F_circleCenterY) == 17,
+              "the chunks t3_chunk readers expect");
 // 16-byte chunk q of triplet record t: 0 {inner segment, outer segment, module 0, module 1}, 1 {module 2,
-// logical layers}, ...; one global load
+// logical layers}, 2 {hits 0-3}, 3 {hits 4-5, betaIn, circle radius}, 4 {circle centre x, y, flags}; one
+// global load
 __device__ __forceinline__ uint4 t3_chunk(const T3SoA& T, uint32_t t, uint32_t q) {
   return ld_global(reinterpret_cast<const uint4*>(T.segmentIndices.p + (size_t)t * kT3ObjBytes) + q);
 }
