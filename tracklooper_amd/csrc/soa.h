@@ -116,13 +116,16 @@ static_assert(kMDw_anchorHitIndices == 0 && kMDw_moduleIndices == 1 && kMDw_dphi
                   kMDw_anchorX == 4 && kMDw_anchorY == 5 && kMDw_anchorZ == 6 && kMDw_anchorRt == 7 &&
                   kMDw_dphis == 8 && kMDw_anchorPhi == 9 && kMDw_anchorHighEdgePhi == 10 &&
                   kMDw_anchorLowEdgePhi == 11 && kMDw_anchorHighEdgeX == 12 && kMDw_anchorHighEdgeY == 13 &&
-                  kMDw_anchorLowEdgeX == 14 && kMDw_anchorLowEdgeY == 15 && kMDw_shiftedXs == kMDHotFields,
+                  kMDw_anchorLowEdgeX == 14 && kMDw_anchorLowEdgeY == 15 && kMDw_outerHitIndices == 16 &&
+                  kMDw_anchorEta == 17 && kMDw_shiftedXs == kMDHotFields,
               "the chunks md_chunk readers expect, and the first cold field");
 // the slope pair of mini-doublet record i (hot words 16, 17), one global load
 __device__ __forceinline__ float2 md_slope(const MDsSoA& D, uint32_t i) {
   return ld_global(reinterpret_cast<const float2*>(D.anchorHitIndices.p + (size_t)i * kMDHotBytes + 4 * kMDSlopeWord));
 }
-// 16-byte chunk q (hot words 4q .. 4q + 3) of mini-doublet record i, one global load
+// 16-byte chunk q (hot words 4q .. 4q + 3) of mini-doublet record i, one global load: 0 {anchor hit, module,
+// dphichange, dz}, 1 {anchor x, y, z, rt}, 2 {dphi, anchor phi, edge phis}, 3 {edge x, y}, 4 {slope pair,
+// outer hit, anchor eta}
 __device__ __forceinline__ uint4 md_chunk(const MDsSoA& D, uint32_t i, uint32_t q) {
   return ld_global(reinterpret_cast<const uint4*>(D.anchorHitIndices.p + (size_t)i * kMDHotBytes) + q);
 }

@@ -796,9 +796,9 @@ __device__ __forceinline__ bool t3_select(const EventView& v, uint32_t m1, uint3
 // addTripletToMemory (SDL/Triplet.h:153-239)
 __device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_t m1, uint32_t inSeg, uint32_t outSeg,
                                          float betaIn, float radius, float gx, float gy) {
-  const uint16_t m2 = v.S.outerLowerModuleIndices[inSeg], m3 = v.S.outerLowerModuleIndices[outSeg];
-  const uint32_t md1 = v.S.mdIndices[2 * inSeg], mdB = v.S.mdIndices[2 * inSeg + 1];
-  const uint32_t md3 = v.S.mdIndices[2 * outSeg + 1];
+  const uint4 sIn = seg_chunk(v.S, inSeg, 0), sOut = seg_chunk(v.S, outSeg, 0);  // {MD in, MD out, modules}
+  const uint16_t m2 = (uint16_t)sIn.w, m3 = (uint16_t)sOut.w;
+  const uint32_t md1 = sIn.x, mdB = sIn.y, md3 = sOut.y;
   // the record built in registers (u16 and u8 fields in the low bytes of their words, flags 0) and
   // written with 16-byte stores
   uint32_t w[kT3ObjWords] = {};
@@ -809,9 +809,11 @@ __device__ __forceinline__ void t3_store(const EventView& v, uint32_t t, uint32_
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     w[t3_word(kT3F_lowerModuleIndices) + k] = mods[k];
-    w[t3_word(kT3F_logicalLayers) + k] = (uint8_t)(v.M.layers[mods[k]] + (v.M.subdets[mods[k]] == kEndcap) * 6);
-    w[t3_word(kT3F_hitIndices) + 2 * k] = v.D.anchorHitIndices[mds[k]];
-    w[t3_word(kT3F_hitIndices) + 2 * k + 1] = v.D.outerHitIndices[mds[k]];
+    ModCtx c;
+    unpack_mod(mod_word(v.M, mods[k]), c);
+    w[t3_word(kT3F_logicalLayers) + k] = (uint8_t)(c.layer + (c.subdet == kEndcap) * 6);
+    w[t3_word(kT3F_hitIndices) + 2 * k] = md_chunk(v.D, mds[k], 0).x;      // anchor hit
+    w[t3_word(kT3F_hitIndices) + 2 * k + 1] = md_chunk(v.D, mds[k], 4).z;  // outer hit
   }
   w[t3_word(kT3F_betaIn)] = __float_as_uint(betaIn);
   w[t3_word(kT3F_circleRadius)] = __float_as_uint(radius);

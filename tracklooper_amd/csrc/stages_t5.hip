@@ -371,21 +371,36 @@ __device__ bool t5_candidate(const EventView& v, const uint16_t* mods, uint32_t 
 // addQuintupletToMemory (SDL/Quintuplet.h:150-215) plus the T3 flags
 __device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_t innerT3, uint32_t outerT3,
                                          const T5Result& r, int layer) {
+  // the two T3 records' first four chunks (segments, modules, logical layers, hits), one load each
+  uint4 a[4], b[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    a[k] = t3_chunk(v.T, innerT3, k);
+    b[k] = t3_chunk(v.T, outerT3, k);
+  }
+  const uint32_t ai[16] = {a[0].x, a[0].y, a[0].z, a[0].w, a[1].x, a[1].y, a[1].z, a[1].w,
+                           a[2].x, a[2].y, a[2].z, a[2].w, a[3].x, a[3].y, a[3].z, a[3].w};
+  const uint32_t bi[16] = {b[0].x, b[0].y, b[0].z, b[0].w, b[1].x, b[1].y, b[1].z, b[1].w,
+                           b[2].x, b[2].y, b[2].z, b[2].w, b[3].x, b[3].y, b[3].z, b[3].w};
+  constexpr uint32_t kMod = t3_word(kT3F_lowerModuleIndices), kLL = t3_word(kT3F_logicalLayers),
+                     kHit = t3_word(kT3F_hitIndices);
+  static_assert(kHit + 6 <= 16, "the hits are in the first four chunks");
   const int adj = layer == 1 ? 1 : 0;
-  const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * innerT3 + adj]];
+  const uint32_t mdAdj = seg_chunk(v.S, adj ? a[0].y : a[0].x, 0).x;  // first MD of the inner or middle segment
+  const uint4 mdA = md_chunk(v.D, mdAdj, 2), mdE = md_chunk(v.D, mdAdj, 4);  // anchor phi, anchor eta
   v.Q.tripletIndices[2 * q] = innerT3;
   v.Q.tripletIndices[2 * q + 1] = outerT3;
-  v.Q.lowerModuleIndices[5 * q + 0] = v.T.lowerModuleIndices[3 * innerT3];
-  v.Q.lowerModuleIndices[5 * q + 1] = v.T.lowerModuleIndices[3 * innerT3 + 1];
-  v.Q.lowerModuleIndices[5 * q + 2] = v.T.lowerModuleIndices[3 * innerT3 + 2];
-  v.Q.lowerModuleIndices[5 * q + 3] = v.T.lowerModuleIndices[3 * outerT3 + 1];
-  v.Q.lowerModuleIndices[5 * q + 4] = v.T.lowerModuleIndices[3 * outerT3 + 2];
+  v.Q.lowerModuleIndices[5 * q + 0] = (uint16_t)ai[kMod];
+  v.Q.lowerModuleIndices[5 * q + 1] = (uint16_t)ai[kMod + 1];
+  v.Q.lowerModuleIndices[5 * q + 2] = (uint16_t)ai[kMod + 2];
+  v.Q.lowerModuleIndices[5 * q + 3] = (uint16_t)bi[kMod + 1];
+  v.Q.lowerModuleIndices[5 * q + 4] = (uint16_t)bi[kMod + 2];
   v.Q.innerRadius[q] = r.innerR;
   v.Q.outerRadius[q] = r.outerR;
   v.Q.bridgeRadius[q] = r.bridgeR;
   v.Q.pt[q] = (r.innerR + r.outerR) * 3.8f * 1.602f / (2 * 100 * 5.39f);
-  v.Q.eta[q] = v.D.anchorEta[mdAdj];
-  v.Q.phi[q] = v.D.anchorPhi[mdAdj];
+  v.Q.eta[q] = __uint_as_float(mdE.w);
+  v.Q.phi[q] = __uint_as_float(mdA.y);
   v.Q.score_rphisum[q] = r.chi2 + r.nonAnchorChi2;
   v.Q.layer[q] = (uint8_t)layer;
   v.Q.isDup[q] = 0;
@@ -397,15 +412,15 @@ __device__ __forceinline__ void t5_store(const EventView& v, uint32_t q, uint32_
   v.Q.rzChiSquared[q] = r.rzChi2;
   v.Q.chiSquared[q] = r.chi2;
   v.Q.nonAnchorChiSquared[q] = r.nonAnchorChi2;
-  v.Q.logicalLayers[5 * q + 0] = v.T.logicalLayers[3 * innerT3];
-  v.Q.logicalLayers[5 * q + 1] = v.T.logicalLayers[3 * innerT3 + 1];
-  v.Q.logicalLayers[5 * q + 2] = v.T.logicalLayers[3 * innerT3 + 2];
-  v.Q.logicalLayers[5 * q + 3] = v.T.logicalLayers[3 * outerT3 + 1];
-  v.Q.logicalLayers[5 * q + 4] = v.T.logicalLayers[3 * outerT3 + 2];
+  v.Q.logicalLayers[5 * q + 0] = (uint8_t)ai[kLL];
+  v.Q.logicalLayers[5 * q + 1] = (uint8_t)ai[kLL + 1];
+  v.Q.logicalLayers[5 * q + 2] = (uint8_t)ai[kLL + 2];
+  v.Q.logicalLayers[5 * q + 3] = (uint8_t)bi[kLL + 1];
+  v.Q.logicalLayers[5 * q + 4] = (uint8_t)bi[kLL + 2];
 #pragma unroll
-  for (int j = 0; j < 6; j++) v.Q.hitIndices[10 * q + j] = v.T.hitIndices[6 * innerT3 + j];
+  for (int j = 0; j < 6; j++) v.Q.hitIndices[10 * q + j] = ai[kHit + j];
 #pragma unroll
-  for (int j = 0; j < 4; j++) v.Q.hitIndices[10 * q + 6 + j] = v.T.hitIndices[6 * outerT3 + 2 + j];
+  for (int j = 0; j < 4; j++) v.Q.hitIndices[10 * q + 6 + j] = bi[kHit + 2 + j];
   v.T.partOfT5[innerT3] = 1;
   v.T.partOfT5[outerT3] = 1;
 }
