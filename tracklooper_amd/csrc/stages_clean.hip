@@ -667,10 +667,15 @@ __global__ __launch_bounds__(256) void k_t5_tc_pairs(LST_VIEWS) {
           }
           const bool xLo = x < y;
           uint32_t hx[10], hy[10];
+          const uint2* px = reinterpret_cast<const uint2*>(&v.Q.hitIndices[10 * x]);  // 10 words, 8-byte aligned
+          const uint2* py = reinterpret_cast<const uint2*>(&v.Q.hitIndices[10 * y]);
 #pragma unroll
-          for (int a = 0; a < 10; a++) {
-            hx[a] = v.Q.hitIndices[10 * x + a];
-            hy[a] = v.Q.hitIndices[10 * y + a];
+          for (int a = 0; a < 5; a++) {
+            const uint2 u = ld_global(px + a), w = ld_global(py + a);
+            hx[2 * a] = u.x;
+            hx[2 * a + 1] = u.y;
+            hy[2 * a] = w.x;
+            hy[2 * a + 1] = w.y;
           }
           int matched = 0;
 #pragma unroll

@@ -145,6 +145,18 @@ __device__ __forceinline__ uint32_t candidate_object(const uint32_t* pre, const 
   return objBase[lo] + (L - pre[lo]);
 }
 
+// a pLS's two lower modules and four hits (anchor and outer hit of its inner, then of its outer
+// mini-doublet), from 16-byte record chunks
+__device__ __forceinline__ void pls_mods_hits(const EventView& v, uint32_t pls, uint16_t mods[2], uint32_t hits[4]) {
+  const uint4 s = seg_chunk(v.S, pls, 0);
+  mods[0] = (uint16_t)s.z;
+  mods[1] = (uint16_t)s.w;
+  hits[0] = md_chunk(v.D, s.x, 0).x;
+  hits[1] = md_chunk(v.D, s.x, 4).z;
+  hits[2] = md_chunk(v.D, s.y, 0).x;
+  hits[3] = md_chunk(v.D, s.y, 4).z;
+}
+
 struct PT5Kind {
   using Res = PT5Result;
   static constexpr uint32_t kBit = 1, kCap = kNMaxPixelQuintuplets;
@@ -196,20 +208,25 @@ struct PT5Kind {
     v.P5.centerY[q] = r.cy;
     v.P5.logicalLayers[7 * q] = 0;
     v.P5.logicalLayers[7 * q + 1] = 0;
-    v.P5.lowerModuleIndices[7 * q] = v.S.innerLowerModuleIndices[pls];
-    v.P5.lowerModuleIndices[7 * q + 1] = v.S.outerLowerModuleIndices[pls];
+    uint16_t pm[2];
+    uint32_t ph[4];
+    pls_mods_hits(v, pls, pm, ph);
+    v.P5.lowerModuleIndices[7 * q] = pm[0];
+    v.P5.lowerModuleIndices[7 * q + 1] = pm[1];
 #pragma unroll
     for (int j = 0; j < 5; j++) {
       v.P5.logicalLayers[7 * q + 2 + j] = v.Q.logicalLayers[5 * t5 + j];
       v.P5.lowerModuleIndices[7 * q + 2 + j] = v.Q.lowerModuleIndices[5 * t5 + j];
     }
-    const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
-    v.P5.hitIndices[14 * q] = v.D.anchorHitIndices[pi];
-    v.P5.hitIndices[14 * q + 1] = v.D.outerHitIndices[pi];
-    v.P5.hitIndices[14 * q + 2] = v.D.anchorHitIndices[po];
-    v.P5.hitIndices[14 * q + 3] = v.D.outerHitIndices[po];
 #pragma unroll
-    for (int j = 0; j < 10; j++) v.P5.hitIndices[14 * q + 4 + j] = v.Q.hitIndices[10 * t5 + j];
+    for (int j = 0; j < 4; j++) v.P5.hitIndices[14 * q + j] = ph[j];
+    const uint2* th = reinterpret_cast<const uint2*>(&v.Q.hitIndices[10 * t5]);  // 10 words, 8-byte aligned
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const uint2 h = ld_global(th + j);
+      v.P5.hitIndices[14 * q + 4 + 2 * j] = h.x;
+      v.P5.hitIndices[14 * q + 5 + 2 * j] = h.y;
+    }
     v.P5.rzChiSquared[q] = r.rz;
     v.P5.rPhiChiSquared[q] = r.rphi;
     v.P5.rPhiChiSquaredInwards[q] = r.rphiIn;
@@ -270,16 +287,26 @@ struct PT3Kind {
   }
   // addPixelTripletToMemory (SDL/PixelTriplet.h:134-204)
   __device__ static void write(const EventView& v, uint32_t q, uint32_t pls, uint32_t i, uint32_t t3, const Res& r) {
-    const uint32_t m = v.T.lowerModuleIndices[3 * t3];
+    uint4 c[4];  // the T3 record's first four chunks (segments, modules, logical layers, hits)
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) c[k] = t3_chunk(v.T, t3, k);
+    const uint32_t w[16] = {c[0].x, c[0].y, c[0].z, c[0].w, c[1].x, c[1].y, c[1].z, c[1].w,
+                            c[2].x, c[2].y, c[2].z, c[2].w, c[3].x, c[3].y, c[3].z, c[3].w};
+    constexpr uint32_t kMod = t3_word(kT3F_lowerModuleIndices), kLL = t3_word(kT3F_logicalLayers),
+                       kHit = t3_word(kT3F_hitIndices);
+    static_assert(kHit + 6 <= 16, "the hits are in the first four chunks");
+    const uint32_t m = (uint16_t)w[kMod];
     const int adj = v.M.layers[m] == 1 ? 1 : 0;
-    const uint32_t mdAdj = v.S.mdIndices[2 * v.T.segmentIndices[2 * t3] + adj];
+    const uint4 sIn = seg_chunk(v.S, c[0].x, 0);
+    const uint32_t mdAdj = adj ? sIn.y : sIn.x;
+    const uint4 mdA = md_chunk(v.D, mdAdj, 2), mdE = md_chunk(v.D, mdAdj, 4);  // anchor phi, anchor eta
     v.P3.pixelSegmentIndices[q] = pls;
     v.P3.tripletIndices[q] = t3;
     v.P3.pixelRadius[q] = r.pixR;
     v.P3.tripletRadius[q] = r.t3R;
     v.P3.pt[q] = v.P.ptIn[i];
-    v.P3.eta[q] = v.D.anchorEta[mdAdj];
-    v.P3.phi[q] = v.D.anchorPhi[mdAdj];
+    v.P3.eta[q] = __uint_as_float(mdE.w);
+    v.P3.phi[q] = __uint_as_float(mdA.y);
     v.P3.eta_pix[q] = v.P.eta[i];
     v.P3.phi_pix[q] = v.P.phi[i];
     v.P3.isDup[q] = 0;
@@ -288,20 +315,20 @@ struct PT3Kind {
     v.P3.centerY[q] = 0;
     v.P3.logicalLayers[5 * q] = 0;
     v.P3.logicalLayers[5 * q + 1] = 0;
-    v.P3.lowerModuleIndices[5 * q] = v.S.innerLowerModuleIndices[pls];
-    v.P3.lowerModuleIndices[5 * q + 1] = v.S.outerLowerModuleIndices[pls];
+    uint16_t pm[2];
+    uint32_t ph[4];
+    pls_mods_hits(v, pls, pm, ph);
+    v.P3.lowerModuleIndices[5 * q] = pm[0];
+    v.P3.lowerModuleIndices[5 * q + 1] = pm[1];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      v.P3.logicalLayers[5 * q + 2 + j] = v.T.logicalLayers[3 * t3 + j];
-      v.P3.lowerModuleIndices[5 * q + 2 + j] = v.T.lowerModuleIndices[3 * t3 + j];
+      v.P3.logicalLayers[5 * q + 2 + j] = (uint8_t)w[kLL + j];
+      v.P3.lowerModuleIndices[5 * q + 2 + j] = (uint16_t)w[kMod + j];
     }
-    const uint32_t pi = v.S.mdIndices[2 * pls], po = v.S.mdIndices[2 * pls + 1];
-    v.P3.hitIndices[10 * q] = v.D.anchorHitIndices[pi];
-    v.P3.hitIndices[10 * q + 1] = v.D.outerHitIndices[pi];
-    v.P3.hitIndices[10 * q + 2] = v.D.anchorHitIndices[po];
-    v.P3.hitIndices[10 * q + 3] = v.D.outerHitIndices[po];
 #pragma unroll
-    for (int j = 0; j < 6; j++) v.P3.hitIndices[10 * q + 4 + j] = v.T.hitIndices[6 * t3 + j];
+    for (int j = 0; j < 4; j++) v.P3.hitIndices[10 * q + j] = ph[j];
+#pragma unroll
+    for (int j = 0; j < 6; j++) v.P3.hitIndices[10 * q + 4 + j] = w[kHit + j];
     v.P3.rzChiSquared[q] = r.rz;
     v.P3.rPhiChiSquared[q] = r.rphi;
     v.P3.rPhiChiSquaredInwards[q] = r.rphiIn;
