@@ -281,8 +281,8 @@ lst_status lst_event::allocate(uint32_t nHitsNeeded) {
   c.take(e.W.plsBinRec, plsCap); c.take(e.W.plsBinHits, plsCap);
   c.take(e.W.t5Binned, t5Cap ? t5Cap : 1); c.take(e.W.t5BinOff, lst::kEtaBinOffsets);
   c.take(e.W.t5BinRec, t5Cap ? t5Cap : 1); c.take(e.W.t5ListRec, t5Cap ? t5Cap : 1);
-  c.take(e.W.pixBinned, pt5Cap + pt3Cap); c.take(e.W.pixBinOff, lst::kEtaBinOffsets);
-  c.take(e.W.tcBinned, tcCap); c.take(e.W.tcBinOff, lst::kEtaBinOffsets); c.take(e.W.tcHitSet, lst::kTCHitSet);
+  c.take(e.W.pixBinRec, pt5Cap + pt3Cap); c.take(e.W.pixBinOff, lst::kEtaBinOffsets);
+  c.take(e.W.tcBinRec, tcCap); c.take(e.W.tcBinOff, lst::kEtaBinOffsets); c.take(e.W.tcHitSet, lst::kTCHitSet);
   c.take(e.W.pt5Mask, lst::kPixMaskWords); c.take(e.W.pt5MaskBase, plsCap);
   c.take(e.W.pt3Mask, lst::kPixMaskWords); c.take(e.W.pt3MaskBase, plsCap);
   c.take(e.W.pt5Shard, 2 * lst::kPixShards); c.take(e.W.pt3Shard, 2 * lst::kPixShards);

@@ -293,10 +293,11 @@ struct PixelWork {
   uint32_t* t5Binned;     // [T5 capacity] listed T5 slots, eta bin by eta bin
   uint32_t* t5BinOff;     // [kEtaBinOffsets]
   float4* t5BinRec;       // [T5 capacity] the same records, eta bin by eta bin
-  uint32_t* tcBinned;     // [TC capacity] track candidates, eta bin by eta bin
+  float4* tcBinRec;       // [TC capacity] per track candidate, eta bin by eta bin: (eta, phi, the dR2 bound
+                          // of crossCleanpLS, 0), the bound -1 for one that is not compared
   uint32_t* tcBinOff;     // [kEtaBinOffsets]
   int32_t* tcHitSet;      // [kTCHitSet] open-addressing set of the pLS hits of the pT3/pT5 candidates
-  uint32_t* pixBinned;    // [pT5 + pT3 capacity] pT5 j and pT3 (n5 + j), eta bin by eta bin
+  float2* pixBinRec;      // [pT5 + pT3 capacity] (eta, phi) of pT5 j and pT3 (n5 + j), eta bin by eta bin
   uint32_t* pixBinOff;    // [kEtaBinOffsets]
   unsigned long long* pt5Mask;  // [kPixMaskWords] pass mask per 64-candidate chunk of each pLS (pT5)
   uint32_t* pt5MaskBase;        // [pLS capacity] first mask word of the pLS, ~0: none

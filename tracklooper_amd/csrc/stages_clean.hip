@@ -701,20 +701,26 @@ __global__ __launch_bounds__(256) void k_t5_tc_pairs(LST_VIEWS) {
 // dropped.  dR2 < 1e-3 needs |d eta| < 0.032, and a NaN direction never flags,
 // so the pT5/pT3 are binned by eta and a T5 tile (the T5s in eta-bin order)
 // meets only the bins next to its own.
+struct PixBinItem {
+  float2 rec;  // (eta, phi)
+  float eta;
+};
 __global__ __launch_bounds__(1024) void k_bin_pix(LST_VIEWS) {
   LST_BIND;
   const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
   bin_by_eta(
       n5 + v.R.scalars[kNPixelTriplets], v.largePaths != 0,
-      [&](uint32_t j) { return BinItem{j, j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5]}; },
-      [&](uint32_t pos, const BinItem& t) { v.W.pixBinned[pos] = t.id; }, v.W.pixBinOff);
+      [&](uint32_t j) {
+        const float eta = j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5], phi = j < n5 ? v.P5.phi[j] : v.P3.phi[j - n5];
+        return PixBinItem{make_float2(eta, phi), eta};
+      },
+      [&](uint32_t pos, const PixBinItem& t) { v.W.pixBinRec[pos] = t.rec; }, v.W.pixBinOff);
 }
 
 __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(LST_VIEWS) {
   LST_BIND;
-  __shared__ float sEta[kTile], sPhi[kTile];
+  __shared__ float2 sRec[kTile];  // (eta, phi)
   const uint32_t n = v.R.scalars[kT5Listed];
-  const uint32_t n5 = v.R.scalars[kNPixelQuintuplets];
   const PairItems items = pair_items(n);
   for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
     const uint32_t x0 = (it / items.shares) * kTile;
@@ -734,17 +740,14 @@ __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(LST_VIEWS) {
     bool dup = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      if (y0 + threadIdx.x < ys.e) {
-        const uint32_t j = v.W.pixBinned[band_at(B, y0 + threadIdx.x)];
-        sEta[threadIdx.x] = j < n5 ? v.P5.eta[j] : v.P3.eta[j - n5];
-        sPhi[threadIdx.x] = j < n5 ? v.P5.phi[j] : v.P3.phi[j - n5];
-      }
+      if (y0 + threadIdx.x < ys.e) sRec[threadIdx.x] = v.W.pixBinRec[band_at(B, y0 + threadIdx.x)];
       __syncthreads();
       const uint32_t m = tile_len(ys, y0);
       if (!active || dup) continue;
       for (uint32_t t = 0; t < m && !dup; t++) {
-        const float dEta = m_abs(eta1 - sEta[t]);
-        const float dPhi = wrap_dphi(phi1, sPhi[t]);
+        const float2 y = sRec[t];
+        const float dEta = m_abs(eta1 - y.x);
+        const float dPhi = wrap_dphi(phi1, y.y);
         dup = dEta * dEta + dPhi * dPhi < 1e-3f;
       }
     }
@@ -762,11 +765,11 @@ __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(LST_VIEWS) {
 // its own; the hit test is a lookup in the set of the candidates' pixel hits
 // (ntuple indices), done once per pLS.
 __device__ __forceinline__ void pls_ntuple_hits(const EventView& v, uint32_t seg, int32_t* p) {
-  const uint32_t a = v.S.mdIndices[2 * seg], b = v.S.mdIndices[2 * seg + 1];
-  p[0] = (int32_t)v.H.idxs[v.D.anchorHitIndices[a]];
-  p[1] = (int32_t)v.H.idxs[v.D.anchorHitIndices[b]];
-  p[2] = (int32_t)v.H.idxs[v.D.outerHitIndices[a]];
-  p[3] = (int32_t)v.H.idxs[v.D.outerHitIndices[b]];
+  const uint4 s = seg_chunk(v.S, seg, 0);  // {inner MD, outer MD, modules}
+  const uint32_t a = s.x, b = s.y;
+  const uint32_t h[4] = {md_chunk(v.D, a, 0).x, md_chunk(v.D, b, 0).x, md_chunk(v.D, a, 4).z, md_chunk(v.D, b, 4).z};
+#pragma unroll
+  for (int k = 0; k < 4; k++) p[k] = (int32_t)v.H.idxs[h[k]];  // anchor hits, then outer hits
 }
 constexpr int32_t kSetEmpty = INT32_MIN;  // never an ntuple index (those are >= -1); k_pls_connected clears the set
 __device__ __forceinline__ uint32_t set_slot(int32_t h) { return ((uint32_t)h * 0x9E3779B1u) >> (32 - 18); }
@@ -777,12 +780,29 @@ __device__ __forceinline__ void set_insert(int32_t* set, int32_t h) {
     if (old == kSetEmpty || old == h) return;
   }
 }
-__device__ __forceinline__ bool set_has(const int32_t* set, int32_t h) {
-  for (uint32_t s = set_slot(h);; s = (s + 1) & (kTCHitSet - 1)) {
-    const int32_t k = set[s];
-    if (k == h) return true;
-    if (k == kSetEmpty) return false;
+// whether any of the four values other than -1 is in the set: the four probe
+// sequences run side by side, one load of each per step
+__device__ __forceinline__ bool set_has_any4(const int32_t* set, const int32_t h[4]) {
+  uint32_t s[4];
+  bool live[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    s[a] = set_slot(h[a]);
+    live[a] = h[a] != -1;
   }
+  while (live[0] || live[1] || live[2] || live[3]) {
+    int32_t k[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) k[a] = live[a] ? set[s[a]] : kSetEmpty;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      if (!live[a]) continue;
+      if (k[a] == h[a]) return true;
+      if (k[a] == kSetEmpty) live[a] = false;
+      else s[a] = (s[a] + 1) & (kTCHitSet - 1);
+    }
+  }
+  return false;
 }
 
 // per track candidate: the direction a pLS is compared with and its kind (0:
@@ -827,17 +847,26 @@ __global__ void k_tc_prep(LST_VIEWS) {
   }
 }
 
+struct TcBinItem {
+  float4 rec;  // (eta, phi, dR2 bound, 0)
+  float eta;
+};
 __global__ __launch_bounds__(1024) void k_bin_tc(LST_VIEWS) {
   LST_BIND;
   bin_by_eta(
-      v.R.scalars[kNTrackCandidates], v.largePaths != 0, [&](uint32_t t) { return BinItem{t, v.W.tcEta[t]}; },
-      [&](uint32_t pos, const BinItem& t) { v.W.tcBinned[pos] = t.id; }, v.W.tcBinOff);
+      v.R.scalars[kNTrackCandidates], v.largePaths != 0,
+      [&](uint32_t t) {
+        const float eta = v.W.tcEta[t];
+        const uint8_t kind = v.W.tcKind[t];  // 1: T5 (dR2 < 1e-3), 2: pT3 or pT5 (dR2 < 1e-6), 0: none
+        const float bound = kind == 1 ? 1e-3f : (kind == 2 ? 0.000001f : -1.f);
+        return TcBinItem{make_float4(eta, v.W.tcPhi[t], bound, 0.f), eta};
+      },
+      [&](uint32_t pos, const TcBinItem& t) { v.W.tcBinRec[pos] = t.rec; }, v.W.tcBinOff);
 }
 
 __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(LST_VIEWS) {
   LST_BIND;
-  __shared__ float sEta[kTile], sPhi[kTile];
-  __shared__ uint8_t sKind[kTile];
+  __shared__ float4 sRec[kTile];  // (eta, phi, dR2 bound, 0)
   const uint32_t n = v.R.scalars[kEvNPLS];
   const uint32_t plsBase = (uint32_t)v.segModuleIndices[v.M.nLowerModules];
   const PairItems items = pair_items(n);
@@ -859,27 +888,21 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(LST_VIEWS) {
       if (active && it % items.shares == 0) {  // the hit test, once per pLS
         int32_t p1[4];
         pls_ntuple_hits(v, plsBase + i, p1);
-        for (int a = 0; a < 4 && !dup; a++) dup = p1[a] != -1 && set_has(v.W.tcHitSet, p1[a]);
+        dup = set_has_any4(v.W.tcHitSet, p1);
       }
     }
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
       __syncthreads();
-      if (y0 + threadIdx.x < ys.e) {
-        const uint32_t t = v.W.tcBinned[band_at(B, y0 + threadIdx.x)];
-        sEta[threadIdx.x] = v.W.tcEta[t];
-        sPhi[threadIdx.x] = v.W.tcPhi[t];
-        sKind[threadIdx.x] = v.W.tcKind[t];
-      }
+      if (y0 + threadIdx.x < ys.e) sRec[threadIdx.x] = v.W.tcBinRec[band_at(B, y0 + threadIdx.x)];
       __syncthreads();
       const uint32_t m = tile_len(ys, y0);
       if (!active || dup) continue;
       for (uint32_t t2 = 0; t2 < m && !dup; t2++) {
-        const uint8_t kind = sKind[t2];
-        if (kind == 0) continue;
-        const float dEta = m_abs(eta1 - sEta[t2]);
-        const float dPhi = wrap_dphi(phi1, sPhi[t2]);
+        const float4 y = sRec[t2];
+        const float dEta = m_abs(eta1 - y.x);
+        const float dPhi = wrap_dphi(phi1, y.y);
         const float dR2 = dEta * dEta + dPhi * dPhi;
-        dup = kind == 1 ? dR2 < 1e-3f : dR2 < 0.000001f;
+        dup = dR2 < y.z;  // never for a bound of -1 (dR2 >= 0 or NaN)
       }
     }
     if (active && dup) v.P.isDup[i] = 1;
