@@ -725,17 +725,18 @@ __global__ __launch_bounds__(kTile) void k_cross_t5_pairs(LST_VIEWS) {
   for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
     const uint32_t x0 = (it / items.shares) * kTile;
     const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
-    const Band B = tile_band(v.W.pixBinOff, eta_bin(v.Q.eta[v.W.t5Binned[x0]]), eta_bin(v.Q.eta[v.W.t5Binned[xl]]));
+    const Band B = tile_band(v.W.pixBinOff, eta_bin(v.W.t5BinRec[x0].x), eta_bin(v.W.t5BinRec[xl].x));
     const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
     const uint32_t xi = x0 + threadIdx.x;
     uint32_t q = 0;
     bool active = xi < n;
     float eta1 = 0, phi1 = 0;
     if (active) {
-      q = v.W.t5Binned[xi];
+      const float4 rx = v.W.t5BinRec[xi];  // (eta, phi, score, slot)
+      q = __float_as_uint(rx.w);
       active = v.Q.isDup[q] == 0 && v.Q.partOfPT5[q] == 0;
-      eta1 = v.Q.eta[q];
-      phi1 = v.Q.phi[q];
+      eta1 = rx.x;
+      phi1 = rx.y;
     }
     bool dup = false;
     for (uint32_t y0 = ys.b; y0 < ys.e; y0 += kTile) {
@@ -873,7 +874,7 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(LST_VIEWS) {
   for (uint32_t it = blockIdx.x; it < items.items; it += gridDim.x) {
     const uint32_t x0 = (it / items.shares) * kTile;
     const uint32_t xl = x0 + kTile - 1 < n ? x0 + kTile - 1 : n - 1;
-    const Band B = tile_band(v.W.tcBinOff, eta_bin(v.P.eta[v.W.plsBinned[x0]]), eta_bin(v.P.eta[v.W.plsBinned[xl]]));
+    const Band B = tile_band(v.W.tcBinOff, eta_bin(v.W.plsBinRec[x0].x), eta_bin(v.W.plsBinRec[xl].x));
     const YShare ys = share_of(B.n1 + B.n2, it % items.shares, items.shares);
     const uint32_t xi = x0 + threadIdx.x;
     uint32_t i = 0;
@@ -881,10 +882,11 @@ __global__ __launch_bounds__(kTile) void k_cross_pls_pairs(LST_VIEWS) {
     float eta1 = 0, phi1 = 0;
     bool dup = false;
     if (active) {
-      i = v.W.plsBinned[xi];
-      active = v.P.isQuad[i] && v.P.isDup[i] == 0;
-      eta1 = v.P.eta[i];
-      phi1 = v.P.phi[i];
+      const float4 rx = v.W.plsBinRec[xi];  // (eta, phi, score, index | isQuad << 31)
+      i = __float_as_uint(rx.w) & 0x7fffffffu;
+      active = (__float_as_uint(rx.w) & kQuadBit) != 0 && v.P.isDup[i] == 0;
+      eta1 = rx.x;
+      phi1 = rx.y;
       if (active && it % items.shares == 0) {  // the hit test, once per pLS
         int32_t p1[4];
         pls_ntuple_hits(v, plsBase + i, p1);
