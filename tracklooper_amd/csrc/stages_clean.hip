@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kTile) void k_pt3_pairs(LST_VIEWS) {
   LST_BIND;
   __shared__ float sScore[kTile];
   __shared__ uint8_t sLayer[kTile];
-  __shared__ uint32_t sHits[10 * kTile];
+  __shared__ __align__(16) uint32_t sHits[10 * kTile];
   const uint32_t n = v.R.scalars[kNPixelTriplets];
   const YShare ys = y_share(n);
   for (uint32_t x0 = blockIdx.x * kTile; x0 < n; x0 += gridDim.x * kTile) {
@@ -405,8 +405,13 @@ __global__ __launch_bounds__(kTile) void k_pt3_pairs(LST_VIEWS) {
     uint8_t li = 0;
     float si = 0;
     if (active) {
+      const uint2* ph = reinterpret_cast<const uint2*>(&v.P3.hitIndices[10 * ix]);  // 10 words, 8-byte aligned
 #pragma unroll
-      for (int a = 0; a < 10; a++) h[a] = v.P3.hitIndices[10 * ix + a];
+      for (int a = 0; a < 5; a++) {
+        const uint2 u = ld_global(ph + a);
+        h[2 * a] = u.x;
+        h[2 * a + 1] = u.y;
+      }
       li = v.P3.logicalLayers[5 * ix + 2];
       si = v.P3.score[ix];
     }
@@ -417,7 +422,10 @@ __global__ __launch_bounds__(kTile) void k_pt3_pairs(LST_VIEWS) {
       if (jl < ys.e) {
         sScore[threadIdx.x] = v.P3.score[jl];
         sLayer[threadIdx.x] = v.P3.logicalLayers[5 * jl + 2];
-        for (int a = 0; a < 10; a++) sHits[threadIdx.x * 10 + a] = v.P3.hitIndices[10 * jl + a];
+        const uint2* ph = reinterpret_cast<const uint2*>(&v.P3.hitIndices[10 * jl]);
+        uint2* sh = reinterpret_cast<uint2*>(&sHits[threadIdx.x * 10]);
+#pragma unroll
+        for (int a = 0; a < 5; a++) sh[a] = ld_global(ph + a);
       }
       __syncthreads();
       const uint32_t m = tile_len(ys, y0);

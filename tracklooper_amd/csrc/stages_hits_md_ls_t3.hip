@@ -646,17 +646,19 @@ __global__ __launch_bounds__(256) void k_ls_place(LST_VIEWS) {
   for_each_passing(v.FS, total, nL, [&](uint32_t L, uint32_t rankL) {
     // the loads that need L, then those that need its mini-doublets and module, each group issued before any use
     const uint2 c = v.WL.cand[L];
-    const float* w = reinterpret_cast<const float*>(&v.WL.park[kLSParkWords * (size_t)L]);
-    const float w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
-    const uint32_t inMD = c.x, outMD = c.y, m = v.D.moduleIndices[inMD], om = v.D.moduleIndices[outMD];
+    static_assert(kLSParkWords % 2 == 0, "parked results in 8-byte pairs");
+    const float2* w = reinterpret_cast<const float2*>(&v.WL.park[kLSParkWords * (size_t)L]);
+    const float2 p0 = ld_global(w), p1 = ld_global(w + 1), p2 = ld_global(w + 2);
+    const uint32_t inMD = c.x, outMD = c.y;
+    const uint4 a = md_chunk(v.D, inMD, 0), b = md_chunk(v.D, outMD, 0);  // {anchor hit, module, ...}
+    const uint32_t m = (uint16_t)a.y, om = (uint16_t)b.y, hitIn = a.x, hitOut = b.x;
     const uint32_t off = v.WL.candOff[m];
     const int occ = v.segModuleOccupancy[m];
     const uint32_t segBase = v.segModuleIndices[m];
-    const uint32_t hitIn = v.D.anchorHitIndices[inMD], hitOut = v.D.anchorHitIndices[outMD];
     const uint32_t rank = rankL - flag_rank(v.FS, off, total);
     if ((int)rank >= occ) return;
     const uint32_t s = segBase + rank;
-    const float d[6] = {w0, w1, w2, w3, w4, w5};
+    const float d[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
     store_seg(v, s, inMD, outMD, m, om, hitIn, hitOut, d);
   }, [&](uint32_t m) {  // then per module its counts and range
     const uint32_t count = flag_rank(v.FS, v.WL.candOff[m + 1], total) - flag_rank(v.FS, v.WL.candOff[m], total);
