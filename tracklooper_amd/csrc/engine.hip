@@ -1543,7 +1543,7 @@ lst_status lst_batch_autotune(lst_batch* b, uint32_t first, uint32_t repeats, fl
   // on its own count only
   std::vector<uint32_t> sid;  // the slot order of a run: largest event first
   for (uint32_t p : largest_first(b, ids.data(), ids.size(), b->slots)) sid.push_back(ids[p]);
-  static const uint32_t cand[] = {0, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096};
+  static const uint32_t cand[] = {0, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096};
   constexpr int nc = sizeof(cand) / sizeof(cand[0]);
   std::vector<float> best((size_t)lst::kNTune * nc, 1e30f);
   uint32_t saved[lst::kNTune];
