@@ -375,19 +375,17 @@ __device__ __forceinline__ T3Rec load_t3_rec(const float* rec) {
 // 826-1024): the pLS against the inner and against the outer segment of T3 t3
 __device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3) {
   const ModulesView& M = v.M;
-  const uint16_t lo = v.T.lowerModuleIndices[3 * t3], mid = v.T.lowerModuleIndices[3 * t3 + 1],
-                 up = v.T.lowerModuleIndices[3 * t3 + 2];
-  const uint32_t segIn = v.T.segmentIndices[2 * t3], segOut = v.T.segmentIndices[2 * t3 + 1];
-  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
+  // record chunks: T3 {segments, modules 0-1}, {module 2, ...}; segment {MDs, modules}, {..., dPhiChange, ...}
+  const uint4 t0 = t3_chunk(v.T, t3, 0);
+  const uint16_t lo = (uint16_t)t0.z, mid = (uint16_t)t0.w, up = (uint16_t)t3_chunk(v.T, t3, 1).x;
+  const uint4 sp = seg_chunk(v.S, pls, 0), sIn = seg_chunk(v.S, t0.x, 0), sOut = seg_chunk(v.S, t0.y, 0);
   const PixelSeed s = load_seed(v.P, arr);
-  const float alphaPix = v.S.dPhiChanges[pls];
-  const uint32_t md1 = v.S.mdIndices[2 * segIn], md2 = v.S.mdIndices[2 * segIn + 1];
-  const uint32_t md3 = v.S.mdIndices[2 * segOut], md4 = v.S.mdIndices[2 * segOut + 1];
-  if (!pixel_tracklet(M, v.D, s, pixIn, pixOut, md1, md2, lo, mid, alphaPix, v.S.dPhiChanges[segIn], v.ptCut,
-                      md_slope(v.D, md1)))
+  const float alphaPix = __uint_as_float(seg_chunk(v.S, pls, 2).y);
+  if (!pixel_tracklet(M, v.D, s, sp.x, sp.y, sIn.x, sIn.y, lo, mid, alphaPix,
+                      __uint_as_float(seg_chunk(v.S, t0.x, 2).y), v.ptCut, md_slope(v.D, sIn.x)))
     return false;
-  return pixel_tracklet(M, v.D, s, pixIn, pixOut, md3, md4, mid, up, alphaPix, v.S.dPhiChanges[segOut], v.ptCut,
-                        md_slope(v.D, md3));
+  return pixel_tracklet(M, v.D, s, sp.x, sp.y, sOut.x, sOut.y, mid, up, alphaPix,
+                        __uint_as_float(seg_chunk(v.S, t0.y, 2).y), v.ptCut, md_slope(v.D, sOut.x));
 }
 
 // one of the two pixel-tracklet tests of pt3_tracklets: part 0 the inner
@@ -395,14 +393,15 @@ __device__ __forceinline__ bool pt3_tracklets(const EventView& v, uint32_t pls, 
 __device__ __forceinline__ bool pt3_tracklet_half(const EventView& v, uint32_t pls, uint32_t arr, uint32_t t3,
                                                   uint32_t part) {
   const ModulesView& M = v.M;
-  const uint16_t mA = v.T.lowerModuleIndices[3 * t3 + part], mB = v.T.lowerModuleIndices[3 * t3 + 1 + part];
-  const uint32_t seg = v.T.segmentIndices[2 * t3 + part];
-  const uint32_t pixIn = v.S.mdIndices[2 * pls], pixOut = v.S.mdIndices[2 * pls + 1];
+  const uint4 t0 = t3_chunk(v.T, t3, 0);  // {segments, modules 0-1}
+  const uint16_t mA = (uint16_t)(part ? t0.w : t0.z);
+  const uint16_t mB = (uint16_t)(part ? t3_chunk(v.T, t3, 1).x : t0.w);
+  const uint32_t seg = part ? t0.y : t0.x;
+  const uint4 sp = seg_chunk(v.S, pls, 0), sg = seg_chunk(v.S, seg, 0);  // {MDs, modules}
   const PixelSeed s = load_seed(v.P, arr);
-  const float alphaPix = v.S.dPhiChanges[pls];
-  const uint32_t mdA = v.S.mdIndices[2 * seg], mdB = v.S.mdIndices[2 * seg + 1];
-  return pixel_tracklet(M, v.D, s, pixIn, pixOut, mdA, mdB, mA, mB, alphaPix, v.S.dPhiChanges[seg], v.ptCut,
-                        md_slope(v.D, mdA));
+  const float alphaPix = __uint_as_float(seg_chunk(v.S, pls, 2).y);  // dPhiChange
+  return pixel_tracklet(M, v.D, s, sp.x, sp.y, sg.x, sg.y, mA, mB, alphaPix,
+                        __uint_as_float(seg_chunk(v.S, seg, 2).y), v.ptCut, md_slope(v.D, sg.x));
 }
 
 // runPixelTripletDefaultAlgo (SDL/PixelTriplet.h:826-1024) is the AND of pure

@@ -123,6 +123,13 @@ static_assert(kMDw_anchorHitIndices == 0 && kMDw_moduleIndices == 1 && kMDw_dphi
 __device__ __forceinline__ float2 md_slope(const MDsSoA& D, uint32_t i) {
   return ld_global(reinterpret_cast<const float2*>(D.anchorHitIndices.p + (size_t)i * kMDHotBytes + 4 * kMDSlopeWord));
 }
+// outer hit x, y of mini-doublet record i (adjacent cold words), one global load
+static_assert(md_slot(kMDw_outerY) == md_slot(kMDw_outerX) + 1 && md_slot(kMDw_outerX) % 2 == 0 &&
+                  kMDw_outerX >= kMDHotFields,
+              "outer x and y are an aligned pair of cold words");
+__device__ __forceinline__ float2 md_outer_xy(const MDsSoA& D, uint32_t i) {
+  return ld_global(reinterpret_cast<const float2*>(&D.outerX[i]));
+}
 // 16-byte chunk q (hot words 4q .. 4q + 3) of mini-doublet record i, one global load: 0 {anchor hit, module,
 // dphichange, dz}, 1 {anchor x, y, z, rt}, 2 {dphi, anchor phi, edge phis}, 3 {edge x, y}, 4 {slope pair,
 // outer hit, anchor eta}

@@ -162,11 +162,12 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
     p[k] = load_point(v.D, c.md[k]);
   }
   float g, f;
-  r.outerR = v.T.circleRadius[outerT3];
+  r.outerR = __uint_as_float(t3_chunk(v.T, outerT3, 3).w);  // circle radius
   r.bridgeR = circle_radius3(p[1].x, p[1].y, p[2].x, p[2].y, p[3].x, p[3].y, g, f);
-  r.innerR = v.T.circleRadius[innerT3];
-  g = v.T.circleCenterX[innerT3];
-  f = v.T.circleCenterY[innerT3];
+  const uint4 i3 = t3_chunk(v.T, innerT3, 3), i4 = t3_chunk(v.T, innerT3, 4);  // radius; centre x, y
+  r.innerR = __uint_as_float(i3.w);
+  g = __uint_as_float(i4.x);
+  f = __uint_as_float(i4.y);
   const float innerPt = 2 * k2Rinv1GeVf * r.innerR;
   r.tight = false;
   if (!t5_rz_chi2(m, p, innerPt, r.innerR, g, f, r.rzChi2, r.tight)) return false;
@@ -192,8 +193,9 @@ __device__ __forceinline__ bool t5_rest(const EventView& v, const T5Cand& c, uin
     return false;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    x[k] = v.D.outerX[c.md[k]];
-    y[k] = v.D.outerY[c.md[k]];
+    const float2 o = md_outer_xy(v.D, c.md[k]);
+    x[k] = o.x;
+    y[k] = o.y;
   }
   regression_sigmas(m, false, d1, d2, flat);
   r.nonAnchorChi2 = circle_chi2(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.regR);
@@ -231,11 +233,12 @@ __device__ __forceinline__ bool t5_rest_half(const EventView& v, const T5Cand& c
   }
   if (part == 0) {
     float g, f;
-    r.outerR = v.T.circleRadius[outerT3];
+    r.outerR = __uint_as_float(t3_chunk(v.T, outerT3, 3).w);  // circle radius
     r.bridgeR = circle_radius3(p[1].x, p[1].y, p[2].x, p[2].y, p[3].x, p[3].y, g, f);
-    r.innerR = v.T.circleRadius[innerT3];
-    g = v.T.circleCenterX[innerT3];
-    f = v.T.circleCenterY[innerT3];
+    const uint4 i3 = t3_chunk(v.T, innerT3, 3), i4 = t3_chunk(v.T, innerT3, 4);  // radius; centre x, y
+    r.innerR = __uint_as_float(i3.w);
+    g = __uint_as_float(i4.x);
+    f = __uint_as_float(i4.y);
     const float innerPt = 2 * k2Rinv1GeVf * r.innerR;
     r.tight = false;
     if (!t5_rz_chi2(m, p, innerPt, r.innerR, g, f, r.rzChi2, r.tight)) return false;
@@ -260,8 +263,9 @@ __device__ __forceinline__ bool t5_rest_half(const EventView& v, const T5Cand& c
     return false;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    x[k] = v.D.outerX[c.md[k]];
-    y[k] = v.D.outerY[c.md[k]];
+    const float2 o = md_outer_xy(v.D, c.md[k]);
+    x[k] = o.x;
+    y[k] = o.y;
   }
   regression_sigmas(m, false, d1, d2, flat);
   r.nonAnchorChi2 = circle_chi2(v.M, c.mods, x, y, d1, d2, flat, r.regG, r.regF, r.regR);
