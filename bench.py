@@ -80,8 +80,8 @@ def algorithmic_bytes(n: dict) -> dict:
                            mini-doublets' anchor hits and modules (12 B), write the 44 B segment
       k_t3_place         : per triplet read its list entry (8 B) and results (24 B), write the 60 B triplet
       k_t5_place         : per quintuplet read its list entry (12 B) and results (48 B), write the 119 B quintuplet
-      k_pix_write<PT5Kind>: per pLS its mask words (8 B) and per pixel quintuplet its parked result (40 B) and
-                           the 150 B record written
+      k_pix_write<PT5Kind>: per stage-A survivor its pLS, mask position and object (12 B) and its mask word
+                           (8 B); per pixel quintuplet its parked result (40 B) and the 150 B record written
     """
     h, md, ls, t3, t5 = n["hits"], n["mds"], n["segs"], n["t3"], n["t5"]
     pls, pt5, pt3, tc = n["pls"], n["pt5"], n["pt3"], n["tc"]
@@ -108,7 +108,7 @@ def algorithmic_bytes(n: dict) -> dict:
         "k_ls_place": 88 * ls,
         "k_t3_place": 92 * t3,
         "k_t5_place": 179 * t5,
-        "k_pix_write<PT5Kind>": 8 * pls + 190 * pt5,
+        "k_pix_write<PT5Kind>": 20 * n["pt5surv"] + 190 * pt5,
     }
 
 
