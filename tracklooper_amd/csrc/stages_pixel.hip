@@ -6,8 +6,8 @@
 // Ordering.  The reference CPU build fills pT5 and pT3 slots in the order of
 // its serial loops (pLS, then connected module, then outer object) and TC
 // slots in kernel order (pT5, pT3, T5, pLS), each truncated at a fixed
-// capacity.  The builders here run one wave per pLS over that pLS's
-// candidates in the same order: a counting pass, a scan over pLS, then a
+// capacity.  The builders here take each pLS's candidates in the same order
+// (stage A one wave per pLS): a counting pass, a scan over pLS, then a
 // writing pass that places every passing candidate at its serial slot.  TC
 // additions rank their flags with workgroup scans.  Duplicate removal and
 // cross cleaning run as all-pairs kernels in stages_clean.hip.
@@ -71,12 +71,13 @@ __global__ void k_pls_connected(LST_VIEWS) {
 //            position, object, partial result) is appended to a list; the
 //            pLS's pass-mask words are cleared
 //   stage B  one lane per survivor: the costly cuts; a pass sets its bit in
-//            the pLS's mask, records where its result is, and counts the pLS;
+//            the pLS's mask, stores its result in its survivor record, and counts the pLS;
 //            pLS whose survivors did not all fit (flagged by stage A) are
 //            counted instead by evaluating every candidate
 //   scan     per-pLS counts to slot offsets
-//   write    one wave per pLS: the passing candidates, in mask order, to
-//            their slots; the flagged pLS are evaluated again instead
+//   write    one lane per survivor: the passing candidates to their slots, the
+//            set mask bits before each giving its rank; the flagged pLS are
+//            evaluated again instead
 
 // the connected-module prefix of pLS i in LDS: pre[k] candidates before
 // module k, objBase[k] the module's first object; returns the total
